@@ -1,0 +1,298 @@
+"""ctypes wrapper of the CPU oracle (``oracle/liboracle.so``).
+
+TEST INFRASTRUCTURE ONLY: imported by ``tests/``, ``__graft_entry__.smoke()``
+and the CPU-baseline leg of ``bench.py`` as the checker.  The product path
+(the HIP library and its host package) never imports this module.
+
+Arrays use the Trixi.jl layout ``u[v, i, j, k, e]`` (Julia column-major), i.e.
+a C-contiguous numpy array of shape ``(E, N, N, N, nv)`` for 3D (reversed
+axes), ``(E, N, N, nv)`` for 2D and ``(E, N, nv)`` for 1D.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+
+EQ_LINADV, EQ_EULER, EQ_GLM_MHD = 0, 1, 2
+VI_WEAK, VI_FD, VI_HG = 0, 1, 2
+FLUX_CENTRAL, FLUX_LLF, FLUX_LLF_NAIVE, FLUX_RANOCHA, FLUX_SHIMA_ETAL, FLUX_HINDENLANG_GASSNER = range(6)
+IC_CONSTANT, IC_CONVERGENCE_TEST, IC_WEAK_BLAST_WAVE, IC_SEEDED_RANDOM, IC_SMOOTH_PERIODIC = range(5)
+
+
+class OracleCfg(ctypes.Structure):
+    _fields_ = [
+        ("ndims", ctypes.c_int32),
+        ("equation", ctypes.c_int32),
+        ("polydeg", ctypes.c_int32),
+        ("volume_integral", ctypes.c_int32),
+        ("volume_flux", ctypes.c_int32),
+        ("surface_flux", ctypes.c_int32),
+        ("fv_flux", ctypes.c_int32),
+        ("alpha_smooth", ctypes.c_int32),
+        ("gamma", ctypes.c_double),
+        ("c_h", ctypes.c_double),
+        ("advection_velocity", ctypes.c_double * 3),
+        ("alpha_max", ctypes.c_double),
+        ("alpha_min", ctypes.c_double),
+        ("ic_length", ctypes.c_double),
+    ]
+
+
+def build():
+    """Compile liboracle.so with the committed Makefile."""
+    subprocess.run(["make", "-s", "-C", _HERE, "liboracle.so"], check=True)
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = ctypes.CDLL(_LIB_PATH)
+        P = ctypes.c_void_p
+        i64, i32, dbl, u64 = ctypes.c_int64, ctypes.c_int32, ctypes.c_double, ctypes.c_uint64
+        cfgp = ctypes.POINTER(OracleCfg)
+        sig = {
+            "oracle_nvars": (ctypes.c_int, [cfgp]),
+            "oracle_basis": (ctypes.c_int, [ctypes.c_int] + [P] * 8),
+            "oracle_interpolation_matrix": (ctypes.c_int, [ctypes.c_int, P, ctypes.c_int, P, P]),
+            "oracle_ln_mean": (dbl, [dbl, dbl]),
+            "oracle_inv_ln_mean": (dbl, [dbl, dbl]),
+            "oracle_flux": (None, [cfgp, P, ctypes.c_int, P]),
+            "oracle_two_point_flux": (None, [cfgp, ctypes.c_int, P, P, ctypes.c_int, P]),
+            "oracle_noncons_flux": (None, [cfgp, P, P, ctypes.c_int, P]),
+            "oracle_max_abs_speeds": (None, [cfgp, P, P]),
+            "oracle_entropy": (dbl, [cfgp, P]),
+            "oracle_cons2entropy": (None, [cfgp, P, P]),
+            "oracle_entropy_potential": (None, [cfgp, P, ctypes.c_int, P]),
+            "oracle_mesh_nelements": (i64, [ctypes.c_int, ctypes.c_int]),
+            "oracle_mesh_uniform": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, dbl, dbl, ctypes.c_int, P, P, P, P, P]),
+            "oracle_initial_condition": (None, [cfgp, ctypes.c_int, P, dbl, u64, u64, P]),
+            "oracle_compute_coefficients": (None, [cfgp, ctypes.c_int, i64, P, dbl, u64, P]),
+            "oracle_rhs": (None, [cfgp, i64, i64, P, P, P, P, P, P]),
+            "oracle_indicator_hg": (None, [cfgp, i64, i64, P, P, P]),
+            "oracle_max_dt": (dbl, [cfgp, i64, P, P]),
+            "oracle_error_norms": (None, [cfgp, ctypes.c_int, i64, P, P, P, dbl, dbl, u64, P, P]),
+            "oracle_integrate": (None, [cfgp, i64, P, P, P]),
+            "oracle_ck2n54_coefficients": (None, [P, P, P]),
+            "oracle_ck2n54_step": (None, [cfgp, i64, i64, P, P, P, P, P, P, dbl, dbl]),
+            "oracle_rhs_jvp": (None, [cfgp, i64, i64, P, P, P, P, P, P, P]),
+            "oracle_set_threads": (ctypes.c_int, [ctypes.c_int]),
+        }
+        for name, (res, args) in sig.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+def make_cfg(ndims, equation, polydeg=3, volume_integral=VI_WEAK, volume_flux=FLUX_CENTRAL,
+             surface_flux=FLUX_LLF, fv_flux=FLUX_LLF, gamma=1.4, c_h=1.0,
+             advection_velocity=(1.0, 1.0, 1.0), alpha_max=0.5, alpha_min=0.001,
+             alpha_smooth=True, ic_length=4.0):
+    c = OracleCfg()
+    c.ndims, c.equation, c.polydeg = ndims, equation, polydeg
+    c.volume_integral, c.volume_flux, c.surface_flux, c.fv_flux = (
+        volume_integral, volume_flux, surface_flux, fv_flux)
+    c.alpha_smooth = 1 if alpha_smooth else 0
+    c.gamma, c.c_h = gamma, c_h
+    for d in range(3):
+        c.advection_velocity[d] = advection_velocity[d] if d < len(advection_velocity) else 0.0
+    c.alpha_max, c.alpha_min, c.ic_length = alpha_max, alpha_min, ic_length
+    return c
+
+
+def nvars(cfg):
+    return lib().oracle_nvars(ctypes.byref(cfg))
+
+
+def basis(nnodes):
+    N = nnodes
+    out = {k: np.zeros(N) for k in ("nodes", "weights", "lhat_minus", "lhat_plus")}
+    for k in ("D", "Dsplit", "Dhat", "Vinv"):
+        out[k] = np.zeros((N, N))
+    rc = lib().oracle_basis(N, _p(out["nodes"]), _p(out["weights"]), _p(out["D"]), _p(out["Dsplit"]),
+                            _p(out["Dhat"]), _p(out["lhat_minus"]), _p(out["lhat_plus"]), _p(out["Vinv"]))
+    assert rc == 0
+    return out
+
+
+def interpolation_matrix(nodes_in, nodes_out):
+    nin, nout = len(nodes_in), len(nodes_out)
+    V = np.zeros((nout, nin))
+    lib().oracle_interpolation_matrix(nin, _p(np.ascontiguousarray(nodes_in, dtype=np.float64)), nout,
+                                      _p(np.ascontiguousarray(nodes_out, dtype=np.float64)), _p(V))
+    return V
+
+
+def ln_mean(x, y):
+    return lib().oracle_ln_mean(float(x), float(y))
+
+
+def inv_ln_mean(x, y):
+    return lib().oracle_inv_ln_mean(float(x), float(y))
+
+
+def flux(cfg, u, orientation):
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    f = np.zeros(nvars(cfg))
+    lib().oracle_flux(ctypes.byref(cfg), _p(u), orientation, _p(f))
+    return f
+
+
+def two_point_flux(cfg, kind, ull, urr, orientation):
+    ull = np.ascontiguousarray(ull, dtype=np.float64)
+    urr = np.ascontiguousarray(urr, dtype=np.float64)
+    f = np.zeros(nvars(cfg))
+    lib().oracle_two_point_flux(ctypes.byref(cfg), kind, _p(ull), _p(urr), orientation, _p(f))
+    return f
+
+
+def noncons_flux(cfg, ull, urr, orientation):
+    ull = np.ascontiguousarray(ull, dtype=np.float64)
+    urr = np.ascontiguousarray(urr, dtype=np.float64)
+    f = np.zeros(nvars(cfg))
+    lib().oracle_noncons_flux(ctypes.byref(cfg), _p(ull), _p(urr), orientation, _p(f))
+    return f
+
+
+def cons2entropy(cfg, u):
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    w = np.zeros(nvars(cfg))
+    lib().oracle_cons2entropy(ctypes.byref(cfg), _p(u), _p(w))
+    return w
+
+
+def entropy(cfg, u):
+    return lib().oracle_entropy(ctypes.byref(cfg), _p(np.ascontiguousarray(u, dtype=np.float64)))
+
+
+def entropy_potential(cfg, u, orientation):
+    out = np.zeros(1)
+    lib().oracle_entropy_potential(ctypes.byref(cfg), _p(np.ascontiguousarray(u, dtype=np.float64)),
+                                   orientation, _p(out))
+    return out[0]
+
+
+def max_abs_speeds(cfg, u):
+    lam = np.zeros(3)
+    lib().oracle_max_abs_speeds(ctypes.byref(cfg), _p(np.ascontiguousarray(u, dtype=np.float64)), _p(lam))
+    return lam[: cfg.ndims]
+
+
+class Mesh:
+    """Uniform periodic TreeMesh as built by the oracle."""
+
+    def __init__(self, ndims, level, cmin, cmax, polydeg):
+        E = lib().oracle_mesh_nelements(ndims, level)
+        I = ndims * E
+        N = polydeg + 1
+        self.ndims, self.level, self.polydeg, self.nelements, self.ninterfaces = ndims, level, polydeg, E, I
+        self.coordinates_min, self.coordinates_max = cmin, cmax
+        self.neighbor_ids = np.zeros((I, 2), dtype=np.int32)
+        self.orientations = np.zeros(I, dtype=np.int32)
+        self.inverse_jacobian = np.zeros(E)
+        self.node_coordinates = np.zeros((E,) + (N,) * ndims + (ndims,))
+        self.cell_index = np.zeros((E, ndims), dtype=np.int32)
+        rc = lib().oracle_mesh_uniform(ndims, level, cmin, cmax, polydeg, _p(self.neighbor_ids),
+                                       _p(self.orientations), _p(self.inverse_jacobian),
+                                       _p(self.node_coordinates), _p(self.cell_index))
+        assert rc == 0
+        self.total_volume = (cmax - cmin) ** ndims
+
+
+def state_shape(cfg, nelements):
+    N = cfg.polydeg + 1
+    return (nelements,) + (N,) * cfg.ndims + (nvars(cfg),)
+
+
+def compute_coefficients(cfg, mesh, ic, t=0.0, seed=20261202):
+    u = np.zeros(state_shape(cfg, mesh.nelements))
+    lib().oracle_compute_coefficients(ctypes.byref(cfg), ic, mesh.nelements, _p(mesh.node_coordinates),
+                                      float(t), seed, _p(u))
+    return u
+
+
+def initial_condition(cfg, ic, x, t=0.0, seed=20261202, node_index=0):
+    u = np.zeros(nvars(cfg))
+    x = np.ascontiguousarray(np.atleast_1d(x), dtype=np.float64)
+    lib().oracle_initial_condition(ctypes.byref(cfg), ic, _p(x), float(t), seed, node_index, _p(u))
+    return u
+
+
+def rhs(cfg, mesh, u, return_alpha=False):
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    du = np.zeros_like(u)
+    alpha = np.zeros(mesh.nelements) if return_alpha else None
+    lib().oracle_rhs(ctypes.byref(cfg), mesh.nelements, mesh.ninterfaces, _p(mesh.neighbor_ids),
+                     _p(mesh.orientations), _p(mesh.inverse_jacobian), _p(u), _p(du), _p(alpha))
+    return (du, alpha) if return_alpha else du
+
+
+def rhs_jvp(cfg, mesh, u, udot):
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    udot = np.ascontiguousarray(udot, dtype=np.float64)
+    du = np.zeros_like(u)
+    dudot = np.zeros_like(u)
+    lib().oracle_rhs_jvp(ctypes.byref(cfg), mesh.nelements, mesh.ninterfaces, _p(mesh.neighbor_ids),
+                         _p(mesh.orientations), _p(mesh.inverse_jacobian), _p(u), _p(udot), _p(du), _p(dudot))
+    return du, dudot
+
+
+def indicator_hg(cfg, mesh, u):
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    alpha = np.zeros(mesh.nelements)
+    lib().oracle_indicator_hg(ctypes.byref(cfg), mesh.nelements, mesh.ninterfaces, _p(mesh.neighbor_ids),
+                              _p(u), _p(alpha))
+    return alpha
+
+
+def max_dt(cfg, mesh, u):
+    return lib().oracle_max_dt(ctypes.byref(cfg), mesh.nelements, _p(mesh.inverse_jacobian),
+                               _p(np.ascontiguousarray(u, dtype=np.float64)))
+
+
+def error_norms(cfg, mesh, u, t, ic, seed=20261202):
+    nv = nvars(cfg)
+    l2, linf = np.zeros(nv), np.zeros(nv)
+    lib().oracle_error_norms(ctypes.byref(cfg), ic, mesh.nelements, _p(mesh.inverse_jacobian),
+                             _p(mesh.node_coordinates), _p(np.ascontiguousarray(u, dtype=np.float64)),
+                             float(t), float(mesh.total_volume), seed, _p(l2), _p(linf))
+    return l2, linf
+
+
+def integrate(cfg, mesh, u):
+    out = np.zeros(nvars(cfg))
+    lib().oracle_integrate(ctypes.byref(cfg), mesh.nelements, _p(mesh.inverse_jacobian),
+                           _p(np.ascontiguousarray(u, dtype=np.float64)), _p(out))
+    return out
+
+
+def ck2n54_coefficients():
+    A, B, c = np.zeros(5), np.zeros(5), np.zeros(5)
+    lib().oracle_ck2n54_coefficients(_p(A), _p(B), _p(c))
+    return A, B, c
+
+
+def ck2n54_step(cfg, mesh, u, k, t, dt):
+    """Advance ``u`` (in place) by one CK2N54 step; ``k`` is the 2N register."""
+    du = np.zeros_like(u)
+    lib().oracle_ck2n54_step(ctypes.byref(cfg), mesh.nelements, mesh.ninterfaces, _p(mesh.neighbor_ids),
+                             _p(mesh.orientations), _p(mesh.inverse_jacobian), _p(u), _p(k), _p(du),
+                             float(t), float(dt))
+
+
+def set_threads(n):
+    return lib().oracle_set_threads(int(n))

@@ -1,0 +1,144 @@
+/*
+ * trixi_oracle.h -- CPU oracle for the DGSEM rhs! on a periodic TreeMesh.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is a C++ (C-ABI), FP64 restatement of the
+ * algorithm Trixi.jl runs on the CPU for the semidiscretization that
+ * TrixiCUDA.jl offloads (reference: src/solvers/dg_{1,2,3}d.jl and
+ * src/solvers/dg_{1,2,3}d_kernel.jl).  It exists to check the HIP library;
+ * it is never linked into, called by, or shipped as part of the product path.
+ * Only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg load it.
+ *
+ * Array conventions follow Trixi.jl (Julia, column-major, 1-based) mapped to
+ * 0-based C offsets:
+ *   u[v + nv*(i + N*(j + N*(k + N*e)))]          ("[v,i,j,k,e]")
+ *   node_coordinates[x + d*(i + N*(j + N*(k + N*e)))]
+ *   neighbor_ids[s + 2*iface]  s=0 left, s=1 right (0-based element ids)
+ *   orientations[iface] in {1,..,ndims}
+ *   matrices row-major: M[r*N + c] == Julia M[r+1, c+1]
+ */
+#ifndef TRIXI_ORACLE_H
+#define TRIXI_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { OR_EQ_LINADV = 0, OR_EQ_EULER = 1, OR_EQ_GLM_MHD = 2 };
+enum { OR_VI_WEAK = 0, OR_VI_FD = 1, OR_VI_HG = 2 };
+enum {
+    OR_FLUX_CENTRAL = 0,
+    OR_FLUX_LLF = 1,          /* FluxLaxFriedrichs(max_abs_speed)            */
+    OR_FLUX_LLF_NAIVE = 2,    /* FluxLaxFriedrichs(max_abs_speed_naive)      */
+    OR_FLUX_RANOCHA = 3,
+    OR_FLUX_SHIMA_ETAL = 4,
+    OR_FLUX_HINDENLANG_GASSNER = 5
+};
+enum {
+    OR_IC_CONSTANT = 0,
+    OR_IC_CONVERGENCE_TEST = 1, /* LinAdv: 1+0.5 sin(pi (x-a t)); MHD: Alfven wave */
+    OR_IC_WEAK_BLAST_WAVE = 2,
+    OR_IC_SEEDED_RANDOM = 3,    /* Euler: rho=1+0.1U, v=0.1N(0,1), p=1+0.05U     */
+    OR_IC_SMOOTH_PERIODIC = 4   /* Euler: seeded sums of periodic sines           */
+};
+
+typedef struct oracle_cfg {
+    int32_t ndims;            /* 1, 2, 3 */
+    int32_t equation;         /* OR_EQ_* */
+    int32_t polydeg;          /* N = polydeg + 1 nodes per direction */
+    int32_t volume_integral;  /* OR_VI_* */
+    int32_t volume_flux;      /* OR_FLUX_* (DG volume flux for FD and HG) */
+    int32_t surface_flux;     /* OR_FLUX_* */
+    int32_t fv_flux;          /* OR_FLUX_* (HG subcell flux) */
+    int32_t alpha_smooth;     /* HG smoothing on/off */
+    double gamma;
+    double c_h;               /* GLM cleaning speed */
+    double advection_velocity[3];
+    double alpha_max, alpha_min;
+    double ic_length;         /* domain period used by the smooth periodic IC */
+} oracle_cfg;
+
+int oracle_nvars(const oracle_cfg* cfg);
+
+/* ---- LGL basis (Trixi basis_lobatto_legendre.jl algorithms) -------------- */
+int oracle_basis(int nnodes, double* nodes, double* weights, double* D,
+                 double* Dsplit, double* Dhat, double* lhat_minus,
+                 double* lhat_plus, double* Vinv);
+int oracle_interpolation_matrix(int nin, const double* nodes_in, int nout,
+                                const double* nodes_out, double* Vdm);
+
+/* ---- ln_mean / inv_ln_mean (Trixi math.jl) -------------------------------- */
+double oracle_ln_mean(double x, double y);
+double oracle_inv_ln_mean(double x, double y);
+
+/* ---- point fluxes ---------------------------------------------------------- */
+void oracle_flux(const oracle_cfg* cfg, const double* u, int orientation, double* f);
+void oracle_two_point_flux(const oracle_cfg* cfg, int kind, const double* ull,
+                           const double* urr, int orientation, double* f);
+void oracle_noncons_flux(const oracle_cfg* cfg, const double* ull,
+                         const double* urr, int orientation, double* f);
+void oracle_max_abs_speeds(const oracle_cfg* cfg, const double* u, double* lam);
+double oracle_entropy(const oracle_cfg* cfg, const double* u);
+void oracle_cons2entropy(const oracle_cfg* cfg, const double* u, double* w);
+void oracle_entropy_potential(const oracle_cfg* cfg, const double* u,
+                              int orientation, double* psi);
+
+/* ---- uniform periodic TreeMesh (Trixi TreeMesh + init_elements/interfaces) */
+int64_t oracle_mesh_nelements(int ndims, int level);
+int oracle_mesh_uniform(int ndims, int level, double coord_min, double coord_max,
+                        int polydeg, int32_t* neighbor_ids, int32_t* orientations,
+                        double* inverse_jacobian, double* node_coordinates,
+                        int32_t* cell_index /* [ndims*E] integer cell coords */);
+
+/* ---- initial conditions ------------------------------------------------------ */
+void oracle_initial_condition(const oracle_cfg* cfg, int ic, const double* x,
+                              double t, uint64_t seed, uint64_t node_index,
+                              double* u);
+void oracle_compute_coefficients(const oracle_cfg* cfg, int ic, int64_t nelements,
+                                 const double* node_coordinates, double t,
+                                 uint64_t seed, double* u);
+
+/* ---- semidiscretization ------------------------------------------------------- */
+void oracle_rhs(const oracle_cfg* cfg, int64_t nelements, int64_t ninterfaces,
+                const int32_t* neighbor_ids, const int32_t* orientations,
+                const double* inverse_jacobian, const double* u, double* du,
+                double* alpha_out /* nullable, HG blending factors [E] */);
+void oracle_indicator_hg(const oracle_cfg* cfg, int64_t nelements,
+                         int64_t ninterfaces, const int32_t* neighbor_ids,
+                         const double* u, double* alpha);
+
+/* ---- callbacks ----------------------------------------------------------------- */
+double oracle_max_dt(const oracle_cfg* cfg, int64_t nelements,
+                     const double* inverse_jacobian, const double* u);
+void oracle_error_norms(const oracle_cfg* cfg, int ic, int64_t nelements,
+                        const double* inverse_jacobian,
+                        const double* node_coordinates, const double* u, double t,
+                        double total_volume, uint64_t seed, double* l2,
+                        double* linf);
+void oracle_integrate(const oracle_cfg* cfg, int64_t nelements,
+                      const double* inverse_jacobian, const double* u,
+                      double* integral /* [nvars] */);
+
+/* ---- time integration: CarpenterKennedy2N54 ------------------------------------- */
+void oracle_ck2n54_coefficients(double* A, double* B, double* c);
+/* One step of size dt; k is the 2N register (overwritten). */
+void oracle_ck2n54_step(const oracle_cfg* cfg, int64_t nelements,
+                        int64_t ninterfaces, const int32_t* neighbor_ids,
+                        const int32_t* orientations,
+                        const double* inverse_jacobian, double* u, double* k,
+                        double* du_scratch, double t, double dt);
+
+/* ---- forward-mode JVP of rhs! by dual numbers (checks for the adjoint) ---------- */
+void oracle_rhs_jvp(const oracle_cfg* cfg, int64_t nelements, int64_t ninterfaces,
+                    const int32_t* neighbor_ids, const int32_t* orientations,
+                    const double* inverse_jacobian, const double* u,
+                    const double* udot, double* du, double* dudot);
+
+/* ---- OpenMP control for the timed CPU baseline ---------------------------------- */
+int oracle_set_threads(int nthreads);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,185 @@
+"""Pins the oracle's point fluxes by properties the mathematics fixes.
+
+* ln_mean / inv_ln_mean against 50-digit mpmath values (tests/golden/ln_mean.json);
+* consistency f*(u,u) = f(u) and symmetry of every two-point flux;
+* Tadmor's entropy-conservation condition [[w]].f* = [[psi]] for flux_ranocha;
+* wave-speed estimates against eigenvalues of the primitive-variable
+  quasi-linear systems (Euler and ideal MHD), computed by numpy;
+* LLF = central - lambda/2 [[u]] with lambda at least the spectral radius.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def rng_state_euler(O, cfg, rng):
+    nd = cfg.ndims
+    rho = 1.0 + 0.5 * rng.random()
+    v = 0.5 * rng.standard_normal(nd)
+    p = 1.0 + 0.5 * rng.random()
+    u = np.zeros(nd + 2)
+    u[0] = rho
+    u[1:1 + nd] = rho * v
+    u[-1] = p / (cfg.gamma - 1) + 0.5 * rho * v @ v
+    return u
+
+
+def rng_state_mhd(O, cfg, rng):
+    rho = 1.0 + 0.5 * rng.random()
+    v = 0.5 * rng.standard_normal(3)
+    p = 1.0 + 0.5 * rng.random()
+    B = 0.7 * rng.standard_normal(3)
+    psi = 0.1 * rng.standard_normal()
+    u = np.zeros(9)
+    u[0] = rho
+    u[1:4] = rho * v
+    u[4] = p / (cfg.gamma - 1) + 0.5 * rho * v @ v + 0.5 * B @ B + 0.5 * psi * psi
+    u[5:8] = B
+    u[8] = psi
+    return u
+
+
+def test_ln_mean_against_mpmath(oracle):
+    with open(os.path.join(GOLDEN, "ln_mean.json")) as fh:
+        cases = json.load(fh)
+    for c in cases:
+        lm = oracle.ln_mean(c["x"], c["y"])
+        ilm = oracle.inv_ln_mean(c["x"], c["y"])
+        ref, iref = float(c["ln_mean"]), float(c["inv_ln_mean"])
+        # the closed-form branch loses digits as f^2 -> 1e-4 from above:
+        # |d log| ~ eps / (y/x - 1) ~ 1e-14 at the threshold
+        assert abs(lm - ref) <= 5e-14 * abs(ref), (c, lm)
+        assert abs(ilm - iref) <= 5e-14 * abs(iref), (c, ilm)
+        assert oracle.ln_mean(c["y"], c["x"]) == pytest.approx(ref, rel=5e-14)
+
+
+@pytest.mark.parametrize("nd", [1, 2, 3])
+@pytest.mark.parametrize("kind", ["RANOCHA", "SHIMA_ETAL", "LLF", "LLF_NAIVE", "CENTRAL"])
+def test_euler_two_point_consistency_and_symmetry(oracle, nd, kind):
+    O = oracle
+    cfg = O.make_cfg(nd, O.EQ_EULER, gamma=1.4)
+    k = getattr(O, "FLUX_" + kind)
+    rng = np.random.default_rng(1)
+    for _ in range(20):
+        ul, ur = rng_state_euler(O, cfg, rng), rng_state_euler(O, cfg, rng)
+        for o in range(1, nd + 1):
+            f = O.flux(cfg, ul, o)
+            np.testing.assert_allclose(O.two_point_flux(cfg, k, ul, ul, o), f, rtol=1e-14, atol=1e-14)
+            if kind not in ("LLF", "LLF_NAIVE"):
+                np.testing.assert_allclose(O.two_point_flux(cfg, k, ul, ur, o),
+                                           O.two_point_flux(cfg, k, ur, ul, o), rtol=1e-13, atol=1e-14)
+
+
+@pytest.mark.parametrize("nd", [1, 2, 3])
+def test_ranocha_is_entropy_conservative(oracle, nd):
+    O = oracle
+    cfg = O.make_cfg(nd, O.EQ_EULER, gamma=1.4)
+    rng = np.random.default_rng(2)
+    for _ in range(50):
+        ul, ur = rng_state_euler(O, cfg, rng), rng_state_euler(O, cfg, rng)
+        # also states close together (series branch of ln_mean)
+        if _ % 2:
+            ur = ul * (1 + 1e-4 * rng.standard_normal(len(ul)))
+        wl, wr = O.cons2entropy(cfg, ul), O.cons2entropy(cfg, ur)
+        for o in range(1, nd + 1):
+            f = O.two_point_flux(cfg, O.FLUX_RANOCHA, ul, ur, o)
+            lhs = (wr - wl) @ f
+            rhs = O.entropy_potential(cfg, ur, o) - O.entropy_potential(cfg, ul, o)
+            assert abs(lhs - rhs) <= 1e-13 * (1 + np.abs(wr - wl) @ np.abs(f)), (lhs, rhs)
+    # shima_etal is kinetic-energy preserving but not entropy conservative
+    ul, ur = rng_state_euler(O, cfg, rng), rng_state_euler(O, cfg, rng)
+    wl, wr = O.cons2entropy(cfg, ul), O.cons2entropy(cfg, ur)
+    f = O.two_point_flux(cfg, O.FLUX_SHIMA_ETAL, ul, ur, 1)
+    assert abs((wr - wl) @ f - (O.entropy_potential(cfg, ur, 1) - O.entropy_potential(cfg, ul, 1))) > 1e-8
+
+
+def euler_prim_matrix(rho, vn, p, gamma):
+    return np.array([[vn, rho, 0.0], [0.0, vn, 1.0 / rho], [0.0, gamma * p, vn]])
+
+
+def mhd_prim_matrix(rho, v, p, B, gamma, o):
+    # 1D ideal MHD in primitive variables (rho, v_n, v_t1, v_t2, B_t1, B_t2, p)
+    # along direction o with the normal field as a parameter.
+    t1, t2 = [d for d in range(3) if d != o]
+    vn, Bn = v[o], B[o]
+    A = np.zeros((7, 7))
+    A[0, 0], A[0, 1] = vn, rho
+    A[1, 1], A[1, 4], A[1, 5], A[1, 6] = vn, B[t1] / rho, B[t2] / rho, 1.0 / rho
+    A[2, 2], A[2, 4] = vn, -Bn / rho
+    A[3, 3], A[3, 5] = vn, -Bn / rho
+    A[4, 1], A[4, 2], A[4, 4] = B[t1], -Bn, vn
+    A[5, 1], A[5, 3], A[5, 5] = B[t2], -Bn, vn
+    A[6, 1], A[6, 6] = gamma * p, vn
+    return A
+
+
+def test_wave_speeds_match_eigenvalues(oracle):
+    O = oracle
+    rng = np.random.default_rng(3)
+    cfg = O.make_cfg(3, O.EQ_EULER, gamma=1.4)
+    for _ in range(20):
+        u = rng_state_euler(O, cfg, rng)
+        rho, v = u[0], u[1:4] / u[0]
+        p = 0.4 * (u[4] - 0.5 * rho * v @ v)
+        lam = O.max_abs_speeds(cfg, u)
+        for o in range(3):
+            ev = np.linalg.eigvals(euler_prim_matrix(rho, v[o], p, 1.4))
+            assert lam[o] == pytest.approx(np.abs(ev).max(), rel=1e-12)
+    cfgm = O.make_cfg(3, O.EQ_GLM_MHD, gamma=5.0 / 3.0, c_h=1.0)
+    for _ in range(20):
+        u = rng_state_mhd(O, cfgm, rng)
+        rho, v, B, psi = u[0], u[1:4] / u[0], u[5:8], u[8]
+        p = (2.0 / 3.0) * (u[4] - 0.5 * rho * v @ v - 0.5 * B @ B - 0.5 * psi**2)
+        lam = O.max_abs_speeds(cfgm, u)
+        for o in range(3):
+            ev = np.linalg.eigvals(mhd_prim_matrix(rho, v, p, B, 5.0 / 3.0, o))
+            assert lam[o] == pytest.approx(np.abs(ev).max(), rel=1e-9)
+
+
+@pytest.mark.parametrize("eq", ["EULER", "GLM_MHD"])
+def test_llf_structure(oracle, eq):
+    O = oracle
+    rng = np.random.default_rng(4)
+    if eq == "EULER":
+        cfg = O.make_cfg(3, O.EQ_EULER, gamma=1.4)
+        gen = rng_state_euler
+    else:
+        cfg = O.make_cfg(3, O.EQ_GLM_MHD, gamma=5.0 / 3.0, c_h=1.0)
+        gen = rng_state_mhd
+    for _ in range(20):
+        ul, ur = gen(O, cfg, rng), gen(O, cfg, rng)
+        for o in (1, 2, 3):
+            fc = O.two_point_flux(cfg, O.FLUX_CENTRAL, ul, ur, o)
+            sl, sr = O.max_abs_speeds(cfg, ul)[o - 1], O.max_abs_speeds(cfg, ur)[o - 1]
+            for kind, lam_expect in ((O.FLUX_LLF, max(sl, sr)), (O.FLUX_LLF_NAIVE, None)):
+                f = O.two_point_flux(cfg, kind, ul, ur, o)
+                d = (fc - f)
+                # dissipation is a multiple of the jump with lambda >= local spectral radius
+                lam = d @ (ur - ul) / ((ur - ul) @ (ur - ul)) * 2
+                np.testing.assert_allclose(d, 0.5 * lam * (ur - ul), rtol=1e-12, atol=1e-13)
+                assert lam >= max(sl, sr) * (1 - 1e-14)
+                if lam_expect is not None:
+                    assert lam == pytest.approx(lam_expect, rel=1e-13)
+
+
+def test_mhd_two_point_consistency_symmetry(oracle):
+    O = oracle
+    cfg = O.make_cfg(3, O.EQ_GLM_MHD, gamma=5.0 / 3.0, c_h=1.0)
+    rng = np.random.default_rng(5)
+    for _ in range(20):
+        ul, ur = rng_state_mhd(O, cfg, rng), rng_state_mhd(O, cfg, rng)
+        for o in (1, 2, 3):
+            np.testing.assert_allclose(O.two_point_flux(cfg, O.FLUX_HINDENLANG_GASSNER, ul, ul, o),
+                                       O.flux(cfg, ul, o), rtol=1e-13, atol=1e-13)
+            np.testing.assert_allclose(O.two_point_flux(cfg, O.FLUX_HINDENLANG_GASSNER, ul, ur, o),
+                                       O.two_point_flux(cfg, O.FLUX_HINDENLANG_GASSNER, ur, ul, o),
+                                       rtol=1e-13, atol=1e-13)
+            # Powell/GLM term is linear in the right state's normal field and psi
+            z = ur.copy()
+            z[5 + o - 1] = 0.0
+            z[8] = 0.0
+            np.testing.assert_allclose(O.noncons_flux(cfg, ul, z, o), 0.0, atol=0)

@@ -1,0 +1,235 @@
+"""Pins the oracle's full semidiscretization by invariants of the scheme.
+
+* free-stream preservation, conservation, entropy conservation (EC volume +
+  EC surface flux, Euler and GLM-MHD with the Powell term), entropy
+  dissipation with LLF;
+* pressure-equilibrium preservation of flux_shima_etal / flux_ranocha;
+* exactness for polynomial data of linear advection away from the periodic seam;
+* CK2N54 order of accuracy against the matrix exponential of the (linear)
+  semidiscretization, and the C1 trajectory against Trixi's published test
+  values (a recalled sanity check only, not part of the reference tree);
+* the HG indicator's special cases and smoothing;
+* the dual-number JVP against central finite differences.
+"""
+import numpy as np
+import pytest
+import scipy.linalg
+
+
+def entropy_rate(O, cfg, mesh, u, du):
+    nv = O.nvars(cfg)
+    w = np.apply_along_axis(lambda x: O.cons2entropy(cfg, x), -1, u.reshape(-1, nv)).reshape(u.shape)
+    wd = (w * du).sum(-1, keepdims=True)
+    return O.integrate(cfg, mesh, np.repeat(wd, nv, axis=-1))[0]
+
+
+@pytest.mark.parametrize("nd,level", [(1, 3), (2, 2), (3, 1)])
+@pytest.mark.parametrize("vi,vf", [("WEAK", "CENTRAL"), ("FD", "RANOCHA"), ("FD", "SHIMA_ETAL"), ("HG", "RANOCHA")])
+def test_euler_free_stream_and_conservation(oracle, nd, level, vi, vf):
+    O = oracle
+    cfg = O.make_cfg(nd, O.EQ_EULER, 3, getattr(O, "VI_" + vi), getattr(O, "FLUX_" + vf), O.FLUX_LLF)
+    mesh = O.Mesh(nd, level, -2.0, 2.0, 3)
+    u = O.compute_coefficients(cfg, mesh, O.IC_CONSTANT)
+    assert np.abs(O.rhs(cfg, mesh, u)).max() < 1e-13
+    u = O.compute_coefficients(cfg, mesh, O.IC_SEEDED_RANDOM)
+    du = O.rhs(cfg, mesh, u)
+    scale = np.abs(du).max() * mesh.total_volume
+    assert np.abs(O.integrate(cfg, mesh, du)).max() < 1e-13 * scale
+
+
+@pytest.mark.parametrize("nd,level", [(1, 3), (2, 2), (3, 1)])
+def test_euler_entropy_conservation_and_dissipation(oracle, nd, level):
+    O = oracle
+    mesh = O.Mesh(nd, level, -2.0, 2.0, 3)
+    cfg = O.make_cfg(nd, O.EQ_EULER, 3, O.VI_FD, O.FLUX_RANOCHA, O.FLUX_RANOCHA)
+    u = O.compute_coefficients(cfg, mesh, O.IC_SEEDED_RANDOM)
+    du = O.rhs(cfg, mesh, u)
+    assert abs(entropy_rate(O, cfg, mesh, u, du)) < 1e-13 * np.abs(du).max() * mesh.total_volume
+    cfg = O.make_cfg(nd, O.EQ_EULER, 3, O.VI_FD, O.FLUX_RANOCHA, O.FLUX_LLF)
+    du = O.rhs(cfg, mesh, u)
+    assert entropy_rate(O, cfg, mesh, u, du) < -1e-6
+
+
+def test_mhd_entropy_conservation_with_powell(oracle):
+    O = oracle
+    mesh = O.Mesh(3, 1, -1.0, 1.0, 3)
+    cfg = O.make_cfg(3, O.EQ_GLM_MHD, 3, O.VI_FD, O.FLUX_HINDENLANG_GASSNER, O.FLUX_HINDENLANG_GASSNER,
+                     gamma=5.0 / 3.0, c_h=1.0)
+    u = O.compute_coefficients(cfg, mesh, O.IC_SEEDED_RANDOM)
+    du = O.rhs(cfg, mesh, u)
+    assert abs(entropy_rate(O, cfg, mesh, u, du)) < 1e-12 * np.abs(du).max() * mesh.total_volume
+    cfg = O.make_cfg(3, O.EQ_GLM_MHD, 3, O.VI_FD, O.FLUX_HINDENLANG_GASSNER, O.FLUX_LLF,
+                     gamma=5.0 / 3.0, c_h=1.0)
+    du = O.rhs(cfg, mesh, u)
+    assert entropy_rate(O, cfg, mesh, u, du) < -1e-6
+    # free stream (constant state, constant field) is preserved
+    u = O.compute_coefficients(cfg, mesh, O.IC_CONSTANT)
+    assert np.abs(O.rhs(cfg, mesh, u)).max() < 1e-13
+
+
+@pytest.mark.parametrize("vf", ["RANOCHA", "SHIMA_ETAL"])
+def test_pressure_equilibrium_preservation(oracle, vf):
+    O = oracle
+    nd = 2
+    mesh = O.Mesh(nd, 2, -2.0, 2.0, 3)
+    k = getattr(O, "FLUX_" + vf)
+    cfg = O.make_cfg(nd, O.EQ_EULER, 3, O.VI_FD, k, k)
+    x = mesh.node_coordinates
+    rho = 1.0 + 0.3 * np.sin(np.pi * x[..., 0] / 2) * np.cos(np.pi * x[..., 1] / 2)
+    v = np.array([0.3, -0.2])
+    p = 1.0
+    u = np.stack([rho, rho * v[0], rho * v[1], p / 0.4 + 0.5 * rho * (v @ v)], axis=-1)
+    du = O.rhs(cfg, mesh, u)
+    # dp/dt = (gamma-1) (dE - v.dm + |v|^2/2 drho) must vanish
+    dp = 0.4 * (du[..., 3] - v[0] * du[..., 1] - v[1] * du[..., 2] + 0.5 * (v @ v) * du[..., 0])
+    assert np.abs(dp).max() < 1e-12 * np.abs(du).max()
+
+
+@pytest.mark.parametrize("nd,level", [(1, 3), (2, 2), (3, 2)])
+def test_linadv_polynomial_exactness(oracle, nd, level):
+    O = oracle
+    a = (1.0, -0.5, 0.25)[:nd]
+    cfg = O.make_cfg(nd, O.EQ_LINADV, 3, O.VI_WEAK, O.FLUX_CENTRAL, O.FLUX_LLF, advection_velocity=a)
+    mesh = O.Mesh(nd, level, -1.0, 1.0, 3)
+    x = mesh.node_coordinates
+    coef = [0.3, -0.7, 0.2]
+    u = 1.0 + sum(coef[d] * x[..., d] ** 3 + 0.1 * x[..., d] ** 2 for d in range(nd))
+    du = O.rhs(cfg, mesh, u[..., None])[..., 0]
+    exact = -sum(a[d] * (3 * coef[d] * x[..., d] ** 2 + 0.2 * x[..., d]) for d in range(nd))
+    n1 = 2**level
+    interior = np.all((mesh.cell_index > 0) & (mesh.cell_index < n1 - 1), axis=1)
+    assert interior.any()
+    np.testing.assert_allclose(du[interior], exact[interior], atol=1e-12)
+
+
+def test_c1_trajectory_matches_published_trixi_values(oracle):
+    # Trixi.jl's elixir_advection_basic.jl (1D, polydeg 3, level 4, CFL 1.6,
+    # t in [0, 1]) test values; recalled, used as a sanity check only.
+    O = oracle
+    cfg = O.make_cfg(1, O.EQ_LINADV, 3, O.VI_WEAK, surface_flux=O.FLUX_LLF, advection_velocity=(1.0,))
+    mesh = O.Mesh(1, 4, -1.0, 1.0, 3)
+    u = O.compute_coefficients(cfg, mesh, O.IC_CONVERGENCE_TEST)
+    dt = 1.6 * O.max_dt(cfg, mesh, u)
+    assert dt == pytest.approx(0.05, rel=1e-15)
+    k = np.zeros_like(u)
+    t = 0.0
+    for _ in range(20):
+        O.ck2n54_step(cfg, mesh, u, k, t, dt)
+        t += dt
+    l2, linf = O.error_norms(cfg, mesh, u, t, O.IC_CONVERGENCE_TEST)
+    assert l2[0] == pytest.approx(6.0388296447998465e-6, rel=1e-8)
+    assert linf[0] == pytest.approx(3.217887726258972e-5, rel=1e-8)
+
+
+def test_ck2n54_is_fourth_order(oracle):
+    O = oracle
+    cfg = O.make_cfg(1, O.EQ_LINADV, 3, O.VI_WEAK, surface_flux=O.FLUX_LLF, advection_velocity=(1.0,))
+    mesh = O.Mesh(1, 2, -1.0, 1.0, 3)
+    u0 = O.compute_coefficients(cfg, mesh, O.IC_CONVERGENCE_TEST)
+    n = u0.size
+    A = np.zeros((n, n))
+    for j in range(n):
+        e = np.zeros(n)
+        e[j] = 1.0
+        A[:, j] = O.rhs(cfg, mesh, e.reshape(u0.shape)).ravel()
+    T = 0.5
+    exact = scipy.linalg.expm(A * T) @ u0.ravel()
+    errs = []
+    for steps in (20, 40, 80):
+        u = u0.copy()
+        k = np.zeros_like(u)
+        dt = T / steps
+        for s in range(steps):
+            O.ck2n54_step(cfg, mesh, u, k, s * dt, dt)
+        errs.append(np.abs(u.ravel() - exact).max())
+    rates = np.log2(np.array(errs[:-1]) / np.array(errs[1:]))
+    assert np.all(rates > 3.8), rates
+
+
+def test_hg_indicator_special_cases(oracle):
+    O = oracle
+    cfg = O.make_cfg(2, O.EQ_EULER, 3, O.VI_HG, O.FLUX_RANOCHA, O.FLUX_LLF, O.FLUX_LLF,
+                     alpha_max=0.5, alpha_min=0.001, alpha_smooth=False)
+    mesh = O.Mesh(2, 2, -2.0, 2.0, 3)
+    u = O.compute_coefficients(cfg, mesh, O.IC_CONSTANT)
+    assert np.all(O.indicator_hg(cfg, mesh, u) == 0.0)
+    # put the highest Legendre mode (in x) into rho*p on element 5
+    b = O.basis(4)
+    V = np.linalg.inv(b["Vinv"])
+    mode = V[:, 3]
+    rho = u[..., 0].copy()
+    p = 1.0 + 0.0 * rho
+    rho[5] = 1.0 + 0.2 * mode[:, None] * np.ones((1, 4))  # varies in y only (axis 1 is j)
+    u2 = np.stack([rho, 0 * rho, 0 * rho, p / 0.4], axis=-1)
+    alpha = O.indicator_hg(cfg, mesh, u2)
+    assert alpha[5] == pytest.approx(0.5)
+    assert np.all(np.delete(alpha, 5) == 0.0)
+    # smoothing: neighbours get max(own, alpha/2)
+    cfg.alpha_smooth = 1
+    alpha_s = O.indicator_hg(cfg, mesh, u2)
+    nbrs = set()
+    for (l, r) in mesh.neighbor_ids:
+        if l == 5:
+            nbrs.add(r)
+        if r == 5:
+            nbrs.add(l)
+    for e in range(mesh.nelements):
+        expect = 0.5 if e == 5 else (0.25 if e in nbrs else 0.0)
+        assert alpha_s[e] == pytest.approx(expect)
+
+
+def test_hg_rhs_reduces_to_fd_and_fv(oracle):
+    O = oracle
+    mesh = O.Mesh(2, 2, -2.0, 2.0, 3)
+    cfg_fd = O.make_cfg(2, O.EQ_EULER, 3, O.VI_FD, O.FLUX_RANOCHA, O.FLUX_LLF)
+    cfg_hg = O.make_cfg(2, O.EQ_EULER, 3, O.VI_HG, O.FLUX_RANOCHA, O.FLUX_LLF, O.FLUX_LLF,
+                        alpha_max=0.0, alpha_min=0.001)
+    u = O.compute_coefficients(cfg_fd, mesh, O.IC_SEEDED_RANDOM)
+    # alpha_max = 0 -> pure DG
+    np.testing.assert_allclose(O.rhs(cfg_hg, mesh, u), O.rhs(cfg_fd, mesh, u), rtol=0, atol=1e-14)
+    # blend is affine in alpha: rhs(alpha) = (1-alpha) DG + alpha FV
+    cfg_hg.alpha_max = 1.0
+    cfg_hg.alpha_min = 0.0
+    u2 = O.compute_coefficients(cfg_fd, mesh, O.IC_WEAK_BLAST_WAVE)
+    du, alpha = O.rhs(cfg_hg, mesh, u2, return_alpha=True)
+    assert alpha.max() > 0
+    # a first-order FV volume term is still conservative
+    assert np.abs(O.integrate(cfg_hg, mesh, du)).max() < 1e-12 * np.abs(du).max() * mesh.total_volume
+
+
+def test_jvp_matches_central_differences(oracle):
+    O = oracle
+    mesh = O.Mesh(2, 2, -2.0, 2.0, 3)
+    cfg = O.make_cfg(2, O.EQ_EULER, 3, O.VI_FD, O.FLUX_RANOCHA, O.FLUX_LLF)
+    u = O.compute_coefficients(cfg, mesh, O.IC_SEEDED_RANDOM)
+    rng = np.random.default_rng(6)
+    w = rng.standard_normal(u.shape)
+    du, jv = O.rhs_jvp(cfg, mesh, u, w)
+    np.testing.assert_array_equal(du, O.rhs(cfg, mesh, u))
+    h = 1e-6
+    fd = (O.rhs(cfg, mesh, u + h * w) - O.rhs(cfg, mesh, u - h * w)) / (2 * h)
+    assert np.abs(jv - fd).max() < 1e-6 * np.abs(jv).max()
+
+
+def test_mesh_morton_order_and_connectivity(oracle):
+    O = oracle
+    for nd in (1, 2, 3):
+        mesh = O.Mesh(nd, 3, -2.0, 2.0, 3)
+        n1 = 8
+        E = mesh.nelements
+        ci = mesh.cell_index
+        # Morton: element id equals the bit interleave of the cell coordinates
+        code = np.zeros(E, dtype=np.int64)
+        for lvl in range(3):
+            for d in range(nd):
+                code |= ((ci[:, d] >> lvl) & 1).astype(np.int64) << (lvl * nd + d)
+        np.testing.assert_array_equal(code, np.arange(E))
+        # interfaces: 3 per element in +x, +y, +z order; right = left + e_d (periodic)
+        assert mesh.ninterfaces == nd * E
+        for i, ((l, r), o) in enumerate(zip(mesh.neighbor_ids, mesh.orientations)):
+            assert l == i // nd and o == i % nd + 1
+            expect = ci[l].copy()
+            expect[o - 1] = (expect[o - 1] + 1) % n1
+            np.testing.assert_array_equal(ci[r], expect)
+        dx = 4.0 / n1
+        np.testing.assert_allclose(mesh.inverse_jacobian, 2.0 / dx)
