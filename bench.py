@@ -1,0 +1,189 @@
+"""Benchmark of the MI355X DGSEM rhs! (one JSON line on stdout).
+
+Workload (default): 3D compressible Euler, polydeg 3, flux differencing with
+flux_ranocha, LLF surface flux, periodic TreeMesh [-2,2]^3 at refinement
+level 5 (32768 elements x 64 nodes x 5 variables), seeded smooth periodic
+state.  A step is one rhs! evaluation (u -> du) with u resident in HBM.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision mixed|fp64]
+
+`value` is DOF-updates/s of the mixed-precision rhs! (FP32 arithmetic, FP64
+state / log-means); the FP64 rhs! is reported beside it with its own
+roofline.  The CPU baseline is the FP64 oracle restatement (OpenMP) on a
+bounded sample of the same workload and state.
+"""
+import argparse
+import importlib.util
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+PKG_DIR = os.path.join(ROOT, "trixicuda.jl_amd")
+
+
+def load(name, path, pkg_dir=None):
+    if name in sys.modules:
+        return sys.modules[name]
+    spec = importlib.util.spec_from_file_location(name, path, submodule_search_locations=pkg_dir)
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules[name] = mod
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def make_semi(th, level, precision):
+    eq = th.CompressibleEulerEquations3D(1.4)
+    solver = th.DGSEMHIP(polydeg=3, surface_flux=th.flux_lax_friedrichs,
+                         volume_integral=th.VolumeIntegralFluxDifferencing(th.flux_ranocha), precision=precision)
+    mesh = th.TreeMesh((-2.0,) * 3, (2.0,) * 3, initial_refinement_level=level, n_cells_max=10**8)
+    return th.SemidiscretizationHyperbolicHIP(mesh, eq, th.initial_condition_smooth_periodic(20261202), solver)
+
+
+def time_rhs(torch, th, semi, u, du, steps, warmup, stream):
+    with torch.cuda.stream(stream):
+        for _ in range(warmup):
+            th.rhs_hip_(du, u, semi, 0.0)
+        torch.cuda.synchronize()
+        start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        t0 = time.perf_counter()
+        start.record(stream)
+        for _ in range(steps):
+            th.rhs_hip_(du, u, semi, 0.0)
+        stop.record(stream)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+    return start.elapsed_time(stop) / steps, wall / steps * 1e3
+
+
+def time_rk(torch, th, semi, u, steps, stream):
+    k = torch.zeros_like(u)
+    uu = u.clone()
+    with torch.cuda.stream(stream):
+        for i in range(3):
+            th.rk_step(uu, k, 0.0, 1e-4, semi, use_graph=True, stream=stream)
+        torch.cuda.synchronize()
+        start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        start.record(stream)
+        for i in range(steps):
+            th.rk_step(uu, k, 0.0, 1e-4, semi, use_graph=True, stream=stream)
+        stop.record(stream)
+        torch.cuda.synchronize()
+    return start.elapsed_time(stop) / steps
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--level", type=int, default=5)
+    ap.add_argument("--precision", choices=("mixed", "fp64"), default="mixed",
+                    help="mode reported as `value` (both are measured)")
+    ap.add_argument("--only", choices=("mixed", "fp64"), default=None, help="measure one mode only (profiling)")
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--rk-steps", type=int, default=20)
+    args = ap.parse_args()
+
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}, WORLD_SIZE {world}); "
+              "the multi-GPU path is not part of this build", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    th = load("trixi_hip", os.path.join(PKG_DIR, "__init__.py"), [PKG_DIR])
+    fm = load("trixi_hip_flop_model", os.path.join(PKG_DIR, "flop_model.py"))
+    if not torch.cuda.is_available():
+        raise SystemExit("bench.py needs a GPU")
+    key = "euler3d_fd_ranocha_llf"
+    flops_dof, bytes_dof = fm.FLOPS_PER_DOF[key], fm.BYTES_PER_DOF[key]
+
+    stream = torch.cuda.Stream()
+    modes = [args.only] if args.only else ["fp64", "mixed"]
+    res, du_out = {}, {}
+    u = None
+    for mode in modes:
+        semi = make_semi(th, args.level, mode)
+        if u is None:
+            u = th.compute_coefficients(0.0, semi)
+            torch.cuda.synchronize()
+        du = torch.empty_like(u)
+        ms, wall_ms = time_rhs(torch, th, semi, u, du, args.steps, args.warmup, stream)
+        dofs = semi.ndofs
+        secs = ms * 1e-3
+        peak = fm.PEAK["fp64_TFLOPs"] if mode == "fp64" else fm.PEAK["fp32_TFLOPs"]
+        achieved_tf = flops_dof * dofs / secs / 1e12
+        res[mode] = {
+            "value": dofs / secs, "ms_per_step": ms, "wall_ms_per_step": wall_ms,
+            "roofline": {"bound": "valu", "achieved": achieved_tf, "peak": peak, "unit": "TFLOP/s",
+                         "frac": achieved_tf / peak, "traffic": None,
+                         "hbm_GBps": bytes_dof * dofs / secs / 1e9,
+                         "hbm_frac": bytes_dof * dofs / secs / 1e9 / fm.PEAK["hbm_GBps"],
+                         "flop_per_dof": flops_dof, "bytes_per_dof": bytes_dof},
+        }
+        if args.rk_steps > 0:
+            res[mode]["rk_step_ms"] = time_rk(torch, th, semi, u, args.rk_steps, stream)
+        du_out[mode] = du.clone()
+        semi_keep = semi
+        if mode != modes[-1]:
+            semi.close()
+    torch.cuda.synchronize()
+
+    errors = {}
+    if "fp64" in du_out and "mixed" in du_out:
+        a, b = du_out["mixed"], du_out["fp64"]
+        errors["mixed_vs_fp64_rel_max"] = float((a - b).abs().max() / b.abs().max())
+        errors["mixed_vs_fp64_rel_l2"] = float(torch.linalg.norm(a - b) / torch.linalg.norm(b))
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+        threads = min(16, os.cpu_count() or 1)
+        O.set_threads(threads)
+        cfg = O.make_cfg(3, O.EQ_EULER, 3, O.VI_FD, O.FLUX_RANOCHA, O.FLUX_LLF, ic_length=4.0)
+        mesh = O.Mesh(3, args.level, -2.0, 2.0, 3)
+        u_np = th.wrap_array_native(u, semi_keep).cpu().numpy()
+        n, t0 = 0, time.perf_counter()
+        du_ref = None
+        while True:
+            du_ref = O.rhs(cfg, mesh, u_np)
+            n += 1
+            if time.perf_counter() - t0 >= args.cpu_seconds and n >= 2:
+                break
+        el = time.perf_counter() - t0
+        dofs = mesh.nelements * 64
+        cpu = {"value": dofs * n / el, "unit": "DOF-updates/s", "cores": threads, "kind": "port",
+               "sample": f"{n} x rhs! of the full workload state ({dofs} DOF), FP64 C++ oracle, OpenMP"}
+        for mode, d in du_out.items():
+            g = th.wrap_array_native(d, semi_keep).cpu().numpy()
+            errors[f"{mode}_vs_oracle_rel_max"] = float(np.abs(g - du_ref).max() / np.abs(du_ref).max())
+            errors[f"{mode}_vs_oracle_rel_l2"] = float(np.linalg.norm(g - du_ref) / np.linalg.norm(du_ref))
+
+    head = args.precision if args.precision in res else modes[0]
+    r = res[head]
+    line = {
+        "metric": "rhs! DOF-updates/s", "value": r["value"], "unit": "DOF-updates/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": r["ms_per_step"], "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32" if head == "mixed" else "f64",
+        "data": "synthetic (seeded smooth periodic state)",
+        "config": {"workload": f"3D Euler FD flux_ranocha + LLF, polydeg 3, TreeMesh [-2,2]^3 level {args.level} "
+                               f"({8 ** args.level} elements, {8 ** args.level * 64} DOF, 5 vars), one rhs!",
+                   "precision": head, "parallelism": "single GPU"},
+        "roofline": r["roofline"],
+        "cpu_baseline": cpu,
+        "modes": res,
+        "errors": errors,
+    }
+    if "fp64" in res and "mixed" in res:
+        line["mixed_over_fp64"] = res["mixed"]["value"] / res["fp64"]["value"]
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

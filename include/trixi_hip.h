@@ -1,0 +1,204 @@
+/*
+ * trixi_hip.h -- C ABI of libtrixihip.so, the gfx950 (AMD Instinct MI355X)
+ * DGSEM semidiscretization for Trixi.jl's TreeMesh.
+ *
+ * This library is the drop-in replacement for the GPU side of TrixiCUDA.jl
+ * (reference: src/solvers/solvers.jl rhs_gpu!, src/solvers/dg_{1,2,3}d.jl,
+ * src/solvers/dg_{1,2,3}d_kernel.jl and src/callbacks_step/).  A host binding
+ * (Julia ccall, Python ctypes) maps Trixi's equation/flux/initial-condition
+ * objects to the enums below and passes plain device pointers.
+ *
+ * Conventions
+ *   - Every function returns 0 on success and a negative TRIXI_HIP_E* code on
+ *     failure; trixi_hip_last_error() then holds a message (thread-local).
+ *     No C++ exception crosses this boundary.
+ *   - State vectors live in device memory (HBM) in the library's native
+ *     layout, element innermost:
+ *         u[(v * N^d + node) * nelements + element],
+ *         node = i + N*j + N^2*k  (i = x index, 0-based),
+ *     with elements in Trixi's TreeMesh leaf (Morton) order.  trixi_hip_pack /
+ *     trixi_hip_unpack convert from/to Trixi's u[v, i, j, k, element] layout.
+ *   - All device work is asynchronous on the caller's stream unless stated.
+ *   - A context is not thread safe; the caller serialises calls on it.
+ *   - Every device buffer a context needs is allocated in trixi_hip_create;
+ *     no call after create allocates device memory.
+ *   - Supported: periodic uniform TreeMesh (no boundaries, no mortars), the
+ *     reference asserts the same on its GPU path (dg_3d.jl:463-466, 542-544).
+ */
+#ifndef TRIXI_HIP_H
+#define TRIXI_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct trixi_hip_ctx trixi_hip_ctx;
+typedef void* trixi_hip_stream; /* a hipStream_t; NULL = the default stream */
+
+/* ---- error codes ----------------------------------------------------------- */
+#define TRIXI_HIP_OK 0
+#define TRIXI_HIP_EINVAL (-1)       /* bad argument / handle / size mismatch     */
+#define TRIXI_HIP_EUNSUPPORTED (-2) /* combination of equation/fluxes/N not built */
+#define TRIXI_HIP_EDEVICE (-3)      /* HIP runtime error                          */
+#define TRIXI_HIP_ENOMEM (-4)       /* device allocation failed                   */
+#define TRIXI_HIP_ESTATE (-5)       /* non-finite or non-positive state found     */
+
+/* ---- enums (values shared with the host bindings) ----------------------------- */
+enum trixi_hip_equation {
+    TRIXI_HIP_LINEAR_SCALAR_ADVECTION = 0, /* LinearScalarAdvectionEquation{1,2,3}D */
+    TRIXI_HIP_COMPRESSIBLE_EULER = 1,      /* CompressibleEulerEquations{1,2,3}D    */
+    TRIXI_HIP_IDEAL_GLM_MHD = 2            /* IdealGlmMhdEquations3D                */
+};
+enum trixi_hip_volume_integral {
+    TRIXI_HIP_VOLUME_WEAK_FORM = 0,          /* VolumeIntegralWeakForm          */
+    TRIXI_HIP_VOLUME_FLUX_DIFFERENCING = 1,  /* VolumeIntegralFluxDifferencing  */
+    TRIXI_HIP_VOLUME_SHOCK_CAPTURING_HG = 2  /* VolumeIntegralShockCapturingHG  */
+};
+enum trixi_hip_flux {
+    TRIXI_HIP_FLUX_CENTRAL = 0,
+    TRIXI_HIP_FLUX_LAX_FRIEDRICHS = 1,       /* FluxLaxFriedrichs(max_abs_speed)       */
+    TRIXI_HIP_FLUX_LAX_FRIEDRICHS_NAIVE = 2, /* FluxLaxFriedrichs(max_abs_speed_naive) */
+    TRIXI_HIP_FLUX_RANOCHA = 3,
+    TRIXI_HIP_FLUX_SHIMA_ETAL = 4,
+    TRIXI_HIP_FLUX_HINDENLANG_GASSNER = 5    /* with flux_nonconservative_powell       */
+};
+enum trixi_hip_precision {
+    TRIXI_HIP_FP64 = 0,  /* all arithmetic in FP64                                    */
+    TRIXI_HIP_MIXED = 1  /* FP32 volume/surface arithmetic; FP64 state, log-means,
+                            RK update, shock indicator and reductions                */
+};
+enum trixi_hip_initial_condition {
+    TRIXI_HIP_IC_CONSTANT = 0,
+    TRIXI_HIP_IC_CONVERGENCE_TEST = 1, /* LinAdv sine wave / GLM-MHD Alfven wave        */
+    TRIXI_HIP_IC_WEAK_BLAST_WAVE = 2,
+    TRIXI_HIP_IC_SEEDED_RANDOM = 3,
+    TRIXI_HIP_IC_SMOOTH_PERIODIC = 4
+};
+
+/* Solver configuration: DGSEMGPU(; polydeg, surface_flux, volume_integral)
+ * (reference src/solvers/dgsem_gpu.jl:43-53) plus the equation parameters. */
+typedef struct trixi_hip_config {
+    int32_t ndims;            /* 1, 2 or 3                                        */
+    int32_t equation;         /* enum trixi_hip_equation                          */
+    int32_t polydeg;          /* N - 1; 3 for every headline configuration        */
+    int32_t volume_integral;  /* enum trixi_hip_volume_integral                   */
+    int32_t volume_flux;      /* DG volume flux (flux differencing / HG)          */
+    int32_t surface_flux;     /* numerical interface flux                         */
+    int32_t fv_flux;          /* HG subcell finite-volume flux                    */
+    int32_t precision;        /* enum trixi_hip_precision                         */
+    int32_t alpha_smooth;     /* HG: smooth blending factors with neighbours      */
+    int32_t reserved;
+    double gamma;             /* ratio of specific heats (Euler, MHD)             */
+    double c_h;               /* GLM divergence-cleaning speed                    */
+    double advection_velocity[3];
+    double alpha_max, alpha_min; /* HG indicator parameters                       */
+    double ic_length;         /* period used by TRIXI_HIP_IC_SMOOTH_PERIODIC      */
+} trixi_hip_config;
+
+/* Mesh and connectivity (host arrays, copied to the device by create): the
+ * cache the reference builds in create_cache_gpu (src/solvers/cache.jl:130-158,
+ * containers_3d.jl:6-96). */
+typedef struct trixi_hip_mesh {
+    int64_t nelements;
+    int64_t ninterfaces;
+    const int32_t* neighbor_ids;     /* [2*ninterfaces]: (left, right), 0-based        */
+    const int32_t* orientations;     /* [ninterfaces]: 1..ndims                        */
+    const double* inverse_jacobian;  /* [nelements]                                    */
+    const double* node_coordinates;  /* Trixi layout [d, i, j, k, element]; may be NULL */
+    double total_volume;
+} trixi_hip_mesh;
+
+/* Library version string. */
+const char* trixi_hip_version(void);
+
+/* Message of the last failed call on this thread. */
+const char* trixi_hip_last_error(void);
+
+/* Lobatto-Legendre basis of `nnodes` nodes, computed on the host (no device
+ * needed): the arrays of LobattoLegendreBasisGPU (basis_lobatto_legendre.jl:
+ * 60-100).  Matrices are row-major N x N; boundary_interpolation is N x 2
+ * (row-major: [j][0] = lhat_j(-1), [j][1] = lhat_j(+1)).  Any pointer may be
+ * NULL. */
+int trixi_hip_basis(int32_t nnodes, double* nodes, double* weights, double* derivative_matrix,
+                    double* derivative_split, double* derivative_dhat,
+                    double* boundary_interpolation, double* inverse_vandermonde_legendre);
+
+/* Build a context: basis, device copies of the mesh, all scratch buffers.
+ * Replaces LobattoLegendreBasisGPU + DGSEMGPU + SemidiscretizationHyperbolicGPU
+ * + create_cache_gpu (basis_lobatto_legendre.jl:60-100, dgsem_gpu.jl:43-53,
+ * semidiscretization_hyperbolic.jl:60-87, cache.jl:130-212).  Returns
+ * TRIXI_HIP_EUNSUPPORTED for combinations that are not built. */
+int trixi_hip_create(const trixi_hip_config* config, const trixi_hip_mesh* mesh,
+                     trixi_hip_ctx** out);
+int trixi_hip_destroy(trixi_hip_ctx* ctx);
+
+/* Sizes: nvars, nnodes per direction, values per state vector (nv*N^d*E). */
+int trixi_hip_sizes(const trixi_hip_ctx* ctx, int32_t* nvars, int32_t* nnodes,
+                    int64_t* nvalues);
+
+/* du = rhs(u, t), in place into du, asynchronous on `stream`.
+ * Replaces rhs_gpu!(du_ode, u_ode, semi, t) (solvers.jl:18-31) and the stage
+ * pipeline rhs_gpu!(du, u, t, mesh::TreeMesh{d}, ...) (dg_1d.jl:523-548,
+ * dg_2d.jl:760-790, dg_3d.jl:895-925).  u and du are device pointers in the
+ * native layout; they must not alias. */
+int trixi_hip_rhs(trixi_hip_ctx* ctx, double* du, const double* u, double t,
+                  trixi_hip_stream stream);
+
+/* Layout conversion, device to device: Trixi u[v,i,j,k,e] <-> native.
+ * Stands in for wrap_array / wrap_array_native (dg.jl:14-32). */
+int trixi_hip_pack(trixi_hip_ctx* ctx, double* u_native, const double* u_trixi,
+                   trixi_hip_stream stream);
+int trixi_hip_unpack(trixi_hip_ctx* ctx, double* u_trixi, const double* u_native,
+                     trixi_hip_stream stream);
+
+/* u = initial_condition(x_node, t) at every node (native layout), incl. the
+ * 1D end-node nudge of compute_coefficients_kernel! (dg.jl:46-68, 148-160). */
+int trixi_hip_compute_coefficients(trixi_hip_ctx* ctx, double* u, int32_t ic,
+                                   double t, uint64_t seed, trixi_hip_stream stream);
+
+/* One CarpenterKennedy2N54 step of size dt from time t (5 fused rhs + FP64
+ * register-update launches): u <- u(t+dt), k is the 2N register.  With
+ * use_graph != 0 the five stages are captured once into a hipGraph per
+ * (u, k) pair and replayed; t and dt are passed through device memory.
+ * Replaces the OrdinaryDiffEq stage loop around rhs_gpu! used by the
+ * reference examples (e.g. examples/euler_ec_3d.jl:53-55). */
+int trixi_hip_rk_step(trixi_hip_ctx* ctx, double* u, double* k, double t, double dt,
+                      int32_t use_graph, trixi_hip_stream stream);
+
+/* max_dt = 2 / (N * max_e invJ_e * sum_d lambda_d) on the device; blocks until
+ * the scalar is on the host.  Replaces max_dt(u::CuArray, ...)
+ * (stepsize_dg_{1,2,3}d.jl, e.g. stepsize_dg_3d.jl:20-45). */
+int trixi_hip_max_dt(trixi_hip_ctx* ctx, const double* u, double* max_dt,
+                     trixi_hip_stream stream);
+
+/* L2 and Linf errors against the initial condition at time t (2N-node
+ * analysis basis); blocks.  Replaces calc_error_norms
+ * (analysis_dg_{1,2,3}d.jl, e.g. analysis_dg_3d.jl:45-89). */
+int trixi_hip_error_norms(trixi_hip_ctx* ctx, const double* u, int32_t ic, double t,
+                          uint64_t seed, double* l2, double* linf,
+                          trixi_hip_stream stream);
+
+/* Quadrature integral of every conservative variable over the domain; blocks.
+ * Replaces integrate / integrate_via_indices (analysis_dg_3d.jl:1-43). */
+int trixi_hip_integrate(trixi_hip_ctx* ctx, const double* u, double* integral,
+                        trixi_hip_stream stream);
+
+/* Hennemann-Gassner blending factors (after smoothing) into the device array
+ * alpha[nelements], in FP64.  Replaces the CPU indicator callable and its
+ * D2H/H2D round trip (indicators.jl:7-42, dg_2d.jl:171-173). */
+int trixi_hip_indicator(trixi_hip_ctx* ctx, const double* u, double* alpha,
+                        trixi_hip_stream stream);
+
+/* Positivity and finiteness check (FP64): writes a per-element flag
+ * (bit 0: non-finite value, bit 1: rho <= 0, bit 2: p <= 0) into
+ * flags[nelements] and the number of flagged elements to *nbad (blocks).
+ * Returns TRIXI_HIP_ESTATE if any element is flagged. */
+int trixi_hip_check_state(trixi_hip_ctx* ctx, const double* u, int32_t* flags,
+                          int64_t* nbad, trixi_hip_stream stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* TRIXI_HIP_H */

@@ -1,0 +1,88 @@
+"""Shared case definitions: one description drives both the HIP path (via the
+host package) and the oracle, so parity tests compare like with like."""
+from dataclasses import dataclass, field
+
+import numpy as np
+
+
+@dataclass
+class Case:
+    name: str
+    ndims: int
+    equation: str           # "linadv" | "euler" | "mhd"
+    level: int
+    vi: str = "weak"        # "weak" | "fd" | "hg"
+    vflux: str = "central"
+    sflux: str = "llf"
+    fvflux: str = "llf"
+    polydeg: int = 3
+    cmin: float = -2.0
+    cmax: float = 2.0
+    ic: str = "seeded_random"
+    gamma: float = 1.4
+    c_h: float = 1.0
+    advection: tuple = (1.0, 0.5, 0.25)
+    alpha_max: float = 0.5
+    alpha_min: float = 0.001
+    alpha_smooth: bool = True
+    precision: str = "fp64"
+    seed: int = 20261202
+
+
+FLUX_NAMES = ("central", "llf", "llf_naive", "ranocha", "shima_etal", "hindenlang_gassner")
+IC_NAMES = ("constant", "convergence_test", "weak_blast_wave", "seeded_random", "smooth_periodic")
+
+
+def oracle_cfg(O, c: Case):
+    eq = {"linadv": O.EQ_LINADV, "euler": O.EQ_EULER, "mhd": O.EQ_GLM_MHD}[c.equation]
+    vi = {"weak": O.VI_WEAK, "fd": O.VI_FD, "hg": O.VI_HG}[c.vi]
+    return O.make_cfg(c.ndims, eq, c.polydeg, vi, FLUX_NAMES.index(c.vflux), FLUX_NAMES.index(c.sflux),
+                      FLUX_NAMES.index(c.fvflux), gamma=c.gamma, c_h=c.c_h,
+                      advection_velocity=c.advection[:c.ndims], alpha_max=c.alpha_max, alpha_min=c.alpha_min,
+                      alpha_smooth=c.alpha_smooth, ic_length=c.cmax - c.cmin)
+
+
+def oracle_ic(O, c: Case):
+    return IC_NAMES.index(c.ic)
+
+
+def _flux(th, name, mhd):
+    f = {"central": th.flux_central, "llf": th.flux_lax_friedrichs,
+         "llf_naive": th.FluxLaxFriedrichs(th.max_abs_speed_naive), "ranocha": th.flux_ranocha,
+         "shima_etal": th.flux_shima_etal, "hindenlang_gassner": th.flux_hindenlang_gassner}[name]
+    return (f, th.flux_nonconservative_powell) if mhd else f
+
+
+def product_semi(th, c: Case, precision=None):
+    mhd = c.equation == "mhd"
+    if c.equation == "linadv":
+        eq = [None, th.LinearScalarAdvectionEquation1D, th.LinearScalarAdvectionEquation2D,
+              th.LinearScalarAdvectionEquation3D][c.ndims](c.advection[:c.ndims])
+    elif c.equation == "euler":
+        eq = [None, th.CompressibleEulerEquations1D, th.CompressibleEulerEquations2D,
+              th.CompressibleEulerEquations3D][c.ndims](c.gamma)
+    else:
+        eq = th.IdealGlmMhdEquations3D(c.gamma, c.c_h)
+    basis = th.LobattoLegendreBasisHIP(c.polydeg)
+    if c.vi == "weak":
+        vi = th.VolumeIntegralWeakForm()
+    elif c.vi == "fd":
+        vi = th.VolumeIntegralFluxDifferencing(_flux(th, c.vflux, mhd))
+    else:
+        ind = th.IndicatorHennemannGassner(eq, basis, alpha_max=c.alpha_max, alpha_min=c.alpha_min,
+                                           alpha_smooth=c.alpha_smooth, variable=th.density_pressure)
+        vi = th.VolumeIntegralShockCapturingHG(ind, volume_flux_dg=_flux(th, c.vflux, mhd),
+                                               volume_flux_fv=_flux(th, c.fvflux, mhd))
+    solver = th.DGSEMHIP(polydeg=c.polydeg, surface_flux=_flux(th, c.sflux, mhd), volume_integral=vi,
+                         precision=precision or c.precision)
+    mesh = th.TreeMesh((c.cmin,) * c.ndims, (c.cmax,) * c.ndims, initial_refinement_level=c.level,
+                       n_cells_max=10**8)
+    ic = {"constant": th.initial_condition_constant, "convergence_test": th.initial_condition_convergence_test,
+          "weak_blast_wave": th.initial_condition_weak_blast_wave,
+          "seeded_random": th.initial_condition_seeded_random(c.seed),
+          "smooth_periodic": th.initial_condition_smooth_periodic(c.seed)}[c.ic]
+    return th.SemidiscretizationHyperbolicHIP(mesh, eq, ic, solver)
+
+
+def rel_max(a, b):
+    return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))

@@ -1,0 +1,240 @@
+"""GPU parity: the HIP library against the CPU oracle on identical inputs.
+
+Bar (FP64): relative max-norm ||du_gpu - du_oracle||_inf / ||du_oracle||_inf
+<= 1e-12; error norms to 1e-10; integer/layout work bit-exact.  Mixed mode:
+relative max-norm of du <= 2e-5 against the FP64 oracle (stated per case).
+"""
+import numpy as np
+import pytest
+
+from cases import Case, oracle_cfg, oracle_ic, product_semi, rel_max
+
+pytestmark = pytest.mark.gpu
+
+FP64_TOL = 1e-12
+MIXED_TOL = 2e-5
+
+
+@pytest.fixture(scope="module")
+def torch():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    return torch
+
+
+def run_pair(th, O, torch, case, precision=None):
+    """(du_gpu in Trixi layout, du_oracle, u) for one case."""
+    cfg = oracle_cfg(O, case)
+    mesh = O.Mesh(case.ndims, case.level, case.cmin, case.cmax, case.polydeg)
+    u = O.compute_coefficients(cfg, mesh, oracle_ic(O, case), 0.0, case.seed)
+    du_ref = O.rhs(cfg, mesh, u)
+    semi = product_semi(th, case, precision)
+    u_dev = th.pack(torch.from_numpy(u).cuda(), semi)
+    du_dev = torch.empty_like(u_dev)
+    th.rhs_hip_(du_dev, u_dev, semi, 0.0)
+    du = th.wrap_array_native(du_dev, semi).cpu().numpy()
+    torch.cuda.synchronize()
+    semi.close()
+    return du, du_ref, u
+
+
+CASES = [
+    Case("linadv1d_weak_llf", 1, "linadv", 4, cmin=-1.0, cmax=1.0, ic="convergence_test"),
+    Case("linadv2d_weak_llf", 2, "linadv", 3, ic="seeded_random"),
+    Case("linadv3d_weak_central", 3, "linadv", 2, sflux="central", ic="seeded_random"),
+    Case("linadv1d_weak_llf_p1", 1, "linadv", 4, polydeg=1),
+    Case("linadv2d_weak_llf_p4", 2, "linadv", 2, polydeg=4),
+    Case("euler1d_weak_llf", 1, "euler", 4),
+    Case("euler2d_weak_llf", 2, "euler", 3),
+    Case("euler3d_weak_llfnaive", 3, "euler", 2, sflux="llf_naive"),
+    Case("euler1d_fd_ranocha_llf", 1, "euler", 5, "fd", "ranocha"),
+    Case("euler2d_fd_ranocha_llf", 2, "euler", 5, "fd", "ranocha"),      # C2 shape
+    Case("euler2d_fd_ranocha_ranocha", 2, "euler", 3, "fd", "ranocha", "ranocha"),
+    Case("euler2d_fd_shima_shima", 2, "euler", 3, "fd", "shima_etal", "shima_etal"),
+    Case("euler2d_fd_shima_llfnaive", 2, "euler", 3, "fd", "shima_etal", "llf_naive"),
+    Case("euler3d_fd_ranocha_llf", 3, "euler", 3, "fd", "ranocha", ic="smooth_periodic"),
+    Case("euler3d_fd_ranocha_llf_random", 3, "euler", 2, "fd", "ranocha"),
+    Case("euler3d_fd_ranocha_llfnaive", 3, "euler", 2, "fd", "ranocha", "llf_naive"),
+    Case("euler3d_fd_shima_llf", 3, "euler", 2, "fd", "shima_etal"),
+    Case("euler2d_fd_ranocha_llf_p1", 2, "euler", 3, "fd", "ranocha", polydeg=1),
+    Case("euler3d_fd_ranocha_llf_p2", 3, "euler", 2, "fd", "ranocha", polydeg=2),
+    Case("mhd3d_fd_hg_llf_alfven", 3, "mhd", 2, "fd", "hindenlang_gassner", cmin=-1.0, cmax=1.0,
+         ic="convergence_test", gamma=5 / 3),
+    Case("mhd3d_fd_hg_hg_random", 3, "mhd", 1, "fd", "hindenlang_gassner", "hindenlang_gassner", cmin=-1.0,
+         cmax=1.0, gamma=5 / 3),
+    Case("mhd3d_fd_hg_llfnaive", 3, "mhd", 1, "fd", "hindenlang_gassner", "llf_naive", cmin=-1.0, cmax=1.0,
+         gamma=5 / 3),
+    Case("euler2d_hg_ranocha_blast", 2, "euler", 4, "hg", "ranocha", ic="weak_blast_wave"),
+    Case("euler2d_hg_shima_blast_nosmooth", 2, "euler", 4, "hg", "shima_etal", ic="weak_blast_wave",
+         alpha_smooth=False),
+    Case("euler2d_hg_ranocha_random", 2, "euler", 3, "hg", "ranocha", alpha_min=0.0),
+    Case("euler1d_hg_ranocha_blast", 1, "euler", 5, "hg", "ranocha", ic="weak_blast_wave"),
+    Case("euler3d_hg_ranocha_blast", 3, "euler", 3, "hg", "ranocha", ic="weak_blast_wave"),
+    Case("euler2d_hg_ranocha_llfnaive", 2, "euler", 4, "hg", "ranocha", "llf_naive", "llf_naive",
+         ic="weak_blast_wave"),
+]
+
+
+@pytest.mark.parametrize("case", CASES, ids=lambda c: c.name)
+def test_rhs_fp64_matches_oracle(th, oracle, torch, case):
+    du, du_ref, _ = run_pair(th, oracle, torch, case)
+    err = rel_max(du, du_ref)
+    print(f"{case.name}: rel max-norm {err:.3e}", flush=True)
+    assert np.all(np.isfinite(du))
+    assert err <= FP64_TOL, err
+
+
+MIXED_CASES = [c for c in CASES if c.vi in ("fd", "hg") and c.polydeg == 3] + \
+    [Case("euler2d_weak_llf_mixed", 2, "euler", 3), Case("linadv1d_mixed", 1, "linadv", 4, cmin=-1.0, cmax=1.0,
+                                                        ic="convergence_test")]
+
+
+@pytest.mark.parametrize("case", MIXED_CASES, ids=lambda c: c.name)
+def test_rhs_mixed_within_stated_error(th, oracle, torch, case):
+    du, du_ref, _ = run_pair(th, oracle, torch, case, precision="mixed")
+    err = rel_max(du, du_ref)
+    l2 = float(np.sqrt(np.mean((du - du_ref) ** 2)) / np.sqrt(np.mean(du_ref ** 2)))
+    print(f"{case.name}: mixed rel max-norm {err:.3e}, rel L2 {l2:.3e}", flush=True)
+    assert err <= MIXED_TOL, err
+
+
+def test_pack_unpack_roundtrip_bit_exact(th, oracle, torch):
+    case = Case("euler3d", 3, "euler", 2, "fd", "ranocha")
+    semi = product_semi(th, case)
+    x = torch.randn(semi.trixi_shape, dtype=torch.float64, device="cuda")
+    nat = th.pack(x, semi)
+    back = th.wrap_array_native(nat, semi)
+    assert torch.equal(back, x)
+    # native layout: u[(v*N^d + node)*E + e] == trixi[e, k, j, i, v]
+    xn = x.cpu().numpy()
+    nn = nat.cpu().numpy().reshape(semi.native_shape)
+    E, N, nv = semi.nelements, semi.nnodes, semi.nvariables
+    assert nn[2, 1 + N * 2 + N * N * 3, 7] == xn[7, 3, 2, 1, 2]
+    semi.close()
+
+
+@pytest.mark.parametrize("case", [Case("linadv1d", 1, "linadv", 4, cmin=-1.0, cmax=1.0, ic="convergence_test"),
+                                  Case("euler2d_random", 2, "euler", 3),
+                                  Case("euler2d_blast", 2, "euler", 3, ic="weak_blast_wave"),
+                                  Case("euler3d_smooth", 3, "euler", 2, ic="smooth_periodic"),
+                                  Case("mhd3d_alfven", 3, "mhd", 2, "fd", "hindenlang_gassner", cmin=-1.0,
+                                       cmax=1.0, ic="convergence_test", gamma=5 / 3)],
+                         ids=lambda c: c.name)
+def test_initial_condition_and_callbacks_match_oracle(th, oracle, torch, case):
+    O = oracle
+    cfg = oracle_cfg(O, case)
+    mesh = O.Mesh(case.ndims, case.level, case.cmin, case.cmax, case.polydeg)
+    u_ref = O.compute_coefficients(cfg, mesh, oracle_ic(O, case), 0.0, case.seed)
+    semi = product_semi(th, case)
+    u = th.compute_coefficients(0.0, semi)
+    u_np = th.wrap_array_native(u, semi).cpu().numpy()
+    np.testing.assert_allclose(u_np, u_ref, rtol=1e-13, atol=1e-14)
+    # max_dt
+    assert th.max_dt(u, semi) == pytest.approx(O.max_dt(cfg, mesh, u_ref), rel=1e-13)
+    # integrals
+    np.testing.assert_allclose(th.integrate(u, semi), O.integrate(cfg, mesh, u_ref), rtol=1e-13, atol=1e-13)
+    # error norms at a later time (nonzero errors)
+    t = 0.1
+    l2, linf = th.calc_error_norms(u, t, semi)
+    l2r, linfr = O.error_norms(cfg, mesh, u_ref, t, oracle_ic(O, case), case.seed)
+    # errors that vanish in exact arithmetic (e.g. the constant Alfven-wave
+    # density) are compared at absolute round-off
+    np.testing.assert_allclose(l2, l2r, rtol=1e-10, atol=1e-13)
+    np.testing.assert_allclose(linf, linfr, rtol=1e-10, atol=1e-13)
+    semi.close()
+
+
+def test_indicator_matches_oracle(th, oracle, torch):
+    O = oracle
+    case = Case("euler2d_hg", 2, "euler", 5, "hg", "ranocha", ic="weak_blast_wave")
+    cfg = oracle_cfg(O, case)
+    mesh = O.Mesh(2, 5, -2.0, 2.0, 3)
+    u = O.compute_coefficients(cfg, mesh, oracle_ic(O, case))
+    alpha_ref = O.indicator_hg(cfg, mesh, u)
+    semi = product_semi(th, case)
+    alpha = th.blending_factors(th.pack(torch.from_numpy(u).cuda(), semi), semi).cpu().numpy()
+    np.testing.assert_allclose(alpha, alpha_ref, rtol=1e-12, atol=1e-14)
+    assert alpha_ref.max() > 0.1 and (alpha_ref == 0).sum() > 0
+    semi.close()
+
+
+@pytest.mark.parametrize("use_graph", [False, True])
+@pytest.mark.parametrize("case", [Case("euler2d_fd", 2, "euler", 3, "fd", "ranocha"),
+                                  Case("euler2d_hg", 2, "euler", 3, "hg", "ranocha", ic="weak_blast_wave"),
+                                  Case("mhd3d", 3, "mhd", 1, "fd", "hindenlang_gassner", cmin=-1.0, cmax=1.0,
+                                       ic="convergence_test", gamma=5 / 3)],
+                         ids=lambda c: c.name)
+def test_rk_step_matches_oracle(th, oracle, torch, case, use_graph):
+    O = oracle
+    cfg = oracle_cfg(O, case)
+    mesh = O.Mesh(case.ndims, case.level, case.cmin, case.cmax, case.polydeg)
+    u = O.compute_coefficients(cfg, mesh, oracle_ic(O, case), 0.0, case.seed)
+    semi = product_semi(th, case)
+    ud = th.pack(torch.from_numpy(u).cuda(), semi)
+    kd = torch.zeros_like(ud)
+    dt = 0.5 * O.max_dt(cfg, mesh, u)
+    k = np.zeros_like(u)
+    uo = u.copy()
+    stream = torch.cuda.Stream()
+    stream.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(stream):
+        for step in range(3):
+            th.rk_step(ud, kd, step * dt, dt, semi, use_graph=use_graph, stream=stream)
+            # eager host work between replays
+            torch.cuda.current_stream().synchronize()
+            O.ck2n54_step(cfg, mesh, uo, k, step * dt, dt)
+            got = th.wrap_array_native(ud, semi).cpu().numpy()
+            err = rel_max(got - u, uo - u)
+            print(f"{case.name} graph={use_graph} step {step}: rel err of increment {err:.3e}", flush=True)
+            assert err < 1e-11, err
+    torch.cuda.synchronize()
+    semi.close()
+
+
+def test_c1_trajectory_matches_oracle(th, oracle, torch):
+    """C1: 1D advection, level 4, CK2N54 with CFL 1.6 to t=1: L2/Linf to 1e-10."""
+    O = oracle
+    case = Case("c1", 1, "linadv", 4, cmin=-1.0, cmax=1.0, ic="convergence_test", advection=(1.0,))
+    semi = product_semi(th, case)
+    ode = th.semidiscretizeHIP(semi, (0.0, 1.0))
+    sol = th.solve(ode, th.CarpenterKennedy2N54(williamson_condition=False),
+                   callback=[th.StepsizeCallback(cfl=1.6)])
+    assert sol.nsteps == 20
+    l2, linf = th.calc_error_norms(sol.u, sol.t, semi)
+    cfg = oracle_cfg(O, case)
+    mesh = O.Mesh(1, 4, -1.0, 1.0, 3)
+    u = O.compute_coefficients(cfg, mesh, O.IC_CONVERGENCE_TEST)
+    k = np.zeros_like(u)
+    t = 0.0
+    for _ in range(20):
+        dt = min(1.6 * O.max_dt(cfg, mesh, u), 1.0 - t)
+        O.ck2n54_step(cfg, mesh, u, k, t, dt)
+        t += dt
+    l2r, linfr = O.error_norms(cfg, mesh, u, t, O.IC_CONVERGENCE_TEST)
+    print(f"C1: gpu l2={l2[0]:.16e} linf={linf[0]:.16e}; oracle l2={l2r[0]:.16e} linf={linfr[0]:.16e}")
+    np.testing.assert_allclose(l2, l2r, rtol=1e-10)
+    np.testing.assert_allclose(linf, linfr, rtol=1e-10)
+    semi.close()
+
+
+def test_check_state_flags(th, oracle, torch):
+    case = Case("euler2d", 2, "euler", 2, "fd", "ranocha")
+    semi = product_semi(th, case)
+    u = th.compute_coefficients(0.0, semi)
+    nbad, flags = th.check_state(u, semi)
+    assert nbad == 0 and int(flags.abs().sum()) == 0
+    v = th.wrap_array(u, semi)
+    v[0, 3, 5] = -1.0          # rho <= 0 in element 5
+    v[1, 0, 9] = float("nan")  # non-finite in element 9
+    nbad, flags = th.check_state(u, semi)
+    f = flags.cpu().numpy()
+    assert nbad == 2 and f[5] & 2 and f[9] & 1
+    semi.close()
+
+
+def test_unsupported_combination_is_an_error(th, torch):
+    c = Case("mhd_hg", 3, "mhd", 1, "hg", "hindenlang_gassner", cmin=-1.0, cmax=1.0, gamma=5 / 3)
+    with pytest.raises(th.TrixiHipError) as ei:
+        product_semi(th, c)
+    assert ei.value.code == -2

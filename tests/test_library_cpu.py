@@ -1,0 +1,109 @@
+"""CPU-side checks of the product: the C-ABI library loads and exports every
+symbol include/trixi_hip.h declares; its host-side basis matches the exact
+fixtures; the host TreeMesh matches the oracle's; solver configuration maps
+to the right enums and rejects unsupported combinations.  No device call."""
+import json
+import os
+import re
+
+import numpy as np
+import pytest
+
+from conftest import ROOT
+
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+
+
+def declared_symbols():
+    with open(os.path.join(ROOT, "include", "trixi_hip.h")) as fh:
+        text = fh.read()
+    return sorted(set(re.findall(r"^\s*(?:const\s+)?\w+\**\s+\**(trixi_hip_\w+)\s*\(", text, re.M)))
+
+
+def test_header_declares_the_abi():
+    syms = declared_symbols()
+    for s in ("trixi_hip_create", "trixi_hip_rhs", "trixi_hip_rk_step", "trixi_hip_max_dt",
+              "trixi_hip_error_norms", "trixi_hip_basis", "trixi_hip_pack", "trixi_hip_unpack"):
+        assert s in syms
+
+
+def test_library_exports_every_declared_symbol(th):
+    import ctypes
+    L = ctypes.CDLL(th.LIBRARY_PATH)
+    for s in declared_symbols():
+        assert hasattr(L, s), s
+    from trixi_hip import _lib
+    assert set(declared_symbols()) == set(_lib.SIGNATURES), "ctypes binding out of sync with the header"
+    assert "gfx950" in th.version()
+
+
+@pytest.mark.parametrize("N", [2, 3, 4, 5, 6])
+def test_library_basis_matches_exact_fixture(th, N):
+    with open(os.path.join(GOLDEN, f"basis_N{N}.json")) as fh:
+        g = {k: np.array(v, dtype=np.float64) for k, v in json.load(fh).items() if k != "N"}
+    b = th.LobattoLegendreBasisHIP(N - 1)
+    np.testing.assert_allclose(b.nodes, g["nodes"], rtol=0, atol=2e-16 * 4)
+    np.testing.assert_allclose(b.weights, g["weights"], rtol=0, atol=4e-16)
+    for key, mine in (("D", b.derivative_matrix), ("Dhat", b.derivative_dhat), ("Vinv", b.inverse_vandermonde_legendre)):
+        np.testing.assert_allclose(mine, g[key], rtol=0, atol=2e-14 * np.abs(g[key]).max(), err_msg=key)
+    np.testing.assert_allclose(b.derivative_split, g["Dsplit"], rtol=0, atol=2e-14 * np.abs(g["Dsplit"]).max())
+    np.testing.assert_allclose(b.boundary_interpolation[:, 0], g["lhat_minus"], atol=1e-14)
+    np.testing.assert_allclose(b.boundary_interpolation[:, 1], g["lhat_plus"], atol=1e-14)
+
+
+@pytest.mark.parametrize("nd,level", [(1, 4), (2, 3), (3, 2)])
+def test_host_treemesh_matches_oracle(th, oracle, nd, level):
+    m = th.TreeMesh((-2.0,) * nd, (2.0,) * nd, initial_refinement_level=level, n_cells_max=10**6)
+    o = oracle.Mesh(nd, level, -2.0, 2.0, 3)
+    assert m.nelements == o.nelements and m.ninterfaces == o.ninterfaces
+    np.testing.assert_array_equal(m.neighbor_ids, o.neighbor_ids)
+    np.testing.assert_array_equal(m.orientations, o.orientations)
+    np.testing.assert_array_equal(m.cell_index, o.cell_index)
+    np.testing.assert_array_equal(m.inverse_jacobian, o.inverse_jacobian)
+    x = m.node_coordinates(th.LobattoLegendreBasisHIP(3).nodes)
+    np.testing.assert_allclose(x, o.node_coordinates, rtol=0, atol=1e-15)
+
+
+def test_config_mapping_and_rejections(th):
+    eq = th.CompressibleEulerEquations3D(1.4)
+    s = th.DGSEMHIP(polydeg=3, surface_flux=th.flux_lax_friedrichs,
+                    volume_integral=th.VolumeIntegralFluxDifferencing(th.flux_ranocha))
+    c = s.config(eq)
+    assert (c.ndims, c.equation, c.polydeg, c.volume_integral, c.volume_flux, c.surface_flux, c.precision) == \
+        (3, 1, 3, 1, 3, 1, 0)
+    s32 = th.DGSEMHIP(polydeg=3, surface_flux=th.FluxLaxFriedrichs(th.max_abs_speed_naive),
+                      volume_integral=th.VolumeIntegralFluxDifferencing(th.flux_shima_etal), RealT=np.float32)
+    c = s32.config(eq)
+    assert (c.surface_flux, c.volume_flux, c.precision) == (2, 4, 1)
+    mhd = th.IdealGlmMhdEquations3D(5 / 3)
+    with pytest.raises(NotImplementedError):
+        th.DGSEMHIP(polydeg=3, surface_flux=th.flux_lax_friedrichs).config(mhd)
+    ok = th.DGSEMHIP(polydeg=3, surface_flux=(th.flux_lax_friedrichs, th.flux_nonconservative_powell),
+                     volume_integral=th.VolumeIntegralFluxDifferencing(
+                         (th.flux_hindenlang_gassner, th.flux_nonconservative_powell)))
+    c = ok.config(mhd)
+    assert (c.equation, c.volume_flux, c.surface_flux, c.c_h) == (2, 5, 1, 1.0)
+    eq2 = th.CompressibleEulerEquations2D(1.4)
+    b = th.LobattoLegendreBasisHIP(3)
+    ind = th.IndicatorHennemannGassner(eq2, b, alpha_max=0.5, alpha_min=0.001, alpha_smooth=True,
+                                       variable=th.density_pressure)
+    hg = th.DGSEMHIP(polydeg=3, surface_flux=th.flux_lax_friedrichs,
+                     volume_integral=th.VolumeIntegralShockCapturingHG(ind, volume_flux_dg=th.flux_ranocha,
+                                                                       volume_flux_fv=th.flux_lax_friedrichs))
+    c = hg.config(eq2)
+    assert (c.volume_integral, c.volume_flux, c.fv_flux, c.alpha_smooth, c.alpha_max) == (2, 3, 1, 1, 0.5)
+    with pytest.raises(NotImplementedError):
+        th.TreeMesh((-1.0,), (1.0,), 2, periodicity=False)
+    with pytest.raises(ValueError):
+        th.TreeMesh((-1.0, -1.0), (1.0, 2.0), 2)
+
+
+def test_device_calls_fail_loudly_without_gpu(th):
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present")
+    mesh = th.TreeMesh((-1.0,), (1.0,), 2)
+    solver = th.DGSEMHIP(polydeg=3, surface_flux=th.flux_lax_friedrichs)
+    with pytest.raises(th.TrixiHipError):
+        th.SemidiscretizationHyperbolicHIP(mesh, th.LinearScalarAdvectionEquation1D(1.0),
+                                           th.initial_condition_convergence_test, solver)

@@ -1,0 +1,33 @@
+"""trixi_hip -- MI355X-native DGSEM semidiscretization for Trixi.jl's TreeMesh.
+
+Host package of libtrixihip.so (hand-written gfx950 HIP kernels behind a C
+ABI, include/trixi_hip.h).  The public names are HIP-named twins of
+TrixiCUDA.jl's exports (src/TrixiCUDA.jl:74-77): LobattoLegendreBasisHIP,
+DGSEMHIP, SemidiscretizationHyperbolicHIP, semidiscretizeHIP, plus rhs_hip_
+(rhs_gpu!) and the GPU-resident callbacks.
+
+This directory is not a valid Python identifier; load it as ``trixi_hip``:
+
+    spec = importlib.util.spec_from_file_location(
+        "trixi_hip", "trixicuda.jl_amd/__init__.py",
+        submodule_search_locations=["trixicuda.jl_amd"])
+"""
+from ._lib import LIBRARY_PATH, TrixiHipError, version
+from .callbacks import (AnalysisCallback, StepsizeCallback, blending_factors, calc_error_norms, check_state,
+                        integrate, max_dt)
+from .equations import (CompressibleEulerEquations1D, CompressibleEulerEquations2D, CompressibleEulerEquations3D,
+                        FluxLaxFriedrichs, IdealGlmMhdEquations3D, LinearScalarAdvectionEquation1D,
+                        LinearScalarAdvectionEquation2D, LinearScalarAdvectionEquation3D, density_pressure,
+                        flux_central, flux_hindenlang_gassner, flux_lax_friedrichs, flux_nonconservative_powell,
+                        flux_ranocha, flux_shima_etal, initial_condition_constant,
+                        initial_condition_convergence_test, initial_condition_seeded_random,
+                        initial_condition_smooth_periodic, initial_condition_weak_blast_wave, max_abs_speed,
+                        max_abs_speed_naive)
+from .mesh import TreeMesh
+from .semidiscretization import (ODEProblem, SemidiscretizationHyperbolicHIP, compute_coefficients, pack,
+                                 rhs_hip_, semidiscretizeHIP, wrap_array, wrap_array_native)
+from .solver import (DGSEMHIP, IndicatorHennemannGassner, LobattoLegendreBasisHIP, VolumeIntegralFluxDifferencing,
+                     VolumeIntegralShockCapturingHG, VolumeIntegralWeakForm)
+from .time_integration import CarpenterKennedy2N54, Solution, rk_step, solve
+
+__all__ = [n for n in dir() if not n.startswith("_")]

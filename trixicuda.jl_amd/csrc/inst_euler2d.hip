@@ -1,0 +1,22 @@
+// Kernel instantiations: compressible Euler 2D, weak form and flux differencing.
+#include "registry.hpp"
+
+namespace trixi_hip {
+
+void register_euler2d() {
+    using EQ = Euler<2>;
+    constexpr int C = FLUX_CENTRAL;
+    Registrar<2, 4, EQ, VI_WEAK, C, FLUX_LLF, C>::add(EQ_EULER);
+    Registrar<2, 4, EQ, VI_WEAK, C, FLUX_LLF_NAIVE, C>::add(EQ_EULER);
+    Registrar<2, 4, EQ, VI_WEAK, C, FLUX_CENTRAL, C>::add(EQ_EULER);
+    Registrar<2, 4, EQ, VI_FD, FLUX_RANOCHA, FLUX_LLF, C>::add(EQ_EULER);
+    Registrar<2, 4, EQ, VI_FD, FLUX_RANOCHA, FLUX_LLF_NAIVE, C>::add(EQ_EULER);
+    Registrar<2, 4, EQ, VI_FD, FLUX_RANOCHA, FLUX_RANOCHA, C>::add(EQ_EULER);
+    Registrar<2, 4, EQ, VI_FD, FLUX_SHIMA_ETAL, FLUX_LLF, C>::add(EQ_EULER);
+    Registrar<2, 4, EQ, VI_FD, FLUX_SHIMA_ETAL, FLUX_LLF_NAIVE, C>::add(EQ_EULER);
+    Registrar<2, 4, EQ, VI_FD, FLUX_SHIMA_ETAL, FLUX_SHIMA_ETAL, C>::add(EQ_EULER);
+    Registrar<2, 2, EQ, VI_FD, FLUX_RANOCHA, FLUX_LLF, C>::add(EQ_EULER);
+    Registrar<2, 3, EQ, VI_FD, FLUX_RANOCHA, FLUX_LLF, C>::add(EQ_EULER);
+}
+
+}  // namespace trixi_hip

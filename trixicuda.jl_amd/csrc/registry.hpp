@@ -1,0 +1,51 @@
+// registry.hpp -- table of the compiled rhs kernel instantiations.
+#pragma once
+
+#include "rhs_kernel.hpp"
+
+namespace trixi_hip {
+
+struct KernelKey {
+    int dim, N, eq, vol, vf, sf, fvf, prec, rk;
+    bool operator<(const KernelKey& o) const {
+        const int a[9] = {dim, N, eq, vol, vf, sf, fvf, prec, rk};
+        const int b[9] = {o.dim, o.N, o.eq, o.vol, o.vf, o.sf, o.fvf, o.prec, o.rk};
+        for (int i = 0; i < 9; ++i)
+            if (a[i] != b[i]) return a[i] < b[i];
+        return false;
+    }
+};
+
+using LaunchFn = hipError_t (*)(const RhsArgs&, hipStream_t);
+
+void register_kernel(const KernelKey& key, LaunchFn fn);
+LaunchFn find_kernel(const KernelKey& key);
+
+// Workgroup width in elements: fill about 256 lanes, keep both LDS tiles
+// (state and accumulator) within 48 KiB so several workgroups share a CU.
+template <int DIM, int N, int NV, int RS>
+constexpr int pick_m() {
+    constexpr int NL = cpow(N, DIM - 1), NN = cpow(N, DIM);
+    int m = 256 / NL;
+    if (m < 1) m = 1;
+    while (DIM > 1 && m > 1 && 2 * NV * NN * (m + 1) * RS > 48 * 1024) m /= 2;
+    return m;
+}
+
+template <int DIM, int N, class EQ, int VOL, int VF, int SF, int FVF>
+struct Registrar {
+    static void add(int eq) {
+        constexpr int MD = pick_m<DIM, N, EQ::NV, 8>();
+        constexpr int MF = pick_m<DIM, N, EQ::NV, 4>();
+        register_kernel({DIM, N, eq, VOL, VF, SF, FVF, 0, 0},
+                        &launch_rhs<DIM, N, EQ, VOL, VF, SF, FVF, double, MD, false>);
+        register_kernel({DIM, N, eq, VOL, VF, SF, FVF, 0, 1},
+                        &launch_rhs<DIM, N, EQ, VOL, VF, SF, FVF, double, MD, true>);
+        register_kernel({DIM, N, eq, VOL, VF, SF, FVF, 1, 0},
+                        &launch_rhs<DIM, N, EQ, VOL, VF, SF, FVF, float, MF, false>);
+        register_kernel({DIM, N, eq, VOL, VF, SF, FVF, 1, 1},
+                        &launch_rhs<DIM, N, EQ, VOL, VF, SF, FVF, float, MF, true>);
+    }
+};
+
+}  // namespace trixi_hip

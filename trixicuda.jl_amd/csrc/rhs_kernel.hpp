@@ -1,0 +1,290 @@
+// rhs_kernel.hpp -- the fused DGSEM rhs! kernel for gfx950.
+//
+// One launch computes, for every element of a periodic TreeMesh,
+//   volume integral (weak form, flux differencing incl. nonconservative
+//   terms, or Hennemann-Gassner DG/FV blending)
+//   + surface integral of the numerical interface fluxes
+//   + Jacobian scaling du *= -1/J,
+// i.e. the reference stages cuda_volume_integral!, cuda_prolong2interfaces!,
+// cuda_interface_flux!, cuda_surface_integral! and cuda_jacobian!
+// (src/solvers/dg_3d.jl:57-102, 361-412, 846-873 and the 1D/2D twins) in a
+// single pass over HBM.  Optionally the low-storage Runge-Kutta register
+// update is fused in as an epilogue (FP64): k = a k + dt du, u_out = u + b k.
+//
+// Work decomposition.  A workgroup owns M consecutive elements (Trixi order,
+// element-innermost SoA so that lanes of a wave read consecutive addresses).
+// Lane (el, line) owns one line of N nodes through element el: for d =
+// 0..DIM-1 it evaluates the N(N-1)/2 symmetric two-point fluxes along its
+// line in direction d, the two interface fluxes at the line's end faces
+// (recomputed on both sides of each interface, so no face buffer ever goes
+// to HBM), and accumulates the line's N nodes in registers.  Directions are
+// combined through an LDS accumulator: the lines of one direction partition
+// the element's nodes, so each phase is race free between barriers.
+#pragma once
+
+#include "physics.hpp"
+
+namespace trixi_hip {
+
+constexpr int MAXN = 8;
+
+__host__ __device__ constexpr int cpow(int b, int e) { return e == 0 ? 1 : b * cpow(b, e - 1); }
+
+struct RhsArgs {
+    const double* u;          // SoA state  u[(v*NN + node)*E + e]
+    double* du;               // SoA output (rhs mode)
+    double* k;                // 2N register (RK mode)
+    double* u_out;            // RK mode: u_out = u + b k
+    const double* time;       // RK mode: device scalars {t, dt}
+    const int32_t* nbr;       // [2*DIM][E] neighbour element per face (-x,+x,-y,+y,-z,+z)
+    const double* invJ;       // [E]
+    const double* alpha_raw;  // [E] HG blending factors before smoothing
+    int64_t E;
+    PhysParams phys;
+    double Ds[MAXN * MAXN];   // derivative_split (row-major)
+    double Dhat[MAXN * MAXN]; // derivative_dhat (row-major)
+    double invw[MAXN];        // inverse weights
+    double lhat_m, lhat_p;    // boundary_interpolation[1,1], [N,2]
+    double atol;              // HG pure-DG threshold
+    int smooth;               // HG smoothing
+    double rk_a, rk_b;        // RK stage coefficients (rk_a unused when rk_first)
+    int rk_first;
+};
+
+// Line geometry: base node of line `line` in direction d, the other
+// directions enumerated in increasing order with the first one fastest
+// (the face-node order of Trixi's interface arrays, containers_3d.jl:58).
+template <int DIM, int N>
+__device__ __forceinline__ int line_base(int line, int d) {
+    int base = 0, rem = line;
+#pragma unroll
+    for (int dd = 0; dd < DIM; ++dd) {
+        if (dd == d) continue;
+        base += (rem % N) * cpow(N, dd);
+        rem /= N;
+    }
+    return base;
+}
+
+template <int DIM, int N, class EQ, int VOL, int VF, int SF, int FVF, typename R, int M, bool RK>
+__global__ void __launch_bounds__(M * cpow(N, DIM - 1))
+rhs_kernel(const RhsArgs a) {
+    constexpr int NL = cpow(N, DIM - 1);
+    constexpr int NN = cpow(N, DIM);
+    constexpr int NV = EQ::NV;
+    constexpr bool STAGE = DIM > 1;
+    constexpr int LDM = M + 1;  // padded LDS row
+    constexpr bool SF_LOGS = (SF == FLUX_RANOCHA || SF == FLUX_HINDENLANG_GASSNER);
+    constexpr bool VF_LOGS = (VOL != VI_WEAK) && (VF == FLUX_RANOCHA || VF == FLUX_HINDENLANG_GASSNER);
+    constexpr bool FV_LOGS = (VOL == VI_HG) && (FVF == FLUX_RANOCHA || FVF == FLUX_HINDENLANG_GASSNER);
+    constexpr bool NODE_LOGS = SF_LOGS || VF_LOGS || FV_LOGS;
+    using Node = typename EQ::template Node<R>;
+
+    __shared__ R us[STAGE ? NV * NN * LDM : 1];
+    __shared__ R acc[STAGE ? NV * NN * LDM : 1];
+
+    const int tid = threadIdx.x;
+    const int el = tid % M;
+    const int line = tid / M;
+    const int64_t E = a.E;
+    const int64_t e0 = (int64_t)blockIdx.x * M;
+    const int64_t e = e0 + el;
+    const bool valid = e < E;
+    const int64_t ec = valid ? e : E - 1;
+    const PhysR<R> P = PhysR<R>::from(a.phys);
+
+    if (STAGE) {
+        for (int idx = tid; idx < NV * NN * M; idx += M * NL) {
+            const int c = idx % M;
+            const int r = idx / M;
+            int64_t ee = e0 + c;
+            if (ee >= E) ee = E - 1;
+            us[r * LDM + c] = (R)a.u[(int64_t)r * E + ee];
+        }
+        __syncthreads();
+    }
+
+    R alpha = R(0);
+    bool dg_only = true;
+    if (VOL == VI_HG) {
+        double al = a.alpha_raw[ec];
+        if (a.smooth) {
+#pragma unroll
+            for (int s = 0; s < 2 * DIM; ++s) {
+                const double an = 0.5 * a.alpha_raw[a.nbr[(int64_t)s * E + ec]];
+                al = an > al ? an : al;
+            }
+        }
+        dg_only = fabs(al) <= a.atol;
+        alpha = (R)al;
+    }
+
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+        const int s = cpow(N, d);
+        const int base = line_base<DIM, N>(line, d);
+
+        Node nd[N];
+#pragma unroll
+        for (int p = 0; p < N; ++p) {
+            R uc[NV];
+#pragma unroll
+            for (int v = 0; v < NV; ++v) {
+                const int row = v * NN + base + p * s;
+                uc[v] = STAGE ? us[row * LDM + el] : (R)a.u[(int64_t)row * E + ec];
+            }
+            nd[p] = EQ::template node<R, NODE_LOGS>(uc, P);
+        }
+
+        R acl[N][NV];
+#pragma unroll
+        for (int p = 0; p < N; ++p)
+#pragma unroll
+            for (int v = 0; v < NV; ++v) acl[p][v] = R(0);
+
+        // ---- volume integral along the line ---------------------------------
+        if (VOL == VI_WEAK) {
+            R fl[N][NV];
+#pragma unroll
+            for (int p = 0; p < N; ++p) EQ::flux(nd[p], d, P, fl[p]);
+#pragma unroll
+            for (int i = 0; i < N; ++i)
+#pragma unroll
+                for (int p = 0; p < N; ++p) {
+                    const R w = (R)a.Dhat[i * N + p];
+#pragma unroll
+                    for (int v = 0; v < NV; ++v) acl[i][v] += w * fl[p][v];
+                }
+        } else {
+            const R sc = (VOL == VI_HG && !dg_only) ? R(1) - alpha : R(1);
+#pragma unroll
+            for (int i = 0; i < N; ++i)
+#pragma unroll
+                for (int j = i + 1; j < N; ++j) {
+                    R f[NV];
+                    EQ::template two_point<VF>(nd[i], nd[j], d, P, f);
+                    const R fa = sc * (R)a.Ds[i * N + j];
+                    const R fb = sc * (R)a.Ds[j * N + i];
+#pragma unroll
+                    for (int v = 0; v < NV; ++v) {
+                        acl[i][v] += fa * f[v];
+                        acl[j][v] += fb * f[v];
+                    }
+                }
+            if constexpr (EQ::NONCONS) {
+#pragma unroll
+                for (int i = 0; i < N; ++i) {
+                    R integral[NV];
+#pragma unroll
+                    for (int v = 0; v < NV; ++v) integral[v] = R(0);
+#pragma unroll
+                    for (int j = 0; j < N; ++j) {
+                        R f[NV];
+                        EQ::noncons(nd[i], nd[j], d, f);
+                        const R w = (R)a.Ds[i * N + j];
+#pragma unroll
+                        for (int v = 0; v < NV; ++v) integral[v] += w * f[v];
+                    }
+                    const R h = sc * R(0.5);
+#pragma unroll
+                    for (int v = 0; v < NV; ++v) acl[i][v] += h * integral[v];
+                }
+            }
+            if (VOL == VI_HG && !dg_only) {
+#pragma unroll
+                for (int p = 0; p + 1 < N; ++p) {
+                    R F[NV];
+                    EQ::template two_point<FVF>(nd[p], nd[p + 1], d, P, F);
+                    const R wl = alpha * (R)a.invw[p];
+                    const R wr = alpha * (R)a.invw[p + 1];
+#pragma unroll
+                    for (int v = 0; v < NV; ++v) {
+                        acl[p][v] += wl * F[v];
+                        acl[p + 1][v] -= wr * F[v];
+                    }
+                }
+            }
+        }
+
+        // ---- interface fluxes at both ends of the line, surface integral ----
+        {
+            const int32_t nbm = a.nbr[(int64_t)(2 * d) * E + ec];
+            const int32_t nbp = a.nbr[(int64_t)(2 * d + 1) * E + ec];
+            R um[NV], up[NV];
+#pragma unroll
+            for (int v = 0; v < NV; ++v) {
+                um[v] = (R)a.u[(int64_t)(v * NN + base + (N - 1) * s) * E + nbm];
+                up[v] = (R)a.u[(int64_t)(v * NN + base) * E + nbp];
+            }
+            const Node ndm = EQ::template node<R, SF_LOGS>(um, P);
+            const Node ndp = EQ::template node<R, SF_LOGS>(up, P);
+            R fm[NV], fp[NV];
+            EQ::template two_point<SF>(ndm, nd[0], d, P, fm);       // this element is "right"
+            EQ::template two_point<SF>(nd[N - 1], ndp, d, P, fp);   // this element is "left"
+            if constexpr (EQ::NONCONS) {
+                R ncm[NV], ncp[NV];
+                EQ::noncons(nd[0], ndm, d, ncm);
+                EQ::noncons(nd[N - 1], ndp, d, ncp);
+#pragma unroll
+                for (int v = 0; v < NV; ++v) {
+                    fm[v] = fm[v] + R(0.5) * ncm[v];
+                    fp[v] = fp[v] + R(0.5) * ncp[v];
+                }
+            }
+            const R lm = (R)a.lhat_m, lp = (R)a.lhat_p;
+#pragma unroll
+            for (int v = 0; v < NV; ++v) {
+                acl[0][v] = acl[0][v] - fm[v] * lm;
+                acl[N - 1][v] = acl[N - 1][v] + fp[v] * lp;
+            }
+        }
+
+        // ---- combine directions ---------------------------------------------
+        if (d + 1 < DIM) {
+#pragma unroll
+            for (int p = 0; p < N; ++p)
+#pragma unroll
+                for (int v = 0; v < NV; ++v) {
+                    const int idx = (v * NN + base + p * s) * LDM + el;
+                    if (d == 0)
+                        acc[idx] = acl[p][v];
+                    else
+                        acc[idx] += acl[p][v];
+                }
+            __syncthreads();
+        } else if (valid) {
+            const double fac = -a.invJ[e];
+            double t_dt = 0.0;
+            if (RK) t_dt = a.time[1];
+#pragma unroll
+            for (int p = 0; p < N; ++p)
+#pragma unroll
+                for (int v = 0; v < NV; ++v) {
+                    const int node = base + p * s;
+                    R total = acl[p][v];
+                    if (STAGE) total = acc[(v * NN + node) * LDM + el] + total;
+                    const int64_t g = (int64_t)(v * NN + node) * E + e;
+                    const double val = (double)total * fac;
+                    if (!RK) {
+                        a.du[g] = val;
+                    } else {
+                        const double kk = a.rk_first ? t_dt * val : a.rk_a * a.k[g] + t_dt * val;
+                        a.k[g] = kk;
+                        a.u_out[g] = a.u[g] + a.rk_b * kk;
+                    }
+                }
+        }
+    }
+}
+
+// Launch helper used by the instantiation units.
+template <int DIM, int N, class EQ, int VOL, int VF, int SF, int FVF, typename R, int M, bool RK>
+hipError_t launch_rhs(const RhsArgs& a, hipStream_t stream) {
+    constexpr int NL = cpow(N, DIM - 1);
+    const int64_t blocks = (a.E + M - 1) / M;
+    hipLaunchKernelGGL((rhs_kernel<DIM, N, EQ, VOL, VF, SF, FVF, R, M, RK>), dim3((unsigned)blocks),
+                       dim3(M * NL), 0, stream, a);
+    return hipGetLastError();
+}
+
+}  // namespace trixi_hip

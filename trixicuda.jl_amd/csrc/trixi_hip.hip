@@ -1,0 +1,1206 @@
+// trixi_hip.hip -- C ABI, context management and auxiliary kernels of
+// libtrixihip.so (see include/trixi_hip.h for the contract).
+#include "../../include/trixi_hip.h"
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "registry.hpp"
+
+namespace trixi_hip {
+
+// ---------------------------------------------------------------------------
+// registry
+// ---------------------------------------------------------------------------
+static std::map<KernelKey, LaunchFn>& table() {
+    static std::map<KernelKey, LaunchFn> t;
+    return t;
+}
+void register_kernel(const KernelKey& key, LaunchFn fn) { table()[key] = fn; }
+LaunchFn find_kernel(const KernelKey& key) {
+    auto it = table().find(key);
+    return it == table().end() ? nullptr : it->second;
+}
+
+void register_linadv();
+void register_euler1d();
+void register_euler2d();
+void register_euler3d();
+void register_euler1d_hg();
+void register_euler2d_hg();
+void register_euler3d_hg();
+void register_mhd3d();
+
+static void ensure_registered() {
+    static std::once_flag once;
+    std::call_once(once, [] {
+        register_linadv();
+        register_euler1d();
+        register_euler2d();
+        register_euler3d();
+        register_euler1d_hg();
+        register_euler2d_hg();
+        register_euler3d_hg();
+        register_mhd3d();
+    });
+}
+
+// ---------------------------------------------------------------------------
+// error reporting
+// ---------------------------------------------------------------------------
+static thread_local std::string g_last_error;
+
+static int fail(int code, const std::string& msg) {
+    g_last_error = msg;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                  \
+    do {                                                                               \
+        hipError_t _e = (expr);                                                        \
+        if (_e != hipSuccess)                                                          \
+            return fail(TRIXI_HIP_EDEVICE, std::string(#expr) + ": " + hipGetErrorString(_e)); \
+    } while (0)
+
+// ---------------------------------------------------------------------------
+// Lobatto-Legendre basis on the host (the matrices of
+// LobattoLegendreBasisGPU, reference basis_lobatto_legendre.jl:60-100).
+// Interior nodes: roots of P_{n-1} - P_{n+1} (proportional to (1-x^2) P_n'),
+// Newton from Chebyshev-Gauss-Lobatto guesses; w_i = 2 / (n (n+1) P_n(x_i)^2).
+// ---------------------------------------------------------------------------
+static void legendre(int n, double x, double& P, double& dP) {
+    double p0 = 1.0, p1 = x, d0 = 0.0, d1 = 1.0;
+    if (n == 0) {
+        P = 1.0;
+        dP = 0.0;
+        return;
+    }
+    for (int k = 2; k <= n; ++k) {
+        const double p2 = ((2.0 * k - 1.0) * x * p1 - (k - 1.0) * p0) / k;
+        const double d2 = d0 + (2.0 * k - 1.0) * p1;
+        p0 = p1;
+        p1 = p2;
+        d0 = d1;
+        d1 = d2;
+    }
+    P = p1;
+    dP = d1;
+}
+
+struct HostBasis {
+    int N;
+    std::vector<double> x, w, wbary, D, Ds, Dhat, lm, lp, Vinv;
+};
+
+static void lgl(int N, std::vector<double>& x, std::vector<double>& w) {
+    const int n = N - 1;
+    x.assign(N, 0.0);
+    w.assign(N, 0.0);
+    x[0] = -1.0;
+    x[n] = 1.0;
+    for (int i = 1; i < n; ++i) {
+        double xi = -cos(M_PI * i / n);
+        for (int it = 0; it < 100; ++it) {
+            double Pm, dPm, Pp, dPp;
+            legendre(n - 1, xi, Pm, dPm);
+            legendre(n + 1, xi, Pp, dPp);
+            const double q = Pm - Pp, dq = dPm - dPp;
+            const double dx = q / dq;
+            xi -= dx;
+            if (fabs(dx) < 1e-17) break;
+        }
+        x[i] = xi;
+    }
+    // enforce symmetry and the centre node
+    for (int i = 0; i < N / 2; ++i) {
+        const double s = 0.5 * (x[n - i] - x[i]);
+        x[i] = -s;
+        x[n - i] = s;
+    }
+    if (N % 2 == 1) x[N / 2] = 0.0;
+    for (int i = 0; i < N; ++i) {
+        double P, dP;
+        legendre(n, x[i], P, dP);
+        w[i] = 2.0 / (n * (n + 1.0) * P * P);
+    }
+}
+
+static bool invert(int n, const std::vector<double>& A, std::vector<double>& Ainv) {
+    std::vector<double> M = A;
+    Ainv.assign(n * n, 0.0);
+    for (int i = 0; i < n; ++i) Ainv[i * n + i] = 1.0;
+    for (int c = 0; c < n; ++c) {
+        int piv = c;
+        for (int r = c + 1; r < n; ++r)
+            if (fabs(M[r * n + c]) > fabs(M[piv * n + c])) piv = r;
+        if (M[piv * n + c] == 0.0) return false;
+        for (int k = 0; k < n; ++k) {
+            std::swap(M[piv * n + k], M[c * n + k]);
+            std::swap(Ainv[piv * n + k], Ainv[c * n + k]);
+        }
+        const double d = M[c * n + c];
+        for (int k = 0; k < n; ++k) {
+            M[c * n + k] /= d;
+            Ainv[c * n + k] /= d;
+        }
+        for (int r = 0; r < n; ++r) {
+            if (r == c) continue;
+            const double f = M[r * n + c];
+            if (f == 0.0) continue;
+            for (int k = 0; k < n; ++k) {
+                M[r * n + k] -= f * M[c * n + k];
+                Ainv[r * n + k] -= f * Ainv[c * n + k];
+            }
+        }
+    }
+    return true;
+}
+
+static std::vector<double> interpolation_matrix(const std::vector<double>& xin,
+                                                const std::vector<double>& wbary,
+                                                const std::vector<double>& xout) {
+    const int nin = (int)xin.size(), nout = (int)xout.size();
+    std::vector<double> V(nout * nin, 0.0);
+    for (int k = 0; k < nout; ++k) {
+        int hit = -1;
+        for (int j = 0; j < nin; ++j)
+            if (xout[k] == xin[j]) hit = j;
+        if (hit >= 0) {
+            V[k * nin + hit] = 1.0;
+            continue;
+        }
+        double s = 0.0;
+        for (int j = 0; j < nin; ++j) {
+            V[k * nin + j] = wbary[j] / (xout[k] - xin[j]);
+            s += V[k * nin + j];
+        }
+        for (int j = 0; j < nin; ++j) V[k * nin + j] /= s;
+    }
+    return V;
+}
+
+static HostBasis make_basis(int N) {
+    HostBasis b;
+    b.N = N;
+    lgl(N, b.x, b.w);
+    b.wbary.assign(N, 1.0);
+    for (int j = 0; j < N; ++j)
+        for (int k = 0; k < N; ++k)
+            if (k != j) b.wbary[j] *= (b.x[j] - b.x[k]);
+    for (int j = 0; j < N; ++j) b.wbary[j] = 1.0 / b.wbary[j];
+    b.D.assign(N * N, 0.0);
+    for (int i = 0; i < N; ++i) {
+        double diag = 0.0;
+        for (int j = 0; j < N; ++j)
+            if (j != i) {
+                b.D[i * N + j] = (b.wbary[j] / b.wbary[i]) / (b.x[i] - b.x[j]);
+                diag -= b.D[i * N + j];
+            }
+        b.D[i * N + i] = diag;
+    }
+    b.Ds.resize(N * N);
+    for (int q = 0; q < N * N; ++q) b.Ds[q] = 2.0 * b.D[q];
+    b.Ds[0] += 1.0 / b.w[0];
+    b.Ds[N * N - 1] -= 1.0 / b.w[N - 1];
+    b.Dhat.resize(N * N);
+    for (int j = 0; j < N; ++j)
+        for (int n = 0; n < N; ++n) b.Dhat[j * N + n] = -b.D[n * N + j] * b.w[n] / b.w[j];
+    b.lm.assign(N, 0.0);
+    b.lp.assign(N, 0.0);
+    b.lm[0] = 1.0 / b.w[0];
+    b.lp[N - 1] = 1.0 / b.w[N - 1];
+    std::vector<double> V(N * N);
+    for (int i = 0; i < N; ++i)
+        for (int m = 0; m < N; ++m) {
+            double P, dP;
+            legendre(m, b.x[i], P, dP);
+            V[i * N + m] = P * sqrt(m + 0.5);
+        }
+    invert(N, V, b.Vinv);
+    return b;
+}
+
+// ---------------------------------------------------------------------------
+// counter-based generator for the seeded initial states (splitmix64)
+// ---------------------------------------------------------------------------
+__host__ __device__ inline uint64_t splitmix64(uint64_t x) {
+    uint64_t z = x + 0x9E3779B97F4A7C15ULL;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+}
+__host__ __device__ inline double u01(uint64_t seed, uint64_t index, uint64_t stream) {
+    const uint64_t r = splitmix64(splitmix64(seed) ^ (index * 16 + stream));
+    return ((double)(r >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+}
+__device__ inline double normal01(uint64_t seed, uint64_t index, uint64_t stream) {
+    const double a = u01(seed, index, stream), b = u01(seed, index, stream + 1);
+    return sqrt(-2.0 * log(a)) * cos(2.0 * M_PI * b);
+}
+
+struct IcParams {
+    int dim, eq, nv, ic;
+    double gamma, inv_gm1, c_h, a[3], length;
+    uint64_t seed;
+};
+
+__device__ double smooth_field(uint64_t seed, int field, int nd, const double* x, double L,
+                               bool use_cos) {
+    double s = 0.0;
+    for (int m = 0; m < 3; ++m) {
+        const uint64_t base = 0xFFFFFFFF00000000ULL + (uint64_t)(field * 3 + m) * 16;
+        double arg = 0.0;
+        for (int d = 0; d < nd; ++d) {
+            const int k = (int)floor(u01(seed, base, d) * 7.0) - 3;
+            arg += k * x[d];
+        }
+        const double amp = (2.0 * u01(seed, base, 3) - 1.0) / 3.0;
+        const double ph = 2.0 * M_PI * u01(seed, base, 4);
+        const double th = 2.0 * M_PI * arg / L + ph;
+        s += amp * (use_cos ? cos(th) : sin(th));
+    }
+    return s;
+}
+
+// Initial conditions (Trixi initial_condition_* functions used by the
+// reference tests and examples, plus the seeded benchmark states).
+__device__ void initial_condition(const IcParams& q, const double* x, double t, uint64_t idx,
+                                  double* u) {
+    if (q.eq == EQ_LINADV) {
+        if (q.ic == TRIXI_HIP_IC_CONVERGENCE_TEST) {
+            double xs = 0.0;
+            for (int d = 0; d < q.dim; ++d) xs += x[d] - q.a[d] * t;
+            const double omega = 2.0 * M_PI * (1.0 / 2.0);
+            u[0] = 1.0 + 0.5 * sin(omega * xs);
+        } else if (q.ic == TRIXI_HIP_IC_SEEDED_RANDOM) {
+            u[0] = 1.0 + 0.1 * (2.0 * u01(q.seed, idx, 0) - 1.0);
+        } else {
+            u[0] = 2.0;
+        }
+        return;
+    }
+    if (q.eq == EQ_EULER) {
+        double v[3] = {0, 0, 0}, rho = 1.0, p = 1.0;
+        if (q.ic == TRIXI_HIP_IC_WEAK_BLAST_WAVE) {
+            double r2 = 0.0;
+            for (int d = 0; d < q.dim; ++d) r2 += x[d] * x[d];
+            const double r = sqrt(r2);
+            const bool inside = !(r > 0.5);
+            if (q.dim == 1) {
+                const double phi = x[0] > 0 ? 0.0 : M_PI;
+                v[0] = inside ? 0.1882 * cos(phi) : 0.0;
+            } else if (q.dim == 2) {
+                const double phi = atan2(x[1], x[0]);
+                v[0] = inside ? 0.1882 * cos(phi) : 0.0;
+                v[1] = inside ? 0.1882 * sin(phi) : 0.0;
+            } else {
+                const double phi = atan2(x[1], x[0]);
+                const double theta = (r == 0.0) ? 0.0 : acos(x[2] / r);
+                v[0] = inside ? 0.1882 * cos(phi) * sin(theta) : 0.0;
+                v[1] = inside ? 0.1882 * sin(phi) * sin(theta) : 0.0;
+                v[2] = inside ? 0.1882 * cos(theta) : 0.0;
+            }
+            rho = inside ? 1.1691 : 1.0;
+            p = inside ? 1.245 : 1.0;
+        } else if (q.ic == TRIXI_HIP_IC_SEEDED_RANDOM) {
+            rho = 1.0 + 0.1 * (2.0 * u01(q.seed, idx, 0) - 1.0);
+            p = 1.0 + 0.05 * (2.0 * u01(q.seed, idx, 1) - 1.0);
+            for (int d = 0; d < q.dim; ++d) v[d] = 0.1 * normal01(q.seed, idx, 2 + 2 * d);
+        } else if (q.ic == TRIXI_HIP_IC_SMOOTH_PERIODIC) {
+            const double L = q.length > 0 ? q.length : 4.0;
+            rho = 1.0 + 0.1 * smooth_field(q.seed, 0, q.dim, x, L, false);
+            for (int d = 0; d < q.dim; ++d) v[d] = 0.1 * smooth_field(q.seed, 1 + d, q.dim, x, L, false);
+            p = 1.0 + 0.05 * smooth_field(q.seed, 4, q.dim, x, L, true);
+        } else if (q.ic == TRIXI_HIP_IC_CONVERGENCE_TEST) {
+            double xs = 0.0;
+            for (int d = 0; d < q.dim; ++d) xs += x[d];
+            const double ini = 2.0 + 0.1 * sin(M_PI * (xs - t));
+            u[0] = ini;
+            for (int d = 0; d < q.dim; ++d) u[1 + d] = ini;
+            u[q.dim + 1] = ini * ini;
+            return;
+        }
+        u[0] = rho;
+        double ke = 0.0;
+        for (int d = 0; d < q.dim; ++d) {
+            u[1 + d] = rho * v[d];
+            ke = (d == 0) ? u[1 + d] * v[d] : ke + u[1 + d] * v[d];
+        }
+        u[q.dim + 1] = p * q.inv_gm1 + 0.5 * ke;
+        return;
+    }
+    double prim[9] = {1.0, 0, 0, 0, 1.0, 0, 0, 0, 0};
+    if (q.ic == TRIXI_HIP_IC_CONVERGENCE_TEST) {
+        const double omega = 2.0 * M_PI, r = 2.0, e = 0.2;
+        const double nx = 1.0 / sqrt(r * r + 1.0), ny = r / sqrt(r * r + 1.0), sqr = 1.0;
+        const double Va = omega / (ny * sqr);
+        const double phi_alv = omega / ny * (nx * (x[0] - 0.5 * r) + ny * (x[1] - 0.5 * r)) - Va * t;
+        const double rho = 1.0;
+        const double v1 = -e * ny * cos(phi_alv) / rho;
+        const double v2 = e * nx * cos(phi_alv) / rho;
+        const double v3 = e * sin(phi_alv) / rho;
+        prim[0] = rho;
+        prim[1] = v1;
+        prim[2] = v2;
+        prim[3] = v3;
+        prim[4] = 1.0;
+        prim[5] = nx - rho * v1 * sqr;
+        prim[6] = ny - rho * v2 * sqr;
+        prim[7] = -rho * v3 * sqr;
+        prim[8] = 0.0;
+    } else if (q.ic == TRIXI_HIP_IC_SEEDED_RANDOM) {
+        prim[0] = 1.0 + 0.1 * (2.0 * u01(q.seed, idx, 0) - 1.0);
+        prim[4] = 1.0 + 0.05 * (2.0 * u01(q.seed, idx, 1) - 1.0);
+        for (int d = 0; d < 3; ++d) prim[1 + d] = 0.1 * normal01(q.seed, idx, 2 + 2 * d);
+        for (int d = 0; d < 3; ++d) prim[5 + d] = 0.5 + 0.1 * (2.0 * u01(q.seed, idx, 8 + d) - 1.0);
+        prim[8] = 0.01 * (2.0 * u01(q.seed, idx, 11) - 1.0);
+    }
+    const double rho = prim[0], p = prim[4], psi = prim[8];
+    u[0] = rho;
+    for (int d = 0; d < 3; ++d) u[1 + d] = rho * prim[1 + d];
+    u[4] = p * q.inv_gm1 + 0.5 * (u[1] * prim[1] + u[2] * prim[2] + u[3] * prim[3]) +
+           0.5 * (prim[5] * prim[5] + prim[6] * prim[6] + prim[7] * prim[7]) + 0.5 * psi * psi;
+    for (int d = 0; d < 3; ++d) u[5 + d] = prim[5 + d];
+    u[8] = psi;
+}
+
+// ---------------------------------------------------------------------------
+// auxiliary kernels
+// ---------------------------------------------------------------------------
+__global__ void pack_kernel(double* __restrict__ dst, const double* __restrict__ src, int nv, int nn,
+                            int64_t E, int to_native) {
+    const int64_t total = (int64_t)nv * nn * E;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        // native index i = (v*nn + n)*E + e  <->  trixi index v + nv*(n + nn*e)
+        const int64_t e = i % E;
+        const int64_t r = i / E;
+        const int64_t n = r % nn, v = r / nn;
+        const int64_t t = v + nv * (n + nn * e);
+        if (to_native)
+            dst[i] = src[t];
+        else
+            dst[t] = src[i];
+    }
+}
+
+__global__ void coefficients_kernel(double* __restrict__ u, const double* __restrict__ xs, IcParams q,
+                                    int N, int nn, int64_t E, double t) {
+    const int64_t total = (int64_t)nn * E;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t e = i % E;
+        const int n = (int)(i / E);
+        double x[3] = {0, 0, 0};
+        for (int d = 0; d < q.dim; ++d) x[d] = xs[((int64_t)d * nn + n) * E + e];
+        if (q.dim == 1) {
+            if (n == 0) x[0] = nextafter(x[0], INFINITY);
+            if (n == N - 1) x[0] = nextafter(x[0], -INFINITY);
+        }
+        double uu[9];
+        initial_condition(q, x, t, (uint64_t)(e * nn + n), uu);
+        for (int v = 0; v < q.nv; ++v) u[((int64_t)v * nn + n) * E + e] = uu[v];
+    }
+}
+
+// Hennemann-Gassner indicator, one element per thread, FP64 (Trixi
+// calc_indicator_hennemann_gassner! with variable density_pressure).
+template <int DIM, int N>
+__global__ void indicator_kernel(const double* __restrict__ u, double* __restrict__ alpha_raw,
+                                 int64_t E, PhysParams phys, const double* __restrict__ Vinv_g,
+                                 double threshold, double parameter_s, double alpha_min,
+                                 double alpha_max) {
+    constexpr int NN = cpow(N, DIM);
+    const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (e >= E) return;
+    const PhysR<double> P = PhysR<double>::from(phys);
+    double Vinv[N * N];
+#pragma unroll
+    for (int q = 0; q < N * N; ++q) Vinv[q] = Vinv_g[q];
+    double ind[NN], tmp[NN];
+#pragma unroll
+    for (int n = 0; n < NN; ++n) {
+        double uc[DIM + 2];
+#pragma unroll
+        for (int v = 0; v < DIM + 2; ++v) uc[v] = u[((int64_t)v * NN + n) * E + e];
+        ind[n] = Euler<DIM>::density_pressure(uc, P);
+    }
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+        const int s = cpow(N, d);
+#pragma unroll
+        for (int n = 0; n < NN; ++n) {
+            const int idx = (n / s) % N, base = n - idx * s;
+            double r = 0.0;
+#pragma unroll
+            for (int ii = 0; ii < N; ++ii) r += Vinv[idx * N + ii] * ind[base + ii * s];
+            tmp[n] = r;
+        }
+#pragma unroll
+        for (int n = 0; n < NN; ++n) ind[n] = tmp[n];
+    }
+    double total = 0.0, clip1 = 0.0, clip2 = 0.0;
+#pragma unroll
+    for (int n = 0; n < NN; ++n) {
+        int maxc = 0;
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) {
+            const int c = (n / cpow(N, d)) % N;
+            maxc = c > maxc ? c : maxc;
+        }
+        const double e2 = ind[n] * ind[n];
+        total += e2;
+        if (maxc < N - 1) clip1 += e2;
+        if (maxc < N - 2) clip2 += e2;
+    }
+    const double frac1 = (total != 0.0) ? (total - clip1) / total : 0.0;
+    const double frac2 = (clip1 != 0.0) ? (clip1 - clip2) / clip1 : 0.0;
+    const double energy = frac2 > frac1 ? frac2 : frac1;
+    double a = 1.0 / (1.0 + exp(-parameter_s / threshold * (energy - threshold)));
+    if (a < alpha_min) a = 0.0;
+    if (a > 1.0 - alpha_min) a = 1.0;
+    alpha_raw[e] = a < alpha_max ? a : alpha_max;
+}
+
+__global__ void smooth_alpha_kernel(const double* __restrict__ raw, double* __restrict__ out,
+                                    const int32_t* __restrict__ nbr, int dim, int64_t E) {
+    const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (e >= E) return;
+    double a = raw[e];
+    for (int s = 0; s < 2 * dim; ++s) {
+        const double an = 0.5 * raw[nbr[(int64_t)s * E + e]];
+        a = an > a ? an : a;
+    }
+    out[e] = a;
+}
+
+__global__ void copy_kernel(double* __restrict__ dst, const double* __restrict__ src, int64_t n) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
+__global__ void set_time_kernel(double* time, double t, double dt, double c0, double c1, double c2,
+                                double c3, double c4) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        const double c[5] = {c0, c1, c2, c3, c4};
+        for (int s = 0; s < 5; ++s) {
+            time[2 * s] = t + c[s] * dt;
+            time[2 * s + 1] = dt;
+        }
+    }
+}
+
+// Per-element max_abs_speeds scaled by invJ, reduced per block (max).
+template <class EQ, int DIM>
+__global__ void max_speed_kernel(const double* __restrict__ u, const double* __restrict__ invJ,
+                                 double* __restrict__ partial, int nn, int64_t E, PhysParams phys) {
+    __shared__ double red[256];
+    const PhysR<double> P = PhysR<double>::from(phys);
+    double best = 0.0;
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < E;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        double lmax[3] = {0, 0, 0};
+        for (int n = 0; n < nn; ++n) {
+            double uc[EQ::NV], lam[3];
+            for (int v = 0; v < EQ::NV; ++v) uc[v] = u[((int64_t)v * nn + n) * E + e];
+            EQ::max_abs_speeds(uc, P, lam);
+            for (int d = 0; d < DIM; ++d) lmax[d] = lam[d] > lmax[d] ? lam[d] : lmax[d];
+        }
+        double s = 0.0;
+        for (int d = 0; d < DIM; ++d) s += lmax[d];
+        const double sc = invJ[e] * s;
+        best = sc > best ? sc : best;
+    }
+    red[threadIdx.x] = best;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + w]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
+// Deterministic final reduction of `nblocks` partial vectors of length m:
+// components [0, nsum) are summed, [nsum, m) take the maximum; one thread
+// per component, fixed order.
+__global__ void reduce_partials_kernel(const double* __restrict__ partial, double* __restrict__ out,
+                                       int nblocks, int m, int nsum) {
+    const int c = threadIdx.x;
+    if (c >= m) return;
+    const bool is_sum = c < nsum;
+    double acc = is_sum ? 0.0 : partial[c];
+    for (int b = is_sum ? 0 : 1; b < nblocks; ++b) {
+        const double v = partial[(int64_t)b * m + c];
+        acc = is_sum ? acc + v : fmax(acc, v);
+    }
+    out[c] = acc;
+}
+
+// Error norms: each thread handles analysis points (element, analysis node),
+// interpolating u and x with the (NA x N) Vandermonde of the 2N analysis
+// basis.  Block partials: [l2 (nv) | linf (nv)].
+template <int DIM>
+__global__ void error_norms_kernel(const double* __restrict__ u, const double* __restrict__ xs,
+                                   const double* __restrict__ invJ, double* __restrict__ partial,
+                                   const double* __restrict__ Vdm_g, const double* __restrict__ wa_g,
+                                   int N, int NA, int64_t E, IcParams q, double t, int nv) {
+    __shared__ double red[256 * 18];
+    const int nn = (int)lrint(pow((double)N, (double)DIM));
+    const int nna = (int)lrint(pow((double)NA, (double)DIM));
+    double l2[9], li[9];
+    for (int v = 0; v < nv; ++v) {
+        l2[v] = 0.0;
+        li[v] = 0.0;
+    }
+    const int64_t total = (int64_t)nna * E;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t e = i / nna;
+        const int a = (int)(i % nna);
+        int ai[3] = {0, 0, 0};
+        {
+            int r = a;
+            for (int d = 0; d < DIM; ++d) {
+                ai[d] = r % NA;
+                r /= NA;
+            }
+        }
+        double ua[9], xa[3] = {0, 0, 0};
+        for (int v = 0; v < nv; ++v) ua[v] = 0.0;
+        for (int n = 0; n < nn; ++n) {
+            double w = 1.0;
+            int r = n;
+            for (int d = 0; d < DIM; ++d) {
+                w *= Vdm_g[ai[d] * N + (r % N)];
+                r /= N;
+            }
+            if (w == 0.0) continue;
+            for (int v = 0; v < nv; ++v) ua[v] += w * u[((int64_t)v * nn + n) * E + e];
+            for (int d = 0; d < DIM; ++d) xa[d] += w * xs[((int64_t)d * nn + n) * E + e];
+        }
+        double uex[9];
+        initial_condition(q, xa, t, 0, uex);
+        double wq = 1.0;
+        for (int d = 0; d < DIM; ++d) wq *= wa_g[ai[d]];
+        const double vj = pow(1.0 / invJ[e], (double)DIM);
+        for (int v = 0; v < nv; ++v) {
+            const double diff = uex[v] - ua[v];
+            l2[v] += diff * diff * (wq * vj);
+            li[v] = fmax(li[v], fabs(diff));
+        }
+    }
+    for (int v = 0; v < nv; ++v) {
+        red[threadIdx.x * 18 + v] = l2[v];
+        red[threadIdx.x * 18 + 9 + v] = li[v];
+    }
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w)
+            for (int v = 0; v < nv; ++v) {
+                red[threadIdx.x * 18 + v] += red[(threadIdx.x + w) * 18 + v];
+                red[threadIdx.x * 18 + 9 + v] =
+                    fmax(red[threadIdx.x * 18 + 9 + v], red[(threadIdx.x + w) * 18 + 9 + v]);
+            }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int v = 0; v < nv; ++v) {
+            partial[(int64_t)blockIdx.x * 2 * nv + v] = red[v];
+            partial[(int64_t)blockIdx.x * 2 * nv + nv + v] = red[9 + v];
+        }
+}
+
+__global__ void integrate_kernel(const double* __restrict__ u, const double* __restrict__ invJ,
+                                 double* __restrict__ partial, const double* __restrict__ w_g, int N,
+                                 int dim, int64_t E, int nv) {
+    __shared__ double red[256 * 9];
+    int nn = 1;
+    for (int d = 0; d < dim; ++d) nn *= N;
+    double acc[9];
+    for (int v = 0; v < nv; ++v) acc[v] = 0.0;
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < E;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        const double vj = pow(1.0 / invJ[e], (double)dim);
+        for (int n = 0; n < nn; ++n) {
+            double w = 1.0;
+            int r = n;
+            for (int d = 0; d < dim; ++d) {
+                w *= w_g[r % N];
+                r /= N;
+            }
+            for (int v = 0; v < nv; ++v) acc[v] += vj * w * u[((int64_t)v * nn + n) * E + e];
+        }
+    }
+    for (int v = 0; v < nv; ++v) red[threadIdx.x * 9 + v] = acc[v];
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w)
+            for (int v = 0; v < nv; ++v) red[threadIdx.x * 9 + v] += red[(threadIdx.x + w) * 9 + v];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int v = 0; v < nv; ++v) partial[(int64_t)blockIdx.x * nv + v] = red[v];
+}
+
+// Positivity / finiteness flags per element (FP64).
+__global__ void check_state_kernel(const double* __restrict__ u, int32_t* __restrict__ flags,
+                                   double* __restrict__ partial, int eq, int dim, int nv, int nn,
+                                   int64_t E, PhysParams phys) {
+    __shared__ double red[256];
+    double cnt = 0.0;
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < E;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        int f = 0;
+        for (int n = 0; n < nn; ++n) {
+            double uc[9];
+            for (int v = 0; v < nv; ++v) {
+                uc[v] = u[((int64_t)v * nn + n) * E + e];
+                if (!isfinite(uc[v])) f |= 1;
+            }
+            if (eq == EQ_LINADV) continue;
+            if (!(uc[0] > 0.0)) f |= 2;
+            double ke = 0.0;
+            const int nvel = (eq == EQ_EULER) ? dim : 3;
+            for (int d = 0; d < nvel; ++d) ke += uc[1 + d] * uc[1 + d] / uc[0];
+            double p = uc[nvel + 1] - 0.5 * ke;
+            if (eq == EQ_GLM_MHD) p -= 0.5 * (uc[5] * uc[5] + uc[6] * uc[6] + uc[7] * uc[7] + uc[8] * uc[8]);
+            p *= phys.gm1;
+            if (!(p > 0.0)) f |= 4;
+        }
+        if (flags) flags[e] = f;
+        if (f) cnt += 1.0;
+    }
+    red[threadIdx.x] = cnt;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
+}  // namespace trixi_hip
+
+using namespace trixi_hip;
+
+// ---------------------------------------------------------------------------
+// context
+// ---------------------------------------------------------------------------
+struct trixi_hip_ctx {
+    trixi_hip_config cfg;
+    int dim, N, nv, nn;
+    int64_t E, I, nvalues;
+    double total_volume;
+    HostBasis basis;
+    // analysis basis
+    int NA;
+    // device buffers (all allocated in create)
+    int32_t* d_nbr = nullptr;
+    double* d_invJ = nullptr;
+    double* d_x = nullptr;
+    double* d_alpha_raw = nullptr;
+    double* d_alpha = nullptr;
+    double* d_time = nullptr;
+    double* d_partial = nullptr;
+    double* d_result = nullptr;
+    double* d_Vinv = nullptr;
+    double* d_Vdm = nullptr;
+    double* d_wa = nullptr;
+    double* d_w = nullptr;
+    double* d_tmp = nullptr;  // RK ping-pong state
+    int nblocks_red = 1024;
+    RhsArgs args;
+    LaunchFn rhs_fn = nullptr, rk_fn = nullptr;
+    // graph cache for rk steps
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    double* graph_u = nullptr;
+    double* graph_k = nullptr;
+    hipStream_t graph_stream = nullptr;
+    bool has_graph = false;
+};
+
+static const double CK_A[5] = {0.0, -567301805773.0 / 1357537059087.0, -2404267990393.0 / 2016746695238.0,
+                               -3550918686646.0 / 2091501179385.0, -1275806237668.0 / 842570457699.0};
+static const double CK_B[5] = {1432997174477.0 / 9575080441755.0, 5161836677717.0 / 13612068292357.0,
+                               1720146321549.0 / 2090206949498.0, 3134564353537.0 / 4481467310338.0,
+                               2277821191437.0 / 14882151754819.0};
+static const double CK_C[5] = {0.0, 1432997174477.0 / 9575080441755.0, 2526269341429.0 / 6820363962896.0,
+                               2006345519317.0 / 3224310063776.0, 2802321613138.0 / 2924317926251.0};
+
+static int nvars_of(const trixi_hip_config& c) {
+    if (c.equation == TRIXI_HIP_LINEAR_SCALAR_ADVECTION) return 1;
+    if (c.equation == TRIXI_HIP_COMPRESSIBLE_EULER) return c.ndims + 2;
+    return 9;
+}
+
+static KernelKey make_key(const trixi_hip_config& c, int rk) {
+    KernelKey k;
+    k.dim = c.ndims;
+    k.N = c.polydeg + 1;
+    k.eq = c.equation;
+    k.vol = c.volume_integral;
+    k.vf = (c.volume_integral == TRIXI_HIP_VOLUME_WEAK_FORM) ? FLUX_CENTRAL : c.volume_flux;
+    k.sf = c.surface_flux;
+    k.fvf = (c.volume_integral == TRIXI_HIP_VOLUME_SHOCK_CAPTURING_HG) ? c.fv_flux : FLUX_CENTRAL;
+    k.prec = c.precision;
+    k.rk = rk;
+    return k;
+}
+
+static void free_ctx(trixi_hip_ctx* c) {
+    if (!c) return;
+    if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
+    if (c->graph) (void)hipGraphDestroy(c->graph);
+    void* bufs[] = {c->d_nbr, c->d_invJ, c->d_x, c->d_alpha_raw, c->d_alpha, c->d_time, c->d_partial,
+                    c->d_result, c->d_Vinv, c->d_Vdm, c->d_wa, c->d_w, c->d_tmp};
+    for (void* p : bufs)
+        if (p) (void)hipFree(p);
+    delete c;
+}
+
+static int launch_indicator(trixi_hip_ctx* c, const double* u, hipStream_t s) {
+    const int threads = 128;
+    const unsigned blocks = (unsigned)((c->E + threads - 1) / threads);
+    const double threshold = 0.5 * pow(10.0, -1.8 * pow((double)c->N, 0.25));
+    const double parameter_s = log((1.0 - 0.0001) / 0.0001);
+    const PhysParams& P = c->args.phys;
+    if (c->N != 4) return fail(TRIXI_HIP_EUNSUPPORTED, "shock capturing is built for polydeg 3 only");
+    if (c->dim == 1)
+        hipLaunchKernelGGL((indicator_kernel<1, 4>), dim3(blocks), dim3(threads), 0, s, u, c->d_alpha_raw, c->E, P,
+                           c->d_Vinv, threshold, parameter_s, c->cfg.alpha_min, c->cfg.alpha_max);
+    else if (c->dim == 2)
+        hipLaunchKernelGGL((indicator_kernel<2, 4>), dim3(blocks), dim3(threads), 0, s, u, c->d_alpha_raw, c->E, P,
+                           c->d_Vinv, threshold, parameter_s, c->cfg.alpha_min, c->cfg.alpha_max);
+    else
+        hipLaunchKernelGGL((indicator_kernel<3, 4>), dim3(blocks), dim3(threads), 0, s, u, c->d_alpha_raw, c->E, P,
+                           c->d_Vinv, threshold, parameter_s, c->cfg.alpha_min, c->cfg.alpha_max);
+    HIP_TRY(hipGetLastError());
+    return TRIXI_HIP_OK;
+}
+
+extern "C" {
+
+const char* trixi_hip_version(void) { return "trixi_hip 0.1.0 (gfx950)"; }
+
+const char* trixi_hip_last_error(void) { return g_last_error.c_str(); }
+
+int trixi_hip_basis(int32_t nnodes, double* nodes, double* weights, double* derivative_matrix,
+                    double* derivative_split, double* derivative_dhat, double* boundary_interpolation,
+                    double* inverse_vandermonde_legendre) {
+    if (nnodes < 2 || nnodes > MAXN) return fail(TRIXI_HIP_EINVAL, "nnodes out of range");
+    const HostBasis b = make_basis(nnodes);
+    const int N = nnodes;
+    if (nodes) memcpy(nodes, b.x.data(), N * sizeof(double));
+    if (weights) memcpy(weights, b.w.data(), N * sizeof(double));
+    if (derivative_matrix) memcpy(derivative_matrix, b.D.data(), N * N * sizeof(double));
+    if (derivative_split) memcpy(derivative_split, b.Ds.data(), N * N * sizeof(double));
+    if (derivative_dhat) memcpy(derivative_dhat, b.Dhat.data(), N * N * sizeof(double));
+    if (boundary_interpolation)
+        for (int j = 0; j < N; ++j) {
+            boundary_interpolation[2 * j] = b.lm[j];
+            boundary_interpolation[2 * j + 1] = b.lp[j];
+        }
+    if (inverse_vandermonde_legendre) memcpy(inverse_vandermonde_legendre, b.Vinv.data(), N * N * sizeof(double));
+    return TRIXI_HIP_OK;
+}
+
+int trixi_hip_create(const trixi_hip_config* config, const trixi_hip_mesh* mesh, trixi_hip_ctx** out) {
+    if (!config || !mesh || !out) return fail(TRIXI_HIP_EINVAL, "null argument");
+    *out = nullptr;
+    ensure_registered();
+    const trixi_hip_config& cfg = *config;
+    if (cfg.ndims < 1 || cfg.ndims > 3) return fail(TRIXI_HIP_EINVAL, "ndims must be 1, 2 or 3");
+    if (cfg.polydeg < 1 || cfg.polydeg + 1 > MAXN) return fail(TRIXI_HIP_EINVAL, "polydeg out of range");
+    if (cfg.equation == TRIXI_HIP_IDEAL_GLM_MHD && cfg.ndims != 3)
+        return fail(TRIXI_HIP_EUNSUPPORTED, "GLM-MHD is built for 3D only");
+    if (cfg.volume_integral == TRIXI_HIP_VOLUME_SHOCK_CAPTURING_HG && cfg.equation != TRIXI_HIP_COMPRESSIBLE_EULER)
+        return fail(TRIXI_HIP_EUNSUPPORTED, "shock capturing is built for compressible Euler only");
+    const KernelKey key = make_key(cfg, 0);
+    LaunchFn fn = find_kernel(key), fn_rk = find_kernel(make_key(cfg, 1));
+    if (!fn || !fn_rk) {
+        char buf[256];
+        snprintf(buf, sizeof buf,
+                 "unsupported combination: ndims=%d equation=%d polydeg=%d volume_integral=%d volume_flux=%d "
+                 "surface_flux=%d fv_flux=%d precision=%d",
+                 cfg.ndims, cfg.equation, cfg.polydeg, cfg.volume_integral, cfg.volume_flux, cfg.surface_flux,
+                 cfg.fv_flux, cfg.precision);
+        return fail(TRIXI_HIP_EUNSUPPORTED, buf);
+    }
+    const int dim = cfg.ndims, N = cfg.polydeg + 1, nv = nvars_of(cfg);
+    int nn = 1;
+    for (int d = 0; d < dim; ++d) nn *= N;
+    const int64_t E = mesh->nelements, I = mesh->ninterfaces;
+    if (E <= 0 || I < 0 || !mesh->neighbor_ids || !mesh->orientations || !mesh->inverse_jacobian)
+        return fail(TRIXI_HIP_EINVAL, "incomplete mesh");
+    if (E > (int64_t)INT32_MAX / 2) return fail(TRIXI_HIP_EINVAL, "too many elements");
+    // Face neighbours from the interfaces; every face must be matched once
+    // (periodic, conforming TreeMesh).
+    std::vector<int32_t> nbr((size_t)2 * dim * E, -1);
+    for (int64_t i = 0; i < I; ++i) {
+        const int32_t L = mesh->neighbor_ids[2 * i], R = mesh->neighbor_ids[2 * i + 1];
+        const int o = mesh->orientations[i];
+        if (L < 0 || L >= E || R < 0 || R >= E || o < 1 || o > dim)
+            return fail(TRIXI_HIP_EINVAL, "interface " + std::to_string(i) + " out of range");
+        int32_t& lp = nbr[(size_t)(2 * (o - 1) + 1) * E + L];
+        int32_t& rm = nbr[(size_t)(2 * (o - 1)) * E + R];
+        if (lp != -1 || rm != -1)
+            return fail(TRIXI_HIP_EINVAL, "face matched twice (non-conforming meshes need mortars)");
+        lp = R;
+        rm = L;
+    }
+    for (size_t q = 0; q < nbr.size(); ++q)
+        if (nbr[q] < 0)
+            return fail(TRIXI_HIP_EUNSUPPORTED,
+                        "element face without interface: only periodic meshes are supported");
+
+    trixi_hip_ctx* c = new trixi_hip_ctx();
+    c->cfg = cfg;
+    c->dim = dim;
+    c->N = N;
+    c->nv = nv;
+    c->nn = nn;
+    c->E = E;
+    c->I = I;
+    c->nvalues = (int64_t)nv * nn * E;
+    c->total_volume = mesh->total_volume;
+    c->basis = make_basis(N);
+    c->NA = 2 * (N - 1) + 1;
+    c->rhs_fn = fn;
+    c->rk_fn = fn_rk;
+
+    auto cleanup_fail = [&](int code, const std::string& msg) {
+        free_ctx(c);
+        return fail(code, msg);
+    };
+#define ALLOC(ptr, bytes)                                                      \
+    do {                                                                       \
+        if (hipMalloc((void**)&(ptr), (bytes)) != hipSuccess)                  \
+            return cleanup_fail(TRIXI_HIP_ENOMEM, "hipMalloc failed for " #ptr); \
+    } while (0)
+    ALLOC(c->d_nbr, nbr.size() * sizeof(int32_t));
+    ALLOC(c->d_invJ, E * sizeof(double));
+    ALLOC(c->d_x, (size_t)dim * nn * E * sizeof(double));
+    ALLOC(c->d_alpha_raw, E * sizeof(double));
+    ALLOC(c->d_alpha, E * sizeof(double));
+    ALLOC(c->d_time, 16 * sizeof(double));
+    ALLOC(c->d_partial, (size_t)c->nblocks_red * 18 * sizeof(double));
+    ALLOC(c->d_result, 32 * sizeof(double));
+    ALLOC(c->d_Vinv, N * N * sizeof(double));
+    ALLOC(c->d_Vdm, c->NA * N * sizeof(double));
+    ALLOC(c->d_wa, c->NA * sizeof(double));
+    ALLOC(c->d_w, N * sizeof(double));
+    ALLOC(c->d_tmp, (size_t)c->nvalues * sizeof(double));
+#undef ALLOC
+    auto up = [&](void* dst, const void* src, size_t bytes) {
+        return hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice) == hipSuccess;
+    };
+    bool ok = up(c->d_nbr, nbr.data(), nbr.size() * sizeof(int32_t));
+    ok = ok && up(c->d_invJ, mesh->inverse_jacobian, E * sizeof(double));
+    if (mesh->node_coordinates) {
+        std::vector<double> xs((size_t)dim * nn * E);
+        for (int64_t e = 0; e < E; ++e)
+            for (int n = 0; n < nn; ++n)
+                for (int d = 0; d < dim; ++d)
+                    xs[((size_t)d * nn + n) * E + e] = mesh->node_coordinates[((size_t)e * nn + n) * dim + d];
+        ok = ok && up(c->d_x, xs.data(), xs.size() * sizeof(double));
+    } else {
+        ok = ok && hipMemset(c->d_x, 0, (size_t)dim * nn * E * sizeof(double)) == hipSuccess;
+    }
+    ok = ok && up(c->d_Vinv, c->basis.Vinv.data(), N * N * sizeof(double));
+    {
+        std::vector<double> xa, wa;
+        lgl(c->NA, xa, wa);
+        std::vector<double> V = interpolation_matrix(c->basis.x, c->basis.wbary, xa);
+        ok = ok && up(c->d_Vdm, V.data(), V.size() * sizeof(double));
+        ok = ok && up(c->d_wa, wa.data(), wa.size() * sizeof(double));
+    }
+    ok = ok && up(c->d_w, c->basis.w.data(), N * sizeof(double));
+    if (!ok) return cleanup_fail(TRIXI_HIP_EDEVICE, "host to device copy failed in create");
+
+    RhsArgs& a = c->args;
+    memset(&a, 0, sizeof(a));
+    a.nbr = c->d_nbr;
+    a.invJ = c->d_invJ;
+    a.alpha_raw = c->d_alpha_raw;
+    a.E = E;
+    a.phys.gamma = cfg.gamma;
+    a.phys.gm1 = cfg.gamma - 1.0;
+    a.phys.inv_gm1 = (cfg.gamma != 1.0) ? 1.0 / (cfg.gamma - 1.0) : 0.0;
+    a.phys.c_h = cfg.c_h;
+    for (int d = 0; d < 3; ++d) a.phys.a[d] = cfg.advection_velocity[d];
+    for (int q = 0; q < N * N; ++q) {
+        a.Ds[q] = c->basis.Ds[q];
+        a.Dhat[q] = c->basis.Dhat[q];
+    }
+    for (int q = 0; q < N; ++q) a.invw[q] = 1.0 / c->basis.w[q];
+    a.lhat_m = c->basis.lm[0];
+    a.lhat_p = c->basis.lp[N - 1];
+    {
+        const double eps = 2.220446049250313e-16;
+        a.atol = std::max(100 * eps, pow(eps, 0.75));
+    }
+    a.smooth = cfg.alpha_smooth;
+    *out = c;
+    return TRIXI_HIP_OK;
+}
+
+int trixi_hip_destroy(trixi_hip_ctx* ctx) {
+    if (!ctx) return fail(TRIXI_HIP_EINVAL, "null context");
+    free_ctx(ctx);
+    return TRIXI_HIP_OK;
+}
+
+int trixi_hip_sizes(const trixi_hip_ctx* ctx, int32_t* nvars, int32_t* nnodes, int64_t* nvalues) {
+    if (!ctx) return fail(TRIXI_HIP_EINVAL, "null context");
+    if (nvars) *nvars = ctx->nv;
+    if (nnodes) *nnodes = ctx->N;
+    if (nvalues) *nvalues = ctx->nvalues;
+    return TRIXI_HIP_OK;
+}
+
+int trixi_hip_rhs(trixi_hip_ctx* ctx, double* du, const double* u, double t, trixi_hip_stream stream) {
+    if (!ctx || !du || !u) return fail(TRIXI_HIP_EINVAL, "null argument");
+    if (du == u) return fail(TRIXI_HIP_EINVAL, "du and u must not alias");
+    (void)t;
+    hipStream_t s = (hipStream_t)stream;
+    if (ctx->cfg.volume_integral == TRIXI_HIP_VOLUME_SHOCK_CAPTURING_HG) {
+        int rc = launch_indicator(ctx, u, s);
+        if (rc) return rc;
+    }
+    RhsArgs a = ctx->args;
+    a.u = u;
+    a.du = du;
+    HIP_TRY(ctx->rhs_fn(a, s));
+    return TRIXI_HIP_OK;
+}
+
+int trixi_hip_pack(trixi_hip_ctx* ctx, double* u_native, const double* u_trixi, trixi_hip_stream stream) {
+    if (!ctx || !u_native || !u_trixi) return fail(TRIXI_HIP_EINVAL, "null argument");
+    hipLaunchKernelGGL(pack_kernel, dim3(1024), dim3(256), 0, (hipStream_t)stream, u_native, u_trixi, ctx->nv,
+                       ctx->nn, ctx->E, 1);
+    HIP_TRY(hipGetLastError());
+    return TRIXI_HIP_OK;
+}
+
+int trixi_hip_unpack(trixi_hip_ctx* ctx, double* u_trixi, const double* u_native, trixi_hip_stream stream) {
+    if (!ctx || !u_native || !u_trixi) return fail(TRIXI_HIP_EINVAL, "null argument");
+    hipLaunchKernelGGL(pack_kernel, dim3(1024), dim3(256), 0, (hipStream_t)stream, u_trixi, u_native, ctx->nv,
+                       ctx->nn, ctx->E, 0);
+    HIP_TRY(hipGetLastError());
+    return TRIXI_HIP_OK;
+}
+
+static IcParams ic_params(const trixi_hip_ctx* c, int ic, uint64_t seed) {
+    IcParams q;
+    q.dim = c->dim;
+    q.eq = c->cfg.equation;
+    q.nv = c->nv;
+    q.ic = ic;
+    q.gamma = c->cfg.gamma;
+    q.inv_gm1 = c->args.phys.inv_gm1;
+    q.c_h = c->cfg.c_h;
+    for (int d = 0; d < 3; ++d) q.a[d] = c->cfg.advection_velocity[d];
+    q.length = c->cfg.ic_length;
+    q.seed = seed;
+    return q;
+}
+
+int trixi_hip_compute_coefficients(trixi_hip_ctx* ctx, double* u, int32_t ic, double t, uint64_t seed,
+                                   trixi_hip_stream stream) {
+    if (!ctx || !u) return fail(TRIXI_HIP_EINVAL, "null argument");
+    if (ic < 0 || ic > TRIXI_HIP_IC_SMOOTH_PERIODIC) return fail(TRIXI_HIP_EINVAL, "unknown initial condition");
+    hipLaunchKernelGGL(coefficients_kernel, dim3(1024), dim3(256), 0, (hipStream_t)stream, u, ctx->d_x,
+                       ic_params(ctx, ic, seed), ctx->N, ctx->nn, ctx->E, t);
+    HIP_TRY(hipGetLastError());
+    return TRIXI_HIP_OK;
+}
+
+// The five stages of CarpenterKennedy2N54, each one fused rhs+update
+// launch (plus the HG indicator when shock capturing).  Stages alternate
+// between the caller's u and the context's ping-pong buffer; the last
+// stage's result is copied back to u.
+static int enqueue_rk_stages(trixi_hip_ctx* c, double* u, double* k, hipStream_t s) {
+    double* cur = u;
+    double* nxt = c->d_tmp;
+    for (int st = 0; st < 5; ++st) {
+        if (c->cfg.volume_integral == TRIXI_HIP_VOLUME_SHOCK_CAPTURING_HG) {
+            int rc = launch_indicator(c, cur, s);
+            if (rc) return rc;
+        }
+        RhsArgs a = c->args;
+        a.u = cur;
+        a.k = k;
+        a.u_out = nxt;
+        a.time = c->d_time + 2 * st;
+        a.rk_a = CK_A[st];
+        a.rk_b = CK_B[st];
+        a.rk_first = (st == 0);
+        HIP_TRY(c->rk_fn(a, s));
+        double* t = cur;
+        cur = nxt;
+        nxt = t;
+    }
+    if (cur != u) {
+        hipLaunchKernelGGL(copy_kernel, dim3(1024), dim3(256), 0, s, u, cur, c->nvalues);
+        HIP_TRY(hipGetLastError());
+    }
+    return TRIXI_HIP_OK;
+}
+
+int trixi_hip_rk_step(trixi_hip_ctx* ctx, double* u, double* k, double t, double dt, int32_t use_graph,
+                      trixi_hip_stream stream) {
+    if (!ctx || !u || !k) return fail(TRIXI_HIP_EINVAL, "null argument");
+    hipStream_t s = (hipStream_t)stream;
+    hipLaunchKernelGGL(set_time_kernel, dim3(1), dim3(1), 0, s, ctx->d_time, t, dt, CK_C[0], CK_C[1], CK_C[2],
+                       CK_C[3], CK_C[4]);
+    HIP_TRY(hipGetLastError());
+    if (!use_graph) return enqueue_rk_stages(ctx, u, k, s);
+    if (!s) return fail(TRIXI_HIP_EINVAL, "graph replay needs an explicit (non-default) stream");
+    if (!ctx->has_graph || ctx->graph_u != u || ctx->graph_k != k) {
+        if (ctx->graph_exec) {
+            HIP_TRY(hipGraphExecDestroy(ctx->graph_exec));
+            ctx->graph_exec = nullptr;
+        }
+        if (ctx->graph) {
+            HIP_TRY(hipGraphDestroy(ctx->graph));
+            ctx->graph = nullptr;
+        }
+        ctx->has_graph = false;
+        HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        int rc = enqueue_rk_stages(ctx, u, k, s);
+        hipGraph_t g = nullptr;
+        hipError_t ec = hipStreamEndCapture(s, &g);
+        if (rc) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc;
+        }
+        if (ec != hipSuccess) return fail(TRIXI_HIP_EDEVICE, std::string("graph capture: ") + hipGetErrorString(ec));
+        ctx->graph = g;
+        HIP_TRY(hipGraphInstantiate(&ctx->graph_exec, g, nullptr, nullptr, 0));
+        ctx->graph_u = u;
+        ctx->graph_k = k;
+        ctx->has_graph = true;
+    }
+    HIP_TRY(hipGraphLaunch(ctx->graph_exec, s));
+    return TRIXI_HIP_OK;
+}
+
+int trixi_hip_max_dt(trixi_hip_ctx* ctx, const double* u, double* max_dt, trixi_hip_stream stream) {
+    if (!ctx || !u || !max_dt) return fail(TRIXI_HIP_EINVAL, "null argument");
+    hipStream_t s = (hipStream_t)stream;
+    const int threads = 256;
+    int blocks = (int)std::min<int64_t>((ctx->E + threads - 1) / threads, ctx->nblocks_red);
+    const PhysParams& P = ctx->args.phys;
+    const int eq = ctx->cfg.equation, dim = ctx->dim;
+#define MS(EQT, D) \
+    hipLaunchKernelGGL((max_speed_kernel<EQT, D>), dim3(blocks), dim3(threads), 0, s, u, ctx->d_invJ, ctx->d_partial, ctx->nn, ctx->E, P)
+    if (eq == TRIXI_HIP_LINEAR_SCALAR_ADVECTION) {
+        if (dim == 1) MS(LinAdv<1>, 1); else if (dim == 2) MS(LinAdv<2>, 2); else MS(LinAdv<3>, 3);
+    } else if (eq == TRIXI_HIP_COMPRESSIBLE_EULER) {
+        if (dim == 1) MS(Euler<1>, 1); else if (dim == 2) MS(Euler<2>, 2); else MS(Euler<3>, 3);
+    } else {
+        MS(GlmMhd3D, 3);
+    }
+#undef MS
+    HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(reduce_partials_kernel, dim3(1), dim3(32), 0, s, ctx->d_partial, ctx->d_result, blocks, 1, 0);
+    HIP_TRY(hipGetLastError());
+    double m = 0.0;
+    HIP_TRY(hipMemcpyAsync(&m, ctx->d_result, sizeof(double), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    const double tiny = nextafter(0.0, 1.0);
+    if (!(m > tiny)) m = tiny;
+    *max_dt = 2.0 / (ctx->N * m);
+    return TRIXI_HIP_OK;
+}
+
+int trixi_hip_error_norms(trixi_hip_ctx* ctx, const double* u, int32_t ic, double t, uint64_t seed, double* l2,
+                          double* linf, trixi_hip_stream stream) {
+    if (!ctx || !u || !l2 || !linf) return fail(TRIXI_HIP_EINVAL, "null argument");
+    hipStream_t s = (hipStream_t)stream;
+    const int threads = 256;
+    const int64_t npts = (int64_t)ctx->E * (int64_t)lrint(pow((double)ctx->NA, (double)ctx->dim));
+    const int blocks = (int)std::min<int64_t>((npts + threads - 1) / threads, ctx->nblocks_red);
+    const IcParams q = ic_params(ctx, ic, seed);
+    const int nv = ctx->nv;
+#define EN(D)                                                                                                   \
+    hipLaunchKernelGGL((error_norms_kernel<D>), dim3(blocks), dim3(threads), 0, s, u, ctx->d_x, ctx->d_invJ,    \
+                       ctx->d_partial, ctx->d_Vdm, ctx->d_wa, ctx->N, ctx->NA, ctx->E, q, t, nv)
+    if (ctx->dim == 1) EN(1); else if (ctx->dim == 2) EN(2); else EN(3);
+#undef EN
+    HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(reduce_partials_kernel, dim3(1), dim3(32), 0, s, ctx->d_partial, ctx->d_result, blocks,
+                       2 * nv, nv);
+    HIP_TRY(hipGetLastError());
+    double res[18];
+    HIP_TRY(hipMemcpyAsync(res, ctx->d_result, 2 * nv * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    for (int v = 0; v < nv; ++v) {
+        l2[v] = sqrt(res[v] / ctx->total_volume);
+        linf[v] = res[nv + v];
+    }
+    return TRIXI_HIP_OK;
+}
+
+int trixi_hip_integrate(trixi_hip_ctx* ctx, const double* u, double* integral, trixi_hip_stream stream) {
+    if (!ctx || !u || !integral) return fail(TRIXI_HIP_EINVAL, "null argument");
+    hipStream_t s = (hipStream_t)stream;
+    const int threads = 256;
+    const int blocks = (int)std::min<int64_t>((ctx->E + threads - 1) / threads, ctx->nblocks_red);
+    hipLaunchKernelGGL(integrate_kernel, dim3(blocks), dim3(threads), 0, s, u, ctx->d_invJ, ctx->d_partial,
+                       ctx->d_w, ctx->N, ctx->dim, ctx->E, ctx->nv);
+    HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(reduce_partials_kernel, dim3(1), dim3(32), 0, s, ctx->d_partial, ctx->d_result, blocks,
+                       ctx->nv, ctx->nv);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(integral, ctx->d_result, ctx->nv * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    return TRIXI_HIP_OK;
+}
+
+int trixi_hip_indicator(trixi_hip_ctx* ctx, const double* u, double* alpha, trixi_hip_stream stream) {
+    if (!ctx || !u || !alpha) return fail(TRIXI_HIP_EINVAL, "null argument");
+    if (ctx->cfg.equation != TRIXI_HIP_COMPRESSIBLE_EULER)
+        return fail(TRIXI_HIP_EUNSUPPORTED, "the HG indicator is built for compressible Euler only");
+    hipStream_t s = (hipStream_t)stream;
+    int rc = launch_indicator(ctx, u, s);
+    if (rc) return rc;
+    if (ctx->cfg.alpha_smooth) {
+        hipLaunchKernelGGL(smooth_alpha_kernel, dim3((unsigned)((ctx->E + 255) / 256)), dim3(256), 0, s,
+                           ctx->d_alpha_raw, alpha, ctx->d_nbr, ctx->dim, ctx->E);
+    } else {
+        hipLaunchKernelGGL(copy_kernel, dim3(256), dim3(256), 0, s, alpha, ctx->d_alpha_raw, ctx->E);
+    }
+    HIP_TRY(hipGetLastError());
+    return TRIXI_HIP_OK;
+}
+
+int trixi_hip_check_state(trixi_hip_ctx* ctx, const double* u, int32_t* flags, int64_t* nbad,
+                          trixi_hip_stream stream) {
+    if (!ctx || !u || !nbad) return fail(TRIXI_HIP_EINVAL, "null argument");
+    hipStream_t s = (hipStream_t)stream;
+    const int threads = 256;
+    const int blocks = (int)std::min<int64_t>((ctx->E + threads - 1) / threads, ctx->nblocks_red);
+    hipLaunchKernelGGL(check_state_kernel, dim3(blocks), dim3(threads), 0, s, u, flags, ctx->d_partial,
+                       ctx->cfg.equation, ctx->dim, ctx->nv, ctx->nn, ctx->E, ctx->args.phys);
+    HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(reduce_partials_kernel, dim3(1), dim3(32), 0, s, ctx->d_partial, ctx->d_result, blocks, 1, 1);
+    HIP_TRY(hipGetLastError());
+    double cnt = 0.0;
+    HIP_TRY(hipMemcpyAsync(&cnt, ctx->d_result, sizeof(double), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    *nbad = (int64_t)cnt;
+    if (*nbad > 0) return fail(TRIXI_HIP_ESTATE, std::to_string(*nbad) + " element(s) with invalid state");
+    return TRIXI_HIP_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,160 @@
+"""Equations, numerical fluxes and initial conditions (host-side identities).
+
+A Julia/Trixi function object cannot cross a C ABI, so each supported
+Trixi.jl equation, flux and initial condition is an object carrying the
+enum value the library understands (reference: the fluxes the tests pass to
+DGSEMGPU, e.g. test/tree_dgsem_3d/euler_ec.jl:13-17,
+test/tree_dgsem_3d/mhd_alfven_wave.jl:13-18, test/tree_dgsem_2d/euler_shock.jl:12-27).
+"""
+from dataclasses import dataclass, field
+
+from . import _lib as L
+
+
+# ----------------------------------------------------------------------------
+# equations
+# ----------------------------------------------------------------------------
+@dataclass(frozen=True)
+class AbstractEquations:
+    ndims: int
+    equation_id: int
+
+    @property
+    def nvariables(self):
+        if self.equation_id == L.EQ_LINADV:
+            return 1
+        if self.equation_id == L.EQ_EULER:
+            return self.ndims + 2
+        return 9
+
+
+def _linadv(ndims):
+    @dataclass(frozen=True)
+    class _LinAdv(AbstractEquations):
+        advection_velocity: tuple = ()
+
+        def __init__(self, advection_velocity):
+            vel = tuple(float(a) for a in (advection_velocity if hasattr(advection_velocity, "__len__")
+                                           else (advection_velocity,)))
+            if len(vel) != ndims:
+                raise ValueError(f"advection velocity must have {ndims} components")
+            object.__setattr__(self, "ndims", ndims)
+            object.__setattr__(self, "equation_id", L.EQ_LINADV)
+            object.__setattr__(self, "advection_velocity", vel)
+
+    _LinAdv.__name__ = _LinAdv.__qualname__ = f"LinearScalarAdvectionEquation{ndims}D"
+    return _LinAdv
+
+
+LinearScalarAdvectionEquation1D = _linadv(1)
+LinearScalarAdvectionEquation2D = _linadv(2)
+LinearScalarAdvectionEquation3D = _linadv(3)
+
+
+def _euler(ndims):
+    @dataclass(frozen=True)
+    class _Euler(AbstractEquations):
+        gamma: float = 1.4
+
+        def __init__(self, gamma):
+            object.__setattr__(self, "ndims", ndims)
+            object.__setattr__(self, "equation_id", L.EQ_EULER)
+            object.__setattr__(self, "gamma", float(gamma))
+
+    _Euler.__name__ = _Euler.__qualname__ = f"CompressibleEulerEquations{ndims}D"
+    return _Euler
+
+
+CompressibleEulerEquations1D = _euler(1)
+CompressibleEulerEquations2D = _euler(2)
+CompressibleEulerEquations3D = _euler(3)
+
+
+@dataclass(frozen=True)
+class IdealGlmMhdEquations3D(AbstractEquations):
+    gamma: float = 5.0 / 3.0
+    c_h: float = 1.0
+
+    def __init__(self, gamma, initial_c_h=1.0):
+        object.__setattr__(self, "ndims", 3)
+        object.__setattr__(self, "equation_id", L.EQ_GLM_MHD)
+        object.__setattr__(self, "gamma", float(gamma))
+        object.__setattr__(self, "c_h", float(initial_c_h))
+
+
+# ----------------------------------------------------------------------------
+# fluxes
+# ----------------------------------------------------------------------------
+@dataclass(frozen=True)
+class NumericalFlux:
+    name: str
+    flux_id: int
+
+    def __repr__(self):
+        return self.name
+
+
+def max_abs_speed(*args):  # identity marker of Trixi's max_abs_speed
+    raise TypeError("wave-speed estimates are selected, not called, on the HIP path")
+
+
+def max_abs_speed_naive(*args):  # identity marker of Trixi's max_abs_speed_naive
+    raise TypeError("wave-speed estimates are selected, not called, on the HIP path")
+
+
+def FluxLaxFriedrichs(max_abs_speed_fn=max_abs_speed):
+    if max_abs_speed_fn is max_abs_speed_naive:
+        return NumericalFlux("FluxLaxFriedrichs(max_abs_speed_naive)", L.FLUX_LLF_NAIVE)
+    if max_abs_speed_fn is max_abs_speed:
+        return NumericalFlux("FluxLaxFriedrichs(max_abs_speed)", L.FLUX_LLF)
+    raise NotImplementedError("unsupported wave-speed estimate")
+
+
+flux_central = NumericalFlux("flux_central", L.FLUX_CENTRAL)
+flux_lax_friedrichs = FluxLaxFriedrichs(max_abs_speed)
+flux_ranocha = NumericalFlux("flux_ranocha", L.FLUX_RANOCHA)
+flux_shima_etal = NumericalFlux("flux_shima_etal", L.FLUX_SHIMA_ETAL)
+flux_hindenlang_gassner = NumericalFlux("flux_hindenlang_gassner", L.FLUX_HINDENLANG_GASSNER)
+flux_nonconservative_powell = NumericalFlux("flux_nonconservative_powell", -1)
+
+
+def density_pressure(*args):  # identity marker of the HG indicator variable
+    raise TypeError("indicator variables are selected, not called, on the HIP path")
+
+
+# ----------------------------------------------------------------------------
+# initial conditions
+# ----------------------------------------------------------------------------
+@dataclass(frozen=True)
+class InitialCondition:
+    name: str
+    ic_id: int
+    seed: int = 0
+
+    def __repr__(self):
+        return self.name
+
+
+initial_condition_constant = InitialCondition("initial_condition_constant", L.IC_CONSTANT)
+initial_condition_convergence_test = InitialCondition("initial_condition_convergence_test", L.IC_CONVERGENCE_TEST)
+initial_condition_weak_blast_wave = InitialCondition("initial_condition_weak_blast_wave", L.IC_WEAK_BLAST_WAVE)
+
+
+def initial_condition_seeded_random(seed=20261202):
+    """rho = 1 + 0.1 U(-1,1), v = 0.1 N(0,1), p = 1 + 0.05 U(-1,1) per node,
+    from a counter-based generator keyed by (seed, global node index)."""
+    return InitialCondition(f"initial_condition_seeded_random(seed={seed})", L.IC_SEEDED_RANDOM, int(seed))
+
+
+def initial_condition_smooth_periodic(seed=20261202):
+    """Seeded sums of three periodic sine modes per primitive variable."""
+    return InitialCondition(f"initial_condition_smooth_periodic(seed={seed})", L.IC_SMOOTH_PERIODIC, int(seed))
+
+
+def split_flux(flux):
+    """Split a (conservative, nonconservative) tuple as Trixi accepts it."""
+    if isinstance(flux, tuple):
+        if len(flux) != 2 or flux[1] is not flux_nonconservative_powell:
+            raise NotImplementedError(f"unsupported flux tuple {flux}")
+        return flux[0], flux[1]
+    return flux, None

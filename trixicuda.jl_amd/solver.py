@@ -1,0 +1,133 @@
+"""DGSEM solver description: basis, volume integrals, indicator, DGSEMHIP.
+
+HIP-named twins of the reference constructors LobattoLegendreBasisGPU
+(src/solvers/basis_lobatto_legendre.jl:25-100) and DGSEMGPU
+(src/solvers/dgsem_gpu.jl:6-53).  The matrices are computed by the library's
+host code (trixi_hip_basis) so host and device agree bit for bit.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib as L
+from .equations import density_pressure, flux_central, flux_ranocha, split_flux
+
+
+class LobattoLegendreBasisHIP:
+    def __init__(self, polydeg, RealT=np.float64):
+        if polydeg < 1 or polydeg > 7:
+            raise ValueError("polydeg must be in 1..7")
+        N = polydeg + 1
+        self.polydeg, self.nnodes, self.RealT = polydeg, N, RealT
+        self.nodes = np.zeros(N)
+        self.weights = np.zeros(N)
+        self.derivative_matrix = np.zeros((N, N))
+        self.derivative_split = np.zeros((N, N))
+        self.derivative_dhat = np.zeros((N, N))
+        self.boundary_interpolation = np.zeros((N, 2))
+        self.inverse_vandermonde_legendre = np.zeros((N, N))
+        p = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+        L.check(L.lib().trixi_hip_basis(N, p(self.nodes), p(self.weights), p(self.derivative_matrix),
+                                        p(self.derivative_split), p(self.derivative_dhat),
+                                        p(self.boundary_interpolation), p(self.inverse_vandermonde_legendre)))
+        self.inverse_weights = 1.0 / self.weights
+        self.derivative_split_transpose = self.derivative_split.T.copy()
+
+    def __repr__(self):
+        return f"LobattoLegendreBasisHIP(polydeg={self.polydeg})"
+
+
+class VolumeIntegralWeakForm:
+    kind = L.VI_WEAK
+
+
+class VolumeIntegralFluxDifferencing:
+    kind = L.VI_FD
+
+    def __init__(self, volume_flux=flux_ranocha):
+        self.volume_flux = volume_flux
+
+
+class IndicatorHennemannGassner:
+    def __init__(self, equations, basis, alpha_max=0.5, alpha_min=0.001, alpha_smooth=True,
+                 variable=density_pressure):
+        if variable is not density_pressure:
+            raise NotImplementedError("only the density_pressure indicator variable is built")
+        self.alpha_max, self.alpha_min, self.alpha_smooth = float(alpha_max), float(alpha_min), bool(alpha_smooth)
+        self.variable = variable
+
+
+class VolumeIntegralShockCapturingHG:
+    kind = L.VI_HG
+
+    def __init__(self, indicator, volume_flux_dg=flux_ranocha, volume_flux_fv=None):
+        self.indicator = indicator
+        self.volume_flux_dg = volume_flux_dg
+        self.volume_flux_fv = volume_flux_fv
+
+
+class DGSEMHIP:
+    """DGSEMHIP(; polydeg, surface_flux=flux_central, volume_integral=VolumeIntegralWeakForm(),
+    RealT=np.float64, precision=None).
+
+    precision "fp64" computes everything in FP64; "mixed" keeps the state,
+    the logarithmic means, the RK update and the shock indicator in FP64 and
+    does the volume/surface arithmetic in FP32.  RealT=np.float32 selects
+    "mixed" (the reference's Float32 mode)."""
+
+    def __init__(self, polydeg, surface_flux=flux_central, volume_integral=None, RealT=np.float64,
+                 precision=None):
+        self.basis = LobattoLegendreBasisHIP(polydeg, RealT)
+        self.surface_flux = surface_flux
+        self.volume_integral = volume_integral if volume_integral is not None else VolumeIntegralWeakForm()
+        if precision is None:
+            precision = "mixed" if RealT == np.float32 else "fp64"
+        if precision not in ("fp64", "mixed"):
+            raise ValueError("precision must be 'fp64' or 'mixed'")
+        self.precision = precision
+
+    @property
+    def polydeg(self):
+        return self.basis.polydeg
+
+    def config(self, equations, ic_length=4.0):
+        """Fill the C-ABI configuration for these equations."""
+        c = L.Config()
+        c.ndims = equations.ndims
+        c.equation = equations.equation_id
+        c.polydeg = self.polydeg
+        vi = self.volume_integral
+        c.volume_integral = vi.kind
+        sflux, s_nc = split_flux(self.surface_flux)
+        c.surface_flux = sflux.flux_id
+        nc_needed = equations.equation_id == L.EQ_GLM_MHD
+        if nc_needed != (s_nc is not None):
+            raise NotImplementedError("GLM-MHD needs (flux, flux_nonconservative_powell) tuples, "
+                                      "other equations plain fluxes")
+        c.volume_flux = L.FLUX_CENTRAL
+        c.fv_flux = L.FLUX_CENTRAL
+        c.alpha_smooth = 0
+        c.alpha_max, c.alpha_min = 0.5, 0.001
+        if vi.kind == L.VI_FD:
+            vflux, v_nc = split_flux(vi.volume_flux)
+            if nc_needed != (v_nc is not None):
+                raise NotImplementedError("GLM-MHD volume flux must be (flux, flux_nonconservative_powell)")
+            c.volume_flux = vflux.flux_id
+        elif vi.kind == L.VI_HG:
+            c.volume_flux = split_flux(vi.volume_flux_dg)[0].flux_id
+            fv = vi.volume_flux_fv if vi.volume_flux_fv is not None else sflux
+            c.fv_flux = split_flux(fv)[0].flux_id
+            c.alpha_smooth = 1 if vi.indicator.alpha_smooth else 0
+            c.alpha_max, c.alpha_min = vi.indicator.alpha_max, vi.indicator.alpha_min
+        c.precision = L.PREC_MIXED if self.precision == "mixed" else L.PREC_FP64
+        c.gamma = getattr(equations, "gamma", 1.4)
+        c.c_h = getattr(equations, "c_h", 0.0)
+        vel = getattr(equations, "advection_velocity", ())
+        for d in range(3):
+            c.advection_velocity[d] = vel[d] if d < len(vel) else 0.0
+        c.ic_length = ic_length
+        return c
+
+    def __repr__(self):
+        return (f"DGSEMHIP(polydeg={self.polydeg}, surface_flux={self.surface_flux}, "
+                f"volume_integral={type(self.volume_integral).__name__}, precision={self.precision})")

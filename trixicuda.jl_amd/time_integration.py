@@ -1,0 +1,70 @@
+"""CarpenterKennedy2N54 time integration on the device.
+
+The reference hands the ODE problem to OrdinaryDiffEq
+(`solve(ode, CarpenterKennedy2N54(williamson_condition = false); dt, callback)`,
+e.g. examples/euler_ec_3d.jl:53-55).  Here each step is one library call:
+five fused rhs+register-update launches, replayed from a hipGraph.
+"""
+from . import _lib as L
+from .callbacks import AnalysisCallback, StepsizeCallback
+from .semidiscretization import _ptr
+
+
+class CarpenterKennedy2N54:
+    def __init__(self, williamson_condition=False):
+        self.williamson_condition = williamson_condition
+
+
+class Solution:
+    def __init__(self, u, t, nsteps, semi):
+        self.u, self.t, self.nsteps, self.semi = u, t, nsteps, semi
+
+    def __repr__(self):
+        return f"Solution(t={self.t}, nsteps={self.nsteps})"
+
+
+def rk_step(u, k, t, dt, semi, use_graph=True, stream=None):
+    """Advance u by one CK2N54 step in place; k is the 2N register."""
+    import ctypes
+    import torch
+    semi.check_vector(u, "u")
+    semi.check_vector(k, "k")
+    s = stream if stream is not None else torch.cuda.current_stream()
+    if use_graph and s.cuda_stream == 0:
+        raise L.TrixiHipError(L.EINVAL, "graph replay needs a non-default stream")
+    L.check(L.lib().trixi_hip_rk_step(semi.ctx, _ptr(u), _ptr(k), float(t), float(dt), 1 if use_graph else 0,
+                                      ctypes.c_void_p(s.cuda_stream)))
+
+
+def solve(ode, alg=None, dt=None, callback=(), use_graph=True, maxiters=10**7):
+    """Integrate ode over its tspan; returns a Solution with the final state."""
+    import torch
+    if alg is not None and not isinstance(alg, CarpenterKennedy2N54):
+        raise NotImplementedError("only CarpenterKennedy2N54 is built")
+    semi = ode.p
+    cbs = list(callback) if isinstance(callback, (list, tuple)) else [callback]
+    stepsize = next((c for c in cbs if isinstance(c, StepsizeCallback)), None)
+    analysis = [c for c in cbs if isinstance(c, AnalysisCallback)]
+    if dt is None and stepsize is None:
+        raise ValueError("give dt or a StepsizeCallback")
+    t0, t1 = ode.tspan
+    u = ode.u0.clone()
+    k = torch.zeros_like(u)
+    stream = torch.cuda.Stream()
+    stream.wait_stream(torch.cuda.current_stream())
+    t, nsteps = t0, 0
+    with torch.cuda.stream(stream):
+        while t < t1 and nsteps < maxiters:
+            h = stepsize.dt(u, semi) if stepsize is not None else dt
+            if t + h > t1:
+                h = t1 - t
+            rk_step(u, k, t, h, semi, use_graph=use_graph, stream=stream)
+            t = t + h
+            nsteps += 1
+            for a in analysis:
+                a(u, t, nsteps)
+        for a in analysis:
+            a(u, t, nsteps, final=True)
+    torch.cuda.current_stream().wait_stream(stream)
+    stream.synchronize()
+    return Solution(u, t, nsteps, semi)
