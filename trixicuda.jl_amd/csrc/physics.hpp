@@ -6,13 +6,19 @@
 // The flux algebra follows the Trixi.jl definitions that TrixiCUDA.jl's
 // kernels inline (reference: fluxes passed in by the tests, e.g.
 // test/tree_dgsem_3d/euler_ec.jl:13-17, test/tree_dgsem_3d/mhd_alfven_wave.jl:13-18;
-// call sites dg_3d_kernel.jl:221-234, 1155-1203).  Orientation o is 0-based here.
+// call sites dg_3d_kernel.jl:221-234, 1155-1203).  Orientation o is 0-based.
+//
+// Node data.  Each equation defines a Node (primitive variables plus, where
+// an entropy-conservative flux needs them, log(rho) and log(p)) built once
+// per node from the conservative state; the conservative variables are
+// reconstructed on demand (Node::cons) where a flux needs them.
 #pragma once
 
 #include <hip/hip_runtime.h>
 
 #include <cmath>
 #include <cstdint>
+#include <type_traits>
 
 namespace trixi_hip {
 
@@ -46,43 +52,68 @@ struct PhysR {
     }
 };
 
-__device__ __forceinline__ double dsqrt(double x) { return sqrt(x); }
-__device__ __forceinline__ float dsqrt(float x) { return sqrtf(x); }
+// ---------------------------------------------------------------------------
+// arithmetic helpers: IEEE division / square root in FP64 mode; in the mixed
+// mode the FP32 quotient is n * rcp(d) and the square root the hardware
+// v_sqrt_f32 (both within 2 ulp of FP32, far below the mixed-mode error).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double rdiv(double n, double d) { return n / d; }
+__device__ __forceinline__ float rdiv(float n, float d) { return n * __builtin_amdgcn_rcpf(d); }
+__device__ __forceinline__ double rsqrt_(double x) { return sqrt(x); }
+__device__ __forceinline__ float rsqrt_(float x) { return __builtin_amdgcn_sqrtf(x); }
 __device__ __forceinline__ double dabs(double x) { return fabs(x); }
 __device__ __forceinline__ float dabs(float x) { return fabsf(x); }
 template <typename R> __device__ __forceinline__ R dmax(R a, R b) { return b > a ? b : a; }
 
-// ---------------------------------------------------------------------------
-// Logarithmic mean from precomputed logarithms.  ln_mean(x, y) equals
-// (y - x) / log(y / x) away from x = y and the 4-term series in
-// f2 = ((x - y) / (x + y))^2 below f2 = 1e-4 (Trixi.jl ln_mean).  The branch
-// test is done without a division (den > 0 for positive x, y).  In the
-// closed branch |log(y/x)| > 0.02, so log(y) - log(x) has a relative error of
-// a few 1e-14, far inside the FP64 parity bound.
-// ---------------------------------------------------------------------------
-struct LnMean {
-    double mean;      // ln_mean(x, y)
-    double inv_mean;  // inv_ln_mean(x, y)
-};
-
-__device__ __forceinline__ double ln_mean_from_logs(double x, double y, double lx, double ly) {
-    const double num = x * (x - 2.0 * y) + y * y;
-    const double den = x * (x + 2.0 * y) + y * y;
-    if (num < 1.0e-4 * den) {
-        const double f2 = num / den;
-        return (x + y) / (2.0 + f2 * (2.0 / 3.0 + f2 * (2.0 / 5.0 + f2 * (2.0 / 7.0))));
+// FP64 quotient used inside the logarithmic means: IEEE in FP64 mode; in the
+// mixed mode (whose result is rounded to FP32) v_rcp_f64 refined by one
+// Newton step, accurate to far better than FP32.
+template <typename R>
+__device__ __forceinline__ double ldiv(double n, double d) {
+    if constexpr (std::is_same<R, double>::value) {
+        return n / d;
+    } else {
+        double r = __builtin_amdgcn_rcp(d);
+        r = fma(fma(-d, r, 1.0), r, r);
+        return n * r;
     }
-    return (y - x) / (ly - lx);
 }
 
-__device__ __forceinline__ double inv_ln_mean_from_logs(double x, double y, double lx, double ly) {
-    const double num = x * (x - 2.0 * y) + y * y;
-    const double den = x * (x + 2.0 * y) + y * y;
+// ---------------------------------------------------------------------------
+// Logarithmic means (Trixi.jl ln_mean / inv_ln_mean) from precomputed
+// logarithms, one division each.
+//   f2 = ((y - x) / (x + y))^2;  branch f2 < 1e-4 tested as num < 1e-4 den.
+//   series:  ln_mean = (x + y) / P(f2),  P(f2) = 2 + 2f2/3 + 2f2^2/5 + 2f2^3/7,
+//            evaluated as (x + y) * Q(f2) with Q = 1/P expanded to the same
+//            order, Q = 1/2 - f2/6 - 2 f2^2/45 - 22 f2^3/945 (difference from
+//            1/P is O(f2^4) < 1e-16 relative);
+//            inv_ln_mean = P(f2) / (x + y).
+//   closed:  ln_mean = (y - x) / (log y - log x), inv_ln_mean its reciprocal;
+//            |log(y/x)| > 0.02 here, so log y - log x carries a relative
+//            error of a few 1e-14, far inside the FP64 parity bound.
+// ---------------------------------------------------------------------------
+template <typename R>
+__device__ __forceinline__ double ln_mean_logs(double x, double y, double lx, double ly) {
+    const double dxy = y - x, s = x + y;
+    const double num = dxy * dxy, den = s * s;
     if (num < 1.0e-4 * den) {
-        const double f2 = num / den;
-        return (2.0 + f2 * (2.0 / 3.0 + f2 * (2.0 / 5.0 + f2 * (2.0 / 7.0)))) / (x + y);
+        const double is = ldiv<R>(1.0, s);
+        const double z = dxy * is, f2 = z * z;
+        return s * (0.5 + f2 * (-1.0 / 6.0 + f2 * (-2.0 / 45.0 + f2 * (-22.0 / 945.0))));
     }
-    return (ly - lx) / (y - x);
+    return ldiv<R>(dxy, ly - lx);
+}
+
+template <typename R>
+__device__ __forceinline__ double inv_ln_mean_logs(double x, double y, double lx, double ly) {
+    const double dxy = y - x, s = x + y;
+    const double num = dxy * dxy, den = s * s;
+    if (num < 1.0e-4 * den) {
+        const double is = ldiv<R>(1.0, s);
+        const double z = dxy * is, f2 = z * z;
+        return (2.0 + f2 * (2.0 / 3.0 + f2 * (2.0 / 5.0 + f2 * (2.0 / 7.0)))) * is;
+    }
+    return ldiv<R>(ly - lx, dxy);
 }
 
 // ===========================================================================
@@ -92,27 +123,41 @@ template <int DIM>
 struct LinAdv {
     static constexpr int NV = 1;
     static constexpr bool NONCONS = false;
+    static constexpr int NRF = 1;  // node record: u
     template <typename R>
     struct Node {
-        R u[1];
+        R u;
+        double lrho, lp;  // unused
+        __device__ R cons(int) const { return u; }
     };
-    template <typename R, bool LOGS = true>
+    template <typename R, bool LOGS>
     __device__ static Node<R> node(const R* u, const PhysR<R>&) {
         Node<R> n;
-        n.u[0] = u[0];
+        n.u = u[0];
+        n.lrho = n.lp = 0.0;
+        return n;
+    }
+    template <typename R>
+    __device__ static void to_rec(const Node<R>& n, R* f) {
+        f[0] = n.u;
+    }
+    template <typename R>
+    __device__ static Node<R> from_rec(const R* f, double, double, const PhysR<R>&) {
+        Node<R> n;
+        n.u = f[0];
+        n.lrho = n.lp = 0.0;
         return n;
     }
     template <typename R>
     __device__ static void flux(const Node<R>& n, int o, const PhysR<R>& P, R* f) {
-        f[0] = P.a[o] * n.u[0];
+        f[0] = P.a[o] * n.u;
     }
     template <int KIND, typename R>
-    __device__ static void two_point(const Node<R>& l, const Node<R>& r, int o, const PhysR<R>& P,
-                                     R* f) {
-        R fl = P.a[o] * l.u[0], fr = P.a[o] * r.u[0];
+    __device__ static void two_point(const Node<R>& l, const Node<R>& r, int o, const PhysR<R>& P, R* f) {
+        const R fl = P.a[o] * l.u, fr = P.a[o] * r.u;
         if (KIND == FLUX_LLF || KIND == FLUX_LLF_NAIVE) {
-            R lam = dabs(P.a[o]);
-            f[0] = R(0.5) * (fl + fr) + (R(-0.5) * lam * (r.u[0] - l.u[0]));
+            const R lam = dabs(P.a[o]);
+            f[0] = R(0.5) * (fl + fr) + (R(-0.5) * lam * (r.u - l.u));
         } else {
             f[0] = R(0.5) * (fl + fr);
         }
@@ -130,64 +175,83 @@ template <int DIM>
 struct Euler {
     static constexpr int NV = DIM + 2;
     static constexpr bool NONCONS = false;
+    static constexpr int NRF = DIM + 3;  // node record: rho, v[DIM], p, c
     template <typename R>
     struct Node {
-        R u[NV];
-        R rho, v[DIM], p;
-        double lrho, lp;  // log(rho), log(p) for the entropy-conservative means
+        R rho, v[DIM], p, c;  // primitives and sound speed
+        double lrho, lp;      // log(rho), log(p) for the entropy-conservative means
+        R inv_gm1;
+        __device__ R cons(int q) const {
+            if (q == 0) return rho;
+            if (q <= DIM) return rho * v[q - 1];
+            R ke = R(0);
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) ke += rho * v[d] * v[d];
+            return p * inv_gm1 + R(0.5) * ke;
+        }
     };
 
-    template <typename R>
-    __device__ static void prim(const R* u, const PhysR<R>& P, R& rho, R* v, R& p) {
-        rho = u[0];
+    template <typename R, bool LOGS>
+    __device__ static Node<R> node(const R* u, const PhysR<R>& P) {
+        Node<R> n;
+        n.rho = u[0];
+        const R inv_rho = rdiv(R(1), n.rho);
         R ke = R(0);
 #pragma unroll
         for (int d = 0; d < DIM; ++d) {
-            v[d] = u[1 + d] / rho;
-            ke = (d == 0) ? u[1 + d] * v[d] : ke + u[1 + d] * v[d];
+            n.v[d] = u[1 + d] * inv_rho;
+            ke = (d == 0) ? u[1 + d] * n.v[d] : ke + u[1 + d] * n.v[d];
         }
-        p = P.gm1 * (u[DIM + 1] - R(0.5) * ke);
-    }
-
-    template <typename R, bool LOGS = true>
-    __device__ static Node<R> node(const R* u, const PhysR<R>& P) {
-        Node<R> n;
-#pragma unroll
-        for (int q = 0; q < NV; ++q) n.u[q] = u[q];
-        prim(u, P, n.rho, n.v, n.p);
+        n.p = P.gm1 * (u[DIM + 1] - R(0.5) * ke);
+        n.c = rsqrt_(P.gamma * n.p * inv_rho);
+        n.inv_gm1 = P.inv_gm1;
         if (LOGS) {
             n.lrho = log((double)n.rho);
             n.lp = log((double)n.p);
         } else {
-            n.lrho = 0.0;
-            n.lp = 0.0;
+            n.lrho = n.lp = 0.0;
         }
+        return n;
+    }
+    template <typename R>
+    __device__ static void to_rec(const Node<R>& n, R* f) {
+        f[0] = n.rho;
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) f[1 + d] = n.v[d];
+        f[DIM + 1] = n.p;
+        f[DIM + 2] = n.c;
+    }
+    template <typename R>
+    __device__ static Node<R> from_rec(const R* f, double lrho, double lp, const PhysR<R>& P) {
+        Node<R> n;
+        n.rho = f[0];
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) n.v[d] = f[1 + d];
+        n.p = f[DIM + 1];
+        n.c = f[DIM + 2];
+        n.lrho = lrho;
+        n.lp = lp;
+        n.inv_gm1 = P.inv_gm1;
         return n;
     }
 
     template <typename R>
     __device__ static void flux(const Node<R>& n, int o, const PhysR<R>& P, R* f) {
-        const R rho_vo = n.u[1 + o];
+        const R rho_vo = n.rho * n.v[o];
         f[0] = rho_vo;
 #pragma unroll
         for (int d = 0; d < DIM; ++d) f[1 + d] = (d == o) ? rho_vo * n.v[d] + n.p : rho_vo * n.v[d];
-        f[DIM + 1] = (n.u[DIM + 1] + n.p) * n.v[o];
-    }
-
-    template <typename R>
-    __device__ static R sound(const Node<R>& n, const PhysR<R>& P) {
-        return dsqrt(P.gamma * n.p / n.rho);
+        f[DIM + 1] = (n.cons(DIM + 1) + n.p) * n.v[o];
     }
 
     template <int KIND, typename R>
-    __device__ static void two_point(const Node<R>& l, const Node<R>& r, int o, const PhysR<R>& P,
-                                     R* f) {
+    __device__ static void two_point(const Node<R>& l, const Node<R>& r, int o, const PhysR<R>& P, R* f) {
         if (KIND == FLUX_RANOCHA) {
-            const double rho_mean_d = ln_mean_from_logs((double)l.rho, (double)r.rho, l.lrho, r.lrho);
+            const double rho_mean_d = ln_mean_logs<R>((double)l.rho, (double)r.rho, l.lrho, r.lrho);
             // inv_ln_mean(rho_ll p_rr, rho_rr p_ll) with logs log(rho)+log(p)
             const double x = (double)l.rho * (double)r.p, y = (double)r.rho * (double)l.p;
-            const double lx = l.lrho + r.lp, ly = r.lrho + l.lp;
-            const double inv_rho_p_mean_d = (double)l.p * (double)r.p * inv_ln_mean_from_logs(x, y, lx, ly);
+            const double inv_rho_p_mean_d =
+                (double)l.p * (double)r.p * inv_ln_mean_logs<R>(x, y, l.lrho + r.lp, r.lrho + l.lp);
             const R rho_mean = (R)rho_mean_d;
             const R inv_rho_p_mean = (R)inv_rho_p_mean_d;
             R vavg[DIM];
@@ -227,15 +291,14 @@ struct Euler {
             flux(l, o, P, fl);
             flux(r, o, P, fr);
             if (KIND == FLUX_LLF || KIND == FLUX_LLF_NAIVE) {
-                const R cl = sound(l, P), cr = sound(r, P);
                 R lam;
                 if (KIND == FLUX_LLF_NAIVE)
-                    lam = dmax(dabs(l.v[o]), dabs(r.v[o])) + dmax(cl, cr);
+                    lam = dmax(dabs(l.v[o]), dabs(r.v[o])) + dmax(l.c, r.c);
                 else
-                    lam = dmax(dabs(l.v[o]) + cl, dabs(r.v[o]) + cr);
+                    lam = dmax(dabs(l.v[o]) + l.c, dabs(r.v[o]) + r.c);
+                const R hl = R(-0.5) * lam;
 #pragma unroll
-                for (int q = 0; q < NV; ++q)
-                    f[q] = R(0.5) * (fl[q] + fr[q]) + (R(-0.5) * lam * (r.u[q] - l.u[q]));
+                for (int q = 0; q < NV; ++q) f[q] = R(0.5) * (fl[q] + fr[q]) + hl * (r.cons(q) - l.cons(q));
             } else {
 #pragma unroll
                 for (int q = 0; q < NV; ++q) f[q] = R(0.5) * (fl[q] + fr[q]);
@@ -245,9 +308,15 @@ struct Euler {
 
     template <typename R>
     __device__ static void max_abs_speeds(const R* u, const PhysR<R>& P, R* lam) {
-        R rho, v[DIM], p;
-        prim(u, P, rho, v, p);
-        const R c = dsqrt(P.gamma * p / rho);
+        const R rho = u[0];
+        R v[DIM], ke = R(0);
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) {
+            v[d] = u[1 + d] / rho;
+            ke = (d == 0) ? u[1 + d] * v[d] : ke + u[1 + d] * v[d];
+        }
+        const R p = P.gm1 * (u[DIM + 1] - R(0.5) * ke);
+        const R c = sqrt(P.gamma * p / rho);
 #pragma unroll
         for (int d = 0; d < DIM; ++d) lam[d] = dabs(v[d]) + c;
     }
@@ -269,49 +338,86 @@ struct Euler {
 struct GlmMhd3D {
     static constexpr int NV = 9;
     static constexpr bool NONCONS = true;
+    static constexpr int NRF = 9;  // node record: rho, v[3], p, B[3], psi
     template <typename R>
     struct Node {
-        R u[9];
-        R rho, v[3], p;
+        R rho, v[3], p, B[3], psi;
         double lrho, lp;
+        R inv_gm1;
+        __device__ R cons(int q) const {
+            if (q == 0) return rho;
+            if (q <= 3) return rho * v[q - 1];
+            if (q == 4)
+                return p * inv_gm1 + R(0.5) * (rho * v[0] * v[0] + rho * v[1] * v[1] + rho * v[2] * v[2]) +
+                       R(0.5) * (B[0] * B[0] + B[1] * B[1] + B[2] * B[2]) + R(0.5) * psi * psi;
+            if (q <= 7) return B[q - 5];
+            return psi;
+        }
     };
-    template <typename R, bool LOGS = true>
+    template <typename R, bool LOGS>
     __device__ static Node<R> node(const R* u, const PhysR<R>& P) {
         Node<R> n;
-#pragma unroll
-        for (int q = 0; q < 9; ++q) n.u[q] = u[q];
         n.rho = u[0];
-        n.v[0] = u[1] / n.rho;
-        n.v[1] = u[2] / n.rho;
-        n.v[2] = u[3] / n.rho;
+        const R inv_rho = rdiv(R(1), n.rho);
+        n.v[0] = u[1] * inv_rho;
+        n.v[1] = u[2] * inv_rho;
+        n.v[2] = u[3] * inv_rho;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) n.B[d] = u[5 + d];
+        n.psi = u[8];
         n.p = P.gm1 * (u[4] - R(0.5) * (u[1] * n.v[0] + u[2] * n.v[1] + u[3] * n.v[2]) -
                        R(0.5) * (u[5] * u[5] + u[6] * u[6] + u[7] * u[7]) - R(0.5) * u[8] * u[8]);
+        n.inv_gm1 = P.inv_gm1;
         if (LOGS) {
             n.lrho = log((double)n.rho);
             n.lp = log((double)n.p);
         } else {
-            n.lrho = 0.0;
-            n.lp = 0.0;
+            n.lrho = n.lp = 0.0;
         }
+        return n;
+    }
+    template <typename R>
+    __device__ static void to_rec(const Node<R>& n, R* f) {
+        f[0] = n.rho;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) f[1 + d] = n.v[d];
+        f[4] = n.p;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) f[5 + d] = n.B[d];
+        f[8] = n.psi;
+    }
+    template <typename R>
+    __device__ static Node<R> from_rec(const R* f, double lrho, double lp, const PhysR<R>& P) {
+        Node<R> n;
+        n.rho = f[0];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) n.v[d] = f[1 + d];
+        n.p = f[4];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) n.B[d] = f[5 + d];
+        n.psi = f[8];
+        n.lrho = lrho;
+        n.lp = lp;
+        n.inv_gm1 = P.inv_gm1;
         return n;
     }
 
     template <typename R>
     __device__ static void flux(const Node<R>& n, int o, const PhysR<R>& P, R* f) {
-        const R* u = n.u;
-        const R rho = u[0];
-        const R v[3] = {n.v[0], n.v[1], n.v[2]};
-        const R B[3] = {u[5], u[6], u[7]};
-        const R psi = u[8];
+        const R rho = n.rho;
+        const R* v = n.v;
+        const R* B = n.B;
+        const R psi = n.psi;
         const R kin_en = R(0.5) * rho * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
         const R mag_en = R(0.5) * (B[0] * B[0] + B[1] * B[1] + B[2] * B[2]);
-        const R p_over_gm1 = u[4] - kin_en - mag_en - R(0.5) * psi * psi;
-        const R p = P.gm1 * p_over_gm1;
+        const R p_over_gm1 = n.p * P.inv_gm1;
+        const R p = n.p;
         const R vB = v[0] * B[0] + v[1] * B[1] + v[2] * B[2];
-        f[0] = u[1 + o];
+        const R rho_vo = rho * v[o];
+        f[0] = rho_vo;
 #pragma unroll
         for (int d = 0; d < 3; ++d)
-            f[1 + d] = (d == o) ? u[1 + o] * v[d] + p + mag_en - B[o] * B[o] : u[1 + o] * v[d] - B[o] * B[d];
+            f[1 + d] = (d == o) ? rho_vo * v[d] + p + mag_en - B[o] * B[o] : rho_vo * v[d] - B[o] * B[d];
         f[4] = (kin_en + P.gamma * p_over_gm1 + R(2) * mag_en) * v[o] - B[o] * vB + P.c_h * psi * B[o];
 #pragma unroll
         for (int d = 0; d < 3; ++d) f[5 + d] = (d == o) ? P.c_h * psi : v[o] * B[d] - v[d] * B[o];
@@ -319,32 +425,25 @@ struct GlmMhd3D {
     }
 
     template <typename R>
-    __device__ static R fast_speed(const R* u, int o, const PhysR<R>& P) {
-        const R rho = u[0];
-        const R v1 = u[1] / rho, v2 = u[2] / rho, v3 = u[3] / rho;
-        const R B1 = u[5], B2 = u[6], B3 = u[7], psi = u[8];
-        const R kin_en = R(0.5) * rho * (v1 * v1 + v2 * v2 + v3 * v3);
-        const R mag_en = R(0.5) * (B1 * B1 + B2 * B2 + B3 * B3);
-        const R p = P.gm1 * (u[4] - kin_en - mag_en - R(0.5) * psi * psi);
-        const R a_square = P.gamma * p / rho;
-        const R sqrt_rho = dsqrt(rho);
-        const R b1 = B1 / sqrt_rho, b2 = B2 / sqrt_rho, b3 = B3 / sqrt_rho;
-        const R b_square = b1 * b1 + b2 * b2 + b3 * b3;
-        const R bo = (o == 0) ? b1 : ((o == 1) ? b2 : b3);
+    __device__ static R fast_speed(const Node<R>& n, int o, const PhysR<R>& P) {
+        const R inv_rho = rdiv(R(1), n.rho);
+        const R a_square = P.gamma * n.p * inv_rho;
+        const R b_square = (n.B[0] * n.B[0] + n.B[1] * n.B[1] + n.B[2] * n.B[2]) * inv_rho;
+        const R bo2 = n.B[o] * n.B[o] * inv_rho;
         const R s = a_square + b_square;
-        return dsqrt(R(0.5) * s + R(0.5) * dsqrt(s * s - R(4) * a_square * bo * bo));
+        return rsqrt_(R(0.5) * s + R(0.5) * rsqrt_(s * s - R(4) * a_square * bo2));
     }
 
     template <int KIND, typename R>
-    __device__ static void two_point(const Node<R>& l, const Node<R>& r, int o, const PhysR<R>& P,
-                                     R* f) {
+    __device__ static void two_point(const Node<R>& l, const Node<R>& r, int o, const PhysR<R>& P, R* f) {
         if (KIND == FLUX_HINDENLANG_GASSNER) {
-            const R Bl[3] = {l.u[5], l.u[6], l.u[7]}, Br[3] = {r.u[5], r.u[6], r.u[7]};
-            const R psil = l.u[8], psir = r.u[8];
-            const double rho_mean_d = ln_mean_from_logs((double)l.rho, (double)r.rho, l.lrho, r.lrho);
+            const R* Bl = l.B;
+            const R* Br = r.B;
+            const R psil = l.psi, psir = r.psi;
+            const double rho_mean_d = ln_mean_logs<R>((double)l.rho, (double)r.rho, l.lrho, r.lrho);
             const double x = (double)l.rho * (double)r.p, y = (double)r.rho * (double)l.p;
             const double inv_rho_p_mean_d =
-                (double)l.p * (double)r.p * inv_ln_mean_from_logs(x, y, l.lrho + r.lp, r.lrho + l.lp);
+                (double)l.p * (double)r.p * inv_ln_mean_logs<R>(x, y, l.lrho + r.lp, r.lrho + l.lp);
             const R rho_mean = (R)rho_mean_d, inv_rho_p_mean = (R)inv_rho_p_mean_d;
             R vavg[3];
 #pragma unroll
@@ -380,15 +479,15 @@ struct GlmMhd3D {
             flux(l, o, P, fl);
             flux(r, o, P, fr);
             if (KIND == FLUX_LLF || KIND == FLUX_LLF_NAIVE) {
-                const R cl = fast_speed(l.u, o, P), cr = fast_speed(r.u, o, P);
+                const R cl = fast_speed(l, o, P), cr = fast_speed(r, o, P);
                 R lam;
                 if (KIND == FLUX_LLF_NAIVE)
                     lam = dmax(dabs(l.v[o]), dabs(r.v[o])) + dmax(cl, cr);
                 else
                     lam = dmax(dabs(l.v[o]) + cl, dabs(r.v[o]) + cr);
+                const R hl = R(-0.5) * lam;
 #pragma unroll
-                for (int q = 0; q < 9; ++q)
-                    f[q] = R(0.5) * (fl[q] + fr[q]) + (R(-0.5) * lam * (r.u[q] - l.u[q]));
+                for (int q = 0; q < 9; ++q) f[q] = R(0.5) * (fl[q] + fr[q]) + hl * (r.cons(q) - l.cons(q));
             } else {
 #pragma unroll
                 for (int q = 0; q < 9; ++q) f[q] = R(0.5) * (fl[q] + fr[q]);
@@ -400,15 +499,13 @@ struct GlmMhd3D {
     // Galilean GLM terms of the left state times the right state's B_o / psi.
     template <typename R>
     __device__ static void noncons(const Node<R>& l, const Node<R>& r, int o, R* f) {
-        const R B_ll[3] = {l.u[5], l.u[6], l.u[7]};
-        const R psi_ll = l.u[8];
-        const R v_dot_B_ll = l.v[0] * B_ll[0] + l.v[1] * B_ll[1] + l.v[2] * B_ll[2];
-        const R Bo_rr = r.u[5 + o];
-        const R psi_rr = r.u[8];
-        const R psi_vo_ll = psi_ll * l.v[o];
+        const R v_dot_B_ll = l.v[0] * l.B[0] + l.v[1] * l.B[1] + l.v[2] * l.B[2];
+        const R Bo_rr = r.B[o];
+        const R psi_rr = r.psi;
+        const R psi_vo_ll = l.psi * l.v[o];
         f[0] = R(0);
 #pragma unroll
-        for (int d = 0; d < 3; ++d) f[1 + d] = B_ll[d] * Bo_rr;
+        for (int d = 0; d < 3; ++d) f[1 + d] = l.B[d] * Bo_rr;
         f[4] = v_dot_B_ll * Bo_rr + psi_vo_ll * psi_rr;
 #pragma unroll
         for (int d = 0; d < 3; ++d) f[5 + d] = l.v[d] * Bo_rr;
@@ -417,8 +514,23 @@ struct GlmMhd3D {
 
     template <typename R>
     __device__ static void max_abs_speeds(const R* u, const PhysR<R>& P, R* lam) {
+        const R rho = u[0];
+        const R v1 = u[1] / rho, v2 = u[2] / rho, v3 = u[3] / rho;
+        const R B1 = u[5], B2 = u[6], B3 = u[7], psi = u[8];
+        const R kin_en = R(0.5) * rho * (v1 * v1 + v2 * v2 + v3 * v3);
+        const R mag_en = R(0.5) * (B1 * B1 + B2 * B2 + B3 * B3);
+        const R p = P.gm1 * (u[4] - kin_en - mag_en - R(0.5) * psi * psi);
+        const R a_square = P.gamma * p / rho;
+        const R sqrt_rho = sqrt(rho);
+        const R b1 = B1 / sqrt_rho, b2 = B2 / sqrt_rho, b3 = B3 / sqrt_rho;
+        const R b_square = b1 * b1 + b2 * b2 + b3 * b3;
+        const R vv[3] = {v1, v2, v3}, bb[3] = {b1, b2, b3};
 #pragma unroll
-        for (int d = 0; d < 3; ++d) lam[d] = dabs(u[1 + d] / u[0]) + fast_speed(u, d, P);
+        for (int d = 0; d < 3; ++d) {
+            const R s = a_square + b_square;
+            const R cf = sqrt(R(0.5) * s + R(0.5) * sqrt(s * s - R(4) * a_square * bb[d] * bb[d]));
+            lam[d] = dabs(vv[d]) + cf;
+        }
     }
 };
 

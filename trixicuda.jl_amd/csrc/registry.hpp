@@ -21,22 +21,24 @@ using LaunchFn = hipError_t (*)(const RhsArgs&, hipStream_t);
 void register_kernel(const KernelKey& key, LaunchFn fn);
 LaunchFn find_kernel(const KernelKey& key);
 
-// Workgroup width in elements: fill about 256 lanes, keep both LDS tiles
-// (state and accumulator) within 48 KiB so several workgroups share a CU.
-template <int DIM, int N, int NV, int RS>
+// Workgroup width in elements: fill about 256 lanes and keep the LDS tiles
+// (node records, FP64 logarithms, direction accumulator) within 40 KiB so
+// that several workgroups share a CU.
+template <int DIM, int N, int NV, int NRF, int RS>
 constexpr int pick_m() {
     constexpr int NL = cpow(N, DIM - 1), NN = cpow(N, DIM);
+    constexpr int per_elem = NN * (NRF * RS + 2 * 8 + (DIM > 1 ? NV * RS : 0));
     int m = 256 / NL;
     if (m < 1) m = 1;
-    while (DIM > 1 && m > 1 && 2 * NV * NN * (m + 1) * RS > 48 * 1024) m /= 2;
+    while (m > 1 && m * per_elem > 40 * 1024) m /= 2;
     return m;
 }
 
 template <int DIM, int N, class EQ, int VOL, int VF, int SF, int FVF>
 struct Registrar {
     static void add(int eq) {
-        constexpr int MD = pick_m<DIM, N, EQ::NV, 8>();
-        constexpr int MF = pick_m<DIM, N, EQ::NV, 4>();
+        constexpr int MD = pick_m<DIM, N, EQ::NV, EQ::NRF, 8>();
+        constexpr int MF = pick_m<DIM, N, EQ::NV, EQ::NRF, 4>();
         register_kernel({DIM, N, eq, VOL, VF, SF, FVF, 0, 0},
                         &launch_rhs<DIM, N, EQ, VOL, VF, SF, FVF, double, MD, false>);
         register_kernel({DIM, N, eq, VOL, VF, SF, FVF, 0, 1},

@@ -49,7 +49,21 @@ struct RhsArgs {
     int smooth;               // HG smoothing
     double rk_a, rk_b;        // RK stage coefficients (rk_a unused when rk_first)
     int rk_first;
+    int nxcd;                 // XCDs the dispatcher deals workgroups to round robin
 };
+
+// Workgroup -> element tile.  The dispatcher deals consecutive workgroups
+// to the XCDs round robin, and each XCD has its own L2; remapping so that
+// XCD x processes the contiguous tiles [x*per, (x+1)*per) keeps every cache
+// line of the element-innermost state, and the face neighbours of a tile
+// (nearby in Morton order), inside one L2.  The grid is launched with
+// nxcd*per workgroups; tiles past the end exit before any barrier.
+__device__ __forceinline__ int64_t xcd_tile(int nxcd) {
+    const int64_t bid = blockIdx.x, nb = gridDim.x;
+    if (nxcd <= 1) return bid;
+    const int64_t per = nb / nxcd;
+    return (bid % nxcd) * per + bid / nxcd;
+}
 
 // Line geometry: base node of line `line` in direction d, the other
 // directions enumerated in increasing order with the first one fastest
@@ -72,37 +86,51 @@ rhs_kernel(const RhsArgs a) {
     constexpr int NL = cpow(N, DIM - 1);
     constexpr int NN = cpow(N, DIM);
     constexpr int NV = EQ::NV;
-    constexpr bool STAGE = DIM > 1;
-    constexpr int LDM = M + 1;  // padded LDS row
+    constexpr int NRF = EQ::NRF;
     constexpr bool SF_LOGS = (SF == FLUX_RANOCHA || SF == FLUX_HINDENLANG_GASSNER);
     constexpr bool VF_LOGS = (VOL != VI_WEAK) && (VF == FLUX_RANOCHA || VF == FLUX_HINDENLANG_GASSNER);
     constexpr bool FV_LOGS = (VOL == VI_HG) && (FVF == FLUX_RANOCHA || FVF == FLUX_HINDENLANG_GASSNER);
     constexpr bool NODE_LOGS = SF_LOGS || VF_LOGS || FV_LOGS;
+    constexpr int NDF = NODE_LOGS ? 2 : 0;
     using Node = typename EQ::template Node<R>;
 
-    __shared__ R us[STAGE ? NV * NN * LDM : 1];
-    __shared__ R acc[STAGE ? NV * NN * LDM : 1];
+    // LDS: node records (R fields and FP64 logarithms) of the M elements,
+    // field-major with the element innermost, and the direction accumulator.
+    __shared__ R rec[NRF * NN * M];
+    __shared__ double recd[(NDF > 0 ? NDF : 1) * NN * M];
+    __shared__ R acc[DIM > 1 ? NV * NN * M : 1];
 
     const int tid = threadIdx.x;
     const int el = tid % M;
     const int line = tid / M;
     const int64_t E = a.E;
-    const int64_t e0 = (int64_t)blockIdx.x * M;
+    const int64_t e0 = xcd_tile(a.nxcd) * M;
+    if (e0 >= E) return;  // whole workgroup: padding tile of the XCD mapping
     const int64_t e = e0 + el;
     const bool valid = e < E;
     const int64_t ec = valid ? e : E - 1;
     const PhysR<R> P = PhysR<R>::from(a.phys);
 
-    if (STAGE) {
-        for (int idx = tid; idx < NV * NN * M; idx += M * NL) {
-            const int c = idx % M;
-            const int r = idx / M;
-            int64_t ee = e0 + c;
-            if (ee >= E) ee = E - 1;
-            us[r * LDM + c] = (R)a.u[(int64_t)r * E + ee];
+    // ---- node data, once per node: coalesced loads (element innermost) -----
+    for (int q = tid; q < NN * M; q += M * NL) {
+        const int c = q % M;
+        const int n = q / M;
+        int64_t ee = e0 + c;
+        if (ee >= E) ee = E - 1;
+        R uc[NV];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) uc[v] = (R)a.u[(int64_t)(v * NN + n) * E + ee];
+        const Node nd = EQ::template node<R, NODE_LOGS>(uc, P);
+        R f[NRF];
+        EQ::to_rec(nd, f);
+#pragma unroll
+        for (int k = 0; k < NRF; ++k) rec[(k * NN + n) * M + c] = f[k];
+        if (NODE_LOGS) {
+            recd[n * M + c] = nd.lrho;
+            recd[(NN + n) * M + c] = nd.lp;
         }
-        __syncthreads();
     }
+    __syncthreads();
 
     R alpha = R(0);
     bool dg_only = true;
@@ -127,13 +155,13 @@ rhs_kernel(const RhsArgs a) {
         Node nd[N];
 #pragma unroll
         for (int p = 0; p < N; ++p) {
-            R uc[NV];
+            const int node = base + p * s;
+            R f[NRF];
 #pragma unroll
-            for (int v = 0; v < NV; ++v) {
-                const int row = v * NN + base + p * s;
-                uc[v] = STAGE ? us[row * LDM + el] : (R)a.u[(int64_t)row * E + ec];
-            }
-            nd[p] = EQ::template node<R, NODE_LOGS>(uc, P);
+            for (int k = 0; k < NRF; ++k) f[k] = rec[(k * NN + node) * M + el];
+            const double lr = NODE_LOGS ? recd[node * M + el] : 0.0;
+            const double lp = NODE_LOGS ? recd[(NN + node) * M + el] : 0.0;
+            nd[p] = EQ::from_rec(f, lr, lp, P);
         }
 
         R acl[N][NV];
@@ -245,7 +273,7 @@ rhs_kernel(const RhsArgs a) {
             for (int p = 0; p < N; ++p)
 #pragma unroll
                 for (int v = 0; v < NV; ++v) {
-                    const int idx = (v * NN + base + p * s) * LDM + el;
+                    const int idx = (v * NN + base + p * s) * M + el;
                     if (d == 0)
                         acc[idx] = acl[p][v];
                     else
@@ -262,7 +290,7 @@ rhs_kernel(const RhsArgs a) {
                 for (int v = 0; v < NV; ++v) {
                     const int node = base + p * s;
                     R total = acl[p][v];
-                    if (STAGE) total = acc[(v * NN + node) * LDM + el] + total;
+                    if (DIM > 1) total = acc[(v * NN + node) * M + el] + total;
                     const int64_t g = (int64_t)(v * NN + node) * E + e;
                     const double val = (double)total * fac;
                     if (!RK) {
@@ -281,7 +309,8 @@ rhs_kernel(const RhsArgs a) {
 template <int DIM, int N, class EQ, int VOL, int VF, int SF, int FVF, typename R, int M, bool RK>
 hipError_t launch_rhs(const RhsArgs& a, hipStream_t stream) {
     constexpr int NL = cpow(N, DIM - 1);
-    const int64_t blocks = (a.E + M - 1) / M;
+    int64_t blocks = (a.E + M - 1) / M;
+    if (a.nxcd > 1) blocks = (blocks + a.nxcd - 1) / a.nxcd * a.nxcd;
     hipLaunchKernelGGL((rhs_kernel<DIM, N, EQ, VOL, VF, SF, FVF, R, M, RK>), dim3((unsigned)blocks),
                        dim3(M * NL), 0, stream, a);
     return hipGetLastError();

@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <cstdlib>
 #include <cstdio>
 #include <cstring>
 #include <map>
@@ -949,6 +950,17 @@ int trixi_hip_create(const trixi_hip_config* config, const trixi_hip_mesh* mesh,
         a.atol = std::max(100 * eps, pow(eps, 0.75));
     }
     a.smooth = cfg.alpha_smooth;
+    a.nxcd = 1;
+    {
+        int dev = 0;
+        hipDeviceProp_t prop;
+        if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess) {
+            // CDNA3/CDNA4 (gfx942, gfx950) parts are built from 8 XCDs.
+            if (strstr(prop.gcnArchName, "gfx950") || strstr(prop.gcnArchName, "gfx942")) a.nxcd = 8;
+        }
+        const char* env = getenv("TRIXI_HIP_NXCD");
+        if (env) a.nxcd = atoi(env) > 0 ? atoi(env) : 1;
+    }
     *out = c;
     return TRIXI_HIP_OK;
 }
