@@ -28,9 +28,9 @@ CFLAGS = [
 ]
 
 
-def _compile(src):
-    obj = os.path.join(BUILD, os.path.basename(src).replace(".hip", ".o"))
-    cmd = [HIPCC] + CFLAGS + ["-c", src, "-o", obj]
+def _compile(src, build_dir=BUILD, extra=()):
+    obj = os.path.join(build_dir, os.path.basename(src).replace(".hip", ".o"))
+    cmd = [HIPCC] + CFLAGS + list(extra) + ["-c", src, "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     msgs = "\n".join(l for l in (r.stdout + r.stderr).splitlines() if "not a recognized feature" not in l
                      and "packed-fp32-ops" not in l)
@@ -39,23 +39,25 @@ def _compile(src):
     return obj, msgs
 
 
-def build(jobs=None, verbose=False):
-    os.makedirs(BUILD, exist_ok=True)
+def build(jobs=None, verbose=False, out=LIB, build_dir=BUILD, defines=()):
+    """Compile and link; `defines` are extra -D options (tuning variants)."""
+    os.makedirs(build_dir, exist_ok=True)
     srcs = sorted(glob.glob(os.path.join(CSRC, "*.hip")))
     jobs = jobs or min(16, os.cpu_count() or 4)
+    extra = [f"-D{d}" for d in defines]
     objs = []
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
-        for obj, msgs in ex.map(_compile, srcs):
+        for obj, msgs in ex.map(lambda s: _compile(s, build_dir, extra), srcs):
             objs.append(obj)
             if verbose and msgs.strip():
                 print(msgs, file=sys.stderr)
-    tmp = LIB + ".tmp"
+    tmp = out + ".tmp"
     cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + objs
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("link failed:\n" + r.stdout + r.stderr)
-    os.replace(tmp, LIB)
-    return LIB
+    os.replace(tmp, out)
+    return out
 
 
 if __name__ == "__main__":

@@ -1,7 +1,7 @@
 // physics.hpp -- device-side equations and numerical fluxes for the gfx950
 // DGSEM kernels.  Everything is templated on the arithmetic type R (double in
-// the FP64 mode, float in the mixed mode); the logarithmic means are always
-// evaluated in double.
+// the FP64 mode, float in the mixed mode); the logarithms inside the
+// entropy-conservative means, and their differences, are always FP64.
 //
 // The flux algebra follows the Trixi.jl definitions that TrixiCUDA.jl's
 // kernels inline (reference: fluxes passed in by the tests, e.g.
@@ -53,11 +53,25 @@ struct PhysR {
 };
 
 // ---------------------------------------------------------------------------
-// arithmetic helpers: IEEE division / square root in FP64 mode; in the mixed
-// mode the FP32 quotient is n * rcp(d) and the square root the hardware
-// v_sqrt_f32 (both within 2 ulp of FP32, far below the mixed-mode error).
+// arithmetic helpers.
+//   FP64: reciprocal = v_rcp_f64 refined by two Newton steps, quotient
+//         n * r corrected by one residual step (within about 1 ulp, far
+//         inside the FP64 parity bound, 6 instructions instead of the
+//         11-instruction IEEE sequence); IEEE square root.
+//   FP32 (mixed mode): v_rcp_f32 and v_sqrt_f32 (within 2 ulp of FP32).
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ double rdiv(double n, double d) { return n / d; }
+__device__ __forceinline__ double rrcp(double d) {
+    double r = __builtin_amdgcn_rcp(d);
+    r = fma(fma(-d, r, 1.0), r, r);
+    r = fma(fma(-d, r, 1.0), r, r);
+    return r;
+}
+__device__ __forceinline__ float rrcp(float d) { return __builtin_amdgcn_rcpf(d); }
+__device__ __forceinline__ double rdiv(double n, double d) {
+    const double r = rrcp(d);
+    const double q = n * r;
+    return fma(fma(-d, q, n), r, q);
+}
 __device__ __forceinline__ float rdiv(float n, float d) { return n * __builtin_amdgcn_rcpf(d); }
 __device__ __forceinline__ double rsqrt_(double x) { return sqrt(x); }
 __device__ __forceinline__ float rsqrt_(float x) { return __builtin_amdgcn_sqrtf(x); }
@@ -65,55 +79,43 @@ __device__ __forceinline__ double dabs(double x) { return fabs(x); }
 __device__ __forceinline__ float dabs(float x) { return fabsf(x); }
 template <typename R> __device__ __forceinline__ R dmax(R a, R b) { return b > a ? b : a; }
 
-// FP64 quotient used inside the logarithmic means: IEEE in FP64 mode; in the
-// mixed mode (whose result is rounded to FP32) v_rcp_f64 refined by one
-// Newton step, accurate to far better than FP32.
-template <typename R>
-__device__ __forceinline__ double ldiv(double n, double d) {
-    if constexpr (std::is_same<R, double>::value) {
-        return n / d;
-    } else {
-        double r = __builtin_amdgcn_rcp(d);
-        r = fma(fma(-d, r, 1.0), r, r);
-        return n * r;
-    }
-}
-
 // ---------------------------------------------------------------------------
-// Logarithmic means (Trixi.jl ln_mean / inv_ln_mean) from precomputed
-// logarithms, one division each.
-//   f2 = ((y - x) / (x + y))^2;  branch f2 < 1e-4 tested as num < 1e-4 den.
-//   series:  ln_mean = (x + y) / P(f2),  P(f2) = 2 + 2f2/3 + 2f2^2/5 + 2f2^3/7,
-//            evaluated as (x + y) * Q(f2) with Q = 1/P expanded to the same
-//            order, Q = 1/2 - f2/6 - 2 f2^2/45 - 22 f2^3/945 (difference from
-//            1/P is O(f2^4) < 1e-16 relative);
-//            inv_ln_mean = P(f2) / (x + y).
-//   closed:  ln_mean = (y - x) / (log y - log x), inv_ln_mean its reciprocal;
-//            |log(y/x)| > 0.02 here, so log y - log x carries a relative
-//            error of a few 1e-14, far inside the FP64 parity bound.
+// Logarithmic means (Trixi.jl ln_mean / inv_ln_mean) from per-node
+// logarithms, branch free with a single reciprocal each.
+//   f2 = ((y - x) / (x + y))^2; the series branch is taken for f2 < 1e-4,
+//   tested as (y-x)^2 < 1e-4 (x+y)^2 (x, y > 0).
+//   series:  ln_mean     = (x + y) Q(f2),  Q = 1/P expanded to the order of
+//                          P(f2) = 2 + 2f2/3 + 2f2^2/5 + 2f2^3/7:
+//                          Q = 1/2 - f2/6 - 2 f2^2/45 - 22 f2^3/945
+//                          (differs from 1/P by O(f2^4) < 1e-16 relative);
+//            inv_ln_mean = P(f2) / (x + y);
+//   closed:  ln_mean     = (y - x) / (log y - log x), inv_ln_mean = 1 / that.
+//   The one reciprocal is of (x + y) in the series branch and of the
+//   closed-branch denominator otherwise.  The logarithms and their
+//   difference are always FP64 (|log(y/x)| > 0.02 in the closed branch, so
+//   the difference of FP64 logarithms is accurate to a few 1e-14); the
+//   remaining arithmetic runs in R (FP32 in the mixed mode, where the
+//   result is consumed in FP32 anyway).
 // ---------------------------------------------------------------------------
 template <typename R>
-__device__ __forceinline__ double ln_mean_logs(double x, double y, double lx, double ly) {
-    const double dxy = y - x, s = x + y;
-    const double num = dxy * dxy, den = s * s;
-    if (num < 1.0e-4 * den) {
-        const double is = ldiv<R>(1.0, s);
-        const double z = dxy * is, f2 = z * z;
-        return s * (0.5 + f2 * (-1.0 / 6.0 + f2 * (-2.0 / 45.0 + f2 * (-22.0 / 945.0))));
-    }
-    return ldiv<R>(dxy, ly - lx);
+__device__ __forceinline__ R ln_mean_logs(R x, R y, double lx, double ly) {
+    const R dxy = y - x, s = x + y;
+    const bool series = dxy * dxy < R(1.0e-4) * (s * s);
+    const R r = rrcp(series ? s : (R)(ly - lx));
+    const R z = dxy * r, f2 = z * z;
+    const R q = R(0.5) + f2 * (R(-1.0 / 6.0) + f2 * (R(-2.0 / 45.0) + f2 * R(-22.0 / 945.0)));
+    return series ? s * q : z;
 }
 
 template <typename R>
-__device__ __forceinline__ double inv_ln_mean_logs(double x, double y, double lx, double ly) {
-    const double dxy = y - x, s = x + y;
-    const double num = dxy * dxy, den = s * s;
-    if (num < 1.0e-4 * den) {
-        const double is = ldiv<R>(1.0, s);
-        const double z = dxy * is, f2 = z * z;
-        return (2.0 + f2 * (2.0 / 3.0 + f2 * (2.0 / 5.0 + f2 * (2.0 / 7.0)))) * is;
-    }
-    return ldiv<R>(ly - lx, dxy);
+__device__ __forceinline__ R inv_ln_mean_logs(R x, R y, double lx, double ly) {
+    const R dxy = y - x, s = x + y;
+    const bool series = dxy * dxy < R(1.0e-4) * (s * s);
+    const R dl = (R)(ly - lx);
+    const R r = rrcp(series ? s : dxy);
+    const R z = dxy * r, f2 = z * z;
+    const R pp = R(2) + f2 * (R(2.0 / 3.0) + f2 * (R(2.0 / 5.0) + f2 * R(2.0 / 7.0)));
+    return series ? pp * r : dl * r;
 }
 
 // ===========================================================================
@@ -247,13 +249,10 @@ struct Euler {
     template <int KIND, typename R>
     __device__ static void two_point(const Node<R>& l, const Node<R>& r, int o, const PhysR<R>& P, R* f) {
         if (KIND == FLUX_RANOCHA) {
-            const double rho_mean_d = ln_mean_logs<R>((double)l.rho, (double)r.rho, l.lrho, r.lrho);
-            // inv_ln_mean(rho_ll p_rr, rho_rr p_ll) with logs log(rho)+log(p)
-            const double x = (double)l.rho * (double)r.p, y = (double)r.rho * (double)l.p;
-            const double inv_rho_p_mean_d =
-                (double)l.p * (double)r.p * inv_ln_mean_logs<R>(x, y, l.lrho + r.lp, r.lrho + l.lp);
-            const R rho_mean = (R)rho_mean_d;
-            const R inv_rho_p_mean = (R)inv_rho_p_mean_d;
+            const R rho_mean = ln_mean_logs<R>(l.rho, r.rho, l.lrho, r.lrho);
+            // p_ll p_rr inv_ln_mean(rho_ll p_rr, rho_rr p_ll), logs log(rho)+log(p)
+            const R inv_rho_p_mean =
+                l.p * r.p * inv_ln_mean_logs<R>(l.rho * r.p, r.rho * l.p, l.lrho + r.lp, r.lrho + l.lp);
             R vavg[DIM];
             R vsq = R(0);
 #pragma unroll
@@ -440,11 +439,9 @@ struct GlmMhd3D {
             const R* Bl = l.B;
             const R* Br = r.B;
             const R psil = l.psi, psir = r.psi;
-            const double rho_mean_d = ln_mean_logs<R>((double)l.rho, (double)r.rho, l.lrho, r.lrho);
-            const double x = (double)l.rho * (double)r.p, y = (double)r.rho * (double)l.p;
-            const double inv_rho_p_mean_d =
-                (double)l.p * (double)r.p * inv_ln_mean_logs<R>(x, y, l.lrho + r.lp, r.lrho + l.lp);
-            const R rho_mean = (R)rho_mean_d, inv_rho_p_mean = (R)inv_rho_p_mean_d;
+            const R rho_mean = ln_mean_logs<R>(l.rho, r.rho, l.lrho, r.lrho);
+            const R inv_rho_p_mean =
+                l.p * r.p * inv_ln_mean_logs<R>(l.rho * r.p, r.rho * l.p, l.lrho + r.lp, r.lrho + l.lp);
             R vavg[3];
 #pragma unroll
             for (int d = 0; d < 3; ++d) vavg[d] = R(0.5) * (l.v[d] + r.v[d]);

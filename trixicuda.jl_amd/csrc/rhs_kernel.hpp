@@ -28,6 +28,20 @@ namespace trixi_hip {
 
 constexpr int MAXN = 8;
 
+// Tuning knobs (compile-time): LDS row padding in elements and a minimum
+// number of waves per SIMD requested from the register allocator.
+#ifndef TRIXI_HIP_LDS_PAD
+#define TRIXI_HIP_LDS_PAD 0
+#endif
+#ifndef TRIXI_HIP_MIN_WAVES
+#define TRIXI_HIP_MIN_WAVES 1
+#endif
+// 1: node records are computed once per node into LDS; 0: every lane
+// computes the node data of its line in each direction phase (no record LDS).
+#ifndef TRIXI_HIP_STAGE_NODES
+#define TRIXI_HIP_STAGE_NODES 1
+#endif
+
 __host__ __device__ constexpr int cpow(int b, int e) { return e == 0 ? 1 : b * cpow(b, e - 1); }
 
 struct RhsArgs {
@@ -81,7 +95,7 @@ __device__ __forceinline__ int line_base(int line, int d) {
 }
 
 template <int DIM, int N, class EQ, int VOL, int VF, int SF, int FVF, typename R, int M, bool RK>
-__global__ void __launch_bounds__(M * cpow(N, DIM - 1))
+__global__ void __launch_bounds__(M * cpow(N, DIM - 1), TRIXI_HIP_MIN_WAVES)
 rhs_kernel(const RhsArgs a) {
     constexpr int NL = cpow(N, DIM - 1);
     constexpr int NN = cpow(N, DIM);
@@ -96,9 +110,11 @@ rhs_kernel(const RhsArgs a) {
 
     // LDS: node records (R fields and FP64 logarithms) of the M elements,
     // field-major with the element innermost, and the direction accumulator.
-    __shared__ R rec[NRF * NN * M];
-    __shared__ double recd[(NDF > 0 ? NDF : 1) * NN * M];
-    __shared__ R acc[DIM > 1 ? NV * NN * M : 1];
+    constexpr int LDM = M + TRIXI_HIP_LDS_PAD;  // LDS row length (elements)
+    constexpr bool STAGE = TRIXI_HIP_STAGE_NODES != 0;
+    __shared__ R rec[STAGE ? NRF * NN * LDM : 1];
+    __shared__ double recd[(STAGE && NDF > 0) ? NDF * NN * LDM : 1];
+    __shared__ R acc[DIM > 1 ? NV * NN * LDM : 1];
 
     const int tid = threadIdx.x;
     const int el = tid % M;
@@ -111,8 +127,24 @@ rhs_kernel(const RhsArgs a) {
     const int64_t ec = valid ? e : E - 1;
     const PhysR<R> P = PhysR<R>::from(a.phys);
 
+    // Face neighbours of all 2*DIM faces, and the neighbour face data of the
+    // first direction, are requested before the node pass so that their
+    // latency overlaps it; later directions are prefetched one phase ahead.
+    int32_t nb[2 * DIM];
+#pragma unroll
+    for (int f = 0; f < 2 * DIM; ++f) nb[f] = a.nbr[(int64_t)f * E + ec];
+    double um_next[NV], up_next[NV];
+    {
+        const int base0 = line_base<DIM, N>(line, 0);
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            um_next[v] = a.u[(int64_t)(v * NN + base0 + (N - 1)) * E + nb[0]];
+            up_next[v] = a.u[(int64_t)(v * NN + base0) * E + nb[1]];
+        }
+    }
+
     // ---- node data, once per node: coalesced loads (element innermost) -----
-    for (int q = tid; q < NN * M; q += M * NL) {
+    for (int q = tid; STAGE && q < NN * M; q += M * NL) {
         const int c = q % M;
         const int n = q / M;
         int64_t ee = e0 + c;
@@ -124,13 +156,13 @@ rhs_kernel(const RhsArgs a) {
         R f[NRF];
         EQ::to_rec(nd, f);
 #pragma unroll
-        for (int k = 0; k < NRF; ++k) rec[(k * NN + n) * M + c] = f[k];
+        for (int k = 0; k < NRF; ++k) rec[(k * NN + n) * LDM + c] = f[k];
         if (NODE_LOGS) {
-            recd[n * M + c] = nd.lrho;
-            recd[(NN + n) * M + c] = nd.lp;
+            recd[n * LDM + c] = nd.lrho;
+            recd[(NN + n) * LDM + c] = nd.lp;
         }
     }
-    __syncthreads();
+    if (STAGE) __syncthreads();
 
     R alpha = R(0);
     bool dg_only = true;
@@ -139,7 +171,7 @@ rhs_kernel(const RhsArgs a) {
         if (a.smooth) {
 #pragma unroll
             for (int s = 0; s < 2 * DIM; ++s) {
-                const double an = 0.5 * a.alpha_raw[a.nbr[(int64_t)s * E + ec]];
+                const double an = 0.5 * a.alpha_raw[nb[s]];
                 al = an > al ? an : al;
             }
         }
@@ -151,17 +183,39 @@ rhs_kernel(const RhsArgs a) {
     for (int d = 0; d < DIM; ++d) {
         const int s = cpow(N, d);
         const int base = line_base<DIM, N>(line, d);
+        R um[NV], up[NV];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            um[v] = (R)um_next[v];
+            up[v] = (R)up_next[v];
+        }
+        if (d + 1 < DIM) {
+            const int sn = cpow(N, d + 1);
+            const int bn = line_base<DIM, N>(line, d + 1);
+#pragma unroll
+            for (int v = 0; v < NV; ++v) {
+                um_next[v] = a.u[(int64_t)(v * NN + bn + (N - 1) * sn) * E + nb[2 * (d + 1)]];
+                up_next[v] = a.u[(int64_t)(v * NN + bn) * E + nb[2 * (d + 1) + 1]];
+            }
+        }
 
         Node nd[N];
 #pragma unroll
         for (int p = 0; p < N; ++p) {
             const int node = base + p * s;
-            R f[NRF];
+            if (STAGE) {
+                R f[NRF];
 #pragma unroll
-            for (int k = 0; k < NRF; ++k) f[k] = rec[(k * NN + node) * M + el];
-            const double lr = NODE_LOGS ? recd[node * M + el] : 0.0;
-            const double lp = NODE_LOGS ? recd[(NN + node) * M + el] : 0.0;
-            nd[p] = EQ::from_rec(f, lr, lp, P);
+                for (int k = 0; k < NRF; ++k) f[k] = rec[(k * NN + node) * LDM + el];
+                const double lr = NODE_LOGS ? recd[node * LDM + el] : 0.0;
+                const double lp = NODE_LOGS ? recd[(NN + node) * LDM + el] : 0.0;
+                nd[p] = EQ::from_rec(f, lr, lp, P);
+            } else {
+                R uc[NV];
+#pragma unroll
+                for (int v = 0; v < NV; ++v) uc[v] = (R)a.u[(int64_t)(v * NN + node) * E + ec];
+                nd[p] = EQ::template node<R, NODE_LOGS>(uc, P);
+            }
         }
 
         R acl[N][NV];
@@ -236,14 +290,6 @@ rhs_kernel(const RhsArgs a) {
 
         // ---- interface fluxes at both ends of the line, surface integral ----
         {
-            const int32_t nbm = a.nbr[(int64_t)(2 * d) * E + ec];
-            const int32_t nbp = a.nbr[(int64_t)(2 * d + 1) * E + ec];
-            R um[NV], up[NV];
-#pragma unroll
-            for (int v = 0; v < NV; ++v) {
-                um[v] = (R)a.u[(int64_t)(v * NN + base + (N - 1) * s) * E + nbm];
-                up[v] = (R)a.u[(int64_t)(v * NN + base) * E + nbp];
-            }
             const Node ndm = EQ::template node<R, SF_LOGS>(um, P);
             const Node ndp = EQ::template node<R, SF_LOGS>(up, P);
             R fm[NV], fp[NV];
@@ -273,7 +319,7 @@ rhs_kernel(const RhsArgs a) {
             for (int p = 0; p < N; ++p)
 #pragma unroll
                 for (int v = 0; v < NV; ++v) {
-                    const int idx = (v * NN + base + p * s) * M + el;
+                    const int idx = (v * NN + base + p * s) * LDM + el;
                     if (d == 0)
                         acc[idx] = acl[p][v];
                     else
@@ -290,7 +336,7 @@ rhs_kernel(const RhsArgs a) {
                 for (int v = 0; v < NV; ++v) {
                     const int node = base + p * s;
                     R total = acl[p][v];
-                    if (DIM > 1) total = acc[(v * NN + node) * M + el] + total;
+                    if (DIM > 1) total = acc[(v * NN + node) * LDM + el] + total;
                     const int64_t g = (int64_t)(v * NN + node) * E + e;
                     const double val = (double)total * fac;
                     if (!RK) {
