@@ -57,6 +57,22 @@ def time_rhs(torch, th, semi, u, du, steps, warmup, stream):
     return start.elapsed_time(stop) / steps, wall / steps * 1e3
 
 
+def time_vjp(torch, th, semi, u, steps, warmup, stream):
+    vbar = torch.ones_like(u)
+    ubar = torch.empty_like(u)
+    with torch.cuda.stream(stream):
+        for _ in range(warmup):
+            th.rhs_vjp(ubar, u, vbar, semi)
+        torch.cuda.synchronize()
+        start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        start.record(stream)
+        for _ in range(steps):
+            th.rhs_vjp(ubar, u, vbar, semi)
+        stop.record(stream)
+        torch.cuda.synchronize()
+    return start.elapsed_time(stop) / steps
+
+
 def time_rk(torch, th, semi, u, steps, stream):
     k = torch.zeros_like(u)
     uu = u.clone()
@@ -85,6 +101,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--rk-steps", type=int, default=20)
+    ap.add_argument("--vjp-steps", type=int, default=50)
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="only run a plain device copy of this many MiB (counter calibration)")
     args = ap.parse_args()
@@ -141,6 +158,10 @@ def main():
         }
         if args.rk_steps > 0:
             res[mode]["rk_step_ms"] = time_rk(torch, th, semi, u, args.rk_steps, stream)
+        if mode == "fp64" and args.vjp_steps > 0:
+            vms = time_vjp(torch, th, semi, u, args.vjp_steps, 3, stream)
+            res[mode]["vjp_ms"] = vms
+            res[mode]["adjoint_over_forward"] = vms / ms
         du_out[mode] = du.clone()
         semi_keep = semi
         if mode != modes[-1]:

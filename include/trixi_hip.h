@@ -191,6 +191,28 @@ int trixi_hip_integrate(trixi_hip_ctx* ctx, const double* u, double* integral,
 int trixi_hip_indicator(trixi_hip_ctx* ctx, const double* u, double* alpha,
                         trixi_hip_stream stream);
 
+/* Adjoint of rhs!: ubar = beta * ubar + (d rhs / d u)^T (alpha * vbar), FP64,
+ * asynchronous.  No reference counterpart (TrixiCUDA.jl has no adjoint); it
+ * backs a ChainRulesCore.rrule / torch.autograd.Function of rhs_gpu!
+ * (solvers.jl:18-31).  Built for linear advection and compressible Euler,
+ * weak form and flux differencing; other configurations return
+ * TRIXI_HIP_EUNSUPPORTED.  ubar must not alias u or vbar. */
+int trixi_hip_rhs_vjp(trixi_hip_ctx* ctx, double* ubar, const double* u, const double* vbar,
+                      double t, double alpha, double beta, trixi_hip_stream stream);
+
+/* Allocate the adjoint's work buffers (7 state vectors).  Call once before
+ * trixi_hip_rk_step_vjp; this is the only allocating call after create. */
+int trixi_hip_enable_adjoint(trixi_hip_ctx* ctx);
+
+/* Reverse of one CarpenterKennedy2N54 step from the step's initial state
+ * u_n (a checkpoint, not modified): on entry ubar holds the cotangent of
+ * u_{n+1}, on exit that of u_n.  The five stage inputs are recomputed in
+ * FP64 from u_n, then the stages are reversed (register cotangent update +
+ * rhs VJP per stage).  No reference counterpart (multi-step gradients with
+ * RK-stage checkpointing). */
+int trixi_hip_rk_step_vjp(trixi_hip_ctx* ctx, const double* u_n, double* ubar, double t,
+                          double dt, trixi_hip_stream stream);
+
 /* Positivity and finiteness check (FP64): writes a per-element flag
  * (bit 0: non-finite value, bit 1: rho <= 0, bit 2: p <= 0) into
  * flags[nelements] and the number of flagged elements to *nbad (blocks).

@@ -85,6 +85,7 @@ def lib():
             "oracle_ck2n54_step": (None, [cfgp, i64, i64, P, P, P, P, P, P, dbl, dbl]),
             "oracle_rhs_jvp": (None, [cfgp, i64, i64, P, P, P, P, P, P, P]),
             "oracle_set_threads": (ctypes.c_int, [ctypes.c_int]),
+            "oracle_ck2n54_step_jvp": (None, [cfgp, i64, i64, P, P, P, P, P, dbl, P, P]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -292,6 +293,28 @@ def ck2n54_step(cfg, mesh, u, k, t, dt):
     lib().oracle_ck2n54_step(ctypes.byref(cfg), mesh.nelements, mesh.ninterfaces, _p(mesh.neighbor_ids),
                              _p(mesh.orientations), _p(mesh.inverse_jacobian), _p(u), _p(k), _p(du),
                              float(t), float(dt))
+
+
+def ck2n54_step_jvp(cfg, mesh, u, udot, dt):
+    """(u_next, d u_next) of one CK2N54 step from u along the tangent udot."""
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    udot = np.ascontiguousarray(udot, dtype=np.float64)
+    un, und = np.zeros_like(u), np.zeros_like(u)
+    lib().oracle_ck2n54_step_jvp(ctypes.byref(cfg), mesh.nelements, mesh.ninterfaces, _p(mesh.neighbor_ids),
+                                 _p(mesh.orientations), _p(mesh.inverse_jacobian), _p(u), _p(udot), float(dt),
+                                 _p(un), _p(und))
+    return un, und
+
+
+def jacobian(cfg, mesh, u):
+    """Dense Jacobian of rhs! (tiny meshes only), column by column from the dual JVP."""
+    n = u.size
+    J = np.zeros((n, n))
+    for j in range(n):
+        e = np.zeros(n)
+        e[j] = 1.0
+        J[:, j] = rhs_jvp(cfg, mesh, u, e.reshape(u.shape))[1].ravel()
+    return J
 
 
 def set_threads(n):

@@ -1391,6 +1391,30 @@ void oracle_ck2n54_step(const oracle_cfg* cfg, int64_t E, int64_t I, const int32
     }
 }
 
+// Forward-mode derivative of one CK2N54 step: (u, k) -> (u_next) with
+// tangent udot; k starts at zero as in the low-storage scheme.
+void oracle_ck2n54_step_jvp(const oracle_cfg* cfg, int64_t E, int64_t I, const int32_t* nbr,
+                            const int32_t* ori, const double* invJ, const double* u, const double* udot,
+                            double dt, double* u_next, double* u_next_dot) {
+    double A[5], B[5], C[5];
+    oracle_ck2n54_coefficients(A, B, C);
+    Cfg c = make_cfg(cfg);
+    const size_t n = (size_t)E * ipow(c.N, c.nd) * c.nv;
+    std::vector<Dual> uu(n), k(n), du(n);
+    for (size_t q = 0; q < n; ++q) uu[q] = Dual(u[q], udot[q]);
+    for (int s = 0; s < 5; ++s) {
+        rhs_impl<Dual>(c, E, I, nbr, ori, invJ, uu.data(), du.data(), (Dual*)nullptr);
+        for (size_t q = 0; q < n; ++q) {
+            k[q] = (s == 0) ? dt * du[q] : A[s] * k[q] + dt * du[q];
+            uu[q] = uu[q] + B[s] * k[q];
+        }
+    }
+    for (size_t q = 0; q < n; ++q) {
+        u_next[q] = uu[q].v;
+        u_next_dot[q] = uu[q].d;
+    }
+}
+
 int oracle_set_threads(int nthreads) {
 #ifdef _OPENMP
     if (nthreads > 0) omp_set_num_threads(nthreads);

@@ -135,6 +135,12 @@ void oracle_rhs_jvp(const oracle_cfg* cfg, int64_t nelements, int64_t ninterface
                     const double* inverse_jacobian, const double* u,
                     const double* udot, double* du, double* dudot);
 
+void oracle_ck2n54_step_jvp(const oracle_cfg* cfg, int64_t nelements, int64_t ninterfaces,
+                            const int32_t* neighbor_ids, const int32_t* orientations,
+                            const double* inverse_jacobian, const double* u,
+                            const double* udot, double dt, double* u_next,
+                            double* u_next_dot);
+
 /* ---- OpenMP control for the timed CPU baseline ---------------------------------- */
 int oracle_set_threads(int nthreads);
 

@@ -1,0 +1,185 @@
+"""GPU adjoint: the hand-written VJP kernels against exact derivatives.
+
+* rhs! VJP == J^T vbar with J assembled column by column from the oracle's
+  dual-number JVP (tiny meshes), relative max-norm <= 1e-11;
+* dot-product test <vbar, J w> (oracle JVP) == <J^T vbar, w> (GPU VJP) on a
+  larger mesh, relative 1e-11;
+* reverse CK2N54 step against the oracle's dual-number step JVP;
+* 20-step gradient of the L2 norm with step checkpointing against
+  fourth-order central differences of the GPU forward map (smooth fluxes,
+  relative 1e-8), and, with LLF, against the oracle's chained step JVPs
+  (relative 1e-10);
+* the autograd function backward == VJP.
+"""
+import numpy as np
+import pytest
+
+from cases import Case, oracle_cfg, oracle_ic, product_semi
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def torch():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    return torch
+
+
+def setup(th, O, torch, case):
+    cfg = oracle_cfg(O, case)
+    mesh = O.Mesh(case.ndims, case.level, case.cmin, case.cmax, case.polydeg)
+    u = O.compute_coefficients(cfg, mesh, oracle_ic(O, case), 0.0, case.seed)
+    semi = product_semi(th, case)
+    ud = th.pack(torch.from_numpy(u).cuda(), semi)
+    return cfg, mesh, u, semi, ud
+
+
+TINY = [
+    Case("linadv1d_weak_llf", 1, "linadv", 2, cmin=-1.0, cmax=1.0, ic="convergence_test", advection=(0.7,)),
+    Case("euler1d_weak_llf", 1, "euler", 2),
+    Case("euler1d_fd_ranocha_llf", 1, "euler", 2, "fd", "ranocha"),
+    Case("euler2d_fd_ranocha_llf", 2, "euler", 1, "fd", "ranocha"),
+    Case("euler2d_fd_ranocha_llfnaive", 2, "euler", 1, "fd", "ranocha", "llf_naive"),
+    Case("euler2d_fd_shima_shima", 2, "euler", 1, "fd", "shima_etal", "shima_etal"),
+    Case("euler2d_weak_central", 2, "euler", 1, sflux="central"),
+    Case("euler3d_fd_ranocha_ranocha", 3, "euler", 1, "fd", "ranocha", "ranocha"),
+]
+
+
+@pytest.mark.parametrize("case", TINY, ids=lambda c: c.name)
+def test_vjp_equals_transposed_oracle_jacobian(th, oracle, torch, case):
+    O = oracle
+    cfg, mesh, u, semi, ud = setup(th, O, torch, case)
+    J = O.jacobian(cfg, mesh, u)
+    rng = np.random.default_rng(7)
+    vbar = rng.standard_normal(u.shape)
+    expect = (J.T @ vbar.ravel()).reshape(u.shape)
+    vd = th.pack(torch.from_numpy(vbar).cuda(), semi)
+    ubar = torch.empty_like(vd)
+    th.rhs_vjp(ubar, ud, vd, semi)
+    got = th.wrap_array_native(ubar, semi).cpu().numpy()
+    err = float(np.abs(got - expect).max() / np.abs(expect).max())
+    print(f"{case.name}: VJP vs J^T vbar rel max {err:.3e}", flush=True)
+    assert err < 1e-11, err
+    # beta accumulation: ubar = 2 ubar + J^T (3 vbar)
+    th.rhs_vjp(ubar, ud, vd, semi, alpha=3.0, beta=2.0)
+    got2 = th.wrap_array_native(ubar, semi).cpu().numpy()
+    assert float(np.abs(got2 - 5 * expect).max() / np.abs(expect).max()) < 1e-11
+    semi.close()
+
+
+@pytest.mark.parametrize("case", [Case("euler3d_fd_ranocha_llf", 3, "euler", 2, "fd", "ranocha", ic="smooth_periodic"),
+                                  Case("euler2d_fd_shima_llf", 2, "euler", 3, "fd", "shima_etal"),
+                                  Case("linadv3d_weak_llf", 3, "linadv", 2)],
+                         ids=lambda c: c.name)
+def test_vjp_dot_product(th, oracle, torch, case):
+    O = oracle
+    cfg, mesh, u, semi, ud = setup(th, O, torch, case)
+    rng = np.random.default_rng(8)
+    w = rng.standard_normal(u.shape)
+    vbar = rng.standard_normal(u.shape)
+    _, jw = O.rhs_jvp(cfg, mesh, u, w)
+    ubar = torch.empty_like(ud)
+    th.rhs_vjp(ubar, ud, th.pack(torch.from_numpy(vbar).cuda(), semi), semi)
+    lhs = float((vbar * jw).sum())
+    rhs = float((th.wrap_array_native(ubar, semi).cpu().numpy() * w).sum())
+    print(f"{case.name}: <v,Jw>={lhs:.15e} <J^T v,w>={rhs:.15e}", flush=True)
+    assert abs(lhs - rhs) <= 1e-11 * max(abs(lhs), np.abs(vbar).sum() * np.abs(jw).max() * 1e-3)
+    semi.close()
+
+
+def test_rk_step_vjp_dot_product(th, oracle, torch):
+    O = oracle
+    case = Case("euler2d_fd", 2, "euler", 2, "fd", "ranocha")
+    cfg, mesh, u, semi, ud = setup(th, O, torch, case)
+    dt = 0.5 * O.max_dt(cfg, mesh, u)
+    rng = np.random.default_rng(9)
+    w = rng.standard_normal(u.shape)
+    vbar = rng.standard_normal(u.shape)
+    _, dphi = O.ck2n54_step_jvp(cfg, mesh, u, w, dt)
+    th.enable_adjoint(semi)
+    ubar = th.pack(torch.from_numpy(vbar).cuda(), semi)
+    th.rk_step_vjp(ud, ubar, 0.0, dt, semi)
+    lhs = float((vbar * dphi).sum())
+    rhs = float((th.wrap_array_native(ubar, semi).cpu().numpy() * w).sum())
+    print(f"rk step: <v,dPhi w>={lhs:.15e} <dPhi^T v,w>={rhs:.15e}", flush=True)
+    assert abs(lhs - rhs) <= 1e-10 * abs(lhs)
+    semi.close()
+
+
+def test_multistep_gradient_matches_finite_differences(th, oracle, torch):
+    """Gradient of ||u(t_20)||_L2 w.r.t. u0 over 20 CK2N54 steps, checkpointed,
+    against fourth-order central differences of the GPU forward map.  The
+    surface flux is flux_ranocha, so the map is smooth (LLF's max wave speed
+    has kinks that finite differences step across: covered by the next test)."""
+    case = Case("euler2d_fd", 2, "euler", 2, "fd", "ranocha", "ranocha", ic="smooth_periodic")
+    semi = product_semi(th, case)
+    u0 = th.compute_coefficients(0.0, semi)
+    dt = 0.5 * th.max_dt(u0, semi)
+    nsteps = 20
+    traj = th.forward_with_checkpoints(semi, u0, dt, nsteps)
+    loss, g_final = th.l2_norm_and_gradient(traj.u_final, semi)
+    grad = th.backward(traj, g_final)
+
+    def loss_of(u_init):
+        tr = th.forward_with_checkpoints(semi, u_init, dt, nsteps)
+        return float(th.l2_norm_and_gradient(tr.u_final, semi)[0])
+
+    gen = torch.Generator(device="cuda").manual_seed(10)
+    for i in range(3):
+        d = torch.randn(u0.shape, dtype=torch.float64, device="cuda", generator=gen)
+        h = 1e-4
+        fd = (-loss_of(u0 + 2 * h * d) + 8 * loss_of(u0 + h * d) - 8 * loss_of(u0 - h * d)
+              + loss_of(u0 - 2 * h * d)) / (12 * h)
+        ad = float((grad * d).sum())
+        print(f"direction {i}: adjoint {ad:.12e} fourth-order FD {fd:.12e}", flush=True)
+        assert abs(ad - fd) <= 1e-8 * abs(fd) + 1e-12
+    semi.close()
+
+
+@pytest.mark.parametrize("sflux", ["llf", "llf_naive"])
+def test_multistep_gradient_matches_oracle_tangent(th, oracle, torch, sflux):
+    """<d u0, adjoint of g> over 20 checkpointed CK2N54 steps with an LLF
+    surface flux == <g, d u_20> from the oracle's chained dual-number step JVPs."""
+    O = oracle
+    case = Case("euler2d_fd", 2, "euler", 2, "fd", "ranocha", sflux, ic="smooth_periodic")
+    cfg, mesh, u, semi, ud = setup(th, O, torch, case)
+    dt = 0.5 * O.max_dt(cfg, mesh, u)
+    nsteps = 20
+    rng = np.random.default_rng(11)
+    d = rng.standard_normal(u.shape)
+    g = rng.standard_normal(u.shape)
+    ut, dt_ = u.copy(), d.copy()
+    for _ in range(nsteps):
+        ut, dt_ = O.ck2n54_step_jvp(cfg, mesh, ut, dt_, dt)
+    exact = float((g * dt_).sum())
+    traj = th.forward_with_checkpoints(semi, ud, dt, nsteps)
+    grad = th.backward(traj, th.pack(torch.from_numpy(g).cuda(), semi))
+    ad = float((th.wrap_array_native(grad, semi).cpu().numpy() * d).sum())
+    print(f"{sflux}: adjoint {ad:.15e} oracle tangent {exact:.15e}", flush=True)
+    assert abs(ad - exact) <= 1e-10 * abs(exact)
+    semi.close()
+
+
+def test_autograd_function_backward(th, oracle, torch):
+    case = Case("euler3d_fd", 3, "euler", 1, "fd", "ranocha")
+    semi = product_semi(th, case)
+    u = th.compute_coefficients(0.0, semi).requires_grad_(True)
+    c = torch.randn_like(u)
+    (th.rhs(u, semi) * c).sum().backward()
+    ubar = torch.empty_like(c)
+    th.rhs_vjp(ubar, u.detach(), c, semi)
+    assert torch.allclose(u.grad, ubar, rtol=0, atol=0)
+    semi.close()
+
+
+def test_adjoint_unsupported_is_an_error(th, torch):
+    c = Case("mhd", 3, "mhd", 1, "fd", "hindenlang_gassner", cmin=-1.0, cmax=1.0, gamma=5 / 3)
+    semi = product_semi(th, c)
+    u = th.compute_coefficients(0.0, semi)
+    with pytest.raises(th.TrixiHipError) as ei:
+        th.rhs_vjp(torch.empty_like(u), u, torch.ones_like(u), semi)
+    assert ei.value.code == -2
+    semi.close()

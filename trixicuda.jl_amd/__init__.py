@@ -29,5 +29,7 @@ from .semidiscretization import (ODEProblem, SemidiscretizationHyperbolicHIP, co
 from .solver import (DGSEMHIP, IndicatorHennemannGassner, LobattoLegendreBasisHIP, VolumeIntegralFluxDifferencing,
                      VolumeIntegralShockCapturingHG, VolumeIntegralWeakForm)
 from .time_integration import CarpenterKennedy2N54, Solution, rk_step, solve
+from .adjoint import (backward, enable_adjoint, forward_with_checkpoints, l2_norm_and_gradient, quadrature_weights,
+                      rhs, rhs_vjp, rk_step_vjp)
 
 __all__ = [n for n in dir() if not n.startswith("_")]

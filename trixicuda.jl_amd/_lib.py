@@ -76,6 +76,9 @@ SIGNATURES = {
     "trixi_hip_integrate": (ctypes.c_int, [_P, _P, _P, _P]),
     "trixi_hip_indicator": (ctypes.c_int, [_P, _P, _P, _P]),
     "trixi_hip_check_state": (ctypes.c_int, [_P, _P, _P, _P, _P]),
+    "trixi_hip_rhs_vjp": (ctypes.c_int, [_P, _P, _P, _P, _D, _D, _D, _P]),
+    "trixi_hip_enable_adjoint": (ctypes.c_int, [_P]),
+    "trixi_hip_rk_step_vjp": (ctypes.c_int, [_P, _P, _P, _D, _D, _P]),
 }
 
 _lib = None

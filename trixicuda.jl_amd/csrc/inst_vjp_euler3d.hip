@@ -1,0 +1,20 @@
+// Adjoint kernel instantiations: compressible Euler 3D, weak form and flux differencing.
+#include "registry.hpp"
+
+namespace trixi_hip {
+
+void register_vjp_euler3d() {
+    using EQ = EulerVjp<3>;
+    constexpr int C = FLUX_CENTRAL;
+    VjpRegistrar<3, 4, EQ, VI_WEAK, C, FLUX_LLF>::add(EQ_EULER);
+    VjpRegistrar<3, 4, EQ, VI_WEAK, C, FLUX_LLF_NAIVE>::add(EQ_EULER);
+    VjpRegistrar<3, 4, EQ, VI_WEAK, C, FLUX_CENTRAL>::add(EQ_EULER);
+    VjpRegistrar<3, 4, EQ, VI_FD, FLUX_RANOCHA, FLUX_LLF>::add(EQ_EULER);
+    VjpRegistrar<3, 4, EQ, VI_FD, FLUX_RANOCHA, FLUX_LLF_NAIVE>::add(EQ_EULER);
+    VjpRegistrar<3, 4, EQ, VI_FD, FLUX_RANOCHA, FLUX_RANOCHA>::add(EQ_EULER);
+    VjpRegistrar<3, 4, EQ, VI_FD, FLUX_SHIMA_ETAL, FLUX_LLF>::add(EQ_EULER);
+    VjpRegistrar<3, 4, EQ, VI_FD, FLUX_SHIMA_ETAL, FLUX_LLF_NAIVE>::add(EQ_EULER);
+    VjpRegistrar<3, 4, EQ, VI_FD, FLUX_SHIMA_ETAL, FLUX_SHIMA_ETAL>::add(EQ_EULER);
+}
+
+}  // namespace trixi_hip

@@ -2,6 +2,7 @@
 #pragma once
 
 #include "rhs_kernel.hpp"
+#include "vjp_kernel.hpp"
 
 namespace trixi_hip {
 
@@ -17,9 +18,32 @@ struct KernelKey {
 };
 
 using LaunchFn = hipError_t (*)(const RhsArgs&, hipStream_t);
+using VjpFn = hipError_t (*)(const VjpArgs&, hipStream_t);
 
 void register_kernel(const KernelKey& key, LaunchFn fn);
 LaunchFn find_kernel(const KernelKey& key);
+void register_vjp(const KernelKey& key, VjpFn fn);
+VjpFn find_vjp(const KernelKey& key);
+
+// Workgroup width of the adjoint kernel (FP64): node records with 1/rho and
+// 1/p, FP64 logarithms and the cotangent accumulator within 40 KiB.
+template <int DIM, int N, int NV, int NRF>
+constexpr int pick_m_vjp() {
+    constexpr int NL = cpow(N, DIM - 1), NN = cpow(N, DIM);
+    constexpr int per_elem = NN * (NRF * 8 + 2 * 8 + 2 * NV * 8);
+    int m = 256 / NL;
+    if (m < 1) m = 1;
+    while (m > 1 && m * per_elem > 44 * 1024) m /= 2;
+    return m;
+}
+
+template <int DIM, int N, class EQ, int VOL, int VF, int SF>
+struct VjpRegistrar {
+    static void add(int eq) {
+        constexpr int M = pick_m_vjp<DIM, N, EQ::NV, EQ::NRF>();
+        register_vjp({DIM, N, eq, VOL, VF, SF, FLUX_CENTRAL, 0, 0}, &launch_vjp<DIM, N, EQ, VOL, VF, SF, double, M>);
+    }
+};
 
 // Workgroup width in elements: fill about 256 lanes and keep the LDS tiles
 // (node records, FP64 logarithms, direction accumulator) within 40 KiB so

@@ -29,6 +29,15 @@ LaunchFn find_kernel(const KernelKey& key) {
     auto it = table().find(key);
     return it == table().end() ? nullptr : it->second;
 }
+static std::map<KernelKey, VjpFn>& vjp_table() {
+    static std::map<KernelKey, VjpFn> t;
+    return t;
+}
+void register_vjp(const KernelKey& key, VjpFn fn) { vjp_table()[key] = fn; }
+VjpFn find_vjp(const KernelKey& key) {
+    auto it = vjp_table().find(key);
+    return it == vjp_table().end() ? nullptr : it->second;
+}
 
 void register_linadv();
 void register_euler1d();
@@ -38,6 +47,10 @@ void register_euler1d_hg();
 void register_euler2d_hg();
 void register_euler3d_hg();
 void register_mhd3d();
+void register_vjp_linadv();
+void register_vjp_euler1d();
+void register_vjp_euler2d();
+void register_vjp_euler3d();
 
 static void ensure_registered() {
     static std::once_flag once;
@@ -50,6 +63,10 @@ static void ensure_registered() {
         register_euler2d_hg();
         register_euler3d_hg();
         register_mhd3d();
+        register_vjp_linadv();
+        register_vjp_euler1d();
+        register_vjp_euler2d();
+        register_vjp_euler3d();
     });
 }
 
@@ -488,6 +505,14 @@ __global__ void copy_kernel(double* __restrict__ dst, const double* __restrict__
         dst[i] = src[i];
 }
 
+// Reverse of one low-storage RK stage's register update:
+// kbar <- a * kbar + b * ubar   (a is ignored, kbar overwritten, when first)
+__global__ void kbar_update_kernel(double* __restrict__ kbar, const double* __restrict__ ubar, double a,
+                                   double b, int first, int64_t n) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        kbar[i] = first ? b * ubar[i] : a * kbar[i] + b * ubar[i];
+}
+
 __global__ void set_time_kernel(double* time, double t, double dt, double c0, double c1, double c2,
                                 double c3, double c4) {
     if (threadIdx.x == 0 && blockIdx.x == 0) {
@@ -720,6 +745,13 @@ struct trixi_hip_ctx {
     int nblocks_red = 1024;
     RhsArgs args;
     LaunchFn rhs_fn = nullptr, rk_fn = nullptr;
+    // adjoint
+    VjpFn vjp_fn = nullptr;
+    LaunchFn rk64_fn = nullptr;   // FP64 forward stage for the adjoint's recomputation
+    double* d_stage = nullptr;    // 5 stage inputs of one RK step
+    double* d_k = nullptr;        // forward register during recomputation
+    double* d_kbar = nullptr;     // adjoint register
+    bool adjoint = false;
     // graph cache for rk steps
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
@@ -761,8 +793,9 @@ static void free_ctx(trixi_hip_ctx* c) {
     if (!c) return;
     if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
     if (c->graph) (void)hipGraphDestroy(c->graph);
-    void* bufs[] = {c->d_nbr, c->d_invJ, c->d_x, c->d_alpha_raw, c->d_alpha, c->d_time, c->d_partial,
-                    c->d_result, c->d_Vinv, c->d_Vdm, c->d_wa, c->d_w, c->d_tmp};
+    void* bufs[] = {c->d_nbr, c->d_invJ, c->d_x,     c->d_alpha_raw, c->d_alpha, c->d_time,
+                    c->d_partial, c->d_result, c->d_Vinv, c->d_Vdm,   c->d_wa,    c->d_w,
+                    c->d_tmp, c->d_stage, c->d_k, c->d_kbar};
     for (void* p : bufs)
         if (p) (void)hipFree(p);
     delete c;
@@ -1193,6 +1226,109 @@ int trixi_hip_indicator(trixi_hip_ctx* ctx, const double* u, double* alpha, trix
         hipLaunchKernelGGL(copy_kernel, dim3(256), dim3(256), 0, s, alpha, ctx->d_alpha_raw, ctx->E);
     }
     HIP_TRY(hipGetLastError());
+    return TRIXI_HIP_OK;
+}
+
+static VjpArgs vjp_args(const trixi_hip_ctx* c) {
+    VjpArgs v;
+    memset(&v, 0, sizeof(v));
+    v.nbr = c->d_nbr;
+    v.invJ = c->d_invJ;
+    v.E = c->E;
+    v.phys = c->args.phys;
+    for (int q = 0; q < MAXN * MAXN; ++q) {
+        v.Ds[q] = c->args.Ds[q];
+        v.Dhat[q] = c->args.Dhat[q];
+    }
+    v.lhat_m = c->args.lhat_m;
+    v.lhat_p = c->args.lhat_p;
+    v.nxcd = c->args.nxcd;
+    return v;
+}
+
+static int require_vjp(trixi_hip_ctx* c) {
+    if (c->vjp_fn) return TRIXI_HIP_OK;
+    KernelKey k = make_key(c->cfg, 0);
+    k.prec = 0;
+    c->vjp_fn = find_vjp(k);
+    if (!c->vjp_fn)
+        return fail(TRIXI_HIP_EUNSUPPORTED,
+                    "no adjoint kernel for this configuration (built: linear advection and compressible "
+                    "Euler, weak form and flux differencing, polydeg 3)");
+    return TRIXI_HIP_OK;
+}
+
+int trixi_hip_rhs_vjp(trixi_hip_ctx* ctx, double* ubar, const double* u, const double* vbar, double t,
+                      double alpha, double beta, trixi_hip_stream stream) {
+    if (!ctx || !ubar || !u || !vbar) return fail(TRIXI_HIP_EINVAL, "null argument");
+    if (ubar == vbar || ubar == u) return fail(TRIXI_HIP_EINVAL, "ubar must not alias u or vbar");
+    (void)t;
+    int rc = require_vjp(ctx);
+    if (rc) return rc;
+    VjpArgs v = vjp_args(ctx);
+    v.u = u;
+    v.vbar = vbar;
+    v.ubar = ubar;
+    v.alpha = alpha;
+    v.beta = beta;
+    HIP_TRY(ctx->vjp_fn(v, (hipStream_t)stream));
+    return TRIXI_HIP_OK;
+}
+
+int trixi_hip_enable_adjoint(trixi_hip_ctx* ctx) {
+    if (!ctx) return fail(TRIXI_HIP_EINVAL, "null context");
+    int rc = require_vjp(ctx);
+    if (rc) return rc;
+    if (ctx->adjoint) return TRIXI_HIP_OK;
+    trixi_hip_config c64 = ctx->cfg;
+    c64.precision = TRIXI_HIP_FP64;
+    ctx->rk64_fn = find_kernel(make_key(c64, 1));
+    if (!ctx->rk64_fn) return fail(TRIXI_HIP_EUNSUPPORTED, "no FP64 stage kernel for the adjoint");
+    const size_t bytes = (size_t)ctx->nvalues * sizeof(double);
+    if (hipMalloc((void**)&ctx->d_stage, 5 * bytes) != hipSuccess ||
+        hipMalloc((void**)&ctx->d_k, bytes) != hipSuccess || hipMalloc((void**)&ctx->d_kbar, bytes) != hipSuccess)
+        return fail(TRIXI_HIP_ENOMEM, "hipMalloc failed for the adjoint buffers");
+    ctx->adjoint = true;
+    return TRIXI_HIP_OK;
+}
+
+int trixi_hip_rk_step_vjp(trixi_hip_ctx* ctx, const double* u_n, double* ubar, double t, double dt,
+                          trixi_hip_stream stream) {
+    if (!ctx || !u_n || !ubar) return fail(TRIXI_HIP_EINVAL, "null argument");
+    if (!ctx->adjoint) return fail(TRIXI_HIP_EINVAL, "call trixi_hip_enable_adjoint first");
+    hipStream_t s = (hipStream_t)stream;
+    const int64_t n = ctx->nvalues;
+    double* U[5];
+    for (int st = 0; st < 5; ++st) U[st] = ctx->d_stage + (size_t)st * n;
+    // recompute the stage inputs U_0 .. U_4 in FP64
+    hipLaunchKernelGGL(set_time_kernel, dim3(1), dim3(1), 0, s, ctx->d_time, t, dt, CK_C[0], CK_C[1], CK_C[2],
+                       CK_C[3], CK_C[4]);
+    hipLaunchKernelGGL(copy_kernel, dim3(1024), dim3(256), 0, s, U[0], u_n, n);
+    HIP_TRY(hipGetLastError());
+    for (int st = 0; st < 4; ++st) {
+        RhsArgs a = ctx->args;
+        a.u = U[st];
+        a.k = ctx->d_k;
+        a.u_out = U[st + 1];
+        a.time = ctx->d_time + 2 * st;
+        a.rk_a = CK_A[st];
+        a.rk_b = CK_B[st];
+        a.rk_first = (st == 0);
+        HIP_TRY(ctx->rk64_fn(a, s));
+    }
+    // reverse the stages
+    for (int st = 4; st >= 0; --st) {
+        hipLaunchKernelGGL(kbar_update_kernel, dim3(1024), dim3(256), 0, s, ctx->d_kbar, (const double*)ubar,
+                           st < 4 ? CK_A[st + 1] : 0.0, CK_B[st], st == 4 ? 1 : 0, n);
+        HIP_TRY(hipGetLastError());
+        VjpArgs v = vjp_args(ctx);
+        v.u = U[st];
+        v.vbar = ctx->d_kbar;
+        v.ubar = ubar;
+        v.alpha = dt;
+        v.beta = 1.0;
+        HIP_TRY(ctx->vjp_fn(v, s));
+    }
     return TRIXI_HIP_OK;
 }
 

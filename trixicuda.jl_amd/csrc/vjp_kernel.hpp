@@ -1,0 +1,223 @@
+// vjp_kernel.hpp -- adjoint (vector-Jacobian product) of the fused rhs!.
+//
+//   ubar = beta * ubar + (d rhs / d u)^T (alpha * vbar)
+//
+// rhs(u)_e = -invJ_e [ volume(u_e) + surface(u_e, u_neighbours) ], so with
+// w_e = -invJ_e * alpha * vbar_e (the cotangent of the element accumulator):
+//  * a volume pair (i, j) along a line, acc_i += Ds[i,j] f*, acc_j += Ds[j,i] f*,
+//    has flux cotangent  g = Ds[i,j] w_i + Ds[j,i] w_j;
+//  * the weak form acc_i += Dhat[i,p] f(u_p) has g_p = sum_i Dhat[i,p] w_i;
+//  * an interface flux F(u_L, u_R) enters acc_L (end node) with +lhat_p and
+//    acc_R (first node) with -lhat_m, so g = lhat_p w_L - lhat_m w_R.
+//    Each element evaluates the interfaces of its own faces and keeps only
+//    its own side's cotangent, reading the neighbour's w at the face node:
+//    every element writes only its own ubar, with no atomics.
+// The work decomposition (M elements per workgroup, one lane per element
+// line, direction phases combined through LDS, XCD-aware tiles) is that of
+// rhs_kernel; cotangents are accumulated in primitive space and converted to
+// conservative space once per node.
+#pragma once
+
+#include "physics_vjp.hpp"
+#include "rhs_kernel.hpp"
+
+namespace trixi_hip {
+
+struct VjpArgs {
+    const double* u;
+    const double* vbar;
+    double* ubar;
+    const int32_t* nbr;
+    const double* invJ;
+    int64_t E;
+    PhysParams phys;
+    double Ds[MAXN * MAXN];
+    double Dhat[MAXN * MAXN];
+    double lhat_m, lhat_p;
+    double alpha, beta;
+    int nxcd;
+};
+
+template <int DIM, int N, class EQ, int VOL, int VF, int SF, typename R, int M>
+__global__ void __launch_bounds__(M * cpow(N, DIM - 1))
+vjp_kernel(const VjpArgs a) {
+    constexpr int NL = cpow(N, DIM - 1);
+    constexpr int NN = cpow(N, DIM);
+    constexpr int NV = EQ::NV;
+    constexpr int NRF = EQ::NRF;
+    using Node = typename EQ::template Node<R>;
+    using Bar = typename EQ::template Bar<R>;
+
+    __shared__ R rec[NRF * NN * M];
+    __shared__ double recd[2 * NN * M];
+    __shared__ R ws[NV * NN * M];   // w = -invJ alpha vbar of the tile
+    __shared__ R acc[NV * NN * M];  // primitive cotangents, summed over directions
+
+    const int tid = threadIdx.x;
+    const int el = tid % M;
+    const int line = tid / M;
+    const int64_t E = a.E;
+    const int64_t e0 = xcd_tile(a.nxcd) * M;
+    if (e0 >= E) return;
+    const int64_t e = e0 + el;
+    const int64_t ec = e < E ? e : E - 1;
+    const PhysR<R> P = PhysR<R>::from(a.phys);
+
+    int32_t nb[2 * DIM];
+#pragma unroll
+    for (int f = 0; f < 2 * DIM; ++f) nb[f] = a.nbr[(int64_t)f * E + ec];
+
+    for (int q = tid; q < NN * M; q += M * NL) {
+        const int c = q % M;
+        const int n = q / M;
+        int64_t ee = e0 + c;
+        if (ee >= E) ee = E - 1;
+        R uc[NV];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) uc[v] = (R)a.u[(int64_t)(v * NN + n) * E + ee];
+        const Node nd = EQ::node(uc, P);
+        R f[NRF];
+        EQ::to_rec(nd, f);
+#pragma unroll
+        for (int k = 0; k < NRF; ++k) rec[(k * NN + n) * M + c] = f[k];
+        recd[n * M + c] = nd.lrho;
+        recd[(NN + n) * M + c] = nd.lp;
+        const R wsc = (R)(-a.invJ[ee] * a.alpha);
+#pragma unroll
+        for (int v = 0; v < NV; ++v) ws[(v * NN + n) * M + c] = wsc * (R)a.vbar[(int64_t)(v * NN + n) * E + ee];
+    }
+    __syncthreads();
+
+    const R lm = (R)a.lhat_m, lp = (R)a.lhat_p;
+    auto load_node = [&](int node) -> Node {
+        R f[NRF];
+#pragma unroll
+        for (int k = 0; k < NRF; ++k) f[k] = rec[(k * NN + node) * M + el];
+        return EQ::template from_rec<R>(f, recd[node * M + el], recd[(NN + node) * M + el]);
+    };
+    // acc (+)= b at a node of this lane's element; the first direction stores
+    auto flush = [&](int d, int node, const Bar& b) {
+        R t[NV];
+        EQ::bar_to_array(b, t);
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            const int idx = (v * NN + node) * M + el;
+            acc[idx] = d == 0 ? t[v] : acc[idx] + t[v];
+        }
+    };
+    // Interface at the -/+ face of the line: neighbour face node state and
+    // cotangent, then the flux VJP keeping only this element's side.
+    auto face = [&](int d, int base, int s, bool plus, const Node& own, Bar& b) {
+        const int64_t fn = nb[2 * d + (plus ? 1 : 0)];
+        const R wn = (R)(-a.invJ[fn] * a.alpha);
+        const int nnode = plus ? base : base + (N - 1) * s;
+        const int onode = plus ? base + (N - 1) * s : base;
+        R un[NV], g[NV];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            const int64_t r = (int64_t)(v * NN + nnode) * E + fn;
+            un[v] = (R)a.u[r];
+            const R wo = ws[(v * NN + onode) * M + el];
+            g[v] = plus ? lp * wo - lm * wn * (R)a.vbar[r] : lp * wn * (R)a.vbar[r] - lm * wo;
+        }
+        const Node ndn = EQ::node(un, P);
+        Bar dummy;
+        EQ::zero(dummy);
+        if (plus)
+            EQ::template two_point_vjp<SF>(own, ndn, d, g, b, dummy, P);
+        else
+            EQ::template two_point_vjp<SF>(ndn, own, d, g, dummy, b, P);
+    };
+
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+        const int s = cpow(N, d);
+        const int base = line_base<DIM, N>(line, d);
+        auto W = [&](int p, int v) -> R { return ws[(v * NN + base + p * s) * M + el]; };
+
+        Bar bb[N];
+#pragma unroll
+        for (int p = 0; p < N; ++p) EQ::zero(bb[p]);
+        {
+            Node nd[N];
+#pragma unroll
+            for (int p = 0; p < N; ++p) nd[p] = load_node(base + p * s);
+            if (VOL == VI_WEAK) {
+#pragma unroll
+                for (int p = 0; p < N; ++p) {
+                    R g[NV];
+#pragma unroll
+                    for (int v = 0; v < NV; ++v) g[v] = R(0);
+#pragma unroll
+                    for (int i = 0; i < N; ++i) {
+                        const R dh = (R)a.Dhat[i * N + p];
+#pragma unroll
+                        for (int v = 0; v < NV; ++v) g[v] += dh * W(i, v);
+                    }
+                    EQ::flux_vjp(nd[p], d, g, bb[p], P);
+                }
+            } else {
+#pragma unroll
+                for (int i = 0; i < N; ++i)
+#pragma unroll
+                    for (int j = i + 1; j < N; ++j) {
+                        const R fa = (R)a.Ds[i * N + j], fb = (R)a.Ds[j * N + i];
+                        R g[NV];
+#pragma unroll
+                        for (int v = 0; v < NV; ++v) g[v] = fa * W(i, v) + fb * W(j, v);
+                        EQ::template two_point_vjp<VF>(nd[i], nd[j], d, g, bb[i], bb[j], P);
+                    }
+            }
+        }
+#pragma unroll
+        for (int p = 1; p < N - 1; ++p) flush(d, base + p * s, bb[p]);
+        {
+            const Node own = load_node(base);
+            face(d, base, s, false, own, bb[0]);
+        }
+        flush(d, base, bb[0]);
+        {
+            const Node own = load_node(base + (N - 1) * s);
+            face(d, base, s, true, own, bb[N - 1]);
+        }
+        flush(d, base + (N - 1) * s, bb[N - 1]);
+        __syncthreads();
+    }
+
+    // (d prim / d cons)^T per node, ubar = beta ubar + result
+    for (int q = tid; q < NN * M; q += M * NL) {
+        const int c = q % M;
+        const int n = q / M;
+        const int64_t ee = e0 + c;
+        if (ee >= E) continue;
+        R f[NRF];
+#pragma unroll
+        for (int k = 0; k < NRF; ++k) f[k] = rec[(k * NN + n) * M + c];
+        const Node nd = EQ::template from_rec<R>(f, recd[n * M + c], recd[(NN + n) * M + c]);
+        R t[NV];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) t[v] = acc[(v * NN + n) * M + c];
+        Bar tot;
+        EQ::array_to_bar(t, tot);
+        R ub[NV];
+        EQ::to_cons_bar(nd, tot, ub, P);
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            const int64_t g = (int64_t)(v * NN + n) * E + ee;
+            const double prev = a.beta != 0.0 ? a.beta * a.ubar[g] : 0.0;
+            a.ubar[g] = prev + (double)ub[v];
+        }
+    }
+}
+
+template <int DIM, int N, class EQ, int VOL, int VF, int SF, typename R, int M>
+hipError_t launch_vjp(const VjpArgs& a, hipStream_t stream) {
+    constexpr int NL = cpow(N, DIM - 1);
+    int64_t blocks = (a.E + M - 1) / M;
+    if (a.nxcd > 1) blocks = (blocks + a.nxcd - 1) / a.nxcd * a.nxcd;
+    hipLaunchKernelGGL((vjp_kernel<DIM, N, EQ, VOL, VF, SF, R, M>), dim3((unsigned)blocks), dim3(M * NL), 0,
+                       stream, a);
+    return hipGetLastError();
+}
+
+}  // namespace trixi_hip

@@ -1,0 +1,201 @@
+# TrixiHIP.jl -- Julia host binding of libtrixihip.so (the gfx950 DGSEM
+# rhs! for Trixi.jl's TreeMesh).  HIP-named twins of TrixiCUDA.jl's exports
+# (reference src/TrixiCUDA.jl:74-77): DGSEMHIP, SemidiscretizationHyperbolicHIP,
+# semidiscretizeHIP, plus rhs_hip! (rhs_gpu!, src/solvers/solvers.jl:18-31).
+#
+# Shipped as source: it needs Julia, Trixi.jl and AMDGPU.jl (for ROCArray
+# device buffers), none of which exist in the build container.  The C ABI it
+# calls (include/trixi_hip.h) is exercised by the Python tests.
+module TrixiHIP
+
+using Trixi
+using AMDGPU
+using SciMLBase: ODEProblem, FullSpecialize
+
+const libtrixihip = get(ENV, "TRIXI_HIP_LIBRARY",
+                        joinpath(@__DIR__, "..", "libtrixihip.so"))
+
+# ---- C structs (must match include/trixi_hip.h) ------------------------------
+struct CConfig
+    ndims::Int32
+    equation::Int32
+    polydeg::Int32
+    volume_integral::Int32
+    volume_flux::Int32
+    surface_flux::Int32
+    fv_flux::Int32
+    precision::Int32
+    alpha_smooth::Int32
+    reserved::Int32
+    gamma::Float64
+    c_h::Float64
+    advection_velocity::NTuple{3, Float64}
+    alpha_max::Float64
+    alpha_min::Float64
+    ic_length::Float64
+end
+
+struct CMesh
+    nelements::Int64
+    ninterfaces::Int64
+    neighbor_ids::Ptr{Int32}
+    orientations::Ptr{Int32}
+    inverse_jacobian::Ptr{Float64}
+    node_coordinates::Ptr{Float64}
+    total_volume::Float64
+end
+
+function check(rc)
+    if rc != 0
+        msg = unsafe_string(ccall((:trixi_hip_last_error, libtrixihip), Cstring, ()))
+        error("trixi_hip error $rc: $msg")
+    end
+    nothing
+end
+
+# ---- identities of Trixi objects (a Julia function cannot cross a C ABI) -----
+equation_id(::LinearScalarAdvectionEquation1D) = 0
+equation_id(::LinearScalarAdvectionEquation2D) = 0
+equation_id(::LinearScalarAdvectionEquation3D) = 0
+equation_id(::CompressibleEulerEquations1D) = 1
+equation_id(::CompressibleEulerEquations2D) = 1
+equation_id(::CompressibleEulerEquations3D) = 1
+equation_id(::IdealGlmMhdEquations3D) = 2
+equation_id(eq) = throw(ArgumentError("equation $(typeof(eq)) is not built for HIP"))
+
+flux_id(::typeof(flux_central)) = 0
+flux_id(f::FluxLaxFriedrichs) = f.max_abs_speed === max_abs_speed_naive ? 2 : 1
+flux_id(::typeof(flux_ranocha)) = 3
+flux_id(::typeof(flux_shima_etal)) = 4
+flux_id(::typeof(flux_hindenlang_gassner)) = 5
+flux_id(f::Tuple) = (f[2] === flux_nonconservative_powell ? flux_id(f[1]) :
+                     throw(ArgumentError("unsupported nonconservative flux $(f[2])")))
+flux_id(f) = throw(ArgumentError("flux $f is not built for HIP"))
+
+ic_id(::typeof(initial_condition_constant)) = 0
+ic_id(::typeof(initial_condition_convergence_test)) = 1
+ic_id(::typeof(initial_condition_weak_blast_wave)) = 2
+ic_id(f) = throw(ArgumentError("initial condition $f is not built for HIP"))
+
+# ---- public API ----------------------------------------------------------------
+struct DGSEMHIP{VI, SF}
+    polydeg::Int
+    surface_flux::SF
+    volume_integral::VI
+    precision::Symbol      # :fp64 or :mixed
+end
+
+function DGSEMHIP(; RealT = Float64, polydeg::Integer, surface_flux = flux_central,
+                  volume_integral = VolumeIntegralWeakForm(), precision = nothing)
+    prec = precision === nothing ? (RealT === Float32 ? :mixed : :fp64) : precision
+    DGSEMHIP(Int(polydeg), surface_flux, volume_integral, prec)
+end
+
+mutable struct SemidiscretizationHyperbolicHIP{Mesh, Equations, IC, Solver}
+    mesh::Mesh
+    equations::Equations
+    initial_condition::IC
+    solver::Solver
+    ctx::Ptr{Cvoid}
+    nvalues::Int
+end
+
+function make_config(solver::DGSEMHIP, equations, mesh)
+    vi = solver.volume_integral
+    volume_integral, vflux, fvflux, smooth, amax, amin = 0, 0, 0, 0, 0.5, 0.001
+    if vi isa VolumeIntegralFluxDifferencing
+        volume_integral, vflux = 1, flux_id(vi.volume_flux)
+    elseif vi isa VolumeIntegralShockCapturingHG
+        volume_integral, vflux, fvflux = 2, flux_id(vi.volume_flux_dg), flux_id(vi.volume_flux_fv)
+        smooth = vi.indicator.alpha_smooth ? 1 : 0
+        amax, amin = vi.indicator.alpha_max, vi.indicator.alpha_min
+    end
+    gamma = hasproperty(equations, :gamma) ? equations.gamma : 1.4
+    c_h = hasproperty(equations, :c_h) ? equations.c_h : 0.0
+    a = hasproperty(equations, :advection_velocity) ? Tuple(equations.advection_velocity) : ()
+    vel = ntuple(d -> d <= length(a) ? Float64(a[d]) : 0.0, 3)
+    len = mesh.tree.length_level_0
+    CConfig(ndims(mesh), equation_id(equations), solver.polydeg, volume_integral, vflux,
+            flux_id(solver.surface_flux), fvflux, solver.precision === :mixed ? 1 : 0,
+            smooth, 0, gamma, c_h, vel, amax, amin, len)
+end
+
+function SemidiscretizationHyperbolicHIP(mesh::TreeMesh, equations, initial_condition,
+                                         solver::DGSEMHIP;
+                                         source_terms = nothing,
+                                         boundary_conditions = boundary_condition_periodic)
+    source_terms === nothing || throw(ArgumentError("source terms are not built for HIP"))
+    boundary_conditions === boundary_condition_periodic ||
+        throw(ArgumentError("only periodic TreeMesh is supported"))
+    # Trixi builds the element and interface containers on the host.
+    dg_cpu = DGSEM(polydeg = solver.polydeg, surface_flux = flux_central)
+    cache = Trixi.create_cache(mesh, equations, dg_cpu, Float64, Float64)
+    e, i = cache.elements, cache.interfaces
+    nbr = Int32.(i.neighbor_ids .- 1)            # 0-based
+    ori = Int32.(i.orientations)
+    invj = Vector{Float64}(e.inverse_jacobian)
+    xyz = Array{Float64}(e.node_coordinates)     # [d, i, j, k, e]
+    cfg = make_config(solver, equations, mesh)
+    GC.@preserve nbr ori invj xyz begin
+        cmesh = CMesh(nelements(e), ninterfaces(i), pointer(nbr), pointer(ori), pointer(invj),
+                      pointer(xyz), mesh.tree.length_level_0^ndims(mesh))
+        ctx = Ref{Ptr{Cvoid}}(C_NULL)
+        check(ccall((:trixi_hip_create, libtrixihip), Cint,
+                    (Ref{CConfig}, Ref{CMesh}, Ref{Ptr{Cvoid}}), cfg, cmesh, ctx))
+    end
+    nv = Ref{Int32}(0); nn = Ref{Int32}(0); nval = Ref{Int64}(0)
+    check(ccall((:trixi_hip_sizes, libtrixihip), Cint, (Ptr{Cvoid}, Ref{Int32}, Ref{Int32}, Ref{Int64}),
+                ctx[], nv, nn, nval))
+    semi = SemidiscretizationHyperbolicHIP(mesh, equations, initial_condition, solver, ctx[], Int(nval[]))
+    finalizer(s -> ccall((:trixi_hip_destroy, libtrixihip), Cint, (Ptr{Cvoid},), s.ctx), semi)
+    semi
+end
+
+stream_ptr() = AMDGPU.stream().stream   # hipStream_t of the task-local stream
+
+"rhs_hip!(du_ode, u_ode, semi, t): the ODE right-hand side on the GPU (rhs_gpu!)."
+function rhs_hip!(du_ode::ROCVector{Float64}, u_ode::ROCVector{Float64},
+                  semi::SemidiscretizationHyperbolicHIP, t)
+    check(ccall((:trixi_hip_rhs, libtrixihip), Cint,
+                (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Float64, Ptr{Cvoid}),
+                semi.ctx, pointer(du_ode), pointer(u_ode), t, stream_ptr()))
+    nothing
+end
+
+function compute_coefficients_hip(t, semi::SemidiscretizationHyperbolicHIP)
+    u = ROCVector{Float64}(undef, semi.nvalues)
+    check(ccall((:trixi_hip_compute_coefficients, libtrixihip), Cint,
+                (Ptr{Cvoid}, Ptr{Float64}, Int32, Float64, UInt64, Ptr{Cvoid}),
+                semi.ctx, pointer(u), ic_id(semi.initial_condition), t, 0, stream_ptr()))
+    u
+end
+
+"semidiscretizeHIP(semi, tspan): ODEProblem whose f is rhs_hip! (semidiscretizeGPU)."
+function semidiscretizeHIP(semi::SemidiscretizationHyperbolicHIP, tspan)
+    u0 = compute_coefficients_hip(first(tspan), semi)
+    ODEProblem{true, FullSpecialize}(rhs_hip!, u0, tspan, semi)
+end
+
+"max_dt on the device (StepsizeCallback)."
+function max_dt_hip(u_ode::ROCVector{Float64}, semi::SemidiscretizationHyperbolicHIP)
+    out = Ref{Float64}(0.0)
+    check(ccall((:trixi_hip_max_dt, libtrixihip), Cint,
+                (Ptr{Cvoid}, Ptr{Float64}, Ref{Float64}, Ptr{Cvoid}),
+                semi.ctx, pointer(u_ode), out, stream_ptr()))
+    out[]
+end
+
+"(l2, linf) against the initial condition at time t (AnalysisCallback)."
+function calc_error_norms_hip(u_ode::ROCVector{Float64}, t, semi::SemidiscretizationHyperbolicHIP)
+    nv = nvariables(semi.equations)
+    l2 = zeros(nv); linf = zeros(nv)
+    check(ccall((:trixi_hip_error_norms, libtrixihip), Cint,
+                (Ptr{Cvoid}, Ptr{Float64}, Int32, Float64, UInt64, Ptr{Float64}, Ptr{Float64}, Ptr{Cvoid}),
+                semi.ctx, pointer(u_ode), ic_id(semi.initial_condition), t, 0, l2, linf, stream_ptr()))
+    l2, linf
+end
+
+export DGSEMHIP, SemidiscretizationHyperbolicHIP, semidiscretizeHIP, rhs_hip!,
+       max_dt_hip, calc_error_norms_hip
+
+end # module
