@@ -21,6 +21,8 @@ import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 PKG_DIR = os.path.join(ROOT, "trixicuda.jl_amd")
+# rocprofv3 FETCH_SIZE/WRITE_SIZE of the rhs kernel on this workload, copy-calibrated
+TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "rhs_kernel_euler3d_fd_ranocha_llf_level5_traffic.txt")
 
 
 def load(name, path, pkg_dir=None):
@@ -128,6 +130,8 @@ def main():
 
     th = load("trixi_hip", os.path.join(PKG_DIR, "__init__.py"), [PKG_DIR])
     fm = load("trixi_hip_flop_model", os.path.join(PKG_DIR, "flop_model.py"))
+    prof = load("trixi_hip_profiling", os.path.join(PKG_DIR, "profiling.py"))
+    traffic = prof.read_traffic(TRAFFIC_SUMMARY) if (args.level == 5 and os.path.exists(TRAFFIC_SUMMARY)) else {}
     if not torch.cuda.is_available():
         raise SystemExit("bench.py needs a GPU")
     key = "euler3d_fd_ranocha_llf"
@@ -146,15 +150,20 @@ def main():
         ms, wall_ms = time_rhs(torch, th, semi, u, du, args.steps, args.warmup, stream)
         dofs = semi.ndofs
         secs = ms * 1e-3
-        peak = fm.PEAK["fp64_TFLOPs"] if mode == "fp64" else fm.PEAK["fp32_TFLOPs"]
+        fpeak = fm.PEAK["fp64_TFLOPs"] if mode == "fp64" else fm.PEAK["fp32_TFLOPs"]
         achieved_tf = flops_dof * dofs / secs / 1e12
+        gbps = bytes_dof * dofs / secs / 1e9
+        # Roofline: 405 flop / 80 B = 5 flop/B lies below the FP64 (9.8) and FP32
+        # (19.7) ridge points, so the HBM roof is the one that bounds the kernel.
         res[mode] = {
             "value": dofs / secs, "ms_per_step": ms, "wall_ms_per_step": wall_ms,
-            "roofline": {"bound": "valu", "achieved": achieved_tf, "peak": peak, "unit": "TFLOP/s",
-                         "frac": achieved_tf / peak, "traffic": None,
-                         "hbm_GBps": bytes_dof * dofs / secs / 1e9,
-                         "hbm_frac": bytes_dof * dofs / secs / 1e9 / fm.PEAK["hbm_GBps"],
-                         "flop_per_dof": flops_dof, "bytes_per_dof": bytes_dof},
+            "roofline": {"bound": "hbm", "achieved": gbps, "peak": fm.PEAK["hbm_GBps"], "unit": "GB/s",
+                         "frac": gbps / fm.PEAK["hbm_GBps"], "traffic": traffic.get(mode),
+                         "kernel": "rhs_kernel<3, 4, Euler<3>, FD, flux_ranocha, LLF, "
+                                   + ("double, M=4>" if mode == "fp64" else "float, M=8>"),
+                         "algorithmic_bytes": bytes_dof * dofs, "bytes_per_dof": bytes_dof,
+                         "flop_per_dof": flops_dof, "flop_TFLOPs": achieved_tf, "flop_peak_TFLOPs": fpeak,
+                         "flop_frac": achieved_tf / fpeak},
         }
         if args.rk_steps > 0:
             res[mode]["rk_step_ms"] = time_rk(torch, th, semi, u, args.rk_steps, stream)

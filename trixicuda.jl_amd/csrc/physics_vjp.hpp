@@ -73,6 +73,7 @@ template <int DIM>
 struct LinAdvVjp {
     using F = LinAdv<DIM>;
     static constexpr int NV = 1;
+    static constexpr bool NONCONS = false;
     static constexpr int NRF = 1;
     template <typename R>
     struct Node {
@@ -132,6 +133,7 @@ struct LinAdvVjp {
 template <int DIM>
 struct EulerVjp {
     static constexpr int NV = DIM + 2;
+    static constexpr bool NONCONS = false;
     static constexpr int NRF = DIM + 5;  // rho, v[DIM], p, c, 1/rho, 1/p
     template <typename R>
     struct Node {
@@ -394,6 +396,374 @@ struct EulerVjp {
                     sound_vjp(l, lb, bl);
             }
         }
+    }
+};
+
+// ===========================================================================
+// Ideal GLM-MHD in 3D with the nonconservative Powell / GLM terms.
+// Primitive variables (rho, v[3], p, B[3], psi).
+// ===========================================================================
+struct GlmMhdVjp {
+    static constexpr int NV = 9;
+    static constexpr bool NONCONS = true;
+    static constexpr int NRF = 11;  // rho, v[3], p, B[3], psi, 1/rho, 1/p
+    template <typename R>
+    struct Node {
+        R rho, v[3], p, B[3], psi, irho, ip;
+        double lrho, lp;
+    };
+    template <typename R>
+    struct Bar {
+        R rho, v[3], p, B[3], psi;
+    };
+
+    template <typename R>
+    __device__ static Node<R> node(const R* u, const PhysR<R>& P) {
+        Node<R> n;
+        n.rho = u[0];
+        n.irho = rrcp(n.rho);
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            n.v[d] = u[1 + d] * n.irho;
+            n.B[d] = u[5 + d];
+        }
+        n.psi = u[8];
+        n.p = P.gm1 * (u[4] - R(0.5) * (u[1] * n.v[0] + u[2] * n.v[1] + u[3] * n.v[2]) -
+                       R(0.5) * (u[5] * u[5] + u[6] * u[6] + u[7] * u[7]) - R(0.5) * u[8] * u[8]);
+        n.ip = rrcp(n.p);
+        n.lrho = log((double)n.rho);
+        n.lp = log((double)n.p);
+        return n;
+    }
+    template <typename R>
+    __device__ static void to_rec(const Node<R>& n, R* f) {
+        f[0] = n.rho;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            f[1 + d] = n.v[d];
+            f[5 + d] = n.B[d];
+        }
+        f[4] = n.p;
+        f[8] = n.psi;
+        f[9] = n.irho;
+        f[10] = n.ip;
+    }
+    template <typename R>
+    __device__ static Node<R> from_rec(const R* f, double lrho, double lp) {
+        Node<R> n;
+        n.rho = f[0];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            n.v[d] = f[1 + d];
+            n.B[d] = f[5 + d];
+        }
+        n.p = f[4];
+        n.psi = f[8];
+        n.irho = f[9];
+        n.ip = f[10];
+        n.lrho = lrho;
+        n.lp = lp;
+        return n;
+    }
+    template <typename R>
+    __device__ static void zero(Bar<R>& b) {
+        b.rho = b.p = b.psi = R(0);
+#pragma unroll
+        for (int d = 0; d < 3; ++d) b.v[d] = b.B[d] = R(0);
+    }
+    template <typename R>
+    __device__ static void bar_to_array(const Bar<R>& b, R* a) {
+        a[0] = b.rho;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            a[1 + d] = b.v[d];
+            a[5 + d] = b.B[d];
+        }
+        a[4] = b.p;
+        a[8] = b.psi;
+    }
+    template <typename R>
+    __device__ static void array_to_bar(const R* a, Bar<R>& b) {
+        b.rho = a[0];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            b.v[d] = a[1 + d];
+            b.B[d] = a[5 + d];
+        }
+        b.p = a[4];
+        b.psi = a[8];
+    }
+    template <typename R>
+    __device__ static R dot3(const R* a, const R* b) {
+        return a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
+    }
+    template <typename R>
+    __device__ static R cons(const Node<R>& n, int q, const PhysR<R>& P) {
+        if (q == 0) return n.rho;
+        if (q <= 3) return n.rho * n.v[q - 1];
+        if (q == 4)
+            return n.p * P.inv_gm1 + R(0.5) * n.rho * dot3(n.v, n.v) + R(0.5) * dot3(n.B, n.B) +
+                   R(0.5) * n.psi * n.psi;
+        if (q <= 7) return n.B[q - 5];
+        return n.psi;
+    }
+    // (d cons / d prim)^T c
+    template <typename R>
+    __device__ static void add_cons_bar(const Node<R>& n, const R* c, Bar<R>& b, const PhysR<R>& P) {
+        const R cE = c[4];
+        R t = c[0] + cE * R(0.5) * dot3(n.v, n.v);
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            t += c[1 + d] * n.v[d];
+            b.v[d] += (c[1 + d] + cE * n.v[d]) * n.rho;
+            b.B[d] += c[5 + d] + cE * n.B[d];
+        }
+        b.rho += t;
+        b.p += cE * P.inv_gm1;
+        b.psi += c[8] + cE * n.psi;
+    }
+    // (d prim / d cons)^T b
+    template <typename R>
+    __device__ static void to_cons_bar(const Node<R>& n, const Bar<R>& b, R* ub, const PhysR<R>& P) {
+        const R pg = b.p * P.gm1;
+        R t = b.rho + pg * R(0.5) * dot3(n.v, n.v);
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            t -= b.v[d] * n.v[d] * n.irho;
+            ub[1 + d] = b.v[d] * n.irho - pg * n.v[d];
+            ub[5 + d] = b.B[d] - pg * n.B[d];
+        }
+        ub[0] = t;
+        ub[4] = pg;
+        ub[8] = b.psi - pg * n.psi;
+    }
+
+    // physical flux f_o (Trixi's flux(u, orientation, ::IdealGlmMhdEquations3D))
+    template <typename R>
+    __device__ static void flux_vjp(const Node<R>& n, int o, const R* g, Bar<R>& b, const PhysR<R>& P) {
+        const R vo = n.v[o], Bo = n.B[o];
+        const R vv = dot3(n.v, n.v), BB = dot3(n.B, n.B), vB = dot3(n.v, n.B);
+        const R gE = g[4];
+        R gv = R(0), gB = R(0), gi = R(0), gvi = R(0);
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            gv += g[1 + d] * n.v[d];
+            gB += g[1 + d] * n.B[d];
+            gi += g[5 + d] * n.B[d];
+            gvi += g[5 + d] * n.v[d];
+        }
+        // f0 = rho v_o; f_{1+d} = rho v_o v_d + delta(p + |B|^2/2) - B_o B_d
+        b.rho += (g[0] + gv) * vo + gE * R(0.5) * vv * vo;
+        R vob = (g[0] + gv) * n.rho;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            b.v[d] += g[1 + d] * n.rho * vo + gE * n.rho * n.v[d] * vo - gE * Bo * n.B[d] - g[5 + d] * Bo;
+            b.B[d] += g[1 + o] * n.B[d] - g[1 + d] * Bo + gE * (R(2) * n.B[d] * vo - Bo * n.v[d]) +
+                      g[5 + d] * vo;
+        }
+        b.p += g[1 + o] + gE * P.gamma * P.inv_gm1 * vo;
+        // f4 = (rho|v|^2/2 + gamma p/(gamma-1) + |B|^2) v_o - B_o v.B + c_h psi B_o
+        vob += gE * (R(0.5) * n.rho * vv + P.gamma * n.p * P.inv_gm1 + BB) + gi;
+        R Bob = -gB - gE * vB + gE * P.c_h * n.psi - gvi + g[8] * P.c_h;
+        b.v[o] += vob;
+        b.B[o] += Bob;
+        // f_{5+d} = v_o B_d - v_d B_o (+ c_h psi for d == o); f8 = c_h B_o
+        b.psi += gE * P.c_h * Bo + g[5 + o] * P.c_h;
+    }
+
+    // fast magnetosonic speed c_f(n, o) and its reverse
+    template <typename R>
+    __device__ static R fast_speed(const Node<R>& n, int o, const PhysR<R>& P, R& a2, R& b2, R& bo2, R& D) {
+        a2 = P.gamma * n.p * n.irho;
+        b2 = dot3(n.B, n.B) * n.irho;
+        bo2 = n.B[o] * n.B[o] * n.irho;
+        const R s = a2 + b2;
+        D = rsqrt_(s * s - R(4) * a2 * bo2);
+        return rsqrt_(R(0.5) * s + R(0.5) * D);
+    }
+    template <typename R>
+    __device__ static void fast_speed_vjp(const Node<R>& n, int o, R cb, Bar<R>& b, const PhysR<R>& P) {
+        R a2, b2, bo2, D;
+        const R cf = fast_speed(n, o, P, a2, b2, bo2, D);
+        const R q = cb * rrcp(R(4) * cf);
+        const R iD = rrcp(D);
+        const R sb = q * (R(1) + (a2 + b2) * iD);
+        const R a2b = sb - q * R(2) * bo2 * iD;
+        const R bo2b = -q * R(2) * a2 * iD;
+        b.p += a2b * P.gamma * n.irho;
+        b.rho -= (a2b * a2 + sb * b2 + bo2b * bo2) * n.irho;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) b.B[d] += sb * R(2) * n.B[d] * n.irho;
+        b.B[o] += bo2b * R(2) * n.B[o] * n.irho;
+    }
+
+    template <int KIND, typename R>
+    __device__ static void two_point_vjp(const Node<R>& l, const Node<R>& r, int o, const R* g, Bar<R>& bl,
+                                         Bar<R>& br, const PhysR<R>& P) {
+        if (KIND == FLUX_HINDENLANG_GASSNER) {
+            R rho_mean, dLx, dLy;
+            ln_mean_grad<R>(l.rho, r.rho, l.lrho, r.lrho, l.irho, r.irho, rho_mean, dLx, dLy);
+            const R X = l.rho * r.p, Y = r.rho * l.p;
+            R I, dIx, dIy;
+            inv_ln_mean_grad<R>(X, Y, l.lrho + r.lp, r.lrho + l.lp, l.irho * r.ip, r.irho * l.ip, I, dIx, dIy);
+            const R iprm = l.p * r.p * I;
+            R va[3];
+#pragma unroll
+            for (int d = 0; d < 3; ++d) va[d] = R(0.5) * (l.v[d] + r.v[d]);
+            const R vsa = R(0.5) * dot3(l.v, r.v);
+            const R f1 = rho_mean * va[o];
+            const R gE = g[4];
+            const R sb = R(0.5) * gE;  // cotangent of the sum s in f4 = ... + s/2
+            R f1b = g[0] + gE * (vsa + iprm * P.inv_gm1);
+#pragma unroll
+            for (int d = 0; d < 3; ++d) f1b += g[1 + d] * va[d];
+            // rho_mean, va
+            const R rmb = f1b * va[o];
+            R vab[3];
+#pragma unroll
+            for (int d = 0; d < 3; ++d) vab[d] = g[1 + d] * f1;
+            vab[o] += f1b * rho_mean;
+            // p_avg + magnetic_square_avg in f_{1+o}
+            const R pab = g[1 + o];
+            // velocity_square_avg and inv_rho_p_mean in f4
+            const R vsab = gE * f1;
+            const R iprmb = gE * f1 * P.inv_gm1;
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                bl.v[d] += R(0.5) * (vab[d] + vsab * r.v[d]);
+                br.v[d] += R(0.5) * (vab[d] + vsab * l.v[d]);
+                // magnetic_square_avg = (Bl . Br)/2 ; -(Bl_o Br_d + Br_o Bl_d)/2
+                bl.B[d] += R(0.5) * pab * r.B[d] - R(0.5) * g[1 + d] * r.B[o];
+                br.B[d] += R(0.5) * pab * l.B[d] - R(0.5) * g[1 + d] * l.B[o];
+                bl.B[o] -= R(0.5) * g[1 + d] * r.B[d];
+                br.B[o] -= R(0.5) * g[1 + d] * l.B[d];
+            }
+            bl.p += R(0.5) * pab;
+            br.p += R(0.5) * pab;
+            // inv_rho_p_mean = p_l p_r I(rho_l p_r, rho_r p_l)
+            const R Ib = iprmb * l.p * r.p;
+            bl.p += iprmb * r.p * I;
+            br.p += iprmb * l.p * I;
+            const R Xb = Ib * dIx, Yb = Ib * dIy;
+            bl.rho += Xb * r.p + rmb * dLx;
+            br.p += Xb * l.rho;
+            br.rho += Yb * l.p + rmb * dLy;
+            bl.p += Yb * r.rho;
+            // s = p_l v_r,o + p_r v_l,o + sum_{d != o} [...] + c_h (Bl_o psi_r + Br_o psi_l)
+            bl.p += sb * r.v[o];
+            br.v[o] += sb * l.p;
+            br.p += sb * l.v[o];
+            bl.v[o] += sb * r.p;
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                if (d == o) continue;
+                const R BB = l.B[d] * r.B[d];
+                bl.v[o] += sb * BB;
+                br.v[o] += sb * BB;
+                bl.B[d] += sb * (l.v[o] + r.v[o]) * r.B[d];
+                br.B[d] += sb * (l.v[o] + r.v[o]) * l.B[d];
+                // - v_l,d Bl_o Br_d - v_r,d Br_o Bl_d
+                bl.v[d] -= sb * l.B[o] * r.B[d];
+                bl.B[o] -= sb * l.v[d] * r.B[d];
+                br.B[d] -= sb * l.v[d] * l.B[o];
+                br.v[d] -= sb * r.B[o] * l.B[d];
+                br.B[o] -= sb * r.v[d] * l.B[d];
+                bl.B[d] -= sb * r.v[d] * r.B[o];
+                // f_{5+d} = (v_l,o Bl_d - v_l,d Bl_o + v_r,o Br_d - v_r,d Br_o)/2
+                const R h = R(0.5) * g[5 + d];
+                bl.v[o] += h * l.B[d];
+                bl.B[d] += h * l.v[o];
+                bl.v[d] -= h * l.B[o];
+                bl.B[o] -= h * l.v[d];
+                br.v[o] += h * r.B[d];
+                br.B[d] += h * r.v[o];
+                br.v[d] -= h * r.B[o];
+                br.B[o] -= h * r.v[d];
+            }
+            bl.B[o] += sb * P.c_h * r.psi + R(0.5) * P.c_h * g[8];
+            br.psi += sb * P.c_h * l.B[o];
+            br.B[o] += sb * P.c_h * l.psi + R(0.5) * P.c_h * g[8];
+            bl.psi += sb * P.c_h * r.B[o];
+            // f_{5+o} = c_h (psi_l + psi_r)/2
+            bl.psi += R(0.5) * P.c_h * g[5 + o];
+            br.psi += R(0.5) * P.c_h * g[5 + o];
+        } else {
+            R h[NV];
+#pragma unroll
+            for (int q = 0; q < NV; ++q) h[q] = R(0.5) * g[q];
+            flux_vjp(l, o, h, bl, P);
+            flux_vjp(r, o, h, br, P);
+            if (KIND == FLUX_LLF || KIND == FLUX_LLF_NAIVE) {
+                R t0, t1, t2, t3;
+                const R cl = fast_speed(l, o, P, t0, t1, t2, t3);
+                const R cr = fast_speed(r, o, P, t0, t1, t2, t3);
+                const R al = dabs(l.v[o]), ar = dabs(r.v[o]);
+                R lam;
+                bool right_v, right_c;
+                if (KIND == FLUX_LLF_NAIVE) {
+                    right_v = ar > al;
+                    right_c = cr > cl;
+                    lam = (right_v ? ar : al) + (right_c ? cr : cl);
+                } else {
+                    right_v = (ar + cr) > (al + cl);
+                    right_c = right_v;
+                    lam = right_v ? ar + cr : al + cl;
+                }
+                R cvl[NV], cvr[NV], dot = R(0);
+#pragma unroll
+                for (int q = 0; q < NV; ++q) {
+                    const R hl = R(0.5) * lam * g[q];
+                    cvl[q] = hl;
+                    cvr[q] = -hl;
+                    dot += g[q] * (cons(r, q, P) - cons(l, q, P));
+                }
+                add_cons_bar(l, cvl, bl, P);
+                add_cons_bar(r, cvr, br, P);
+                const R lb = R(-0.5) * dot;
+                if (right_v)
+                    br.v[o] += lb * (r.v[o] >= R(0) ? R(1) : R(-1));
+                else
+                    bl.v[o] += lb * (l.v[o] >= R(0) ? R(1) : R(-1));
+                if (right_c)
+                    fast_speed_vjp(r, o, lb, br, P);
+                else
+                    fast_speed_vjp(l, o, lb, bl, P);
+            }
+        }
+    }
+
+    // flux_nonconservative_powell(l, r, o) is linear in (r.B_o, r.psi):
+    //   f = A(l) r.B_o + C(l) r.psi,
+    //   A = (0, B_l, v_l.B_l, v_l, 0), C = (0, 0, psi_l v_l,o, 0, v_l,o).
+    // Reverse of sum_j c_j f(l, r_j) for cotangent g: the own side is the
+    // derivative of A, C at (beta, phi) = sum_j c_j (r_j.B_o, r_j.psi); the
+    // other side gets a = g.A(l) (for B_o) and c = g.C(l) (for psi).
+    template <typename R>
+    __device__ static void other_vals(const Node<R>& n, int o, R& bo, R& ps) {
+        bo = n.B[o];
+        ps = n.psi;
+    }
+    template <typename R>
+    __device__ static void add_other(Bar<R>& b, int o, R a, R c) {
+        b.B[o] += a;
+        b.psi += c;
+    }
+    template <typename R>
+    __device__ static void noncons_other(const Node<R>& l, int o, const R* g, R& a, R& c) {
+        a = g[4] * dot3(l.v, l.B);
+#pragma unroll
+        for (int d = 0; d < 3; ++d) a += g[1 + d] * l.B[d] + g[5 + d] * l.v[d];
+        c = (g[4] * l.psi + g[8]) * l.v[o];
+    }
+    template <typename R>
+    __device__ static void noncons_own(const Node<R>& l, int o, const R* g, R beta, R phi, Bar<R>& bl) {
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            bl.B[d] += (g[1 + d] + g[4] * l.v[d]) * beta;
+            bl.v[d] += (g[4] * l.B[d] + g[5 + d]) * beta;
+        }
+        bl.psi += g[4] * l.v[o] * phi;
+        bl.v[o] += (g[4] * l.psi + g[8]) * phi;
     }
 };
 

@@ -51,6 +51,7 @@ void register_vjp_linadv();
 void register_vjp_euler1d();
 void register_vjp_euler2d();
 void register_vjp_euler3d();
+void register_vjp_mhd3d();
 
 static void ensure_registered() {
     static std::once_flag once;
@@ -67,6 +68,7 @@ static void ensure_registered() {
         register_vjp_euler1d();
         register_vjp_euler2d();
         register_vjp_euler3d();
+        register_vjp_mhd3d();
     });
 }
 
@@ -1253,8 +1255,8 @@ static int require_vjp(trixi_hip_ctx* c) {
     c->vjp_fn = find_vjp(k);
     if (!c->vjp_fn)
         return fail(TRIXI_HIP_EUNSUPPORTED,
-                    "no adjoint kernel for this configuration (built: linear advection and compressible "
-                    "Euler, weak form and flux differencing, polydeg 3)");
+                    "no adjoint kernel for this configuration (built: linear advection, compressible "
+                    "Euler (weak form, flux differencing) and GLM-MHD flux differencing, polydeg 3)");
     return TRIXI_HIP_OK;
 }
 

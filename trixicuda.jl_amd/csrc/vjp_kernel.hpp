@@ -18,6 +18,8 @@
 // conservative space once per node.
 #pragma once
 
+#include <type_traits>
+
 #include "physics_vjp.hpp"
 #include "rhs_kernel.hpp"
 
@@ -47,6 +49,8 @@ vjp_kernel(const VjpArgs a) {
     constexpr int NRF = EQ::NRF;
     using Node = typename EQ::template Node<R>;
     using Bar = typename EQ::template Bar<R>;
+    // flux differencing with node records of more than 10 values reloads them per pair
+    constexpr bool RELOAD = VOL != VI_WEAK && NRF > 10;
 
     __shared__ R rec[NRF * NN * M];
     __shared__ double recd[2 * NN * M];
@@ -85,6 +89,9 @@ vjp_kernel(const VjpArgs a) {
         const R wsc = (R)(-a.invJ[ee] * a.alpha);
 #pragma unroll
         for (int v = 0; v < NV; ++v) ws[(v * NN + n) * M + c] = wsc * (R)a.vbar[(int64_t)(v * NN + n) * E + ee];
+        if constexpr (RELOAD)
+#pragma unroll
+            for (int v = 0; v < NV; ++v) acc[(v * NN + n) * M + c] = R(0);
     }
     __syncthreads();
 
@@ -105,6 +112,12 @@ vjp_kernel(const VjpArgs a) {
             acc[idx] = d == 0 ? t[v] : acc[idx] + t[v];
         }
     };
+    auto add_acc = [&](int node, const Bar& b) {
+        R t[NV];
+        EQ::bar_to_array(b, t);
+#pragma unroll
+        for (int v = 0; v < NV; ++v) acc[(v * NN + node) * M + el] += t[v];
+    };
     // Interface at the -/+ face of the line: neighbour face node state and
     // cotangent, then the flux VJP keeping only this element's side.
     auto face = [&](int d, int base, int s, bool plus, const Node& own, Bar& b) {
@@ -112,13 +125,15 @@ vjp_kernel(const VjpArgs a) {
         const R wn = (R)(-a.invJ[fn] * a.alpha);
         const int nnode = plus ? base : base + (N - 1) * s;
         const int onode = plus ? base + (N - 1) * s : base;
-        R un[NV], g[NV];
+        R un[NV], g[NV], go[NV], gn[NV];
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
             const int64_t r = (int64_t)(v * NN + nnode) * E + fn;
             un[v] = (R)a.u[r];
-            const R wo = ws[(v * NN + onode) * M + el];
-            g[v] = plus ? lp * wo - lm * wn * (R)a.vbar[r] : lp * wn * (R)a.vbar[r] - lm * wo;
+            // cotangents of this element's and the neighbour's face flux
+            go[v] = (plus ? lp : -lm) * ws[(v * NN + onode) * M + el];
+            gn[v] = (plus ? -lm : lp) * wn * (R)a.vbar[r];
+            g[v] = go[v] + gn[v];
         }
         const Node ndn = EQ::node(un, P);
         Bar dummy;
@@ -127,18 +142,110 @@ vjp_kernel(const VjpArgs a) {
             EQ::template two_point_vjp<SF>(own, ndn, d, g, b, dummy, P);
         else
             EQ::template two_point_vjp<SF>(ndn, own, d, g, dummy, b, P);
+        if constexpr (EQ::NONCONS) {
+            // each side adds half the nonconservative flux with itself first:
+            // own term noncons(own, nbr), neighbour term noncons(nbr, own)
+            R bo, ps, ca, cc;
+#pragma unroll
+            for (int v = 0; v < NV; ++v) {
+                go[v] *= R(0.5);
+                gn[v] *= R(0.5);
+            }
+            EQ::other_vals(ndn, d, bo, ps);
+            EQ::noncons_own(own, d, go, bo, ps, b);
+            EQ::noncons_other(ndn, d, gn, ca, cc);
+            EQ::add_other(b, d, ca, cc);
+        }
     };
 
-#pragma unroll
-    for (int d = 0; d < DIM; ++d) {
+    // one phase per direction; the direction is a compile-time constant so
+    // that every component index (v[d], B[d], g[1 + d]) stays static
+    auto phase = [&](auto dc) {
+        constexpr int d = decltype(dc)::value;
         const int s = cpow(N, d);
         const int base = line_base<DIM, N>(line, d);
         auto W = [&](int p, int v) -> R { return ws[(v * NN + base + p * s) * M + el]; };
 
-        Bar bb[N];
+        if constexpr (RELOAD) {
+            // Large node records: the pair's nodes are reloaded from LDS (the
+            // empty asm with a memory clobber keeps the compiler from merging
+            // the reloads back into N live records) and cotangents go straight
+            // to the LDS accumulator, zeroed in the node pass.
 #pragma unroll
-        for (int p = 0; p < N; ++p) EQ::zero(bb[p]);
-        {
+            for (int i = 0; i < N; ++i) {
+                const Node ni = load_node(base + i * s);
+                Bar bi;
+                EQ::zero(bi);
+#pragma unroll
+                for (int j = i + 1; j < N; ++j) {
+                    asm volatile("" ::: "memory");
+                    const Node nj = load_node(base + j * s);
+                    const R fa = (R)a.Ds[i * N + j], fb = (R)a.Ds[j * N + i];
+                    R g[NV];
+#pragma unroll
+                    for (int v = 0; v < NV; ++v) g[v] = fa * W(i, v) + fb * W(j, v);
+                    Bar bj;
+                    EQ::zero(bj);
+                    EQ::template two_point_vjp<VF>(ni, nj, d, g, bi, bj, P);
+                    add_acc(base + j * s, bj);
+                }
+                add_acc(base + i * s, bi);
+            }
+            if constexpr (EQ::NONCONS) {
+                R ca[N], cc[N];
+#pragma unroll
+                for (int i = 0; i < N; ++i) {
+                    asm volatile("" ::: "memory");
+                    R beta = R(0), phi = R(0);
+#pragma unroll
+                    for (int j = 0; j < N; ++j) {
+                        R bo, ps;
+                        EQ::other_vals(load_node(base + j * s), d, bo, ps);
+                        beta += (R)a.Ds[i * N + j] * bo;
+                        phi += (R)a.Ds[i * N + j] * ps;
+                    }
+                    const Node ni = load_node(base + i * s);
+                    R g[NV];
+#pragma unroll
+                    for (int v = 0; v < NV; ++v) g[v] = R(0.5) * W(i, v);
+                    Bar bi;
+                    EQ::zero(bi);
+                    EQ::noncons_own(ni, d, g, beta, phi, bi);
+                    EQ::noncons_other(ni, d, g, ca[i], cc[i]);
+                    add_acc(base + i * s, bi);
+                }
+#pragma unroll
+                for (int j = 0; j < N; ++j) {
+                    R sa = R(0), sc = R(0);
+#pragma unroll
+                    for (int i = 0; i < N; ++i) {
+                        sa += (R)a.Ds[i * N + j] * ca[i];
+                        sc += (R)a.Ds[i * N + j] * cc[i];
+                    }
+                    Bar t;
+                    EQ::zero(t);
+                    EQ::add_other(t, d, sa, sc);
+                    add_acc(base + j * s, t);
+                }
+            }
+            {
+                asm volatile("" ::: "memory");
+                Bar b;
+                EQ::zero(b);
+                face(d, base, s, false, load_node(base), b);
+                add_acc(base, b);
+            }
+            {
+                asm volatile("" ::: "memory");
+                Bar b;
+                EQ::zero(b);
+                face(d, base, s, true, load_node(base + (N - 1) * s), b);
+                add_acc(base + (N - 1) * s, b);
+            }
+        } else {
+            Bar bb[N];
+#pragma unroll
+            for (int p = 0; p < N; ++p) EQ::zero(bb[p]);
             Node nd[N];
 #pragma unroll
             for (int p = 0; p < N; ++p) nd[p] = load_node(base + p * s);
@@ -167,22 +274,55 @@ vjp_kernel(const VjpArgs a) {
                         for (int v = 0; v < NV; ++v) g[v] = fa * W(i, v) + fb * W(j, v);
                         EQ::template two_point_vjp<VF>(nd[i], nd[j], d, g, bb[i], bb[j], P);
                     }
-            }
-        }
+                if constexpr (EQ::NONCONS) {
+                    // acc_i += 1/2 sum_j Ds[i,j] noncons(u_i, u_j), linear in u_j's (B_d, psi)
+                    R ca[N], cc[N];
 #pragma unroll
-        for (int p = 1; p < N - 1; ++p) flush(d, base + p * s, bb[p]);
-        {
-            const Node own = load_node(base);
-            face(d, base, s, false, own, bb[0]);
+                    for (int i = 0; i < N; ++i) {
+                        R beta = R(0), phi = R(0);
+#pragma unroll
+                        for (int j = 0; j < N; ++j) {
+                            R bo, ps;
+                            EQ::other_vals(nd[j], d, bo, ps);
+                            beta += (R)a.Ds[i * N + j] * bo;
+                            phi += (R)a.Ds[i * N + j] * ps;
+                        }
+                        R g[NV];
+#pragma unroll
+                        for (int v = 0; v < NV; ++v) g[v] = R(0.5) * W(i, v);
+                        EQ::noncons_own(nd[i], d, g, beta, phi, bb[i]);
+                        EQ::noncons_other(nd[i], d, g, ca[i], cc[i]);
+                    }
+#pragma unroll
+                    for (int j = 0; j < N; ++j) {
+                        R sa = R(0), sc = R(0);
+#pragma unroll
+                        for (int i = 0; i < N; ++i) {
+                            sa += (R)a.Ds[i * N + j] * ca[i];
+                            sc += (R)a.Ds[i * N + j] * cc[i];
+                        }
+                        EQ::add_other(bb[j], d, sa, sc);
+                    }
+                }
+            }
+#pragma unroll
+            for (int p = 1; p < N - 1; ++p) flush(d, base + p * s, bb[p]);
+            {
+                const Node own = load_node(base);
+                face(d, base, s, false, own, bb[0]);
+            }
+            flush(d, base, bb[0]);
+            {
+                const Node own = load_node(base + (N - 1) * s);
+                face(d, base, s, true, own, bb[N - 1]);
+            }
+            flush(d, base + (N - 1) * s, bb[N - 1]);
         }
-        flush(d, base, bb[0]);
-        {
-            const Node own = load_node(base + (N - 1) * s);
-            face(d, base, s, true, own, bb[N - 1]);
-        }
-        flush(d, base + (N - 1) * s, bb[N - 1]);
         __syncthreads();
-    }
+    };
+    phase(std::integral_constant<int, 0>{});
+    if constexpr (DIM > 1) phase(std::integral_constant<int, 1>{});
+    if constexpr (DIM > 2) phase(std::integral_constant<int, 2>{});
 
     // (d prim / d cons)^T per node, ubar = beta ubar + result
     for (int q = tid; q < NN * M; q += M * NL) {

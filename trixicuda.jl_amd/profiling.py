@@ -1,0 +1,132 @@
+"""Text summaries of rocprofv3 databases (rocpd SQLite, ``*_results.db``).
+
+    python trixicuda.jl_amd/profiling.py kernels DB [--match S] [--command CMD]
+    python trixicuda.jl_amd/profiling.py traffic --match S --fetch DB --write DB \
+        --copy-fetch DB --copy-write DB --copy-mib MIB --label L --algorithmic-bytes B [--command CMD]
+
+``kernels`` lists per-kernel dispatch counts and durations (the numbers of
+``rocprofv3 --kernel-trace --stats``) with the launch resources.  ``traffic``
+turns FETCH_SIZE / WRITE_SIZE (KiB per dispatch, one counter per run) into
+HBM bytes per launch, each scaled by what the same counter reads for a plain
+device copy of known size (the largest ``__amd_rocclr_copyBuffer`` dispatch of
+a ``bench.py --copy-calibration-mib MIB`` run), and prints a
+``traffic_bytes <label> <bytes>`` line that bench.py reads.
+"""
+import argparse
+import collections
+import sqlite3
+import statistics
+import sys
+
+
+def kernel_rows(db):
+    c = sqlite3.connect(db)
+    rows = c.execute("select name, duration, lds_size, vgpr_count, scratch_size, workgroup_x, grid_x "
+                     "from kernels").fetchall()
+    by = collections.defaultdict(list)
+    meta = {}
+    for name, dur, lds, vgpr, scratch, wg, grid in rows:
+        by[name].append(dur)
+        meta[name] = (lds, vgpr, scratch, wg, grid)
+    out = []
+    for name, ds in by.items():
+        lds, vgpr, scratch, wg, grid = meta[name]
+        out.append({"name": name, "calls": len(ds), "avg_us": statistics.fmean(ds) / 1e3, "min_us": min(ds) / 1e3,
+                    "max_us": max(ds) / 1e3, "total_us": sum(ds) / 1e3, "lds": lds, "vgpr": vgpr,
+                    "scratch": scratch, "workgroup": wg, "grid": grid})
+    out.sort(key=lambda r: -r["total_us"])
+    return out
+
+
+def counter_per_dispatch(db, counter):
+    """{kernel name: [value summed over the counter's instances, per dispatch]}"""
+    c = sqlite3.connect(db)
+    rows = c.execute("select dispatch_id, kernel_name, sum(value) from counters_collection "
+                     "where counter_name = ? group by dispatch_id, kernel_name", (counter,)).fetchall()
+    by = collections.defaultdict(list)
+    for _, name, v in rows:
+        by[name].append(v)
+    return by
+
+
+def _pick(by, match):
+    names = [n for n in by if match in n]
+    if len(names) != 1:
+        raise SystemExit(f"'{match}' matches {len(names)} kernels: {names}")
+    return names[0], by[names[0]]
+
+
+def _copy_factor(db, counter, nbytes):
+    by = counter_per_dispatch(db, counter)
+    vals = by.get("__amd_rocclr_copyBuffer")
+    if not vals:
+        raise SystemExit(f"no copy dispatches in {db}")
+    return nbytes / (max(vals) * 1024.0), max(vals)
+
+
+def cmd_kernels(a):
+    lines = []
+    if a.command:
+        lines.append(f"command: {a.command}")
+    lines.append(f"{'calls':>6} {'avg_us':>10} {'min_us':>10} {'max_us':>10} {'total_us':>11}  vgpr  lds  scratch  wg  grid  kernel")
+    for r in kernel_rows(a.db):
+        if a.match and a.match not in r["name"]:
+            continue
+        lines.append(f"{r['calls']:6d} {r['avg_us']:10.2f} {r['min_us']:10.2f} {r['max_us']:10.2f} {r['total_us']:11.1f}"
+                     f"  {r['vgpr']:4d} {r['lds']:5d} {r['scratch']:4d} {r['workgroup']:4d} {r['grid']:8d}  {r['name']}")
+    print("\n".join(lines))
+
+
+def cmd_traffic(a):
+    nbytes = a.copy_mib * (1 << 20)
+    ff, fcopy = _copy_factor(a.copy_fetch, "FETCH_SIZE", nbytes)
+    wf, wcopy = _copy_factor(a.copy_write, "WRITE_SIZE", nbytes)
+    name, fetch = _pick(counter_per_dispatch(a.fetch, "FETCH_SIZE"), a.match)
+    name_w, write = _pick(counter_per_dispatch(a.write, "WRITE_SIZE"), a.match)
+    assert name == name_w, (name, name_w)
+    f_kib, w_kib = statistics.fmean(fetch), statistics.fmean(write)
+    fb, wb = f_kib * 1024 * ff, w_kib * 1024 * wf
+    if a.command:
+        print(f"command: {a.command}")
+    print(f"kernel: {name}")
+    print(f"dispatches: FETCH_SIZE run {len(fetch)}, WRITE_SIZE run {len(write)}")
+    print(f"copy calibration ({a.copy_mib} MiB read + written): FETCH_SIZE {fcopy:.0f} KiB -> factor {ff:.4f}; "
+          f"WRITE_SIZE {wcopy:.0f} KiB -> factor {wf:.4f}")
+    print(f"FETCH_SIZE {f_kib:.1f} KiB/dispatch -> {fb / 1e6:.2f} MB; WRITE_SIZE {w_kib:.1f} KiB/dispatch -> "
+          f"{wb / 1e6:.2f} MB")
+    if a.algorithmic_bytes:
+        print(f"algorithmic bytes {a.algorithmic_bytes / 1e6:.2f} MB; measured / algorithmic "
+              f"{(fb + wb) / a.algorithmic_bytes:.3f}")
+    print(f"traffic_bytes {a.label} {fb + wb:.0f}")
+
+
+def read_traffic(path):
+    """{label: bytes} from the ``traffic_bytes`` lines of a summary file."""
+    out = {}
+    with open(path) as f:
+        for line in f:
+            p = line.split()
+            if len(p) == 3 and p[0] == "traffic_bytes":
+                out[p[1]] = float(p[2])
+    return out
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    sub = ap.add_subparsers(dest="cmd", required=True)
+    k = sub.add_parser("kernels")
+    k.add_argument("db")
+    k.add_argument("--match", default="")
+    k.add_argument("--command", default="")
+    t = sub.add_parser("traffic")
+    for n in ("--fetch", "--write", "--copy-fetch", "--copy-write", "--match", "--label"):
+        t.add_argument(n, required=True)
+    t.add_argument("--copy-mib", type=int, required=True)
+    t.add_argument("--algorithmic-bytes", type=float, default=0.0)
+    t.add_argument("--command", default="")
+    a = ap.parse_args(argv)
+    {"kernels": cmd_kernels, "traffic": cmd_traffic}[a.cmd](a)
+
+
+if __name__ == "__main__":
+    main(sys.argv[1:])
