@@ -194,8 +194,9 @@ int trixi_hip_indicator(trixi_hip_ctx* ctx, const double* u, double* alpha,
 /* Adjoint of rhs!: ubar = beta * ubar + (d rhs / d u)^T (alpha * vbar), FP64,
  * asynchronous.  No reference counterpart (TrixiCUDA.jl has no adjoint); it
  * backs a ChainRulesCore.rrule / torch.autograd.Function of rhs_gpu!
- * (solvers.jl:18-31).  Built for linear advection and compressible Euler,
- * weak form and flux differencing; other configurations return
+ * (solvers.jl:18-31).  Built for linear advection, compressible Euler (weak
+ * form and flux differencing) and ideal GLM-MHD 3D (flux differencing with the
+ * nonconservative Powell terms); other configurations return
  * TRIXI_HIP_EUNSUPPORTED.  ubar must not alias u or vbar. */
 int trixi_hip_rhs_vjp(trixi_hip_ctx* ctx, double* ubar, const double* u, const double* vbar,
                       double t, double alpha, double beta, trixi_hip_stream stream);
