@@ -74,7 +74,9 @@ enum trixi_hip_initial_condition {
     TRIXI_HIP_IC_CONVERGENCE_TEST = 1, /* LinAdv sine wave / GLM-MHD Alfven wave        */
     TRIXI_HIP_IC_WEAK_BLAST_WAVE = 2,
     TRIXI_HIP_IC_SEEDED_RANDOM = 3,
-    TRIXI_HIP_IC_SMOOTH_PERIODIC = 4
+    TRIXI_HIP_IC_SMOOTH_PERIODIC = 4,
+    TRIXI_HIP_IC_WEAK_BLAST_WAVE_SEEDED = 5 /* Euler: weak blast + seeded U(-0.01, 0.01)
+                                               on rho, v, p per node               */
 };
 
 /* Solver configuration: DGSEMGPU(; polydeg, surface_flux, volume_integral)

@@ -879,7 +879,7 @@ void initial_condition(const Cfg& c, int ic, const double* x, double t, uint64_t
     }
     if (c.eq == OR_EQ_EULER) {
         double v[3] = {0, 0, 0}, rho = 1.0, p = 1.0;
-        if (ic == OR_IC_WEAK_BLAST_WAVE) {
+        if (ic == OR_IC_WEAK_BLAST_WAVE || ic == OR_IC_WEAK_BLAST_WAVE_SEEDED) {
             double r2 = 0.0;
             for (int d = 0; d < c.nd; ++d) r2 += x[d] * x[d];
             const double r = sqrt(r2);
@@ -900,6 +900,12 @@ void initial_condition(const Cfg& c, int ic, const double* x, double t, uint64_t
             }
             rho = inside ? 1.1691 : 1.0;
             p = inside ? 1.245 : 1.0;
+            if (ic == OR_IC_WEAK_BLAST_WAVE_SEEDED) {
+                // breaks the exact symmetries (v = 0, equal neighbour states)
+                rho += 0.01 * (2.0 * u01(seed, idx, 0) - 1.0);
+                p += 0.01 * (2.0 * u01(seed, idx, 1) - 1.0);
+                for (int d = 0; d < c.nd; ++d) v[d] += 0.01 * (2.0 * u01(seed, idx, 2 + d) - 1.0);
+            }
         } else if (ic == OR_IC_SEEDED_RANDOM) {
             rho = 1.0 + 0.1 * (2.0 * u01(seed, idx, 0) - 1.0);
             p = 1.0 + 0.05 * (2.0 * u01(seed, idx, 1) - 1.0);

@@ -40,7 +40,8 @@ enum {
     OR_IC_CONVERGENCE_TEST = 1, /* LinAdv: 1+0.5 sin(pi (x-a t)); MHD: Alfven wave */
     OR_IC_WEAK_BLAST_WAVE = 2,
     OR_IC_SEEDED_RANDOM = 3,    /* Euler: rho=1+0.1U, v=0.1N(0,1), p=1+0.05U     */
-    OR_IC_SMOOTH_PERIODIC = 4   /* Euler: seeded sums of periodic sines           */
+    OR_IC_SMOOTH_PERIODIC = 4,  /* Euler: seeded sums of periodic sines           */
+    OR_IC_WEAK_BLAST_WAVE_SEEDED = 5 /* Euler: weak blast + U(-0.01,0.01) on rho,v,p */
 };
 
 typedef struct oracle_cfg {

@@ -30,7 +30,8 @@ class Case:
 
 
 FLUX_NAMES = ("central", "llf", "llf_naive", "ranocha", "shima_etal", "hindenlang_gassner")
-IC_NAMES = ("constant", "convergence_test", "weak_blast_wave", "seeded_random", "smooth_periodic")
+IC_NAMES = ("constant", "convergence_test", "weak_blast_wave", "seeded_random", "smooth_periodic",
+            "weak_blast_wave_seeded")
 
 
 def oracle_cfg(O, c: Case):
@@ -80,7 +81,8 @@ def product_semi(th, c: Case, precision=None):
     ic = {"constant": th.initial_condition_constant, "convergence_test": th.initial_condition_convergence_test,
           "weak_blast_wave": th.initial_condition_weak_blast_wave,
           "seeded_random": th.initial_condition_seeded_random(c.seed),
-          "smooth_periodic": th.initial_condition_smooth_periodic(c.seed)}[c.ic]
+          "smooth_periodic": th.initial_condition_smooth_periodic(c.seed),
+          "weak_blast_wave_seeded": th.initial_condition_weak_blast_wave_seeded(c.seed)}[c.ic]
     return th.SemidiscretizationHyperbolicHIP(mesh, eq, ic, solver)
 
 

@@ -46,6 +46,14 @@ TINY = [
     Case("euler2d_weak_central", 2, "euler", 1, sflux="central"),
     Case("euler3d_fd_ranocha_ranocha", 3, "euler", 1, "fd", "ranocha", "ranocha"),
     Case("mhd3d_fd_hg_llf", 3, "mhd", 1, "fd", "hindenlang_gassner", cmin=-1.0, cmax=1.0, gamma=5 / 3),
+    # Hennemann-Gassner shock capturing: blending factors inside the logistic
+    # range (weak blast, level 1: 0.0065), smoothed from a neighbour and
+    # clipped (seeded random, level 2), and alpha_max = 1
+    Case("euler2d_hg_ranocha_blast", 2, "euler", 1, "hg", "ranocha", ic="weak_blast_wave_seeded"),
+    Case("euler2d_hg_ranocha_random", 2, "euler", 2, "hg", "ranocha"),
+    Case("euler2d_hg_shima_llfnaive_amax1", 2, "euler", 2, "hg", "shima_etal", "llf_naive", "llf_naive",
+         alpha_max=1.0),
+    Case("euler1d_hg_ranocha_blast", 1, "euler", 2, "hg", "ranocha", ic="weak_blast_wave_seeded"),
     Case("mhd3d_fd_hg_llfnaive", 3, "mhd", 1, "fd", "hindenlang_gassner", "llf_naive", cmin=-1.0, cmax=1.0,
          gamma=5 / 3),
     Case("mhd3d_fd_hg_hg", 3, "mhd", 1, "fd", "hindenlang_gassner", "hindenlang_gassner", cmin=-1.0, cmax=1.0,
@@ -79,7 +87,9 @@ def test_vjp_equals_transposed_oracle_jacobian(th, oracle, torch, case):
                                   Case("euler2d_fd_shima_llf", 2, "euler", 3, "fd", "shima_etal"),
                                   Case("linadv3d_weak_llf", 3, "linadv", 2),
                                   Case("mhd3d_fd_hg_llf_alfven", 3, "mhd", 2, "fd", "hindenlang_gassner",
-                                       cmin=-1.0, cmax=1.0, ic="convergence_test", gamma=5 / 3)],
+                                       cmin=-1.0, cmax=1.0, ic="convergence_test", gamma=5 / 3),
+                                  Case("euler3d_hg_ranocha_blast", 3, "euler", 2, "hg", "ranocha",
+                                       ic="weak_blast_wave_seeded")],
                          ids=lambda c: c.name)
 def test_vjp_dot_product(th, oracle, torch, case):
     O = oracle
@@ -230,6 +240,60 @@ def test_mhd_l2_gradient_matches_finite_differences(th, oracle, torch, sflux):
     semi.close()
 
 
+HG_BLAST = dict(ic="weak_blast_wave_seeded")
+
+
+def test_hg_multistep_gradient_matches_oracle_tangent(th, oracle, torch):
+    """2D Euler with Hennemann-Gassner shock capturing (weak blast, level 3,
+    alpha_max 0.5, flux_ranocha, FV subcells with LLF): <d u0, adjoint of g>
+    over 100 checkpointed CK2N54 steps == <g, d u_100> from the oracle's
+    chained dual-number step JVPs, which differentiate through the blending
+    factor, its smoothing and clipping."""
+    O = oracle
+    case = Case("euler2d_hg_blast", 2, "euler", 3, "hg", "ranocha", **HG_BLAST)
+    cfg, mesh, u, semi, ud = setup(th, O, torch, case)
+    dt = 0.5 * O.max_dt(cfg, mesh, u)
+    nsteps = 100
+    rng = np.random.default_rng(14)
+    d = rng.standard_normal(u.shape)
+    g = rng.standard_normal(u.shape)
+    ut, dt_ = u.copy(), d.copy()
+    for _ in range(nsteps):
+        ut, dt_ = O.ck2n54_step_jvp(cfg, mesh, ut, dt_, dt)
+    exact = float((g * dt_).sum())
+    traj = th.forward_with_checkpoints(semi, ud, dt, nsteps)
+    grad = th.backward(traj, th.pack(torch.from_numpy(g).cuda(), semi))
+    ad = float((th.wrap_array_native(grad, semi).cpu().numpy() * d).sum())
+    print(f"hg: adjoint {ad:.15e} oracle tangent {exact:.15e}", flush=True)
+    assert abs(ad - exact) <= 1e-9 * abs(exact)
+    semi.close()
+
+
+def test_hg_mixed_forward_checkpointed_adjoint(th, oracle, torch):
+    """The mixed-precision forward trajectory (FP32 fluxes, FP64 state and RK
+    update) with the FP64 adjoint evaluated from its checkpoints, against the
+    all-FP64 gradient of the same loss over 100 steps: the difference is the
+    forward's FP32 rounding carried through the gradient (stated, bounded
+    by 1e-3 relative in the L2 norm)."""
+    case = Case("euler2d_hg_blast", 2, "euler", 3, "hg", "ranocha", **HG_BLAST)
+    s64 = product_semi(th, case)
+    smx = product_semi(th, case, precision="mixed")
+    u0 = th.compute_coefficients(0.0, s64)
+    dt = 0.5 * th.max_dt(u0, s64)
+    nsteps = 100
+    grads = {}
+    for name, semi in (("fp64", s64), ("mixed", smx)):
+        traj = th.forward_with_checkpoints(semi, u0, dt, nsteps)
+        _, g_final = th.l2_norm_and_gradient(traj.u_final, semi)
+        grads[name] = th.backward(traj, g_final)
+    a, b = grads["mixed"], grads["fp64"]
+    rel = float(torch.linalg.norm(a - b) / torch.linalg.norm(b))
+    print(f"hg gradient, mixed forward vs FP64 forward: relative L2 difference {rel:.3e}", flush=True)
+    assert rel < 1e-3
+    s64.close()
+    smx.close()
+
+
 def test_autograd_function_backward(th, oracle, torch):
     case = Case("euler3d_fd", 3, "euler", 1, "fd", "ranocha")
     semi = product_semi(th, case)
@@ -243,7 +307,7 @@ def test_autograd_function_backward(th, oracle, torch):
 
 
 def test_adjoint_unsupported_is_an_error(th, torch):
-    c = Case("euler2d_hg", 2, "euler", 1, "hg", "ranocha", ic="weak_blast_wave")
+    c = Case("linadv2d_p2", 2, "linadv", 1, polydeg=2)
     semi = product_semi(th, c)
     u = th.compute_coefficients(0.0, semi)
     with pytest.raises(th.TrixiHipError) as ei:

@@ -117,6 +117,7 @@ def test_pack_unpack_roundtrip_bit_exact(th, oracle, torch):
 @pytest.mark.parametrize("case", [Case("linadv1d", 1, "linadv", 4, cmin=-1.0, cmax=1.0, ic="convergence_test"),
                                   Case("euler2d_random", 2, "euler", 3),
                                   Case("euler2d_blast", 2, "euler", 3, ic="weak_blast_wave"),
+                                  Case("euler2d_blast_seeded", 2, "euler", 3, ic="weak_blast_wave_seeded"),
                                   Case("euler3d_smooth", 3, "euler", 2, ic="smooth_periodic"),
                                   Case("mhd3d_alfven", 3, "mhd", 2, "fd", "hindenlang_gassner", cmin=-1.0,
                                        cmax=1.0, ic="convergence_test", gamma=5 / 3)],

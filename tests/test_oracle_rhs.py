@@ -211,6 +211,28 @@ def test_jvp_matches_central_differences(oracle):
     assert np.abs(jv - fd).max() < 1e-6 * np.abs(jv).max()
 
 
+def test_shock_capturing_jvp_matches_central_differences(oracle):
+    """Dual-number derivative through the Hennemann-Gassner blending factor
+    (modal energy, logistic, clipping, neighbour smoothing) and the DG/FV
+    blend.  The seeded weak blast has no exact ties or zero velocities, where
+    LLF's wave-speed max and |v| have kinks that central differences average."""
+    O = oracle
+    mesh = O.Mesh(2, 3, -2.0, 2.0, 3)
+    cfg = O.make_cfg(2, O.EQ_EULER, 3, O.VI_HG, O.FLUX_RANOCHA, O.FLUX_LLF, O.FLUX_LLF)
+    u = O.compute_coefficients(cfg, mesh, O.IC_WEAK_BLAST_WAVE_SEEDED)
+    alpha = O.indicator_hg(cfg, mesh, u)
+    assert ((alpha > 1e-3) & (alpha < 0.5)).sum() >= 4  # blending factors on the logistic branch
+    rng = np.random.default_rng(16)
+    w = rng.standard_normal(u.shape)
+    _, jv = O.rhs_jvp(cfg, mesh, u, w)
+    h = 1e-6
+    fd = (O.rhs(cfg, mesh, u + h * w) - O.rhs(cfg, mesh, u - h * w)) / (2 * h)
+    assert np.abs(jv - fd).max() < 1e-6 * np.abs(jv).max()
+    # and the blending factor alone
+    ja = (O.indicator_hg(cfg, mesh, u + h * w) - O.indicator_hg(cfg, mesh, u - h * w)) / (2 * h)
+    assert np.abs(ja).max() > 1e-2
+
+
 def test_mesh_morton_order_and_connectivity(oracle):
     O = oracle
     for nd in (1, 2, 3):

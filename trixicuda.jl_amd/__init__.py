@@ -21,7 +21,8 @@ from .equations import (CompressibleEulerEquations1D, CompressibleEulerEquations
                         flux_central, flux_hindenlang_gassner, flux_lax_friedrichs, flux_nonconservative_powell,
                         flux_ranocha, flux_shima_etal, initial_condition_constant,
                         initial_condition_convergence_test, initial_condition_seeded_random,
-                        initial_condition_smooth_periodic, initial_condition_weak_blast_wave, max_abs_speed,
+                        initial_condition_smooth_periodic, initial_condition_weak_blast_wave,
+                        initial_condition_weak_blast_wave_seeded, max_abs_speed,
                         max_abs_speed_naive)
 from .mesh import TreeMesh
 from .semidiscretization import (ODEProblem, SemidiscretizationHyperbolicHIP, compute_coefficients, pack,

@@ -15,7 +15,8 @@ EQ_LINADV, EQ_EULER, EQ_GLM_MHD = 0, 1, 2
 VI_WEAK, VI_FD, VI_HG = 0, 1, 2
 FLUX_CENTRAL, FLUX_LLF, FLUX_LLF_NAIVE, FLUX_RANOCHA, FLUX_SHIMA_ETAL, FLUX_HINDENLANG_GASSNER = range(6)
 PREC_FP64, PREC_MIXED = 0, 1
-IC_CONSTANT, IC_CONVERGENCE_TEST, IC_WEAK_BLAST_WAVE, IC_SEEDED_RANDOM, IC_SMOOTH_PERIODIC = range(5)
+(IC_CONSTANT, IC_CONVERGENCE_TEST, IC_WEAK_BLAST_WAVE, IC_SEEDED_RANDOM, IC_SMOOTH_PERIODIC,
+ IC_WEAK_BLAST_WAVE_SEEDED) = range(6)
 
 
 class TrixiHipError(RuntimeError):

@@ -1,4 +1,5 @@
-// Adjoint kernel instantiations: compressible Euler 1D, weak form and flux differencing.
+// Adjoint kernel instantiations: compressible Euler 1D, weak form, flux differencing and
+// Hennemann-Gassner shock capturing.
 #include "registry.hpp"
 
 namespace trixi_hip {
@@ -15,6 +16,10 @@ void register_vjp_euler1d() {
     VjpRegistrar<1, 4, EQ, VI_FD, FLUX_SHIMA_ETAL, FLUX_LLF>::add(EQ_EULER);
     VjpRegistrar<1, 4, EQ, VI_FD, FLUX_SHIMA_ETAL, FLUX_LLF_NAIVE>::add(EQ_EULER);
     VjpRegistrar<1, 4, EQ, VI_FD, FLUX_SHIMA_ETAL, FLUX_SHIMA_ETAL>::add(EQ_EULER);
+    VjpRegistrar<1, 4, EQ, VI_HG, FLUX_RANOCHA, FLUX_LLF, FLUX_LLF>::add(EQ_EULER);
+    VjpRegistrar<1, 4, EQ, VI_HG, FLUX_RANOCHA, FLUX_LLF_NAIVE, FLUX_LLF_NAIVE>::add(EQ_EULER);
+    VjpRegistrar<1, 4, EQ, VI_HG, FLUX_SHIMA_ETAL, FLUX_LLF, FLUX_LLF>::add(EQ_EULER);
+    VjpRegistrar<1, 4, EQ, VI_HG, FLUX_SHIMA_ETAL, FLUX_LLF_NAIVE, FLUX_LLF_NAIVE>::add(EQ_EULER);
 }
 
 }  // namespace trixi_hip

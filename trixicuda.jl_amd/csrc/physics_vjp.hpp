@@ -270,6 +270,31 @@ struct EulerVjp {
         b.p += Eb * P.inv_gm1;
     }
 
+    // forward two-point flux value (the forward kernel's formulas), used for
+    // the cotangent of the shock-capturing blending factor
+    template <int KIND, typename R>
+    __device__ static void two_point_value(const Node<R>& l, const Node<R>& r, int o, const PhysR<R>& P, R* f) {
+        using F = Euler<DIM>;
+        typename F::template Node<R> a, b;
+        a.rho = l.rho;
+        b.rho = r.rho;
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) {
+            a.v[d] = l.v[d];
+            b.v[d] = r.v[d];
+        }
+        a.p = l.p;
+        b.p = r.p;
+        a.c = l.c;
+        b.c = r.c;
+        a.lrho = l.lrho;
+        b.lrho = r.lrho;
+        a.lp = l.lp;
+        b.lp = r.lp;
+        a.inv_gm1 = b.inv_gm1 = P.inv_gm1;
+        F::template two_point<KIND>(a, b, o, P, f);
+    }
+
     // sound speed c = sqrt(gamma p / rho): dc/dp = c/(2p), dc/drho = -c/(2 rho)
     template <typename R>
     __device__ static void sound_vjp(const Node<R>& n, R cb, Bar<R>& b) {

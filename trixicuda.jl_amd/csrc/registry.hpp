@@ -37,11 +37,11 @@ constexpr int pick_m_vjp() {
     return m;
 }
 
-template <int DIM, int N, class EQ, int VOL, int VF, int SF>
+template <int DIM, int N, class EQ, int VOL, int VF, int SF, int FVF = FLUX_CENTRAL>
 struct VjpRegistrar {
     static void add(int eq) {
         constexpr int M = pick_m_vjp<DIM, N, EQ::NV, EQ::NRF>();
-        register_vjp({DIM, N, eq, VOL, VF, SF, FLUX_CENTRAL, 0, 0}, &launch_vjp<DIM, N, EQ, VOL, VF, SF, double, M>);
+        register_vjp({DIM, N, eq, VOL, VF, SF, FVF, 0, 0}, &launch_vjp<DIM, N, EQ, VOL, VF, SF, FVF, double, M>);
     }
 };
 

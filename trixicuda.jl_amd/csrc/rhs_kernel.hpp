@@ -112,9 +112,13 @@ rhs_kernel(const RhsArgs a) {
     // field-major with the element innermost, and the direction accumulator.
     constexpr int LDM = M + TRIXI_HIP_LDS_PAD;  // LDS row length (elements)
     constexpr bool STAGE = TRIXI_HIP_STAGE_NODES != 0;
+    // RK stage: the node pass keeps u (and prefetches k) in registers, and the
+    // register update runs over the same (node, element) pairs at the end,
+    // from the accumulator, instead of re-reading u and k line by line.
+    constexpr bool COOP = RK && STAGE;
     __shared__ R rec[STAGE ? NRF * NN * LDM : 1];
     __shared__ double recd[(STAGE && NDF > 0) ? NDF * NN * LDM : 1];
-    __shared__ R acc[DIM > 1 ? NV * NN * LDM : 1];
+    __shared__ R acc[(DIM > 1 || COOP) ? NV * NN * LDM : 1];
 
     const int tid = threadIdx.x;
     const int el = tid % M;
@@ -144,14 +148,26 @@ rhs_kernel(const RhsArgs a) {
     }
 
     // ---- node data, once per node: coalesced loads (element innermost) -----
-    for (int q = tid; STAGE && q < NN * M; q += M * NL) {
+    // Every lane visits exactly N (node, element) pairs: NN * M / (M * NL).
+    double ukeep[COOP ? N : 1][NV], kkeep[COOP ? N : 1][NV];
+#pragma unroll
+    for (int it = 0; it < (STAGE ? N : 0); ++it) {
+        const int q = tid + it * M * NL;
         const int c = q % M;
         const int n = q / M;
         int64_t ee = e0 + c;
         if (ee >= E) ee = E - 1;
         R uc[NV];
 #pragma unroll
-        for (int v = 0; v < NV; ++v) uc[v] = (R)a.u[(int64_t)(v * NN + n) * E + ee];
+        for (int v = 0; v < NV; ++v) {
+            const int64_t g = (int64_t)(v * NN + n) * E + ee;
+            const double ud = a.u[g];
+            uc[v] = (R)ud;
+            if constexpr (COOP) {
+                ukeep[it][v] = ud;
+                kkeep[it][v] = a.rk_first ? 0.0 : a.k[g];
+            }
+        }
         const Node nd = EQ::template node<R, NODE_LOGS>(uc, P);
         R f[NRF];
         EQ::to_rec(nd, f);
@@ -314,7 +330,7 @@ rhs_kernel(const RhsArgs a) {
         }
 
         // ---- combine directions ---------------------------------------------
-        if (d + 1 < DIM) {
+        if (d + 1 < DIM || COOP) {
 #pragma unroll
             for (int p = 0; p < N; ++p)
 #pragma unroll
@@ -347,6 +363,30 @@ rhs_kernel(const RhsArgs a) {
                         a.u_out[g] = a.u[g] + a.rk_b * kk;
                     }
                 }
+        }
+    }
+
+    if constexpr (COOP) {
+        // k = a_s k + dt du, u_out = u + b_s k over the node pass's pairs;
+        // the accumulator holds every direction (same summation order as the
+        // line-wise path)
+        const double t_dt = a.time[1];
+#pragma unroll
+        for (int it = 0; it < N; ++it) {
+            const int q = tid + it * M * NL;
+            const int c = q % M;
+            const int n = q / M;
+            const int64_t ee = e0 + c;
+            if (ee >= E) continue;
+            const double fac = -a.invJ[ee];
+#pragma unroll
+            for (int v = 0; v < NV; ++v) {
+                const int64_t g = (int64_t)(v * NN + n) * E + ee;
+                const double val = (double)acc[(v * NN + n) * LDM + c] * fac;
+                const double kk = a.rk_first ? t_dt * val : a.rk_a * kkeep[it][v] + t_dt * val;
+                a.k[g] = kk;
+                a.u_out[g] = ukeep[it][v] + a.rk_b * kk;
+            }
         }
     }
 }

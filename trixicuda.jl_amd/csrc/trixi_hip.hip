@@ -308,7 +308,7 @@ __device__ void initial_condition(const IcParams& q, const double* x, double t, 
     }
     if (q.eq == EQ_EULER) {
         double v[3] = {0, 0, 0}, rho = 1.0, p = 1.0;
-        if (q.ic == TRIXI_HIP_IC_WEAK_BLAST_WAVE) {
+        if (q.ic == TRIXI_HIP_IC_WEAK_BLAST_WAVE || q.ic == TRIXI_HIP_IC_WEAK_BLAST_WAVE_SEEDED) {
             double r2 = 0.0;
             for (int d = 0; d < q.dim; ++d) r2 += x[d] * x[d];
             const double r = sqrt(r2);
@@ -329,6 +329,12 @@ __device__ void initial_condition(const IcParams& q, const double* x, double t, 
             }
             rho = inside ? 1.1691 : 1.0;
             p = inside ? 1.245 : 1.0;
+            if (q.ic == TRIXI_HIP_IC_WEAK_BLAST_WAVE_SEEDED) {
+                // breaks the exact symmetries (v = 0, equal neighbour states)
+                rho += 0.01 * (2.0 * u01(q.seed, idx, 0) - 1.0);
+                p += 0.01 * (2.0 * u01(q.seed, idx, 1) - 1.0);
+                for (int d = 0; d < q.dim; ++d) v[d] += 0.01 * (2.0 * u01(q.seed, idx, 2 + d) - 1.0);
+            }
         } else if (q.ic == TRIXI_HIP_IC_SEEDED_RANDOM) {
             rho = 1.0 + 0.1 * (2.0 * u01(q.seed, idx, 0) - 1.0);
             p = 1.0 + 0.05 * (2.0 * u01(q.seed, idx, 1) - 1.0);
@@ -499,6 +505,134 @@ __global__ void smooth_alpha_kernel(const double* __restrict__ raw, double* __re
         a = an > a ? an : a;
     }
     out[e] = a;
+}
+
+// Reverse of the HG blending factor (smoothing max, clipping, logistic,
+// modal energy, density * pressure), one element per thread:
+//   ubar_e += abar_raw_e * d alpha_raw_e / d u_e,
+// where abar_raw_e gathers the cotangents of the smoothed factors that took
+// their value from element e: its own (winner -1) and, for each face f, the
+// neighbour n = nbr(e, f) whose max was half of e's factor (winner(n) = f ^ 1,
+// the face of n that points back to e).
+template <int DIM, int N>
+__global__ void indicator_vjp_kernel(const double* __restrict__ u, double* __restrict__ ubar,
+                                     const double* __restrict__ alphabar, const int32_t* __restrict__ winner,
+                                     const int32_t* __restrict__ nbr, int smooth, int64_t E, PhysParams phys,
+                                     const double* __restrict__ Vinv_g, double threshold, double parameter_s,
+                                     double alpha_min, double alpha_max) {
+    constexpr int NN = cpow(N, DIM);
+    constexpr int NV = DIM + 2;
+    const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (e >= E) return;
+    double ab;
+    if (smooth) {
+        ab = winner[e] < 0 ? alphabar[e] : 0.0;
+        for (int f = 0; f < 2 * DIM; ++f) {
+            const int32_t n = nbr[(int64_t)f * E + e];
+            if (winner[n] == (f ^ 1)) ab += 0.5 * alphabar[n];
+        }
+    } else {
+        ab = alphabar[e];
+    }
+    if (ab == 0.0) return;
+    const PhysR<double> P = PhysR<double>::from(phys);
+    double Vinv[N * N];
+#pragma unroll
+    for (int q = 0; q < N * N; ++q) Vinv[q] = Vinv_g[q];
+    double m[NN], tmp[NN];
+#pragma unroll
+    for (int n = 0; n < NN; ++n) {
+        double uc[NV];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) uc[v] = u[((int64_t)v * NN + n) * E + e];
+        m[n] = Euler<DIM>::density_pressure(uc, P);
+    }
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+        const int s = cpow(N, d);
+#pragma unroll
+        for (int n = 0; n < NN; ++n) {
+            const int idx = (n / s) % N, base = n - idx * s;
+            double r = 0.0;
+#pragma unroll
+            for (int ii = 0; ii < N; ++ii) r += Vinv[idx * N + ii] * m[base + ii * s];
+            tmp[n] = r;
+        }
+#pragma unroll
+        for (int n = 0; n < NN; ++n) m[n] = tmp[n];
+    }
+    double total = 0.0, clip1 = 0.0, clip2 = 0.0;
+#pragma unroll
+    for (int n = 0; n < NN; ++n) {
+        int maxc = 0;
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) {
+            const int c = (n / cpow(N, d)) % N;
+            maxc = c > maxc ? c : maxc;
+        }
+        const double e2 = m[n] * m[n];
+        total += e2;
+        if (maxc < N - 1) clip1 += e2;
+        if (maxc < N - 2) clip2 += e2;
+    }
+    const double frac1 = (total != 0.0) ? (total - clip1) / total : 0.0;
+    const double frac2 = (clip1 != 0.0) ? (clip1 - clip2) / clip1 : 0.0;
+    const bool second = frac2 > frac1;
+    const double energy = second ? frac2 : frac1;
+    const double k = parameter_s / threshold;
+    const double a = 1.0 / (1.0 + exp(-k * (energy - threshold)));
+    // clipped to 0, to 1 or to alpha_max: locally constant
+    if (a < alpha_min || a > 1.0 - alpha_min || !(a < alpha_max)) return;
+    const double eb = ab * k * a * (1.0 - a);
+    double tb = 0.0, c1b = 0.0, c2b = 0.0;
+    if (second) {
+        if (clip1 == 0.0) return;
+        c1b = eb * clip2 / (clip1 * clip1);
+        c2b = -eb / clip1;
+    } else {
+        if (total == 0.0) return;
+        tb = eb * clip1 / (total * total);
+        c1b = -eb / total;
+    }
+#pragma unroll
+    for (int n = 0; n < NN; ++n) {
+        int maxc = 0;
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) {
+            const int c = (n / cpow(N, d)) % N;
+            maxc = c > maxc ? c : maxc;
+        }
+        m[n] = 2.0 * m[n] * (tb + (maxc < N - 1 ? c1b : 0.0) + (maxc < N - 2 ? c2b : 0.0));
+    }
+    // transpose of the dimension-wise modal transform, last direction first
+#pragma unroll
+    for (int d = DIM - 1; d >= 0; --d) {
+        const int s = cpow(N, d);
+#pragma unroll
+        for (int n = 0; n < NN; ++n) {
+            const int idx = (n / s) % N, base = n - idx * s;
+            double r = 0.0;
+#pragma unroll
+            for (int jj = 0; jj < N; ++jj) r += Vinv[jj * N + idx] * m[base + jj * s];
+            tmp[n] = r;
+        }
+#pragma unroll
+        for (int n = 0; n < NN; ++n) m[n] = tmp[n];
+    }
+    // density * pressure = (gamma - 1) (rho E - |rho v|^2 / 2)
+#pragma unroll
+    for (int n = 0; n < NN; ++n) {
+        const double g = m[n] * P.gm1;
+        const int64_t r0 = (int64_t)n * E + e;
+        const double rho = u[r0], en = u[(int64_t)(DIM + 1) * NN * E + r0];
+        ubar[r0] += g * en;
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) {
+            const int64_t rd = (int64_t)(1 + d) * NN * E + r0;
+            ubar[rd] -= g * u[rd];
+        }
+        ubar[(int64_t)(DIM + 1) * NN * E + r0] += g * rho;
+    }
 }
 
 __global__ void copy_kernel(double* __restrict__ dst, const double* __restrict__ src, int64_t n) {
@@ -753,6 +887,8 @@ struct trixi_hip_ctx {
     double* d_stage = nullptr;    // 5 stage inputs of one RK step
     double* d_k = nullptr;        // forward register during recomputation
     double* d_kbar = nullptr;     // adjoint register
+    double* d_alphabar = nullptr; // HG: cotangent of the smoothed blending factor
+    int32_t* d_winner = nullptr;  // HG: argument of the smoothing max
     bool adjoint = false;
     // graph cache for rk steps
     hipGraph_t graph = nullptr;
@@ -797,7 +933,7 @@ static void free_ctx(trixi_hip_ctx* c) {
     if (c->graph) (void)hipGraphDestroy(c->graph);
     void* bufs[] = {c->d_nbr, c->d_invJ, c->d_x,     c->d_alpha_raw, c->d_alpha, c->d_time,
                     c->d_partial, c->d_result, c->d_Vinv, c->d_Vdm,   c->d_wa,    c->d_w,
-                    c->d_tmp, c->d_stage, c->d_k, c->d_kbar};
+                    c->d_tmp, c->d_stage, c->d_k, c->d_kbar, c->d_alphabar, c->d_winner};
     for (void* p : bufs)
         if (p) (void)hipFree(p);
     delete c;
@@ -1064,7 +1200,7 @@ static IcParams ic_params(const trixi_hip_ctx* c, int ic, uint64_t seed) {
 int trixi_hip_compute_coefficients(trixi_hip_ctx* ctx, double* u, int32_t ic, double t, uint64_t seed,
                                    trixi_hip_stream stream) {
     if (!ctx || !u) return fail(TRIXI_HIP_EINVAL, "null argument");
-    if (ic < 0 || ic > TRIXI_HIP_IC_SMOOTH_PERIODIC) return fail(TRIXI_HIP_EINVAL, "unknown initial condition");
+    if (ic < 0 || ic > TRIXI_HIP_IC_WEAK_BLAST_WAVE_SEEDED) return fail(TRIXI_HIP_EINVAL, "unknown initial condition");
     hipLaunchKernelGGL(coefficients_kernel, dim3(1024), dim3(256), 0, (hipStream_t)stream, u, ctx->d_x,
                        ic_params(ctx, ic, seed), ctx->N, ctx->nn, ctx->E, t);
     HIP_TRY(hipGetLastError());
@@ -1242,21 +1378,69 @@ static VjpArgs vjp_args(const trixi_hip_ctx* c) {
         v.Ds[q] = c->args.Ds[q];
         v.Dhat[q] = c->args.Dhat[q];
     }
+    for (int q = 0; q < MAXN; ++q) v.invw[q] = c->args.invw[q];
     v.lhat_m = c->args.lhat_m;
     v.lhat_p = c->args.lhat_p;
     v.nxcd = c->args.nxcd;
+    v.alpha_raw = c->d_alpha_raw;
+    v.alphabar = c->d_alphabar;
+    v.winner = c->d_winner;
+    v.atol = c->args.atol;
+    v.smooth = c->args.smooth;
     return v;
+}
+
+static bool is_hg(const trixi_hip_ctx* c) {
+    return c->cfg.volume_integral == TRIXI_HIP_VOLUME_SHOCK_CAPTURING_HG;
 }
 
 static int require_vjp(trixi_hip_ctx* c) {
     if (c->vjp_fn) return TRIXI_HIP_OK;
     KernelKey k = make_key(c->cfg, 0);
     k.prec = 0;
-    c->vjp_fn = find_vjp(k);
-    if (!c->vjp_fn)
+    VjpFn fn = find_vjp(k);
+    if (!fn)
         return fail(TRIXI_HIP_EUNSUPPORTED,
                     "no adjoint kernel for this configuration (built: linear advection, compressible "
-                    "Euler (weak form, flux differencing) and GLM-MHD flux differencing, polydeg 3)");
+                    "Euler (weak form, flux differencing, HG shock capturing) and GLM-MHD flux differencing, "
+                    "polydeg 3)");
+    if (is_hg(c) && !c->d_alphabar) {
+        if (hipMalloc((void**)&c->d_alphabar, (size_t)c->E * sizeof(double)) != hipSuccess ||
+            hipMalloc((void**)&c->d_winner, (size_t)c->E * sizeof(int32_t)) != hipSuccess)
+            return fail(TRIXI_HIP_ENOMEM, "hipMalloc failed for the shock-capturing adjoint buffers");
+    }
+    c->vjp_fn = fn;
+    return TRIXI_HIP_OK;
+}
+
+// ubar = beta ubar + J(v.u)^T (alpha vbar): the VJP kernel and, for HG shock
+// capturing, the blending factor of v.u before it and its reverse after it.
+static int enqueue_vjp(trixi_hip_ctx* c, const VjpArgs& v, hipStream_t s) {
+    if (is_hg(c)) {
+        int rc = launch_indicator(c, v.u, s);
+        if (rc) return rc;
+    }
+    HIP_TRY(c->vjp_fn(v, s));
+    if (is_hg(c)) {
+        const int threads = 64;
+        const unsigned blocks = (unsigned)((c->E + threads - 1) / threads);
+        const double threshold = 0.5 * pow(10.0, -1.8 * pow((double)c->N, 0.25));
+        const double parameter_s = log((1.0 - 0.0001) / 0.0001);
+        const PhysParams& P = c->args.phys;
+#define TRIXI_HIP_IND_VJP(D)                                                                                    \
+    hipLaunchKernelGGL((indicator_vjp_kernel<D, 4>), dim3(blocks), dim3(threads), 0, s, v.u, v.ubar,            \
+                       (const double*)c->d_alphabar, (const int32_t*)c->d_winner, (const int32_t*)c->d_nbr,    \
+                       c->cfg.alpha_smooth, c->E, P, (const double*)c->d_Vinv, threshold, parameter_s,       \
+                       c->cfg.alpha_min, c->cfg.alpha_max)
+        if (c->dim == 1)
+            TRIXI_HIP_IND_VJP(1);
+        else if (c->dim == 2)
+            TRIXI_HIP_IND_VJP(2);
+        else
+            TRIXI_HIP_IND_VJP(3);
+#undef TRIXI_HIP_IND_VJP
+        HIP_TRY(hipGetLastError());
+    }
     return TRIXI_HIP_OK;
 }
 
@@ -1273,8 +1457,7 @@ int trixi_hip_rhs_vjp(trixi_hip_ctx* ctx, double* ubar, const double* u, const d
     v.ubar = ubar;
     v.alpha = alpha;
     v.beta = beta;
-    HIP_TRY(ctx->vjp_fn(v, (hipStream_t)stream));
-    return TRIXI_HIP_OK;
+    return enqueue_vjp(ctx, v, (hipStream_t)stream);
 }
 
 int trixi_hip_enable_adjoint(trixi_hip_ctx* ctx) {
@@ -1308,6 +1491,10 @@ int trixi_hip_rk_step_vjp(trixi_hip_ctx* ctx, const double* u_n, double* ubar, d
     hipLaunchKernelGGL(copy_kernel, dim3(1024), dim3(256), 0, s, U[0], u_n, n);
     HIP_TRY(hipGetLastError());
     for (int st = 0; st < 4; ++st) {
+        if (is_hg(ctx)) {
+            int rc = launch_indicator(ctx, U[st], s);
+            if (rc) return rc;
+        }
         RhsArgs a = ctx->args;
         a.u = U[st];
         a.k = ctx->d_k;
@@ -1329,7 +1516,8 @@ int trixi_hip_rk_step_vjp(trixi_hip_ctx* ctx, const double* u_n, double* ubar, d
         v.ubar = ubar;
         v.alpha = dt;
         v.beta = 1.0;
-        HIP_TRY(ctx->vjp_fn(v, s));
+        int rc = enqueue_vjp(ctx, v, s);
+        if (rc) return rc;
     }
     return TRIXI_HIP_OK;
 }

@@ -38,9 +38,16 @@ struct VjpArgs {
     double lhat_m, lhat_p;
     double alpha, beta;
     int nxcd;
+    // Hennemann-Gassner shock capturing
+    const double* alpha_raw;  // [E] blending factors before smoothing (from u)
+    double* alphabar;         // [E] out: cotangent of the smoothed blending factor
+    int32_t* winner;          // [E] out: argument of the smoothing max (-1 own, else face)
+    double invw[MAXN];        // inverse LGL weights (FV subcells)
+    double atol;              // pure flux-differencing threshold
+    int smooth;
 };
 
-template <int DIM, int N, class EQ, int VOL, int VF, int SF, typename R, int M>
+template <int DIM, int N, class EQ, int VOL, int VF, int SF, int FVF, typename R, int M>
 __global__ void __launch_bounds__(M * cpow(N, DIM - 1))
 vjp_kernel(const VjpArgs a) {
     constexpr int NL = cpow(N, DIM - 1);
@@ -51,11 +58,13 @@ vjp_kernel(const VjpArgs a) {
     using Bar = typename EQ::template Bar<R>;
     // flux differencing with node records of more than 10 values reloads them per pair
     constexpr bool RELOAD = VOL != VI_WEAK && NRF > 10;
+    static_assert(!(RELOAD && VOL == VI_HG), "shock capturing uses the register-resident node path");
 
     __shared__ R rec[NRF * NN * M];
     __shared__ double recd[2 * NN * M];
     __shared__ R ws[NV * NN * M];   // w = -invJ alpha vbar of the tile
     __shared__ R acc[NV * NN * M];  // primitive cotangents, summed over directions
+    __shared__ double abar_s[VOL == VI_HG ? NL * M : 1];
 
     const int tid = threadIdx.x;
     const int el = tid % M;
@@ -70,6 +79,29 @@ vjp_kernel(const VjpArgs a) {
     int32_t nb[2 * DIM];
 #pragma unroll
     for (int f = 0; f < 2 * DIM; ++f) nb[f] = a.nbr[(int64_t)f * E + ec];
+
+    // HG: the smoothed blending factor of this lane's element, as in the
+    // forward kernel, and which argument of the max produced it
+    constexpr bool HG = VOL == VI_HG;
+    R alpha = R(0);
+    bool dg_only = true;
+    int win = -1;
+    double abar_part = 0.0;
+    if constexpr (HG) {
+        double al = a.alpha_raw[ec];
+        if (a.smooth) {
+#pragma unroll
+            for (int f = 0; f < 2 * DIM; ++f) {
+                const double an = 0.5 * a.alpha_raw[nb[f]];
+                if (an > al) {
+                    al = an;
+                    win = f;
+                }
+            }
+        }
+        dg_only = fabs(al) <= a.atol;
+        alpha = (R)al;
+    }
 
     for (int q = tid; q < NN * M; q += M * NL) {
         const int c = q % M;
@@ -264,6 +296,10 @@ vjp_kernel(const VjpArgs a) {
                     EQ::flux_vjp(nd[p], d, g, bb[p], P);
                 }
             } else {
+                // HG: acc = (1 - alpha) V_DG + alpha V_FV + surface, so the DG
+                // cotangents scale by 1 - alpha and alphabar = <w, V_FV - V_DG>
+                const bool blend = HG && !dg_only;
+                const R sc = blend ? R(1) - alpha : R(1);
 #pragma unroll
                 for (int i = 0; i < N; ++i)
 #pragma unroll
@@ -272,8 +308,35 @@ vjp_kernel(const VjpArgs a) {
                         R g[NV];
 #pragma unroll
                         for (int v = 0; v < NV; ++v) g[v] = fa * W(i, v) + fb * W(j, v);
+                        if constexpr (HG) {
+                            if (blend) {
+                                R f[NV];
+                                EQ::template two_point_value<VF>(nd[i], nd[j], d, P, f);
+#pragma unroll
+                                for (int v = 0; v < NV; ++v) abar_part -= (double)(g[v] * f[v]);
+                            }
+#pragma unroll
+                            for (int v = 0; v < NV; ++v) g[v] *= sc;
+                        }
                         EQ::template two_point_vjp<VF>(nd[i], nd[j], d, g, bb[i], bb[j], P);
                     }
+                if constexpr (HG) {
+                    if (blend) {
+                        // FV subcells: acc_p += alpha invw_p F, acc_{p+1} -= alpha invw_{p+1} F
+#pragma unroll
+                        for (int p = 0; p + 1 < N; ++p) {
+                            R h[NV], F[NV];
+                            EQ::template two_point_value<FVF>(nd[p], nd[p + 1], d, P, F);
+#pragma unroll
+                            for (int v = 0; v < NV; ++v) {
+                                h[v] = (R)a.invw[p] * W(p, v) - (R)a.invw[p + 1] * W(p + 1, v);
+                                abar_part += (double)(h[v] * F[v]);
+                                h[v] *= alpha;
+                            }
+                            EQ::template two_point_vjp<FVF>(nd[p], nd[p + 1], d, h, bb[p], bb[p + 1], P);
+                        }
+                    }
+                }
                 if constexpr (EQ::NONCONS) {
                     // acc_i += 1/2 sum_j Ds[i,j] noncons(u_i, u_j), linear in u_j's (B_d, psi)
                     R ca[N], cc[N];
@@ -324,6 +387,18 @@ vjp_kernel(const VjpArgs a) {
     if constexpr (DIM > 1) phase(std::integral_constant<int, 1>{});
     if constexpr (DIM > 2) phase(std::integral_constant<int, 2>{});
 
+    if constexpr (HG) {
+        // alphabar of each element: sum of its lines' parts in a fixed order
+        abar_s[tid] = abar_part;
+        __syncthreads();
+        if (line == 0 && e < E) {
+            double sum = 0.0;
+            for (int l = 0; l < NL; ++l) sum += abar_s[l * M + el];
+            a.alphabar[e] = dg_only ? 0.0 : sum;
+            a.winner[e] = win;
+        }
+    }
+
     // (d prim / d cons)^T per node, ubar = beta ubar + result
     for (int q = tid; q < NN * M; q += M * NL) {
         const int c = q % M;
@@ -350,12 +425,12 @@ vjp_kernel(const VjpArgs a) {
     }
 }
 
-template <int DIM, int N, class EQ, int VOL, int VF, int SF, typename R, int M>
+template <int DIM, int N, class EQ, int VOL, int VF, int SF, int FVF, typename R, int M>
 hipError_t launch_vjp(const VjpArgs& a, hipStream_t stream) {
     constexpr int NL = cpow(N, DIM - 1);
     int64_t blocks = (a.E + M - 1) / M;
     if (a.nxcd > 1) blocks = (blocks + a.nxcd - 1) / a.nxcd * a.nxcd;
-    hipLaunchKernelGGL((vjp_kernel<DIM, N, EQ, VOL, VF, SF, R, M>), dim3((unsigned)blocks), dim3(M * NL), 0,
+    hipLaunchKernelGGL((vjp_kernel<DIM, N, EQ, VOL, VF, SF, FVF, R, M>), dim3((unsigned)blocks), dim3(M * NL), 0,
                        stream, a);
     return hipGetLastError();
 }

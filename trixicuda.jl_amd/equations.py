@@ -146,6 +146,15 @@ def initial_condition_seeded_random(seed=20261202):
     return InitialCondition(f"initial_condition_seeded_random(seed={seed})", L.IC_SEEDED_RANDOM, int(seed))
 
 
+def initial_condition_weak_blast_wave_seeded(seed=20261202):
+    """initial_condition_weak_blast_wave plus U(-0.01, 0.01) on rho, each v
+    component and p per node (same generator), which removes the exact
+    symmetries (zero velocity, identical neighbour states) at which the flux
+    derivatives have kinks."""
+    return InitialCondition(f"initial_condition_weak_blast_wave_seeded(seed={seed})", L.IC_WEAK_BLAST_WAVE_SEEDED,
+                            int(seed))
+
+
 def initial_condition_smooth_periodic(seed=20261202):
     """Seeded sums of three periodic sine modes per primitive variable."""
     return InitialCondition(f"initial_condition_smooth_periodic(seed={seed})", L.IC_SMOOTH_PERIODIC, int(seed))
