@@ -75,6 +75,100 @@ def time_vjp(torch, th, semi, u, steps, warmup, stream):
     return start.elapsed_time(stop) / steps
 
 
+def _events_ms(torch, stream, fn, reps, warmup=3):
+    """Average ms per call of fn() on `stream` (HIP events on that stream)."""
+    with torch.cuda.stream(stream):
+        for _ in range(warmup):
+            fn()
+        torch.cuda.synchronize()
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        for _ in range(reps):
+            fn()
+        b.record(stream)
+        torch.cuda.synchronize()
+    return a.elapsed_time(b) / reps
+
+
+def measure_configs(torch, th, stream, reps):
+    """The other workloads of the benchmark set, each on one GPU:
+    C2 2D Euler FD (flux_ranocha + LLF), [-2,2]^2 level 5, seeded random state;
+    C4 3D GLM-MHD Alfven wave (flux_hindenlang_gassner + Powell, LLF), [-1,1]^3 level 4;
+    C5 2D Euler HG shock capturing (alpha_max 0.5), [-2,2]^2 level 6, seeded weak blast:
+    FP64 and mixed rhs! times, the VJP and its ratio to the FP64 rhs!, and for
+    C5 a 100-step mixed forward with the checkpointed FP64 adjoint."""
+    out = {}
+    specs = {
+        "C2": dict(eq=th.CompressibleEulerEquations2D(1.4), nd=2, lo=-2.0, hi=2.0, level=5,
+                   vi=th.VolumeIntegralFluxDifferencing(th.flux_ranocha), sf=th.flux_lax_friedrichs,
+                   ic=th.initial_condition_seeded_random(20261202), vjp=True),
+        "C4": dict(eq=th.IdealGlmMhdEquations3D(5.0 / 3.0, 1.0), nd=3, lo=-1.0, hi=1.0, level=4,
+                   vi=th.VolumeIntegralFluxDifferencing((th.flux_hindenlang_gassner, th.flux_nonconservative_powell)),
+                   sf=(th.flux_lax_friedrichs, th.flux_nonconservative_powell),
+                   ic=th.initial_condition_convergence_test, vjp=True),
+        "C5": dict(eq=th.CompressibleEulerEquations2D(1.4), nd=2, lo=-2.0, hi=2.0, level=6,
+                   vi=th.VolumeIntegralShockCapturingHG(
+                       th.IndicatorHennemannGassner(th.CompressibleEulerEquations2D(1.4), th.LobattoLegendreBasisHIP(3),
+                                                    alpha_max=0.5, alpha_min=0.001, alpha_smooth=True,
+                                                    variable=th.density_pressure),
+                       volume_flux_dg=th.flux_ranocha, volume_flux_fv=th.flux_lax_friedrichs),
+                   sf=th.flux_lax_friedrichs, ic=th.initial_condition_weak_blast_wave_seeded(20261202), vjp=True),
+    }
+    for name, c in specs.items():
+        mesh = th.TreeMesh((c["lo"],) * c["nd"], (c["hi"],) * c["nd"], initial_refinement_level=c["level"],
+                           n_cells_max=10**8)
+        r, du = {}, {}
+        semis = {}
+        for prec in ("fp64", "mixed"):
+            solver = th.DGSEMHIP(polydeg=3, surface_flux=c["sf"], volume_integral=c["vi"], precision=prec)
+            semi = th.SemidiscretizationHyperbolicHIP(mesh, c["eq"], c["ic"], solver)
+            semis[prec] = semi
+            u = th.compute_coefficients(0.0, semi)
+            d = torch.empty_like(u)
+            ms = _events_ms(torch, stream, lambda: th.rhs_hip_(d, u, semi, 0.0), reps)
+            torch.cuda.synchronize()
+            du[prec] = d
+            r[prec] = {"rhs_ms": ms, "dof_per_s": semi.ndofs / (ms * 1e-3)}
+        r["ndofs"] = semis["fp64"].ndofs
+        a, b = du["mixed"], du["fp64"]
+        r["mixed_vs_fp64_rel_l2"] = float(torch.linalg.norm(a - b) / torch.linalg.norm(b))
+        r["mixed_vs_fp64_rel_max"] = float((a - b).abs().max() / b.abs().max())
+        r["mixed_over_fp64"] = r["fp64"]["rhs_ms"] / r["mixed"]["rhs_ms"]
+        if c["vjp"]:
+            s64 = semis["fp64"]
+            u = th.compute_coefficients(0.0, s64)
+            vb = torch.randn_like(u)
+            ub = torch.empty_like(u)
+            vms = _events_ms(torch, stream, lambda: th.rhs_vjp(ub, u, vb, s64), max(reps // 4, 5))
+            r["vjp_ms"] = vms
+            r["adjoint_over_forward"] = vms / r["fp64"]["rhs_ms"]
+        if name == "C5":
+            u0 = th.compute_coefficients(0.0, semis["fp64"])
+            dt = 0.5 * th.max_dt(u0, semis["fp64"])
+            for prec in ("mixed", "fp64"):
+                semi = semis[prec]
+                with torch.cuda.stream(stream):
+                    torch.cuda.synchronize()
+                    t0 = time.perf_counter()
+                    traj = th.forward_with_checkpoints(semi, u0, dt, 100)
+                    torch.cuda.synchronize()
+                    t1 = time.perf_counter()
+                    _, g = th.l2_norm_and_gradient(traj.u_final, semi)
+                    grad = th.backward(traj, g)
+                    torch.cuda.synchronize()
+                    t2 = time.perf_counter()
+                r[f"trajectory_100_steps_{prec}_forward"] = {"forward_ms": (t1 - t0) * 1e3,
+                                                             "checkpointed_adjoint_ms": (t2 - t1) * 1e3}
+                r.setdefault("_grads", {})[prec] = grad
+            ga, gb = r["_grads"]["mixed"], r["_grads"]["fp64"]
+            r["gradient_mixed_vs_fp64_rel_l2"] = float(torch.linalg.norm(ga - gb) / torch.linalg.norm(gb))
+            del r["_grads"]
+        for semi in semis.values():
+            semi.close()
+        out[name] = r
+    return out
+
+
 def time_rk(torch, th, semi, u, steps, stream):
     k = torch.zeros_like(u)
     uu = u.clone()
@@ -104,6 +198,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--rk-steps", type=int, default=20)
     ap.add_argument("--vjp-steps", type=int, default=50)
+    ap.add_argument("--no-extra", action="store_true", help="skip the C2/C4/C5 measurements")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="only run a plain device copy of this many MiB (counter calibration)")
     args = ap.parse_args()
@@ -177,6 +272,10 @@ def main():
             semi.close()
     torch.cuda.synchronize()
 
+    extra = None
+    if not args.no_extra and not args.only:
+        extra = measure_configs(torch, th, stream, 100)
+
     errors = {}
     if "fp64" in du_out and "mixed" in du_out:
         a, b = du_out["mixed"], du_out["fp64"]
@@ -222,6 +321,7 @@ def main():
         "cpu_baseline": cpu,
         "modes": res,
         "errors": errors,
+        "other_configs": extra,
     }
     if "fp64" in res and "mixed" in res:
         line["mixed_over_fp64"] = res["mixed"]["value"] / res["fp64"]["value"]

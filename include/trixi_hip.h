@@ -169,6 +169,21 @@ int trixi_hip_compute_coefficients(trixi_hip_ctx* ctx, double* u, int32_t ic,
 int trixi_hip_rk_step(trixi_hip_ctx* ctx, double* u, double* k, double t, double dt,
                       int32_t use_graph, trixi_hip_stream stream);
 
+/* Stage s (0..4) of the CarpenterKennedy2N54 step that starts at time t:
+ * k <- A_s k + dt rhs(u, t + c_s dt) (k is overwritten when s == 0),
+ * u_out <- u + B_s k, in one fused launch (plus the indicator for shock
+ * capturing).  u_out must not alias u: neighbouring elements read u's face
+ * nodes while the stage writes.  For drivers that step stage by stage, as
+ * OrdinaryDiffEq's 2N loop around rhs_gpu! does (solvers.jl:18-31). */
+int trixi_hip_rk_stage(trixi_hip_ctx* ctx, const double* u, double* k, double* u_out, int32_t stage,
+                       double t, double dt, trixi_hip_stream stream);
+
+/* Block until the work queued on the stream has finished; reports an
+ * asynchronous device error of earlier calls as TRIXI_HIP_EDEVICE.  The
+ * counterpart of CUDA.synchronize() in the reference benchmarks
+ * (benchmark/euler_ec_3d.jl:58-72). */
+int trixi_hip_sync(trixi_hip_stream stream);
+
 /* max_dt = 2 / (N * max_e invJ_e * sum_d lambda_d) on the device; blocks until
  * the scalar is on the host.  Replaces max_dt(u::CuArray, ...)
  * (stepsize_dg_{1,2,3}d.jl, e.g. stepsize_dg_3d.jl:20-45). */

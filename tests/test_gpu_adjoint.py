@@ -240,6 +240,40 @@ def test_mhd_l2_gradient_matches_finite_differences(th, oracle, torch, sflux):
     semi.close()
 
 
+def test_c4_l2_gradient_full_config(th, torch):
+    """C4 as stated: GLM-MHD Alfven wave on [-1,1]^3 at level 4 (4096
+    elements), flux_hindenlang_gassner + Powell volume terms, LLF + Powell
+    surface terms, c_h = 1, gamma = 5/3; gradient of ||u(t_20)||_L2 with
+    respect to u0 over 20 CK2N54 steps of dt = max_dt (CFL 1.0), against
+    fourth-order central differences on 16 random directions (relative 1e-5:
+    LLF's wave-speed max is not differentiable everywhere)."""
+    case = Case("c4", 3, "mhd", 4, "fd", "hindenlang_gassner", "llf", **MHD_ALFVEN)
+    semi = product_semi(th, case)
+    u0 = th.compute_coefficients(0.0, semi)
+    dt = 1.0 * th.max_dt(u0, semi)
+    nsteps = 20
+    traj = th.forward_with_checkpoints(semi, u0, dt, nsteps)
+    _, g_final = th.l2_norm_and_gradient(traj.u_final, semi)
+    grad = th.backward(traj, g_final)
+
+    def loss_of(u_init):
+        tr = th.forward_with_checkpoints(semi, u_init, dt, nsteps)
+        return float(th.l2_norm_and_gradient(tr.u_final, semi)[0])
+
+    gen = torch.Generator(device="cuda").manual_seed(20261202)
+    worst = 0.0
+    for i in range(16):
+        d = torch.randn(u0.shape, dtype=torch.float64, device="cuda", generator=gen)
+        h = 1e-4
+        fd = (-loss_of(u0 + 2 * h * d) + 8 * loss_of(u0 + h * d) - 8 * loss_of(u0 - h * d)
+              + loss_of(u0 - 2 * h * d)) / (12 * h)
+        ad = float((grad * d).sum())
+        worst = max(worst, abs(ad - fd) / abs(fd))
+        assert abs(ad - fd) <= 1e-5 * abs(fd), (i, ad, fd)
+    print(f"c4: 16 directions, worst relative difference adjoint vs FD {worst:.3e}", flush=True)
+    semi.close()
+
+
 HG_BLAST = dict(ic="weak_blast_wave_seeded")
 
 

@@ -193,6 +193,32 @@ def test_rk_step_matches_oracle(th, oracle, torch, case, use_graph):
     semi.close()
 
 
+@pytest.mark.parametrize("case", [Case("euler3d_fd", 3, "euler", 2, "fd", "ranocha", ic="smooth_periodic"),
+                                  Case("euler2d_hg", 2, "euler", 3, "hg", "ranocha", ic="weak_blast_wave_seeded"),
+                                  Case("euler2d_fd_mixed", 2, "euler", 3, "fd", "ranocha", precision="mixed")],
+                         ids=lambda c: c.name)
+def test_rk_stages_equal_rk_step(th, torch, case):
+    """Five trixi_hip_rk_stage calls (ping-pong states) == one trixi_hip_rk_step, bit for bit."""
+    semi = product_semi(th, case)
+    u0 = th.compute_coefficients(0.0, semi)
+    dt = 0.5 * th.max_dt(u0, semi)
+    t = 0.25
+    u_step, k_step = u0.clone(), torch.zeros_like(u0)
+    th.rk_step(u_step, k_step, t, dt, semi, use_graph=False)
+    a, b, k = u0.clone(), torch.empty_like(u0), torch.full_like(u0, float("nan"))  # stage 0 overwrites k
+    for st in range(5):
+        th.rk_stage(a, k, b, st, t, dt, semi)
+        a, b = b, a
+    th.synchronize()
+    assert torch.equal(a, u_step)
+    assert torch.equal(k, k_step)
+    with pytest.raises(th.TrixiHipError):
+        th.rk_stage(a, k, a, 0, t, dt, semi)
+    with pytest.raises(th.TrixiHipError):
+        th.rk_stage(a, k, b, 5, t, dt, semi)
+    semi.close()
+
+
 def test_c1_trajectory_matches_oracle(th, oracle, torch):
     """C1: 1D advection, level 4, CK2N54 with CFL 1.6 to t=1: L2/Linf to 1e-10."""
     O = oracle

@@ -29,7 +29,7 @@ from .semidiscretization import (ODEProblem, SemidiscretizationHyperbolicHIP, co
                                  rhs_hip_, semidiscretizeHIP, wrap_array, wrap_array_native)
 from .solver import (DGSEMHIP, IndicatorHennemannGassner, LobattoLegendreBasisHIP, VolumeIntegralFluxDifferencing,
                      VolumeIntegralShockCapturingHG, VolumeIntegralWeakForm)
-from .time_integration import CarpenterKennedy2N54, Solution, rk_step, solve
+from .time_integration import CarpenterKennedy2N54, Solution, rk_stage, rk_step, solve, synchronize
 from .adjoint import (backward, enable_adjoint, forward_with_checkpoints, l2_norm_and_gradient, quadrature_weights,
                       rhs, rhs_vjp, rk_step_vjp)
 

@@ -72,6 +72,8 @@ SIGNATURES = {
     "trixi_hip_unpack": (ctypes.c_int, [_P, _P, _P, _P]),
     "trixi_hip_compute_coefficients": (ctypes.c_int, [_P, _P, _I32, _D, _U64, _P]),
     "trixi_hip_rk_step": (ctypes.c_int, [_P, _P, _P, _D, _D, _I32, _P]),
+    "trixi_hip_rk_stage": (ctypes.c_int, [_P, _P, _P, _P, _I32, _D, _D, _P]),
+    "trixi_hip_sync": (ctypes.c_int, [_P]),
     "trixi_hip_max_dt": (ctypes.c_int, [_P, _P, _P, _P]),
     "trixi_hip_error_norms": (ctypes.c_int, [_P, _P, _I32, _D, _U64, _P, _P, _P]),
     "trixi_hip_integrate": (ctypes.c_int, [_P, _P, _P, _P]),

@@ -1239,6 +1239,37 @@ static int enqueue_rk_stages(trixi_hip_ctx* c, double* u, double* k, hipStream_t
     return TRIXI_HIP_OK;
 }
 
+int trixi_hip_rk_stage(trixi_hip_ctx* ctx, const double* u, double* k, double* u_out, int32_t stage, double t,
+                       double dt, trixi_hip_stream stream) {
+    if (!ctx || !u || !k || !u_out) return fail(TRIXI_HIP_EINVAL, "null argument");
+    if (stage < 0 || stage > 4) return fail(TRIXI_HIP_EINVAL, "stage must be 0..4");
+    if (u_out == u || k == u || k == u_out) return fail(TRIXI_HIP_EINVAL, "u, k and u_out must be distinct");
+    hipStream_t s = (hipStream_t)stream;
+    hipLaunchKernelGGL(set_time_kernel, dim3(1), dim3(1), 0, s, ctx->d_time, t, dt, CK_C[0], CK_C[1], CK_C[2],
+                       CK_C[3], CK_C[4]);
+    HIP_TRY(hipGetLastError());
+    if (ctx->cfg.volume_integral == TRIXI_HIP_VOLUME_SHOCK_CAPTURING_HG) {
+        int rc = launch_indicator(ctx, u, s);
+        if (rc) return rc;
+    }
+    RhsArgs a = ctx->args;
+    a.u = u;
+    a.k = k;
+    a.u_out = u_out;
+    a.time = ctx->d_time + 2 * stage;
+    a.rk_a = CK_A[stage];
+    a.rk_b = CK_B[stage];
+    a.rk_first = (stage == 0);
+    HIP_TRY(ctx->rk_fn(a, s));
+    return TRIXI_HIP_OK;
+}
+
+int trixi_hip_sync(trixi_hip_stream stream) {
+    HIP_TRY(hipStreamSynchronize((hipStream_t)stream));
+    HIP_TRY(hipGetLastError());
+    return TRIXI_HIP_OK;
+}
+
 int trixi_hip_rk_step(trixi_hip_ctx* ctx, double* u, double* k, double t, double dt, int32_t use_graph,
                       trixi_hip_stream stream) {
     if (!ctx || !u || !k) return fail(TRIXI_HIP_EINVAL, "null argument");

@@ -10,6 +10,7 @@ module TrixiHIP
 
 using Trixi
 using AMDGPU
+using ChainRulesCore
 using SciMLBase: ODEProblem, FullSpecialize
 
 const libtrixihip = get(ENV, "TRIXI_HIP_LIBRARY",
@@ -76,6 +77,17 @@ ic_id(::typeof(initial_condition_constant)) = 0
 ic_id(::typeof(initial_condition_convergence_test)) = 1
 ic_id(::typeof(initial_condition_weak_blast_wave)) = 2
 ic_id(f) = throw(ArgumentError("initial condition $f is not built for HIP"))
+# seeded benchmark states (ids 3, 4, 5 of trixi_hip_initial_condition) carry their seed
+struct SeededInitialCondition
+    id::Int32
+    seed::UInt64
+end
+initial_condition_seeded_random(seed = 20261202) = SeededInitialCondition(3, seed)
+initial_condition_smooth_periodic(seed = 20261202) = SeededInitialCondition(4, seed)
+initial_condition_weak_blast_wave_seeded(seed = 20261202) = SeededInitialCondition(5, seed)
+ic_id(ic::SeededInitialCondition) = ic.id
+ic_seed(ic::SeededInitialCondition) = ic.seed
+ic_seed(ic) = UInt64(0)
 
 # ---- public API ----------------------------------------------------------------
 struct DGSEMHIP{VI, SF}
@@ -166,7 +178,8 @@ function compute_coefficients_hip(t, semi::SemidiscretizationHyperbolicHIP)
     u = ROCVector{Float64}(undef, semi.nvalues)
     check(ccall((:trixi_hip_compute_coefficients, libtrixihip), Cint,
                 (Ptr{Cvoid}, Ptr{Float64}, Int32, Float64, UInt64, Ptr{Cvoid}),
-                semi.ctx, pointer(u), ic_id(semi.initial_condition), t, 0, stream_ptr()))
+                semi.ctx, pointer(u), ic_id(semi.initial_condition), t, ic_seed(semi.initial_condition),
+                stream_ptr()))
     u
 end
 
@@ -195,7 +208,76 @@ function calc_error_norms_hip(u_ode::ROCVector{Float64}, t, semi::Semidiscretiza
     l2, linf
 end
 
-export DGSEMHIP, SemidiscretizationHyperbolicHIP, semidiscretizeHIP, rhs_hip!,
-       max_dt_hip, calc_error_norms_hip
+# ---- time stepping ---------------------------------------------------------------
+"One CarpenterKennedy2N54 step (5 fused stages; hipGraph replay when use_graph)."
+function rk_step_hip!(u::ROCVector{Float64}, k::ROCVector{Float64}, t, dt,
+                      semi::SemidiscretizationHyperbolicHIP; use_graph = true)
+    check(ccall((:trixi_hip_rk_step, libtrixihip), Cint,
+                (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Float64, Float64, Int32, Ptr{Cvoid}),
+                semi.ctx, pointer(u), pointer(k), t, dt, use_graph ? 1 : 0, stream_ptr()))
+    nothing
+end
+
+"Stage s (0..4) of the CK2N54 step from t: k = A_s k + dt rhs(u), u_out = u + B_s k."
+function rk_stage_hip!(u_out::ROCVector{Float64}, k::ROCVector{Float64}, u::ROCVector{Float64},
+                       stage::Integer, t, dt, semi::SemidiscretizationHyperbolicHIP)
+    check(ccall((:trixi_hip_rk_stage, libtrixihip), Cint,
+                (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Int32, Float64, Float64, Ptr{Cvoid}),
+                semi.ctx, pointer(u), pointer(k), pointer(u_out), stage, t, dt, stream_ptr()))
+    nothing
+end
+
+hip_synchronize() = check(ccall((:trixi_hip_sync, libtrixihip), Cint, (Ptr{Cvoid},), stream_ptr()))
+
+# ---- adjoint -----------------------------------------------------------------------
+"Out-of-place rhs for reverse-mode AD; its rrule pulls back with trixi_hip_rhs_vjp."
+function rhs_hip(u_ode::ROCVector{Float64}, semi::SemidiscretizationHyperbolicHIP, t)
+    du = similar(u_ode)
+    rhs_hip!(du, u_ode, semi, t)
+    du
+end
+
+function ChainRulesCore.rrule(::typeof(rhs_hip), u_ode::ROCVector{Float64},
+                              semi::SemidiscretizationHyperbolicHIP, t)
+    du = rhs_hip(u_ode, semi, t)
+    function rhs_hip_pullback(dbar)
+        vbar = convert(ROCVector{Float64}, unthunk(dbar))
+        ubar = similar(u_ode)
+        check(ccall((:trixi_hip_rhs_vjp, libtrixihip), Cint,
+                    (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Float64, Float64, Float64, Ptr{Cvoid}),
+                    semi.ctx, pointer(ubar), pointer(u_ode), pointer(vbar), t, 1.0, 0.0, stream_ptr()))
+        return NoTangent(), ubar, NoTangent(), NoTangent()
+    end
+    du, rhs_hip_pullback
+end
+
+"Reverse one CK2N54 step from its checkpoint u_n: ubar (of u_{n+1}) becomes the cotangent of u_n."
+function rk_step_vjp_hip!(ubar::ROCVector{Float64}, u_n::ROCVector{Float64}, t, dt,
+                          semi::SemidiscretizationHyperbolicHIP)
+    check(ccall((:trixi_hip_enable_adjoint, libtrixihip), Cint, (Ptr{Cvoid},), semi.ctx))
+    check(ccall((:trixi_hip_rk_step_vjp, libtrixihip), Cint,
+                (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Float64, Float64, Ptr{Cvoid}),
+                semi.ctx, pointer(u_n), pointer(ubar), t, dt, stream_ptr()))
+    nothing
+end
+
+"Gradient of a loss of u(t0 + n dt) with respect to u0, keeping one checkpoint per step."
+function gradient_checkpointed(u0::ROCVector{Float64}, ubar_final::ROCVector{Float64}, t0, dt, nsteps,
+                               semi::SemidiscretizationHyperbolicHIP)
+    u = copy(u0); k = similar(u0); cps = ROCVector{Float64}[]
+    for n in 1:nsteps
+        push!(cps, copy(u))
+        rk_step_hip!(u, k, t0 + (n - 1) * dt, dt, semi; use_graph = false)
+    end
+    ubar = copy(ubar_final)
+    for n in nsteps:-1:1
+        rk_step_vjp_hip!(ubar, cps[n], t0 + (n - 1) * dt, dt, semi)
+    end
+    ubar
+end
+
+export DGSEMHIP, SemidiscretizationHyperbolicHIP, semidiscretizeHIP, rhs_hip!, rhs_hip,
+       max_dt_hip, calc_error_norms_hip, rk_step_hip!, rk_stage_hip!, hip_synchronize,
+       rk_step_vjp_hip!, gradient_checkpointed
 
 end # module

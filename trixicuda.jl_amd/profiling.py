@@ -3,6 +3,7 @@
     python trixicuda.jl_amd/profiling.py kernels DB [--match S] [--command CMD]
     python trixicuda.jl_amd/profiling.py traffic --match S --fetch DB --write DB \
         --copy-fetch DB --copy-write DB --copy-mib MIB --label L --algorithmic-bytes B [--command CMD]
+    python trixicuda.jl_amd/profiling.py flops --match S --f64 DB [--f32 DB] --dofs N --ms T [--label L]
 
 ``kernels`` lists per-kernel dispatch counts and durations (the numbers of
 ``rocprofv3 --kernel-trace --stats``) with the launch resources.  ``traffic``
@@ -10,7 +11,9 @@ turns FETCH_SIZE / WRITE_SIZE (KiB per dispatch, one counter per run) into
 HBM bytes per launch, each scaled by what the same counter reads for a plain
 device copy of known size (the largest ``__amd_rocclr_copyBuffer`` dispatch of
 a ``bench.py --copy-calibration-mib MIB`` run), and prints a
-``traffic_bytes <label> <bytes>`` line that bench.py reads.
+``traffic_bytes <label> <bytes>`` line that bench.py reads.  ``flops`` turns
+SQ_INSTS_VALU_{ADD,MUL,FMA,TRANS}_F64 (and _F32) wave-instruction counts into
+executed flop per dispatch: 64 lanes x (ADD + MUL + TRANS + 2 FMA).
 """
 import argparse
 import collections
@@ -100,6 +103,27 @@ def cmd_traffic(a):
     print(f"traffic_bytes {a.label} {fb + wb:.0f}")
 
 
+def cmd_flops(a):
+    def one(db, prec):
+        r = {}
+        name = None
+        for ctr in ("ADD", "MUL", "FMA", "TRANS"):
+            name, vals = _pick(counter_per_dispatch(db, f"SQ_INSTS_VALU_{ctr}_{prec}"), a.match)
+            r[f"{ctr}_{prec}"] = statistics.fmean(vals)
+        flop = 64 * (r[f"ADD_{prec}"] + r[f"MUL_{prec}"] + r[f"TRANS_{prec}"] + 2 * r[f"FMA_{prec}"])
+        return name, r, flop
+    name, r64, f64 = one(a.f64, "F64")
+    r32, f32 = {}, 0.0
+    if a.f32:
+        _, r32, f32 = one(a.f32, "F32")
+    print(f"## {a.label}: {name}")
+    print("wave instructions per dispatch:", ", ".join(f"{k} {v:.0f}" for k, v in {**r64, **r32}.items()))
+    print(f"executed FP64 flop {f64:.4e} ({f64 / a.dofs:.0f}/DOF), FP32 flop {f32:.4e} ({f32 / a.dofs:.0f}/DOF) "
+          f"per dispatch")
+    print(f"at {a.ms * 1e3:.1f} us per launch: FP64 {f64 / a.ms / 1e9:.1f} TFLOP/s of 78.6, "
+          f"FP32 {f32 / a.ms / 1e9:.1f} TFLOP/s of 157.3")
+
+
 def read_traffic(path):
     """{label: bytes} from the ``traffic_bytes`` lines of a summary file."""
     out = {}
@@ -124,8 +148,15 @@ def main(argv=None):
     t.add_argument("--copy-mib", type=int, required=True)
     t.add_argument("--algorithmic-bytes", type=float, default=0.0)
     t.add_argument("--command", default="")
+    f = sub.add_parser("flops")
+    f.add_argument("--match", required=True)
+    f.add_argument("--f64", required=True)
+    f.add_argument("--f32", default="")
+    f.add_argument("--dofs", type=float, required=True)
+    f.add_argument("--ms", type=float, required=True)
+    f.add_argument("--label", default="")
     a = ap.parse_args(argv)
-    {"kernels": cmd_kernels, "traffic": cmd_traffic}[a.cmd](a)
+    {"kernels": cmd_kernels, "traffic": cmd_traffic, "flops": cmd_flops}[a.cmd](a)
 
 
 if __name__ == "__main__":

@@ -36,6 +36,26 @@ def rk_step(u, k, t, dt, semi, use_graph=True, stream=None):
                                       ctypes.c_void_p(s.cuda_stream)))
 
 
+def rk_stage(u, k, u_out, stage, t, dt, semi, stream=None):
+    """Stage `stage` (0..4) of the CK2N54 step starting at t: k <- A_s k +
+    dt rhs(u, t + c_s dt), u_out <- u + B_s k (u_out must not alias u)."""
+    import ctypes
+    import torch
+    for x, n in ((u, "u"), (k, "k"), (u_out, "u_out")):
+        semi.check_vector(x, n)
+    s = stream if stream is not None else torch.cuda.current_stream()
+    L.check(L.lib().trixi_hip_rk_stage(semi.ctx, _ptr(u), _ptr(k), _ptr(u_out), int(stage), float(t), float(dt),
+                                       ctypes.c_void_p(s.cuda_stream)))
+
+
+def synchronize(stream=None):
+    """Wait for the stream's work; asynchronous device errors raise here."""
+    import ctypes
+    import torch
+    s = stream if stream is not None else torch.cuda.current_stream()
+    L.check(L.lib().trixi_hip_sync(ctypes.c_void_p(s.cuda_stream)))
+
+
 def solve(ode, alg=None, dt=None, callback=(), use_graph=True, maxiters=10**7):
     """Integrate ode over its tspan; returns a Solution with the final state."""
     import torch
