@@ -25,15 +25,20 @@ LaunchFn find_kernel(const KernelKey& key);
 void register_vjp(const KernelKey& key, VjpFn fn);
 VjpFn find_vjp(const KernelKey& key);
 
+#ifndef TRIXI_HIP_VJP_LDS_KIB
+#define TRIXI_HIP_VJP_LDS_KIB 64
+#endif
+
 // Workgroup width of the adjoint kernel (FP64): node records with 1/rho and
-// 1/p, FP64 logarithms and the cotangent accumulator within 40 KiB.
+// 1/p, FP64 logarithms, the scaled cotangents and the accumulator within
+// TRIXI_HIP_VJP_LDS_KIB.
 template <int DIM, int N, int NV, int NRF>
 constexpr int pick_m_vjp() {
     constexpr int NL = cpow(N, DIM - 1), NN = cpow(N, DIM);
     constexpr int per_elem = NN * (NRF * 8 + 2 * 8 + 2 * NV * 8);
     int m = 256 / NL;
     if (m < 1) m = 1;
-    while (m > 1 && m * per_elem > 44 * 1024) m /= 2;
+    while (m > 1 && m * per_elem > TRIXI_HIP_VJP_LDS_KIB * 1024) m /= 2;
     return m;
 }
 

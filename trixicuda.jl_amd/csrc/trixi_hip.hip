@@ -438,54 +438,82 @@ __global__ void coefficients_kernel(double* __restrict__ u, const double* __rest
 
 // Hennemann-Gassner indicator, one element per thread, FP64 (Trixi
 // calc_indicator_hennemann_gassner! with variable density_pressure).
+// Hennemann-Gassner blending factor, one lane per node: 64-lane blocks of
+// 64 / N^d elements.  The dimension-wise modal transform goes through LDS;
+// the element's first lane sums the modal energies in node order (the order
+// of the element-per-thread formulation) and applies the logistic and clips.
 template <int DIM, int N>
-__global__ void indicator_kernel(const double* __restrict__ u, double* __restrict__ alpha_raw,
-                                 int64_t E, PhysParams phys, const double* __restrict__ Vinv_g,
-                                 double threshold, double parameter_s, double alpha_min,
-                                 double alpha_max) {
-    constexpr int NN = cpow(N, DIM);
-    const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (e >= E) return;
-    const PhysR<double> P = PhysR<double>::from(phys);
-    double Vinv[N * N];
+__device__ __forceinline__ int max_mode(int n) {
+    int maxc = 0;
 #pragma unroll
-    for (int q = 0; q < N * N; ++q) Vinv[q] = Vinv_g[q];
-    double ind[NN], tmp[NN];
-#pragma unroll
-    for (int n = 0; n < NN; ++n) {
-        double uc[DIM + 2];
-#pragma unroll
-        for (int v = 0; v < DIM + 2; ++v) uc[v] = u[((int64_t)v * NN + n) * E + e];
-        ind[n] = Euler<DIM>::density_pressure(uc, P);
+    for (int d = 0; d < DIM; ++d) {
+        const int c = (n / cpow(N, d)) % N;
+        maxc = c > maxc ? c : maxc;
     }
+    return maxc;
+}
+
+// modal coefficients of density * pressure at this lane's node (all lanes of
+// the block take part: it synchronises)
+template <int DIM, int N>
+__device__ __forceinline__ double modal_density_pressure(const double* __restrict__ u, int64_t ec, int64_t E,
+                                                         const PhysR<double>& P, const double* __restrict__ Vinv,
+                                                         double* buf, int el, int n) {
+    constexpr int NN = cpow(N, DIM);
+    constexpr int NV = DIM + 2;
+    double uc[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) uc[v] = u[((int64_t)v * NN + n) * E + ec];
+    double m = Euler<DIM>::density_pressure(uc, P);
 #pragma unroll
     for (int d = 0; d < DIM; ++d) {
         const int s = cpow(N, d);
+        const int idx = (n / s) % N, base = n - idx * s;
+        buf[threadIdx.x] = m;
+        __syncthreads();
+        double r = 0.0;
 #pragma unroll
-        for (int n = 0; n < NN; ++n) {
-            const int idx = (n / s) % N, base = n - idx * s;
-            double r = 0.0;
-#pragma unroll
-            for (int ii = 0; ii < N; ++ii) r += Vinv[idx * N + ii] * ind[base + ii * s];
-            tmp[n] = r;
-        }
-#pragma unroll
-        for (int n = 0; n < NN; ++n) ind[n] = tmp[n];
+        for (int ii = 0; ii < N; ++ii) r += Vinv[idx * N + ii] * buf[el * NN + base + ii * s];
+        __syncthreads();
+        m = r;
     }
-    double total = 0.0, clip1 = 0.0, clip2 = 0.0;
-#pragma unroll
-    for (int n = 0; n < NN; ++n) {
-        int maxc = 0;
-#pragma unroll
-        for (int d = 0; d < DIM; ++d) {
-            const int c = (n / cpow(N, d)) % N;
-            maxc = c > maxc ? c : maxc;
-        }
-        const double e2 = ind[n] * ind[n];
-        total += e2;
-        if (maxc < N - 1) clip1 += e2;
-        if (maxc < N - 2) clip2 += e2;
+    return m;
+}
+
+template <int DIM, int N>
+__device__ __forceinline__ void modal_energies(const double* e2, int el, double& total, double& clip1,
+                                               double& clip2) {
+    constexpr int NN = cpow(N, DIM);
+    total = clip1 = clip2 = 0.0;
+    for (int q = 0; q < NN; ++q) {
+        const int maxc = max_mode<DIM, N>(q);
+        const double v = e2[el * NN + q];
+        total += v;
+        if (maxc < N - 1) clip1 += v;
+        if (maxc < N - 2) clip2 += v;
     }
+}
+
+template <int DIM, int N>
+__global__ void __launch_bounds__(64) indicator_kernel(const double* __restrict__ u, double* __restrict__ alpha_raw,
+                                                       int64_t E, PhysParams phys, const double* __restrict__ Vinv,
+                                                       double threshold, double parameter_s, double alpha_min,
+                                                       double alpha_max) {
+    constexpr int NN = cpow(N, DIM);
+    constexpr int EPB = 64 / NN;
+    static_assert(NN <= 64 && 64 % NN == 0, "one block holds whole elements");
+    __shared__ double buf[64];
+    __shared__ double e2[64];
+    const int el = threadIdx.x / NN, n = threadIdx.x % NN;
+    const int64_t e = (int64_t)blockIdx.x * EPB + el;
+    const int64_t ec = e < E ? e : E - 1;
+    const PhysR<double> P = PhysR<double>::from(phys);
+    const double m = modal_density_pressure<DIM, N>(u, ec, E, P, Vinv, buf, el, n);
+    e2[threadIdx.x] = m * m;
+    __syncthreads();
+    if (n != 0 || e >= E) return;
+    double total, clip1, clip2;
+    modal_energies<DIM, N>(e2, el, total, clip1, clip2);
     const double frac1 = (total != 0.0) ? (total - clip1) / total : 0.0;
     const double frac2 = (clip1 != 0.0) ? (clip1 - clip2) / clip1 : 0.0;
     const double energy = frac2 > frac1 ? frac2 : frac1;
@@ -508,131 +536,104 @@ __global__ void smooth_alpha_kernel(const double* __restrict__ raw, double* __re
 }
 
 // Reverse of the HG blending factor (smoothing max, clipping, logistic,
-// modal energy, density * pressure), one element per thread:
+// modal energy, density * pressure), one lane per node like the forward:
 //   ubar_e += abar_raw_e * d alpha_raw_e / d u_e,
 // where abar_raw_e gathers the cotangents of the smoothed factors that took
 // their value from element e: its own (winner -1) and, for each face f, the
 // neighbour n = nbr(e, f) whose max was half of e's factor (winner(n) = f ^ 1,
 // the face of n that points back to e).
 template <int DIM, int N>
-__global__ void indicator_vjp_kernel(const double* __restrict__ u, double* __restrict__ ubar,
-                                     const double* __restrict__ alphabar, const int32_t* __restrict__ winner,
-                                     const int32_t* __restrict__ nbr, int smooth, int64_t E, PhysParams phys,
-                                     const double* __restrict__ Vinv_g, double threshold, double parameter_s,
-                                     double alpha_min, double alpha_max) {
+__global__ void __launch_bounds__(64) indicator_vjp_kernel(
+    const double* __restrict__ u, double* __restrict__ ubar, const double* __restrict__ alphabar,
+    const int32_t* __restrict__ winner, const int32_t* __restrict__ nbr, int smooth, int64_t E, PhysParams phys,
+    const double* __restrict__ Vinv, double threshold, double parameter_s, double alpha_min, double alpha_max) {
     constexpr int NN = cpow(N, DIM);
     constexpr int NV = DIM + 2;
-    const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (e >= E) return;
-    double ab;
-    if (smooth) {
-        ab = winner[e] < 0 ? alphabar[e] : 0.0;
-        for (int f = 0; f < 2 * DIM; ++f) {
-            const int32_t n = nbr[(int64_t)f * E + e];
-            if (winner[n] == (f ^ 1)) ab += 0.5 * alphabar[n];
-        }
-    } else {
-        ab = alphabar[e];
-    }
-    if (ab == 0.0) return;
+    constexpr int EPB = 64 / NN;
+    static_assert(NN <= 64 && 64 % NN == 0, "one block holds whole elements");
+    __shared__ double buf[64];
+    __shared__ double e2[64];
+    __shared__ double coef[EPB][3];  // cotangents of total, clip1, clip2 (0: nothing to do)
+    const int el = threadIdx.x / NN, n = threadIdx.x % NN;
+    const int64_t e = (int64_t)blockIdx.x * EPB + el;
+    const bool valid = e < E;
+    const int64_t ec = valid ? e : E - 1;
     const PhysR<double> P = PhysR<double>::from(phys);
-    double Vinv[N * N];
-#pragma unroll
-    for (int q = 0; q < N * N; ++q) Vinv[q] = Vinv_g[q];
-    double m[NN], tmp[NN];
-#pragma unroll
-    for (int n = 0; n < NN; ++n) {
-        double uc[NV];
-#pragma unroll
-        for (int v = 0; v < NV; ++v) uc[v] = u[((int64_t)v * NN + n) * E + e];
-        m[n] = Euler<DIM>::density_pressure(uc, P);
-    }
-#pragma unroll
-    for (int d = 0; d < DIM; ++d) {
-        const int s = cpow(N, d);
-#pragma unroll
-        for (int n = 0; n < NN; ++n) {
-            const int idx = (n / s) % N, base = n - idx * s;
-            double r = 0.0;
-#pragma unroll
-            for (int ii = 0; ii < N; ++ii) r += Vinv[idx * N + ii] * m[base + ii * s];
-            tmp[n] = r;
+    const double m = modal_density_pressure<DIM, N>(u, ec, E, P, Vinv, buf, el, n);
+    e2[threadIdx.x] = m * m;
+    __syncthreads();
+    if (n == 0) {
+        double tb = 0.0, c1b = 0.0, c2b = 0.0;
+        double ab = 0.0;
+        if (valid) {
+            if (smooth) {
+                ab = winner[e] < 0 ? alphabar[e] : 0.0;
+                for (int f = 0; f < 2 * DIM; ++f) {
+                    const int32_t nb = nbr[(int64_t)f * E + e];
+                    if (winner[nb] == (f ^ 1)) ab += 0.5 * alphabar[nb];
+                }
+            } else {
+                ab = alphabar[e];
+            }
         }
-#pragma unroll
-        for (int n = 0; n < NN; ++n) m[n] = tmp[n];
-    }
-    double total = 0.0, clip1 = 0.0, clip2 = 0.0;
-#pragma unroll
-    for (int n = 0; n < NN; ++n) {
-        int maxc = 0;
-#pragma unroll
-        for (int d = 0; d < DIM; ++d) {
-            const int c = (n / cpow(N, d)) % N;
-            maxc = c > maxc ? c : maxc;
+        if (ab != 0.0) {
+            double total, clip1, clip2;
+            modal_energies<DIM, N>(e2, el, total, clip1, clip2);
+            const double frac1 = (total != 0.0) ? (total - clip1) / total : 0.0;
+            const double frac2 = (clip1 != 0.0) ? (clip1 - clip2) / clip1 : 0.0;
+            const bool second = frac2 > frac1;
+            const double energy = second ? frac2 : frac1;
+            const double k = parameter_s / threshold;
+            const double a = 1.0 / (1.0 + exp(-k * (energy - threshold)));
+            // clipped to 0, to 1 or to alpha_max: locally constant
+            if (!(a < alpha_min || a > 1.0 - alpha_min || !(a < alpha_max))) {
+                const double eb = ab * k * a * (1.0 - a);
+                if (second) {
+                    if (clip1 != 0.0) {
+                        c1b = eb * clip2 / (clip1 * clip1);
+                        c2b = -eb / clip1;
+                    }
+                } else if (total != 0.0) {
+                    tb = eb * clip1 / (total * total);
+                    c1b = -eb / total;
+                }
+            }
         }
-        const double e2 = m[n] * m[n];
-        total += e2;
-        if (maxc < N - 1) clip1 += e2;
-        if (maxc < N - 2) clip2 += e2;
+        coef[el][0] = tb;
+        coef[el][1] = c1b;
+        coef[el][2] = c2b;
     }
-    const double frac1 = (total != 0.0) ? (total - clip1) / total : 0.0;
-    const double frac2 = (clip1 != 0.0) ? (clip1 - clip2) / clip1 : 0.0;
-    const bool second = frac2 > frac1;
-    const double energy = second ? frac2 : frac1;
-    const double k = parameter_s / threshold;
-    const double a = 1.0 / (1.0 + exp(-k * (energy - threshold)));
-    // clipped to 0, to 1 or to alpha_max: locally constant
-    if (a < alpha_min || a > 1.0 - alpha_min || !(a < alpha_max)) return;
-    const double eb = ab * k * a * (1.0 - a);
-    double tb = 0.0, c1b = 0.0, c2b = 0.0;
-    if (second) {
-        if (clip1 == 0.0) return;
-        c1b = eb * clip2 / (clip1 * clip1);
-        c2b = -eb / clip1;
-    } else {
-        if (total == 0.0) return;
-        tb = eb * clip1 / (total * total);
-        c1b = -eb / total;
-    }
-#pragma unroll
-    for (int n = 0; n < NN; ++n) {
-        int maxc = 0;
-#pragma unroll
-        for (int d = 0; d < DIM; ++d) {
-            const int c = (n / cpow(N, d)) % N;
-            maxc = c > maxc ? c : maxc;
-        }
-        m[n] = 2.0 * m[n] * (tb + (maxc < N - 1 ? c1b : 0.0) + (maxc < N - 2 ? c2b : 0.0));
-    }
+    __syncthreads();
+    const double tb = coef[el][0], c1b = coef[el][1], c2b = coef[el][2];
+    const bool active = valid && (tb != 0.0 || c1b != 0.0 || c2b != 0.0);
+    const int maxc = max_mode<DIM, N>(n);
+    double mb = 2.0 * m * (tb + (maxc < N - 1 ? c1b : 0.0) + (maxc < N - 2 ? c2b : 0.0));
     // transpose of the dimension-wise modal transform, last direction first
 #pragma unroll
     for (int d = DIM - 1; d >= 0; --d) {
         const int s = cpow(N, d);
+        const int idx = (n / s) % N, base = n - idx * s;
+        buf[threadIdx.x] = mb;
+        __syncthreads();
+        double r = 0.0;
 #pragma unroll
-        for (int n = 0; n < NN; ++n) {
-            const int idx = (n / s) % N, base = n - idx * s;
-            double r = 0.0;
-#pragma unroll
-            for (int jj = 0; jj < N; ++jj) r += Vinv[jj * N + idx] * m[base + jj * s];
-            tmp[n] = r;
-        }
-#pragma unroll
-        for (int n = 0; n < NN; ++n) m[n] = tmp[n];
+        for (int jj = 0; jj < N; ++jj) r += Vinv[jj * N + idx] * buf[el * NN + base + jj * s];
+        __syncthreads();
+        mb = r;
     }
+    if (!active) return;
     // density * pressure = (gamma - 1) (rho E - |rho v|^2 / 2)
+    const double g = mb * P.gm1;
+    const int64_t r0 = (int64_t)n * E + e;
+    const double rho = u[r0], en = u[(int64_t)(DIM + 1) * NN * E + r0];
+    ubar[r0] += g * en;
 #pragma unroll
-    for (int n = 0; n < NN; ++n) {
-        const double g = m[n] * P.gm1;
-        const int64_t r0 = (int64_t)n * E + e;
-        const double rho = u[r0], en = u[(int64_t)(DIM + 1) * NN * E + r0];
-        ubar[r0] += g * en;
-#pragma unroll
-        for (int d = 0; d < DIM; ++d) {
-            const int64_t rd = (int64_t)(1 + d) * NN * E + r0;
-            ubar[rd] -= g * u[rd];
-        }
-        ubar[(int64_t)(DIM + 1) * NN * E + r0] += g * rho;
+    for (int d = 0; d < DIM; ++d) {
+        const int64_t rd = (int64_t)(1 + d) * NN * E + r0;
+        ubar[rd] -= g * u[rd];
     }
+    ubar[(int64_t)(DIM + 1) * NN * E + r0] += g * rho;
+    (void)NV;
 }
 
 __global__ void copy_kernel(double* __restrict__ dst, const double* __restrict__ src, int64_t n) {
@@ -940,20 +941,20 @@ static void free_ctx(trixi_hip_ctx* c) {
 }
 
 static int launch_indicator(trixi_hip_ctx* c, const double* u, hipStream_t s) {
-    const int threads = 128;
-    const unsigned blocks = (unsigned)((c->E + threads - 1) / threads);
     const double threshold = 0.5 * pow(10.0, -1.8 * pow((double)c->N, 0.25));
     const double parameter_s = log((1.0 - 0.0001) / 0.0001);
     const PhysParams& P = c->args.phys;
     if (c->N != 4) return fail(TRIXI_HIP_EUNSUPPORTED, "shock capturing is built for polydeg 3 only");
+    const int epb = 64 / (c->dim == 1 ? 4 : c->dim == 2 ? 16 : 64);
+    const unsigned blocks = (unsigned)((c->E + epb - 1) / epb);
     if (c->dim == 1)
-        hipLaunchKernelGGL((indicator_kernel<1, 4>), dim3(blocks), dim3(threads), 0, s, u, c->d_alpha_raw, c->E, P,
+        hipLaunchKernelGGL((indicator_kernel<1, 4>), dim3(blocks), dim3(64), 0, s, u, c->d_alpha_raw, c->E, P,
                            c->d_Vinv, threshold, parameter_s, c->cfg.alpha_min, c->cfg.alpha_max);
     else if (c->dim == 2)
-        hipLaunchKernelGGL((indicator_kernel<2, 4>), dim3(blocks), dim3(threads), 0, s, u, c->d_alpha_raw, c->E, P,
+        hipLaunchKernelGGL((indicator_kernel<2, 4>), dim3(blocks), dim3(64), 0, s, u, c->d_alpha_raw, c->E, P,
                            c->d_Vinv, threshold, parameter_s, c->cfg.alpha_min, c->cfg.alpha_max);
     else
-        hipLaunchKernelGGL((indicator_kernel<3, 4>), dim3(blocks), dim3(threads), 0, s, u, c->d_alpha_raw, c->E, P,
+        hipLaunchKernelGGL((indicator_kernel<3, 4>), dim3(blocks), dim3(64), 0, s, u, c->d_alpha_raw, c->E, P,
                            c->d_Vinv, threshold, parameter_s, c->cfg.alpha_min, c->cfg.alpha_max);
     HIP_TRY(hipGetLastError());
     return TRIXI_HIP_OK;
@@ -1453,8 +1454,9 @@ static int enqueue_vjp(trixi_hip_ctx* c, const VjpArgs& v, hipStream_t s) {
     }
     HIP_TRY(c->vjp_fn(v, s));
     if (is_hg(c)) {
+        const int epb = 64 / (c->dim == 1 ? 4 : c->dim == 2 ? 16 : 64);
+        const unsigned blocks = (unsigned)((c->E + epb - 1) / epb);
         const int threads = 64;
-        const unsigned blocks = (unsigned)((c->E + threads - 1) / threads);
         const double threshold = 0.5 * pow(10.0, -1.8 * pow((double)c->N, 0.25));
         const double parameter_s = log((1.0 - 0.0001) / 0.0001);
         const PhysParams& P = c->args.phys;

@@ -56,9 +56,10 @@ vjp_kernel(const VjpArgs a) {
     constexpr int NRF = EQ::NRF;
     using Node = typename EQ::template Node<R>;
     using Bar = typename EQ::template Bar<R>;
-    // flux differencing with node records of more than 10 values reloads them per pair
-    constexpr bool RELOAD = VOL != VI_WEAK && NRF > 10;
-    static_assert(!(RELOAD && VOL == VI_HG), "shock capturing uses the register-resident node path");
+    // flux differencing with node records of more than 10 values, and shock
+    // capturing (which also recomputes the forward fluxes), reload the nodes
+    // per pair and accumulate cotangents in LDS
+    constexpr bool RELOAD = VOL != VI_WEAK && (NRF > 10 || VOL == VI_HG);
 
     __shared__ R rec[NRF * NN * M];
     __shared__ double recd[2 * NN * M];
@@ -203,6 +204,10 @@ vjp_kernel(const VjpArgs a) {
             // empty asm with a memory clobber keeps the compiler from merging
             // the reloads back into N live records) and cotangents go straight
             // to the LDS accumulator, zeroed in the node pass.
+            // HG: acc = (1 - alpha) V_DG + alpha V_FV + surface, so the DG
+            // cotangents scale by 1 - alpha and alphabar = <w, V_FV - V_DG>
+            const bool blend = HG && !dg_only;
+            const R sc = blend ? R(1) - alpha : R(1);
 #pragma unroll
             for (int i = 0; i < N; ++i) {
                 const Node ni = load_node(base + i * s);
@@ -216,12 +221,47 @@ vjp_kernel(const VjpArgs a) {
                     R g[NV];
 #pragma unroll
                     for (int v = 0; v < NV; ++v) g[v] = fa * W(i, v) + fb * W(j, v);
+                    if constexpr (HG) {
+                        if (blend) {
+                            R f[NV];
+                            EQ::template two_point_value<VF>(ni, nj, d, P, f);
+#pragma unroll
+                            for (int v = 0; v < NV; ++v) abar_part -= (double)(g[v] * f[v]);
+                        }
+#pragma unroll
+                        for (int v = 0; v < NV; ++v) g[v] *= sc;
+                    }
                     Bar bj;
                     EQ::zero(bj);
                     EQ::template two_point_vjp<VF>(ni, nj, d, g, bi, bj, P);
                     add_acc(base + j * s, bj);
                 }
                 add_acc(base + i * s, bi);
+            }
+            if constexpr (HG) {
+                if (blend) {
+                    // FV subcells: acc_p += alpha invw_p F, acc_{p+1} -= alpha invw_{p+1} F
+#pragma unroll
+                    for (int p = 0; p + 1 < N; ++p) {
+                        asm volatile("" ::: "memory");
+                        const Node np0 = load_node(base + p * s);
+                        const Node np1 = load_node(base + (p + 1) * s);
+                        R h[NV], F[NV];
+                        EQ::template two_point_value<FVF>(np0, np1, d, P, F);
+#pragma unroll
+                        for (int v = 0; v < NV; ++v) {
+                            h[v] = (R)a.invw[p] * W(p, v) - (R)a.invw[p + 1] * W(p + 1, v);
+                            abar_part += (double)(h[v] * F[v]);
+                            h[v] *= alpha;
+                        }
+                        Bar b0, b1;
+                        EQ::zero(b0);
+                        EQ::zero(b1);
+                        EQ::template two_point_vjp<FVF>(np0, np1, d, h, b0, b1, P);
+                        add_acc(base + p * s, b0);
+                        add_acc(base + (p + 1) * s, b1);
+                    }
+                }
             }
             if constexpr (EQ::NONCONS) {
                 R ca[N], cc[N];
@@ -296,10 +336,6 @@ vjp_kernel(const VjpArgs a) {
                     EQ::flux_vjp(nd[p], d, g, bb[p], P);
                 }
             } else {
-                // HG: acc = (1 - alpha) V_DG + alpha V_FV + surface, so the DG
-                // cotangents scale by 1 - alpha and alphabar = <w, V_FV - V_DG>
-                const bool blend = HG && !dg_only;
-                const R sc = blend ? R(1) - alpha : R(1);
 #pragma unroll
                 for (int i = 0; i < N; ++i)
 #pragma unroll
@@ -308,35 +344,8 @@ vjp_kernel(const VjpArgs a) {
                         R g[NV];
 #pragma unroll
                         for (int v = 0; v < NV; ++v) g[v] = fa * W(i, v) + fb * W(j, v);
-                        if constexpr (HG) {
-                            if (blend) {
-                                R f[NV];
-                                EQ::template two_point_value<VF>(nd[i], nd[j], d, P, f);
-#pragma unroll
-                                for (int v = 0; v < NV; ++v) abar_part -= (double)(g[v] * f[v]);
-                            }
-#pragma unroll
-                            for (int v = 0; v < NV; ++v) g[v] *= sc;
-                        }
                         EQ::template two_point_vjp<VF>(nd[i], nd[j], d, g, bb[i], bb[j], P);
                     }
-                if constexpr (HG) {
-                    if (blend) {
-                        // FV subcells: acc_p += alpha invw_p F, acc_{p+1} -= alpha invw_{p+1} F
-#pragma unroll
-                        for (int p = 0; p + 1 < N; ++p) {
-                            R h[NV], F[NV];
-                            EQ::template two_point_value<FVF>(nd[p], nd[p + 1], d, P, F);
-#pragma unroll
-                            for (int v = 0; v < NV; ++v) {
-                                h[v] = (R)a.invw[p] * W(p, v) - (R)a.invw[p + 1] * W(p + 1, v);
-                                abar_part += (double)(h[v] * F[v]);
-                                h[v] *= alpha;
-                            }
-                            EQ::template two_point_vjp<FVF>(nd[p], nd[p + 1], d, h, bb[p], bb[p + 1], P);
-                        }
-                    }
-                }
                 if constexpr (EQ::NONCONS) {
                     // acc_i += 1/2 sum_j Ds[i,j] noncons(u_i, u_j), linear in u_j's (B_d, psi)
                     R ca[N], cc[N];
