@@ -87,7 +87,11 @@ def test_rhs_fp64_matches_oracle(th, oracle, torch, case):
 
 MIXED_CASES = [c for c in CASES if c.vi in ("fd", "hg") and c.polydeg == 3] + \
     [Case("euler2d_weak_llf_mixed", 2, "euler", 3), Case("linadv1d_mixed", 1, "linadv", 4, cmin=-1.0, cmax=1.0,
-                                                        ic="convergence_test")]
+                                                        ic="convergence_test"),
+     # weak form with N = 4 applies D-hat on the matrix cores in mixed precision
+     Case("euler3d_weak_llf_mixed", 3, "euler", 2, ic="smooth_periodic"),
+     Case("linadv3d_weak_mixed", 3, "linadv", 2, ic="seeded_random"),
+     Case("linadv2d_weak_mixed", 2, "linadv", 3, ic="seeded_random")]
 
 
 @pytest.mark.parametrize("case", MIXED_CASES, ids=lambda c: c.name)

@@ -22,9 +22,17 @@
 // the element's nodes, so each phase is race free between barriers.
 #pragma once
 
+#include <type_traits>
+
 #include "physics.hpp"
 
 namespace trixi_hip {
+
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+#ifndef TRIXI_HIP_WEAK_MFMA
+#define TRIXI_HIP_WEAK_MFMA 1  // mixed-precision weak form: D-hat on MFMA
+#endif
 
 constexpr int MAXN = 8;
 
@@ -107,6 +115,9 @@ rhs_kernel(const RhsArgs a) {
     constexpr bool NODE_LOGS = SF_LOGS || VF_LOGS || FV_LOGS;
     constexpr int NDF = NODE_LOGS ? 2 : 0;
     using Node = typename EQ::template Node<R>;
+    // mixed precision weak form with N = 4 on whole waves: D-hat on MFMA
+    constexpr bool WEAK_MFMA = TRIXI_HIP_WEAK_MFMA && VOL == VI_WEAK && std::is_same<R, float>::value &&
+                               N == 4 && (M * NL) % 64 == 0;
 
     // LDS: node records (R fields and FP64 logarithms) of the M elements,
     // field-major with the element innermost, and the direction accumulator.
@@ -245,14 +256,34 @@ rhs_kernel(const RhsArgs a) {
             R fl[N][NV];
 #pragma unroll
             for (int p = 0; p < N; ++p) EQ::flux(nd[p], d, P, fl[p]);
+            if constexpr (WEAK_MFMA) {
+                // D-hat on the matrix cores: v_mfma_f32_4x4x1_16b_f32 runs 16
+                // blocks of 4x4x1 per wave, block b on lanes 4b..4b+3.  Each
+                // lane gives row (lane % 4) of D-hat as A and its own line's
+                // flux at node k as B, so after k = 0..3 it holds its line's
+                // contraction sum_k Dhat[i][k] f_k in VGPRs i = 0..3.
+                const int r = threadIdx.x & 3;
+                float dh[N];
 #pragma unroll
-            for (int i = 0; i < N; ++i)
+                for (int k = 0; k < N; ++k) dh[k] = (float)a.Dhat[r * N + k];
 #pragma unroll
-                for (int p = 0; p < N; ++p) {
-                    const R w = (R)a.Dhat[i * N + p];
+                for (int v = 0; v < NV; ++v) {
+                    f32x4 c = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-                    for (int v = 0; v < NV; ++v) acl[i][v] += w * fl[p][v];
+                    for (int k = 0; k < N; ++k) c = __builtin_amdgcn_mfma_f32_4x4x1f32(dh[k], fl[k][v], c, 0, 0, 0);
+#pragma unroll
+                    for (int i = 0; i < N; ++i) acl[i][v] += c[i];
                 }
+            } else {
+#pragma unroll
+                for (int i = 0; i < N; ++i)
+#pragma unroll
+                    for (int p = 0; p < N; ++p) {
+                        const R w = (R)a.Dhat[i * N + p];
+#pragma unroll
+                        for (int v = 0; v < NV; ++v) acl[i][v] += w * fl[p][v];
+                    }
+            }
         } else {
             const R sc = (VOL == VI_HG && !dg_only) ? R(1) - alpha : R(1);
 #pragma unroll
