@@ -79,6 +79,24 @@ enum trixi_hip_initial_condition {
                                                on rho, v, p per node               */
 };
 
+/* Boundary condition on the faces of a non-periodic TreeMesh that have no
+ * interface (Trixi's boundary_conditions argument; reference cuda_boundary_flux!
+ * dg_2d.jl:453-494 and boundary_flux_kernel!, the same condition on every side
+ * as in test/tree_dgsem_2d/euler_source_terms_nonperiodic.jl:12-17). */
+enum trixi_hip_boundary_condition {
+    TRIXI_HIP_BC_PERIODIC = 0,  /* every face has an interface                     */
+    TRIXI_HIP_BC_DIRICHLET = 1  /* BoundaryConditionDirichlet(boundary_ic): state
+                                   boundary_ic(x_face, t); flux(u_inner, u_b) on
+                                   + faces, flux(u_b, u_inner) on - faces        */
+};
+
+/* Source terms added after the Jacobian (reference cuda_sources!,
+ * dg_2d.jl:746-757; source_terms_kernel!). */
+enum trixi_hip_source_terms {
+    TRIXI_HIP_SOURCE_NONE = 0,
+    TRIXI_HIP_SOURCE_CONVERGENCE_TEST = 1 /* source_terms_convergence_test (Euler)  */
+};
+
 /* Solver configuration: DGSEMGPU(; polydeg, surface_flux, volume_integral)
  * (reference src/solvers/dgsem_gpu.jl:43-53) plus the equation parameters. */
 typedef struct trixi_hip_config {
@@ -97,6 +115,11 @@ typedef struct trixi_hip_config {
     double advection_velocity[3];
     double alpha_max, alpha_min; /* HG indicator parameters                       */
     double ic_length;         /* period used by TRIXI_HIP_IC_SMOOTH_PERIODIC      */
+    int32_t boundary_condition; /* enum trixi_hip_boundary_condition              */
+    int32_t boundary_ic;      /* Dirichlet state: enum trixi_hip_initial_condition */
+    uint64_t boundary_seed;   /* seed of a seeded boundary_ic                     */
+    int32_t source_terms;     /* enum trixi_hip_source_terms                      */
+    int32_t reserved2;
 } trixi_hip_config;
 
 /* Mesh and connectivity (host arrays, copied to the device by create): the

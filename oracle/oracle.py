@@ -21,6 +21,8 @@ EQ_LINADV, EQ_EULER, EQ_GLM_MHD = 0, 1, 2
 VI_WEAK, VI_FD, VI_HG = 0, 1, 2
 FLUX_CENTRAL, FLUX_LLF, FLUX_LLF_NAIVE, FLUX_RANOCHA, FLUX_SHIMA_ETAL, FLUX_HINDENLANG_GASSNER = range(6)
 IC_CONSTANT, IC_CONVERGENCE_TEST, IC_WEAK_BLAST_WAVE, IC_SEEDED_RANDOM, IC_SMOOTH_PERIODIC, IC_WEAK_BLAST_WAVE_SEEDED = range(6)
+BC_PERIODIC, BC_DIRICHLET = 0, 1
+SOURCE_NONE, SOURCE_CONVERGENCE_TEST = 0, 1
 
 
 class OracleCfg(ctypes.Structure):
@@ -39,6 +41,13 @@ class OracleCfg(ctypes.Structure):
         ("alpha_max", ctypes.c_double),
         ("alpha_min", ctypes.c_double),
         ("ic_length", ctypes.c_double),
+        ("boundary_condition", ctypes.c_int32),
+        ("boundary_ic", ctypes.c_int32),
+        ("boundary_seed", ctypes.c_uint64),
+        ("source_terms", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+        ("node_coordinates", ctypes.c_void_p),
+        ("time", ctypes.c_double),
     ]
 
 
@@ -74,6 +83,10 @@ def lib():
             "oracle_entropy_potential": (None, [cfgp, P, ctypes.c_int, P]),
             "oracle_mesh_nelements": (i64, [ctypes.c_int, ctypes.c_int]),
             "oracle_mesh_uniform": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, dbl, dbl, ctypes.c_int, P, P, P, P, P]),
+            "oracle_mesh_ninterfaces": (i64, [ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+            "oracle_mesh": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, dbl, dbl, ctypes.c_int, ctypes.c_int,
+                                           P, P, P, P, P]),
+            "oracle_source_convergence_test": (None, [cfgp, P, dbl, P]),
             "oracle_initial_condition": (None, [cfgp, ctypes.c_int, P, dbl, u64, u64, P]),
             "oracle_compute_coefficients": (None, [cfgp, ctypes.c_int, i64, P, dbl, u64, P]),
             "oracle_rhs": (None, [cfgp, i64, i64, P, P, P, P, P, P]),
@@ -102,8 +115,11 @@ def _p(a):
 def make_cfg(ndims, equation, polydeg=3, volume_integral=VI_WEAK, volume_flux=FLUX_CENTRAL,
              surface_flux=FLUX_LLF, fv_flux=FLUX_LLF, gamma=1.4, c_h=1.0,
              advection_velocity=(1.0, 1.0, 1.0), alpha_max=0.5, alpha_min=0.001,
-             alpha_smooth=True, ic_length=4.0):
+             alpha_smooth=True, ic_length=4.0, boundary_condition=BC_PERIODIC, boundary_ic=IC_CONSTANT,
+             boundary_seed=0, source_terms=SOURCE_NONE):
     c = OracleCfg()
+    c.boundary_condition, c.boundary_ic, c.boundary_seed = boundary_condition, boundary_ic, boundary_seed
+    c.source_terms = source_terms
     c.ndims, c.equation, c.polydeg = ndims, equation, polydeg
     c.volume_integral, c.volume_flux, c.surface_flux, c.fv_flux = (
         volume_integral, volume_flux, surface_flux, fv_flux)
@@ -194,11 +210,12 @@ def max_abs_speeds(cfg, u):
 
 
 class Mesh:
-    """Uniform periodic TreeMesh as built by the oracle."""
+    """Uniform TreeMesh as built by the oracle (periodic unless periodic=False)."""
 
-    def __init__(self, ndims, level, cmin, cmax, polydeg):
+    def __init__(self, ndims, level, cmin, cmax, polydeg, periodic=True):
         E = lib().oracle_mesh_nelements(ndims, level)
-        I = ndims * E
+        I = lib().oracle_mesh_ninterfaces(ndims, level, 1 if periodic else 0)
+        self.periodic = bool(periodic)
         N = polydeg + 1
         self.ndims, self.level, self.polydeg, self.nelements, self.ninterfaces = ndims, level, polydeg, E, I
         self.coordinates_min, self.coordinates_max = cmin, cmax
@@ -207,9 +224,9 @@ class Mesh:
         self.inverse_jacobian = np.zeros(E)
         self.node_coordinates = np.zeros((E,) + (N,) * ndims + (ndims,))
         self.cell_index = np.zeros((E, ndims), dtype=np.int32)
-        rc = lib().oracle_mesh_uniform(ndims, level, cmin, cmax, polydeg, _p(self.neighbor_ids),
-                                       _p(self.orientations), _p(self.inverse_jacobian),
-                                       _p(self.node_coordinates), _p(self.cell_index))
+        rc = lib().oracle_mesh(ndims, level, cmin, cmax, polydeg, 1 if periodic else 0, _p(self.neighbor_ids),
+                               _p(self.orientations), _p(self.inverse_jacobian), _p(self.node_coordinates),
+                               _p(self.cell_index))
         assert rc == 0
         self.total_volume = (cmax - cmin) ** ndims
 
@@ -233,7 +250,24 @@ def initial_condition(cfg, ic, x, t=0.0, seed=20261202, node_index=0):
     return u
 
 
-def rhs(cfg, mesh, u, return_alpha=False):
+def _bind(cfg, mesh, t=0.0):
+    """A copy of cfg that carries the mesh's node coordinates (boundary states,
+    sources) and the time."""
+    c = OracleCfg.from_buffer_copy(cfg)
+    c.node_coordinates = mesh.node_coordinates.ctypes.data
+    c.time = float(t)
+    return c
+
+
+def source_convergence_test(cfg, x, t):
+    s = np.zeros(nvars(cfg))
+    lib().oracle_source_convergence_test(ctypes.byref(cfg), _p(np.ascontiguousarray(np.atleast_1d(x), dtype=np.float64)),
+                                         float(t), _p(s))
+    return s
+
+
+def rhs(cfg, mesh, u, return_alpha=False, t=0.0):
+    cfg = _bind(cfg, mesh, t)
     u = np.ascontiguousarray(u, dtype=np.float64)
     du = np.zeros_like(u)
     alpha = np.zeros(mesh.nelements) if return_alpha else None
@@ -242,7 +276,8 @@ def rhs(cfg, mesh, u, return_alpha=False):
     return (du, alpha) if return_alpha else du
 
 
-def rhs_jvp(cfg, mesh, u, udot):
+def rhs_jvp(cfg, mesh, u, udot, t=0.0):
+    cfg = _bind(cfg, mesh, t)
     u = np.ascontiguousarray(u, dtype=np.float64)
     udot = np.ascontiguousarray(udot, dtype=np.float64)
     du = np.zeros_like(u)
@@ -289,14 +324,16 @@ def ck2n54_coefficients():
 
 def ck2n54_step(cfg, mesh, u, k, t, dt):
     """Advance ``u`` (in place) by one CK2N54 step; ``k`` is the 2N register."""
+    cfg = _bind(cfg, mesh, t)
     du = np.zeros_like(u)
     lib().oracle_ck2n54_step(ctypes.byref(cfg), mesh.nelements, mesh.ninterfaces, _p(mesh.neighbor_ids),
                              _p(mesh.orientations), _p(mesh.inverse_jacobian), _p(u), _p(k), _p(du),
                              float(t), float(dt))
 
 
-def ck2n54_step_jvp(cfg, mesh, u, udot, dt):
-    """(u_next, d u_next) of one CK2N54 step from u along the tangent udot."""
+def ck2n54_step_jvp(cfg, mesh, u, udot, dt, t=0.0):
+    """(u_next, d u_next) of one CK2N54 step from (t, u) along the tangent udot."""
+    cfg = _bind(cfg, mesh, t)
     u = np.ascontiguousarray(u, dtype=np.float64)
     udot = np.ascontiguousarray(udot, dtype=np.float64)
     un, und = np.zeros_like(u), np.zeros_like(u)
@@ -306,14 +343,14 @@ def ck2n54_step_jvp(cfg, mesh, u, udot, dt):
     return un, und
 
 
-def jacobian(cfg, mesh, u):
+def jacobian(cfg, mesh, u, t=0.0):
     """Dense Jacobian of rhs! (tiny meshes only), column by column from the dual JVP."""
     n = u.size
     J = np.zeros((n, n))
     for j in range(n):
         e = np.zeros(n)
         e[j] = 1.0
-        J[:, j] = rhs_jvp(cfg, mesh, u, e.reshape(u.shape))[1].ravel()
+        J[:, j] = rhs_jvp(cfg, mesh, u, e.reshape(u.shape), t)[1].ravel()
     return J
 
 

@@ -82,6 +82,10 @@ constexpr int MAXV = 9;
 struct Cfg {
     int nd, eq, N, nv, vi, vflux, sflux, fvflux, smooth;
     double gamma, inv_gm1, c_h, a[3], amax, amin, ic_length;
+    int bc, bc_ic, src;
+    uint64_t bc_seed;
+    const double* xnodes;
+    double t;
 };
 
 Cfg make_cfg(const oracle_cfg* c) {
@@ -102,6 +106,12 @@ Cfg make_cfg(const oracle_cfg* c) {
     k.amax = c->alpha_max;
     k.amin = c->alpha_min;
     k.ic_length = c->ic_length;
+    k.bc = c->boundary_condition;
+    k.bc_ic = c->boundary_ic;
+    k.bc_seed = c->boundary_seed;
+    k.src = c->source_terms;
+    k.xnodes = c->node_coordinates;
+    k.t = c->time;
     return k;
 }
 
@@ -691,6 +701,28 @@ void indicator_hg(const Cfg& c, const Basis& b, int64_t E, int64_t I, const int3
 // integral, Jacobian); periodic meshes only, no mortars, no sources -- the
 // reference asserts the same (dg_3d.jl:463-466, 542-544).
 // ----------------------------------------------------------------------------
+void initial_condition(const Cfg& c, int ic, const double* x, double t, uint64_t seed, uint64_t node_index,
+                       double* u);
+
+// source_terms_convergence_test (compressible Euler): with
+// g(s) = 2 + 0.1 sin(pi s), s = x_1 + ... + x_d - t, the convergence-test state
+// rho = rho v_i = g, rho e = g^2 (v_i = 1, p = (gamma - 1)(g^2 - d g / 2)) solves
+// u_t + div f(u) = S with S the terms below (dg/ds = 0.1 pi cos(pi s)):
+//   S_rho      = d g' - g'                                (continuity)
+//   S_{rho v_i} = d g' - g' + p'                           (momentum, i = 1..d)
+//   S_{rho e}  = d (2 g g' + p') - 2 g g'                 (energy)
+// with p' = (gamma - 1)(2 g - d/2) g'.
+void source_convergence_test(const Cfg& c, const double* x, double t, double* sv) {
+    const int d = c.nd;
+    double s = -t;
+    for (int q = 0; q < d; ++q) s += x[q];
+    const double g = 2.0 + 0.1 * sin(M_PI * s), gp = 0.1 * M_PI * cos(M_PI * s);
+    const double pp = (c.gamma - 1.0) * (2.0 * g - 0.5 * d) * gp;
+    sv[0] = d * gp - gp;
+    for (int q = 0; q < d; ++q) sv[1 + q] = d * gp - gp + pp;
+    sv[d + 1] = d * (2.0 * g * gp + pp) - 2.0 * g * gp;
+}
+
 template <class T>
 void rhs_impl(const Cfg& c, int64_t E, int64_t I, const int32_t* nbr, const int32_t* ori,
               const double* invJ, const T* u, T* du, T* alpha_out) {
@@ -779,6 +811,45 @@ void rhs_impl(const Cfg& c, int64_t E, int64_t I, const int32_t* nbr, const int3
         }
     }
 
+    // calc_boundary_flux! (Trixi BoundaryConditionDirichlet, as in the reference
+    // test test/tree_dgsem_2d/euler_source_terms_nonperiodic.jl:12-17 and the
+    // GPU kernel boundary_flux_kernel!): every element face without an
+    // interface is a boundary face; the state outside is boundary_ic(x, t) at
+    // the face node; the flux is surface_flux(u_inner, u_b) when the boundary
+    // lies on the element's + side (Trixi direction even), else
+    // surface_flux(u_b, u_inner).
+    if (c.bc == OR_BC_DIRICHLET) {
+        std::vector<char> has_interface((size_t)2 * nd * E, 0);
+        for (int64_t i = 0; i < I; ++i) {
+            const int o = ori[i];
+            has_interface[(size_t)(2 * o - 1) * E + nbr[2 * i]] = 1;
+            has_interface[(size_t)(2 * o - 2) * E + nbr[2 * i + 1]] = 1;
+        }
+#pragma omp parallel for schedule(static)
+        for (int64_t e = 0; e < E; ++e)
+            for (int dir = 0; dir < 2 * nd; ++dir) {
+                if (has_interface[(size_t)dir * E + e]) continue;
+                const int od = dir / 2;
+                const bool plus = (dir % 2) == 1;
+                T uin[MAXV], ub[MAXV], flux[MAXV];
+                for (int f = 0; f < nf; ++f) {
+                    const int node = face_node(N, nd, od, f, plus ? N - 1 : 0);
+                    double x[3] = {0, 0, 0}, ubd[MAXV];
+                    for (int d = 0; d < nd; ++d) x[d] = c.xnodes[((size_t)e * nn + node) * nd + d];
+                    initial_condition(c, c.bc_ic, x, c.t, c.bc_seed, (uint64_t)(e * nn + node), ubd);
+                    for (int v = 0; v < nv; ++v) {
+                        uin[v] = u[e * esz + nv * node + v];
+                        ub[v] = T(ubd[v]);
+                    }
+                    if (plus)
+                        two_point(c, c.sflux, uin, ub, od + 1, flux);
+                    else
+                        two_point(c, c.sflux, ub, uin, od + 1, flux);
+                    for (int v = 0; v < nv; ++v) SF(v, f, dir, e) = flux[v];
+                }
+            }
+    }
+
     // calc_surface_integral!
     const double factor_1 = b.lm[0], factor_2 = b.lp[N - 1];
 #pragma omp parallel for schedule(static)
@@ -798,6 +869,19 @@ void rhs_impl(const Cfg& c, int64_t E, int64_t I, const int32_t* nbr, const int3
     for (int64_t e = 0; e < E; ++e) {
         const double factor = -invJ[e];
         for (size_t q = 0; q < esz; ++q) du[e * esz + q] = du[e * esz + q] * factor;
+    }
+
+    // calc_sources! (after the Jacobian, as in Trixi and the reference
+    // cuda_sources!, dg_2d.jl:746-757)
+    if (c.src == OR_SOURCE_CONVERGENCE_TEST) {
+#pragma omp parallel for schedule(static)
+        for (int64_t e = 0; e < E; ++e)
+            for (int n = 0; n < nn; ++n) {
+                double x[3] = {0, 0, 0}, sv[MAXV];
+                for (int d = 0; d < nd; ++d) x[d] = c.xnodes[((size_t)e * nn + n) * nd + d];
+                source_convergence_test(c, x, c.t, sv);
+                for (int v = 0; v < nv; ++v) du[e * esz + nv * n + v] = du[e * esz + nv * n + v] + sv[v];
+            }
     }
 }
 
@@ -1131,9 +1215,23 @@ int64_t oracle_mesh_nelements(int ndims, int level) { return (int64_t)1 << (ndim
 // level most significant.  Elements are the leaves in cell order; interfaces
 // are created per element for the +x, +y, +z neighbours in that order
 // (Trixi TreeMesh + init_elements!/init_interfaces!).
+int64_t oracle_mesh_ninterfaces(int nd, int level, int periodic) {
+    const int64_t E = oracle_mesh_nelements(nd, level);
+    if (periodic) return nd * E;
+    // faces on the + boundary of each direction have no interface
+    return nd * (E - (E >> level));
+}
+
 int oracle_mesh_uniform(int nd, int level, double cmin, double cmax, int polydeg,
                         int32_t* neighbor_ids, int32_t* orientations, double* invJ,
                         double* node_coordinates, int32_t* cell_index) {
+    return oracle_mesh(nd, level, cmin, cmax, polydeg, 1, neighbor_ids, orientations, invJ, node_coordinates,
+                       cell_index);
+}
+
+int oracle_mesh(int nd, int level, double cmin, double cmax, int polydeg, int periodic,
+                int32_t* neighbor_ids, int32_t* orientations, double* invJ,
+                double* node_coordinates, int32_t* cell_index) {
     if (nd < 1 || nd > 3 || level < 0 || level > 10) return -1;
     const int64_t E = oracle_mesh_nelements(nd, level);
     const int64_t n1 = (int64_t)1 << level;
@@ -1188,6 +1286,7 @@ int oracle_mesh_uniform(int nd, int level, double cmin, double cmax, int polydeg
                 }
         }
         for (int d = 0; d < nd; ++d) {
+            if (!periodic && cells[e][d] + 1 == n1) continue;  // boundary face
             std::vector<int64_t> k = cells[e];
             k[d] = (k[d] + 1) % n1;
             int64_t key = 0;
@@ -1201,6 +1300,10 @@ int oracle_mesh_uniform(int nd, int level, double cmin, double cmax, int polydeg
         }
     }
     return 0;
+}
+
+void oracle_source_convergence_test(const oracle_cfg* cfg, const double* x, double t, double* s) {
+    source_convergence_test(make_cfg(cfg), x, t, s);
 }
 
 void oracle_initial_condition(const oracle_cfg* cfg, int ic, const double* x, double t,
@@ -1388,12 +1491,12 @@ void oracle_ck2n54_step(const oracle_cfg* cfg, int64_t E, int64_t I, const int32
     Cfg c = make_cfg(cfg);
     const size_t n = (size_t)E * ipow(c.N, c.nd) * c.nv;
     for (int s = 0; s < 5; ++s) {
+        c.t = t + C[s] * dt;
         rhs_impl<double>(c, E, I, nbr, ori, invJ, u, du, (double*)nullptr);
         for (size_t q = 0; q < n; ++q) {
             k[q] = (s == 0) ? dt * du[q] : A[s] * k[q] + dt * du[q];
             u[q] = u[q] + B[s] * k[q];
         }
-        (void)t;
     }
 }
 
@@ -1408,7 +1511,9 @@ void oracle_ck2n54_step_jvp(const oracle_cfg* cfg, int64_t E, int64_t I, const i
     const size_t n = (size_t)E * ipow(c.N, c.nd) * c.nv;
     std::vector<Dual> uu(n), k(n), du(n);
     for (size_t q = 0; q < n; ++q) uu[q] = Dual(u[q], udot[q]);
+    const double t0 = c.t;  // start of the step
     for (int s = 0; s < 5; ++s) {
+        c.t = t0 + C[s] * dt;
         rhs_impl<Dual>(c, E, I, nbr, ori, invJ, uu.data(), du.data(), (Dual*)nullptr);
         for (size_t q = 0; q < n; ++q) {
             k[q] = (s == 0) ? dt * du[q] : A[s] * k[q] + dt * du[q];

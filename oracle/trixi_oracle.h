@@ -43,6 +43,8 @@ enum {
     OR_IC_SMOOTH_PERIODIC = 4,  /* Euler: seeded sums of periodic sines           */
     OR_IC_WEAK_BLAST_WAVE_SEEDED = 5 /* Euler: weak blast + U(-0.01,0.01) on rho,v,p */
 };
+enum { OR_BC_PERIODIC = 0, OR_BC_DIRICHLET = 1 };
+enum { OR_SOURCE_NONE = 0, OR_SOURCE_CONVERGENCE_TEST = 1 };
 
 typedef struct oracle_cfg {
     int32_t ndims;            /* 1, 2, 3 */
@@ -58,6 +60,13 @@ typedef struct oracle_cfg {
     double advection_velocity[3];
     double alpha_max, alpha_min;
     double ic_length;         /* domain period used by the smooth periodic IC */
+    int32_t boundary_condition; /* OR_BC_*: faces without an interface          */
+    int32_t boundary_ic;      /* Dirichlet state: OR_IC_*                        */
+    uint64_t boundary_seed;
+    int32_t source_terms;     /* OR_SOURCE_*                                     */
+    int32_t reserved;
+    const double* node_coordinates; /* [E][nodes][d], for boundary states and sources */
+    double time;              /* rhs time; the start of the step for the RK step JVP */
 } oracle_cfg;
 
 int oracle_nvars(const oracle_cfg* cfg);
@@ -91,6 +100,14 @@ int oracle_mesh_uniform(int ndims, int level, double coord_min, double coord_max
                         int polydeg, int32_t* neighbor_ids, int32_t* orientations,
                         double* inverse_jacobian, double* node_coordinates,
                         int32_t* cell_index /* [ndims*E] integer cell coords */);
+/* The same mesh, non-periodic when periodic == 0: no interfaces across the
+ * domain boundary (those faces are boundary faces). */
+int64_t oracle_mesh_ninterfaces(int ndims, int level, int periodic);
+int oracle_mesh(int ndims, int level, double coord_min, double coord_max, int polydeg, int periodic,
+                int32_t* neighbor_ids, int32_t* orientations, double* inverse_jacobian,
+                double* node_coordinates, int32_t* cell_index);
+/* source_terms_convergence_test at x, t (compressible Euler) */
+void oracle_source_convergence_test(const oracle_cfg* cfg, const double* x, double t, double* s);
 
 /* ---- initial conditions ------------------------------------------------------ */
 void oracle_initial_condition(const oracle_cfg* cfg, int ic, const double* x,

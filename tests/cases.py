@@ -27,6 +27,10 @@ class Case:
     alpha_smooth: bool = True
     precision: str = "fp64"
     seed: int = 20261202
+    periodic: bool = True
+    bc_state: str = ""      # Dirichlet state on the boundary faces of a non-periodic mesh
+    source: str = ""        # "convergence_test" for source_terms_convergence_test
+    t: float = 0.0          # time of the rhs! evaluation
 
 
 FLUX_NAMES = ("central", "llf", "llf_naive", "ranocha", "shima_etal", "hindenlang_gassner")
@@ -34,13 +38,20 @@ IC_NAMES = ("constant", "convergence_test", "weak_blast_wave", "seeded_random", 
             "weak_blast_wave_seeded")
 
 
+def oracle_mesh(O, c: Case):
+    return O.Mesh(c.ndims, c.level, c.cmin, c.cmax, c.polydeg, periodic=c.periodic)
+
+
 def oracle_cfg(O, c: Case):
     eq = {"linadv": O.EQ_LINADV, "euler": O.EQ_EULER, "mhd": O.EQ_GLM_MHD}[c.equation]
     vi = {"weak": O.VI_WEAK, "fd": O.VI_FD, "hg": O.VI_HG}[c.vi]
+    ext = dict(boundary_condition=O.BC_DIRICHLET if c.bc_state else O.BC_PERIODIC,
+               boundary_ic=IC_NAMES.index(c.bc_state) if c.bc_state else 0,
+               source_terms=O.SOURCE_CONVERGENCE_TEST if c.source == "convergence_test" else O.SOURCE_NONE)
     return O.make_cfg(c.ndims, eq, c.polydeg, vi, FLUX_NAMES.index(c.vflux), FLUX_NAMES.index(c.sflux),
                       FLUX_NAMES.index(c.fvflux), gamma=c.gamma, c_h=c.c_h,
                       advection_velocity=c.advection[:c.ndims], alpha_max=c.alpha_max, alpha_min=c.alpha_min,
-                      alpha_smooth=c.alpha_smooth, ic_length=c.cmax - c.cmin)
+                      alpha_smooth=c.alpha_smooth, ic_length=c.cmax - c.cmin, **ext)
 
 
 def oracle_ic(O, c: Case):
@@ -77,13 +88,19 @@ def product_semi(th, c: Case, precision=None):
     solver = th.DGSEMHIP(polydeg=c.polydeg, surface_flux=_flux(th, c.sflux, mhd), volume_integral=vi,
                          precision=precision or c.precision)
     mesh = th.TreeMesh((c.cmin,) * c.ndims, (c.cmax,) * c.ndims, initial_refinement_level=c.level,
-                       n_cells_max=10**8)
-    ic = {"constant": th.initial_condition_constant, "convergence_test": th.initial_condition_convergence_test,
-          "weak_blast_wave": th.initial_condition_weak_blast_wave,
-          "seeded_random": th.initial_condition_seeded_random(c.seed),
-          "smooth_periodic": th.initial_condition_smooth_periodic(c.seed),
-          "weak_blast_wave_seeded": th.initial_condition_weak_blast_wave_seeded(c.seed)}[c.ic]
-    return th.SemidiscretizationHyperbolicHIP(mesh, eq, ic, solver)
+                       n_cells_max=10**8, periodicity=c.periodic)
+    ics = {"constant": th.initial_condition_constant, "convergence_test": th.initial_condition_convergence_test,
+           "weak_blast_wave": th.initial_condition_weak_blast_wave,
+           "seeded_random": th.initial_condition_seeded_random(c.seed),
+           "smooth_periodic": th.initial_condition_smooth_periodic(c.seed),
+           "weak_blast_wave_seeded": th.initial_condition_weak_blast_wave_seeded(c.seed)}
+    kw = {}
+    if c.bc_state:
+        bc = th.BoundaryConditionDirichlet(ics[c.bc_state])
+        kw["boundary_conditions"] = {s: bc for s in ("x_neg", "x_pos", "y_neg", "y_pos", "z_neg", "z_pos")[:2 * c.ndims]}
+    if c.source == "convergence_test":
+        kw["source_terms"] = th.source_terms_convergence_test
+    return th.SemidiscretizationHyperbolicHIP(mesh, eq, ics[c.ic], solver, **kw)
 
 
 def rel_max(a, b):

@@ -14,7 +14,7 @@
 import numpy as np
 import pytest
 
-from cases import Case, oracle_cfg, oracle_ic, product_semi
+from cases import Case, oracle_cfg, oracle_ic, oracle_mesh, product_semi
 
 pytestmark = pytest.mark.gpu
 
@@ -29,8 +29,8 @@ def torch():
 
 def setup(th, O, torch, case):
     cfg = oracle_cfg(O, case)
-    mesh = O.Mesh(case.ndims, case.level, case.cmin, case.cmax, case.polydeg)
-    u = O.compute_coefficients(cfg, mesh, oracle_ic(O, case), 0.0, case.seed)
+    mesh = oracle_mesh(O, case)
+    u = O.compute_coefficients(cfg, mesh, oracle_ic(O, case), case.t, case.seed)
     semi = product_semi(th, case)
     ud = th.pack(torch.from_numpy(u).cuda(), semi)
     return cfg, mesh, u, semi, ud
@@ -54,6 +54,11 @@ TINY = [
     Case("euler2d_hg_shima_llfnaive_amax1", 2, "euler", 2, "hg", "shima_etal", "llf_naive", "llf_naive",
          alpha_max=1.0),
     Case("euler1d_hg_ranocha_blast", 1, "euler", 2, "hg", "ranocha", ic="weak_blast_wave_seeded"),
+    # non-periodic: Dirichlet states do not depend on u, the source depends on x and t only
+    Case("euler2d_weak_dirichlet_src", 2, "euler", 1, cmin=0.0, cmax=2.0, periodic=False, bc_state="convergence_test",
+         source="convergence_test", t=0.3),
+    Case("euler1d_fd_dirichlet", 1, "euler", 2, "fd", "ranocha", cmin=0.0, cmax=2.0, periodic=False,
+         bc_state="convergence_test", t=0.6),
     Case("mhd3d_fd_hg_llfnaive", 3, "mhd", 1, "fd", "hindenlang_gassner", "llf_naive", cmin=-1.0, cmax=1.0,
          gamma=5 / 3),
     Case("mhd3d_fd_hg_hg", 3, "mhd", 1, "fd", "hindenlang_gassner", "hindenlang_gassner", cmin=-1.0, cmax=1.0,
@@ -65,19 +70,19 @@ TINY = [
 def test_vjp_equals_transposed_oracle_jacobian(th, oracle, torch, case):
     O = oracle
     cfg, mesh, u, semi, ud = setup(th, O, torch, case)
-    J = O.jacobian(cfg, mesh, u)
+    J = O.jacobian(cfg, mesh, u, case.t)
     rng = np.random.default_rng(7)
     vbar = rng.standard_normal(u.shape)
     expect = (J.T @ vbar.ravel()).reshape(u.shape)
     vd = th.pack(torch.from_numpy(vbar).cuda(), semi)
     ubar = torch.empty_like(vd)
-    th.rhs_vjp(ubar, ud, vd, semi)
+    th.rhs_vjp(ubar, ud, vd, semi, t=case.t)
     got = th.wrap_array_native(ubar, semi).cpu().numpy()
     err = float(np.abs(got - expect).max() / np.abs(expect).max())
     print(f"{case.name}: VJP vs J^T vbar rel max {err:.3e}", flush=True)
     assert err < 1e-11, err
     # beta accumulation: ubar = 2 ubar + J^T (3 vbar)
-    th.rhs_vjp(ubar, ud, vd, semi, alpha=3.0, beta=2.0)
+    th.rhs_vjp(ubar, ud, vd, semi, t=case.t, alpha=3.0, beta=2.0)
     got2 = th.wrap_array_native(ubar, semi).cpu().numpy()
     assert float(np.abs(got2 - 5 * expect).max() / np.abs(expect).max()) < 1e-11
     semi.close()
@@ -271,6 +276,32 @@ def test_c4_l2_gradient_full_config(th, torch):
         worst = max(worst, abs(ad - fd) / abs(fd))
         assert abs(ad - fd) <= 1e-5 * abs(fd), (i, ad, fd)
     print(f"c4: 16 directions, worst relative difference adjoint vs FD {worst:.3e}", flush=True)
+    semi.close()
+
+
+def test_nonperiodic_multistep_gradient_matches_oracle_tangent(th, oracle, torch):
+    """2D Euler on a non-periodic mesh with the exact convergence-test state on
+    the boundary and source_terms_convergence_test: 10 checkpointed CK2N54
+    steps from t = 0 (time-dependent boundary states and sources at every
+    stage) against the oracle's chained step JVPs."""
+    O = oracle
+    case = Case("euler2d_dirichlet_src", 2, "euler", 2, cmin=0.0, cmax=2.0, ic="convergence_test", periodic=False,
+                bc_state="convergence_test", source="convergence_test")
+    cfg, mesh, u, semi, ud = setup(th, O, torch, case)
+    dt = 0.5 * O.max_dt(cfg, mesh, u)
+    nsteps = 10
+    rng = np.random.default_rng(21)
+    d = rng.standard_normal(u.shape)
+    g = rng.standard_normal(u.shape)
+    ut, dt_ = u.copy(), d.copy()
+    for n in range(nsteps):
+        ut, dt_ = O.ck2n54_step_jvp(cfg, mesh, ut, dt_, dt, t=n * dt)
+    exact = float((g * dt_).sum())
+    traj = th.forward_with_checkpoints(semi, ud, dt, nsteps)
+    grad = th.backward(traj, th.pack(torch.from_numpy(g).cuda(), semi))
+    ad = float((th.wrap_array_native(grad, semi).cpu().numpy() * d).sum())
+    print(f"non-periodic: adjoint {ad:.15e} oracle tangent {exact:.15e}", flush=True)
+    assert abs(ad - exact) <= 1e-10 * abs(exact)
     semi.close()
 
 

@@ -7,7 +7,7 @@ relative max-norm of du <= 2e-5 against the FP64 oracle (stated per case).
 import numpy as np
 import pytest
 
-from cases import Case, oracle_cfg, oracle_ic, product_semi, rel_max
+from cases import oracle_mesh, Case, oracle_cfg, oracle_ic, product_semi, rel_max
 
 pytestmark = pytest.mark.gpu
 
@@ -26,13 +26,13 @@ def torch():
 def run_pair(th, O, torch, case, precision=None):
     """(du_gpu in Trixi layout, du_oracle, u) for one case."""
     cfg = oracle_cfg(O, case)
-    mesh = O.Mesh(case.ndims, case.level, case.cmin, case.cmax, case.polydeg)
-    u = O.compute_coefficients(cfg, mesh, oracle_ic(O, case), 0.0, case.seed)
-    du_ref = O.rhs(cfg, mesh, u)
+    mesh = oracle_mesh(O, case)
+    u = O.compute_coefficients(cfg, mesh, oracle_ic(O, case), case.t, case.seed)
+    du_ref = O.rhs(cfg, mesh, u, t=case.t)
     semi = product_semi(th, case, precision)
     u_dev = th.pack(torch.from_numpy(u).cuda(), semi)
     du_dev = torch.empty_like(u_dev)
-    th.rhs_hip_(du_dev, u_dev, semi, 0.0)
+    th.rhs_hip_(du_dev, u_dev, semi, case.t)
     du = th.wrap_array_native(du_dev, semi).cpu().numpy()
     torch.cuda.synchronize()
     semi.close()
@@ -65,6 +65,29 @@ CASES = [
          cmax=1.0, gamma=5 / 3),
     Case("mhd3d_fd_hg_llfnaive", 3, "mhd", 1, "fd", "hindenlang_gassner", "llf_naive", cmin=-1.0, cmax=1.0,
          gamma=5 / 3),
+    # non-periodic meshes with Dirichlet states (the exact convergence-test
+    # solution) and source_terms_convergence_test, evaluated at t = 0.3
+    # (reference tests tree_dgsem_{1,2}d/euler_source_terms_nonperiodic.jl)
+    Case("euler1d_weak_dirichlet_src", 1, "euler", 4, cmin=0.0, cmax=2.0, ic="convergence_test", periodic=False,
+         bc_state="convergence_test", source="convergence_test", t=0.3),
+    Case("euler2d_weak_dirichlet_src", 2, "euler", 3, cmin=0.0, cmax=2.0, ic="convergence_test", periodic=False,
+         bc_state="convergence_test", source="convergence_test", t=0.3),
+    Case("euler3d_weak_dirichlet_src", 3, "euler", 2, cmin=0.0, cmax=2.0, ic="convergence_test", periodic=False,
+         bc_state="convergence_test", source="convergence_test", t=0.3),
+    Case("euler2d_fd_dirichlet_random", 2, "euler", 3, "fd", "ranocha", cmin=0.0, cmax=2.0, periodic=False,
+         bc_state="convergence_test", t=0.7),
+    Case("euler3d_fd_dirichlet_const", 3, "euler", 2, "fd", "ranocha", cmin=0.0, cmax=2.0, periodic=False,
+         bc_state="constant", source="convergence_test", t=0.1),
+    # periodic with source terms (reference tests tree_dgsem_{1,2,3}d/euler_source_terms.jl)
+    Case("euler1d_weak_periodic_src", 1, "euler", 4, cmin=0.0, cmax=2.0, ic="convergence_test",
+         source="convergence_test", t=0.4),
+    Case("euler2d_weak_periodic_src", 2, "euler", 3, cmin=0.0, cmax=2.0, ic="convergence_test",
+         source="convergence_test", t=0.4),
+    # (a random state: at the convergence-test state du is the small difference
+    # of a large flux divergence and the source, and the relative error of du
+    # grows with that cancellation)
+    Case("euler3d_fd_periodic_src", 3, "euler", 2, "fd", "ranocha", cmin=0.0, cmax=2.0, source="convergence_test",
+         t=0.4),
     Case("euler2d_hg_ranocha_blast", 2, "euler", 4, "hg", "ranocha", ic="weak_blast_wave"),
     Case("euler2d_hg_shima_blast_nosmooth", 2, "euler", 4, "hg", "shima_etal", ic="weak_blast_wave",
          alpha_smooth=False),
@@ -168,12 +191,20 @@ def test_indicator_matches_oracle(th, oracle, torch):
 @pytest.mark.parametrize("case", [Case("euler2d_fd", 2, "euler", 3, "fd", "ranocha"),
                                   Case("euler2d_hg", 2, "euler", 3, "hg", "ranocha", ic="weak_blast_wave"),
                                   Case("mhd3d", 3, "mhd", 1, "fd", "hindenlang_gassner", cmin=-1.0, cmax=1.0,
-                                       ic="convergence_test", gamma=5 / 3)],
+                                       ic="convergence_test", gamma=5 / 3),
+                                  Case("euler2d_weak_dirichlet_src", 2, "euler", 3, cmin=0.0, cmax=2.0,
+                                       ic="convergence_test", periodic=False, bc_state="convergence_test",
+                                       source="convergence_test"),
+                                  Case("euler3d_fd_dirichlet_src", 3, "euler", 2, "fd", "ranocha", cmin=0.0,
+                                       cmax=2.0, ic="convergence_test", periodic=False, bc_state="convergence_test",
+                                       source="convergence_test")],
                          ids=lambda c: c.name)
 def test_rk_step_matches_oracle(th, oracle, torch, case, use_graph):
+    """Three CK2N54 steps from t = 0 (stage times through device memory: the
+    boundary states and sources of the non-periodic cases depend on them)."""
     O = oracle
     cfg = oracle_cfg(O, case)
-    mesh = O.Mesh(case.ndims, case.level, case.cmin, case.cmax, case.polydeg)
+    mesh = oracle_mesh(O, case)
     u = O.compute_coefficients(cfg, mesh, oracle_ic(O, case), 0.0, case.seed)
     semi = product_semi(th, case)
     ud = th.pack(torch.from_numpy(u).cuda(), semi)

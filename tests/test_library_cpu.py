@@ -64,6 +64,17 @@ def test_host_treemesh_matches_oracle(th, oracle, nd, level):
     np.testing.assert_allclose(x, o.node_coordinates, rtol=0, atol=1e-15)
 
 
+@pytest.mark.parametrize("nd", [1, 2, 3])
+def test_nonperiodic_mesh_matches_oracle(th, oracle, nd):
+    lvl = 2
+    m = th.TreeMesh((0.0,) * nd, (2.0,) * nd, lvl, periodicity=False)
+    om = oracle.Mesh(nd, lvl, 0.0, 2.0, 3, periodic=False)
+    assert m.ninterfaces == om.ninterfaces == nd * (m.nelements - m.nelements // (1 << lvl))
+    np.testing.assert_array_equal(m.neighbor_ids, om.neighbor_ids)
+    np.testing.assert_array_equal(m.orientations, om.orientations)
+    assert len(m.boundary_faces) == 2 * nd * (1 << (lvl * (nd - 1)))
+
+
 def test_config_mapping_and_rejections(th):
     eq = th.CompressibleEulerEquations3D(1.4)
     s = th.DGSEMHIP(polydeg=3, surface_flux=th.flux_lax_friedrichs,
@@ -92,8 +103,18 @@ def test_config_mapping_and_rejections(th):
                                                                        volume_flux_fv=th.flux_lax_friedrichs))
     c = hg.config(eq2)
     assert (c.volume_integral, c.volume_flux, c.fv_flux, c.alpha_smooth, c.alpha_max) == (2, 3, 1, 1, 0.5)
-    with pytest.raises(NotImplementedError):
-        th.TreeMesh((-1.0,), (1.0,), 2, periodicity=False)
+    m = th.TreeMesh((-1.0,), (1.0,), 2, periodicity=False)
+    assert m.ninterfaces == 3 and m.boundary_faces.tolist() == [[0, 0], [3, 1]]
+    with pytest.raises(ValueError):  # a non-periodic mesh needs boundary conditions
+        th.SemidiscretizationHyperbolicHIP(m, th.CompressibleEulerEquations1D(1.4),
+                                           th.initial_condition_convergence_test,
+                                           th.DGSEMHIP(polydeg=3, surface_flux=th.flux_lax_friedrichs))
+    with pytest.raises(NotImplementedError):  # one Dirichlet state for all sides
+        th.SemidiscretizationHyperbolicHIP(
+            m, th.CompressibleEulerEquations1D(1.4), th.initial_condition_convergence_test,
+            th.DGSEMHIP(polydeg=3, surface_flux=th.flux_lax_friedrichs),
+            boundary_conditions={"x_neg": th.BoundaryConditionDirichlet(th.initial_condition_convergence_test),
+                                 "x_pos": th.BoundaryConditionDirichlet(th.initial_condition_constant)})
     with pytest.raises(ValueError):
         th.TreeMesh((-1.0, -1.0), (1.0, 2.0), 2)
 

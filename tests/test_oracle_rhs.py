@@ -233,6 +233,59 @@ def test_shock_capturing_jvp_matches_central_differences(oracle):
     assert np.abs(ja).max() > 1e-2
 
 
+@pytest.mark.parametrize("nd", [1, 2, 3])
+def test_source_terms_manufacture_the_convergence_test(oracle, nd):
+    """S = u_t + div f(u) at the convergence-test state (central differences
+    in x and t of the initial condition and the physical flux)."""
+    O = oracle
+    cfg = O.make_cfg(nd, O.EQ_EULER, 3, O.VI_WEAK)
+    rng = np.random.default_rng(20 + nd)
+    h = 1e-5
+    for _ in range(6):
+        x, t = rng.uniform(0, 2, nd), rng.uniform(0, 1)
+        ic = lambda xx, tt: O.initial_condition(cfg, O.IC_CONVERGENCE_TEST, xx, tt)
+        ut = (ic(x, t + h) - ic(x, t - h)) / (2 * h)
+        div = sum((O.flux(cfg, ic(x + h * np.eye(nd)[d], t), d + 1) - O.flux(cfg, ic(x - h * np.eye(nd)[d], t), d + 1))
+                  / (2 * h) for d in range(nd))
+        S = O.source_convergence_test(cfg, x, t)
+        assert np.abs(S - (ut + div)).max() < 1e-8 * max(1.0, np.abs(S).max())
+
+
+@pytest.mark.parametrize("nd", [1, 2, 3])
+def test_dirichlet_constant_state_is_preserved(oracle, nd):
+    """Constant state inside and on the boundary: du = 0 (free stream through
+    the boundary fluxes)."""
+    O = oracle
+    cfg = O.make_cfg(nd, O.EQ_EULER, 3, O.VI_FD, O.FLUX_RANOCHA, O.FLUX_LLF, boundary_condition=O.BC_DIRICHLET,
+                     boundary_ic=O.IC_CONSTANT)
+    mesh = O.Mesh(nd, 2, 0.0, 2.0, 3, periodic=False)
+    u = O.compute_coefficients(cfg, mesh, O.IC_CONSTANT)
+    assert np.abs(O.rhs(cfg, mesh, u)).max() < 1e-13
+
+
+@pytest.mark.parametrize("nd,vi", [(1, "weak"), (2, "weak"), (2, "fd"), (3, "weak")])
+def test_dirichlet_and_sources_converge_to_the_time_derivative(oracle, nd, vi):
+    """With Dirichlet states and source_terms_convergence_test, rhs!(u_exact(t))
+    approximates d/dt u_exact; the max-norm residual of polydeg 3 drops like
+    h^3 on the non-periodic mesh (a wrong boundary flux or source leaves an
+    O(1) residual that does not converge)."""
+    O = oracle
+    kind = O.VI_WEAK if vi == "weak" else O.VI_FD
+    cfg = O.make_cfg(nd, O.EQ_EULER, 3, kind, O.FLUX_RANOCHA, O.FLUX_LLF, boundary_condition=O.BC_DIRICHLET,
+                     boundary_ic=O.IC_CONVERGENCE_TEST, source_terms=O.SOURCE_CONVERGENCE_TEST, ic_length=2.0)
+    errs = []
+    t = 0.3
+    for lvl in {1: (3, 4), 2: (2, 3), 3: (2, 3)}[nd]:
+        mesh = O.Mesh(nd, lvl, 0.0, 2.0, 3, periodic=False)
+        u = O.compute_coefficients(cfg, mesh, O.IC_CONVERGENCE_TEST, t)
+        h = 1e-6
+        ut = (O.compute_coefficients(cfg, mesh, O.IC_CONVERGENCE_TEST, t + h)
+              - O.compute_coefficients(cfg, mesh, O.IC_CONVERGENCE_TEST, t - h)) / (2 * h)
+        errs.append(np.abs(O.rhs(cfg, mesh, u, t=t) - ut).max())
+    rate = np.log2(errs[0] / errs[1])
+    assert rate > 2.5, (errs, rate)
+
+
 def test_mesh_morton_order_and_connectivity(oracle):
     O = oracle
     for nd in (1, 2, 3):

@@ -15,7 +15,8 @@ This directory is not a valid Python identifier; load it as ``trixi_hip``:
 from ._lib import LIBRARY_PATH, TrixiHipError, version
 from .callbacks import (AnalysisCallback, StepsizeCallback, blending_factors, calc_error_norms, check_state,
                         integrate, max_dt)
-from .equations import (CompressibleEulerEquations1D, CompressibleEulerEquations2D, CompressibleEulerEquations3D,
+from .equations import (BoundaryConditionDirichlet, boundary_condition_periodic, source_terms_convergence_test,
+                        CompressibleEulerEquations1D, CompressibleEulerEquations2D, CompressibleEulerEquations3D,
                         FluxLaxFriedrichs, IdealGlmMhdEquations3D, LinearScalarAdvectionEquation1D,
                         LinearScalarAdvectionEquation2D, LinearScalarAdvectionEquation3D, density_pressure,
                         flux_central, flux_hindenlang_gassner, flux_lax_friedrichs, flux_nonconservative_powell,

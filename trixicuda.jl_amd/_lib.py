@@ -15,6 +15,8 @@ EQ_LINADV, EQ_EULER, EQ_GLM_MHD = 0, 1, 2
 VI_WEAK, VI_FD, VI_HG = 0, 1, 2
 FLUX_CENTRAL, FLUX_LLF, FLUX_LLF_NAIVE, FLUX_RANOCHA, FLUX_SHIMA_ETAL, FLUX_HINDENLANG_GASSNER = range(6)
 PREC_FP64, PREC_MIXED = 0, 1
+BC_PERIODIC, BC_DIRICHLET = 0, 1
+SOURCE_NONE, SOURCE_CONVERGENCE_TEST = 0, 1
 (IC_CONSTANT, IC_CONVERGENCE_TEST, IC_WEAK_BLAST_WAVE, IC_SEEDED_RANDOM, IC_SMOOTH_PERIODIC,
  IC_WEAK_BLAST_WAVE_SEEDED) = range(6)
 
@@ -43,6 +45,11 @@ class Config(ctypes.Structure):
         ("alpha_max", ctypes.c_double),
         ("alpha_min", ctypes.c_double),
         ("ic_length", ctypes.c_double),
+        ("boundary_condition", ctypes.c_int32),
+        ("boundary_ic", ctypes.c_int32),
+        ("boundary_seed", ctypes.c_uint64),
+        ("source_terms", ctypes.c_int32),
+        ("reserved2", ctypes.c_int32),
     ]
 
 

@@ -1,0 +1,199 @@
+// initial_conditions.hpp -- initial conditions on the device: the Trixi
+// initial_condition_* functions used by the reference tests and examples
+// (also the Dirichlet boundary states), plus the seeded benchmark states.
+#pragma once
+
+#include "../../include/trixi_hip.h"
+#include "physics.hpp"
+
+namespace trixi_hip {
+
+// ---------------------------------------------------------------------------
+// counter-based generator for the seeded initial states (splitmix64)
+// ---------------------------------------------------------------------------
+__host__ __device__ inline uint64_t splitmix64(uint64_t x) {
+    uint64_t z = x + 0x9E3779B97F4A7C15ULL;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+}
+__host__ __device__ inline double u01(uint64_t seed, uint64_t index, uint64_t stream) {
+    const uint64_t r = splitmix64(splitmix64(seed) ^ (index * 16 + stream));
+    return ((double)(r >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+}
+__device__ inline double normal01(uint64_t seed, uint64_t index, uint64_t stream) {
+    const double a = u01(seed, index, stream), b = u01(seed, index, stream + 1);
+    return sqrt(-2.0 * log(a)) * cos(2.0 * M_PI * b);
+}
+
+struct IcParams {
+    int dim, eq, nv, ic;
+    double gamma, inv_gm1, c_h, a[3], length;
+    uint64_t seed;
+};
+
+__device__ inline double smooth_field(uint64_t seed, int field, int nd, const double* x, double L,
+                               bool use_cos) {
+    double s = 0.0;
+    for (int m = 0; m < 3; ++m) {
+        const uint64_t base = 0xFFFFFFFF00000000ULL + (uint64_t)(field * 3 + m) * 16;
+        double arg = 0.0;
+        for (int d = 0; d < nd; ++d) {
+            const int k = (int)floor(u01(seed, base, d) * 7.0) - 3;
+            arg += k * x[d];
+        }
+        const double amp = (2.0 * u01(seed, base, 3) - 1.0) / 3.0;
+        const double ph = 2.0 * M_PI * u01(seed, base, 4);
+        const double th = 2.0 * M_PI * arg / L + ph;
+        s += amp * (use_cos ? cos(th) : sin(th));
+    }
+    return s;
+}
+
+// Initial conditions (Trixi initial_condition_* functions used by the
+// reference tests and examples, plus the seeded benchmark states).
+__device__ inline void initial_condition(const IcParams& q, const double* x, double t, uint64_t idx,
+                                  double* u) {
+    if (q.eq == EQ_LINADV) {
+        if (q.ic == TRIXI_HIP_IC_CONVERGENCE_TEST) {
+            double xs = 0.0;
+            for (int d = 0; d < q.dim; ++d) xs += x[d] - q.a[d] * t;
+            const double omega = 2.0 * M_PI * (1.0 / 2.0);
+            u[0] = 1.0 + 0.5 * sin(omega * xs);
+        } else if (q.ic == TRIXI_HIP_IC_SEEDED_RANDOM) {
+            u[0] = 1.0 + 0.1 * (2.0 * u01(q.seed, idx, 0) - 1.0);
+        } else {
+            u[0] = 2.0;
+        }
+        return;
+    }
+    if (q.eq == EQ_EULER) {
+        double v[3] = {0, 0, 0}, rho = 1.0, p = 1.0;
+        if (q.ic == TRIXI_HIP_IC_WEAK_BLAST_WAVE || q.ic == TRIXI_HIP_IC_WEAK_BLAST_WAVE_SEEDED) {
+            double r2 = 0.0;
+            for (int d = 0; d < q.dim; ++d) r2 += x[d] * x[d];
+            const double r = sqrt(r2);
+            const bool inside = !(r > 0.5);
+            if (q.dim == 1) {
+                const double phi = x[0] > 0 ? 0.0 : M_PI;
+                v[0] = inside ? 0.1882 * cos(phi) : 0.0;
+            } else if (q.dim == 2) {
+                const double phi = atan2(x[1], x[0]);
+                v[0] = inside ? 0.1882 * cos(phi) : 0.0;
+                v[1] = inside ? 0.1882 * sin(phi) : 0.0;
+            } else {
+                const double phi = atan2(x[1], x[0]);
+                const double theta = (r == 0.0) ? 0.0 : acos(x[2] / r);
+                v[0] = inside ? 0.1882 * cos(phi) * sin(theta) : 0.0;
+                v[1] = inside ? 0.1882 * sin(phi) * sin(theta) : 0.0;
+                v[2] = inside ? 0.1882 * cos(theta) : 0.0;
+            }
+            rho = inside ? 1.1691 : 1.0;
+            p = inside ? 1.245 : 1.0;
+            if (q.ic == TRIXI_HIP_IC_WEAK_BLAST_WAVE_SEEDED) {
+                // breaks the exact symmetries (v = 0, equal neighbour states)
+                rho += 0.01 * (2.0 * u01(q.seed, idx, 0) - 1.0);
+                p += 0.01 * (2.0 * u01(q.seed, idx, 1) - 1.0);
+                for (int d = 0; d < q.dim; ++d) v[d] += 0.01 * (2.0 * u01(q.seed, idx, 2 + d) - 1.0);
+            }
+        } else if (q.ic == TRIXI_HIP_IC_SEEDED_RANDOM) {
+            rho = 1.0 + 0.1 * (2.0 * u01(q.seed, idx, 0) - 1.0);
+            p = 1.0 + 0.05 * (2.0 * u01(q.seed, idx, 1) - 1.0);
+            for (int d = 0; d < q.dim; ++d) v[d] = 0.1 * normal01(q.seed, idx, 2 + 2 * d);
+        } else if (q.ic == TRIXI_HIP_IC_SMOOTH_PERIODIC) {
+            const double L = q.length > 0 ? q.length : 4.0;
+            rho = 1.0 + 0.1 * smooth_field(q.seed, 0, q.dim, x, L, false);
+            for (int d = 0; d < q.dim; ++d) v[d] = 0.1 * smooth_field(q.seed, 1 + d, q.dim, x, L, false);
+            p = 1.0 + 0.05 * smooth_field(q.seed, 4, q.dim, x, L, true);
+        } else if (q.ic == TRIXI_HIP_IC_CONVERGENCE_TEST) {
+            double xs = 0.0;
+            for (int d = 0; d < q.dim; ++d) xs += x[d];
+            const double ini = 2.0 + 0.1 * sin(M_PI * (xs - t));
+            u[0] = ini;
+            for (int d = 0; d < q.dim; ++d) u[1 + d] = ini;
+            u[q.dim + 1] = ini * ini;
+            return;
+        }
+        u[0] = rho;
+        double ke = 0.0;
+        for (int d = 0; d < q.dim; ++d) {
+            u[1 + d] = rho * v[d];
+            ke = (d == 0) ? u[1 + d] * v[d] : ke + u[1 + d] * v[d];
+        }
+        u[q.dim + 1] = p * q.inv_gm1 + 0.5 * ke;
+        return;
+    }
+    double prim[9] = {1.0, 0, 0, 0, 1.0, 0, 0, 0, 0};
+    if (q.ic == TRIXI_HIP_IC_CONVERGENCE_TEST) {
+        const double omega = 2.0 * M_PI, r = 2.0, e = 0.2;
+        const double nx = 1.0 / sqrt(r * r + 1.0), ny = r / sqrt(r * r + 1.0), sqr = 1.0;
+        const double Va = omega / (ny * sqr);
+        const double phi_alv = omega / ny * (nx * (x[0] - 0.5 * r) + ny * (x[1] - 0.5 * r)) - Va * t;
+        const double rho = 1.0;
+        const double v1 = -e * ny * cos(phi_alv) / rho;
+        const double v2 = e * nx * cos(phi_alv) / rho;
+        const double v3 = e * sin(phi_alv) / rho;
+        prim[0] = rho;
+        prim[1] = v1;
+        prim[2] = v2;
+        prim[3] = v3;
+        prim[4] = 1.0;
+        prim[5] = nx - rho * v1 * sqr;
+        prim[6] = ny - rho * v2 * sqr;
+        prim[7] = -rho * v3 * sqr;
+        prim[8] = 0.0;
+    } else if (q.ic == TRIXI_HIP_IC_SEEDED_RANDOM) {
+        prim[0] = 1.0 + 0.1 * (2.0 * u01(q.seed, idx, 0) - 1.0);
+        prim[4] = 1.0 + 0.05 * (2.0 * u01(q.seed, idx, 1) - 1.0);
+        for (int d = 0; d < 3; ++d) prim[1 + d] = 0.1 * normal01(q.seed, idx, 2 + 2 * d);
+        for (int d = 0; d < 3; ++d) prim[5 + d] = 0.5 + 0.1 * (2.0 * u01(q.seed, idx, 8 + d) - 1.0);
+        prim[8] = 0.01 * (2.0 * u01(q.seed, idx, 11) - 1.0);
+    }
+    const double rho = prim[0], p = prim[4], psi = prim[8];
+    u[0] = rho;
+    for (int d = 0; d < 3; ++d) u[1 + d] = rho * prim[1 + d];
+    u[4] = p * q.inv_gm1 + 0.5 * (u[1] * prim[1] + u[2] * prim[2] + u[3] * prim[3]) +
+           0.5 * (prim[5] * prim[5] + prim[6] * prim[6] + prim[7] * prim[7]) + 0.5 * psi * psi;
+    for (int d = 0; d < 3; ++d) u[5 + d] = prim[5 + d];
+    u[8] = psi;
+}
+
+
+// Dirichlet boundary states built into the non-periodic kernels: the
+// convergence-test state of compressible Euler (initial_condition_convergence_test)
+// and the constant state; the library rejects other boundary states.
+__host__ __device__ inline void boundary_value(const IcParams& q, const double* x, double t, double* u) {
+    if (q.ic == TRIXI_HIP_IC_CONVERGENCE_TEST) {
+        double xs = 0.0;
+        for (int d = 0; d < q.dim; ++d) xs += x[d];
+        const double ini = 2.0 + 0.1 * sin(M_PI * (xs - t));
+        u[0] = ini;
+        for (int d = 0; d < q.dim; ++d) u[1 + d] = ini;
+        u[q.dim + 1] = ini * ini;
+    } else {
+        u[0] = 1.0;
+        for (int d = 0; d < q.dim; ++d) u[1 + d] = 0.0;
+        u[q.dim + 1] = q.inv_gm1;
+    }
+}
+
+// source_terms_convergence_test (compressible Euler, d dimensions): the
+// source that makes initial_condition_convergence_test,
+//   rho = rho v_k = g,  rho e = g^2,  g = 2 + 0.1 sin(pi (sum_k x_k - t)),
+// an exact solution.  With g' = 0.1 pi cos(pi (sum_k x_k - t)), v_k = 1 and
+// p = (gamma - 1)(g^2 - d g / 2), S = d_t u + div f(u) gives
+//   S_rho = (d - 1) g',  S_{rho v_k} = (d - 1) g' + p',
+//   S_{rho e} = 2 (d - 1) g g' + d p',  p' = (gamma - 1)(2 g - d / 2) g'.
+__host__ __device__ inline void source_convergence_test(int dim, double gm1, const double* x, double t, double* s) {
+    double xs = 0.0;
+    for (int d = 0; d < dim; ++d) xs += x[d];
+    const double arg = M_PI * (xs - t);
+    const double g = 2.0 + 0.1 * sin(arg);
+    const double gp = 0.1 * M_PI * cos(arg);
+    const double pp = gm1 * (2.0 * g - 0.5 * dim) * gp;
+    s[0] = (dim - 1) * gp;
+    for (int d = 0; d < dim; ++d) s[1 + d] = (dim - 1) * gp + pp;
+    s[dim + 1] = 2.0 * (dim - 1) * g * gp + dim * pp;
+}
+
+}  // namespace trixi_hip

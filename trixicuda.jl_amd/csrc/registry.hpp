@@ -8,10 +8,11 @@ namespace trixi_hip {
 
 struct KernelKey {
     int dim, N, eq, vol, vf, sf, fvf, prec, rk;
+    int ext = 0;  // 1: boundary faces and source terms (non-periodic meshes)
     bool operator<(const KernelKey& o) const {
-        const int a[9] = {dim, N, eq, vol, vf, sf, fvf, prec, rk};
-        const int b[9] = {o.dim, o.N, o.eq, o.vol, o.vf, o.sf, o.fvf, o.prec, o.rk};
-        for (int i = 0; i < 9; ++i)
+        const int a[10] = {dim, N, eq, vol, vf, sf, fvf, prec, rk, ext};
+        const int b[10] = {o.dim, o.N, o.eq, o.vol, o.vf, o.sf, o.fvf, o.prec, o.rk, o.ext};
+        for (int i = 0; i < 10; ++i)
             if (a[i] != b[i]) return a[i] < b[i];
         return false;
     }
@@ -42,11 +43,12 @@ constexpr int pick_m_vjp() {
     return m;
 }
 
-template <int DIM, int N, class EQ, int VOL, int VF, int SF, int FVF = FLUX_CENTRAL>
+template <int DIM, int N, class EQ, int VOL, int VF, int SF, int FVF = FLUX_CENTRAL, bool EXT = false>
 struct VjpRegistrar {
     static void add(int eq) {
         constexpr int M = pick_m_vjp<DIM, N, EQ::NV, EQ::NRF>();
-        register_vjp({DIM, N, eq, VOL, VF, SF, FVF, 0, 0}, &launch_vjp<DIM, N, EQ, VOL, VF, SF, FVF, double, M>);
+        register_vjp({DIM, N, eq, VOL, VF, SF, FVF, 0, 0, EXT ? 1 : 0},
+                     &launch_vjp<DIM, N, EQ, VOL, VF, SF, FVF, double, M, EXT>);
     }
 };
 
@@ -85,6 +87,23 @@ struct Registrar {
                         &launch_rhs<DIM, N, EQ, VOL, VF, SF, FVF, float, MF, false>);
         register_kernel({DIM, N, eq, VOL, VF, SF, FVF, 1, 1},
                         &launch_rhs<DIM, N, EQ, VOL, VF, SF, FVF, float, MF, true>);
+    }
+};
+
+// The same four variants with boundary faces and source terms.
+template <int DIM, int N, class EQ, int VOL, int VF, int SF, int FVF>
+struct RegistrarExt {
+    static void add(int eq) {
+        constexpr int MD = pick_m<DIM, N, EQ::NV, EQ::NRF, 8>();
+        constexpr int MF = pick_m<DIM, N, EQ::NV, EQ::NRF, 4>();
+        register_kernel({DIM, N, eq, VOL, VF, SF, FVF, 0, 0, 1},
+                        &launch_rhs<DIM, N, EQ, VOL, VF, SF, FVF, double, MD, false, true>);
+        register_kernel({DIM, N, eq, VOL, VF, SF, FVF, 0, 1, 1},
+                        &launch_rhs<DIM, N, EQ, VOL, VF, SF, FVF, double, MD, true, true>);
+        register_kernel({DIM, N, eq, VOL, VF, SF, FVF, 1, 0, 1},
+                        &launch_rhs<DIM, N, EQ, VOL, VF, SF, FVF, float, MF, false, true>);
+        register_kernel({DIM, N, eq, VOL, VF, SF, FVF, 1, 1, 1},
+                        &launch_rhs<DIM, N, EQ, VOL, VF, SF, FVF, float, MF, true, true>);
     }
 };
 

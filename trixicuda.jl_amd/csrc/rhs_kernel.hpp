@@ -24,6 +24,7 @@
 
 #include <type_traits>
 
+#include "initial_conditions.hpp"
 #include "physics.hpp"
 
 namespace trixi_hip {
@@ -72,7 +73,38 @@ struct RhsArgs {
     double rk_a, rk_b;        // RK stage coefficients (rk_a unused when rk_first)
     int rk_first;
     int nxcd;                 // XCDs the dispatcher deals workgroups to round robin
+    // non-periodic meshes and source terms
+    const double* x;          // node coordinates x[(d*NN + node)*E + e]
+    IcParams bc_ic;           // Dirichlet boundary state (faces with nbr < 0)
+    int bc;                   // enum trixi_hip_boundary_condition
+    int src;                  // enum trixi_hip_source_terms
+    double t_host;            // rhs mode: time t
 };
+
+// Dirichlet state boundary_ic(x, t) at node `node` of element e (its face
+// node on a boundary face), in R.
+template <int DIM, int NN, int NV, typename R>
+__device__ __forceinline__ void boundary_state(const RhsArgs& a, int64_t e, int node, double t, R* ub) {
+    double xx[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) xx[d] = a.x[((int64_t)d * NN + node) * a.E + e];
+    double uu[5];
+    boundary_value(a.bc_ic, xx, t, uu);
+#pragma unroll
+    for (int v = 0; v < NV; ++v) ub[v] = (R)uu[v];
+}
+
+// source_terms_convergence_test at node `node` of element e, FP64.
+template <int DIM, int NN, int NV>
+__device__ __forceinline__ void source_at(const RhsArgs& a, int64_t e, int node, double t, double* sv) {
+    double xx[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) xx[d] = a.x[((int64_t)d * NN + node) * a.E + e];
+    double s5[5];
+    source_convergence_test(DIM, a.phys.gm1, xx, t, s5);
+#pragma unroll
+    for (int v = 0; v < NV; ++v) sv[v] = s5[v];
+}
 
 // Workgroup -> element tile.  The dispatcher deals consecutive workgroups
 // to the XCDs round robin, and each XCD has its own L2; remapping so that
@@ -102,7 +134,9 @@ __device__ __forceinline__ int line_base(int line, int d) {
     return base;
 }
 
-template <int DIM, int N, class EQ, int VOL, int VF, int SF, int FVF, typename R, int M, bool RK>
+// EXT: the variant with boundary faces (nbr < 0, Dirichlet states) and source
+// terms; the periodic, source-free kernels carry none of that code.
+template <int DIM, int N, class EQ, int VOL, int VF, int SF, int FVF, typename R, int M, bool RK, bool EXT>
 __global__ void __launch_bounds__(M * cpow(N, DIM - 1), TRIXI_HIP_MIN_WAVES)
 rhs_kernel(const RhsArgs a) {
     constexpr int NL = cpow(N, DIM - 1);
@@ -145,16 +179,24 @@ rhs_kernel(const RhsArgs a) {
     // Face neighbours of all 2*DIM faces, and the neighbour face data of the
     // first direction, are requested before the node pass so that their
     // latency overlaps it; later directions are prefetched one phase ahead.
-    int32_t nb[2 * DIM];
+    // A face with nbr < 0 is a boundary face (non-periodic mesh): its loads
+    // read this element instead and the state is replaced by the boundary
+    // state in the direction phase.
+    int32_t nb[2 * DIM], nbl[2 * DIM];
 #pragma unroll
-    for (int f = 0; f < 2 * DIM; ++f) nb[f] = a.nbr[(int64_t)f * E + ec];
+    for (int f = 0; f < 2 * DIM; ++f) {
+        nb[f] = a.nbr[(int64_t)f * E + ec];
+        nbl[f] = (!EXT || nb[f] >= 0) ? nb[f] : (int32_t)ec;
+    }
+    double tnow = 0.0;
+    if constexpr (EXT) tnow = RK ? a.time[0] : a.t_host;
     double um_next[NV], up_next[NV];
     {
         const int base0 = line_base<DIM, N>(line, 0);
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
-            um_next[v] = a.u[(int64_t)(v * NN + base0 + (N - 1)) * E + nb[0]];
-            up_next[v] = a.u[(int64_t)(v * NN + base0) * E + nb[1]];
+            um_next[v] = a.u[(int64_t)(v * NN + base0 + (N - 1)) * E + nbl[0]];
+            up_next[v] = a.u[(int64_t)(v * NN + base0) * E + nbl[1]];
         }
     }
 
@@ -198,6 +240,7 @@ rhs_kernel(const RhsArgs a) {
         if (a.smooth) {
 #pragma unroll
             for (int s = 0; s < 2 * DIM; ++s) {
+                if (EXT && nb[s] < 0) continue;  // smoothing runs over interfaces only
                 const double an = 0.5 * a.alpha_raw[nb[s]];
                 al = an > al ? an : al;
             }
@@ -216,13 +259,17 @@ rhs_kernel(const RhsArgs a) {
             um[v] = (R)um_next[v];
             up[v] = (R)up_next[v];
         }
+        if (EXT && a.bc) {
+            if (nb[2 * d] < 0) boundary_state<DIM, NN, NV, R>(a, ec, base, tnow, um);
+            if (nb[2 * d + 1] < 0) boundary_state<DIM, NN, NV, R>(a, ec, base + (N - 1) * s, tnow, up);
+        }
         if (d + 1 < DIM) {
             const int sn = cpow(N, d + 1);
             const int bn = line_base<DIM, N>(line, d + 1);
 #pragma unroll
             for (int v = 0; v < NV; ++v) {
-                um_next[v] = a.u[(int64_t)(v * NN + bn + (N - 1) * sn) * E + nb[2 * (d + 1)]];
-                up_next[v] = a.u[(int64_t)(v * NN + bn) * E + nb[2 * (d + 1) + 1]];
+                um_next[v] = a.u[(int64_t)(v * NN + bn + (N - 1) * sn) * E + nbl[2 * (d + 1)]];
+                up_next[v] = a.u[(int64_t)(v * NN + bn) * E + nbl[2 * (d + 1) + 1]];
             }
         }
 
@@ -378,14 +425,18 @@ rhs_kernel(const RhsArgs a) {
             double t_dt = 0.0;
             if (RK) t_dt = a.time[1];
 #pragma unroll
-            for (int p = 0; p < N; ++p)
+            for (int p = 0; p < N; ++p) {
+                const int node = base + p * s;
+                double sv[NV];
+#pragma unroll
+                for (int v = 0; v < NV; ++v) sv[v] = 0.0;
+                if (EXT && a.src) source_at<DIM, NN, NV>(a, e, node, tnow, sv);
 #pragma unroll
                 for (int v = 0; v < NV; ++v) {
-                    const int node = base + p * s;
                     R total = acl[p][v];
                     if (DIM > 1) total = acc[(v * NN + node) * LDM + el] + total;
                     const int64_t g = (int64_t)(v * NN + node) * E + e;
-                    const double val = (double)total * fac;
+                    const double val = (double)total * fac + sv[v];
                     if (!RK) {
                         a.du[g] = val;
                     } else {
@@ -394,6 +445,7 @@ rhs_kernel(const RhsArgs a) {
                         a.u_out[g] = a.u[g] + a.rk_b * kk;
                     }
                 }
+            }
         }
     }
 
@@ -410,10 +462,14 @@ rhs_kernel(const RhsArgs a) {
             const int64_t ee = e0 + c;
             if (ee >= E) continue;
             const double fac = -a.invJ[ee];
+            double sv[NV];
+#pragma unroll
+            for (int v = 0; v < NV; ++v) sv[v] = 0.0;
+            if (EXT && a.src) source_at<DIM, NN, NV>(a, ee, n, tnow, sv);
 #pragma unroll
             for (int v = 0; v < NV; ++v) {
                 const int64_t g = (int64_t)(v * NN + n) * E + ee;
-                const double val = (double)acc[(v * NN + n) * LDM + c] * fac;
+                const double val = (double)acc[(v * NN + n) * LDM + c] * fac + sv[v];
                 const double kk = a.rk_first ? t_dt * val : a.rk_a * kkeep[it][v] + t_dt * val;
                 a.k[g] = kk;
                 a.u_out[g] = ukeep[it][v] + a.rk_b * kk;
@@ -423,12 +479,12 @@ rhs_kernel(const RhsArgs a) {
 }
 
 // Launch helper used by the instantiation units.
-template <int DIM, int N, class EQ, int VOL, int VF, int SF, int FVF, typename R, int M, bool RK>
+template <int DIM, int N, class EQ, int VOL, int VF, int SF, int FVF, typename R, int M, bool RK, bool EXT = false>
 hipError_t launch_rhs(const RhsArgs& a, hipStream_t stream) {
     constexpr int NL = cpow(N, DIM - 1);
     int64_t blocks = (a.E + M - 1) / M;
     if (a.nxcd > 1) blocks = (blocks + a.nxcd - 1) / a.nxcd * a.nxcd;
-    hipLaunchKernelGGL((rhs_kernel<DIM, N, EQ, VOL, VF, SF, FVF, R, M, RK>), dim3((unsigned)blocks),
+    hipLaunchKernelGGL((rhs_kernel<DIM, N, EQ, VOL, VF, SF, FVF, R, M, RK, EXT>), dim3((unsigned)blocks),
                        dim3(M * NL), 0, stream, a);
     return hipGetLastError();
 }

@@ -14,6 +14,7 @@
 #include <vector>
 
 #include "registry.hpp"
+#include "initial_conditions.hpp"
 
 namespace trixi_hip {
 
@@ -52,6 +53,8 @@ void register_vjp_euler1d();
 void register_vjp_euler2d();
 void register_vjp_euler3d();
 void register_vjp_mhd3d();
+void register_euler_ext();
+void register_vjp_euler_ext();
 
 static void ensure_registered() {
     static std::once_flag once;
@@ -69,6 +72,8 @@ static void ensure_registered() {
         register_vjp_euler2d();
         register_vjp_euler3d();
         register_vjp_mhd3d();
+        register_euler_ext();
+        register_vjp_euler_ext();
     });
 }
 
@@ -248,156 +253,6 @@ static HostBasis make_basis(int N) {
 }
 
 // ---------------------------------------------------------------------------
-// counter-based generator for the seeded initial states (splitmix64)
-// ---------------------------------------------------------------------------
-__host__ __device__ inline uint64_t splitmix64(uint64_t x) {
-    uint64_t z = x + 0x9E3779B97F4A7C15ULL;
-    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
-    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
-    return z ^ (z >> 31);
-}
-__host__ __device__ inline double u01(uint64_t seed, uint64_t index, uint64_t stream) {
-    const uint64_t r = splitmix64(splitmix64(seed) ^ (index * 16 + stream));
-    return ((double)(r >> 11) + 0.5) * (1.0 / 9007199254740992.0);
-}
-__device__ inline double normal01(uint64_t seed, uint64_t index, uint64_t stream) {
-    const double a = u01(seed, index, stream), b = u01(seed, index, stream + 1);
-    return sqrt(-2.0 * log(a)) * cos(2.0 * M_PI * b);
-}
-
-struct IcParams {
-    int dim, eq, nv, ic;
-    double gamma, inv_gm1, c_h, a[3], length;
-    uint64_t seed;
-};
-
-__device__ double smooth_field(uint64_t seed, int field, int nd, const double* x, double L,
-                               bool use_cos) {
-    double s = 0.0;
-    for (int m = 0; m < 3; ++m) {
-        const uint64_t base = 0xFFFFFFFF00000000ULL + (uint64_t)(field * 3 + m) * 16;
-        double arg = 0.0;
-        for (int d = 0; d < nd; ++d) {
-            const int k = (int)floor(u01(seed, base, d) * 7.0) - 3;
-            arg += k * x[d];
-        }
-        const double amp = (2.0 * u01(seed, base, 3) - 1.0) / 3.0;
-        const double ph = 2.0 * M_PI * u01(seed, base, 4);
-        const double th = 2.0 * M_PI * arg / L + ph;
-        s += amp * (use_cos ? cos(th) : sin(th));
-    }
-    return s;
-}
-
-// Initial conditions (Trixi initial_condition_* functions used by the
-// reference tests and examples, plus the seeded benchmark states).
-__device__ void initial_condition(const IcParams& q, const double* x, double t, uint64_t idx,
-                                  double* u) {
-    if (q.eq == EQ_LINADV) {
-        if (q.ic == TRIXI_HIP_IC_CONVERGENCE_TEST) {
-            double xs = 0.0;
-            for (int d = 0; d < q.dim; ++d) xs += x[d] - q.a[d] * t;
-            const double omega = 2.0 * M_PI * (1.0 / 2.0);
-            u[0] = 1.0 + 0.5 * sin(omega * xs);
-        } else if (q.ic == TRIXI_HIP_IC_SEEDED_RANDOM) {
-            u[0] = 1.0 + 0.1 * (2.0 * u01(q.seed, idx, 0) - 1.0);
-        } else {
-            u[0] = 2.0;
-        }
-        return;
-    }
-    if (q.eq == EQ_EULER) {
-        double v[3] = {0, 0, 0}, rho = 1.0, p = 1.0;
-        if (q.ic == TRIXI_HIP_IC_WEAK_BLAST_WAVE || q.ic == TRIXI_HIP_IC_WEAK_BLAST_WAVE_SEEDED) {
-            double r2 = 0.0;
-            for (int d = 0; d < q.dim; ++d) r2 += x[d] * x[d];
-            const double r = sqrt(r2);
-            const bool inside = !(r > 0.5);
-            if (q.dim == 1) {
-                const double phi = x[0] > 0 ? 0.0 : M_PI;
-                v[0] = inside ? 0.1882 * cos(phi) : 0.0;
-            } else if (q.dim == 2) {
-                const double phi = atan2(x[1], x[0]);
-                v[0] = inside ? 0.1882 * cos(phi) : 0.0;
-                v[1] = inside ? 0.1882 * sin(phi) : 0.0;
-            } else {
-                const double phi = atan2(x[1], x[0]);
-                const double theta = (r == 0.0) ? 0.0 : acos(x[2] / r);
-                v[0] = inside ? 0.1882 * cos(phi) * sin(theta) : 0.0;
-                v[1] = inside ? 0.1882 * sin(phi) * sin(theta) : 0.0;
-                v[2] = inside ? 0.1882 * cos(theta) : 0.0;
-            }
-            rho = inside ? 1.1691 : 1.0;
-            p = inside ? 1.245 : 1.0;
-            if (q.ic == TRIXI_HIP_IC_WEAK_BLAST_WAVE_SEEDED) {
-                // breaks the exact symmetries (v = 0, equal neighbour states)
-                rho += 0.01 * (2.0 * u01(q.seed, idx, 0) - 1.0);
-                p += 0.01 * (2.0 * u01(q.seed, idx, 1) - 1.0);
-                for (int d = 0; d < q.dim; ++d) v[d] += 0.01 * (2.0 * u01(q.seed, idx, 2 + d) - 1.0);
-            }
-        } else if (q.ic == TRIXI_HIP_IC_SEEDED_RANDOM) {
-            rho = 1.0 + 0.1 * (2.0 * u01(q.seed, idx, 0) - 1.0);
-            p = 1.0 + 0.05 * (2.0 * u01(q.seed, idx, 1) - 1.0);
-            for (int d = 0; d < q.dim; ++d) v[d] = 0.1 * normal01(q.seed, idx, 2 + 2 * d);
-        } else if (q.ic == TRIXI_HIP_IC_SMOOTH_PERIODIC) {
-            const double L = q.length > 0 ? q.length : 4.0;
-            rho = 1.0 + 0.1 * smooth_field(q.seed, 0, q.dim, x, L, false);
-            for (int d = 0; d < q.dim; ++d) v[d] = 0.1 * smooth_field(q.seed, 1 + d, q.dim, x, L, false);
-            p = 1.0 + 0.05 * smooth_field(q.seed, 4, q.dim, x, L, true);
-        } else if (q.ic == TRIXI_HIP_IC_CONVERGENCE_TEST) {
-            double xs = 0.0;
-            for (int d = 0; d < q.dim; ++d) xs += x[d];
-            const double ini = 2.0 + 0.1 * sin(M_PI * (xs - t));
-            u[0] = ini;
-            for (int d = 0; d < q.dim; ++d) u[1 + d] = ini;
-            u[q.dim + 1] = ini * ini;
-            return;
-        }
-        u[0] = rho;
-        double ke = 0.0;
-        for (int d = 0; d < q.dim; ++d) {
-            u[1 + d] = rho * v[d];
-            ke = (d == 0) ? u[1 + d] * v[d] : ke + u[1 + d] * v[d];
-        }
-        u[q.dim + 1] = p * q.inv_gm1 + 0.5 * ke;
-        return;
-    }
-    double prim[9] = {1.0, 0, 0, 0, 1.0, 0, 0, 0, 0};
-    if (q.ic == TRIXI_HIP_IC_CONVERGENCE_TEST) {
-        const double omega = 2.0 * M_PI, r = 2.0, e = 0.2;
-        const double nx = 1.0 / sqrt(r * r + 1.0), ny = r / sqrt(r * r + 1.0), sqr = 1.0;
-        const double Va = omega / (ny * sqr);
-        const double phi_alv = omega / ny * (nx * (x[0] - 0.5 * r) + ny * (x[1] - 0.5 * r)) - Va * t;
-        const double rho = 1.0;
-        const double v1 = -e * ny * cos(phi_alv) / rho;
-        const double v2 = e * nx * cos(phi_alv) / rho;
-        const double v3 = e * sin(phi_alv) / rho;
-        prim[0] = rho;
-        prim[1] = v1;
-        prim[2] = v2;
-        prim[3] = v3;
-        prim[4] = 1.0;
-        prim[5] = nx - rho * v1 * sqr;
-        prim[6] = ny - rho * v2 * sqr;
-        prim[7] = -rho * v3 * sqr;
-        prim[8] = 0.0;
-    } else if (q.ic == TRIXI_HIP_IC_SEEDED_RANDOM) {
-        prim[0] = 1.0 + 0.1 * (2.0 * u01(q.seed, idx, 0) - 1.0);
-        prim[4] = 1.0 + 0.05 * (2.0 * u01(q.seed, idx, 1) - 1.0);
-        for (int d = 0; d < 3; ++d) prim[1 + d] = 0.1 * normal01(q.seed, idx, 2 + 2 * d);
-        for (int d = 0; d < 3; ++d) prim[5 + d] = 0.5 + 0.1 * (2.0 * u01(q.seed, idx, 8 + d) - 1.0);
-        prim[8] = 0.01 * (2.0 * u01(q.seed, idx, 11) - 1.0);
-    }
-    const double rho = prim[0], p = prim[4], psi = prim[8];
-    u[0] = rho;
-    for (int d = 0; d < 3; ++d) u[1 + d] = rho * prim[1 + d];
-    u[4] = p * q.inv_gm1 + 0.5 * (u[1] * prim[1] + u[2] * prim[2] + u[3] * prim[3]) +
-           0.5 * (prim[5] * prim[5] + prim[6] * prim[6] + prim[7] * prim[7]) + 0.5 * psi * psi;
-    for (int d = 0; d < 3; ++d) u[5 + d] = prim[5 + d];
-    u[8] = psi;
-}
-
-// ---------------------------------------------------------------------------
 // auxiliary kernels
 // ---------------------------------------------------------------------------
 __global__ void pack_kernel(double* __restrict__ dst, const double* __restrict__ src, int nv, int nn,
@@ -436,8 +291,6 @@ __global__ void coefficients_kernel(double* __restrict__ u, const double* __rest
     }
 }
 
-// Hennemann-Gassner indicator, one element per thread, FP64 (Trixi
-// calc_indicator_hennemann_gassner! with variable density_pressure).
 // Hennemann-Gassner blending factor, one lane per node: 64-lane blocks of
 // 64 / N^d elements.  The dimension-wise modal transform goes through LDS;
 // the element's first lane sums the modal energies in node order (the order
@@ -529,7 +382,9 @@ __global__ void smooth_alpha_kernel(const double* __restrict__ raw, double* __re
     if (e >= E) return;
     double a = raw[e];
     for (int s = 0; s < 2 * dim; ++s) {
-        const double an = 0.5 * raw[nbr[(int64_t)s * E + e]];
+        const int32_t nb = nbr[(int64_t)s * E + e];
+        if (nb < 0) continue;  // boundary face: smoothing runs over interfaces only
+        const double an = 0.5 * raw[nb];
         a = an > a ? an : a;
     }
     out[e] = a;
@@ -570,7 +425,7 @@ __global__ void __launch_bounds__(64) indicator_vjp_kernel(
                 ab = winner[e] < 0 ? alphabar[e] : 0.0;
                 for (int f = 0; f < 2 * DIM; ++f) {
                     const int32_t nb = nbr[(int64_t)f * E + e];
-                    if (winner[nb] == (f ^ 1)) ab += 0.5 * alphabar[nb];
+                    if (nb >= 0 && winner[nb] == (f ^ 1)) ab += 0.5 * alphabar[nb];
                 }
             } else {
                 ab = alphabar[e];
@@ -925,6 +780,7 @@ static KernelKey make_key(const trixi_hip_config& c, int rk) {
     k.fvf = (c.volume_integral == TRIXI_HIP_VOLUME_SHOCK_CAPTURING_HG) ? c.fv_flux : FLUX_CENTRAL;
     k.prec = c.precision;
     k.rk = rk;
+    k.ext = (c.boundary_condition != TRIXI_HIP_BC_PERIODIC || c.source_terms != TRIXI_HIP_SOURCE_NONE) ? 1 : 0;
     return k;
 }
 
@@ -959,6 +815,8 @@ static int launch_indicator(trixi_hip_ctx* c, const double* u, hipStream_t s) {
     HIP_TRY(hipGetLastError());
     return TRIXI_HIP_OK;
 }
+
+static IcParams ic_params(const trixi_hip_ctx* c, int ic, uint64_t seed);
 
 extern "C" {
 
@@ -997,15 +855,29 @@ int trixi_hip_create(const trixi_hip_config* config, const trixi_hip_mesh* mesh,
         return fail(TRIXI_HIP_EUNSUPPORTED, "GLM-MHD is built for 3D only");
     if (cfg.volume_integral == TRIXI_HIP_VOLUME_SHOCK_CAPTURING_HG && cfg.equation != TRIXI_HIP_COMPRESSIBLE_EULER)
         return fail(TRIXI_HIP_EUNSUPPORTED, "shock capturing is built for compressible Euler only");
+    if (cfg.boundary_condition != TRIXI_HIP_BC_PERIODIC) {
+        if (cfg.boundary_condition != TRIXI_HIP_BC_DIRICHLET)
+            return fail(TRIXI_HIP_EINVAL, "unknown boundary condition");
+        if (cfg.equation != TRIXI_HIP_COMPRESSIBLE_EULER ||
+            (cfg.boundary_ic != TRIXI_HIP_IC_CONVERGENCE_TEST && cfg.boundary_ic != TRIXI_HIP_IC_CONSTANT))
+            return fail(TRIXI_HIP_EUNSUPPORTED,
+                        "Dirichlet boundaries are built for compressible Euler with the convergence-test or "
+                        "constant state");
+    }
+    if (cfg.source_terms != TRIXI_HIP_SOURCE_NONE) {
+        if (cfg.source_terms != TRIXI_HIP_SOURCE_CONVERGENCE_TEST) return fail(TRIXI_HIP_EINVAL, "unknown source terms");
+        if (cfg.equation != TRIXI_HIP_COMPRESSIBLE_EULER)
+            return fail(TRIXI_HIP_EUNSUPPORTED, "source_terms_convergence_test is built for compressible Euler");
+    }
     const KernelKey key = make_key(cfg, 0);
     LaunchFn fn = find_kernel(key), fn_rk = find_kernel(make_key(cfg, 1));
     if (!fn || !fn_rk) {
-        char buf[256];
+        char buf[320];
         snprintf(buf, sizeof buf,
                  "unsupported combination: ndims=%d equation=%d polydeg=%d volume_integral=%d volume_flux=%d "
-                 "surface_flux=%d fv_flux=%d precision=%d",
+                 "surface_flux=%d fv_flux=%d precision=%d boundaries/sources=%d",
                  cfg.ndims, cfg.equation, cfg.polydeg, cfg.volume_integral, cfg.volume_flux, cfg.surface_flux,
-                 cfg.fv_flux, cfg.precision);
+                 cfg.fv_flux, cfg.precision, key.ext);
         return fail(TRIXI_HIP_EUNSUPPORTED, buf);
     }
     const int dim = cfg.ndims, N = cfg.polydeg + 1, nv = nvars_of(cfg);
@@ -1015,8 +887,9 @@ int trixi_hip_create(const trixi_hip_config* config, const trixi_hip_mesh* mesh,
     if (E <= 0 || I < 0 || !mesh->neighbor_ids || !mesh->orientations || !mesh->inverse_jacobian)
         return fail(TRIXI_HIP_EINVAL, "incomplete mesh");
     if (E > (int64_t)INT32_MAX / 2) return fail(TRIXI_HIP_EINVAL, "too many elements");
-    // Face neighbours from the interfaces; every face must be matched once
-    // (periodic, conforming TreeMesh).
+    // Face neighbours from the interfaces; every face must be matched at most
+    // once (conforming TreeMesh), and faces without an interface are boundary
+    // faces, which need a boundary condition.
     std::vector<int32_t> nbr((size_t)2 * dim * E, -1);
     for (int64_t i = 0; i < I; ++i) {
         const int32_t L = mesh->neighbor_ids[2 * i], R = mesh->neighbor_ids[2 * i + 1];
@@ -1030,10 +903,13 @@ int trixi_hip_create(const trixi_hip_config* config, const trixi_hip_mesh* mesh,
         lp = R;
         rm = L;
     }
-    for (size_t q = 0; q < nbr.size(); ++q)
-        if (nbr[q] < 0)
-            return fail(TRIXI_HIP_EUNSUPPORTED,
-                        "element face without interface: only periodic meshes are supported");
+    if (cfg.boundary_condition == TRIXI_HIP_BC_PERIODIC)
+        for (size_t q = 0; q < nbr.size(); ++q)
+            if (nbr[q] < 0)
+                return fail(TRIXI_HIP_EUNSUPPORTED,
+                            "element face without interface: a non-periodic mesh needs a boundary condition");
+    if (cfg.boundary_condition != TRIXI_HIP_BC_PERIODIC && !mesh->node_coordinates)
+        return fail(TRIXI_HIP_EINVAL, "boundary states need the node coordinates");
 
     trixi_hip_ctx* c = new trixi_hip_ctx();
     c->cfg = cfg;
@@ -1110,6 +986,7 @@ int trixi_hip_create(const trixi_hip_config* config, const trixi_hip_mesh* mesh,
     a.phys.inv_gm1 = (cfg.gamma != 1.0) ? 1.0 / (cfg.gamma - 1.0) : 0.0;
     a.phys.c_h = cfg.c_h;
     for (int d = 0; d < 3; ++d) a.phys.a[d] = cfg.advection_velocity[d];
+    a.bc_ic = ic_params(c, cfg.boundary_ic, cfg.boundary_seed);
     for (int q = 0; q < N * N; ++q) {
         a.Ds[q] = c->basis.Ds[q];
         a.Dhat[q] = c->basis.Dhat[q];
@@ -1122,6 +999,9 @@ int trixi_hip_create(const trixi_hip_config* config, const trixi_hip_mesh* mesh,
         a.atol = std::max(100 * eps, pow(eps, 0.75));
     }
     a.smooth = cfg.alpha_smooth;
+    a.x = c->d_x;
+    a.bc = cfg.boundary_condition;
+    a.src = cfg.source_terms;
     a.nxcd = 1;
     {
         int dev = 0;
@@ -1163,6 +1043,7 @@ int trixi_hip_rhs(trixi_hip_ctx* ctx, double* du, const double* u, double t, tri
     RhsArgs a = ctx->args;
     a.u = u;
     a.du = du;
+    a.t_host = t;
     HIP_TRY(ctx->rhs_fn(a, s));
     return TRIXI_HIP_OK;
 }
@@ -1419,6 +1300,9 @@ static VjpArgs vjp_args(const trixi_hip_ctx* c) {
     v.winner = c->d_winner;
     v.atol = c->args.atol;
     v.smooth = c->args.smooth;
+    v.x = c->d_x;
+    v.bc_ic = c->args.bc_ic;
+    v.bc = c->args.bc;
     return v;
 }
 
@@ -1481,7 +1365,6 @@ int trixi_hip_rhs_vjp(trixi_hip_ctx* ctx, double* ubar, const double* u, const d
                       double alpha, double beta, trixi_hip_stream stream) {
     if (!ctx || !ubar || !u || !vbar) return fail(TRIXI_HIP_EINVAL, "null argument");
     if (ubar == vbar || ubar == u) return fail(TRIXI_HIP_EINVAL, "ubar must not alias u or vbar");
-    (void)t;
     int rc = require_vjp(ctx);
     if (rc) return rc;
     VjpArgs v = vjp_args(ctx);
@@ -1490,6 +1373,7 @@ int trixi_hip_rhs_vjp(trixi_hip_ctx* ctx, double* ubar, const double* u, const d
     v.ubar = ubar;
     v.alpha = alpha;
     v.beta = beta;
+    v.t = t;
     return enqueue_vjp(ctx, v, (hipStream_t)stream);
 }
 
@@ -1549,6 +1433,7 @@ int trixi_hip_rk_step_vjp(trixi_hip_ctx* ctx, const double* u_n, double* ubar, d
         v.ubar = ubar;
         v.alpha = dt;
         v.beta = 1.0;
+        v.t = t + CK_C[st] * dt;
         int rc = enqueue_vjp(ctx, v, s);
         if (rc) return rc;
     }

@@ -45,9 +45,14 @@ struct VjpArgs {
     double invw[MAXN];        // inverse LGL weights (FV subcells)
     double atol;              // pure flux-differencing threshold
     int smooth;
+    // non-periodic meshes: boundary faces (nbr < 0) with Dirichlet states
+    const double* x;          // node coordinates x[(d*NN + node)*E + e]
+    IcParams bc_ic;
+    int bc;
+    double t;                 // time of the linearisation point
 };
 
-template <int DIM, int N, class EQ, int VOL, int VF, int SF, int FVF, typename R, int M>
+template <int DIM, int N, class EQ, int VOL, int VF, int SF, int FVF, typename R, int M, bool EXT>
 __global__ void __launch_bounds__(M * cpow(N, DIM - 1))
 vjp_kernel(const VjpArgs a) {
     constexpr int NL = cpow(N, DIM - 1);
@@ -93,6 +98,7 @@ vjp_kernel(const VjpArgs a) {
         if (a.smooth) {
 #pragma unroll
             for (int f = 0; f < 2 * DIM; ++f) {
+                if (EXT && nb[f] < 0) continue;  // smoothing runs over interfaces only
                 const double an = 0.5 * a.alpha_raw[nb[f]];
                 if (an > al) {
                     al = an;
@@ -153,9 +159,13 @@ vjp_kernel(const VjpArgs a) {
     };
     // Interface at the -/+ face of the line: neighbour face node state and
     // cotangent, then the flux VJP keeping only this element's side.
+    // A boundary face (EXT, nbr < 0) has a Dirichlet state that does not
+    // depend on u, and no neighbour cotangent.
     auto face = [&](int d, int base, int s, bool plus, const Node& own, Bar& b) {
-        const int64_t fn = nb[2 * d + (plus ? 1 : 0)];
-        const R wn = (R)(-a.invJ[fn] * a.alpha);
+        const int32_t fnb = nb[2 * d + (plus ? 1 : 0)];
+        const bool bnd = EXT && fnb < 0;
+        const int64_t fn = bnd ? ec : fnb;
+        const R wn = bnd ? R(0) : (R)(-a.invJ[fn] * a.alpha);
         const int nnode = plus ? base : base + (N - 1) * s;
         const int onode = plus ? base + (N - 1) * s : base;
         R un[NV], g[NV], go[NV], gn[NV];
@@ -167,6 +177,17 @@ vjp_kernel(const VjpArgs a) {
             go[v] = (plus ? lp : -lm) * ws[(v * NN + onode) * M + el];
             gn[v] = (plus ? -lm : lp) * wn * (R)a.vbar[r];
             g[v] = go[v] + gn[v];
+        }
+        if constexpr (EXT) {
+            if (bnd) {
+                double xx[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+                for (int dd = 0; dd < DIM; ++dd) xx[dd] = a.x[((int64_t)dd * NN + onode) * E + ec];
+                double uu[5];
+                boundary_value(a.bc_ic, xx, a.t, uu);
+#pragma unroll
+                for (int v = 0; v < NV; ++v) un[v] = (R)uu[v];
+            }
         }
         const Node ndn = EQ::node(un, P);
         Bar dummy;
@@ -434,12 +455,12 @@ vjp_kernel(const VjpArgs a) {
     }
 }
 
-template <int DIM, int N, class EQ, int VOL, int VF, int SF, int FVF, typename R, int M>
+template <int DIM, int N, class EQ, int VOL, int VF, int SF, int FVF, typename R, int M, bool EXT = false>
 hipError_t launch_vjp(const VjpArgs& a, hipStream_t stream) {
     constexpr int NL = cpow(N, DIM - 1);
     int64_t blocks = (a.E + M - 1) / M;
     if (a.nxcd > 1) blocks = (blocks + a.nxcd - 1) / a.nxcd * a.nxcd;
-    hipLaunchKernelGGL((vjp_kernel<DIM, N, EQ, VOL, VF, SF, FVF, R, M>), dim3((unsigned)blocks), dim3(M * NL), 0,
+    hipLaunchKernelGGL((vjp_kernel<DIM, N, EQ, VOL, VF, SF, FVF, R, M, EXT>), dim3((unsigned)blocks), dim3(M * NL), 0,
                        stream, a);
     return hipGetLastError();
 }

@@ -155,6 +155,46 @@ def initial_condition_weak_blast_wave_seeded(seed=20261202):
                             int(seed))
 
 
+# ----------------------------------------------------------------------------
+# boundary conditions and source terms
+# ----------------------------------------------------------------------------
+class _Periodic:
+    def __repr__(self):
+        return "boundary_condition_periodic"
+
+
+boundary_condition_periodic = _Periodic()
+
+
+class BoundaryConditionDirichlet:
+    """Trixi's BoundaryConditionDirichlet(boundary_value_function): the state
+    boundary_value_function(x, t) outside the face; flux(u_inner, u_b) on +
+    faces, flux(u_b, u_inner) on - faces.  Built for compressible Euler with
+    initial_condition_convergence_test or initial_condition_constant."""
+
+    def __init__(self, boundary_value_function):
+        if not isinstance(boundary_value_function, InitialCondition):
+            raise NotImplementedError("the boundary state must be one of the built initial conditions")
+        self.boundary_value_function = boundary_value_function
+
+    def __repr__(self):
+        return f"BoundaryConditionDirichlet({self.boundary_value_function})"
+
+
+@dataclass(frozen=True)
+class SourceTerms:
+    name: str
+    source_id: int
+
+    def __repr__(self):
+        return self.name
+
+
+# Trixi's source_terms_convergence_test for compressible Euler: makes
+# initial_condition_convergence_test an exact solution.
+source_terms_convergence_test = SourceTerms("source_terms_convergence_test", L.SOURCE_CONVERGENCE_TEST)
+
+
 def initial_condition_smooth_periodic(seed=20261202):
     """Seeded sums of three periodic sine modes per primitive variable."""
     return InitialCondition(f"initial_condition_smooth_periodic(seed={seed})", L.IC_SMOOTH_PERIODIC, int(seed))

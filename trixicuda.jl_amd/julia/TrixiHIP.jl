@@ -34,6 +34,11 @@ struct CConfig
     alpha_max::Float64
     alpha_min::Float64
     ic_length::Float64
+    boundary_condition::Int32
+    boundary_ic::Int32
+    boundary_seed::UInt64
+    source_terms::Int32
+    reserved2::Int32
 end
 
 struct CMesh

@@ -1,4 +1,4 @@
-"""Uniform periodic TreeMesh on the host.
+"""Uniform TreeMesh on the host (periodic or not).
 
 Mirrors what the reference obtains from Trixi's TreeMesh and
 init_elements!/init_interfaces! before copying to the GPU
@@ -9,7 +9,9 @@ init_elements!/init_interfaces! before copying to the GPU
   ((c - 1) >> d) & 1, so the leaf (element) order is the Morton order with
   the coarsest level most significant;
 * one interface per element and positive direction (+x, +y, +z in that
-  order), left = the element, right = its periodic neighbour;
+  order), left = the element, right = its neighbour (periodic wrap-around in
+  periodic directions); in a non-periodic direction the faces on the domain
+  boundary get no interface and are the mesh's boundary faces;
 * inverse Jacobian 2/dx, node coordinates centre + dx/2 * LGL node.
 """
 import numpy as np
@@ -25,9 +27,11 @@ class TreeMesh:
         lengths = cmax - cmin
         if not np.all(lengths == lengths[0]):
             raise ValueError("TreeMesh domains must be cubes")
-        if periodicity is not True:
-            raise NotImplementedError("only fully periodic TreeMesh is supported")
         self.ndims = nd = cmin.size
+        per = (bool(periodicity),) * nd if isinstance(periodicity, (bool, np.bool_)) else tuple(map(bool, periodicity))
+        if len(per) != nd:
+            raise ValueError("periodicity needs one entry per dimension")
+        self.periodicity = per
         self.level = L = int(initial_refinement_level)
         self.coordinates_min, self.coordinates_max = cmin, cmax
         self.length = float(lengths[0])
@@ -49,17 +53,28 @@ class TreeMesh:
                     code |= ((k[:, d] >> lvl) & 1) << (lvl * nd + d)
             return code
 
-        self.ninterfaces = nd * E
         nbr = np.zeros((E, nd, 2), dtype=np.int32)
         ori = np.zeros((E, nd), dtype=np.int32)
+        keep = np.ones((E, nd), dtype=bool)
         for d in range(nd):
             k = cells.copy()
             k[:, d] = (k[:, d] + 1) % n1
             nbr[:, d, 0] = e
             nbr[:, d, 1] = encode(k)
             ori[:, d] = d + 1
-        self.neighbor_ids = nbr.reshape(-1, 2)
-        self.orientations = ori.reshape(-1)
+            if not per[d]:
+                keep[:, d] = cells[:, d] + 1 < n1
+        self.neighbor_ids = nbr[keep].reshape(-1, 2)
+        self.orientations = ori[keep].reshape(-1)
+        self.ninterfaces = int(keep.sum())
+        # boundary faces (element, direction 0..2d-1 as -x, +x, -y, +y, -z, +z)
+        bnd = []
+        for d in range(nd):
+            if not per[d]:
+                for side, at in ((0, 0), (1, n1 - 1)):
+                    for el in np.nonzero(cells[:, d] == at)[0]:
+                        bnd.append((int(el), 2 * d + side))
+        self.boundary_faces = np.array(sorted(bnd), dtype=np.int32).reshape(-1, 2)
         dx = self.length / n1
         self.inverse_jacobian = np.full(E, 1.0 / (dx / 2.0))
         # Cell centres by halving from the root centre (the tree's own
@@ -92,4 +107,4 @@ class TreeMesh:
 
     def __repr__(self):
         return (f"TreeMesh{self.ndims}D(level={self.level}, nelements={self.nelements}, "
-                f"domain=[{self.coordinates_min}, {self.coordinates_max}], periodic)")
+                f"domain=[{self.coordinates_min}, {self.coordinates_max}], periodicity={self.periodicity})")

@@ -12,6 +12,7 @@ import ctypes
 import numpy as np
 
 from . import _lib as L
+from .equations import BoundaryConditionDirichlet, SourceTerms, boundary_condition_periodic
 
 
 def _torch():
@@ -30,14 +31,11 @@ def _stream():
 
 class SemidiscretizationHyperbolicHIP:
     def __init__(self, mesh, equations, initial_condition, solver, source_terms=None,
-                 boundary_conditions="periodic"):
+                 boundary_conditions=boundary_condition_periodic):
         if mesh.ndims != equations.ndims:
             raise ValueError("mesh and equations dimensions differ")
-        if source_terms is not None:
-            raise NotImplementedError("source terms are not built on the HIP path")
-        if boundary_conditions not in ("periodic", None):
-            raise NotImplementedError("only boundary_condition_periodic is supported")
         self.mesh, self.equations, self.initial_condition, self.solver = mesh, equations, initial_condition, solver
+        self.source_terms, self.boundary_conditions = source_terms, boundary_conditions
         self.nvariables = equations.nvariables
         self.nnodes = solver.basis.nnodes
         self.nelements = mesh.nelements
@@ -45,6 +43,7 @@ class SemidiscretizationHyperbolicHIP:
         self.ndofs = self.nelements * self.nnodes_per_element
         self.nvalues = self.ndofs * self.nvariables
         self.config = solver.config(equations, ic_length=mesh.length)
+        self._set_boundaries_and_sources(mesh, boundary_conditions, source_terms)
         self._node_coordinates = np.ascontiguousarray(mesh.node_coordinates(solver.basis.nodes))
         self._nbr = np.ascontiguousarray(mesh.neighbor_ids, dtype=np.int32)
         self._ori = np.ascontiguousarray(mesh.orientations, dtype=np.int32)
@@ -59,6 +58,35 @@ class SemidiscretizationHyperbolicHIP:
         handle = ctypes.c_void_p()
         L.check(L.lib().trixi_hip_create(ctypes.byref(self.config), ctypes.byref(md), ctypes.byref(handle)))
         self._ctx = handle
+
+    def _set_boundaries_and_sources(self, mesh, bcs, source_terms):
+        """Trixi's boundary_conditions (one condition, or one per side as a
+        dict/namedtuple x_neg, x_pos, ...: all sides must then be the same
+        Dirichlet condition) and source_terms, into the C-ABI configuration."""
+        c = self.config
+        if source_terms is None:
+            c.source_terms = L.SOURCE_NONE
+        elif isinstance(source_terms, SourceTerms):
+            c.source_terms = source_terms.source_id
+        else:
+            raise NotImplementedError(f"source terms {source_terms!r} are not built")
+        periodic_mesh = all(mesh.periodicity)
+        if bcs is None or bcs is boundary_condition_periodic or bcs == "periodic":
+            if not periodic_mesh:
+                raise ValueError("a non-periodic mesh needs boundary_conditions")
+            c.boundary_condition = L.BC_PERIODIC
+            return
+        sides = list(bcs.values()) if isinstance(bcs, dict) else \
+            (list(bcs._asdict().values()) if hasattr(bcs, "_asdict") else [bcs])
+        dirichlet = [b for b in sides if b is not boundary_condition_periodic]
+        if not dirichlet or not all(isinstance(b, BoundaryConditionDirichlet) for b in dirichlet):
+            raise NotImplementedError("only BoundaryConditionDirichlet and periodic boundaries are built")
+        f = dirichlet[0].boundary_value_function
+        if any(b.boundary_value_function != f for b in dirichlet):
+            raise NotImplementedError("all boundary faces must carry the same Dirichlet state")
+        c.boundary_condition = L.BC_DIRICHLET
+        c.boundary_ic = f.ic_id
+        c.boundary_seed = f.seed
 
     # -- lifetime ------------------------------------------------------------
     def close(self):
