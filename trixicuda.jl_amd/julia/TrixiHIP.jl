@@ -117,7 +117,26 @@ mutable struct SemidiscretizationHyperbolicHIP{Mesh, Equations, IC, Solver}
     nvalues::Int
 end
 
-function make_config(solver::DGSEMHIP, equations, mesh)
+# boundary_conditions: periodic, one BoundaryConditionDirichlet, or a NamedTuple
+# of them (x_neg, x_pos, ...) that all carry the same state
+function boundary_config(bcs)
+    bcs === boundary_condition_periodic && return (0, 0, UInt64(0))
+    sides = bcs isa NamedTuple ? collect(values(bcs)) : [bcs]
+    dir = [b for b in sides if b !== boundary_condition_periodic]
+    all(b -> b isa BoundaryConditionDirichlet, dir) ||
+        throw(ArgumentError("only BoundaryConditionDirichlet and periodic boundaries are built for HIP"))
+    f = dir[1].boundary_value_function
+    all(b -> b.boundary_value_function === f, dir) ||
+        throw(ArgumentError("all boundary faces must carry the same Dirichlet state"))
+    (1, ic_id(f), ic_seed(f))
+end
+
+source_id(::Nothing) = 0
+source_id(::typeof(source_terms_convergence_test)) = 1
+source_id(f) = throw(ArgumentError("source terms $f are not built for HIP"))
+
+function make_config(solver::DGSEMHIP, equations, mesh; boundary_conditions = boundary_condition_periodic,
+                     source_terms = nothing)
     vi = solver.volume_integral
     volume_integral, vflux, fvflux, smooth, amax, amin = 0, 0, 0, 0, 0.5, 0.001
     if vi isa VolumeIntegralFluxDifferencing
@@ -132,19 +151,19 @@ function make_config(solver::DGSEMHIP, equations, mesh)
     a = hasproperty(equations, :advection_velocity) ? Tuple(equations.advection_velocity) : ()
     vel = ntuple(d -> d <= length(a) ? Float64(a[d]) : 0.0, 3)
     len = mesh.tree.length_level_0
+    bc, bc_ic, bc_seed = boundary_config(boundary_conditions)
     CConfig(ndims(mesh), equation_id(equations), solver.polydeg, volume_integral, vflux,
             flux_id(solver.surface_flux), fvflux, solver.precision === :mixed ? 1 : 0,
-            smooth, 0, gamma, c_h, vel, amax, amin, len)
+            smooth, 0, gamma, c_h, vel, amax, amin, len, bc, bc_ic, bc_seed, source_id(source_terms), 0)
 end
 
 function SemidiscretizationHyperbolicHIP(mesh::TreeMesh, equations, initial_condition,
                                          solver::DGSEMHIP;
                                          source_terms = nothing,
                                          boundary_conditions = boundary_condition_periodic)
-    source_terms === nothing || throw(ArgumentError("source terms are not built for HIP"))
-    boundary_conditions === boundary_condition_periodic ||
-        throw(ArgumentError("only periodic TreeMesh is supported"))
-    # Trixi builds the element and interface containers on the host.
+    # Trixi builds the element and interface containers on the host (faces on a
+    # non-periodic boundary get no interface: the library treats them as
+    # boundary faces).
     dg_cpu = DGSEM(polydeg = solver.polydeg, surface_flux = flux_central)
     cache = Trixi.create_cache(mesh, equations, dg_cpu, Float64, Float64)
     e, i = cache.elements, cache.interfaces
@@ -152,7 +171,7 @@ function SemidiscretizationHyperbolicHIP(mesh::TreeMesh, equations, initial_cond
     ori = Int32.(i.orientations)
     invj = Vector{Float64}(e.inverse_jacobian)
     xyz = Array{Float64}(e.node_coordinates)     # [d, i, j, k, e]
-    cfg = make_config(solver, equations, mesh)
+    cfg = make_config(solver, equations, mesh; boundary_conditions, source_terms)
     GC.@preserve nbr ori invj xyz begin
         cmesh = CMesh(nelements(e), ninterfaces(i), pointer(nbr), pointer(ori), pointer(invj),
                       pointer(xyz), mesh.tree.length_level_0^ndims(mesh))
