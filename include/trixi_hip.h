@@ -133,6 +133,16 @@ typedef struct trixi_hip_mesh {
     const double* inverse_jacobian;  /* [nelements]                                    */
     const double* node_coordinates;  /* Trixi layout [d, i, j, k, element]; may be NULL */
     double total_volume;
+    /* L2 mortars of a refined mesh (Trixi's init_mortars! container, as the
+     * reference copies it in create_cache_gpu, cache.jl:150-158): 0 for a
+     * conforming mesh, then the pointers may be NULL.  2D and 3D only. */
+    int64_t nmortars;
+    const int32_t* mortar_neighbor_ids; /* [nmortars][2^(d-1)+1]: the small elements
+                                           (lower, upper; 3D lower_left, lower_right,
+                                           upper_left, upper_right), then the large */
+    const int32_t* mortar_orientations; /* [nmortars]: 1..ndims                      */
+    const int32_t* mortar_large_sides;  /* [nmortars]: 1 large element on the left
+                                           (-side), 2 on the right                   */
 } trixi_hip_mesh;
 
 /* Library version string. */

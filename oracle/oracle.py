@@ -48,6 +48,10 @@ class OracleCfg(ctypes.Structure):
         ("reserved", ctypes.c_int32),
         ("node_coordinates", ctypes.c_void_p),
         ("time", ctypes.c_double),
+        ("nmortars", ctypes.c_int64),
+        ("mortar_neighbor_ids", ctypes.c_void_p),
+        ("mortar_orientations", ctypes.c_void_p),
+        ("mortar_large_sides", ctypes.c_void_p),
     ]
 
 
@@ -229,6 +233,151 @@ class Mesh:
                                _p(self.cell_index))
         assert rc == 0
         self.total_volume = (cmax - cmin) ** ndims
+        self.nmortars = 0
+        self.mortar_neighbor_ids = np.zeros((0, (1 << (ndims - 1)) + 1), dtype=np.int32)
+        self.mortar_orientations = np.zeros(0, dtype=np.int32)
+        self.mortar_large_sides = np.zeros(0, dtype=np.int32)
+
+
+class RefinedMesh:
+    """TreeMesh with box refinement patches, built geometrically.
+
+    Leaves are (level, integer coordinates) cells.  Starting from the uniform
+    level-``level`` grid, each box patch refines the leaves whose centre lies
+    strictly inside it; then any leaf with a face neighbour leaf two or more
+    levels finer is refined until none is left (the unique minimal 2:1 face
+    balance, which Trixi's rebalance! reaches one refined cell at a time,
+    abstract_tree.jl).  The element order is the depth-first order of the
+    tree with children in Morton order (Trixi inserts a cell's children
+    directly behind it), i.e. leaves sorted by the Morton code of their
+    lower-left corner at the finest level.  Interfaces join equal-level face
+    neighbours (left = element, + direction), an L2 mortar joins a leaf to the
+    2^(d-1) leaves of its refined same-level neighbour facing it (small
+    elements in position order, first tangential direction fastest, then the
+    large element; large_side 1 when the large element is on the left), and
+    faces outside a non-periodic domain are boundary faces.
+    """
+
+    def __init__(self, ndims, level, cmin, cmax, polydeg, patches, periodic=True):
+        if ndims == 1:
+            raise ValueError("1D trees have no mortars")
+        nd = ndims
+        per = periodic if isinstance(periodic, tuple) else (bool(periodic),) * nd
+        leaves = {(level, k) for k in np.ndindex(*((1 << level),) * nd)}
+
+        def inside(lv, k, lo, hi):
+            length = cmax - cmin
+            for d in range(nd):
+                c = cmin + (k[d] + 0.5) * length / (1 << lv)
+                if not (lo[d] < c < hi[d]):
+                    return False
+            return True
+
+        def split(lv, k):
+            leaves.discard((lv, k))
+            for ch in range(1 << nd):
+                leaves.add((lv + 1, tuple(2 * k[d] + ((ch >> d) & 1) for d in range(nd))))
+
+        def wrap(lv, k, d, s):
+            kk = list(k)
+            kk[d] += s
+            n1 = 1 << lv
+            if not 0 <= kk[d] < n1:
+                if not per[d]:
+                    return None
+                kk[d] %= n1
+            return tuple(kk)
+
+        def covering_leaf(lv, k):
+            # the leaf containing cell (lv, k), if it is at level <= lv
+            for l in range(lv, -1, -1):
+                kk = tuple(x >> (lv - l) for x in k)
+                if (l, kk) in leaves:
+                    return (l, kk)
+            return None
+
+        for lo, hi in patches:
+            for lv, k in [c for c in leaves if inside(c[0], c[1], lo, hi)]:
+                split(lv, k)
+            changed = True
+            while changed:
+                changed = False
+                for lv, k in sorted(leaves, reverse=True):
+                    if (lv, k) not in leaves:
+                        continue
+                    for d in range(nd):
+                        for s in (-1, 1):
+                            nk = wrap(lv, k, d, s)
+                            if nk is None:
+                                continue
+                            cov = covering_leaf(lv, nk)
+                            if cov is not None and cov[0] < lv - 1:
+                                split(*cov)
+                                changed = True
+        lmax = max(lv for lv, _ in leaves)
+
+        def morton(c):
+            lv, k = c
+            code = 0
+            for b in range(lmax - 1, -1, -1):
+                for d in range(nd - 1, -1, -1):
+                    code = (code << 1) | (((k[d] << (lmax - lv)) >> b) & 1)
+            return code
+
+        elems = sorted(leaves, key=morton)
+        E = len(elems)
+        eid = {c: i for i, c in enumerate(elems)}
+        N = polydeg + 1
+        nodes = basis(N)["nodes"]
+        self.ndims, self.polydeg, self.level, self.nelements = nd, polydeg, lmax, E
+        self.coordinates_min, self.coordinates_max, self.periodic = cmin, cmax, per
+        self.levels = np.array([lv for lv, _ in elems], dtype=np.int32)
+        self.cell_index = np.array([k for _, k in elems], dtype=np.int32).reshape(E, nd)
+        self.inverse_jacobian = np.array([1.0 / ((cmax - cmin) / (1 << lv) / 2.0) for lv, _ in elems])
+        self.node_coordinates = np.zeros((E,) + (N,) * nd + (nd,))
+        for e, (lv, k) in enumerate(elems):
+            jac = (cmax - cmin) / (1 << lv) / 2.0
+            for d in range(nd):
+                cc, ln = 0.5 * (cmin + cmax), cmax - cmin
+                for b in range(lv - 1, -1, -1):
+                    ln *= 0.5
+                    cc += (0.5 if (k[d] >> b) & 1 else -0.5) * ln
+                shp = [1] * nd
+                shp[nd - 1 - d] = N
+                self.node_coordinates[e, ..., d] = cc + jac * nodes.reshape(shp)
+        ifc, mort, bnd = [], [], []
+        for e, (lv, k) in enumerate(elems):
+            for d in range(nd):
+                for s in (-1, 1):
+                    nk = wrap(lv, k, d, s)
+                    if nk is None:
+                        bnd.append((e, 2 * d + (s > 0)))
+                        continue
+                    if (lv, nk) in leaves:
+                        if s > 0:
+                            ifc.append((e, eid[(lv, nk)], d + 1))
+                        continue
+                    if covering_leaf(lv, nk) is not None:
+                        continue  # a coarser leaf: this element is a small side of its mortar
+                    tdims = [q for q in range(nd) if q != d]
+                    small = []
+                    for pos in range(1 << (nd - 1)):
+                        kk = [2 * x for x in nk]
+                        if s < 0:
+                            kk[d] += 1
+                        for t, q in enumerate(tdims):
+                            kk[q] += (pos >> t) & 1
+                        small.append(eid[(lv + 1, tuple(kk))])
+                    mort.append((small + [e], d + 1, 1 if s > 0 else 2))
+        self.neighbor_ids = np.array([(a, b) for a, b, _ in ifc], dtype=np.int32).reshape(-1, 2)
+        self.orientations = np.array([o for _, _, o in ifc], dtype=np.int32)
+        self.ninterfaces = len(ifc)
+        self.boundary_faces = np.array(sorted(bnd), dtype=np.int32).reshape(-1, 2)
+        self.nmortars = len(mort)
+        self.mortar_neighbor_ids = np.array([m[0] for m in mort], dtype=np.int32).reshape(-1, (1 << (nd - 1)) + 1)
+        self.mortar_orientations = np.array([m[1] for m in mort], dtype=np.int32)
+        self.mortar_large_sides = np.array([m[2] for m in mort], dtype=np.int32)
+        self.total_volume = (cmax - cmin) ** nd
 
 
 def state_shape(cfg, nelements):
@@ -256,7 +405,20 @@ def _bind(cfg, mesh, t=0.0):
     c = OracleCfg.from_buffer_copy(cfg)
     c.node_coordinates = mesh.node_coordinates.ctypes.data
     c.time = float(t)
+    c.nmortars = mesh.nmortars
+    if mesh.nmortars:
+        c.mortar_neighbor_ids = mesh.mortar_neighbor_ids.ctypes.data
+        c.mortar_orientations = mesh.mortar_orientations.ctypes.data
+        c.mortar_large_sides = mesh.mortar_large_sides.ctypes.data
     return c
+
+
+def mortar_matrices(polydeg):
+    """(forward_lower, forward_upper, reverse_lower, reverse_upper), N x N each."""
+    N = polydeg + 1
+    m = [np.zeros((N, N)) for _ in range(4)]
+    assert lib().oracle_mortar_matrices(N, *[_p(a) for a in m]) == 0
+    return tuple(m)
 
 
 def source_convergence_test(cfg, x, t):

@@ -86,6 +86,8 @@ struct Cfg {
     uint64_t bc_seed;
     const double* xnodes;
     double t;
+    int64_t nmortars;
+    const int32_t *mids, *mori, *mls;
 };
 
 Cfg make_cfg(const oracle_cfg* c) {
@@ -112,6 +114,10 @@ Cfg make_cfg(const oracle_cfg* c) {
     k.src = c->source_terms;
     k.xnodes = c->node_coordinates;
     k.t = c->time;
+    k.nmortars = c->nmortars;
+    k.mids = c->mortar_neighbor_ids;
+    k.mori = c->mortar_orientations;
+    k.mls = c->mortar_large_sides;
     return k;
 }
 
@@ -447,6 +453,83 @@ void lagrange_polys(double x, int n, const double* nodes, const double* wbary, d
         total += out[j];
     }
     for (int j = 0; j < n; ++j) out[j] /= total;
+}
+
+// Legendre-Gauss nodes and weights (Newton iteration on P_n from the
+// Chebyshev-like first guess; w_i = 2 / ((1 - x_i^2) P_n'(x_i)^2)).
+void gauss_nodes_weights(int n, double* x, double* w) {
+    for (int i = 0; i < n; ++i) {
+        double xi = -cos(M_PI * (i + 0.75) / (n + 0.5));
+        double dp = 1.0;
+        for (int it = 0; it < 100; ++it) {
+            double p0 = 1.0, p1 = xi;
+            for (int k = 2; k <= n; ++k) {
+                const double p2 = ((2 * k - 1) * xi * p1 - (k - 1) * p0) / k;
+                p0 = p1;
+                p1 = p2;
+            }
+            const double pn = (n == 1) ? xi : p1, pm = (n == 1) ? 1.0 : p0;
+            dp = n * (xi * pn - pm) / (xi * xi - 1.0);
+            const double dx = pn / dp;
+            xi -= dx;
+            if (fabs(dx) < 1e-16) break;
+        }
+        double p0 = 1.0, p1 = xi;
+        for (int k = 2; k <= n; ++k) {
+            const double p2 = ((2 * k - 1) * xi * p1 - (k - 1) * p0) / k;
+            p0 = p1;
+            p1 = p2;
+        }
+        dp = n * (xi * ((n == 1) ? xi : p1) - ((n == 1) ? 1.0 : p0)) / (xi * xi - 1.0);
+        x[i] = xi;
+        w[i] = 2.0 / ((1.0 - xi * xi) * dp * dp);
+    }
+}
+
+// Mortar operators of Trixi's MortarL2 (external to the reference, used
+// through MortarL2GPU, basis_lobatto_legendre.jl:155-173):
+//   forward_{lower,upper}[j][i] = l_i((x_j -+ 1) / 2)  (Lobatto interpolation),
+//   reverse_{lower,upper} = G2L * P * L2G with the Gauss-basis projection
+//   P[i][j] = 1/2 * lg_i((g_j -+ 1)/2) * wg_j / wg_i.
+void mortar_matrices(int N, double* fl, double* fu, double* rl, double* ru) {
+    double x[16], w[16], wb[16], g[16], wg[16], gb[16], poly[16];
+    gauss_lobatto_nodes_weights(N, x, w);
+    barycentric_weights(N, x, wb);
+    gauss_nodes_weights(N, g, wg);
+    barycentric_weights(N, g, gb);
+    for (int j = 0; j < N; ++j) {
+        lagrange_polys(0.5 * (x[j] - 1.0), N, x, wb, poly);
+        for (int i = 0; i < N; ++i) fl[j * N + i] = poly[i];
+        lagrange_polys(0.5 * (x[j] + 1.0), N, x, wb, poly);
+        for (int i = 0; i < N; ++i) fu[j * N + i] = poly[i];
+    }
+    double G2L[256], L2G[256];
+    for (int j = 0; j < N; ++j) {
+        lagrange_polys(x[j], N, g, gb, poly);  // Gauss -> Lobatto
+        for (int i = 0; i < N; ++i) G2L[j * N + i] = poly[i];
+        lagrange_polys(g[j], N, x, wb, poly);  // Lobatto -> Gauss
+        for (int i = 0; i < N; ++i) L2G[j * N + i] = poly[i];
+    }
+    for (int half = 0; half < 2; ++half) {
+        double P[256], T[256];
+        for (int j = 0; j < N; ++j) {
+            lagrange_polys(0.5 * (g[j] + (half ? 1.0 : -1.0)), N, g, gb, poly);
+            for (int i = 0; i < N; ++i) P[i * N + j] = 0.5 * poly[i] * wg[j] / wg[i];
+        }
+        for (int i = 0; i < N; ++i)
+            for (int j = 0; j < N; ++j) {
+                double acc = 0.0;
+                for (int k = 0; k < N; ++k) acc += P[i * N + k] * L2G[k * N + j];
+                T[i * N + j] = acc;
+            }
+        double* R = half ? ru : rl;
+        for (int i = 0; i < N; ++i)
+            for (int j = 0; j < N; ++j) {
+                double acc = 0.0;
+                for (int k = 0; k < N; ++k) acc += G2L[i * N + k] * T[k * N + j];
+                R[i * N + j] = acc;
+            }
+    }
 }
 
 double legendre_normalized(int N, double x) {
@@ -811,6 +894,76 @@ void rhs_impl(const Cfg& c, int64_t E, int64_t I, const int32_t* nbr, const int3
         }
     }
 
+    // prolong2mortars! / calc_mortar_flux! / mortar_fluxes_to_elements! (L2
+    // mortars; reference dg_2d_kernel.jl:1167-1390, dg_3d_kernel.jl:1361-1790):
+    // the large element's face values are interpolated to each small face
+    // (forward_lower / forward_upper per tangential direction, position bits
+    // first tangential direction fastest), the surface flux is evaluated with
+    // the large side on its side of the mortar, the small elements take it as
+    // their face flux, and the large element takes the L2 projection
+    // (reverse_lower / reverse_upper) of all small-face fluxes.
+    std::vector<char> mortar_face((size_t)2 * nd * E, 0);
+    if (c.nmortars > 0) {
+        const int K = 1 << (nd - 1);
+        double Fm[2][256], Rm[2][256];
+        mortar_matrices(N, Fm[0], Fm[1], Rm[0], Rm[1]);
+        for (int64_t m = 0; m < c.nmortars; ++m) {
+            const int o = c.mori[m], ls = c.mls[m];
+            const int32_t* ids = c.mids + m * (K + 1);
+            const int32_t large = ids[K];
+            const int dir_large = 2 * (o - 1) + (ls == 1 ? 1 : 0), dir_small = 2 * (o - 1) + (ls == 1 ? 0 : 1);
+            const int pos_large = (ls == 1) ? N - 1 : 0, pos_small = (ls == 1) ? 0 : N - 1;
+            mortar_face[(size_t)dir_large * E + large] = 1;
+            std::vector<T> fstar((size_t)K * nf * nv);
+            for (int p = 0; p < K; ++p) {
+                const int32_t small = ids[p];
+                mortar_face[(size_t)dir_small * E + small] = 1;
+                const int p1 = p & 1, p2 = (p >> 1) & 1;
+                for (int q = 0; q < nf; ++q) {
+                    const int q1 = q % N, q2 = q / N;
+                    T uL[MAXV], uS[MAXV], flux[MAXV];
+                    for (int v = 0; v < nv; ++v) {
+                        T acc = 0.0;
+                        for (int a = 0; a < N; ++a) {
+                            if (nd == 2) {
+                                acc = acc + Fm[p1][q1 * N + a] * u[large * esz + nv * face_node(N, nd, o - 1, a, pos_large) + v];
+                            } else {
+                                for (int bb = 0; bb < N; ++bb)
+                                    acc = acc + Fm[p1][q1 * N + a] * Fm[p2][q2 * N + bb] *
+                                                    u[large * esz + nv * face_node(N, nd, o - 1, a + N * bb, pos_large) + v];
+                            }
+                        }
+                        uL[v] = acc;
+                        uS[v] = u[small * esz + nv * face_node(N, nd, o - 1, q, pos_small) + v];
+                    }
+                    if (ls == 1)
+                        two_point(c, c.sflux, uL, uS, o, flux);
+                    else
+                        two_point(c, c.sflux, uS, uL, o, flux);
+                    for (int v = 0; v < nv; ++v) {
+                        SF(v, q, dir_small, small) = flux[v];
+                        fstar[((size_t)p * nf + q) * nv + v] = flux[v];
+                    }
+                }
+            }
+            for (int j = 0; j < nf; ++j) {
+                const int j1 = j % N, j2 = j / N;
+                for (int v = 0; v < nv; ++v) {
+                    T acc = 0.0;
+                    for (int p = 0; p < K; ++p) {
+                        const int p1 = p & 1, p2 = (p >> 1) & 1;
+                        for (int q = 0; q < nf; ++q) {
+                            const int q1 = q % N, q2 = q / N;
+                            const double r = (nd == 2) ? Rm[p1][j1 * N + q1] : Rm[p1][j1 * N + q1] * Rm[p2][j2 * N + q2];
+                            acc = acc + r * fstar[((size_t)p * nf + q) * nv + v];
+                        }
+                    }
+                    SF(v, j, dir_large, large) = acc;
+                }
+            }
+        }
+    }
+
     // calc_boundary_flux! (Trixi BoundaryConditionDirichlet, as in the reference
     // test test/tree_dgsem_2d/euler_source_terms_nonperiodic.jl:12-17 and the
     // GPU kernel boundary_flux_kernel!): every element face without an
@@ -825,6 +978,8 @@ void rhs_impl(const Cfg& c, int64_t E, int64_t I, const int32_t* nbr, const int3
             has_interface[(size_t)(2 * o - 1) * E + nbr[2 * i]] = 1;
             has_interface[(size_t)(2 * o - 2) * E + nbr[2 * i + 1]] = 1;
         }
+        for (size_t q = 0; q < has_interface.size(); ++q)
+            if (mortar_face[q]) has_interface[q] = 1;
 #pragma omp parallel for schedule(static)
         for (int64_t e = 0; e < E; ++e)
             for (int dir = 0; dir < 2 * nd; ++dir) {
@@ -1299,6 +1454,12 @@ int oracle_mesh(int nd, int level, double cmin, double cmax, int polydeg, int pe
             ++count;
         }
     }
+    return 0;
+}
+
+int oracle_mortar_matrices(int N, double* fl, double* fu, double* rl, double* ru) {
+    if (N < 2 || N > 16) return -1;
+    mortar_matrices(N, fl, fu, rl, ru);
     return 0;
 }
 

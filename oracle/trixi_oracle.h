@@ -67,6 +67,10 @@ typedef struct oracle_cfg {
     int32_t reserved;
     const double* node_coordinates; /* [E][nodes][d], for boundary states and sources */
     double time;              /* rhs time; the start of the step for the RK step JVP */
+    int64_t nmortars;         /* L2 mortars (Trixi init_mortars! container):          */
+    const int32_t* mortar_neighbor_ids; /* [nmortars][2^(d-1)+1] small..., large   */
+    const int32_t* mortar_orientations; /* [nmortars] 1..d                          */
+    const int32_t* mortar_large_sides;  /* [nmortars] 1: large on the left, 2: right */
 } oracle_cfg;
 
 int oracle_nvars(const oracle_cfg* cfg);
@@ -106,6 +110,11 @@ int64_t oracle_mesh_ninterfaces(int ndims, int level, int periodic);
 int oracle_mesh(int ndims, int level, double coord_min, double coord_max, int polydeg, int periodic,
                 int32_t* neighbor_ids, int32_t* orientations, double* inverse_jacobian,
                 double* node_coordinates, int32_t* cell_index);
+/* L2 mortar operators (row-major N x N): forward = interpolation from the
+ * large face to the lower / upper half, reverse = L2 projection back
+ * (Gauss quadrature). */
+int oracle_mortar_matrices(int nnodes, double* forward_lower, double* forward_upper,
+                           double* reverse_lower, double* reverse_upper);
 /* source_terms_convergence_test at x, t (compressible Euler) */
 void oracle_source_convergence_test(const oracle_cfg* cfg, const double* x, double t, double* s);
 

@@ -31,6 +31,7 @@ class Case:
     bc_state: str = ""      # Dirichlet state on the boundary faces of a non-periodic mesh
     source: str = ""        # "convergence_test" for source_terms_convergence_test
     t: float = 0.0          # time of the rhs! evaluation
+    patches: tuple = ()     # box refinement patches ((lo, hi), ...): L2 mortars
 
 
 FLUX_NAMES = ("central", "llf", "llf_naive", "ranocha", "shima_etal", "hindenlang_gassner")
@@ -39,6 +40,8 @@ IC_NAMES = ("constant", "convergence_test", "weak_blast_wave", "seeded_random", 
 
 
 def oracle_mesh(O, c: Case):
+    if c.patches:
+        return O.RefinedMesh(c.ndims, c.level, c.cmin, c.cmax, c.polydeg, c.patches, periodic=c.periodic)
     return O.Mesh(c.ndims, c.level, c.cmin, c.cmax, c.polydeg, periodic=c.periodic)
 
 
@@ -87,8 +90,9 @@ def product_semi(th, c: Case, precision=None):
                                                volume_flux_fv=_flux(th, c.fvflux, mhd))
     solver = th.DGSEMHIP(polydeg=c.polydeg, surface_flux=_flux(th, c.sflux, mhd), volume_integral=vi,
                          precision=precision or c.precision)
+    patches = tuple(dict(type="box", coordinates_min=lo, coordinates_max=hi) for lo, hi in c.patches)
     mesh = th.TreeMesh((c.cmin,) * c.ndims, (c.cmax,) * c.ndims, initial_refinement_level=c.level,
-                       n_cells_max=10**8, periodicity=c.periodic)
+                       n_cells_max=10**8, periodicity=c.periodic, refinement_patches=patches)
     ics = {"constant": th.initial_condition_constant, "convergence_test": th.initial_condition_convergence_test,
            "weak_blast_wave": th.initial_condition_weak_blast_wave,
            "seeded_random": th.initial_condition_seeded_random(c.seed),

@@ -371,8 +371,10 @@ def test_autograd_function_backward(th, oracle, torch):
     semi.close()
 
 
-def test_adjoint_unsupported_is_an_error(th, torch):
-    c = Case("linadv2d_p2", 2, "linadv", 1, polydeg=2)
+@pytest.mark.parametrize("c", [Case("linadv2d_p2", 2, "linadv", 1, polydeg=2),
+                               Case("linadv2d_mortar", 2, "linadv", 2, cmin=-1.0, cmax=1.0,
+                                    patches=(((0.0, -1.0), (1.0, 1.0)),))], ids=lambda c: c.name)
+def test_adjoint_unsupported_is_an_error(th, torch, c):
     semi = product_semi(th, c)
     u = th.compute_coefficients(0.0, semi)
     with pytest.raises(th.TrixiHipError) as ei:

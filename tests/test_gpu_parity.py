@@ -13,6 +13,9 @@ pytestmark = pytest.mark.gpu
 
 FP64_TOL = 1e-12
 MIXED_TOL = 2e-5
+MORTAR_2D = (((0.0, -1.0), (1.0, 1.0)),)
+MORTAR_3D = (((0.0, -1.0, -1.0), (1.0, 1.0, 1.0)), ((0.0, -0.5, -0.5), (0.5, 0.5, 0.5)))
+EULER_MORTAR_3D = (((0.5, 0.5, 0.5), (1.5, 1.5, 1.5)),)
 
 
 @pytest.fixture(scope="module")
@@ -96,6 +99,32 @@ CASES = [
     Case("euler3d_hg_ranocha_blast", 3, "euler", 3, "hg", "ranocha", ic="weak_blast_wave"),
     Case("euler2d_hg_ranocha_llfnaive", 2, "euler", 4, "hg", "ranocha", "llf_naive", "llf_naive",
          ic="weak_blast_wave"),
+] + [
+    # refined meshes with L2 mortars: the reference's mortar meshes
+    # (test/tree_dgsem_2d/advection_mortar.jl, test/tree_dgsem_3d/advection_mortar.jl,
+    # test/tree_dgsem_3d/euler_mortar.jl) and non-periodic / flux-differencing variants
+    Case("linadv2d_mortar_llf", 2, "linadv", 2, cmin=-1.0, cmax=1.0, ic="convergence_test", advection=(0.2, -0.7),
+         patches=MORTAR_2D),
+    Case("linadv2d_mortar_llf_random", 2, "linadv", 2, cmin=-1.0, cmax=1.0, advection=(0.2, -0.7),
+         patches=MORTAR_2D),
+    Case("linadv3d_mortar_llf_random", 3, "linadv", 2, cmin=-1.0, cmax=1.0, advection=(0.2, -0.7, 0.5),
+         patches=MORTAR_3D),
+    # (the reference's velocity sums to zero, so the convergence-test state is
+    # stationary and du is pure discretization error; mixed precision is
+    # checked on the random state)
+    Case("linadv3d_mortar_central", 3, "linadv", 2, sflux="central", cmin=-1.0, cmax=1.0, ic="convergence_test",
+         advection=(0.2, -0.7, 0.5), patches=MORTAR_3D),
+    Case("linadv3d_mortar_central_random", 3, "linadv", 2, sflux="central", cmin=-1.0, cmax=1.0,
+         advection=(0.2, -0.7, 0.5), patches=MORTAR_3D),
+    Case("euler3d_mortar_weak_llf_src", 3, "euler", 2, cmin=0.0, cmax=2.0, ic="convergence_test",
+         source="convergence_test", t=0.3, patches=EULER_MORTAR_3D),
+    Case("euler3d_mortar_weak_llf_random", 3, "euler", 2, cmin=0.0, cmax=2.0, patches=EULER_MORTAR_3D),
+    Case("euler3d_mortar_fd_ranocha_random", 3, "euler", 2, "fd", "ranocha", cmin=0.0, cmax=2.0,
+         patches=EULER_MORTAR_3D),
+    Case("euler2d_mortar_fd_dirichlet", 2, "euler", 3, "fd", "ranocha", cmin=0.0, cmax=2.0, periodic=False,
+         bc_state="convergence_test", source="convergence_test", t=0.2, patches=(((0.5, 0.5), (1.5, 1.5)),)),
+    Case("euler2d_mortar_weak_random_nested", 2, "euler", 2, cmin=0.0, cmax=2.0,
+         patches=(((0.5, 0.5), (1.5, 1.5)), ((0.8, 0.8), (1.2, 1.2)))),
 ]
 
 
@@ -114,7 +143,9 @@ MIXED_CASES = [c for c in CASES if c.vi in ("fd", "hg") and c.polydeg == 3] + \
      # weak form with N = 4 applies D-hat on the matrix cores in mixed precision
      Case("euler3d_weak_llf_mixed", 3, "euler", 2, ic="smooth_periodic"),
      Case("linadv3d_weak_mixed", 3, "linadv", 2, ic="seeded_random"),
-     Case("linadv2d_weak_mixed", 2, "linadv", 3, ic="seeded_random")]
+     Case("linadv2d_weak_mixed", 2, "linadv", 3, ic="seeded_random")] + \
+    [c for c in CASES if c.patches and c.name != "linadv3d_mortar_central" and
+     not (c.ic == "convergence_test" and c.source)]  # (cancellation at the manufactured state, as above)
 
 
 @pytest.mark.parametrize("case", MIXED_CASES, ids=lambda c: c.name)
@@ -147,12 +178,14 @@ def test_pack_unpack_roundtrip_bit_exact(th, oracle, torch):
                                   Case("euler2d_blast_seeded", 2, "euler", 3, ic="weak_blast_wave_seeded"),
                                   Case("euler3d_smooth", 3, "euler", 2, ic="smooth_periodic"),
                                   Case("mhd3d_alfven", 3, "mhd", 2, "fd", "hindenlang_gassner", cmin=-1.0,
-                                       cmax=1.0, ic="convergence_test", gamma=5 / 3)],
+                                       cmax=1.0, ic="convergence_test", gamma=5 / 3),
+                                  Case("euler3d_mortar", 3, "euler", 2, cmin=0.0, cmax=2.0, ic="convergence_test",
+                                       patches=EULER_MORTAR_3D)],
                          ids=lambda c: c.name)
 def test_initial_condition_and_callbacks_match_oracle(th, oracle, torch, case):
     O = oracle
     cfg = oracle_cfg(O, case)
-    mesh = O.Mesh(case.ndims, case.level, case.cmin, case.cmax, case.polydeg)
+    mesh = oracle_mesh(O, case)
     u_ref = O.compute_coefficients(cfg, mesh, oracle_ic(O, case), 0.0, case.seed)
     semi = product_semi(th, case)
     u = th.compute_coefficients(0.0, semi)
@@ -197,7 +230,11 @@ def test_indicator_matches_oracle(th, oracle, torch):
                                        source="convergence_test"),
                                   Case("euler3d_fd_dirichlet_src", 3, "euler", 2, "fd", "ranocha", cmin=0.0,
                                        cmax=2.0, ic="convergence_test", periodic=False, bc_state="convergence_test",
-                                       source="convergence_test")],
+                                       source="convergence_test"),
+                                  Case("linadv3d_mortar", 3, "linadv", 2, cmin=-1.0, cmax=1.0, ic="convergence_test",
+                                       advection=(0.2, -0.7, 0.5), patches=MORTAR_3D),
+                                  Case("euler3d_mortar_src", 3, "euler", 2, cmin=0.0, cmax=2.0, ic="convergence_test",
+                                       source="convergence_test", patches=EULER_MORTAR_3D)],
                          ids=lambda c: c.name)
 def test_rk_step_matches_oracle(th, oracle, torch, case, use_graph):
     """Three CK2N54 steps from t = 0 (stage times through device memory: the
@@ -230,7 +267,9 @@ def test_rk_step_matches_oracle(th, oracle, torch, case, use_graph):
 
 @pytest.mark.parametrize("case", [Case("euler3d_fd", 3, "euler", 2, "fd", "ranocha", ic="smooth_periodic"),
                                   Case("euler2d_hg", 2, "euler", 3, "hg", "ranocha", ic="weak_blast_wave_seeded"),
-                                  Case("euler2d_fd_mixed", 2, "euler", 3, "fd", "ranocha", precision="mixed")],
+                                  Case("euler2d_fd_mixed", 2, "euler", 3, "fd", "ranocha", precision="mixed"),
+                                  Case("linadv2d_mortar", 2, "linadv", 2, cmin=-1.0, cmax=1.0, ic="convergence_test",
+                                       advection=(0.2, -0.7), patches=MORTAR_2D)],
                          ids=lambda c: c.name)
 def test_rk_stages_equal_rk_step(th, torch, case):
     """Five trixi_hip_rk_stage calls (ping-pong states) == one trixi_hip_rk_step, bit for bit."""

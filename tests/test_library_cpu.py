@@ -128,3 +128,52 @@ def test_device_calls_fail_loudly_without_gpu(th):
     with pytest.raises(th.TrixiHipError):
         th.SemidiscretizationHyperbolicHIP(mesh, th.LinearScalarAdvectionEquation1D(1.0),
                                            th.initial_condition_convergence_test, solver)
+
+
+# the reference's mortar meshes: test/tree_dgsem_2d/advection_mortar.jl:16-23,
+# test/tree_dgsem_3d/advection_mortar.jl:16-25, test/tree_dgsem_3d/euler_mortar.jl:15-22,
+# test/tree_dgsem_2d/euler_blob_mortar.jl:59-68
+REFERENCE_MORTAR_MESHES = [
+    (2, 2, -1.0, 1.0, [((0.0, -1.0), (1.0, 1.0))]),
+    (3, 2, -1.0, 1.0, [((0.0, -1.0, -1.0), (1.0, 1.0, 1.0)), ((0.0, -0.5, -0.5), (0.5, 0.5, 0.5))]),
+    (3, 2, 0.0, 2.0, [((0.5,) * 3, (1.5,) * 3)]),
+    (2, 4, -32.0, 32.0, [((-40.0, -5.0), (40.0, 5.0)), ((-40.0, -5.0), (40.0, 5.0))]),
+]
+
+
+@pytest.mark.parametrize("spec", REFERENCE_MORTAR_MESHES, ids=["adv2d", "adv3d", "euler3d", "blob2d"])
+@pytest.mark.parametrize("periodic", [True, False])
+def test_refined_treemesh_matches_oracle(th, oracle, spec, periodic):
+    """The host tree (Trixi's refine!/rebalance! replayed cell by cell) and the
+    oracle's geometric construction (minimal 2:1 balance, leaves in Morton
+    order) give the same elements, interfaces, mortars and boundary faces."""
+    nd, lvl, a, b, patches = spec
+    m = th.TreeMesh((a,) * nd, (b,) * nd, lvl, periodicity=periodic,
+                    refinement_patches=[dict(type="box", coordinates_min=lo, coordinates_max=hi) for lo, hi in patches])
+    o = oracle.RefinedMesh(nd, lvl, a, b, 3, patches, periodic=periodic)
+    assert m.nelements == o.nelements and m.ninterfaces == o.ninterfaces and m.nmortars == o.nmortars > 0
+    np.testing.assert_array_equal(m.cell_index, o.cell_index)
+    np.testing.assert_array_equal(m.levels, o.levels)
+    np.testing.assert_array_equal(m.neighbor_ids, o.neighbor_ids)
+    np.testing.assert_array_equal(m.orientations, o.orientations)
+    np.testing.assert_array_equal(m.boundary_faces, o.boundary_faces)
+    np.testing.assert_array_equal(m.inverse_jacobian, o.inverse_jacobian)
+    key = lambda X: sorted(zip(map(tuple, X.mortar_neighbor_ids), X.mortar_orientations, X.mortar_large_sides))
+    assert key(m) == key(o)
+    x = m.node_coordinates(th.LobattoLegendreBasisHIP(3).nodes)
+    np.testing.assert_array_equal(x, o.node_coordinates)
+
+
+def test_mortar_mesh_rejections(th):
+    with pytest.raises(NotImplementedError):
+        th.TreeMesh((-1.0,), (1.0,), 2, refinement_patches=[dict(type="box", coordinates_min=(0.0,),
+                                                                  coordinates_max=(1.0,))])
+    m = th.TreeMesh((-1.0, -1.0), (1.0, 1.0), 2,
+                    refinement_patches=[dict(type="box", coordinates_min=(0.0, -1.0), coordinates_max=(1.0, 1.0))])
+    bad = m.mortar_neighbor_ids.copy()
+    bad[0, 0] = m.nelements  # out of range: rejected before any device call
+    m.mortar_neighbor_ids = bad
+    with pytest.raises(th.TrixiHipError, match="mortar 0 out of range"):
+        th.SemidiscretizationHyperbolicHIP(m, th.LinearScalarAdvectionEquation2D((0.2, -0.7)),
+                                           th.initial_condition_convergence_test,
+                                           th.DGSEMHIP(polydeg=3, surface_flux=th.flux_lax_friedrichs))

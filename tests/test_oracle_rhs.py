@@ -308,3 +308,125 @@ def test_mesh_morton_order_and_connectivity(oracle):
             np.testing.assert_array_equal(ci[r], expect)
         dx = 4.0 / n1
         np.testing.assert_allclose(mesh.inverse_jacobian, 2.0 / dx)
+
+
+@pytest.mark.parametrize("polydeg", [1, 2, 3, 4, 5, 7])
+def test_mortar_operators_are_interpolation_and_l2_projection(oracle, polydeg):
+    """forward_{lower,upper} interpolate the Lobatto polynomial to the lower /
+    upper half (exact on monomials), reverse_{lower,upper} are the L2
+    projection of the two halves back to one polynomial: Galerkin
+    orthogonality against every Legendre polynomial of degree < N, integrated
+    with a 2N-point Gauss rule of numpy (independent of the oracle)."""
+    from numpy.polynomial import legendre as Lg
+    O = oracle
+    N = polydeg + 1
+    fl, fu, rl, ru = O.mortar_matrices(polydeg)
+    x = O.basis(N)["nodes"]
+    V = np.vander(x, N, increasing=True)
+    np.testing.assert_allclose(fl @ V, np.vander((x - 1) / 2, N, increasing=True), atol=1e-14)
+    np.testing.assert_allclose(fu @ V, np.vander((x + 1) / 2, N, increasing=True), atol=1e-14)
+    g, wg = Lg.leggauss(2 * N)
+
+    def interp(vals, s):
+        return np.vander(s, N, increasing=True) @ np.linalg.solve(V, vals)
+
+    rng = np.random.default_rng(polydeg)
+    for _ in range(3):
+        f1, f2 = rng.standard_normal(N), rng.standard_normal(N)
+        q = rl @ f1 + ru @ f2
+        for k in range(N):
+            phi = Lg.Legendre.basis(k)
+            lhs = np.sum(wg * interp(q, g) * phi(g))
+            rhs = 0.5 * np.sum(wg * interp(f1, g) * phi((g - 1) / 2)) + 0.5 * np.sum(wg * interp(f2, g) * phi((g + 1) / 2))
+            assert abs(lhs - rhs) < 1e-13 * max(1.0, abs(rhs))
+    # projection of an interpolated polynomial reproduces it
+    np.testing.assert_allclose(rl @ fl + ru @ fu, np.eye(N), atol=1e-13)
+
+
+# the reference's mortar meshes (test/tree_dgsem_2d/advection_mortar.jl:16-23,
+# test/tree_dgsem_3d/advection_mortar.jl:16-25, test/tree_dgsem_3d/euler_mortar.jl:15-22)
+MORTAR_MESHES = {
+    "advection_mortar_2d": (2, 2, -1.0, 1.0, [((0.0, -1.0), (1.0, 1.0))]),
+    "advection_mortar_3d": (3, 2, -1.0, 1.0, [((0.0, -1.0, -1.0), (1.0, 1.0, 1.0)), ((0.0, -0.5, -0.5), (0.5, 0.5, 0.5))]),
+    "euler_mortar_3d": (3, 2, 0.0, 2.0, [((0.5,) * 3, (1.5,) * 3)]),
+}
+
+
+def test_refined_mesh_mortar_geometry(oracle):
+    """Every mortar joins 2^(d-1) leaves of half the size that tile the large
+    element's face, each small face touches the large face, and the face
+    count balances: 2d faces per element = interfaces both sides + mortar
+    faces + boundary faces."""
+    O = oracle
+    for name, (nd, lvl, a, b, patches) in MORTAR_MESHES.items():
+        for per in (True, False):
+            m = O.RefinedMesh(nd, lvl, a, b, 3, patches, periodic=per)
+            K = 1 << (nd - 1)
+            h = (b - a) / (1 << m.levels.astype(np.int64))
+            ctr = m.node_coordinates.reshape(m.nelements, -1, nd).mean(axis=1)
+            for ids, o, ls in zip(m.mortar_neighbor_ids, m.mortar_orientations, m.mortar_large_sides):
+                large, small = ids[K], ids[:K]
+                assert np.all(h[small] == h[large] / 2)
+                d = o - 1
+                sgn = 1 if ls == 1 else -1   # small elements on the + side of the large one
+                for s in small:
+                    gap = (ctr[s, d] - ctr[large, d]) * sgn
+                    if per:
+                        gap = (gap + (b - a) / 2) % (b - a) - (b - a) / 2
+                    assert abs(gap - 0.75 * h[large]) < 1e-12
+                    for t in range(nd):
+                        if t != d:
+                            assert abs(ctr[s, t] - ctr[large, t]) < h[large] / 2
+                tc = np.delete(ctr[small], d, axis=1)
+                assert len({tuple(np.round(r, 12)) for r in tc}) == K
+            faces = 2 * m.ninterfaces + m.nmortars * (K + 1) + len(m.boundary_faces)
+            assert faces == 2 * nd * m.nelements, name
+
+
+@pytest.mark.parametrize("name", list(MORTAR_MESHES))
+def test_mortar_free_stream_and_conservation(oracle, name):
+    """Constant states stay constant across mortars, and the rhs integrates to
+    zero over a periodic refined mesh (the L2 projection preserves the face
+    integral of the small-face fluxes)."""
+    O = oracle
+    nd, lvl, a, b, patches = MORTAR_MESHES[name]
+    mesh = O.RefinedMesh(nd, lvl, a, b, 3, patches)
+    for eq, vi, vf in ((O.EQ_LINADV, O.VI_WEAK, O.FLUX_CENTRAL), (O.EQ_EULER, O.VI_WEAK, O.FLUX_CENTRAL),
+                       (O.EQ_EULER, O.VI_FD, O.FLUX_RANOCHA)):
+        cfg = O.make_cfg(nd, eq, 3, vi, vf, O.FLUX_LLF, advection_velocity=(0.2, -0.7, 0.5)[:nd])
+        u = O.compute_coefficients(cfg, mesh, O.IC_CONSTANT)
+        assert np.abs(O.rhs(cfg, mesh, u)).max() < 1e-12
+        u = O.compute_coefficients(cfg, mesh, O.IC_SEEDED_RANDOM)
+        du = O.rhs(cfg, mesh, u)
+        assert np.abs(O.integrate(cfg, mesh, du)).max() < 1e-13 * np.abs(du).max() * mesh.total_volume
+
+
+@pytest.mark.parametrize("nd,eq,vi,per,src", [(2, "linadv", "weak", True, False), (3, "linadv", "weak", True, False),
+                                              (2, "euler", "fd", True, True), (2, "euler", "weak", False, True),
+                                              (3, "euler", "fd", True, True)])
+def test_mortar_meshes_converge_to_the_time_derivative(oracle, nd, eq, vi, per, src):
+    """rhs!(u_exact(t)) on a mesh refined in [0.5, 1.5]^d approaches
+    d/dt u_exact at the DG rate (polydeg 3: about h^3 in the max norm) when
+    both levels are refined; a wrong mortar interpolation or projection (a
+    half swapped, an operator transposed) leaves an O(1) residual at the
+    mortars that does not converge."""
+    O = oracle
+    E = {"linadv": O.EQ_LINADV, "euler": O.EQ_EULER}[eq]
+    kw = dict(ic_length=2.0, advection_velocity=(1.0, 0.5, 0.25)[:nd])
+    if not per:
+        kw.update(boundary_condition=O.BC_DIRICHLET, boundary_ic=O.IC_CONVERGENCE_TEST)
+    if src:
+        kw.update(source_terms=O.SOURCE_CONVERGENCE_TEST)
+    cfg = O.make_cfg(nd, E, 3, O.VI_WEAK if vi == "weak" else O.VI_FD, O.FLUX_RANOCHA if vi == "fd" else O.FLUX_CENTRAL,
+                     O.FLUX_LLF, **kw)
+    errs = []
+    t, h = 0.3, 1e-6
+    for lvl in ((3, 4) if (nd == 2 and eq == "linadv") else (2, 3)):
+        mesh = O.RefinedMesh(nd, lvl, 0.0, 2.0, 3, [((0.5,) * nd, (1.5,) * nd)], periodic=per)
+        assert mesh.nmortars > 0
+        u = O.compute_coefficients(cfg, mesh, O.IC_CONVERGENCE_TEST, t)
+        ut = (O.compute_coefficients(cfg, mesh, O.IC_CONVERGENCE_TEST, t + h)
+              - O.compute_coefficients(cfg, mesh, O.IC_CONVERGENCE_TEST, t - h)) / (2 * h)
+        errs.append(np.abs(O.rhs(cfg, mesh, u, t=t) - ut).max())
+    rate = np.log2(errs[0] / errs[1])
+    assert rate > 2.5, (errs, rate)

@@ -62,6 +62,10 @@ class MeshDesc(ctypes.Structure):
         ("inverse_jacobian", ctypes.c_void_p),
         ("node_coordinates", ctypes.c_void_p),
         ("total_volume", ctypes.c_double),
+        ("nmortars", ctypes.c_int64),
+        ("mortar_neighbor_ids", ctypes.c_void_p),
+        ("mortar_orientations", ctypes.c_void_p),
+        ("mortar_large_sides", ctypes.c_void_p),
     ]
 
 

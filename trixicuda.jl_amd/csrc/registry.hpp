@@ -1,6 +1,7 @@
 // registry.hpp -- table of the compiled rhs kernel instantiations.
 #pragma once
 
+#include "mortar_kernel.hpp"
 #include "rhs_kernel.hpp"
 #include "vjp_kernel.hpp"
 
@@ -20,11 +21,20 @@ struct KernelKey {
 
 using LaunchFn = hipError_t (*)(const RhsArgs&, hipStream_t);
 using VjpFn = hipError_t (*)(const VjpArgs&, hipStream_t);
+using MortarFn = hipError_t (*)(const MortarArgs&, hipStream_t);
 
 void register_kernel(const KernelKey& key, LaunchFn fn);
 LaunchFn find_kernel(const KernelKey& key);
 void register_vjp(const KernelKey& key, VjpFn fn);
 VjpFn find_vjp(const KernelKey& key);
+// mortar passes are keyed by dim, N, eq and sf only (other fields zero)
+void register_mortar(const KernelKey& key, MortarFn fn);
+MortarFn find_mortar(const KernelKey& key);
+
+template <int DIM, int N, class EQ, int SF>
+struct MortarRegistrar {
+    static void add(int eq) { register_mortar({DIM, N, eq, 0, 0, SF, 0, 0, 0, 0}, &launch_mortar<DIM, N, EQ, SF>); }
+};
 
 #ifndef TRIXI_HIP_VJP_LDS_KIB
 #define TRIXI_HIP_VJP_LDS_KIB 64

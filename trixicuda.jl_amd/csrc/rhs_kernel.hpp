@@ -79,6 +79,12 @@ struct RhsArgs {
     int bc;                   // enum trixi_hip_boundary_condition
     int src;                  // enum trixi_hip_source_terms
     double t_host;            // rhs mode: time t
+    // L2 mortars: a face with nbr = -2 - slot takes its surface flux from
+    // mflux[(slot*NL + face node)*NV + v]; mnb[slot*mK + k] lists the elements
+    // across that face (-1 padded) for the blending-factor smoothing
+    const double* mflux;
+    const int32_t* mnb;
+    int mK;
 };
 
 // Dirichlet state boundary_ic(x, t) at node `node` of element e (its face
@@ -134,8 +140,9 @@ __device__ __forceinline__ int line_base(int line, int d) {
     return base;
 }
 
-// EXT: the variant with boundary faces (nbr < 0, Dirichlet states) and source
-// terms; the periodic, source-free kernels carry none of that code.
+// EXT: the variant with boundary faces (nbr = -1, Dirichlet states), L2
+// mortar faces (nbr = -2 - slot, fluxes from the mortar pass) and source
+// terms; the periodic, conforming, source-free kernels carry none of that code.
 template <int DIM, int N, class EQ, int VOL, int VF, int SF, int FVF, typename R, int M, bool RK, bool EXT>
 __global__ void __launch_bounds__(M * cpow(N, DIM - 1), TRIXI_HIP_MIN_WAVES)
 rhs_kernel(const RhsArgs a) {
@@ -179,9 +186,10 @@ rhs_kernel(const RhsArgs a) {
     // Face neighbours of all 2*DIM faces, and the neighbour face data of the
     // first direction, are requested before the node pass so that their
     // latency overlaps it; later directions are prefetched one phase ahead.
-    // A face with nbr < 0 is a boundary face (non-periodic mesh): its loads
-    // read this element instead and the state is replaced by the boundary
-    // state in the direction phase.
+    // A face with nbr < 0 is a boundary face (-1, non-periodic mesh) or a
+    // mortar face (<= -2): its loads read this element instead, and the state
+    // is replaced by the boundary state, or the flux by the mortar flux, in
+    // the direction phase.
     int32_t nb[2 * DIM], nbl[2 * DIM];
 #pragma unroll
     for (int f = 0; f < 2 * DIM; ++f) {
@@ -240,7 +248,18 @@ rhs_kernel(const RhsArgs a) {
         if (a.smooth) {
 #pragma unroll
             for (int s = 0; s < 2 * DIM; ++s) {
-                if (EXT && nb[s] < 0) continue;  // smoothing runs over interfaces only
+                if (EXT && nb[s] < 0) {
+                    // boundary faces have no neighbour; a mortar face has the
+                    // elements on the other side of the mortar
+                    if (nb[s] == -1) continue;
+                    const int32_t* mn = a.mnb + (int64_t)(-2 - nb[s]) * a.mK;
+                    for (int k = 0; k < a.mK; ++k) {
+                        if (mn[k] < 0) continue;
+                        const double an = 0.5 * a.alpha_raw[mn[k]];
+                        al = an > al ? an : al;
+                    }
+                    continue;
+                }
                 const double an = 0.5 * a.alpha_raw[nb[s]];
                 al = an > al ? an : al;
             }
@@ -260,8 +279,8 @@ rhs_kernel(const RhsArgs a) {
             up[v] = (R)up_next[v];
         }
         if (EXT && a.bc) {
-            if (nb[2 * d] < 0) boundary_state<DIM, NN, NV, R>(a, ec, base, tnow, um);
-            if (nb[2 * d + 1] < 0) boundary_state<DIM, NN, NV, R>(a, ec, base + (N - 1) * s, tnow, up);
+            if (nb[2 * d] == -1) boundary_state<DIM, NN, NV, R>(a, ec, base, tnow, um);
+            if (nb[2 * d + 1] == -1) boundary_state<DIM, NN, NV, R>(a, ec, base + (N - 1) * s, tnow, up);
         }
         if (d + 1 < DIM) {
             const int sn = cpow(N, d + 1);
@@ -397,6 +416,19 @@ rhs_kernel(const RhsArgs a) {
                 for (int v = 0; v < NV; ++v) {
                     fm[v] = fm[v] + R(0.5) * ncm[v];
                     fp[v] = fp[v] + R(0.5) * ncp[v];
+                }
+            }
+            if constexpr (EXT) {
+                // mortar faces: the face node of this line is `line`
+                if (nb[2 * d] <= -2) {
+                    const double* mf = a.mflux + ((int64_t)(-2 - nb[2 * d]) * NL + line) * NV;
+#pragma unroll
+                    for (int v = 0; v < NV; ++v) fm[v] = (R)mf[v];
+                }
+                if (nb[2 * d + 1] <= -2) {
+                    const double* mf = a.mflux + ((int64_t)(-2 - nb[2 * d + 1]) * NL + line) * NV;
+#pragma unroll
+                    for (int v = 0; v < NV; ++v) fp[v] = (R)mf[v];
                 }
             }
             const R lm = (R)a.lhat_m, lp = (R)a.lhat_p;

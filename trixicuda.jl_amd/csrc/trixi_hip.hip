@@ -39,6 +39,15 @@ VjpFn find_vjp(const KernelKey& key) {
     auto it = vjp_table().find(key);
     return it == vjp_table().end() ? nullptr : it->second;
 }
+static std::map<KernelKey, MortarFn>& mortar_table() {
+    static std::map<KernelKey, MortarFn> t;
+    return t;
+}
+void register_mortar(const KernelKey& key, MortarFn fn) { mortar_table()[key] = fn; }
+MortarFn find_mortar(const KernelKey& key) {
+    auto it = mortar_table().find(key);
+    return it == mortar_table().end() ? nullptr : it->second;
+}
 
 void register_linadv();
 void register_euler1d();
@@ -55,6 +64,7 @@ void register_vjp_euler3d();
 void register_vjp_mhd3d();
 void register_euler_ext();
 void register_vjp_euler_ext();
+void register_mortar_kernels();
 
 static void ensure_registered() {
     static std::once_flag once;
@@ -74,6 +84,7 @@ static void ensure_registered() {
         register_vjp_mhd3d();
         register_euler_ext();
         register_vjp_euler_ext();
+        register_mortar_kernels();
     });
 }
 
@@ -376,14 +387,28 @@ __global__ void __launch_bounds__(64) indicator_kernel(const double* __restrict_
     alpha_raw[e] = a < alpha_max ? a : alpha_max;
 }
 
+// alpha_e = max(raw_e, max over the face neighbours n of raw_n / 2), the
+// neighbours being the elements across interfaces and L2 mortars (Trixi's
+// apply_smoothing! over interfaces and mortars); boundary faces (nbr = -1)
+// have none, a mortar face (nbr = -2 - slot) lists its in mnb[slot*mK ..].
 __global__ void smooth_alpha_kernel(const double* __restrict__ raw, double* __restrict__ out,
-                                    const int32_t* __restrict__ nbr, int dim, int64_t E) {
+                                    const int32_t* __restrict__ nbr, int dim, int64_t E,
+                                    const int32_t* __restrict__ mnb, int mK) {
     const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (e >= E) return;
     double a = raw[e];
     for (int s = 0; s < 2 * dim; ++s) {
         const int32_t nb = nbr[(int64_t)s * E + e];
-        if (nb < 0) continue;  // boundary face: smoothing runs over interfaces only
+        if (nb == -1) continue;
+        if (nb <= -2) {
+            for (int k = 0; k < mK; ++k) {
+                const int32_t n = mnb[(int64_t)(-2 - nb) * mK + k];
+                if (n < 0) continue;
+                const double an = 0.5 * raw[n];
+                a = an > a ? an : a;
+            }
+            continue;
+        }
         const double an = 0.5 * raw[nb];
         a = an > a ? an : a;
     }
@@ -746,6 +771,16 @@ struct trixi_hip_ctx {
     double* d_alphabar = nullptr; // HG: cotangent of the smoothed blending factor
     int32_t* d_winner = nullptr;  // HG: argument of the smoothing max
     bool adjoint = false;
+    // L2 mortars
+    int64_t nmortars = 0;
+    int mK = 0;                   // small elements per mortar, 2^(d-1)
+    int32_t* d_mids = nullptr;    // [nmortars][mK+1]
+    int32_t* d_mori = nullptr;
+    int32_t* d_mls = nullptr;
+    int32_t* d_mnb = nullptr;     // [slots][mK] elements across each mortar face
+    double* d_mflux = nullptr;    // [slots][NL][nv]
+    MortarArgs margs;
+    MortarFn mortar_fn = nullptr;
     // graph cache for rk steps
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
@@ -769,7 +804,7 @@ static int nvars_of(const trixi_hip_config& c) {
     return 9;
 }
 
-static KernelKey make_key(const trixi_hip_config& c, int rk) {
+static KernelKey make_key(const trixi_hip_config& c, int rk, bool mortars = false) {
     KernelKey k;
     k.dim = c.ndims;
     k.N = c.polydeg + 1;
@@ -780,7 +815,8 @@ static KernelKey make_key(const trixi_hip_config& c, int rk) {
     k.fvf = (c.volume_integral == TRIXI_HIP_VOLUME_SHOCK_CAPTURING_HG) ? c.fv_flux : FLUX_CENTRAL;
     k.prec = c.precision;
     k.rk = rk;
-    k.ext = (c.boundary_condition != TRIXI_HIP_BC_PERIODIC || c.source_terms != TRIXI_HIP_SOURCE_NONE) ? 1 : 0;
+    k.ext = (c.boundary_condition != TRIXI_HIP_BC_PERIODIC || c.source_terms != TRIXI_HIP_SOURCE_NONE || mortars)
+                ? 1 : 0;
     return k;
 }
 
@@ -790,7 +826,8 @@ static void free_ctx(trixi_hip_ctx* c) {
     if (c->graph) (void)hipGraphDestroy(c->graph);
     void* bufs[] = {c->d_nbr, c->d_invJ, c->d_x,     c->d_alpha_raw, c->d_alpha, c->d_time,
                     c->d_partial, c->d_result, c->d_Vinv, c->d_Vdm,   c->d_wa,    c->d_w,
-                    c->d_tmp, c->d_stage, c->d_k, c->d_kbar, c->d_alphabar, c->d_winner};
+                    c->d_tmp, c->d_stage, c->d_k, c->d_kbar, c->d_alphabar, c->d_winner,
+                    c->d_mids, c->d_mori, c->d_mls, c->d_mnb, c->d_mflux};
     for (void* p : bufs)
         if (p) (void)hipFree(p);
     delete c;
@@ -816,7 +853,78 @@ static int launch_indicator(trixi_hip_ctx* c, const double* u, hipStream_t s) {
     return TRIXI_HIP_OK;
 }
 
+// Mortar fluxes of state u, before the rhs kernel that reads them.
+static int launch_mortars(trixi_hip_ctx* c, const double* u, hipStream_t s) {
+    if (c->nmortars == 0) return TRIXI_HIP_OK;
+    MortarArgs m = c->margs;
+    m.u = u;
+    HIP_TRY(c->mortar_fn(m, s));
+    return TRIXI_HIP_OK;
+}
+
 static IcParams ic_params(const trixi_hip_ctx* c, int ic, uint64_t seed);
+
+// Mortar operators of MortarL2 (Lobatto basis of N nodes): forward_{lower,upper}
+// [j][i] = l_i((x_j -+ 1)/2); reverse = G2L * P * L2G with the Gauss-basis
+// projection P[i][j] = 1/2 lg_i((g_j -+ 1)/2) wg_j / wg_i (Trixi's
+// calc_forward_{lower,upper}, calc_reverse_{lower,upper}(n, Val(:gauss)),
+// used by the reference's MortarL2GPU, basis_lobatto_legendre.jl:155-173).
+static void gauss_legendre(int n, std::vector<double>& x, std::vector<double>& w) {
+    x.assign(n, 0.0);
+    w.assign(n, 0.0);
+    for (int i = 0; i < n; ++i) {
+        double xi = -cos(M_PI * (i + 0.75) / (n + 0.5)), P = 0.0, dP = 1.0;
+        for (int it = 0; it < 100; ++it) {
+            legendre(n, xi, P, dP);
+            const double dx = P / dP;
+            xi -= dx;
+            if (fabs(dx) < 1e-16) break;
+        }
+        legendre(n, xi, P, dP);
+        x[i] = xi;
+        w[i] = 2.0 / ((1.0 - xi * xi) * dP * dP);
+    }
+}
+
+static void mortar_operators(const HostBasis& b, int N, double F[2][MAXN * MAXN], double R[2][MAXN * MAXN]) {
+    std::vector<double> g, wg;
+    gauss_legendre(N, g, wg);
+    std::vector<double> gb(N);
+    for (int j = 0; j < N; ++j) {
+        double p = 1.0;
+        for (int k = 0; k < N; ++k)
+            if (k != j) p *= g[j] - g[k];
+        gb[j] = 1.0 / p;
+    }
+    for (int half = 0; half < 2; ++half) {
+        const double sh = half ? 1.0 : -1.0;
+        std::vector<double> xs(N), gs(N);
+        for (int j = 0; j < N; ++j) {
+            xs[j] = 0.5 * (b.x[j] + sh);
+            gs[j] = 0.5 * (g[j] + sh);
+        }
+        const std::vector<double> Fh = interpolation_matrix(b.x, b.wbary, xs);  // [j][i]
+        const std::vector<double> Pg = interpolation_matrix(g, gb, gs);         // [j][i] = lg_i(gs_j)
+        const std::vector<double> L2G = interpolation_matrix(b.x, b.wbary, g);
+        const std::vector<double> G2L = interpolation_matrix(g, gb, b.x);
+        std::vector<double> P(N * N), T(N * N);
+        for (int i = 0; i < N; ++i)
+            for (int j = 0; j < N; ++j) P[i * N + j] = 0.5 * Pg[j * N + i] * wg[j] / wg[i];
+        for (int i = 0; i < N; ++i)
+            for (int j = 0; j < N; ++j) {
+                double acc = 0.0;
+                for (int k = 0; k < N; ++k) acc += P[i * N + k] * L2G[k * N + j];
+                T[i * N + j] = acc;
+            }
+        for (int i = 0; i < N; ++i)
+            for (int j = 0; j < N; ++j) {
+                double acc = 0.0;
+                for (int k = 0; k < N; ++k) acc += G2L[i * N + k] * T[k * N + j];
+                R[half][i * N + j] = acc;
+                F[half][i * N + j] = Fh[i * N + j];
+            }
+    }
+}
 
 extern "C" {
 
@@ -869,13 +977,21 @@ int trixi_hip_create(const trixi_hip_config* config, const trixi_hip_mesh* mesh,
         if (cfg.equation != TRIXI_HIP_COMPRESSIBLE_EULER)
             return fail(TRIXI_HIP_EUNSUPPORTED, "source_terms_convergence_test is built for compressible Euler");
     }
-    const KernelKey key = make_key(cfg, 0);
-    LaunchFn fn = find_kernel(key), fn_rk = find_kernel(make_key(cfg, 1));
+    const int64_t NM = mesh->nmortars;
+    if (NM < 0) return fail(TRIXI_HIP_EINVAL, "negative mortar count");
+    if (NM > 0 && cfg.ndims == 1) return fail(TRIXI_HIP_EINVAL, "1D meshes have no mortars");
+    const KernelKey key = make_key(cfg, 0, NM > 0);
+    LaunchFn fn = find_kernel(key), fn_rk = find_kernel(make_key(cfg, 1, NM > 0));
+    MortarFn mfn = nullptr;
+    if (NM > 0) {
+        mfn = find_mortar({cfg.ndims, cfg.polydeg + 1, cfg.equation, 0, 0, cfg.surface_flux, 0, 0, 0, 0});
+        if (!mfn) fn = nullptr;
+    }
     if (!fn || !fn_rk) {
         char buf[320];
         snprintf(buf, sizeof buf,
                  "unsupported combination: ndims=%d equation=%d polydeg=%d volume_integral=%d volume_flux=%d "
-                 "surface_flux=%d fv_flux=%d precision=%d boundaries/sources=%d",
+                 "surface_flux=%d fv_flux=%d precision=%d boundaries/sources/mortars=%d",
                  cfg.ndims, cfg.equation, cfg.polydeg, cfg.volume_integral, cfg.volume_flux, cfg.surface_flux,
                  cfg.fv_flux, cfg.precision, key.ext);
         return fail(TRIXI_HIP_EUNSUPPORTED, buf);
@@ -903,9 +1019,40 @@ int trixi_hip_create(const trixi_hip_config* config, const trixi_hip_mesh* mesh,
         lp = R;
         rm = L;
     }
+    // Mortar faces: nbr = -2 - slot, slot m*(K+1)+p for the small element at
+    // position p, m*(K+1)+K for the large element; mnb lists the elements
+    // across each slot's face.
+    const int K = 1 << (dim - 1);
+    std::vector<int32_t> mnb;
+    if (NM > 0) {
+        if (!mesh->mortar_neighbor_ids || !mesh->mortar_orientations || !mesh->mortar_large_sides)
+            return fail(TRIXI_HIP_EINVAL, "incomplete mortar arrays");
+        if (NM * (K + 1) > (int64_t)INT32_MAX / 2) return fail(TRIXI_HIP_EINVAL, "too many mortars");
+        mnb.assign((size_t)NM * (K + 1) * K, -1);
+        for (int64_t m = 0; m < NM; ++m) {
+            const int o = mesh->mortar_orientations[m], ls = mesh->mortar_large_sides[m];
+            const int32_t* ids = mesh->mortar_neighbor_ids + m * (K + 1);
+            bool in_range = o >= 1 && o <= dim && (ls == 1 || ls == 2);
+            for (int p = 0; p <= K; ++p) in_range = in_range && ids[p] >= 0 && ids[p] < E;
+            if (!in_range) return fail(TRIXI_HIP_EINVAL, "mortar " + std::to_string(m) + " out of range");
+            const int dir_large = 2 * (o - 1) + (ls == 1 ? 1 : 0), dir_small = 2 * (o - 1) + (ls == 1 ? 0 : 1);
+            const int64_t slot_large = m * (K + 1) + K;
+            for (int p = 0; p < K; ++p) {
+                int32_t& f = nbr[(size_t)dir_small * E + ids[p]];
+                if (f != -1) return fail(TRIXI_HIP_EINVAL, "face matched twice (mortar " + std::to_string(m) + ")");
+                const int64_t slot = m * (K + 1) + p;
+                f = (int32_t)(-2 - slot);
+                mnb[(size_t)slot * K] = ids[K];
+                mnb[(size_t)slot_large * K + p] = ids[p];
+            }
+            int32_t& f = nbr[(size_t)dir_large * E + ids[K]];
+            if (f != -1) return fail(TRIXI_HIP_EINVAL, "face matched twice (mortar " + std::to_string(m) + ")");
+            f = (int32_t)(-2 - slot_large);
+        }
+    }
     if (cfg.boundary_condition == TRIXI_HIP_BC_PERIODIC)
         for (size_t q = 0; q < nbr.size(); ++q)
-            if (nbr[q] < 0)
+            if (nbr[q] == -1)
                 return fail(TRIXI_HIP_EUNSUPPORTED,
                             "element face without interface: a non-periodic mesh needs a boundary condition");
     if (cfg.boundary_condition != TRIXI_HIP_BC_PERIODIC && !mesh->node_coordinates)
@@ -925,6 +1072,9 @@ int trixi_hip_create(const trixi_hip_config* config, const trixi_hip_mesh* mesh,
     c->NA = 2 * (N - 1) + 1;
     c->rhs_fn = fn;
     c->rk_fn = fn_rk;
+    c->nmortars = NM;
+    c->mK = K;
+    c->mortar_fn = mfn;
 
     auto cleanup_fail = [&](int code, const std::string& msg) {
         free_ctx(c);
@@ -948,6 +1098,14 @@ int trixi_hip_create(const trixi_hip_config* config, const trixi_hip_mesh* mesh,
     ALLOC(c->d_wa, c->NA * sizeof(double));
     ALLOC(c->d_w, N * sizeof(double));
     ALLOC(c->d_tmp, (size_t)c->nvalues * sizeof(double));
+    if (NM > 0) {
+        const size_t slots = (size_t)NM * (K + 1);
+        ALLOC(c->d_mids, slots * sizeof(int32_t));
+        ALLOC(c->d_mori, NM * sizeof(int32_t));
+        ALLOC(c->d_mls, NM * sizeof(int32_t));
+        ALLOC(c->d_mnb, slots * K * sizeof(int32_t));
+        ALLOC(c->d_mflux, slots * (nn / N) * nv * sizeof(double));
+    }
 #undef ALLOC
     auto up = [&](void* dst, const void* src, size_t bytes) {
         return hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice) == hipSuccess;
@@ -973,6 +1131,12 @@ int trixi_hip_create(const trixi_hip_config* config, const trixi_hip_mesh* mesh,
         ok = ok && up(c->d_wa, wa.data(), wa.size() * sizeof(double));
     }
     ok = ok && up(c->d_w, c->basis.w.data(), N * sizeof(double));
+    if (NM > 0) {
+        ok = ok && up(c->d_mids, mesh->mortar_neighbor_ids, (size_t)NM * (K + 1) * sizeof(int32_t));
+        ok = ok && up(c->d_mori, mesh->mortar_orientations, NM * sizeof(int32_t));
+        ok = ok && up(c->d_mls, mesh->mortar_large_sides, NM * sizeof(int32_t));
+        ok = ok && up(c->d_mnb, mnb.data(), mnb.size() * sizeof(int32_t));
+    }
     if (!ok) return cleanup_fail(TRIXI_HIP_EDEVICE, "host to device copy failed in create");
 
     RhsArgs& a = c->args;
@@ -1002,6 +1166,21 @@ int trixi_hip_create(const trixi_hip_config* config, const trixi_hip_mesh* mesh,
     a.x = c->d_x;
     a.bc = cfg.boundary_condition;
     a.src = cfg.source_terms;
+    a.mflux = c->d_mflux;
+    a.mnb = c->d_mnb;
+    a.mK = K;
+    {
+        MortarArgs& m = c->margs;
+        memset(&m, 0, sizeof(m));
+        m.mflux = c->d_mflux;
+        m.ids = c->d_mids;
+        m.ori = c->d_mori;
+        m.ls = c->d_mls;
+        m.nmortars = NM;
+        m.E = E;
+        m.phys = a.phys;
+        if (NM > 0) mortar_operators(c->basis, N, m.F, m.R);
+    }
     a.nxcd = 1;
     {
         int dev = 0;
@@ -1038,6 +1217,10 @@ int trixi_hip_rhs(trixi_hip_ctx* ctx, double* du, const double* u, double t, tri
     hipStream_t s = (hipStream_t)stream;
     if (ctx->cfg.volume_integral == TRIXI_HIP_VOLUME_SHOCK_CAPTURING_HG) {
         int rc = launch_indicator(ctx, u, s);
+        if (rc) return rc;
+    }
+    {
+        int rc = launch_mortars(ctx, u, s);
         if (rc) return rc;
     }
     RhsArgs a = ctx->args;
@@ -1101,6 +1284,10 @@ static int enqueue_rk_stages(trixi_hip_ctx* c, double* u, double* k, hipStream_t
             int rc = launch_indicator(c, cur, s);
             if (rc) return rc;
         }
+        {
+            int rc = launch_mortars(c, cur, s);
+            if (rc) return rc;
+        }
         RhsArgs a = c->args;
         a.u = cur;
         a.k = k;
@@ -1132,6 +1319,10 @@ int trixi_hip_rk_stage(trixi_hip_ctx* ctx, const double* u, double* k, double* u
     HIP_TRY(hipGetLastError());
     if (ctx->cfg.volume_integral == TRIXI_HIP_VOLUME_SHOCK_CAPTURING_HG) {
         int rc = launch_indicator(ctx, u, s);
+        if (rc) return rc;
+    }
+    {
+        int rc = launch_mortars(ctx, u, s);
         if (rc) return rc;
     }
     RhsArgs a = ctx->args;
@@ -1272,7 +1463,7 @@ int trixi_hip_indicator(trixi_hip_ctx* ctx, const double* u, double* alpha, trix
     if (rc) return rc;
     if (ctx->cfg.alpha_smooth) {
         hipLaunchKernelGGL(smooth_alpha_kernel, dim3((unsigned)((ctx->E + 255) / 256)), dim3(256), 0, s,
-                           ctx->d_alpha_raw, alpha, ctx->d_nbr, ctx->dim, ctx->E);
+                           ctx->d_alpha_raw, alpha, ctx->d_nbr, ctx->dim, ctx->E, ctx->d_mnb, ctx->mK);
     } else {
         hipLaunchKernelGGL(copy_kernel, dim3(256), dim3(256), 0, s, alpha, ctx->d_alpha_raw, ctx->E);
     }
@@ -1312,6 +1503,7 @@ static bool is_hg(const trixi_hip_ctx* c) {
 
 static int require_vjp(trixi_hip_ctx* c) {
     if (c->vjp_fn) return TRIXI_HIP_OK;
+    if (c->nmortars > 0) return fail(TRIXI_HIP_EUNSUPPORTED, "no adjoint kernel for meshes with mortars");
     KernelKey k = make_key(c->cfg, 0);
     k.prec = 0;
     VjpFn fn = find_vjp(k);

@@ -49,6 +49,10 @@ struct CMesh
     inverse_jacobian::Ptr{Float64}
     node_coordinates::Ptr{Float64}
     total_volume::Float64
+    nmortars::Int64
+    mortar_neighbor_ids::Ptr{Int32}
+    mortar_orientations::Ptr{Int32}
+    mortar_large_sides::Ptr{Int32}
 end
 
 function check(rc)
@@ -161,9 +165,9 @@ function SemidiscretizationHyperbolicHIP(mesh::TreeMesh, equations, initial_cond
                                          solver::DGSEMHIP;
                                          source_terms = nothing,
                                          boundary_conditions = boundary_condition_periodic)
-    # Trixi builds the element and interface containers on the host (faces on a
-    # non-periodic boundary get no interface: the library treats them as
-    # boundary faces).
+    # Trixi builds the element, interface and L2 mortar containers on the host
+    # (faces on a non-periodic boundary get no interface: the library treats
+    # them as boundary faces).
     dg_cpu = DGSEM(polydeg = solver.polydeg, surface_flux = flux_central)
     cache = Trixi.create_cache(mesh, equations, dg_cpu, Float64, Float64)
     e, i = cache.elements, cache.interfaces
@@ -171,10 +175,20 @@ function SemidiscretizationHyperbolicHIP(mesh::TreeMesh, equations, initial_cond
     ori = Int32.(i.orientations)
     invj = Vector{Float64}(e.inverse_jacobian)
     xyz = Array{Float64}(e.node_coordinates)     # [d, i, j, k, e]
+    if hasproperty(cache, :mortars)
+        m = cache.mortars                        # neighbor_ids [2^(d-1)+1, nmortars]
+        mids = Int32.(m.neighbor_ids .- 1)
+        mori = Int32.(m.orientations)
+        mls = Int32.(m.large_sides)
+    else
+        mids = Int32[]; mori = Int32[]; mls = Int32[]
+    end
     cfg = make_config(solver, equations, mesh; boundary_conditions, source_terms)
-    GC.@preserve nbr ori invj xyz begin
+    GC.@preserve nbr ori invj xyz mids mori mls begin
         cmesh = CMesh(nelements(e), ninterfaces(i), pointer(nbr), pointer(ori), pointer(invj),
-                      pointer(xyz), mesh.tree.length_level_0^ndims(mesh))
+                      pointer(xyz), mesh.tree.length_level_0^ndims(mesh), length(mori),
+                      isempty(mori) ? C_NULL : pointer(mids), isempty(mori) ? C_NULL : pointer(mori),
+                      isempty(mori) ? C_NULL : pointer(mls))
         ctx = Ref{Ptr{Cvoid}}(C_NULL)
         check(ccall((:trixi_hip_create, libtrixihip), Cint,
                     (Ref{CConfig}, Ref{CMesh}, Ref{Ptr{Cvoid}}), cfg, cmesh, ctx))

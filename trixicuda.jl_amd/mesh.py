@@ -1,4 +1,4 @@
-"""Uniform TreeMesh on the host (periodic or not).
+"""TreeMesh on the host: uniform or with refinement patches, periodic or not.
 
 Mirrors what the reference obtains from Trixi's TreeMesh and
 init_elements!/init_interfaces! before copying to the GPU
@@ -13,13 +13,24 @@ init_elements!/init_interfaces! before copying to the GPU
   periodic directions); in a non-periodic direction the faces on the domain
   boundary get no interface and are the mesh's boundary faces;
 * inverse Jacobian 2/dx, node coordinates centre + dx/2 * LGL node.
+
+With refinement patches the tree is built as Trixi's TreeMesh does it
+(abstract_tree.jl refine!, refine_box!, rebalance!): refining a cell inserts
+its 2^d children directly behind it (so the element order is the depth-first
+order of the leaves); "box" patches refine the leaves whose centre lies
+strictly inside the box; after each refinement the tree is rebalanced to 2:1
+across faces.  Interfaces join same-level leaves (+ directions), mortars join
+a leaf to the 2^(d-1) children of its same-level neighbour (L2 mortars, the
+container of Trixi's init_mortars!: small elements in position order, first
+tangential direction fastest, then the large element), and faces without any
+neighbour are boundary faces.
 """
 import numpy as np
 
 
 class TreeMesh:
     def __init__(self, coordinates_min, coordinates_max, initial_refinement_level, n_cells_max=None,
-                 periodicity=True):
+                 periodicity=True, refinement_patches=()):
         cmin = np.atleast_1d(np.asarray(coordinates_min, dtype=np.float64))
         cmax = np.atleast_1d(np.asarray(coordinates_max, dtype=np.float64))
         if cmin.shape != cmax.shape or cmin.ndim != 1 or not 1 <= cmin.size <= 3:
@@ -32,9 +43,18 @@ class TreeMesh:
         if len(per) != nd:
             raise ValueError("periodicity needs one entry per dimension")
         self.periodicity = per
-        self.level = L = int(initial_refinement_level)
+        self.nmortars = 0
+        self.mortar_neighbor_ids = np.zeros((0, (1 << (nd - 1)) + 1), dtype=np.int32)
+        self.mortar_orientations = np.zeros(0, dtype=np.int32)
+        self.mortar_large_sides = np.zeros(0, dtype=np.int32)
         self.coordinates_min, self.coordinates_max = cmin, cmax
         self.length = float(lengths[0])
+        if refinement_patches:
+            if nd == 1:
+                raise NotImplementedError("refinement patches are built for 2D and 3D meshes (L2 mortars)")
+            self._build_tree(cmin, cmax, int(initial_refinement_level), per, refinement_patches, n_cells_max)
+            return
+        self.level = L = int(initial_refinement_level)
         self.nelements = E = 1 << (nd * L)
         if n_cells_max is not None and E > n_cells_max:
             raise ValueError(f"n_cells_max={n_cells_max} too small for {E} leaf cells")
@@ -91,10 +111,152 @@ class TreeMesh:
         self.centers = centers
         self.total_volume = self.length ** nd
 
+    # -- general tree (refinement patches) -------------------------------------
+    def _build_tree(self, cmin, cmax, level0, per, patches, n_cells_max):
+        nd = self.ndims
+        nch = 1 << nd
+        length = float(cmax[0] - cmin[0])
+        # cell: [level, integer coords at that level, children ids or None]
+        cells = [[0, (0,) * nd, None]]
+        order = [0]                       # storage order (Trixi's cell array)
+        index = {(0, (0,) * nd): 0}       # (level, coords) -> cell id
+
+        def same_level_neighbour(cid, d, side):
+            lvl, k, _ = cells[cid]
+            n1 = 1 << lvl
+            kk = list(k)
+            kk[d] += 1 if side else -1
+            if not 0 <= kk[d] < n1:
+                if not per[d]:
+                    return None
+                kk[d] %= n1
+            return index.get((lvl, tuple(kk)))
+
+        def refine_unbalanced(ids):
+            pos = {c: i for i, c in enumerate(order)}
+            new_order, done = [], []
+            todo = set(ids)
+            for c in order:
+                new_order.append(c)
+                if c in todo:
+                    lvl, k, ch = cells[c]
+                    assert ch is None, "only leaves are refined"
+                    kids = []
+                    for child in range(nch):
+                        kc = tuple(2 * k[d] + ((child >> d) & 1) for d in range(nd))
+                        cells.append([lvl + 1, kc, None])
+                        cid = len(cells) - 1
+                        index[(lvl + 1, kc)] = cid
+                        kids.append(cid)
+                    cells[c][2] = kids
+                    new_order.extend(kids)
+                    done.append(c)
+            order[:] = new_order
+            del pos
+            return done
+
+        def rebalance(refined):
+            need = []
+            for c in refined:
+                lvl, k, _ = cells[c]
+                if lvl == 0:
+                    continue
+                parent = index[(lvl - 1, tuple(x >> 1 for x in k))]
+                for d in range(nd):
+                    for side in (0, 1):
+                        if same_level_neighbour(c, d, side) is not None:
+                            continue
+                        coarse = same_level_neighbour(parent, d, side)
+                        if coarse is None:
+                            continue  # domain boundary
+                        if cells[coarse][2] is None:
+                            need.append(coarse)
+            return refine_unbalanced(sorted(set(need)))
+
+        def refine(ids):
+            refined = refine_unbalanced(ids)
+            while refined:
+                refined = rebalance(refined)
+
+        def leaves():
+            return [c for c in order if cells[c][2] is None]
+
+        def centre(c):
+            lvl, k, _ = cells[c]
+            # halving from the root centre, as the tree computes child centres
+            out = np.empty(nd)
+            for d in range(nd):
+                cc, ln = 0.5 * (cmin[d] + cmax[d]), length
+                for b in range(lvl - 1, -1, -1):
+                    ln *= 0.5
+                    cc += (0.5 if (k[d] >> b) & 1 else -0.5) * ln
+                out[d] = cc
+            return out
+
+        for _ in range(level0):
+            refine([c for c in leaves() if cells[c][0] < level0])
+        for patch in patches:
+            ptype = patch["type"] if isinstance(patch, dict) else patch.type
+            if ptype != "box":
+                raise NotImplementedError("only box refinement patches are built")
+            lo = np.asarray(patch["coordinates_min"] if isinstance(patch, dict) else patch.coordinates_min, float)
+            hi = np.asarray(patch["coordinates_max"] if isinstance(patch, dict) else patch.coordinates_max, float)
+            refine([c for c in leaves() if np.all(lo < centre(c)) and np.all(hi > centre(c))])
+
+        elems = leaves()
+        E = len(elems)
+        if n_cells_max is not None and len(cells) > n_cells_max:
+            raise ValueError(f"n_cells_max={n_cells_max} too small for {len(cells)} cells")
+        c2e = {c: i for i, c in enumerate(elems)}
+        self.nelements = E
+        self.level = max(cells[c][0] for c in elems)
+        self.levels = np.array([cells[c][0] for c in elems], dtype=np.int32)
+        self.cell_index = np.array([cells[c][1] for c in elems], dtype=np.int32).reshape(E, nd)
+        self.centers = np.array([centre(c) for c in elems]).reshape(E, nd)
+        self.inverse_jacobian = np.array([1.0 / ((length / (1 << cells[c][0])) / 2.0) for c in elems])
+        ifc, mort, bnd = [], [], []
+        for e, c in enumerate(elems):
+            for d in range(nd):
+                for side in (0, 1):
+                    nb = same_level_neighbour(c, d, side)
+                    if nb is None:
+                        lvl, k, _ = cells[c]
+                        if lvl > 0:
+                            parent = index[(lvl - 1, tuple(x >> 1 for x in k))]
+                            if same_level_neighbour(parent, d, side) is not None:
+                                continue  # small side of a mortar: owned by the large element
+                        bnd.append((e, 2 * d + side))
+                        continue
+                    if cells[nb][2] is None:
+                        if side == 1:
+                            ifc.append((e, c2e[nb], d + 1))
+                        continue
+                    # mortar: the neighbour's children on the face towards this element
+                    tdims = [q for q in range(nd) if q != d]
+                    small = []
+                    for pos in range(1 << (nd - 1)):
+                        child = 0 if side else (1 << d)
+                        for t, q in enumerate(tdims):
+                            child |= ((pos >> t) & 1) << q
+                        small.append(c2e[cells[nb][2][child]])
+                    mort.append((small + [e], d + 1, 1 if side else 2))
+        self.neighbor_ids = np.array([(a, b) for a, b, _ in ifc], dtype=np.int32).reshape(-1, 2)
+        self.orientations = np.array([o for _, _, o in ifc], dtype=np.int32)
+        self.ninterfaces = len(ifc)
+        self.boundary_faces = np.array(sorted(bnd), dtype=np.int32).reshape(-1, 2)
+        self.nmortars = len(mort)
+        self.mortar_neighbor_ids = np.array([m[0] for m in mort], dtype=np.int32).reshape(-1, (1 << (nd - 1)) + 1)
+        self.mortar_orientations = np.array([m[1] for m in mort], dtype=np.int32)
+        self.mortar_large_sides = np.array([m[2] for m in mort], dtype=np.int32)
+        self.total_volume = length ** nd
+
     def node_coordinates(self, nodes):
         """Node coordinates [E, (N,)*nd, nd] in Trixi order (i fastest = last axis)."""
         nd, N = self.ndims, len(nodes)
-        jac = (self.length / (1 << self.level)) / 2.0
+        if hasattr(self, "levels"):
+            jac = ((self.length / (1 << self.levels)) / 2.0).reshape((-1,) + (1,) * nd)
+        else:
+            jac = (self.length / (1 << self.level)) / 2.0
         shape = (self.nelements,) + (N,) * nd + (nd,)
         x = np.empty(shape)
         nodes = np.asarray(nodes, dtype=np.float64)
@@ -102,7 +264,7 @@ class TreeMesh:
             # node axes are (k, j, i): direction d varies along node axis nd-1-d
             shp = [1] * nd
             shp[nd - 1 - d] = N
-            x[..., d] = self.centers[:, d].reshape((-1,) + (1,) * nd) + (jac * nodes).reshape(shp)[None]
+            x[..., d] = self.centers[:, d].reshape((-1,) + (1,) * nd) + jac * nodes.reshape(shp)[None]
         return x
 
     def __repr__(self):

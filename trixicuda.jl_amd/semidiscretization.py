@@ -55,6 +55,14 @@ class SemidiscretizationHyperbolicHIP:
         md.inverse_jacobian = self._invj.ctypes.data
         md.node_coordinates = self._node_coordinates.ctypes.data
         md.total_volume = mesh.total_volume
+        md.nmortars = mesh.nmortars
+        if mesh.nmortars:
+            self._mids = np.ascontiguousarray(mesh.mortar_neighbor_ids, dtype=np.int32)
+            self._mori = np.ascontiguousarray(mesh.mortar_orientations, dtype=np.int32)
+            self._mls = np.ascontiguousarray(mesh.mortar_large_sides, dtype=np.int32)
+            md.mortar_neighbor_ids = self._mids.ctypes.data
+            md.mortar_orientations = self._mori.ctypes.data
+            md.mortar_large_sides = self._mls.ctypes.data
         handle = ctypes.c_void_p()
         L.check(L.lib().trixi_hip_create(ctypes.byref(self.config), ctypes.byref(md), ctypes.byref(handle)))
         self._ctx = handle
