@@ -97,6 +97,14 @@ enum trixi_hip_source_terms {
     TRIXI_HIP_SOURCE_CONVERGENCE_TEST = 1 /* source_terms_convergence_test (Euler)  */
 };
 
+/* IndicatorHennemannGassner(...; variable) (Trixi's density_pressure,
+ * pressure and density functions of compressible Euler). */
+enum trixi_hip_indicator_variable {
+    TRIXI_HIP_INDICATOR_DENSITY_PRESSURE = 0,
+    TRIXI_HIP_INDICATOR_PRESSURE = 1,
+    TRIXI_HIP_INDICATOR_DENSITY = 2
+};
+
 /* Solver configuration: DGSEMGPU(; polydeg, surface_flux, volume_integral)
  * (reference src/solvers/dgsem_gpu.jl:43-53) plus the equation parameters. */
 typedef struct trixi_hip_config {
@@ -109,7 +117,7 @@ typedef struct trixi_hip_config {
     int32_t fv_flux;          /* HG subcell finite-volume flux                    */
     int32_t precision;        /* enum trixi_hip_precision                         */
     int32_t alpha_smooth;     /* HG: smooth blending factors with neighbours      */
-    int32_t reserved;
+    int32_t indicator_variable; /* HG: enum trixi_hip_indicator_variable          */
     double gamma;             /* ratio of specific heats (Euler, MHD)             */
     double c_h;               /* GLM divergence-cleaning speed                    */
     double advection_velocity[3];

@@ -23,6 +23,7 @@ FLUX_CENTRAL, FLUX_LLF, FLUX_LLF_NAIVE, FLUX_RANOCHA, FLUX_SHIMA_ETAL, FLUX_HIND
 IC_CONSTANT, IC_CONVERGENCE_TEST, IC_WEAK_BLAST_WAVE, IC_SEEDED_RANDOM, IC_SMOOTH_PERIODIC, IC_WEAK_BLAST_WAVE_SEEDED = range(6)
 BC_PERIODIC, BC_DIRICHLET = 0, 1
 SOURCE_NONE, SOURCE_CONVERGENCE_TEST = 0, 1
+IND_DENSITY_PRESSURE, IND_PRESSURE, IND_DENSITY = 0, 1, 2
 
 
 class OracleCfg(ctypes.Structure):
@@ -45,7 +46,7 @@ class OracleCfg(ctypes.Structure):
         ("boundary_ic", ctypes.c_int32),
         ("boundary_seed", ctypes.c_uint64),
         ("source_terms", ctypes.c_int32),
-        ("reserved", ctypes.c_int32),
+        ("indicator_variable", ctypes.c_int32),
         ("node_coordinates", ctypes.c_void_p),
         ("time", ctypes.c_double),
         ("nmortars", ctypes.c_int64),
@@ -120,8 +121,9 @@ def make_cfg(ndims, equation, polydeg=3, volume_integral=VI_WEAK, volume_flux=FL
              surface_flux=FLUX_LLF, fv_flux=FLUX_LLF, gamma=1.4, c_h=1.0,
              advection_velocity=(1.0, 1.0, 1.0), alpha_max=0.5, alpha_min=0.001,
              alpha_smooth=True, ic_length=4.0, boundary_condition=BC_PERIODIC, boundary_ic=IC_CONSTANT,
-             boundary_seed=0, source_terms=SOURCE_NONE):
+             boundary_seed=0, source_terms=SOURCE_NONE, indicator_variable=0):
     c = OracleCfg()
+    c.indicator_variable = indicator_variable
     c.boundary_condition, c.boundary_ic, c.boundary_seed = boundary_condition, boundary_ic, boundary_seed
     c.source_terms = source_terms
     c.ndims, c.equation, c.polydeg = ndims, equation, polydeg
@@ -450,6 +452,7 @@ def rhs_jvp(cfg, mesh, u, udot, t=0.0):
 
 
 def indicator_hg(cfg, mesh, u):
+    cfg = _bind(cfg, mesh)
     u = np.ascontiguousarray(u, dtype=np.float64)
     alpha = np.zeros(mesh.nelements)
     lib().oracle_indicator_hg(ctypes.byref(cfg), mesh.nelements, mesh.ninterfaces, _p(mesh.neighbor_ids),

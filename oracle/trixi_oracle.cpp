@@ -82,7 +82,7 @@ constexpr int MAXV = 9;
 struct Cfg {
     int nd, eq, N, nv, vi, vflux, sflux, fvflux, smooth;
     double gamma, inv_gm1, c_h, a[3], amax, amin, ic_length;
-    int bc, bc_ic, src;
+    int bc, bc_ic, src, ivar;
     uint64_t bc_seed;
     const double* xnodes;
     double t;
@@ -114,6 +114,7 @@ Cfg make_cfg(const oracle_cfg* c) {
     k.src = c->source_terms;
     k.xnodes = c->node_coordinates;
     k.t = c->time;
+    k.ivar = c->indicator_variable;
     k.nmortars = c->nmortars;
     k.mids = c->mortar_neighbor_ids;
     k.mori = c->mortar_orientations;
@@ -719,9 +720,14 @@ void fv_element(const Cfg& c, const Basis& b, const T* ue, T* due, T alpha) {
 // Hennemann-Gassner indicator (reference callable src/solvers/indicators.jl:7-42
 // with Trixi calc_indicator_hennemann_gassner! and apply_smoothing!).
 // ----------------------------------------------------------------------------
-template <class T> T density_pressure(const Cfg& c, const T* u) {
+// The indicator variable of compressible Euler: Trixi's density_pressure
+// rho p = (gamma - 1)(rho rho_e - |rho v|^2 / 2), pressure
+// p = (gamma - 1)(rho_e - |rho v|^2 / (2 rho)) or density rho.
+template <class T> T indicator_variable(const Cfg& c, const T* u) {
+    if (c.ivar == 2) return u[0];
     T ke = 0.0;
     for (int d = 0; d < c.nd; ++d) ke = (d == 0) ? u[1 + d] * u[1 + d] : ke + u[1 + d] * u[1 + d];
+    if (c.ivar == 1) return (c.gamma - 1.0) * (u[c.nd + 1] - 0.5 * ke / u[0]);
     return (c.gamma - 1.0) * (u[0] * u[c.nd + 1] - 0.5 * ke);
 }
 
@@ -730,7 +736,7 @@ T indicator_element(const Cfg& c, const Basis& b, const T* ue, double threshold,
                     double parameter_s) {
     const int N = c.N, nd = c.nd, nv = c.nv, nn = ipow(N, nd);
     std::vector<T> ind(nn), tmp(nn);
-    for (int n = 0; n < nn; ++n) ind[n] = density_pressure(c, ue + nv * n);
+    for (int n = 0; n < nn; ++n) ind[n] = indicator_variable(c, ue + nv * n);
     // multiply_scalar_dimensionwise!: apply Vinv along x, then y, then z.
     for (int d = 0; d < nd; ++d) {
         const int s = ipow(N, d);
@@ -775,6 +781,18 @@ void indicator_hg(const Cfg& c, const Basis& b, int64_t E, int64_t I, const int3
             const int32_t l = nbr[2 * i], r = nbr[2 * i + 1];
             alpha[l] = tmax(tmax(tmp[l], 0.5 * tmp[r]), alpha[l]);
             alpha[r] = tmax(tmax(tmp[r], 0.5 * tmp[l]), alpha[r]);
+        }
+        // L2 mortars: each small element with the large one (Trixi's
+        // apply_smoothing! loops over the mortars after the interfaces)
+        const int K = 1 << (c.nd - 1);
+        for (int64_t m = 0; m < c.nmortars; ++m) {
+            const int32_t* ids = c.mids + m * (K + 1);
+            const int32_t large = ids[K];
+            for (int p = 0; p < K; ++p) {
+                const int32_t s = ids[p];
+                alpha[s] = tmax(tmax(tmp[s], 0.5 * tmp[large]), alpha[s]);
+                alpha[large] = tmax(tmax(tmp[large], 0.5 * tmp[s]), alpha[large]);
+            }
         }
     }
 }

@@ -64,7 +64,7 @@ typedef struct oracle_cfg {
     int32_t boundary_ic;      /* Dirichlet state: OR_IC_*                        */
     uint64_t boundary_seed;
     int32_t source_terms;     /* OR_SOURCE_*                                     */
-    int32_t reserved;
+    int32_t indicator_variable; /* HG: 0 density*pressure, 1 pressure, 2 density */
     const double* node_coordinates; /* [E][nodes][d], for boundary states and sources */
     double time;              /* rhs time; the start of the step for the RK step JVP */
     int64_t nmortars;         /* L2 mortars (Trixi init_mortars! container):          */

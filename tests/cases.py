@@ -32,6 +32,7 @@ class Case:
     source: str = ""        # "convergence_test" for source_terms_convergence_test
     t: float = 0.0          # time of the rhs! evaluation
     patches: tuple = ()     # box refinement patches ((lo, hi), ...): L2 mortars
+    indicator: str = "density_pressure"  # HG indicator variable: density_pressure | pressure | density
 
 
 FLUX_NAMES = ("central", "llf", "llf_naive", "ranocha", "shima_etal", "hindenlang_gassner")
@@ -54,7 +55,8 @@ def oracle_cfg(O, c: Case):
     return O.make_cfg(c.ndims, eq, c.polydeg, vi, FLUX_NAMES.index(c.vflux), FLUX_NAMES.index(c.sflux),
                       FLUX_NAMES.index(c.fvflux), gamma=c.gamma, c_h=c.c_h,
                       advection_velocity=c.advection[:c.ndims], alpha_max=c.alpha_max, alpha_min=c.alpha_min,
-                      alpha_smooth=c.alpha_smooth, ic_length=c.cmax - c.cmin, **ext)
+                      alpha_smooth=c.alpha_smooth, ic_length=c.cmax - c.cmin,
+                      indicator_variable=("density_pressure", "pressure", "density").index(c.indicator), **ext)
 
 
 def oracle_ic(O, c: Case):
@@ -85,7 +87,7 @@ def product_semi(th, c: Case, precision=None):
         vi = th.VolumeIntegralFluxDifferencing(_flux(th, c.vflux, mhd))
     else:
         ind = th.IndicatorHennemannGassner(eq, basis, alpha_max=c.alpha_max, alpha_min=c.alpha_min,
-                                           alpha_smooth=c.alpha_smooth, variable=th.density_pressure)
+                                           alpha_smooth=c.alpha_smooth, variable=getattr(th, c.indicator))
         vi = th.VolumeIntegralShockCapturingHG(ind, volume_flux_dg=_flux(th, c.vflux, mhd),
                                                volume_flux_fv=_flux(th, c.fvflux, mhd))
     solver = th.DGSEMHIP(polydeg=c.polydeg, surface_flux=_flux(th, c.sflux, mhd), volume_integral=vi,

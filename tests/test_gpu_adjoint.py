@@ -54,6 +54,9 @@ TINY = [
     Case("euler2d_hg_shima_llfnaive_amax1", 2, "euler", 2, "hg", "shima_etal", "llf_naive", "llf_naive",
          alpha_max=1.0),
     Case("euler1d_hg_ranocha_blast", 1, "euler", 2, "hg", "ranocha", ic="weak_blast_wave_seeded"),
+    # indicator variables pressure and density
+    Case("euler2d_hg_pressure_blast", 2, "euler", 1, "hg", "ranocha", ic="weak_blast_wave_seeded", indicator="pressure"),
+    Case("euler2d_hg_density_random", 2, "euler", 2, "hg", "ranocha", indicator="density"),
     # non-periodic: Dirichlet states do not depend on u, the source depends on x and t only
     Case("euler2d_weak_dirichlet_src", 2, "euler", 1, cmin=0.0, cmax=2.0, periodic=False, bc_state="convergence_test",
          source="convergence_test", t=0.3),

@@ -430,3 +430,90 @@ def test_mortar_meshes_converge_to_the_time_derivative(oracle, nd, eq, vi, per, 
         errs.append(np.abs(O.rhs(cfg, mesh, u, t=t) - ut).max())
     rate = np.log2(errs[0] / errs[1])
     assert rate > 2.5, (errs, rate)
+
+
+def _prim2cons(rho, v, p, gamma):
+    return np.concatenate([rho[..., None], rho[..., None] * v, (p / (gamma - 1) + 0.5 * rho * (v ** 2).sum(-1))[..., None]],
+                          axis=-1)
+
+
+@pytest.mark.parametrize("nd", [1, 2, 3])
+def test_indicator_variables_reduce_to_density_pressure(oracle, nd):
+    """pressure and density as HG indicator variables: with rho = 1 the
+    pressure indicator equals the density_pressure one, with p = 1 the
+    density indicator does (rho p is then p, resp. rho); and on a state whose
+    density carries the top mode while the pressure is smooth, the density
+    indicator fires where the pressure one does not."""
+    O = oracle
+    gamma = 1.4
+    mesh = O.Mesh(nd, {1: 4, 2: 2, 3: 1}[nd], -1.0, 1.0, 3)
+    shape = mesh.node_coordinates.shape[:-1]
+    rng = np.random.default_rng(nd)
+    mk = lambda var: O.make_cfg(nd, O.EQ_EULER, 3, O.VI_HG, O.FLUX_RANOCHA, O.FLUX_LLF, O.FLUX_LLF, gamma=gamma,
+                                alpha_smooth=False, indicator_variable=var)
+    v = 0.3 * rng.standard_normal(shape + (nd,))
+    field = 1.0 + 0.3 * rng.uniform(-1, 1, shape)
+    one = np.ones(shape)
+    u = _prim2cons(one, v, field, gamma)      # rho = 1
+    a_dp = O.indicator_hg(mk(O.IND_DENSITY_PRESSURE), mesh, u)
+    assert a_dp.max() > 0.1
+    np.testing.assert_allclose(O.indicator_hg(mk(O.IND_PRESSURE), mesh, u), a_dp, rtol=1e-12, atol=1e-14)
+    u = _prim2cons(field, v, one, gamma)      # p = 1
+    a_dp = O.indicator_hg(mk(O.IND_DENSITY_PRESSURE), mesh, u)
+    np.testing.assert_allclose(O.indicator_hg(mk(O.IND_DENSITY), mesh, u), a_dp, rtol=1e-12, atol=1e-14)
+    # top Legendre mode in rho only (element 0), p constant
+    V = np.linalg.inv(O.basis(4)["Vinv"])
+    rho = np.ones(shape)
+    rho[0] = 1.0 + 0.2 * V[:, 3].reshape((1,) * (nd - 1) + (4,))
+    u = _prim2cons(rho, 0 * v, one, gamma)
+    assert O.indicator_hg(mk(O.IND_DENSITY), mesh, u)[0] > 0.1
+    assert np.all(O.indicator_hg(mk(O.IND_PRESSURE), mesh, u) == 0.0)
+
+
+def test_hg_smoothing_crosses_mortars(oracle):
+    """Blending-factor smoothing over interfaces and mortars: a small element
+    takes half of its large neighbour's factor and the large one half of each
+    small neighbour's, as across an interface."""
+    O = oracle
+    nd, lvl, a, b, patches = MORTAR_MESHES["advection_mortar_2d"]
+    mesh = O.RefinedMesh(nd, lvl, a, b, 3, patches)
+    cfg = O.make_cfg(2, O.EQ_EULER, 3, O.VI_HG, O.FLUX_RANOCHA, O.FLUX_LLF, O.FLUX_LLF, alpha_smooth=False)
+    u = O.compute_coefficients(cfg, mesh, O.IC_CONSTANT)
+    V = np.linalg.inv(O.basis(4)["Vinv"])
+    m = 0
+    ids = mesh.mortar_neighbor_ids[m]
+    large, small = ids[2], ids[:2]
+    rho = u[..., 0].copy()
+    rho[large] = 1.0 + 0.2 * V[:, 3][:, None] * np.ones((1, 4))
+    u2 = u.copy()
+    u2[..., 0] = rho
+    raw = O.indicator_hg(cfg, mesh, u2)
+    assert raw[large] > 0.1 and np.count_nonzero(raw) == 1
+    cfg.alpha_smooth = 1
+    sm = O.indicator_hg(cfg, mesh, u2)
+    for s in small:
+        assert sm[s] == pytest.approx(0.5 * raw[large])
+    # the other direction: the small element's factor reaches the large one
+    u3 = u.copy()
+    rho = u[..., 0].copy()
+    rho[small[0]] = 1.0 + 0.2 * V[:, 3][:, None] * np.ones((1, 4))
+    u3[..., 0] = rho
+    cfg.alpha_smooth = 0
+    raw = O.indicator_hg(cfg, mesh, u3)
+    cfg.alpha_smooth = 1
+    assert O.indicator_hg(cfg, mesh, u3)[large] == pytest.approx(0.5 * raw[small[0]])
+
+
+def test_hg_on_mortar_meshes_is_conservative(oracle):
+    O = oracle
+    for name in ("advection_mortar_2d", "euler_mortar_3d"):
+        nd, lvl, a, b, patches = MORTAR_MESHES[name]
+        mesh = O.RefinedMesh(nd, lvl, a, b, 3, patches)
+        cfg = O.make_cfg(nd, O.EQ_EULER, 3, O.VI_HG, O.FLUX_RANOCHA, O.FLUX_LLF, O.FLUX_LLF, alpha_min=0.0,
+                         indicator_variable=O.IND_PRESSURE)
+        u = O.compute_coefficients(cfg, mesh, O.IC_CONSTANT)
+        assert np.abs(O.rhs(cfg, mesh, u)).max() < 1e-12
+        u = O.compute_coefficients(cfg, mesh, O.IC_WEAK_BLAST_WAVE_SEEDED)
+        du, alpha = O.rhs(cfg, mesh, u, return_alpha=True)
+        assert alpha.max() > 0.05
+        assert np.abs(O.integrate(cfg, mesh, du)).max() < 1e-13 * np.abs(du).max() * mesh.total_volume

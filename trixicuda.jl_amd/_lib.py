@@ -38,7 +38,7 @@ class Config(ctypes.Structure):
         ("fv_flux", ctypes.c_int32),
         ("precision", ctypes.c_int32),
         ("alpha_smooth", ctypes.c_int32),
-        ("reserved", ctypes.c_int32),
+        ("indicator_variable", ctypes.c_int32),
         ("gamma", ctypes.c_double),
         ("c_h", ctypes.c_double),
         ("advection_velocity", ctypes.c_double * 3),

@@ -4,7 +4,8 @@
 // (linear advection, weak form, LLF) and test/tree_dgsem_3d/euler_mortar.jl
 // (compressible Euler, weak form, LLF, source terms); the compressible Euler
 // rhs variants with boundary faces (weak form and flux differencing with
-// flux_ranocha, LLF surface flux) also read mortar faces.
+// flux_ranocha, LLF surface flux) also read mortar faces, and so do the
+// shock-capturing variants below.
 #include "registry.hpp"
 
 namespace trixi_hip {
@@ -21,6 +22,10 @@ void register_mortar_kernels() {
     RegistrarExt<3, 4, LinAdv<3>, VI_WEAK, C, FLUX_LLF, C>::add(EQ_LINADV);
     RegistrarExt<2, 4, LinAdv<2>, VI_WEAK, C, C, C>::add(EQ_LINADV);
     RegistrarExt<3, 4, LinAdv<3>, VI_WEAK, C, C, C>::add(EQ_LINADV);
+    // shock capturing on refined or non-periodic meshes
+    // (test/tree_dgsem_2d/euler_blob_mortar.jl: DG flux_ranocha, FV and surface LLF)
+    RegistrarExt<2, 4, Euler<2>, VI_HG, FLUX_RANOCHA, FLUX_LLF, FLUX_LLF>::add(EQ_EULER);
+    RegistrarExt<3, 4, Euler<3>, VI_HG, FLUX_RANOCHA, FLUX_LLF, FLUX_LLF>::add(EQ_EULER);
 }
 
 }  // namespace trixi_hip

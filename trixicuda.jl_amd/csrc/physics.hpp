@@ -328,6 +328,19 @@ struct Euler {
         for (int d = 0; d < DIM; ++d) ke = (d == 0) ? u[1 + d] * u[1 + d] : ke + u[1 + d] * u[1 + d];
         return P.gm1 * (u[0] * u[DIM + 1] - R(0.5) * ke);
     }
+
+    // the HG indicator variable: 0 density * pressure, 1 pressure (Trixi
+    // pressure), 2 density
+    __device__ static double indicator_variable(const double* u, const PhysR<double>& P, int var) {
+        if (var == 2) return u[0];
+        if (var == 1) {
+            double ke = 0.0;
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) ke += u[1 + d] * u[1 + d];
+            return P.gm1 * (u[DIM + 1] - 0.5 * ke / u[0]);
+        }
+        return density_pressure(u, P);
+    }
 };
 
 // ===========================================================================

@@ -317,18 +317,19 @@ __device__ __forceinline__ int max_mode(int n) {
     return maxc;
 }
 
-// modal coefficients of density * pressure at this lane's node (all lanes of
-// the block take part: it synchronises)
+// modal coefficients of the indicator variable (density * pressure,
+// pressure or density) at this lane's node (all lanes of the block take part:
+// it synchronises)
 template <int DIM, int N>
-__device__ __forceinline__ double modal_density_pressure(const double* __restrict__ u, int64_t ec, int64_t E,
-                                                         const PhysR<double>& P, const double* __restrict__ Vinv,
-                                                         double* buf, int el, int n) {
+__device__ __forceinline__ double modal_indicator(const double* __restrict__ u, int64_t ec, int64_t E,
+                                                  const PhysR<double>& P, const double* __restrict__ Vinv,
+                                                  double* buf, int el, int n, int var) {
     constexpr int NN = cpow(N, DIM);
     constexpr int NV = DIM + 2;
     double uc[NV];
 #pragma unroll
     for (int v = 0; v < NV; ++v) uc[v] = u[((int64_t)v * NN + n) * E + ec];
-    double m = Euler<DIM>::density_pressure(uc, P);
+    double m = Euler<DIM>::indicator_variable(uc, P, var);
 #pragma unroll
     for (int d = 0; d < DIM; ++d) {
         const int s = cpow(N, d);
@@ -362,7 +363,7 @@ template <int DIM, int N>
 __global__ void __launch_bounds__(64) indicator_kernel(const double* __restrict__ u, double* __restrict__ alpha_raw,
                                                        int64_t E, PhysParams phys, const double* __restrict__ Vinv,
                                                        double threshold, double parameter_s, double alpha_min,
-                                                       double alpha_max) {
+                                                       double alpha_max, int var) {
     constexpr int NN = cpow(N, DIM);
     constexpr int EPB = 64 / NN;
     static_assert(NN <= 64 && 64 % NN == 0, "one block holds whole elements");
@@ -372,7 +373,7 @@ __global__ void __launch_bounds__(64) indicator_kernel(const double* __restrict_
     const int64_t e = (int64_t)blockIdx.x * EPB + el;
     const int64_t ec = e < E ? e : E - 1;
     const PhysR<double> P = PhysR<double>::from(phys);
-    const double m = modal_density_pressure<DIM, N>(u, ec, E, P, Vinv, buf, el, n);
+    const double m = modal_indicator<DIM, N>(u, ec, E, P, Vinv, buf, el, n, var);
     e2[threadIdx.x] = m * m;
     __syncthreads();
     if (n != 0 || e >= E) return;
@@ -426,7 +427,8 @@ template <int DIM, int N>
 __global__ void __launch_bounds__(64) indicator_vjp_kernel(
     const double* __restrict__ u, double* __restrict__ ubar, const double* __restrict__ alphabar,
     const int32_t* __restrict__ winner, const int32_t* __restrict__ nbr, int smooth, int64_t E, PhysParams phys,
-    const double* __restrict__ Vinv, double threshold, double parameter_s, double alpha_min, double alpha_max) {
+    const double* __restrict__ Vinv, double threshold, double parameter_s, double alpha_min, double alpha_max,
+    int var) {
     constexpr int NN = cpow(N, DIM);
     constexpr int NV = DIM + 2;
     constexpr int EPB = 64 / NN;
@@ -439,7 +441,7 @@ __global__ void __launch_bounds__(64) indicator_vjp_kernel(
     const bool valid = e < E;
     const int64_t ec = valid ? e : E - 1;
     const PhysR<double> P = PhysR<double>::from(phys);
-    const double m = modal_density_pressure<DIM, N>(u, ec, E, P, Vinv, buf, el, n);
+    const double m = modal_indicator<DIM, N>(u, ec, E, P, Vinv, buf, el, n, var);
     e2[threadIdx.x] = m * m;
     __syncthreads();
     if (n == 0) {
@@ -502,10 +504,29 @@ __global__ void __launch_bounds__(64) indicator_vjp_kernel(
         mb = r;
     }
     if (!active) return;
-    // density * pressure = (gamma - 1) (rho E - |rho v|^2 / 2)
-    const double g = mb * P.gm1;
     const int64_t r0 = (int64_t)n * E + e;
+    if (var == 2) {  // density
+        ubar[r0] += mb;
+        return;
+    }
+    const double g = mb * P.gm1;
     const double rho = u[r0], en = u[(int64_t)(DIM + 1) * NN * E + r0];
+    if (var == 1) {
+        // pressure = (gamma - 1) (rho E - |rho v|^2 / (2 rho))
+        const double irho = 1.0 / rho;
+        double ke = 0.0;
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) {
+            const int64_t rd = (int64_t)(1 + d) * NN * E + r0;
+            const double vd = u[rd] * irho;
+            ke += vd * vd;
+            ubar[rd] -= g * vd;
+        }
+        ubar[r0] += g * 0.5 * ke;
+        ubar[(int64_t)(DIM + 1) * NN * E + r0] += g;
+        return;
+    }
+    // density * pressure = (gamma - 1) (rho E - |rho v|^2 / 2)
     ubar[r0] += g * en;
 #pragma unroll
     for (int d = 0; d < DIM; ++d) {
@@ -842,13 +863,16 @@ static int launch_indicator(trixi_hip_ctx* c, const double* u, hipStream_t s) {
     const unsigned blocks = (unsigned)((c->E + epb - 1) / epb);
     if (c->dim == 1)
         hipLaunchKernelGGL((indicator_kernel<1, 4>), dim3(blocks), dim3(64), 0, s, u, c->d_alpha_raw, c->E, P,
-                           c->d_Vinv, threshold, parameter_s, c->cfg.alpha_min, c->cfg.alpha_max);
+                           c->d_Vinv, threshold, parameter_s, c->cfg.alpha_min, c->cfg.alpha_max,
+                           c->cfg.indicator_variable);
     else if (c->dim == 2)
         hipLaunchKernelGGL((indicator_kernel<2, 4>), dim3(blocks), dim3(64), 0, s, u, c->d_alpha_raw, c->E, P,
-                           c->d_Vinv, threshold, parameter_s, c->cfg.alpha_min, c->cfg.alpha_max);
+                           c->d_Vinv, threshold, parameter_s, c->cfg.alpha_min, c->cfg.alpha_max,
+                           c->cfg.indicator_variable);
     else
         hipLaunchKernelGGL((indicator_kernel<3, 4>), dim3(blocks), dim3(64), 0, s, u, c->d_alpha_raw, c->E, P,
-                           c->d_Vinv, threshold, parameter_s, c->cfg.alpha_min, c->cfg.alpha_max);
+                           c->d_Vinv, threshold, parameter_s, c->cfg.alpha_min, c->cfg.alpha_max,
+                           c->cfg.indicator_variable);
     HIP_TRY(hipGetLastError());
     return TRIXI_HIP_OK;
 }
@@ -963,6 +987,9 @@ int trixi_hip_create(const trixi_hip_config* config, const trixi_hip_mesh* mesh,
         return fail(TRIXI_HIP_EUNSUPPORTED, "GLM-MHD is built for 3D only");
     if (cfg.volume_integral == TRIXI_HIP_VOLUME_SHOCK_CAPTURING_HG && cfg.equation != TRIXI_HIP_COMPRESSIBLE_EULER)
         return fail(TRIXI_HIP_EUNSUPPORTED, "shock capturing is built for compressible Euler only");
+    if (cfg.indicator_variable < TRIXI_HIP_INDICATOR_DENSITY_PRESSURE ||
+        cfg.indicator_variable > TRIXI_HIP_INDICATOR_DENSITY)
+        return fail(TRIXI_HIP_EINVAL, "unknown indicator variable");
     if (cfg.boundary_condition != TRIXI_HIP_BC_PERIODIC) {
         if (cfg.boundary_condition != TRIXI_HIP_BC_DIRICHLET)
             return fail(TRIXI_HIP_EINVAL, "unknown boundary condition");
@@ -1540,7 +1567,7 @@ static int enqueue_vjp(trixi_hip_ctx* c, const VjpArgs& v, hipStream_t s) {
     hipLaunchKernelGGL((indicator_vjp_kernel<D, 4>), dim3(blocks), dim3(threads), 0, s, v.u, v.ubar,            \
                        (const double*)c->d_alphabar, (const int32_t*)c->d_winner, (const int32_t*)c->d_nbr,    \
                        c->cfg.alpha_smooth, c->E, P, (const double*)c->d_Vinv, threshold, parameter_s,       \
-                       c->cfg.alpha_min, c->cfg.alpha_max)
+                       c->cfg.alpha_min, c->cfg.alpha_max, c->cfg.indicator_variable)
         if (c->dim == 1)
             TRIXI_HIP_IND_VJP(1);
         else if (c->dim == 2)

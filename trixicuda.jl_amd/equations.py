@@ -118,7 +118,15 @@ flux_hindenlang_gassner = NumericalFlux("flux_hindenlang_gassner", L.FLUX_HINDEN
 flux_nonconservative_powell = NumericalFlux("flux_nonconservative_powell", -1)
 
 
-def density_pressure(*args):  # identity marker of the HG indicator variable
+def density_pressure(*args):  # identity markers of the HG indicator variables
+    raise TypeError("indicator variables are selected, not called, on the HIP path")
+
+
+def pressure(*args):
+    raise TypeError("indicator variables are selected, not called, on the HIP path")
+
+
+def density(*args):
     raise TypeError("indicator variables are selected, not called, on the HIP path")
 
 

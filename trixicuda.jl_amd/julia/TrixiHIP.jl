@@ -27,7 +27,7 @@ struct CConfig
     fv_flux::Int32
     precision::Int32
     alpha_smooth::Int32
-    reserved::Int32
+    indicator_variable::Int32
     gamma::Float64
     c_h::Float64
     advection_velocity::NTuple{3, Float64}
@@ -135,6 +135,11 @@ function boundary_config(bcs)
     (1, ic_id(f), ic_seed(f))
 end
 
+indicator_variable_id(::typeof(density_pressure)) = 0
+indicator_variable_id(::typeof(pressure)) = 1
+indicator_variable_id(::typeof(density)) = 2
+indicator_variable_id(f) = throw(ArgumentError("indicator variable $f is not built for HIP"))
+
 source_id(::Nothing) = 0
 source_id(::typeof(source_terms_convergence_test)) = 1
 source_id(f) = throw(ArgumentError("source terms $f are not built for HIP"))
@@ -142,13 +147,14 @@ source_id(f) = throw(ArgumentError("source terms $f are not built for HIP"))
 function make_config(solver::DGSEMHIP, equations, mesh; boundary_conditions = boundary_condition_periodic,
                      source_terms = nothing)
     vi = solver.volume_integral
-    volume_integral, vflux, fvflux, smooth, amax, amin = 0, 0, 0, 0, 0.5, 0.001
+    volume_integral, vflux, fvflux, smooth, amax, amin, var = 0, 0, 0, 0, 0.5, 0.001, 0
     if vi isa VolumeIntegralFluxDifferencing
         volume_integral, vflux = 1, flux_id(vi.volume_flux)
     elseif vi isa VolumeIntegralShockCapturingHG
         volume_integral, vflux, fvflux = 2, flux_id(vi.volume_flux_dg), flux_id(vi.volume_flux_fv)
         smooth = vi.indicator.alpha_smooth ? 1 : 0
         amax, amin = vi.indicator.alpha_max, vi.indicator.alpha_min
+        var = indicator_variable_id(vi.indicator.variable)
     end
     gamma = hasproperty(equations, :gamma) ? equations.gamma : 1.4
     c_h = hasproperty(equations, :c_h) ? equations.c_h : 0.0
@@ -158,7 +164,7 @@ function make_config(solver::DGSEMHIP, equations, mesh; boundary_conditions = bo
     bc, bc_ic, bc_seed = boundary_config(boundary_conditions)
     CConfig(ndims(mesh), equation_id(equations), solver.polydeg, volume_integral, vflux,
             flux_id(solver.surface_flux), fvflux, solver.precision === :mixed ? 1 : 0,
-            smooth, 0, gamma, c_h, vel, amax, amin, len, bc, bc_ic, bc_seed, source_id(source_terms), 0)
+            smooth, var, gamma, c_h, vel, amax, amin, len, bc, bc_ic, bc_seed, source_id(source_terms), 0)
 end
 
 function SemidiscretizationHyperbolicHIP(mesh::TreeMesh, equations, initial_condition,
