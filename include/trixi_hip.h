@@ -49,7 +49,7 @@ typedef void* trixi_hip_stream; /* a hipStream_t; NULL = the default stream */
 enum trixi_hip_equation {
     TRIXI_HIP_LINEAR_SCALAR_ADVECTION = 0, /* LinearScalarAdvectionEquation{1,2,3}D */
     TRIXI_HIP_COMPRESSIBLE_EULER = 1,      /* CompressibleEulerEquations{1,2,3}D    */
-    TRIXI_HIP_IDEAL_GLM_MHD = 2            /* IdealGlmMhdEquations3D                */
+    TRIXI_HIP_IDEAL_GLM_MHD = 2            /* IdealGlmMhdEquations2D / 3D           */
 };
 enum trixi_hip_volume_integral {
     TRIXI_HIP_VOLUME_WEAK_FORM = 0,          /* VolumeIntegralWeakForm          */
@@ -253,8 +253,9 @@ int trixi_hip_indicator(trixi_hip_ctx* ctx, const double* u, double* alpha,
  * asynchronous.  No reference counterpart (TrixiCUDA.jl has no adjoint); it
  * backs a ChainRulesCore.rrule / torch.autograd.Function of rhs_gpu!
  * (solvers.jl:18-31).  Built for linear advection, compressible Euler (weak
- * form and flux differencing) and ideal GLM-MHD 3D (flux differencing with the
- * nonconservative Powell terms); other configurations return
+ * form, flux differencing, HG shock capturing) and ideal GLM-MHD 2D and 3D
+ * (flux differencing with the nonconservative Powell terms) on conforming
+ * meshes; other configurations (and meshes with mortars) return
  * TRIXI_HIP_EUNSUPPORTED.  ubar must not alias u or vbar. */
 int trixi_hip_rhs_vjp(trixi_hip_ctx* ctx, double* ubar, const double* u, const double* vbar,
                       double t, double alpha, double beta, trixi_hip_stream stream);

@@ -147,7 +147,7 @@ template <class T> T inv_ln_mean(T x, T y) {
 // Equations.  State layouts (Trixi.jl conventions):
 //   LinAdv:  (u)
 //   Euler:   (rho, rho_v[1..nd], rho_e)
-//   GLM-MHD: (rho, rho_v1..3, rho_e, B1..3, psi)   (3D only here)
+//   GLM-MHD: (rho, rho_v1..3, rho_e, B1..3, psi)   (2D: orientations 1, 2)
 // Orientation o is 1-based as in Trixi.
 // ----------------------------------------------------------------------------
 template <class T> void euler_prim(const Cfg& c, const T* u, T& rho, T* v, T& p) {
@@ -1188,9 +1188,19 @@ void initial_condition(const Cfg& c, int ic, const double* x, double t, uint64_t
         euler_prim2cons(c, rho, v, p, u);
         return;
     }
-    // GLM-MHD 3D
+    // GLM-MHD
     double prim[9] = {1.0, 0, 0, 0, 1.0, 0, 0, 0, 0};
-    if (ic == OR_IC_CONVERGENCE_TEST) {
+    if (ic == OR_IC_CONVERGENCE_TEST && c.nd == 2) {
+        // Alfven wave of Trixi's IdealGlmMhdEquations2D convergence test
+        // (Derigs et al. 2016): rho = 1, p = 0.1, along x_perp = (x + y)/sqrt 2
+        const double alpha = 0.25 * M_PI;
+        const double x_perp = x[0] * cos(alpha) + x[1] * sin(alpha);
+        const double B_perp = 0.1 * sin(2.0 * M_PI * (x_perp + t));
+        const double v1 = -B_perp * sin(alpha), v2 = B_perp * cos(alpha);
+        const double v3 = 0.1 * cos(2.0 * M_PI * (x_perp + t));
+        const double pr[9] = {1.0, v1, v2, v3, 0.1, cos(alpha) + v1, sin(alpha) + v2, v3, 0.0};
+        memcpy(prim, pr, sizeof(pr));
+    } else if (ic == OR_IC_CONVERGENCE_TEST) {
         // Alfven wave (Trixi initial_condition_convergence_test, ideal GLM-MHD).
         const double p = 1.0, omega = 2.0 * M_PI, r = 2.0, e = 0.2;
         const double nx = 1.0 / sqrt(r * r + 1.0), ny = r / sqrt(r * r + 1.0), sqr = 1.0;

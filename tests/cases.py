@@ -79,7 +79,7 @@ def product_semi(th, c: Case, precision=None):
         eq = [None, th.CompressibleEulerEquations1D, th.CompressibleEulerEquations2D,
               th.CompressibleEulerEquations3D][c.ndims](c.gamma)
     else:
-        eq = th.IdealGlmMhdEquations3D(c.gamma, c.c_h)
+        eq = [None, None, th.IdealGlmMhdEquations2D, th.IdealGlmMhdEquations3D][c.ndims](c.gamma, c.c_h)
     basis = th.LobattoLegendreBasisHIP(c.polydeg)
     if c.vi == "weak":
         vi = th.VolumeIntegralWeakForm()

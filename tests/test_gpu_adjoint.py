@@ -66,6 +66,9 @@ TINY = [
          gamma=5 / 3),
     Case("mhd3d_fd_hg_hg", 3, "mhd", 1, "fd", "hindenlang_gassner", "hindenlang_gassner", cmin=-1.0, cmax=1.0,
          gamma=5 / 3),
+    Case("mhd2d_fd_central_llf", 2, "mhd", 2, "fd", "central", cmin=-1.0, cmax=1.0, gamma=5 / 3),
+    Case("mhd2d_fd_hg_hg", 2, "mhd", 2, "fd", "hindenlang_gassner", "hindenlang_gassner", cmin=-1.0, cmax=1.0),
+    Case("mhd2d_fd_hg_llfnaive", 2, "mhd", 1, "fd", "hindenlang_gassner", "llf_naive", cmin=-1.0, cmax=1.0),
 ]
 
 

@@ -135,6 +135,17 @@ CASES = [
          patches=(((-0.5, -0.5, -0.5), (0.5, 0.5, 0.5)),)),
     Case("euler2d_hg_dirichlet_density", 2, "euler", 3, "hg", "ranocha", cmin=0.0, cmax=2.0, periodic=False,
          bc_state="convergence_test", indicator="density", alpha_min=0.0),
+    # 2D GLM-MHD (test/tree_dgsem_2d/mhd_alfven_wave.jl: volume flux_central,
+    # surface LLF, Powell terms, gamma 5/3 on [0, sqrt 2]^2; mhd_ec.jl:
+    # flux_hindenlang_gassner throughout)
+    Case("mhd2d_fd_central_llf_alfven", 2, "mhd", 3, "fd", "central", cmin=0.0, cmax=2 ** 0.5,
+         ic="convergence_test", gamma=5 / 3),
+    Case("mhd2d_fd_central_llf_random", 2, "mhd", 3, "fd", "central", cmin=-1.0, cmax=1.0, gamma=5 / 3),
+    Case("mhd2d_fd_hg_hg_random", 2, "mhd", 3, "fd", "hindenlang_gassner", "hindenlang_gassner", cmin=-1.0,
+         cmax=1.0, gamma=1.4),
+    Case("mhd2d_fd_hg_llf_random", 2, "mhd", 3, "fd", "hindenlang_gassner", cmin=-1.0, cmax=1.0, gamma=5 / 3),
+    Case("mhd2d_fd_hg_llfnaive_random", 2, "mhd", 2, "fd", "hindenlang_gassner", "llf_naive", cmin=-1.0,
+         cmax=1.0, gamma=5 / 3),
 ]
 
 
@@ -190,7 +201,9 @@ def test_pack_unpack_roundtrip_bit_exact(th, oracle, torch):
                                   Case("mhd3d_alfven", 3, "mhd", 2, "fd", "hindenlang_gassner", cmin=-1.0,
                                        cmax=1.0, ic="convergence_test", gamma=5 / 3),
                                   Case("euler3d_mortar", 3, "euler", 2, cmin=0.0, cmax=2.0, ic="convergence_test",
-                                       patches=EULER_MORTAR_3D)],
+                                       patches=EULER_MORTAR_3D),
+                                  Case("mhd2d_alfven", 2, "mhd", 3, "fd", "central", cmin=0.0, cmax=2 ** 0.5,
+                                       ic="convergence_test", gamma=5 / 3)],
                          ids=lambda c: c.name)
 def test_initial_condition_and_callbacks_match_oracle(th, oracle, torch, case):
     O = oracle
@@ -254,7 +267,9 @@ def test_indicator_matches_oracle(th, oracle, torch, case):
                                        source="convergence_test", patches=EULER_MORTAR_3D),
                                   Case("euler2d_hg_mortar", 2, "euler", 3, "hg", "ranocha", cmin=-1.0, cmax=1.0,
                                        gamma=5 / 3, ic="weak_blast_wave_seeded", indicator="pressure",
-                                       patches=MORTAR_2D)],
+                                       patches=MORTAR_2D),
+                                  Case("mhd2d_alfven", 2, "mhd", 3, "fd", "central", cmin=0.0, cmax=2 ** 0.5,
+                                       ic="convergence_test", gamma=5 / 3)],
                          ids=lambda c: c.name)
 def test_rk_step_matches_oracle(th, oracle, torch, case, use_graph):
     """Three CK2N54 steps from t = 0 (stage times through device memory: the

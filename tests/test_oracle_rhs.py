@@ -517,3 +517,42 @@ def test_hg_on_mortar_meshes_is_conservative(oracle):
         du, alpha = O.rhs(cfg, mesh, u, return_alpha=True)
         assert alpha.max() > 0.05
         assert np.abs(O.integrate(cfg, mesh, du)).max() < 1e-13 * np.abs(du).max() * mesh.total_volume
+
+
+@pytest.mark.parametrize("nd", [2, 3])
+def test_mhd_alfven_wave_is_an_exact_solution(oracle, nd):
+    """The GLM-MHD convergence-test states (2D: Derigs et al.'s Alfven wave
+    along the diagonal of [0, sqrt 2]^2; 3D: Altmann's wave on [-1, 1]^3)
+    solve the equations with the Powell terms: rhs!(u_exact(t)) - u_t drops
+    at the DG rate.  A wrong sign of t in the wave or a flux term that breaks
+    the 2D reduction leaves an O(1) residual."""
+    O = oracle
+    lo, hi = (0.0, np.sqrt(2.0)) if nd == 2 else (-1.0, 1.0)
+    cfg = O.make_cfg(nd, O.EQ_GLM_MHD, 3, O.VI_FD, O.FLUX_HINDENLANG_GASSNER, O.FLUX_LLF, gamma=5 / 3, c_h=1.0)
+    errs = []
+    t, h = 0.1, 1e-6
+    for lvl in ((3, 4) if nd == 2 else (2, 3)):
+        mesh = O.Mesh(nd, lvl, lo, hi, 3)
+        u = O.compute_coefficients(cfg, mesh, O.IC_CONVERGENCE_TEST, t)
+        ut = (O.compute_coefficients(cfg, mesh, O.IC_CONVERGENCE_TEST, t + h)
+              - O.compute_coefficients(cfg, mesh, O.IC_CONVERGENCE_TEST, t - h)) / (2 * h)
+        assert np.abs(ut).max() > 0.5
+        errs.append(np.abs(O.rhs(cfg, mesh, u, t=t) - ut).max())
+    rate = np.log2(errs[0] / errs[1])
+    assert rate > 2.5, (errs, rate)
+
+
+def test_mhd2d_entropy_conservation_with_powell(oracle):
+    O = oracle
+    mesh = O.Mesh(2, 2, -1.0, 1.0, 3)
+    cfg = O.make_cfg(2, O.EQ_GLM_MHD, 3, O.VI_FD, O.FLUX_HINDENLANG_GASSNER, O.FLUX_HINDENLANG_GASSNER,
+                     gamma=1.4, c_h=1.0)
+    u = O.compute_coefficients(cfg, mesh, O.IC_SEEDED_RANDOM)
+    du = O.rhs(cfg, mesh, u)
+    assert abs(entropy_rate(O, cfg, mesh, u, du)) < 1e-12 * np.abs(du).max() * mesh.total_volume
+    cfg = O.make_cfg(2, O.EQ_GLM_MHD, 3, O.VI_FD, O.FLUX_HINDENLANG_GASSNER, O.FLUX_LLF, gamma=1.4, c_h=1.0)
+    du = O.rhs(cfg, mesh, u)
+    assert entropy_rate(O, cfg, mesh, u, du) < -1e-6
+    # a constant state with constant B stays constant
+    u = O.compute_coefficients(cfg, mesh, O.IC_CONSTANT)
+    assert np.abs(O.rhs(cfg, mesh, u)).max() < 1e-13

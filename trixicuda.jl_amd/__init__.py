@@ -17,7 +17,7 @@ from .callbacks import (AnalysisCallback, StepsizeCallback, blending_factors, ca
                         integrate, max_dt)
 from .equations import (BoundaryConditionDirichlet, boundary_condition_periodic, source_terms_convergence_test,
                         CompressibleEulerEquations1D, CompressibleEulerEquations2D, CompressibleEulerEquations3D,
-                        FluxLaxFriedrichs, IdealGlmMhdEquations3D, LinearScalarAdvectionEquation1D,
+                        FluxLaxFriedrichs, IdealGlmMhdEquations2D, IdealGlmMhdEquations3D, LinearScalarAdvectionEquation1D,
                         LinearScalarAdvectionEquation2D, LinearScalarAdvectionEquation3D, density, density_pressure, pressure,
                         flux_central, flux_hindenlang_gassner, flux_lax_friedrichs, flux_nonconservative_powell,
                         flux_ranocha, flux_shima_etal, initial_condition_constant,

@@ -124,7 +124,22 @@ __device__ inline void initial_condition(const IcParams& q, const double* x, dou
         return;
     }
     double prim[9] = {1.0, 0, 0, 0, 1.0, 0, 0, 0, 0};
-    if (q.ic == TRIXI_HIP_IC_CONVERGENCE_TEST) {
+    if (q.ic == TRIXI_HIP_IC_CONVERGENCE_TEST && q.dim == 2) {
+        // Alfven wave of IdealGlmMhdEquations2D (Derigs et al. 2016): along
+        // x_perp = x cos(pi/4) + y sin(pi/4), B_perp = 0.1 sin(2 pi (x_perp + t))
+        const double ca = cos(0.25 * M_PI), sa = sin(0.25 * M_PI);
+        const double xp = x[0] * ca + x[1] * sa;
+        const double Bp = 0.1 * sin(2.0 * M_PI * (xp + t));
+        prim[0] = 1.0;
+        prim[1] = -Bp * sa;
+        prim[2] = Bp * ca;
+        prim[3] = 0.1 * cos(2.0 * M_PI * (xp + t));
+        prim[4] = 0.1;
+        prim[5] = ca + prim[1];
+        prim[6] = sa + prim[2];
+        prim[7] = prim[3];
+        prim[8] = 0.0;
+    } else if (q.ic == TRIXI_HIP_IC_CONVERGENCE_TEST) {
         const double omega = 2.0 * M_PI, r = 2.0, e = 0.2;
         const double nx = 1.0 / sqrt(r * r + 1.0), ny = r / sqrt(r * r + 1.0), sqr = 1.0;
         const double Va = omega / (ny * sqr);

@@ -5,7 +5,7 @@
 namespace trixi_hip {
 
 void register_mhd3d() {
-    using EQ = GlmMhd3D;
+    using EQ = GlmMhd;
     constexpr int C = FLUX_CENTRAL;
     Registrar<3, 4, EQ, VI_FD, FLUX_HINDENLANG_GASSNER, FLUX_LLF, C>::add(EQ_GLM_MHD);
     Registrar<3, 4, EQ, VI_FD, FLUX_HINDENLANG_GASSNER, FLUX_LLF_NAIVE, C>::add(EQ_GLM_MHD);

@@ -344,10 +344,11 @@ struct Euler {
 };
 
 // ===========================================================================
-// Ideal GLM-MHD in 3D: (rho, rho v1..3, rho e, B1..3, psi), Powell and
-// Galilean nonconservative terms.
+// Ideal GLM-MHD: (rho, rho v1..3, rho e, B1..3, psi), Powell and Galilean
+// nonconservative terms.  IdealGlmMhdEquations3D uses orientations 0..2,
+// IdealGlmMhdEquations2D the same nine-variable system with orientations 0, 1.
 // ===========================================================================
-struct GlmMhd3D {
+struct GlmMhd {
     static constexpr int NV = 9;
     static constexpr bool NONCONS = true;
     static constexpr int NRF = 9;  // node record: rho, v[3], p, B[3], psi

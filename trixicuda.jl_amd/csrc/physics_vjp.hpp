@@ -425,7 +425,7 @@ struct EulerVjp {
 };
 
 // ===========================================================================
-// Ideal GLM-MHD in 3D with the nonconservative Powell / GLM terms.
+// Ideal GLM-MHD (2D and 3D) with the nonconservative Powell / GLM terms.
 // Primitive variables (rho, v[3], p, B[3], psi).
 // ===========================================================================
 struct GlmMhdVjp {

@@ -57,6 +57,7 @@ void register_euler1d_hg();
 void register_euler2d_hg();
 void register_euler3d_hg();
 void register_mhd3d();
+void register_mhd2d();
 void register_vjp_linadv();
 void register_vjp_euler1d();
 void register_vjp_euler2d();
@@ -77,6 +78,7 @@ static void ensure_registered() {
         register_euler2d_hg();
         register_euler3d_hg();
         register_mhd3d();
+        register_mhd2d();
         register_vjp_linadv();
         register_vjp_euler1d();
         register_vjp_euler2d();
@@ -983,8 +985,8 @@ int trixi_hip_create(const trixi_hip_config* config, const trixi_hip_mesh* mesh,
     const trixi_hip_config& cfg = *config;
     if (cfg.ndims < 1 || cfg.ndims > 3) return fail(TRIXI_HIP_EINVAL, "ndims must be 1, 2 or 3");
     if (cfg.polydeg < 1 || cfg.polydeg + 1 > MAXN) return fail(TRIXI_HIP_EINVAL, "polydeg out of range");
-    if (cfg.equation == TRIXI_HIP_IDEAL_GLM_MHD && cfg.ndims != 3)
-        return fail(TRIXI_HIP_EUNSUPPORTED, "GLM-MHD is built for 3D only");
+    if (cfg.equation == TRIXI_HIP_IDEAL_GLM_MHD && cfg.ndims == 1)
+        return fail(TRIXI_HIP_EUNSUPPORTED, "GLM-MHD is built for 2D and 3D");
     if (cfg.volume_integral == TRIXI_HIP_VOLUME_SHOCK_CAPTURING_HG && cfg.equation != TRIXI_HIP_COMPRESSIBLE_EULER)
         return fail(TRIXI_HIP_EUNSUPPORTED, "shock capturing is built for compressible Euler only");
     if (cfg.indicator_variable < TRIXI_HIP_INDICATOR_DENSITY_PRESSURE ||
@@ -1422,7 +1424,7 @@ int trixi_hip_max_dt(trixi_hip_ctx* ctx, const double* u, double* max_dt, trixi_
     } else if (eq == TRIXI_HIP_COMPRESSIBLE_EULER) {
         if (dim == 1) MS(Euler<1>, 1); else if (dim == 2) MS(Euler<2>, 2); else MS(Euler<3>, 3);
     } else {
-        MS(GlmMhd3D, 3);
+        if (dim == 2) MS(GlmMhd, 2); else MS(GlmMhd, 3);
     }
 #undef MS
     HIP_TRY(hipGetLastError());

@@ -70,16 +70,24 @@ CompressibleEulerEquations2D = _euler(2)
 CompressibleEulerEquations3D = _euler(3)
 
 
-@dataclass(frozen=True)
-class IdealGlmMhdEquations3D(AbstractEquations):
-    gamma: float = 5.0 / 3.0
-    c_h: float = 1.0
+def _glm_mhd(ndims):
+    @dataclass(frozen=True)
+    class _GlmMhd(AbstractEquations):
+        gamma: float = 5.0 / 3.0
+        c_h: float = 1.0
 
-    def __init__(self, gamma, initial_c_h=1.0):
-        object.__setattr__(self, "ndims", 3)
-        object.__setattr__(self, "equation_id", L.EQ_GLM_MHD)
-        object.__setattr__(self, "gamma", float(gamma))
-        object.__setattr__(self, "c_h", float(initial_c_h))
+        def __init__(self, gamma, initial_c_h=1.0):
+            object.__setattr__(self, "ndims", ndims)
+            object.__setattr__(self, "equation_id", L.EQ_GLM_MHD)
+            object.__setattr__(self, "gamma", float(gamma))
+            object.__setattr__(self, "c_h", float(initial_c_h))
+
+    _GlmMhd.__name__ = _GlmMhd.__qualname__ = f"IdealGlmMhdEquations{ndims}D"
+    return _GlmMhd
+
+
+IdealGlmMhdEquations2D = _glm_mhd(2)
+IdealGlmMhdEquations3D = _glm_mhd(3)
 
 
 # ----------------------------------------------------------------------------
