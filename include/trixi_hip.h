@@ -49,7 +49,8 @@ typedef void* trixi_hip_stream; /* a hipStream_t; NULL = the default stream */
 enum trixi_hip_equation {
     TRIXI_HIP_LINEAR_SCALAR_ADVECTION = 0, /* LinearScalarAdvectionEquation{1,2,3}D */
     TRIXI_HIP_COMPRESSIBLE_EULER = 1,      /* CompressibleEulerEquations{1,2,3}D    */
-    TRIXI_HIP_IDEAL_GLM_MHD = 2            /* IdealGlmMhdEquations2D / 3D           */
+    TRIXI_HIP_IDEAL_GLM_MHD = 2,           /* IdealGlmMhdEquations2D / 3D           */
+    TRIXI_HIP_INVISCID_BURGERS = 3         /* InviscidBurgersEquation1D             */
 };
 enum trixi_hip_volume_integral {
     TRIXI_HIP_VOLUME_WEAK_FORM = 0,          /* VolumeIntegralWeakForm          */
@@ -62,7 +63,8 @@ enum trixi_hip_flux {
     TRIXI_HIP_FLUX_LAX_FRIEDRICHS_NAIVE = 2, /* FluxLaxFriedrichs(max_abs_speed_naive) */
     TRIXI_HIP_FLUX_RANOCHA = 3,
     TRIXI_HIP_FLUX_SHIMA_ETAL = 4,
-    TRIXI_HIP_FLUX_HINDENLANG_GASSNER = 5    /* with flux_nonconservative_powell       */
+    TRIXI_HIP_FLUX_HINDENLANG_GASSNER = 5,   /* with flux_nonconservative_powell       */
+    TRIXI_HIP_FLUX_EC = 6                    /* flux_ec (InviscidBurgersEquation1D)    */
 };
 enum trixi_hip_precision {
     TRIXI_HIP_FP64 = 0,  /* all arithmetic in FP64                                    */
@@ -71,12 +73,17 @@ enum trixi_hip_precision {
 };
 enum trixi_hip_initial_condition {
     TRIXI_HIP_IC_CONSTANT = 0,
-    TRIXI_HIP_IC_CONVERGENCE_TEST = 1, /* LinAdv sine wave / GLM-MHD Alfven wave        */
+    TRIXI_HIP_IC_CONVERGENCE_TEST = 1, /* per equation: LinAdv / Burgers sine wave, Euler
+                                          manufactured state, GLM-MHD Alfven wave  */
     TRIXI_HIP_IC_WEAK_BLAST_WAVE = 2,
     TRIXI_HIP_IC_SEEDED_RANDOM = 3,
     TRIXI_HIP_IC_SMOOTH_PERIODIC = 4,
-    TRIXI_HIP_IC_WEAK_BLAST_WAVE_SEEDED = 5 /* Euler: weak blast + seeded U(-0.01, 0.01)
+    TRIXI_HIP_IC_WEAK_BLAST_WAVE_SEEDED = 5, /* Euler: weak blast + seeded U(-0.01, 0.01)
                                                on rho, v, p per node               */
+    TRIXI_HIP_IC_SHOCK = 6,            /* Burgers: 1.5 for x < 0.5, else 0.5
+                                          (test/tree_dgsem_1d/burgers_shock.jl)    */
+    TRIXI_HIP_IC_RAREFACTION = 7       /* Burgers: 0.5 for x < 0.5, else 1.5
+                                          (test/tree_dgsem_1d/burgers_rarefraction.jl) */
 };
 
 /* Boundary condition on the faces of a non-periodic TreeMesh that have no
@@ -127,7 +134,10 @@ typedef struct trixi_hip_config {
     int32_t boundary_ic;      /* Dirichlet state: enum trixi_hip_initial_condition */
     uint64_t boundary_seed;   /* seed of a seeded boundary_ic                     */
     int32_t source_terms;     /* enum trixi_hip_source_terms                      */
-    int32_t reserved2;
+    int32_t boundary_outflow_faces; /* bit f (face direction -x,+x,-y,+y,-z,+z = 0..5)
+                                       set: that side takes the outflow state
+                                       (the inner state, flux(u_inner)) instead of
+                                       the Dirichlet state                        */
 } trixi_hip_config;
 
 /* Mesh and connectivity (host arrays, copied to the device by create): the

@@ -17,10 +17,11 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_PATH = os.path.join(_HERE, "liboracle.so")
 
-EQ_LINADV, EQ_EULER, EQ_GLM_MHD = 0, 1, 2
+EQ_LINADV, EQ_EULER, EQ_GLM_MHD, EQ_BURGERS = 0, 1, 2, 3
 VI_WEAK, VI_FD, VI_HG = 0, 1, 2
-FLUX_CENTRAL, FLUX_LLF, FLUX_LLF_NAIVE, FLUX_RANOCHA, FLUX_SHIMA_ETAL, FLUX_HINDENLANG_GASSNER = range(6)
-IC_CONSTANT, IC_CONVERGENCE_TEST, IC_WEAK_BLAST_WAVE, IC_SEEDED_RANDOM, IC_SMOOTH_PERIODIC, IC_WEAK_BLAST_WAVE_SEEDED = range(6)
+FLUX_CENTRAL, FLUX_LLF, FLUX_LLF_NAIVE, FLUX_RANOCHA, FLUX_SHIMA_ETAL, FLUX_HINDENLANG_GASSNER, FLUX_EC = range(7)
+(IC_CONSTANT, IC_CONVERGENCE_TEST, IC_WEAK_BLAST_WAVE, IC_SEEDED_RANDOM, IC_SMOOTH_PERIODIC, IC_WEAK_BLAST_WAVE_SEEDED,
+ IC_SHOCK, IC_RAREFACTION) = range(8)
 BC_PERIODIC, BC_DIRICHLET = 0, 1
 SOURCE_NONE, SOURCE_CONVERGENCE_TEST = 0, 1
 IND_DENSITY_PRESSURE, IND_PRESSURE, IND_DENSITY = 0, 1, 2
@@ -53,6 +54,8 @@ class OracleCfg(ctypes.Structure):
         ("mortar_neighbor_ids", ctypes.c_void_p),
         ("mortar_orientations", ctypes.c_void_p),
         ("mortar_large_sides", ctypes.c_void_p),
+        ("boundary_outflow_faces", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
     ]
 
 
@@ -121,8 +124,9 @@ def make_cfg(ndims, equation, polydeg=3, volume_integral=VI_WEAK, volume_flux=FL
              surface_flux=FLUX_LLF, fv_flux=FLUX_LLF, gamma=1.4, c_h=1.0,
              advection_velocity=(1.0, 1.0, 1.0), alpha_max=0.5, alpha_min=0.001,
              alpha_smooth=True, ic_length=4.0, boundary_condition=BC_PERIODIC, boundary_ic=IC_CONSTANT,
-             boundary_seed=0, source_terms=SOURCE_NONE, indicator_variable=0):
+             boundary_seed=0, source_terms=SOURCE_NONE, indicator_variable=0, boundary_outflow_faces=0):
     c = OracleCfg()
+    c.boundary_outflow_faces = boundary_outflow_faces
     c.indicator_variable = indicator_variable
     c.boundary_condition, c.boundary_ic, c.boundary_seed = boundary_condition, boundary_ic, boundary_seed
     c.source_terms = source_terms

@@ -82,7 +82,7 @@ constexpr int MAXV = 9;
 struct Cfg {
     int nd, eq, N, nv, vi, vflux, sflux, fvflux, smooth;
     double gamma, inv_gm1, c_h, a[3], amax, amin, ic_length;
-    int bc, bc_ic, src, ivar;
+    int bc, bc_ic, src, ivar, outflow;
     uint64_t bc_seed;
     const double* xnodes;
     double t;
@@ -115,6 +115,7 @@ Cfg make_cfg(const oracle_cfg* c) {
     k.xnodes = c->node_coordinates;
     k.t = c->time;
     k.ivar = c->indicator_variable;
+    k.outflow = c->boundary_outflow_faces;
     k.nmortars = c->nmortars;
     k.mids = c->mortar_neighbor_ids;
     k.mori = c->mortar_orientations;
@@ -173,7 +174,9 @@ void mhd_prim(const Cfg& c, const T* u, T& rho, T* v, T& p, T* B, T& psi) {
 
 template <class T> void phys_flux(const Cfg& c, const T* u, int o, T* f) {
     const int oi = o - 1;
-    if (c.eq == OR_EQ_LINADV) {
+    if (c.eq == OR_EQ_BURGERS) {
+        f[0] = 0.5 * u[0] * u[0];  // Trixi InviscidBurgersEquation1D flux
+    } else if (c.eq == OR_EQ_LINADV) {
         f[0] = c.a[oi] * u[0];
     } else if (c.eq == OR_EQ_EULER) {
         T rho, v[3], p;
@@ -225,6 +228,9 @@ template <class T> void normal_speed(const Cfg& c, const T* u, int o, T& vn, T& 
     const int oi = o - 1;
     if (c.eq == OR_EQ_LINADV) {
         vn = c.a[oi];
+        cs = 0.0;
+    } else if (c.eq == OR_EQ_BURGERS) {
+        vn = u[0];  // max_abs_speed(_naive) of Burgers: max(|u_ll|, |u_rr|)
         cs = 0.0;
     } else if (c.eq == OR_EQ_EULER) {
         T rho, v[3], p;
@@ -354,6 +360,9 @@ template <class T> void two_point(const Cfg& c, int kind, const T* ull, const T*
             return;
         case OR_FLUX_HINDENLANG_GASSNER:
             flux_hindenlang_gassner_mhd(c, ull, urr, o, f);
+            return;
+        case OR_FLUX_EC:  // Trixi flux_ec(u_ll, u_rr, orientation, ::InviscidBurgersEquation1D)
+            f[0] = (ull[0] * ull[0] + ull[0] * urr[0] + urr[0] * urr[0]) / 6.0;
             return;
         case OR_FLUX_LLF:
         case OR_FLUX_LLF_NAIVE: {
@@ -814,6 +823,12 @@ void initial_condition(const Cfg& c, int ic, const double* x, double t, uint64_t
 //   S_{rho e}  = d (2 g g' + p') - 2 g g'                 (energy)
 // with p' = (gamma - 1)(2 g - d/2) g'.
 void source_convergence_test(const Cfg& c, const double* x, double t, double* sv) {
+    if (c.eq == OR_EQ_BURGERS) {
+        // u = 2 + sin(2 pi (x - t)): u_t + u u_x = 2 pi cos(.) (u - 1)
+        const double w = 2.0 * M_PI, arg = w * (x[0] - t);
+        sv[0] = w * cos(arg) * (2.0 + sin(arg) - 1.0);
+        return;
+    }
     const int d = c.nd;
     double s = -t;
     for (int q = 0; q < d; ++q) s += x[q];
@@ -1014,7 +1029,9 @@ void rhs_impl(const Cfg& c, int64_t E, int64_t I, const int32_t* nbr, const int3
                         uin[v] = u[e * esz + nv * node + v];
                         ub[v] = T(ubd[v]);
                     }
-                    if (plus)
+                    if ((c.outflow >> dir) & 1)
+                        phys_flux(c, uin, od + 1, flux);  // outflow: flux(u_inner)
+                    else if (plus)
                         two_point(c, c.sflux, uin, ub, od + 1, flux);
                     else
                         two_point(c, c.sflux, ub, uin, od + 1, flux);
@@ -1121,6 +1138,24 @@ double smooth_field(uint64_t seed, int field, int nd, const double* x, double L,
 
 void initial_condition(const Cfg& c, int ic, const double* x, double t, uint64_t seed,
                        uint64_t idx, double* u) {
+    if (c.eq == OR_EQ_BURGERS) {
+        // Trixi initial_condition_convergence_test(x, t, ::InviscidBurgersEquation1D):
+        // c + A sin(omega (x - t)), c = 2, A = 1, L = 1, omega = 2 pi / L; and
+        // the shock / rarefaction steps of the reference's Burgers tests
+        if (ic == OR_IC_CONVERGENCE_TEST) {
+            const double cc = 2.0, A = 1.0, L = 1.0, omega = 2.0 * M_PI / L;
+            u[0] = cc + A * sin(omega * (x[0] - t));
+        } else if (ic == OR_IC_SHOCK) {
+            u[0] = x[0] < 0.5 ? 1.5 : 0.5;
+        } else if (ic == OR_IC_RAREFACTION) {
+            u[0] = x[0] < 0.5 ? 0.5 : 1.5;
+        } else if (ic == OR_IC_SEEDED_RANDOM) {
+            u[0] = 1.0 + 0.5 * (2.0 * u01(seed, idx, 0) - 1.0);
+        } else {
+            u[0] = 2.0;
+        }
+        return;
+    }
     if (c.eq == OR_EQ_LINADV) {
         if (ic == OR_IC_CONVERGENCE_TEST) {
             double xs = 0.0;
@@ -1225,7 +1260,7 @@ void initial_condition(const Cfg& c, int ic, const double* x, double t, uint64_t
 
 // Entropy and entropy variables (Trixi entropy / cons2entropy).
 template <class T> T entropy_math(const Cfg& c, const T* u) {
-    if (c.eq == OR_EQ_LINADV) return 0.5 * u[0] * u[0];
+    if (c.eq == OR_EQ_LINADV || c.eq == OR_EQ_BURGERS) return 0.5 * u[0] * u[0];
     T rho, v[3], p;
     if (c.eq == OR_EQ_EULER) {
         euler_prim(c, u, rho, v, p);
@@ -1245,7 +1280,7 @@ template <class T> T entropy_math(const Cfg& c, const T* u) {
 extern "C" {
 
 int oracle_nvars(const oracle_cfg* cfg) {
-    if (cfg->equation == OR_EQ_LINADV) return 1;
+    if (cfg->equation == OR_EQ_LINADV || cfg->equation == OR_EQ_BURGERS) return 1;
     if (cfg->equation == OR_EQ_EULER) return cfg->ndims + 2;
     return 9;
 }
@@ -1340,6 +1375,8 @@ void oracle_max_abs_speeds(const oracle_cfg* cfg, const double* u, double* lam) 
     for (int d = 0; d < c.nd; ++d) {
         if (c.eq == OR_EQ_LINADV) {
             lam[d] = fabs(c.a[d]);
+        } else if (c.eq == OR_EQ_BURGERS) {
+            lam[d] = fabs(u[0]);
         } else if (c.eq == OR_EQ_EULER) {
             double rho, v[3], p;
             euler_prim(c, u, rho, v, p);
@@ -1356,7 +1393,7 @@ double oracle_entropy(const oracle_cfg* cfg, const double* u) {
 
 void oracle_cons2entropy(const oracle_cfg* cfg, const double* u, double* w) {
     Cfg c = make_cfg(cfg);
-    if (c.eq == OR_EQ_LINADV) {
+    if (c.eq == OR_EQ_LINADV || c.eq == OR_EQ_BURGERS) {
         w[0] = u[0];
         return;
     }
@@ -1380,10 +1417,13 @@ void oracle_cons2entropy(const oracle_cfg* cfg, const double* u, double* w) {
 }
 
 // Entropy flux potential psi_o with [[w]].f* = [[psi_o]] for EC fluxes:
-// Euler: rho v_o; LinAdv (U = u^2/2): a_o u^2 / 2.
+// Euler: rho v_o; LinAdv (U = u^2/2): a_o u^2 / 2; Burgers (U = u^2/2,
+// F = u^3/3): w f - F = u^3 / 6.
 void oracle_entropy_potential(const oracle_cfg* cfg, const double* u, int o, double* psi) {
     Cfg c = make_cfg(cfg);
-    if (c.eq == OR_EQ_LINADV) {
+    if (c.eq == OR_EQ_BURGERS) {
+        *psi = u[0] * u[0] * u[0] / 6.0;
+    } else if (c.eq == OR_EQ_LINADV) {
         *psi = 0.5 * c.a[o - 1] * u[0] * u[0];
     } else {
         *psi = u[o];

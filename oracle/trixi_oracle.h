@@ -25,7 +25,7 @@
 extern "C" {
 #endif
 
-enum { OR_EQ_LINADV = 0, OR_EQ_EULER = 1, OR_EQ_GLM_MHD = 2 };
+enum { OR_EQ_LINADV = 0, OR_EQ_EULER = 1, OR_EQ_GLM_MHD = 2, OR_EQ_BURGERS = 3 };
 enum { OR_VI_WEAK = 0, OR_VI_FD = 1, OR_VI_HG = 2 };
 enum {
     OR_FLUX_CENTRAL = 0,
@@ -33,7 +33,8 @@ enum {
     OR_FLUX_LLF_NAIVE = 2,    /* FluxLaxFriedrichs(max_abs_speed_naive)      */
     OR_FLUX_RANOCHA = 3,
     OR_FLUX_SHIMA_ETAL = 4,
-    OR_FLUX_HINDENLANG_GASSNER = 5
+    OR_FLUX_HINDENLANG_GASSNER = 5,
+    OR_FLUX_EC = 6            /* Burgers flux_ec                             */
 };
 enum {
     OR_IC_CONSTANT = 0,
@@ -41,7 +42,9 @@ enum {
     OR_IC_WEAK_BLAST_WAVE = 2,
     OR_IC_SEEDED_RANDOM = 3,    /* Euler: rho=1+0.1U, v=0.1N(0,1), p=1+0.05U     */
     OR_IC_SMOOTH_PERIODIC = 4,  /* Euler: seeded sums of periodic sines           */
-    OR_IC_WEAK_BLAST_WAVE_SEEDED = 5 /* Euler: weak blast + U(-0.01,0.01) on rho,v,p */
+    OR_IC_WEAK_BLAST_WAVE_SEEDED = 5, /* Euler: weak blast + U(-0.01,0.01) on rho,v,p */
+    OR_IC_SHOCK = 6,          /* Burgers: 1.5 for x < 0.5, else 0.5 */
+    OR_IC_RAREFACTION = 7     /* Burgers: 0.5 for x < 0.5, else 1.5 */
 };
 enum { OR_BC_PERIODIC = 0, OR_BC_DIRICHLET = 1 };
 enum { OR_SOURCE_NONE = 0, OR_SOURCE_CONVERGENCE_TEST = 1 };
@@ -71,6 +74,8 @@ typedef struct oracle_cfg {
     const int32_t* mortar_neighbor_ids; /* [nmortars][2^(d-1)+1] small..., large   */
     const int32_t* mortar_orientations; /* [nmortars] 1..d                          */
     const int32_t* mortar_large_sides;  /* [nmortars] 1: large on the left, 2: right */
+    int32_t boundary_outflow_faces; /* bit f (-x,+x,-y,+y,-z,+z): boundary flux f(u_inner) */
+    int32_t reserved;
 } oracle_cfg;
 
 int oracle_nvars(const oracle_cfg* cfg);

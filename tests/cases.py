@@ -9,7 +9,7 @@ import numpy as np
 class Case:
     name: str
     ndims: int
-    equation: str           # "linadv" | "euler" | "mhd"
+    equation: str           # "linadv" | "euler" | "mhd" | "burgers"
     level: int
     vi: str = "weak"        # "weak" | "fd" | "hg"
     vflux: str = "central"
@@ -32,12 +32,15 @@ class Case:
     source: str = ""        # "convergence_test" for source_terms_convergence_test
     t: float = 0.0          # time of the rhs! evaluation
     patches: tuple = ()     # box refinement patches ((lo, hi), ...): L2 mortars
-    indicator: str = "density_pressure"  # HG indicator variable: density_pressure | pressure | density
+    indicator: str = "density_pressure"  # HG indicator variable: density_pressure | pressure | density | first
+    outflow: tuple = ()     # boundary sides ("x_neg", ...) with the outflow condition
 
 
-FLUX_NAMES = ("central", "llf", "llf_naive", "ranocha", "shima_etal", "hindenlang_gassner")
+FLUX_NAMES = ("central", "llf", "llf_naive", "ranocha", "shima_etal", "hindenlang_gassner", "ec")
 IC_NAMES = ("constant", "convergence_test", "weak_blast_wave", "seeded_random", "smooth_periodic",
-            "weak_blast_wave_seeded")
+            "weak_blast_wave_seeded", "shock", "rarefaction")
+SIDES = ("x_neg", "x_pos", "y_neg", "y_pos", "z_neg", "z_pos")
+INDICATORS = {"density_pressure": 0, "pressure": 1, "density": 2, "first": 2}
 
 
 def oracle_mesh(O, c: Case):
@@ -47,16 +50,17 @@ def oracle_mesh(O, c: Case):
 
 
 def oracle_cfg(O, c: Case):
-    eq = {"linadv": O.EQ_LINADV, "euler": O.EQ_EULER, "mhd": O.EQ_GLM_MHD}[c.equation]
+    eq = {"linadv": O.EQ_LINADV, "euler": O.EQ_EULER, "mhd": O.EQ_GLM_MHD, "burgers": O.EQ_BURGERS}[c.equation]
     vi = {"weak": O.VI_WEAK, "fd": O.VI_FD, "hg": O.VI_HG}[c.vi]
     ext = dict(boundary_condition=O.BC_DIRICHLET if c.bc_state else O.BC_PERIODIC,
                boundary_ic=IC_NAMES.index(c.bc_state) if c.bc_state else 0,
-               source_terms=O.SOURCE_CONVERGENCE_TEST if c.source == "convergence_test" else O.SOURCE_NONE)
+               source_terms=O.SOURCE_CONVERGENCE_TEST if c.source == "convergence_test" else O.SOURCE_NONE,
+               boundary_outflow_faces=sum(1 << SIDES.index(s) for s in c.outflow))
     return O.make_cfg(c.ndims, eq, c.polydeg, vi, FLUX_NAMES.index(c.vflux), FLUX_NAMES.index(c.sflux),
                       FLUX_NAMES.index(c.fvflux), gamma=c.gamma, c_h=c.c_h,
                       advection_velocity=c.advection[:c.ndims], alpha_max=c.alpha_max, alpha_min=c.alpha_min,
                       alpha_smooth=c.alpha_smooth, ic_length=c.cmax - c.cmin,
-                      indicator_variable=("density_pressure", "pressure", "density").index(c.indicator), **ext)
+                      indicator_variable=INDICATORS[c.indicator], **ext)
 
 
 def oracle_ic(O, c: Case):
@@ -66,7 +70,7 @@ def oracle_ic(O, c: Case):
 def _flux(th, name, mhd):
     f = {"central": th.flux_central, "llf": th.flux_lax_friedrichs,
          "llf_naive": th.FluxLaxFriedrichs(th.max_abs_speed_naive), "ranocha": th.flux_ranocha,
-         "shima_etal": th.flux_shima_etal, "hindenlang_gassner": th.flux_hindenlang_gassner}[name]
+         "shima_etal": th.flux_shima_etal, "hindenlang_gassner": th.flux_hindenlang_gassner, "ec": th.flux_ec}[name]
     return (f, th.flux_nonconservative_powell) if mhd else f
 
 
@@ -75,6 +79,8 @@ def product_semi(th, c: Case, precision=None):
     if c.equation == "linadv":
         eq = [None, th.LinearScalarAdvectionEquation1D, th.LinearScalarAdvectionEquation2D,
               th.LinearScalarAdvectionEquation3D][c.ndims](c.advection[:c.ndims])
+    elif c.equation == "burgers":
+        eq = th.InviscidBurgersEquation1D()
     elif c.equation == "euler":
         eq = [None, th.CompressibleEulerEquations1D, th.CompressibleEulerEquations2D,
               th.CompressibleEulerEquations3D][c.ndims](c.gamma)
@@ -99,11 +105,13 @@ def product_semi(th, c: Case, precision=None):
            "weak_blast_wave": th.initial_condition_weak_blast_wave,
            "seeded_random": th.initial_condition_seeded_random(c.seed),
            "smooth_periodic": th.initial_condition_smooth_periodic(c.seed),
-           "weak_blast_wave_seeded": th.initial_condition_weak_blast_wave_seeded(c.seed)}
+           "weak_blast_wave_seeded": th.initial_condition_weak_blast_wave_seeded(c.seed),
+           "shock": th.initial_condition_shock, "rarefaction": th.initial_condition_rarefaction}
     kw = {}
     if c.bc_state:
         bc = th.BoundaryConditionDirichlet(ics[c.bc_state])
-        kw["boundary_conditions"] = {s: bc for s in ("x_neg", "x_pos", "y_neg", "y_pos", "z_neg", "z_pos")[:2 * c.ndims]}
+        kw["boundary_conditions"] = {s: (th.boundary_condition_outflow if s in c.outflow else bc)
+                                     for s in SIDES[:2 * c.ndims]}
     if c.source == "convergence_test":
         kw["source_terms"] = th.source_terms_convergence_test
     return th.SemidiscretizationHyperbolicHIP(mesh, eq, ics[c.ic], solver, **kw)

@@ -146,6 +146,23 @@ CASES = [
     Case("mhd2d_fd_hg_llf_random", 2, "mhd", 3, "fd", "hindenlang_gassner", cmin=-1.0, cmax=1.0, gamma=5 / 3),
     Case("mhd2d_fd_hg_llfnaive_random", 2, "mhd", 2, "fd", "hindenlang_gassner", "llf_naive", cmin=-1.0,
          cmax=1.0, gamma=5 / 3),
+    # inviscid Burgers 1D (test/tree_dgsem_1d/burgers_basic.jl: weak form, LLF,
+    # source terms, level 4 on [0, 1]; burgers_shock.jl / burgers_rarefraction.jl:
+    # HG with flux_ec / LLF, variable first, alpha_max 1, level 6, inflow
+    # Dirichlet step on x_neg and outflow on x_pos)
+    Case("burgers1d_weak_llf_src", 1, "burgers", 4, cmin=0.0, cmax=1.0, ic="convergence_test",
+         source="convergence_test", t=0.3),
+    Case("burgers1d_fd_ec_llf_random", 1, "burgers", 4, "fd", "ec", cmin=0.0, cmax=1.0),
+    Case("burgers1d_fd_ec_ec_random", 1, "burgers", 4, "fd", "ec", "ec", cmin=0.0, cmax=1.0),
+    Case("burgers1d_hg_shock", 1, "burgers", 6, "hg", "ec", "llf", "llf", cmin=0.0, cmax=1.0, ic="shock",
+         periodic=False, bc_state="shock", outflow=("x_pos",), indicator="first", alpha_max=1.0),
+    Case("burgers1d_hg_rarefaction", 1, "burgers", 6, "hg", "ec", "llf", "llf", cmin=0.0, cmax=1.0,
+         ic="rarefaction", periodic=False, bc_state="rarefaction", outflow=("x_pos",), indicator="first",
+         alpha_max=1.0),
+    Case("burgers1d_hg_random_periodic", 1, "burgers", 5, "hg", "ec", "llf", "llf", cmin=0.0, cmax=1.0,
+         indicator="first", alpha_max=1.0, alpha_min=0.0),
+    Case("burgers1d_weak_inflow_outflow_src", 1, "burgers", 4, cmin=0.0, cmax=1.0, ic="convergence_test",
+         periodic=False, bc_state="convergence_test", outflow=("x_pos",), source="convergence_test", t=0.2),
 ]
 
 
@@ -203,7 +220,9 @@ def test_pack_unpack_roundtrip_bit_exact(th, oracle, torch):
                                   Case("euler3d_mortar", 3, "euler", 2, cmin=0.0, cmax=2.0, ic="convergence_test",
                                        patches=EULER_MORTAR_3D),
                                   Case("mhd2d_alfven", 2, "mhd", 3, "fd", "central", cmin=0.0, cmax=2 ** 0.5,
-                                       ic="convergence_test", gamma=5 / 3)],
+                                       ic="convergence_test", gamma=5 / 3),
+                                  Case("burgers1d", 1, "burgers", 4, cmin=0.0, cmax=1.0, ic="convergence_test",
+                                       source="convergence_test")],
                          ids=lambda c: c.name)
 def test_initial_condition_and_callbacks_match_oracle(th, oracle, torch, case):
     O = oracle
@@ -269,7 +288,12 @@ def test_indicator_matches_oracle(th, oracle, torch, case):
                                        gamma=5 / 3, ic="weak_blast_wave_seeded", indicator="pressure",
                                        patches=MORTAR_2D),
                                   Case("mhd2d_alfven", 2, "mhd", 3, "fd", "central", cmin=0.0, cmax=2 ** 0.5,
-                                       ic="convergence_test", gamma=5 / 3)],
+                                       ic="convergence_test", gamma=5 / 3),
+                                  Case("burgers1d_hg_shock", 1, "burgers", 6, "hg", "ec", "llf", "llf", cmin=0.0,
+                                       cmax=1.0, ic="shock", periodic=False, bc_state="shock", outflow=("x_pos",),
+                                       indicator="first", alpha_max=1.0),
+                                  Case("burgers1d_src", 1, "burgers", 4, cmin=0.0, cmax=1.0, ic="convergence_test",
+                                       source="convergence_test")],
                          ids=lambda c: c.name)
 def test_rk_step_matches_oracle(th, oracle, torch, case, use_graph):
     """Three CK2N54 steps from t = 0 (stage times through device memory: the

@@ -183,3 +183,23 @@ def test_mhd_two_point_consistency_symmetry(oracle):
             z[5 + o - 1] = 0.0
             z[8] = 0.0
             np.testing.assert_allclose(O.noncons_flux(cfg, ul, z, o), 0.0, atol=0)
+
+
+def test_burgers_flux_ec_and_llf(oracle):
+    """flux_ec of the inviscid Burgers equation: consistent (f*(u, u) = u^2/2),
+    symmetric, and entropy conservative for U = u^2/2, w = u, F = u^3/3:
+    (u_r - u_l) f* = psi_r - psi_l with psi = w f - F = u^3/6.  LLF: the
+    central flux minus max(|u_l|, |u_r|) / 2 [[u]], both wave-speed variants."""
+    O = oracle
+    cfg = O.make_cfg(1, O.EQ_BURGERS, 3)
+    rng = np.random.default_rng(3)
+    for _ in range(50):
+        ul, ur = rng.uniform(-2, 2, 1), rng.uniform(-2, 2, 1)
+        f = O.two_point_flux(cfg, O.FLUX_EC, ul, ur, 1)
+        assert f[0] == pytest.approx(O.two_point_flux(cfg, O.FLUX_EC, ur, ul, 1)[0], rel=1e-15)
+        assert (ur - ul)[0] * f[0] == pytest.approx((ur[0] ** 3 - ul[0] ** 3) / 6, rel=1e-12, abs=1e-14)
+        assert O.two_point_flux(cfg, O.FLUX_EC, ul, ul, 1)[0] == pytest.approx(0.5 * ul[0] ** 2, rel=1e-15)
+        lam = max(abs(ul[0]), abs(ur[0]))
+        expect = 0.25 * (ul[0] ** 2 + ur[0] ** 2) - 0.5 * lam * (ur[0] - ul[0])
+        for kind in (O.FLUX_LLF, O.FLUX_LLF_NAIVE):
+            assert O.two_point_flux(cfg, kind, ul, ur, 1)[0] == pytest.approx(expect, rel=1e-14, abs=1e-15)

@@ -556,3 +556,60 @@ def test_mhd2d_entropy_conservation_with_powell(oracle):
     # a constant state with constant B stays constant
     u = O.compute_coefficients(cfg, mesh, O.IC_CONSTANT)
     assert np.abs(O.rhs(cfg, mesh, u)).max() < 1e-13
+
+
+def test_burgers_source_terms_manufacture_the_convergence_test(oracle):
+    O = oracle
+    cfg = O.make_cfg(1, O.EQ_BURGERS, 3)
+    rng = np.random.default_rng(5)
+    h = 1e-5
+    ic = lambda xx, tt: O.initial_condition(cfg, O.IC_CONVERGENCE_TEST, np.array([xx]), tt)
+    for _ in range(8):
+        x, t = rng.uniform(0, 1), rng.uniform(0, 1)
+        ut = (ic(x, t + h) - ic(x, t - h)) / (2 * h)
+        fx = (O.flux(cfg, ic(x + h, t), 1) - O.flux(cfg, ic(x - h, t), 1)) / (2 * h)
+        S = O.source_convergence_test(cfg, np.array([x]), t)
+        assert abs(S[0] - (ut + fx)[0]) < 1e-7 * max(1.0, abs(S[0]))
+
+
+@pytest.mark.parametrize("periodic,vi", [(True, "weak"), (True, "fd"), (False, "weak"), (False, "fd")])
+def test_burgers_converges_to_the_time_derivative(oracle, periodic, vi):
+    """rhs!(u_exact(t)) - u_t -> 0 at the DG rate for the Burgers convergence
+    test with its source, periodic or with the exact state as inflow
+    (Dirichlet, x_neg) and the outflow condition flux(u_inner) on x_pos (the
+    wave moves right: u >= 1)."""
+    O = oracle
+    kw = dict(source_terms=O.SOURCE_CONVERGENCE_TEST)
+    if not periodic:
+        kw.update(boundary_condition=O.BC_DIRICHLET, boundary_ic=O.IC_CONVERGENCE_TEST, boundary_outflow_faces=2)
+    cfg = O.make_cfg(1, O.EQ_BURGERS, 3, O.VI_WEAK if vi == "weak" else O.VI_FD, O.FLUX_EC, O.FLUX_LLF, **kw)
+    errs = []
+    t, h = 0.3, 1e-6
+    for lvl in (4, 5):
+        mesh = O.Mesh(1, lvl, 0.0, 1.0, 3, periodic=periodic)
+        u = O.compute_coefficients(cfg, mesh, O.IC_CONVERGENCE_TEST, t)
+        ut = (O.compute_coefficients(cfg, mesh, O.IC_CONVERGENCE_TEST, t + h)
+              - O.compute_coefficients(cfg, mesh, O.IC_CONVERGENCE_TEST, t - h)) / (2 * h)
+        errs.append(np.abs(O.rhs(cfg, mesh, u, t=t) - ut).max())
+    rate = np.log2(errs[0] / errs[1])
+    assert rate > 2.5, (errs, rate)
+
+
+def test_burgers_entropy_conservation_and_outflow(oracle):
+    O = oracle
+    mesh = O.Mesh(1, 4, 0.0, 1.0, 3)
+    cfg = O.make_cfg(1, O.EQ_BURGERS, 3, O.VI_FD, O.FLUX_EC, O.FLUX_EC)
+    u = O.compute_coefficients(cfg, mesh, O.IC_SEEDED_RANDOM)
+    du = O.rhs(cfg, mesh, u)
+    assert abs(entropy_rate(O, cfg, mesh, u, du)) < 1e-13 * np.abs(du).max()
+    cfg = O.make_cfg(1, O.EQ_BURGERS, 3, O.VI_FD, O.FLUX_EC, O.FLUX_LLF)
+    du = O.rhs(cfg, mesh, u)
+    assert entropy_rate(O, cfg, mesh, u, du) < -1e-6
+    # outflow on both sides: the total changes only by the boundary fluxes
+    # f(u) = u^2/2 of the end nodes, d/dt int u = u(0)^2/2 - u(1)^2/2
+    mesh = O.Mesh(1, 4, 0.0, 1.0, 3, periodic=False)
+    cfg = O.make_cfg(1, O.EQ_BURGERS, 3, O.VI_FD, O.FLUX_EC, O.FLUX_LLF, boundary_condition=O.BC_DIRICHLET,
+                     boundary_outflow_faces=3)
+    du = O.rhs(cfg, mesh, u)
+    expect = 0.5 * u[0, 0, 0] ** 2 - 0.5 * u[-1, -1, 0] ** 2
+    assert O.integrate(cfg, mesh, du)[0] == pytest.approx(expect, rel=1e-12)

@@ -19,6 +19,8 @@ from .equations import (BoundaryConditionDirichlet, boundary_condition_periodic,
                         CompressibleEulerEquations1D, CompressibleEulerEquations2D, CompressibleEulerEquations3D,
                         FluxLaxFriedrichs, IdealGlmMhdEquations2D, IdealGlmMhdEquations3D, LinearScalarAdvectionEquation1D,
                         LinearScalarAdvectionEquation2D, LinearScalarAdvectionEquation3D, density, density_pressure, pressure,
+                        InviscidBurgersEquation1D, first, flux_ec, initial_condition_shock,
+                        initial_condition_rarefaction, boundary_condition_outflow,
                         flux_central, flux_hindenlang_gassner, flux_lax_friedrichs, flux_nonconservative_powell,
                         flux_ranocha, flux_shima_etal, initial_condition_constant,
                         initial_condition_convergence_test, initial_condition_seeded_random,

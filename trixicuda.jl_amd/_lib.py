@@ -11,14 +11,14 @@ LIBRARY_PATH = os.environ.get("TRIXI_HIP_LIBRARY") or os.path.join(HERE, "libtri
 
 OK, EINVAL, EUNSUPPORTED, EDEVICE, ENOMEM, ESTATE = 0, -1, -2, -3, -4, -5
 
-EQ_LINADV, EQ_EULER, EQ_GLM_MHD = 0, 1, 2
+EQ_LINADV, EQ_EULER, EQ_GLM_MHD, EQ_BURGERS = 0, 1, 2, 3
 VI_WEAK, VI_FD, VI_HG = 0, 1, 2
-FLUX_CENTRAL, FLUX_LLF, FLUX_LLF_NAIVE, FLUX_RANOCHA, FLUX_SHIMA_ETAL, FLUX_HINDENLANG_GASSNER = range(6)
+FLUX_CENTRAL, FLUX_LLF, FLUX_LLF_NAIVE, FLUX_RANOCHA, FLUX_SHIMA_ETAL, FLUX_HINDENLANG_GASSNER, FLUX_EC = range(7)
 PREC_FP64, PREC_MIXED = 0, 1
 BC_PERIODIC, BC_DIRICHLET = 0, 1
 SOURCE_NONE, SOURCE_CONVERGENCE_TEST = 0, 1
 (IC_CONSTANT, IC_CONVERGENCE_TEST, IC_WEAK_BLAST_WAVE, IC_SEEDED_RANDOM, IC_SMOOTH_PERIODIC,
- IC_WEAK_BLAST_WAVE_SEEDED) = range(6)
+ IC_WEAK_BLAST_WAVE_SEEDED, IC_SHOCK, IC_RAREFACTION) = range(8)
 
 
 class TrixiHipError(RuntimeError):
@@ -49,7 +49,7 @@ class Config(ctypes.Structure):
         ("boundary_ic", ctypes.c_int32),
         ("boundary_seed", ctypes.c_uint64),
         ("source_terms", ctypes.c_int32),
-        ("reserved2", ctypes.c_int32),
+        ("boundary_outflow_faces", ctypes.c_int32),
     ]
 
 

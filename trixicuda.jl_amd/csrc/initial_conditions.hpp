@@ -54,6 +54,21 @@ __device__ inline double smooth_field(uint64_t seed, int field, int nd, const do
 // reference tests and examples, plus the seeded benchmark states).
 __device__ inline void initial_condition(const IcParams& q, const double* x, double t, uint64_t idx,
                                   double* u) {
+    if (q.eq == EQ_BURGERS) {
+        if (q.ic == TRIXI_HIP_IC_CONVERGENCE_TEST) {
+            // c + A sin(omega (x - t)), c = 2, A = 1, omega = 2 pi
+            u[0] = 2.0 + sin(2.0 * M_PI * (x[0] - t));
+        } else if (q.ic == TRIXI_HIP_IC_SHOCK) {
+            u[0] = x[0] < 0.5 ? 1.5 : 0.5;
+        } else if (q.ic == TRIXI_HIP_IC_RAREFACTION) {
+            u[0] = x[0] < 0.5 ? 0.5 : 1.5;
+        } else if (q.ic == TRIXI_HIP_IC_SEEDED_RANDOM) {
+            u[0] = 1.0 + 0.5 * (2.0 * u01(q.seed, idx, 0) - 1.0);
+        } else {
+            u[0] = 2.0;
+        }
+        return;
+    }
     if (q.eq == EQ_LINADV) {
         if (q.ic == TRIXI_HIP_IC_CONVERGENCE_TEST) {
             double xs = 0.0;
@@ -176,8 +191,20 @@ __device__ inline void initial_condition(const IcParams& q, const double* x, dou
 
 // Dirichlet boundary states built into the non-periodic kernels: the
 // convergence-test state of compressible Euler (initial_condition_convergence_test)
-// and the constant state; the library rejects other boundary states.
+// and the constant state; for the Burgers equation its convergence-test wave
+// and the shock / rarefaction steps.  The library rejects other boundary states.
 __host__ __device__ inline void boundary_value(const IcParams& q, const double* x, double t, double* u) {
+    if (q.eq == EQ_BURGERS) {
+        if (q.ic == TRIXI_HIP_IC_CONVERGENCE_TEST)
+            u[0] = 2.0 + sin(2.0 * M_PI * (x[0] - t));
+        else if (q.ic == TRIXI_HIP_IC_SHOCK)
+            u[0] = x[0] < 0.5 ? 1.5 : 0.5;
+        else if (q.ic == TRIXI_HIP_IC_RAREFACTION)
+            u[0] = x[0] < 0.5 ? 0.5 : 1.5;
+        else
+            u[0] = 2.0;
+        return;
+    }
     if (q.ic == TRIXI_HIP_IC_CONVERGENCE_TEST) {
         double xs = 0.0;
         for (int d = 0; d < q.dim; ++d) xs += x[d];
@@ -209,6 +236,14 @@ __host__ __device__ inline void source_convergence_test(int dim, double gm1, con
     s[0] = (dim - 1) * gp;
     for (int d = 0; d < dim; ++d) s[1 + d] = (dim - 1) * gp + pp;
     s[dim + 1] = 2.0 * (dim - 1) * g * gp + dim * pp;
+}
+
+// source_terms_convergence_test of the inviscid Burgers equation: with
+// u = c + A sin(omega (x - t)) (c = 2, A = 1, omega = 2 pi),
+// S = u_t + u u_x = omega A cos(omega (x - t)) (c - 1 + A sin(omega (x - t))).
+__host__ __device__ inline double source_burgers_convergence_test(double x, double t) {
+    const double arg = 2.0 * M_PI * (x - t);
+    return 2.0 * M_PI * cos(arg) * (1.0 + sin(arg));
 }
 
 }  // namespace trixi_hip

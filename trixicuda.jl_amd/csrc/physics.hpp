@@ -22,7 +22,7 @@
 
 namespace trixi_hip {
 
-enum Equation : int { EQ_LINADV = 0, EQ_EULER = 1, EQ_GLM_MHD = 2 };
+enum Equation : int { EQ_LINADV = 0, EQ_EULER = 1, EQ_GLM_MHD = 2, EQ_BURGERS = 3 };
 enum VolumeIntegral : int { VI_WEAK = 0, VI_FD = 1, VI_HG = 2 };
 enum Flux : int {
     FLUX_CENTRAL = 0,
@@ -30,7 +30,8 @@ enum Flux : int {
     FLUX_LLF_NAIVE = 2,   // FluxLaxFriedrichs(max_abs_speed_naive)
     FLUX_RANOCHA = 3,
     FLUX_SHIMA_ETAL = 4,
-    FLUX_HINDENLANG_GASSNER = 5
+    FLUX_HINDENLANG_GASSNER = 5,
+    FLUX_EC = 6           // flux_ec of the inviscid Burgers equation
 };
 
 struct PhysParams {
@@ -167,6 +168,64 @@ struct LinAdv {
     template <typename R>
     __device__ static void max_abs_speeds(const R* u, const PhysR<R>& P, R* lam) {
         for (int d = 0; d < DIM; ++d) lam[d] = dabs(P.a[d]);
+    }
+};
+
+// ===========================================================================
+// Inviscid Burgers equation in 1D (Trixi InviscidBurgersEquation1D):
+// u_t + (u^2 / 2)_x = 0.  flux_ec is the entropy-conservative two-point flux
+// (u_l^2 + u_l u_r + u_r^2) / 6 for the entropy u^2 / 2; the wave speed of
+// both LLF variants is max(|u_l|, |u_r|).
+// ===========================================================================
+struct Burgers {
+    static constexpr int NV = 1;
+    static constexpr bool NONCONS = false;
+    static constexpr int NRF = 1;
+    template <typename R>
+    struct Node {
+        R u;
+        double lrho, lp;  // unused
+        __device__ R cons(int) const { return u; }
+    };
+    template <typename R, bool LOGS>
+    __device__ static Node<R> node(const R* u, const PhysR<R>&) {
+        Node<R> n;
+        n.u = u[0];
+        n.lrho = n.lp = 0.0;
+        return n;
+    }
+    template <typename R>
+    __device__ static void to_rec(const Node<R>& n, R* f) {
+        f[0] = n.u;
+    }
+    template <typename R>
+    __device__ static Node<R> from_rec(const R* f, double, double, const PhysR<R>&) {
+        Node<R> n;
+        n.u = f[0];
+        n.lrho = n.lp = 0.0;
+        return n;
+    }
+    template <typename R>
+    __device__ static void flux(const Node<R>& n, int, const PhysR<R>&, R* f) {
+        f[0] = R(0.5) * n.u * n.u;
+    }
+    template <int KIND, typename R>
+    __device__ static void two_point(const Node<R>& l, const Node<R>& r, int, const PhysR<R>&, R* f) {
+        if (KIND == FLUX_EC) {
+            f[0] = (l.u * l.u + l.u * r.u + r.u * r.u) * R(1.0 / 6.0);
+        } else {
+            const R fl = R(0.5) * l.u * l.u, fr = R(0.5) * r.u * r.u;
+            if (KIND == FLUX_LLF || KIND == FLUX_LLF_NAIVE) {
+                const R lam = dmax(dabs(l.u), dabs(r.u));
+                f[0] = R(0.5) * (fl + fr) + (R(-0.5) * lam * (r.u - l.u));
+            } else {
+                f[0] = R(0.5) * (fl + fr);
+            }
+        }
+    }
+    template <typename R>
+    __device__ static void max_abs_speeds(const R* u, const PhysR<R>&, R* lam) {
+        lam[0] = dabs(u[0]);
     }
 };
 

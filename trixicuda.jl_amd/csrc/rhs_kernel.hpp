@@ -83,6 +83,7 @@ struct RhsArgs {
     IcParams bc_ic;           // Dirichlet boundary state (faces with nbr < 0)
     int bc;                   // enum trixi_hip_boundary_condition
     int src;                  // enum trixi_hip_source_terms
+    int outflow;              // bit f: boundary face direction f takes the inner state
     double t_host;            // rhs mode: time t
     // L2 mortars: a face with nbr = -2 - slot takes its surface flux from
     // mflux[(slot*NL + face node)*NV + v]; mnb[slot*mK + k] lists the elements
@@ -105,12 +106,18 @@ __device__ __forceinline__ void boundary_state(const RhsArgs& a, int64_t e, int 
     for (int v = 0; v < NV; ++v) ub[v] = (R)uu[v];
 }
 
-// source_terms_convergence_test at node `node` of element e, FP64.
+// source_terms_convergence_test at node `node` of element e, FP64
+// (compressible Euler, or the inviscid Burgers equation: the only scalar
+// equation built with source terms).
 template <int DIM, int NN, int NV>
 __device__ __forceinline__ void source_at(const RhsArgs& a, int64_t e, int node, double t, double* sv) {
     double xx[3] = {0.0, 0.0, 0.0};
 #pragma unroll
     for (int d = 0; d < DIM; ++d) xx[d] = a.x[((int64_t)d * NN + node) * a.E + e];
+    if constexpr (NV == 1) {
+        sv[0] = source_burgers_convergence_test(xx[0], t);
+        return;
+    }
     double s5[5];
     source_convergence_test(DIM, a.phys.gm1, xx, t, s5);
 #pragma unroll
@@ -352,8 +359,24 @@ rhs_kernel(const RhsArgs a) {
             up[v] = (R)up_next[v];
         }
         if (EXT && a.bc) {
-            if (nb[2 * d] == -1) boundary_state<DIM, NN, NV, R>(a, ec, base, tnow, um);
-            if (nb[2 * d + 1] == -1) boundary_state<DIM, NN, NV, R>(a, ec, base + (N - 1) * s, tnow, up);
+            // outflow sides: the inner face state on both sides, so that the
+            // (consistent) surface flux is flux(u_inner)
+            if (nb[2 * d] == -1) {
+                if ((a.outflow >> (2 * d)) & 1) {
+#pragma unroll
+                    for (int v = 0; v < NV; ++v) um[v] = (R)a.u[(int64_t)(v * NN + base) * E + ec];
+                } else {
+                    boundary_state<DIM, NN, NV, R>(a, ec, base, tnow, um);
+                }
+            }
+            if (nb[2 * d + 1] == -1) {
+                if ((a.outflow >> (2 * d + 1)) & 1) {
+#pragma unroll
+                    for (int v = 0; v < NV; ++v) up[v] = (R)a.u[(int64_t)(v * NN + base + (N - 1) * s) * E + ec];
+                } else {
+                    boundary_state<DIM, NN, NV, R>(a, ec, base + (N - 1) * s, tnow, up);
+                }
+            }
         }
         if (!PD && d + 1 < DIM) {
             const int sn = cpow(N, d + 1);

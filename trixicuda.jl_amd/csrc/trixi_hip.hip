@@ -58,6 +58,7 @@ void register_euler2d_hg();
 void register_euler3d_hg();
 void register_mhd3d();
 void register_mhd2d();
+void register_burgers();
 void register_vjp_linadv();
 void register_vjp_euler1d();
 void register_vjp_euler2d();
@@ -79,6 +80,7 @@ static void ensure_registered() {
         register_euler3d_hg();
         register_mhd3d();
         register_mhd2d();
+        register_burgers();
         register_vjp_linadv();
         register_vjp_euler1d();
         register_vjp_euler2d();
@@ -325,13 +327,19 @@ __device__ __forceinline__ int max_mode(int n) {
 template <int DIM, int N>
 __device__ __forceinline__ double modal_indicator(const double* __restrict__ u, int64_t ec, int64_t E,
                                                   const PhysR<double>& P, const double* __restrict__ Vinv,
-                                                  double* buf, int el, int n, int var) {
+                                                  double* buf, int el, int n, int var, int nv) {
     constexpr int NN = cpow(N, DIM);
-    constexpr int NV = DIM + 2;
-    double uc[NV];
+    // compressible Euler (DIM + 2 variables) or a scalar equation (nv = 1:
+    // the variable is the solution itself)
+    double m;
+    if (nv == 1) {
+        m = u[(int64_t)n * E + ec];
+    } else {
+        double uc[DIM + 2];
 #pragma unroll
-    for (int v = 0; v < NV; ++v) uc[v] = u[((int64_t)v * NN + n) * E + ec];
-    double m = Euler<DIM>::indicator_variable(uc, P, var);
+        for (int v = 0; v < DIM + 2; ++v) uc[v] = u[((int64_t)v * NN + n) * E + ec];
+        m = Euler<DIM>::indicator_variable(uc, P, var);
+    }
 #pragma unroll
     for (int d = 0; d < DIM; ++d) {
         const int s = cpow(N, d);
@@ -365,7 +373,7 @@ template <int DIM, int N>
 __global__ void __launch_bounds__(64) indicator_kernel(const double* __restrict__ u, double* __restrict__ alpha_raw,
                                                        int64_t E, PhysParams phys, const double* __restrict__ Vinv,
                                                        double threshold, double parameter_s, double alpha_min,
-                                                       double alpha_max, int var) {
+                                                       double alpha_max, int var, int nv) {
     constexpr int NN = cpow(N, DIM);
     constexpr int EPB = 64 / NN;
     static_assert(NN <= 64 && 64 % NN == 0, "one block holds whole elements");
@@ -375,7 +383,7 @@ __global__ void __launch_bounds__(64) indicator_kernel(const double* __restrict_
     const int64_t e = (int64_t)blockIdx.x * EPB + el;
     const int64_t ec = e < E ? e : E - 1;
     const PhysR<double> P = PhysR<double>::from(phys);
-    const double m = modal_indicator<DIM, N>(u, ec, E, P, Vinv, buf, el, n, var);
+    const double m = modal_indicator<DIM, N>(u, ec, E, P, Vinv, buf, el, n, var, nv);
     e2[threadIdx.x] = m * m;
     __syncthreads();
     if (n != 0 || e >= E) return;
@@ -430,7 +438,7 @@ __global__ void __launch_bounds__(64) indicator_vjp_kernel(
     const double* __restrict__ u, double* __restrict__ ubar, const double* __restrict__ alphabar,
     const int32_t* __restrict__ winner, const int32_t* __restrict__ nbr, int smooth, int64_t E, PhysParams phys,
     const double* __restrict__ Vinv, double threshold, double parameter_s, double alpha_min, double alpha_max,
-    int var) {
+    int var, int nv) {
     constexpr int NN = cpow(N, DIM);
     constexpr int NV = DIM + 2;
     constexpr int EPB = 64 / NN;
@@ -443,7 +451,7 @@ __global__ void __launch_bounds__(64) indicator_vjp_kernel(
     const bool valid = e < E;
     const int64_t ec = valid ? e : E - 1;
     const PhysR<double> P = PhysR<double>::from(phys);
-    const double m = modal_indicator<DIM, N>(u, ec, E, P, Vinv, buf, el, n, var);
+    const double m = modal_indicator<DIM, N>(u, ec, E, P, Vinv, buf, el, n, var, nv);
     e2[threadIdx.x] = m * m;
     __syncthreads();
     if (n == 0) {
@@ -731,7 +739,7 @@ __global__ void check_state_kernel(const double* __restrict__ u, int32_t* __rest
                 uc[v] = u[((int64_t)v * nn + n) * E + e];
                 if (!isfinite(uc[v])) f |= 1;
             }
-            if (eq == EQ_LINADV) continue;
+            if (eq == EQ_LINADV || eq == EQ_BURGERS) continue;
             if (!(uc[0] > 0.0)) f |= 2;
             double ke = 0.0;
             const int nvel = (eq == EQ_EULER) ? dim : 3;
@@ -822,7 +830,7 @@ static const double CK_C[5] = {0.0, 1432997174477.0 / 9575080441755.0, 252626934
                                2006345519317.0 / 3224310063776.0, 2802321613138.0 / 2924317926251.0};
 
 static int nvars_of(const trixi_hip_config& c) {
-    if (c.equation == TRIXI_HIP_LINEAR_SCALAR_ADVECTION) return 1;
+    if (c.equation == TRIXI_HIP_LINEAR_SCALAR_ADVECTION || c.equation == TRIXI_HIP_INVISCID_BURGERS) return 1;
     if (c.equation == TRIXI_HIP_COMPRESSIBLE_EULER) return c.ndims + 2;
     return 9;
 }
@@ -866,15 +874,15 @@ static int launch_indicator(trixi_hip_ctx* c, const double* u, hipStream_t s) {
     if (c->dim == 1)
         hipLaunchKernelGGL((indicator_kernel<1, 4>), dim3(blocks), dim3(64), 0, s, u, c->d_alpha_raw, c->E, P,
                            c->d_Vinv, threshold, parameter_s, c->cfg.alpha_min, c->cfg.alpha_max,
-                           c->cfg.indicator_variable);
+                           c->cfg.indicator_variable, c->nv);
     else if (c->dim == 2)
         hipLaunchKernelGGL((indicator_kernel<2, 4>), dim3(blocks), dim3(64), 0, s, u, c->d_alpha_raw, c->E, P,
                            c->d_Vinv, threshold, parameter_s, c->cfg.alpha_min, c->cfg.alpha_max,
-                           c->cfg.indicator_variable);
+                           c->cfg.indicator_variable, c->nv);
     else
         hipLaunchKernelGGL((indicator_kernel<3, 4>), dim3(blocks), dim3(64), 0, s, u, c->d_alpha_raw, c->E, P,
                            c->d_Vinv, threshold, parameter_s, c->cfg.alpha_min, c->cfg.alpha_max,
-                           c->cfg.indicator_variable);
+                           c->cfg.indicator_variable, c->nv);
     HIP_TRY(hipGetLastError());
     return TRIXI_HIP_OK;
 }
@@ -987,24 +995,39 @@ int trixi_hip_create(const trixi_hip_config* config, const trixi_hip_mesh* mesh,
     if (cfg.polydeg < 1 || cfg.polydeg + 1 > MAXN) return fail(TRIXI_HIP_EINVAL, "polydeg out of range");
     if (cfg.equation == TRIXI_HIP_IDEAL_GLM_MHD && cfg.ndims == 1)
         return fail(TRIXI_HIP_EUNSUPPORTED, "GLM-MHD is built for 2D and 3D");
-    if (cfg.volume_integral == TRIXI_HIP_VOLUME_SHOCK_CAPTURING_HG && cfg.equation != TRIXI_HIP_COMPRESSIBLE_EULER)
-        return fail(TRIXI_HIP_EUNSUPPORTED, "shock capturing is built for compressible Euler only");
+    if (cfg.equation == TRIXI_HIP_INVISCID_BURGERS && cfg.ndims != 1)
+        return fail(TRIXI_HIP_EUNSUPPORTED, "the inviscid Burgers equation is built for 1D");
+    if (cfg.volume_integral == TRIXI_HIP_VOLUME_SHOCK_CAPTURING_HG && cfg.equation != TRIXI_HIP_COMPRESSIBLE_EULER &&
+        cfg.equation != TRIXI_HIP_INVISCID_BURGERS)
+        return fail(TRIXI_HIP_EUNSUPPORTED, "shock capturing is built for compressible Euler and Burgers");
+    if (cfg.volume_integral == TRIXI_HIP_VOLUME_SHOCK_CAPTURING_HG && cfg.equation == TRIXI_HIP_INVISCID_BURGERS &&
+        cfg.indicator_variable != TRIXI_HIP_INDICATOR_DENSITY)
+        return fail(TRIXI_HIP_EUNSUPPORTED, "the Burgers indicator variable is the solution itself (first)");
     if (cfg.indicator_variable < TRIXI_HIP_INDICATOR_DENSITY_PRESSURE ||
         cfg.indicator_variable > TRIXI_HIP_INDICATOR_DENSITY)
         return fail(TRIXI_HIP_EINVAL, "unknown indicator variable");
     if (cfg.boundary_condition != TRIXI_HIP_BC_PERIODIC) {
         if (cfg.boundary_condition != TRIXI_HIP_BC_DIRICHLET)
             return fail(TRIXI_HIP_EINVAL, "unknown boundary condition");
-        if (cfg.equation != TRIXI_HIP_COMPRESSIBLE_EULER ||
-            (cfg.boundary_ic != TRIXI_HIP_IC_CONVERGENCE_TEST && cfg.boundary_ic != TRIXI_HIP_IC_CONSTANT))
+        const bool euler_ok = cfg.equation == TRIXI_HIP_COMPRESSIBLE_EULER &&
+                              (cfg.boundary_ic == TRIXI_HIP_IC_CONVERGENCE_TEST || cfg.boundary_ic == TRIXI_HIP_IC_CONSTANT);
+        const bool burgers_ok = cfg.equation == TRIXI_HIP_INVISCID_BURGERS &&
+                                (cfg.boundary_ic == TRIXI_HIP_IC_CONVERGENCE_TEST || cfg.boundary_ic == TRIXI_HIP_IC_CONSTANT ||
+                                 cfg.boundary_ic == TRIXI_HIP_IC_SHOCK || cfg.boundary_ic == TRIXI_HIP_IC_RAREFACTION);
+        if (!euler_ok && !burgers_ok)
             return fail(TRIXI_HIP_EUNSUPPORTED,
-                        "Dirichlet boundaries are built for compressible Euler with the convergence-test or "
-                        "constant state");
+                        "Dirichlet boundaries are built for compressible Euler (convergence-test or constant state) "
+                        "and Burgers (convergence-test wave, shock, rarefaction, constant)");
+        if (cfg.boundary_outflow_faces < 0 || cfg.boundary_outflow_faces >= (1 << (2 * cfg.ndims)))
+            return fail(TRIXI_HIP_EINVAL, "outflow face bits beyond the mesh's faces");
+    } else if (cfg.boundary_outflow_faces != 0) {
+        return fail(TRIXI_HIP_EINVAL, "outflow faces need a non-periodic boundary configuration");
     }
     if (cfg.source_terms != TRIXI_HIP_SOURCE_NONE) {
         if (cfg.source_terms != TRIXI_HIP_SOURCE_CONVERGENCE_TEST) return fail(TRIXI_HIP_EINVAL, "unknown source terms");
-        if (cfg.equation != TRIXI_HIP_COMPRESSIBLE_EULER)
-            return fail(TRIXI_HIP_EUNSUPPORTED, "source_terms_convergence_test is built for compressible Euler");
+        if (cfg.equation != TRIXI_HIP_COMPRESSIBLE_EULER && cfg.equation != TRIXI_HIP_INVISCID_BURGERS)
+            return fail(TRIXI_HIP_EUNSUPPORTED,
+                        "source_terms_convergence_test is built for compressible Euler and Burgers");
     }
     const int64_t NM = mesh->nmortars;
     if (NM < 0) return fail(TRIXI_HIP_EINVAL, "negative mortar count");
@@ -1195,6 +1218,7 @@ int trixi_hip_create(const trixi_hip_config* config, const trixi_hip_mesh* mesh,
     a.x = c->d_x;
     a.bc = cfg.boundary_condition;
     a.src = cfg.source_terms;
+    a.outflow = cfg.boundary_outflow_faces;
     a.mflux = c->d_mflux;
     a.mnb = c->d_mnb;
     a.mK = K;
@@ -1423,6 +1447,8 @@ int trixi_hip_max_dt(trixi_hip_ctx* ctx, const double* u, double* max_dt, trixi_
         if (dim == 1) MS(LinAdv<1>, 1); else if (dim == 2) MS(LinAdv<2>, 2); else MS(LinAdv<3>, 3);
     } else if (eq == TRIXI_HIP_COMPRESSIBLE_EULER) {
         if (dim == 1) MS(Euler<1>, 1); else if (dim == 2) MS(Euler<2>, 2); else MS(Euler<3>, 3);
+    } else if (eq == TRIXI_HIP_INVISCID_BURGERS) {
+        MS(Burgers, 1);
     } else {
         if (dim == 2) MS(GlmMhd, 2); else MS(GlmMhd, 3);
     }
@@ -1533,6 +1559,8 @@ static bool is_hg(const trixi_hip_ctx* c) {
 static int require_vjp(trixi_hip_ctx* c) {
     if (c->vjp_fn) return TRIXI_HIP_OK;
     if (c->nmortars > 0) return fail(TRIXI_HIP_EUNSUPPORTED, "no adjoint kernel for meshes with mortars");
+    if (c->cfg.boundary_outflow_faces != 0)
+        return fail(TRIXI_HIP_EUNSUPPORTED, "no adjoint kernel for outflow boundaries");
     KernelKey k = make_key(c->cfg, 0);
     k.prec = 0;
     VjpFn fn = find_vjp(k);
@@ -1569,7 +1597,7 @@ static int enqueue_vjp(trixi_hip_ctx* c, const VjpArgs& v, hipStream_t s) {
     hipLaunchKernelGGL((indicator_vjp_kernel<D, 4>), dim3(blocks), dim3(threads), 0, s, v.u, v.ubar,            \
                        (const double*)c->d_alphabar, (const int32_t*)c->d_winner, (const int32_t*)c->d_nbr,    \
                        c->cfg.alpha_smooth, c->E, P, (const double*)c->d_Vinv, threshold, parameter_s,       \
-                       c->cfg.alpha_min, c->cfg.alpha_max, c->cfg.indicator_variable)
+                       c->cfg.alpha_min, c->cfg.alpha_max, c->cfg.indicator_variable, c->nv)
         if (c->dim == 1)
             TRIXI_HIP_IND_VJP(1);
         else if (c->dim == 2)

@@ -21,7 +21,7 @@ class AbstractEquations:
 
     @property
     def nvariables(self):
-        if self.equation_id == L.EQ_LINADV:
+        if self.equation_id in (L.EQ_LINADV, L.EQ_BURGERS):
             return 1
         if self.equation_id == L.EQ_EULER:
             return self.ndims + 2
@@ -90,6 +90,13 @@ IdealGlmMhdEquations2D = _glm_mhd(2)
 IdealGlmMhdEquations3D = _glm_mhd(3)
 
 
+@dataclass(frozen=True)
+class InviscidBurgersEquation1D(AbstractEquations):
+    def __init__(self):
+        object.__setattr__(self, "ndims", 1)
+        object.__setattr__(self, "equation_id", L.EQ_BURGERS)
+
+
 # ----------------------------------------------------------------------------
 # fluxes
 # ----------------------------------------------------------------------------
@@ -123,6 +130,7 @@ flux_lax_friedrichs = FluxLaxFriedrichs(max_abs_speed)
 flux_ranocha = NumericalFlux("flux_ranocha", L.FLUX_RANOCHA)
 flux_shima_etal = NumericalFlux("flux_shima_etal", L.FLUX_SHIMA_ETAL)
 flux_hindenlang_gassner = NumericalFlux("flux_hindenlang_gassner", L.FLUX_HINDENLANG_GASSNER)
+flux_ec = NumericalFlux("flux_ec", L.FLUX_EC)  # entropy-conservative flux of the Burgers equation
 flux_nonconservative_powell = NumericalFlux("flux_nonconservative_powell", -1)
 
 
@@ -135,6 +143,10 @@ def pressure(*args):
 
 
 def density(*args):
+    raise TypeError("indicator variables are selected, not called, on the HIP path")
+
+
+def first(*args):  # Julia's Base.first: the solution of a scalar equation as indicator variable
     raise TypeError("indicator variables are selected, not called, on the HIP path")
 
 
@@ -154,6 +166,9 @@ class InitialCondition:
 initial_condition_constant = InitialCondition("initial_condition_constant", L.IC_CONSTANT)
 initial_condition_convergence_test = InitialCondition("initial_condition_convergence_test", L.IC_CONVERGENCE_TEST)
 initial_condition_weak_blast_wave = InitialCondition("initial_condition_weak_blast_wave", L.IC_WEAK_BLAST_WAVE)
+# the Burgers step states of test/tree_dgsem_1d/burgers_shock.jl and burgers_rarefraction.jl
+initial_condition_shock = InitialCondition("initial_condition_shock", L.IC_SHOCK)
+initial_condition_rarefaction = InitialCondition("initial_condition_rarefaction", L.IC_RAREFACTION)
 
 
 def initial_condition_seeded_random(seed=20261202):
@@ -180,6 +195,17 @@ class _Periodic:
 
 
 boundary_condition_periodic = _Periodic()
+
+
+class _Outflow:
+    """The outflow boundary of the reference's Burgers tests: the boundary flux
+    is the physical flux of the inner state, flux(u_inner)."""
+
+    def __repr__(self):
+        return "boundary_condition_outflow"
+
+
+boundary_condition_outflow = _Outflow()
 
 
 class BoundaryConditionDirichlet:

@@ -9,6 +9,7 @@
 module TrixiHIP
 
 using Trixi
+using StaticArrays: SVector
 using AMDGPU
 using ChainRulesCore
 using SciMLBase: ODEProblem, FullSpecialize
@@ -38,7 +39,7 @@ struct CConfig
     boundary_ic::Int32
     boundary_seed::UInt64
     source_terms::Int32
-    reserved2::Int32
+    boundary_outflow_faces::Int32
 end
 
 struct CMesh
@@ -72,6 +73,7 @@ equation_id(::CompressibleEulerEquations2D) = 1
 equation_id(::CompressibleEulerEquations3D) = 1
 equation_id(::IdealGlmMhdEquations3D) = 2
 equation_id(::IdealGlmMhdEquations2D) = 2
+equation_id(::InviscidBurgersEquation1D) = 3
 equation_id(eq) = throw(ArgumentError("equation $(typeof(eq)) is not built for HIP"))
 
 flux_id(::typeof(flux_central)) = 0
@@ -79,6 +81,7 @@ flux_id(f::FluxLaxFriedrichs) = f.max_abs_speed === max_abs_speed_naive ? 2 : 1
 flux_id(::typeof(flux_ranocha)) = 3
 flux_id(::typeof(flux_shima_etal)) = 4
 flux_id(::typeof(flux_hindenlang_gassner)) = 5
+flux_id(::typeof(flux_ec)) = 6
 flux_id(f::Tuple) = (f[2] === flux_nonconservative_powell ? flux_id(f[1]) :
                      throw(ArgumentError("unsupported nonconservative flux $(f[2])")))
 flux_id(f) = throw(ArgumentError("flux $f is not built for HIP"))
@@ -86,6 +89,12 @@ flux_id(f) = throw(ArgumentError("flux $f is not built for HIP"))
 ic_id(::typeof(initial_condition_constant)) = 0
 ic_id(::typeof(initial_condition_convergence_test)) = 1
 ic_id(::typeof(initial_condition_weak_blast_wave)) = 2
+# the Burgers step states of test/tree_dgsem_1d/burgers_shock.jl and
+# burgers_rarefraction.jl, as functions Trixi can evaluate too
+initial_condition_shock(x, t, ::InviscidBurgersEquation1D) = SVector(x[1] < 0.5 ? 1.5 : 0.5)
+initial_condition_rarefaction(x, t, ::InviscidBurgersEquation1D) = SVector(x[1] < 0.5 ? 0.5 : 1.5)
+ic_id(::typeof(initial_condition_shock)) = 6
+ic_id(::typeof(initial_condition_rarefaction)) = 7
 ic_id(f) = throw(ArgumentError("initial condition $f is not built for HIP"))
 # seeded benchmark states (ids 3, 4, 5 of trixi_hip_initial_condition) carry their seed
 struct SeededInitialCondition
@@ -122,22 +131,37 @@ mutable struct SemidiscretizationHyperbolicHIP{Mesh, Equations, IC, Solver}
     nvalues::Int
 end
 
+# The outflow boundary of the reference's Burgers tests: flux(u_inner).
+function boundary_condition_outflow(u_inner, orientation, direction, x, t, surface_flux_function, equations)
+    Trixi.flux(u_inner, orientation, equations)
+end
+
 # boundary_conditions: periodic, one BoundaryConditionDirichlet, or a NamedTuple
-# of them (x_neg, x_pos, ...) that all carry the same state
+# (x_neg, x_pos, ...) of Dirichlet conditions that all carry the same state and
+# boundary_condition_outflow sides (bit f of the outflow mask)
 function boundary_config(bcs)
-    bcs === boundary_condition_periodic && return (0, 0, UInt64(0))
+    bcs === boundary_condition_periodic && return (0, 0, UInt64(0), 0)
     sides = bcs isa NamedTuple ? collect(values(bcs)) : [bcs]
-    dir = [b for b in sides if b !== boundary_condition_periodic]
+    outflow = 0
+    if bcs isa NamedTuple
+        names = (:x_neg, :x_pos, :y_neg, :y_pos, :z_neg, :z_pos)
+        for (k, b) in pairs(bcs)
+            b === boundary_condition_outflow && (outflow |= 1 << (findfirst(==(k), names) - 1))
+        end
+    end
+    dir = [b for b in sides if b !== boundary_condition_periodic && b !== boundary_condition_outflow]
     all(b -> b isa BoundaryConditionDirichlet, dir) ||
-        throw(ArgumentError("only BoundaryConditionDirichlet and periodic boundaries are built for HIP"))
+        throw(ArgumentError("only BoundaryConditionDirichlet, outflow and periodic boundaries are built for HIP"))
+    isempty(dir) && return (1, 0, UInt64(0), outflow)
     f = dir[1].boundary_value_function
     all(b -> b.boundary_value_function === f, dir) ||
-        throw(ArgumentError("all boundary faces must carry the same Dirichlet state"))
-    (1, ic_id(f), ic_seed(f))
+        throw(ArgumentError("all Dirichlet boundary faces must carry the same state"))
+    (1, ic_id(f), ic_seed(f), outflow)
 end
 
 indicator_variable_id(::typeof(density_pressure)) = 0
 indicator_variable_id(::typeof(pressure)) = 1
+indicator_variable_id(::typeof(first)) = 2
 indicator_variable_id(::typeof(density)) = 2
 indicator_variable_id(f) = throw(ArgumentError("indicator variable $f is not built for HIP"))
 
@@ -162,10 +186,10 @@ function make_config(solver::DGSEMHIP, equations, mesh; boundary_conditions = bo
     a = hasproperty(equations, :advection_velocity) ? Tuple(equations.advection_velocity) : ()
     vel = ntuple(d -> d <= length(a) ? Float64(a[d]) : 0.0, 3)
     len = mesh.tree.length_level_0
-    bc, bc_ic, bc_seed = boundary_config(boundary_conditions)
+    bc, bc_ic, bc_seed, outflow = boundary_config(boundary_conditions)
     CConfig(ndims(mesh), equation_id(equations), solver.polydeg, volume_integral, vflux,
             flux_id(solver.surface_flux), fvflux, solver.precision === :mixed ? 1 : 0,
-            smooth, var, gamma, c_h, vel, amax, amin, len, bc, bc_ic, bc_seed, source_id(source_terms), 0)
+            smooth, var, gamma, c_h, vel, amax, amin, len, bc, bc_ic, bc_seed, source_id(source_terms), outflow)
 end
 
 function SemidiscretizationHyperbolicHIP(mesh::TreeMesh, equations, initial_condition,
@@ -322,6 +346,7 @@ function gradient_checkpointed(u0::ROCVector{Float64}, ubar_final::ROCVector{Flo
 end
 
 export DGSEMHIP, SemidiscretizationHyperbolicHIP, semidiscretizeHIP, rhs_hip!, rhs_hip,
+       initial_condition_shock, initial_condition_rarefaction, boundary_condition_outflow,
        max_dt_hip, calc_error_norms_hip, rk_step_hip!, rk_stage_hip!, hip_synchronize,
        rk_step_vjp_hip!, gradient_checkpointed
 

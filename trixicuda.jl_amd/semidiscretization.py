@@ -12,7 +12,8 @@ import ctypes
 import numpy as np
 
 from . import _lib as L
-from .equations import BoundaryConditionDirichlet, SourceTerms, boundary_condition_periodic
+from .equations import (BoundaryConditionDirichlet, SourceTerms, boundary_condition_outflow,
+                        boundary_condition_periodic, initial_condition_constant)
 
 
 def _torch():
@@ -84,17 +85,32 @@ class SemidiscretizationHyperbolicHIP:
                 raise ValueError("a non-periodic mesh needs boundary_conditions")
             c.boundary_condition = L.BC_PERIODIC
             return
-        sides = list(bcs.values()) if isinstance(bcs, dict) else \
-            (list(bcs._asdict().values()) if hasattr(bcs, "_asdict") else [bcs])
-        dirichlet = [b for b in sides if b is not boundary_condition_periodic]
-        if not dirichlet or not all(isinstance(b, BoundaryConditionDirichlet) for b in dirichlet):
-            raise NotImplementedError("only BoundaryConditionDirichlet and periodic boundaries are built")
-        f = dirichlet[0].boundary_value_function
-        if any(b.boundary_value_function != f for b in dirichlet):
-            raise NotImplementedError("all boundary faces must carry the same Dirichlet state")
+        names = ("x_neg", "x_pos", "y_neg", "y_pos", "z_neg", "z_pos")[:2 * mesh.ndims]
+        if isinstance(bcs, dict) or hasattr(bcs, "_asdict"):
+            per_side = dict(bcs) if isinstance(bcs, dict) else bcs._asdict()
+            if set(per_side) - set(names):
+                raise ValueError(f"unknown boundary names {sorted(set(per_side) - set(names))}")
+            sides = [per_side.get(n, boundary_condition_periodic) for n in names]
+        else:
+            sides = [bcs] * len(names)
+        outflow = 0
+        for f_dir, b in enumerate(sides):
+            if b is boundary_condition_outflow:
+                outflow |= 1 << f_dir
+        dirichlet = [b for b in sides if b is not boundary_condition_periodic and b is not boundary_condition_outflow]
+        if not all(isinstance(b, BoundaryConditionDirichlet) for b in dirichlet):
+            raise NotImplementedError("only BoundaryConditionDirichlet, boundary_condition_outflow and periodic "
+                                      "boundaries are built")
+        if dirichlet:
+            f = dirichlet[0].boundary_value_function
+            if any(b.boundary_value_function != f for b in dirichlet):
+                raise NotImplementedError("all Dirichlet boundary faces must carry the same state")
+        else:
+            f = initial_condition_constant
         c.boundary_condition = L.BC_DIRICHLET
         c.boundary_ic = f.ic_id
         c.boundary_seed = f.seed
+        c.boundary_outflow_faces = outflow
 
     # -- lifetime ------------------------------------------------------------
     def close(self):

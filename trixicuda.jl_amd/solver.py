@@ -10,7 +10,7 @@ import ctypes
 import numpy as np
 
 from . import _lib as L
-from .equations import density, density_pressure, flux_central, flux_ranocha, pressure, split_flux
+from .equations import density, density_pressure, first, flux_central, flux_ranocha, pressure, split_flux
 
 
 class LobattoLegendreBasisHIP:
@@ -51,8 +51,8 @@ class VolumeIntegralFluxDifferencing:
 class IndicatorHennemannGassner:
     def __init__(self, equations, basis, alpha_max=0.5, alpha_min=0.001, alpha_smooth=True,
                  variable=density_pressure):
-        if variable not in (density_pressure, pressure, density):
-            raise NotImplementedError("the indicator variable must be density_pressure, pressure or density")
+        if variable not in (density_pressure, pressure, density, first):
+            raise NotImplementedError("the indicator variable must be density_pressure, pressure, density or first")
         self.alpha_max, self.alpha_min, self.alpha_smooth = float(alpha_max), float(alpha_min), bool(alpha_smooth)
         self.variable = variable
 
@@ -118,7 +118,8 @@ class DGSEMHIP:
             fv = vi.volume_flux_fv if vi.volume_flux_fv is not None else sflux
             c.fv_flux = split_flux(fv)[0].flux_id
             c.alpha_smooth = 1 if vi.indicator.alpha_smooth else 0
-            c.indicator_variable = {density_pressure: 0, pressure: 1, density: 2}[vi.indicator.variable]
+            # first (the solution of a scalar equation) and density both take u[0]
+            c.indicator_variable = {density_pressure: 0, pressure: 1, density: 2, first: 2}[vi.indicator.variable]
             c.alpha_max, c.alpha_min = vi.indicator.alpha_max, vi.indicator.alpha_min
         c.precision = L.PREC_MIXED if self.precision == "mixed" else L.PREC_FP64
         c.gamma = getattr(equations, "gamma", 1.4)
