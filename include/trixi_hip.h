@@ -49,7 +49,8 @@ typedef void* trixi_hip_stream; /* a hipStream_t; NULL = the default stream */
 enum trixi_hip_equation {
     TRIXI_HIP_LINEAR_SCALAR_ADVECTION = 0, /* LinearScalarAdvectionEquation{1,2,3}D */
     TRIXI_HIP_COMPRESSIBLE_EULER = 1,      /* CompressibleEulerEquations{1,2,3}D    */
-    TRIXI_HIP_IDEAL_GLM_MHD = 2,           /* IdealGlmMhdEquations2D / 3D           */
+    TRIXI_HIP_IDEAL_GLM_MHD = 2,           /* IdealGlmMhdEquations1D / 2D / 3D (1D:
+                                              8 variables, no psi / Powell terms)  */
     TRIXI_HIP_INVISCID_BURGERS = 3         /* InviscidBurgersEquation1D             */
 };
 enum trixi_hip_volume_integral {

@@ -103,7 +103,7 @@ Cfg make_cfg(const oracle_cfg* c) {
     k.smooth = c->alpha_smooth;
     k.gamma = c->gamma;
     k.inv_gm1 = 1.0 / (c->gamma - 1.0);
-    k.c_h = c->c_h;
+    k.c_h = (c->equation == OR_EQ_GLM_MHD && c->ndims == 1) ? 0.0 : c->c_h;  // no cleaning field in 1D
     for (int d = 0; d < 3; ++d) k.a[d] = c->advection_velocity[d];
     k.amax = c->alpha_max;
     k.amin = c->alpha_min;
@@ -166,7 +166,7 @@ void mhd_prim(const Cfg& c, const T* u, T& rho, T* v, T& p, T* B, T& psi) {
     rho = u[0];
     for (int d = 0; d < 3; ++d) v[d] = u[1 + d] / rho;
     for (int d = 0; d < 3; ++d) B[d] = u[5 + d];
-    psi = u[8];
+    psi = (c.nv > 8) ? u[8] : T(0.0);  // 1D ideal MHD: no cleaning field
     p = (c.gamma - 1.0) *
         (u[4] - 0.5 * (u[1] * v[0] + u[2] * v[1] + u[3] * v[2]) -
          0.5 * (B[0] * B[0] + B[1] * B[1] + B[2] * B[2]) - 0.5 * psi * psi);
@@ -189,7 +189,7 @@ template <class T> void phys_flux(const Cfg& c, const T* u, int o, T* f) {
         T rho = u[0];
         T v[3] = {u[1] / rho, u[2] / rho, u[3] / rho};
         T B[3] = {u[5], u[6], u[7]};
-        T psi = u[8];
+        T psi = (c.nv > 8) ? u[8] : T(0.0);
         T kin_en = 0.5 * rho * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
         T mag_en = 0.5 * (B[0] * B[0] + B[1] * B[1] + B[2] * B[2]);
         T p_over_gm1 = u[4] - kin_en - mag_en - 0.5 * psi * psi;
@@ -210,7 +210,7 @@ template <class T> void phys_flux(const Cfg& c, const T* u, int o, T* f) {
 template <class T> T mhd_fast_wavespeed(const Cfg& c, const T* u, int o) {
     T rho = u[0];
     T v1 = u[1] / rho, v2 = u[2] / rho, v3 = u[3] / rho;
-    T B1 = u[5], B2 = u[6], B3 = u[7], psi = u[8];
+    T B1 = u[5], B2 = u[6], B3 = u[7], psi = (c.nv > 8) ? u[8] : T(0.0);
     T kin_en = 0.5 * rho * (v1 * v1 + v2 * v2 + v3 * v3);
     T mag_en = 0.5 * (B1 * B1 + B2 * B2 + B3 * B3);
     T p = (c.gamma - 1.0) * (u[4] - kin_en - mag_en - 0.5 * psi * psi);
@@ -385,7 +385,8 @@ template <class T> void two_point(const Cfg& c, int kind, const T* ull, const T*
     }
 }
 
-inline bool has_noncons(const Cfg& c) { return c.eq == OR_EQ_GLM_MHD; }
+// (1D ideal MHD has no nonconservative terms: B1 is constant)
+inline bool has_noncons(const Cfg& c) { return c.eq == OR_EQ_GLM_MHD && c.nd > 1; }
 
 // ----------------------------------------------------------------------------
 // Basis (Trixi.jl basis_lobatto_legendre.jl; used by the reference at
@@ -1113,7 +1114,7 @@ void mhd_prim2cons(const Cfg& c, const double* prim, double* u) {
     u[4] = p * c.inv_gm1 + 0.5 * (u[1] * v[0] + u[2] * v[1] + u[3] * v[2]) +
            0.5 * (B[0] * B[0] + B[1] * B[1] + B[2] * B[2]) + 0.5 * psi * psi;
     for (int d = 0; d < 3; ++d) u[5 + d] = B[d];
-    u[8] = psi;
+    if (c.nv > 8) u[8] = psi;
 }
 
 // Seeded smooth periodic field: sum over 3 modes of amp*trig(2 pi k.x/L + phase),
@@ -1225,7 +1226,13 @@ void initial_condition(const Cfg& c, int ic, const double* x, double t, uint64_t
     }
     // GLM-MHD
     double prim[9] = {1.0, 0, 0, 0, 1.0, 0, 0, 0, 0};
-    if (ic == OR_IC_CONVERGENCE_TEST && c.nd == 2) {
+    if (ic == OR_IC_CONVERGENCE_TEST && c.nd == 1) {
+        // Alfven wave of Trixi's IdealGlmMhdEquations1D convergence test:
+        // rho = 1, v1 = 0, p = 0.1, B1 = 1, v_perp = B_perp = 0.1 (sin, cos)(2 pi (x + t))
+        const double arg = 2.0 * M_PI * (x[0] + t);
+        const double pr[9] = {1.0, 0.0, 0.1 * sin(arg), 0.1 * cos(arg), 0.1, 1.0, 0.1 * sin(arg), 0.1 * cos(arg), 0.0};
+        memcpy(prim, pr, sizeof(pr));
+    } else if (ic == OR_IC_CONVERGENCE_TEST && c.nd == 2) {
         // Alfven wave of Trixi's IdealGlmMhdEquations2D convergence test
         // (Derigs et al. 2016): rho = 1, p = 0.1, along x_perp = (x + y)/sqrt 2
         const double alpha = 0.25 * M_PI;
@@ -1282,7 +1289,7 @@ extern "C" {
 int oracle_nvars(const oracle_cfg* cfg) {
     if (cfg->equation == OR_EQ_LINADV || cfg->equation == OR_EQ_BURGERS) return 1;
     if (cfg->equation == OR_EQ_EULER) return cfg->ndims + 2;
-    return 9;
+    return cfg->ndims == 1 ? 8 : 9;  // ideal MHD: (rho, rho v, rho e, B) in 1D, + psi in 2D/3D
 }
 
 int oracle_basis(int N, double* nodes, double* weights, double* D, double* Dsplit, double* Dhat,
@@ -1412,7 +1419,7 @@ void oracle_cons2entropy(const oracle_cfg* cfg, const double* u, double* w) {
     w[nvel + 1] = -rho_p;
     if (c.eq == OR_EQ_GLM_MHD) {
         for (int d = 0; d < 3; ++d) w[5 + d] = rho_p * B[d];
-        w[8] = rho_p * psi;
+        if (c.nv > 8) w[8] = rho_p * psi;
     }
 }
 

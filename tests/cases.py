@@ -75,7 +75,7 @@ def _flux(th, name, mhd):
 
 
 def product_semi(th, c: Case, precision=None):
-    mhd = c.equation == "mhd"
+    mhd = c.equation == "mhd" and c.ndims > 1  # (flux, flux_nonconservative_powell) pairs
     if c.equation == "linadv":
         eq = [None, th.LinearScalarAdvectionEquation1D, th.LinearScalarAdvectionEquation2D,
               th.LinearScalarAdvectionEquation3D][c.ndims](c.advection[:c.ndims])
@@ -85,7 +85,8 @@ def product_semi(th, c: Case, precision=None):
         eq = [None, th.CompressibleEulerEquations1D, th.CompressibleEulerEquations2D,
               th.CompressibleEulerEquations3D][c.ndims](c.gamma)
     else:
-        eq = [None, None, th.IdealGlmMhdEquations2D, th.IdealGlmMhdEquations3D][c.ndims](c.gamma, c.c_h)
+        eq = [None, th.IdealGlmMhdEquations1D, th.IdealGlmMhdEquations2D, th.IdealGlmMhdEquations3D][c.ndims](c.gamma,
+                                                                                                        c.c_h)
     basis = th.LobattoLegendreBasisHIP(c.polydeg)
     if c.vi == "weak":
         vi = th.VolumeIntegralWeakForm()

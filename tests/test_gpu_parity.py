@@ -163,6 +163,15 @@ CASES = [
          indicator="first", alpha_max=1.0, alpha_min=0.0),
     Case("burgers1d_weak_inflow_outflow_src", 1, "burgers", 4, cmin=0.0, cmax=1.0, ic="convergence_test",
          periodic=False, bc_state="convergence_test", outflow=("x_pos",), source="convergence_test", t=0.2),
+    # 1D ideal MHD, eight variables (test/tree_dgsem_1d/mhd_alfven_wave.jl:
+    # flux_hindenlang_gassner volume, LLF surface, polydeg 4, level 2;
+    # mhd_ec.jl: flux_hindenlang_gassner throughout, polydeg 3)
+    Case("mhd1d_fd_hg_llf_alfven_p4", 1, "mhd", 2, "fd", "hindenlang_gassner", cmin=0.0, cmax=1.0,
+         ic="convergence_test", gamma=5 / 3, polydeg=4),
+    Case("mhd1d_fd_hg_hg_random", 1, "mhd", 4, "fd", "hindenlang_gassner", "hindenlang_gassner", cmin=0.0,
+         cmax=1.0, gamma=5 / 3),
+    Case("mhd1d_fd_hg_llf_random", 1, "mhd", 4, "fd", "hindenlang_gassner", cmin=0.0, cmax=1.0, gamma=5 / 3),
+    Case("mhd1d_weak_llf_random", 1, "mhd", 4, cmin=0.0, cmax=1.0, gamma=5 / 3),
 ]
 
 
@@ -222,7 +231,9 @@ def test_pack_unpack_roundtrip_bit_exact(th, oracle, torch):
                                   Case("mhd2d_alfven", 2, "mhd", 3, "fd", "central", cmin=0.0, cmax=2 ** 0.5,
                                        ic="convergence_test", gamma=5 / 3),
                                   Case("burgers1d", 1, "burgers", 4, cmin=0.0, cmax=1.0, ic="convergence_test",
-                                       source="convergence_test")],
+                                       source="convergence_test"),
+                                  Case("mhd1d_alfven", 1, "mhd", 3, "fd", "hindenlang_gassner", cmin=0.0, cmax=1.0,
+                                       ic="convergence_test", gamma=5 / 3)],
                          ids=lambda c: c.name)
 def test_initial_condition_and_callbacks_match_oracle(th, oracle, torch, case):
     O = oracle
@@ -293,7 +304,9 @@ def test_indicator_matches_oracle(th, oracle, torch, case):
                                        cmax=1.0, ic="shock", periodic=False, bc_state="shock", outflow=("x_pos",),
                                        indicator="first", alpha_max=1.0),
                                   Case("burgers1d_src", 1, "burgers", 4, cmin=0.0, cmax=1.0, ic="convergence_test",
-                                       source="convergence_test")],
+                                       source="convergence_test"),
+                                  Case("mhd1d_alfven", 1, "mhd", 3, "fd", "hindenlang_gassner", cmin=0.0, cmax=1.0,
+                                       ic="convergence_test", gamma=5 / 3)],
                          ids=lambda c: c.name)
 def test_rk_step_matches_oracle(th, oracle, torch, case, use_graph):
     """Three CK2N54 steps from t = 0 (stage times through device memory: the

@@ -519,19 +519,20 @@ def test_hg_on_mortar_meshes_is_conservative(oracle):
         assert np.abs(O.integrate(cfg, mesh, du)).max() < 1e-13 * np.abs(du).max() * mesh.total_volume
 
 
-@pytest.mark.parametrize("nd", [2, 3])
+@pytest.mark.parametrize("nd", [1, 2, 3])
 def test_mhd_alfven_wave_is_an_exact_solution(oracle, nd):
-    """The GLM-MHD convergence-test states (2D: Derigs et al.'s Alfven wave
-    along the diagonal of [0, sqrt 2]^2; 3D: Altmann's wave on [-1, 1]^3)
+    """The MHD convergence-test states (1D: circularly polarised Alfven wave
+    on [0, 1], no cleaning field; 2D: Derigs et al.'s Alfven wave along the
+    diagonal of [0, sqrt 2]^2; 3D: Altmann's wave on [-1, 1]^3)
     solve the equations with the Powell terms: rhs!(u_exact(t)) - u_t drops
     at the DG rate.  A wrong sign of t in the wave or a flux term that breaks
     the 2D reduction leaves an O(1) residual."""
     O = oracle
-    lo, hi = (0.0, np.sqrt(2.0)) if nd == 2 else (-1.0, 1.0)
+    lo, hi = {1: (0.0, 1.0), 2: (0.0, np.sqrt(2.0)), 3: (-1.0, 1.0)}[nd]
     cfg = O.make_cfg(nd, O.EQ_GLM_MHD, 3, O.VI_FD, O.FLUX_HINDENLANG_GASSNER, O.FLUX_LLF, gamma=5 / 3, c_h=1.0)
     errs = []
     t, h = 0.1, 1e-6
-    for lvl in ((3, 4) if nd == 2 else (2, 3)):
+    for lvl in ((3, 4) if nd < 3 else (2, 3)):
         mesh = O.Mesh(nd, lvl, lo, hi, 3)
         u = O.compute_coefficients(cfg, mesh, O.IC_CONVERGENCE_TEST, t)
         ut = (O.compute_coefficients(cfg, mesh, O.IC_CONVERGENCE_TEST, t + h)
@@ -613,3 +614,22 @@ def test_burgers_entropy_conservation_and_outflow(oracle):
     du = O.rhs(cfg, mesh, u)
     expect = 0.5 * u[0, 0, 0] ** 2 - 0.5 * u[-1, -1, 0] ** 2
     assert O.integrate(cfg, mesh, du)[0] == pytest.approx(expect, rel=1e-12)
+
+
+def test_mhd1d_entropy_conservation(oracle):
+    """1D ideal MHD (eight variables, no Powell terms): with B1 constant the
+    EC flux conserves entropy and LLF dissipates it; constant states stay."""
+    O = oracle
+    mesh = O.Mesh(1, 4, 0.0, 1.0, 3)
+    cfg = O.make_cfg(1, O.EQ_GLM_MHD, 3, O.VI_FD, O.FLUX_HINDENLANG_GASSNER, O.FLUX_HINDENLANG_GASSNER, gamma=5 / 3)
+    assert O.nvars(cfg) == 8
+    u = O.compute_coefficients(cfg, mesh, O.IC_SEEDED_RANDOM)
+    u[..., 4] += 0.5 * (1.0 - u[..., 5] ** 2)  # B1 = 1 at the same pressure
+    u[..., 5] = 1.0
+    du = O.rhs(cfg, mesh, u)
+    assert abs(entropy_rate(O, cfg, mesh, u, du)) < 1e-12 * np.abs(du).max()
+    cfg = O.make_cfg(1, O.EQ_GLM_MHD, 3, O.VI_FD, O.FLUX_HINDENLANG_GASSNER, O.FLUX_LLF, gamma=5 / 3)
+    du = O.rhs(cfg, mesh, u)
+    assert entropy_rate(O, cfg, mesh, u, du) < -1e-6
+    u = O.compute_coefficients(cfg, mesh, O.IC_CONSTANT)
+    assert np.abs(O.rhs(cfg, mesh, u)).max() < 1e-13

@@ -139,7 +139,18 @@ __device__ inline void initial_condition(const IcParams& q, const double* x, dou
         return;
     }
     double prim[9] = {1.0, 0, 0, 0, 1.0, 0, 0, 0, 0};
-    if (q.ic == TRIXI_HIP_IC_CONVERGENCE_TEST && q.dim == 2) {
+    if (q.ic == TRIXI_HIP_IC_CONVERGENCE_TEST && q.dim == 1) {
+        // circularly polarised Alfven wave of IdealGlmMhdEquations1D on [0, 1]:
+        // rho = 1, p = 0.1, B1 = 1, (v2, v3) = (B2, B3) = 0.1 (sin, cos)(2 pi (x + t))
+        const double arg = 2.0 * M_PI * (x[0] + t);
+        prim[1] = 0.0;
+        prim[2] = 0.1 * sin(arg);
+        prim[3] = 0.1 * cos(arg);
+        prim[4] = 0.1;
+        prim[5] = 1.0;
+        prim[6] = prim[2];
+        prim[7] = prim[3];
+    } else if (q.ic == TRIXI_HIP_IC_CONVERGENCE_TEST && q.dim == 2) {
         // Alfven wave of IdealGlmMhdEquations2D (Derigs et al. 2016): along
         // x_perp = x cos(pi/4) + y sin(pi/4), B_perp = 0.1 sin(2 pi (x_perp + t))
         const double ca = cos(0.25 * M_PI), sa = sin(0.25 * M_PI);
@@ -185,7 +196,7 @@ __device__ inline void initial_condition(const IcParams& q, const double* x, dou
     u[4] = p * q.inv_gm1 + 0.5 * (u[1] * prim[1] + u[2] * prim[2] + u[3] * prim[3]) +
            0.5 * (prim[5] * prim[5] + prim[6] * prim[6] + prim[7] * prim[7]) + 0.5 * psi * psi;
     for (int d = 0; d < 3; ++d) u[5 + d] = prim[5 + d];
-    u[8] = psi;
+    if (q.nv > 8) u[8] = psi;  // 1D ideal MHD has no cleaning field
 }
 
 

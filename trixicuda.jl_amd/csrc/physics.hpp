@@ -604,4 +604,61 @@ struct GlmMhd {
     }
 };
 
+// ===========================================================================
+// Ideal MHD in 1D (Trixi IdealGlmMhdEquations1D): (rho, rho v1..3, rho e,
+// B1..3), no cleaning field and no nonconservative terms (B1 is constant).
+// The GLM-MHD fluxes with psi = 0 and c_h = 0 (the host sets c_h = 0 for 1D)
+// are its fluxes, so this wraps them with the cleaning field held at zero.
+// ===========================================================================
+struct GlmMhd1D {
+    static constexpr int NV = 8;
+    static constexpr bool NONCONS = false;
+    static constexpr int NRF = GlmMhd::NRF;
+    template <typename R>
+    using Node = GlmMhd::Node<R>;
+    template <typename R, bool LOGS>
+    __device__ static Node<R> node(const R* u, const PhysR<R>& P) {
+        R u9[9];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) u9[q] = u[q];
+        u9[8] = R(0);
+        return GlmMhd::template node<R, LOGS>(u9, P);
+    }
+    template <typename R>
+    __device__ static void to_rec(const Node<R>& n, R* f) {
+        GlmMhd::to_rec(n, f);
+    }
+    template <typename R>
+    __device__ static Node<R> from_rec(const R* f, double lrho, double lp, const PhysR<R>& P) {
+        return GlmMhd::from_rec(f, lrho, lp, P);
+    }
+    template <typename R>
+    __device__ static void flux(const Node<R>& n, int o, const PhysR<R>& P, R* f) {
+        R f9[9];
+        GlmMhd::flux(n, o, P, f9);
+#pragma unroll
+        for (int q = 0; q < 8; ++q) f[q] = f9[q];
+    }
+    template <int KIND, typename R>
+    __device__ static void two_point(const Node<R>& l, const Node<R>& r, int o, const PhysR<R>& P, R* f) {
+        R f9[9];
+        GlmMhd::template two_point<KIND>(l, r, o, P, f9);
+#pragma unroll
+        for (int q = 0; q < 8; ++q) f[q] = f9[q];
+    }
+    template <typename R>
+    __device__ static void noncons(const Node<R>&, const Node<R>&, int, R* f) {
+#pragma unroll
+        for (int q = 0; q < 8; ++q) f[q] = R(0);
+    }
+    template <typename R>
+    __device__ static void max_abs_speeds(const R* u, const PhysR<R>& P, R* lam) {
+        R u9[9];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) u9[q] = u[q];
+        u9[8] = R(0);
+        GlmMhd::max_abs_speeds(u9, P, lam);
+    }
+};
+
 }  // namespace trixi_hip

@@ -59,6 +59,7 @@ void register_euler3d_hg();
 void register_mhd3d();
 void register_mhd2d();
 void register_burgers();
+void register_mhd1d();
 void register_vjp_linadv();
 void register_vjp_euler1d();
 void register_vjp_euler2d();
@@ -81,6 +82,7 @@ static void ensure_registered() {
         register_mhd3d();
         register_mhd2d();
         register_burgers();
+        register_mhd1d();
         register_vjp_linadv();
         register_vjp_euler1d();
         register_vjp_euler2d();
@@ -745,7 +747,8 @@ __global__ void check_state_kernel(const double* __restrict__ u, int32_t* __rest
             const int nvel = (eq == EQ_EULER) ? dim : 3;
             for (int d = 0; d < nvel; ++d) ke += uc[1 + d] * uc[1 + d] / uc[0];
             double p = uc[nvel + 1] - 0.5 * ke;
-            if (eq == EQ_GLM_MHD) p -= 0.5 * (uc[5] * uc[5] + uc[6] * uc[6] + uc[7] * uc[7] + uc[8] * uc[8]);
+            if (eq == EQ_GLM_MHD)
+                p -= 0.5 * (uc[5] * uc[5] + uc[6] * uc[6] + uc[7] * uc[7] + (nv > 8 ? uc[8] * uc[8] : 0.0));
             p *= phys.gm1;
             if (!(p > 0.0)) f |= 4;
         }
@@ -832,7 +835,7 @@ static const double CK_C[5] = {0.0, 1432997174477.0 / 9575080441755.0, 252626934
 static int nvars_of(const trixi_hip_config& c) {
     if (c.equation == TRIXI_HIP_LINEAR_SCALAR_ADVECTION || c.equation == TRIXI_HIP_INVISCID_BURGERS) return 1;
     if (c.equation == TRIXI_HIP_COMPRESSIBLE_EULER) return c.ndims + 2;
-    return 9;
+    return c.ndims == 1 ? 8 : 9;  // ideal MHD: no cleaning field in 1D
 }
 
 static KernelKey make_key(const trixi_hip_config& c, int rk, bool mortars = false) {
@@ -993,8 +996,6 @@ int trixi_hip_create(const trixi_hip_config* config, const trixi_hip_mesh* mesh,
     const trixi_hip_config& cfg = *config;
     if (cfg.ndims < 1 || cfg.ndims > 3) return fail(TRIXI_HIP_EINVAL, "ndims must be 1, 2 or 3");
     if (cfg.polydeg < 1 || cfg.polydeg + 1 > MAXN) return fail(TRIXI_HIP_EINVAL, "polydeg out of range");
-    if (cfg.equation == TRIXI_HIP_IDEAL_GLM_MHD && cfg.ndims == 1)
-        return fail(TRIXI_HIP_EUNSUPPORTED, "GLM-MHD is built for 2D and 3D");
     if (cfg.equation == TRIXI_HIP_INVISCID_BURGERS && cfg.ndims != 1)
         return fail(TRIXI_HIP_EUNSUPPORTED, "the inviscid Burgers equation is built for 1D");
     if (cfg.volume_integral == TRIXI_HIP_VOLUME_SHOCK_CAPTURING_HG && cfg.equation != TRIXI_HIP_COMPRESSIBLE_EULER &&
@@ -1200,7 +1201,7 @@ int trixi_hip_create(const trixi_hip_config* config, const trixi_hip_mesh* mesh,
     a.phys.gamma = cfg.gamma;
     a.phys.gm1 = cfg.gamma - 1.0;
     a.phys.inv_gm1 = (cfg.gamma != 1.0) ? 1.0 / (cfg.gamma - 1.0) : 0.0;
-    a.phys.c_h = cfg.c_h;
+    a.phys.c_h = (cfg.equation == TRIXI_HIP_IDEAL_GLM_MHD && dim == 1) ? 0.0 : cfg.c_h;
     for (int d = 0; d < 3; ++d) a.phys.a[d] = cfg.advection_velocity[d];
     a.bc_ic = ic_params(c, cfg.boundary_ic, cfg.boundary_seed);
     for (int q = 0; q < N * N; ++q) {
@@ -1450,7 +1451,7 @@ int trixi_hip_max_dt(trixi_hip_ctx* ctx, const double* u, double* max_dt, trixi_
     } else if (eq == TRIXI_HIP_INVISCID_BURGERS) {
         MS(Burgers, 1);
     } else {
-        if (dim == 2) MS(GlmMhd, 2); else MS(GlmMhd, 3);
+        if (dim == 1) MS(GlmMhd1D, 1); else if (dim == 2) MS(GlmMhd, 2); else MS(GlmMhd, 3);
     }
 #undef MS
     HIP_TRY(hipGetLastError());

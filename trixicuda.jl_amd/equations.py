@@ -25,7 +25,7 @@ class AbstractEquations:
             return 1
         if self.equation_id == L.EQ_EULER:
             return self.ndims + 2
-        return 9
+        return 9 if self.ndims > 1 else 8  # ideal MHD: no cleaning field in 1D
 
 
 def _linadv(ndims):
@@ -86,6 +86,7 @@ def _glm_mhd(ndims):
     return _GlmMhd
 
 
+IdealGlmMhdEquations1D = _glm_mhd(1)  # (rho, rho v1..3, rho e, B1..3); no psi, no Powell terms
 IdealGlmMhdEquations2D = _glm_mhd(2)
 IdealGlmMhdEquations3D = _glm_mhd(3)
 

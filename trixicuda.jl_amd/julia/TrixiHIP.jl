@@ -73,6 +73,7 @@ equation_id(::CompressibleEulerEquations2D) = 1
 equation_id(::CompressibleEulerEquations3D) = 1
 equation_id(::IdealGlmMhdEquations3D) = 2
 equation_id(::IdealGlmMhdEquations2D) = 2
+equation_id(::IdealGlmMhdEquations1D) = 2
 equation_id(::InviscidBurgersEquation1D) = 3
 equation_id(eq) = throw(ArgumentError("equation $(typeof(eq)) is not built for HIP"))
 

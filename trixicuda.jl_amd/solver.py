@@ -100,10 +100,10 @@ class DGSEMHIP:
         c.volume_integral = vi.kind
         sflux, s_nc = split_flux(self.surface_flux)
         c.surface_flux = sflux.flux_id
-        nc_needed = equations.equation_id == L.EQ_GLM_MHD
+        nc_needed = equations.equation_id == L.EQ_GLM_MHD and equations.ndims > 1
         if nc_needed != (s_nc is not None):
-            raise NotImplementedError("GLM-MHD needs (flux, flux_nonconservative_powell) tuples, "
-                                      "other equations plain fluxes")
+            raise NotImplementedError("GLM-MHD in 2D/3D needs (flux, flux_nonconservative_powell) tuples, "
+                                      "other equations (and 1D MHD) plain fluxes")
         c.volume_flux = L.FLUX_CENTRAL
         c.fv_flux = L.FLUX_CENTRAL
         c.alpha_smooth = 0
