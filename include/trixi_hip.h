@@ -65,7 +65,9 @@ enum trixi_hip_flux {
     TRIXI_HIP_FLUX_RANOCHA = 3,
     TRIXI_HIP_FLUX_SHIMA_ETAL = 4,
     TRIXI_HIP_FLUX_HINDENLANG_GASSNER = 5,   /* with flux_nonconservative_powell       */
-    TRIXI_HIP_FLUX_EC = 6                    /* flux_ec (InviscidBurgersEquation1D)    */
+    TRIXI_HIP_FLUX_EC = 6,                   /* flux_ec (InviscidBurgersEquation1D)    */
+    TRIXI_HIP_FLUX_HLL = 7                   /* flux_hll = FluxHLL(min_max_speed_davis),
+                                                compressible Euler                     */
 };
 enum trixi_hip_precision {
     TRIXI_HIP_FP64 = 0,  /* all arithmetic in FP64                                    */
@@ -83,8 +85,10 @@ enum trixi_hip_initial_condition {
                                                on rho, v, p per node               */
     TRIXI_HIP_IC_SHOCK = 6,            /* Burgers: 1.5 for x < 0.5, else 0.5
                                           (test/tree_dgsem_1d/burgers_shock.jl)    */
-    TRIXI_HIP_IC_RAREFACTION = 7       /* Burgers: 0.5 for x < 0.5, else 1.5
+    TRIXI_HIP_IC_RAREFACTION = 7,      /* Burgers: 0.5 for x < 0.5, else 1.5
                                           (test/tree_dgsem_1d/burgers_rarefraction.jl) */
+    TRIXI_HIP_IC_EOC_TEST_COUPLED_EULER_GRAVITY = 8 /* Euler 2D/3D: rho = 2 + 0.1 sin(pi (sum x - t)),
+                                          v = 1, p = 2 rho^2 / (d pi) (gamma = 2) */
 };
 
 /* Boundary condition on the faces of a non-periodic TreeMesh that have no
@@ -102,7 +106,8 @@ enum trixi_hip_boundary_condition {
  * dg_2d.jl:746-757; source_terms_kernel!). */
 enum trixi_hip_source_terms {
     TRIXI_HIP_SOURCE_NONE = 0,
-    TRIXI_HIP_SOURCE_CONVERGENCE_TEST = 1 /* source_terms_convergence_test (Euler)  */
+    TRIXI_HIP_SOURCE_CONVERGENCE_TEST = 1, /* source_terms_convergence_test (Euler, Burgers) */
+    TRIXI_HIP_SOURCE_EOC_TEST_EULER = 2    /* source_terms_eoc_test_euler (Euler 2D/3D)     */
 };
 
 /* IndicatorHennemannGassner(...; variable) (Trixi's density_pressure,

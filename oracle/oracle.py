@@ -19,11 +19,11 @@ _LIB_PATH = os.path.join(_HERE, "liboracle.so")
 
 EQ_LINADV, EQ_EULER, EQ_GLM_MHD, EQ_BURGERS = 0, 1, 2, 3
 VI_WEAK, VI_FD, VI_HG = 0, 1, 2
-FLUX_CENTRAL, FLUX_LLF, FLUX_LLF_NAIVE, FLUX_RANOCHA, FLUX_SHIMA_ETAL, FLUX_HINDENLANG_GASSNER, FLUX_EC = range(7)
+FLUX_CENTRAL, FLUX_LLF, FLUX_LLF_NAIVE, FLUX_RANOCHA, FLUX_SHIMA_ETAL, FLUX_HINDENLANG_GASSNER, FLUX_EC, FLUX_HLL = range(8)
 (IC_CONSTANT, IC_CONVERGENCE_TEST, IC_WEAK_BLAST_WAVE, IC_SEEDED_RANDOM, IC_SMOOTH_PERIODIC, IC_WEAK_BLAST_WAVE_SEEDED,
- IC_SHOCK, IC_RAREFACTION) = range(8)
+ IC_SHOCK, IC_RAREFACTION, IC_EOC_GRAVITY) = range(9)
 BC_PERIODIC, BC_DIRICHLET = 0, 1
-SOURCE_NONE, SOURCE_CONVERGENCE_TEST = 0, 1
+SOURCE_NONE, SOURCE_CONVERGENCE_TEST, SOURCE_EOC_TEST_EULER = 0, 1, 2
 IND_DENSITY_PRESSURE, IND_PRESSURE, IND_DENSITY = 0, 1, 2
 
 

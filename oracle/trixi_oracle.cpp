@@ -361,6 +361,26 @@ template <class T> void two_point(const Cfg& c, int kind, const T* ull, const T*
         case OR_FLUX_HINDENLANG_GASSNER:
             flux_hindenlang_gassner_mhd(c, ull, urr, o, f);
             return;
+        case OR_FLUX_HLL: {
+            // Trixi FluxHLL with min_max_speed_davis (the flux_hll default since
+            // Trixi 0.7): lambda_min = min(v_ll - c_ll, v_rr - c_rr),
+            // lambda_max = max(v_ll + c_ll, v_rr + c_rr)
+            T fl[MAXV], fr[MAXV], vl, cl, vr, cr;
+            phys_flux(c, ull, o, fl);
+            phys_flux(c, urr, o, fr);
+            normal_speed(c, ull, o, vl, cl);
+            normal_speed(c, urr, o, vr, cr);
+            const T lmin = tmin(vl - cl, vr - cr), lmax = tmax(vl + cl, vr + cr);
+            if (val(lmin) >= 0.0 && val(lmax) >= 0.0) {
+                for (int v = 0; v < c.nv; ++v) f[v] = fl[v];
+            } else if (val(lmax) <= 0.0 && val(lmin) <= 0.0) {
+                for (int v = 0; v < c.nv; ++v) f[v] = fr[v];
+            } else {
+                for (int v = 0; v < c.nv; ++v)
+                    f[v] = (lmax * fl[v] - lmin * fr[v] + lmin * lmax * (urr[v] - ull[v])) / (lmax - lmin);
+            }
+            return;
+        }
         case OR_FLUX_EC:  // Trixi flux_ec(u_ll, u_rr, orientation, ::InviscidBurgersEquation1D)
             f[0] = (ull[0] * ull[0] + ull[0] * urr[0] + urr[0] * urr[0]) / 6.0;
             return;
@@ -824,6 +844,21 @@ void initial_condition(const Cfg& c, int ic, const double* x, double t, uint64_t
 //   S_{rho e}  = d (2 g g' + p') - 2 g g'                 (energy)
 // with p' = (gamma - 1)(2 g - d/2) g'.
 void source_convergence_test(const Cfg& c, const double* x, double t, double* sv) {
+    if (c.src == OR_SOURCE_EOC_TEST_EULER) {
+        // source_terms_eoc_test_euler (Trixi, d = 2, 3) in its own form:
+        // rho_x = 0.1 pi cos(pi (sum x - t)), C_grav = -4 G / (d pi) (G = 1),
+        // du_rho = (d-1) rho_x, du_{rho v} = rho_x ((d-1) - C_grav rho),
+        // du_E = rho_x (d (d-1)/2 - ((d-1)/(gamma-1) + d) C_grav rho)
+        const int d = c.nd;
+        double s = -t;
+        for (int q = 0; q < d; ++q) s += x[q];
+        const double rhox = 0.1 * M_PI * cos(M_PI * s), rho = 2.0 + 0.1 * sin(M_PI * s);
+        const double C_grav = -4.0 / (d * M_PI);
+        sv[0] = (d - 1) * rhox;
+        for (int q = 0; q < d; ++q) sv[1 + q] = rhox * ((d - 1) - C_grav * rho);
+        sv[d + 1] = rhox * (0.5 * d * (d - 1) - ((d - 1) * c.inv_gm1 + d) * C_grav * rho);
+        return;
+    }
     if (c.eq == OR_EQ_BURGERS) {
         // u = 2 + sin(2 pi (x - t)): u_t + u u_x = 2 pi cos(.) (u - 1)
         const double w = 2.0 * M_PI, arg = w * (x[0] - t);
@@ -1064,7 +1099,7 @@ void rhs_impl(const Cfg& c, int64_t E, int64_t I, const int32_t* nbr, const int3
 
     // calc_sources! (after the Jacobian, as in Trixi and the reference
     // cuda_sources!, dg_2d.jl:746-757)
-    if (c.src == OR_SOURCE_CONVERGENCE_TEST) {
+    if (c.src == OR_SOURCE_CONVERGENCE_TEST || c.src == OR_SOURCE_EOC_TEST_EULER) {
 #pragma omp parallel for schedule(static)
         for (int64_t e = 0; e < E; ++e)
             for (int n = 0; n < nn; ++n) {
@@ -1172,6 +1207,18 @@ void initial_condition(const Cfg& c, int ic, const double* x, double t, uint64_t
     }
     if (c.eq == OR_EQ_EULER) {
         double v[3] = {0, 0, 0}, rho = 1.0, p = 1.0;
+        if (ic == OR_IC_EOC_GRAVITY) {
+            // initial_condition_eoc_test_coupled_euler_gravity: c = 2, A = 0.1,
+            // G = 1; rho = c + A sin(pi (sum x - t)), v = 1, p = rho^2 G 2/(d pi)
+            double s = -t;
+            for (int d = 0; d < c.nd; ++d) s += x[d];
+            const double ini = 2.0 + 0.1 * sin(M_PI * s);
+            rho = ini;
+            for (int d = 0; d < c.nd; ++d) v[d] = 1.0;
+            p = ini * ini * 1.0 * 2.0 / (c.nd * M_PI);
+            euler_prim2cons(c, rho, v, p, u);
+            return;
+        }
         if (ic == OR_IC_WEAK_BLAST_WAVE || ic == OR_IC_WEAK_BLAST_WAVE_SEEDED) {
             double r2 = 0.0;
             for (int d = 0; d < c.nd; ++d) r2 += x[d] * x[d];

@@ -34,7 +34,8 @@ enum {
     OR_FLUX_RANOCHA = 3,
     OR_FLUX_SHIMA_ETAL = 4,
     OR_FLUX_HINDENLANG_GASSNER = 5,
-    OR_FLUX_EC = 6            /* Burgers flux_ec                             */
+    OR_FLUX_EC = 6,           /* Burgers flux_ec                             */
+    OR_FLUX_HLL = 7           /* FluxHLL(min_max_speed_davis), Euler         */
 };
 enum {
     OR_IC_CONSTANT = 0,
@@ -44,10 +45,11 @@ enum {
     OR_IC_SMOOTH_PERIODIC = 4,  /* Euler: seeded sums of periodic sines           */
     OR_IC_WEAK_BLAST_WAVE_SEEDED = 5, /* Euler: weak blast + U(-0.01,0.01) on rho,v,p */
     OR_IC_SHOCK = 6,          /* Burgers: 1.5 for x < 0.5, else 0.5 */
-    OR_IC_RAREFACTION = 7     /* Burgers: 0.5 for x < 0.5, else 1.5 */
+    OR_IC_RAREFACTION = 7,    /* Burgers: 0.5 for x < 0.5, else 1.5 */
+    OR_IC_EOC_GRAVITY = 8     /* Euler: initial_condition_eoc_test_coupled_euler_gravity */
 };
 enum { OR_BC_PERIODIC = 0, OR_BC_DIRICHLET = 1 };
-enum { OR_SOURCE_NONE = 0, OR_SOURCE_CONVERGENCE_TEST = 1 };
+enum { OR_SOURCE_NONE = 0, OR_SOURCE_CONVERGENCE_TEST = 1, OR_SOURCE_EOC_TEST_EULER = 2 };
 
 typedef struct oracle_cfg {
     int32_t ndims;            /* 1, 2, 3 */

@@ -36,9 +36,10 @@ class Case:
     outflow: tuple = ()     # boundary sides ("x_neg", ...) with the outflow condition
 
 
-FLUX_NAMES = ("central", "llf", "llf_naive", "ranocha", "shima_etal", "hindenlang_gassner", "ec")
+FLUX_NAMES = ("central", "llf", "llf_naive", "ranocha", "shima_etal", "hindenlang_gassner", "ec", "hll")
 IC_NAMES = ("constant", "convergence_test", "weak_blast_wave", "seeded_random", "smooth_periodic",
-            "weak_blast_wave_seeded", "shock", "rarefaction")
+            "weak_blast_wave_seeded", "shock", "rarefaction", "eoc_gravity")
+SOURCES = {"": 0, "convergence_test": 1, "eoc_test_euler": 2}
 SIDES = ("x_neg", "x_pos", "y_neg", "y_pos", "z_neg", "z_pos")
 INDICATORS = {"density_pressure": 0, "pressure": 1, "density": 2, "first": 2}
 
@@ -54,7 +55,7 @@ def oracle_cfg(O, c: Case):
     vi = {"weak": O.VI_WEAK, "fd": O.VI_FD, "hg": O.VI_HG}[c.vi]
     ext = dict(boundary_condition=O.BC_DIRICHLET if c.bc_state else O.BC_PERIODIC,
                boundary_ic=IC_NAMES.index(c.bc_state) if c.bc_state else 0,
-               source_terms=O.SOURCE_CONVERGENCE_TEST if c.source == "convergence_test" else O.SOURCE_NONE,
+               source_terms=SOURCES[c.source],
                boundary_outflow_faces=sum(1 << SIDES.index(s) for s in c.outflow))
     return O.make_cfg(c.ndims, eq, c.polydeg, vi, FLUX_NAMES.index(c.vflux), FLUX_NAMES.index(c.sflux),
                       FLUX_NAMES.index(c.fvflux), gamma=c.gamma, c_h=c.c_h,
@@ -70,7 +71,8 @@ def oracle_ic(O, c: Case):
 def _flux(th, name, mhd):
     f = {"central": th.flux_central, "llf": th.flux_lax_friedrichs,
          "llf_naive": th.FluxLaxFriedrichs(th.max_abs_speed_naive), "ranocha": th.flux_ranocha,
-         "shima_etal": th.flux_shima_etal, "hindenlang_gassner": th.flux_hindenlang_gassner, "ec": th.flux_ec}[name]
+         "shima_etal": th.flux_shima_etal, "hindenlang_gassner": th.flux_hindenlang_gassner, "ec": th.flux_ec,
+         "hll": th.flux_hll}[name]
     return (f, th.flux_nonconservative_powell) if mhd else f
 
 
@@ -107,14 +109,16 @@ def product_semi(th, c: Case, precision=None):
            "seeded_random": th.initial_condition_seeded_random(c.seed),
            "smooth_periodic": th.initial_condition_smooth_periodic(c.seed),
            "weak_blast_wave_seeded": th.initial_condition_weak_blast_wave_seeded(c.seed),
-           "shock": th.initial_condition_shock, "rarefaction": th.initial_condition_rarefaction}
+           "shock": th.initial_condition_shock, "rarefaction": th.initial_condition_rarefaction,
+           "eoc_gravity": th.initial_condition_eoc_test_coupled_euler_gravity}
     kw = {}
     if c.bc_state:
         bc = th.BoundaryConditionDirichlet(ics[c.bc_state])
         kw["boundary_conditions"] = {s: (th.boundary_condition_outflow if s in c.outflow else bc)
                                      for s in SIDES[:2 * c.ndims]}
-    if c.source == "convergence_test":
-        kw["source_terms"] = th.source_terms_convergence_test
+    if c.source:
+        kw["source_terms"] = {"convergence_test": th.source_terms_convergence_test,
+                              "eoc_test_euler": th.source_terms_eoc_test_euler}[c.source]
     return th.SemidiscretizationHyperbolicHIP(mesh, eq, ics[c.ic], solver, **kw)
 
 

@@ -100,7 +100,9 @@ def test_vjp_equals_transposed_oracle_jacobian(th, oracle, torch, case):
                                   Case("mhd3d_fd_hg_llf_alfven", 3, "mhd", 2, "fd", "hindenlang_gassner",
                                        cmin=-1.0, cmax=1.0, ic="convergence_test", gamma=5 / 3),
                                   Case("euler3d_hg_ranocha_blast", 3, "euler", 2, "hg", "ranocha",
-                                       ic="weak_blast_wave_seeded")],
+                                       ic="weak_blast_wave_seeded"),
+                                  Case("euler3d_hg_ranocha_fv_ranocha", 3, "euler", 2, "hg", "ranocha", "ranocha",
+                                       "ranocha", ic="weak_blast_wave_seeded")],
                          ids=lambda c: c.name)
 def test_vjp_dot_product(th, oracle, torch, case):
     O = oracle

@@ -172,6 +172,18 @@ CASES = [
          cmax=1.0, gamma=5 / 3),
     Case("mhd1d_fd_hg_llf_random", 1, "mhd", 4, "fd", "hindenlang_gassner", cmin=0.0, cmax=1.0, gamma=5 / 3),
     Case("mhd1d_weak_llf_random", 1, "mhd", 4, cmin=0.0, cmax=1.0, gamma=5 / 3),
+    # FluxHLL (Davis) with the coupled Euler-gravity EOC state and
+    # source_terms_eoc_test_euler (test/tree_dgsem_3d/euler_convergence.jl:
+    # gamma 2, weak form, level 2 on [0, 2]^3)
+    Case("euler3d_weak_hll_eoc", 3, "euler", 2, sflux="hll", cmin=0.0, cmax=2.0, ic="eoc_gravity",
+         source="eoc_test_euler", gamma=2.0, t=0.3),
+    Case("euler2d_weak_hll_eoc", 2, "euler", 3, sflux="hll", cmin=0.0, cmax=2.0, ic="eoc_gravity",
+         source="eoc_test_euler", gamma=2.0, t=0.7),
+    Case("euler1d_weak_hll_random", 1, "euler", 4, sflux="hll"),
+    Case("euler2d_weak_hll_random", 2, "euler", 3, sflux="hll"),
+    # test/tree_dgsem_3d/euler_shock.jl: HG with flux_ranocha as DG, FV and surface flux
+    Case("euler3d_hg_ranocha_ranocha_blast", 3, "euler", 3, "hg", "ranocha", "ranocha", "ranocha",
+         ic="weak_blast_wave"),
 ]
 
 
@@ -190,7 +202,8 @@ MIXED_CASES = [c for c in CASES if c.vi in ("fd", "hg") and c.polydeg == 3] + \
      # weak form with N = 4 applies D-hat on the matrix cores in mixed precision
      Case("euler3d_weak_llf_mixed", 3, "euler", 2, ic="smooth_periodic"),
      Case("linadv3d_weak_mixed", 3, "linadv", 2, ic="seeded_random"),
-     Case("linadv2d_weak_mixed", 2, "linadv", 3, ic="seeded_random")] + \
+     Case("linadv2d_weak_mixed", 2, "linadv", 3, ic="seeded_random"),
+     Case("euler3d_weak_hll_mixed", 3, "euler", 2, sflux="hll", ic="smooth_periodic")] + \
     [c for c in CASES if c.patches and c.name != "linadv3d_mortar_central" and
      not (c.ic == "convergence_test" and c.source)]  # (cancellation at the manufactured state, as above)
 
@@ -306,7 +319,9 @@ def test_indicator_matches_oracle(th, oracle, torch, case):
                                   Case("burgers1d_src", 1, "burgers", 4, cmin=0.0, cmax=1.0, ic="convergence_test",
                                        source="convergence_test"),
                                   Case("mhd1d_alfven", 1, "mhd", 3, "fd", "hindenlang_gassner", cmin=0.0, cmax=1.0,
-                                       ic="convergence_test", gamma=5 / 3)],
+                                       ic="convergence_test", gamma=5 / 3),
+                                  Case("euler3d_weak_hll_eoc", 3, "euler", 2, sflux="hll", cmin=0.0, cmax=2.0,
+                                       ic="eoc_gravity", source="eoc_test_euler", gamma=2.0)],
                          ids=lambda c: c.name)
 def test_rk_step_matches_oracle(th, oracle, torch, case, use_graph):
     """Three CK2N54 steps from t = 0 (stage times through device memory: the

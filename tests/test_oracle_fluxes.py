@@ -203,3 +203,40 @@ def test_burgers_flux_ec_and_llf(oracle):
         expect = 0.25 * (ul[0] ** 2 + ur[0] ** 2) - 0.5 * lam * (ur[0] - ul[0])
         for kind in (O.FLUX_LLF, O.FLUX_LLF_NAIVE):
             assert O.two_point_flux(cfg, kind, ul, ur, 1)[0] == pytest.approx(expect, rel=1e-14, abs=1e-15)
+
+
+@pytest.mark.parametrize("nd", [1, 2, 3])
+def test_hll_flux_structure(oracle, nd):
+    """FluxHLL(min_max_speed_davis) for Euler: consistent; the left (right)
+    physical flux when every wave moves right (left); otherwise
+    (l_max f_l - l_min f_r + l_min l_max [[u]]) / (l_max - l_min) with the
+    Davis estimates min / max of v -+ c over both states; and its
+    dissipation never exceeds LLF's."""
+    O = oracle
+    cfg = O.make_cfg(nd, O.EQ_EULER, 3, gamma=1.4)
+    rng = np.random.default_rng(40 + nd)
+
+    def prim2cons(rho, v, p):
+        return np.concatenate([[rho], rho * v, [p / 0.4 + 0.5 * rho * v @ v]])
+
+    for shift in (0.0, 3.0, -3.0):  # subsonic, supersonic right, supersonic left
+        for _ in range(20):
+            rl, rr = rng.uniform(0.5, 2, 2)
+            pl, pr = rng.uniform(0.5, 2, 2)
+            vl, vr = rng.normal(0, 0.3, nd), rng.normal(0, 0.3, nd)
+            vl[0] += shift
+            vr[0] += shift
+            ul, ur = prim2cons(rl, vl, pl), prim2cons(rr, vr, pr)
+            f = O.two_point_flux(cfg, O.FLUX_HLL, ul, ur, 1)
+            fl, fr = O.flux(cfg, ul, 1), O.flux(cfg, ur, 1)
+            cl, cr = np.sqrt(1.4 * pl / rl), np.sqrt(1.4 * pr / rr)
+            lmin, lmax = min(vl[0] - cl, vr[0] - cr), max(vl[0] + cl, vr[0] + cr)
+            if lmin >= 0:
+                np.testing.assert_allclose(f, fl, rtol=1e-14, atol=1e-14)
+            elif lmax <= 0:
+                np.testing.assert_allclose(f, fr, rtol=1e-14, atol=1e-14)
+            else:
+                expect = (lmax * fl - lmin * fr + lmin * lmax * (ur - ul)) / (lmax - lmin)
+                np.testing.assert_allclose(f, expect, rtol=1e-12, atol=1e-13)
+        np.testing.assert_allclose(O.two_point_flux(cfg, O.FLUX_HLL, ul, ul, 1), O.flux(cfg, ul, 1), rtol=1e-14,
+                                   atol=1e-14)

@@ -633,3 +633,34 @@ def test_mhd1d_entropy_conservation(oracle):
     assert entropy_rate(O, cfg, mesh, u, du) < -1e-6
     u = O.compute_coefficients(cfg, mesh, O.IC_CONSTANT)
     assert np.abs(O.rhs(cfg, mesh, u)).max() < 1e-13
+
+
+@pytest.mark.parametrize("nd", [2, 3])
+def test_eoc_test_euler_source_is_manufactured(oracle, nd):
+    """source_terms_eoc_test_euler = u_t + div f(u) at the coupled
+    Euler-gravity EOC state (gamma = 2; central differences), and with it
+    rhs!(u_exact) - u_t drops at the DG rate on the weak form with flux_hll
+    (test/tree_dgsem_3d/euler_convergence.jl)."""
+    O = oracle
+    cfg = O.make_cfg(nd, O.EQ_EULER, 3, O.VI_WEAK, O.FLUX_CENTRAL, O.FLUX_HLL, gamma=2.0,
+                     source_terms=O.SOURCE_EOC_TEST_EULER)
+    rng = np.random.default_rng(60 + nd)
+    h = 1e-5
+    ic = lambda xx, tt: O.initial_condition(cfg, O.IC_EOC_GRAVITY, xx, tt)
+    for _ in range(6):
+        x, t = rng.uniform(0, 2, nd), rng.uniform(0, 1)
+        ut = (ic(x, t + h) - ic(x, t - h)) / (2 * h)
+        div = sum((O.flux(cfg, ic(x + h * np.eye(nd)[d], t), d + 1) - O.flux(cfg, ic(x - h * np.eye(nd)[d], t), d + 1))
+                  / (2 * h) for d in range(nd))
+        S = O.source_convergence_test(cfg, x, t)
+        assert np.abs(S - (ut + div)).max() < 1e-8 * max(1.0, np.abs(S).max())
+    errs = []
+    t, hh = 0.3, 1e-6
+    for lvl in (2, 3):
+        mesh = O.Mesh(nd, lvl, 0.0, 2.0, 3)
+        u = O.compute_coefficients(cfg, mesh, O.IC_EOC_GRAVITY, t)
+        ut = (O.compute_coefficients(cfg, mesh, O.IC_EOC_GRAVITY, t + hh)
+              - O.compute_coefficients(cfg, mesh, O.IC_EOC_GRAVITY, t - hh)) / (2 * hh)
+        errs.append(np.abs(O.rhs(cfg, mesh, u, t=t) - ut).max())
+    rate = np.log2(errs[0] / errs[1])
+    assert rate > 2.5, (errs, rate)

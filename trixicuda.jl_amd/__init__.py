@@ -19,7 +19,9 @@ from .equations import (BoundaryConditionDirichlet, boundary_condition_periodic,
                         CompressibleEulerEquations1D, CompressibleEulerEquations2D, CompressibleEulerEquations3D,
                         FluxLaxFriedrichs, IdealGlmMhdEquations1D, IdealGlmMhdEquations2D, IdealGlmMhdEquations3D, LinearScalarAdvectionEquation1D,
                         LinearScalarAdvectionEquation2D, LinearScalarAdvectionEquation3D, density, density_pressure, pressure,
-                        InviscidBurgersEquation1D, first, flux_ec, initial_condition_shock,
+                        InviscidBurgersEquation1D, first, flux_ec, initial_condition_shock, FluxHLL, flux_hll,
+                        min_max_speed_davis, initial_condition_eoc_test_coupled_euler_gravity,
+                        source_terms_eoc_test_euler,
                         initial_condition_rarefaction, boundary_condition_outflow,
                         flux_central, flux_hindenlang_gassner, flux_lax_friedrichs, flux_nonconservative_powell,
                         flux_ranocha, flux_shima_etal, initial_condition_constant,

@@ -13,12 +13,12 @@ OK, EINVAL, EUNSUPPORTED, EDEVICE, ENOMEM, ESTATE = 0, -1, -2, -3, -4, -5
 
 EQ_LINADV, EQ_EULER, EQ_GLM_MHD, EQ_BURGERS = 0, 1, 2, 3
 VI_WEAK, VI_FD, VI_HG = 0, 1, 2
-FLUX_CENTRAL, FLUX_LLF, FLUX_LLF_NAIVE, FLUX_RANOCHA, FLUX_SHIMA_ETAL, FLUX_HINDENLANG_GASSNER, FLUX_EC = range(7)
+FLUX_CENTRAL, FLUX_LLF, FLUX_LLF_NAIVE, FLUX_RANOCHA, FLUX_SHIMA_ETAL, FLUX_HINDENLANG_GASSNER, FLUX_EC, FLUX_HLL = range(8)
 PREC_FP64, PREC_MIXED = 0, 1
 BC_PERIODIC, BC_DIRICHLET = 0, 1
-SOURCE_NONE, SOURCE_CONVERGENCE_TEST = 0, 1
+SOURCE_NONE, SOURCE_CONVERGENCE_TEST, SOURCE_EOC_TEST_EULER = 0, 1, 2
 (IC_CONSTANT, IC_CONVERGENCE_TEST, IC_WEAK_BLAST_WAVE, IC_SEEDED_RANDOM, IC_SMOOTH_PERIODIC,
- IC_WEAK_BLAST_WAVE_SEEDED, IC_SHOCK, IC_RAREFACTION) = range(8)
+ IC_WEAK_BLAST_WAVE_SEEDED, IC_SHOCK, IC_RAREFACTION, IC_EOC_TEST_COUPLED_EULER_GRAVITY) = range(9)
 
 
 class TrixiHipError(RuntimeError):

@@ -111,6 +111,14 @@ __device__ inline void initial_condition(const IcParams& q, const double* x, dou
                 p += 0.01 * (2.0 * u01(q.seed, idx, 1) - 1.0);
                 for (int d = 0; d < q.dim; ++d) v[d] += 0.01 * (2.0 * u01(q.seed, idx, 2 + d) - 1.0);
             }
+        } else if (q.ic == TRIXI_HIP_IC_EOC_TEST_COUPLED_EULER_GRAVITY) {
+            // c = 2, A = 0.1, G = 1: rho = c + A sin(pi (sum x - t)), v = 1,
+            // p = rho^2 G 2 / (d pi)
+            double xs = -t;
+            for (int d = 0; d < q.dim; ++d) xs += x[d];
+            rho = 2.0 + 0.1 * sin(M_PI * xs);
+            for (int d = 0; d < q.dim; ++d) v[d] = 1.0;
+            p = rho * rho * 2.0 / (q.dim * M_PI);
         } else if (q.ic == TRIXI_HIP_IC_SEEDED_RANDOM) {
             rho = 1.0 + 0.1 * (2.0 * u01(q.seed, idx, 0) - 1.0);
             p = 1.0 + 0.05 * (2.0 * u01(q.seed, idx, 1) - 1.0);
@@ -247,6 +255,24 @@ __host__ __device__ inline void source_convergence_test(int dim, double gm1, con
     s[0] = (dim - 1) * gp;
     for (int d = 0; d < dim; ++d) s[1 + d] = (dim - 1) * gp + pp;
     s[dim + 1] = 2.0 * (dim - 1) * g * gp + dim * pp;
+}
+
+// source_terms_eoc_test_euler (compressible Euler, d dimensions) for the state
+// of initial_condition_eoc_test_coupled_euler_gravity: rho = g =
+// 2 + 0.1 sin(pi (sum_k x_k - t)), v_k = 1, p = K g^2 with K = 2 / (d pi).
+// With g' = 0.1 pi cos(pi (sum_k x_k - t)) and p' = 2 K g g', S = d_t u + div f(u):
+//   S_rho = (d - 1) g',  S_{rho v_k} = (d - 1) g' + p',
+//   S_{rho e} = (d - 1) e' + d p',  e' = p' / (gamma - 1) + d g' / 2.
+__host__ __device__ inline void source_eoc_test_euler(int dim, double gm1, const double* x, double t, double* s) {
+    double xs = -t;
+    for (int d = 0; d < dim; ++d) xs += x[d];
+    const double g = 2.0 + 0.1 * sin(M_PI * xs);
+    const double gp = 0.1 * M_PI * cos(M_PI * xs);
+    const double pp = 2.0 * (2.0 / (dim * M_PI)) * g * gp;
+    s[0] = (dim - 1) * gp;
+    for (int d = 0; d < dim; ++d) s[1 + d] = (dim - 1) * gp + pp;
+    const double ep = pp / gm1 + 0.5 * dim * gp;
+    s[dim + 1] = (dim - 1) * ep + dim * pp;
 }
 
 // source_terms_convergence_test of the inviscid Burgers equation: with

@@ -9,6 +9,7 @@ void register_euler2d() {
     Registrar<2, 4, EQ, VI_WEAK, C, FLUX_LLF, C>::add(EQ_EULER);
     Registrar<2, 4, EQ, VI_WEAK, C, FLUX_LLF_NAIVE, C>::add(EQ_EULER);
     Registrar<2, 4, EQ, VI_WEAK, C, FLUX_CENTRAL, C>::add(EQ_EULER);
+    Registrar<2, 4, EQ, VI_WEAK, C, FLUX_HLL, C>::add(EQ_EULER);
     Registrar<2, 4, EQ, VI_FD, FLUX_RANOCHA, FLUX_LLF, C>::add(EQ_EULER);
     Registrar<2, 4, EQ, VI_FD, FLUX_RANOCHA, FLUX_LLF_NAIVE, C>::add(EQ_EULER);
     Registrar<2, 4, EQ, VI_FD, FLUX_RANOCHA, FLUX_RANOCHA, C>::add(EQ_EULER);

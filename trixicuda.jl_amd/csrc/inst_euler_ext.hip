@@ -15,6 +15,10 @@ void register_euler_ext() {
     RegistrarExt<1, 4, Euler<1>, VI_FD, FLUX_RANOCHA, FLUX_LLF, C>::add(EQ_EULER);
     RegistrarExt<2, 4, Euler<2>, VI_FD, FLUX_RANOCHA, FLUX_LLF, C>::add(EQ_EULER);
     RegistrarExt<3, 4, Euler<3>, VI_FD, FLUX_RANOCHA, FLUX_LLF, C>::add(EQ_EULER);
+    // test/tree_dgsem_3d/euler_convergence.jl: weak form, flux_hll,
+    // source_terms_eoc_test_euler
+    RegistrarExt<2, 4, Euler<2>, VI_WEAK, C, FLUX_HLL, C>::add(EQ_EULER);
+    RegistrarExt<3, 4, Euler<3>, VI_WEAK, C, FLUX_HLL, C>::add(EQ_EULER);
 }
 
 }  // namespace trixi_hip

@@ -20,6 +20,7 @@ void register_vjp_euler3d() {
     VjpRegistrar<3, 4, EQ, VI_HG, FLUX_RANOCHA, FLUX_LLF_NAIVE, FLUX_LLF_NAIVE>::add(EQ_EULER);
     VjpRegistrar<3, 4, EQ, VI_HG, FLUX_SHIMA_ETAL, FLUX_LLF, FLUX_LLF>::add(EQ_EULER);
     VjpRegistrar<3, 4, EQ, VI_HG, FLUX_SHIMA_ETAL, FLUX_LLF_NAIVE, FLUX_LLF_NAIVE>::add(EQ_EULER);
+    VjpRegistrar<3, 4, EQ, VI_HG, FLUX_RANOCHA, FLUX_RANOCHA, FLUX_RANOCHA>::add(EQ_EULER);
 }
 
 }  // namespace trixi_hip

@@ -31,7 +31,8 @@ enum Flux : int {
     FLUX_RANOCHA = 3,
     FLUX_SHIMA_ETAL = 4,
     FLUX_HINDENLANG_GASSNER = 5,
-    FLUX_EC = 6           // flux_ec of the inviscid Burgers equation
+    FLUX_EC = 6,          // flux_ec of the inviscid Burgers equation
+    FLUX_HLL = 7          // FluxHLL(min_max_speed_davis) (compressible Euler)
 };
 
 struct PhysParams {
@@ -348,7 +349,26 @@ struct Euler {
             R fl[NV], fr[NV];
             flux(l, o, P, fl);
             flux(r, o, P, fr);
-            if (KIND == FLUX_LLF || KIND == FLUX_LLF_NAIVE) {
+            if (KIND == FLUX_HLL) {
+                // HLL with Davis' estimates lambda_min = min(v_l - c_l, v_r - c_r),
+                // lambda_max = max(v_l + c_l, v_r + c_r): upwind when both have
+                // one sign, else (lmax f_l - lmin f_r + lmin lmax [[u]]) / (lmax - lmin)
+                const R a = l.v[o] - l.c, b = r.v[o] - r.c, cc = l.v[o] + l.c, dd = r.v[o] + r.c;
+                const R lmin = b < a ? b : a;
+                const R lmax = dmax(cc, dd);
+                if (lmin >= R(0) && lmax >= R(0)) {
+#pragma unroll
+                    for (int q = 0; q < NV; ++q) f[q] = fl[q];
+                } else if (lmax <= R(0) && lmin <= R(0)) {
+#pragma unroll
+                    for (int q = 0; q < NV; ++q) f[q] = fr[q];
+                } else {
+                    const R inv = rdiv(R(1), lmax - lmin);
+                    const R fll = lmax * inv, frr = lmin * inv, fdiss = lmin * lmax * inv;
+#pragma unroll
+                    for (int q = 0; q < NV; ++q) f[q] = fll * fl[q] - frr * fr[q] + fdiss * (r.cons(q) - l.cons(q));
+                }
+            } else if (KIND == FLUX_LLF || KIND == FLUX_LLF_NAIVE) {
                 R lam;
                 if (KIND == FLUX_LLF_NAIVE)
                     lam = dmax(dabs(l.v[o]), dabs(r.v[o])) + dmax(l.c, r.c);

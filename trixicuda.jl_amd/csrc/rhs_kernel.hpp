@@ -119,7 +119,10 @@ __device__ __forceinline__ void source_at(const RhsArgs& a, int64_t e, int node,
         return;
     }
     double s5[5];
-    source_convergence_test(DIM, a.phys.gm1, xx, t, s5);
+    if (a.src == TRIXI_HIP_SOURCE_EOC_TEST_EULER)
+        source_eoc_test_euler(DIM, a.phys.gm1, xx, t, s5);
+    else
+        source_convergence_test(DIM, a.phys.gm1, xx, t, s5);
 #pragma unroll
     for (int v = 0; v < NV; ++v) sv[v] = s5[v];
 }

@@ -1024,7 +1024,10 @@ int trixi_hip_create(const trixi_hip_config* config, const trixi_hip_mesh* mesh,
     } else if (cfg.boundary_outflow_faces != 0) {
         return fail(TRIXI_HIP_EINVAL, "outflow faces need a non-periodic boundary configuration");
     }
-    if (cfg.source_terms != TRIXI_HIP_SOURCE_NONE) {
+    if (cfg.source_terms == TRIXI_HIP_SOURCE_EOC_TEST_EULER) {
+        if (cfg.equation != TRIXI_HIP_COMPRESSIBLE_EULER || cfg.ndims == 1)
+            return fail(TRIXI_HIP_EUNSUPPORTED, "source_terms_eoc_test_euler is built for compressible Euler 2D/3D");
+    } else if (cfg.source_terms != TRIXI_HIP_SOURCE_NONE) {
         if (cfg.source_terms != TRIXI_HIP_SOURCE_CONVERGENCE_TEST) return fail(TRIXI_HIP_EINVAL, "unknown source terms");
         if (cfg.equation != TRIXI_HIP_COMPRESSIBLE_EULER && cfg.equation != TRIXI_HIP_INVISCID_BURGERS)
             return fail(TRIXI_HIP_EUNSUPPORTED,

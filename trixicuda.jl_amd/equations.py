@@ -132,6 +132,19 @@ flux_ranocha = NumericalFlux("flux_ranocha", L.FLUX_RANOCHA)
 flux_shima_etal = NumericalFlux("flux_shima_etal", L.FLUX_SHIMA_ETAL)
 flux_hindenlang_gassner = NumericalFlux("flux_hindenlang_gassner", L.FLUX_HINDENLANG_GASSNER)
 flux_ec = NumericalFlux("flux_ec", L.FLUX_EC)  # entropy-conservative flux of the Burgers equation
+
+
+def min_max_speed_davis(*args):  # identity marker of Trixi's min_max_speed_davis
+    raise TypeError("wave-speed estimates are selected, not called, on the HIP path")
+
+
+def FluxHLL(min_max_speed=min_max_speed_davis):
+    if min_max_speed is not min_max_speed_davis:
+        raise NotImplementedError("FluxHLL is built with min_max_speed_davis")
+    return NumericalFlux("FluxHLL(min_max_speed_davis)", L.FLUX_HLL)
+
+
+flux_hll = FluxHLL()  # compressible Euler
 flux_nonconservative_powell = NumericalFlux("flux_nonconservative_powell", -1)
 
 
@@ -170,6 +183,9 @@ initial_condition_weak_blast_wave = InitialCondition("initial_condition_weak_bla
 # the Burgers step states of test/tree_dgsem_1d/burgers_shock.jl and burgers_rarefraction.jl
 initial_condition_shock = InitialCondition("initial_condition_shock", L.IC_SHOCK)
 initial_condition_rarefaction = InitialCondition("initial_condition_rarefaction", L.IC_RAREFACTION)
+# compressible Euler 2D/3D: rho = 2 + 0.1 sin(pi (sum x - t)), v = 1, p = 2 rho^2 / (d pi) (gamma = 2)
+initial_condition_eoc_test_coupled_euler_gravity = InitialCondition(
+    "initial_condition_eoc_test_coupled_euler_gravity", L.IC_EOC_TEST_COUPLED_EULER_GRAVITY)
 
 
 def initial_condition_seeded_random(seed=20261202):
@@ -236,6 +252,8 @@ class SourceTerms:
 # Trixi's source_terms_convergence_test for compressible Euler: makes
 # initial_condition_convergence_test an exact solution.
 source_terms_convergence_test = SourceTerms("source_terms_convergence_test", L.SOURCE_CONVERGENCE_TEST)
+# the manufactured source of initial_condition_eoc_test_coupled_euler_gravity (Euler 2D/3D)
+source_terms_eoc_test_euler = SourceTerms("source_terms_eoc_test_euler", L.SOURCE_EOC_TEST_EULER)
 
 
 def initial_condition_smooth_periodic(seed=20261202):

@@ -83,6 +83,8 @@ flux_id(::typeof(flux_ranocha)) = 3
 flux_id(::typeof(flux_shima_etal)) = 4
 flux_id(::typeof(flux_hindenlang_gassner)) = 5
 flux_id(::typeof(flux_ec)) = 6
+flux_id(f::FluxHLL) = f.min_max_speed === min_max_speed_davis ? 7 :
+    throw(ArgumentError("FluxHLL is built with min_max_speed_davis"))
 flux_id(f::Tuple) = (f[2] === flux_nonconservative_powell ? flux_id(f[1]) :
                      throw(ArgumentError("unsupported nonconservative flux $(f[2])")))
 flux_id(f) = throw(ArgumentError("flux $f is not built for HIP"))
@@ -96,6 +98,7 @@ initial_condition_shock(x, t, ::InviscidBurgersEquation1D) = SVector(x[1] < 0.5 
 initial_condition_rarefaction(x, t, ::InviscidBurgersEquation1D) = SVector(x[1] < 0.5 ? 0.5 : 1.5)
 ic_id(::typeof(initial_condition_shock)) = 6
 ic_id(::typeof(initial_condition_rarefaction)) = 7
+ic_id(::typeof(initial_condition_eoc_test_coupled_euler_gravity)) = 8
 ic_id(f) = throw(ArgumentError("initial condition $f is not built for HIP"))
 # seeded benchmark states (ids 3, 4, 5 of trixi_hip_initial_condition) carry their seed
 struct SeededInitialCondition
@@ -168,6 +171,7 @@ indicator_variable_id(f) = throw(ArgumentError("indicator variable $f is not bui
 
 source_id(::Nothing) = 0
 source_id(::typeof(source_terms_convergence_test)) = 1
+source_id(::typeof(source_terms_eoc_test_euler)) = 2
 source_id(f) = throw(ArgumentError("source terms $f are not built for HIP"))
 
 function make_config(solver::DGSEMHIP, equations, mesh; boundary_conditions = boundary_condition_periodic,
