@@ -161,29 +161,31 @@ __device__ __forceinline__ int line_base(int line, int d) {
 // element, which pays where LDS limits the occupancy of the sequential form:
 // the FP64 rhs! (3D Euler flux differencing at level 5: 93.8 -> 89.5 us).  In
 // FP32 (63.4 -> 76.8 us) and in the fused RK stage (no gain) the sequential
-// form stays, with one copy of each direction's code live at a time.
-template <int DIM, int N, int M, typename R, bool RK>
+// form stays, with one copy of each direction's code live at a time, and so
+// it does for the nine-variable MHD system, whose parallel form spills
+// (3D GLM-MHD at level 4: 28.6 -> 45.5 us).
+template <int DIM, int N, int M, typename R, bool RK, int NV>
 __host__ __device__ constexpr bool rhs_par_dirs() {
     return TRIXI_HIP_PAR_DIRS != 0 && DIM > 1 && (M * cpow(N, DIM - 1)) % 64 == 0 &&
-           std::is_same<R, double>::value && !RK;
+           std::is_same<R, double>::value && !RK && NV <= 5;
 }
-template <int DIM, int N, int M, typename R, bool RK>
+template <int DIM, int N, int M, typename R, bool RK, int NV>
 __host__ __device__ constexpr int rhs_threads() {
-    return M * cpow(N, DIM - 1) * (rhs_par_dirs<DIM, N, M, R, RK>() ? DIM : 1);
+    return M * cpow(N, DIM - 1) * (rhs_par_dirs<DIM, N, M, R, RK, NV>() ? DIM : 1);
 }
 
 // EXT: the variant with boundary faces (nbr = -1, Dirichlet states), L2
 // mortar faces (nbr = -2 - slot, fluxes from the mortar pass) and source
 // terms; the periodic, conforming, source-free kernels carry none of that code.
 template <int DIM, int N, class EQ, int VOL, int VF, int SF, int FVF, typename R, int M, bool RK, bool EXT>
-__global__ void __launch_bounds__((rhs_threads<DIM, N, M, R, RK>()), TRIXI_HIP_MIN_WAVES)
+__global__ void __launch_bounds__((rhs_threads<DIM, N, M, R, RK, EQ::NV>()), TRIXI_HIP_MIN_WAVES)
 rhs_kernel(const RhsArgs a) {
     constexpr int NL = cpow(N, DIM - 1);
     constexpr int NN = cpow(N, DIM);
     // PD: lane group g (whole waves) evaluates direction g; the groups meet in
     // the LDS accumulator in direction order (x stores, y adds, z adds and
     // writes), the same summation order as the sequential phases.
-    constexpr bool PD = rhs_par_dirs<DIM, N, M, R, RK>();
+    constexpr bool PD = rhs_par_dirs<DIM, N, M, R, RK, EQ::NV>();
     constexpr int NG = PD ? DIM : 1;
     constexpr int NT = M * NL * NG;         // workgroup size
     constexpr int NIT = (N + NG - 1) / NG;  // node-pass (node, element) pairs per lane
@@ -627,7 +629,7 @@ hipError_t launch_rhs(const RhsArgs& a, hipStream_t stream) {
     int64_t blocks = (a.E + M - 1) / M;
     if (a.nxcd > 1) blocks = (blocks + a.nxcd - 1) / a.nxcd * a.nxcd;
     hipLaunchKernelGGL((rhs_kernel<DIM, N, EQ, VOL, VF, SF, FVF, R, M, RK, EXT>), dim3((unsigned)blocks),
-                       dim3(rhs_threads<DIM, N, M, R, RK>()), 0, stream, a);
+                       dim3(rhs_threads<DIM, N, M, R, RK, EQ::NV>()), 0, stream, a);
     (void)NL;
     return hipGetLastError();
 }
