@@ -51,7 +51,8 @@ enum trixi_hip_equation {
     TRIXI_HIP_COMPRESSIBLE_EULER = 1,      /* CompressibleEulerEquations{1,2,3}D    */
     TRIXI_HIP_IDEAL_GLM_MHD = 2,           /* IdealGlmMhdEquations1D / 2D / 3D (1D:
                                               8 variables, no psi / Powell terms)  */
-    TRIXI_HIP_INVISCID_BURGERS = 3         /* InviscidBurgersEquation1D             */
+    TRIXI_HIP_INVISCID_BURGERS = 3,        /* InviscidBurgersEquation1D             */
+    TRIXI_HIP_HYPERBOLIC_DIFFUSION = 4     /* HyperbolicDiffusionEquations{1,2,3}D  */
 };
 enum trixi_hip_volume_integral {
     TRIXI_HIP_VOLUME_WEAK_FORM = 0,          /* VolumeIntegralWeakForm          */
@@ -87,8 +88,15 @@ enum trixi_hip_initial_condition {
                                           (test/tree_dgsem_1d/burgers_shock.jl)    */
     TRIXI_HIP_IC_RAREFACTION = 7,      /* Burgers: 0.5 for x < 0.5, else 1.5
                                           (test/tree_dgsem_1d/burgers_rarefraction.jl) */
-    TRIXI_HIP_IC_EOC_TEST_COUPLED_EULER_GRAVITY = 8 /* Euler 2D/3D: rho = 2 + 0.1 sin(pi (sum x - t)),
+    TRIXI_HIP_IC_EOC_TEST_COUPLED_EULER_GRAVITY = 8, /* Euler 2D/3D: rho = 2 + 0.1 sin(pi (sum x - t)),
                                           v = 1, p = 2 rho^2 / (d pi) (gamma = 2) */
+    TRIXI_HIP_IC_POISSON_NONPERIODIC = 9,  /* hyperbolic diffusion: the initial guess at
+                                          t = 0, else the steady solution (as a
+                                          boundary state: always the steady solution,
+                                          boundary_condition_poisson_nonperiodic) */
+    TRIXI_HIP_IC_HARMONIC_NONPERIODIC = 10 /* hyperbolic diffusion 1D: phi = 5, q = 0 at
+                                          t = 0, else phi = 3 + e x, q = e
+                                          (test/tree_dgsem_1d/hypdiff_harmonic_nonperiodic.jl) */
 };
 
 /* Boundary condition on the faces of a non-periodic TreeMesh that have no
@@ -107,7 +115,10 @@ enum trixi_hip_boundary_condition {
 enum trixi_hip_source_terms {
     TRIXI_HIP_SOURCE_NONE = 0,
     TRIXI_HIP_SOURCE_CONVERGENCE_TEST = 1, /* source_terms_convergence_test (Euler, Burgers) */
-    TRIXI_HIP_SOURCE_EOC_TEST_EULER = 2    /* source_terms_eoc_test_euler (Euler 2D/3D)     */
+    TRIXI_HIP_SOURCE_EOC_TEST_EULER = 2,   /* source_terms_eoc_test_euler (Euler 2D/3D)     */
+    TRIXI_HIP_SOURCE_POISSON_NONPERIODIC = 3, /* hyperbolic diffusion: f of the steady state,
+                                             -q / T_r                                  */
+    TRIXI_HIP_SOURCE_HARMONIC = 4          /* hyperbolic diffusion: 0, -q / T_r            */
 };
 
 /* IndicatorHennemannGassner(...; variable) (Trixi's density_pressure,
@@ -144,6 +155,8 @@ typedef struct trixi_hip_config {
                                        set: that side takes the outflow state
                                        (the inner state, flux(u_inner)) instead of
                                        the Dirichlet state                        */
+    double nu;                /* hyperbolic diffusion: diffusion constant (1)     */
+    double Lr;                /* hyperbolic diffusion: reference length 1/(2 pi); T_r = Lr^2 / nu */
 } trixi_hip_config;
 
 /* Mesh and connectivity (host arrays, copied to the device by create): the

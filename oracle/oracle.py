@@ -17,13 +17,13 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_PATH = os.path.join(_HERE, "liboracle.so")
 
-EQ_LINADV, EQ_EULER, EQ_GLM_MHD, EQ_BURGERS = 0, 1, 2, 3
+EQ_LINADV, EQ_EULER, EQ_GLM_MHD, EQ_BURGERS, EQ_HYPDIFF = 0, 1, 2, 3, 4
 VI_WEAK, VI_FD, VI_HG = 0, 1, 2
 FLUX_CENTRAL, FLUX_LLF, FLUX_LLF_NAIVE, FLUX_RANOCHA, FLUX_SHIMA_ETAL, FLUX_HINDENLANG_GASSNER, FLUX_EC, FLUX_HLL = range(8)
 (IC_CONSTANT, IC_CONVERGENCE_TEST, IC_WEAK_BLAST_WAVE, IC_SEEDED_RANDOM, IC_SMOOTH_PERIODIC, IC_WEAK_BLAST_WAVE_SEEDED,
- IC_SHOCK, IC_RAREFACTION, IC_EOC_GRAVITY) = range(9)
+ IC_SHOCK, IC_RAREFACTION, IC_EOC_GRAVITY, IC_POISSON_NONPERIODIC, IC_HARMONIC_NONPERIODIC) = range(11)
 BC_PERIODIC, BC_DIRICHLET = 0, 1
-SOURCE_NONE, SOURCE_CONVERGENCE_TEST, SOURCE_EOC_TEST_EULER = 0, 1, 2
+SOURCE_NONE, SOURCE_CONVERGENCE_TEST, SOURCE_EOC_TEST_EULER, SOURCE_POISSON_NONPERIODIC, SOURCE_HARMONIC = range(5)
 IND_DENSITY_PRESSURE, IND_PRESSURE, IND_DENSITY = 0, 1, 2
 
 
@@ -56,6 +56,8 @@ class OracleCfg(ctypes.Structure):
         ("mortar_large_sides", ctypes.c_void_p),
         ("boundary_outflow_faces", ctypes.c_int32),
         ("reserved", ctypes.c_int32),
+        ("nu", ctypes.c_double),
+        ("Lr", ctypes.c_double),
     ]
 
 
@@ -124,8 +126,10 @@ def make_cfg(ndims, equation, polydeg=3, volume_integral=VI_WEAK, volume_flux=FL
              surface_flux=FLUX_LLF, fv_flux=FLUX_LLF, gamma=1.4, c_h=1.0,
              advection_velocity=(1.0, 1.0, 1.0), alpha_max=0.5, alpha_min=0.001,
              alpha_smooth=True, ic_length=4.0, boundary_condition=BC_PERIODIC, boundary_ic=IC_CONSTANT,
-             boundary_seed=0, source_terms=SOURCE_NONE, indicator_variable=0, boundary_outflow_faces=0):
+             boundary_seed=0, source_terms=SOURCE_NONE, indicator_variable=0, boundary_outflow_faces=0,
+             nu=1.0, Lr=1.0 / (2.0 * np.pi)):
     c = OracleCfg()
+    c.nu, c.Lr = nu, Lr
     c.boundary_outflow_faces = boundary_outflow_faces
     c.indicator_variable = indicator_variable
     c.boundary_condition, c.boundary_ic, c.boundary_seed = boundary_condition, boundary_ic, boundary_seed
@@ -224,8 +228,11 @@ class Mesh:
 
     def __init__(self, ndims, level, cmin, cmax, polydeg, periodic=True):
         E = lib().oracle_mesh_nelements(ndims, level)
-        I = lib().oracle_mesh_ninterfaces(ndims, level, 1 if periodic else 0)
-        self.periodic = bool(periodic)
+        per = tuple(map(bool, periodic)) if isinstance(periodic, (tuple, list)) else (bool(periodic),) * ndims
+        mask = sum(1 << d for d in range(ndims) if per[d])
+        I = lib().oracle_mesh_ninterfaces(ndims, level, mask)
+        self.periodic = all(per)
+        self.periodicity = per
         N = polydeg + 1
         self.ndims, self.level, self.polydeg, self.nelements, self.ninterfaces = ndims, level, polydeg, E, I
         self.coordinates_min, self.coordinates_max = cmin, cmax
@@ -234,7 +241,7 @@ class Mesh:
         self.inverse_jacobian = np.zeros(E)
         self.node_coordinates = np.zeros((E,) + (N,) * ndims + (ndims,))
         self.cell_index = np.zeros((E, ndims), dtype=np.int32)
-        rc = lib().oracle_mesh(ndims, level, cmin, cmax, polydeg, 1 if periodic else 0, _p(self.neighbor_ids),
+        rc = lib().oracle_mesh(ndims, level, cmin, cmax, polydeg, mask, _p(self.neighbor_ids),
                                _p(self.orientations), _p(self.inverse_jacobian), _p(self.node_coordinates),
                                _p(self.cell_index))
         assert rc == 0

@@ -81,7 +81,7 @@ constexpr int MAXV = 9;
 
 struct Cfg {
     int nd, eq, N, nv, vi, vflux, sflux, fvflux, smooth;
-    double gamma, inv_gm1, c_h, a[3], amax, amin, ic_length;
+    double gamma, inv_gm1, c_h, a[3], amax, amin, ic_length, nu, inv_Tr;
     int bc, bc_ic, src, ivar, outflow;
     uint64_t bc_seed;
     const double* xnodes;
@@ -115,6 +115,8 @@ Cfg make_cfg(const oracle_cfg* c) {
     k.xnodes = c->node_coordinates;
     k.t = c->time;
     k.ivar = c->indicator_variable;
+    k.nu = c->nu;
+    k.inv_Tr = (c->Lr > 0.0) ? c->nu / (c->Lr * c->Lr) : 0.0;
     k.outflow = c->boundary_outflow_faces;
     k.nmortars = c->nmortars;
     k.mids = c->mortar_neighbor_ids;
@@ -174,7 +176,11 @@ void mhd_prim(const Cfg& c, const T* u, T& rho, T* v, T& p, T* B, T& psi) {
 
 template <class T> void phys_flux(const Cfg& c, const T* u, int o, T* f) {
     const int oi = o - 1;
-    if (c.eq == OR_EQ_BURGERS) {
+    if (c.eq == OR_EQ_HYPDIFF) {
+        // Trixi HyperbolicDiffusionEquations flux: (-nu q_o, -phi / T_r e_o)
+        f[0] = -c.nu * u[o];
+        for (int d = 0; d < c.nd; ++d) f[1 + d] = (d == oi) ? -u[0] * c.inv_Tr : T(0.0);
+    } else if (c.eq == OR_EQ_BURGERS) {
         f[0] = 0.5 * u[0] * u[0];  // Trixi InviscidBurgersEquation1D flux
     } else if (c.eq == OR_EQ_LINADV) {
         f[0] = c.a[oi] * u[0];
@@ -232,6 +238,9 @@ template <class T> void normal_speed(const Cfg& c, const T* u, int o, T& vn, T& 
     } else if (c.eq == OR_EQ_BURGERS) {
         vn = u[0];  // max_abs_speed(_naive) of Burgers: max(|u_ll|, |u_rr|)
         cs = 0.0;
+    } else if (c.eq == OR_EQ_HYPDIFF) {
+        vn = 0.0;  // constant speed sqrt(nu / T_r)
+        cs = sqrt(c.nu * c.inv_Tr);
     } else if (c.eq == OR_EQ_EULER) {
         T rho, v[3], p;
         euler_prim(c, u, rho, v, p);
@@ -834,6 +843,8 @@ void indicator_hg(const Cfg& c, const Basis& b, int64_t E, int64_t I, const int3
 // ----------------------------------------------------------------------------
 void initial_condition(const Cfg& c, int ic, const double* x, double t, uint64_t seed, uint64_t node_index,
                        double* u);
+void hypdiff_state(const Cfg& c, int ic, const double* x, double t, bool steady, uint64_t seed, uint64_t idx,
+                   double* u);
 
 // source_terms_convergence_test (compressible Euler): with
 // g(s) = 2 + 0.1 sin(pi s), s = x_1 + ... + x_d - t, the convergence-test state
@@ -1060,7 +1071,11 @@ void rhs_impl(const Cfg& c, int64_t E, int64_t I, const int32_t* nbr, const int3
                     const int node = face_node(N, nd, od, f, plus ? N - 1 : 0);
                     double x[3] = {0, 0, 0}, ubd[MAXV];
                     for (int d = 0; d < nd; ++d) x[d] = c.xnodes[((size_t)e * nn + node) * nd + d];
-                    initial_condition(c, c.bc_ic, x, c.t, c.bc_seed, (uint64_t)(e * nn + node), ubd);
+                    if (c.eq == OR_EQ_HYPDIFF)
+                        hypdiff_state(c, c.bc_ic, x, c.t, c.bc_ic == OR_IC_POISSON_NONPERIODIC, c.bc_seed,
+                                      (uint64_t)(e * nn + node), ubd);
+                    else
+                        initial_condition(c, c.bc_ic, x, c.t, c.bc_seed, (uint64_t)(e * nn + node), ubd);
                     for (int v = 0; v < nv; ++v) {
                         uin[v] = u[e * esz + nv * node + v];
                         ub[v] = T(ubd[v]);
@@ -1099,13 +1114,33 @@ void rhs_impl(const Cfg& c, int64_t E, int64_t I, const int32_t* nbr, const int3
 
     // calc_sources! (after the Jacobian, as in Trixi and the reference
     // cuda_sources!, dg_2d.jl:746-757)
-    if (c.src == OR_SOURCE_CONVERGENCE_TEST || c.src == OR_SOURCE_EOC_TEST_EULER) {
+    if (c.src != OR_SOURCE_NONE) {
 #pragma omp parallel for schedule(static)
         for (int64_t e = 0; e < E; ++e)
             for (int n = 0; n < nn; ++n) {
                 double x[3] = {0, 0, 0}, sv[MAXV];
                 for (int d = 0; d < nd; ++d) x[d] = c.xnodes[((size_t)e * nn + n) * nd + d];
-                source_convergence_test(c, x, c.t, sv);
+                if (c.eq == OR_EQ_HYPDIFF) {
+                    // source_terms_poisson_nonperiodic / source_terms_harmonic:
+                    // f = -nu lap phi of the steady state (sum of the second
+                    // derivatives along each direction), and -q / T_r
+                    double f = 0.0;
+                    if (c.src == OR_SOURCE_POISSON_NONPERIODIC) {
+                        if (nd == 1) {
+                            f = c.nu * M_PI * M_PI * sin(M_PI * x[0]);
+                        } else {
+                            double k2 = M_PI * M_PI;
+                            for (int k = 1; k < nd; ++k) k2 += 4.0 * M_PI * M_PI;
+                            double prod = 2.0 * cos(M_PI * x[0]);
+                            for (int k = 1; k < nd; ++k) prod *= sin(2.0 * M_PI * x[k]);
+                            f = c.nu * k2 * prod;
+                        }
+                    }
+                    sv[0] = f;
+                    for (int k = 0; k < nd; ++k) sv[1 + k] = -c.inv_Tr * val(u[e * esz + nv * n + 1 + k]);
+                } else {
+                    source_convergence_test(c, x, c.t, sv);
+                }
                 for (int v = 0; v < nv; ++v) du[e * esz + nv * n + v] = du[e * esz + nv * n + v] + sv[v];
             }
     }
@@ -1172,8 +1207,58 @@ double smooth_field(uint64_t seed, int field, int nd, const double* x, double L,
     return s;
 }
 
+// Hyperbolic diffusion states (Trixi initial_condition_poisson_nonperiodic:
+// the steady solution phi of -lap phi = f, q = grad phi; the initial guess
+// phi = q = 1 at t = 0; boundary_condition_poisson_nonperiodic always the
+// steady solution) and the 1D harmonic state phi = A + B x, q = B (A = 3,
+// B = e; phi = 5, q = 0 at t = 0).
+void hypdiff_state(const Cfg& c, int ic, const double* x, double t, bool steady, uint64_t seed, uint64_t idx,
+                   double* u) {
+    const int d = c.nd;
+    if (ic == OR_IC_SEEDED_RANDOM) {
+        for (int k = 0; k <= d; ++k) u[k] = 1.0 + 0.5 * (2.0 * u01(seed, idx, k) - 1.0);
+    } else if (ic == OR_IC_POISSON_NONPERIODIC) {
+        if (t == 0.0 && !steady) {
+            for (int k = 0; k <= d; ++k) u[k] = 1.0;
+        } else if (d == 1) {
+            u[0] = sin(M_PI * x[0]);
+            u[1] = M_PI * cos(M_PI * x[0]);
+        } else {
+            // phi = 2 cos(pi x) prod_{k>0} sin(2 pi x_k) + 2 and its gradient
+            double fac[3], dfac[3];
+            fac[0] = cos(M_PI * x[0]);
+            dfac[0] = -M_PI * sin(M_PI * x[0]);
+            for (int k = 1; k < d; ++k) {
+                fac[k] = sin(2.0 * M_PI * x[k]);
+                dfac[k] = 2.0 * M_PI * cos(2.0 * M_PI * x[k]);
+            }
+            double prod = 2.0;
+            for (int k = 0; k < d; ++k) prod *= fac[k];
+            u[0] = prod + 2.0;
+            for (int j = 0; j < d; ++j) {
+                double g = 2.0 * dfac[j];
+                for (int k = 0; k < d; ++k)
+                    if (k != j) g *= fac[k];
+                u[1 + j] = g;
+            }
+        }
+    } else if (ic == OR_IC_HARMONIC_NONPERIODIC) {
+        const double A = 3.0, B = exp(1.0);
+        u[0] = (t == 0.0) ? 5.0 : A + B * x[0];
+        u[1] = (t == 0.0) ? 0.0 : B;
+        for (int k = 2; k <= d; ++k) u[k] = 0.0;
+    } else {
+        u[0] = 1.0;
+        for (int k = 1; k <= d; ++k) u[k] = 0.0;
+    }
+}
+
 void initial_condition(const Cfg& c, int ic, const double* x, double t, uint64_t seed,
                        uint64_t idx, double* u) {
+    if (c.eq == OR_EQ_HYPDIFF) {
+        hypdiff_state(c, ic, x, t, false, seed, idx, u);
+        return;
+    }
     if (c.eq == OR_EQ_BURGERS) {
         // Trixi initial_condition_convergence_test(x, t, ::InviscidBurgersEquation1D):
         // c + A sin(omega (x - t)), c = 2, A = 1, L = 1, omega = 2 pi / L; and
@@ -1335,6 +1420,7 @@ extern "C" {
 
 int oracle_nvars(const oracle_cfg* cfg) {
     if (cfg->equation == OR_EQ_LINADV || cfg->equation == OR_EQ_BURGERS) return 1;
+    if (cfg->equation == OR_EQ_HYPDIFF) return cfg->ndims + 1;
     if (cfg->equation == OR_EQ_EULER) return cfg->ndims + 2;
     return cfg->ndims == 1 ? 8 : 9;  // ideal MHD: (rho, rho v, rho e, B) in 1D, + psi in 2D/3D
 }
@@ -1431,6 +1517,8 @@ void oracle_max_abs_speeds(const oracle_cfg* cfg, const double* u, double* lam) 
             lam[d] = fabs(c.a[d]);
         } else if (c.eq == OR_EQ_BURGERS) {
             lam[d] = fabs(u[0]);
+        } else if (c.eq == OR_EQ_HYPDIFF) {
+            lam[d] = sqrt(c.nu * c.inv_Tr);
         } else if (c.eq == OR_EQ_EULER) {
             double rho, v[3], p;
             euler_prim(c, u, rho, v, p);
@@ -1494,16 +1582,18 @@ int64_t oracle_mesh_nelements(int ndims, int level) { return (int64_t)1 << (ndim
 // (Trixi TreeMesh + init_elements!/init_interfaces!).
 int64_t oracle_mesh_ninterfaces(int nd, int level, int periodic) {
     const int64_t E = oracle_mesh_nelements(nd, level);
-    if (periodic) return nd * E;
-    // faces on the + boundary of each direction have no interface
-    return nd * (E - (E >> level));
+    // per direction: E interfaces when periodic (bit d), else the faces on
+    // the + boundary have none
+    int64_t I = 0;
+    for (int d = 0; d < nd; ++d) I += ((periodic >> d) & 1) ? E : E - (E >> level);
+    return I;
 }
 
 int oracle_mesh_uniform(int nd, int level, double cmin, double cmax, int polydeg,
                         int32_t* neighbor_ids, int32_t* orientations, double* invJ,
                         double* node_coordinates, int32_t* cell_index) {
-    return oracle_mesh(nd, level, cmin, cmax, polydeg, 1, neighbor_ids, orientations, invJ, node_coordinates,
-                       cell_index);
+    return oracle_mesh(nd, level, cmin, cmax, polydeg, (1 << nd) - 1, neighbor_ids, orientations, invJ,
+                       node_coordinates, cell_index);
 }
 
 int oracle_mesh(int nd, int level, double cmin, double cmax, int polydeg, int periodic,
@@ -1563,7 +1653,7 @@ int oracle_mesh(int nd, int level, double cmin, double cmax, int polydeg, int pe
                 }
         }
         for (int d = 0; d < nd; ++d) {
-            if (!periodic && cells[e][d] + 1 == n1) continue;  // boundary face
+            if (!((periodic >> d) & 1) && cells[e][d] + 1 == n1) continue;  // boundary face
             std::vector<int64_t> k = cells[e];
             k[d] = (k[d] + 1) % n1;
             int64_t key = 0;

@@ -25,7 +25,7 @@
 extern "C" {
 #endif
 
-enum { OR_EQ_LINADV = 0, OR_EQ_EULER = 1, OR_EQ_GLM_MHD = 2, OR_EQ_BURGERS = 3 };
+enum { OR_EQ_LINADV = 0, OR_EQ_EULER = 1, OR_EQ_GLM_MHD = 2, OR_EQ_BURGERS = 3, OR_EQ_HYPDIFF = 4 };
 enum { OR_VI_WEAK = 0, OR_VI_FD = 1, OR_VI_HG = 2 };
 enum {
     OR_FLUX_CENTRAL = 0,
@@ -46,10 +46,13 @@ enum {
     OR_IC_WEAK_BLAST_WAVE_SEEDED = 5, /* Euler: weak blast + U(-0.01,0.01) on rho,v,p */
     OR_IC_SHOCK = 6,          /* Burgers: 1.5 for x < 0.5, else 0.5 */
     OR_IC_RAREFACTION = 7,    /* Burgers: 0.5 for x < 0.5, else 1.5 */
-    OR_IC_EOC_GRAVITY = 8     /* Euler: initial_condition_eoc_test_coupled_euler_gravity */
+    OR_IC_EOC_GRAVITY = 8,    /* Euler: initial_condition_eoc_test_coupled_euler_gravity */
+    OR_IC_POISSON_NONPERIODIC = 9,  /* hyperbolic diffusion: guess at t = 0, else steady */
+    OR_IC_HARMONIC_NONPERIODIC = 10 /* hyperbolic diffusion 1D harmonic state */
 };
 enum { OR_BC_PERIODIC = 0, OR_BC_DIRICHLET = 1 };
-enum { OR_SOURCE_NONE = 0, OR_SOURCE_CONVERGENCE_TEST = 1, OR_SOURCE_EOC_TEST_EULER = 2 };
+enum { OR_SOURCE_NONE = 0, OR_SOURCE_CONVERGENCE_TEST = 1, OR_SOURCE_EOC_TEST_EULER = 2,
+       OR_SOURCE_POISSON_NONPERIODIC = 3, OR_SOURCE_HARMONIC = 4 };
 
 typedef struct oracle_cfg {
     int32_t ndims;            /* 1, 2, 3 */
@@ -78,6 +81,7 @@ typedef struct oracle_cfg {
     const int32_t* mortar_large_sides;  /* [nmortars] 1: large on the left, 2: right */
     int32_t boundary_outflow_faces; /* bit f (-x,+x,-y,+y,-z,+z): boundary flux f(u_inner) */
     int32_t reserved;
+    double nu, Lr;            /* hyperbolic diffusion: T_r = Lr^2 / nu */
 } oracle_cfg;
 
 int oracle_nvars(const oracle_cfg* cfg);
@@ -111,8 +115,9 @@ int oracle_mesh_uniform(int ndims, int level, double coord_min, double coord_max
                         int polydeg, int32_t* neighbor_ids, int32_t* orientations,
                         double* inverse_jacobian, double* node_coordinates,
                         int32_t* cell_index /* [ndims*E] integer cell coords */);
-/* The same mesh, non-periodic when periodic == 0: no interfaces across the
- * domain boundary (those faces are boundary faces). */
+/* The same mesh, periodic in direction d iff bit d of `periodic` is set:
+ * no interfaces across the domain boundary in the other directions (those
+ * faces are boundary faces). */
 int64_t oracle_mesh_ninterfaces(int ndims, int level, int periodic);
 int oracle_mesh(int ndims, int level, double coord_min, double coord_max, int polydeg, int periodic,
                 int32_t* neighbor_ids, int32_t* orientations, double* inverse_jacobian,

@@ -9,7 +9,7 @@ import numpy as np
 class Case:
     name: str
     ndims: int
-    equation: str           # "linadv" | "euler" | "mhd" | "burgers"
+    equation: str           # "linadv" | "euler" | "mhd" | "burgers" | "hypdiff"
     level: int
     vi: str = "weak"        # "weak" | "fd" | "hg"
     vflux: str = "central"
@@ -34,12 +34,14 @@ class Case:
     patches: tuple = ()     # box refinement patches ((lo, hi), ...): L2 mortars
     indicator: str = "density_pressure"  # HG indicator variable: density_pressure | pressure | density | first
     outflow: tuple = ()     # boundary sides ("x_neg", ...) with the outflow condition
+    nu: float = 1.0         # hyperbolic diffusion constant
 
 
 FLUX_NAMES = ("central", "llf", "llf_naive", "ranocha", "shima_etal", "hindenlang_gassner", "ec", "hll")
 IC_NAMES = ("constant", "convergence_test", "weak_blast_wave", "seeded_random", "smooth_periodic",
-            "weak_blast_wave_seeded", "shock", "rarefaction", "eoc_gravity")
-SOURCES = {"": 0, "convergence_test": 1, "eoc_test_euler": 2}
+            "weak_blast_wave_seeded", "shock", "rarefaction", "eoc_gravity", "poisson_nonperiodic",
+            "harmonic_nonperiodic")
+SOURCES = {"": 0, "convergence_test": 1, "eoc_test_euler": 2, "poisson_nonperiodic": 3, "harmonic": 4}
 SIDES = ("x_neg", "x_pos", "y_neg", "y_pos", "z_neg", "z_pos")
 INDICATORS = {"density_pressure": 0, "pressure": 1, "density": 2, "first": 2}
 
@@ -51,7 +53,8 @@ def oracle_mesh(O, c: Case):
 
 
 def oracle_cfg(O, c: Case):
-    eq = {"linadv": O.EQ_LINADV, "euler": O.EQ_EULER, "mhd": O.EQ_GLM_MHD, "burgers": O.EQ_BURGERS}[c.equation]
+    eq = {"linadv": O.EQ_LINADV, "euler": O.EQ_EULER, "mhd": O.EQ_GLM_MHD, "burgers": O.EQ_BURGERS,
+          "hypdiff": O.EQ_HYPDIFF}[c.equation]
     vi = {"weak": O.VI_WEAK, "fd": O.VI_FD, "hg": O.VI_HG}[c.vi]
     ext = dict(boundary_condition=O.BC_DIRICHLET if c.bc_state else O.BC_PERIODIC,
                boundary_ic=IC_NAMES.index(c.bc_state) if c.bc_state else 0,
@@ -61,7 +64,7 @@ def oracle_cfg(O, c: Case):
                       FLUX_NAMES.index(c.fvflux), gamma=c.gamma, c_h=c.c_h,
                       advection_velocity=c.advection[:c.ndims], alpha_max=c.alpha_max, alpha_min=c.alpha_min,
                       alpha_smooth=c.alpha_smooth, ic_length=c.cmax - c.cmin,
-                      indicator_variable=INDICATORS[c.indicator], **ext)
+                      indicator_variable=INDICATORS[c.indicator], nu=c.nu, **ext)
 
 
 def oracle_ic(O, c: Case):
@@ -83,6 +86,9 @@ def product_semi(th, c: Case, precision=None):
               th.LinearScalarAdvectionEquation3D][c.ndims](c.advection[:c.ndims])
     elif c.equation == "burgers":
         eq = th.InviscidBurgersEquation1D()
+    elif c.equation == "hypdiff":
+        eq = [None, th.HyperbolicDiffusionEquations1D, th.HyperbolicDiffusionEquations2D,
+              th.HyperbolicDiffusionEquations3D][c.ndims](nu=c.nu)
     elif c.equation == "euler":
         eq = [None, th.CompressibleEulerEquations1D, th.CompressibleEulerEquations2D,
               th.CompressibleEulerEquations3D][c.ndims](c.gamma)
@@ -110,15 +116,22 @@ def product_semi(th, c: Case, precision=None):
            "smooth_periodic": th.initial_condition_smooth_periodic(c.seed),
            "weak_blast_wave_seeded": th.initial_condition_weak_blast_wave_seeded(c.seed),
            "shock": th.initial_condition_shock, "rarefaction": th.initial_condition_rarefaction,
-           "eoc_gravity": th.initial_condition_eoc_test_coupled_euler_gravity}
+           "eoc_gravity": th.initial_condition_eoc_test_coupled_euler_gravity,
+           "poisson_nonperiodic": th.initial_condition_poisson_nonperiodic,
+           "harmonic_nonperiodic": th.initial_condition_harmonic_nonperiodic}
     kw = {}
     if c.bc_state:
-        bc = th.BoundaryConditionDirichlet(ics[c.bc_state])
-        kw["boundary_conditions"] = {s: (th.boundary_condition_outflow if s in c.outflow else bc)
-                                     for s in SIDES[:2 * c.ndims]}
+        bc = (th.boundary_condition_poisson_nonperiodic if c.bc_state == "poisson_nonperiodic"
+              else th.BoundaryConditionDirichlet(ics[c.bc_state]))
+        per = c.periodic if isinstance(c.periodic, tuple) else (c.periodic,) * c.ndims
+        kw["boundary_conditions"] = {s: (th.boundary_condition_periodic if per[i // 2] else
+                                         th.boundary_condition_outflow if s in c.outflow else bc)
+                                     for i, s in enumerate(SIDES[:2 * c.ndims])}
     if c.source:
         kw["source_terms"] = {"convergence_test": th.source_terms_convergence_test,
-                              "eoc_test_euler": th.source_terms_eoc_test_euler}[c.source]
+                              "eoc_test_euler": th.source_terms_eoc_test_euler,
+                              "poisson_nonperiodic": th.source_terms_poisson_nonperiodic,
+                              "harmonic": th.source_terms_harmonic}[c.source]
     return th.SemidiscretizationHyperbolicHIP(mesh, eq, ics[c.ic], solver, **kw)
 
 

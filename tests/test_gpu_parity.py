@@ -184,6 +184,22 @@ CASES = [
     # test/tree_dgsem_3d/euler_shock.jl: HG with flux_ranocha as DG, FV and surface flux
     Case("euler3d_hg_ranocha_ranocha_blast", 3, "euler", 3, "hg", "ranocha", "ranocha", "ranocha",
          ic="weak_blast_wave"),
+    # hyperbolic diffusion (test/tree_dgsem_{1,2,3}d/hypdiff_nonperiodic.jl: polydeg 4,
+    # LLF, the Poisson steady state on the x boundaries, periodic elsewhere,
+    # source_terms_poisson_nonperiodic; 1D hypdiff_harmonic_nonperiodic.jl: nu 1.25,
+    # polydeg 3, the harmonic state as Dirichlet state, source_terms_harmonic)
+    Case("hypdiff1d_poisson", 1, "hypdiff", 3, polydeg=4, cmin=0.0, cmax=1.0, ic="poisson_nonperiodic",
+         periodic=False, bc_state="poisson_nonperiodic", source="poisson_nonperiodic"),
+    Case("hypdiff2d_poisson", 2, "hypdiff", 3, polydeg=4, cmin=0.0, cmax=1.0, ic="poisson_nonperiodic",
+         periodic=(False, True), bc_state="poisson_nonperiodic", source="poisson_nonperiodic"),
+    Case("hypdiff3d_poisson", 3, "hypdiff", 2, polydeg=4, cmin=0.0, cmax=1.0, ic="poisson_nonperiodic",
+         periodic=(False, True, True), bc_state="poisson_nonperiodic", source="poisson_nonperiodic"),
+    # (a random state: at the steady state du is only the discretisation residual)
+    Case("hypdiff2d_poisson_random_t", 2, "hypdiff", 3, polydeg=4, cmin=0.0, cmax=1.0,
+         periodic=(False, True), bc_state="poisson_nonperiodic", source="poisson_nonperiodic", t=0.5),
+    Case("hypdiff1d_harmonic_bc_random", 1, "hypdiff", 2, cmin=-1.0, cmax=2.0, periodic=False,
+         bc_state="harmonic_nonperiodic", source="harmonic", nu=1.25, t=0.3),
+    Case("hypdiff3d_random_periodic", 3, "hypdiff", 2, polydeg=3, cmin=0.0, cmax=1.0),
 ]
 
 
@@ -246,7 +262,10 @@ def test_pack_unpack_roundtrip_bit_exact(th, oracle, torch):
                                   Case("burgers1d", 1, "burgers", 4, cmin=0.0, cmax=1.0, ic="convergence_test",
                                        source="convergence_test"),
                                   Case("mhd1d_alfven", 1, "mhd", 3, "fd", "hindenlang_gassner", cmin=0.0, cmax=1.0,
-                                       ic="convergence_test", gamma=5 / 3)],
+                                       ic="convergence_test", gamma=5 / 3),
+                                  Case("hypdiff2d_poisson", 2, "hypdiff", 3, polydeg=4, cmin=0.0, cmax=1.0,
+                                       ic="poisson_nonperiodic", periodic=(False, True),
+                                       bc_state="poisson_nonperiodic", source="poisson_nonperiodic")],
                          ids=lambda c: c.name)
 def test_initial_condition_and_callbacks_match_oracle(th, oracle, torch, case):
     O = oracle
@@ -321,7 +340,10 @@ def test_indicator_matches_oracle(th, oracle, torch, case):
                                   Case("mhd1d_alfven", 1, "mhd", 3, "fd", "hindenlang_gassner", cmin=0.0, cmax=1.0,
                                        ic="convergence_test", gamma=5 / 3),
                                   Case("euler3d_weak_hll_eoc", 3, "euler", 2, sflux="hll", cmin=0.0, cmax=2.0,
-                                       ic="eoc_gravity", source="eoc_test_euler", gamma=2.0)],
+                                       ic="eoc_gravity", source="eoc_test_euler", gamma=2.0),
+                                  Case("hypdiff2d_poisson", 2, "hypdiff", 3, polydeg=4, cmin=0.0, cmax=1.0,
+                                       ic="poisson_nonperiodic", periodic=(False, True), bc_state="poisson_nonperiodic",
+                                       source="poisson_nonperiodic")],
                          ids=lambda c: c.name)
 def test_rk_step_matches_oracle(th, oracle, torch, case, use_graph):
     """Three CK2N54 steps from t = 0 (stage times through device memory: the

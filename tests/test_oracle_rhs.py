@@ -664,3 +664,42 @@ def test_eoc_test_euler_source_is_manufactured(oracle, nd):
         errs.append(np.abs(O.rhs(cfg, mesh, u, t=t) - ut).max())
     rate = np.log2(errs[0] / errs[1])
     assert rate > 2.5, (errs, rate)
+
+
+@pytest.mark.parametrize("nd", [1, 2, 3])
+def test_hypdiff_poisson_steady_state_is_an_equilibrium(oracle, nd):
+    """Hyperbolic diffusion with source_terms_poisson_nonperiodic and the
+    steady solution on the x boundaries (periodic in the other directions):
+    rhs!(u_steady) -> 0 at the DG rate (polydeg 4: about h^4); a wrong sign
+    in the flux, the relaxation or f leaves an O(1) residual."""
+    O = oracle
+    per = (False,) + (True,) * (nd - 1)
+    cfg = O.make_cfg(nd, O.EQ_HYPDIFF, 4, O.VI_WEAK, O.FLUX_CENTRAL, O.FLUX_LLF,
+                     boundary_condition=O.BC_DIRICHLET, boundary_ic=O.IC_POISSON_NONPERIODIC,
+                     source_terms=O.SOURCE_POISSON_NONPERIODIC)
+    errs = []
+    for lvl in ((3, 4) if nd < 3 else (2, 3)):
+        mesh = O.Mesh(nd, lvl, 0.0, 1.0, 4, periodic=per)
+        u = O.compute_coefficients(cfg, mesh, O.IC_POISSON_NONPERIODIC, 1.0)  # the steady state (t != 0)
+        errs.append(np.abs(O.rhs(cfg, mesh, u, t=0.0)).max())
+    rate = np.log2(errs[0] / errs[1])
+    assert rate > 3.3, (errs, rate)
+
+
+def test_hypdiff_harmonic_and_llf(oracle):
+    """The harmonic state phi = 3 + e x, q = e is an exact equilibrium of
+    hyperbolic diffusion (nu = 1.25) with source_terms_harmonic and itself as
+    the Dirichlet state; LLF's wave speed is the constant sqrt(nu / T_r) = nu / Lr."""
+    O = oracle
+    cfg = O.make_cfg(1, O.EQ_HYPDIFF, 3, O.VI_WEAK, O.FLUX_CENTRAL, O.FLUX_LLF, nu=1.25,
+                     boundary_condition=O.BC_DIRICHLET, boundary_ic=O.IC_HARMONIC_NONPERIODIC,
+                     source_terms=O.SOURCE_HARMONIC)
+    mesh = O.Mesh(1, 2, -1.0, 2.0, 3, periodic=False)
+    u = O.compute_coefficients(cfg, mesh, O.IC_HARMONIC_NONPERIODIC, 0.5)
+    assert np.abs(O.rhs(cfg, mesh, u, t=0.5)).max() < 1e-10  # terms of size phi / T_r * invJ ~ 1e3
+    ul, ur = np.array([0.3, -0.2]), np.array([1.1, 0.4])
+    lam = 1.25 * 2 * np.pi
+    fl, fr = O.flux(cfg, ul, 1), O.flux(cfg, ur, 1)
+    np.testing.assert_allclose(fl, [-1.25 * -0.2, -0.3 * lam ** 2 / 1.25], rtol=1e-14)
+    np.testing.assert_allclose(O.two_point_flux(cfg, O.FLUX_LLF, ul, ur, 1), 0.5 * (fl + fr) - 0.5 * lam * (ur - ul),
+                               rtol=1e-14)

@@ -21,7 +21,10 @@ from .equations import (BoundaryConditionDirichlet, boundary_condition_periodic,
                         LinearScalarAdvectionEquation2D, LinearScalarAdvectionEquation3D, density, density_pressure, pressure,
                         InviscidBurgersEquation1D, first, flux_ec, initial_condition_shock, FluxHLL, flux_hll,
                         min_max_speed_davis, initial_condition_eoc_test_coupled_euler_gravity,
-                        source_terms_eoc_test_euler,
+                        source_terms_eoc_test_euler, HyperbolicDiffusionEquations1D, HyperbolicDiffusionEquations2D,
+                        HyperbolicDiffusionEquations3D, initial_condition_poisson_nonperiodic,
+                        initial_condition_harmonic_nonperiodic, source_terms_poisson_nonperiodic,
+                        source_terms_harmonic, boundary_condition_poisson_nonperiodic,
                         initial_condition_rarefaction, boundary_condition_outflow,
                         flux_central, flux_hindenlang_gassner, flux_lax_friedrichs, flux_nonconservative_powell,
                         flux_ranocha, flux_shima_etal, initial_condition_constant,

@@ -11,14 +11,15 @@ LIBRARY_PATH = os.environ.get("TRIXI_HIP_LIBRARY") or os.path.join(HERE, "libtri
 
 OK, EINVAL, EUNSUPPORTED, EDEVICE, ENOMEM, ESTATE = 0, -1, -2, -3, -4, -5
 
-EQ_LINADV, EQ_EULER, EQ_GLM_MHD, EQ_BURGERS = 0, 1, 2, 3
+EQ_LINADV, EQ_EULER, EQ_GLM_MHD, EQ_BURGERS, EQ_HYPDIFF = 0, 1, 2, 3, 4
 VI_WEAK, VI_FD, VI_HG = 0, 1, 2
 FLUX_CENTRAL, FLUX_LLF, FLUX_LLF_NAIVE, FLUX_RANOCHA, FLUX_SHIMA_ETAL, FLUX_HINDENLANG_GASSNER, FLUX_EC, FLUX_HLL = range(8)
 PREC_FP64, PREC_MIXED = 0, 1
 BC_PERIODIC, BC_DIRICHLET = 0, 1
-SOURCE_NONE, SOURCE_CONVERGENCE_TEST, SOURCE_EOC_TEST_EULER = 0, 1, 2
+SOURCE_NONE, SOURCE_CONVERGENCE_TEST, SOURCE_EOC_TEST_EULER, SOURCE_POISSON_NONPERIODIC, SOURCE_HARMONIC = range(5)
 (IC_CONSTANT, IC_CONVERGENCE_TEST, IC_WEAK_BLAST_WAVE, IC_SEEDED_RANDOM, IC_SMOOTH_PERIODIC,
- IC_WEAK_BLAST_WAVE_SEEDED, IC_SHOCK, IC_RAREFACTION, IC_EOC_TEST_COUPLED_EULER_GRAVITY) = range(9)
+ IC_WEAK_BLAST_WAVE_SEEDED, IC_SHOCK, IC_RAREFACTION, IC_EOC_TEST_COUPLED_EULER_GRAVITY, IC_POISSON_NONPERIODIC,
+ IC_HARMONIC_NONPERIODIC) = range(11)
 
 
 class TrixiHipError(RuntimeError):
@@ -50,6 +51,8 @@ class Config(ctypes.Structure):
         ("boundary_seed", ctypes.c_uint64),
         ("source_terms", ctypes.c_int32),
         ("boundary_outflow_faces", ctypes.c_int32),
+        ("nu", ctypes.c_double),
+        ("Lr", ctypes.c_double),
     ]
 
 

@@ -50,10 +50,56 @@ __device__ inline double smooth_field(uint64_t seed, int field, int nd, const do
     return s;
 }
 
+// Hyperbolic diffusion states (phi, q_1..d): initial_condition_poisson_nonperiodic
+// is the steady solution of -lap phi = f with q = grad phi,
+//   1D: phi = sin(pi x);  2D: 2 cos(pi x) sin(2 pi y) + 2;
+//   3D: 2 cos(pi x) sin(2 pi y) sin(2 pi z) + 2,
+// and at t = 0 the initial guess phi = q = 1; as a boundary state (steady =
+// true) it is always the steady solution.  The harmonic state of the 1D test
+// is phi = 5, q = 0 at t = 0, else phi = 3 + e x, q = e.
+__host__ __device__ inline void hypdiff_state(const IcParams& q, const double* x, double t, bool steady, double* u) {
+    const int d = q.dim;
+    if (q.ic == TRIXI_HIP_IC_POISSON_NONPERIODIC) {
+        if (t == 0.0 && !steady) {
+            for (int k = 0; k <= d; ++k) u[k] = 1.0;
+            return;
+        }
+        if (d == 1) {
+            u[0] = sin(M_PI * x[0]);
+            u[1] = M_PI * cos(M_PI * x[0]);
+        } else {
+            const double sx = sin(M_PI * x[0]), cx = cos(M_PI * x[0]);
+            const double sy = sin(2.0 * M_PI * x[1]), cy = cos(2.0 * M_PI * x[1]);
+            const double sz = d == 3 ? sin(2.0 * M_PI * x[2]) : 1.0, cz = d == 3 ? cos(2.0 * M_PI * x[2]) : 0.0;
+            u[0] = 2.0 * cx * sy * sz + 2.0;
+            u[1] = -2.0 * M_PI * sx * sy * sz;
+            u[2] = 4.0 * M_PI * cx * cy * sz;
+            if (d == 3) u[3] = 4.0 * M_PI * cx * sy * cz;
+        }
+        return;
+    }
+    if (q.ic == TRIXI_HIP_IC_HARMONIC_NONPERIODIC) {
+        const double A = 3.0, B = exp(1.0);
+        u[0] = (t == 0.0) ? 5.0 : A + B * x[0];
+        for (int k = 1; k <= d; ++k) u[k] = (t == 0.0 || k > 1) ? 0.0 : B;
+        return;
+    }
+    u[0] = 1.0;
+    for (int k = 1; k <= d; ++k) u[k] = 0.0;
+}
+
 // Initial conditions (Trixi initial_condition_* functions used by the
 // reference tests and examples, plus the seeded benchmark states).
 __device__ inline void initial_condition(const IcParams& q, const double* x, double t, uint64_t idx,
                                   double* u) {
+    if (q.eq == EQ_HYPDIFF) {
+        if (q.ic == TRIXI_HIP_IC_SEEDED_RANDOM) {
+            for (int k = 0; k <= q.dim; ++k) u[k] = 1.0 + 0.5 * (2.0 * u01(q.seed, idx, k) - 1.0);
+            return;
+        }
+        hypdiff_state(q, x, t, false, u);
+        return;
+    }
     if (q.eq == EQ_BURGERS) {
         if (q.ic == TRIXI_HIP_IC_CONVERGENCE_TEST) {
             // c + A sin(omega (x - t)), c = 2, A = 1, omega = 2 pi
@@ -213,6 +259,10 @@ __device__ inline void initial_condition(const IcParams& q, const double* x, dou
 // and the constant state; for the Burgers equation its convergence-test wave
 // and the shock / rarefaction steps.  The library rejects other boundary states.
 __host__ __device__ inline void boundary_value(const IcParams& q, const double* x, double t, double* u) {
+    if (q.eq == EQ_HYPDIFF) {
+        hypdiff_state(q, x, t, q.ic == TRIXI_HIP_IC_POISSON_NONPERIODIC, u);
+        return;
+    }
     if (q.eq == EQ_BURGERS) {
         if (q.ic == TRIXI_HIP_IC_CONVERGENCE_TEST)
             u[0] = 2.0 + sin(2.0 * M_PI * (x[0] - t));
@@ -273,6 +323,25 @@ __host__ __device__ inline void source_eoc_test_euler(int dim, double gm1, const
     for (int d = 0; d < dim; ++d) s[1 + d] = (dim - 1) * gp + pp;
     const double ep = pp / gm1 + 0.5 * dim * gp;
     s[dim + 1] = (dim - 1) * ep + dim * pp;
+}
+
+// Hyperbolic diffusion sources: source_terms_poisson_nonperiodic
+// (f = -nu lap phi of the steady state: nu pi^2 sin(pi x) in 1D,
+// 10 nu pi^2 cos(pi x) sin(2 pi y) in 2D, 18 nu pi^2 cos(pi x) sin(2 pi y) sin(2 pi z)
+// in 3D) and source_terms_harmonic (f = 0); both relax q: -q_k / T_r.
+__host__ __device__ inline void source_hypdiff(int dim, int src, double nu, double inv_Tr, const double* x,
+                                               const double* qv, double* s) {
+    double f = 0.0;
+    if (src == TRIXI_HIP_SOURCE_POISSON_NONPERIODIC) {
+        if (dim == 1)
+            f = nu * M_PI * M_PI * sin(M_PI * x[0]);
+        else if (dim == 2)
+            f = 10.0 * nu * M_PI * M_PI * cos(M_PI * x[0]) * sin(2.0 * M_PI * x[1]);
+        else
+            f = 18.0 * nu * M_PI * M_PI * cos(M_PI * x[0]) * sin(2.0 * M_PI * x[1]) * sin(2.0 * M_PI * x[2]);
+    }
+    s[0] = f;
+    for (int k = 0; k < dim; ++k) s[1 + k] = -inv_Tr * qv[k];
 }
 
 // source_terms_convergence_test of the inviscid Burgers equation: with

@@ -22,7 +22,7 @@
 
 namespace trixi_hip {
 
-enum Equation : int { EQ_LINADV = 0, EQ_EULER = 1, EQ_GLM_MHD = 2, EQ_BURGERS = 3 };
+enum Equation : int { EQ_LINADV = 0, EQ_EULER = 1, EQ_GLM_MHD = 2, EQ_BURGERS = 3, EQ_HYPDIFF = 4 };
 enum VolumeIntegral : int { VI_WEAK = 0, VI_FD = 1, VI_HG = 2 };
 enum Flux : int {
     FLUX_CENTRAL = 0,
@@ -38,17 +38,20 @@ enum Flux : int {
 struct PhysParams {
     double gamma, inv_gm1, gm1, c_h;
     double a[3];
+    double nu, inv_Tr;  // hyperbolic diffusion: diffusion constant, 1 / reference time
 };
 
 template <typename R>
 struct PhysR {
-    R gamma, inv_gm1, gm1, c_h, a[3];
+    R gamma, inv_gm1, gm1, c_h, a[3], nu, inv_Tr;
     __host__ __device__ static PhysR from(const PhysParams& p) {
         PhysR r;
         r.gamma = (R)p.gamma;
         r.inv_gm1 = (R)p.inv_gm1;
         r.gm1 = (R)p.gm1;
         r.c_h = (R)p.c_h;
+        r.nu = (R)p.nu;
+        r.inv_Tr = (R)p.inv_Tr;
         for (int d = 0; d < 3; ++d) r.a[d] = (R)p.a[d];
         return r;
     }
@@ -169,6 +172,72 @@ struct LinAdv {
     template <typename R>
     __device__ static void max_abs_speeds(const R* u, const PhysR<R>& P, R* lam) {
         for (int d = 0; d < DIM; ++d) lam[d] = dabs(P.a[d]);
+    }
+};
+
+// ===========================================================================
+// Hyperbolic diffusion (Trixi HyperbolicDiffusionEquations{1,2,3}D): (phi,
+// q_1..DIM), the first-order hyperbolic system phi_t - nu div q = f,
+// q_t - (grad phi - q) / T_r = 0 whose steady state solves -nu lap phi = f.
+// Flux in direction o: f_phi = -nu q_o, f_{q_o} = -phi / T_r, others 0; the
+// wave speed of LLF is the constant sqrt(nu / T_r).
+// ===========================================================================
+template <int DIM>
+struct HypDiff {
+    static constexpr int NV = DIM + 1;
+    static constexpr bool NONCONS = false;
+    static constexpr int NRF = DIM + 1;
+    template <typename R>
+    struct Node {
+        R u[DIM + 1];
+        double lrho, lp;  // unused
+        __device__ R cons(int q) const { return u[q]; }
+    };
+    template <typename R, bool LOGS>
+    __device__ static Node<R> node(const R* u, const PhysR<R>&) {
+        Node<R> n;
+#pragma unroll
+        for (int q = 0; q < NV; ++q) n.u[q] = u[q];
+        n.lrho = n.lp = 0.0;
+        return n;
+    }
+    template <typename R>
+    __device__ static void to_rec(const Node<R>& n, R* f) {
+#pragma unroll
+        for (int q = 0; q < NV; ++q) f[q] = n.u[q];
+    }
+    template <typename R>
+    __device__ static Node<R> from_rec(const R* f, double, double, const PhysR<R>&) {
+        Node<R> n;
+#pragma unroll
+        for (int q = 0; q < NV; ++q) n.u[q] = f[q];
+        n.lrho = n.lp = 0.0;
+        return n;
+    }
+    template <typename R>
+    __device__ static void flux(const Node<R>& n, int o, const PhysR<R>& P, R* f) {
+        f[0] = -P.nu * n.u[1 + o];
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) f[1 + d] = (d == o) ? -n.u[0] * P.inv_Tr : R(0);
+    }
+    template <int KIND, typename R>
+    __device__ static void two_point(const Node<R>& l, const Node<R>& r, int o, const PhysR<R>& P, R* f) {
+        R fl[NV], fr[NV];
+        flux(l, o, P, fl);
+        flux(r, o, P, fr);
+        if (KIND == FLUX_LLF || KIND == FLUX_LLF_NAIVE) {
+            const R lam = sqrt(P.nu * P.inv_Tr);
+#pragma unroll
+            for (int q = 0; q < NV; ++q) f[q] = R(0.5) * (fl[q] + fr[q]) + (R(-0.5) * lam * (r.u[q] - l.u[q]));
+        } else {
+#pragma unroll
+            for (int q = 0; q < NV; ++q) f[q] = R(0.5) * (fl[q] + fr[q]);
+        }
+    }
+    template <typename R>
+    __device__ static void max_abs_speeds(const R*, const PhysR<R>& P, R* lam) {
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) lam[d] = sqrt(P.nu * P.inv_Tr);
     }
 };
 

@@ -118,6 +118,17 @@ __device__ __forceinline__ void source_at(const RhsArgs& a, int64_t e, int node,
         sv[0] = source_burgers_convergence_test(xx[0], t);
         return;
     }
+    if (a.src == TRIXI_HIP_SOURCE_POISSON_NONPERIODIC || a.src == TRIXI_HIP_SOURCE_HARMONIC) {
+        // hyperbolic diffusion: the relaxation -q / T_r reads this node's q
+        double qv[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) qv[d] = a.u[((int64_t)(1 + d) * NN + node) * a.E + e];
+        double s4[4];
+        source_hypdiff(DIM, a.src, a.phys.nu, a.phys.inv_Tr, xx, qv, s4);
+#pragma unroll
+        for (int v = 0; v < NV && v < 4; ++v) sv[v] = s4[v];
+        return;
+    }
     double s5[5];
     if (a.src == TRIXI_HIP_SOURCE_EOC_TEST_EULER)
         source_eoc_test_euler(DIM, a.phys.gm1, xx, t, s5);

@@ -60,6 +60,7 @@ void register_mhd3d();
 void register_mhd2d();
 void register_burgers();
 void register_mhd1d();
+void register_hypdiff();
 void register_vjp_linadv();
 void register_vjp_euler1d();
 void register_vjp_euler2d();
@@ -83,6 +84,7 @@ static void ensure_registered() {
         register_mhd2d();
         register_burgers();
         register_mhd1d();
+        register_hypdiff();
         register_vjp_linadv();
         register_vjp_euler1d();
         register_vjp_euler2d();
@@ -741,7 +743,7 @@ __global__ void check_state_kernel(const double* __restrict__ u, int32_t* __rest
                 uc[v] = u[((int64_t)v * nn + n) * E + e];
                 if (!isfinite(uc[v])) f |= 1;
             }
-            if (eq == EQ_LINADV || eq == EQ_BURGERS) continue;
+            if (eq == EQ_LINADV || eq == EQ_BURGERS || eq == EQ_HYPDIFF) continue;
             if (!(uc[0] > 0.0)) f |= 2;
             double ke = 0.0;
             const int nvel = (eq == EQ_EULER) ? dim : 3;
@@ -835,6 +837,7 @@ static const double CK_C[5] = {0.0, 1432997174477.0 / 9575080441755.0, 252626934
 static int nvars_of(const trixi_hip_config& c) {
     if (c.equation == TRIXI_HIP_LINEAR_SCALAR_ADVECTION || c.equation == TRIXI_HIP_INVISCID_BURGERS) return 1;
     if (c.equation == TRIXI_HIP_COMPRESSIBLE_EULER) return c.ndims + 2;
+    if (c.equation == TRIXI_HIP_HYPERBOLIC_DIFFUSION) return c.ndims + 1;
     return c.ndims == 1 ? 8 : 9;  // ideal MHD: no cleaning field in 1D
 }
 
@@ -1012,19 +1015,33 @@ int trixi_hip_create(const trixi_hip_config* config, const trixi_hip_mesh* mesh,
             return fail(TRIXI_HIP_EINVAL, "unknown boundary condition");
         const bool euler_ok = cfg.equation == TRIXI_HIP_COMPRESSIBLE_EULER &&
                               (cfg.boundary_ic == TRIXI_HIP_IC_CONVERGENCE_TEST || cfg.boundary_ic == TRIXI_HIP_IC_CONSTANT);
+        const bool hypdiff_ok = cfg.equation == TRIXI_HIP_HYPERBOLIC_DIFFUSION &&
+                                (cfg.boundary_ic == TRIXI_HIP_IC_POISSON_NONPERIODIC ||
+                                 cfg.boundary_ic == TRIXI_HIP_IC_HARMONIC_NONPERIODIC ||
+                                 cfg.boundary_ic == TRIXI_HIP_IC_CONSTANT);
         const bool burgers_ok = cfg.equation == TRIXI_HIP_INVISCID_BURGERS &&
                                 (cfg.boundary_ic == TRIXI_HIP_IC_CONVERGENCE_TEST || cfg.boundary_ic == TRIXI_HIP_IC_CONSTANT ||
                                  cfg.boundary_ic == TRIXI_HIP_IC_SHOCK || cfg.boundary_ic == TRIXI_HIP_IC_RAREFACTION);
-        if (!euler_ok && !burgers_ok)
+        if (!euler_ok && !burgers_ok && !hypdiff_ok)
             return fail(TRIXI_HIP_EUNSUPPORTED,
                         "Dirichlet boundaries are built for compressible Euler (convergence-test or constant state) "
-                        "and Burgers (convergence-test wave, shock, rarefaction, constant)");
+                        "Burgers (convergence-test wave, shock, rarefaction, constant) and hyperbolic "
+                        "diffusion (Poisson steady state, harmonic state, constant)");
         if (cfg.boundary_outflow_faces < 0 || cfg.boundary_outflow_faces >= (1 << (2 * cfg.ndims)))
             return fail(TRIXI_HIP_EINVAL, "outflow face bits beyond the mesh's faces");
     } else if (cfg.boundary_outflow_faces != 0) {
         return fail(TRIXI_HIP_EINVAL, "outflow faces need a non-periodic boundary configuration");
     }
-    if (cfg.source_terms == TRIXI_HIP_SOURCE_EOC_TEST_EULER) {
+    if (cfg.equation == TRIXI_HIP_HYPERBOLIC_DIFFUSION) {
+        if (!(cfg.nu > 0.0) || !(cfg.Lr > 0.0))
+            return fail(TRIXI_HIP_EINVAL, "hyperbolic diffusion needs nu > 0 and Lr > 0");
+        if (cfg.source_terms != TRIXI_HIP_SOURCE_NONE && cfg.source_terms != TRIXI_HIP_SOURCE_POISSON_NONPERIODIC &&
+            cfg.source_terms != TRIXI_HIP_SOURCE_HARMONIC)
+            return fail(TRIXI_HIP_EUNSUPPORTED, "hyperbolic diffusion takes the Poisson or harmonic source terms");
+    } else if (cfg.source_terms == TRIXI_HIP_SOURCE_POISSON_NONPERIODIC ||
+               cfg.source_terms == TRIXI_HIP_SOURCE_HARMONIC) {
+        return fail(TRIXI_HIP_EUNSUPPORTED, "these source terms are built for hyperbolic diffusion");
+    } else if (cfg.source_terms == TRIXI_HIP_SOURCE_EOC_TEST_EULER) {
         if (cfg.equation != TRIXI_HIP_COMPRESSIBLE_EULER || cfg.ndims == 1)
             return fail(TRIXI_HIP_EUNSUPPORTED, "source_terms_eoc_test_euler is built for compressible Euler 2D/3D");
     } else if (cfg.source_terms != TRIXI_HIP_SOURCE_NONE) {
@@ -1205,6 +1222,10 @@ int trixi_hip_create(const trixi_hip_config* config, const trixi_hip_mesh* mesh,
     a.phys.gm1 = cfg.gamma - 1.0;
     a.phys.inv_gm1 = (cfg.gamma != 1.0) ? 1.0 / (cfg.gamma - 1.0) : 0.0;
     a.phys.c_h = (cfg.equation == TRIXI_HIP_IDEAL_GLM_MHD && dim == 1) ? 0.0 : cfg.c_h;
+    if (cfg.equation == TRIXI_HIP_HYPERBOLIC_DIFFUSION) {
+        a.phys.nu = cfg.nu;
+        a.phys.inv_Tr = cfg.nu / (cfg.Lr * cfg.Lr);  // T_r = Lr^2 / nu
+    }
     for (int d = 0; d < 3; ++d) a.phys.a[d] = cfg.advection_velocity[d];
     a.bc_ic = ic_params(c, cfg.boundary_ic, cfg.boundary_seed);
     for (int q = 0; q < N * N; ++q) {
@@ -1453,6 +1474,8 @@ int trixi_hip_max_dt(trixi_hip_ctx* ctx, const double* u, double* max_dt, trixi_
         if (dim == 1) MS(Euler<1>, 1); else if (dim == 2) MS(Euler<2>, 2); else MS(Euler<3>, 3);
     } else if (eq == TRIXI_HIP_INVISCID_BURGERS) {
         MS(Burgers, 1);
+    } else if (eq == TRIXI_HIP_HYPERBOLIC_DIFFUSION) {
+        if (dim == 1) MS(HypDiff<1>, 1); else if (dim == 2) MS(HypDiff<2>, 2); else MS(HypDiff<3>, 3);
     } else {
         if (dim == 1) MS(GlmMhd1D, 1); else if (dim == 2) MS(GlmMhd, 2); else MS(GlmMhd, 3);
     }

@@ -6,6 +6,7 @@ enum value the library understands (reference: the fluxes the tests pass to
 DGSEMGPU, e.g. test/tree_dgsem_3d/euler_ec.jl:13-17,
 test/tree_dgsem_3d/mhd_alfven_wave.jl:13-18, test/tree_dgsem_2d/euler_shock.jl:12-27).
 """
+import math
 from dataclasses import dataclass, field
 
 from . import _lib as L
@@ -23,6 +24,8 @@ class AbstractEquations:
     def nvariables(self):
         if self.equation_id in (L.EQ_LINADV, L.EQ_BURGERS):
             return 1
+        if self.equation_id == L.EQ_HYPDIFF:
+            return self.ndims + 1
         if self.equation_id == L.EQ_EULER:
             return self.ndims + 2
         return 9 if self.ndims > 1 else 8  # ideal MHD: no cleaning field in 1D
@@ -89,6 +92,29 @@ def _glm_mhd(ndims):
 IdealGlmMhdEquations1D = _glm_mhd(1)  # (rho, rho v1..3, rho e, B1..3); no psi, no Powell terms
 IdealGlmMhdEquations2D = _glm_mhd(2)
 IdealGlmMhdEquations3D = _glm_mhd(3)
+
+
+def _hypdiff(ndims):
+    @dataclass(frozen=True)
+    class _HypDiff(AbstractEquations):
+        """(phi, q_1..d): phi_t - nu div q = f, q_t = (grad phi - q) / T_r,
+        T_r = Lr^2 / nu (Trixi HyperbolicDiffusionEquations{1,2,3}D)."""
+        nu: float = 1.0
+        Lr: float = 1.0 / (2.0 * 3.141592653589793)
+
+        def __init__(self, nu=1.0, Lr=None):
+            object.__setattr__(self, "ndims", ndims)
+            object.__setattr__(self, "equation_id", L.EQ_HYPDIFF)
+            object.__setattr__(self, "nu", float(nu))
+            object.__setattr__(self, "Lr", float(Lr) if Lr is not None else 1.0 / (2.0 * math.pi))
+
+    _HypDiff.__name__ = _HypDiff.__qualname__ = f"HyperbolicDiffusionEquations{ndims}D"
+    return _HypDiff
+
+
+HyperbolicDiffusionEquations1D = _hypdiff(1)
+HyperbolicDiffusionEquations2D = _hypdiff(2)
+HyperbolicDiffusionEquations3D = _hypdiff(3)
 
 
 @dataclass(frozen=True)
@@ -186,6 +212,13 @@ initial_condition_rarefaction = InitialCondition("initial_condition_rarefaction"
 # compressible Euler 2D/3D: rho = 2 + 0.1 sin(pi (sum x - t)), v = 1, p = 2 rho^2 / (d pi) (gamma = 2)
 initial_condition_eoc_test_coupled_euler_gravity = InitialCondition(
     "initial_condition_eoc_test_coupled_euler_gravity", L.IC_EOC_TEST_COUPLED_EULER_GRAVITY)
+# hyperbolic diffusion: the initial guess phi = q = 1 at t = 0, else the steady
+# solution of -lap phi = f (1D sin(pi x); 2D / 3D 2 cos(pi x) prod sin(2 pi x_k) + 2)
+initial_condition_poisson_nonperiodic = InitialCondition("initial_condition_poisson_nonperiodic",
+                                                         L.IC_POISSON_NONPERIODIC)
+# test/tree_dgsem_1d/hypdiff_harmonic_nonperiodic.jl: phi = 5, q = 0 at t = 0, else 3 + e x, q = e
+initial_condition_harmonic_nonperiodic = InitialCondition("initial_condition_harmonic_nonperiodic",
+                                                          L.IC_HARMONIC_NONPERIODIC)
 
 
 def initial_condition_seeded_random(seed=20261202):
@@ -225,16 +258,28 @@ class _Outflow:
 boundary_condition_outflow = _Outflow()
 
 
+class _PoissonNonperiodic:
+    """Trixi's boundary_condition_poisson_nonperiodic: the steady Poisson state
+    outside the face at any time (hyperbolic diffusion)."""
+
+    def __repr__(self):
+        return "boundary_condition_poisson_nonperiodic"
+
+
+boundary_condition_poisson_nonperiodic = _PoissonNonperiodic()
+
+
 class BoundaryConditionDirichlet:
     """Trixi's BoundaryConditionDirichlet(boundary_value_function): the state
     boundary_value_function(x, t) outside the face; flux(u_inner, u_b) on +
     faces, flux(u_b, u_inner) on - faces.  Built for compressible Euler with
     initial_condition_convergence_test or initial_condition_constant."""
 
-    def __init__(self, boundary_value_function):
+    def __init__(self, boundary_value_function, _steady=False):
         if not isinstance(boundary_value_function, InitialCondition):
             raise NotImplementedError("the boundary state must be one of the built initial conditions")
         self.boundary_value_function = boundary_value_function
+        self._steady = _steady  # set by boundary_condition_poisson_nonperiodic
 
     def __repr__(self):
         return f"BoundaryConditionDirichlet({self.boundary_value_function})"
@@ -254,6 +299,9 @@ class SourceTerms:
 source_terms_convergence_test = SourceTerms("source_terms_convergence_test", L.SOURCE_CONVERGENCE_TEST)
 # the manufactured source of initial_condition_eoc_test_coupled_euler_gravity (Euler 2D/3D)
 source_terms_eoc_test_euler = SourceTerms("source_terms_eoc_test_euler", L.SOURCE_EOC_TEST_EULER)
+# hyperbolic diffusion: f = -nu lap phi of the Poisson steady state, and -q / T_r
+source_terms_poisson_nonperiodic = SourceTerms("source_terms_poisson_nonperiodic", L.SOURCE_POISSON_NONPERIODIC)
+source_terms_harmonic = SourceTerms("source_terms_harmonic", L.SOURCE_HARMONIC)
 
 
 def initial_condition_smooth_periodic(seed=20261202):

@@ -40,6 +40,8 @@ struct CConfig
     boundary_seed::UInt64
     source_terms::Int32
     boundary_outflow_faces::Int32
+    nu::Float64
+    Lr::Float64
 end
 
 struct CMesh
@@ -75,6 +77,8 @@ equation_id(::IdealGlmMhdEquations3D) = 2
 equation_id(::IdealGlmMhdEquations2D) = 2
 equation_id(::IdealGlmMhdEquations1D) = 2
 equation_id(::InviscidBurgersEquation1D) = 3
+equation_id(::Union{HyperbolicDiffusionEquations1D, HyperbolicDiffusionEquations2D,
+                    HyperbolicDiffusionEquations3D}) = 4
 equation_id(eq) = throw(ArgumentError("equation $(typeof(eq)) is not built for HIP"))
 
 flux_id(::typeof(flux_central)) = 0
@@ -99,6 +103,12 @@ initial_condition_rarefaction(x, t, ::InviscidBurgersEquation1D) = SVector(x[1] 
 ic_id(::typeof(initial_condition_shock)) = 6
 ic_id(::typeof(initial_condition_rarefaction)) = 7
 ic_id(::typeof(initial_condition_eoc_test_coupled_euler_gravity)) = 8
+ic_id(::typeof(initial_condition_poisson_nonperiodic)) = 9
+# the harmonic state of test/tree_dgsem_1d/hypdiff_harmonic_nonperiodic.jl
+function initial_condition_harmonic_nonperiodic(x, t, ::HyperbolicDiffusionEquations1D)
+    t == 0 ? SVector(5.0, 0.0) : SVector(3.0 + exp(1.0) * x[1], exp(1.0))
+end
+ic_id(::typeof(initial_condition_harmonic_nonperiodic)) = 10
 ic_id(f) = throw(ArgumentError("initial condition $f is not built for HIP"))
 # seeded benchmark states (ids 3, 4, 5 of trixi_hip_initial_condition) carry their seed
 struct SeededInitialCondition
@@ -153,6 +163,9 @@ function boundary_config(bcs)
             b === boundary_condition_outflow && (outflow |= 1 << (findfirst(==(k), names) - 1))
         end
     end
+    # boundary_condition_poisson_nonperiodic: the steady Poisson state at any time
+    sides = [b === boundary_condition_poisson_nonperiodic ?
+             BoundaryConditionDirichlet(initial_condition_poisson_nonperiodic) : b for b in sides]
     dir = [b for b in sides if b !== boundary_condition_periodic && b !== boundary_condition_outflow]
     all(b -> b isa BoundaryConditionDirichlet, dir) ||
         throw(ArgumentError("only BoundaryConditionDirichlet, outflow and periodic boundaries are built for HIP"))
@@ -172,6 +185,8 @@ indicator_variable_id(f) = throw(ArgumentError("indicator variable $f is not bui
 source_id(::Nothing) = 0
 source_id(::typeof(source_terms_convergence_test)) = 1
 source_id(::typeof(source_terms_eoc_test_euler)) = 2
+source_id(::typeof(source_terms_poisson_nonperiodic)) = 3
+source_id(::typeof(source_terms_harmonic)) = 4
 source_id(f) = throw(ArgumentError("source terms $f are not built for HIP"))
 
 function make_config(solver::DGSEMHIP, equations, mesh; boundary_conditions = boundary_condition_periodic,
@@ -194,7 +209,9 @@ function make_config(solver::DGSEMHIP, equations, mesh; boundary_conditions = bo
     bc, bc_ic, bc_seed, outflow = boundary_config(boundary_conditions)
     CConfig(ndims(mesh), equation_id(equations), solver.polydeg, volume_integral, vflux,
             flux_id(solver.surface_flux), fvflux, solver.precision === :mixed ? 1 : 0,
-            smooth, var, gamma, c_h, vel, amax, amin, len, bc, bc_ic, bc_seed, source_id(source_terms), outflow)
+            smooth, var, gamma, c_h, vel, amax, amin, len, bc, bc_ic, bc_seed, source_id(source_terms), outflow,
+            hasproperty(equations, :nu) ? Float64(equations.nu) : 0.0,
+            hasproperty(equations, :Lr) ? Float64(equations.Lr) : 0.0)
 end
 
 function SemidiscretizationHyperbolicHIP(mesh::TreeMesh, equations, initial_condition,
@@ -352,6 +369,7 @@ end
 
 export DGSEMHIP, SemidiscretizationHyperbolicHIP, semidiscretizeHIP, rhs_hip!, rhs_hip,
        initial_condition_shock, initial_condition_rarefaction, boundary_condition_outflow,
+       initial_condition_harmonic_nonperiodic,
        max_dt_hip, calc_error_norms_hip, rk_step_hip!, rk_stage_hip!, hip_synchronize,
        rk_step_vjp_hip!, gradient_checkpointed
 

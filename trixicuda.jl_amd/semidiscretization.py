@@ -13,7 +13,8 @@ import numpy as np
 
 from . import _lib as L
 from .equations import (BoundaryConditionDirichlet, SourceTerms, boundary_condition_outflow,
-                        boundary_condition_periodic, initial_condition_constant)
+                        boundary_condition_periodic, boundary_condition_poisson_nonperiodic,
+                        initial_condition_constant, initial_condition_poisson_nonperiodic)
 
 
 def _torch():
@@ -97,6 +98,11 @@ class SemidiscretizationHyperbolicHIP:
         for f_dir, b in enumerate(sides):
             if b is boundary_condition_outflow:
                 outflow |= 1 << f_dir
+        if any(isinstance(b, BoundaryConditionDirichlet) and
+               b.boundary_value_function is initial_condition_poisson_nonperiodic for b in sides):
+            raise NotImplementedError("use boundary_condition_poisson_nonperiodic (the steady state at any time)")
+        sides = [BoundaryConditionDirichlet(initial_condition_poisson_nonperiodic, _steady=True)
+                 if b is boundary_condition_poisson_nonperiodic else b for b in sides]
         dirichlet = [b for b in sides if b is not boundary_condition_periodic and b is not boundary_condition_outflow]
         if not all(isinstance(b, BoundaryConditionDirichlet) for b in dirichlet):
             raise NotImplementedError("only BoundaryConditionDirichlet, boundary_condition_outflow and periodic "

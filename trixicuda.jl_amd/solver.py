@@ -128,6 +128,8 @@ class DGSEMHIP:
         for d in range(3):
             c.advection_velocity[d] = vel[d] if d < len(vel) else 0.0
         c.ic_length = ic_length
+        c.nu = getattr(equations, "nu", 0.0)
+        c.Lr = getattr(equations, "Lr", 0.0)
         return c
 
     def __repr__(self):
