@@ -274,7 +274,9 @@ int trixi_hip_integrate(trixi_hip_ctx* ctx, const double* u, double* integral,
 
 /* Hennemann-Gassner blending factors (after smoothing) into the device array
  * alpha[nelements], in FP64.  Replaces the CPU indicator callable and its
- * D2H/H2D round trip (indicators.jl:7-42, dg_2d.jl:171-173). */
+ * D2H/H2D round trip (indicators.jl:7-42, dg_2d.jl:171-173).  Built for
+ * compressible Euler (any indicator variable), 2D/3D GLM-MHD (density_pressure)
+ * and Burgers (first); other equations return TRIXI_HIP_EUNSUPPORTED. */
 int trixi_hip_indicator(trixi_hip_ctx* ctx, const double* u, double* alpha,
                         trixi_hip_stream stream);
 

@@ -721,13 +721,17 @@ void flux_differencing_element(const Cfg& c, const Basis& b, int kind, const T* 
 
 template <class T>
 void fv_element(const Cfg& c, const Basis& b, const T* ue, T* due, T alpha) {
+    // Trixi calcflux_fv! / fv_kernel! (reference dg_2d_kernel.jl:646-654,
+    // 744-748): at the subcell interface between nodes idx - 1 and idx,
+    // fstar_L = f + nc(u_{idx-1}, u_idx)/2 (used by the left node) and
+    // fstar_R = f + nc(u_idx, u_{idx-1})/2 (used by the right node); without
+    // nonconservative terms both are f.  Zero at the element ends.
     const int N = c.N, nd = c.nd, nv = c.nv, nn = ipow(N, nd);
-    // fstar[d][v][pos along d 0..N][other node index] with zero end values.
-    std::vector<T> fstar((size_t)nd * nv * (N + 1) * nn, T(0.0));
-    auto F = [&](int d, int v, int pos, int n_other) -> T& {
-        return fstar[(((size_t)d * nv + v) * (N + 1) + pos) * nn + n_other];
+    std::vector<T> fl((size_t)nd * nv * (N + 1) * nn, T(0.0)), fr(fl.size(), T(0.0));
+    auto at = [&](int d, int v, int pos, int n_other) -> size_t {
+        return (((size_t)d * nv + v) * (N + 1) + pos) * nn + n_other;
     };
-    T f[MAXV];
+    T f[MAXV], ncl[MAXV], ncr[MAXV];
     for (int d = 0; d < nd; ++d) {
         const int s = ipow(N, d);
         for (int n = 0; n < nn; ++n) {
@@ -735,19 +739,22 @@ void fv_element(const Cfg& c, const Basis& b, const T* ue, T* due, T alpha) {
             if (idx == 0) continue;
             two_point(c, c.fvflux, ue + nv * (n - s), ue + nv * n, d + 1, f);
             const int base = n - idx * s;
-            for (int v = 0; v < nv; ++v) F(d, v, idx, base) = f[v];
+            if (has_noncons(c)) {
+                noncons_powell(c, ue + nv * (n - s), ue + nv * n, d + 1, ncl);
+                noncons_powell(c, ue + nv * n, ue + nv * (n - s), d + 1, ncr);
+            }
+            for (int v = 0; v < nv; ++v) {
+                fl[at(d, v, idx, base)] = has_noncons(c) ? f[v] + 0.5 * ncl[v] : f[v];
+                fr[at(d, v, idx, base)] = has_noncons(c) ? f[v] + 0.5 * ncr[v] : f[v];
+            }
         }
-    }
-    if (has_noncons(c)) {
-        // Nonconservative FV subcell fluxes are not part of the supported
-        // configurations (shock capturing is Euler-only here).
     }
     for (int n = 0; n < nn; ++n) {
         for (int v = 0; v < nv; ++v) {
             T acc = 0.0;
             for (int d = 0; d < nd; ++d) {
                 const int s = ipow(N, d), idx = (n / s) % N, base = n - idx * s;
-                T term = b.invw[idx] * (F(d, v, idx + 1, base) - F(d, v, idx, base));
+                T term = b.invw[idx] * (fl[at(d, v, idx + 1, base)] - fr[at(d, v, idx, base)]);
                 acc = (d == 0) ? term : acc + term;
             }
             due[nv * n + v] = due[nv * n + v] + alpha * acc;
@@ -763,6 +770,12 @@ void fv_element(const Cfg& c, const Basis& b, const T* ue, T* due, T alpha) {
 // rho p = (gamma - 1)(rho rho_e - |rho v|^2 / 2), pressure
 // p = (gamma - 1)(rho_e - |rho v|^2 / (2 rho)) or density rho.
 template <class T> T indicator_variable(const Cfg& c, const T* u) {
+    if (c.eq == OR_EQ_GLM_MHD) {
+        // Trixi density_pressure(u, ::IdealGlmMhdEquations): rho p
+        T rho, v[3], p, B[3], psi;
+        mhd_prim(c, u, rho, v, p, B, psi);
+        return rho * p;
+    }
     if (c.ivar == 2) return u[0];
     T ke = 0.0;
     for (int d = 0; d < c.nd; ++d) ke = (d == 0) ? u[1 + d] * u[1 + d] : ke + u[1 + d] * u[1 + d];
@@ -1387,6 +1400,24 @@ void initial_condition(const Cfg& c, int ic, const double* x, double t, uint64_t
         const double B1 = nx - rho * v1 * sqr, B2 = ny - rho * v2 * sqr, B3 = -rho * v3 * sqr;
         const double pr[9] = {rho, v1, v2, v3, p, B1, B2, B3, 0.0};
         memcpy(prim, pr, sizeof(pr));
+    } else if (ic == OR_IC_WEAK_BLAST_WAVE || ic == OR_IC_WEAK_BLAST_WAVE_SEEDED) {
+        // Trixi's MHD weak blast wave: the Euler blast (Hennemann & Gassner
+        // 2020) with B = (1, 1, 1) and psi = 0 (via the Euler state of the
+        // same dimension)
+        Cfg ce = c;
+        ce.eq = OR_EQ_EULER;
+        ce.nv = c.nd + 2;
+        double ue[5];
+        initial_condition(ce, ic, x, t, seed, idx, ue);
+        double ke = 0.0;
+        prim[0] = ue[0];
+        for (int d = 0; d < c.nd; ++d) {
+            prim[1 + d] = ue[1 + d] / ue[0];
+            ke += ue[1 + d] * prim[1 + d];
+        }
+        prim[4] = (c.gamma - 1.0) * (ue[c.nd + 1] - 0.5 * ke);
+        prim[5] = prim[6] = prim[7] = 1.0;
+        prim[8] = 0.0;
     } else if (ic == OR_IC_SEEDED_RANDOM) {
         prim[0] = 1.0 + 0.1 * (2.0 * u01(seed, idx, 0) - 1.0);
         prim[4] = 1.0 + 0.05 * (2.0 * u01(seed, idx, 1) - 1.0);

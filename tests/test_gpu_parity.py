@@ -42,6 +42,18 @@ def run_pair(th, O, torch, case, precision=None):
     return du, du_ref, u
 
 
+# test/tree_dgsem_{2,3}d/mhd_shock.jl as written: polydeg 4, level 3 on
+# [-2, 2]^d, weak blast wave, alpha_max 0.5, alpha_min 0.001, smoothing
+MHD_HG_P4 = [
+    Case("mhd2d_hg_ec_blast_p4", 2, "mhd", 3, "hg", "hindenlang_gassner", "hindenlang_gassner", "hindenlang_gassner",
+         polydeg=4, ic="weak_blast_wave", gamma=1.4),
+    Case("mhd3d_hg_ec_blast_p4", 3, "mhd", 3, "hg", "hindenlang_gassner", "hindenlang_gassner", "hindenlang_gassner",
+         polydeg=4, ic="weak_blast_wave", gamma=1.4),
+    Case("mhd3d_hg_ec_blast_seeded_p4", 3, "mhd", 2, "hg", "hindenlang_gassner", "hindenlang_gassner",
+         "hindenlang_gassner", polydeg=4, ic="weak_blast_wave_seeded", gamma=1.4),
+]
+
+
 CASES = [
     Case("linadv1d_weak_llf", 1, "linadv", 4, cmin=-1.0, cmax=1.0, ic="convergence_test"),
     Case("linadv2d_weak_llf", 2, "linadv", 3, ic="seeded_random"),
@@ -200,6 +212,17 @@ CASES = [
     Case("hypdiff1d_harmonic_bc_random", 1, "hypdiff", 2, cmin=-1.0, cmax=2.0, periodic=False,
          bc_state="harmonic_nonperiodic", source="harmonic", nu=1.25, t=0.3),
     Case("hypdiff3d_random_periodic", 3, "hypdiff", 2, polydeg=3, cmin=0.0, cmax=1.0),
+    # GLM-MHD shock capturing (test/tree_dgsem_{2,3}d/mhd_shock.jl: the EC flux +
+    # Powell terms as DG, FV and surface flux, indicator density_pressure,
+    # MHD weak blast wave)
+    Case("mhd2d_hg_ec_blast", 2, "mhd", 3, "hg", "hindenlang_gassner", "hindenlang_gassner", "hindenlang_gassner",
+         cmin=-2.0, cmax=2.0, ic="weak_blast_wave", gamma=1.4),
+    Case("mhd2d_hg_llf_blast_seeded", 2, "mhd", 3, "hg", "hindenlang_gassner", "llf", "llf", cmin=-2.0, cmax=2.0,
+         ic="weak_blast_wave_seeded", gamma=1.4, alpha_min=0.0),
+    Case("mhd3d_hg_ec_blast", 3, "mhd", 2, "hg", "hindenlang_gassner", "hindenlang_gassner", "hindenlang_gassner",
+         cmin=-2.0, cmax=2.0, ic="weak_blast_wave_seeded", gamma=1.4),
+    # the tests' own configuration: polydeg 4, level 3 on [-2, 2]^d
+    *MHD_HG_P4,
 ]
 
 
@@ -212,7 +235,7 @@ def test_rhs_fp64_matches_oracle(th, oracle, torch, case):
     assert err <= FP64_TOL, err
 
 
-MIXED_CASES = [c for c in CASES if c.vi in ("fd", "hg") and c.polydeg == 3] + \
+MIXED_CASES = [c for c in CASES if c.vi in ("fd", "hg") and c.polydeg == 3] + MHD_HG_P4 + \
     [Case("euler2d_weak_llf_mixed", 2, "euler", 3), Case("linadv1d_mixed", 1, "linadv", 4, cmin=-1.0, cmax=1.0,
                                                         ic="convergence_test"),
      # weak form with N = 4 applies D-hat on the matrix cores in mixed precision
@@ -297,7 +320,16 @@ def test_initial_condition_and_callbacks_match_oracle(th, oracle, torch, case):
                                   Case("euler3d_hg_density", 3, "euler", 3, "hg", "ranocha", ic="weak_blast_wave",
                                        indicator="density"),
                                   Case("euler2d_hg_mortar", 2, "euler", 4, "hg", "ranocha", ic="weak_blast_wave",
-                                       indicator="pressure", patches=(((-0.5, -0.5), (0.5, 0.5)),))],
+                                       indicator="pressure", patches=(((-0.5, -0.5), (0.5, 0.5)),)),
+                                  Case("mhd2d_hg", 2, "mhd", 5, "hg", "hindenlang_gassner", "hindenlang_gassner",
+                                       "hindenlang_gassner", ic="weak_blast_wave", gamma=1.4),
+                                  # polydeg 4: blocks with idle lanes (2D: two 25-node
+                                  # elements per wave; 3D: 125 nodes on 128 lanes)
+                                  MHD_HG_P4[0], MHD_HG_P4[1],
+                                  # x = 0.5 falls inside element 29 of 64 on [0, 1.1]
+                                  Case("burgers1d_hg_shock", 1, "burgers", 6, "hg", "ec", "llf", "llf", cmin=0.0,
+                                       cmax=1.1, ic="shock", periodic=False, bc_state="shock", outflow=("x_pos",),
+                                       indicator="first", alpha_max=1.0)],
                          ids=lambda c: c.name)
 def test_indicator_matches_oracle(th, oracle, torch, case):
     O = oracle
@@ -343,7 +375,11 @@ def test_indicator_matches_oracle(th, oracle, torch, case):
                                        ic="eoc_gravity", source="eoc_test_euler", gamma=2.0),
                                   Case("hypdiff2d_poisson", 2, "hypdiff", 3, polydeg=4, cmin=0.0, cmax=1.0,
                                        ic="poisson_nonperiodic", periodic=(False, True), bc_state="poisson_nonperiodic",
-                                       source="poisson_nonperiodic")],
+                                       source="poisson_nonperiodic"),
+                                  Case("mhd2d_hg_blast", 2, "mhd", 3, "hg", "hindenlang_gassner", "hindenlang_gassner",
+                                       "hindenlang_gassner", cmin=-2.0, cmax=2.0, ic="weak_blast_wave_seeded",
+                                       gamma=1.4),
+                                  MHD_HG_P4[0]],
                          ids=lambda c: c.name)
 def test_rk_step_matches_oracle(th, oracle, torch, case, use_graph):
     """Three CK2N54 steps from t = 0 (stage times through device memory: the

@@ -94,6 +94,16 @@ def test_config_mapping_and_rejections(th):
                          (th.flux_hindenlang_gassner, th.flux_nonconservative_powell)))
     c = ok.config(mhd)
     assert (c.equation, c.volume_flux, c.surface_flux, c.c_h) == (2, 5, 1, 1.0)
+    # mhd_shock.jl: HG with (flux, flux_nonconservative_powell) as DG and FV flux
+    pair = (th.flux_hindenlang_gassner, th.flux_nonconservative_powell)
+    ind_mhd = th.IndicatorHennemannGassner(mhd, th.LobattoLegendreBasisHIP(4), alpha_max=0.5, alpha_min=0.001,
+                                           alpha_smooth=True, variable=th.density_pressure)
+    hg = th.DGSEMHIP(polydeg=4, surface_flux=pair, volume_integral=th.VolumeIntegralShockCapturingHG(
+        ind_mhd, volume_flux_dg=pair, volume_flux_fv=pair)).config(mhd)
+    assert (hg.volume_integral, hg.volume_flux, hg.fv_flux, hg.surface_flux, hg.indicator_variable) == (2, 5, 5, 5, 0)
+    with pytest.raises(NotImplementedError):
+        th.DGSEMHIP(polydeg=4, surface_flux=pair, volume_integral=th.VolumeIntegralShockCapturingHG(
+            ind_mhd, volume_flux_dg=th.flux_hindenlang_gassner, volume_flux_fv=pair)).config(mhd)
     eq2 = th.CompressibleEulerEquations2D(1.4)
     b = th.LobattoLegendreBasisHIP(3)
     ind = th.IndicatorHennemannGassner(eq2, b, alpha_max=0.5, alpha_min=0.001, alpha_smooth=True,

@@ -703,3 +703,26 @@ def test_hypdiff_harmonic_and_llf(oracle):
     np.testing.assert_allclose(fl, [-1.25 * -0.2, -0.3 * lam ** 2 / 1.25], rtol=1e-14)
     np.testing.assert_allclose(O.two_point_flux(cfg, O.FLUX_LLF, ul, ur, 1), 0.5 * (fl + fr) - 0.5 * lam * (ur - ul),
                                rtol=1e-14)
+
+
+@pytest.mark.parametrize("nd", [2, 3])
+def test_mhd_shock_capturing_entropy_and_limits(oracle, nd):
+    """GLM-MHD shock capturing (test/tree_dgsem_{2,3}d/mhd_shock.jl): with the EC
+    flux + Powell terms as DG, FV (subcell interfaces f + nc(u_l, u_r)/2 on the
+    left, f + nc(u_r, u_l)/2 on the right) and surface flux, the blended scheme
+    conserves entropy at any blending factor; alpha_max = 0 reduces it to flux
+    differencing; LLF dissipates."""
+    O = oracle
+    mesh = O.Mesh(nd, 2 if nd == 2 else 1, -1.0, 1.0, 3)
+    HG = O.FLUX_HINDENLANG_GASSNER
+    cfg = O.make_cfg(nd, O.EQ_GLM_MHD, 3, O.VI_HG, HG, HG, HG, gamma=1.4, c_h=1.0, alpha_min=0.0)
+    u = O.compute_coefficients(cfg, mesh, O.IC_WEAK_BLAST_WAVE_SEEDED)
+    du, alpha = O.rhs(cfg, mesh, u, return_alpha=True)
+    assert alpha.max() > 0.01
+    assert abs(entropy_rate(O, cfg, mesh, u, du)) < 1e-12 * np.abs(du).max() * mesh.total_volume
+    cfg0 = O.make_cfg(nd, O.EQ_GLM_MHD, 3, O.VI_HG, HG, HG, HG, gamma=1.4, c_h=1.0, alpha_max=0.0)
+    cfg_fd = O.make_cfg(nd, O.EQ_GLM_MHD, 3, O.VI_FD, HG, HG, gamma=1.4, c_h=1.0)
+    np.testing.assert_allclose(O.rhs(cfg0, mesh, u), O.rhs(cfg_fd, mesh, u), rtol=0, atol=1e-12 * np.abs(du).max())
+    cfg = O.make_cfg(nd, O.EQ_GLM_MHD, 3, O.VI_HG, HG, O.FLUX_LLF, O.FLUX_LLF, gamma=1.4, c_h=1.0, alpha_min=0.0)
+    du = O.rhs(cfg, mesh, u)
+    assert entropy_rate(O, cfg, mesh, u, du) < -1e-6

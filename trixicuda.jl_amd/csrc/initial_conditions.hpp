@@ -88,6 +88,39 @@ __host__ __device__ inline void hypdiff_state(const IcParams& q, const double* x
     for (int k = 1; k <= d; ++k) u[k] = 0.0;
 }
 
+// Weak blast wave (Hennemann & Gassner 2020, Sec. 6.3): inside r <= 0.5
+// rho = 1.1691, p = 1.245 and an outward velocity of 0.1882; outside at rest
+// with rho = p = 1; the seeded variant adds U(-0.01, 0.01) per node.
+__device__ inline void weak_blast_prim(const IcParams& q, const double* x, uint64_t idx, double& rho, double* v,
+                                       double& p) {
+    double r2 = 0.0;
+    for (int d = 0; d < q.dim; ++d) r2 += x[d] * x[d];
+    const double r = sqrt(r2);
+    const bool inside = !(r > 0.5);
+    if (q.dim == 1) {
+        const double phi = x[0] > 0 ? 0.0 : M_PI;
+        v[0] = inside ? 0.1882 * cos(phi) : 0.0;
+    } else if (q.dim == 2) {
+        const double phi = atan2(x[1], x[0]);
+        v[0] = inside ? 0.1882 * cos(phi) : 0.0;
+        v[1] = inside ? 0.1882 * sin(phi) : 0.0;
+    } else {
+        const double phi = atan2(x[1], x[0]);
+        const double theta = (r == 0.0) ? 0.0 : acos(x[2] / r);
+        v[0] = inside ? 0.1882 * cos(phi) * sin(theta) : 0.0;
+        v[1] = inside ? 0.1882 * sin(phi) * sin(theta) : 0.0;
+        v[2] = inside ? 0.1882 * cos(theta) : 0.0;
+    }
+    rho = inside ? 1.1691 : 1.0;
+    p = inside ? 1.245 : 1.0;
+    if (q.ic == TRIXI_HIP_IC_WEAK_BLAST_WAVE_SEEDED) {
+        // breaks the exact symmetries (v = 0, equal neighbour states)
+        rho += 0.01 * (2.0 * u01(q.seed, idx, 0) - 1.0);
+        p += 0.01 * (2.0 * u01(q.seed, idx, 1) - 1.0);
+        for (int d = 0; d < q.dim; ++d) v[d] += 0.01 * (2.0 * u01(q.seed, idx, 2 + d) - 1.0);
+    }
+}
+
 // Initial conditions (Trixi initial_condition_* functions used by the
 // reference tests and examples, plus the seeded benchmark states).
 __device__ inline void initial_condition(const IcParams& q, const double* x, double t, uint64_t idx,
@@ -131,32 +164,7 @@ __device__ inline void initial_condition(const IcParams& q, const double* x, dou
     if (q.eq == EQ_EULER) {
         double v[3] = {0, 0, 0}, rho = 1.0, p = 1.0;
         if (q.ic == TRIXI_HIP_IC_WEAK_BLAST_WAVE || q.ic == TRIXI_HIP_IC_WEAK_BLAST_WAVE_SEEDED) {
-            double r2 = 0.0;
-            for (int d = 0; d < q.dim; ++d) r2 += x[d] * x[d];
-            const double r = sqrt(r2);
-            const bool inside = !(r > 0.5);
-            if (q.dim == 1) {
-                const double phi = x[0] > 0 ? 0.0 : M_PI;
-                v[0] = inside ? 0.1882 * cos(phi) : 0.0;
-            } else if (q.dim == 2) {
-                const double phi = atan2(x[1], x[0]);
-                v[0] = inside ? 0.1882 * cos(phi) : 0.0;
-                v[1] = inside ? 0.1882 * sin(phi) : 0.0;
-            } else {
-                const double phi = atan2(x[1], x[0]);
-                const double theta = (r == 0.0) ? 0.0 : acos(x[2] / r);
-                v[0] = inside ? 0.1882 * cos(phi) * sin(theta) : 0.0;
-                v[1] = inside ? 0.1882 * sin(phi) * sin(theta) : 0.0;
-                v[2] = inside ? 0.1882 * cos(theta) : 0.0;
-            }
-            rho = inside ? 1.1691 : 1.0;
-            p = inside ? 1.245 : 1.0;
-            if (q.ic == TRIXI_HIP_IC_WEAK_BLAST_WAVE_SEEDED) {
-                // breaks the exact symmetries (v = 0, equal neighbour states)
-                rho += 0.01 * (2.0 * u01(q.seed, idx, 0) - 1.0);
-                p += 0.01 * (2.0 * u01(q.seed, idx, 1) - 1.0);
-                for (int d = 0; d < q.dim; ++d) v[d] += 0.01 * (2.0 * u01(q.seed, idx, 2 + d) - 1.0);
-            }
+            weak_blast_prim(q, x, idx, rho, v, p);
         } else if (q.ic == TRIXI_HIP_IC_EOC_TEST_COUPLED_EULER_GRAVITY) {
             // c = 2, A = 0.1, G = 1: rho = c + A sin(pi (sum x - t)), v = 1,
             // p = rho^2 G 2 / (d pi)
@@ -236,6 +244,15 @@ __device__ inline void initial_condition(const IcParams& q, const double* x, dou
         prim[5] = nx - rho * v1 * sqr;
         prim[6] = ny - rho * v2 * sqr;
         prim[7] = -rho * v3 * sqr;
+        prim[8] = 0.0;
+    } else if (q.ic == TRIXI_HIP_IC_WEAK_BLAST_WAVE || q.ic == TRIXI_HIP_IC_WEAK_BLAST_WAVE_SEEDED) {
+        // the MHD weak blast: the Euler blast with B = (1, 1, 1), psi = 0
+        double rho, v[3] = {0.0, 0.0, 0.0}, p;
+        weak_blast_prim(q, x, idx, rho, v, p);
+        prim[0] = rho;
+        for (int d = 0; d < 3; ++d) prim[1 + d] = v[d];
+        prim[4] = p;
+        for (int d = 0; d < 3; ++d) prim[5 + d] = 1.0;
         prim[8] = 0.0;
     } else if (q.ic == TRIXI_HIP_IC_SEEDED_RANDOM) {
         prim[0] = 1.0 + 0.1 * (2.0 * u01(q.seed, idx, 0) - 1.0);

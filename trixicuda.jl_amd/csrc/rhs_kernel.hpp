@@ -504,10 +504,23 @@ rhs_kernel(const RhsArgs a) {
                     EQ::template two_point<FVF>(nd[p], nd[p + 1], d, P, F);
                     const R wl = alpha * (R)a.invw[p];
                     const R wr = alpha * (R)a.invw[p + 1];
+                    if constexpr (EQ::NONCONS) {
+                        // subcell interface with nonconservative terms: the left
+                        // node takes f + nc(u_l, u_r)/2, the right f + nc(u_r, u_l)/2
+                        R ncl[NV], ncr[NV];
+                        EQ::noncons(nd[p], nd[p + 1], d, ncl);
+                        EQ::noncons(nd[p + 1], nd[p], d, ncr);
 #pragma unroll
-                    for (int v = 0; v < NV; ++v) {
-                        acl[p][v] += wl * F[v];
-                        acl[p + 1][v] -= wr * F[v];
+                        for (int v = 0; v < NV; ++v) {
+                            acl[p][v] += wl * (F[v] + R(0.5) * ncl[v]);
+                            acl[p + 1][v] -= wr * (F[v] + R(0.5) * ncr[v]);
+                        }
+                    } else {
+#pragma unroll
+                        for (int v = 0; v < NV; ++v) {
+                            acl[p][v] += wl * F[v];
+                            acl[p + 1][v] -= wr * F[v];
+                        }
                     }
                 }
             }

@@ -310,10 +310,22 @@ __global__ void coefficients_kernel(double* __restrict__ u, const double* __rest
     }
 }
 
-// Hennemann-Gassner blending factor, one lane per node: 64-lane blocks of
-// 64 / N^d elements.  The dimension-wise modal transform goes through LDS;
-// the element's first lane sums the modal energies in node order (the order
-// of the element-per-thread formulation) and applies the logistic and clips.
+// Hennemann-Gassner blending factor, one lane per node: blocks of whole
+// elements, one wave (64 lanes) when an element fits, else the element's
+// node count rounded up to whole waves (polydeg 4 in 3D: 125 nodes, 128
+// lanes); lanes past the last whole element idle.  The dimension-wise modal
+// transform goes through LDS; the element's first lane sums the modal
+// energies in node order (the order of the element-per-thread formulation)
+// and applies the logistic and clips.
+template <int DIM, int N>
+constexpr int ind_block() {
+    return cpow(N, DIM) <= 64 ? 64 : (cpow(N, DIM) + 63) / 64 * 64;
+}
+template <int DIM, int N>
+constexpr int ind_epb() {
+    return ind_block<DIM, N>() / cpow(N, DIM);
+}
+
 template <int DIM, int N>
 __device__ __forceinline__ int max_mode(int n) {
     int maxc = 0;
@@ -338,11 +350,20 @@ __device__ __forceinline__ double modal_indicator(const double* __restrict__ u, 
     double m;
     if (nv == 1) {
         m = u[(int64_t)n * E + ec];
-    } else {
+    } else if (nv == DIM + 2) {
         double uc[DIM + 2];
 #pragma unroll
         for (int v = 0; v < DIM + 2; ++v) uc[v] = u[((int64_t)v * NN + n) * E + ec];
         m = Euler<DIM>::indicator_variable(uc, P, var);
+    } else {
+        // ideal GLM-MHD (nine variables): density_pressure with the magnetic
+        // and cleaning energies taken out of the pressure
+        double uc[9];
+#pragma unroll
+        for (int v = 0; v < 9; ++v) uc[v] = u[((int64_t)v * NN + n) * E + ec];
+        const double ke = (uc[1] * uc[1] + uc[2] * uc[2] + uc[3] * uc[3]) / uc[0];
+        const double me = uc[5] * uc[5] + uc[6] * uc[6] + uc[7] * uc[7] + uc[8] * uc[8];
+        m = uc[0] * P.gm1 * (uc[4] - 0.5 * ke - 0.5 * me);
     }
 #pragma unroll
     for (int d = 0; d < DIM; ++d) {
@@ -374,23 +395,27 @@ __device__ __forceinline__ void modal_energies(const double* e2, int el, double&
 }
 
 template <int DIM, int N>
-__global__ void __launch_bounds__(64) indicator_kernel(const double* __restrict__ u, double* __restrict__ alpha_raw,
-                                                       int64_t E, PhysParams phys, const double* __restrict__ Vinv,
-                                                       double threshold, double parameter_s, double alpha_min,
-                                                       double alpha_max, int var, int nv) {
+__global__ void __launch_bounds__(256) indicator_kernel(const double* __restrict__ u, double* __restrict__ alpha_raw,
+                                                        int64_t E, PhysParams phys, const double* __restrict__ Vinv,
+                                                        double threshold, double parameter_s, double alpha_min,
+                                                        double alpha_max, int var, int nv) {
     constexpr int NN = cpow(N, DIM);
-    constexpr int EPB = 64 / NN;
-    static_assert(NN <= 64 && 64 % NN == 0, "one block holds whole elements");
-    __shared__ double buf[64];
-    __shared__ double e2[64];
+    constexpr int BS = ind_block<DIM, N>();
+    constexpr int EPB = ind_epb<DIM, N>();
+    static_assert(BS <= 256 && EPB >= 1, "one block holds whole elements");
+    __shared__ double buf[BS];
+    __shared__ double e2[BS];
+    // idle lanes (el >= EPB) shadow the last element: their LDS slots lie
+    // past EPB * NN, which no element reads
     const int el = threadIdx.x / NN, n = threadIdx.x % NN;
-    const int64_t e = (int64_t)blockIdx.x * EPB + el;
+    const int elc = el < EPB ? el : EPB - 1;
+    const int64_t e = (int64_t)blockIdx.x * EPB + elc;
     const int64_t ec = e < E ? e : E - 1;
     const PhysR<double> P = PhysR<double>::from(phys);
-    const double m = modal_indicator<DIM, N>(u, ec, E, P, Vinv, buf, el, n, var, nv);
+    const double m = modal_indicator<DIM, N>(u, ec, E, P, Vinv, buf, elc, n, var, nv);
     e2[threadIdx.x] = m * m;
     __syncthreads();
-    if (n != 0 || e >= E) return;
+    if (n != 0 || el >= EPB || e >= E) return;
     double total, clip1, clip2;
     modal_energies<DIM, N>(e2, el, total, clip1, clip2);
     const double frac1 = (total != 0.0) ? (total - clip1) / total : 0.0;
@@ -874,21 +899,18 @@ static int launch_indicator(trixi_hip_ctx* c, const double* u, hipStream_t s) {
     const double threshold = 0.5 * pow(10.0, -1.8 * pow((double)c->N, 0.25));
     const double parameter_s = log((1.0 - 0.0001) / 0.0001);
     const PhysParams& P = c->args.phys;
-    if (c->N != 4) return fail(TRIXI_HIP_EUNSUPPORTED, "shock capturing is built for polydeg 3 only");
-    const int epb = 64 / (c->dim == 1 ? 4 : c->dim == 2 ? 16 : 64);
-    const unsigned blocks = (unsigned)((c->E + epb - 1) / epb);
-    if (c->dim == 1)
-        hipLaunchKernelGGL((indicator_kernel<1, 4>), dim3(blocks), dim3(64), 0, s, u, c->d_alpha_raw, c->E, P,
-                           c->d_Vinv, threshold, parameter_s, c->cfg.alpha_min, c->cfg.alpha_max,
-                           c->cfg.indicator_variable, c->nv);
-    else if (c->dim == 2)
-        hipLaunchKernelGGL((indicator_kernel<2, 4>), dim3(blocks), dim3(64), 0, s, u, c->d_alpha_raw, c->E, P,
-                           c->d_Vinv, threshold, parameter_s, c->cfg.alpha_min, c->cfg.alpha_max,
-                           c->cfg.indicator_variable, c->nv);
-    else
-        hipLaunchKernelGGL((indicator_kernel<3, 4>), dim3(blocks), dim3(64), 0, s, u, c->d_alpha_raw, c->E, P,
-                           c->d_Vinv, threshold, parameter_s, c->cfg.alpha_min, c->cfg.alpha_max,
-                           c->cfg.indicator_variable, c->nv);
+    if (c->N != 4 && c->N != 5) return fail(TRIXI_HIP_EUNSUPPORTED, "shock capturing is built for polydeg 3 and 4");
+#define TRIXI_HIP_IND(D, NQ)                                                                                  \
+    hipLaunchKernelGGL((indicator_kernel<D, NQ>),                                                             \
+                       dim3((unsigned)((c->E + ind_epb<D, NQ>() - 1) / ind_epb<D, NQ>())),                    \
+                       dim3(ind_block<D, NQ>()), 0, s, u, c->d_alpha_raw, c->E, P, c->d_Vinv, threshold,     \
+                       parameter_s, c->cfg.alpha_min, c->cfg.alpha_max, c->cfg.indicator_variable, c->nv)
+    if (c->N == 4) {
+        if (c->dim == 1) TRIXI_HIP_IND(1, 4); else if (c->dim == 2) TRIXI_HIP_IND(2, 4); else TRIXI_HIP_IND(3, 4);
+    } else {
+        if (c->dim == 1) TRIXI_HIP_IND(1, 5); else if (c->dim == 2) TRIXI_HIP_IND(2, 5); else TRIXI_HIP_IND(3, 5);
+    }
+#undef TRIXI_HIP_IND
     HIP_TRY(hipGetLastError());
     return TRIXI_HIP_OK;
 }
@@ -1002,8 +1024,11 @@ int trixi_hip_create(const trixi_hip_config* config, const trixi_hip_mesh* mesh,
     if (cfg.equation == TRIXI_HIP_INVISCID_BURGERS && cfg.ndims != 1)
         return fail(TRIXI_HIP_EUNSUPPORTED, "the inviscid Burgers equation is built for 1D");
     if (cfg.volume_integral == TRIXI_HIP_VOLUME_SHOCK_CAPTURING_HG && cfg.equation != TRIXI_HIP_COMPRESSIBLE_EULER &&
-        cfg.equation != TRIXI_HIP_INVISCID_BURGERS)
-        return fail(TRIXI_HIP_EUNSUPPORTED, "shock capturing is built for compressible Euler and Burgers");
+        cfg.equation != TRIXI_HIP_INVISCID_BURGERS && cfg.equation != TRIXI_HIP_IDEAL_GLM_MHD)
+        return fail(TRIXI_HIP_EUNSUPPORTED, "shock capturing is built for compressible Euler, Burgers and GLM-MHD");
+    if (cfg.volume_integral == TRIXI_HIP_VOLUME_SHOCK_CAPTURING_HG && cfg.equation == TRIXI_HIP_IDEAL_GLM_MHD &&
+        (cfg.indicator_variable != TRIXI_HIP_INDICATOR_DENSITY_PRESSURE || cfg.ndims == 1))
+        return fail(TRIXI_HIP_EUNSUPPORTED, "GLM-MHD shock capturing is built for 2D/3D with density_pressure");
     if (cfg.volume_integral == TRIXI_HIP_VOLUME_SHOCK_CAPTURING_HG && cfg.equation == TRIXI_HIP_INVISCID_BURGERS &&
         cfg.indicator_variable != TRIXI_HIP_INDICATOR_DENSITY)
         return fail(TRIXI_HIP_EUNSUPPORTED, "the Burgers indicator variable is the solution itself (first)");
@@ -1343,7 +1368,7 @@ static IcParams ic_params(const trixi_hip_ctx* c, int ic, uint64_t seed) {
 int trixi_hip_compute_coefficients(trixi_hip_ctx* ctx, double* u, int32_t ic, double t, uint64_t seed,
                                    trixi_hip_stream stream) {
     if (!ctx || !u) return fail(TRIXI_HIP_EINVAL, "null argument");
-    if (ic < 0 || ic > TRIXI_HIP_IC_WEAK_BLAST_WAVE_SEEDED) return fail(TRIXI_HIP_EINVAL, "unknown initial condition");
+    if (ic < 0 || ic > TRIXI_HIP_IC_HARMONIC_NONPERIODIC) return fail(TRIXI_HIP_EINVAL, "unknown initial condition");
     hipLaunchKernelGGL(coefficients_kernel, dim3(1024), dim3(256), 0, (hipStream_t)stream, u, ctx->d_x,
                        ic_params(ctx, ic, seed), ctx->N, ctx->nn, ctx->E, t);
     HIP_TRY(hipGetLastError());
@@ -1495,6 +1520,7 @@ int trixi_hip_max_dt(trixi_hip_ctx* ctx, const double* u, double* max_dt, trixi_
 int trixi_hip_error_norms(trixi_hip_ctx* ctx, const double* u, int32_t ic, double t, uint64_t seed, double* l2,
                           double* linf, trixi_hip_stream stream) {
     if (!ctx || !u || !l2 || !linf) return fail(TRIXI_HIP_EINVAL, "null argument");
+    if (ic < 0 || ic > TRIXI_HIP_IC_HARMONIC_NONPERIODIC) return fail(TRIXI_HIP_EINVAL, "unknown initial condition");
     hipStream_t s = (hipStream_t)stream;
     const int threads = 256;
     const int64_t npts = (int64_t)ctx->E * (int64_t)lrint(pow((double)ctx->NA, (double)ctx->dim));
@@ -1538,8 +1564,16 @@ int trixi_hip_integrate(trixi_hip_ctx* ctx, const double* u, double* integral, t
 
 int trixi_hip_indicator(trixi_hip_ctx* ctx, const double* u, double* alpha, trixi_hip_stream stream) {
     if (!ctx || !u || !alpha) return fail(TRIXI_HIP_EINVAL, "null argument");
-    if (ctx->cfg.equation != TRIXI_HIP_COMPRESSIBLE_EULER)
-        return fail(TRIXI_HIP_EUNSUPPORTED, "the HG indicator is built for compressible Euler only");
+    const trixi_hip_config& cfg = ctx->cfg;
+    const bool built = cfg.equation == TRIXI_HIP_COMPRESSIBLE_EULER ||
+                       (cfg.equation == TRIXI_HIP_IDEAL_GLM_MHD && cfg.ndims > 1 &&
+                        cfg.indicator_variable == TRIXI_HIP_INDICATOR_DENSITY_PRESSURE) ||
+                       (cfg.equation == TRIXI_HIP_INVISCID_BURGERS &&
+                        cfg.indicator_variable == TRIXI_HIP_INDICATOR_DENSITY);
+    if (!built)
+        return fail(TRIXI_HIP_EUNSUPPORTED,
+                    "the HG indicator is built for compressible Euler, 2D/3D GLM-MHD (density_pressure) and "
+                    "Burgers (first)");
     hipStream_t s = (hipStream_t)stream;
     int rc = launch_indicator(ctx, u, s);
     if (rc) return rc;

@@ -114,9 +114,12 @@ class DGSEMHIP:
                 raise NotImplementedError("GLM-MHD volume flux must be (flux, flux_nonconservative_powell)")
             c.volume_flux = vflux.flux_id
         elif vi.kind == L.VI_HG:
-            c.volume_flux = split_flux(vi.volume_flux_dg)[0].flux_id
-            fv = vi.volume_flux_fv if vi.volume_flux_fv is not None else sflux
-            c.fv_flux = split_flux(fv)[0].flux_id
+            dg_flux, dg_nc = split_flux(vi.volume_flux_dg)
+            fv_flux, fv_nc = split_flux(vi.volume_flux_fv if vi.volume_flux_fv is not None else self.surface_flux)
+            if nc_needed != (dg_nc is not None) or nc_needed != (fv_nc is not None):
+                raise NotImplementedError("GLM-MHD DG and FV volume fluxes must be (flux, flux_nonconservative_powell)")
+            c.volume_flux = dg_flux.flux_id
+            c.fv_flux = fv_flux.flux_id
             c.alpha_smooth = 1 if vi.indicator.alpha_smooth else 0
             # first (the solution of a scalar equation) and density both take u[0]
             c.indicator_variable = {density_pressure: 0, pressure: 1, density: 2, first: 2}[vi.indicator.variable]
