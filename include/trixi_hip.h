@@ -52,7 +52,10 @@ enum trixi_hip_equation {
     TRIXI_HIP_IDEAL_GLM_MHD = 2,           /* IdealGlmMhdEquations1D / 2D / 3D (1D:
                                               8 variables, no psi / Powell terms)  */
     TRIXI_HIP_INVISCID_BURGERS = 3,        /* InviscidBurgersEquation1D             */
-    TRIXI_HIP_HYPERBOLIC_DIFFUSION = 4     /* HyperbolicDiffusionEquations{1,2,3}D  */
+    TRIXI_HIP_HYPERBOLIC_DIFFUSION = 4,    /* HyperbolicDiffusionEquations{1,2,3}D  */
+    TRIXI_HIP_COMPRESSIBLE_EULER_MULTICOMPONENT = 5 /* CompressibleEulerMulticomponent-
+                                              Equations{1,2}D: (rho v, rho e, rho_1..
+                                              rho_ncomponents), 1 or 2 components  */
 };
 enum trixi_hip_volume_integral {
     TRIXI_HIP_VOLUME_WEAK_FORM = 0,          /* VolumeIntegralWeakForm          */
@@ -157,6 +160,10 @@ typedef struct trixi_hip_config {
                                        the Dirichlet state                        */
     double nu;                /* hyperbolic diffusion: diffusion constant (1)     */
     double Lr;                /* hyperbolic diffusion: reference length 1/(2 pi); T_r = Lr^2 / nu */
+    int32_t ncomponents;      /* multicomponent Euler: number of gas components   */
+    int32_t reserved;         /* 0                                                */
+    double gammas[4];         /* multicomponent Euler: gamma_k of each component  */
+    double gas_constants[4];  /* multicomponent Euler: R_k of each component      */
 } trixi_hip_config;
 
 /* Mesh and connectivity (host arrays, copied to the device by create): the

@@ -17,7 +17,7 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_PATH = os.path.join(_HERE, "liboracle.so")
 
-EQ_LINADV, EQ_EULER, EQ_GLM_MHD, EQ_BURGERS, EQ_HYPDIFF = 0, 1, 2, 3, 4
+EQ_LINADV, EQ_EULER, EQ_GLM_MHD, EQ_BURGERS, EQ_HYPDIFF, EQ_EULER_MULTI = 0, 1, 2, 3, 4, 5
 VI_WEAK, VI_FD, VI_HG = 0, 1, 2
 FLUX_CENTRAL, FLUX_LLF, FLUX_LLF_NAIVE, FLUX_RANOCHA, FLUX_SHIMA_ETAL, FLUX_HINDENLANG_GASSNER, FLUX_EC, FLUX_HLL = range(8)
 (IC_CONSTANT, IC_CONVERGENCE_TEST, IC_WEAK_BLAST_WAVE, IC_SEEDED_RANDOM, IC_SMOOTH_PERIODIC, IC_WEAK_BLAST_WAVE_SEEDED,
@@ -58,6 +58,10 @@ class OracleCfg(ctypes.Structure):
         ("reserved", ctypes.c_int32),
         ("nu", ctypes.c_double),
         ("Lr", ctypes.c_double),
+        ("ncomponents", ctypes.c_int32),
+        ("reserved2", ctypes.c_int32),
+        ("gammas", ctypes.c_double * 4),
+        ("gas_constants", ctypes.c_double * 4),
     ]
 
 
@@ -127,9 +131,12 @@ def make_cfg(ndims, equation, polydeg=3, volume_integral=VI_WEAK, volume_flux=FL
              advection_velocity=(1.0, 1.0, 1.0), alpha_max=0.5, alpha_min=0.001,
              alpha_smooth=True, ic_length=4.0, boundary_condition=BC_PERIODIC, boundary_ic=IC_CONSTANT,
              boundary_seed=0, source_terms=SOURCE_NONE, indicator_variable=0, boundary_outflow_faces=0,
-             nu=1.0, Lr=1.0 / (2.0 * np.pi)):
+             nu=1.0, Lr=1.0 / (2.0 * np.pi), gammas=(), gas_constants=()):
     c = OracleCfg()
     c.nu, c.Lr = nu, Lr
+    c.ncomponents = len(gammas)
+    for k, (g, r) in enumerate(zip(gammas, gas_constants)):
+        c.gammas[k], c.gas_constants[k] = g, r
     c.boundary_outflow_faces = boundary_outflow_faces
     c.indicator_variable = indicator_variable
     c.boundary_condition, c.boundary_ic, c.boundary_seed = boundary_condition, boundary_ic, boundary_seed

@@ -81,6 +81,8 @@ constexpr int MAXV = 9;
 
 struct Cfg {
     int nd, eq, N, nv, vi, vflux, sflux, fvflux, smooth;
+    int nc;                     // multicomponent Euler: components
+    double mgamma[4], mR[4], mcv[4];  // gamma_k, R_k, c_v,k = R_k / (gamma_k - 1)
     double gamma, inv_gm1, c_h, a[3], amax, amin, ic_length, nu, inv_Tr;
     int bc, bc_ic, src, ivar, outflow;
     uint64_t bc_seed;
@@ -122,6 +124,12 @@ Cfg make_cfg(const oracle_cfg* c) {
     k.mids = c->mortar_neighbor_ids;
     k.mori = c->mortar_orientations;
     k.mls = c->mortar_large_sides;
+    k.nc = (c->equation == OR_EQ_EULER_MULTI) ? c->ncomponents : 0;
+    for (int i = 0; i < 4; ++i) {
+        k.mgamma[i] = (i < k.nc) ? c->gammas[i] : 0.0;
+        k.mR[i] = (i < k.nc) ? c->gas_constants[i] : 0.0;
+        k.mcv[i] = (i < k.nc) ? c->gas_constants[i] / (c->gammas[i] - 1.0) : 0.0;
+    }
     return k;
 }
 
@@ -174,8 +182,44 @@ void mhd_prim(const Cfg& c, const T* u, T& rho, T* v, T& p, T* B, T& psi) {
          0.5 * (B[0] * B[0] + B[1] * B[1] + B[2] * B[2]) - 0.5 * psi * psi);
 }
 
+// Multicomponent Euler (Trixi CompressibleEulerMulticomponentEquations{1,2}D):
+// u = (rho v_1..nd, rho e, rho_1..rho_nc); density = sum of the partial
+// densities, totalgamma = sum rho_k c_v,k gamma_k / sum rho_k c_v,k,
+// p = (gamma - 1)(rho e - rho |v|^2 / 2).
+template <class T> T multi_density(const Cfg& c, const T* u) {
+    T rho = u[c.nd + 1];
+    for (int k = 1; k < c.nc; ++k) rho = rho + u[c.nd + 1 + k];
+    return rho;
+}
+template <class T> T multi_totalgamma(const Cfg& c, const T* u) {
+    T h1 = 0.0, h2 = 0.0;
+    for (int k = 0; k < c.nc; ++k) {
+        h1 += u[c.nd + 1 + k] * c.mcv[k] * c.mgamma[k];
+        h2 += u[c.nd + 1 + k] * c.mcv[k];
+    }
+    return h1 / h2;
+}
+template <class T> void multi_prim(const Cfg& c, const T* u, T& rho, T* v, T& p, T& gamma) {
+    rho = multi_density(c, u);
+    T vsq = 0.0;
+    for (int d = 0; d < c.nd; ++d) {
+        v[d] = u[d] / rho;
+        vsq += v[d] * v[d];
+    }
+    gamma = multi_totalgamma(c, u);
+    p = (gamma - 1.0) * (u[c.nd] - 0.5 * rho * vsq);
+}
+
 template <class T> void phys_flux(const Cfg& c, const T* u, int o, T* f) {
     const int oi = o - 1;
+    if (c.eq == OR_EQ_EULER_MULTI) {
+        T rho, v[3], p, gamma;
+        multi_prim(c, u, rho, v, p, gamma);
+        for (int d = 0; d < c.nd; ++d) f[d] = (d == oi) ? u[d] * v[oi] + p : u[d] * v[oi];
+        f[c.nd] = (u[c.nd] + p) * v[oi];
+        for (int k = 0; k < c.nc; ++k) f[c.nd + 1 + k] = u[c.nd + 1 + k] * v[oi];
+        return;
+    }
     if (c.eq == OR_EQ_HYPDIFF) {
         // Trixi HyperbolicDiffusionEquations flux: (-nu q_o, -phi / T_r e_o)
         f[0] = -c.nu * u[o];
@@ -246,6 +290,11 @@ template <class T> void normal_speed(const Cfg& c, const T* u, int o, T& vn, T& 
         euler_prim(c, u, rho, v, p);
         vn = v[oi];
         cs = tsqrt(c.gamma * p / rho);
+    } else if (c.eq == OR_EQ_EULER_MULTI) {
+        T rho, v[3], p, gamma;
+        multi_prim(c, u, rho, v, p, gamma);
+        vn = v[oi];
+        cs = tsqrt(gamma * p / rho);
     } else {
         vn = u[1 + oi] / u[0];
         cs = mhd_fast_wavespeed(c, u, o);
@@ -281,6 +330,48 @@ template <class T> void flux_ranocha_euler(const Cfg& c, const T* ull, const T* 
     for (int d = 0; d < c.nd; ++d) f[1 + d] = (d == oi) ? f1 * vavg[d] + p_avg : f1 * vavg[d];
     f[c.nd + 1] = f1 * (velocity_square_avg + inv_rho_p_mean * c.inv_gm1) +
                   0.5 * (pl * vr[oi] + pr * vl[oi]);
+}
+
+// Trixi flux_ranocha(u_ll, u_rr, orientation, ::CompressibleEulerMulticomponentEquations{1,2}D):
+// ln-means of the partial densities times the mean normal velocity; the
+// momentum and energy fluxes of Ranocha's flux with the temperature-based
+// pressures p = T sum rho_k R_k, T = (rho e - rho |v|^2 / 2) / sum rho_k c_v,k,
+// and the gamma of the averaged state.
+template <class T> void flux_ranocha_multi(const Cfg& c, const T* ull, const T* urr, int o, T* f) {
+    const int oi = o - 1, nd = c.nd;
+    T rl = multi_density(c, ull), rr = multi_density(c, urr);
+    T uavg[MAXV];
+    for (int v = 0; v < c.nv; ++v) uavg[v] = 0.5 * (ull[v] + urr[v]);
+    T gamma = multi_totalgamma(c, uavg);
+    T inv_gm1 = 1.0 / (gamma - 1.0);
+    T vl[3], vr[3], vavg[3], vsq = 0.0, kel = 0.0, ker = 0.0;
+    for (int d = 0; d < nd; ++d) {
+        vl[d] = ull[d] / rl;
+        vr[d] = urr[d] / rr;
+        vavg[d] = 0.5 * (vl[d] + vr[d]);
+        vsq += vl[d] * vr[d];
+        kel += vl[d] * vl[d];
+        ker += vr[d] * vr[d];
+    }
+    T velocity_square_avg = 0.5 * vsq;
+    T enth_l = 0.0, enth_r = 0.0, h_l = 0.0, h_r = 0.0;
+    for (int k = 0; k < c.nc; ++k) {
+        enth_l += ull[nd + 1 + k] * c.mR[k];
+        enth_r += urr[nd + 1 + k] * c.mR[k];
+        h_l += ull[nd + 1 + k] * c.mcv[k];
+        h_r += urr[nd + 1 + k] * c.mcv[k];
+    }
+    T Tl = (ull[nd] - 0.5 * rl * kel) / h_l, Tr = (urr[nd] - 0.5 * rr * ker) / h_r;
+    T pl = Tl * enth_l, pr = Tr * enth_r;
+    T p_avg = 0.5 * (pl + pr);
+    T inv_rho_p_mean = pl * pr * inv_ln_mean(rl * pr, rr * pl);
+    T f_rho_sum = 0.0;
+    for (int k = 0; k < c.nc; ++k) {
+        f[nd + 1 + k] = ln_mean(ull[nd + 1 + k], urr[nd + 1 + k]) * vavg[oi];
+        f_rho_sum += f[nd + 1 + k];
+    }
+    for (int d = 0; d < nd; ++d) f[d] = (d == oi) ? f_rho_sum * vavg[d] + p_avg : f_rho_sum * vavg[d];
+    f[nd] = f_rho_sum * (velocity_square_avg + inv_rho_p_mean * inv_gm1) + 0.5 * (pl * vr[oi] + pr * vl[oi]);
 }
 
 template <class T> void flux_shima_euler(const Cfg& c, const T* ull, const T* urr, int o, T* f) {
@@ -362,7 +453,10 @@ template <class T> void noncons_powell(const Cfg& c, const T* ull, const T* urr,
 template <class T> void two_point(const Cfg& c, int kind, const T* ull, const T* urr, int o, T* f) {
     switch (kind) {
         case OR_FLUX_RANOCHA:
-            flux_ranocha_euler(c, ull, urr, o, f);
+            if (c.eq == OR_EQ_EULER_MULTI)
+                flux_ranocha_multi(c, ull, urr, o, f);
+            else
+                flux_ranocha_euler(c, ull, urr, o, f);
             return;
         case OR_FLUX_SHIMA_ETAL:
             flux_shima_euler(c, ull, urr, o, f);
@@ -1266,6 +1360,39 @@ void hypdiff_state(const Cfg& c, int ic, const double* x, double t, bool steady,
     }
 }
 
+// Weak blast wave primitives (Trixi initial_condition_weak_blast_wave of
+// CompressibleEulerEquations{1,2,3}D, Hennemann & Gassner 2020 Sec. 6.3);
+// the seeded variant adds U(-0.01, 0.01) per node.
+void weak_blast_prim(const Cfg& c, int ic, const double* x, uint64_t seed, uint64_t idx, double& rho,
+                     double* v, double& p) {
+    double r2 = 0.0;
+    for (int d = 0; d < c.nd; ++d) r2 += x[d] * x[d];
+    const double r = sqrt(r2);
+    const bool inside = !(r > 0.5);
+    if (c.nd == 1) {
+        const double phi = x[0] > 0 ? 0.0 : M_PI;
+        v[0] = inside ? 0.1882 * cos(phi) : 0.0;
+    } else if (c.nd == 2) {
+        const double phi = atan2(x[1], x[0]);
+        v[0] = inside ? 0.1882 * cos(phi) : 0.0;
+        v[1] = inside ? 0.1882 * sin(phi) : 0.0;
+    } else {
+        const double phi = atan2(x[1], x[0]);
+        const double theta = (r == 0.0) ? 0.0 : acos(x[2] / r);
+        v[0] = inside ? 0.1882 * cos(phi) * sin(theta) : 0.0;
+        v[1] = inside ? 0.1882 * sin(phi) * sin(theta) : 0.0;
+        v[2] = inside ? 0.1882 * cos(theta) : 0.0;
+    }
+    rho = inside ? 1.1691 : 1.0;
+    p = inside ? 1.245 : 1.0;
+    if (ic == OR_IC_WEAK_BLAST_WAVE_SEEDED) {
+        // breaks the exact symmetries (v = 0, equal neighbour states)
+        rho += 0.01 * (2.0 * u01(seed, idx, 0) - 1.0);
+        p += 0.01 * (2.0 * u01(seed, idx, 1) - 1.0);
+        for (int d = 0; d < c.nd; ++d) v[d] += 0.01 * (2.0 * u01(seed, idx, 2 + d) - 1.0);
+    }
+}
+
 void initial_condition(const Cfg& c, int ic, const double* x, double t, uint64_t seed,
                        uint64_t idx, double* u) {
     if (c.eq == OR_EQ_HYPDIFF) {
@@ -1303,6 +1430,31 @@ void initial_condition(const Cfg& c, int ic, const double* x, double t, uint64_t
         }
         return;
     }
+    if (c.eq == OR_EQ_EULER_MULTI) {
+        // Trixi's multicomponent initial_condition_weak_blast_wave: the Euler
+        // blast with rho_k = 2^(k-1) (1 - 2) rho / (1 - 2^nc) (prim2cons with
+        // the totalgamma of the partial densities); the seeded state
+        // rho_k = 2^(k-1) / (2^nc - 1) (1 + 0.1 U), v = 0.1 N(0, 1), p = 1 + 0.05 U.
+        double v[3] = {0, 0, 0}, rho = 1.0, p = 1.0, prim[MAXV];
+        if (ic == OR_IC_WEAK_BLAST_WAVE || ic == OR_IC_WEAK_BLAST_WAVE_SEEDED)
+            weak_blast_prim(c, ic, x, seed, idx, rho, v, p);
+        for (int k = 0; k < c.nc; ++k)
+            prim[c.nd + 1 + k] = pow(2.0, k) * (1.0 - 2.0) * rho / (1.0 - pow(2.0, c.nc));
+        if (ic == OR_IC_SEEDED_RANDOM) {
+            for (int k = 0; k < c.nc; ++k) prim[c.nd + 1 + k] *= 1.0 + 0.1 * (2.0 * u01(seed, idx, 8 + k) - 1.0);
+            p = 1.0 + 0.05 * (2.0 * u01(seed, idx, 1) - 1.0);
+            for (int d = 0; d < c.nd; ++d) v[d] = 0.1 * normal01(seed, idx, 2 + 2 * d);
+        }
+        const double rho_t = multi_density(c, prim), gamma = multi_totalgamma(c, prim);
+        double ke = 0.0;
+        for (int d = 0; d < c.nd; ++d) {
+            u[d] = rho_t * v[d];
+            ke += u[d] * v[d];
+        }
+        u[c.nd] = p / (gamma - 1.0) + 0.5 * ke;
+        for (int k = 0; k < c.nc; ++k) u[c.nd + 1 + k] = prim[c.nd + 1 + k];
+        return;
+    }
     if (c.eq == OR_EQ_EULER) {
         double v[3] = {0, 0, 0}, rho = 1.0, p = 1.0;
         if (ic == OR_IC_EOC_GRAVITY) {
@@ -1318,32 +1470,7 @@ void initial_condition(const Cfg& c, int ic, const double* x, double t, uint64_t
             return;
         }
         if (ic == OR_IC_WEAK_BLAST_WAVE || ic == OR_IC_WEAK_BLAST_WAVE_SEEDED) {
-            double r2 = 0.0;
-            for (int d = 0; d < c.nd; ++d) r2 += x[d] * x[d];
-            const double r = sqrt(r2);
-            const bool inside = !(r > 0.5);
-            if (c.nd == 1) {
-                const double phi = x[0] > 0 ? 0.0 : M_PI;
-                v[0] = inside ? 0.1882 * cos(phi) : 0.0;
-            } else if (c.nd == 2) {
-                const double phi = atan2(x[1], x[0]);
-                v[0] = inside ? 0.1882 * cos(phi) : 0.0;
-                v[1] = inside ? 0.1882 * sin(phi) : 0.0;
-            } else {
-                const double phi = atan2(x[1], x[0]);
-                const double theta = (r == 0.0) ? 0.0 : acos(x[2] / r);
-                v[0] = inside ? 0.1882 * cos(phi) * sin(theta) : 0.0;
-                v[1] = inside ? 0.1882 * sin(phi) * sin(theta) : 0.0;
-                v[2] = inside ? 0.1882 * cos(theta) : 0.0;
-            }
-            rho = inside ? 1.1691 : 1.0;
-            p = inside ? 1.245 : 1.0;
-            if (ic == OR_IC_WEAK_BLAST_WAVE_SEEDED) {
-                // breaks the exact symmetries (v = 0, equal neighbour states)
-                rho += 0.01 * (2.0 * u01(seed, idx, 0) - 1.0);
-                p += 0.01 * (2.0 * u01(seed, idx, 1) - 1.0);
-                for (int d = 0; d < c.nd; ++d) v[d] += 0.01 * (2.0 * u01(seed, idx, 2 + d) - 1.0);
-            }
+            weak_blast_prim(c, ic, x, seed, idx, rho, v, p);
         } else if (ic == OR_IC_SEEDED_RANDOM) {
             rho = 1.0 + 0.1 * (2.0 * u01(seed, idx, 0) - 1.0);
             p = 1.0 + 0.05 * (2.0 * u01(seed, idx, 1) - 1.0);
@@ -1453,6 +1580,7 @@ int oracle_nvars(const oracle_cfg* cfg) {
     if (cfg->equation == OR_EQ_LINADV || cfg->equation == OR_EQ_BURGERS) return 1;
     if (cfg->equation == OR_EQ_HYPDIFF) return cfg->ndims + 1;
     if (cfg->equation == OR_EQ_EULER) return cfg->ndims + 2;
+    if (cfg->equation == OR_EQ_EULER_MULTI) return cfg->ndims + 1 + cfg->ncomponents;
     return cfg->ndims == 1 ? 8 : 9;  // ideal MHD: (rho, rho v, rho e, B) in 1D, + psi in 2D/3D
 }
 
@@ -1554,6 +1682,10 @@ void oracle_max_abs_speeds(const oracle_cfg* cfg, const double* u, double* lam) 
             double rho, v[3], p;
             euler_prim(c, u, rho, v, p);
             lam[d] = fabs(v[d]) + sqrt(c.gamma * p / rho);
+        } else if (c.eq == OR_EQ_EULER_MULTI) {
+            double rho, v[3], p, gamma;
+            multi_prim(c, u, rho, v, p, gamma);
+            lam[d] = fabs(v[d]) + sqrt(gamma * p / rho);
         } else {
             lam[d] = fabs(u[1 + d] / u[0]) + mhd_fast_wavespeed(c, u, d + 1);
         }

@@ -25,7 +25,8 @@
 extern "C" {
 #endif
 
-enum { OR_EQ_LINADV = 0, OR_EQ_EULER = 1, OR_EQ_GLM_MHD = 2, OR_EQ_BURGERS = 3, OR_EQ_HYPDIFF = 4 };
+enum { OR_EQ_LINADV = 0, OR_EQ_EULER = 1, OR_EQ_GLM_MHD = 2, OR_EQ_BURGERS = 3, OR_EQ_HYPDIFF = 4,
+       OR_EQ_EULER_MULTI = 5 /* (rho v, rho e, rho_1..rho_nc) */ };
 enum { OR_VI_WEAK = 0, OR_VI_FD = 1, OR_VI_HG = 2 };
 enum {
     OR_FLUX_CENTRAL = 0,
@@ -82,6 +83,9 @@ typedef struct oracle_cfg {
     int32_t boundary_outflow_faces; /* bit f (-x,+x,-y,+y,-z,+z): boundary flux f(u_inner) */
     int32_t reserved;
     double nu, Lr;            /* hyperbolic diffusion: T_r = Lr^2 / nu */
+    int32_t ncomponents;      /* multicomponent Euler */
+    int32_t reserved2;
+    double gammas[4], gas_constants[4];
 } oracle_cfg;
 
 int oracle_nvars(const oracle_cfg* cfg);

@@ -9,7 +9,7 @@ import numpy as np
 class Case:
     name: str
     ndims: int
-    equation: str           # "linadv" | "euler" | "mhd" | "burgers" | "hypdiff"
+    equation: str           # "linadv" | "euler" | "mhd" | "burgers" | "hypdiff" | "eulermulti"
     level: int
     vi: str = "weak"        # "weak" | "fd" | "hg"
     vflux: str = "central"
@@ -35,6 +35,8 @@ class Case:
     indicator: str = "density_pressure"  # HG indicator variable: density_pressure | pressure | density | first
     outflow: tuple = ()     # boundary sides ("x_neg", ...) with the outflow condition
     nu: float = 1.0         # hyperbolic diffusion constant
+    gammas: tuple = (1.4, 1.4)          # multicomponent Euler
+    gas_constants: tuple = (0.4, 0.4)
 
 
 FLUX_NAMES = ("central", "llf", "llf_naive", "ranocha", "shima_etal", "hindenlang_gassner", "ec", "hll")
@@ -54,7 +56,7 @@ def oracle_mesh(O, c: Case):
 
 def oracle_cfg(O, c: Case):
     eq = {"linadv": O.EQ_LINADV, "euler": O.EQ_EULER, "mhd": O.EQ_GLM_MHD, "burgers": O.EQ_BURGERS,
-          "hypdiff": O.EQ_HYPDIFF}[c.equation]
+          "hypdiff": O.EQ_HYPDIFF, "eulermulti": O.EQ_EULER_MULTI}[c.equation]
     vi = {"weak": O.VI_WEAK, "fd": O.VI_FD, "hg": O.VI_HG}[c.vi]
     ext = dict(boundary_condition=O.BC_DIRICHLET if c.bc_state else O.BC_PERIODIC,
                boundary_ic=IC_NAMES.index(c.bc_state) if c.bc_state else 0,
@@ -64,7 +66,9 @@ def oracle_cfg(O, c: Case):
                       FLUX_NAMES.index(c.fvflux), gamma=c.gamma, c_h=c.c_h,
                       advection_velocity=c.advection[:c.ndims], alpha_max=c.alpha_max, alpha_min=c.alpha_min,
                       alpha_smooth=c.alpha_smooth, ic_length=c.cmax - c.cmin,
-                      indicator_variable=INDICATORS[c.indicator], nu=c.nu, **ext)
+                      indicator_variable=INDICATORS[c.indicator], nu=c.nu,
+                      gammas=c.gammas if c.equation == "eulermulti" else (),
+                      gas_constants=c.gas_constants if c.equation == "eulermulti" else (), **ext)
 
 
 def oracle_ic(O, c: Case):
@@ -92,6 +96,10 @@ def product_semi(th, c: Case, precision=None):
     elif c.equation == "euler":
         eq = [None, th.CompressibleEulerEquations1D, th.CompressibleEulerEquations2D,
               th.CompressibleEulerEquations3D][c.ndims](c.gamma)
+    elif c.equation == "eulermulti":
+        eq = [None, th.CompressibleEulerMulticomponentEquations1D,
+              th.CompressibleEulerMulticomponentEquations2D][c.ndims](gammas=c.gammas,
+                                                                      gas_constants=c.gas_constants)
     else:
         eq = [None, th.IdealGlmMhdEquations1D, th.IdealGlmMhdEquations2D, th.IdealGlmMhdEquations3D][c.ndims](c.gamma,
                                                                                                         c.c_h)
@@ -137,3 +145,29 @@ def product_semi(th, c: Case, precision=None):
 
 def rel_max(a, b):
     return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
+
+
+# multicomponent Euler gases: equal components (the reference tests), one
+# component, distinct components
+MULTI_GASES = [((1.4, 1.4), (0.4, 0.4)), ((1.4,), (0.4,)), ((1.4, 1.648), (0.287, 1.578))]
+
+
+def multi_entropy(u, nd, gammas, gas_constants):
+    """U = -sum_k rho_k s_k, s_k = c_v,k log T - R_k log rho_k (complex-step safe)."""
+    g, R = np.array(gammas), np.array(gas_constants)
+    cv = R / (g - 1)
+    rk = u[nd + 1:]
+    rho = rk.sum()
+    v = u[:nd] / rho
+    T = (u[nd] - 0.5 * rho * np.sum(v * v)) / np.sum(rk * cv)
+    return -np.sum(rk * (cv * np.log(T) - R * np.log(rk)))
+
+
+def multi_entropy_vars(u, nd, gammas, gas_constants):
+    h = 1e-30
+    w = np.zeros(len(u))
+    for j in range(len(u)):
+        uc = u.astype(complex)
+        uc[j] += 1j * h
+        w[j] = multi_entropy(uc, nd, gammas, gas_constants).imag / h
+    return w

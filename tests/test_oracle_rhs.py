@@ -726,3 +726,55 @@ def test_mhd_shock_capturing_entropy_and_limits(oracle, nd):
     cfg = O.make_cfg(nd, O.EQ_GLM_MHD, 3, O.VI_HG, HG, O.FLUX_LLF, O.FLUX_LLF, gamma=1.4, c_h=1.0, alpha_min=0.0)
     du = O.rhs(cfg, mesh, u)
     assert entropy_rate(O, cfg, mesh, u, du) < -1e-6
+
+
+@pytest.mark.parametrize("nd,level", [(1, 3), (2, 2)])
+def test_multicomponent_euler_invariants(oracle, nd, level):
+    """Multicomponent Euler (test/tree_dgsem_{1,2}d/eulermulti_{ec,es}.jl):
+    free stream, conservation of each partial mass, momentum and energy,
+    entropy conservation with flux_ranocha throughout (equal gases, entropy
+    -sum rho_k (c_v log T - R log rho_k) with complex-step entropy variables)
+    and dissipation with an LLF surface flux."""
+    from cases import multi_entropy_vars
+    O = oracle
+    mesh = O.Mesh(nd, level, -2.0, 2.0, 3)
+    gas = ((1.4, 1.4), (0.4, 0.4))
+    kw = dict(gammas=gas[0], gas_constants=gas[1])
+    cfg = O.make_cfg(nd, O.EQ_EULER_MULTI, 3, O.VI_FD, O.FLUX_RANOCHA, O.FLUX_RANOCHA, **kw)
+    u = O.compute_coefficients(cfg, mesh, O.IC_CONSTANT)
+    assert np.abs(O.rhs(cfg, mesh, u)).max() < 1e-13
+    u = O.compute_coefficients(cfg, mesh, O.IC_SEEDED_RANDOM)
+    nv = nd + 3
+
+    def entropy_rate(du):
+        w = np.apply_along_axis(lambda x: multi_entropy_vars(x, nd, *gas), -1, u.reshape(-1, nv)).reshape(u.shape)
+        return O.integrate(cfg, mesh, np.repeat((w * du).sum(-1, keepdims=True), nv, axis=-1))[0]
+
+    for sflux, vi in ((O.FLUX_RANOCHA, O.VI_FD), (O.FLUX_LLF, O.VI_FD), (O.FLUX_LLF, O.VI_WEAK)):
+        c = O.make_cfg(nd, O.EQ_EULER_MULTI, 3, vi, O.FLUX_RANOCHA, sflux, **kw)
+        du = O.rhs(c, mesh, u)
+        scale = np.abs(du).max() * mesh.total_volume
+        assert np.abs(O.integrate(c, mesh, du)).max() < 1e-13 * scale
+        if vi == O.VI_FD and sflux == O.FLUX_RANOCHA:
+            assert abs(entropy_rate(du)) < 1e-13 * scale
+        elif vi == O.VI_FD:
+            assert entropy_rate(du) < -1e-6
+
+
+@pytest.mark.parametrize("nd", [1, 2])
+def test_multicomponent_weak_blast_state(oracle, nd):
+    """Trixi's multicomponent weak blast: partial densities 2^(k-1) / (2^n - 1)
+    of the Euler blast's density, the same velocity and pressure (checked by
+    the Euler flux of the summed state), and the single-gas case equal to the
+    Euler state."""
+    O = oracle
+    mesh = O.Mesh(nd, 3, -2.0, 2.0, 3)
+    ce = O.make_cfg(nd, O.EQ_EULER, gamma=1.4)
+    ue = O.compute_coefficients(ce, mesh, O.IC_WEAK_BLAST_WAVE)
+    for gas in (((1.4, 1.4), (0.4, 0.4)), ((1.4,), (0.4,))):
+        cm = O.make_cfg(nd, O.EQ_EULER_MULTI, gammas=gas[0], gas_constants=gas[1])
+        um = O.compute_coefficients(cm, mesh, O.IC_WEAK_BLAST_WAVE)
+        n = len(gas[0])
+        frac = np.array([2.0 ** k / (2.0 ** n - 1) for k in range(n)])
+        np.testing.assert_allclose(um[..., nd + 1:], ue[..., :1] * frac, rtol=1e-15)
+        np.testing.assert_allclose(um[..., :nd + 1], ue[..., 1:], rtol=1e-14, atol=1e-15)

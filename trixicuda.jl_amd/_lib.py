@@ -11,7 +11,7 @@ LIBRARY_PATH = os.environ.get("TRIXI_HIP_LIBRARY") or os.path.join(HERE, "libtri
 
 OK, EINVAL, EUNSUPPORTED, EDEVICE, ENOMEM, ESTATE = 0, -1, -2, -3, -4, -5
 
-EQ_LINADV, EQ_EULER, EQ_GLM_MHD, EQ_BURGERS, EQ_HYPDIFF = 0, 1, 2, 3, 4
+EQ_LINADV, EQ_EULER, EQ_GLM_MHD, EQ_BURGERS, EQ_HYPDIFF, EQ_EULER_MULTI = 0, 1, 2, 3, 4, 5
 VI_WEAK, VI_FD, VI_HG = 0, 1, 2
 FLUX_CENTRAL, FLUX_LLF, FLUX_LLF_NAIVE, FLUX_RANOCHA, FLUX_SHIMA_ETAL, FLUX_HINDENLANG_GASSNER, FLUX_EC, FLUX_HLL = range(8)
 PREC_FP64, PREC_MIXED = 0, 1
@@ -53,6 +53,10 @@ class Config(ctypes.Structure):
         ("boundary_outflow_faces", ctypes.c_int32),
         ("nu", ctypes.c_double),
         ("Lr", ctypes.c_double),
+        ("ncomponents", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+        ("gammas", ctypes.c_double * 4),
+        ("gas_constants", ctypes.c_double * 4),
     ]
 
 

@@ -30,6 +30,8 @@ struct IcParams {
     int dim, eq, nv, ic;
     double gamma, inv_gm1, c_h, a[3], length;
     uint64_t seed;
+    int nc;                          // multicomponent Euler: components
+    double cv[MAXC], cvg[MAXC];      // c_v,k and c_v,k gamma_k
 };
 
 __device__ inline double smooth_field(uint64_t seed, int field, int nd, const double* x, double L,
@@ -146,6 +148,37 @@ __device__ inline void initial_condition(const IcParams& q, const double* x, dou
         } else {
             u[0] = 2.0;
         }
+        return;
+    }
+    if (q.eq == EQ_EULER_MULTI) {
+        // (rho v, rho e, rho_1..rho_nc).  Weak blast wave (Trixi's
+        // multicomponent initial_condition_weak_blast_wave): the Euler blast
+        // with rho_k = 2^(k-1) (1 - 2) rho / (1 - 2^nc); the seeded state:
+        // rho_k = 2^(k-1) / (2^nc - 1) (1 + 0.1 U), v = 0.1 N(0, 1), p = 1 + 0.05 U.
+        double v[3] = {0, 0, 0}, rho = 1.0, p = 1.0, rk[MAXC];
+        const double den = 1.0 - (double)(1 << q.nc);
+        if (q.ic == TRIXI_HIP_IC_WEAK_BLAST_WAVE || q.ic == TRIXI_HIP_IC_WEAK_BLAST_WAVE_SEEDED)
+            weak_blast_prim(q, x, idx, rho, v, p);
+        for (int k = 0; k < q.nc; ++k) rk[k] = (double)(1 << k) * (1.0 - 2.0) * rho / den;
+        if (q.ic == TRIXI_HIP_IC_SEEDED_RANDOM) {
+            for (int k = 0; k < q.nc; ++k) rk[k] *= 1.0 + 0.1 * (2.0 * u01(q.seed, idx, 8 + k) - 1.0);
+            p = 1.0 + 0.05 * (2.0 * u01(q.seed, idx, 1) - 1.0);
+            for (int d = 0; d < q.dim; ++d) v[d] = 0.1 * normal01(q.seed, idx, 2 + 2 * d);
+        }
+        double rho_t = 0.0, h1 = 0.0, h2 = 0.0;
+        for (int k = 0; k < q.nc; ++k) {
+            rho_t += rk[k];
+            h1 += rk[k] * q.cvg[k];
+            h2 += rk[k] * q.cv[k];
+        }
+        const double gamma = h1 / h2;
+        double ke = 0.0;
+        for (int d = 0; d < q.dim; ++d) {
+            u[d] = rho_t * v[d];
+            ke += u[d] * v[d];
+        }
+        u[q.dim] = p / (gamma - 1.0) + 0.5 * ke;
+        for (int k = 0; k < q.nc; ++k) u[q.dim + 1 + k] = rk[k];
         return;
     }
     if (q.eq == EQ_LINADV) {

@@ -22,7 +22,8 @@
 
 namespace trixi_hip {
 
-enum Equation : int { EQ_LINADV = 0, EQ_EULER = 1, EQ_GLM_MHD = 2, EQ_BURGERS = 3, EQ_HYPDIFF = 4 };
+enum Equation : int { EQ_LINADV = 0, EQ_EULER = 1, EQ_GLM_MHD = 2, EQ_BURGERS = 3, EQ_HYPDIFF = 4, EQ_EULER_MULTI = 5 };
+constexpr int MAXC = 4;  // gas components of multicomponent Euler (parameter slots)
 enum VolumeIntegral : int { VI_WEAK = 0, VI_FD = 1, VI_HG = 2 };
 enum Flux : int {
     FLUX_CENTRAL = 0,
@@ -39,11 +40,13 @@ struct PhysParams {
     double gamma, inv_gm1, gm1, c_h;
     double a[3];
     double nu, inv_Tr;  // hyperbolic diffusion: diffusion constant, 1 / reference time
+    // multicomponent Euler: per component c_v = R / (gamma - 1) and c_v gamma
+    double cv[MAXC], cvg[MAXC];
 };
 
 template <typename R>
 struct PhysR {
-    R gamma, inv_gm1, gm1, c_h, a[3], nu, inv_Tr;
+    R gamma, inv_gm1, gm1, c_h, a[3], nu, inv_Tr, cv[MAXC], cvg[MAXC];
     __host__ __device__ static PhysR from(const PhysParams& p) {
         PhysR r;
         r.gamma = (R)p.gamma;
@@ -53,6 +56,10 @@ struct PhysR {
         r.nu = (R)p.nu;
         r.inv_Tr = (R)p.inv_Tr;
         for (int d = 0; d < 3; ++d) r.a[d] = (R)p.a[d];
+        for (int k = 0; k < MAXC; ++k) {
+            r.cv[k] = (R)p.cv[k];
+            r.cvg[k] = (R)p.cvg[k];
+        }
         return r;
     }
 };
@@ -488,6 +495,171 @@ struct Euler {
             return P.gm1 * (u[DIM + 1] - 0.5 * ke / u[0]);
         }
         return density_pressure(u, P);
+    }
+};
+
+// ===========================================================================
+// Multicomponent compressible Euler (Trixi CompressibleEulerMulticomponent-
+// Equations{1,2}D): (rho v_1..DIM, rho e, rho_1..rho_NC).  The mixture has
+// rho = sum rho_k and gamma = sum rho_k c_v,k gamma_k / sum rho_k c_v,k
+// (Trixi totalgamma), p = (gamma - 1)(rho e - rho |v|^2 / 2).
+// flux_ranocha is Trixi's multicomponent form: ln-means of the partial
+// densities, the Euler momentum and energy structure with the gamma of the
+// averaged state; the partial-density logarithms are taken per pair (the
+// node record stages log(rho) and log(p) only).
+// ===========================================================================
+template <int DIM, int NC>
+struct EulerMulti {
+    static constexpr int NV = DIM + 1 + NC;
+    static constexpr bool NONCONS = false;
+    static constexpr int NRF = NC + DIM + 2;  // node record: rho_k[NC], v[DIM], p, c
+    template <typename R>
+    struct Node {
+        R rk[NC], v[DIM], p, c;
+        R rho, inv_gm1;   // derived: total density, 1 / (gamma - 1)
+        double lrho, lp;  // log(rho), log(p)
+        __device__ R cons(int q) const {
+            if (q < DIM) return rho * v[q];
+            if (q == DIM) {
+                R ke = R(0);
+#pragma unroll
+                for (int d = 0; d < DIM; ++d) ke += rho * v[d] * v[d];
+                return p * inv_gm1 + R(0.5) * ke;
+            }
+            return rk[q - DIM - 1];
+        }
+    };
+
+    // sum rho_k c_v,k gamma_k and sum rho_k c_v,k
+    template <typename R>
+    __device__ static R gamma_of(const R* rk, const PhysR<R>& P) {
+        R h1 = R(0), h2 = R(0);
+#pragma unroll
+        for (int k = 0; k < NC; ++k) {
+            h1 += rk[k] * P.cvg[k];
+            h2 += rk[k] * P.cv[k];
+        }
+        return rdiv(h1, h2);
+    }
+
+    template <typename R, bool LOGS>
+    __device__ static Node<R> node(const R* u, const PhysR<R>& P) {
+        Node<R> n;
+        R rho = R(0);
+#pragma unroll
+        for (int k = 0; k < NC; ++k) {
+            n.rk[k] = u[DIM + 1 + k];
+            rho += n.rk[k];
+        }
+        n.rho = rho;
+        const R inv_rho = rdiv(R(1), rho);
+        R vsq = R(0);
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) {
+            n.v[d] = u[d] * inv_rho;
+            vsq += n.v[d] * n.v[d];
+        }
+        const R gamma = gamma_of(n.rk, P);
+        n.p = (gamma - R(1)) * (u[DIM] - R(0.5) * rho * vsq);
+        n.c = rsqrt_(gamma * n.p * inv_rho);
+        n.inv_gm1 = rdiv(R(1), gamma - R(1));
+        if (LOGS) {
+            n.lrho = log((double)n.rho);
+            n.lp = log((double)n.p);
+        } else {
+            n.lrho = n.lp = 0.0;
+        }
+        return n;
+    }
+    template <typename R>
+    __device__ static void to_rec(const Node<R>& n, R* f) {
+#pragma unroll
+        for (int k = 0; k < NC; ++k) f[k] = n.rk[k];
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) f[NC + d] = n.v[d];
+        f[NC + DIM] = n.p;
+        f[NC + DIM + 1] = n.c;
+    }
+    template <typename R>
+    __device__ static Node<R> from_rec(const R* f, double lrho, double lp, const PhysR<R>& P) {
+        Node<R> n;
+        R rho = R(0);
+#pragma unroll
+        for (int k = 0; k < NC; ++k) {
+            n.rk[k] = f[k];
+            rho += f[k];
+        }
+        n.rho = rho;
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) n.v[d] = f[NC + d];
+        n.p = f[NC + DIM];
+        n.c = f[NC + DIM + 1];
+        n.inv_gm1 = rdiv(R(1), gamma_of(n.rk, P) - R(1));
+        n.lrho = lrho;
+        n.lp = lp;
+        return n;
+    }
+    template <typename R>
+    __device__ static void flux(const Node<R>& n, int o, const PhysR<R>& P, R* f) {
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) f[d] = (d == o) ? n.rho * n.v[d] * n.v[o] + n.p : n.rho * n.v[d] * n.v[o];
+        f[DIM] = (n.cons(DIM) + n.p) * n.v[o];
+#pragma unroll
+        for (int k = 0; k < NC; ++k) f[DIM + 1 + k] = n.rk[k] * n.v[o];
+    }
+
+    template <int KIND, typename R>
+    __device__ static void two_point(const Node<R>& l, const Node<R>& r, int o, const PhysR<R>& P, R* f) {
+        if (KIND == FLUX_RANOCHA) {
+            R f_rho_sum = R(0), ravg[NC];
+            const R vo_avg = R(0.5) * (l.v[o] + r.v[o]);
+#pragma unroll
+            for (int k = 0; k < NC; ++k) {
+                const R fk = ln_mean_logs<R>(l.rk[k], r.rk[k], log((double)l.rk[k]), log((double)r.rk[k])) * vo_avg;
+                f[DIM + 1 + k] = fk;
+                f_rho_sum += fk;
+                ravg[k] = R(0.5) * (l.rk[k] + r.rk[k]);
+            }
+            const R inv_gm1 = rdiv(R(1), gamma_of(ravg, P) - R(1));
+            const R inv_rho_p_mean =
+                l.p * r.p * inv_ln_mean_logs<R>(l.rho * r.p, r.rho * l.p, l.lrho + r.lp, r.lrho + l.lp);
+            R vsq = R(0);
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) vsq += l.v[d] * r.v[d];
+            const R velocity_square_avg = R(0.5) * vsq;
+            const R p_avg = R(0.5) * (l.p + r.p);
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) {
+                const R vd = R(0.5) * (l.v[d] + r.v[d]);
+                f[d] = (d == o) ? f_rho_sum * vd + p_avg : f_rho_sum * vd;
+            }
+            f[DIM] = f_rho_sum * (velocity_square_avg + inv_rho_p_mean * inv_gm1) +
+                     R(0.5) * (l.p * r.v[o] + r.p * l.v[o]);
+        } else {
+            R fl[NV], fr[NV];
+            flux(l, o, P, fl);
+            flux(r, o, P, fr);
+            if (KIND == FLUX_LLF || KIND == FLUX_LLF_NAIVE) {
+                R lam;
+                if (KIND == FLUX_LLF_NAIVE)
+                    lam = dmax(dabs(l.v[o]), dabs(r.v[o])) + dmax(l.c, r.c);
+                else
+                    lam = dmax(dabs(l.v[o]) + l.c, dabs(r.v[o]) + r.c);
+                const R hl = R(-0.5) * lam;
+#pragma unroll
+                for (int q = 0; q < NV; ++q) f[q] = R(0.5) * (fl[q] + fr[q]) + hl * (r.cons(q) - l.cons(q));
+            } else {
+#pragma unroll
+                for (int q = 0; q < NV; ++q) f[q] = R(0.5) * (fl[q] + fr[q]);
+            }
+        }
+    }
+
+    template <typename R>
+    __device__ static void max_abs_speeds(const R* u, const PhysR<R>& P, R* lam) {
+        const Node<R> n = node<R, false>(u, P);
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) lam[d] = dabs(n.v[d]) + n.c;
     }
 };
 

@@ -19,6 +19,9 @@ struct KernelKey {
     }
 };
 
+// registry equation id of multicomponent Euler with nc components
+constexpr int multi_key(int nc) { return 100 + nc; }
+
 using LaunchFn = hipError_t (*)(const RhsArgs&, hipStream_t);
 using VjpFn = hipError_t (*)(const VjpArgs&, hipStream_t);
 using MortarFn = hipError_t (*)(const MortarArgs&, hipStream_t);

@@ -61,6 +61,7 @@ void register_mhd2d();
 void register_burgers();
 void register_mhd1d();
 void register_hypdiff();
+void register_euler_multi();
 void register_vjp_linadv();
 void register_vjp_euler1d();
 void register_vjp_euler2d();
@@ -85,6 +86,7 @@ static void ensure_registered() {
         register_burgers();
         register_mhd1d();
         register_hypdiff();
+        register_euler_multi();
         register_vjp_linadv();
         register_vjp_euler1d();
         register_vjp_euler2d();
@@ -769,6 +771,19 @@ __global__ void check_state_kernel(const double* __restrict__ u, int32_t* __rest
                 if (!isfinite(uc[v])) f |= 1;
             }
             if (eq == EQ_LINADV || eq == EQ_BURGERS || eq == EQ_HYPDIFF) continue;
+            if (eq == EQ_EULER_MULTI) {
+                // total density and the mixture pressure
+                double rho = 0.0, h1 = 0.0, h2 = 0.0, ke = 0.0;
+                for (int k = 0; k < nv - dim - 1; ++k) {
+                    rho += uc[dim + 1 + k];
+                    h1 += uc[dim + 1 + k] * phys.cvg[k];
+                    h2 += uc[dim + 1 + k] * phys.cv[k];
+                }
+                if (!(rho > 0.0)) f |= 2;
+                for (int d = 0; d < dim; ++d) ke += uc[d] * uc[d] / rho;
+                if (!((h1 / h2 - 1.0) * (uc[dim] - 0.5 * ke) > 0.0)) f |= 4;
+                continue;
+            }
             if (!(uc[0] > 0.0)) f |= 2;
             double ke = 0.0;
             const int nvel = (eq == EQ_EULER) ? dim : 3;
@@ -861,6 +876,7 @@ static const double CK_C[5] = {0.0, 1432997174477.0 / 9575080441755.0, 252626934
 
 static int nvars_of(const trixi_hip_config& c) {
     if (c.equation == TRIXI_HIP_LINEAR_SCALAR_ADVECTION || c.equation == TRIXI_HIP_INVISCID_BURGERS) return 1;
+    if (c.equation == TRIXI_HIP_COMPRESSIBLE_EULER_MULTICOMPONENT) return c.ndims + 1 + c.ncomponents;
     if (c.equation == TRIXI_HIP_COMPRESSIBLE_EULER) return c.ndims + 2;
     if (c.equation == TRIXI_HIP_HYPERBOLIC_DIFFUSION) return c.ndims + 1;
     return c.ndims == 1 ? 8 : 9;  // ideal MHD: no cleaning field in 1D
@@ -870,7 +886,8 @@ static KernelKey make_key(const trixi_hip_config& c, int rk, bool mortars = fals
     KernelKey k;
     k.dim = c.ndims;
     k.N = c.polydeg + 1;
-    k.eq = c.equation;
+    // multicomponent Euler: one kernel family per component count
+    k.eq = (c.equation == TRIXI_HIP_COMPRESSIBLE_EULER_MULTICOMPONENT) ? multi_key(c.ncomponents) : c.equation;
     k.vol = c.volume_integral;
     k.vf = (c.volume_integral == TRIXI_HIP_VOLUME_WEAK_FORM) ? FLUX_CENTRAL : c.volume_flux;
     k.sf = c.surface_flux;
@@ -1021,8 +1038,20 @@ int trixi_hip_create(const trixi_hip_config* config, const trixi_hip_mesh* mesh,
     const trixi_hip_config& cfg = *config;
     if (cfg.ndims < 1 || cfg.ndims > 3) return fail(TRIXI_HIP_EINVAL, "ndims must be 1, 2 or 3");
     if (cfg.polydeg < 1 || cfg.polydeg + 1 > MAXN) return fail(TRIXI_HIP_EINVAL, "polydeg out of range");
+    if (cfg.equation < TRIXI_HIP_LINEAR_SCALAR_ADVECTION || cfg.equation > TRIXI_HIP_COMPRESSIBLE_EULER_MULTICOMPONENT)
+        return fail(TRIXI_HIP_EINVAL, "unknown equation");
     if (cfg.equation == TRIXI_HIP_INVISCID_BURGERS && cfg.ndims != 1)
         return fail(TRIXI_HIP_EUNSUPPORTED, "the inviscid Burgers equation is built for 1D");
+    if (cfg.equation == TRIXI_HIP_COMPRESSIBLE_EULER_MULTICOMPONENT) {
+        if (cfg.ndims > 2) return fail(TRIXI_HIP_EUNSUPPORTED, "multicomponent Euler is built for 1D and 2D");
+        if (cfg.ncomponents < 1 || cfg.ncomponents > 2)
+            return fail(TRIXI_HIP_EUNSUPPORTED, "multicomponent Euler is built for 1 or 2 components");
+        for (int k = 0; k < cfg.ncomponents; ++k)
+            if (!(cfg.gammas[k] > 1.0) || !(cfg.gas_constants[k] > 0.0))
+                return fail(TRIXI_HIP_EINVAL, "multicomponent Euler needs gamma_k > 1 and R_k > 0");
+        if (cfg.boundary_condition != TRIXI_HIP_BC_PERIODIC || cfg.source_terms != TRIXI_HIP_SOURCE_NONE)
+            return fail(TRIXI_HIP_EUNSUPPORTED, "multicomponent Euler is built for periodic meshes without sources");
+    }
     if (cfg.volume_integral == TRIXI_HIP_VOLUME_SHOCK_CAPTURING_HG && cfg.equation != TRIXI_HIP_COMPRESSIBLE_EULER &&
         cfg.equation != TRIXI_HIP_INVISCID_BURGERS && cfg.equation != TRIXI_HIP_IDEAL_GLM_MHD)
         return fail(TRIXI_HIP_EUNSUPPORTED, "shock capturing is built for compressible Euler, Burgers and GLM-MHD");
@@ -1251,6 +1280,13 @@ int trixi_hip_create(const trixi_hip_config* config, const trixi_hip_mesh* mesh,
         a.phys.nu = cfg.nu;
         a.phys.inv_Tr = cfg.nu / (cfg.Lr * cfg.Lr);  // T_r = Lr^2 / nu
     }
+    if (cfg.equation == TRIXI_HIP_COMPRESSIBLE_EULER_MULTICOMPONENT) {
+        // c_v,k = R_k / (gamma_k - 1) (Trixi's cv), and c_v,k gamma_k
+        for (int k = 0; k < cfg.ncomponents; ++k) {
+            a.phys.cv[k] = cfg.gas_constants[k] / (cfg.gammas[k] - 1.0);
+            a.phys.cvg[k] = a.phys.cv[k] * cfg.gammas[k];
+        }
+    }
     for (int d = 0; d < 3; ++d) a.phys.a[d] = cfg.advection_velocity[d];
     a.bc_ic = ic_params(c, cfg.boundary_ic, cfg.boundary_seed);
     for (int q = 0; q < N * N; ++q) {
@@ -1362,6 +1398,11 @@ static IcParams ic_params(const trixi_hip_ctx* c, int ic, uint64_t seed) {
     for (int d = 0; d < 3; ++d) q.a[d] = c->cfg.advection_velocity[d];
     q.length = c->cfg.ic_length;
     q.seed = seed;
+    q.nc = c->cfg.equation == TRIXI_HIP_COMPRESSIBLE_EULER_MULTICOMPONENT ? c->cfg.ncomponents : 0;
+    for (int k = 0; k < MAXC; ++k) {
+        q.cv[k] = c->args.phys.cv[k];
+        q.cvg[k] = c->args.phys.cvg[k];
+    }
     return q;
 }
 
@@ -1501,6 +1542,17 @@ int trixi_hip_max_dt(trixi_hip_ctx* ctx, const double* u, double* max_dt, trixi_
         MS(Burgers, 1);
     } else if (eq == TRIXI_HIP_HYPERBOLIC_DIFFUSION) {
         if (dim == 1) MS(HypDiff<1>, 1); else if (dim == 2) MS(HypDiff<2>, 2); else MS(HypDiff<3>, 3);
+    } else if (eq == TRIXI_HIP_COMPRESSIBLE_EULER_MULTICOMPONENT) {
+        using M11 = EulerMulti<1, 1>;
+        using M12 = EulerMulti<1, 2>;
+        using M21 = EulerMulti<2, 1>;
+        using M22 = EulerMulti<2, 2>;
+        const int nc = ctx->cfg.ncomponents;
+        if (dim == 1) {
+            if (nc == 1) MS(M11, 1); else MS(M12, 1);
+        } else {
+            if (nc == 1) MS(M21, 2); else MS(M22, 2);
+        }
     } else {
         if (dim == 1) MS(GlmMhd1D, 1); else if (dim == 2) MS(GlmMhd, 2); else MS(GlmMhd, 3);
     }

@@ -28,6 +28,8 @@ class AbstractEquations:
             return self.ndims + 1
         if self.equation_id == L.EQ_EULER:
             return self.ndims + 2
+        if self.equation_id == L.EQ_EULER_MULTI:
+            return self.ndims + 1 + len(self.gammas)
         return 9 if self.ndims > 1 else 8  # ideal MHD: no cleaning field in 1D
 
 
@@ -71,6 +73,37 @@ def _euler(ndims):
 CompressibleEulerEquations1D = _euler(1)
 CompressibleEulerEquations2D = _euler(2)
 CompressibleEulerEquations3D = _euler(3)
+
+
+def _euler_multi(ndims):
+    @dataclass(frozen=True)
+    class _EulerMulti(AbstractEquations):
+        """(rho v_1..d, rho e, rho_1..rho_n): n gas components with their own
+        gamma_k and R_k (Trixi CompressibleEulerMulticomponentEquations{1,2}D;
+        keyword arguments, a scalar meaning one component)."""
+        gammas: tuple = ()
+        gas_constants: tuple = ()
+
+        def __init__(self, *, gammas, gas_constants):
+            g = tuple(float(x) for x in (gammas if hasattr(gammas, "__len__") else (gammas,)))
+            r = tuple(float(x) for x in (gas_constants if hasattr(gas_constants, "__len__") else (gas_constants,)))
+            if len(g) != len(r) or not g:
+                raise ValueError("gammas and gas_constants must have the same (non-zero) length")
+            object.__setattr__(self, "ndims", ndims)
+            object.__setattr__(self, "equation_id", L.EQ_EULER_MULTI)
+            object.__setattr__(self, "gammas", g)
+            object.__setattr__(self, "gas_constants", r)
+
+        @property
+        def ncomponents(self):
+            return len(self.gammas)
+
+    _EulerMulti.__name__ = _EulerMulti.__qualname__ = f"CompressibleEulerMulticomponentEquations{ndims}D"
+    return _EulerMulti
+
+
+CompressibleEulerMulticomponentEquations1D = _euler_multi(1)
+CompressibleEulerMulticomponentEquations2D = _euler_multi(2)
 
 
 def _glm_mhd(ndims):

@@ -42,6 +42,10 @@ struct CConfig
     boundary_outflow_faces::Int32
     nu::Float64
     Lr::Float64
+    ncomponents::Int32
+    reserved::Int32
+    gammas::NTuple{4, Float64}
+    gas_constants::NTuple{4, Float64}
 end
 
 struct CMesh
@@ -79,6 +83,8 @@ equation_id(::IdealGlmMhdEquations1D) = 2
 equation_id(::InviscidBurgersEquation1D) = 3
 equation_id(::Union{HyperbolicDiffusionEquations1D, HyperbolicDiffusionEquations2D,
                     HyperbolicDiffusionEquations3D}) = 4
+equation_id(::Union{CompressibleEulerMulticomponentEquations1D,
+                    CompressibleEulerMulticomponentEquations2D}) = 5
 equation_id(eq) = throw(ArgumentError("equation $(typeof(eq)) is not built for HIP"))
 
 flux_id(::typeof(flux_central)) = 0
@@ -207,11 +213,17 @@ function make_config(solver::DGSEMHIP, equations, mesh; boundary_conditions = bo
     vel = ntuple(d -> d <= length(a) ? Float64(a[d]) : 0.0, 3)
     len = mesh.tree.length_level_0
     bc, bc_ic, bc_seed, outflow = boundary_config(boundary_conditions)
+    # multicomponent Euler: per-component gamma_k and R_k
+    gammas = hasproperty(equations, :gammas) ? Tuple(equations.gammas) : ()
+    gas_r = hasproperty(equations, :gas_constants) ? Tuple(equations.gas_constants) : ()
     CConfig(ndims(mesh), equation_id(equations), solver.polydeg, volume_integral, vflux,
             flux_id(solver.surface_flux), fvflux, solver.precision === :mixed ? 1 : 0,
             smooth, var, gamma, c_h, vel, amax, amin, len, bc, bc_ic, bc_seed, source_id(source_terms), outflow,
             hasproperty(equations, :nu) ? Float64(equations.nu) : 0.0,
-            hasproperty(equations, :Lr) ? Float64(equations.Lr) : 0.0)
+            hasproperty(equations, :Lr) ? Float64(equations.Lr) : 0.0,
+            Int32(length(gammas)), Int32(0),
+            ntuple(k -> k <= length(gammas) ? Float64(gammas[k]) : 0.0, 4),
+            ntuple(k -> k <= length(gas_r) ? Float64(gas_r[k]) : 0.0, 4))
 end
 
 function SemidiscretizationHyperbolicHIP(mesh::TreeMesh, equations, initial_condition,

@@ -133,6 +133,12 @@ class DGSEMHIP:
         c.ic_length = ic_length
         c.nu = getattr(equations, "nu", 0.0)
         c.Lr = getattr(equations, "Lr", 0.0)
+        gammas = getattr(equations, "gammas", ())
+        if len(gammas) > 4:
+            raise NotImplementedError("at most four gas components")
+        c.ncomponents = len(gammas)
+        for k, (g, r) in enumerate(zip(gammas, getattr(equations, "gas_constants", ()))):
+            c.gammas[k], c.gas_constants[k] = g, r
         return c
 
     def __repr__(self):
