@@ -53,9 +53,11 @@ enum trixi_hip_equation {
                                               8 variables, no psi / Powell terms)  */
     TRIXI_HIP_INVISCID_BURGERS = 3,        /* InviscidBurgersEquation1D             */
     TRIXI_HIP_HYPERBOLIC_DIFFUSION = 4,    /* HyperbolicDiffusionEquations{1,2,3}D  */
-    TRIXI_HIP_COMPRESSIBLE_EULER_MULTICOMPONENT = 5 /* CompressibleEulerMulticomponent-
+    TRIXI_HIP_COMPRESSIBLE_EULER_MULTICOMPONENT = 5, /* CompressibleEulerMulticomponent-
                                               Equations{1,2}D: (rho v, rho e, rho_1..
                                               rho_ncomponents), 1 or 2 components  */
+    TRIXI_HIP_COMPRESSIBLE_EULER_QUASI1D = 6 /* CompressibleEulerEquationsQuasi1D:
+                                              (a rho, a rho v, a e, a)              */
 };
 enum trixi_hip_volume_integral {
     TRIXI_HIP_VOLUME_WEAK_FORM = 0,          /* VolumeIntegralWeakForm          */
@@ -70,8 +72,10 @@ enum trixi_hip_flux {
     TRIXI_HIP_FLUX_SHIMA_ETAL = 4,
     TRIXI_HIP_FLUX_HINDENLANG_GASSNER = 5,   /* with flux_nonconservative_powell       */
     TRIXI_HIP_FLUX_EC = 6,                   /* flux_ec (InviscidBurgersEquation1D)    */
-    TRIXI_HIP_FLUX_HLL = 7                   /* flux_hll = FluxHLL(min_max_speed_davis),
+    TRIXI_HIP_FLUX_HLL = 7,                  /* flux_hll = FluxHLL(min_max_speed_davis),
                                                 compressible Euler                     */
+    TRIXI_HIP_FLUX_CHAN_ETAL = 8             /* (flux_chan_etal,
+                                                flux_nonconservative_chan_etal), quasi-1D */
 };
 enum trixi_hip_precision {
     TRIXI_HIP_FP64 = 0,  /* all arithmetic in FP64                                    */
@@ -117,7 +121,8 @@ enum trixi_hip_boundary_condition {
  * dg_2d.jl:746-757; source_terms_kernel!). */
 enum trixi_hip_source_terms {
     TRIXI_HIP_SOURCE_NONE = 0,
-    TRIXI_HIP_SOURCE_CONVERGENCE_TEST = 1, /* source_terms_convergence_test (Euler, Burgers) */
+    TRIXI_HIP_SOURCE_CONVERGENCE_TEST = 1, /* source_terms_convergence_test (Euler, Burgers,
+                                              quasi-1D Euler)                            */
     TRIXI_HIP_SOURCE_EOC_TEST_EULER = 2,   /* source_terms_eoc_test_euler (Euler 2D/3D)     */
     TRIXI_HIP_SOURCE_POISSON_NONPERIODIC = 3, /* hyperbolic diffusion: f of the steady state,
                                              -q / T_r                                  */

@@ -210,8 +210,30 @@ template <class T> void multi_prim(const Cfg& c, const T* u, T& rho, T* v, T& p,
     p = (gamma - 1.0) * (u[c.nd] - 0.5 * rho * vsq);
 }
 
+// Quasi-1D Euler (Trixi CompressibleEulerEquationsQuasi1D): u = (a rho,
+// a rho v, a e, a); cons2prim divides by a and applies the Euler cons2prim.
+template <class T> void quasi_prim(const Cfg& c, const T* u, T& rho, T& v, T& p, T& a) {
+    a = u[3];
+    rho = u[0] / a;
+    const T rho_v = u[1] / a, rho_e = u[2] / a;
+    v = rho_v / rho;
+    p = (c.gamma - 1.0) * (rho_e - 0.5 * rho_v * v);
+}
+
 template <class T> void phys_flux(const Cfg& c, const T* u, int o, T* f) {
     const int oi = o - 1;
+    if (c.eq == OR_EQ_EULER_QUASI1D) {
+        // Trixi flux: (a rho v, a rho v^2, a v (e + p), 0), e = (a e) / a; the
+        // pressure gradient is the nonconservative term a p_x
+        T rho, v, p, a;
+        quasi_prim(c, u, rho, v, p, a);
+        const T e = u[2] / a;
+        f[0] = u[1];
+        f[1] = u[1] * v;
+        f[2] = a * v * (e + p);
+        f[3] = 0.0;
+        return;
+    }
     if (c.eq == OR_EQ_EULER_MULTI) {
         T rho, v[3], p, gamma;
         multi_prim(c, u, rho, v, p, gamma);
@@ -295,6 +317,10 @@ template <class T> void normal_speed(const Cfg& c, const T* u, int o, T& vn, T& 
         multi_prim(c, u, rho, v, p, gamma);
         vn = v[oi];
         cs = tsqrt(gamma * p / rho);
+    } else if (c.eq == OR_EQ_EULER_QUASI1D) {
+        T rho, p, a;
+        quasi_prim(c, u, rho, vn, p, a);
+        cs = tsqrt(c.gamma * p / rho);
     } else {
         vn = u[1 + oi] / u[0];
         cs = mhd_fast_wavespeed(c, u, o);
@@ -374,6 +400,24 @@ template <class T> void flux_ranocha_multi(const Cfg& c, const T* ull, const T* 
     f[nd] = f_rho_sum * (velocity_square_avg + inv_rho_p_mean * inv_gm1) + 0.5 * (pl * vr[oi] + pr * vl[oi]);
 }
 
+// Trixi flux_chan_etal (CompressibleEulerEquationsQuasi1D; Chan, Shukla, Wu,
+// Liu, Nalluri 2023): ln-mean density times the mean of a v, Ranocha's
+// energy flux with a inside the pressure work term; the a component is 0.
+template <class T> void flux_chan_etal(const Cfg& c, const T* ull, const T* urr, T* f) {
+    T rl, vl, pl, al, rr, vr, pr, ar;
+    quasi_prim(c, ull, rl, vl, pl, al);
+    quasi_prim(c, urr, rr, vr, pr, ar);
+    const T rho_mean = ln_mean(rl, rr);
+    const T inv_rho_p_mean = pl * pr * inv_ln_mean(rl * pr, rr * pl);
+    const T v1_avg = 0.5 * (vl + vr);
+    const T a_v1_avg = 0.5 * (al * vl + ar * vr);
+    const T velocity_square_avg = 0.5 * (vl * vr);
+    f[0] = rho_mean * a_v1_avg;
+    f[1] = rho_mean * a_v1_avg * v1_avg;
+    f[2] = f[0] * (velocity_square_avg + inv_rho_p_mean * c.inv_gm1) + 0.5 * (pl * ar * vr + pr * al * vl);
+    f[3] = 0.0;
+}
+
 template <class T> void flux_shima_euler(const Cfg& c, const T* ull, const T* urr, int o, T* f) {
     const int oi = o - 1;
     T rl, vl[3], pl, rr, vr[3], pr;
@@ -433,7 +477,18 @@ void flux_hindenlang_gassner_mhd(const Cfg& c, const T* ull, const T* urr, int o
 // Powell term (0, B, v.B, v, 0) and Galilean GLM term, multiplied by the
 // right state's normal B (resp. psi); callers apply the factor 0.5.
 template <class T> void noncons_powell(const Cfg& c, const T* ull, const T* urr, int o, T* f) {
-    (void)c;
+    if (c.eq == OR_EQ_EULER_QUASI1D) {
+        // flux_nonconservative_chan_etal: (0, a_ll p_rr, 0, 0), the pressure
+        // gradient a p_x in the same left-state-times-right-state form
+        T rl, vl, pl, al, rr, vr, pr, ar;
+        quasi_prim(c, ull, rl, vl, pl, al);
+        quasi_prim(c, urr, rr, vr, pr, ar);
+        f[0] = 0.0;
+        f[1] = al * pr;
+        f[2] = 0.0;
+        f[3] = 0.0;
+        return;
+    }
     const int oi = o - 1;
     T rho_ll = ull[0];
     T v_ll[3] = {ull[1] / rho_ll, ull[2] / rho_ll, ull[3] / rho_ll};
@@ -460,6 +515,9 @@ template <class T> void two_point(const Cfg& c, int kind, const T* ull, const T*
             return;
         case OR_FLUX_SHIMA_ETAL:
             flux_shima_euler(c, ull, urr, o, f);
+            return;
+        case OR_FLUX_CHAN_ETAL:
+            flux_chan_etal(c, ull, urr, f);
             return;
         case OR_FLUX_HINDENLANG_GASSNER:
             flux_hindenlang_gassner_mhd(c, ull, urr, o, f);
@@ -509,7 +567,7 @@ template <class T> void two_point(const Cfg& c, int kind, const T* ull, const T*
 }
 
 // (1D ideal MHD has no nonconservative terms: B1 is constant)
-inline bool has_noncons(const Cfg& c) { return c.eq == OR_EQ_GLM_MHD && c.nd > 1; }
+inline bool has_noncons(const Cfg& c) { return (c.eq == OR_EQ_GLM_MHD && c.nd > 1) || c.eq == OR_EQ_EULER_QUASI1D; }
 
 // ----------------------------------------------------------------------------
 // Basis (Trixi.jl basis_lobatto_legendre.jl; used by the reference at
@@ -977,6 +1035,28 @@ void source_convergence_test(const Cfg& c, const double* x, double t, double* sv
         sv[d + 1] = rhox * (0.5 * d * (d - 1) - ((d - 1) * c.inv_gm1 + d) * C_grav * rho);
         return;
     }
+    if (c.eq == OR_EQ_EULER_QUASI1D) {
+        // Trixi's quasi-1D source_terms_convergence_test: the residual of the
+        // manufactured state rho = ini, v = 1, e = ini^2 / rho,
+        // p = (gamma - 1)(e - rho v^2 / 2), a = 1.5 - 0.5 cos(pi x):
+        //   du1 = (a rho)_t + (a rho v)_x, du2 = (a rho v)_t + (a rho v v)_x + a p_x,
+        //   du3 = (a e)_t + (a v (e + p))_x,
+        // by the product rule with ini_t = -w A cos, ini_x = w A cos (w = pi, A = 0.1)
+        const double A = 0.1, w = M_PI, arg = w * (x[0] - t);
+        const double ini = 2.0 + A * sin(arg), ini_x = w * A * cos(arg), ini_t = -ini_x;
+        const double rho = ini, v = 1.0, e = ini * ini / rho;
+        const double rho_x = ini_x, rho_t = ini_t;
+        const double e_x = 2.0 * ini * ini_x / rho - ini * ini * rho_x / (rho * rho);
+        const double e_t = 2.0 * ini * ini_t / rho - ini * ini * rho_t / (rho * rho);
+        const double p = (c.gamma - 1.0) * (e - 0.5 * rho * v * v);
+        const double p_x = (c.gamma - 1.0) * (e_x - 0.5 * rho_x * v * v);
+        const double a = 1.5 - 0.5 * cos(x[0] * M_PI), a_x = 0.5 * M_PI * sin(x[0] * M_PI);
+        sv[0] = a * rho_t + (a_x * rho * v + a * rho_x * v);
+        sv[1] = a * rho_t * v + (a_x * rho * v * v + a * rho_x * v * v) + a * p_x;
+        sv[2] = a * e_t + (a_x * v * (e + p) + a * v * (e_x + p_x));
+        sv[3] = 0.0;
+        return;
+    }
     if (c.eq == OR_EQ_BURGERS) {
         // u = 2 + sin(2 pi (x - t)): u_t + u u_x = 2 pi cos(.) (u - 1)
         const double w = 2.0 * M_PI, arg = w * (x[0] - t);
@@ -1430,6 +1510,32 @@ void initial_condition(const Cfg& c, int ic, const double* x, double t, uint64_t
         }
         return;
     }
+    if (c.eq == OR_EQ_EULER_QUASI1D) {
+        // Trixi's quasi-1D initial_condition_convergence_test (c = 2, A = 0.1,
+        // L = 2, omega = 2 pi / L); seeded: rho = 1 + 0.1 U, p = 1 + 0.05 U,
+        // v = 0.1 N(0, 1), a = 1 + 0.2 U; prim2cons (a rho, a rho v, a (p/(gamma-1)
+        // + rho v^2 / 2), a)
+        double rho = 1.0, v = 0.0, p = 1.0, a = 1.0;
+        if (ic == OR_IC_CONVERGENCE_TEST) {
+            const double cc = 2.0, A = 0.1, L = 2.0, f = 1.0 / L, omega = 2.0 * M_PI * f;
+            const double ini = cc + A * sin(omega * (x[0] - t));
+            rho = ini;
+            v = 1.0;
+            const double e = ini * ini / rho;
+            p = (c.gamma - 1.0) * (e - 0.5 * rho * v * v);
+            a = 1.5 - 0.5 * cos(x[0] * M_PI);
+        } else if (ic == OR_IC_SEEDED_RANDOM) {
+            rho = 1.0 + 0.1 * (2.0 * u01(seed, idx, 0) - 1.0);
+            p = 1.0 + 0.05 * (2.0 * u01(seed, idx, 1) - 1.0);
+            v = 0.1 * normal01(seed, idx, 2);
+            a = 1.0 + 0.2 * (2.0 * u01(seed, idx, 4) - 1.0);
+        }
+        u[0] = a * rho;
+        u[1] = a * rho * v;
+        u[2] = a * (p * c.inv_gm1 + 0.5 * rho * v * v);
+        u[3] = a;
+        return;
+    }
     if (c.eq == OR_EQ_EULER_MULTI) {
         // Trixi's multicomponent initial_condition_weak_blast_wave: the Euler
         // blast with rho_k = 2^(k-1) (1 - 2) rho / (1 - 2^nc) (prim2cons with
@@ -1581,6 +1687,7 @@ int oracle_nvars(const oracle_cfg* cfg) {
     if (cfg->equation == OR_EQ_HYPDIFF) return cfg->ndims + 1;
     if (cfg->equation == OR_EQ_EULER) return cfg->ndims + 2;
     if (cfg->equation == OR_EQ_EULER_MULTI) return cfg->ndims + 1 + cfg->ncomponents;
+    if (cfg->equation == OR_EQ_EULER_QUASI1D) return 4;
     return cfg->ndims == 1 ? 8 : 9;  // ideal MHD: (rho, rho v, rho e, B) in 1D, + psi in 2D/3D
 }
 
@@ -1686,6 +1793,10 @@ void oracle_max_abs_speeds(const oracle_cfg* cfg, const double* u, double* lam) 
             double rho, v[3], p, gamma;
             multi_prim(c, u, rho, v, p, gamma);
             lam[d] = fabs(v[d]) + sqrt(gamma * p / rho);
+        } else if (c.eq == OR_EQ_EULER_QUASI1D) {
+            double rho, v, p, a;
+            quasi_prim(c, u, rho, v, p, a);
+            lam[d] = fabs(v) + sqrt(c.gamma * p / rho);
         } else {
             lam[d] = fabs(u[1 + d] / u[0]) + mhd_fast_wavespeed(c, u, d + 1);
         }

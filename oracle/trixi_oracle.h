@@ -26,7 +26,8 @@ extern "C" {
 #endif
 
 enum { OR_EQ_LINADV = 0, OR_EQ_EULER = 1, OR_EQ_GLM_MHD = 2, OR_EQ_BURGERS = 3, OR_EQ_HYPDIFF = 4,
-       OR_EQ_EULER_MULTI = 5 /* (rho v, rho e, rho_1..rho_nc) */ };
+       OR_EQ_EULER_MULTI = 5 /* (rho v, rho e, rho_1..rho_nc) */,
+       OR_EQ_EULER_QUASI1D = 6 /* (a rho, a rho v, a e, a) */ };
 enum { OR_VI_WEAK = 0, OR_VI_FD = 1, OR_VI_HG = 2 };
 enum {
     OR_FLUX_CENTRAL = 0,
@@ -36,7 +37,8 @@ enum {
     OR_FLUX_SHIMA_ETAL = 4,
     OR_FLUX_HINDENLANG_GASSNER = 5,
     OR_FLUX_EC = 6,           /* Burgers flux_ec                             */
-    OR_FLUX_HLL = 7           /* FluxHLL(min_max_speed_davis), Euler         */
+    OR_FLUX_HLL = 7,          /* FluxHLL(min_max_speed_davis), Euler         */
+    OR_FLUX_CHAN_ETAL = 8     /* quasi-1D: flux_chan_etal + flux_nonconservative_chan_etal */
 };
 enum {
     OR_IC_CONSTANT = 0,

@@ -9,7 +9,7 @@ import numpy as np
 class Case:
     name: str
     ndims: int
-    equation: str           # "linadv" | "euler" | "mhd" | "burgers" | "hypdiff" | "eulermulti"
+    equation: str           # "linadv" | "euler" | "mhd" | "burgers" | "hypdiff" | "eulermulti" | "quasi1d"
     level: int
     vi: str = "weak"        # "weak" | "fd" | "hg"
     vflux: str = "central"
@@ -37,9 +37,10 @@ class Case:
     nu: float = 1.0         # hyperbolic diffusion constant
     gammas: tuple = (1.4, 1.4)          # multicomponent Euler
     gas_constants: tuple = (0.4, 0.4)
+    ic_fn: object = None    # initial condition as a Python function (x, t, equations) -> conservative state
 
 
-FLUX_NAMES = ("central", "llf", "llf_naive", "ranocha", "shima_etal", "hindenlang_gassner", "ec", "hll")
+FLUX_NAMES = ("central", "llf", "llf_naive", "ranocha", "shima_etal", "hindenlang_gassner", "ec", "hll", "chan_etal")
 IC_NAMES = ("constant", "convergence_test", "weak_blast_wave", "seeded_random", "smooth_periodic",
             "weak_blast_wave_seeded", "shock", "rarefaction", "eoc_gravity", "poisson_nonperiodic",
             "harmonic_nonperiodic")
@@ -56,7 +57,7 @@ def oracle_mesh(O, c: Case):
 
 def oracle_cfg(O, c: Case):
     eq = {"linadv": O.EQ_LINADV, "euler": O.EQ_EULER, "mhd": O.EQ_GLM_MHD, "burgers": O.EQ_BURGERS,
-          "hypdiff": O.EQ_HYPDIFF, "eulermulti": O.EQ_EULER_MULTI}[c.equation]
+          "hypdiff": O.EQ_HYPDIFF, "eulermulti": O.EQ_EULER_MULTI, "quasi1d": O.EQ_EULER_QUASI1D}[c.equation]
     vi = {"weak": O.VI_WEAK, "fd": O.VI_FD, "hg": O.VI_HG}[c.vi]
     ext = dict(boundary_condition=O.BC_DIRICHLET if c.bc_state else O.BC_PERIODIC,
                boundary_ic=IC_NAMES.index(c.bc_state) if c.bc_state else 0,
@@ -75,16 +76,21 @@ def oracle_ic(O, c: Case):
     return IC_NAMES.index(c.ic)
 
 
-def _flux(th, name, mhd):
+def _flux(th, name, pair):
     f = {"central": th.flux_central, "llf": th.flux_lax_friedrichs,
          "llf_naive": th.FluxLaxFriedrichs(th.max_abs_speed_naive), "ranocha": th.flux_ranocha,
          "shima_etal": th.flux_shima_etal, "hindenlang_gassner": th.flux_hindenlang_gassner, "ec": th.flux_ec,
-         "hll": th.flux_hll}[name]
-    return (f, th.flux_nonconservative_powell) if mhd else f
+         "hll": th.flux_hll, "chan_etal": th.flux_chan_etal}[name]
+    if pair == "mhd":
+        return (f, th.flux_nonconservative_powell)
+    if pair == "quasi1d":
+        return (f, th.flux_nonconservative_chan_etal)
+    return f
 
 
 def product_semi(th, c: Case, precision=None):
-    mhd = c.equation == "mhd" and c.ndims > 1  # (flux, flux_nonconservative_powell) pairs
+    # (flux, nonconservative flux) pairs
+    mhd = "mhd" if (c.equation == "mhd" and c.ndims > 1) else ("quasi1d" if c.equation == "quasi1d" else None)
     if c.equation == "linadv":
         eq = [None, th.LinearScalarAdvectionEquation1D, th.LinearScalarAdvectionEquation2D,
               th.LinearScalarAdvectionEquation3D][c.ndims](c.advection[:c.ndims])
@@ -96,6 +102,8 @@ def product_semi(th, c: Case, precision=None):
     elif c.equation == "euler":
         eq = [None, th.CompressibleEulerEquations1D, th.CompressibleEulerEquations2D,
               th.CompressibleEulerEquations3D][c.ndims](c.gamma)
+    elif c.equation == "quasi1d":
+        eq = th.CompressibleEulerEquationsQuasi1D(c.gamma)
     elif c.equation == "eulermulti":
         eq = [None, th.CompressibleEulerMulticomponentEquations1D,
               th.CompressibleEulerMulticomponentEquations2D][c.ndims](gammas=c.gammas,
@@ -140,7 +148,7 @@ def product_semi(th, c: Case, precision=None):
                               "eoc_test_euler": th.source_terms_eoc_test_euler,
                               "poisson_nonperiodic": th.source_terms_poisson_nonperiodic,
                               "harmonic": th.source_terms_harmonic}[c.source]
-    return th.SemidiscretizationHyperbolicHIP(mesh, eq, ics[c.ic], solver, **kw)
+    return th.SemidiscretizationHyperbolicHIP(mesh, eq, c.ic_fn if c.ic_fn is not None else ics[c.ic], solver, **kw)
 
 
 def rel_max(a, b):
@@ -171,3 +179,40 @@ def multi_entropy_vars(u, nd, gammas, gas_constants):
         uc[j] += 1j * h
         w[j] = multi_entropy(uc, nd, gammas, gas_constants).imag / h
     return w
+
+
+# initial conditions written as functions in the reference's test files
+def eulerquasi_ec_state(x, t, equations):
+    """initial_condition_ec of test/tree_dgsem_1d/eulerquasi_ec.jl: v = 0.1,
+    rho = 2 + 0.1 x, p = 3, a = 2 + x, as (a rho, a rho v, a e, a)."""
+    v1, rho, p, a = 0.1, 2.0 + 0.1 * x[0], 3.0, 2.0 + x[0]
+    return [a * rho, a * rho * v1, a * (p / (equations.gamma - 1) + 0.5 * rho * v1 ** 2), a]
+
+
+def euler_blast_wave_1d_state(x, t, equations):
+    """initial_condition_blast_wave of test/tree_dgsem_1d/euler_blast_wave.jl:
+    inside |x| <= 0.5 rho = 1.1691, v = 0.1882 sign(x), p = 1.245; outside
+    rho = 1, v = 0, p = 1e-3."""
+    r = abs(x[0])
+    cos_phi = 1.0 if x[0] > 0 else -1.0
+    rho = 1.0 if r > 0.5 else 1.1691
+    v1 = 0.0 if r > 0.5 else 0.1882 * cos_phi
+    p = 1.0e-3 if r > 0.5 else 1.245
+    return [rho, rho * v1, p / (equations.gamma - 1) + 0.5 * rho * v1 ** 2]
+
+
+def oracle_state(O, c, mesh, t=0.0):
+    """The case's initial state on the oracle mesh: the oracle's initial
+    condition, or the case's function at the node coordinates (in 1D with
+    the element end nodes one ulp inwards, as Trixi samples them)."""
+    if c.ic_fn is None:
+        return O.compute_coefficients(oracle_cfg(O, c), mesh, oracle_ic(O, c), t, c.seed)
+    from types import SimpleNamespace
+    eq = SimpleNamespace(gamma=c.gamma)
+    x = mesh.node_coordinates.copy()
+    if c.ndims == 1:
+        x[:, 0, 0] = np.nextafter(x[:, 0, 0], np.inf)
+        x[:, -1, 0] = np.nextafter(x[:, -1, 0], -np.inf)
+    pts = x.reshape(-1, c.ndims)
+    u = np.array([c.ic_fn(tuple(p), t, eq) for p in pts], dtype=np.float64)
+    return u.reshape(x.shape[:-1] + (u.shape[-1],))

@@ -7,7 +7,8 @@ relative max-norm of du <= 2e-5 against the FP64 oracle (stated per case).
 import numpy as np
 import pytest
 
-from cases import oracle_mesh, Case, oracle_cfg, oracle_ic, product_semi, rel_max
+from cases import (oracle_mesh, Case, oracle_cfg, oracle_ic, oracle_state, product_semi, rel_max, eulerquasi_ec_state,
+                   euler_blast_wave_1d_state)
 
 pytestmark = pytest.mark.gpu
 
@@ -30,7 +31,7 @@ def run_pair(th, O, torch, case, precision=None):
     """(du_gpu in Trixi layout, du_oracle, u) for one case."""
     cfg = oracle_cfg(O, case)
     mesh = oracle_mesh(O, case)
-    u = O.compute_coefficients(cfg, mesh, oracle_ic(O, case), case.t, case.seed)
+    u = oracle_state(O, case, mesh, case.t)
     du_ref = O.rhs(cfg, mesh, u, t=case.t)
     semi = product_semi(th, case, precision)
     u_dev = th.pack(torch.from_numpy(u).cuda(), semi)
@@ -71,6 +72,22 @@ MULTI_CASES = [
     Case("eulermulti2d_weak_single_random", 2, "eulermulti", 3, "weak", "central", "llf", ic="seeded_random",
          gammas=(1.4,), gas_constants=(0.4,)),
 ]
+# quasi-1D Euler: test/tree_dgsem_1d/eulerquasi_ec.jl (its initial_condition_ec,
+# polydeg 4, level 6 on [-1, 1]) and eulerquasi_source_terms.jl (convergence
+# test with sources, level 4), (flux_chan_etal, flux_nonconservative_chan_etal)
+# as volume and surface flux; seeded states with a random nozzle width
+QUASI_CASES = [
+    Case("quasi1d_ec", 1, "quasi1d", 6, "fd", "chan_etal", "chan_etal", polydeg=4, cmin=-1.0, cmax=1.0,
+         ic="constant", ic_fn=eulerquasi_ec_state),
+    Case("quasi1d_src", 1, "quasi1d", 4, "fd", "chan_etal", "chan_etal", polydeg=4, cmin=-1.0, cmax=1.0,
+         ic="convergence_test", source="convergence_test", t=0.3),
+    Case("quasi1d_random_p4", 1, "quasi1d", 4, "fd", "chan_etal", "chan_etal", polydeg=4, cmin=-1.0, cmax=1.0),
+    Case("quasi1d_random_p3", 1, "quasi1d", 5, "fd", "chan_etal", "chan_etal", cmin=-1.0, cmax=1.0),
+]
+# test/tree_dgsem_1d/euler_blast_wave.jl: its initial_condition_blast_wave,
+# HG with flux_ranocha (DG) and LLF (FV, surface), polydeg 3, level 6
+EULER_BLAST_1D = Case("euler1d_blast_wave_hg", 1, "euler", 6, "hg", "ranocha", "llf", "llf", ic="constant",
+                      ic_fn=euler_blast_wave_1d_state)
 
 
 CASES = [
@@ -243,6 +260,8 @@ CASES = [
     # the tests' own configuration: polydeg 4, level 3 on [-2, 2]^d
     *MHD_HG_P4,
     *MULTI_CASES,
+    *QUASI_CASES,
+    EULER_BLAST_1D,
 ]
 
 
@@ -256,6 +275,7 @@ def test_rhs_fp64_matches_oracle(th, oracle, torch, case):
 
 
 MIXED_CASES = [c for c in CASES if c.vi in ("fd", "hg") and c.polydeg == 3] + MHD_HG_P4 + \
+    [QUASI_CASES[0], QUASI_CASES[2]] + \
     [Case("euler2d_weak_llf_mixed", 2, "euler", 3), Case("linadv1d_mixed", 1, "linadv", 4, cmin=-1.0, cmax=1.0,
                                                         ic="convergence_test"),
      # weak form with N = 4 applies D-hat on the matrix cores in mixed precision
@@ -265,6 +285,26 @@ MIXED_CASES = [c for c in CASES if c.vi in ("fd", "hg") and c.polydeg == 3] + MH
      Case("euler3d_weak_hll_mixed", 3, "euler", 2, sflux="hll", ic="smooth_periodic")] + \
     [c for c in CASES if c.patches and c.name != "linadv3d_mortar_central" and
      not (c.ic == "convergence_test" and c.source)]  # (cancellation at the manufactured state, as above)
+
+
+@pytest.mark.parametrize("case", [QUASI_CASES[0], EULER_BLAST_1D,
+                                  Case("euler2d_function_blast", 2, "euler", 3, ic="constant",
+                                       ic_fn=lambda x, t, eq: [1.0 + 0.1 * x[0], 0.2, -0.1 * x[1],
+                                                               2.5 + (0.3 if x[0] * x[0] + x[1] * x[1] < 0.25
+                                                                      else 0.0)])],
+                         ids=lambda c: c.name)
+def test_function_initial_condition(th, oracle, torch, case):
+    """An initial condition written as a Python function (as the reference's
+    test files define them) is sampled on the host at the node coordinates,
+    with Trixi's 1D end-node shift, and packed into the device state."""
+    mesh = oracle_mesh(oracle, case)
+    u_ref = oracle_state(oracle, case, mesh)
+    semi = product_semi(th, case)
+    u = th.wrap_array_native(th.compute_coefficients(0.0, semi), semi).cpu().numpy()
+    np.testing.assert_array_equal(u, u_ref)
+    with pytest.raises(NotImplementedError):
+        th.calc_error_norms(th.compute_coefficients(0.0, semi), 0.0, semi)
+    semi.close()
 
 
 @pytest.mark.parametrize("case", MIXED_CASES, ids=lambda c: c.name)
@@ -309,7 +349,7 @@ def test_pack_unpack_roundtrip_bit_exact(th, oracle, torch):
                                   Case("hypdiff2d_poisson", 2, "hypdiff", 3, polydeg=4, cmin=0.0, cmax=1.0,
                                        ic="poisson_nonperiodic", periodic=(False, True),
                                        bc_state="poisson_nonperiodic", source="poisson_nonperiodic"),
-                                  MULTI_CASES[1], MULTI_CASES[3], MULTI_CASES[4]],
+                                  MULTI_CASES[1], MULTI_CASES[3], MULTI_CASES[4], QUASI_CASES[1]],
                          ids=lambda c: c.name)
 def test_initial_condition_and_callbacks_match_oracle(th, oracle, torch, case):
     O = oracle
@@ -401,7 +441,7 @@ def test_indicator_matches_oracle(th, oracle, torch, case):
                                   Case("mhd2d_hg_blast", 2, "mhd", 3, "hg", "hindenlang_gassner", "hindenlang_gassner",
                                        "hindenlang_gassner", cmin=-2.0, cmax=2.0, ic="weak_blast_wave_seeded",
                                        gamma=1.4),
-                                  MHD_HG_P4[0], MULTI_CASES[3], MULTI_CASES[4]],
+                                  MHD_HG_P4[0], MULTI_CASES[3], MULTI_CASES[4], QUASI_CASES[1]],
                          ids=lambda c: c.name)
 def test_rk_step_matches_oracle(th, oracle, torch, case, use_graph):
     """Three CK2N54 steps from t = 0 (stage times through device memory: the

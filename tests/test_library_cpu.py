@@ -187,3 +187,27 @@ def test_mortar_mesh_rejections(th):
         th.SemidiscretizationHyperbolicHIP(m, th.LinearScalarAdvectionEquation2D((0.2, -0.7)),
                                            th.initial_condition_convergence_test,
                                            th.DGSEMHIP(polydeg=3, surface_flux=th.flux_lax_friedrichs))
+
+
+def test_prim2cons_and_quasi1d_flux_pairs(th):
+    """prim2cons for the function-written initial conditions, and the
+    (flux, nonconservative flux) pairs quasi-1D Euler requires."""
+    e1 = th.CompressibleEulerEquations1D(1.4)
+    assert th.prim2cons((2.0, 0.5, 1.0), e1) == pytest.approx([2.0, 1.0, 1.0 / 0.4 + 0.25])
+    q = th.CompressibleEulerEquationsQuasi1D(1.4)
+    assert q.nvariables == 4
+    assert th.prim2cons((2.0, 0.5, 1.0, 3.0), q) == pytest.approx([6.0, 3.0, 3.0 * (2.5 + 0.25), 3.0])
+    m = th.CompressibleEulerMulticomponentEquations2D(gammas=(1.4, 2.0), gas_constants=(0.4, 1.0))
+    # gamma of the mixture: sum rho_k c_v,k gamma_k / sum rho_k c_v,k, c_v = R / (gamma - 1)
+    gam = (1.0 * 1.0 * 1.4 + 3.0 * 1.0 * 2.0) / (1.0 * 1.0 + 3.0 * 1.0)
+    assert th.prim2cons((0.5, 0.0, 1.0, 1.0, 3.0), m) == pytest.approx(
+        [2.0, 0.0, 1.0 / (gam - 1) + 0.5 * 4.0 * 0.25, 1.0, 3.0])
+    pair = (th.flux_chan_etal, th.flux_nonconservative_chan_etal)
+    c = th.DGSEMHIP(polydeg=4, surface_flux=pair, volume_integral=th.VolumeIntegralFluxDifferencing(pair)).config(q)
+    assert (c.equation, c.volume_flux, c.surface_flux) == (6, 8, 8)
+    with pytest.raises(NotImplementedError):
+        th.DGSEMHIP(polydeg=4, surface_flux=th.flux_chan_etal,
+                    volume_integral=th.VolumeIntegralFluxDifferencing(pair)).config(q)
+    with pytest.raises(NotImplementedError):  # the MHD term with quasi-1D Euler
+        th.DGSEMHIP(polydeg=4, surface_flux=(th.flux_chan_etal, th.flux_nonconservative_powell),
+                    volume_integral=th.VolumeIntegralFluxDifferencing(pair)).config(q)

@@ -778,3 +778,98 @@ def test_multicomponent_weak_blast_state(oracle, nd):
         frac = np.array([2.0 ** k / (2.0 ** n - 1) for k in range(n)])
         np.testing.assert_allclose(um[..., nd + 1:], ue[..., :1] * frac, rtol=1e-15)
         np.testing.assert_allclose(um[..., :nd + 1], ue[..., 1:], rtol=1e-14, atol=1e-15)
+
+
+# ---------------------------------------------------------------------------
+# quasi-1D compressible Euler: (a rho, a rho v, a e, a)
+# ---------------------------------------------------------------------------
+def quasi_entropy(u, gamma=1.4):
+    """a U_Euler(u / a) with U_Euler = -rho s / (gamma - 1), s = log p - gamma log rho."""
+    a = u[3]
+    rho, rv, re = u[0] / a, u[1] / a, u[2] / a
+    p = (gamma - 1) * (re - 0.5 * rv * rv / rho)
+    return -a * rho * (np.log(p) - gamma * np.log(rho)) / (gamma - 1)
+
+
+@pytest.mark.parametrize("N", [4, 5])
+def test_quasi1d_reduces_to_euler_at_unit_width(oracle, N):
+    """With a = 1 the quasi-1D scheme (flux_chan_etal + a_ll p_rr in the
+    nonconservative form with Trixi's factor 1/2) is the 1D Euler flux
+    differencing scheme with flux_ranocha, node for node; a stays constant.
+    A nonconservative term with a mean pressure, or without the factor 1/2,
+    fails this."""
+    O = oracle
+    mesh = O.Mesh(1, 3, -1.0, 1.0, N - 1)
+    cq = O.make_cfg(1, O.EQ_EULER_QUASI1D, N - 1, O.VI_FD, O.FLUX_CHAN_ETAL, O.FLUX_CHAN_ETAL)
+    ce = O.make_cfg(1, O.EQ_EULER, N - 1, O.VI_FD, O.FLUX_RANOCHA, O.FLUX_RANOCHA)
+    ue = O.compute_coefficients(ce, mesh, O.IC_SEEDED_RANDOM)
+    uq = np.concatenate([ue, np.ones(ue.shape[:-1] + (1,))], axis=-1)
+    dq, de = O.rhs(cq, mesh, uq), O.rhs(ce, mesh, ue)
+    assert np.abs(dq[..., :3] - de).max() < 1e-13 * np.abs(de).max()
+    assert np.abs(dq[..., 3]).max() == 0.0
+
+
+def test_quasi1d_flux_consistency_and_entropy_conservation(oracle):
+    """flux_chan_etal is consistent (f*(u, u) = (a rho v, a rho v^2, a v (e + p), 0))
+    and symmetric, and the semidiscretisation with it conserves the entropy
+    a U_Euler(u / a) on a periodic mesh with a varying width (entropy
+    variables by complex step)."""
+    O = oracle
+    cfg = O.make_cfg(1, O.EQ_EULER_QUASI1D, 4, O.VI_FD, O.FLUX_CHAN_ETAL, O.FLUX_CHAN_ETAL)
+    rng = np.random.default_rng(30)
+    for _ in range(20):
+        ul = np.array([1.2, 0.3, 3.0, 1.5]) * (1 + 0.2 * rng.random(4))
+        ur = np.array([1.0, -0.2, 2.5, 2.0]) * (1 + 0.2 * rng.random(4))
+        np.testing.assert_allclose(O.two_point_flux(cfg, O.FLUX_CHAN_ETAL, ul, ul, 1), O.flux(cfg, ul, 1),
+                                   rtol=1e-14, atol=1e-15)
+        fs = O.two_point_flux(cfg, O.FLUX_CHAN_ETAL, ul, ur, 1)
+        # (the energy component cancels terms of size one: absolute round-off)
+        np.testing.assert_allclose(fs, O.two_point_flux(cfg, O.FLUX_CHAN_ETAL, ur, ul, 1), rtol=1e-13, atol=1e-15)
+    mesh = O.Mesh(1, 3, -1.0, 1.0, 4)
+    u = O.compute_coefficients(cfg, mesh, O.IC_SEEDED_RANDOM)
+    du = O.rhs(cfg, mesh, u)
+    assert np.abs(du[..., 3]).max() == 0.0
+    h = 1e-30
+    w = np.zeros_like(u)
+    for j in range(4):
+        uc = u.astype(complex)
+        uc[..., j] += 1j * h
+        w[..., j] = quasi_entropy(np.moveaxis(uc, -1, 0)).imag / h
+    rate = O.integrate(cfg, mesh, np.repeat((w * du).sum(-1, keepdims=True), 4, axis=-1))[0]
+    assert abs(rate) < 1e-13 * np.abs(du).max() * mesh.total_volume, rate
+
+
+def test_quasi1d_source_and_convergence(oracle):
+    """source_terms_convergence_test equals the residual of the manufactured
+    state (sympy derivatives of the quasi-1D equations), and with it
+    rhs!(u_exact(t)) approximates d/dt u_exact at the DG rate (polydeg 4)."""
+    import sympy as sp
+    O = oracle
+    gamma = 1.4
+    cfg = O.make_cfg(1, O.EQ_EULER_QUASI1D, 4, O.VI_FD, O.FLUX_CHAN_ETAL, O.FLUX_CHAN_ETAL,
+                     source_terms=O.SOURCE_CONVERGENCE_TEST, gamma=gamma)
+    X, T = sp.symbols("x t")
+    ini = 2 + sp.Rational(1, 10) * sp.sin(sp.pi * (X - T))
+    rho, v, a = ini, 1, sp.Rational(3, 2) - sp.cos(sp.pi * X) / 2
+    e = ini ** 2 / rho
+    p = (gamma - 1) * (e - rho * v ** 2 / 2)
+    S = [sp.diff(a * rho, T) + sp.diff(a * rho * v, X),
+         sp.diff(a * rho * v, T) + sp.diff(a * rho * v * v, X) + a * sp.diff(p, X),
+         sp.diff(a * e, T) + sp.diff(a * v * (e + p), X)]
+    Sf = sp.lambdify((X, T), S, "numpy")
+    rng = np.random.default_rng(31)
+    for _ in range(8):
+        x, t = rng.uniform(-1, 1), rng.uniform(0, 1)
+        s = O.source_convergence_test(cfg, np.array([x]), t)
+        np.testing.assert_allclose(s[:3], np.array(Sf(x, t), dtype=float), rtol=1e-13, atol=1e-14)
+        assert s[3] == 0.0
+    errs = []
+    t = 0.3
+    for lvl in (3, 4):
+        mesh = O.Mesh(1, lvl, -1.0, 1.0, 4)
+        u = O.compute_coefficients(cfg, mesh, O.IC_CONVERGENCE_TEST, t)
+        h = 1e-6
+        ut = (O.compute_coefficients(cfg, mesh, O.IC_CONVERGENCE_TEST, t + h)
+              - O.compute_coefficients(cfg, mesh, O.IC_CONVERGENCE_TEST, t - h)) / (2 * h)
+        errs.append(np.abs(O.rhs(cfg, mesh, u, t=t) - ut).max())
+    assert np.log2(errs[0] / errs[1]) > 3.5, errs

@@ -18,6 +18,7 @@ from .callbacks import (AnalysisCallback, StepsizeCallback, blending_factors, ca
 from .equations import (BoundaryConditionDirichlet, boundary_condition_periodic, source_terms_convergence_test,
                         CompressibleEulerEquations1D, CompressibleEulerEquations2D, CompressibleEulerEquations3D,
                         CompressibleEulerMulticomponentEquations1D, CompressibleEulerMulticomponentEquations2D,
+                        CompressibleEulerEquationsQuasi1D, flux_chan_etal, flux_nonconservative_chan_etal, prim2cons,
                         FluxLaxFriedrichs, IdealGlmMhdEquations1D, IdealGlmMhdEquations2D, IdealGlmMhdEquations3D, LinearScalarAdvectionEquation1D,
                         LinearScalarAdvectionEquation2D, LinearScalarAdvectionEquation3D, density, density_pressure, pressure,
                         InviscidBurgersEquation1D, first, flux_ec, initial_condition_shock, FluxHLL, flux_hll,

@@ -24,6 +24,8 @@ def calc_error_norms(u_ode, t, semi, initial_condition=None):
     """(l2, linf) per variable against the initial condition at time t."""
     semi.check_vector(u_ode, "u_ode")
     ic = initial_condition or semi.initial_condition
+    if not hasattr(ic, "ic_id"):
+        raise NotImplementedError("error norms are computed on the device against a built initial condition")
     l2 = np.zeros(semi.nvariables)
     linf = np.zeros(semi.nvariables)
     L.check(L.lib().trixi_hip_error_norms(semi.ctx, _ptr(u_ode), ic.ic_id, float(t), ic.seed,

@@ -150,6 +150,30 @@ __device__ inline void initial_condition(const IcParams& q, const double* x, dou
         }
         return;
     }
+    if (q.eq == EQ_EULER_QUASI1D) {
+        // initial_condition_convergence_test of CompressibleEulerEquationsQuasi1D:
+        // ini = 2 + 0.1 sin(pi (x - t)), rho = ini, v = 1, e = ini^2 / rho,
+        // p = (gamma - 1)(e - rho v^2 / 2), a = 1.5 - 0.5 cos(pi x)
+        double rho = 1.0, v = 0.0, p = 1.0, a = 1.0;
+        if (q.ic == TRIXI_HIP_IC_CONVERGENCE_TEST) {
+            const double ini = 2.0 + 0.1 * sin(M_PI * (x[0] - t));
+            rho = ini;
+            v = 1.0;
+            const double e = ini * ini / rho;
+            p = (q.gamma - 1.0) * (e - 0.5 * rho * v * v);
+            a = 1.5 - 0.5 * cos(x[0] * M_PI);
+        } else if (q.ic == TRIXI_HIP_IC_SEEDED_RANDOM) {
+            rho = 1.0 + 0.1 * (2.0 * u01(q.seed, idx, 0) - 1.0);
+            p = 1.0 + 0.05 * (2.0 * u01(q.seed, idx, 1) - 1.0);
+            v = 0.1 * normal01(q.seed, idx, 2);
+            a = 1.0 + 0.2 * (2.0 * u01(q.seed, idx, 4) - 1.0);
+        }
+        u[0] = a * rho;
+        u[1] = a * rho * v;
+        u[2] = a * (p * q.inv_gm1 + 0.5 * rho * v * v);
+        u[3] = a;
+        return;
+    }
     if (q.eq == EQ_EULER_MULTI) {
         // (rho v, rho e, rho_1..rho_nc).  Weak blast wave (Trixi's
         // multicomponent initial_condition_weak_blast_wave): the Euler blast
@@ -355,6 +379,27 @@ __host__ __device__ inline void source_convergence_test(int dim, double gm1, con
     s[0] = (dim - 1) * gp;
     for (int d = 0; d < dim; ++d) s[1 + d] = (dim - 1) * gp + pp;
     s[dim + 1] = 2.0 * (dim - 1) * g * gp + dim * pp;
+}
+
+// source_terms_convergence_test of CompressibleEulerEquationsQuasi1D: with
+// ini = 2 + 0.1 sin(pi (x - t)), g' = 0.1 pi cos(pi (x - t)) (so ini_t = -g',
+// ini_x = g'), rho = e = ini, v = 1, p = (gamma - 1) ini / 2,
+// a = 1.5 - 0.5 cos(pi x), a_x = pi/2 sin(pi x):
+//   S1 = (a rho)_t + (a rho v)_x              = a_x ini,
+//   S2 = (a rho v)_t + (a rho v^2)_x + a p_x   = a_x ini + a p_x,
+//   S3 = (a e)_t + (a v (e + p))_x             = a_x (ini + p) + a p_x,
+// p_x = (gamma - 1) g' / 2, S4 = 0.
+__host__ __device__ inline void source_quasi1d_convergence_test(double gamma, const double* x, double t, double* s) {
+    const double arg = M_PI * (x[0] - t);
+    const double ini = 2.0 + 0.1 * sin(arg);
+    const double gp = 0.1 * M_PI * cos(arg);
+    const double a = 1.5 - 0.5 * cos(x[0] * M_PI);
+    const double a_x = 0.5 * M_PI * sin(x[0] * M_PI);
+    const double p = 0.5 * (gamma - 1.0) * ini, p_x = 0.5 * (gamma - 1.0) * gp;
+    s[0] = a_x * ini;
+    s[1] = a_x * ini + a * p_x;
+    s[2] = a_x * (ini + p) + a * p_x;
+    s[3] = 0.0;
 }
 
 // source_terms_eoc_test_euler (compressible Euler, d dimensions) for the state

@@ -52,7 +52,7 @@ __global__ void __launch_bounds__(256) mortar_kernel(const MortarArgs a) {
     constexpr int K = 1 << (DIM - 1);
     constexpr int LPM = K * NL;             // lanes per mortar
     constexpr int MPB = 256 / LPM;          // mortars per workgroup
-    constexpr bool LOGS = (SF == FLUX_RANOCHA || SF == FLUX_HINDENLANG_GASSNER);
+    constexpr bool LOGS = flux_needs_logs(SF);
     static_assert(LPM <= 256 && 256 % LPM == 0, "mortar lanes must tile the workgroup");
     using Node = typename EQ::template Node<double>;
     __shared__ double large_face[MPB][NV][NL];

@@ -22,7 +22,15 @@
 
 namespace trixi_hip {
 
-enum Equation : int { EQ_LINADV = 0, EQ_EULER = 1, EQ_GLM_MHD = 2, EQ_BURGERS = 3, EQ_HYPDIFF = 4, EQ_EULER_MULTI = 5 };
+enum Equation : int {
+    EQ_LINADV = 0,
+    EQ_EULER = 1,
+    EQ_GLM_MHD = 2,
+    EQ_BURGERS = 3,
+    EQ_HYPDIFF = 4,
+    EQ_EULER_MULTI = 5,
+    EQ_EULER_QUASI1D = 6
+};
 constexpr int MAXC = 4;  // gas components of multicomponent Euler (parameter slots)
 enum VolumeIntegral : int { VI_WEAK = 0, VI_FD = 1, VI_HG = 2 };
 enum Flux : int {
@@ -33,8 +41,14 @@ enum Flux : int {
     FLUX_SHIMA_ETAL = 4,
     FLUX_HINDENLANG_GASSNER = 5,
     FLUX_EC = 6,          // flux_ec of the inviscid Burgers equation
-    FLUX_HLL = 7          // FluxHLL(min_max_speed_davis) (compressible Euler)
+    FLUX_HLL = 7,         // FluxHLL(min_max_speed_davis) (compressible Euler)
+    FLUX_CHAN_ETAL = 8    // flux_chan_etal + flux_nonconservative_chan_etal (quasi-1D Euler)
 };
+
+// two-point fluxes that take ln-means (they need log(rho), log(p) per node)
+constexpr bool flux_needs_logs(int f) {
+    return f == FLUX_RANOCHA || f == FLUX_HINDENLANG_GASSNER || f == FLUX_CHAN_ETAL;
+}
 
 struct PhysParams {
     double gamma, inv_gm1, gm1, c_h;
@@ -660,6 +674,124 @@ struct EulerMulti {
         const Node<R> n = node<R, false>(u, P);
 #pragma unroll
         for (int d = 0; d < DIM; ++d) lam[d] = dabs(n.v[d]) + n.c;
+    }
+};
+
+// ===========================================================================
+// Quasi-one-dimensional compressible Euler (Trixi
+// CompressibleEulerEquationsQuasi1D): (a rho, a rho v, a e, a) with the nozzle
+// width a, e = p / (gamma - 1) + rho v^2 / 2;
+//   (a rho)_t + (a rho v)_x = 0,  (a rho v)_t + (a rho v^2)_x + a p_x = 0,
+//   (a e)_t + (a v (e + p))_x = 0,  a_t = 0.
+// flux_chan_etal is the entropy-conservative two-point flux of Chan et al.
+// (2023) and flux_nonconservative_chan_etal(u_l, u_r) = (0, a_l p_r, 0, 0)
+// the pressure-gradient term.
+// ===========================================================================
+struct EulerQuasi1D {
+    static constexpr int NV = 4;
+    static constexpr bool NONCONS = true;
+    static constexpr int NRF = 5;  // node record: rho, v, p, c, a
+    template <typename R>
+    struct Node {
+        R rho, v[1], p, c, a;
+        double lrho, lp;
+        R inv_gm1;
+        __device__ R cons(int q) const {
+            if (q == 0) return a * rho;
+            if (q == 1) return a * rho * v[0];
+            if (q == 2) return a * (p * inv_gm1 + R(0.5) * rho * v[0] * v[0]);
+            return a;
+        }
+    };
+    template <typename R, bool LOGS>
+    __device__ static Node<R> node(const R* u, const PhysR<R>& P) {
+        Node<R> n;
+        n.a = u[3];
+        const R inv_a = rdiv(R(1), n.a);
+        n.rho = u[0] * inv_a;
+        const R rv = u[1] * inv_a;
+        n.v[0] = rdiv(rv, n.rho);
+        n.p = P.gm1 * (u[2] * inv_a - R(0.5) * rv * n.v[0]);
+        n.c = rsqrt_(P.gamma * n.p * rdiv(R(1), n.rho));
+        n.inv_gm1 = P.inv_gm1;
+        if (LOGS) {
+            n.lrho = log((double)n.rho);
+            n.lp = log((double)n.p);
+        } else {
+            n.lrho = n.lp = 0.0;
+        }
+        return n;
+    }
+    template <typename R>
+    __device__ static void to_rec(const Node<R>& n, R* f) {
+        f[0] = n.rho;
+        f[1] = n.v[0];
+        f[2] = n.p;
+        f[3] = n.c;
+        f[4] = n.a;
+    }
+    template <typename R>
+    __device__ static Node<R> from_rec(const R* f, double lrho, double lp, const PhysR<R>& P) {
+        Node<R> n;
+        n.rho = f[0];
+        n.v[0] = f[1];
+        n.p = f[2];
+        n.c = f[3];
+        n.a = f[4];
+        n.lrho = lrho;
+        n.lp = lp;
+        n.inv_gm1 = P.inv_gm1;
+        return n;
+    }
+    template <typename R>
+    __device__ static void flux(const Node<R>& n, int, const PhysR<R>& P, R* f) {
+        const R arv = n.a * n.rho * n.v[0];
+        f[0] = arv;
+        f[1] = arv * n.v[0];
+        f[2] = n.a * n.v[0] * (n.p * n.inv_gm1 + R(0.5) * n.rho * n.v[0] * n.v[0] + n.p);
+        f[3] = R(0);
+    }
+    template <int KIND, typename R>
+    __device__ static void two_point(const Node<R>& l, const Node<R>& r, int o, const PhysR<R>& P, R* f) {
+        if (KIND == FLUX_CHAN_ETAL) {
+            const R rho_mean = ln_mean_logs<R>(l.rho, r.rho, l.lrho, r.lrho);
+            const R inv_rho_p_mean =
+                l.p * r.p * inv_ln_mean_logs<R>(l.rho * r.p, r.rho * l.p, l.lrho + r.lp, r.lrho + l.lp);
+            const R v_avg = R(0.5) * (l.v[0] + r.v[0]);
+            const R a_v_avg = R(0.5) * (l.a * l.v[0] + r.a * r.v[0]);
+            const R velocity_square_avg = R(0.5) * (l.v[0] * r.v[0]);
+            const R f1 = rho_mean * a_v_avg;
+            f[0] = f1;
+            f[1] = f1 * v_avg;
+            f[2] = f1 * (velocity_square_avg + inv_rho_p_mean * P.inv_gm1) +
+                   R(0.5) * (l.p * r.a * r.v[0] + r.p * l.a * l.v[0]);
+            f[3] = R(0);
+        } else {
+            R fl[NV], fr[NV];
+            flux(l, o, P, fl);
+            flux(r, o, P, fr);
+            if (KIND == FLUX_LLF || KIND == FLUX_LLF_NAIVE) {
+                const R lam = dmax(dabs(l.v[0]), dabs(r.v[0])) + dmax(l.c, r.c);
+#pragma unroll
+                for (int q = 0; q < NV; ++q)
+                    f[q] = R(0.5) * (fl[q] + fr[q]) + (R(-0.5) * lam * (r.cons(q) - l.cons(q)));
+            } else {
+#pragma unroll
+                for (int q = 0; q < NV; ++q) f[q] = R(0.5) * (fl[q] + fr[q]);
+            }
+        }
+    }
+    template <typename R>
+    __device__ static void noncons(const Node<R>& l, const Node<R>& r, int, R* f) {
+        f[0] = R(0);
+        f[1] = l.a * r.p;
+        f[2] = R(0);
+        f[3] = R(0);
+    }
+    template <typename R>
+    __device__ static void max_abs_speeds(const R* u, const PhysR<R>& P, R* lam) {
+        const Node<R> n = node<R, false>(u, P);
+        lam[0] = dabs(n.v[0]) + n.c;
     }
 };
 

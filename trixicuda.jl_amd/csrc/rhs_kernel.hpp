@@ -130,7 +130,9 @@ __device__ __forceinline__ void source_at(const RhsArgs& a, int64_t e, int node,
         return;
     }
     double s5[5];
-    if (a.src == TRIXI_HIP_SOURCE_EOC_TEST_EULER)
+    if (a.bc_ic.eq == EQ_EULER_QUASI1D)
+        source_quasi1d_convergence_test(a.phys.gamma, xx, t, s5);
+    else if (a.src == TRIXI_HIP_SOURCE_EOC_TEST_EULER)
         source_eoc_test_euler(DIM, a.phys.gm1, xx, t, s5);
     else
         source_convergence_test(DIM, a.phys.gm1, xx, t, s5);
@@ -202,9 +204,9 @@ rhs_kernel(const RhsArgs a) {
     constexpr int NIT = (N + NG - 1) / NG;  // node-pass (node, element) pairs per lane
     constexpr int NV = EQ::NV;
     constexpr int NRF = EQ::NRF;
-    constexpr bool SF_LOGS = (SF == FLUX_RANOCHA || SF == FLUX_HINDENLANG_GASSNER);
-    constexpr bool VF_LOGS = (VOL != VI_WEAK) && (VF == FLUX_RANOCHA || VF == FLUX_HINDENLANG_GASSNER);
-    constexpr bool FV_LOGS = (VOL == VI_HG) && (FVF == FLUX_RANOCHA || FVF == FLUX_HINDENLANG_GASSNER);
+    constexpr bool SF_LOGS = flux_needs_logs(SF);
+    constexpr bool VF_LOGS = (VOL != VI_WEAK) && flux_needs_logs(VF);
+    constexpr bool FV_LOGS = (VOL == VI_HG) && flux_needs_logs(FVF);
     constexpr bool NODE_LOGS = SF_LOGS || VF_LOGS || FV_LOGS;
     constexpr int NDF = NODE_LOGS ? 2 : 0;
     using Node = typename EQ::template Node<R>;

@@ -62,6 +62,7 @@ void register_burgers();
 void register_mhd1d();
 void register_hypdiff();
 void register_euler_multi();
+void register_euler_quasi1d();
 void register_vjp_linadv();
 void register_vjp_euler1d();
 void register_vjp_euler2d();
@@ -87,6 +88,7 @@ static void ensure_registered() {
         register_mhd1d();
         register_hypdiff();
         register_euler_multi();
+        register_euler_quasi1d();
         register_vjp_linadv();
         register_vjp_euler1d();
         register_vjp_euler2d();
@@ -771,6 +773,13 @@ __global__ void check_state_kernel(const double* __restrict__ u, int32_t* __rest
                 if (!isfinite(uc[v])) f |= 1;
             }
             if (eq == EQ_LINADV || eq == EQ_BURGERS || eq == EQ_HYPDIFF) continue;
+            if (eq == EQ_EULER_QUASI1D) {
+                // rho = (a rho) / a, p = (gamma - 1)((a e) / a - (a rho v)^2 / (2 a (a rho)))
+                const double a = uc[3];
+                if (!(a > 0.0) || !(uc[0] / a > 0.0)) f |= 2;
+                if (!(phys.gm1 * (uc[2] / a - 0.5 * uc[1] * uc[1] / (a * uc[0])) > 0.0)) f |= 4;
+                continue;
+            }
             if (eq == EQ_EULER_MULTI) {
                 // total density and the mixture pressure
                 double rho = 0.0, h1 = 0.0, h2 = 0.0, ke = 0.0;
@@ -877,6 +886,7 @@ static const double CK_C[5] = {0.0, 1432997174477.0 / 9575080441755.0, 252626934
 static int nvars_of(const trixi_hip_config& c) {
     if (c.equation == TRIXI_HIP_LINEAR_SCALAR_ADVECTION || c.equation == TRIXI_HIP_INVISCID_BURGERS) return 1;
     if (c.equation == TRIXI_HIP_COMPRESSIBLE_EULER_MULTICOMPONENT) return c.ndims + 1 + c.ncomponents;
+    if (c.equation == TRIXI_HIP_COMPRESSIBLE_EULER_QUASI1D) return 4;
     if (c.equation == TRIXI_HIP_COMPRESSIBLE_EULER) return c.ndims + 2;
     if (c.equation == TRIXI_HIP_HYPERBOLIC_DIFFUSION) return c.ndims + 1;
     return c.ndims == 1 ? 8 : 9;  // ideal MHD: no cleaning field in 1D
@@ -1038,8 +1048,15 @@ int trixi_hip_create(const trixi_hip_config* config, const trixi_hip_mesh* mesh,
     const trixi_hip_config& cfg = *config;
     if (cfg.ndims < 1 || cfg.ndims > 3) return fail(TRIXI_HIP_EINVAL, "ndims must be 1, 2 or 3");
     if (cfg.polydeg < 1 || cfg.polydeg + 1 > MAXN) return fail(TRIXI_HIP_EINVAL, "polydeg out of range");
-    if (cfg.equation < TRIXI_HIP_LINEAR_SCALAR_ADVECTION || cfg.equation > TRIXI_HIP_COMPRESSIBLE_EULER_MULTICOMPONENT)
+    if (cfg.equation < TRIXI_HIP_LINEAR_SCALAR_ADVECTION || cfg.equation > TRIXI_HIP_COMPRESSIBLE_EULER_QUASI1D)
         return fail(TRIXI_HIP_EINVAL, "unknown equation");
+    if (cfg.equation == TRIXI_HIP_COMPRESSIBLE_EULER_QUASI1D) {
+        if (cfg.ndims != 1) return fail(TRIXI_HIP_EINVAL, "quasi-1D Euler is one-dimensional");
+        if (cfg.boundary_condition != TRIXI_HIP_BC_PERIODIC ||
+            (cfg.source_terms != TRIXI_HIP_SOURCE_NONE && cfg.source_terms != TRIXI_HIP_SOURCE_CONVERGENCE_TEST))
+            return fail(TRIXI_HIP_EUNSUPPORTED,
+                        "quasi-1D Euler is built for periodic meshes, with or without source_terms_convergence_test");
+    }
     if (cfg.equation == TRIXI_HIP_INVISCID_BURGERS && cfg.ndims != 1)
         return fail(TRIXI_HIP_EUNSUPPORTED, "the inviscid Burgers equation is built for 1D");
     if (cfg.equation == TRIXI_HIP_COMPRESSIBLE_EULER_MULTICOMPONENT) {
@@ -1100,9 +1117,10 @@ int trixi_hip_create(const trixi_hip_config* config, const trixi_hip_mesh* mesh,
             return fail(TRIXI_HIP_EUNSUPPORTED, "source_terms_eoc_test_euler is built for compressible Euler 2D/3D");
     } else if (cfg.source_terms != TRIXI_HIP_SOURCE_NONE) {
         if (cfg.source_terms != TRIXI_HIP_SOURCE_CONVERGENCE_TEST) return fail(TRIXI_HIP_EINVAL, "unknown source terms");
-        if (cfg.equation != TRIXI_HIP_COMPRESSIBLE_EULER && cfg.equation != TRIXI_HIP_INVISCID_BURGERS)
+        if (cfg.equation != TRIXI_HIP_COMPRESSIBLE_EULER && cfg.equation != TRIXI_HIP_INVISCID_BURGERS &&
+            cfg.equation != TRIXI_HIP_COMPRESSIBLE_EULER_QUASI1D)
             return fail(TRIXI_HIP_EUNSUPPORTED,
-                        "source_terms_convergence_test is built for compressible Euler and Burgers");
+                        "source_terms_convergence_test is built for compressible Euler, Burgers and quasi-1D Euler");
     }
     const int64_t NM = mesh->nmortars;
     if (NM < 0) return fail(TRIXI_HIP_EINVAL, "negative mortar count");
@@ -1542,6 +1560,8 @@ int trixi_hip_max_dt(trixi_hip_ctx* ctx, const double* u, double* max_dt, trixi_
         MS(Burgers, 1);
     } else if (eq == TRIXI_HIP_HYPERBOLIC_DIFFUSION) {
         if (dim == 1) MS(HypDiff<1>, 1); else if (dim == 2) MS(HypDiff<2>, 2); else MS(HypDiff<3>, 3);
+    } else if (eq == TRIXI_HIP_COMPRESSIBLE_EULER_QUASI1D) {
+        MS(EulerQuasi1D, 1);
     } else if (eq == TRIXI_HIP_COMPRESSIBLE_EULER_MULTICOMPONENT) {
         using M11 = EulerMulti<1, 1>;
         using M12 = EulerMulti<1, 2>;

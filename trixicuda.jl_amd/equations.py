@@ -30,6 +30,8 @@ class AbstractEquations:
             return self.ndims + 2
         if self.equation_id == L.EQ_EULER_MULTI:
             return self.ndims + 1 + len(self.gammas)
+        if self.equation_id == L.EQ_EULER_QUASI1D:
+            return 4
         return 9 if self.ndims > 1 else 8  # ideal MHD: no cleaning field in 1D
 
 
@@ -104,6 +106,39 @@ def _euler_multi(ndims):
 
 CompressibleEulerMulticomponentEquations1D = _euler_multi(1)
 CompressibleEulerMulticomponentEquations2D = _euler_multi(2)
+
+
+@dataclass(frozen=True)
+class CompressibleEulerEquationsQuasi1D(AbstractEquations):
+    """(a rho, a rho v, a e, a) with the nozzle width a (Trixi
+    CompressibleEulerEquationsQuasi1D); nonconservative pressure term a p_x."""
+    gamma: float = 1.4
+
+    def __init__(self, gamma):
+        object.__setattr__(self, "ndims", 1)
+        object.__setattr__(self, "equation_id", L.EQ_EULER_QUASI1D)
+        object.__setattr__(self, "gamma", float(gamma))
+
+
+def prim2cons(prim, equations):
+    """Trixi's prim2cons for compressible Euler (rho, v.., p), quasi-1D Euler
+    (rho, v, p, a) and multicomponent Euler (v.., p, rho_1..rho_n), for
+    initial conditions written as Python functions."""
+    prim = [float(x) for x in prim]
+    nd = equations.ndims
+    if equations.equation_id == L.EQ_EULER:
+        rho, v, p = prim[0], prim[1:1 + nd], prim[1 + nd]
+        return [rho] + [rho * vi for vi in v] + [p / (equations.gamma - 1) + 0.5 * rho * sum(vi * vi for vi in v)]
+    if equations.equation_id == L.EQ_EULER_QUASI1D:
+        rho, v, p, a = prim
+        return [a * rho, a * rho * v, a * (p / (equations.gamma - 1) + 0.5 * rho * v * v), a]
+    if equations.equation_id == L.EQ_EULER_MULTI:
+        v, p, rk = prim[:nd], prim[nd], prim[nd + 1:]
+        cv = [r / (g - 1) for g, r in zip(equations.gammas, equations.gas_constants)]
+        gamma = sum(x * c * g for x, c, g in zip(rk, cv, equations.gammas)) / sum(x * c for x, c in zip(rk, cv))
+        rho = sum(rk)
+        return [rho * vi for vi in v] + [p / (gamma - 1) + 0.5 * rho * sum(vi * vi for vi in v)] + rk
+    raise NotImplementedError(f"prim2cons is not provided for {type(equations).__name__}")
 
 
 def _glm_mhd(ndims):
@@ -205,6 +240,10 @@ def FluxHLL(min_max_speed=min_max_speed_davis):
 
 flux_hll = FluxHLL()  # compressible Euler
 flux_nonconservative_powell = NumericalFlux("flux_nonconservative_powell", -1)
+# quasi-1D Euler: Chan et al. (2023) entropy-conservative flux and its
+# nonconservative pressure-gradient term (used as a pair)
+flux_chan_etal = NumericalFlux("flux_chan_etal", L.FLUX_CHAN_ETAL)
+flux_nonconservative_chan_etal = NumericalFlux("flux_nonconservative_chan_etal", -2)
 
 
 def density_pressure(*args):  # identity markers of the HG indicator variables
@@ -345,7 +384,7 @@ def initial_condition_smooth_periodic(seed=20261202):
 def split_flux(flux):
     """Split a (conservative, nonconservative) tuple as Trixi accepts it."""
     if isinstance(flux, tuple):
-        if len(flux) != 2 or flux[1] is not flux_nonconservative_powell:
+        if len(flux) != 2 or flux[1] not in (flux_nonconservative_powell, flux_nonconservative_chan_etal):
             raise NotImplementedError(f"unsupported flux tuple {flux}")
         return flux[0], flux[1]
     return flux, None

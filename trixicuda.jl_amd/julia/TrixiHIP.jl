@@ -85,6 +85,7 @@ equation_id(::Union{HyperbolicDiffusionEquations1D, HyperbolicDiffusionEquations
                     HyperbolicDiffusionEquations3D}) = 4
 equation_id(::Union{CompressibleEulerMulticomponentEquations1D,
                     CompressibleEulerMulticomponentEquations2D}) = 5
+equation_id(::CompressibleEulerEquationsQuasi1D) = 6
 equation_id(eq) = throw(ArgumentError("equation $(typeof(eq)) is not built for HIP"))
 
 flux_id(::typeof(flux_central)) = 0
@@ -95,8 +96,9 @@ flux_id(::typeof(flux_hindenlang_gassner)) = 5
 flux_id(::typeof(flux_ec)) = 6
 flux_id(f::FluxHLL) = f.min_max_speed === min_max_speed_davis ? 7 :
     throw(ArgumentError("FluxHLL is built with min_max_speed_davis"))
-flux_id(f::Tuple) = (f[2] === flux_nonconservative_powell ? flux_id(f[1]) :
-                     throw(ArgumentError("unsupported nonconservative flux $(f[2])")))
+flux_id(::typeof(flux_chan_etal)) = 8
+flux_id(f::Tuple) = (f[2] === flux_nonconservative_powell || f[2] === flux_nonconservative_chan_etal ?
+                     flux_id(f[1]) : throw(ArgumentError("unsupported nonconservative flux $(f[2])")))
 flux_id(f) = throw(ArgumentError("flux $f is not built for HIP"))
 
 ic_id(::typeof(initial_condition_constant)) = 0
@@ -116,6 +118,7 @@ function initial_condition_harmonic_nonperiodic(x, t, ::HyperbolicDiffusionEquat
 end
 ic_id(::typeof(initial_condition_harmonic_nonperiodic)) = 10
 ic_id(f) = throw(ArgumentError("initial condition $f is not built for HIP"))
+is_builtin_ic(f) = try ic_id(f); true catch; false end
 # seeded benchmark states (ids 3, 4, 5 of trixi_hip_initial_condition) carry their seed
 struct SeededInitialCondition
     id::Int32
@@ -142,13 +145,15 @@ function DGSEMHIP(; RealT = Float64, polydeg::Integer, surface_flux = flux_centr
     DGSEMHIP(Int(polydeg), surface_flux, volume_integral, prec)
 end
 
-mutable struct SemidiscretizationHyperbolicHIP{Mesh, Equations, IC, Solver}
+mutable struct SemidiscretizationHyperbolicHIP{Mesh, Equations, IC, Solver, Cache}
     mesh::Mesh
     equations::Equations
     initial_condition::IC
     solver::Solver
     ctx::Ptr{Cvoid}
     nvalues::Int
+    dg_cpu::DGSEM        # host containers, for initial conditions written as functions
+    cache_cpu::Cache
 end
 
 # The outflow boundary of the reference's Burgers tests: flux(u_inner).
@@ -261,7 +266,8 @@ function SemidiscretizationHyperbolicHIP(mesh::TreeMesh, equations, initial_cond
     nv = Ref{Int32}(0); nn = Ref{Int32}(0); nval = Ref{Int64}(0)
     check(ccall((:trixi_hip_sizes, libtrixihip), Cint, (Ptr{Cvoid}, Ref{Int32}, Ref{Int32}, Ref{Int64}),
                 ctx[], nv, nn, nval))
-    semi = SemidiscretizationHyperbolicHIP(mesh, equations, initial_condition, solver, ctx[], Int(nval[]))
+    semi = SemidiscretizationHyperbolicHIP(mesh, equations, initial_condition, solver, ctx[], Int(nval[]),
+                                           dg_cpu, cache)
     finalizer(s -> ccall((:trixi_hip_destroy, libtrixihip), Cint, (Ptr{Cvoid},), s.ctx), semi)
     semi
 end
@@ -279,6 +285,16 @@ end
 
 function compute_coefficients_hip(t, semi::SemidiscretizationHyperbolicHIP)
     u = ROCVector{Float64}(undef, semi.nvalues)
+    if !is_builtin_ic(semi.initial_condition)
+        # an initial condition written as a Julia function: Trixi samples it
+        # on the host (Trixi's layout), the library packs it on the device
+        u_host = Trixi.compute_coefficients(semi.initial_condition, t, semi.mesh, semi.equations,
+                                            semi.dg_cpu, semi.cache_cpu)
+        u_trixi = ROCVector{Float64}(u_host)
+        check(ccall((:trixi_hip_pack, libtrixihip), Cint, (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Ptr{Cvoid}),
+                    semi.ctx, pointer(u), pointer(u_trixi), stream_ptr()))
+        return u
+    end
     check(ccall((:trixi_hip_compute_coefficients, libtrixihip), Cint,
                 (Ptr{Cvoid}, Ptr{Float64}, Int32, Float64, UInt64, Ptr{Cvoid}),
                 semi.ctx, pointer(u), ic_id(semi.initial_condition), t, ic_seed(semi.initial_condition),

@@ -12,7 +12,7 @@ import ctypes
 import numpy as np
 
 from . import _lib as L
-from .equations import (BoundaryConditionDirichlet, SourceTerms, boundary_condition_outflow,
+from .equations import (BoundaryConditionDirichlet, InitialCondition, SourceTerms, boundary_condition_outflow,
                         boundary_condition_periodic, boundary_condition_poisson_nonperiodic,
                         initial_condition_constant, initial_condition_poisson_nonperiodic)
 
@@ -163,11 +163,44 @@ class SemidiscretizationHyperbolicHIP:
 
 
 def compute_coefficients(t, semi, initial_condition=None):
-    """u(t) of the initial condition at every node (native layout)."""
+    """u(t) of the initial condition at every node (native layout).
+
+    A built initial condition is evaluated on the device.  A Python function
+    ``initial_condition(x, t, equations)`` returning the conservative variables
+    (Trixi's signature; ``prim2cons`` converts primitive states) is evaluated
+    on the host at every node, as Trixi's compute_coefficients! does (in 1D
+    the element end nodes move one ulp inwards, so that a discontinuity on an
+    element face is sampled from the element's own side), and packed into the
+    device state."""
     ic = initial_condition or semi.initial_condition
+    if not isinstance(ic, InitialCondition):
+        if not callable(ic):
+            raise TypeError(f"initial condition {ic!r} is neither built nor callable")
+        torch = _torch()
+        host = host_initial_state(ic, t, semi)
+        return pack(torch.from_numpy(host).to("cuda"), semi)
     u = semi.allocate_coefficients()
     L.check(L.lib().trixi_hip_compute_coefficients(semi.ctx, _ptr(u), ic.ic_id, float(t), ic.seed, _stream()))
     return u
+
+
+def host_initial_state(initial_condition, t, semi):
+    """Trixi-layout numpy array [E, (N,)*d, nvars] of initial_condition(x, t,
+    equations) at the node coordinates (Trixi's 1D end-node shift included)."""
+    x = semi._node_coordinates
+    nd, N, nv = semi.mesh.ndims, semi.nnodes, semi.nvariables
+    if nd == 1:
+        x = x.copy()
+        x[:, 0, 0] = np.nextafter(x[:, 0, 0], np.inf)
+        x[:, N - 1, 0] = np.nextafter(x[:, N - 1, 0], -np.inf)
+    pts = x.reshape(-1, nd)
+    out = np.empty((pts.shape[0], nv))
+    for i, xi in enumerate(pts):
+        val = np.asarray(initial_condition(tuple(xi), float(t), semi.equations), dtype=np.float64)
+        if val.shape != (nv,):
+            raise ValueError(f"initial condition returned {val.shape}, expected ({nv},)")
+        out[i] = val
+    return out.reshape(x.shape[:-1] + (nv,))
 
 
 def rhs_hip_(du_ode, u_ode, semi, t):

@@ -10,6 +10,7 @@ import ctypes
 import numpy as np
 
 from . import _lib as L
+from . import equations as E
 from .equations import density, density_pressure, first, flux_central, flux_ranocha, pressure, split_flux
 
 
@@ -100,18 +101,22 @@ class DGSEMHIP:
         c.volume_integral = vi.kind
         sflux, s_nc = split_flux(self.surface_flux)
         c.surface_flux = sflux.flux_id
-        nc_needed = equations.equation_id == L.EQ_GLM_MHD and equations.ndims > 1
-        if nc_needed != (s_nc is not None):
-            raise NotImplementedError("GLM-MHD in 2D/3D needs (flux, flux_nonconservative_powell) tuples, "
-                                      "other equations (and 1D MHD) plain fluxes")
+        quasi = equations.equation_id == L.EQ_EULER_QUASI1D
+        nc_needed = (equations.equation_id == L.EQ_GLM_MHD and equations.ndims > 1) or quasi
+        nc_kind = E.flux_nonconservative_chan_etal if quasi else E.flux_nonconservative_powell
+        if nc_needed != (s_nc is not None) or (s_nc is not None and s_nc is not nc_kind):
+            raise NotImplementedError("GLM-MHD in 2D/3D needs (flux, flux_nonconservative_powell) tuples, quasi-1D "
+                                      "Euler (flux_chan_etal, flux_nonconservative_chan_etal), other equations "
+                                      "(and 1D MHD) plain fluxes")
         c.volume_flux = L.FLUX_CENTRAL
         c.fv_flux = L.FLUX_CENTRAL
         c.alpha_smooth = 0
         c.alpha_max, c.alpha_min = 0.5, 0.001
         if vi.kind == L.VI_FD:
             vflux, v_nc = split_flux(vi.volume_flux)
-            if nc_needed != (v_nc is not None):
-                raise NotImplementedError("GLM-MHD volume flux must be (flux, flux_nonconservative_powell)")
+            if nc_needed != (v_nc is not None) or (v_nc is not None and v_nc is not nc_kind):
+                raise NotImplementedError("the volume flux must pair the flux with the equation's nonconservative "
+                                          "term (flux_nonconservative_powell, flux_nonconservative_chan_etal)")
             c.volume_flux = vflux.flux_id
         elif vi.kind == L.VI_HG:
             dg_flux, dg_nc = split_flux(vi.volume_flux_dg)
