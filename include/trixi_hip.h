@@ -74,8 +74,10 @@ enum trixi_hip_flux {
     TRIXI_HIP_FLUX_EC = 6,                   /* flux_ec (InviscidBurgersEquation1D)    */
     TRIXI_HIP_FLUX_HLL = 7,                  /* flux_hll = FluxHLL(min_max_speed_davis),
                                                 compressible Euler                     */
-    TRIXI_HIP_FLUX_CHAN_ETAL = 8             /* (flux_chan_etal,
+    TRIXI_HIP_FLUX_CHAN_ETAL = 8,            /* (flux_chan_etal,
                                                 flux_nonconservative_chan_etal), quasi-1D */
+    TRIXI_HIP_FLUX_HLLE = 9                  /* flux_hlle = FluxHLL(min_max_speed_einfeldt),
+                                                GLM-MHD 2D/3D                          */
 };
 enum trixi_hip_precision {
     TRIXI_HIP_FP64 = 0,  /* all arithmetic in FP64                                    */

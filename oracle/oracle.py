@@ -20,7 +20,7 @@ _LIB_PATH = os.path.join(_HERE, "liboracle.so")
 EQ_LINADV, EQ_EULER, EQ_GLM_MHD, EQ_BURGERS, EQ_HYPDIFF, EQ_EULER_MULTI, EQ_EULER_QUASI1D = range(7)
 VI_WEAK, VI_FD, VI_HG = 0, 1, 2
 (FLUX_CENTRAL, FLUX_LLF, FLUX_LLF_NAIVE, FLUX_RANOCHA, FLUX_SHIMA_ETAL, FLUX_HINDENLANG_GASSNER, FLUX_EC, FLUX_HLL,
- FLUX_CHAN_ETAL) = range(9)
+ FLUX_CHAN_ETAL, FLUX_HLLE) = range(10)
 (IC_CONSTANT, IC_CONVERGENCE_TEST, IC_WEAK_BLAST_WAVE, IC_SEEDED_RANDOM, IC_SMOOTH_PERIODIC, IC_WEAK_BLAST_WAVE_SEEDED,
  IC_SHOCK, IC_RAREFACTION, IC_EOC_GRAVITY, IC_POISSON_NONPERIODIC, IC_HARMONIC_NONPERIODIC) = range(11)
 BC_PERIODIC, BC_DIRICHLET = 0, 1

@@ -327,6 +327,42 @@ template <class T> void normal_speed(const Cfg& c, const T* u, int o, T& vn, T& 
     }
 }
 
+// Trixi calc_fast_wavespeed_roe (ideal GLM-MHD): Roe-averaged normal
+// velocity and fast magnetoacoustic speed (Cargo & Gallice 1997), with
+// total pressure p + |B|^2/2, enthalpy H = (rho e + p_total) / rho,
+// X = |[[B]]|^2 / (2 (sqrt(rho_l) + sqrt(rho_r))^2), b^2 = |B_roe|^2 / sqrt(rho_l rho_r),
+// a^2 = (2 - gamma) X + (gamma - 1)(H_roe - |v_roe|^2 / 2 - b^2).
+template <class T> void mhd_fast_wavespeed_roe(const Cfg& c, const T* ull, const T* urr, int o, T& vel, T& cf) {
+    T rl = ull[0], rr = urr[0];
+    T vl[3] = {ull[1] / rl, ull[2] / rl, ull[3] / rl}, vr[3] = {urr[1] / rr, urr[2] / rr, urr[3] / rr};
+    T Bl[3] = {ull[5], ull[6], ull[7]}, Br[3] = {urr[5], urr[6], urr[7]};
+    T psil = (c.nv > 8) ? ull[8] : T(0.0), psir = (c.nv > 8) ? urr[8] : T(0.0);
+    T kel = 0.5 * (ull[1] * vl[0] + ull[2] * vl[1] + ull[3] * vl[2]);
+    T ker = 0.5 * (urr[1] * vr[0] + urr[2] * vr[1] + urr[3] * vr[2]);
+    T mbl = Bl[0] * Bl[0] + Bl[1] * Bl[1] + Bl[2] * Bl[2], mbr = Br[0] * Br[0] + Br[1] * Br[1] + Br[2] * Br[2];
+    T pl = (c.gamma - 1.0) * (ull[4] - kel - 0.5 * mbl - 0.5 * psil * psil);
+    T pr = (c.gamma - 1.0) * (urr[4] - ker - 0.5 * mbr - 0.5 * psir * psir);
+    T ptl = pl + 0.5 * mbl, ptr = pr + 0.5 * mbr;
+    T sl = tsqrt(rl), sr = tsqrt(rr);
+    T inv_add = 1.0 / (sl + sr), inv_prod = 1.0 / (sl * sr);
+    T wl = sl * inv_add, wr = sr * inv_add;
+    T v_roe[3], B_roe[3];
+    for (int d = 0; d < 3; ++d) {
+        v_roe[d] = vl[d] * wl + vr[d] * wr;
+        B_roe[d] = Bl[d] * wl + Br[d] * wr;
+    }
+    T H_roe = (ull[4] + ptl) / rl * wl + (urr[4] + ptr) / rr * wr;
+    T dB0 = Bl[0] - Br[0], dB1 = Bl[1] - Br[1], dB2 = Bl[2] - Br[2];
+    T X = 0.5 * (dB0 * dB0 + dB1 * dB1 + dB2 * dB2) * inv_add * inv_add;
+    T b2 = (B_roe[0] * B_roe[0] + B_roe[1] * B_roe[1] + B_roe[2] * B_roe[2]) * inv_prod;
+    T a2 = (2.0 - c.gamma) * X +
+           (c.gamma - 1.0) * (H_roe - 0.5 * (v_roe[0] * v_roe[0] + v_roe[1] * v_roe[1] + v_roe[2] * v_roe[2]) - b2);
+    T ca = B_roe[o - 1] * B_roe[o - 1] * inv_prod;
+    T astar = tsqrt((a2 + b2) * (a2 + b2) - 4.0 * a2 * ca);
+    cf = tsqrt(0.5 * (a2 + b2 + astar));
+    vel = v_roe[o - 1];
+}
+
 // Trixi max_abs_speed (Davis-type) and max_abs_speed_naive.
 template <class T> T lambda_max(const Cfg& c, const T* ull, const T* urr, int o, bool naive) {
     if (c.eq == OR_EQ_LINADV) return T(fabs(c.a[o - 1]));
@@ -532,6 +568,27 @@ template <class T> void two_point(const Cfg& c, int kind, const T* ull, const T*
             normal_speed(c, ull, o, vl, cl);
             normal_speed(c, urr, o, vr, cr);
             const T lmin = tmin(vl - cl, vr - cr), lmax = tmax(vl + cl, vr + cr);
+            if (val(lmin) >= 0.0 && val(lmax) >= 0.0) {
+                for (int v = 0; v < c.nv; ++v) f[v] = fl[v];
+            } else if (val(lmax) <= 0.0 && val(lmin) <= 0.0) {
+                for (int v = 0; v < c.nv; ++v) f[v] = fr[v];
+            } else {
+                for (int v = 0; v < c.nv; ++v)
+                    f[v] = (lmax * fl[v] - lmin * fr[v] + lmin * lmax * (urr[v] - ull[v])) / (lmax - lmin);
+            }
+            return;
+        }
+        case OR_FLUX_HLLE: {
+            // Trixi flux_hlle = FluxHLL(min_max_speed_einfeldt), ideal GLM-MHD:
+            // lambda_min = min(v_ll - c_f,ll, v_roe - c_f,roe),
+            // lambda_max = max(v_rr + c_f,rr, v_roe + c_f,roe)
+            T fl[MAXV], fr[MAXV], vl, cl, vr, cr, vroe, croe;
+            phys_flux(c, ull, o, fl);
+            phys_flux(c, urr, o, fr);
+            normal_speed(c, ull, o, vl, cl);
+            normal_speed(c, urr, o, vr, cr);
+            mhd_fast_wavespeed_roe(c, ull, urr, o, vroe, croe);
+            const T lmin = tmin(vl - cl, vroe - croe), lmax = tmax(vr + cr, vroe + croe);
             if (val(lmin) >= 0.0 && val(lmax) >= 0.0) {
                 for (int v = 0; v < c.nv; ++v) f[v] = fl[v];
             } else if (val(lmax) <= 0.0 && val(lmin) <= 0.0) {
@@ -1207,9 +1264,19 @@ void rhs_impl(const Cfg& c, int64_t E, int64_t I, const int32_t* nbr, const int3
                         two_point(c, c.sflux, uL, uS, o, flux);
                     else
                         two_point(c, c.sflux, uS, uL, o, flux);
+                    // nonconservative terms as the reference's mortar_flux_kernel!
+                    // (dg_2d_kernel.jl:1318-1337, dg_3d_kernel.jl:1598-1640): the small
+                    // faces take 1/2 nc(u_large, u_small), the projected large-face
+                    // fluxes 1/2 nc(u_small, u_large)
+                    T nc_small[MAXV], nc_large[MAXV];
+                    for (int v = 0; v < nv; ++v) nc_small[v] = nc_large[v] = 0.0;
+                    if (has_noncons(c)) {
+                        noncons_powell(c, uL, uS, o, nc_small);
+                        noncons_powell(c, uS, uL, o, nc_large);
+                    }
                     for (int v = 0; v < nv; ++v) {
-                        SF(v, q, dir_small, small) = flux[v];
-                        fstar[((size_t)p * nf + q) * nv + v] = flux[v];
+                        SF(v, q, dir_small, small) = flux[v] + 0.5 * nc_small[v];
+                        fstar[((size_t)p * nf + q) * nv + v] = flux[v] + 0.5 * nc_large[v];
                     }
                 }
             }

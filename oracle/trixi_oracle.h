@@ -38,7 +38,8 @@ enum {
     OR_FLUX_HINDENLANG_GASSNER = 5,
     OR_FLUX_EC = 6,           /* Burgers flux_ec                             */
     OR_FLUX_HLL = 7,          /* FluxHLL(min_max_speed_davis), Euler         */
-    OR_FLUX_CHAN_ETAL = 8     /* quasi-1D: flux_chan_etal + flux_nonconservative_chan_etal */
+    OR_FLUX_CHAN_ETAL = 8,    /* quasi-1D: flux_chan_etal + flux_nonconservative_chan_etal */
+    OR_FLUX_HLLE = 9          /* GLM-MHD: FluxHLL(min_max_speed_einfeldt) */
 };
 enum {
     OR_IC_CONSTANT = 0,

@@ -40,7 +40,8 @@ class Case:
     ic_fn: object = None    # initial condition as a Python function (x, t, equations) -> conservative state
 
 
-FLUX_NAMES = ("central", "llf", "llf_naive", "ranocha", "shima_etal", "hindenlang_gassner", "ec", "hll", "chan_etal")
+FLUX_NAMES = ("central", "llf", "llf_naive", "ranocha", "shima_etal", "hindenlang_gassner", "ec", "hll", "chan_etal",
+              "hlle")
 IC_NAMES = ("constant", "convergence_test", "weak_blast_wave", "seeded_random", "smooth_periodic",
             "weak_blast_wave_seeded", "shock", "rarefaction", "eoc_gravity", "poisson_nonperiodic",
             "harmonic_nonperiodic")
@@ -80,7 +81,7 @@ def _flux(th, name, pair):
     f = {"central": th.flux_central, "llf": th.flux_lax_friedrichs,
          "llf_naive": th.FluxLaxFriedrichs(th.max_abs_speed_naive), "ranocha": th.flux_ranocha,
          "shima_etal": th.flux_shima_etal, "hindenlang_gassner": th.flux_hindenlang_gassner, "ec": th.flux_ec,
-         "hll": th.flux_hll, "chan_etal": th.flux_chan_etal}[name]
+         "hll": th.flux_hll, "chan_etal": th.flux_chan_etal, "hlle": th.flux_hlle}[name]
     if pair == "mhd":
         return (f, th.flux_nonconservative_powell)
     if pair == "quasi1d":

@@ -262,6 +262,17 @@ CASES = [
     *MULTI_CASES,
     *QUASI_CASES,
     EULER_BLAST_1D,
+    # test/tree_dgsem_{2,3}d/mhd_alfven_wave_mortar.jl as written: volume
+    # (flux_hindenlang_gassner, powell), surface (flux_hlle, powell), polydeg 3
+    Case("mhd2d_mortar_hlle", 2, "mhd", 4, "fd", "hindenlang_gassner", "hlle", cmin=0.0, cmax=2 ** 0.5,
+         ic="convergence_test", gamma=5 / 3, patches=(((0.25 * 2 ** 0.5,) * 2, (0.75 * 2 ** 0.5,) * 2),)),
+    Case("mhd3d_mortar_hlle", 3, "mhd", 2, "fd", "hindenlang_gassner", "hlle", cmin=-1.0, cmax=1.0,
+         ic="convergence_test", gamma=5 / 3, patches=(((-0.5,) * 3, (0.5,) * 3),)),
+    Case("mhd2d_mortar_llf_random", 2, "mhd", 3, "fd", "hindenlang_gassner", "llf", gamma=5 / 3,
+         patches=(((-1.0, -1.0), (0.5, 1.0)),)),
+    Case("mhd3d_mortar_hlle_random", 3, "mhd", 2, "fd", "hindenlang_gassner", "hlle", gamma=5 / 3,
+         patches=MORTAR_3D),
+    Case("mhd2d_hlle_periodic_random", 2, "mhd", 3, "fd", "hindenlang_gassner", "hlle", gamma=5 / 3),
 ]
 
 
@@ -274,7 +285,11 @@ def test_rhs_fp64_matches_oracle(th, oracle, torch, case):
     assert err <= FP64_TOL, err
 
 
-MIXED_CASES = [c for c in CASES if c.vi in ("fd", "hg") and c.polydeg == 3] + MHD_HG_P4 + \
+# (the MHD Alfven wave on a refined mesh: du is a cancellation of O(1) flux
+# terms, relative errors are amplified; the seeded mortar states cover mixed)
+_MIXED_SKIP = ("mhd2d_mortar_hlle", "mhd3d_mortar_hlle")
+MIXED_CASES = [c for c in CASES if c.vi in ("fd", "hg") and c.polydeg == 3 and not c.patches
+               and c.name not in _MIXED_SKIP] + MHD_HG_P4 + \
     [QUASI_CASES[0], QUASI_CASES[2]] + \
     [Case("euler2d_weak_llf_mixed", 2, "euler", 3), Case("linadv1d_mixed", 1, "linadv", 4, cmin=-1.0, cmax=1.0,
                                                         ic="convergence_test"),
@@ -283,7 +298,7 @@ MIXED_CASES = [c for c in CASES if c.vi in ("fd", "hg") and c.polydeg == 3] + MH
      Case("linadv3d_weak_mixed", 3, "linadv", 2, ic="seeded_random"),
      Case("linadv2d_weak_mixed", 2, "linadv", 3, ic="seeded_random"),
      Case("euler3d_weak_hll_mixed", 3, "euler", 2, sflux="hll", ic="smooth_periodic")] + \
-    [c for c in CASES if c.patches and c.name != "linadv3d_mortar_central" and
+    [c for c in CASES if c.patches and c.name != "linadv3d_mortar_central" and c.name not in _MIXED_SKIP and
      not (c.ic == "convergence_test" and c.source)]  # (cancellation at the manufactured state, as above)
 
 

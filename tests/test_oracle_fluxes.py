@@ -331,3 +331,47 @@ def test_multicomponent_single_gas_reduces_to_euler(oracle, nd):
             for k in (O.FLUX_RANOCHA, O.FLUX_LLF, O.FLUX_LLF_NAIVE):
                 np.testing.assert_allclose(O.two_point_flux(cm, k, ul, ur, o)[perm],
                                            O.two_point_flux(ce, k, el, er, o), rtol=1e-12, atol=1e-14)
+
+
+def test_mhd_hlle_structure_and_euler_limit(oracle):
+    """flux_hlle (FluxHLL with Einfeldt speeds) of GLM-MHD: consistent, and
+    with B = psi = 0 it is the textbook HLLE flux of the Euler system (Roe
+    averages with sqrt(rho) weights of v and H, c_roe^2 = (gamma-1)(H_roe -
+    |v_roe|^2/2); lambda_min = min(v_l - c_l, v_roe - c_roe), lambda_max =
+    max(v_r + c_r, v_roe + c_roe)), computed here independently."""
+    O = oracle
+    g = 5.0 / 3.0
+    rng = np.random.default_rng(40)
+    for nd in (2, 3):
+        cfg = O.make_cfg(nd, O.EQ_GLM_MHD, gamma=g, c_h=1.0)
+        for _ in range(20):
+            ul, ur = rng_state_mhd(O, cfg, rng), rng_state_mhd(O, cfg, rng)
+            for o in range(1, nd + 1):
+                np.testing.assert_allclose(O.two_point_flux(cfg, O.FLUX_HLLE, ul, ul, o), O.flux(cfg, ul, o),
+                                           rtol=1e-13, atol=1e-14)
+            for u in (ul, ur):
+                u[4] -= 0.5 * (u[5:8] @ u[5:8] + u[8] ** 2)
+                u[5:9] = 0.0
+            for o in range(1, nd + 1):
+                def prim(u):
+                    rho = u[0]
+                    v = u[1:4] / rho
+                    p = (g - 1) * (u[4] - 0.5 * rho * v @ v)
+                    return rho, v, p, (u[4] + p) / rho
+                rl, vl, pl, Hl = prim(ul)
+                rr, vr, pr, Hr = prim(ur)
+                wl, wr = np.sqrt(rl) / (np.sqrt(rl) + np.sqrt(rr)), np.sqrt(rr) / (np.sqrt(rl) + np.sqrt(rr))
+                v_roe, H_roe = wl * vl + wr * vr, wl * Hl + wr * Hr
+                c_roe = np.sqrt((g - 1) * (H_roe - 0.5 * v_roe @ v_roe))
+                cl, cr = np.sqrt(g * pl / rl), np.sqrt(g * pr / rr)
+                lmin = min(vl[o - 1] - cl, v_roe[o - 1] - c_roe)
+                lmax = max(vr[o - 1] + cr, v_roe[o - 1] + c_roe)
+                fl, fr = O.flux(cfg, ul, o), O.flux(cfg, ur, o)
+                if lmin >= 0:
+                    ref = fl
+                elif lmax <= 0:
+                    ref = fr
+                else:
+                    ref = (lmax * fl - lmin * fr + lmin * lmax * (ur - ul)) / (lmax - lmin)
+                np.testing.assert_allclose(O.two_point_flux(cfg, O.FLUX_HLLE, ul, ur, o), ref, rtol=1e-12,
+                                           atol=1e-13)

@@ -873,3 +873,39 @@ def test_quasi1d_source_and_convergence(oracle):
               - O.compute_coefficients(cfg, mesh, O.IC_CONVERGENCE_TEST, t - h)) / (2 * h)
         errs.append(np.abs(O.rhs(cfg, mesh, u, t=t) - ut).max())
     assert np.log2(errs[0] / errs[1]) > 3.5, errs
+
+
+@pytest.mark.parametrize("nd", [2, 3])
+def test_mhd_mortar_free_stream_and_side_order(oracle, nd):
+    """GLM-MHD on refined meshes (test/tree_dgsem_{2,3}d/mhd_alfven_wave_mortar.jl):
+    a constant state stays constant through the mortars (Powell terms
+    included); mass, momentum-free components and B stay conservative where
+    the nonconservative terms vanish (rho); and the Alfven wave is
+    approximated at the DG rate across the refinement.  The side order of
+    the nonconservative mortar terms (small faces 1/2 nc(u_large, u_small))
+    follows the reference's mortar_flux_kernel! text."""
+    O = oracle
+    patches = (((0.25 * 2 ** 0.5,) * nd, (0.75 * 2 ** 0.5,) * nd),)
+    for sflux in (O.FLUX_HLLE, O.FLUX_LLF):
+        cfg = O.make_cfg(nd, O.EQ_GLM_MHD, 3, O.VI_FD, O.FLUX_HINDENLANG_GASSNER, sflux, gamma=5 / 3)
+        mesh = O.RefinedMesh(nd, 2, 0.0, 2 ** 0.5, 3, patches)
+        assert mesh.nmortars > 0
+        u = O.compute_coefficients(cfg, mesh, O.IC_CONSTANT)
+        u[..., 5:8] = (0.3, -0.2, 0.1)
+        u[..., 4] += 0.5 * (0.09 + 0.04 + 0.01)
+        assert np.abs(O.rhs(cfg, mesh, u)).max() < 1e-13
+        u = O.compute_coefficients(cfg, mesh, O.IC_SEEDED_RANDOM)
+        du = O.rhs(cfg, mesh, u)
+        # the density equation has no nonconservative term
+        assert abs(O.integrate(cfg, mesh, du)[0]) < 1e-13 * np.abs(du).max() * mesh.total_volume
+        if nd == 2:
+            errs = []
+            t = 0.1
+            for lvl in (2, 3):
+                mesh = O.RefinedMesh(nd, lvl, 0.0, 2 ** 0.5, 3, patches)
+                u = O.compute_coefficients(cfg, mesh, O.IC_CONVERGENCE_TEST, t)
+                h = 1e-6
+                ut = (O.compute_coefficients(cfg, mesh, O.IC_CONVERGENCE_TEST, t + h)
+                      - O.compute_coefficients(cfg, mesh, O.IC_CONVERGENCE_TEST, t - h)) / (2 * h)
+                errs.append(np.abs(O.rhs(cfg, mesh, u, t=t) - ut).max())
+            assert np.log2(errs[0] / errs[1]) > 2.5, errs

@@ -103,16 +103,26 @@ __global__ void __launch_bounds__(256) mortar_kernel(const MortarArgs a) {
         for (int v = 0; v < NV; ++v) uS[v] = a.u[(int64_t)(v * NN + node) * E + small];
         const Node nl = EQ::template node<double, LOGS>(uL, P);
         const Node ns = EQ::template node<double, LOGS>(uS, P);
-        double f[NV];
+        double f[NV], nc_small[NV], nc_large[NV];
         if (side == 1)
             EQ::template two_point<SF>(nl, ns, od, P, f);
         else
             EQ::template two_point<SF>(ns, nl, od, P, f);
+        // nonconservative terms in the reference's mortar order
+        // (dg_2d_kernel.jl:1318-1337, dg_3d_kernel.jl:1598-1640): the small
+        // face takes 1/2 nc(u_large, u_small), the projected large-face flux
+        // 1/2 nc(u_small, u_large)
+#pragma unroll
+        for (int v = 0; v < NV; ++v) nc_small[v] = nc_large[v] = 0.0;
+        if constexpr (EQ::NONCONS) {
+            EQ::noncons(nl, ns, od, nc_small);
+            EQ::noncons(ns, nl, od, nc_large);
+        }
         double* out = a.mflux + ((m * (K + 1) + p) * NL + q) * NV;
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
-            out[v] = f[v];
-            fstar[lm][p][q][v] = f[v];
+            out[v] = f[v] + 0.5 * nc_small[v];
+            fstar[lm][p][q][v] = f[v] + 0.5 * nc_large[v];
         }
     }
     __syncthreads();

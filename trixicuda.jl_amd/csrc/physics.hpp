@@ -42,7 +42,8 @@ enum Flux : int {
     FLUX_HINDENLANG_GASSNER = 5,
     FLUX_EC = 6,          // flux_ec of the inviscid Burgers equation
     FLUX_HLL = 7,         // FluxHLL(min_max_speed_davis) (compressible Euler)
-    FLUX_CHAN_ETAL = 8    // flux_chan_etal + flux_nonconservative_chan_etal (quasi-1D Euler)
+    FLUX_CHAN_ETAL = 8,   // flux_chan_etal + flux_nonconservative_chan_etal (quasi-1D Euler)
+    FLUX_HLLE = 9         // flux_hlle = FluxHLL(min_max_speed_einfeldt) (GLM-MHD)
 };
 
 // two-point fluxes that take ln-means (they need log(rho), log(p) per node)
@@ -899,6 +900,43 @@ struct GlmMhd {
         return rsqrt_(R(0.5) * s + R(0.5) * rsqrt_(s * s - R(4) * a_square * bo2));
     }
 
+    // Roe-averaged normal velocity and fast magnetoacoustic speed (Cargo &
+    // Gallice 1997): weights sqrt(rho) / (sqrt(rho_l) + sqrt(rho_r)) for v, B
+    // and the total enthalpy H = (rho e + p + |B|^2/2) / rho, the jump term
+    // X = |[[B]]|^2 / (2 (sqrt(rho_l) + sqrt(rho_r))^2), b^2 = |B|^2 / sqrt(rho_l rho_r),
+    // a^2 = (2 - gamma) X + (gamma - 1)(H - |v|^2/2 - b^2)
+    template <typename R>
+    __device__ static void roe_fast_speed(const Node<R>& l, const Node<R>& r, int o, const PhysR<R>& P, R& vel,
+                                          R& cf) {
+        const R sl = rsqrt_(l.rho), sr = rsqrt_(r.rho);
+        const R inv_add = rdiv(R(1), sl + sr), inv_prod = rdiv(R(1), sl * sr);
+        const R wl = sl * inv_add, wr = sr * inv_add;
+        R v2 = R(0), B2 = R(0), dB2 = R(0), Bo = R(0);
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            const R vd = l.v[d] * wl + r.v[d] * wr;
+            const R Bd = l.B[d] * wl + r.B[d] * wr;
+            const R db = l.B[d] - r.B[d];
+            v2 += vd * vd;
+            B2 += Bd * Bd;
+            dB2 += db * db;
+            if (d == o) {
+                vel = vd;
+                Bo = Bd;
+            }
+        }
+        const R mbl = l.B[0] * l.B[0] + l.B[1] * l.B[1] + l.B[2] * l.B[2];
+        const R mbr = r.B[0] * r.B[0] + r.B[1] * r.B[1] + r.B[2] * r.B[2];
+        const R Hl = rdiv(l.cons(4) + l.p + R(0.5) * mbl, l.rho), Hr = rdiv(r.cons(4) + r.p + R(0.5) * mbr, r.rho);
+        const R H = Hl * wl + Hr * wr;
+        const R X = R(0.5) * dB2 * inv_add * inv_add;
+        const R b2 = B2 * inv_prod;
+        const R a2 = (R(2) - P.gamma) * X + P.gm1 * (H - R(0.5) * v2 - b2);
+        const R ca = Bo * Bo * inv_prod;
+        const R astar = rsqrt_((a2 + b2) * (a2 + b2) - R(4) * a2 * ca);
+        cf = rsqrt_(R(0.5) * (a2 + b2 + astar));
+    }
+
     template <int KIND, typename R>
     __device__ static void two_point(const Node<R>& l, const Node<R>& r, int o, const PhysR<R>& P, R* f) {
         if (KIND == FLUX_HINDENLANG_GASSNER) {
@@ -951,6 +989,27 @@ struct GlmMhd {
                 const R hl = R(-0.5) * lam;
 #pragma unroll
                 for (int q = 0; q < 9; ++q) f[q] = R(0.5) * (fl[q] + fr[q]) + hl * (r.cons(q) - l.cons(q));
+            } else if (KIND == FLUX_HLLE) {
+                // HLL with Einfeldt's estimates: the states' fast speeds and the
+                // Roe-averaged fast speed of Cargo & Gallice (1997)
+                R vroe, croe;
+                roe_fast_speed(l, r, o, P, vroe, croe);
+                const R a = l.v[o] - fast_speed(l, o, P), b = vroe - croe;
+                const R cc = r.v[o] + fast_speed(r, o, P), dd = vroe + croe;
+                const R lmin = b < a ? b : a;
+                const R lmax = dmax(cc, dd);
+                if (lmin >= R(0) && lmax >= R(0)) {
+#pragma unroll
+                    for (int q = 0; q < 9; ++q) f[q] = fl[q];
+                } else if (lmax <= R(0) && lmin <= R(0)) {
+#pragma unroll
+                    for (int q = 0; q < 9; ++q) f[q] = fr[q];
+                } else {
+                    const R inv = rdiv(R(1), lmax - lmin);
+                    const R fll = lmax * inv, frr = lmin * inv, fdiss = lmin * lmax * inv;
+#pragma unroll
+                    for (int q = 0; q < 9; ++q) f[q] = fll * fl[q] - frr * fr[q] + fdiss * (r.cons(q) - l.cons(q));
+                }
             } else {
 #pragma unroll
                 for (int q = 0; q < 9; ++q) f[q] = R(0.5) * (fl[q] + fr[q]);

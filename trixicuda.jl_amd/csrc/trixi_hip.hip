@@ -63,6 +63,7 @@ void register_mhd1d();
 void register_hypdiff();
 void register_euler_multi();
 void register_euler_quasi1d();
+void register_mhd_mortar();
 void register_vjp_linadv();
 void register_vjp_euler1d();
 void register_vjp_euler2d();
@@ -89,6 +90,7 @@ static void ensure_registered() {
         register_hypdiff();
         register_euler_multi();
         register_euler_quasi1d();
+        register_mhd_mortar();
         register_vjp_linadv();
         register_vjp_euler1d();
         register_vjp_euler2d();

@@ -232,13 +232,20 @@ def min_max_speed_davis(*args):  # identity marker of Trixi's min_max_speed_davi
     raise TypeError("wave-speed estimates are selected, not called, on the HIP path")
 
 
+def min_max_speed_einfeldt(*args):  # identity marker of Trixi's min_max_speed_einfeldt
+    raise TypeError("wave-speed estimates are selected, not called, on the HIP path")
+
+
 def FluxHLL(min_max_speed=min_max_speed_davis):
+    if min_max_speed is min_max_speed_einfeldt:
+        return NumericalFlux("FluxHLL(min_max_speed_einfeldt)", L.FLUX_HLLE)
     if min_max_speed is not min_max_speed_davis:
-        raise NotImplementedError("FluxHLL is built with min_max_speed_davis")
+        raise NotImplementedError("FluxHLL is built with min_max_speed_davis and min_max_speed_einfeldt")
     return NumericalFlux("FluxHLL(min_max_speed_davis)", L.FLUX_HLL)
 
 
 flux_hll = FluxHLL()  # compressible Euler
+flux_hlle = FluxHLL(min_max_speed_einfeldt)  # GLM-MHD
 flux_nonconservative_powell = NumericalFlux("flux_nonconservative_powell", -1)
 # quasi-1D Euler: Chan et al. (2023) entropy-conservative flux and its
 # nonconservative pressure-gradient term (used as a pair)

@@ -94,7 +94,8 @@ flux_id(::typeof(flux_ranocha)) = 3
 flux_id(::typeof(flux_shima_etal)) = 4
 flux_id(::typeof(flux_hindenlang_gassner)) = 5
 flux_id(::typeof(flux_ec)) = 6
-flux_id(f::FluxHLL) = f.min_max_speed === min_max_speed_davis ? 7 :
+flux_id(f::FluxHLL) = f.min_max_speed === min_max_speed_einfeldt ? 9 :
+                      f.min_max_speed === min_max_speed_davis ? 7 :
     throw(ArgumentError("FluxHLL is built with min_max_speed_davis"))
 flux_id(::typeof(flux_chan_etal)) = 8
 flux_id(f::Tuple) = (f[2] === flux_nonconservative_powell || f[2] === flux_nonconservative_chan_etal ?
