@@ -56,8 +56,13 @@ enum trixi_hip_equation {
     TRIXI_HIP_COMPRESSIBLE_EULER_MULTICOMPONENT = 5, /* CompressibleEulerMulticomponent-
                                               Equations{1,2}D: (rho v, rho e, rho_1..
                                               rho_ncomponents), 1 or 2 components  */
-    TRIXI_HIP_COMPRESSIBLE_EULER_QUASI1D = 6 /* CompressibleEulerEquationsQuasi1D:
+    TRIXI_HIP_COMPRESSIBLE_EULER_QUASI1D = 6, /* CompressibleEulerEquationsQuasi1D:
                                               (a rho, a rho v, a e, a)              */
+    TRIXI_HIP_SHALLOW_WATER = 7            /* ShallowWaterEquations2D: (h, h v1, h v2, b);
+                                              volume (flux_wintermeyer_etal,
+                                              flux_nonconservative_wintermeyer_etal),
+                                              surface (any flux,
+                                              flux_nonconservative_fjordholm_etal) */
 };
 enum trixi_hip_volume_integral {
     TRIXI_HIP_VOLUME_WEAK_FORM = 0,          /* VolumeIntegralWeakForm          */
@@ -76,8 +81,10 @@ enum trixi_hip_flux {
                                                 compressible Euler                     */
     TRIXI_HIP_FLUX_CHAN_ETAL = 8,            /* (flux_chan_etal,
                                                 flux_nonconservative_chan_etal), quasi-1D */
-    TRIXI_HIP_FLUX_HLLE = 9                  /* flux_hlle = FluxHLL(min_max_speed_einfeldt),
+    TRIXI_HIP_FLUX_HLLE = 9,                 /* flux_hlle = FluxHLL(min_max_speed_einfeldt),
                                                 GLM-MHD 2D/3D                          */
+    TRIXI_HIP_FLUX_WINTERMEYER_ETAL = 10,    /* shallow water                          */
+    TRIXI_HIP_FLUX_FJORDHOLM_ETAL = 11       /* shallow water                          */
 };
 enum trixi_hip_precision {
     TRIXI_HIP_FP64 = 0,  /* all arithmetic in FP64                                    */
@@ -171,6 +178,9 @@ typedef struct trixi_hip_config {
     int32_t reserved;         /* 0                                                */
     double gammas[4];         /* multicomponent Euler: gamma_k of each component  */
     double gas_constants[4];  /* multicomponent Euler: R_k of each component      */
+    double gravity;           /* shallow water: gravitational acceleration g      */
+    double H0;                /* shallow water: reference water height (unused by
+                                 the built states)                                */
 } trixi_hip_config;
 
 /* Mesh and connectivity (host arrays, copied to the device by create): the

@@ -17,10 +17,10 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_PATH = os.path.join(_HERE, "liboracle.so")
 
-EQ_LINADV, EQ_EULER, EQ_GLM_MHD, EQ_BURGERS, EQ_HYPDIFF, EQ_EULER_MULTI, EQ_EULER_QUASI1D = range(7)
+EQ_LINADV, EQ_EULER, EQ_GLM_MHD, EQ_BURGERS, EQ_HYPDIFF, EQ_EULER_MULTI, EQ_EULER_QUASI1D, EQ_SHALLOW_WATER = range(8)
 VI_WEAK, VI_FD, VI_HG = 0, 1, 2
 (FLUX_CENTRAL, FLUX_LLF, FLUX_LLF_NAIVE, FLUX_RANOCHA, FLUX_SHIMA_ETAL, FLUX_HINDENLANG_GASSNER, FLUX_EC, FLUX_HLL,
- FLUX_CHAN_ETAL, FLUX_HLLE) = range(10)
+ FLUX_CHAN_ETAL, FLUX_HLLE, FLUX_WINTERMEYER_ETAL, FLUX_FJORDHOLM_ETAL) = range(12)
 (IC_CONSTANT, IC_CONVERGENCE_TEST, IC_WEAK_BLAST_WAVE, IC_SEEDED_RANDOM, IC_SMOOTH_PERIODIC, IC_WEAK_BLAST_WAVE_SEEDED,
  IC_SHOCK, IC_RAREFACTION, IC_EOC_GRAVITY, IC_POISSON_NONPERIODIC, IC_HARMONIC_NONPERIODIC) = range(11)
 BC_PERIODIC, BC_DIRICHLET = 0, 1
@@ -63,6 +63,8 @@ class OracleCfg(ctypes.Structure):
         ("reserved2", ctypes.c_int32),
         ("gammas", ctypes.c_double * 4),
         ("gas_constants", ctypes.c_double * 4),
+        ("gravity", ctypes.c_double),
+        ("H0", ctypes.c_double),
     ]
 
 
@@ -132,8 +134,9 @@ def make_cfg(ndims, equation, polydeg=3, volume_integral=VI_WEAK, volume_flux=FL
              advection_velocity=(1.0, 1.0, 1.0), alpha_max=0.5, alpha_min=0.001,
              alpha_smooth=True, ic_length=4.0, boundary_condition=BC_PERIODIC, boundary_ic=IC_CONSTANT,
              boundary_seed=0, source_terms=SOURCE_NONE, indicator_variable=0, boundary_outflow_faces=0,
-             nu=1.0, Lr=1.0 / (2.0 * np.pi), gammas=(), gas_constants=()):
+             nu=1.0, Lr=1.0 / (2.0 * np.pi), gammas=(), gas_constants=(), gravity=9.81):
     c = OracleCfg()
+    c.gravity = gravity
     c.nu, c.Lr = nu, Lr
     c.ncomponents = len(gammas)
     for k, (g, r) in enumerate(zip(gammas, gas_constants)):

@@ -83,6 +83,7 @@ struct Cfg {
     int nd, eq, N, nv, vi, vflux, sflux, fvflux, smooth;
     int nc;                     // multicomponent Euler: components
     double mgamma[4], mR[4], mcv[4];  // gamma_k, R_k, c_v,k = R_k / (gamma_k - 1)
+    double grav;                      // shallow water
     double gamma, inv_gm1, c_h, a[3], amax, amin, ic_length, nu, inv_Tr;
     int bc, bc_ic, src, ivar, outflow;
     uint64_t bc_seed;
@@ -125,6 +126,7 @@ Cfg make_cfg(const oracle_cfg* c) {
     k.mori = c->mortar_orientations;
     k.mls = c->mortar_large_sides;
     k.nc = (c->equation == OR_EQ_EULER_MULTI) ? c->ncomponents : 0;
+    k.grav = c->gravity;
     for (int i = 0; i < 4; ++i) {
         k.mgamma[i] = (i < k.nc) ? c->gammas[i] : 0.0;
         k.mR[i] = (i < k.nc) ? c->gas_constants[i] : 0.0;
@@ -222,6 +224,17 @@ template <class T> void quasi_prim(const Cfg& c, const T* u, T& rho, T& v, T& p,
 
 template <class T> void phys_flux(const Cfg& c, const T* u, int o, T* f) {
     const int oi = o - 1;
+    if (c.eq == OR_EQ_SHALLOW_WATER) {
+        // Trixi ShallowWaterEquations2D flux: (h v_o, h v1 v_o + g h^2/2 [o=1],
+        // h v2 v_o + g h^2/2 [o=2], 0)
+        const T h = u[0], v1 = u[1] / h, v2 = u[2] / h, p = 0.5 * c.grav * h * h;
+        const T hvo = u[o];
+        f[0] = hvo;
+        f[1] = (o == 1) ? u[1] * v1 + p : u[1] * v2;
+        f[2] = (o == 1) ? u[2] * v1 : u[2] * v2 + p;
+        f[3] = 0.0;
+        return;
+    }
     if (c.eq == OR_EQ_EULER_QUASI1D) {
         // Trixi flux: (a rho v, a rho v^2, a v (e + p), 0), e = (a e) / a; the
         // pressure gradient is the nonconservative term a p_x
@@ -321,6 +334,9 @@ template <class T> void normal_speed(const Cfg& c, const T* u, int o, T& vn, T& 
         T rho, p, a;
         quasi_prim(c, u, rho, vn, p, a);
         cs = tsqrt(c.gamma * p / rho);
+    } else if (c.eq == OR_EQ_SHALLOW_WATER) {
+        vn = u[o] / u[0];
+        cs = tsqrt(c.grav * u[0]);
     } else {
         vn = u[1 + oi] / u[0];
         cs = mhd_fast_wavespeed(c, u, o);
@@ -512,7 +528,20 @@ void flux_hindenlang_gassner_mhd(const Cfg& c, const T* ull, const T* urr, int o
 // flux_nonconservative_powell (Trixi ideal_glm_mhd_3d.jl, orientation form):
 // Powell term (0, B, v.B, v, 0) and Galilean GLM term, multiplied by the
 // right state's normal B (resp. psi); callers apply the factor 0.5.
-template <class T> void noncons_powell(const Cfg& c, const T* ull, const T* urr, int o, T* f) {
+// Nonconservative two-point terms; `surf` selects the surface term where the
+// volume and surface terms differ (shallow water).
+template <class T> void noncons_flux(const Cfg& c, const T* ull, const T* urr, int o, T* f, bool surf) {
+    if (c.eq == OR_EQ_SHALLOW_WATER) {
+        // flux_nonconservative_wintermeyer_etal (volume): g h_ll b_rr;
+        // flux_nonconservative_fjordholm_etal (surface): g h_ll b_ll + g h_avg (b_rr - b_ll)
+        const T t = surf ? c.grav * ull[0] * ull[3] + c.grav * (0.5 * (ull[0] + urr[0])) * (urr[3] - ull[3])
+                         : c.grav * ull[0] * urr[3];
+        f[0] = 0.0;
+        f[1] = (o == 1) ? t : T(0.0);
+        f[2] = (o == 2) ? t : T(0.0);
+        f[3] = 0.0;
+        return;
+    }
     if (c.eq == OR_EQ_EULER_QUASI1D) {
         // flux_nonconservative_chan_etal: (0, a_ll p_rr, 0, 0), the pressure
         // gradient a p_x in the same left-state-times-right-state form
@@ -555,6 +584,27 @@ template <class T> void two_point(const Cfg& c, int kind, const T* ull, const T*
         case OR_FLUX_CHAN_ETAL:
             flux_chan_etal(c, ull, urr, f);
             return;
+        case OR_FLUX_WINTERMEYER_ETAL:
+        case OR_FLUX_FJORDHOLM_ETAL: {
+            // Trixi flux_wintermeyer_etal: mean h v_o, p = g h_ll h_rr / 2;
+            // flux_fjordholm_etal: h_avg v_o,avg, p = g (h_ll^2 + h_rr^2) / 4
+            const T hl = ull[0], hr = urr[0];
+            const T v1l = ull[1] / hl, v2l = ull[2] / hl, v1r = urr[1] / hr, v2r = urr[2] / hr;
+            const T v1_avg = 0.5 * (v1l + v1r), v2_avg = 0.5 * (v2l + v2r);
+            T f1, p_avg;
+            if (kind == OR_FLUX_WINTERMEYER_ETAL) {
+                f1 = 0.5 * (ull[o] + urr[o]);
+                p_avg = 0.5 * c.grav * hl * hr;
+            } else {
+                f1 = 0.5 * (hl + hr) * ((o == 1) ? v1_avg : v2_avg);
+                p_avg = 0.25 * c.grav * (hl * hl + hr * hr);
+            }
+            f[0] = f1;
+            f[1] = f1 * v1_avg + ((o == 1) ? p_avg : T(0.0));
+            f[2] = f1 * v2_avg + ((o == 2) ? p_avg : T(0.0));
+            f[3] = 0.0;
+            return;
+        }
         case OR_FLUX_HINDENLANG_GASSNER:
             flux_hindenlang_gassner_mhd(c, ull, urr, o, f);
             return;
@@ -610,6 +660,8 @@ template <class T> void two_point(const Cfg& c, int kind, const T* ull, const T*
             T lam = lambda_max(c, ull, urr, o, kind == OR_FLUX_LLF_NAIVE);
             for (int v = 0; v < c.nv; ++v)
                 f[v] = 0.5 * (fl[v] + fr[v]) + (-0.5 * lam * (urr[v] - ull[v]));
+            // Trixi's shallow-water LLF dissipation leaves the bottom topography out
+            if (c.eq == OR_EQ_SHALLOW_WATER) f[3] = 0.0;
             return;
         }
         case OR_FLUX_CENTRAL:
@@ -624,7 +676,9 @@ template <class T> void two_point(const Cfg& c, int kind, const T* ull, const T*
 }
 
 // (1D ideal MHD has no nonconservative terms: B1 is constant)
-inline bool has_noncons(const Cfg& c) { return (c.eq == OR_EQ_GLM_MHD && c.nd > 1) || c.eq == OR_EQ_EULER_QUASI1D; }
+inline bool has_noncons(const Cfg& c) {
+    return (c.eq == OR_EQ_GLM_MHD && c.nd > 1) || c.eq == OR_EQ_EULER_QUASI1D || c.eq == OR_EQ_SHALLOW_WATER;
+}
 
 // ----------------------------------------------------------------------------
 // Basis (Trixi.jl basis_lobatto_legendre.jl; used by the reference at
@@ -917,7 +971,7 @@ void flux_differencing_element(const Cfg& c, const Basis& b, int kind, const T* 
                 const int s = ipow(N, d), idx = (n / s) % N;
                 for (int ii = 0; ii < N; ++ii) {
                     const int m = n + (ii - idx) * s;
-                    noncons_powell(c, ue + nv * n, ue + nv * m, d + 1, f);
+                    noncons_flux(c, ue + nv * n, ue + nv * m, d + 1, f, false);
                     for (int v = 0; v < nv; ++v)
                         integral[v] = integral[v] + b.Ds[idx * N + ii] * f[v];
                 }
@@ -949,8 +1003,8 @@ void fv_element(const Cfg& c, const Basis& b, const T* ue, T* due, T alpha) {
             two_point(c, c.fvflux, ue + nv * (n - s), ue + nv * n, d + 1, f);
             const int base = n - idx * s;
             if (has_noncons(c)) {
-                noncons_powell(c, ue + nv * (n - s), ue + nv * n, d + 1, ncl);
-                noncons_powell(c, ue + nv * n, ue + nv * (n - s), d + 1, ncr);
+                noncons_flux(c, ue + nv * (n - s), ue + nv * n, d + 1, ncl, true);
+                noncons_flux(c, ue + nv * n, ue + nv * (n - s), d + 1, ncr, true);
             }
             for (int v = 0; v < nv; ++v) {
                 fl[at(d, v, idx, base)] = has_noncons(c) ? f[v] + 0.5 * ncl[v] : f[v];
@@ -1092,6 +1146,25 @@ void source_convergence_test(const Cfg& c, const double* x, double t, double* sv
         sv[d + 1] = rhox * (0.5 * d * (d - 1) - ((d - 1) * c.inv_gm1 + d) * C_grav * rho);
         return;
     }
+    if (c.eq == OR_EQ_SHALLOW_WATER) {
+        // Trixi's ShallowWaterEquations2D source_terms_convergence_test for the
+        // state H = 7 + cos(w x) sin(w y) cos(w_t t), v = (0.5, 1.5),
+        // b = 2 + (sin(w_b x) + sin(w_b y)) / 2, w = 2 pi sqrt 2, w_t = 2 pi,
+        // w_b = sqrt 2 pi: du1 = H_t + v1 (H_x - b_x) + v2 (H_y - b_y),
+        // du2 = v1 du1 + g (H - b) H_x, du3 = v2 du1 + g (H - b) H_y
+        const double w = 2.0 * M_PI * sqrt(2.0), wt = 2.0 * M_PI, wb = sqrt(2.0) * M_PI, v1 = 0.5, v2 = 1.5;
+        const double H = 7.0 + cos(w * x[0]) * sin(w * x[1]) * cos(wt * t);
+        const double H_x = -w * sin(w * x[0]) * sin(w * x[1]) * cos(wt * t);
+        const double H_y = w * cos(w * x[0]) * cos(w * x[1]) * cos(wt * t);
+        const double H_t = -wt * cos(w * x[0]) * sin(w * x[1]) * sin(wt * t);
+        const double b = 2.0 + 0.5 * sin(wb * x[0]) + 0.5 * sin(wb * x[1]);
+        const double b_x = 0.5 * wb * cos(wb * x[0]), b_y = 0.5 * wb * cos(wb * x[1]);
+        sv[0] = H_t + v1 * (H_x - b_x) + v2 * (H_y - b_y);
+        sv[1] = v1 * sv[0] + c.grav * (H - b) * H_x;
+        sv[2] = v2 * sv[0] + c.grav * (H - b) * H_y;
+        sv[3] = 0.0;
+        return;
+    }
     if (c.eq == OR_EQ_EULER_QUASI1D) {
         // Trixi's quasi-1D source_terms_convergence_test: the residual of the
         // manufactured state rho = ini, v = 1, e = ini^2 / rho,
@@ -1203,8 +1276,8 @@ void rhs_impl(const Cfg& c, int64_t E, int64_t I, const int32_t* nbr, const int3
             }
             two_point(c, c.sflux, ull, urr, o, flux);
             if (has_noncons(c)) {
-                noncons_powell(c, ull, urr, o, ncl);
-                noncons_powell(c, urr, ull, o, ncr);
+                noncons_flux(c, ull, urr, o, ncl, true);
+                noncons_flux(c, urr, ull, o, ncr, true);
                 for (int v = 0; v < nv; ++v) {
                     SF(v, f, ldir, L) = flux[v] + 0.5 * ncl[v];
                     SF(v, f, rdir, R) = flux[v] + 0.5 * ncr[v];
@@ -1271,8 +1344,8 @@ void rhs_impl(const Cfg& c, int64_t E, int64_t I, const int32_t* nbr, const int3
                     T nc_small[MAXV], nc_large[MAXV];
                     for (int v = 0; v < nv; ++v) nc_small[v] = nc_large[v] = 0.0;
                     if (has_noncons(c)) {
-                        noncons_powell(c, uL, uS, o, nc_small);
-                        noncons_powell(c, uS, uL, o, nc_large);
+                        noncons_flux(c, uL, uS, o, nc_small, true);
+                        noncons_flux(c, uS, uL, o, nc_large, true);
                     }
                     for (int v = 0; v < nv; ++v) {
                         SF(v, q, dir_small, small) = flux[v] + 0.5 * nc_small[v];
@@ -1340,6 +1413,14 @@ void rhs_impl(const Cfg& c, int64_t E, int64_t I, const int32_t* nbr, const int3
                         two_point(c, c.sflux, uin, ub, od + 1, flux);
                     else
                         two_point(c, c.sflux, ub, uin, od + 1, flux);
+                    // nonconservative terms (reference boundary_flux_kernel!,
+                    // dg_2d_kernel.jl:1144-1159): half of Trixi's Dirichlet
+                    // nonconservative flux, inner state first on either side
+                    if (has_noncons(c) && !((c.outflow >> dir) & 1)) {
+                        T nc[MAXV];
+                        noncons_flux(c, uin, ub, od + 1, nc, true);
+                        for (int v = 0; v < nv; ++v) flux[v] = flux[v] + 0.5 * nc[v];
+                    }
                     for (int v = 0; v < nv; ++v) SF(v, f, dir, e) = flux[v];
                 }
             }
@@ -1577,6 +1658,38 @@ void initial_condition(const Cfg& c, int ic, const double* x, double t, uint64_t
         }
         return;
     }
+    if (c.eq == OR_EQ_SHALLOW_WATER) {
+        // Trixi's ShallowWaterEquations2D initial conditions via prim2cons
+        // (H, v1, v2, b) -> (H - b, (H - b) v1, (H - b) v2, b): the convergence
+        // test (periodic on [0, sqrt 2]^2) and the weak blast wave (centre
+        // (0.7, 0.7), H = 4 / 3.25, b = 0); seeded: b = 0.1 U, h = 1 + 0.1 U,
+        // v = 0.1 N(0, 1); constant: h = 1, v = 0, b = 0
+        double H = 1.0, v1 = 0.0, v2 = 0.0, b = 0.0;
+        if (ic == OR_IC_CONVERGENCE_TEST) {
+            const double cc = 7.0, omega_x = 2.0 * M_PI * sqrt(2.0), omega_t = 2.0 * M_PI;
+            H = cc + cos(omega_x * x[0]) * sin(omega_x * x[1]) * cos(omega_t * t);
+            v1 = 0.5;
+            v2 = 1.5;
+            b = 2.0 + 0.5 * sin(sqrt(2.0) * M_PI * x[0]) + 0.5 * sin(sqrt(2.0) * M_PI * x[1]);
+        } else if (ic == OR_IC_WEAK_BLAST_WAVE) {
+            const double xn = x[0] - 0.7, yn = x[1] - 0.7;
+            const double r = sqrt(xn * xn + yn * yn), phi = atan2(yn, xn);
+            H = r > 0.5 ? 3.25 : 4.0;
+            v1 = r > 0.5 ? 0.0 : 0.1882 * cos(phi);
+            v2 = r > 0.5 ? 0.0 : 0.1882 * sin(phi);
+        } else if (ic == OR_IC_SEEDED_RANDOM) {
+            b = 0.1 * u01(seed, idx, 6);
+            H = 1.0 + 0.1 * (2.0 * u01(seed, idx, 0) - 1.0) + b;
+            v1 = 0.1 * normal01(seed, idx, 2);
+            v2 = 0.1 * normal01(seed, idx, 4);
+        }
+        const double h = H - b;
+        u[0] = h;
+        u[1] = h * v1;
+        u[2] = h * v2;
+        u[3] = b;
+        return;
+    }
     if (c.eq == OR_EQ_EULER_QUASI1D) {
         // Trixi's quasi-1D initial_condition_convergence_test (c = 2, A = 0.1,
         // L = 2, omega = 2 pi / L); seeded: rho = 1 + 0.1 U, p = 1 + 0.05 U,
@@ -1754,7 +1867,7 @@ int oracle_nvars(const oracle_cfg* cfg) {
     if (cfg->equation == OR_EQ_HYPDIFF) return cfg->ndims + 1;
     if (cfg->equation == OR_EQ_EULER) return cfg->ndims + 2;
     if (cfg->equation == OR_EQ_EULER_MULTI) return cfg->ndims + 1 + cfg->ncomponents;
-    if (cfg->equation == OR_EQ_EULER_QUASI1D) return 4;
+    if (cfg->equation == OR_EQ_EULER_QUASI1D || cfg->equation == OR_EQ_SHALLOW_WATER) return 4;
     return cfg->ndims == 1 ? 8 : 9;  // ideal MHD: (rho, rho v, rho e, B) in 1D, + psi in 2D/3D
 }
 
@@ -1840,7 +1953,7 @@ void oracle_two_point_flux(const oracle_cfg* cfg, int kind, const double* ull, c
 }
 void oracle_noncons_flux(const oracle_cfg* cfg, const double* ull, const double* urr, int o,
                          double* f) {
-    noncons_powell(make_cfg(cfg), ull, urr, o, f);
+    noncons_flux(make_cfg(cfg), ull, urr, o, f, true);
 }
 
 void oracle_max_abs_speeds(const oracle_cfg* cfg, const double* u, double* lam) {
@@ -1864,6 +1977,8 @@ void oracle_max_abs_speeds(const oracle_cfg* cfg, const double* u, double* lam) 
             double rho, v, p, a;
             quasi_prim(c, u, rho, v, p, a);
             lam[d] = fabs(v) + sqrt(c.gamma * p / rho);
+        } else if (c.eq == OR_EQ_SHALLOW_WATER) {
+            lam[d] = fabs(u[1 + d] / u[0]) + sqrt(c.grav * u[0]);
         } else {
             lam[d] = fabs(u[1 + d] / u[0]) + mhd_fast_wavespeed(c, u, d + 1);
         }

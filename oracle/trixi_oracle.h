@@ -27,7 +27,8 @@ extern "C" {
 
 enum { OR_EQ_LINADV = 0, OR_EQ_EULER = 1, OR_EQ_GLM_MHD = 2, OR_EQ_BURGERS = 3, OR_EQ_HYPDIFF = 4,
        OR_EQ_EULER_MULTI = 5 /* (rho v, rho e, rho_1..rho_nc) */,
-       OR_EQ_EULER_QUASI1D = 6 /* (a rho, a rho v, a e, a) */ };
+       OR_EQ_EULER_QUASI1D = 6 /* (a rho, a rho v, a e, a) */,
+       OR_EQ_SHALLOW_WATER = 7 /* 2D: (h, h v1, h v2, b) */ };
 enum { OR_VI_WEAK = 0, OR_VI_FD = 1, OR_VI_HG = 2 };
 enum {
     OR_FLUX_CENTRAL = 0,
@@ -39,7 +40,9 @@ enum {
     OR_FLUX_EC = 6,           /* Burgers flux_ec                             */
     OR_FLUX_HLL = 7,          /* FluxHLL(min_max_speed_davis), Euler         */
     OR_FLUX_CHAN_ETAL = 8,    /* quasi-1D: flux_chan_etal + flux_nonconservative_chan_etal */
-    OR_FLUX_HLLE = 9          /* GLM-MHD: FluxHLL(min_max_speed_einfeldt) */
+    OR_FLUX_HLLE = 9,         /* GLM-MHD: FluxHLL(min_max_speed_einfeldt) */
+    OR_FLUX_WINTERMEYER_ETAL = 10, /* shallow water (volume noncons g h_l b_r) */
+    OR_FLUX_FJORDHOLM_ETAL = 11    /* shallow water (surface noncons: Fjordholm) */
 };
 enum {
     OR_IC_CONSTANT = 0,
@@ -89,6 +92,7 @@ typedef struct oracle_cfg {
     int32_t ncomponents;      /* multicomponent Euler */
     int32_t reserved2;
     double gammas[4], gas_constants[4];
+    double gravity, H0;       /* shallow water */
 } oracle_cfg;
 
 int oracle_nvars(const oracle_cfg* cfg);

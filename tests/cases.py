@@ -9,7 +9,7 @@ import numpy as np
 class Case:
     name: str
     ndims: int
-    equation: str           # "linadv" | "euler" | "mhd" | "burgers" | "hypdiff" | "eulermulti" | "quasi1d"
+    equation: str           # "linadv" | "euler" | "mhd" | "burgers" | "hypdiff" | "eulermulti" | "quasi1d" | "swe"
     level: int
     vi: str = "weak"        # "weak" | "fd" | "hg"
     vflux: str = "central"
@@ -38,10 +38,11 @@ class Case:
     gammas: tuple = (1.4, 1.4)          # multicomponent Euler
     gas_constants: tuple = (0.4, 0.4)
     ic_fn: object = None    # initial condition as a Python function (x, t, equations) -> conservative state
+    gravity: float = 9.81   # shallow water
 
 
 FLUX_NAMES = ("central", "llf", "llf_naive", "ranocha", "shima_etal", "hindenlang_gassner", "ec", "hll", "chan_etal",
-              "hlle")
+              "hlle", "wintermeyer_etal", "fjordholm_etal")
 IC_NAMES = ("constant", "convergence_test", "weak_blast_wave", "seeded_random", "smooth_periodic",
             "weak_blast_wave_seeded", "shock", "rarefaction", "eoc_gravity", "poisson_nonperiodic",
             "harmonic_nonperiodic")
@@ -58,7 +59,8 @@ def oracle_mesh(O, c: Case):
 
 def oracle_cfg(O, c: Case):
     eq = {"linadv": O.EQ_LINADV, "euler": O.EQ_EULER, "mhd": O.EQ_GLM_MHD, "burgers": O.EQ_BURGERS,
-          "hypdiff": O.EQ_HYPDIFF, "eulermulti": O.EQ_EULER_MULTI, "quasi1d": O.EQ_EULER_QUASI1D}[c.equation]
+          "hypdiff": O.EQ_HYPDIFF, "eulermulti": O.EQ_EULER_MULTI, "quasi1d": O.EQ_EULER_QUASI1D,
+          "swe": O.EQ_SHALLOW_WATER}[c.equation]
     vi = {"weak": O.VI_WEAK, "fd": O.VI_FD, "hg": O.VI_HG}[c.vi]
     ext = dict(boundary_condition=O.BC_DIRICHLET if c.bc_state else O.BC_PERIODIC,
                boundary_ic=IC_NAMES.index(c.bc_state) if c.bc_state else 0,
@@ -70,7 +72,7 @@ def oracle_cfg(O, c: Case):
                       alpha_smooth=c.alpha_smooth, ic_length=c.cmax - c.cmin,
                       indicator_variable=INDICATORS[c.indicator], nu=c.nu,
                       gammas=c.gammas if c.equation == "eulermulti" else (),
-                      gas_constants=c.gas_constants if c.equation == "eulermulti" else (), **ext)
+                      gas_constants=c.gas_constants if c.equation == "eulermulti" else (), gravity=c.gravity, **ext)
 
 
 def oracle_ic(O, c: Case):
@@ -81,11 +83,16 @@ def _flux(th, name, pair):
     f = {"central": th.flux_central, "llf": th.flux_lax_friedrichs,
          "llf_naive": th.FluxLaxFriedrichs(th.max_abs_speed_naive), "ranocha": th.flux_ranocha,
          "shima_etal": th.flux_shima_etal, "hindenlang_gassner": th.flux_hindenlang_gassner, "ec": th.flux_ec,
-         "hll": th.flux_hll, "chan_etal": th.flux_chan_etal, "hlle": th.flux_hlle}[name]
+         "hll": th.flux_hll, "chan_etal": th.flux_chan_etal, "hlle": th.flux_hlle,
+         "wintermeyer_etal": th.flux_wintermeyer_etal, "fjordholm_etal": th.flux_fjordholm_etal}[name]
     if pair == "mhd":
         return (f, th.flux_nonconservative_powell)
     if pair == "quasi1d":
         return (f, th.flux_nonconservative_chan_etal)
+    if pair == "swe_volume":
+        return (f, th.flux_nonconservative_wintermeyer_etal)
+    if pair == "swe_surface":
+        return (f, th.flux_nonconservative_fjordholm_etal)
     return f
 
 
@@ -105,6 +112,8 @@ def product_semi(th, c: Case, precision=None):
               th.CompressibleEulerEquations3D][c.ndims](c.gamma)
     elif c.equation == "quasi1d":
         eq = th.CompressibleEulerEquationsQuasi1D(c.gamma)
+    elif c.equation == "swe":
+        eq = th.ShallowWaterEquations2D(gravity_constant=c.gravity)
     elif c.equation == "eulermulti":
         eq = [None, th.CompressibleEulerMulticomponentEquations1D,
               th.CompressibleEulerMulticomponentEquations2D][c.ndims](gammas=c.gammas,
@@ -116,13 +125,14 @@ def product_semi(th, c: Case, precision=None):
     if c.vi == "weak":
         vi = th.VolumeIntegralWeakForm()
     elif c.vi == "fd":
-        vi = th.VolumeIntegralFluxDifferencing(_flux(th, c.vflux, mhd))
+        vi = th.VolumeIntegralFluxDifferencing(_flux(th, c.vflux, "swe_volume" if c.equation == "swe" else mhd))
     else:
         ind = th.IndicatorHennemannGassner(eq, basis, alpha_max=c.alpha_max, alpha_min=c.alpha_min,
                                            alpha_smooth=c.alpha_smooth, variable=getattr(th, c.indicator))
         vi = th.VolumeIntegralShockCapturingHG(ind, volume_flux_dg=_flux(th, c.vflux, mhd),
                                                volume_flux_fv=_flux(th, c.fvflux, mhd))
-    solver = th.DGSEMHIP(polydeg=c.polydeg, surface_flux=_flux(th, c.sflux, mhd), volume_integral=vi,
+    solver = th.DGSEMHIP(polydeg=c.polydeg, surface_flux=_flux(th, c.sflux, "swe_surface" if c.equation == "swe" else mhd),
+                         volume_integral=vi,
                          precision=precision or c.precision)
     patches = tuple(dict(type="box", coordinates_min=lo, coordinates_max=hi) for lo, hi in c.patches)
     mesh = th.TreeMesh((c.cmin,) * c.ndims, (c.cmax,) * c.ndims, initial_refinement_level=c.level,

@@ -84,6 +84,21 @@ QUASI_CASES = [
     Case("quasi1d_random_p4", 1, "quasi1d", 4, "fd", "chan_etal", "chan_etal", polydeg=4, cmin=-1.0, cmax=1.0),
     Case("quasi1d_random_p3", 1, "quasi1d", 5, "fd", "chan_etal", "chan_etal", cmin=-1.0, cmax=1.0),
 ]
+# shallow water 2D: test/tree_dgsem_2d/shallowwater_ec.jl (polydeg 4, level 2 on
+# [-1, 1]^2, weak blast wave, volume (wintermeyer, nc wintermeyer), surface
+# (fjordholm, nc fjordholm)), shallowwater_source_terms.jl and
+# shawllowwater_source_terms_nonperiodic.jl (polydeg 3, level 3 on [0, sqrt 2]^2,
+# LLF surface flux, convergence test with sources, periodic or Dirichlet)
+SWE_CASES = [
+    Case("swe2d_ec", 2, "swe", 2, "fd", "wintermeyer_etal", "fjordholm_etal", polydeg=4, cmin=-1.0, cmax=1.0,
+         ic="weak_blast_wave"),
+    Case("swe2d_src", 2, "swe", 3, "fd", "wintermeyer_etal", "llf", cmin=0.0, cmax=2 ** 0.5, ic="convergence_test",
+         source="convergence_test", t=0.3),
+    Case("swe2d_src_nonperiodic", 2, "swe", 3, "fd", "wintermeyer_etal", "llf", cmin=0.0, cmax=2 ** 0.5,
+         ic="convergence_test", source="convergence_test", t=0.3, periodic=False, bc_state="convergence_test"),
+    Case("swe2d_random_p4", 2, "swe", 3, "fd", "wintermeyer_etal", "fjordholm_etal", polydeg=4),
+    Case("swe2d_random_llf", 2, "swe", 3, "fd", "wintermeyer_etal", "llf"),
+]
 # test/tree_dgsem_1d/euler_blast_wave.jl: its initial_condition_blast_wave,
 # HG with flux_ranocha (DG) and LLF (FV, surface), polydeg 3, level 6
 EULER_BLAST_1D = Case("euler1d_blast_wave_hg", 1, "euler", 6, "hg", "ranocha", "llf", "llf", ic="constant",
@@ -273,6 +288,7 @@ CASES = [
     Case("mhd3d_mortar_hlle_random", 3, "mhd", 2, "fd", "hindenlang_gassner", "hlle", gamma=5 / 3,
          patches=MORTAR_3D),
     Case("mhd2d_hlle_periodic_random", 2, "mhd", 3, "fd", "hindenlang_gassner", "hlle", gamma=5 / 3),
+    *SWE_CASES,
 ]
 
 
@@ -285,9 +301,10 @@ def test_rhs_fp64_matches_oracle(th, oracle, torch, case):
     assert err <= FP64_TOL, err
 
 
-# (the MHD Alfven wave on a refined mesh: du is a cancellation of O(1) flux
-# terms, relative errors are amplified; the seeded mortar states cover mixed)
-_MIXED_SKIP = ("mhd2d_mortar_hlle", "mhd3d_mortar_hlle")
+# (the MHD Alfven wave on a refined mesh, and the shallow-water manufactured
+# state with its source: du is a cancellation of flux terms much larger than
+# du, relative errors are amplified; the seeded states cover mixed)
+_MIXED_SKIP = ("mhd2d_mortar_hlle", "mhd3d_mortar_hlle", "swe2d_src", "swe2d_src_nonperiodic")
 MIXED_CASES = [c for c in CASES if c.vi in ("fd", "hg") and c.polydeg == 3 and not c.patches
                and c.name not in _MIXED_SKIP] + MHD_HG_P4 + \
     [QUASI_CASES[0], QUASI_CASES[2]] + \
@@ -364,7 +381,8 @@ def test_pack_unpack_roundtrip_bit_exact(th, oracle, torch):
                                   Case("hypdiff2d_poisson", 2, "hypdiff", 3, polydeg=4, cmin=0.0, cmax=1.0,
                                        ic="poisson_nonperiodic", periodic=(False, True),
                                        bc_state="poisson_nonperiodic", source="poisson_nonperiodic"),
-                                  MULTI_CASES[1], MULTI_CASES[3], MULTI_CASES[4], QUASI_CASES[1]],
+                                  MULTI_CASES[1], MULTI_CASES[3], MULTI_CASES[4], QUASI_CASES[1], SWE_CASES[0],
+                                  SWE_CASES[2]],
                          ids=lambda c: c.name)
 def test_initial_condition_and_callbacks_match_oracle(th, oracle, torch, case):
     O = oracle
@@ -456,7 +474,7 @@ def test_indicator_matches_oracle(th, oracle, torch, case):
                                   Case("mhd2d_hg_blast", 2, "mhd", 3, "hg", "hindenlang_gassner", "hindenlang_gassner",
                                        "hindenlang_gassner", cmin=-2.0, cmax=2.0, ic="weak_blast_wave_seeded",
                                        gamma=1.4),
-                                  MHD_HG_P4[0], MULTI_CASES[3], MULTI_CASES[4], QUASI_CASES[1]],
+                                  MHD_HG_P4[0], MULTI_CASES[3], MULTI_CASES[4], QUASI_CASES[1], SWE_CASES[2]],
                          ids=lambda c: c.name)
 def test_rk_step_matches_oracle(th, oracle, torch, case, use_graph):
     """Three CK2N54 steps from t = 0 (stage times through device memory: the

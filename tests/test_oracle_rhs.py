@@ -909,3 +909,112 @@ def test_mhd_mortar_free_stream_and_side_order(oracle, nd):
                       - O.compute_coefficients(cfg, mesh, O.IC_CONVERGENCE_TEST, t - h)) / (2 * h)
                 errs.append(np.abs(O.rhs(cfg, mesh, u, t=t) - ut).max())
             assert np.log2(errs[0] / errs[1]) > 2.5, errs
+
+
+# ---------------------------------------------------------------------------
+# shallow water 2D: (h, h v1, h v2, b)
+# ---------------------------------------------------------------------------
+def _swe_cfg(O, vflux, sflux, polydeg=4, **kw):
+    return O.make_cfg(2, O.EQ_SHALLOW_WATER, polydeg, O.VI_FD, vflux, sflux, gravity=9.81, **kw)
+
+
+def test_shallow_water_fluxes_and_lake_at_rest(oracle):
+    """flux_wintermeyer_etal and flux_fjordholm_etal are consistent and
+    symmetric; the lake at rest (H = h + b constant, v = 0) is a discrete
+    steady state with the volume pair (wintermeyer, g h_l b_r) and the
+    surface pair (fjordholm, Fjordholm's term) even for a bottom that jumps
+    at every node, and with LLF at the surface for a continuous bottom (LLF
+    leaves b undissipated)."""
+    O = oracle
+    rng = np.random.default_rng(50)
+    cfg = _swe_cfg(O, O.FLUX_WINTERMEYER_ETAL, O.FLUX_FJORDHOLM_ETAL)
+    for _ in range(20):
+        ul = np.array([1.0 + rng.random(), 0.3 * rng.standard_normal(), 0.3 * rng.standard_normal(), rng.random()])
+        ur = np.array([1.0 + rng.random(), 0.3 * rng.standard_normal(), 0.3 * rng.standard_normal(), rng.random()])
+        for o in (1, 2):
+            for k in (O.FLUX_WINTERMEYER_ETAL, O.FLUX_FJORDHOLM_ETAL, O.FLUX_LLF):
+                np.testing.assert_allclose(O.two_point_flux(cfg, k, ul, ul, o), O.flux(cfg, ul, o), rtol=1e-14,
+                                           atol=1e-14)
+            for k in (O.FLUX_WINTERMEYER_ETAL, O.FLUX_FJORDHOLM_ETAL):
+                np.testing.assert_allclose(O.two_point_flux(cfg, k, ul, ur, o), O.two_point_flux(cfg, k, ur, ul, o),
+                                           rtol=1e-14, atol=1e-14)
+    mesh = O.Mesh(2, 2, 0.0, 2 ** 0.5, 4)
+    u = np.zeros((mesh.nelements, 5, 5, 4))
+    b = 0.5 * rng.random(u.shape[:-1])
+    u[..., 0], u[..., 3] = 3.0 - b, b
+    # the cancelling terms are of size g H^2 N^2 invJ (about 1e4): round-off 1e-16 of that
+    tol = 1e-16 * 9.81 * 3.0 ** 2 * 25 * mesh.inverse_jacobian.max() * 10
+    assert np.abs(O.rhs(cfg, mesh, u)).max() < tol
+    x = mesh.node_coordinates
+    b = 0.5 + 0.2 * np.sin(2 * np.pi * x[..., 0] / 2 ** 0.5) * np.cos(2 * np.pi * x[..., 1] / 2 ** 0.5)
+    u[..., 0], u[..., 3] = 3.0 - b, b
+    for sflux in (O.FLUX_FJORDHOLM_ETAL, O.FLUX_LLF):
+        assert np.abs(O.rhs(_swe_cfg(O, O.FLUX_WINTERMEYER_ETAL, sflux), mesh, u)).max() < tol
+
+
+def test_shallow_water_energy_conservation(oracle):
+    """The total energy E = h |v|^2 / 2 + g h^2 / 2 + g h b is conserved by the
+    semidiscretisation with the reference's EC pairs (entropy variables by
+    complex step); LLF at the surface dissipates it."""
+    O = oracle
+    g = 9.81
+    mesh = O.Mesh(2, 2, 0.0, 2 ** 0.5, 4)
+    cfg = _swe_cfg(O, O.FLUX_WINTERMEYER_ETAL, O.FLUX_FJORDHOLM_ETAL)
+    u = O.compute_coefficients(cfg, mesh, O.IC_SEEDED_RANDOM)
+
+    def energy(q):
+        h, hv1, hv2, b = q
+        return 0.5 * (hv1 * hv1 + hv2 * hv2) / h + 0.5 * g * h * h + g * h * b
+
+    w = np.zeros_like(u)
+    for j in range(4):
+        uc = u.astype(complex)
+        uc[..., j] += 1j * 1e-30
+        w[..., j] = energy(np.moveaxis(uc, -1, 0)).imag / 1e-30
+    for sflux, ec in ((O.FLUX_FJORDHOLM_ETAL, True), (O.FLUX_LLF, False)):
+        du = O.rhs(_swe_cfg(O, O.FLUX_WINTERMEYER_ETAL, sflux), mesh, u)
+        assert np.abs(du[..., 3]).max() == 0.0
+        rate = O.integrate(cfg, mesh, np.repeat((w * du).sum(-1, keepdims=True), 4, axis=-1))[0]
+        scale = np.abs(du).max() * mesh.total_volume
+        if ec:
+            assert abs(rate) < 1e-12 * scale, rate
+        else:
+            assert rate < -1e-6 * scale, rate
+
+
+def test_shallow_water_source_and_convergence(oracle):
+    """source_terms_convergence_test equals the sympy residual of the
+    manufactured state (h = H - b, v constant, with g h grad b), and with it
+    rhs!(u_exact) - u_t converges at the DG rate, periodic and with the
+    Dirichlet state on every side."""
+    import sympy as sp
+    O = oracle
+    g = 9.81
+    X, Y, T = sp.symbols("x y t")
+    w, wt, wb = 2 * sp.pi * sp.sqrt(2), 2 * sp.pi, sp.sqrt(2) * sp.pi
+    H = 7 + sp.cos(w * X) * sp.sin(w * Y) * sp.cos(wt * T)
+    b = 2 + sp.sin(wb * X) / 2 + sp.sin(wb * Y) / 2
+    h, v1, v2 = H - b, sp.Rational(1, 2), sp.Rational(3, 2)
+    S = [sp.diff(h, T) + sp.diff(h * v1, X) + sp.diff(h * v2, Y),
+         sp.diff(h * v1, T) + sp.diff(h * v1 * v1 + g * h * h / 2, X) + sp.diff(h * v1 * v2, Y) + g * h * sp.diff(b, X),
+         sp.diff(h * v2, T) + sp.diff(h * v1 * v2, X) + sp.diff(h * v2 * v2 + g * h * h / 2, Y) + g * h * sp.diff(b, Y)]
+    Sf = sp.lambdify((X, Y, T), S, "numpy")
+    cfg = _swe_cfg(O, O.FLUX_WINTERMEYER_ETAL, O.FLUX_LLF, 3, source_terms=O.SOURCE_CONVERGENCE_TEST)
+    rng = np.random.default_rng(51)
+    for _ in range(8):
+        x, t = rng.uniform(0, 2 ** 0.5, 2), rng.uniform(0, 1)
+        s = O.source_convergence_test(cfg, x, t)
+        np.testing.assert_allclose(s[:3], np.array(Sf(x[0], x[1], t), dtype=float), rtol=1e-12, atol=1e-11)
+        assert s[3] == 0.0
+    for periodic in (True, False):
+        kw = {} if periodic else dict(boundary_condition=O.BC_DIRICHLET, boundary_ic=O.IC_CONVERGENCE_TEST)
+        c = _swe_cfg(O, O.FLUX_WINTERMEYER_ETAL, O.FLUX_LLF, 3, source_terms=O.SOURCE_CONVERGENCE_TEST, **kw)
+        errs, t = [], 0.1
+        for lvl in (2, 3):
+            mesh = O.Mesh(2, lvl, 0.0, 2 ** 0.5, 3, periodic=periodic)
+            u = O.compute_coefficients(c, mesh, O.IC_CONVERGENCE_TEST, t)
+            hh = 1e-6
+            ut = (O.compute_coefficients(c, mesh, O.IC_CONVERGENCE_TEST, t + hh)
+                  - O.compute_coefficients(c, mesh, O.IC_CONVERGENCE_TEST, t - hh)) / (2 * hh)
+            errs.append(np.abs(O.rhs(c, mesh, u, t=t) - ut).max())
+        assert np.log2(errs[0] / errs[1]) > 2.5, (periodic, errs)

@@ -11,10 +11,10 @@ LIBRARY_PATH = os.environ.get("TRIXI_HIP_LIBRARY") or os.path.join(HERE, "libtri
 
 OK, EINVAL, EUNSUPPORTED, EDEVICE, ENOMEM, ESTATE = 0, -1, -2, -3, -4, -5
 
-EQ_LINADV, EQ_EULER, EQ_GLM_MHD, EQ_BURGERS, EQ_HYPDIFF, EQ_EULER_MULTI, EQ_EULER_QUASI1D = range(7)
+EQ_LINADV, EQ_EULER, EQ_GLM_MHD, EQ_BURGERS, EQ_HYPDIFF, EQ_EULER_MULTI, EQ_EULER_QUASI1D, EQ_SHALLOW_WATER = range(8)
 VI_WEAK, VI_FD, VI_HG = 0, 1, 2
 (FLUX_CENTRAL, FLUX_LLF, FLUX_LLF_NAIVE, FLUX_RANOCHA, FLUX_SHIMA_ETAL, FLUX_HINDENLANG_GASSNER, FLUX_EC, FLUX_HLL,
- FLUX_CHAN_ETAL, FLUX_HLLE) = range(10)
+ FLUX_CHAN_ETAL, FLUX_HLLE, FLUX_WINTERMEYER_ETAL, FLUX_FJORDHOLM_ETAL) = range(12)
 PREC_FP64, PREC_MIXED = 0, 1
 BC_PERIODIC, BC_DIRICHLET = 0, 1
 SOURCE_NONE, SOURCE_CONVERGENCE_TEST, SOURCE_EOC_TEST_EULER, SOURCE_POISSON_NONPERIODIC, SOURCE_HARMONIC = range(5)
@@ -58,6 +58,8 @@ class Config(ctypes.Structure):
         ("reserved", ctypes.c_int32),
         ("gammas", ctypes.c_double * 4),
         ("gas_constants", ctypes.c_double * 4),
+        ("gravity", ctypes.c_double),
+        ("H0", ctypes.c_double),
     ]
 
 

@@ -21,7 +21,7 @@ __host__ __device__ inline double u01(uint64_t seed, uint64_t index, uint64_t st
     const uint64_t r = splitmix64(splitmix64(seed) ^ (index * 16 + stream));
     return ((double)(r >> 11) + 0.5) * (1.0 / 9007199254740992.0);
 }
-__device__ inline double normal01(uint64_t seed, uint64_t index, uint64_t stream) {
+__host__ __device__ inline double normal01(uint64_t seed, uint64_t index, uint64_t stream) {
     const double a = u01(seed, index, stream), b = u01(seed, index, stream + 1);
     return sqrt(-2.0 * log(a)) * cos(2.0 * M_PI * b);
 }
@@ -32,7 +32,64 @@ struct IcParams {
     uint64_t seed;
     int nc;                          // multicomponent Euler: components
     double cv[MAXC], cvg[MAXC];      // c_v,k and c_v,k gamma_k
+    double grav;                     // shallow water: gravitational acceleration
 };
+
+// Shallow water 2D states (h, h v1, h v2, b) from (H = h + b, v1, v2, b):
+//   convergence test (Trixi initial_condition_convergence_test, periodic on
+//   [0, sqrt 2]^2): H = 7 + cos(w x) sin(w y) cos(2 pi t), w = 2 pi sqrt 2,
+//   v = (0.5, 1.5), b = 2 + 0.5 sin(sqrt 2 pi x) + 0.5 sin(sqrt 2 pi y);
+//   weak blast wave: centre (0.7, 0.7), H = 4 inside r <= 0.5 with the outward
+//   velocity 0.1882, else H = 3.25 at rest, b = 0;
+//   seeded: h = 1 + 0.1 U, v = 0.1 N(0, 1), b = 0.1 U; constant: h = 1, v = 0, b = 0.
+__host__ __device__ inline void shallow_water_state(const IcParams& q, const double* x, double t, uint64_t idx,
+                                                    double* u) {
+    double H = 1.0, v1 = 0.0, v2 = 0.0, b = 0.0;
+    if (q.ic == TRIXI_HIP_IC_CONVERGENCE_TEST) {
+        const double wx = 2.0 * M_PI * sqrt(2.0), wt = 2.0 * M_PI;
+        H = 7.0 + cos(wx * x[0]) * sin(wx * x[1]) * cos(wt * t);
+        v1 = 0.5;
+        v2 = 1.5;
+        b = 2.0 + 0.5 * sin(sqrt(2.0) * M_PI * x[0]) + 0.5 * sin(sqrt(2.0) * M_PI * x[1]);
+    } else if (q.ic == TRIXI_HIP_IC_WEAK_BLAST_WAVE) {
+        const double xn = x[0] - 0.7, yn = x[1] - 0.7;
+        const double r = sqrt(xn * xn + yn * yn), phi = atan2(yn, xn);
+        const bool inside = !(r > 0.5);
+        H = inside ? 4.0 : 3.25;
+        v1 = inside ? 0.1882 * cos(phi) : 0.0;
+        v2 = inside ? 0.1882 * sin(phi) : 0.0;
+    } else if (q.ic == TRIXI_HIP_IC_SEEDED_RANDOM) {
+        b = 0.1 * u01(q.seed, idx, 6);
+        H = 1.0 + 0.1 * (2.0 * u01(q.seed, idx, 0) - 1.0) + b;
+        v1 = 0.1 * normal01(q.seed, idx, 2);
+        v2 = 0.1 * normal01(q.seed, idx, 4);
+    }
+    const double h = H - b;
+    u[0] = h;
+    u[1] = h * v1;
+    u[2] = h * v2;
+    u[3] = b;
+}
+
+// source_terms_convergence_test of ShallowWaterEquations2D: the residual of
+// the convergence-test state, with H_t = -2 pi cos(w x) sin(w y) sin(2 pi t),
+// H_x, H_y its gradient and b_x, b_y the bottom slope (v is constant):
+//   S_h = H_t + v1 (H_x - b_x) + v2 (H_y - b_y),  S_{h v_i} = v_i S_h + g (H - b) H_{x_i}.
+__host__ __device__ inline void source_shallow_water_convergence_test(double g, const double* x, double t,
+                                                                      double* s) {
+    const double wx = 2.0 * M_PI * sqrt(2.0), wt = 2.0 * M_PI, wb = sqrt(2.0) * M_PI;
+    const double v1 = 0.5, v2 = 1.5;
+    const double sX = sin(wx * x[0]), cX = cos(wx * x[0]), sY = sin(wx * x[1]), cY = cos(wx * x[1]);
+    const double sT = sin(wt * t), cT = cos(wt * t);
+    const double H = 7.0 + cX * sY * cT, H_x = -wx * sX * sY * cT, H_y = wx * cX * cY * cT;
+    const double H_t = -wt * cX * sY * sT;
+    const double b = 2.0 + 0.5 * sin(wb * x[0]) + 0.5 * sin(wb * x[1]);
+    const double b_x = 0.5 * wb * cos(wb * x[0]), b_y = 0.5 * wb * cos(wb * x[1]);
+    s[0] = H_t + v1 * (H_x - b_x) + v2 * (H_y - b_y);
+    s[1] = v1 * s[0] + g * (H - b) * H_x;
+    s[2] = v2 * s[0] + g * (H - b) * H_y;
+    s[3] = 0.0;
+}
 
 __device__ inline double smooth_field(uint64_t seed, int field, int nd, const double* x, double L,
                                bool use_cos) {
@@ -148,6 +205,10 @@ __device__ inline void initial_condition(const IcParams& q, const double* x, dou
         } else {
             u[0] = 2.0;
         }
+        return;
+    }
+    if (q.eq == EQ_SHALLOW_WATER) {
+        shallow_water_state(q, x, t, idx, u);
         return;
     }
     if (q.eq == EQ_EULER_QUASI1D) {
@@ -333,6 +394,10 @@ __device__ inline void initial_condition(const IcParams& q, const double* x, dou
 // and the constant state; for the Burgers equation its convergence-test wave
 // and the shock / rarefaction steps.  The library rejects other boundary states.
 __host__ __device__ inline void boundary_value(const IcParams& q, const double* x, double t, double* u) {
+    if (q.eq == EQ_SHALLOW_WATER) {
+        shallow_water_state(q, x, t, 0, u);
+        return;
+    }
     if (q.eq == EQ_HYPDIFF) {
         hypdiff_state(q, x, t, q.ic == TRIXI_HIP_IC_POISSON_NONPERIODIC, u);
         return;

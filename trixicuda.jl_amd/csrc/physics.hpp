@@ -29,7 +29,8 @@ enum Equation : int {
     EQ_BURGERS = 3,
     EQ_HYPDIFF = 4,
     EQ_EULER_MULTI = 5,
-    EQ_EULER_QUASI1D = 6
+    EQ_EULER_QUASI1D = 6,
+    EQ_SHALLOW_WATER = 7
 };
 constexpr int MAXC = 4;  // gas components of multicomponent Euler (parameter slots)
 enum VolumeIntegral : int { VI_WEAK = 0, VI_FD = 1, VI_HG = 2 };
@@ -43,7 +44,9 @@ enum Flux : int {
     FLUX_EC = 6,          // flux_ec of the inviscid Burgers equation
     FLUX_HLL = 7,         // FluxHLL(min_max_speed_davis) (compressible Euler)
     FLUX_CHAN_ETAL = 8,   // flux_chan_etal + flux_nonconservative_chan_etal (quasi-1D Euler)
-    FLUX_HLLE = 9         // flux_hlle = FluxHLL(min_max_speed_einfeldt) (GLM-MHD)
+    FLUX_HLLE = 9,        // flux_hlle = FluxHLL(min_max_speed_einfeldt) (GLM-MHD)
+    FLUX_WINTERMEYER_ETAL = 10,  // shallow water (volume; nonconservative term g h_l b_r)
+    FLUX_FJORDHOLM_ETAL = 11     // shallow water (surface; nonconservative term g h_l b_l + g h_avg [[b]])
 };
 
 // two-point fluxes that take ln-means (they need log(rho), log(p) per node)
@@ -57,11 +60,12 @@ struct PhysParams {
     double nu, inv_Tr;  // hyperbolic diffusion: diffusion constant, 1 / reference time
     // multicomponent Euler: per component c_v = R / (gamma - 1) and c_v gamma
     double cv[MAXC], cvg[MAXC];
+    double grav;  // shallow water: gravitational acceleration
 };
 
 template <typename R>
 struct PhysR {
-    R gamma, inv_gm1, gm1, c_h, a[3], nu, inv_Tr, cv[MAXC], cvg[MAXC];
+    R gamma, inv_gm1, gm1, c_h, a[3], nu, inv_Tr, cv[MAXC], cvg[MAXC], grav;
     __host__ __device__ static PhysR from(const PhysParams& p) {
         PhysR r;
         r.gamma = (R)p.gamma;
@@ -70,6 +74,7 @@ struct PhysR {
         r.c_h = (R)p.c_h;
         r.nu = (R)p.nu;
         r.inv_Tr = (R)p.inv_Tr;
+        r.grav = (R)p.grav;
         for (int d = 0; d < 3; ++d) r.a[d] = (R)p.a[d];
         for (int k = 0; k < MAXC; ++k) {
             r.cv[k] = (R)p.cv[k];
@@ -782,8 +787,8 @@ struct EulerQuasi1D {
             }
         }
     }
-    template <typename R>
-    __device__ static void noncons(const Node<R>& l, const Node<R>& r, int, R* f) {
+    template <bool SURF, typename R>
+    __device__ static void noncons(const Node<R>& l, const Node<R>& r, int, const PhysR<R>&, R* f) {
         f[0] = R(0);
         f[1] = l.a * r.p;
         f[2] = R(0);
@@ -793,6 +798,120 @@ struct EulerQuasi1D {
     __device__ static void max_abs_speeds(const R* u, const PhysR<R>& P, R* lam) {
         const Node<R> n = node<R, false>(u, P);
         lam[0] = dabs(n.v[0]) + n.c;
+    }
+};
+
+// ===========================================================================
+// Shallow water in 2D (Trixi ShallowWaterEquations2D): (h, h v1, h v2, b)
+// with the bottom topography b; flux (h v_o, h v v_o + g h^2/2 e_o, 0) and the
+// nonconservative bottom term g h grad b.  The volume pairs
+// flux_wintermeyer_etal with g h_l b_r, the surface any flux with Fjordholm's
+// g h_l b_l + g (h_l + h_r)/2 (b_r - b_l) (the reference tests' pairs);
+// LLF does not dissipate b.
+// ===========================================================================
+struct ShallowWater2D {
+    static constexpr int NV = 4;
+    static constexpr bool NONCONS = true;
+    static constexpr int NRF = 4;  // node record: h, v1, v2, b
+    template <typename R>
+    struct Node {
+        R h, v[2], b;
+        double lrho, lp;  // unused
+        __device__ R cons(int q) const {
+            if (q == 0) return h;
+            if (q <= 2) return h * v[q - 1];
+            return b;
+        }
+    };
+    template <typename R, bool LOGS>
+    __device__ static Node<R> node(const R* u, const PhysR<R>&) {
+        Node<R> n;
+        n.h = u[0];
+        const R inv_h = rdiv(R(1), n.h);
+        n.v[0] = u[1] * inv_h;
+        n.v[1] = u[2] * inv_h;
+        n.b = u[3];
+        n.lrho = n.lp = 0.0;
+        return n;
+    }
+    template <typename R>
+    __device__ static void to_rec(const Node<R>& n, R* f) {
+        f[0] = n.h;
+        f[1] = n.v[0];
+        f[2] = n.v[1];
+        f[3] = n.b;
+    }
+    template <typename R>
+    __device__ static Node<R> from_rec(const R* f, double, double, const PhysR<R>&) {
+        Node<R> n;
+        n.h = f[0];
+        n.v[0] = f[1];
+        n.v[1] = f[2];
+        n.b = f[3];
+        n.lrho = n.lp = 0.0;
+        return n;
+    }
+    template <typename R>
+    __device__ static void flux(const Node<R>& n, int o, const PhysR<R>& P, R* f) {
+        const R hvo = n.h * n.v[o];
+        const R p = R(0.5) * P.grav * n.h * n.h;
+        f[0] = hvo;
+        f[1] = hvo * n.v[0] + (o == 0 ? p : R(0));
+        f[2] = hvo * n.v[1] + (o == 1 ? p : R(0));
+        f[3] = R(0);
+    }
+    template <int KIND, typename R>
+    __device__ static void two_point(const Node<R>& l, const Node<R>& r, int o, const PhysR<R>& P, R* f) {
+        if (KIND == FLUX_WINTERMEYER_ETAL || KIND == FLUX_FJORDHOLM_ETAL) {
+            R f0, p_avg;
+            if (KIND == FLUX_WINTERMEYER_ETAL) {
+                f0 = R(0.5) * (l.h * l.v[o] + r.h * r.v[o]);
+                p_avg = R(0.5) * P.grav * l.h * r.h;
+            } else {
+                f0 = R(0.5) * (l.h + r.h) * (R(0.5) * (l.v[o] + r.v[o]));
+                p_avg = R(0.25) * P.grav * (l.h * l.h + r.h * r.h);
+            }
+            f[0] = f0;
+            f[1] = f0 * (R(0.5) * (l.v[0] + r.v[0])) + (o == 0 ? p_avg : R(0));
+            f[2] = f0 * (R(0.5) * (l.v[1] + r.v[1])) + (o == 1 ? p_avg : R(0));
+            f[3] = R(0);
+        } else {
+            R fl[NV], fr[NV];
+            flux(l, o, P, fl);
+            flux(r, o, P, fr);
+            if (KIND == FLUX_LLF || KIND == FLUX_LLF_NAIVE) {
+                const R cl = rsqrt_(P.grav * l.h), cr = rsqrt_(P.grav * r.h);
+                R lam;
+                if (KIND == FLUX_LLF_NAIVE)
+                    lam = dmax(dabs(l.v[o]), dabs(r.v[o])) + dmax(cl, cr);
+                else
+                    lam = dmax(dabs(l.v[o]) + cl, dabs(r.v[o]) + cr);
+                const R hl = R(-0.5) * lam;
+#pragma unroll
+                for (int q = 0; q < 3; ++q) f[q] = R(0.5) * (fl[q] + fr[q]) + hl * (r.cons(q) - l.cons(q));
+                f[3] = R(0);
+            } else {
+#pragma unroll
+                for (int q = 0; q < NV; ++q) f[q] = R(0.5) * (fl[q] + fr[q]);
+            }
+        }
+    }
+    // volume: flux_nonconservative_wintermeyer_etal g h_l b_r; surface:
+    // flux_nonconservative_fjordholm_etal g h_l b_l + g h_avg (b_r - b_l)
+    template <bool SURF, typename R>
+    __device__ static void noncons(const Node<R>& l, const Node<R>& r, int o, const PhysR<R>& P, R* f) {
+        const R t = SURF ? P.grav * l.h * l.b + P.grav * (R(0.5) * (l.h + r.h)) * (r.b - l.b) : P.grav * l.h * r.b;
+        f[0] = R(0);
+        f[1] = (o == 0) ? t : R(0);
+        f[2] = (o == 1) ? t : R(0);
+        f[3] = R(0);
+    }
+    template <typename R>
+    __device__ static void max_abs_speeds(const R* u, const PhysR<R>& P, R* lam) {
+        const Node<R> n = node<R, false>(u, P);
+        const R c = rsqrt_(P.grav * n.h);
+        lam[0] = dabs(n.v[0]) + c;
+        lam[1] = dabs(n.v[1]) + c;
     }
 };
 
@@ -1019,8 +1138,8 @@ struct GlmMhd {
 
     // flux_nonconservative_powell(u_ll, u_rr, o): Powell (0, B, v.B, v, 0) and
     // Galilean GLM terms of the left state times the right state's B_o / psi.
-    template <typename R>
-    __device__ static void noncons(const Node<R>& l, const Node<R>& r, int o, R* f) {
+    template <bool SURF, typename R>
+    __device__ static void noncons(const Node<R>& l, const Node<R>& r, int o, const PhysR<R>&, R* f) {
         const R v_dot_B_ll = l.v[0] * l.B[0] + l.v[1] * l.B[1] + l.v[2] * l.B[2];
         const R Bo_rr = r.B[o];
         const R psi_rr = r.psi;
@@ -1098,8 +1217,8 @@ struct GlmMhd1D {
 #pragma unroll
         for (int q = 0; q < 8; ++q) f[q] = f9[q];
     }
-    template <typename R>
-    __device__ static void noncons(const Node<R>&, const Node<R>&, int, R* f) {
+    template <bool SURF, typename R>
+    __device__ static void noncons(const Node<R>&, const Node<R>&, int, const PhysR<R>&, R* f) {
 #pragma unroll
         for (int q = 0; q < 8; ++q) f[q] = R(0);
     }

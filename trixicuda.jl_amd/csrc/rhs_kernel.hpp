@@ -130,7 +130,9 @@ __device__ __forceinline__ void source_at(const RhsArgs& a, int64_t e, int node,
         return;
     }
     double s5[5];
-    if (a.bc_ic.eq == EQ_EULER_QUASI1D)
+    if (a.bc_ic.eq == EQ_SHALLOW_WATER)
+        source_shallow_water_convergence_test(a.phys.grav, xx, t, s5);
+    else if (a.bc_ic.eq == EQ_EULER_QUASI1D)
         source_quasi1d_convergence_test(a.phys.gamma, xx, t, s5);
     else if (a.src == TRIXI_HIP_SOURCE_EOC_TEST_EULER)
         source_eoc_test_euler(DIM, a.phys.gm1, xx, t, s5);
@@ -489,7 +491,7 @@ rhs_kernel(const RhsArgs a) {
 #pragma unroll
                     for (int j = 0; j < N; ++j) {
                         R f[NV];
-                        EQ::noncons(nd[i], nd[j], d, f);
+                        EQ::template noncons<false>(nd[i], nd[j], d, P, f);
                         const R w = (R)a.Ds[i * N + j];
 #pragma unroll
                         for (int v = 0; v < NV; ++v) integral[v] += w * f[v];
@@ -510,8 +512,8 @@ rhs_kernel(const RhsArgs a) {
                         // subcell interface with nonconservative terms: the left
                         // node takes f + nc(u_l, u_r)/2, the right f + nc(u_r, u_l)/2
                         R ncl[NV], ncr[NV];
-                        EQ::noncons(nd[p], nd[p + 1], d, ncl);
-                        EQ::noncons(nd[p + 1], nd[p], d, ncr);
+                        EQ::template noncons<true>(nd[p], nd[p + 1], d, P, ncl);
+                        EQ::template noncons<true>(nd[p + 1], nd[p], d, P, ncr);
 #pragma unroll
                         for (int v = 0; v < NV; ++v) {
                             acl[p][v] += wl * (F[v] + R(0.5) * ncl[v]);
@@ -537,8 +539,8 @@ rhs_kernel(const RhsArgs a) {
             EQ::template two_point<SF>(nd[N - 1], ndp, d, P, fp);   // this element is "left"
             if constexpr (EQ::NONCONS) {
                 R ncm[NV], ncp[NV];
-                EQ::noncons(nd[0], ndm, d, ncm);
-                EQ::noncons(nd[N - 1], ndp, d, ncp);
+                EQ::template noncons<true>(nd[0], ndm, d, P, ncm);
+                EQ::template noncons<true>(nd[N - 1], ndp, d, P, ncp);
 #pragma unroll
                 for (int v = 0; v < NV; ++v) {
                     fm[v] = fm[v] + R(0.5) * ncm[v];

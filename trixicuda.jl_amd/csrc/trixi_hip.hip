@@ -64,6 +64,7 @@ void register_hypdiff();
 void register_euler_multi();
 void register_euler_quasi1d();
 void register_mhd_mortar();
+void register_shallow_water();
 void register_vjp_linadv();
 void register_vjp_euler1d();
 void register_vjp_euler2d();
@@ -91,6 +92,7 @@ static void ensure_registered() {
         register_euler_multi();
         register_euler_quasi1d();
         register_mhd_mortar();
+        register_shallow_water();
         register_vjp_linadv();
         register_vjp_euler1d();
         register_vjp_euler2d();
@@ -775,6 +777,10 @@ __global__ void check_state_kernel(const double* __restrict__ u, int32_t* __rest
                 if (!isfinite(uc[v])) f |= 1;
             }
             if (eq == EQ_LINADV || eq == EQ_BURGERS || eq == EQ_HYPDIFF) continue;
+            if (eq == EQ_SHALLOW_WATER) {
+                if (!(uc[0] > 0.0)) f |= 2;  // water height
+                continue;
+            }
             if (eq == EQ_EULER_QUASI1D) {
                 // rho = (a rho) / a, p = (gamma - 1)((a e) / a - (a rho v)^2 / (2 a (a rho)))
                 const double a = uc[3];
@@ -888,7 +894,7 @@ static const double CK_C[5] = {0.0, 1432997174477.0 / 9575080441755.0, 252626934
 static int nvars_of(const trixi_hip_config& c) {
     if (c.equation == TRIXI_HIP_LINEAR_SCALAR_ADVECTION || c.equation == TRIXI_HIP_INVISCID_BURGERS) return 1;
     if (c.equation == TRIXI_HIP_COMPRESSIBLE_EULER_MULTICOMPONENT) return c.ndims + 1 + c.ncomponents;
-    if (c.equation == TRIXI_HIP_COMPRESSIBLE_EULER_QUASI1D) return 4;
+    if (c.equation == TRIXI_HIP_COMPRESSIBLE_EULER_QUASI1D || c.equation == TRIXI_HIP_SHALLOW_WATER) return 4;
     if (c.equation == TRIXI_HIP_COMPRESSIBLE_EULER) return c.ndims + 2;
     if (c.equation == TRIXI_HIP_HYPERBOLIC_DIFFUSION) return c.ndims + 1;
     return c.ndims == 1 ? 8 : 9;  // ideal MHD: no cleaning field in 1D
@@ -1050,8 +1056,16 @@ int trixi_hip_create(const trixi_hip_config* config, const trixi_hip_mesh* mesh,
     const trixi_hip_config& cfg = *config;
     if (cfg.ndims < 1 || cfg.ndims > 3) return fail(TRIXI_HIP_EINVAL, "ndims must be 1, 2 or 3");
     if (cfg.polydeg < 1 || cfg.polydeg + 1 > MAXN) return fail(TRIXI_HIP_EINVAL, "polydeg out of range");
-    if (cfg.equation < TRIXI_HIP_LINEAR_SCALAR_ADVECTION || cfg.equation > TRIXI_HIP_COMPRESSIBLE_EULER_QUASI1D)
+    if (cfg.equation < TRIXI_HIP_LINEAR_SCALAR_ADVECTION || cfg.equation > TRIXI_HIP_SHALLOW_WATER)
         return fail(TRIXI_HIP_EINVAL, "unknown equation");
+    if (cfg.equation == TRIXI_HIP_SHALLOW_WATER) {
+        if (cfg.ndims != 2) return fail(TRIXI_HIP_EUNSUPPORTED, "shallow water is built for 2D");
+        if (!(cfg.gravity > 0.0)) return fail(TRIXI_HIP_EINVAL, "shallow water needs gravity > 0");
+        if (cfg.boundary_condition != TRIXI_HIP_BC_PERIODIC &&
+            cfg.boundary_ic != TRIXI_HIP_IC_CONVERGENCE_TEST && cfg.boundary_ic != TRIXI_HIP_IC_CONSTANT)
+            return fail(TRIXI_HIP_EUNSUPPORTED, "shallow water Dirichlet states: the convergence test or constant");
+        if (cfg.boundary_outflow_faces != 0) return fail(TRIXI_HIP_EUNSUPPORTED, "no outflow sides for shallow water");
+    }
     if (cfg.equation == TRIXI_HIP_COMPRESSIBLE_EULER_QUASI1D) {
         if (cfg.ndims != 1) return fail(TRIXI_HIP_EINVAL, "quasi-1D Euler is one-dimensional");
         if (cfg.boundary_condition != TRIXI_HIP_BC_PERIODIC ||
@@ -1086,6 +1100,7 @@ int trixi_hip_create(const trixi_hip_config* config, const trixi_hip_mesh* mesh,
     if (cfg.boundary_condition != TRIXI_HIP_BC_PERIODIC) {
         if (cfg.boundary_condition != TRIXI_HIP_BC_DIRICHLET)
             return fail(TRIXI_HIP_EINVAL, "unknown boundary condition");
+        const bool swe_ok = cfg.equation == TRIXI_HIP_SHALLOW_WATER;
         const bool euler_ok = cfg.equation == TRIXI_HIP_COMPRESSIBLE_EULER &&
                               (cfg.boundary_ic == TRIXI_HIP_IC_CONVERGENCE_TEST || cfg.boundary_ic == TRIXI_HIP_IC_CONSTANT);
         const bool hypdiff_ok = cfg.equation == TRIXI_HIP_HYPERBOLIC_DIFFUSION &&
@@ -1095,7 +1110,7 @@ int trixi_hip_create(const trixi_hip_config* config, const trixi_hip_mesh* mesh,
         const bool burgers_ok = cfg.equation == TRIXI_HIP_INVISCID_BURGERS &&
                                 (cfg.boundary_ic == TRIXI_HIP_IC_CONVERGENCE_TEST || cfg.boundary_ic == TRIXI_HIP_IC_CONSTANT ||
                                  cfg.boundary_ic == TRIXI_HIP_IC_SHOCK || cfg.boundary_ic == TRIXI_HIP_IC_RAREFACTION);
-        if (!euler_ok && !burgers_ok && !hypdiff_ok)
+        if (!euler_ok && !burgers_ok && !hypdiff_ok && !swe_ok)
             return fail(TRIXI_HIP_EUNSUPPORTED,
                         "Dirichlet boundaries are built for compressible Euler (convergence-test or constant state) "
                         "Burgers (convergence-test wave, shock, rarefaction, constant) and hyperbolic "
@@ -1120,9 +1135,9 @@ int trixi_hip_create(const trixi_hip_config* config, const trixi_hip_mesh* mesh,
     } else if (cfg.source_terms != TRIXI_HIP_SOURCE_NONE) {
         if (cfg.source_terms != TRIXI_HIP_SOURCE_CONVERGENCE_TEST) return fail(TRIXI_HIP_EINVAL, "unknown source terms");
         if (cfg.equation != TRIXI_HIP_COMPRESSIBLE_EULER && cfg.equation != TRIXI_HIP_INVISCID_BURGERS &&
-            cfg.equation != TRIXI_HIP_COMPRESSIBLE_EULER_QUASI1D)
-            return fail(TRIXI_HIP_EUNSUPPORTED,
-                        "source_terms_convergence_test is built for compressible Euler, Burgers and quasi-1D Euler");
+            cfg.equation != TRIXI_HIP_COMPRESSIBLE_EULER_QUASI1D && cfg.equation != TRIXI_HIP_SHALLOW_WATER)
+            return fail(TRIXI_HIP_EUNSUPPORTED, "source_terms_convergence_test is built for compressible Euler, "
+                                                "Burgers, quasi-1D Euler and shallow water");
     }
     const int64_t NM = mesh->nmortars;
     if (NM < 0) return fail(TRIXI_HIP_EINVAL, "negative mortar count");
@@ -1300,6 +1315,7 @@ int trixi_hip_create(const trixi_hip_config* config, const trixi_hip_mesh* mesh,
         a.phys.nu = cfg.nu;
         a.phys.inv_Tr = cfg.nu / (cfg.Lr * cfg.Lr);  // T_r = Lr^2 / nu
     }
+    a.phys.grav = cfg.gravity;
     if (cfg.equation == TRIXI_HIP_COMPRESSIBLE_EULER_MULTICOMPONENT) {
         // c_v,k = R_k / (gamma_k - 1) (Trixi's cv), and c_v,k gamma_k
         for (int k = 0; k < cfg.ncomponents; ++k) {
@@ -1419,6 +1435,7 @@ static IcParams ic_params(const trixi_hip_ctx* c, int ic, uint64_t seed) {
     q.length = c->cfg.ic_length;
     q.seed = seed;
     q.nc = c->cfg.equation == TRIXI_HIP_COMPRESSIBLE_EULER_MULTICOMPONENT ? c->cfg.ncomponents : 0;
+    q.grav = c->cfg.gravity;
     for (int k = 0; k < MAXC; ++k) {
         q.cv[k] = c->args.phys.cv[k];
         q.cvg[k] = c->args.phys.cvg[k];
@@ -1564,6 +1581,8 @@ int trixi_hip_max_dt(trixi_hip_ctx* ctx, const double* u, double* max_dt, trixi_
         if (dim == 1) MS(HypDiff<1>, 1); else if (dim == 2) MS(HypDiff<2>, 2); else MS(HypDiff<3>, 3);
     } else if (eq == TRIXI_HIP_COMPRESSIBLE_EULER_QUASI1D) {
         MS(EulerQuasi1D, 1);
+    } else if (eq == TRIXI_HIP_SHALLOW_WATER) {
+        MS(ShallowWater2D, 2);
     } else if (eq == TRIXI_HIP_COMPRESSIBLE_EULER_MULTICOMPONENT) {
         using M11 = EulerMulti<1, 1>;
         using M12 = EulerMulti<1, 2>;

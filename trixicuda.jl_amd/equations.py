@@ -30,7 +30,7 @@ class AbstractEquations:
             return self.ndims + 2
         if self.equation_id == L.EQ_EULER_MULTI:
             return self.ndims + 1 + len(self.gammas)
-        if self.equation_id == L.EQ_EULER_QUASI1D:
+        if self.equation_id in (L.EQ_EULER_QUASI1D, L.EQ_SHALLOW_WATER):
             return 4
         return 9 if self.ndims > 1 else 8  # ideal MHD: no cleaning field in 1D
 
@@ -120,15 +120,33 @@ class CompressibleEulerEquationsQuasi1D(AbstractEquations):
         object.__setattr__(self, "gamma", float(gamma))
 
 
+@dataclass(frozen=True)
+class ShallowWaterEquations2D(AbstractEquations):
+    """(h, h v1, h v2, b) with the bottom topography b (Trixi
+    ShallowWaterEquations2D(; gravity_constant, H0))."""
+    gravity: float = 9.81
+    H0: float = 0.0
+
+    def __init__(self, *, gravity_constant, H0=0.0):
+        object.__setattr__(self, "ndims", 2)
+        object.__setattr__(self, "equation_id", L.EQ_SHALLOW_WATER)
+        object.__setattr__(self, "gravity", float(gravity_constant))
+        object.__setattr__(self, "H0", float(H0))
+
+
 def prim2cons(prim, equations):
     """Trixi's prim2cons for compressible Euler (rho, v.., p), quasi-1D Euler
-    (rho, v, p, a) and multicomponent Euler (v.., p, rho_1..rho_n), for
+    (rho, v, p, a), multicomponent Euler (v.., p, rho_1..rho_n) and shallow
+    water (H, v1, v2, b), for
     initial conditions written as Python functions."""
     prim = [float(x) for x in prim]
     nd = equations.ndims
     if equations.equation_id == L.EQ_EULER:
         rho, v, p = prim[0], prim[1:1 + nd], prim[1 + nd]
         return [rho] + [rho * vi for vi in v] + [p / (equations.gamma - 1) + 0.5 * rho * sum(vi * vi for vi in v)]
+    if equations.equation_id == L.EQ_SHALLOW_WATER:
+        H, v1, v2, b = prim
+        return [H - b, (H - b) * v1, (H - b) * v2, b]
     if equations.equation_id == L.EQ_EULER_QUASI1D:
         rho, v, p, a = prim
         return [a * rho, a * rho * v, a * (p / (equations.gamma - 1) + 0.5 * rho * v * v), a]
@@ -251,6 +269,12 @@ flux_nonconservative_powell = NumericalFlux("flux_nonconservative_powell", -1)
 # nonconservative pressure-gradient term (used as a pair)
 flux_chan_etal = NumericalFlux("flux_chan_etal", L.FLUX_CHAN_ETAL)
 flux_nonconservative_chan_etal = NumericalFlux("flux_nonconservative_chan_etal", -2)
+# shallow water: the volume pair (flux_wintermeyer_etal, flux_nonconservative_wintermeyer_etal)
+# and a surface flux with flux_nonconservative_fjordholm_etal
+flux_wintermeyer_etal = NumericalFlux("flux_wintermeyer_etal", L.FLUX_WINTERMEYER_ETAL)
+flux_fjordholm_etal = NumericalFlux("flux_fjordholm_etal", L.FLUX_FJORDHOLM_ETAL)
+flux_nonconservative_wintermeyer_etal = NumericalFlux("flux_nonconservative_wintermeyer_etal", -3)
+flux_nonconservative_fjordholm_etal = NumericalFlux("flux_nonconservative_fjordholm_etal", -4)
 
 
 def density_pressure(*args):  # identity markers of the HG indicator variables
@@ -391,7 +415,9 @@ def initial_condition_smooth_periodic(seed=20261202):
 def split_flux(flux):
     """Split a (conservative, nonconservative) tuple as Trixi accepts it."""
     if isinstance(flux, tuple):
-        if len(flux) != 2 or flux[1] not in (flux_nonconservative_powell, flux_nonconservative_chan_etal):
+        if len(flux) != 2 or flux[1] not in (flux_nonconservative_powell, flux_nonconservative_chan_etal,
+                                             flux_nonconservative_wintermeyer_etal,
+                                             flux_nonconservative_fjordholm_etal):
             raise NotImplementedError(f"unsupported flux tuple {flux}")
         return flux[0], flux[1]
     return flux, None

@@ -46,6 +46,8 @@ struct CConfig
     reserved::Int32
     gammas::NTuple{4, Float64}
     gas_constants::NTuple{4, Float64}
+    gravity::Float64
+    H0::Float64
 end
 
 struct CMesh
@@ -86,6 +88,7 @@ equation_id(::Union{HyperbolicDiffusionEquations1D, HyperbolicDiffusionEquations
 equation_id(::Union{CompressibleEulerMulticomponentEquations1D,
                     CompressibleEulerMulticomponentEquations2D}) = 5
 equation_id(::CompressibleEulerEquationsQuasi1D) = 6
+equation_id(::ShallowWaterEquations2D) = 7
 equation_id(eq) = throw(ArgumentError("equation $(typeof(eq)) is not built for HIP"))
 
 flux_id(::typeof(flux_central)) = 0
@@ -98,7 +101,10 @@ flux_id(f::FluxHLL) = f.min_max_speed === min_max_speed_einfeldt ? 9 :
                       f.min_max_speed === min_max_speed_davis ? 7 :
     throw(ArgumentError("FluxHLL is built with min_max_speed_davis"))
 flux_id(::typeof(flux_chan_etal)) = 8
-flux_id(f::Tuple) = (f[2] === flux_nonconservative_powell || f[2] === flux_nonconservative_chan_etal ?
+flux_id(::typeof(flux_wintermeyer_etal)) = 10
+flux_id(::typeof(flux_fjordholm_etal)) = 11
+flux_id(f::Tuple) = (f[2] in (flux_nonconservative_powell, flux_nonconservative_chan_etal,
+                            flux_nonconservative_wintermeyer_etal, flux_nonconservative_fjordholm_etal) ?
                      flux_id(f[1]) : throw(ArgumentError("unsupported nonconservative flux $(f[2])")))
 flux_id(f) = throw(ArgumentError("flux $f is not built for HIP"))
 
@@ -229,7 +235,9 @@ function make_config(solver::DGSEMHIP, equations, mesh; boundary_conditions = bo
             hasproperty(equations, :Lr) ? Float64(equations.Lr) : 0.0,
             Int32(length(gammas)), Int32(0),
             ntuple(k -> k <= length(gammas) ? Float64(gammas[k]) : 0.0, 4),
-            ntuple(k -> k <= length(gas_r) ? Float64(gas_r[k]) : 0.0, 4))
+            ntuple(k -> k <= length(gas_r) ? Float64(gas_r[k]) : 0.0, 4),
+            hasproperty(equations, :gravity) ? Float64(equations.gravity) : 0.0,
+            hasproperty(equations, :H0) ? Float64(equations.H0) : 0.0)
 end
 
 function SemidiscretizationHyperbolicHIP(mesh::TreeMesh, equations, initial_condition,

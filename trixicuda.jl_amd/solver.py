@@ -102,8 +102,13 @@ class DGSEMHIP:
         sflux, s_nc = split_flux(self.surface_flux)
         c.surface_flux = sflux.flux_id
         quasi = equations.equation_id == L.EQ_EULER_QUASI1D
-        nc_needed = (equations.equation_id == L.EQ_GLM_MHD and equations.ndims > 1) or quasi
-        nc_kind = E.flux_nonconservative_chan_etal if quasi else E.flux_nonconservative_powell
+        swe = equations.equation_id == L.EQ_SHALLOW_WATER
+        nc_needed = (equations.equation_id == L.EQ_GLM_MHD and equations.ndims > 1) or quasi or swe
+        # the nonconservative term each position takes (shallow water: Wintermeyer
+        # in the volume, Fjordholm at the surface, the reference tests' pairs)
+        nc_kind = (E.flux_nonconservative_fjordholm_etal if swe else
+                   E.flux_nonconservative_chan_etal if quasi else E.flux_nonconservative_powell)
+        nc_vol = E.flux_nonconservative_wintermeyer_etal if swe else nc_kind
         if nc_needed != (s_nc is not None) or (s_nc is not None and s_nc is not nc_kind):
             raise NotImplementedError("GLM-MHD in 2D/3D needs (flux, flux_nonconservative_powell) tuples, quasi-1D "
                                       "Euler (flux_chan_etal, flux_nonconservative_chan_etal), other equations "
@@ -114,7 +119,7 @@ class DGSEMHIP:
         c.alpha_max, c.alpha_min = 0.5, 0.001
         if vi.kind == L.VI_FD:
             vflux, v_nc = split_flux(vi.volume_flux)
-            if nc_needed != (v_nc is not None) or (v_nc is not None and v_nc is not nc_kind):
+            if nc_needed != (v_nc is not None) or (v_nc is not None and v_nc is not nc_vol):
                 raise NotImplementedError("the volume flux must pair the flux with the equation's nonconservative "
                                           "term (flux_nonconservative_powell, flux_nonconservative_chan_etal)")
             c.volume_flux = vflux.flux_id
@@ -142,6 +147,8 @@ class DGSEMHIP:
         if len(gammas) > 4:
             raise NotImplementedError("at most four gas components")
         c.ncomponents = len(gammas)
+        c.gravity = getattr(equations, "gravity", 0.0)
+        c.H0 = getattr(equations, "H0", 0.0)
         for k, (g, r) in enumerate(zip(gammas, getattr(equations, "gas_constants", ()))):
             c.gammas[k], c.gas_constants[k] = g, r
         return c
