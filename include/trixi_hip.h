@@ -58,7 +58,7 @@ enum trixi_hip_equation {
                                               rho_ncomponents), 1 or 2 components  */
     TRIXI_HIP_COMPRESSIBLE_EULER_QUASI1D = 6, /* CompressibleEulerEquationsQuasi1D:
                                               (a rho, a rho v, a e, a)              */
-    TRIXI_HIP_SHALLOW_WATER = 7            /* ShallowWaterEquations2D: (h, h v1, h v2, b);
+    TRIXI_HIP_SHALLOW_WATER = 7            /* ShallowWaterEquations{1,2}D: (h, h v, b);
                                               volume (flux_wintermeyer_etal,
                                               flux_nonconservative_wintermeyer_etal),
                                               surface (any flux,
@@ -84,7 +84,10 @@ enum trixi_hip_flux {
     TRIXI_HIP_FLUX_HLLE = 9,                 /* flux_hlle = FluxHLL(min_max_speed_einfeldt),
                                                 GLM-MHD 2D/3D                          */
     TRIXI_HIP_FLUX_WINTERMEYER_ETAL = 10,    /* shallow water                          */
-    TRIXI_HIP_FLUX_FJORDHOLM_ETAL = 11       /* shallow water                          */
+    TRIXI_HIP_FLUX_FJORDHOLM_ETAL = 11,      /* shallow water                          */
+    TRIXI_HIP_FLUX_HYDROSTATIC_LLF = 12      /* (FluxHydrostaticReconstruction(flux_lax_friedrichs,
+                                                hydrostatic_reconstruction_audusse_etal),
+                                                flux_nonconservative_audusse_etal), shallow water */
 };
 enum trixi_hip_precision {
     TRIXI_HIP_FP64 = 0,  /* all arithmetic in FP64                                    */
@@ -121,9 +124,11 @@ enum trixi_hip_initial_condition {
  * as in test/tree_dgsem_2d/euler_source_terms_nonperiodic.jl:12-17). */
 enum trixi_hip_boundary_condition {
     TRIXI_HIP_BC_PERIODIC = 0,  /* every face has an interface                     */
-    TRIXI_HIP_BC_DIRICHLET = 1  /* BoundaryConditionDirichlet(boundary_ic): state
+    TRIXI_HIP_BC_DIRICHLET = 1, /* BoundaryConditionDirichlet(boundary_ic): state
                                    boundary_ic(x_face, t); flux(u_inner, u_b) on
                                    + faces, flux(u_b, u_inner) on - faces        */
+    TRIXI_HIP_BC_SLIP_WALL = 2  /* boundary_condition_slip_wall (shallow water): the
+                                   inner state with the normal momentum reversed */
 };
 
 /* Source terms added after the Jacobian (reference cuda_sources!,

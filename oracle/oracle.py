@@ -20,10 +20,10 @@ _LIB_PATH = os.path.join(_HERE, "liboracle.so")
 EQ_LINADV, EQ_EULER, EQ_GLM_MHD, EQ_BURGERS, EQ_HYPDIFF, EQ_EULER_MULTI, EQ_EULER_QUASI1D, EQ_SHALLOW_WATER = range(8)
 VI_WEAK, VI_FD, VI_HG = 0, 1, 2
 (FLUX_CENTRAL, FLUX_LLF, FLUX_LLF_NAIVE, FLUX_RANOCHA, FLUX_SHIMA_ETAL, FLUX_HINDENLANG_GASSNER, FLUX_EC, FLUX_HLL,
- FLUX_CHAN_ETAL, FLUX_HLLE, FLUX_WINTERMEYER_ETAL, FLUX_FJORDHOLM_ETAL) = range(12)
+ FLUX_CHAN_ETAL, FLUX_HLLE, FLUX_WINTERMEYER_ETAL, FLUX_FJORDHOLM_ETAL, FLUX_HYDROSTATIC_LLF) = range(13)
 (IC_CONSTANT, IC_CONVERGENCE_TEST, IC_WEAK_BLAST_WAVE, IC_SEEDED_RANDOM, IC_SMOOTH_PERIODIC, IC_WEAK_BLAST_WAVE_SEEDED,
  IC_SHOCK, IC_RAREFACTION, IC_EOC_GRAVITY, IC_POISSON_NONPERIODIC, IC_HARMONIC_NONPERIODIC) = range(11)
-BC_PERIODIC, BC_DIRICHLET = 0, 1
+BC_PERIODIC, BC_DIRICHLET, BC_SLIP_WALL = 0, 1, 2
 SOURCE_NONE, SOURCE_CONVERGENCE_TEST, SOURCE_EOC_TEST_EULER, SOURCE_POISSON_NONPERIODIC, SOURCE_HARMONIC = range(5)
 IND_DENSITY_PRESSURE, IND_PRESSURE, IND_DENSITY = 0, 1, 2
 

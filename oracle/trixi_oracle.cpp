@@ -225,14 +225,12 @@ template <class T> void quasi_prim(const Cfg& c, const T* u, T& rho, T& v, T& p,
 template <class T> void phys_flux(const Cfg& c, const T* u, int o, T* f) {
     const int oi = o - 1;
     if (c.eq == OR_EQ_SHALLOW_WATER) {
-        // Trixi ShallowWaterEquations2D flux: (h v_o, h v1 v_o + g h^2/2 [o=1],
-        // h v2 v_o + g h^2/2 [o=2], 0)
-        const T h = u[0], v1 = u[1] / h, v2 = u[2] / h, p = 0.5 * c.grav * h * h;
-        const T hvo = u[o];
-        f[0] = hvo;
-        f[1] = (o == 1) ? u[1] * v1 + p : u[1] * v2;
-        f[2] = (o == 1) ? u[2] * v1 : u[2] * v2 + p;
-        f[3] = 0.0;
+        // Trixi ShallowWaterEquations{1,2}D flux: (h v_o, h v_d v_o + g h^2/2 [d = o], 0)
+        const T h = u[0], p = 0.5 * c.grav * h * h;
+        const T vo = u[o] / h;
+        f[0] = u[o];
+        for (int d = 0; d < c.nd; ++d) f[1 + d] = (d == oi) ? u[1 + d] * vo + p : u[1 + d] * vo;
+        f[c.nd + 1] = 0.0;
         return;
     }
     if (c.eq == OR_EQ_EULER_QUASI1D) {
@@ -528,18 +526,30 @@ void flux_hindenlang_gassner_mhd(const Cfg& c, const T* ull, const T* urr, int o
 // flux_nonconservative_powell (Trixi ideal_glm_mhd_3d.jl, orientation form):
 // Powell term (0, B, v.B, v, 0) and Galilean GLM term, multiplied by the
 // right state's normal B (resp. psi); callers apply the factor 0.5.
-// Nonconservative two-point terms; `surf` selects the surface term where the
-// volume and surface terms differ (shallow water).
-template <class T> void noncons_flux(const Cfg& c, const T* ull, const T* urr, int o, T* f, bool surf) {
+// Nonconservative two-point terms.  `pos` is -1 for the volume term, else the
+// surface (or shock-capturing subcell) flux the term is paired with; the
+// terms differ by position only for shallow water.
+template <class T> void noncons_flux(const Cfg& c, const T* ull, const T* urr, int o, T* f, int pos) {
     if (c.eq == OR_EQ_SHALLOW_WATER) {
-        // flux_nonconservative_wintermeyer_etal (volume): g h_ll b_rr;
-        // flux_nonconservative_fjordholm_etal (surface): g h_ll b_ll + g h_avg (b_rr - b_ll)
-        const T t = surf ? c.grav * ull[0] * ull[3] + c.grav * (0.5 * (ull[0] + urr[0])) * (urr[3] - ull[3])
-                         : c.grav * ull[0] * urr[3];
-        f[0] = 0.0;
-        f[1] = (o == 1) ? t : T(0.0);
-        f[2] = (o == 2) ? t : T(0.0);
-        f[3] = 0.0;
+        // volume: flux_nonconservative_wintermeyer_etal g h_ll b_rr; with the
+        // hydrostatic reconstruction: flux_nonconservative_audusse_etal
+        // g h_ll b_ll - g (h_ll* + h_ll)(h_ll* - h_ll), h* = max(0, h + b - max(b_ll, b_rr))
+        // (half of it plus the reconstructed flux g h*^2 / 2 gives the left
+        // cell g h_ll^2 / 2 + g h_ll b_ll / 2 at a lake at rest);
+        // otherwise flux_nonconservative_fjordholm_etal g h_ll b_ll + g h_avg (b_rr - b_ll)
+        const int nd = c.nd;
+        const T hl = ull[0], hr = urr[0], bl = ull[nd + 1], br = urr[nd + 1];
+        T t;
+        if (pos < 0) {
+            t = c.grav * hl * br;
+        } else if (pos == OR_FLUX_HYDROSTATIC_LLF) {
+            const T hs = tmax(T(0.0), hl + bl - tmax(bl, br));
+            t = c.grav * hl * bl - c.grav * (hs + hl) * (hs - hl);
+        } else {
+            t = c.grav * hl * bl + c.grav * (0.5 * (hl + hr)) * (br - bl);
+        }
+        for (int v = 0; v < c.nv; ++v) f[v] = 0.0;
+        f[o] = t;
         return;
     }
     if (c.eq == OR_EQ_EULER_QUASI1D) {
@@ -588,21 +598,40 @@ template <class T> void two_point(const Cfg& c, int kind, const T* ull, const T*
         case OR_FLUX_FJORDHOLM_ETAL: {
             // Trixi flux_wintermeyer_etal: mean h v_o, p = g h_ll h_rr / 2;
             // flux_fjordholm_etal: h_avg v_o,avg, p = g (h_ll^2 + h_rr^2) / 4
+            const int nd = c.nd;
             const T hl = ull[0], hr = urr[0];
-            const T v1l = ull[1] / hl, v2l = ull[2] / hl, v1r = urr[1] / hr, v2r = urr[2] / hr;
-            const T v1_avg = 0.5 * (v1l + v1r), v2_avg = 0.5 * (v2l + v2r);
+            T vavg[3];
+            for (int d = 0; d < nd; ++d) vavg[d] = 0.5 * (ull[1 + d] / hl + urr[1 + d] / hr);
             T f1, p_avg;
             if (kind == OR_FLUX_WINTERMEYER_ETAL) {
                 f1 = 0.5 * (ull[o] + urr[o]);
                 p_avg = 0.5 * c.grav * hl * hr;
             } else {
-                f1 = 0.5 * (hl + hr) * ((o == 1) ? v1_avg : v2_avg);
+                f1 = 0.5 * (hl + hr) * vavg[o - 1];
                 p_avg = 0.25 * c.grav * (hl * hl + hr * hr);
             }
             f[0] = f1;
-            f[1] = f1 * v1_avg + ((o == 1) ? p_avg : T(0.0));
-            f[2] = f1 * v2_avg + ((o == 2) ? p_avg : T(0.0));
-            f[3] = 0.0;
+            for (int d = 0; d < nd; ++d) f[1 + d] = f1 * vavg[d] + ((d == o - 1) ? p_avg : T(0.0));
+            f[nd + 1] = 0.0;
+            return;
+        }
+        case OR_FLUX_HYDROSTATIC_LLF: {
+            // Trixi FluxHydrostaticReconstruction(flux_lax_friedrichs,
+            // hydrostatic_reconstruction_audusse_etal): LLF of the states with
+            // h* = max(0, h + b - max(b_ll, b_rr)) and h* v (the velocities kept)
+            const int nd = c.nd;
+            const T bmax = tmax(ull[nd + 1], urr[nd + 1]);
+            T ls[MAXV], rs[MAXV];
+            const T hls = tmax(T(0.0), ull[0] + ull[nd + 1] - bmax), hrs = tmax(T(0.0), urr[0] + urr[nd + 1] - bmax);
+            ls[0] = hls;
+            rs[0] = hrs;
+            for (int d = 0; d < nd; ++d) {
+                ls[1 + d] = hls * (ull[1 + d] / ull[0]);
+                rs[1 + d] = hrs * (urr[1 + d] / urr[0]);
+            }
+            ls[nd + 1] = ull[nd + 1];
+            rs[nd + 1] = urr[nd + 1];
+            two_point(c, OR_FLUX_LLF, ls, rs, o, f);
             return;
         }
         case OR_FLUX_HINDENLANG_GASSNER:
@@ -661,7 +690,7 @@ template <class T> void two_point(const Cfg& c, int kind, const T* ull, const T*
             for (int v = 0; v < c.nv; ++v)
                 f[v] = 0.5 * (fl[v] + fr[v]) + (-0.5 * lam * (urr[v] - ull[v]));
             // Trixi's shallow-water LLF dissipation leaves the bottom topography out
-            if (c.eq == OR_EQ_SHALLOW_WATER) f[3] = 0.0;
+            if (c.eq == OR_EQ_SHALLOW_WATER) f[c.nd + 1] = 0.0;
             return;
         }
         case OR_FLUX_CENTRAL:
@@ -971,7 +1000,7 @@ void flux_differencing_element(const Cfg& c, const Basis& b, int kind, const T* 
                 const int s = ipow(N, d), idx = (n / s) % N;
                 for (int ii = 0; ii < N; ++ii) {
                     const int m = n + (ii - idx) * s;
-                    noncons_flux(c, ue + nv * n, ue + nv * m, d + 1, f, false);
+                    noncons_flux(c, ue + nv * n, ue + nv * m, d + 1, f, -1);
                     for (int v = 0; v < nv; ++v)
                         integral[v] = integral[v] + b.Ds[idx * N + ii] * f[v];
                 }
@@ -1003,8 +1032,8 @@ void fv_element(const Cfg& c, const Basis& b, const T* ue, T* due, T alpha) {
             two_point(c, c.fvflux, ue + nv * (n - s), ue + nv * n, d + 1, f);
             const int base = n - idx * s;
             if (has_noncons(c)) {
-                noncons_flux(c, ue + nv * (n - s), ue + nv * n, d + 1, ncl, true);
-                noncons_flux(c, ue + nv * n, ue + nv * (n - s), d + 1, ncr, true);
+                noncons_flux(c, ue + nv * (n - s), ue + nv * n, d + 1, ncl, c.fvflux);
+                noncons_flux(c, ue + nv * n, ue + nv * (n - s), d + 1, ncr, c.fvflux);
             }
             for (int v = 0; v < nv; ++v) {
                 fl[at(d, v, idx, base)] = has_noncons(c) ? f[v] + 0.5 * ncl[v] : f[v];
@@ -1033,6 +1062,10 @@ void fv_element(const Cfg& c, const Basis& b, const T* ue, T* due, T alpha) {
 // rho p = (gamma - 1)(rho rho_e - |rho v|^2 / 2), pressure
 // p = (gamma - 1)(rho_e - |rho v|^2 / (2 rho)) or density rho.
 template <class T> T indicator_variable(const Cfg& c, const T* u) {
+    if (c.eq == OR_EQ_SHALLOW_WATER) {
+        // Trixi waterheight_pressure: h * (g h^2 / 2)
+        return u[0] * (0.5 * c.grav * u[0] * u[0]);
+    }
     if (c.eq == OR_EQ_GLM_MHD) {
         // Trixi density_pressure(u, ::IdealGlmMhdEquations): rho p
         T rho, v[3], p, B[3], psi;
@@ -1276,8 +1309,8 @@ void rhs_impl(const Cfg& c, int64_t E, int64_t I, const int32_t* nbr, const int3
             }
             two_point(c, c.sflux, ull, urr, o, flux);
             if (has_noncons(c)) {
-                noncons_flux(c, ull, urr, o, ncl, true);
-                noncons_flux(c, urr, ull, o, ncr, true);
+                noncons_flux(c, ull, urr, o, ncl, c.sflux);
+                noncons_flux(c, urr, ull, o, ncr, c.sflux);
                 for (int v = 0; v < nv; ++v) {
                     SF(v, f, ldir, L) = flux[v] + 0.5 * ncl[v];
                     SF(v, f, rdir, R) = flux[v] + 0.5 * ncr[v];
@@ -1344,8 +1377,8 @@ void rhs_impl(const Cfg& c, int64_t E, int64_t I, const int32_t* nbr, const int3
                     T nc_small[MAXV], nc_large[MAXV];
                     for (int v = 0; v < nv; ++v) nc_small[v] = nc_large[v] = 0.0;
                     if (has_noncons(c)) {
-                        noncons_flux(c, uL, uS, o, nc_small, true);
-                        noncons_flux(c, uS, uL, o, nc_large, true);
+                        noncons_flux(c, uL, uS, o, nc_small, c.sflux);
+                        noncons_flux(c, uS, uL, o, nc_large, c.sflux);
                     }
                     for (int v = 0; v < nv; ++v) {
                         SF(v, q, dir_small, small) = flux[v] + 0.5 * nc_small[v];
@@ -1378,7 +1411,7 @@ void rhs_impl(const Cfg& c, int64_t E, int64_t I, const int32_t* nbr, const int3
     // the face node; the flux is surface_flux(u_inner, u_b) when the boundary
     // lies on the element's + side (Trixi direction even), else
     // surface_flux(u_b, u_inner).
-    if (c.bc == OR_BC_DIRICHLET) {
+    if (c.bc == OR_BC_DIRICHLET || c.bc == OR_BC_SLIP_WALL) {
         std::vector<char> has_interface((size_t)2 * nd * E, 0);
         for (int64_t i = 0; i < I; ++i) {
             const int o = ori[i];
@@ -1398,15 +1431,19 @@ void rhs_impl(const Cfg& c, int64_t E, int64_t I, const int32_t* nbr, const int3
                     const int node = face_node(N, nd, od, f, plus ? N - 1 : 0);
                     double x[3] = {0, 0, 0}, ubd[MAXV];
                     for (int d = 0; d < nd; ++d) x[d] = c.xnodes[((size_t)e * nn + node) * nd + d];
-                    if (c.eq == OR_EQ_HYPDIFF)
-                        hypdiff_state(c, c.bc_ic, x, c.t, c.bc_ic == OR_IC_POISSON_NONPERIODIC, c.bc_seed,
-                                      (uint64_t)(e * nn + node), ubd);
-                    else
-                        initial_condition(c, c.bc_ic, x, c.t, c.bc_seed, (uint64_t)(e * nn + node), ubd);
+                    if (c.bc == OR_BC_DIRICHLET) {  // (slip walls: the mirrored inner state below)
+                        if (c.eq == OR_EQ_HYPDIFF)
+                            hypdiff_state(c, c.bc_ic, x, c.t, c.bc_ic == OR_IC_POISSON_NONPERIODIC, c.bc_seed,
+                                          (uint64_t)(e * nn + node), ubd);
+                        else
+                            initial_condition(c, c.bc_ic, x, c.t, c.bc_seed, (uint64_t)(e * nn + node), ubd);
+                    }
                     for (int v = 0; v < nv; ++v) {
                         uin[v] = u[e * esz + nv * node + v];
-                        ub[v] = T(ubd[v]);
+                        ub[v] = (c.bc == OR_BC_SLIP_WALL) ? uin[v] : T(ubd[v]);
                     }
+                    // boundary_condition_slip_wall (shallow water): (h, -h v_n, b) outside
+                    if (c.bc == OR_BC_SLIP_WALL) ub[1 + od] = -uin[1 + od];
                     if ((c.outflow >> dir) & 1)
                         phys_flux(c, uin, od + 1, flux);  // outflow: flux(u_inner)
                     else if (plus)
@@ -1418,7 +1455,7 @@ void rhs_impl(const Cfg& c, int64_t E, int64_t I, const int32_t* nbr, const int3
                     // nonconservative flux, inner state first on either side
                     if (has_noncons(c) && !((c.outflow >> dir) & 1)) {
                         T nc[MAXV];
-                        noncons_flux(c, uin, ub, od + 1, nc, true);
+                        noncons_flux(c, uin, ub, od + 1, nc, c.sflux);
                         for (int v = 0; v < nv; ++v) flux[v] = flux[v] + 0.5 * nc[v];
                     }
                     for (int v = 0; v < nv; ++v) SF(v, f, dir, e) = flux[v];
@@ -1665,6 +1702,18 @@ void initial_condition(const Cfg& c, int ic, const double* x, double t, uint64_t
         // (0.7, 0.7), H = 4 / 3.25, b = 0); seeded: b = 0.1 U, h = 1 + 0.1 U,
         // v = 0.1 N(0, 1); constant: h = 1, v = 0, b = 0
         double H = 1.0, v1 = 0.0, v2 = 0.0, b = 0.0;
+        if (c.nd == 1) {
+            // 1D: seeded h = 1 + 0.1 U, v = 0.1 N(0, 1), b = 0.1 U; else h = 1 at rest
+            if (ic == OR_IC_SEEDED_RANDOM) {
+                b = 0.1 * u01(seed, idx, 6);
+                H = 1.0 + 0.1 * (2.0 * u01(seed, idx, 0) - 1.0) + b;
+                v1 = 0.1 * normal01(seed, idx, 2);
+            }
+            u[0] = H - b;
+            u[1] = (H - b) * v1;
+            u[2] = b;
+            return;
+        }
         if (ic == OR_IC_CONVERGENCE_TEST) {
             const double cc = 7.0, omega_x = 2.0 * M_PI * sqrt(2.0), omega_t = 2.0 * M_PI;
             H = cc + cos(omega_x * x[0]) * sin(omega_x * x[1]) * cos(omega_t * t);
@@ -1867,7 +1916,8 @@ int oracle_nvars(const oracle_cfg* cfg) {
     if (cfg->equation == OR_EQ_HYPDIFF) return cfg->ndims + 1;
     if (cfg->equation == OR_EQ_EULER) return cfg->ndims + 2;
     if (cfg->equation == OR_EQ_EULER_MULTI) return cfg->ndims + 1 + cfg->ncomponents;
-    if (cfg->equation == OR_EQ_EULER_QUASI1D || cfg->equation == OR_EQ_SHALLOW_WATER) return 4;
+    if (cfg->equation == OR_EQ_EULER_QUASI1D) return 4;
+    if (cfg->equation == OR_EQ_SHALLOW_WATER) return cfg->ndims + 2;
     return cfg->ndims == 1 ? 8 : 9;  // ideal MHD: (rho, rho v, rho e, B) in 1D, + psi in 2D/3D
 }
 
@@ -1953,7 +2003,8 @@ void oracle_two_point_flux(const oracle_cfg* cfg, int kind, const double* ull, c
 }
 void oracle_noncons_flux(const oracle_cfg* cfg, const double* ull, const double* urr, int o,
                          double* f) {
-    noncons_flux(make_cfg(cfg), ull, urr, o, f, true);
+    const Cfg c = make_cfg(cfg);
+    noncons_flux(c, ull, urr, o, f, c.sflux);
 }
 
 void oracle_max_abs_speeds(const oracle_cfg* cfg, const double* u, double* lam) {
@@ -1978,7 +2029,7 @@ void oracle_max_abs_speeds(const oracle_cfg* cfg, const double* u, double* lam) 
             quasi_prim(c, u, rho, v, p, a);
             lam[d] = fabs(v) + sqrt(c.gamma * p / rho);
         } else if (c.eq == OR_EQ_SHALLOW_WATER) {
-            lam[d] = fabs(u[1 + d] / u[0]) + sqrt(c.grav * u[0]);
+            lam[d] = fabs(u[1 + d] / u[0]) + sqrt(c.grav * u[0]);  // any dimension
         } else {
             lam[d] = fabs(u[1 + d] / u[0]) + mhd_fast_wavespeed(c, u, d + 1);
         }

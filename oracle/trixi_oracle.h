@@ -42,7 +42,8 @@ enum {
     OR_FLUX_CHAN_ETAL = 8,    /* quasi-1D: flux_chan_etal + flux_nonconservative_chan_etal */
     OR_FLUX_HLLE = 9,         /* GLM-MHD: FluxHLL(min_max_speed_einfeldt) */
     OR_FLUX_WINTERMEYER_ETAL = 10, /* shallow water (volume noncons g h_l b_r) */
-    OR_FLUX_FJORDHOLM_ETAL = 11    /* shallow water (surface noncons: Fjordholm) */
+    OR_FLUX_FJORDHOLM_ETAL = 11,   /* shallow water (surface noncons: Fjordholm) */
+    OR_FLUX_HYDROSTATIC_LLF = 12   /* shallow water: LLF of Audusse's hydrostatic reconstruction */
 };
 enum {
     OR_IC_CONSTANT = 0,
@@ -57,7 +58,7 @@ enum {
     OR_IC_POISSON_NONPERIODIC = 9,  /* hyperbolic diffusion: guess at t = 0, else steady */
     OR_IC_HARMONIC_NONPERIODIC = 10 /* hyperbolic diffusion 1D harmonic state */
 };
-enum { OR_BC_PERIODIC = 0, OR_BC_DIRICHLET = 1 };
+enum { OR_BC_PERIODIC = 0, OR_BC_DIRICHLET = 1, OR_BC_SLIP_WALL = 2 };
 enum { OR_SOURCE_NONE = 0, OR_SOURCE_CONVERGENCE_TEST = 1, OR_SOURCE_EOC_TEST_EULER = 2,
        OR_SOURCE_POISSON_NONPERIODIC = 3, OR_SOURCE_HARMONIC = 4 };
 

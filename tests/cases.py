@@ -39,16 +39,17 @@ class Case:
     gas_constants: tuple = (0.4, 0.4)
     ic_fn: object = None    # initial condition as a Python function (x, t, equations) -> conservative state
     gravity: float = 9.81   # shallow water
+    rk_cfl: float = 0.5     # RK step tests: dt = rk_cfl * max_dt
 
 
 FLUX_NAMES = ("central", "llf", "llf_naive", "ranocha", "shima_etal", "hindenlang_gassner", "ec", "hll", "chan_etal",
-              "hlle", "wintermeyer_etal", "fjordholm_etal")
+              "hlle", "wintermeyer_etal", "fjordholm_etal", "hydrostatic_llf")
 IC_NAMES = ("constant", "convergence_test", "weak_blast_wave", "seeded_random", "smooth_periodic",
             "weak_blast_wave_seeded", "shock", "rarefaction", "eoc_gravity", "poisson_nonperiodic",
             "harmonic_nonperiodic")
 SOURCES = {"": 0, "convergence_test": 1, "eoc_test_euler": 2, "poisson_nonperiodic": 3, "harmonic": 4}
 SIDES = ("x_neg", "x_pos", "y_neg", "y_pos", "z_neg", "z_pos")
-INDICATORS = {"density_pressure": 0, "pressure": 1, "density": 2, "first": 2}
+INDICATORS = {"density_pressure": 0, "pressure": 1, "density": 2, "first": 2, "waterheight_pressure": 0}
 
 
 def oracle_mesh(O, c: Case):
@@ -62,8 +63,9 @@ def oracle_cfg(O, c: Case):
           "hypdiff": O.EQ_HYPDIFF, "eulermulti": O.EQ_EULER_MULTI, "quasi1d": O.EQ_EULER_QUASI1D,
           "swe": O.EQ_SHALLOW_WATER}[c.equation]
     vi = {"weak": O.VI_WEAK, "fd": O.VI_FD, "hg": O.VI_HG}[c.vi]
-    ext = dict(boundary_condition=O.BC_DIRICHLET if c.bc_state else O.BC_PERIODIC,
-               boundary_ic=IC_NAMES.index(c.bc_state) if c.bc_state else 0,
+    ext = dict(boundary_condition=(O.BC_SLIP_WALL if c.bc_state == "slip_wall" else
+                                   O.BC_DIRICHLET if c.bc_state else O.BC_PERIODIC),
+               boundary_ic=IC_NAMES.index(c.bc_state) if c.bc_state and c.bc_state != "slip_wall" else 0,
                source_terms=SOURCES[c.source],
                boundary_outflow_faces=sum(1 << SIDES.index(s) for s in c.outflow))
     return O.make_cfg(c.ndims, eq, c.polydeg, vi, FLUX_NAMES.index(c.vflux), FLUX_NAMES.index(c.sflux),
@@ -84,15 +86,18 @@ def _flux(th, name, pair):
          "llf_naive": th.FluxLaxFriedrichs(th.max_abs_speed_naive), "ranocha": th.flux_ranocha,
          "shima_etal": th.flux_shima_etal, "hindenlang_gassner": th.flux_hindenlang_gassner, "ec": th.flux_ec,
          "hll": th.flux_hll, "chan_etal": th.flux_chan_etal, "hlle": th.flux_hlle,
-         "wintermeyer_etal": th.flux_wintermeyer_etal, "fjordholm_etal": th.flux_fjordholm_etal}[name]
+         "wintermeyer_etal": th.flux_wintermeyer_etal, "fjordholm_etal": th.flux_fjordholm_etal,
+         "hydrostatic_llf": th.FluxHydrostaticReconstruction(th.flux_lax_friedrichs,
+                                                             th.hydrostatic_reconstruction_audusse_etal)}[name]
     if pair == "mhd":
         return (f, th.flux_nonconservative_powell)
     if pair == "quasi1d":
         return (f, th.flux_nonconservative_chan_etal)
     if pair == "swe_volume":
         return (f, th.flux_nonconservative_wintermeyer_etal)
-    if pair == "swe_surface":
-        return (f, th.flux_nonconservative_fjordholm_etal)
+    if pair == "swe_surface":  # (the term belongs to the surface / subcell flux)
+        return (f, th.flux_nonconservative_audusse_etal if name == "hydrostatic_llf"
+                else th.flux_nonconservative_fjordholm_etal)
     return f
 
 
@@ -113,7 +118,7 @@ def product_semi(th, c: Case, precision=None):
     elif c.equation == "quasi1d":
         eq = th.CompressibleEulerEquationsQuasi1D(c.gamma)
     elif c.equation == "swe":
-        eq = th.ShallowWaterEquations2D(gravity_constant=c.gravity)
+        eq = [None, th.ShallowWaterEquations1D, th.ShallowWaterEquations2D][c.ndims](gravity_constant=c.gravity)
     elif c.equation == "eulermulti":
         eq = [None, th.CompressibleEulerMulticomponentEquations1D,
               th.CompressibleEulerMulticomponentEquations2D][c.ndims](gammas=c.gammas,
@@ -129,8 +134,9 @@ def product_semi(th, c: Case, precision=None):
     else:
         ind = th.IndicatorHennemannGassner(eq, basis, alpha_max=c.alpha_max, alpha_min=c.alpha_min,
                                            alpha_smooth=c.alpha_smooth, variable=getattr(th, c.indicator))
-        vi = th.VolumeIntegralShockCapturingHG(ind, volume_flux_dg=_flux(th, c.vflux, mhd),
-                                               volume_flux_fv=_flux(th, c.fvflux, mhd))
+        swe = c.equation == "swe"
+        vi = th.VolumeIntegralShockCapturingHG(ind, volume_flux_dg=_flux(th, c.vflux, "swe_volume" if swe else mhd),
+                                               volume_flux_fv=_flux(th, c.fvflux, "swe_surface" if swe else mhd))
     solver = th.DGSEMHIP(polydeg=c.polydeg, surface_flux=_flux(th, c.sflux, "swe_surface" if c.equation == "swe" else mhd),
                          volume_integral=vi,
                          precision=precision or c.precision)
@@ -147,7 +153,9 @@ def product_semi(th, c: Case, precision=None):
            "poisson_nonperiodic": th.initial_condition_poisson_nonperiodic,
            "harmonic_nonperiodic": th.initial_condition_harmonic_nonperiodic}
     kw = {}
-    if c.bc_state:
+    if c.bc_state == "slip_wall":
+        kw["boundary_conditions"] = th.boundary_condition_slip_wall
+    elif c.bc_state:
         bc = (th.boundary_condition_poisson_nonperiodic if c.bc_state == "poisson_nonperiodic"
               else th.BoundaryConditionDirichlet(ics[c.bc_state]))
         per = c.periodic if isinstance(c.periodic, tuple) else (c.periodic,) * c.ndims
@@ -227,3 +235,21 @@ def oracle_state(O, c, mesh, t=0.0):
     pts = x.reshape(-1, c.ndims)
     u = np.array([c.ic_fn(tuple(p), t, eq) for p in pts], dtype=np.float64)
     return u.reshape(x.shape[:-1] + (u.shape[-1],))
+
+
+def stone_throw_state(x, t, equations):
+    """initial_condition_stone_throw_discontinuous_bottom of
+    test/tree_dgsem_1d/shallowwater_shock.jl: flat lake H = H0 = 1.75,
+    v = -1 on [-0.75, 0] and 1 on [0, 0.75], bottom
+    1.5 exp(-(x-1)^2/2) + 0.75 exp(-(x+1)^2/2), set to 0.5 on [-1.5, 0]."""
+    x1 = x[0]
+    H = 1.75
+    v = 0.0
+    if -0.75 <= x1 <= 0.0:
+        v = -1.0
+    elif 0.0 <= x1 <= 0.75:
+        v = 1.0
+    b = 1.5 / np.exp(0.5 * (x1 - 1.0) ** 2) + 0.75 / np.exp(0.5 * (x1 + 1.0) ** 2)
+    if -1.5 <= x1 <= 0.0:
+        b = 0.5
+    return [H - b, (H - b) * v, b]

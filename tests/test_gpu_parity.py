@@ -8,7 +8,7 @@ import numpy as np
 import pytest
 
 from cases import (oracle_mesh, Case, oracle_cfg, oracle_ic, oracle_state, product_semi, rel_max, eulerquasi_ec_state,
-                   euler_blast_wave_1d_state)
+                   euler_blast_wave_1d_state, stone_throw_state)
 
 pytestmark = pytest.mark.gpu
 
@@ -98,6 +98,18 @@ SWE_CASES = [
          ic="convergence_test", source="convergence_test", t=0.3, periodic=False, bc_state="convergence_test"),
     Case("swe2d_random_p4", 2, "swe", 3, "fd", "wintermeyer_etal", "fjordholm_etal", polydeg=4),
     Case("swe2d_random_llf", 2, "swe", 3, "fd", "wintermeyer_etal", "llf"),
+    # test/tree_dgsem_1d/shallowwater_shock.jl as written: its stone-throw state
+    # (g = 9.812, H0 = 1.75, discontinuous bottom), polydeg 4, level 3 on
+    # [-3, 3], slip walls, HG with (wintermeyer, nc wintermeyer) as DG flux and
+    # (hydrostatic reconstruction LLF, nc audusse) as FV and surface flux,
+    # indicator waterheight_pressure
+    Case("swe1d_shock", 1, "swe", 3, "hg", "wintermeyer_etal", "hydrostatic_llf", "hydrostatic_llf", polydeg=4,
+         cmin=-3.0, cmax=3.0, ic="constant", ic_fn=stone_throw_state, gravity=9.812, periodic=False,
+         bc_state="slip_wall", indicator="waterheight_pressure", rk_cfl=0.25),  # (0.5 max_dt is unstable here)
+    Case("swe1d_hg_random", 1, "swe", 4, "hg", "wintermeyer_etal", "hydrostatic_llf", "hydrostatic_llf",
+         polydeg=4, gravity=9.812, alpha_min=0.0, alpha_max=1.0),
+    Case("swe1d_fd_random_walls", 1, "swe", 4, "fd", "wintermeyer_etal", "hydrostatic_llf", polydeg=4,
+         gravity=9.812, periodic=False, bc_state="slip_wall"),
 ]
 # test/tree_dgsem_1d/euler_blast_wave.jl: its initial_condition_blast_wave,
 # HG with flux_ranocha (DG) and LLF (FV, surface), polydeg 3, level 6
@@ -421,6 +433,8 @@ def test_initial_condition_and_callbacks_match_oracle(th, oracle, torch, case):
                                   # polydeg 4: blocks with idle lanes (2D: two 25-node
                                   # elements per wave; 3D: 125 nodes on 128 lanes)
                                   MHD_HG_P4[0], MHD_HG_P4[1],
+                                  # test/tree_dgsem_1d/shallowwater_shock.jl: waterheight_pressure
+                                  SWE_CASES[5],
                                   # x = 0.5 falls inside element 29 of 64 on [0, 1.1]
                                   Case("burgers1d_hg_shock", 1, "burgers", 6, "hg", "ec", "llf", "llf", cmin=0.0,
                                        cmax=1.1, ic="shock", periodic=False, bc_state="shock", outflow=("x_pos",),
@@ -430,7 +444,7 @@ def test_indicator_matches_oracle(th, oracle, torch, case):
     O = oracle
     cfg = oracle_cfg(O, case)
     mesh = oracle_mesh(O, case)
-    u = O.compute_coefficients(cfg, mesh, oracle_ic(O, case))
+    u = oracle_state(O, case, mesh)
     alpha_ref = O.indicator_hg(cfg, mesh, u)
     semi = product_semi(th, case)
     alpha = th.blending_factors(th.pack(torch.from_numpy(u).cuda(), semi), semi).cpu().numpy()
@@ -474,7 +488,8 @@ def test_indicator_matches_oracle(th, oracle, torch, case):
                                   Case("mhd2d_hg_blast", 2, "mhd", 3, "hg", "hindenlang_gassner", "hindenlang_gassner",
                                        "hindenlang_gassner", cmin=-2.0, cmax=2.0, ic="weak_blast_wave_seeded",
                                        gamma=1.4),
-                                  MHD_HG_P4[0], MULTI_CASES[3], MULTI_CASES[4], QUASI_CASES[1], SWE_CASES[2]],
+                                  MHD_HG_P4[0], MULTI_CASES[3], MULTI_CASES[4], QUASI_CASES[1], SWE_CASES[2],
+                                  SWE_CASES[5]],
                          ids=lambda c: c.name)
 def test_rk_step_matches_oracle(th, oracle, torch, case, use_graph):
     """Three CK2N54 steps from t = 0 (stage times through device memory: the
@@ -482,11 +497,11 @@ def test_rk_step_matches_oracle(th, oracle, torch, case, use_graph):
     O = oracle
     cfg = oracle_cfg(O, case)
     mesh = oracle_mesh(O, case)
-    u = O.compute_coefficients(cfg, mesh, oracle_ic(O, case), 0.0, case.seed)
+    u = oracle_state(O, case, mesh)
     semi = product_semi(th, case)
     ud = th.pack(torch.from_numpy(u).cuda(), semi)
     kd = torch.zeros_like(ud)
-    dt = 0.5 * O.max_dt(cfg, mesh, u)
+    dt = case.rk_cfl * O.max_dt(cfg, mesh, u)
     k = np.zeros_like(u)
     uo = u.copy()
     stream = torch.cuda.Stream()

@@ -211,3 +211,23 @@ def test_prim2cons_and_quasi1d_flux_pairs(th):
     with pytest.raises(NotImplementedError):  # the MHD term with quasi-1D Euler
         th.DGSEMHIP(polydeg=4, surface_flux=(th.flux_chan_etal, th.flux_nonconservative_powell),
                     volume_integral=th.VolumeIntegralFluxDifferencing(pair)).config(q)
+
+
+def test_shallow_water_1d_configuration(th):
+    """shallowwater_shock.jl's solver: HG with the Wintermeyer pair (DG) and
+    the hydrostatic reconstruction with Audusse's term (FV, surface)."""
+    eq = th.ShallowWaterEquations1D(gravity_constant=9.812, H0=1.75)
+    assert eq.nvariables == 3
+    hr = (th.FluxHydrostaticReconstruction(th.flux_lax_friedrichs, th.hydrostatic_reconstruction_audusse_etal),
+          th.flux_nonconservative_audusse_etal)
+    vol = (th.flux_wintermeyer_etal, th.flux_nonconservative_wintermeyer_etal)
+    ind = th.IndicatorHennemannGassner(eq, th.LobattoLegendreBasisHIP(4), alpha_max=0.5, alpha_min=0.001,
+                                       alpha_smooth=True, variable=th.waterheight_pressure)
+    c = th.DGSEMHIP(polydeg=4, surface_flux=hr, volume_integral=th.VolumeIntegralShockCapturingHG(
+        ind, volume_flux_dg=vol, volume_flux_fv=hr)).config(eq)
+    assert (c.equation, c.volume_integral, c.volume_flux, c.fv_flux, c.surface_flux, c.indicator_variable,
+            c.gravity) == (7, 2, 10, 12, 12, 0, 9.812)
+    with pytest.raises(NotImplementedError):  # Fjordholm's term with the reconstruction
+        th.DGSEMHIP(polydeg=4, surface_flux=(hr[0], th.flux_nonconservative_fjordholm_etal),
+                    volume_integral=th.VolumeIntegralFluxDifferencing(vol)).config(eq)
+    assert th.prim2cons((1.75, 1.0, 0.5), eq) == pytest.approx([1.25, 1.25, 0.5])

@@ -1018,3 +1018,52 @@ def test_shallow_water_source_and_convergence(oracle):
                   - O.compute_coefficients(c, mesh, O.IC_CONVERGENCE_TEST, t - hh)) / (2 * hh)
             errs.append(np.abs(O.rhs(c, mesh, u, t=t) - ut).max())
         assert np.log2(errs[0] / errs[1]) > 2.5, (periodic, errs)
+
+
+def test_shallow_water_1d_hydrostatic_reconstruction(oracle):
+    """1D shallow water with the hydrostatic reconstruction
+    (test/tree_dgsem_1d/shallowwater_shock.jl): the reconstructed LLF flux is
+    consistent; the lake at rest stays at rest with a bottom that jumps at
+    every node, for flux differencing and for shock capturing at any blending
+    (Audusse's term; its sign and factor are fixed by this); the total
+    energy does not grow with the reconstruction at the surface; slip walls
+    keep a lake at rest and reflect: with v = 0 at the wall the mass flux
+    through it vanishes (total mass conserved)."""
+    O = oracle
+    g = 9.812
+    rng = np.random.default_rng(60)
+    kw = dict(gravity=g)
+    cfg = O.make_cfg(1, O.EQ_SHALLOW_WATER, 4, O.VI_FD, O.FLUX_WINTERMEYER_ETAL, O.FLUX_HYDROSTATIC_LLF, **kw)
+    for _ in range(10):
+        u = np.array([1.0 + rng.random(), 0.3 * rng.standard_normal(), rng.random()])
+        np.testing.assert_allclose(O.two_point_flux(cfg, O.FLUX_HYDROSTATIC_LLF, u, u, 1), O.flux(cfg, u, 1),
+                                   rtol=1e-14, atol=1e-14)
+    for periodic, bc in ((True, O.BC_PERIODIC), (False, O.BC_SLIP_WALL)):
+        mesh = O.Mesh(1, 3, -3.0, 3.0, 4, periodic=periodic)
+        u = np.zeros((mesh.nelements, 5, 3))
+        b = 0.8 * rng.random(u.shape[:-1])
+        u[..., 0], u[..., 2] = 1.75 - b, b
+        tol = 1e-15 * g * 1.75 ** 2 * 25 * mesh.inverse_jacobian.max() * 10
+        for vi, fv in ((O.VI_FD, O.FLUX_CENTRAL), (O.VI_HG, O.FLUX_HYDROSTATIC_LLF)):
+            for amax in (0.5, 1.0):
+                c = O.make_cfg(1, O.EQ_SHALLOW_WATER, 4, vi, O.FLUX_WINTERMEYER_ETAL, O.FLUX_HYDROSTATIC_LLF, fv,
+                               alpha_max=amax, alpha_min=0.0, boundary_condition=bc, **kw)
+                assert np.abs(O.rhs(c, mesh, u)).max() < tol, (periodic, vi, amax)
+    mesh = O.Mesh(1, 3, -3.0, 3.0, 4, periodic=False)
+    c = O.make_cfg(1, O.EQ_SHALLOW_WATER, 4, O.VI_FD, O.FLUX_WINTERMEYER_ETAL, O.FLUX_HYDROSTATIC_LLF,
+                   boundary_condition=O.BC_SLIP_WALL, **kw)
+    u = O.compute_coefficients(c, mesh, O.IC_SEEDED_RANDOM)
+    du = O.rhs(c, mesh, u)
+    assert abs(O.integrate(c, mesh, du)[0]) < 1e-13 * np.abs(du).max() * mesh.total_volume
+
+    def energy(q):
+        h, hv, bb = q
+        return 0.5 * hv * hv / h + 0.5 * g * h * h + g * h * bb
+
+    w = np.zeros_like(u)
+    for j in range(3):
+        uc = u.astype(complex)
+        uc[..., j] += 1j * 1e-30
+        w[..., j] = energy(np.moveaxis(uc, -1, 0)).imag / 1e-30
+    rate = O.integrate(c, mesh, np.repeat((w * du).sum(-1, keepdims=True), 3, axis=-1))[0]
+    assert rate <= 1e-12 * np.abs(du).max() * mesh.total_volume, rate

@@ -21,7 +21,9 @@ from .equations import (BoundaryConditionDirichlet, boundary_condition_periodic,
                         CompressibleEulerEquationsQuasi1D, flux_chan_etal, flux_nonconservative_chan_etal, prim2cons,
                         flux_hlle, min_max_speed_einfeldt, ShallowWaterEquations2D, flux_wintermeyer_etal,
                         flux_fjordholm_etal, flux_nonconservative_wintermeyer_etal,
-                        flux_nonconservative_fjordholm_etal,
+                        flux_nonconservative_fjordholm_etal, ShallowWaterEquations1D, FluxHydrostaticReconstruction,
+                        hydrostatic_reconstruction_audusse_etal, flux_nonconservative_audusse_etal,
+                        boundary_condition_slip_wall, waterheight_pressure,
                         FluxLaxFriedrichs, IdealGlmMhdEquations1D, IdealGlmMhdEquations2D, IdealGlmMhdEquations3D, LinearScalarAdvectionEquation1D,
                         LinearScalarAdvectionEquation2D, LinearScalarAdvectionEquation3D, density, density_pressure, pressure,
                         InviscidBurgersEquation1D, first, flux_ec, initial_condition_shock, FluxHLL, flux_hll,

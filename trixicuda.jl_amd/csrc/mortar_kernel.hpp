@@ -115,8 +115,8 @@ __global__ void __launch_bounds__(256) mortar_kernel(const MortarArgs a) {
 #pragma unroll
         for (int v = 0; v < NV; ++v) nc_small[v] = nc_large[v] = 0.0;
         if constexpr (EQ::NONCONS) {
-            EQ::template noncons<true>(nl, ns, od, P, nc_small);
-            EQ::template noncons<true>(ns, nl, od, P, nc_large);
+            EQ::template noncons<SF>(nl, ns, od, P, nc_small);
+            EQ::template noncons<SF>(ns, nl, od, P, nc_large);
         }
         double* out = a.mflux + ((m * (K + 1) + p) * NL + q) * NV;
 #pragma unroll

@@ -46,7 +46,8 @@ enum Flux : int {
     FLUX_CHAN_ETAL = 8,   // flux_chan_etal + flux_nonconservative_chan_etal (quasi-1D Euler)
     FLUX_HLLE = 9,        // flux_hlle = FluxHLL(min_max_speed_einfeldt) (GLM-MHD)
     FLUX_WINTERMEYER_ETAL = 10,  // shallow water (volume; nonconservative term g h_l b_r)
-    FLUX_FJORDHOLM_ETAL = 11     // shallow water (surface; nonconservative term g h_l b_l + g h_avg [[b]])
+    FLUX_FJORDHOLM_ETAL = 11,    // shallow water (surface; nonconservative term g h_l b_l + g h_avg [[b]])
+    FLUX_HYDROSTATIC_LLF = 12    // shallow water: LLF of Audusse's hydrostatic reconstruction (+ Audusse's term)
 };
 
 // two-point fluxes that take ln-means (they need log(rho), log(p) per node)
@@ -787,7 +788,7 @@ struct EulerQuasi1D {
             }
         }
     }
-    template <bool SURF, typename R>
+    template <int POS, typename R>
     __device__ static void noncons(const Node<R>& l, const Node<R>& r, int, const PhysR<R>&, R* f) {
         f[0] = R(0);
         f[1] = l.a * r.p;
@@ -802,24 +803,27 @@ struct EulerQuasi1D {
 };
 
 // ===========================================================================
-// Shallow water in 2D (Trixi ShallowWaterEquations2D): (h, h v1, h v2, b)
-// with the bottom topography b; flux (h v_o, h v v_o + g h^2/2 e_o, 0) and the
+// Shallow water (Trixi ShallowWaterEquations{1,2}D): (h, h v_1..DIM, b) with
+// the bottom topography b; flux (h v_o, h v v_o + g h^2/2 e_o, 0) and the
 // nonconservative bottom term g h grad b.  The volume pairs
-// flux_wintermeyer_etal with g h_l b_r, the surface any flux with Fjordholm's
-// g h_l b_l + g (h_l + h_r)/2 (b_r - b_l) (the reference tests' pairs);
-// LLF does not dissipate b.
+// flux_wintermeyer_etal with g h_l b_r; at surfaces (and in shock-capturing
+// subcells) the term belongs to the surface flux: Fjordholm's
+// g h_l b_l + g (h_l + h_r)/2 (b_r - b_l), or, with the hydrostatic
+// reconstruction, Audusse's g h_l b_l - g (h_l* + h_l)(h_l* - h_l) with
+// h* = max(0, h + b - max(b_l, b_r)).  LLF does not dissipate b.
 // ===========================================================================
-struct ShallowWater2D {
-    static constexpr int NV = 4;
+template <int DIM>
+struct ShallowWater {
+    static constexpr int NV = DIM + 2;
     static constexpr bool NONCONS = true;
-    static constexpr int NRF = 4;  // node record: h, v1, v2, b
+    static constexpr int NRF = DIM + 2;  // node record: h, v[DIM], b
     template <typename R>
     struct Node {
-        R h, v[2], b;
+        R h, v[DIM], b;
         double lrho, lp;  // unused
         __device__ R cons(int q) const {
             if (q == 0) return h;
-            if (q <= 2) return h * v[q - 1];
+            if (q <= DIM) return h * v[q - 1];
             return b;
         }
     };
@@ -828,26 +832,26 @@ struct ShallowWater2D {
         Node<R> n;
         n.h = u[0];
         const R inv_h = rdiv(R(1), n.h);
-        n.v[0] = u[1] * inv_h;
-        n.v[1] = u[2] * inv_h;
-        n.b = u[3];
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) n.v[d] = u[1 + d] * inv_h;
+        n.b = u[DIM + 1];
         n.lrho = n.lp = 0.0;
         return n;
     }
     template <typename R>
     __device__ static void to_rec(const Node<R>& n, R* f) {
         f[0] = n.h;
-        f[1] = n.v[0];
-        f[2] = n.v[1];
-        f[3] = n.b;
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) f[1 + d] = n.v[d];
+        f[DIM + 1] = n.b;
     }
     template <typename R>
     __device__ static Node<R> from_rec(const R* f, double, double, const PhysR<R>&) {
         Node<R> n;
         n.h = f[0];
-        n.v[0] = f[1];
-        n.v[1] = f[2];
-        n.b = f[3];
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) n.v[d] = f[1 + d];
+        n.b = f[DIM + 1];
         n.lrho = n.lp = 0.0;
         return n;
     }
@@ -856,9 +860,26 @@ struct ShallowWater2D {
         const R hvo = n.h * n.v[o];
         const R p = R(0.5) * P.grav * n.h * n.h;
         f[0] = hvo;
-        f[1] = hvo * n.v[0] + (o == 0 ? p : R(0));
-        f[2] = hvo * n.v[1] + (o == 1 ? p : R(0));
-        f[3] = R(0);
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) f[1 + d] = hvo * n.v[d] + (d == o ? p : R(0));
+        f[DIM + 1] = R(0);
+    }
+    // LLF of two states, b not dissipated
+    template <int KIND, typename R>
+    __device__ static void llf(const Node<R>& l, const Node<R>& r, int o, const PhysR<R>& P, R* f) {
+        R fl[NV], fr[NV];
+        flux(l, o, P, fl);
+        flux(r, o, P, fr);
+        const R cl = rsqrt_(P.grav * l.h), cr = rsqrt_(P.grav * r.h);
+        R lam;
+        if (KIND == FLUX_LLF_NAIVE)
+            lam = dmax(dabs(l.v[o]), dabs(r.v[o])) + dmax(cl, cr);
+        else
+            lam = dmax(dabs(l.v[o]) + cl, dabs(r.v[o]) + cr);
+        const R hl = R(-0.5) * lam;
+#pragma unroll
+        for (int q = 0; q <= DIM; ++q) f[q] = R(0.5) * (fl[q] + fr[q]) + hl * (r.cons(q) - l.cons(q));
+        f[DIM + 1] = R(0);
     }
     template <int KIND, typename R>
     __device__ static void two_point(const Node<R>& l, const Node<R>& r, int o, const PhysR<R>& P, R* f) {
@@ -872,46 +893,56 @@ struct ShallowWater2D {
                 p_avg = R(0.25) * P.grav * (l.h * l.h + r.h * r.h);
             }
             f[0] = f0;
-            f[1] = f0 * (R(0.5) * (l.v[0] + r.v[0])) + (o == 0 ? p_avg : R(0));
-            f[2] = f0 * (R(0.5) * (l.v[1] + r.v[1])) + (o == 1 ? p_avg : R(0));
-            f[3] = R(0);
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) f[1 + d] = f0 * (R(0.5) * (l.v[d] + r.v[d])) + (d == o ? p_avg : R(0));
+            f[DIM + 1] = R(0);
+        } else if (KIND == FLUX_HYDROSTATIC_LLF) {
+            // FluxHydrostaticReconstruction(flux_lax_friedrichs,
+            // hydrostatic_reconstruction_audusse_etal): LLF of the states with
+            // h* = max(0, h + b - max(b_l, b_r)), the velocities kept
+            const R bmax = dmax(l.b, r.b);
+            Node<R> ls = l, rs = r;
+            ls.h = dmax(R(0), l.h + l.b - bmax);
+            rs.h = dmax(R(0), r.h + r.b - bmax);
+            llf<FLUX_LLF>(ls, rs, o, P, f);
+        } else if (KIND == FLUX_LLF || KIND == FLUX_LLF_NAIVE) {
+            llf<KIND>(l, r, o, P, f);
         } else {
             R fl[NV], fr[NV];
             flux(l, o, P, fl);
             flux(r, o, P, fr);
-            if (KIND == FLUX_LLF || KIND == FLUX_LLF_NAIVE) {
-                const R cl = rsqrt_(P.grav * l.h), cr = rsqrt_(P.grav * r.h);
-                R lam;
-                if (KIND == FLUX_LLF_NAIVE)
-                    lam = dmax(dabs(l.v[o]), dabs(r.v[o])) + dmax(cl, cr);
-                else
-                    lam = dmax(dabs(l.v[o]) + cl, dabs(r.v[o]) + cr);
-                const R hl = R(-0.5) * lam;
 #pragma unroll
-                for (int q = 0; q < 3; ++q) f[q] = R(0.5) * (fl[q] + fr[q]) + hl * (r.cons(q) - l.cons(q));
-                f[3] = R(0);
-            } else {
-#pragma unroll
-                for (int q = 0; q < NV; ++q) f[q] = R(0.5) * (fl[q] + fr[q]);
-            }
+            for (int q = 0; q < NV; ++q) f[q] = R(0.5) * (fl[q] + fr[q]);
         }
     }
-    // volume: flux_nonconservative_wintermeyer_etal g h_l b_r; surface:
-    // flux_nonconservative_fjordholm_etal g h_l b_l + g h_avg (b_r - b_l)
-    template <bool SURF, typename R>
+    // POS < 0: the volume term (flux_nonconservative_wintermeyer_etal); else
+    // the term paired with surface / subcell flux POS
+    template <int POS, typename R>
     __device__ static void noncons(const Node<R>& l, const Node<R>& r, int o, const PhysR<R>& P, R* f) {
-        const R t = SURF ? P.grav * l.h * l.b + P.grav * (R(0.5) * (l.h + r.h)) * (r.b - l.b) : P.grav * l.h * r.b;
+        R t;
+        if (POS < 0) {
+            t = P.grav * l.h * r.b;
+        } else if (POS == FLUX_HYDROSTATIC_LLF) {
+            const R hs = dmax(R(0), l.h + l.b - dmax(l.b, r.b));
+            t = P.grav * l.h * l.b - P.grav * (hs + l.h) * (hs - l.h);
+        } else {
+            t = P.grav * l.h * l.b + P.grav * (R(0.5) * (l.h + r.h)) * (r.b - l.b);
+        }
         f[0] = R(0);
-        f[1] = (o == 0) ? t : R(0);
-        f[2] = (o == 1) ? t : R(0);
-        f[3] = R(0);
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) f[1 + d] = (d == o) ? t : R(0);
+        f[DIM + 1] = R(0);
     }
     template <typename R>
     __device__ static void max_abs_speeds(const R* u, const PhysR<R>& P, R* lam) {
         const Node<R> n = node<R, false>(u, P);
         const R c = rsqrt_(P.grav * n.h);
-        lam[0] = dabs(n.v[0]) + c;
-        lam[1] = dabs(n.v[1]) + c;
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) lam[d] = dabs(n.v[d]) + c;
+    }
+    // waterheight_pressure = h g h^2 / 2, the HG indicator variable
+    __device__ static double waterheight_pressure(const double* u, const PhysR<double>& P) {
+        return u[0] * (0.5 * P.grav * u[0] * u[0]);
     }
 };
 
@@ -1138,7 +1169,7 @@ struct GlmMhd {
 
     // flux_nonconservative_powell(u_ll, u_rr, o): Powell (0, B, v.B, v, 0) and
     // Galilean GLM terms of the left state times the right state's B_o / psi.
-    template <bool SURF, typename R>
+    template <int POS, typename R>
     __device__ static void noncons(const Node<R>& l, const Node<R>& r, int o, const PhysR<R>&, R* f) {
         const R v_dot_B_ll = l.v[0] * l.B[0] + l.v[1] * l.B[1] + l.v[2] * l.B[2];
         const R Bo_rr = r.B[o];
@@ -1217,7 +1248,7 @@ struct GlmMhd1D {
 #pragma unroll
         for (int q = 0; q < 8; ++q) f[q] = f9[q];
     }
-    template <bool SURF, typename R>
+    template <int POS, typename R>
     __device__ static void noncons(const Node<R>&, const Node<R>&, int, const PhysR<R>&, R* f) {
 #pragma unroll
         for (int q = 0; q < 8; ++q) f[q] = R(0);

@@ -385,6 +385,11 @@ rhs_kernel(const RhsArgs a) {
                 if ((a.outflow >> (2 * d)) & 1) {
 #pragma unroll
                     for (int v = 0; v < NV; ++v) um[v] = (R)a.u[(int64_t)(v * NN + base) * E + ec];
+                } else if (a.bc == TRIXI_HIP_BC_SLIP_WALL) {
+                    // slip wall: the inner state with its normal momentum reversed
+#pragma unroll
+                    for (int v = 0; v < NV; ++v) um[v] = (R)a.u[(int64_t)(v * NN + base) * E + ec];
+                    um[1 + d] = -um[1 + d];
                 } else {
                     boundary_state<DIM, NN, NV, R>(a, ec, base, tnow, um);
                 }
@@ -393,6 +398,10 @@ rhs_kernel(const RhsArgs a) {
                 if ((a.outflow >> (2 * d + 1)) & 1) {
 #pragma unroll
                     for (int v = 0; v < NV; ++v) up[v] = (R)a.u[(int64_t)(v * NN + base + (N - 1) * s) * E + ec];
+                } else if (a.bc == TRIXI_HIP_BC_SLIP_WALL) {
+#pragma unroll
+                    for (int v = 0; v < NV; ++v) up[v] = (R)a.u[(int64_t)(v * NN + base + (N - 1) * s) * E + ec];
+                    up[1 + d] = -up[1 + d];
                 } else {
                     boundary_state<DIM, NN, NV, R>(a, ec, base + (N - 1) * s, tnow, up);
                 }
@@ -491,7 +500,7 @@ rhs_kernel(const RhsArgs a) {
 #pragma unroll
                     for (int j = 0; j < N; ++j) {
                         R f[NV];
-                        EQ::template noncons<false>(nd[i], nd[j], d, P, f);
+                        EQ::template noncons<-1>(nd[i], nd[j], d, P, f);
                         const R w = (R)a.Ds[i * N + j];
 #pragma unroll
                         for (int v = 0; v < NV; ++v) integral[v] += w * f[v];
@@ -512,8 +521,8 @@ rhs_kernel(const RhsArgs a) {
                         // subcell interface with nonconservative terms: the left
                         // node takes f + nc(u_l, u_r)/2, the right f + nc(u_r, u_l)/2
                         R ncl[NV], ncr[NV];
-                        EQ::template noncons<true>(nd[p], nd[p + 1], d, P, ncl);
-                        EQ::template noncons<true>(nd[p + 1], nd[p], d, P, ncr);
+                        EQ::template noncons<FVF>(nd[p], nd[p + 1], d, P, ncl);
+                        EQ::template noncons<FVF>(nd[p + 1], nd[p], d, P, ncr);
 #pragma unroll
                         for (int v = 0; v < NV; ++v) {
                             acl[p][v] += wl * (F[v] + R(0.5) * ncl[v]);
@@ -539,8 +548,8 @@ rhs_kernel(const RhsArgs a) {
             EQ::template two_point<SF>(nd[N - 1], ndp, d, P, fp);   // this element is "left"
             if constexpr (EQ::NONCONS) {
                 R ncm[NV], ncp[NV];
-                EQ::template noncons<true>(nd[0], ndm, d, P, ncm);
-                EQ::template noncons<true>(nd[N - 1], ndp, d, P, ncp);
+                EQ::template noncons<SF>(nd[0], ndm, d, P, ncm);
+                EQ::template noncons<SF>(nd[N - 1], ndp, d, P, ncp);
 #pragma unroll
                 for (int v = 0; v < NV; ++v) {
                     fm[v] = fm[v] + R(0.5) * ncm[v];

@@ -334,6 +334,11 @@ constexpr int ind_epb() {
     return ind_block<DIM, N>() / cpow(N, DIM);
 }
 
+// internal indicator-variable code of shallow water's waterheight_pressure
+// (its configuration value is density_pressure, the same position in Trixi's
+// API)
+constexpr int IND_WATERHEIGHT_PRESSURE = 3;
+
 template <int DIM, int N>
 __device__ __forceinline__ int max_mode(int n) {
     int maxc = 0;
@@ -356,7 +361,11 @@ __device__ __forceinline__ double modal_indicator(const double* __restrict__ u, 
     // compressible Euler (DIM + 2 variables) or a scalar equation (nv = 1:
     // the variable is the solution itself)
     double m;
-    if (nv == 1) {
+    if (var == IND_WATERHEIGHT_PRESSURE) {
+        // shallow water: waterheight_pressure = h g h^2 / 2
+        const double h = u[(int64_t)n * E + ec];
+        m = h * (0.5 * P.grav * h * h);
+    } else if (nv == 1) {
         m = u[(int64_t)n * E + ec];
     } else if (nv == DIM + 2) {
         double uc[DIM + 2];
@@ -894,7 +903,8 @@ static const double CK_C[5] = {0.0, 1432997174477.0 / 9575080441755.0, 252626934
 static int nvars_of(const trixi_hip_config& c) {
     if (c.equation == TRIXI_HIP_LINEAR_SCALAR_ADVECTION || c.equation == TRIXI_HIP_INVISCID_BURGERS) return 1;
     if (c.equation == TRIXI_HIP_COMPRESSIBLE_EULER_MULTICOMPONENT) return c.ndims + 1 + c.ncomponents;
-    if (c.equation == TRIXI_HIP_COMPRESSIBLE_EULER_QUASI1D || c.equation == TRIXI_HIP_SHALLOW_WATER) return 4;
+    if (c.equation == TRIXI_HIP_COMPRESSIBLE_EULER_QUASI1D) return 4;
+    if (c.equation == TRIXI_HIP_SHALLOW_WATER) return c.ndims + 2;
     if (c.equation == TRIXI_HIP_COMPRESSIBLE_EULER) return c.ndims + 2;
     if (c.equation == TRIXI_HIP_HYPERBOLIC_DIFFUSION) return c.ndims + 1;
     return c.ndims == 1 ? 8 : 9;  // ideal MHD: no cleaning field in 1D
@@ -935,11 +945,12 @@ static int launch_indicator(trixi_hip_ctx* c, const double* u, hipStream_t s) {
     const double parameter_s = log((1.0 - 0.0001) / 0.0001);
     const PhysParams& P = c->args.phys;
     if (c->N != 4 && c->N != 5) return fail(TRIXI_HIP_EUNSUPPORTED, "shock capturing is built for polydeg 3 and 4");
+    const int var = c->cfg.equation == TRIXI_HIP_SHALLOW_WATER ? IND_WATERHEIGHT_PRESSURE : c->cfg.indicator_variable;
 #define TRIXI_HIP_IND(D, NQ)                                                                                  \
     hipLaunchKernelGGL((indicator_kernel<D, NQ>),                                                             \
                        dim3((unsigned)((c->E + ind_epb<D, NQ>() - 1) / ind_epb<D, NQ>())),                    \
                        dim3(ind_block<D, NQ>()), 0, s, u, c->d_alpha_raw, c->E, P, c->d_Vinv, threshold,     \
-                       parameter_s, c->cfg.alpha_min, c->cfg.alpha_max, c->cfg.indicator_variable, c->nv)
+                       parameter_s, c->cfg.alpha_min, c->cfg.alpha_max, var, c->nv)
     if (c->N == 4) {
         if (c->dim == 1) TRIXI_HIP_IND(1, 4); else if (c->dim == 2) TRIXI_HIP_IND(2, 4); else TRIXI_HIP_IND(3, 4);
     } else {
@@ -1059,9 +1070,9 @@ int trixi_hip_create(const trixi_hip_config* config, const trixi_hip_mesh* mesh,
     if (cfg.equation < TRIXI_HIP_LINEAR_SCALAR_ADVECTION || cfg.equation > TRIXI_HIP_SHALLOW_WATER)
         return fail(TRIXI_HIP_EINVAL, "unknown equation");
     if (cfg.equation == TRIXI_HIP_SHALLOW_WATER) {
-        if (cfg.ndims != 2) return fail(TRIXI_HIP_EUNSUPPORTED, "shallow water is built for 2D");
+        if (cfg.ndims > 2) return fail(TRIXI_HIP_EUNSUPPORTED, "shallow water is built for 1D and 2D");
         if (!(cfg.gravity > 0.0)) return fail(TRIXI_HIP_EINVAL, "shallow water needs gravity > 0");
-        if (cfg.boundary_condition != TRIXI_HIP_BC_PERIODIC &&
+        if (cfg.boundary_condition == TRIXI_HIP_BC_DIRICHLET &&
             cfg.boundary_ic != TRIXI_HIP_IC_CONVERGENCE_TEST && cfg.boundary_ic != TRIXI_HIP_IC_CONSTANT)
             return fail(TRIXI_HIP_EUNSUPPORTED, "shallow water Dirichlet states: the convergence test or constant");
         if (cfg.boundary_outflow_faces != 0) return fail(TRIXI_HIP_EUNSUPPORTED, "no outflow sides for shallow water");
@@ -1086,8 +1097,13 @@ int trixi_hip_create(const trixi_hip_config* config, const trixi_hip_mesh* mesh,
             return fail(TRIXI_HIP_EUNSUPPORTED, "multicomponent Euler is built for periodic meshes without sources");
     }
     if (cfg.volume_integral == TRIXI_HIP_VOLUME_SHOCK_CAPTURING_HG && cfg.equation != TRIXI_HIP_COMPRESSIBLE_EULER &&
-        cfg.equation != TRIXI_HIP_INVISCID_BURGERS && cfg.equation != TRIXI_HIP_IDEAL_GLM_MHD)
-        return fail(TRIXI_HIP_EUNSUPPORTED, "shock capturing is built for compressible Euler, Burgers and GLM-MHD");
+        cfg.equation != TRIXI_HIP_INVISCID_BURGERS && cfg.equation != TRIXI_HIP_IDEAL_GLM_MHD &&
+        cfg.equation != TRIXI_HIP_SHALLOW_WATER)
+        return fail(TRIXI_HIP_EUNSUPPORTED,
+                    "shock capturing is built for compressible Euler, Burgers, GLM-MHD and shallow water");
+    if (cfg.volume_integral == TRIXI_HIP_VOLUME_SHOCK_CAPTURING_HG && cfg.equation == TRIXI_HIP_SHALLOW_WATER &&
+        cfg.indicator_variable != TRIXI_HIP_INDICATOR_DENSITY_PRESSURE)
+        return fail(TRIXI_HIP_EUNSUPPORTED, "the shallow-water indicator variable is waterheight_pressure");
     if (cfg.volume_integral == TRIXI_HIP_VOLUME_SHOCK_CAPTURING_HG && cfg.equation == TRIXI_HIP_IDEAL_GLM_MHD &&
         (cfg.indicator_variable != TRIXI_HIP_INDICATOR_DENSITY_PRESSURE || cfg.ndims == 1))
         return fail(TRIXI_HIP_EUNSUPPORTED, "GLM-MHD shock capturing is built for 2D/3D with density_pressure");
@@ -1098,8 +1114,10 @@ int trixi_hip_create(const trixi_hip_config* config, const trixi_hip_mesh* mesh,
         cfg.indicator_variable > TRIXI_HIP_INDICATOR_DENSITY)
         return fail(TRIXI_HIP_EINVAL, "unknown indicator variable");
     if (cfg.boundary_condition != TRIXI_HIP_BC_PERIODIC) {
-        if (cfg.boundary_condition != TRIXI_HIP_BC_DIRICHLET)
+        if (cfg.boundary_condition != TRIXI_HIP_BC_DIRICHLET && cfg.boundary_condition != TRIXI_HIP_BC_SLIP_WALL)
             return fail(TRIXI_HIP_EINVAL, "unknown boundary condition");
+        if (cfg.boundary_condition == TRIXI_HIP_BC_SLIP_WALL && cfg.equation != TRIXI_HIP_SHALLOW_WATER)
+            return fail(TRIXI_HIP_EUNSUPPORTED, "slip walls are built for shallow water");
         const bool swe_ok = cfg.equation == TRIXI_HIP_SHALLOW_WATER;
         const bool euler_ok = cfg.equation == TRIXI_HIP_COMPRESSIBLE_EULER &&
                               (cfg.boundary_ic == TRIXI_HIP_IC_CONVERGENCE_TEST || cfg.boundary_ic == TRIXI_HIP_IC_CONSTANT);
@@ -1582,7 +1600,7 @@ int trixi_hip_max_dt(trixi_hip_ctx* ctx, const double* u, double* max_dt, trixi_
     } else if (eq == TRIXI_HIP_COMPRESSIBLE_EULER_QUASI1D) {
         MS(EulerQuasi1D, 1);
     } else if (eq == TRIXI_HIP_SHALLOW_WATER) {
-        MS(ShallowWater2D, 2);
+        if (dim == 1) MS(ShallowWater<1>, 1); else MS(ShallowWater<2>, 2);
     } else if (eq == TRIXI_HIP_COMPRESSIBLE_EULER_MULTICOMPONENT) {
         using M11 = EulerMulti<1, 1>;
         using M12 = EulerMulti<1, 2>;
@@ -1662,7 +1680,9 @@ int trixi_hip_indicator(trixi_hip_ctx* ctx, const double* u, double* alpha, trix
                        (cfg.equation == TRIXI_HIP_IDEAL_GLM_MHD && cfg.ndims > 1 &&
                         cfg.indicator_variable == TRIXI_HIP_INDICATOR_DENSITY_PRESSURE) ||
                        (cfg.equation == TRIXI_HIP_INVISCID_BURGERS &&
-                        cfg.indicator_variable == TRIXI_HIP_INDICATOR_DENSITY);
+                        cfg.indicator_variable == TRIXI_HIP_INDICATOR_DENSITY) ||
+                       (cfg.equation == TRIXI_HIP_SHALLOW_WATER &&
+                        cfg.indicator_variable == TRIXI_HIP_INDICATOR_DENSITY_PRESSURE);
     if (!built)
         return fail(TRIXI_HIP_EUNSUPPORTED,
                     "the HG indicator is built for compressible Euler, 2D/3D GLM-MHD (density_pressure) and "

@@ -30,8 +30,10 @@ class AbstractEquations:
             return self.ndims + 2
         if self.equation_id == L.EQ_EULER_MULTI:
             return self.ndims + 1 + len(self.gammas)
-        if self.equation_id in (L.EQ_EULER_QUASI1D, L.EQ_SHALLOW_WATER):
+        if self.equation_id == L.EQ_EULER_QUASI1D:
             return 4
+        if self.equation_id == L.EQ_SHALLOW_WATER:
+            return self.ndims + 2
         return 9 if self.ndims > 1 else 8  # ideal MHD: no cleaning field in 1D
 
 
@@ -120,18 +122,26 @@ class CompressibleEulerEquationsQuasi1D(AbstractEquations):
         object.__setattr__(self, "gamma", float(gamma))
 
 
-@dataclass(frozen=True)
-class ShallowWaterEquations2D(AbstractEquations):
-    """(h, h v1, h v2, b) with the bottom topography b (Trixi
-    ShallowWaterEquations2D(; gravity_constant, H0))."""
-    gravity: float = 9.81
-    H0: float = 0.0
+def _shallow_water(ndims):
+    @dataclass(frozen=True)
+    class _ShallowWater(AbstractEquations):
+        """(h, h v, b) with the bottom topography b (Trixi
+        ShallowWaterEquations{1,2}D(; gravity_constant, H0))."""
+        gravity: float = 9.81
+        H0: float = 0.0
 
-    def __init__(self, *, gravity_constant, H0=0.0):
-        object.__setattr__(self, "ndims", 2)
-        object.__setattr__(self, "equation_id", L.EQ_SHALLOW_WATER)
-        object.__setattr__(self, "gravity", float(gravity_constant))
-        object.__setattr__(self, "H0", float(H0))
+        def __init__(self, *, gravity_constant, H0=0.0):
+            object.__setattr__(self, "ndims", ndims)
+            object.__setattr__(self, "equation_id", L.EQ_SHALLOW_WATER)
+            object.__setattr__(self, "gravity", float(gravity_constant))
+            object.__setattr__(self, "H0", float(H0))
+
+    _ShallowWater.__name__ = _ShallowWater.__qualname__ = f"ShallowWaterEquations{ndims}D"
+    return _ShallowWater
+
+
+ShallowWaterEquations1D = _shallow_water(1)
+ShallowWaterEquations2D = _shallow_water(2)
 
 
 def prim2cons(prim, equations):
@@ -145,8 +155,8 @@ def prim2cons(prim, equations):
         rho, v, p = prim[0], prim[1:1 + nd], prim[1 + nd]
         return [rho] + [rho * vi for vi in v] + [p / (equations.gamma - 1) + 0.5 * rho * sum(vi * vi for vi in v)]
     if equations.equation_id == L.EQ_SHALLOW_WATER:
-        H, v1, v2, b = prim
-        return [H - b, (H - b) * v1, (H - b) * v2, b]
+        H, v, b = prim[0], prim[1:-1], prim[-1]
+        return [H - b] + [(H - b) * vi for vi in v] + [b]
     if equations.equation_id == L.EQ_EULER_QUASI1D:
         rho, v, p, a = prim
         return [a * rho, a * rho * v, a * (p / (equations.gamma - 1) + 0.5 * rho * v * v), a]
@@ -275,6 +285,22 @@ flux_wintermeyer_etal = NumericalFlux("flux_wintermeyer_etal", L.FLUX_WINTERMEYE
 flux_fjordholm_etal = NumericalFlux("flux_fjordholm_etal", L.FLUX_FJORDHOLM_ETAL)
 flux_nonconservative_wintermeyer_etal = NumericalFlux("flux_nonconservative_wintermeyer_etal", -3)
 flux_nonconservative_fjordholm_etal = NumericalFlux("flux_nonconservative_fjordholm_etal", -4)
+flux_nonconservative_audusse_etal = NumericalFlux("flux_nonconservative_audusse_etal", -5)
+
+
+def hydrostatic_reconstruction_audusse_etal(*args):  # identity marker of Trixi's reconstruction
+    raise TypeError("reconstructions are selected, not called, on the HIP path")
+
+
+def FluxHydrostaticReconstruction(numerical_flux, hydrostatic_reconstruction):
+    """LLF of Audusse's hydrostatically reconstructed states (shallow water),
+    paired with flux_nonconservative_audusse_etal."""
+    if hydrostatic_reconstruction is not hydrostatic_reconstruction_audusse_etal or \
+            getattr(numerical_flux, "flux_id", None) != L.FLUX_LLF:
+        raise NotImplementedError("FluxHydrostaticReconstruction is built for "
+                                  "(flux_lax_friedrichs, hydrostatic_reconstruction_audusse_etal)")
+    return NumericalFlux("FluxHydrostaticReconstruction(flux_lax_friedrichs, "
+                         "hydrostatic_reconstruction_audusse_etal)", L.FLUX_HYDROSTATIC_LLF)
 
 
 def density_pressure(*args):  # identity markers of the HG indicator variables
@@ -372,6 +398,21 @@ class _PoissonNonperiodic:
 boundary_condition_poisson_nonperiodic = _PoissonNonperiodic()
 
 
+class _SlipWall:
+    """Trixi's boundary_condition_slip_wall (shallow water): the inner state
+    with its normal momentum reversed outside the face."""
+
+    def __repr__(self):
+        return "boundary_condition_slip_wall"
+
+
+boundary_condition_slip_wall = _SlipWall()
+
+
+def waterheight_pressure(*args):  # identity marker of the shallow-water HG indicator variable
+    raise TypeError("indicator variables are selected, not called, on the HIP path")
+
+
 class BoundaryConditionDirichlet:
     """Trixi's BoundaryConditionDirichlet(boundary_value_function): the state
     boundary_value_function(x, t) outside the face; flux(u_inner, u_b) on +
@@ -417,7 +458,8 @@ def split_flux(flux):
     if isinstance(flux, tuple):
         if len(flux) != 2 or flux[1] not in (flux_nonconservative_powell, flux_nonconservative_chan_etal,
                                              flux_nonconservative_wintermeyer_etal,
-                                             flux_nonconservative_fjordholm_etal):
+                                             flux_nonconservative_fjordholm_etal,
+                                             flux_nonconservative_audusse_etal):
             raise NotImplementedError(f"unsupported flux tuple {flux}")
         return flux[0], flux[1]
     return flux, None

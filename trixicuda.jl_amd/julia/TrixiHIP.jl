@@ -88,7 +88,7 @@ equation_id(::Union{HyperbolicDiffusionEquations1D, HyperbolicDiffusionEquations
 equation_id(::Union{CompressibleEulerMulticomponentEquations1D,
                     CompressibleEulerMulticomponentEquations2D}) = 5
 equation_id(::CompressibleEulerEquationsQuasi1D) = 6
-equation_id(::ShallowWaterEquations2D) = 7
+equation_id(::Union{ShallowWaterEquations1D, ShallowWaterEquations2D}) = 7
 equation_id(eq) = throw(ArgumentError("equation $(typeof(eq)) is not built for HIP"))
 
 flux_id(::typeof(flux_central)) = 0
@@ -103,8 +103,12 @@ flux_id(f::FluxHLL) = f.min_max_speed === min_max_speed_einfeldt ? 9 :
 flux_id(::typeof(flux_chan_etal)) = 8
 flux_id(::typeof(flux_wintermeyer_etal)) = 10
 flux_id(::typeof(flux_fjordholm_etal)) = 11
+flux_id(f::FluxHydrostaticReconstruction) =
+    f.numerical_flux === flux_lax_friedrichs && f.hydrostatic_reconstruction === hydrostatic_reconstruction_audusse_etal ?
+    12 : throw(ArgumentError("FluxHydrostaticReconstruction is built for (flux_lax_friedrichs, audusse_etal)"))
 flux_id(f::Tuple) = (f[2] in (flux_nonconservative_powell, flux_nonconservative_chan_etal,
-                            flux_nonconservative_wintermeyer_etal, flux_nonconservative_fjordholm_etal) ?
+                            flux_nonconservative_wintermeyer_etal, flux_nonconservative_fjordholm_etal,
+                            flux_nonconservative_audusse_etal) ?
                      flux_id(f[1]) : throw(ArgumentError("unsupported nonconservative flux $(f[2])")))
 flux_id(f) = throw(ArgumentError("flux $f is not built for HIP"))
 
@@ -173,6 +177,7 @@ end
 # boundary_condition_outflow sides (bit f of the outflow mask)
 function boundary_config(bcs)
     bcs === boundary_condition_periodic && return (0, 0, UInt64(0), 0)
+    bcs === boundary_condition_slip_wall && return (2, 0, UInt64(0), 0)  # shallow water
     sides = bcs isa NamedTuple ? collect(values(bcs)) : [bcs]
     outflow = 0
     if bcs isa NamedTuple
@@ -198,6 +203,7 @@ indicator_variable_id(::typeof(density_pressure)) = 0
 indicator_variable_id(::typeof(pressure)) = 1
 indicator_variable_id(::typeof(first)) = 2
 indicator_variable_id(::typeof(density)) = 2
+indicator_variable_id(::typeof(waterheight_pressure)) = 0
 indicator_variable_id(f) = throw(ArgumentError("indicator variable $f is not built for HIP"))
 
 source_id(::Nothing) = 0

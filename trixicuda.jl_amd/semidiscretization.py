@@ -13,6 +13,7 @@ import numpy as np
 
 from . import _lib as L
 from .equations import (BoundaryConditionDirichlet, InitialCondition, SourceTerms, boundary_condition_outflow,
+                        boundary_condition_slip_wall,
                         boundary_condition_periodic, boundary_condition_poisson_nonperiodic,
                         initial_condition_constant, initial_condition_poisson_nonperiodic)
 
@@ -68,6 +69,14 @@ class SemidiscretizationHyperbolicHIP:
         handle = ctypes.c_void_p()
         L.check(L.lib().trixi_hip_create(ctypes.byref(self.config), ctypes.byref(md), ctypes.byref(handle)))
         self._ctx = handle
+        # the library's state size must be the host's: every device vector is
+        # sized here and addressed there
+        nv, nn, nval = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int64()
+        L.check(L.lib().trixi_hip_sizes(handle, ctypes.byref(nv), ctypes.byref(nn), ctypes.byref(nval)))
+        if (nv.value, nn.value, nval.value) != (self.nvariables, self.nnodes, self.nvalues):
+            self.close()
+            raise L.TrixiHipError(L.EINVAL, f"state size mismatch: library ({nv.value}, {nn.value}, {nval.value}), "
+                                            f"host ({self.nvariables}, {self.nnodes}, {self.nvalues})")
 
     def _set_boundaries_and_sources(self, mesh, bcs, source_terms):
         """Trixi's boundary_conditions (one condition, or one per side as a
@@ -94,6 +103,11 @@ class SemidiscretizationHyperbolicHIP:
             sides = [per_side.get(n, boundary_condition_periodic) for n in names]
         else:
             sides = [bcs] * len(names)
+        if any(b is boundary_condition_slip_wall for b in sides):
+            if not all(b is boundary_condition_slip_wall or b is boundary_condition_periodic for b in sides):
+                raise NotImplementedError("slip walls cannot be combined with other boundary conditions")
+            c.boundary_condition = L.BC_SLIP_WALL
+            return
         outflow = 0
         for f_dir, b in enumerate(sides):
             if b is boundary_condition_outflow:

@@ -52,8 +52,9 @@ class VolumeIntegralFluxDifferencing:
 class IndicatorHennemannGassner:
     def __init__(self, equations, basis, alpha_max=0.5, alpha_min=0.001, alpha_smooth=True,
                  variable=density_pressure):
-        if variable not in (density_pressure, pressure, density, first):
-            raise NotImplementedError("the indicator variable must be density_pressure, pressure, density or first")
+        if variable not in (density_pressure, pressure, density, first, E.waterheight_pressure):
+            raise NotImplementedError("the indicator variable must be density_pressure, pressure, density, first "
+                                      "or waterheight_pressure")
         self.alpha_max, self.alpha_min, self.alpha_smooth = float(alpha_max), float(alpha_min), bool(alpha_smooth)
         self.variable = variable
 
@@ -106,7 +107,8 @@ class DGSEMHIP:
         nc_needed = (equations.equation_id == L.EQ_GLM_MHD and equations.ndims > 1) or quasi or swe
         # the nonconservative term each position takes (shallow water: Wintermeyer
         # in the volume, Fjordholm at the surface, the reference tests' pairs)
-        nc_kind = (E.flux_nonconservative_fjordholm_etal if swe else
+        hr = getattr(sflux, "flux_id", None) == L.FLUX_HYDROSTATIC_LLF
+        nc_kind = ((E.flux_nonconservative_audusse_etal if hr else E.flux_nonconservative_fjordholm_etal) if swe else
                    E.flux_nonconservative_chan_etal if quasi else E.flux_nonconservative_powell)
         nc_vol = E.flux_nonconservative_wintermeyer_etal if swe else nc_kind
         if nc_needed != (s_nc is not None) or (s_nc is not None and s_nc is not nc_kind):
@@ -127,12 +129,20 @@ class DGSEMHIP:
             dg_flux, dg_nc = split_flux(vi.volume_flux_dg)
             fv_flux, fv_nc = split_flux(vi.volume_flux_fv if vi.volume_flux_fv is not None else self.surface_flux)
             if nc_needed != (dg_nc is not None) or nc_needed != (fv_nc is not None):
-                raise NotImplementedError("GLM-MHD DG and FV volume fluxes must be (flux, flux_nonconservative_powell)")
+                raise NotImplementedError("equations with nonconservative terms need (flux, nonconservative flux) "
+                                          "pairs as DG and FV volume fluxes")
+            if nc_needed and (dg_nc is not nc_vol or
+                              (swe and fv_nc is not (E.flux_nonconservative_audusse_etal
+                                                     if fv_flux.flux_id == L.FLUX_HYDROSTATIC_LLF
+                                                     else E.flux_nonconservative_fjordholm_etal))):
+                raise NotImplementedError("the DG / FV volume fluxes do not pair with the equation's "
+                                          "nonconservative terms")
             c.volume_flux = dg_flux.flux_id
             c.fv_flux = fv_flux.flux_id
             c.alpha_smooth = 1 if vi.indicator.alpha_smooth else 0
             # first (the solution of a scalar equation) and density both take u[0]
-            c.indicator_variable = {density_pressure: 0, pressure: 1, density: 2, first: 2}[vi.indicator.variable]
+            c.indicator_variable = {density_pressure: 0, pressure: 1, density: 2, first: 2,
+                                    E.waterheight_pressure: 0}[vi.indicator.variable]
             c.alpha_max, c.alpha_min = vi.indicator.alpha_max, vi.indicator.alpha_min
         c.precision = L.PREC_MIXED if self.precision == "mixed" else L.PREC_FP64
         c.gamma = getattr(equations, "gamma", 1.4)
