@@ -1,7 +1,9 @@
 // physics.hpp -- device-side equations and numerical fluxes for the gfx950
 // DGSEM kernels.  Everything is templated on the arithmetic type R (double in
-// the FP64 mode, float in the mixed mode); the logarithms inside the
-// entropy-conservative means, and their differences, are always FP64.
+// the FP64 mode, float in the mixed mode), the per-node logarithms inside
+// the entropy-conservative means included: the closed-form ln-mean is taken
+// only where |log y - log x| > 0.02, so FP32 logarithms add at most about
+// 5e-6 relative error to a mean in the mixed mode.
 //
 // The flux algebra follows the Trixi.jl definitions that TrixiCUDA.jl's
 // kernels inline (reference: fluxes passed in by the tests, e.g.
@@ -124,14 +126,12 @@ template <typename R> __device__ __forceinline__ R dmax(R a, R b) { return b > a
 //            inv_ln_mean = P(f2) / (x + y);
 //   closed:  ln_mean     = (y - x) / (log y - log x), inv_ln_mean = 1 / that.
 //   The one reciprocal is of (x + y) in the series branch and of the
-//   closed-branch denominator otherwise.  The logarithms and their
-//   difference are always FP64 (|log(y/x)| > 0.02 in the closed branch, so
-//   the difference of FP64 logarithms is accurate to a few 1e-14); the
-//   remaining arithmetic runs in R (FP32 in the mixed mode, where the
-//   result is consumed in FP32 anyway).
+//   closed-branch denominator otherwise.  The logarithms are taken in R:
+//   |log(y/x)| > 0.02 in the closed branch, so their difference is accurate
+//   to a few 1e-14 in FP64 and a few 1e-6 in FP32 (mixed mode).
 // ---------------------------------------------------------------------------
 template <typename R>
-__device__ __forceinline__ R ln_mean_logs(R x, R y, double lx, double ly) {
+__device__ __forceinline__ R ln_mean_logs(R x, R y, R lx, R ly) {
     const R dxy = y - x, s = x + y;
     const bool series = dxy * dxy < R(1.0e-4) * (s * s);
     const R r = rrcp(series ? s : (R)(ly - lx));
@@ -141,7 +141,7 @@ __device__ __forceinline__ R ln_mean_logs(R x, R y, double lx, double ly) {
 }
 
 template <typename R>
-__device__ __forceinline__ R inv_ln_mean_logs(R x, R y, double lx, double ly) {
+__device__ __forceinline__ R inv_ln_mean_logs(R x, R y, R lx, R ly) {
     const R dxy = y - x, s = x + y;
     const bool series = dxy * dxy < R(1.0e-4) * (s * s);
     const R dl = (R)(ly - lx);
@@ -162,7 +162,7 @@ struct LinAdv {
     template <typename R>
     struct Node {
         R u;
-        double lrho, lp;  // unused
+        R lrho, lp;  // unused
         __device__ R cons(int) const { return u; }
     };
     template <typename R, bool LOGS>
@@ -177,7 +177,7 @@ struct LinAdv {
         f[0] = n.u;
     }
     template <typename R>
-    __device__ static Node<R> from_rec(const R* f, double, double, const PhysR<R>&) {
+    __device__ static Node<R> from_rec(const R* f, R, R, const PhysR<R>&) {
         Node<R> n;
         n.u = f[0];
         n.lrho = n.lp = 0.0;
@@ -218,7 +218,7 @@ struct HypDiff {
     template <typename R>
     struct Node {
         R u[DIM + 1];
-        double lrho, lp;  // unused
+        R lrho, lp;  // unused
         __device__ R cons(int q) const { return u[q]; }
     };
     template <typename R, bool LOGS>
@@ -235,7 +235,7 @@ struct HypDiff {
         for (int q = 0; q < NV; ++q) f[q] = n.u[q];
     }
     template <typename R>
-    __device__ static Node<R> from_rec(const R* f, double, double, const PhysR<R>&) {
+    __device__ static Node<R> from_rec(const R* f, R, R, const PhysR<R>&) {
         Node<R> n;
 #pragma unroll
         for (int q = 0; q < NV; ++q) n.u[q] = f[q];
@@ -282,7 +282,7 @@ struct Burgers {
     template <typename R>
     struct Node {
         R u;
-        double lrho, lp;  // unused
+        R lrho, lp;  // unused
         __device__ R cons(int) const { return u; }
     };
     template <typename R, bool LOGS>
@@ -297,7 +297,7 @@ struct Burgers {
         f[0] = n.u;
     }
     template <typename R>
-    __device__ static Node<R> from_rec(const R* f, double, double, const PhysR<R>&) {
+    __device__ static Node<R> from_rec(const R* f, R, R, const PhysR<R>&) {
         Node<R> n;
         n.u = f[0];
         n.lrho = n.lp = 0.0;
@@ -338,7 +338,7 @@ struct Euler {
     template <typename R>
     struct Node {
         R rho, v[DIM], p, c;  // primitives and sound speed
-        double lrho, lp;      // log(rho), log(p) for the entropy-conservative means
+        R lrho, lp;      // log(rho), log(p) for the entropy-conservative means
         R inv_gm1;
         __device__ R cons(int q) const {
             if (q == 0) return rho;
@@ -365,8 +365,8 @@ struct Euler {
         n.c = rsqrt_(P.gamma * n.p * inv_rho);
         n.inv_gm1 = P.inv_gm1;
         if (LOGS) {
-            n.lrho = log((double)n.rho);
-            n.lp = log((double)n.p);
+            n.lrho = log(n.rho);
+            n.lp = log(n.p);
         } else {
             n.lrho = n.lp = 0.0;
         }
@@ -381,7 +381,7 @@ struct Euler {
         f[DIM + 2] = n.c;
     }
     template <typename R>
-    __device__ static Node<R> from_rec(const R* f, double lrho, double lp, const PhysR<R>& P) {
+    __device__ static Node<R> from_rec(const R* f, R lrho, R lp, const PhysR<R>& P) {
         Node<R> n;
         n.rho = f[0];
 #pragma unroll
@@ -538,7 +538,7 @@ struct EulerMulti {
     struct Node {
         R rk[NC], v[DIM], p, c;
         R rho, inv_gm1;   // derived: total density, 1 / (gamma - 1)
-        double lrho, lp;  // log(rho), log(p)
+        R lrho, lp;  // log(rho), log(p)
         __device__ R cons(int q) const {
             if (q < DIM) return rho * v[q];
             if (q == DIM) {
@@ -585,8 +585,8 @@ struct EulerMulti {
         n.c = rsqrt_(gamma * n.p * inv_rho);
         n.inv_gm1 = rdiv(R(1), gamma - R(1));
         if (LOGS) {
-            n.lrho = log((double)n.rho);
-            n.lp = log((double)n.p);
+            n.lrho = log(n.rho);
+            n.lp = log(n.p);
         } else {
             n.lrho = n.lp = 0.0;
         }
@@ -602,7 +602,7 @@ struct EulerMulti {
         f[NC + DIM + 1] = n.c;
     }
     template <typename R>
-    __device__ static Node<R> from_rec(const R* f, double lrho, double lp, const PhysR<R>& P) {
+    __device__ static Node<R> from_rec(const R* f, R lrho, R lp, const PhysR<R>& P) {
         Node<R> n;
         R rho = R(0);
 #pragma unroll
@@ -636,7 +636,7 @@ struct EulerMulti {
             const R vo_avg = R(0.5) * (l.v[o] + r.v[o]);
 #pragma unroll
             for (int k = 0; k < NC; ++k) {
-                const R fk = ln_mean_logs<R>(l.rk[k], r.rk[k], log((double)l.rk[k]), log((double)r.rk[k])) * vo_avg;
+                const R fk = ln_mean_logs<R>(l.rk[k], r.rk[k], log(l.rk[k]), log(r.rk[k])) * vo_avg;
                 f[DIM + 1 + k] = fk;
                 f_rho_sum += fk;
                 ravg[k] = R(0.5) * (l.rk[k] + r.rk[k]);
@@ -701,7 +701,7 @@ struct EulerQuasi1D {
     template <typename R>
     struct Node {
         R rho, v[1], p, c, a;
-        double lrho, lp;
+        R lrho, lp;
         R inv_gm1;
         __device__ R cons(int q) const {
             if (q == 0) return a * rho;
@@ -722,8 +722,8 @@ struct EulerQuasi1D {
         n.c = rsqrt_(P.gamma * n.p * rdiv(R(1), n.rho));
         n.inv_gm1 = P.inv_gm1;
         if (LOGS) {
-            n.lrho = log((double)n.rho);
-            n.lp = log((double)n.p);
+            n.lrho = log(n.rho);
+            n.lp = log(n.p);
         } else {
             n.lrho = n.lp = 0.0;
         }
@@ -738,7 +738,7 @@ struct EulerQuasi1D {
         f[4] = n.a;
     }
     template <typename R>
-    __device__ static Node<R> from_rec(const R* f, double lrho, double lp, const PhysR<R>& P) {
+    __device__ static Node<R> from_rec(const R* f, R lrho, R lp, const PhysR<R>& P) {
         Node<R> n;
         n.rho = f[0];
         n.v[0] = f[1];
@@ -820,7 +820,7 @@ struct ShallowWater {
     template <typename R>
     struct Node {
         R h, v[DIM], b;
-        double lrho, lp;  // unused
+        R lrho, lp;  // unused
         __device__ R cons(int q) const {
             if (q == 0) return h;
             if (q <= DIM) return h * v[q - 1];
@@ -846,7 +846,7 @@ struct ShallowWater {
         f[DIM + 1] = n.b;
     }
     template <typename R>
-    __device__ static Node<R> from_rec(const R* f, double, double, const PhysR<R>&) {
+    __device__ static Node<R> from_rec(const R* f, R, R, const PhysR<R>&) {
         Node<R> n;
         n.h = f[0];
 #pragma unroll
@@ -958,7 +958,7 @@ struct GlmMhd {
     template <typename R>
     struct Node {
         R rho, v[3], p, B[3], psi;
-        double lrho, lp;
+        R lrho, lp;
         R inv_gm1;
         __device__ R cons(int q) const {
             if (q == 0) return rho;
@@ -985,8 +985,8 @@ struct GlmMhd {
                        R(0.5) * (u[5] * u[5] + u[6] * u[6] + u[7] * u[7]) - R(0.5) * u[8] * u[8]);
         n.inv_gm1 = P.inv_gm1;
         if (LOGS) {
-            n.lrho = log((double)n.rho);
-            n.lp = log((double)n.p);
+            n.lrho = log(n.rho);
+            n.lp = log(n.p);
         } else {
             n.lrho = n.lp = 0.0;
         }
@@ -1003,7 +1003,7 @@ struct GlmMhd {
         f[8] = n.psi;
     }
     template <typename R>
-    __device__ static Node<R> from_rec(const R* f, double lrho, double lp, const PhysR<R>& P) {
+    __device__ static Node<R> from_rec(const R* f, R lrho, R lp, const PhysR<R>& P) {
         Node<R> n;
         n.rho = f[0];
 #pragma unroll
@@ -1231,7 +1231,7 @@ struct GlmMhd1D {
         GlmMhd::to_rec(n, f);
     }
     template <typename R>
-    __device__ static Node<R> from_rec(const R* f, double lrho, double lp, const PhysR<R>& P) {
+    __device__ static Node<R> from_rec(const R* f, R lrho, R lp, const PhysR<R>& P) {
         return GlmMhd::from_rec(f, lrho, lp, P);
     }
     template <typename R>

@@ -225,7 +225,7 @@ rhs_kernel(const RhsArgs a) {
     // from the accumulator, instead of re-reading u and k line by line.
     constexpr bool COOP = RK && STAGE;
     __shared__ R rec[STAGE ? NRF * NN * LDM : 1];
-    __shared__ double recd[(STAGE && NDF > 0) ? NDF * NN * LDM : 1];
+    __shared__ R recd[(STAGE && NDF > 0) ? NDF * NN * LDM : 1];
     __shared__ R acc[(DIM > 1 || COOP) ? NV * NN * LDM : 1];
 
     const int tid = threadIdx.x;
@@ -425,8 +425,8 @@ rhs_kernel(const RhsArgs a) {
                 R f[NRF];
 #pragma unroll
                 for (int k = 0; k < NRF; ++k) f[k] = rec[(k * NN + node) * LDM + el];
-                const double lr = NODE_LOGS ? recd[node * LDM + el] : 0.0;
-                const double lp = NODE_LOGS ? recd[(NN + node) * LDM + el] : 0.0;
+                const R lr = NODE_LOGS ? recd[node * LDM + el] : R(0);
+                const R lp = NODE_LOGS ? recd[(NN + node) * LDM + el] : R(0);
                 nd[p] = EQ::from_rec(f, lr, lp, P);
             } else {
                 R uc[NV];
