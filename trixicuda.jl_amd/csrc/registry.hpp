@@ -66,8 +66,8 @@ struct VjpRegistrar {
 };
 
 // Workgroup width in elements: fill about 256 lanes and keep the LDS tiles
-// (node records, FP64 logarithms, direction accumulator) within 40 KiB so
-// that several workgroups share a CU.
+// (node records with their two logarithms, direction accumulator) within
+// 40 KiB so that several workgroups share a CU.
 #ifndef TRIXI_HIP_LDS_BUDGET_KIB
 #define TRIXI_HIP_LDS_BUDGET_KIB 40
 #endif
@@ -80,7 +80,7 @@ struct VjpRegistrar {
 template <int DIM, int N, int NV, int NRF, int RS>
 constexpr int pick_m() {
     constexpr int NL = cpow(N, DIM - 1), NN = cpow(N, DIM);
-    constexpr int per_elem = NN * ((TRIXI_HIP_STAGE_NODES ? NRF * RS + 2 * 8 : 0) + (DIM > 1 ? NV * RS : 0));
+    constexpr int per_elem = NN * ((TRIXI_HIP_STAGE_NODES ? (NRF + 2) * RS : 0) + (DIM > 1 ? NV * RS : 0));
     int m = TRIXI_HIP_MAX_THREADS / NL;
     if (m < 1) m = 1;
     while (m > 1 && m * per_elem > TRIXI_HIP_LDS_BUDGET_KIB * 1024) m /= 2;

@@ -55,6 +55,10 @@ constexpr int MAXN = 8;
 #ifndef TRIXI_HIP_PAR_DIRS
 #define TRIXI_HIP_PAR_DIRS 1
 #endif
+// ... also in the mixed (FP32) mode
+#ifndef TRIXI_HIP_PAR_DIRS_F32
+#define TRIXI_HIP_PAR_DIRS_F32 0
+#endif
 
 __host__ __device__ constexpr int cpow(int b, int e) { return e == 0 ? 1 : b * cpow(b, e - 1); }
 
@@ -182,7 +186,7 @@ __device__ __forceinline__ int line_base(int line, int d) {
 template <int DIM, int N, int M, typename R, bool RK, int NV>
 __host__ __device__ constexpr bool rhs_par_dirs() {
     return TRIXI_HIP_PAR_DIRS != 0 && DIM > 1 && (M * cpow(N, DIM - 1)) % 64 == 0 &&
-           std::is_same<R, double>::value && !RK && NV <= 5;
+           (std::is_same<R, double>::value || TRIXI_HIP_PAR_DIRS_F32 != 0) && !RK && NV <= 5;
 }
 template <int DIM, int N, int M, typename R, bool RK, int NV>
 __host__ __device__ constexpr int rhs_threads() {
