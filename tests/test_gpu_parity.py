@@ -594,3 +594,63 @@ def test_unsupported_combination_is_an_error(th, torch):
     with pytest.raises(th.TrixiHipError) as ei:
         product_semi(th, c)
     assert ei.value.code == -2
+
+
+HEADLINE = Case("euler3d_level5_headline", 3, "euler", 5, "fd", "ranocha", "llf", ic="smooth_periodic")
+
+
+# (free stream: absolute bound on du for flux terms of size 3.5 times D
+# entries of size 10 times 1/J = 16, at FP64 and FP32 round-off)
+@pytest.mark.parametrize("precision,tol,cons,fs", [("fp64", 1e-12, 1e-13, 1e-11), ("mixed", 1e-4, 1e-6, 1e-3)])
+def test_headline_configuration_full_size(th, oracle, torch, precision, tol, cons, fs):
+    """The measured configuration at its full size (3D Euler, flux_ranocha +
+    LLF, polydeg 3, level 5: 32768 elements, 2097152 DOF): the whole state
+    against the oracle, conservation of every variable (the integral of du
+    vanishes on the periodic mesh) and free-stream preservation."""
+    O = oracle
+    case = HEADLINE
+    cfg, mesh = oracle_cfg(O, case), oracle_mesh(O, case)
+    u = O.compute_coefficients(cfg, mesh, oracle_ic(O, case), 0.0, case.seed)
+    du_ref = O.rhs(cfg, mesh, u)
+    semi = product_semi(th, case, precision)
+    ud = th.pack(torch.from_numpy(u).cuda(), semi)
+    dud = torch.empty_like(ud)
+    th.rhs_hip_(dud, ud, semi, 0.0)
+    du = th.wrap_array_native(dud, semi).cpu().numpy()
+    err = rel_max(du, du_ref)
+    print(f"headline {precision}: rel max-norm {err:.3e}", flush=True)
+    assert err <= tol, err
+    scale = np.abs(du_ref).max() * mesh.total_volume
+    integral = th.integrate(dud, semi)
+    assert np.abs(integral).max() <= cons * scale, integral
+    uc = np.zeros_like(u)
+    uc[..., 0], uc[..., 4] = 1.0, 1.0 / 0.4
+    th.rhs_hip_(dud, th.pack(torch.from_numpy(uc).cuda(), semi), semi, 0.0)
+    print(f"headline {precision}: free stream max |du| {float(dud.abs().max()):.3e}", flush=True)
+    assert float(dud.abs().max()) <= fs, float(dud.abs().max())
+    semi.close()
+
+
+def test_headline_rk_step_full_size(th, oracle, torch):
+    """One CK2N54 step (hipGraph) of the measured configuration at full size
+    against the oracle's step."""
+    O = oracle
+    case = HEADLINE
+    cfg, mesh = oracle_cfg(O, case), oracle_mesh(O, case)
+    u = O.compute_coefficients(cfg, mesh, oracle_ic(O, case), 0.0, case.seed)
+    dt = 0.5 * O.max_dt(cfg, mesh, u)
+    semi = product_semi(th, case)
+    ud = th.pack(torch.from_numpy(u).cuda(), semi)
+    kd = torch.zeros_like(ud)
+    stream = torch.cuda.Stream()  # graph replay runs on a non-default stream
+    stream.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(stream):
+        th.rk_step(ud, kd, 0.0, dt, semi, use_graph=True, stream=stream)
+    torch.cuda.synchronize()
+    uo, k = u.copy(), np.zeros_like(u)
+    O.ck2n54_step(cfg, mesh, uo, k, 0.0, dt)
+    got = th.wrap_array_native(ud, semi).cpu().numpy()
+    err = rel_max(got - u, uo - u)
+    print(f"headline RK step: rel err of increment {err:.3e}", flush=True)
+    assert err < 1e-11, err
+    semi.close()
