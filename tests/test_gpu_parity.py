@@ -589,11 +589,27 @@ def test_check_state_flags(th, oracle, torch):
     semi.close()
 
 
-def test_unsupported_combination_is_an_error(th, torch):
-    c = Case("mhd_hg", 3, "mhd", 1, "hg", "hindenlang_gassner", cmin=-1.0, cmax=1.0, gamma=5 / 3)
+@pytest.mark.parametrize("case", [
+    Case("mhd_hg_llf_3d", 3, "mhd", 1, "hg", "hindenlang_gassner", cmin=-1.0, cmax=1.0, gamma=5 / 3),
+    # three gas components (the library builds one or two)
+    Case("eulermulti_3comp", 1, "eulermulti", 2, "fd", "ranocha", "ranocha", gammas=(1.4,) * 3,
+         gas_constants=(0.4,) * 3),
+    # slip walls are built for shallow water only
+    Case("euler_slip_wall", 2, "euler", 2, periodic=False, bc_state="slip_wall"),
+    # multicomponent Euler has no sources
+    Case("eulermulti_source", 2, "eulermulti", 2, "fd", "ranocha", "llf", source="convergence_test"),
+    # quasi-1D Euler needs flux differencing
+    Case("quasi1d_weak", 1, "quasi1d", 2, "weak", "central", "chan_etal", polydeg=4),
+    # the shallow-water indicator variable is waterheight_pressure
+    Case("swe1d_hg_pressure", 1, "swe", 2, "hg", "wintermeyer_etal", "hydrostatic_llf", "hydrostatic_llf",
+         polydeg=4, gravity=9.812, indicator="pressure"),
+], ids=lambda c: c.name)
+def test_unsupported_combination_is_an_error(th, torch, case):
+    """Configurations outside what is built fail in create with
+    EUNSUPPORTED, naming the reason, and allocate nothing on the device."""
     with pytest.raises(th.TrixiHipError) as ei:
-        product_semi(th, c)
-    assert ei.value.code == -2
+        product_semi(th, case)
+    assert ei.value.code == -2, ei.value
 
 
 HEADLINE = Case("euler3d_level5_headline", 3, "euler", 5, "fd", "ranocha", "llf", ic="smooth_periodic")
