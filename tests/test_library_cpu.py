@@ -231,3 +231,56 @@ def test_shallow_water_1d_configuration(th):
         th.DGSEMHIP(polydeg=4, surface_flux=(hr[0], th.flux_nonconservative_fjordholm_etal),
                     volume_integral=th.VolumeIntegralFluxDifferencing(vol)).config(eq)
     assert th.prim2cons((1.75, 1.0, 0.5), eq) == pytest.approx([1.25, 1.25, 0.5])
+
+
+def test_abi_struct_layout_matches_the_header(th, tmp_path):
+    """The ctypes mirrors of trixi_hip_config and trixi_hip_mesh have the C
+    header's size and field offsets (a C probe compiled here with gcc)."""
+    import ctypes
+    import shutil
+    import subprocess
+    L = th._lib
+    gcc = shutil.which("gcc") or shutil.which("cc")
+    if gcc is None:
+        pytest.skip("no C compiler")
+    include = os.path.join(ROOT, "include")
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "trixi_hip.h"', 'int main(void) {']
+    for cname, cls in (("trixi_hip_config", L.Config), ("trixi_hip_mesh", L.MeshDesc)):
+        lines.append(f'printf("{cname} size %zu\\n", sizeof({cname}));')
+        for fname, _ in cls._fields_:
+            lines.append(f'printf("{cname} {fname} %zu\\n", offsetof({cname}, {fname}));')
+    lines.append("return 0; }")
+    src = tmp_path / "probe.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "probe"
+    subprocess.run([gcc, "-I", include, str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    got = {tuple(l.split()[:2]): int(l.split()[2]) for l in out if l}
+    for cname, cls in (("trixi_hip_config", L.Config), ("trixi_hip_mesh", L.MeshDesc)):
+        assert got[(cname, "size")] == ctypes.sizeof(cls), cname
+        for fname, _ in cls._fields_:
+            assert got[(cname, fname)] == getattr(cls, fname).offset, (cname, fname)
+
+
+def test_oracle_cfg_layout_matches_its_header(oracle, tmp_path):
+    """The oracle's ctypes OracleCfg has oracle_cfg's size and field offsets."""
+    import ctypes
+    import shutil
+    import subprocess
+    gcc = shutil.which("gcc") or shutil.which("cc")
+    if gcc is None:
+        pytest.skip("no C compiler")
+    cls = oracle.OracleCfg
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "trixi_oracle.h"', 'int main(void) {',
+             'printf("size %zu\\n", sizeof(oracle_cfg));']
+    lines += [f'printf("{f} %zu\\n", offsetof(oracle_cfg, {f}));' for f, _ in cls._fields_]
+    lines.append("return 0; }")
+    src = tmp_path / "probe.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "probe"
+    subprocess.run([gcc, "-I", os.path.join(ROOT, "oracle"), str(src), "-o", str(exe)], check=True)
+    got = dict((l.split()[0], int(l.split()[1]))
+               for l in subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n") if l)
+    assert got["size"] == ctypes.sizeof(cls)
+    for f, _ in cls._fields_:
+        assert got[f] == getattr(cls, f).offset, f
