@@ -670,3 +670,32 @@ def test_headline_rk_step_full_size(th, oracle, torch):
     print(f"headline RK step: rel err of increment {err:.3e}", flush=True)
     assert err < 1e-11, err
     semi.close()
+
+
+def test_save_solution_callback_files_match_device_state(th, torch, tmp_path):
+    """SaveSolutionCallback during a 2D Euler shock-capturing solve: the
+    initial, every-2nd-step and final files hold cons2prim of the unpacked
+    device state bit for bit, with the blending factors as element variable."""
+    import os
+    if not os.path.exists(th.save_solution.H5_LIBRARY_PATH):
+        pytest.skip("libtrixih5.so not built (no HDF5 library)")
+    from test_save_solution import H5DUMP, h5_dataset
+    if H5DUMP is None:
+        pytest.skip("h5dump absent")
+    case = Case("euler2d_hg_save", 2, "euler", 3, "hg", "ranocha", "llf", "llf", ic="weak_blast_wave")
+    semi = product_semi(th, case)
+    u0 = th.compute_coefficients(0.0, semi)
+    save = th.SaveSolutionCallback(interval=2, output_directory=str(tmp_path), solution_variables=th.cons2prim)
+    ode = th.semidiscretizeHIP(semi, (0.0, 0.05))
+    sol = th.solve(ode, th.CarpenterKennedy2N54(williamson_condition=False),
+                   callback=[th.StepsizeCallback(cfl=0.5), save])
+    names = [os.path.basename(f) for f in save.files]
+    steps = sorted(set([0] + list(range(2, sol.nsteps + 1, 2)) + [sol.nsteps]))
+    assert names == ["solution_%09d.h5" % s for s in steps], (names, sol.nsteps)
+    for path, u in ((save.files[0], u0), (save.files[-1], sol.u)):
+        prim = th.cons2prim(th.wrap_array_native(u, semi).cpu().numpy(), semi.equations)
+        for v in range(4):
+            np.testing.assert_array_equal(h5_dataset(path, f"/variables_{v + 1}", tmp_path), prim[..., v].reshape(-1))
+        alpha = th.blending_factors(u, semi).cpu().numpy()
+        np.testing.assert_array_equal(h5_dataset(path, "/element_variables_1", tmp_path), alpha)
+    semi.close()

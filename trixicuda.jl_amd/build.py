@@ -60,9 +60,31 @@ def build(jobs=None, verbose=False, out=LIB, build_dir=BUILD, defines=()):
     return out
 
 
+H5_INCLUDE = os.environ.get("TRIXI_H5_INCLUDE", "/opt/conda/include")
+H5_LIBDIR = os.environ.get("TRIXI_H5_LIBDIR", "/opt/conda/lib")
+H5_LIB = os.path.join(HERE, "libtrixih5.so")
+
+
+def build_h5(out=H5_LIB):
+    """Host-only HDF5 solution writer (csrc/trixi_h5.cpp) against the HDF5 C
+    library; returns None, building nothing, when its header is absent."""
+    if not os.path.exists(os.path.join(H5_INCLUDE, "hdf5.h")):
+        return None
+    tmp = out + ".tmp"
+    cmd = ["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall", f"-I{H5_INCLUDE}",
+           os.path.join(CSRC, "trixi_h5.cpp"), "-o", tmp, f"-L{H5_LIBDIR}", "-lhdf5",
+           f"-Wl,-rpath,{H5_LIBDIR}"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("g++ failed for trixi_h5.cpp:\n" + r.stdout + r.stderr)
+    os.replace(tmp, out)
+    return out
+
+
 if __name__ == "__main__":
     ap = argparse.ArgumentParser()
     ap.add_argument("--jobs", type=int, default=None)
     ap.add_argument("-v", "--verbose", action="store_true")
     a = ap.parse_args()
     print(build(a.jobs, a.verbose))
+    print(build_h5())

@@ -7,6 +7,7 @@ five fused rhs+register-update launches, replayed from a hipGraph.
 """
 from . import _lib as L
 from .callbacks import AnalysisCallback, StepsizeCallback
+from .save_solution import SaveSolutionCallback
 from .semidiscretization import _ptr
 
 
@@ -65,6 +66,7 @@ def solve(ode, alg=None, dt=None, callback=(), use_graph=True, maxiters=10**7):
     cbs = list(callback) if isinstance(callback, (list, tuple)) else [callback]
     stepsize = next((c for c in cbs if isinstance(c, StepsizeCallback)), None)
     analysis = [c for c in cbs if isinstance(c, AnalysisCallback)]
+    saves = [c for c in cbs if isinstance(c, SaveSolutionCallback)]
     if dt is None and stepsize is None:
         raise ValueError("give dt or a StepsizeCallback")
     t0, t1 = ode.tspan
@@ -72,8 +74,10 @@ def solve(ode, alg=None, dt=None, callback=(), use_graph=True, maxiters=10**7):
     k = torch.zeros_like(u)
     stream = torch.cuda.Stream()
     stream.wait_stream(torch.cuda.current_stream())
-    t, nsteps = t0, 0
+    t, nsteps, h = t0, 0, 0.0
     with torch.cuda.stream(stream):
+        for c in saves:
+            c.initialize(u, t, stepsize.dt(u, semi) if stepsize is not None else dt, semi)
         while t < t1 and nsteps < maxiters:
             h = stepsize.dt(u, semi) if stepsize is not None else dt
             if t + h > t1:
@@ -83,8 +87,12 @@ def solve(ode, alg=None, dt=None, callback=(), use_graph=True, maxiters=10**7):
             nsteps += 1
             for a in analysis:
                 a(u, t, nsteps)
+            for c in saves:
+                c(u, t, h, nsteps, semi)
         for a in analysis:
             a(u, t, nsteps, final=True)
+        for c in saves:
+            c(u, t, h, nsteps, semi, final=True)
     torch.cuda.current_stream().wait_stream(stream)
     stream.synchronize()
     return Solution(u, t, nsteps, semi)
