@@ -318,6 +318,21 @@ function compute_coefficients_hip(t, semi::SemidiscretizationHyperbolicHIP)
 end
 
 "semidiscretizeHIP(semi, tspan): ODEProblem whose f is rhs_hip! (semidiscretizeGPU)."
+# Trixi-layout host copy u[v, i, j, k, e] of a device state, unpacked on the
+# device (trixi_hip_unpack): the state Trixi's SaveSolutionCallback reads
+# through wrap_array_native (reference src/solvers/dg.jl:24-32).  Save it with
+# Trixi.save_solution_file(wrap_array_native_hip(u, semi), t, dt, timestep,
+# semi.mesh, semi.equations, semi.dg_cpu, semi.cache_cpu, save_solution).
+function wrap_array_native_hip(u_ode::ROCVector{Float64}, semi::SemidiscretizationHyperbolicHIP)
+    u_trixi = ROCVector{Float64}(undef, semi.nvalues)
+    check(ccall((:trixi_hip_unpack, libtrixihip), Cint, (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Ptr{Cvoid}),
+                semi.ctx, pointer(u_trixi), pointer(u_ode), stream_ptr()))
+    hip_synchronize()
+    N = Trixi.nnodes(semi.dg_cpu)
+    reshape(Array(u_trixi), Trixi.nvariables(semi.equations), ntuple(_ -> N, ndims(semi.mesh))...,
+            Trixi.nelements(semi.dg_cpu, semi.cache_cpu))
+end
+
 function semidiscretizeHIP(semi::SemidiscretizationHyperbolicHIP, tspan)
     u0 = compute_coefficients_hip(first(tspan), semi)
     ODEProblem{true, FullSpecialize}(rhs_hip!, u0, tspan, semi)
@@ -410,7 +425,7 @@ function gradient_checkpointed(u0::ROCVector{Float64}, ubar_final::ROCVector{Flo
     ubar
 end
 
-export DGSEMHIP, SemidiscretizationHyperbolicHIP, semidiscretizeHIP, rhs_hip!, rhs_hip,
+export DGSEMHIP, SemidiscretizationHyperbolicHIP, semidiscretizeHIP, rhs_hip!, rhs_hip, wrap_array_native_hip,
        initial_condition_shock, initial_condition_rarefaction, boundary_condition_outflow,
        initial_condition_harmonic_nonperiodic,
        max_dt_hip, calc_error_norms_hip, rk_step_hip!, rk_stage_hip!, hip_synchronize,
