@@ -699,3 +699,29 @@ def test_save_solution_callback_files_match_device_state(th, torch, tmp_path):
         alpha = th.blending_factors(u, semi).cpu().numpy()
         np.testing.assert_array_equal(h5_dataset(path, "/element_variables_1", tmp_path), alpha)
     semi.close()
+
+
+def test_save_solution_callback_dt_lands_on_save_times(th, torch, tmp_path):
+    """SaveSolutionCallback(dt=0.25) on the C1 advection run (CFL 1.6, t in
+    [0, 1]): the steps are shortened to land on 0.25, 0.5, 0.75 and 1, each
+    file's time attribute is that time, and the final state still meets the
+    oracle trajectory's accuracy (errors within 1e-3 of the exact solution)."""
+    import os
+    if not os.path.exists(th.save_solution.H5_LIBRARY_PATH):
+        pytest.skip("libtrixih5.so not built (no HDF5 library)")
+    from test_save_solution import H5DUMP, h5_attributes
+    if H5DUMP is None:
+        pytest.skip("h5dump absent")
+    case = Case("c1_save_dt", 1, "linadv", 4, cmin=-1.0, cmax=1.0, ic="convergence_test", advection=(1.0,))
+    semi = product_semi(th, case)
+    save = th.SaveSolutionCallback(dt=0.25, output_directory=str(tmp_path))
+    sol = th.solve(th.semidiscretizeHIP(semi, (0.0, 1.0)), th.CarpenterKennedy2N54(williamson_condition=False),
+                   callback=[th.StepsizeCallback(cfl=1.6), save])
+    times = [float(h5_attributes(f)[0][("/", "time")]) for f in save.files]
+    assert times == [0.0, 0.25, 0.5, 0.75, 1.0], (times, save.files)
+    assert sol.t == 1.0
+    steps = [int(h5_attributes(f)[0][("/", "timestep")]) for f in save.files]
+    assert steps == sorted(steps) and steps[-1] == sol.nsteps
+    l2, linf = th.calc_error_norms(sol.u, sol.t, semi)
+    assert linf[0] < 1e-3
+    semi.close()

@@ -107,6 +107,10 @@ def test_writer_rejections(th, tmp_path):
         th.write_solution_file(str(tmp_path / "missing_dir" / "a.h5"), np.zeros((3, 4, 4, 4)), eq, 3)
     with pytest.raises(ValueError, match="cons2prim or cons2cons"):
         th.SaveSolutionCallback(solution_variables=lambda u, e: u)
+    with pytest.raises(ValueError, match="interval or a positive dt"):
+        th.SaveSolutionCallback(interval=10, dt=0.1)
+    with pytest.raises(ValueError, match="interval or a positive dt"):
+        th.SaveSolutionCallback(dt=0.0)
 
 
 def _families(th):

@@ -182,13 +182,18 @@ def save_solution_file(u_ode, semi, t, dt, timestep, output_directory="out", sol
 
 
 class SaveSolutionCallback:
-    """Saves every `interval` steps, and at the initial and final times when
-    asked (Trixi's SaveSolutionCallback keywords; `dt`-based saving is not built)."""
+    """Saves every `interval` steps or, with `dt`, at the times t0 + n dt
+    (solve() shortens the step that would pass such a time so that it lands on
+    it, as Trixi does with stop times), and at the initial and final times when
+    asked (Trixi's SaveSolutionCallback keywords)."""
 
-    def __init__(self, interval=0, save_initial_solution=True, save_final_solution=True, output_directory="out",
-                 solution_variables=cons2prim):
+    def __init__(self, interval=0, dt=None, save_initial_solution=True, save_final_solution=True,
+                 output_directory="out", solution_variables=cons2prim):
         if solution_variables not in (cons2cons, cons2prim):
             raise ValueError("solution_variables must be cons2prim or cons2cons")
+        if dt is not None and (int(interval) > 0 or not float(dt) > 0):
+            raise ValueError("give either interval or a positive dt")
+        self.save_dt, self.next_time = (float(dt) if dt is not None else None), None
         self.interval, self.output_directory = int(interval), output_directory
         self.save_initial_solution, self.save_final_solution = save_initial_solution, save_final_solution
         self.solution_variables = solution_variables
@@ -198,6 +203,9 @@ class SaveSolutionCallback:
         self.files.append(save_solution_file(u, semi, t, dt, step, self.output_directory, self.solution_variables))
 
     def initialize(self, u, t, dt, semi):
+        self.t0, self.nsaved = t, 0
+        if self.save_dt is not None:
+            self.next_time = t + self.save_dt
         if self.save_initial_solution:
             self._save(u, t, dt, 0, semi)
 
@@ -205,5 +213,10 @@ class SaveSolutionCallback:
         if final:
             if self.save_final_solution and not (self.files and self.files[-1].endswith("%09d.h5" % step)):
                 self._save(u, t, dt, step, semi)
+        elif self.save_dt is not None:
+            if t >= self.next_time:
+                self._save(u, t, dt, step, semi)
+                self.nsaved += 1
+                self.next_time = self.t0 + (self.nsaved + 1) * self.save_dt
         elif self.interval > 0 and step % self.interval == 0:
             self._save(u, t, dt, step, semi)

@@ -82,8 +82,14 @@ def solve(ode, alg=None, dt=None, callback=(), use_graph=True, maxiters=10**7):
             h = stepsize.dt(u, semi) if stepsize is not None else dt
             if t + h > t1:
                 h = t1 - t
+            for c in saves:  # land on the next save time instead of stepping past it
+                if c.next_time is not None and t + h > c.next_time > t:
+                    h = c.next_time - t
             rk_step(u, k, t, h, semi, use_graph=use_graph, stream=stream)
             t = t + h
+            for c in saves:
+                if c.next_time is not None and abs(t - c.next_time) <= 1e-12 * max(1.0, abs(t)):
+                    t = c.next_time  # the shortened step hit the save time; remove the rounding of t + h
             nsteps += 1
             for a in analysis:
                 a(u, t, nsteps)
