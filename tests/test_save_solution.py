@@ -111,6 +111,8 @@ def test_writer_rejections(th, tmp_path):
         th.SaveSolutionCallback(interval=10, dt=0.1)
     with pytest.raises(ValueError, match="interval or a positive dt"):
         th.SaveSolutionCallback(dt=0.0)
+    with pytest.raises(RuntimeError, match="initialize"):
+        th.SaveSolutionCallback(dt=0.1)(None, 0.1, 0.1, 1, None)
 
 
 def _families(th):

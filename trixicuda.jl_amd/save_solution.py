@@ -214,6 +214,8 @@ class SaveSolutionCallback:
             if self.save_final_solution and not (self.files and self.files[-1].endswith("%09d.h5" % step)):
                 self._save(u, t, dt, step, semi)
         elif self.save_dt is not None:
+            if self.next_time is None:
+                raise RuntimeError("SaveSolutionCallback(dt=...) needs initialize(u, t0, dt, semi) before the first step")
             if t >= self.next_time:
                 self._save(u, t, dt, step, semi)
                 self.nsaved += 1
