@@ -50,6 +50,35 @@ int trixi_h5_save_solution(const char* filename, const trixi_h5_solution_info* i
                            const char* const* varnames, const double* element_data,
                            const char* const* element_varnames);
 
+/* Cell tree of a TreeMesh, as Trixi's save_mesh_file writes it to mesh.h5
+ * (Trixi.jl src/meshes/mesh_io.jl, TreeMesh method; read back by Trixi's
+ * load_mesh and by Trixi2Vtk to place the solution files' elements).  The
+ * reference's TreeMesh is Trixi's own (src/solvers/cache.jl:130-158), so the
+ * mesh file of a HIP run is the one Trixi would write for the same mesh. */
+typedef struct {
+    int32_t ndims;             /* 1, 2 or 3 */
+    int32_t periodicity[3];    /* per dimension, 0 or 1 */
+    int64_t n_cells;           /* cells of the tree, leaves and parents */
+    int64_t capacity;          /* n_cells_max of the TreeMesh */
+    int64_t n_leaf_cells;
+    int64_t minimum_level;     /* over the leaf cells */
+    int64_t maximum_level;
+    double center_level_0[3];  /* centre of the root cell */
+    double length_level_0;     /* edge length of the root cell */
+} trixi_h5_mesh_info;
+
+/* Writes (truncating) `filename`: root attributes mesh_type ("TreeMesh"),
+ * ndims, n_cells, capacity, n_leaf_cells, minimum_level, maximum_level,
+ * center_level_0, length_level_0, periodicity and the int64 datasets
+ * parent_ids [n_cells], child_ids [n_cells][2^ndims], neighbor_ids
+ * [n_cells][2 ndims], levels [n_cells] and the float64 dataset coordinates
+ * [n_cells][ndims] (row-major, the transposes of Trixi's column-major
+ * arrays, as HDF5.jl stores a Julia matrix).  Cell ids are 1-based, 0 for
+ * none. */
+int trixi_h5_save_mesh(const char* filename, const trixi_h5_mesh_info* info, const int64_t* parent_ids,
+                       const int64_t* child_ids, const int64_t* neighbor_ids, const int64_t* levels,
+                       const double* coordinates);
+
 const char* trixi_h5_last_error(void);
 
 #ifdef __cplusplus

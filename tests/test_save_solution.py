@@ -51,7 +51,7 @@ def test_header_declares_the_writer():
     with open(os.path.join(ROOT, "include", "trixi_h5.h")) as fh:
         text = fh.read()
     syms = set(re.findall(r"^\s*(?:const\s+)?\w+\**\s+\**(trixi_h5_\w+)\s*\(", text, re.M))
-    assert syms == {"trixi_h5_save_solution", "trixi_h5_last_error"}
+    assert syms == {"trixi_h5_save_solution", "trixi_h5_save_mesh", "trixi_h5_last_error"}
     if os.path.exists(os.path.join(ROOT, "trixicuda.jl_amd", "libtrixih5.so")):
         lib = ctypes.CDLL(os.path.join(ROOT, "trixicuda.jl_amd", "libtrixih5.so"))
         for s in syms:
@@ -165,3 +165,99 @@ def test_varnames_of_every_family(th):
     assert th.varnames(th.cons2cons, s1) == ["h", "h_v", "b"] and th.varnames(th.cons2prim, s1) == ["H", "v", "b"]
     s2 = th.ShallowWaterEquations2D(gravity_constant=9.81)
     assert th.varnames(th.cons2prim, s2) == ["H", "v1", "v2", "b"]
+
+
+def h5_int_dataset(path, name, tmp_path):
+    out = str(tmp_path / (name.strip("/") + ".bin"))
+    subprocess.run([H5DUMP, "-d", name, "-b", "LE", "-o", out, path], check=True, capture_output=True)
+    return np.fromfile(out, dtype="<i8")
+
+
+# the reference's mesh shapes: uniform periodic (test/tree_dgsem_3d/euler_ec.jl),
+# non-periodic, and refined with the same box applied twice
+# (test/tree_dgsem_2d/euler_blob_mortar.jl:59-68)
+MESHES = [
+    dict(lo=(-2.0,) * 3, hi=(2.0,) * 3, level=2, periodicity=True, patches=()),
+    dict(lo=(0.0, 0.0), hi=(2.0, 2.0), level=3, periodicity=(False, True), patches=()),
+    dict(lo=(-32.0, -32.0), hi=(32.0, 32.0), level=4, periodicity=True,
+         patches=[dict(type="box", coordinates_min=(-40.0, -5.0), coordinates_max=(40.0, 5.0))] * 2),
+    dict(lo=(-1.0,), hi=(1.0,), level=4, periodicity=False, patches=()),
+]
+
+
+@pytest.mark.parametrize("spec", MESHES, ids=["uniform3d", "nonperiodic2d", "blob2d", "uniform1d"])
+def test_tree_arrays_are_a_consistent_treemesh(th, spec):
+    """The cell tree written to mesh.h5: the leaves in storage order are the
+    elements (same centres and levels), storage order is depth first (each
+    parent directly before its first child, children consecutive), a child's
+    centre is its parent's shifted by a quarter of the parent's length along
+    each axis, and neighbour links are symmetric and join same-level cells
+    whose centres differ by one cell length along one axis (modulo the
+    domain length in periodic directions)."""
+    m = th.TreeMesh(spec["lo"], spec["hi"], spec["level"], periodicity=spec["periodicity"],
+                    refinement_patches=spec["patches"])
+    t = m.tree_arrays()
+    nd, n = m.ndims, t["n_cells"]
+    P, C, Nb, lev, X = t["parent_ids"], t["child_ids"], t["neighbor_ids"], t["levels"], t["coordinates"]
+    leaf = np.all(C == 0, axis=1)
+    assert t["n_leaf_cells"] == m.nelements == leaf.sum()
+    np.testing.assert_array_equal(X[leaf], m.centers)
+    assert P[0] == 0 and lev[0] == 0
+    np.testing.assert_array_equal(X[0], 0.5 * (np.array(spec["lo"]) + np.array(spec["hi"])))
+    L0 = t["length_level_0"]
+    for i in range(n):
+        kids = C[i][C[i] > 0]
+        if kids.size:
+            assert kids.size == 1 << nd and kids[0] == i + 2 and np.all(np.diff(kids) > 0)
+            for c, k in enumerate(C[i]):
+                assert P[k - 1] == i + 1 and lev[k - 1] == lev[i] + 1
+                off = np.array([0.5 if (c >> d) & 1 else -0.5 for d in range(nd)]) * L0 / 2 ** (lev[i] + 1)
+                np.testing.assert_array_equal(X[k - 1], X[i] + off)
+        per = m.periodicity
+        for f in range(2 * nd):
+            j = Nb[i, f]
+            if j == 0:
+                continue
+            d, side = f // 2, f % 2
+            assert lev[j - 1] == lev[i] and Nb[j - 1, f ^ 1] == i + 1
+            h = L0 / 2 ** lev[i]
+            delta = X[j - 1] - X[i]
+            expect = np.zeros(nd)
+            expect[d] = h if side else -h
+            diff = delta - expect
+            if per[d]:
+                diff[d] = (diff[d] + L0 / 2) % L0 - L0 / 2 if lev[i] > 0 else 0.0
+            np.testing.assert_allclose(diff, 0.0, atol=1e-12 * L0)
+    # a same-level neighbour exists exactly where a cell's neighbour position is inside the tree
+    if not any(m.periodicity):
+        root_nbrs = Nb[0]
+        assert np.all(root_nbrs == 0)
+
+
+@needs_h5
+def test_mesh_file_matches_the_tree(th, tmp_path):
+    """mesh.h5 as Trixi's save_mesh_file writes it for a TreeMesh: the tree
+    datasets (row-major transposes of Trixi's column-major arrays) and the
+    root attributes, read back with h5dump."""
+    m = th.TreeMesh((-32.0, -32.0), (32.0, 32.0), 4, n_cells_max=100_000,
+                    refinement_patches=[dict(type="box", coordinates_min=(-40.0, -5.0),
+                                             coordinates_max=(40.0, 5.0))] * 2)
+    path = th.save_mesh_file(m, str(tmp_path))
+    assert os.path.basename(path) == "mesh.h5"
+    t = m.tree_arrays()
+    for k in ("parent_ids", "child_ids", "neighbor_ids", "levels"):
+        np.testing.assert_array_equal(h5_int_dataset(path, "/" + k, tmp_path), t[k].reshape(-1))
+    np.testing.assert_array_equal(h5_dataset(path, "/coordinates", tmp_path), t["coordinates"].reshape(-1))
+    attrs, text = h5_attributes(path)
+    assert attrs[("/", "mesh_type")] == "TreeMesh"
+    assert int(attrs[("/", "ndims")]) == 2
+    assert int(attrs[("/", "n_cells")]) == t["n_cells"]
+    assert int(attrs[("/", "capacity")]) == 100_000
+    assert int(attrs[("/", "n_leaf_cells")]) == m.nelements
+    assert int(attrs[("/", "minimum_level")]) == 4 and int(attrs[("/", "maximum_level")]) == 6
+    assert float(attrs[("/", "length_level_0")]) == 64.0
+    for a in ("center_level_0", "periodicity"):
+        assert ("/", a) in attrs
+    assert re.search(r'DATASET "child_ids" \{\s*DATATYPE\s+H5T_STD_I64LE\s*DATASPACE\s+SIMPLE \{ \( %d, 4 \)'
+                     % t["n_cells"], subprocess.run([H5DUMP, "-H", path], check=True, capture_output=True,
+                                                    text=True).stdout)

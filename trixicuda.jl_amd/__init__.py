@@ -13,7 +13,7 @@ This directory is not a valid Python identifier; load it as ``trixi_hip``:
         submodule_search_locations=["trixicuda.jl_amd"])
 """
 from ._lib import LIBRARY_PATH, TrixiHipError, version
-from .save_solution import (SaveSolutionCallback, cons2cons, cons2prim, save_solution_file, varnames,
+from .save_solution import (SaveSolutionCallback, cons2cons, cons2prim, save_mesh_file, save_solution_file, varnames,
                             write_solution_file)
 from .callbacks import (AnalysisCallback, StepsizeCallback, blending_factors, calc_error_norms, check_state,
                         integrate, max_dt)

@@ -1,4 +1,4 @@
-// Host-side HDF5 writer of Trixi's solution files (libtrixih5.so).
+// Host-side HDF5 writer of Trixi's solution and TreeMesh files (libtrixih5.so).
 //
 // Writes the file that Trixi's SaveSolutionCallback writes for a DG solver
 // (Trixi.jl src/callbacks_step/save_solution_dg.jl, `save_solution_file`;
@@ -78,6 +78,36 @@ bool dataset_f64(hid_t file, const char* name, const double* data, int64_t n, co
     return attr_str(ds.id, "name", varname);
 }
 
+bool attr_f64_vec(hid_t obj, const char* name, const double* v, hsize_t n) {
+    Handle space{H5Screate_simple(1, &n, nullptr), H5Sclose};
+    if (space.id < 0) return false;
+    Handle attr{H5Acreate2(obj, name, H5T_IEEE_F64LE, space.id, H5P_DEFAULT, H5P_DEFAULT), H5Aclose};
+    if (attr.id < 0) return false;
+    return H5Awrite(attr.id, H5T_NATIVE_DOUBLE, v) >= 0;
+}
+
+// Julia Bool vectors: one byte per value (HDF5.jl's 8-bit Bool)
+bool attr_bool_vec(hid_t obj, const char* name, const int32_t* v, hsize_t n) {
+    unsigned char b[3] = {0, 0, 0};
+    for (hsize_t i = 0; i < n; ++i) b[i] = v[i] ? 1 : 0;
+    Handle space{H5Screate_simple(1, &n, nullptr), H5Sclose};
+    if (space.id < 0) return false;
+    Handle attr{H5Acreate2(obj, name, H5T_STD_B8LE, space.id, H5P_DEFAULT, H5P_DEFAULT), H5Aclose};
+    if (attr.id < 0) return false;
+    return H5Awrite(attr.id, H5T_NATIVE_B8, b) >= 0;
+}
+
+bool dataset_nd(hid_t file, const char* name, hid_t mem_type, hid_t file_type, const void* data, int rank,
+                const hsize_t* dims) {
+    Handle space{H5Screate_simple(rank, dims, nullptr), H5Sclose};
+    if (space.id < 0) return false;
+    Handle ds{H5Dcreate2(file, name, file_type, space.id, H5P_DEFAULT, H5P_DEFAULT, H5P_DEFAULT), H5Dclose};
+    if (ds.id < 0) return false;
+    hsize_t total = 1;
+    for (int r = 0; r < rank; ++r) total *= dims[r];
+    return total == 0 || H5Dwrite(ds.id, mem_type, H5S_ALL, H5S_ALL, H5P_DEFAULT, data) >= 0;
+}
+
 }  // namespace
 
 extern "C" {
@@ -136,6 +166,50 @@ int trixi_h5_save_solution(const char* filename, const trixi_h5_solution_info* i
                          element_varnames[v]))
             return fail(std::string("writing ") + name + " failed");
     }
+    if (H5Fflush(file.id, H5F_SCOPE_LOCAL) < 0) return fail("flushing the file failed");
+    return TRIXI_H5_OK;
+}
+
+int trixi_h5_save_mesh(const char* filename, const trixi_h5_mesh_info* info, const int64_t* parent_ids,
+                       const int64_t* child_ids, const int64_t* neighbor_ids, const int64_t* levels,
+                       const double* coordinates) {
+    if (!filename || !info) {
+        g_err = "null filename or info";
+        return TRIXI_H5_EINVAL;
+    }
+    const int nd = info->ndims;
+    if (nd < 1 || nd > 3 || info->n_cells < 1 || info->capacity < info->n_cells || info->n_leaf_cells < 1 ||
+        info->n_leaf_cells > info->n_cells || info->minimum_level < 0 || info->maximum_level < info->minimum_level) {
+        g_err = "invalid ndims, cell counts or levels";
+        return TRIXI_H5_EINVAL;
+    }
+    if (!parent_ids || !child_ids || !neighbor_ids || !levels || !coordinates) {
+        g_err = "null tree array";
+        return TRIXI_H5_EINVAL;
+    }
+    H5Eset_auto2(H5E_DEFAULT, nullptr, nullptr);
+    Handle file{H5Fcreate(filename, H5F_ACC_TRUNC, H5P_DEFAULT, H5P_DEFAULT), H5Fclose};
+    if (file.id < 0) return fail(std::string("cannot create ") + filename);
+    const hsize_t n = static_cast<hsize_t>(info->n_cells);
+    bool ok = attr_str(file.id, "mesh_type", "TreeMesh") && attr_i64(file.id, "ndims", nd) &&
+              attr_i64(file.id, "n_cells", info->n_cells) && attr_i64(file.id, "capacity", info->capacity) &&
+              attr_i64(file.id, "n_leaf_cells", info->n_leaf_cells) &&
+              attr_i64(file.id, "minimum_level", info->minimum_level) &&
+              attr_i64(file.id, "maximum_level", info->maximum_level) &&
+              attr_f64_vec(file.id, "center_level_0", info->center_level_0, nd) &&
+              attr_f64(file.id, "length_level_0", info->length_level_0) &&
+              attr_bool_vec(file.id, "periodicity", info->periodicity, nd);
+    if (!ok) return fail("writing the mesh attributes failed");
+    const hsize_t d1[1] = {n};
+    const hsize_t dch[2] = {n, static_cast<hsize_t>(1) << nd};
+    const hsize_t dnb[2] = {n, static_cast<hsize_t>(2 * nd)};
+    const hsize_t dxy[2] = {n, static_cast<hsize_t>(nd)};
+    ok = dataset_nd(file.id, "parent_ids", H5T_NATIVE_INT64, H5T_STD_I64LE, parent_ids, 1, d1) &&
+         dataset_nd(file.id, "child_ids", H5T_NATIVE_INT64, H5T_STD_I64LE, child_ids, 2, dch) &&
+         dataset_nd(file.id, "neighbor_ids", H5T_NATIVE_INT64, H5T_STD_I64LE, neighbor_ids, 2, dnb) &&
+         dataset_nd(file.id, "levels", H5T_NATIVE_INT64, H5T_STD_I64LE, levels, 1, d1) &&
+         dataset_nd(file.id, "coordinates", H5T_NATIVE_DOUBLE, H5T_IEEE_F64LE, coordinates, 2, dxy);
+    if (!ok) return fail("writing the tree datasets failed");
     if (H5Fflush(file.id, H5F_SCOPE_LOCAL) < 0) return fail("flushing the file failed");
     return TRIXI_H5_OK;
 }

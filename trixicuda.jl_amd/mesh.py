@@ -55,6 +55,8 @@ class TreeMesh:
             self._build_tree(cmin, cmax, int(initial_refinement_level), per, refinement_patches, n_cells_max)
             return
         self.level = L = int(initial_refinement_level)
+        self._tree = None
+        self._n_cells_max = n_cells_max
         self.nelements = E = 1 << (nd * L)
         if n_cells_max is not None and E > n_cells_max:
             raise ValueError(f"n_cells_max={n_cells_max} too small for {E} leaf cells")
@@ -205,6 +207,8 @@ class TreeMesh:
 
         elems = leaves()
         E = len(elems)
+        self._tree = (cells, list(order), index, per)
+        self._n_cells_max = n_cells_max
         if n_cells_max is not None and len(cells) > n_cells_max:
             raise ValueError(f"n_cells_max={n_cells_max} too small for {len(cells)} cells")
         c2e = {c: i for i, c in enumerate(elems)}
@@ -249,6 +253,85 @@ class TreeMesh:
         self.mortar_orientations = np.array([m[1] for m in mort], dtype=np.int32)
         self.mortar_large_sides = np.array([m[2] for m in mort], dtype=np.int32)
         self.total_volume = length ** nd
+
+    def tree_arrays(self):
+        """The cell tree as Trixi's TreeMesh stores it and its mesh file holds
+        (abstract_tree.jl / serial_tree.jl; save_mesh_file of a TreeMesh):
+        cells in storage order (depth first: refining a cell inserts its
+        children directly behind it), 1-based cell ids, 0 for none.
+
+        Returns a dict of parent_ids [n], child_ids [n, 2^d] (child c has
+        offset bit d set iff (c >> d) & 1), neighbor_ids [n, 2d] (directions
+        -x, +x, -y, +y, -z, +z: the same-level cell across the face, wrapping
+        around periodic directions, so a periodic root is its own neighbour),
+        levels [n], coordinates [n, d] (cell centres, halved from the root
+        centre) and the scalars n_cells, capacity, n_leaf_cells,
+        minimum_level, maximum_level (over the leaves), center_level_0,
+        length_level_0, periodicity.  The leaves in storage order are the
+        elements in order."""
+        nd = self.ndims
+        nch = 1 << nd
+        per = self.periodicity
+        if self._tree is not None:
+            cells, order, index, _ = self._tree
+        else:
+            # a uniform tree: every cell refined down to self.level, depth first
+            cells, order, index = [], [], {}
+
+            def add(lvl, k):
+                cid = len(cells)
+                cells.append([lvl, k, None])
+                index[(lvl, k)] = cid
+                order.append(cid)
+                if lvl < self.level:
+                    kids = []
+                    for child in range(nch):
+                        kc = tuple(2 * k[d] + ((child >> d) & 1) for d in range(nd))
+                        kids.append(add(lvl + 1, kc))
+                    cells[cid][2] = kids
+                return cid
+
+            add(0, (0,) * nd)
+        pos = {c: i for i, c in enumerate(order)}  # storage position, 0-based
+        n = len(order)
+        parent = np.zeros(n, dtype=np.int64)
+        child_ids = np.zeros((n, nch), dtype=np.int64)
+        nbr = np.zeros((n, 2 * nd), dtype=np.int64)
+        levels = np.zeros(n, dtype=np.int64)
+        coords = np.zeros((n, nd))
+        centre0 = 0.5 * (self.coordinates_min + self.coordinates_max)
+        for c in order:
+            i = pos[c]
+            lvl, k, kids = cells[c]
+            levels[i] = lvl
+            if kids is not None:
+                for j, kc in enumerate(kids):
+                    child_ids[i, j] = pos[kc] + 1
+                    parent[pos[kc]] = i + 1
+            n1 = 1 << lvl
+            for d in range(nd):
+                # centre by halving from the root centre (exact binary fractions)
+                cc, ln = centre0[d], self.length
+                for b in range(lvl - 1, -1, -1):
+                    ln *= 0.5
+                    cc += (0.5 if (k[d] >> b) & 1 else -0.5) * ln
+                coords[i, d] = cc
+                for side in (0, 1):
+                    kk = list(k)
+                    kk[d] += 1 if side else -1
+                    if not 0 <= kk[d] < n1:
+                        if not per[d]:
+                            continue
+                        kk[d] %= n1
+                    nb = index.get((lvl, tuple(kk)))
+                    if nb is not None:
+                        nbr[i, 2 * d + side] = pos[nb] + 1
+        leaf = np.all(child_ids == 0, axis=1)
+        return dict(parent_ids=parent, child_ids=child_ids, neighbor_ids=nbr, levels=levels, coordinates=coords,
+                    n_cells=n, capacity=int(self._n_cells_max) if self._n_cells_max is not None else n,
+                    n_leaf_cells=int(leaf.sum()), minimum_level=int(levels[leaf].min()),
+                    maximum_level=int(levels[leaf].max()), center_level_0=centre0.copy(),
+                    length_level_0=self.length, periodicity=tuple(per))
 
     def node_coordinates(self, nodes):
         """Node coordinates [E, (N,)*nd, nd] in Trixi order (i fastest = last axis)."""

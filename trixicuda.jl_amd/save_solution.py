@@ -31,6 +31,13 @@ class _Info(ctypes.Structure):
                 ("mesh_type", ctypes.c_char_p), ("mesh_file", ctypes.c_char_p)]
 
 
+class _MeshInfo(ctypes.Structure):
+    _fields_ = [("ndims", ctypes.c_int32), ("periodicity", ctypes.c_int32 * 3), ("n_cells", ctypes.c_int64),
+                ("capacity", ctypes.c_int64), ("n_leaf_cells", ctypes.c_int64), ("minimum_level", ctypes.c_int64),
+                ("maximum_level", ctypes.c_int64), ("center_level_0", ctypes.c_double * 3),
+                ("length_level_0", ctypes.c_double)]
+
+
 def h5lib():
     """libtrixih5.so; raises when it was not built (no HDF5 headers at build time)."""
     global _h5
@@ -41,9 +48,37 @@ def h5lib():
         lib.trixi_h5_save_solution.restype = ctypes.c_int
         lib.trixi_h5_save_solution.argtypes = [ctypes.c_char_p, ctypes.POINTER(_Info), ctypes.c_void_p,
                                                ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+        lib.trixi_h5_save_mesh.restype = ctypes.c_int
+        lib.trixi_h5_save_mesh.argtypes = [ctypes.c_char_p, ctypes.POINTER(_MeshInfo)] + [ctypes.c_void_p] * 5
         lib.trixi_h5_last_error.restype = ctypes.c_char_p
         _h5 = lib
     return _h5
+
+
+def save_mesh_file(mesh, output_directory="out"):
+    """Trixi's save_mesh_file for a TreeMesh: `mesh.h5` in output_directory
+    with the cell tree (TreeMesh.tree_arrays); returns the path."""
+    t = mesh.tree_arrays()
+    nd = mesh.ndims
+    info = _MeshInfo()
+    info.ndims = nd
+    for d in range(nd):
+        info.periodicity[d] = 1 if t["periodicity"][d] else 0
+        info.center_level_0[d] = t["center_level_0"][d]
+    info.n_cells, info.capacity, info.n_leaf_cells = t["n_cells"], t["capacity"], t["n_leaf_cells"]
+    info.minimum_level, info.maximum_level = t["minimum_level"], t["maximum_level"]
+    info.length_level_0 = t["length_level_0"]
+    arrays = [np.ascontiguousarray(t[k], dtype=np.int64) for k in ("parent_ids", "child_ids", "neighbor_ids",
+                                                                   "levels")]
+    coords = np.ascontiguousarray(t["coordinates"], dtype=np.float64)
+    os.makedirs(output_directory, exist_ok=True)
+    path = os.path.join(output_directory, "mesh.h5")
+    lib = h5lib()
+    rc = lib.trixi_h5_save_mesh(os.fsencode(path), ctypes.byref(info), *[a.ctypes.data for a in arrays],
+                                coords.ctypes.data)
+    if rc != 0:
+        raise L.TrixiHipError(L.EINVAL, lib.trixi_h5_last_error().decode())
+    return path
 
 
 def cons2cons(u, equations):
@@ -164,8 +199,11 @@ def write_solution_file(filename, data, equations, polydeg, t=0.0, dt=0.0, times
     return filename
 
 
-def save_solution_file(u_ode, semi, t, dt, timestep, output_directory="out", solution_variables=cons2prim):
+def save_solution_file(u_ode, semi, t, dt, timestep, output_directory="out", solution_variables=cons2prim,
+                       mesh_file=""):
     """Trixi's save_solution_file for a device state vector; returns the path.
+    `mesh_file` is the name of the mesh file the solution refers to (Trixi
+    writes the mesh file's name, without its directory).
     Shock capturing adds the blending factors as `indicator_shock_capturing`,
     the element variable Trixi saves for VolumeIntegralShockCapturingHG."""
     from .semidiscretization import wrap_array_native
@@ -178,7 +216,7 @@ def save_solution_file(u_ode, semi, t, dt, timestep, output_directory="out", sol
     os.makedirs(output_directory, exist_ok=True)
     path = os.path.join(output_directory, "solution_%09d.h5" % int(timestep))
     return write_solution_file(path, data, semi.equations, semi.solver.polydeg, t, dt, timestep,
-                               solution_variables, element_variables)
+                               solution_variables, element_variables, mesh_file=mesh_file)
 
 
 class SaveSolutionCallback:
@@ -198,9 +236,15 @@ class SaveSolutionCallback:
         self.save_initial_solution, self.save_final_solution = save_initial_solution, save_final_solution
         self.solution_variables = solution_variables
         self.files = []
+        self.mesh_file = None
 
     def _save(self, u, t, dt, step, semi):
-        self.files.append(save_solution_file(u, semi, t, dt, step, self.output_directory, self.solution_variables))
+        # the mesh file once, before the first solution file (Trixi writes it
+        # when the callback first saves and the mesh has unsaved changes)
+        if self.mesh_file is None:
+            self.mesh_file = save_mesh_file(semi.mesh, self.output_directory)
+        self.files.append(save_solution_file(u, semi, t, dt, step, self.output_directory, self.solution_variables,
+                                             mesh_file=os.path.basename(self.mesh_file)))
 
     def initialize(self, u, t, dt, semi):
         self.t0, self.nsaved = t, 0
