@@ -90,7 +90,20 @@ def _events_ms(torch, stream, fn, reps, warmup=3):
     return a.elapsed_time(b) / reps
 
 
-def measure_configs(torch, th, stream, reps):
+def roofline_of(fm, key, dofs, ms, precision):
+    """HBM roof (algorithmic bytes: read u, write du) with the flop fraction
+    beside it, for one rhs! of `dofs` DOF taking `ms` milliseconds."""
+    secs = ms * 1e-3
+    gbps = fm.BYTES_PER_DOF[key] * dofs / secs / 1e9
+    fpeak = fm.PEAK["fp64_TFLOPs"] if precision == "fp64" else fm.PEAK["fp32_TFLOPs"]
+    tf = fm.FLOPS_PER_DOF[key] * dofs / secs / 1e12
+    return {"bound": "hbm", "achieved": gbps, "peak": fm.PEAK["hbm_GBps"], "unit": "GB/s",
+            "frac": gbps / fm.PEAK["hbm_GBps"], "bytes_per_dof": fm.BYTES_PER_DOF[key],
+            "flop_per_dof": fm.FLOPS_PER_DOF[key], "flop_TFLOPs": tf, "flop_peak_TFLOPs": fpeak,
+            "flop_frac": tf / fpeak, "flop_model": key}
+
+
+def measure_configs(torch, th, fm, stream, reps):
     """The other workloads of the benchmark set, each on one GPU:
     C2 2D Euler FD (flux_ranocha + LLF), [-2,2]^2 level 5, seeded random state;
     C4 3D GLM-MHD Alfven wave (flux_hindenlang_gassner + Powell, LLF), [-1,1]^3 level 4;
@@ -99,14 +112,17 @@ def measure_configs(torch, th, stream, reps):
     C5 a 100-step mixed forward with the checkpointed FP64 adjoint."""
     out = {}
     specs = {
-        "C2": dict(eq=th.CompressibleEulerEquations2D(1.4), nd=2, lo=-2.0, hi=2.0, level=5,
+        "C2": dict(key="euler2d_fd_ranocha_llf", eq=th.CompressibleEulerEquations2D(1.4), nd=2, lo=-2.0, hi=2.0,
+                   level=5,
                    vi=th.VolumeIntegralFluxDifferencing(th.flux_ranocha), sf=th.flux_lax_friedrichs,
                    ic=th.initial_condition_seeded_random(20261202), vjp=True),
-        "C4": dict(eq=th.IdealGlmMhdEquations3D(5.0 / 3.0, 1.0), nd=3, lo=-1.0, hi=1.0, level=4,
+        "C4": dict(key="mhd3d_fd_hindenlang_gassner_llf", eq=th.IdealGlmMhdEquations3D(5.0 / 3.0, 1.0), nd=3,
+                   lo=-1.0, hi=1.0, level=4,
                    vi=th.VolumeIntegralFluxDifferencing((th.flux_hindenlang_gassner, th.flux_nonconservative_powell)),
                    sf=(th.flux_lax_friedrichs, th.flux_nonconservative_powell),
                    ic=th.initial_condition_convergence_test, vjp=True),
-        "C5": dict(eq=th.CompressibleEulerEquations2D(1.4), nd=2, lo=-2.0, hi=2.0, level=6,
+        "C5": dict(key="euler2d_hg_ranocha_llf", eq=th.CompressibleEulerEquations2D(1.4), nd=2, lo=-2.0, hi=2.0,
+                   level=6,
                    vi=th.VolumeIntegralShockCapturingHG(
                        th.IndicatorHennemannGassner(th.CompressibleEulerEquations2D(1.4), th.LobattoLegendreBasisHIP(3),
                                                     alpha_max=0.5, alpha_min=0.001, alpha_smooth=True,
@@ -128,7 +144,8 @@ def measure_configs(torch, th, stream, reps):
             ms = _events_ms(torch, stream, lambda: th.rhs_hip_(d, u, semi, 0.0), reps)
             torch.cuda.synchronize()
             du[prec] = d
-            r[prec] = {"rhs_ms": ms, "dof_per_s": semi.ndofs / (ms * 1e-3)}
+            r[prec] = {"rhs_ms": ms, "dof_per_s": semi.ndofs / (ms * 1e-3),
+                       "roofline": roofline_of(fm, c["key"], semi.ndofs, ms, prec)}
         r["ndofs"] = semis["fp64"].ndofs
         a, b = du["mixed"], du["fp64"]
         r["mixed_vs_fp64_rel_l2"] = float(torch.linalg.norm(a - b) / torch.linalg.norm(b))
@@ -278,7 +295,7 @@ def main():
 
     extra = None
     if not args.no_extra and not args.only:
-        extra = measure_configs(torch, th, stream, 100)
+        extra = measure_configs(torch, th, fm, stream, 100)
 
     errors = {}
     if "fp64" in du_out and "mixed" in du_out:
@@ -290,7 +307,8 @@ def main():
     if not args.no_cpu_baseline:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle as O
-        threads = min(16, os.cpu_count() or 1)
+        # the host's own cores: what `nproc` reports (the process's CPU affinity)
+        threads = len(os.sched_getaffinity(0))
         O.set_threads(threads)
         cfg = O.make_cfg(3, O.EQ_EULER, 3, O.VI_FD, O.FLUX_RANOCHA, O.FLUX_LLF, ic_length=4.0)
         mesh = O.Mesh(3, args.level, -2.0, 2.0, 3)
@@ -304,7 +322,8 @@ def main():
                 break
         el = time.perf_counter() - t0
         dofs = mesh.nelements * 64
-        cpu = {"value": dofs * n / el, "unit": "DOF-updates/s", "cores": threads, "kind": "port",
+        cpu = {"value": dofs * n / el, "unit": "DOF-updates/s", "cores": threads, "nproc": threads,
+               "os_cpu_count": os.cpu_count(), "kind": "port",
                "sample": f"{n} x rhs! of the full workload state ({dofs} DOF), FP64 C++ oracle, OpenMP"}
         for mode, d in du_out.items():
             g = th.wrap_array_native(d, semi_keep).cpu().numpy()
