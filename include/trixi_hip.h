@@ -20,10 +20,17 @@
  *     trixi_hip_unpack convert from/to Trixi's u[v, i, j, k, element] layout.
  *   - All device work is asynchronous on the caller's stream unless stated.
  *   - A context is not thread safe; the caller serialises calls on it.
- *   - Every device buffer a context needs is allocated in trixi_hip_create;
- *     no call after create allocates device memory.
- *   - Supported: periodic uniform TreeMesh (no boundaries, no mortars), the
- *     reference asserts the same on its GPU path (dg_3d.jl:463-466, 542-544).
+ *   - Every device buffer of the forward path (rhs!, RK stages and steps,
+ *     callbacks) is allocated in trixi_hip_create; none of those calls
+ *     allocates device memory.  The adjoint allocates its own buffers once:
+ *     trixi_hip_enable_adjoint (the stage checkpoints and registers) and the
+ *     first trixi_hip_rhs_vjp of a shock-capturing context (two per-element
+ *     arrays); call trixi_hip_enable_adjoint before capturing adjoint work.
+ *   - Supported: TreeMesh, uniform or refined (L2 mortars), periodic or with
+ *     the boundary conditions of enum trixi_hip_boundary_condition, with the
+ *     equations, fluxes and precisions listed below; any other combination is
+ *     TRIXI_HIP_EUNSUPPORTED (the reference's GPU path asserts periodicity and
+ *     no mortars, dg_3d.jl:463-466, 542-544).
  */
 #ifndef TRIXI_HIP_H
 #define TRIXI_HIP_H
@@ -91,8 +98,14 @@ enum trixi_hip_flux {
 };
 enum trixi_hip_precision {
     TRIXI_HIP_FP64 = 0,  /* all arithmetic in FP64                                    */
-    TRIXI_HIP_MIXED = 1  /* FP32 volume/surface arithmetic; FP64 state, log-means,
-                            RK update, shock indicator and reductions                */
+    TRIXI_HIP_MIXED = 1, /* FP32 volume/surface arithmetic; FP64 state, the logarithms
+                            of the ln-means (per node, and their differences), RK
+                            update, shock indicator and reductions                   */
+    TRIXI_HIP_MIXED_F32LOG = 2 /* as TRIXI_HIP_MIXED with the ln-mean logarithms in
+                            FP32 as well (built for 3D Euler flux_ranocha + LLF, 2D
+                            Euler flux_ranocha + LLF with flux differencing or HG
+                            shock capturing, 3D GLM-MHD flux_hindenlang_gassner +
+                            LLF; periodic meshes): faster, less accurate             */
 };
 enum trixi_hip_initial_condition {
     TRIXI_HIP_IC_CONSTANT = 0,

@@ -15,7 +15,7 @@ EQ_LINADV, EQ_EULER, EQ_GLM_MHD, EQ_BURGERS, EQ_HYPDIFF, EQ_EULER_MULTI, EQ_EULE
 VI_WEAK, VI_FD, VI_HG = 0, 1, 2
 (FLUX_CENTRAL, FLUX_LLF, FLUX_LLF_NAIVE, FLUX_RANOCHA, FLUX_SHIMA_ETAL, FLUX_HINDENLANG_GASSNER, FLUX_EC, FLUX_HLL,
  FLUX_CHAN_ETAL, FLUX_HLLE, FLUX_WINTERMEYER_ETAL, FLUX_FJORDHOLM_ETAL, FLUX_HYDROSTATIC_LLF) = range(13)
-PREC_FP64, PREC_MIXED = 0, 1
+PREC_FP64, PREC_MIXED, PREC_MIXED_F32LOG = 0, 1, 2
 BC_PERIODIC, BC_DIRICHLET, BC_SLIP_WALL = 0, 1, 2
 SOURCE_NONE, SOURCE_CONVERGENCE_TEST, SOURCE_EOC_TEST_EULER, SOURCE_POISSON_NONPERIODIC, SOURCE_HARMONIC = range(5)
 (IC_CONSTANT, IC_CONVERGENCE_TEST, IC_WEAK_BLAST_WAVE, IC_SEEDED_RANDOM, IC_SMOOTH_PERIODIC,

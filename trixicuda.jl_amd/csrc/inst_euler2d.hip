@@ -11,6 +11,7 @@ void register_euler2d() {
     Registrar<2, 4, EQ, VI_WEAK, C, FLUX_CENTRAL, C>::add(EQ_EULER);
     Registrar<2, 4, EQ, VI_WEAK, C, FLUX_HLL, C>::add(EQ_EULER);
     Registrar<2, 4, EQ, VI_FD, FLUX_RANOCHA, FLUX_LLF, C>::add(EQ_EULER);
+    RegistrarF32Log<2, 4, EQ, VI_FD, FLUX_RANOCHA, FLUX_LLF, C>::add(EQ_EULER);
     Registrar<2, 4, EQ, VI_FD, FLUX_RANOCHA, FLUX_LLF_NAIVE, C>::add(EQ_EULER);
     Registrar<2, 4, EQ, VI_FD, FLUX_RANOCHA, FLUX_RANOCHA, C>::add(EQ_EULER);
     Registrar<2, 4, EQ, VI_FD, FLUX_SHIMA_ETAL, FLUX_LLF, C>::add(EQ_EULER);

@@ -1,9 +1,10 @@
 // physics.hpp -- device-side equations and numerical fluxes for the gfx950
 // DGSEM kernels.  Everything is templated on the arithmetic type R (double in
-// the FP64 mode, float in the mixed mode), the per-node logarithms inside
-// the entropy-conservative means included: the closed-form ln-mean is taken
-// only where |log y - log x| > 0.02, so FP32 logarithms add at most about
-// 5e-6 relative error to a mean in the mixed mode.
+// the FP64 mode, float in the mixed modes) and on the type L of the per-node
+// logarithms inside the entropy-conservative means (double in the FP64 and
+// mixed modes; float only in the mixed_f32log mode, where the closed-form
+// ln-mean, taken where |log y - log x| > 0.02, carries up to about 5e-6
+// relative error from FP32 logarithms).
 //
 // The flux algebra follows the Trixi.jl definitions that TrixiCUDA.jl's
 // kernels inline (reference: fluxes passed in by the tests, e.g.
@@ -126,12 +127,13 @@ template <typename R> __device__ __forceinline__ R dmax(R a, R b) { return b > a
 //            inv_ln_mean = P(f2) / (x + y);
 //   closed:  ln_mean     = (y - x) / (log y - log x), inv_ln_mean = 1 / that.
 //   The one reciprocal is of (x + y) in the series branch and of the
-//   closed-branch denominator otherwise.  The logarithms are taken in R:
-//   |log(y/x)| > 0.02 in the closed branch, so their difference is accurate
-//   to a few 1e-14 in FP64 and a few 1e-6 in FP32 (mixed mode).
+//   closed-branch denominator otherwise.  The logarithms and their
+//   difference are in L: |log(y/x)| > 0.02 in the closed branch, so the
+//   difference is accurate to a few 1e-14 with FP64 logarithms (then rounded
+//   once to R) and to a few 1e-6 with FP32 ones.
 // ---------------------------------------------------------------------------
-template <typename R>
-__device__ __forceinline__ R ln_mean_logs(R x, R y, R lx, R ly) {
+template <typename R, typename L>
+__device__ __forceinline__ R ln_mean_logs(R x, R y, L lx, L ly) {
     const R dxy = y - x, s = x + y;
     const bool series = dxy * dxy < R(1.0e-4) * (s * s);
     const R r = rrcp(series ? s : (R)(ly - lx));
@@ -140,8 +142,8 @@ __device__ __forceinline__ R ln_mean_logs(R x, R y, R lx, R ly) {
     return series ? s * q : z;
 }
 
-template <typename R>
-__device__ __forceinline__ R inv_ln_mean_logs(R x, R y, R lx, R ly) {
+template <typename R, typename L>
+__device__ __forceinline__ R inv_ln_mean_logs(R x, R y, L lx, L ly) {
     const R dxy = y - x, s = x + y;
     const bool series = dxy * dxy < R(1.0e-4) * (s * s);
     const R dl = (R)(ly - lx);
@@ -159,36 +161,36 @@ struct LinAdv {
     static constexpr int NV = 1;
     static constexpr bool NONCONS = false;
     static constexpr int NRF = 1;  // node record: u
-    template <typename R>
+    template <typename R, typename L = R>
     struct Node {
         R u;
-        R lrho, lp;  // unused
+        L lrho, lp;  // unused
         __device__ R cons(int) const { return u; }
     };
-    template <typename R, bool LOGS>
-    __device__ static Node<R> node(const R* u, const PhysR<R>&) {
-        Node<R> n;
+    template <typename R, bool LOGS, typename L = R>
+    __device__ static Node<R, L> node(const R* u, const PhysR<R>&) {
+        Node<R, L> n;
         n.u = u[0];
         n.lrho = n.lp = 0.0;
         return n;
     }
-    template <typename R>
-    __device__ static void to_rec(const Node<R>& n, R* f) {
+    template <typename R, typename L>
+    __device__ static void to_rec(const Node<R, L>& n, R* f) {
         f[0] = n.u;
     }
-    template <typename R>
-    __device__ static Node<R> from_rec(const R* f, R, R, const PhysR<R>&) {
-        Node<R> n;
+    template <typename R, typename L>
+    __device__ static Node<R, L> from_rec(const R* f, L, L, const PhysR<R>&) {
+        Node<R, L> n;
         n.u = f[0];
         n.lrho = n.lp = 0.0;
         return n;
     }
-    template <typename R>
-    __device__ static void flux(const Node<R>& n, int o, const PhysR<R>& P, R* f) {
+    template <typename R, typename L>
+    __device__ static void flux(const Node<R, L>& n, int o, const PhysR<R>& P, R* f) {
         f[0] = P.a[o] * n.u;
     }
-    template <int KIND, typename R>
-    __device__ static void two_point(const Node<R>& l, const Node<R>& r, int o, const PhysR<R>& P, R* f) {
+    template <int KIND, typename R, typename L>
+    __device__ static void two_point(const Node<R, L>& l, const Node<R, L>& r, int o, const PhysR<R>& P, R* f) {
         const R fl = P.a[o] * l.u, fr = P.a[o] * r.u;
         if (KIND == FLUX_LLF || KIND == FLUX_LLF_NAIVE) {
             const R lam = dabs(P.a[o]);
@@ -215,41 +217,41 @@ struct HypDiff {
     static constexpr int NV = DIM + 1;
     static constexpr bool NONCONS = false;
     static constexpr int NRF = DIM + 1;
-    template <typename R>
+    template <typename R, typename L = R>
     struct Node {
         R u[DIM + 1];
-        R lrho, lp;  // unused
+        L lrho, lp;  // unused
         __device__ R cons(int q) const { return u[q]; }
     };
-    template <typename R, bool LOGS>
-    __device__ static Node<R> node(const R* u, const PhysR<R>&) {
-        Node<R> n;
+    template <typename R, bool LOGS, typename L = R>
+    __device__ static Node<R, L> node(const R* u, const PhysR<R>&) {
+        Node<R, L> n;
 #pragma unroll
         for (int q = 0; q < NV; ++q) n.u[q] = u[q];
         n.lrho = n.lp = 0.0;
         return n;
     }
-    template <typename R>
-    __device__ static void to_rec(const Node<R>& n, R* f) {
+    template <typename R, typename L>
+    __device__ static void to_rec(const Node<R, L>& n, R* f) {
 #pragma unroll
         for (int q = 0; q < NV; ++q) f[q] = n.u[q];
     }
-    template <typename R>
-    __device__ static Node<R> from_rec(const R* f, R, R, const PhysR<R>&) {
-        Node<R> n;
+    template <typename R, typename L>
+    __device__ static Node<R, L> from_rec(const R* f, L, L, const PhysR<R>&) {
+        Node<R, L> n;
 #pragma unroll
         for (int q = 0; q < NV; ++q) n.u[q] = f[q];
         n.lrho = n.lp = 0.0;
         return n;
     }
-    template <typename R>
-    __device__ static void flux(const Node<R>& n, int o, const PhysR<R>& P, R* f) {
+    template <typename R, typename L>
+    __device__ static void flux(const Node<R, L>& n, int o, const PhysR<R>& P, R* f) {
         f[0] = -P.nu * n.u[1 + o];
 #pragma unroll
         for (int d = 0; d < DIM; ++d) f[1 + d] = (d == o) ? -n.u[0] * P.inv_Tr : R(0);
     }
-    template <int KIND, typename R>
-    __device__ static void two_point(const Node<R>& l, const Node<R>& r, int o, const PhysR<R>& P, R* f) {
+    template <int KIND, typename R, typename L>
+    __device__ static void two_point(const Node<R, L>& l, const Node<R, L>& r, int o, const PhysR<R>& P, R* f) {
         R fl[NV], fr[NV];
         flux(l, o, P, fl);
         flux(r, o, P, fr);
@@ -279,36 +281,36 @@ struct Burgers {
     static constexpr int NV = 1;
     static constexpr bool NONCONS = false;
     static constexpr int NRF = 1;
-    template <typename R>
+    template <typename R, typename L = R>
     struct Node {
         R u;
-        R lrho, lp;  // unused
+        L lrho, lp;  // unused
         __device__ R cons(int) const { return u; }
     };
-    template <typename R, bool LOGS>
-    __device__ static Node<R> node(const R* u, const PhysR<R>&) {
-        Node<R> n;
+    template <typename R, bool LOGS, typename L = R>
+    __device__ static Node<R, L> node(const R* u, const PhysR<R>&) {
+        Node<R, L> n;
         n.u = u[0];
         n.lrho = n.lp = 0.0;
         return n;
     }
-    template <typename R>
-    __device__ static void to_rec(const Node<R>& n, R* f) {
+    template <typename R, typename L>
+    __device__ static void to_rec(const Node<R, L>& n, R* f) {
         f[0] = n.u;
     }
-    template <typename R>
-    __device__ static Node<R> from_rec(const R* f, R, R, const PhysR<R>&) {
-        Node<R> n;
+    template <typename R, typename L>
+    __device__ static Node<R, L> from_rec(const R* f, L, L, const PhysR<R>&) {
+        Node<R, L> n;
         n.u = f[0];
         n.lrho = n.lp = 0.0;
         return n;
     }
-    template <typename R>
-    __device__ static void flux(const Node<R>& n, int, const PhysR<R>&, R* f) {
+    template <typename R, typename L>
+    __device__ static void flux(const Node<R, L>& n, int, const PhysR<R>&, R* f) {
         f[0] = R(0.5) * n.u * n.u;
     }
-    template <int KIND, typename R>
-    __device__ static void two_point(const Node<R>& l, const Node<R>& r, int, const PhysR<R>&, R* f) {
+    template <int KIND, typename R, typename L>
+    __device__ static void two_point(const Node<R, L>& l, const Node<R, L>& r, int, const PhysR<R>&, R* f) {
         if (KIND == FLUX_EC) {
             f[0] = (l.u * l.u + l.u * r.u + r.u * r.u) * R(1.0 / 6.0);
         } else {
@@ -335,10 +337,10 @@ struct Euler {
     static constexpr int NV = DIM + 2;
     static constexpr bool NONCONS = false;
     static constexpr int NRF = DIM + 3;  // node record: rho, v[DIM], p, c
-    template <typename R>
+    template <typename R, typename L = R>
     struct Node {
         R rho, v[DIM], p, c;  // primitives and sound speed
-        R lrho, lp;      // log(rho), log(p) for the entropy-conservative means
+        L lrho, lp;      // log(rho), log(p) for the entropy-conservative means
         R inv_gm1;
         __device__ R cons(int q) const {
             if (q == 0) return rho;
@@ -350,9 +352,9 @@ struct Euler {
         }
     };
 
-    template <typename R, bool LOGS>
-    __device__ static Node<R> node(const R* u, const PhysR<R>& P) {
-        Node<R> n;
+    template <typename R, bool LOGS, typename L = R>
+    __device__ static Node<R, L> node(const R* u, const PhysR<R>& P) {
+        Node<R, L> n;
         n.rho = u[0];
         const R inv_rho = rdiv(R(1), n.rho);
         R ke = R(0);
@@ -365,24 +367,24 @@ struct Euler {
         n.c = rsqrt_(P.gamma * n.p * inv_rho);
         n.inv_gm1 = P.inv_gm1;
         if (LOGS) {
-            n.lrho = log(n.rho);
-            n.lp = log(n.p);
+            n.lrho = log((L)n.rho);
+            n.lp = log((L)n.p);
         } else {
             n.lrho = n.lp = 0.0;
         }
         return n;
     }
-    template <typename R>
-    __device__ static void to_rec(const Node<R>& n, R* f) {
+    template <typename R, typename L>
+    __device__ static void to_rec(const Node<R, L>& n, R* f) {
         f[0] = n.rho;
 #pragma unroll
         for (int d = 0; d < DIM; ++d) f[1 + d] = n.v[d];
         f[DIM + 1] = n.p;
         f[DIM + 2] = n.c;
     }
-    template <typename R>
-    __device__ static Node<R> from_rec(const R* f, R lrho, R lp, const PhysR<R>& P) {
-        Node<R> n;
+    template <typename R, typename L>
+    __device__ static Node<R, L> from_rec(const R* f, L lrho, L lp, const PhysR<R>& P) {
+        Node<R, L> n;
         n.rho = f[0];
 #pragma unroll
         for (int d = 0; d < DIM; ++d) n.v[d] = f[1 + d];
@@ -394,8 +396,8 @@ struct Euler {
         return n;
     }
 
-    template <typename R>
-    __device__ static void flux(const Node<R>& n, int o, const PhysR<R>& P, R* f) {
+    template <typename R, typename L>
+    __device__ static void flux(const Node<R, L>& n, int o, const PhysR<R>& P, R* f) {
         const R rho_vo = n.rho * n.v[o];
         f[0] = rho_vo;
 #pragma unroll
@@ -403,8 +405,8 @@ struct Euler {
         f[DIM + 1] = (n.cons(DIM + 1) + n.p) * n.v[o];
     }
 
-    template <int KIND, typename R>
-    __device__ static void two_point(const Node<R>& l, const Node<R>& r, int o, const PhysR<R>& P, R* f) {
+    template <int KIND, typename R, typename L>
+    __device__ static void two_point(const Node<R, L>& l, const Node<R, L>& r, int o, const PhysR<R>& P, R* f) {
         if (KIND == FLUX_RANOCHA) {
             const R rho_mean = ln_mean_logs<R>(l.rho, r.rho, l.lrho, r.lrho);
             // p_ll p_rr inv_ln_mean(rho_ll p_rr, rho_rr p_ll), logs log(rho)+log(p)
@@ -534,11 +536,11 @@ struct EulerMulti {
     static constexpr int NV = DIM + 1 + NC;
     static constexpr bool NONCONS = false;
     static constexpr int NRF = NC + DIM + 2;  // node record: rho_k[NC], v[DIM], p, c
-    template <typename R>
+    template <typename R, typename L = R>
     struct Node {
         R rk[NC], v[DIM], p, c;
         R rho, inv_gm1;   // derived: total density, 1 / (gamma - 1)
-        R lrho, lp;  // log(rho), log(p)
+        L lrho, lp;  // log(rho), log(p)
         __device__ R cons(int q) const {
             if (q < DIM) return rho * v[q];
             if (q == DIM) {
@@ -563,9 +565,9 @@ struct EulerMulti {
         return rdiv(h1, h2);
     }
 
-    template <typename R, bool LOGS>
-    __device__ static Node<R> node(const R* u, const PhysR<R>& P) {
-        Node<R> n;
+    template <typename R, bool LOGS, typename L = R>
+    __device__ static Node<R, L> node(const R* u, const PhysR<R>& P) {
+        Node<R, L> n;
         R rho = R(0);
 #pragma unroll
         for (int k = 0; k < NC; ++k) {
@@ -585,15 +587,15 @@ struct EulerMulti {
         n.c = rsqrt_(gamma * n.p * inv_rho);
         n.inv_gm1 = rdiv(R(1), gamma - R(1));
         if (LOGS) {
-            n.lrho = log(n.rho);
-            n.lp = log(n.p);
+            n.lrho = log((L)n.rho);
+            n.lp = log((L)n.p);
         } else {
             n.lrho = n.lp = 0.0;
         }
         return n;
     }
-    template <typename R>
-    __device__ static void to_rec(const Node<R>& n, R* f) {
+    template <typename R, typename L>
+    __device__ static void to_rec(const Node<R, L>& n, R* f) {
 #pragma unroll
         for (int k = 0; k < NC; ++k) f[k] = n.rk[k];
 #pragma unroll
@@ -601,9 +603,9 @@ struct EulerMulti {
         f[NC + DIM] = n.p;
         f[NC + DIM + 1] = n.c;
     }
-    template <typename R>
-    __device__ static Node<R> from_rec(const R* f, R lrho, R lp, const PhysR<R>& P) {
-        Node<R> n;
+    template <typename R, typename L>
+    __device__ static Node<R, L> from_rec(const R* f, L lrho, L lp, const PhysR<R>& P) {
+        Node<R, L> n;
         R rho = R(0);
 #pragma unroll
         for (int k = 0; k < NC; ++k) {
@@ -620,8 +622,8 @@ struct EulerMulti {
         n.lp = lp;
         return n;
     }
-    template <typename R>
-    __device__ static void flux(const Node<R>& n, int o, const PhysR<R>& P, R* f) {
+    template <typename R, typename L>
+    __device__ static void flux(const Node<R, L>& n, int o, const PhysR<R>& P, R* f) {
 #pragma unroll
         for (int d = 0; d < DIM; ++d) f[d] = (d == o) ? n.rho * n.v[d] * n.v[o] + n.p : n.rho * n.v[d] * n.v[o];
         f[DIM] = (n.cons(DIM) + n.p) * n.v[o];
@@ -629,14 +631,14 @@ struct EulerMulti {
         for (int k = 0; k < NC; ++k) f[DIM + 1 + k] = n.rk[k] * n.v[o];
     }
 
-    template <int KIND, typename R>
-    __device__ static void two_point(const Node<R>& l, const Node<R>& r, int o, const PhysR<R>& P, R* f) {
+    template <int KIND, typename R, typename L>
+    __device__ static void two_point(const Node<R, L>& l, const Node<R, L>& r, int o, const PhysR<R>& P, R* f) {
         if (KIND == FLUX_RANOCHA) {
             R f_rho_sum = R(0), ravg[NC];
             const R vo_avg = R(0.5) * (l.v[o] + r.v[o]);
 #pragma unroll
             for (int k = 0; k < NC; ++k) {
-                const R fk = ln_mean_logs<R>(l.rk[k], r.rk[k], log(l.rk[k]), log(r.rk[k])) * vo_avg;
+                const R fk = ln_mean_logs<R>(l.rk[k], r.rk[k], log((L)l.rk[k]), log((L)r.rk[k])) * vo_avg;
                 f[DIM + 1 + k] = fk;
                 f_rho_sum += fk;
                 ravg[k] = R(0.5) * (l.rk[k] + r.rk[k]);
@@ -698,10 +700,10 @@ struct EulerQuasi1D {
     static constexpr int NV = 4;
     static constexpr bool NONCONS = true;
     static constexpr int NRF = 5;  // node record: rho, v, p, c, a
-    template <typename R>
+    template <typename R, typename L = R>
     struct Node {
         R rho, v[1], p, c, a;
-        R lrho, lp;
+        L lrho, lp;
         R inv_gm1;
         __device__ R cons(int q) const {
             if (q == 0) return a * rho;
@@ -710,9 +712,9 @@ struct EulerQuasi1D {
             return a;
         }
     };
-    template <typename R, bool LOGS>
-    __device__ static Node<R> node(const R* u, const PhysR<R>& P) {
-        Node<R> n;
+    template <typename R, bool LOGS, typename L = R>
+    __device__ static Node<R, L> node(const R* u, const PhysR<R>& P) {
+        Node<R, L> n;
         n.a = u[3];
         const R inv_a = rdiv(R(1), n.a);
         n.rho = u[0] * inv_a;
@@ -722,24 +724,24 @@ struct EulerQuasi1D {
         n.c = rsqrt_(P.gamma * n.p * rdiv(R(1), n.rho));
         n.inv_gm1 = P.inv_gm1;
         if (LOGS) {
-            n.lrho = log(n.rho);
-            n.lp = log(n.p);
+            n.lrho = log((L)n.rho);
+            n.lp = log((L)n.p);
         } else {
             n.lrho = n.lp = 0.0;
         }
         return n;
     }
-    template <typename R>
-    __device__ static void to_rec(const Node<R>& n, R* f) {
+    template <typename R, typename L>
+    __device__ static void to_rec(const Node<R, L>& n, R* f) {
         f[0] = n.rho;
         f[1] = n.v[0];
         f[2] = n.p;
         f[3] = n.c;
         f[4] = n.a;
     }
-    template <typename R>
-    __device__ static Node<R> from_rec(const R* f, R lrho, R lp, const PhysR<R>& P) {
-        Node<R> n;
+    template <typename R, typename L>
+    __device__ static Node<R, L> from_rec(const R* f, L lrho, L lp, const PhysR<R>& P) {
+        Node<R, L> n;
         n.rho = f[0];
         n.v[0] = f[1];
         n.p = f[2];
@@ -750,16 +752,16 @@ struct EulerQuasi1D {
         n.inv_gm1 = P.inv_gm1;
         return n;
     }
-    template <typename R>
-    __device__ static void flux(const Node<R>& n, int, const PhysR<R>& P, R* f) {
+    template <typename R, typename L>
+    __device__ static void flux(const Node<R, L>& n, int, const PhysR<R>& P, R* f) {
         const R arv = n.a * n.rho * n.v[0];
         f[0] = arv;
         f[1] = arv * n.v[0];
         f[2] = n.a * n.v[0] * (n.p * n.inv_gm1 + R(0.5) * n.rho * n.v[0] * n.v[0] + n.p);
         f[3] = R(0);
     }
-    template <int KIND, typename R>
-    __device__ static void two_point(const Node<R>& l, const Node<R>& r, int o, const PhysR<R>& P, R* f) {
+    template <int KIND, typename R, typename L>
+    __device__ static void two_point(const Node<R, L>& l, const Node<R, L>& r, int o, const PhysR<R>& P, R* f) {
         if (KIND == FLUX_CHAN_ETAL) {
             const R rho_mean = ln_mean_logs<R>(l.rho, r.rho, l.lrho, r.lrho);
             const R inv_rho_p_mean =
@@ -788,8 +790,8 @@ struct EulerQuasi1D {
             }
         }
     }
-    template <int POS, typename R>
-    __device__ static void noncons(const Node<R>& l, const Node<R>& r, int, const PhysR<R>&, R* f) {
+    template <int POS, typename R, typename L>
+    __device__ static void noncons(const Node<R, L>& l, const Node<R, L>& r, int, const PhysR<R>&, R* f) {
         f[0] = R(0);
         f[1] = l.a * r.p;
         f[2] = R(0);
@@ -817,19 +819,19 @@ struct ShallowWater {
     static constexpr int NV = DIM + 2;
     static constexpr bool NONCONS = true;
     static constexpr int NRF = DIM + 2;  // node record: h, v[DIM], b
-    template <typename R>
+    template <typename R, typename L = R>
     struct Node {
         R h, v[DIM], b;
-        R lrho, lp;  // unused
+        L lrho, lp;  // unused
         __device__ R cons(int q) const {
             if (q == 0) return h;
             if (q <= DIM) return h * v[q - 1];
             return b;
         }
     };
-    template <typename R, bool LOGS>
-    __device__ static Node<R> node(const R* u, const PhysR<R>&) {
-        Node<R> n;
+    template <typename R, bool LOGS, typename L = R>
+    __device__ static Node<R, L> node(const R* u, const PhysR<R>&) {
+        Node<R, L> n;
         n.h = u[0];
         const R inv_h = rdiv(R(1), n.h);
 #pragma unroll
@@ -838,16 +840,16 @@ struct ShallowWater {
         n.lrho = n.lp = 0.0;
         return n;
     }
-    template <typename R>
-    __device__ static void to_rec(const Node<R>& n, R* f) {
+    template <typename R, typename L>
+    __device__ static void to_rec(const Node<R, L>& n, R* f) {
         f[0] = n.h;
 #pragma unroll
         for (int d = 0; d < DIM; ++d) f[1 + d] = n.v[d];
         f[DIM + 1] = n.b;
     }
-    template <typename R>
-    __device__ static Node<R> from_rec(const R* f, R, R, const PhysR<R>&) {
-        Node<R> n;
+    template <typename R, typename L>
+    __device__ static Node<R, L> from_rec(const R* f, L, L, const PhysR<R>&) {
+        Node<R, L> n;
         n.h = f[0];
 #pragma unroll
         for (int d = 0; d < DIM; ++d) n.v[d] = f[1 + d];
@@ -855,8 +857,8 @@ struct ShallowWater {
         n.lrho = n.lp = 0.0;
         return n;
     }
-    template <typename R>
-    __device__ static void flux(const Node<R>& n, int o, const PhysR<R>& P, R* f) {
+    template <typename R, typename L>
+    __device__ static void flux(const Node<R, L>& n, int o, const PhysR<R>& P, R* f) {
         const R hvo = n.h * n.v[o];
         const R p = R(0.5) * P.grav * n.h * n.h;
         f[0] = hvo;
@@ -865,8 +867,8 @@ struct ShallowWater {
         f[DIM + 1] = R(0);
     }
     // LLF of two states, b not dissipated
-    template <int KIND, typename R>
-    __device__ static void llf(const Node<R>& l, const Node<R>& r, int o, const PhysR<R>& P, R* f) {
+    template <int KIND, typename R, typename L>
+    __device__ static void llf(const Node<R, L>& l, const Node<R, L>& r, int o, const PhysR<R>& P, R* f) {
         R fl[NV], fr[NV];
         flux(l, o, P, fl);
         flux(r, o, P, fr);
@@ -881,8 +883,8 @@ struct ShallowWater {
         for (int q = 0; q <= DIM; ++q) f[q] = R(0.5) * (fl[q] + fr[q]) + hl * (r.cons(q) - l.cons(q));
         f[DIM + 1] = R(0);
     }
-    template <int KIND, typename R>
-    __device__ static void two_point(const Node<R>& l, const Node<R>& r, int o, const PhysR<R>& P, R* f) {
+    template <int KIND, typename R, typename L>
+    __device__ static void two_point(const Node<R, L>& l, const Node<R, L>& r, int o, const PhysR<R>& P, R* f) {
         if (KIND == FLUX_WINTERMEYER_ETAL || KIND == FLUX_FJORDHOLM_ETAL) {
             R f0, p_avg;
             if (KIND == FLUX_WINTERMEYER_ETAL) {
@@ -901,7 +903,7 @@ struct ShallowWater {
             // hydrostatic_reconstruction_audusse_etal): LLF of the states with
             // h* = max(0, h + b - max(b_l, b_r)), the velocities kept
             const R bmax = dmax(l.b, r.b);
-            Node<R> ls = l, rs = r;
+            Node<R, L> ls = l, rs = r;
             ls.h = dmax(R(0), l.h + l.b - bmax);
             rs.h = dmax(R(0), r.h + r.b - bmax);
             llf<FLUX_LLF>(ls, rs, o, P, f);
@@ -917,8 +919,8 @@ struct ShallowWater {
     }
     // POS < 0: the volume term (flux_nonconservative_wintermeyer_etal); else
     // the term paired with surface / subcell flux POS
-    template <int POS, typename R>
-    __device__ static void noncons(const Node<R>& l, const Node<R>& r, int o, const PhysR<R>& P, R* f) {
+    template <int POS, typename R, typename L>
+    __device__ static void noncons(const Node<R, L>& l, const Node<R, L>& r, int o, const PhysR<R>& P, R* f) {
         R t;
         if (POS < 0) {
             t = P.grav * l.h * r.b;
@@ -955,10 +957,10 @@ struct GlmMhd {
     static constexpr int NV = 9;
     static constexpr bool NONCONS = true;
     static constexpr int NRF = 9;  // node record: rho, v[3], p, B[3], psi
-    template <typename R>
+    template <typename R, typename L = R>
     struct Node {
         R rho, v[3], p, B[3], psi;
-        R lrho, lp;
+        L lrho, lp;
         R inv_gm1;
         __device__ R cons(int q) const {
             if (q == 0) return rho;
@@ -970,9 +972,9 @@ struct GlmMhd {
             return psi;
         }
     };
-    template <typename R, bool LOGS>
-    __device__ static Node<R> node(const R* u, const PhysR<R>& P) {
-        Node<R> n;
+    template <typename R, bool LOGS, typename L = R>
+    __device__ static Node<R, L> node(const R* u, const PhysR<R>& P) {
+        Node<R, L> n;
         n.rho = u[0];
         const R inv_rho = rdiv(R(1), n.rho);
         n.v[0] = u[1] * inv_rho;
@@ -985,15 +987,15 @@ struct GlmMhd {
                        R(0.5) * (u[5] * u[5] + u[6] * u[6] + u[7] * u[7]) - R(0.5) * u[8] * u[8]);
         n.inv_gm1 = P.inv_gm1;
         if (LOGS) {
-            n.lrho = log(n.rho);
-            n.lp = log(n.p);
+            n.lrho = log((L)n.rho);
+            n.lp = log((L)n.p);
         } else {
             n.lrho = n.lp = 0.0;
         }
         return n;
     }
-    template <typename R>
-    __device__ static void to_rec(const Node<R>& n, R* f) {
+    template <typename R, typename L>
+    __device__ static void to_rec(const Node<R, L>& n, R* f) {
         f[0] = n.rho;
 #pragma unroll
         for (int d = 0; d < 3; ++d) f[1 + d] = n.v[d];
@@ -1002,9 +1004,9 @@ struct GlmMhd {
         for (int d = 0; d < 3; ++d) f[5 + d] = n.B[d];
         f[8] = n.psi;
     }
-    template <typename R>
-    __device__ static Node<R> from_rec(const R* f, R lrho, R lp, const PhysR<R>& P) {
-        Node<R> n;
+    template <typename R, typename L>
+    __device__ static Node<R, L> from_rec(const R* f, L lrho, L lp, const PhysR<R>& P) {
+        Node<R, L> n;
         n.rho = f[0];
 #pragma unroll
         for (int d = 0; d < 3; ++d) n.v[d] = f[1 + d];
@@ -1018,8 +1020,8 @@ struct GlmMhd {
         return n;
     }
 
-    template <typename R>
-    __device__ static void flux(const Node<R>& n, int o, const PhysR<R>& P, R* f) {
+    template <typename R, typename L>
+    __device__ static void flux(const Node<R, L>& n, int o, const PhysR<R>& P, R* f) {
         const R rho = n.rho;
         const R* v = n.v;
         const R* B = n.B;
@@ -1040,8 +1042,8 @@ struct GlmMhd {
         f[8] = P.c_h * B[o];
     }
 
-    template <typename R>
-    __device__ static R fast_speed(const Node<R>& n, int o, const PhysR<R>& P) {
+    template <typename R, typename L>
+    __device__ static R fast_speed(const Node<R, L>& n, int o, const PhysR<R>& P) {
         const R inv_rho = rdiv(R(1), n.rho);
         const R a_square = P.gamma * n.p * inv_rho;
         const R b_square = (n.B[0] * n.B[0] + n.B[1] * n.B[1] + n.B[2] * n.B[2]) * inv_rho;
@@ -1055,8 +1057,8 @@ struct GlmMhd {
     // and the total enthalpy H = (rho e + p + |B|^2/2) / rho, the jump term
     // X = |[[B]]|^2 / (2 (sqrt(rho_l) + sqrt(rho_r))^2), b^2 = |B|^2 / sqrt(rho_l rho_r),
     // a^2 = (2 - gamma) X + (gamma - 1)(H - |v|^2/2 - b^2)
-    template <typename R>
-    __device__ static void roe_fast_speed(const Node<R>& l, const Node<R>& r, int o, const PhysR<R>& P, R& vel,
+    template <typename R, typename L>
+    __device__ static void roe_fast_speed(const Node<R, L>& l, const Node<R, L>& r, int o, const PhysR<R>& P, R& vel,
                                           R& cf) {
         const R sl = rsqrt_(l.rho), sr = rsqrt_(r.rho);
         const R inv_add = rdiv(R(1), sl + sr), inv_prod = rdiv(R(1), sl * sr);
@@ -1087,8 +1089,8 @@ struct GlmMhd {
         cf = rsqrt_(R(0.5) * (a2 + b2 + astar));
     }
 
-    template <int KIND, typename R>
-    __device__ static void two_point(const Node<R>& l, const Node<R>& r, int o, const PhysR<R>& P, R* f) {
+    template <int KIND, typename R, typename L>
+    __device__ static void two_point(const Node<R, L>& l, const Node<R, L>& r, int o, const PhysR<R>& P, R* f) {
         if (KIND == FLUX_HINDENLANG_GASSNER) {
             const R* Bl = l.B;
             const R* Br = r.B;
@@ -1169,8 +1171,8 @@ struct GlmMhd {
 
     // flux_nonconservative_powell(u_ll, u_rr, o): Powell (0, B, v.B, v, 0) and
     // Galilean GLM terms of the left state times the right state's B_o / psi.
-    template <int POS, typename R>
-    __device__ static void noncons(const Node<R>& l, const Node<R>& r, int o, const PhysR<R>&, R* f) {
+    template <int POS, typename R, typename L>
+    __device__ static void noncons(const Node<R, L>& l, const Node<R, L>& r, int o, const PhysR<R>&, R* f) {
         const R v_dot_B_ll = l.v[0] * l.B[0] + l.v[1] * l.B[1] + l.v[2] * l.B[2];
         const R Bo_rr = r.B[o];
         const R psi_rr = r.psi;
@@ -1216,40 +1218,40 @@ struct GlmMhd1D {
     static constexpr int NV = 8;
     static constexpr bool NONCONS = false;
     static constexpr int NRF = GlmMhd::NRF;
-    template <typename R>
-    using Node = GlmMhd::Node<R>;
-    template <typename R, bool LOGS>
-    __device__ static Node<R> node(const R* u, const PhysR<R>& P) {
+    template <typename R, typename L = R>
+    using Node = GlmMhd::Node<R, L>;
+    template <typename R, bool LOGS, typename L = R>
+    __device__ static Node<R, L> node(const R* u, const PhysR<R>& P) {
         R u9[9];
 #pragma unroll
         for (int q = 0; q < 8; ++q) u9[q] = u[q];
         u9[8] = R(0);
-        return GlmMhd::template node<R, LOGS>(u9, P);
+        return GlmMhd::template node<R, LOGS, L>(u9, P);
     }
-    template <typename R>
-    __device__ static void to_rec(const Node<R>& n, R* f) {
+    template <typename R, typename L>
+    __device__ static void to_rec(const Node<R, L>& n, R* f) {
         GlmMhd::to_rec(n, f);
     }
-    template <typename R>
-    __device__ static Node<R> from_rec(const R* f, R lrho, R lp, const PhysR<R>& P) {
+    template <typename R, typename L>
+    __device__ static Node<R, L> from_rec(const R* f, L lrho, L lp, const PhysR<R>& P) {
         return GlmMhd::from_rec(f, lrho, lp, P);
     }
-    template <typename R>
-    __device__ static void flux(const Node<R>& n, int o, const PhysR<R>& P, R* f) {
+    template <typename R, typename L>
+    __device__ static void flux(const Node<R, L>& n, int o, const PhysR<R>& P, R* f) {
         R f9[9];
         GlmMhd::flux(n, o, P, f9);
 #pragma unroll
         for (int q = 0; q < 8; ++q) f[q] = f9[q];
     }
-    template <int KIND, typename R>
-    __device__ static void two_point(const Node<R>& l, const Node<R>& r, int o, const PhysR<R>& P, R* f) {
+    template <int KIND, typename R, typename L>
+    __device__ static void two_point(const Node<R, L>& l, const Node<R, L>& r, int o, const PhysR<R>& P, R* f) {
         R f9[9];
         GlmMhd::template two_point<KIND>(l, r, o, P, f9);
 #pragma unroll
         for (int q = 0; q < 8; ++q) f[q] = f9[q];
     }
-    template <int POS, typename R>
-    __device__ static void noncons(const Node<R>&, const Node<R>&, int, const PhysR<R>&, R* f) {
+    template <int POS, typename R, typename L>
+    __device__ static void noncons(const Node<R, L>&, const Node<R, L>&, int, const PhysR<R>&, R* f) {
 #pragma unroll
         for (int q = 0; q < 8; ++q) f[q] = R(0);
     }

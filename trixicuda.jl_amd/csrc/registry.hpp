@@ -77,10 +77,11 @@ struct VjpRegistrar {
 #ifndef TRIXI_HIP_MAX_THREADS
 #define TRIXI_HIP_MAX_THREADS 256
 #endif
-template <int DIM, int N, int NV, int NRF, int RS>
+// RS: bytes of the arithmetic type; LS: bytes of the staged logarithms.
+template <int DIM, int N, int NV, int NRF, int RS, int LS>
 constexpr int pick_m() {
     constexpr int NL = cpow(N, DIM - 1), NN = cpow(N, DIM);
-    constexpr int per_elem = NN * ((TRIXI_HIP_STAGE_NODES ? (NRF + 2) * RS : 0) + (DIM > 1 ? NV * RS : 0));
+    constexpr int per_elem = NN * ((TRIXI_HIP_STAGE_NODES ? NRF * RS + 2 * LS : 0) + (DIM > 1 ? NV * RS : 0));
     int m = TRIXI_HIP_MAX_THREADS / NL;
     if (m < 1) m = 1;
     while (m > 1 && m * per_elem > TRIXI_HIP_LDS_BUDGET_KIB * 1024) m /= 2;
@@ -90,8 +91,8 @@ constexpr int pick_m() {
 template <int DIM, int N, class EQ, int VOL, int VF, int SF, int FVF>
 struct Registrar {
     static void add(int eq) {
-        constexpr int MD = pick_m<DIM, N, EQ::NV, EQ::NRF, 8>();
-        constexpr int MF = pick_m<DIM, N, EQ::NV, EQ::NRF, 4>();
+        constexpr int MD = pick_m<DIM, N, EQ::NV, EQ::NRF, 8, 8>();
+        constexpr int MF = pick_m<DIM, N, EQ::NV, EQ::NRF, 4, 8>();
         register_kernel({DIM, N, eq, VOL, VF, SF, FVF, 0, 0},
                         &launch_rhs<DIM, N, EQ, VOL, VF, SF, FVF, double, MD, false>);
         register_kernel({DIM, N, eq, VOL, VF, SF, FVF, 0, 1},
@@ -107,8 +108,8 @@ struct Registrar {
 template <int DIM, int N, class EQ, int VOL, int VF, int SF, int FVF>
 struct RegistrarExt {
     static void add(int eq) {
-        constexpr int MD = pick_m<DIM, N, EQ::NV, EQ::NRF, 8>();
-        constexpr int MF = pick_m<DIM, N, EQ::NV, EQ::NRF, 4>();
+        constexpr int MD = pick_m<DIM, N, EQ::NV, EQ::NRF, 8, 8>();
+        constexpr int MF = pick_m<DIM, N, EQ::NV, EQ::NRF, 4, 8>();
         register_kernel({DIM, N, eq, VOL, VF, SF, FVF, 0, 0, 1},
                         &launch_rhs<DIM, N, EQ, VOL, VF, SF, FVF, double, MD, false, true>);
         register_kernel({DIM, N, eq, VOL, VF, SF, FVF, 0, 1, 1},
@@ -117,6 +118,20 @@ struct RegistrarExt {
                         &launch_rhs<DIM, N, EQ, VOL, VF, SF, FVF, float, MF, false, true>);
         register_kernel({DIM, N, eq, VOL, VF, SF, FVF, 1, 1, 1},
                         &launch_rhs<DIM, N, EQ, VOL, VF, SF, FVF, float, MF, true, true>);
+    }
+};
+
+// Precision 2 (TRIXI_HIP_MIXED_F32LOG): the mixed mode with the per-node
+// logarithms of the ln-means taken and staged in FP32, for the configurations
+// whose fluxes take ln-means (periodic meshes).
+template <int DIM, int N, class EQ, int VOL, int VF, int SF, int FVF>
+struct RegistrarF32Log {
+    static void add(int eq) {
+        constexpr int MF = pick_m<DIM, N, EQ::NV, EQ::NRF, 4, 4>();
+        register_kernel({DIM, N, eq, VOL, VF, SF, FVF, 2, 0},
+                        &launch_rhs<DIM, N, EQ, VOL, VF, SF, FVF, float, MF, false, false, float>);
+        register_kernel({DIM, N, eq, VOL, VF, SF, FVF, 2, 1},
+                        &launch_rhs<DIM, N, EQ, VOL, VF, SF, FVF, float, MF, true, false, float>);
     }
 };
 

@@ -196,7 +196,8 @@ __host__ __device__ constexpr int rhs_threads() {
 // EXT: the variant with boundary faces (nbr = -1, Dirichlet states), L2
 // mortar faces (nbr = -2 - slot, fluxes from the mortar pass) and source
 // terms; the periodic, conforming, source-free kernels carry none of that code.
-template <int DIM, int N, class EQ, int VOL, int VF, int SF, int FVF, typename R, int M, bool RK, bool EXT>
+template <int DIM, int N, class EQ, int VOL, int VF, int SF, int FVF, typename R, int M, bool RK, bool EXT,
+          typename L>
 __global__ void __launch_bounds__((rhs_threads<DIM, N, M, R, RK, EQ::NV>()), TRIXI_HIP_MIN_WAVES)
 rhs_kernel(const RhsArgs a) {
     constexpr int NL = cpow(N, DIM - 1);
@@ -215,7 +216,7 @@ rhs_kernel(const RhsArgs a) {
     constexpr bool FV_LOGS = (VOL == VI_HG) && flux_needs_logs(FVF);
     constexpr bool NODE_LOGS = SF_LOGS || VF_LOGS || FV_LOGS;
     constexpr int NDF = NODE_LOGS ? 2 : 0;
-    using Node = typename EQ::template Node<R>;
+    using Node = typename EQ::template Node<R, L>;
     // mixed precision weak form with N = 4 on whole waves: D-hat on MFMA
     constexpr bool WEAK_MFMA = TRIXI_HIP_WEAK_MFMA && VOL == VI_WEAK && std::is_same<R, float>::value &&
                                N == 4 && (M * NL) % 64 == 0;
@@ -229,7 +230,7 @@ rhs_kernel(const RhsArgs a) {
     // from the accumulator, instead of re-reading u and k line by line.
     constexpr bool COOP = RK && STAGE;
     __shared__ R rec[STAGE ? NRF * NN * LDM : 1];
-    __shared__ R recd[(STAGE && NDF > 0) ? NDF * NN * LDM : 1];
+    __shared__ L recd[(STAGE && NDF > 0) ? NDF * NN * LDM : 1];
     __shared__ R acc[(DIM > 1 || COOP) ? NV * NN * LDM : 1];
 
     const int tid = threadIdx.x;
@@ -301,7 +302,7 @@ rhs_kernel(const RhsArgs a) {
                 kkeep[it][v] = a.rk_first ? 0.0 : a.k[g];
             }
         }
-        const Node nd = EQ::template node<R, NODE_LOGS>(uc, P);
+        const Node nd = EQ::template node<R, NODE_LOGS, L>(uc, P);
         R f[NRF];
         EQ::to_rec(nd, f);
 #pragma unroll
@@ -429,14 +430,14 @@ rhs_kernel(const RhsArgs a) {
                 R f[NRF];
 #pragma unroll
                 for (int k = 0; k < NRF; ++k) f[k] = rec[(k * NN + node) * LDM + el];
-                const R lr = NODE_LOGS ? recd[node * LDM + el] : R(0);
-                const R lp = NODE_LOGS ? recd[(NN + node) * LDM + el] : R(0);
+                const L lr = NODE_LOGS ? recd[node * LDM + el] : L(0);
+                const L lp = NODE_LOGS ? recd[(NN + node) * LDM + el] : L(0);
                 nd[p] = EQ::from_rec(f, lr, lp, P);
             } else {
                 R uc[NV];
 #pragma unroll
                 for (int v = 0; v < NV; ++v) uc[v] = (R)a.u[(int64_t)(v * NN + node) * E + ec];
-                nd[p] = EQ::template node<R, NODE_LOGS>(uc, P);
+                nd[p] = EQ::template node<R, NODE_LOGS, L>(uc, P);
             }
         }
 
@@ -545,8 +546,8 @@ rhs_kernel(const RhsArgs a) {
 
         // ---- interface fluxes at both ends of the line, surface integral ----
         {
-            const Node ndm = EQ::template node<R, SF_LOGS>(um, P);
-            const Node ndp = EQ::template node<R, SF_LOGS>(up, P);
+            const Node ndm = EQ::template node<R, SF_LOGS, L>(um, P);
+            const Node ndp = EQ::template node<R, SF_LOGS, L>(up, P);
             R fm[NV], fp[NV];
             EQ::template two_point<SF>(ndm, nd[0], d, P, fm);       // this element is "right"
             EQ::template two_point<SF>(nd[N - 1], ndp, d, P, fp);   // this element is "left"
@@ -664,12 +665,13 @@ rhs_kernel(const RhsArgs a) {
 }
 
 // Launch helper used by the instantiation units.
-template <int DIM, int N, class EQ, int VOL, int VF, int SF, int FVF, typename R, int M, bool RK, bool EXT = false>
+template <int DIM, int N, class EQ, int VOL, int VF, int SF, int FVF, typename R, int M, bool RK, bool EXT = false,
+          typename L = double>
 hipError_t launch_rhs(const RhsArgs& a, hipStream_t stream) {
     constexpr int NL = cpow(N, DIM - 1);
     int64_t blocks = (a.E + M - 1) / M;
     if (a.nxcd > 1) blocks = (blocks + a.nxcd - 1) / a.nxcd * a.nxcd;
-    hipLaunchKernelGGL((rhs_kernel<DIM, N, EQ, VOL, VF, SF, FVF, R, M, RK, EXT>), dim3((unsigned)blocks),
+    hipLaunchKernelGGL((rhs_kernel<DIM, N, EQ, VOL, VF, SF, FVF, R, M, RK, EXT, L>), dim3((unsigned)blocks),
                        dim3(rhs_threads<DIM, N, M, R, RK, EQ::NV>()), 0, stream, a);
     (void)NL;
     return hipGetLastError();

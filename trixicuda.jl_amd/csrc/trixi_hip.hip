@@ -860,7 +860,7 @@ struct trixi_hip_ctx {
     double* d_Vdm = nullptr;
     double* d_wa = nullptr;
     double* d_w = nullptr;
-    double* d_tmp = nullptr;  // RK ping-pong state
+    double* d_tmp = nullptr;  // RK stage states (two buffers of nvalues)
     int nblocks_red = 1024;
     RhsArgs args;
     LaunchFn rhs_fn = nullptr, rk_fn = nullptr;
@@ -1277,7 +1277,7 @@ int trixi_hip_create(const trixi_hip_config* config, const trixi_hip_mesh* mesh,
     ALLOC(c->d_Vdm, c->NA * N * sizeof(double));
     ALLOC(c->d_wa, c->NA * sizeof(double));
     ALLOC(c->d_w, N * sizeof(double));
-    ALLOC(c->d_tmp, (size_t)c->nvalues * sizeof(double));
+    ALLOC(c->d_tmp, 2 * (size_t)c->nvalues * sizeof(double));
     if (NM > 0) {
         const size_t slots = (size_t)NM * (K + 1);
         ALLOC(c->d_mids, slots * sizeof(int32_t));
@@ -1472,13 +1472,19 @@ int trixi_hip_compute_coefficients(trixi_hip_ctx* ctx, double* u, int32_t ic, do
 }
 
 // The five stages of CarpenterKennedy2N54, each one fused rhs+update
-// launch (plus the HG indicator when shock capturing).  Stages alternate
-// between the caller's u and the context's ping-pong buffer; the last
-// stage's result is copied back to u.
+// launch (plus the HG indicator when shock capturing).  A stage reads its
+// input state while other workgroups still read that state's face nodes, so
+// it cannot update in place: stage 0 reads u and writes buffer A, then the
+// stages alternate A -> B -> A -> B, and stage 4 reads B and writes the
+// caller's u, which no stage after 0 reads.
 static int enqueue_rk_stages(trixi_hip_ctx* c, double* u, double* k, hipStream_t s) {
-    double* cur = u;
-    double* nxt = c->d_tmp;
+    double* bufA = c->d_tmp;
+    double* bufB = c->d_tmp + c->nvalues;
+    double* const in[5] = {u, bufA, bufB, bufA, bufB};
+    double* const out[5] = {bufA, bufB, bufA, bufB, u};
     for (int st = 0; st < 5; ++st) {
+        double* cur = in[st];
+        double* nxt = out[st];
         if (c->cfg.volume_integral == TRIXI_HIP_VOLUME_SHOCK_CAPTURING_HG) {
             int rc = launch_indicator(c, cur, s);
             if (rc) return rc;
@@ -1496,13 +1502,6 @@ static int enqueue_rk_stages(trixi_hip_ctx* c, double* u, double* k, hipStream_t
         a.rk_b = CK_B[st];
         a.rk_first = (st == 0);
         HIP_TRY(c->rk_fn(a, s));
-        double* t = cur;
-        cur = nxt;
-        nxt = t;
-    }
-    if (cur != u) {
-        hipLaunchKernelGGL(copy_kernel, dim3(1024), dim3(256), 0, s, u, cur, c->nvalues);
-        HIP_TRY(hipGetLastError());
     }
     return TRIXI_HIP_OK;
 }
@@ -1743,10 +1742,21 @@ static int require_vjp(trixi_hip_ctx* c) {
                     "no adjoint kernel for this configuration (built: linear advection, compressible "
                     "Euler (weak form, flux differencing, HG shock capturing) and GLM-MHD flux differencing, "
                     "polydeg 3)");
+    // the reverse indicator (indicator_vjp_kernel) is built for polydeg 3 and
+    // the compressible Euler variables only
+    if (is_hg(c) && (c->N != 4 || c->cfg.equation != TRIXI_HIP_COMPRESSIBLE_EULER))
+        return fail(TRIXI_HIP_EUNSUPPORTED, "the shock-capturing adjoint is built for compressible Euler, polydeg 3");
     if (is_hg(c) && !c->d_alphabar) {
-        if (hipMalloc((void**)&c->d_alphabar, (size_t)c->E * sizeof(double)) != hipSuccess ||
-            hipMalloc((void**)&c->d_winner, (size_t)c->E * sizeof(int32_t)) != hipSuccess)
+        if (hipMalloc((void**)&c->d_alphabar, (size_t)c->E * sizeof(double)) != hipSuccess) {
+            c->d_alphabar = nullptr;
             return fail(TRIXI_HIP_ENOMEM, "hipMalloc failed for the shock-capturing adjoint buffers");
+        }
+        if (hipMalloc((void**)&c->d_winner, (size_t)c->E * sizeof(int32_t)) != hipSuccess) {
+            (void)hipFree(c->d_alphabar);
+            c->d_alphabar = nullptr;
+            c->d_winner = nullptr;
+            return fail(TRIXI_HIP_ENOMEM, "hipMalloc failed for the shock-capturing adjoint buffers");
+        }
     }
     c->vjp_fn = fn;
     return TRIXI_HIP_OK;
@@ -1810,9 +1820,20 @@ int trixi_hip_enable_adjoint(trixi_hip_ctx* ctx) {
     ctx->rk64_fn = find_kernel(make_key(c64, 1));
     if (!ctx->rk64_fn) return fail(TRIXI_HIP_EUNSUPPORTED, "no FP64 stage kernel for the adjoint");
     const size_t bytes = (size_t)ctx->nvalues * sizeof(double);
-    if (hipMalloc((void**)&ctx->d_stage, 5 * bytes) != hipSuccess ||
-        hipMalloc((void**)&ctx->d_k, bytes) != hipSuccess || hipMalloc((void**)&ctx->d_kbar, bytes) != hipSuccess)
-        return fail(TRIXI_HIP_ENOMEM, "hipMalloc failed for the adjoint buffers");
+    double** bufs[3] = {&ctx->d_stage, &ctx->d_k, &ctx->d_kbar};
+    const size_t sizes[3] = {5 * bytes, bytes, bytes};
+    for (int b = 0; b < 3; ++b) {
+        if (*bufs[b]) continue;
+        if (hipMalloc((void**)bufs[b], sizes[b]) != hipSuccess) {
+            // leave no partial set behind: a retry starts from scratch
+            *bufs[b] = nullptr;
+            for (int q = 0; q < b; ++q) {
+                (void)hipFree(*bufs[q]);
+                *bufs[q] = nullptr;
+            }
+            return fail(TRIXI_HIP_ENOMEM, "hipMalloc failed for the adjoint buffers");
+        }
+    }
     ctx->adjoint = true;
     return TRIXI_HIP_OK;
 }

@@ -147,8 +147,12 @@ struct DGSEMHIP{VI, SF}
     polydeg::Int
     surface_flux::SF
     volume_integral::VI
-    precision::Symbol      # :fp64 or :mixed
+    precision::Symbol      # :fp64, :mixed or :mixed_f32log
 end
+
+# enum trixi_hip_precision
+precision_id(p::Symbol) = p === :fp64 ? Int32(0) : p === :mixed ? Int32(1) : p === :mixed_f32log ? Int32(2) :
+    throw(ArgumentError("precision must be :fp64, :mixed or :mixed_f32log"))
 
 function DGSEMHIP(; RealT = Float64, polydeg::Integer, surface_flux = flux_central,
                   volume_integral = VolumeIntegralWeakForm(), precision = nothing)
@@ -235,7 +239,7 @@ function make_config(solver::DGSEMHIP, equations, mesh; boundary_conditions = bo
     gammas = hasproperty(equations, :gammas) ? Tuple(equations.gammas) : ()
     gas_r = hasproperty(equations, :gas_constants) ? Tuple(equations.gas_constants) : ()
     CConfig(ndims(mesh), equation_id(equations), solver.polydeg, volume_integral, vflux,
-            flux_id(solver.surface_flux), fvflux, solver.precision === :mixed ? 1 : 0,
+            flux_id(solver.surface_flux), fvflux, precision_id(solver.precision),
             smooth, var, gamma, c_h, vel, amax, amin, len, bc, bc_ic, bc_seed, source_id(source_terms), outflow,
             hasproperty(equations, :nu) ? Float64(equations.nu) : 0.0,
             hasproperty(equations, :Lr) ? Float64(equations.Lr) : 0.0,

@@ -68,14 +68,21 @@ class VolumeIntegralShockCapturingHG:
         self.volume_flux_fv = volume_flux_fv
 
 
+# precision name -> enum trixi_hip_precision
+PRECISIONS = {"fp64": L.PREC_FP64, "mixed": L.PREC_MIXED, "mixed_f32log": L.PREC_MIXED_F32LOG}
+
+
 class DGSEMHIP:
     """DGSEMHIP(; polydeg, surface_flux=flux_central, volume_integral=VolumeIntegralWeakForm(),
     RealT=np.float64, precision=None).
 
     precision "fp64" computes everything in FP64; "mixed" keeps the state,
-    the logarithmic means, the RK update and the shock indicator in FP64 and
-    does the volume/surface arithmetic in FP32.  RealT=np.float32 selects
-    "mixed" (the reference's Float32 mode)."""
+    the logarithms of the logarithmic means, the RK update and the shock
+    indicator in FP64 and does the volume/surface arithmetic in FP32;
+    "mixed_f32log" is "mixed" with the ln-mean logarithms in FP32 too (built
+    for the benchmark's periodic flux_ranocha / flux_hindenlang_gassner
+    configurations).  RealT=np.float32 selects "mixed" (the reference's
+    Float32 mode)."""
 
     def __init__(self, polydeg, surface_flux=flux_central, volume_integral=None, RealT=np.float64,
                  precision=None):
@@ -84,8 +91,8 @@ class DGSEMHIP:
         self.volume_integral = volume_integral if volume_integral is not None else VolumeIntegralWeakForm()
         if precision is None:
             precision = "mixed" if RealT == np.float32 else "fp64"
-        if precision not in ("fp64", "mixed"):
-            raise ValueError("precision must be 'fp64' or 'mixed'")
+        if precision not in PRECISIONS:
+            raise ValueError("precision must be one of " + ", ".join(repr(p) for p in PRECISIONS))
         self.precision = precision
 
     @property
@@ -144,7 +151,7 @@ class DGSEMHIP:
             c.indicator_variable = {density_pressure: 0, pressure: 1, density: 2, first: 2,
                                     E.waterheight_pressure: 0}[vi.indicator.variable]
             c.alpha_max, c.alpha_min = vi.indicator.alpha_max, vi.indicator.alpha_min
-        c.precision = L.PREC_MIXED if self.precision == "mixed" else L.PREC_FP64
+        c.precision = PRECISIONS[self.precision]
         c.gamma = getattr(equations, "gamma", 1.4)
         c.c_h = getattr(equations, "c_h", 0.0)
         vel = getattr(equations, "advection_velocity", ())

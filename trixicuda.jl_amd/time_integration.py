@@ -57,6 +57,20 @@ def synchronize(stream=None):
     L.check(L.lib().trixi_hip_sync(ctypes.c_void_p(s.cuda_stream)))
 
 
+def select_step(t, h, t1, save_times=()):
+    """Step from t with the proposed size h: shortened to end at t1, or at the
+    earliest save time strictly inside (t, t + h].  Returns (h, t_next), where
+    t_next is exactly that end time for a shortened step (no rounding of
+    t + h) and t + h otherwise."""
+    end = t1 if t + h >= t1 else None
+    for ts in save_times:
+        if ts is not None and t < ts < t + h and (end is None or ts < end):
+            end = ts
+    if end is None:
+        return h, t + h
+    return end - t, end
+
+
 def solve(ode, alg=None, dt=None, callback=(), use_graph=True, maxiters=10**7):
     """Integrate ode over its tspan; returns a Solution with the final state."""
     import torch
@@ -80,16 +94,9 @@ def solve(ode, alg=None, dt=None, callback=(), use_graph=True, maxiters=10**7):
             c.initialize(u, t, stepsize.dt(u, semi) if stepsize is not None else dt, semi)
         while t < t1 and nsteps < maxiters:
             h = stepsize.dt(u, semi) if stepsize is not None else dt
-            if t + h > t1:
-                h = t1 - t
-            for c in saves:  # land on the next save time instead of stepping past it
-                if c.next_time is not None and t + h > c.next_time > t:
-                    h = c.next_time - t
+            h, t_next = select_step(t, h, t1, [c.next_time for c in saves])
             rk_step(u, k, t, h, semi, use_graph=use_graph, stream=stream)
-            t = t + h
-            for c in saves:
-                if c.next_time is not None and abs(t - c.next_time) <= 1e-12 * max(1.0, abs(t)):
-                    t = c.next_time  # the shortened step hit the save time; remove the rounding of t + h
+            t = t_next
             nsteps += 1
             for a in analysis:
                 a(u, t, nsteps)
