@@ -284,3 +284,20 @@ def test_oracle_cfg_layout_matches_its_header(oracle, tmp_path):
     assert got["size"] == ctypes.sizeof(cls)
     for f, _ in cls._fields_:
         assert got[f] == getattr(cls, f).offset, f
+
+
+def test_reference_configurations_are_built(th):
+    """Every configuration of the reference's test files (and the benchmark
+    configurations) has a registered kernel in both precisions: creating the
+    semidiscretization gets past the kernel lookup, which comes before any
+    device allocation (without a GPU it then fails allocating, never with
+    'unsupported combination')."""
+    from cases import product_semi
+    from reference_configs import BENCHMARK_CASES, REFERENCE_CASES
+    L = th._lib
+    for c in REFERENCE_CASES + BENCHMARK_CASES:
+        for prec in ("fp64", "mixed"):
+            try:
+                product_semi(th, c, prec).close()
+            except th.TrixiHipError as e:
+                assert e.code != L.EUNSUPPORTED, (c.name, prec, str(e))

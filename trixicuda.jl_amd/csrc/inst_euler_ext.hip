@@ -19,6 +19,8 @@ void register_euler_ext() {
     // source_terms_eoc_test_euler
     RegistrarExt<2, 4, Euler<2>, VI_WEAK, C, FLUX_HLL, C>::add(EQ_EULER);
     RegistrarExt<3, 4, Euler<3>, VI_WEAK, C, FLUX_HLL, C>::add(EQ_EULER);
+    // test/tree_dgsem_1d/euler_source_terms.jl: polydeg 4
+    RegistrarExt<1, 5, Euler<1>, VI_WEAK, C, FLUX_LLF, C>::add(EQ_EULER);
 }
 
 }  // namespace trixi_hip
