@@ -302,6 +302,20 @@ int trixi_hip_sync(trixi_hip_stream stream);
 int trixi_hip_max_dt(trixi_hip_ctx* ctx, const double* u, double* max_dt,
                      trixi_hip_stream stream);
 
+/* trixi_hip_max_dt without any allocation: the block partials and the maximum
+ * go to a static device scratch of 2048 doubles (16 KiB, held by every GPU
+ * the library runs on until the process ends) instead of the context's
+ * buffers; the result is the same, and like trixi_hip_max_dt it blocks until
+ * the scalar is on the host.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (tight time-stepping loops,
+ * harnesses that forbid hipMalloc).
+ * Limit: at most 2047 block partials (the grid is at most 1024 blocks of 256
+ * elements, so every mesh fits); larger grids are rejected with
+ * TRIXI_HIP_EINVAL.  All calls on one GPU share the scratch: calls that
+ * overlap on one GPU (several host threads or streams) race. */
+int trixi_hip_max_dt_noalloc(trixi_hip_ctx* ctx, const double* u, double* max_dt,
+                             trixi_hip_stream stream);
+
 /* L2 and Linf errors against the initial condition at time t (2N-node
  * analysis basis); blocks.  Replaces calc_error_norms
  * (analysis_dg_{1,2,3}d.jl, e.g. analysis_dg_3d.jl:45-89). */

@@ -96,6 +96,7 @@ SIGNATURES = {
     "trixi_hip_rk_stage": (ctypes.c_int, [_P, _P, _P, _P, _I32, _D, _D, _P]),
     "trixi_hip_sync": (ctypes.c_int, [_P]),
     "trixi_hip_max_dt": (ctypes.c_int, [_P, _P, _P, _P]),
+    "trixi_hip_max_dt_noalloc": (ctypes.c_int, [_P, _P, _P, _P]),
     "trixi_hip_error_norms": (ctypes.c_int, [_P, _P, _I32, _D, _U64, _P, _P, _P]),
     "trixi_hip_integrate": (ctypes.c_int, [_P, _P, _P, _P]),
     "trixi_hip_indicator": (ctypes.c_int, [_P, _P, _P, _P]),

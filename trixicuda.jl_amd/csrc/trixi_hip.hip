@@ -122,6 +122,37 @@ static int fail(int code, const std::string& msg) {
     } while (0)
 
 // ---------------------------------------------------------------------------
+// Static device scratch of the *_noalloc entry points: a file-scope array that
+// every GPU the library runs on holds until the process ends.  In host code
+// `scratch` names a host-side stand-in, never the device memory: the device
+// address is looked up once per device (hipGetSymbolAddress, with that device
+// current) and only that address is handed to kernels.  All calls on one GPU
+// share the array; calls that overlap on one GPU race.
+// ---------------------------------------------------------------------------
+#define TRIXI_HIP_SCRATCH_DOUBLES 2048  // 16 KiB per device
+static __device__ __align__(256) double scratch[TRIXI_HIP_SCRATCH_DOUBLES];
+
+static constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mutex;
+static double* g_scratch_addr[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device.
+static int scratch_ptr(double** out) {
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return fail(TRIXI_HIP_EDEVICE, "device ordinal out of range for scratch");
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_addr[dev]) {
+        void* p = nullptr;
+        HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) return fail(TRIXI_HIP_EDEVICE, "hipGetSymbolAddress returned no address for scratch");
+        g_scratch_addr[dev] = (double*)p;
+    }
+    *out = g_scratch_addr[dev];
+    return TRIXI_HIP_OK;
+}
+
+// ---------------------------------------------------------------------------
 // Lobatto-Legendre basis on the host (the matrices of
 // LobattoLegendreBasisGPU, reference basis_lobatto_legendre.jl:60-100).
 // Interior nodes: roots of P_{n-1} - P_{n+1} (proportional to (1-x^2) P_n'),
@@ -1385,6 +1416,16 @@ int trixi_hip_create(const trixi_hip_config* config, const trixi_hip_mesh* mesh,
         const char* env = getenv("TRIXI_HIP_NXCD");
         if (env) a.nxcd = atoi(env) > 0 ? atoi(env) : 1;
     }
+    {
+        // look up this device's static scratch once, at set-up, with the
+        // device current (used by the *_noalloc entry points)
+        double* sp = nullptr;
+        const int rc = scratch_ptr(&sp);
+        if (rc) {
+            free_ctx(c);
+            return rc;
+        }
+    }
     *out = c;
     return TRIXI_HIP_OK;
 }
@@ -1579,15 +1620,15 @@ int trixi_hip_rk_step(trixi_hip_ctx* ctx, double* u, double* k, double t, double
     return TRIXI_HIP_OK;
 }
 
-int trixi_hip_max_dt(trixi_hip_ctx* ctx, const double* u, double* max_dt, trixi_hip_stream stream) {
-    if (!ctx || !u || !max_dt) return fail(TRIXI_HIP_EINVAL, "null argument");
-    hipStream_t s = (hipStream_t)stream;
+// max_dt with its block partials in `partial` (at least `blocks` doubles) and
+// the reduced maximum in `result`.
+static int max_dt_impl(trixi_hip_ctx* ctx, const double* u, double* max_dt, hipStream_t s, int blocks,
+                       double* partial, double* result) {
     const int threads = 256;
-    int blocks = (int)std::min<int64_t>((ctx->E + threads - 1) / threads, ctx->nblocks_red);
     const PhysParams& P = ctx->args.phys;
     const int eq = ctx->cfg.equation, dim = ctx->dim;
 #define MS(EQT, D) \
-    hipLaunchKernelGGL((max_speed_kernel<EQT, D>), dim3(blocks), dim3(threads), 0, s, u, ctx->d_invJ, ctx->d_partial, ctx->nn, ctx->E, P)
+    hipLaunchKernelGGL((max_speed_kernel<EQT, D>), dim3(blocks), dim3(threads), 0, s, u, ctx->d_invJ, partial, ctx->nn, ctx->E, P)
     if (eq == TRIXI_HIP_LINEAR_SCALAR_ADVECTION) {
         if (dim == 1) MS(LinAdv<1>, 1); else if (dim == 2) MS(LinAdv<2>, 2); else MS(LinAdv<3>, 3);
     } else if (eq == TRIXI_HIP_COMPRESSIBLE_EULER) {
@@ -1616,15 +1657,37 @@ int trixi_hip_max_dt(trixi_hip_ctx* ctx, const double* u, double* max_dt, trixi_
     }
 #undef MS
     HIP_TRY(hipGetLastError());
-    hipLaunchKernelGGL(reduce_partials_kernel, dim3(1), dim3(32), 0, s, ctx->d_partial, ctx->d_result, blocks, 1, 0);
+    hipLaunchKernelGGL(reduce_partials_kernel, dim3(1), dim3(32), 0, s, partial, result, blocks, 1, 0);
     HIP_TRY(hipGetLastError());
     double m = 0.0;
-    HIP_TRY(hipMemcpyAsync(&m, ctx->d_result, sizeof(double), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(&m, result, sizeof(double), hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
     const double tiny = nextafter(0.0, 1.0);
     if (!(m > tiny)) m = tiny;
     *max_dt = 2.0 / (ctx->N * m);
     return TRIXI_HIP_OK;
+}
+
+int trixi_hip_max_dt(trixi_hip_ctx* ctx, const double* u, double* max_dt, trixi_hip_stream stream) {
+    if (!ctx || !u || !max_dt) return fail(TRIXI_HIP_EINVAL, "null argument");
+    const int blocks = (int)std::min<int64_t>((ctx->E + 255) / 256, ctx->nblocks_red);
+    return max_dt_impl(ctx, u, max_dt, (hipStream_t)stream, blocks, ctx->d_partial, ctx->d_result);
+}
+
+int trixi_hip_max_dt_noalloc(trixi_hip_ctx* ctx, const double* u, double* max_dt, trixi_hip_stream stream) {
+    if (!ctx || !u || !max_dt) return fail(TRIXI_HIP_EINVAL, "null argument");
+    // the same grid as trixi_hip_max_dt: the same partials and maximum
+    const int blocks = (int)std::min<int64_t>((ctx->E + 255) / 256, ctx->nblocks_red);
+    if (blocks + 1 > TRIXI_HIP_SCRATCH_DOUBLES)
+        return fail(TRIXI_HIP_EINVAL, "max_dt_noalloc: " + std::to_string(blocks) +
+                                          " block partials exceed the static scratch of " +
+                                          std::to_string(TRIXI_HIP_SCRATCH_DOUBLES - 1));
+    double* sp = nullptr;
+    const int rc = scratch_ptr(&sp);
+    if (rc) return rc;
+    // partials in [0, blocks), the maximum in the last slot; every partial is
+    // written before it is read, so nothing needs clearing between calls
+    return max_dt_impl(ctx, u, max_dt, (hipStream_t)stream, blocks, sp, sp + TRIXI_HIP_SCRATCH_DOUBLES - 1);
 }
 
 int trixi_hip_error_norms(trixi_hip_ctx* ctx, const double* u, int32_t ic, double t, uint64_t seed, double* l2,
