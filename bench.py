@@ -90,6 +90,37 @@ def _events_ms(torch, stream, fn, reps, warmup=3):
     return a.elapsed_time(b) / reps
 
 
+def _graph_ms(torch, fn, reps, check=None):
+    """Average ms per call of fn() replayed from a HIP graph of `reps` calls
+    (no host launch overhead: the device time of back-to-back rhs! calls).
+    `check()` returns the output tensor; the graph's result must equal the
+    eager one bit for bit."""
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        for _ in range(3):
+            fn()
+    torch.cuda.current_stream().wait_stream(side)
+    torch.cuda.synchronize()
+    eager = check().clone() if check is not None else None
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        for _ in range(reps):
+            fn()
+    for _ in range(2):
+        g.replay()
+    torch.cuda.synchronize()
+    if check is not None and not torch.equal(check(), eager):
+        raise RuntimeError("graph replay of rhs! differs from the eager result")
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record()
+    for _ in range(3):
+        g.replay()
+    b.record()
+    torch.cuda.synchronize()
+    return a.elapsed_time(b) / (3 * reps)
+
+
 def roofline_of(fm, key, dofs, ms, precision):
     """HBM roof (algorithmic bytes: read u, write du) with the flop fraction
     beside it, for one rhs! of `dofs` DOF taking `ms` milliseconds."""
@@ -104,7 +135,10 @@ def roofline_of(fm, key, dofs, ms, precision):
 
 
 def measure_configs(torch, th, fm, stream, reps):
-    """The other workloads of the benchmark set, each on one GPU:
+    """The other workloads of the benchmark set, each on one GPU (rhs_ms:
+    device time per rhs! from a HIP graph of back-to-back calls; rhs_ms_eager:
+    the same calls launched one by one from Python, which bounds the small
+    meshes by the host's launch rate):
     C2 2D Euler FD (flux_ranocha + LLF), [-2,2]^2 level 5, seeded random state;
     C4 3D GLM-MHD Alfven wave (flux_hindenlang_gassner + Powell, LLF), [-1,1]^3 level 4;
     C5 2D Euler HG shock capturing (alpha_max 0.5), [-2,2]^2 level 6, seeded weak blast:
@@ -141,10 +175,11 @@ def measure_configs(torch, th, fm, stream, reps):
             semis[prec] = semi
             u = th.compute_coefficients(0.0, semi)
             d = torch.empty_like(u)
-            ms = _events_ms(torch, stream, lambda: th.rhs_hip_(d, u, semi, 0.0), reps)
+            eager_ms = _events_ms(torch, stream, lambda: th.rhs_hip_(d, u, semi, 0.0), reps)
+            ms = _graph_ms(torch, lambda: th.rhs_hip_(d, u, semi, 0.0), reps, check=lambda: d)
             torch.cuda.synchronize()
             du[prec] = d
-            r[prec] = {"rhs_ms": ms, "dof_per_s": semi.ndofs / (ms * 1e-3),
+            r[prec] = {"rhs_ms": ms, "rhs_ms_eager": eager_ms, "dof_per_s": semi.ndofs / (ms * 1e-3),
                        "roofline": roofline_of(fm, c["key"], semi.ndofs, ms, prec)}
         r["ndofs"] = semis["fp64"].ndofs
         a, b = du["mixed"], du["fp64"]
@@ -156,7 +191,7 @@ def measure_configs(torch, th, fm, stream, reps):
             u = th.compute_coefficients(0.0, s64)
             vb = torch.randn_like(u)
             ub = torch.empty_like(u)
-            vms = _events_ms(torch, stream, lambda: th.rhs_vjp(ub, u, vb, s64), max(reps // 4, 5))
+            vms = _graph_ms(torch, lambda: th.rhs_vjp(ub, u, vb, s64), max(reps // 4, 5), check=lambda: ub)
             r["vjp_ms"] = vms
             r["adjoint_over_forward"] = vms / r["fp64"]["rhs_ms"]
         if name == "C5":

@@ -25,6 +25,8 @@
 
 namespace trixi_hip {
 
+__host__ __device__ constexpr int cpow(int b, int e) { return e == 0 ? 1 : b * cpow(b, e - 1); }
+
 enum Equation : int {
     EQ_LINADV = 0,
     EQ_EULER = 1,

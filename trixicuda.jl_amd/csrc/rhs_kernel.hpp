@@ -60,8 +60,6 @@ constexpr int MAXN = 8;
 #define TRIXI_HIP_PAR_DIRS_F32 0
 #endif
 
-__host__ __device__ constexpr int cpow(int b, int e) { return e == 0 ? 1 : b * cpow(b, e - 1); }
-
 struct RhsArgs {
     const double* u;          // SoA state  u[(v*NN + node)*E + e]
     double* du;               // SoA output (rhs mode)
@@ -95,6 +93,8 @@ struct RhsArgs {
     const double* mflux;
     const int32_t* mnb;
     int mK;
+    // node-parallel form (small periodic conforming meshes, chosen on the host)
+    int np;
 };
 
 // Dirichlet state boundary_ic(x, t) at node `node` of element e (its face
@@ -664,16 +664,311 @@ rhs_kernel(const RhsArgs a) {
     }
 }
 
-// Launch helper used by the instantiation units.
+// ---------------------------------------------------------------------------
+// Node-parallel form for small meshes.  The line form gives each lane a whole
+// line (N(N-1)/2 two-point fluxes and two interface fluxes per direction), so
+// a mesh of a few thousand elements runs one or two waves per SIMD, each a
+// long dependent chain.  Here lane (el, node, d) evaluates only its node's
+// row of the line in direction d: the N-1 two-point fluxes with the other
+// nodes of the line (each pair is evaluated by both of its nodes, twice the
+// volume-flux work), the subcell fluxes on either side of the node and, at
+// the line's ends, the interface flux; N^d * DIM lanes per element.
+//
+// Every node receives the same terms in the same order as in the line form
+// (partners in increasing order, then the nonconservative sum, the subcell
+// fluxes, the interface flux; directions added x, y, z); the two forms
+// differ only where the compiler fuses a multiply-add in one code shape and
+// not in the other.
+//
+// The last direction's lanes also do the node pass (node records to LDS,
+// and for a fused RK stage u and k kept in registers) and the epilogue.
+// ---------------------------------------------------------------------------
+// Either of two nodes, chosen field by field through their node records
+// (register selects; a conditional copy of the whole struct would go through
+// scratch).  The record holds every field the fluxes read.
+template <class EQ, typename R, typename L>
+__device__ __forceinline__ typename EQ::template Node<R, L> select_node(
+    bool c, const typename EQ::template Node<R, L>& a, const typename EQ::template Node<R, L>& b,
+    const PhysR<R>& P) {
+    R fa[EQ::NRF], fb[EQ::NRF];
+    EQ::to_rec(a, fa);
+    EQ::to_rec(b, fb);
+#pragma unroll
+    for (int k = 0; k < EQ::NRF; ++k) fa[k] = c ? fa[k] : fb[k];
+    return EQ::from_rec(fa, c ? a.lrho : b.lrho, c ? a.lp : b.lp, P);
+}
+
+template <int DIM, int N>
+__host__ __device__ constexpr int np_elems() {
+    return (256 / (cpow(N, DIM) * DIM)) > 0 ? 256 / (cpow(N, DIM) * DIM) : 1;
+}
+
+template <int DIM, int N, class EQ, int VOL, int VF, int SF, int FVF, typename R, bool RK, typename L>
+__global__ void __launch_bounds__((np_elems<DIM, N>() * cpow(N, DIM) * DIM))
+rhs_np_kernel(const RhsArgs a) {
+    constexpr int NN = cpow(N, DIM);
+    constexpr int M = np_elems<DIM, N>();
+    constexpr int NV = EQ::NV;
+    constexpr int NRF = EQ::NRF;
+    constexpr bool SF_LOGS = flux_needs_logs(SF);
+    constexpr bool VF_LOGS = flux_needs_logs(VF);
+    constexpr bool FV_LOGS = (VOL == VI_HG) && flux_needs_logs(FVF);
+    constexpr bool NODE_LOGS = SF_LOGS || VF_LOGS || FV_LOGS;
+    constexpr int NDF = NODE_LOGS ? 2 : 0;
+    using Node = typename EQ::template Node<R, L>;
+    static_assert(VOL != VI_WEAK, "the node-parallel form is built for flux differencing and HG");
+
+    __shared__ R rec[NRF * NN * M];
+    __shared__ L recd[NDF > 0 ? NDF * NN * M : 1];
+    __shared__ R acc[DIM > 1 ? (DIM - 1) * NV * NN * M : 1];
+
+    const int tid = threadIdx.x;
+    const int el = tid % M;
+    const int node = (tid / M) % NN;
+    const int d = tid / (M * NN);
+    const int64_t E = a.E;
+    const int64_t e0 = xcd_tile(a.nxcd) * M;
+    if (e0 >= E) return;  // whole workgroup: padding tile of the XCD mapping
+    const int64_t e = e0 + el;
+    const bool valid = e < E;
+    const int64_t ec = valid ? e : E - 1;
+    const PhysR<R> P = PhysR<R>::from(a.phys);
+    const bool last = d == DIM - 1;  // node pass and epilogue lanes
+
+    // this node's position along direction d, and the interface state at the
+    // line's ends (requested before the node pass to overlap its latency)
+    const int s = d == 0 ? 1 : (d == 1 ? N : N * N);
+    const int p = (node / s) % N;
+    const int base = node - p * s;
+    const bool face = p == 0 || p == N - 1;
+    R ub[NV];
+    if (face) {
+        const int32_t nbe = a.nbr[(int64_t)(2 * d + (p == 0 ? 0 : 1)) * E + ec];
+        const int nn_face = p == 0 ? node + (N - 1) * s : node - (N - 1) * s;
+#pragma unroll
+        for (int v = 0; v < NV; ++v) ub[v] = (R)a.u[(int64_t)(v * NN + nn_face) * E + nbe];
+    }
+
+    if (last) {
+        R uc[NV];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) uc[v] = (R)a.u[(int64_t)(v * NN + node) * E + ec];
+        const Node nd = EQ::template node<R, NODE_LOGS, L>(uc, P);
+        R f[NRF];
+        EQ::to_rec(nd, f);
+#pragma unroll
+        for (int k = 0; k < NRF; ++k) rec[(k * NN + node) * M + el] = f[k];
+        if (NODE_LOGS) {
+            recd[node * M + el] = nd.lrho;
+            recd[(NN + node) * M + el] = nd.lp;
+        }
+    }
+    __syncthreads();
+
+    R alpha = R(0);
+    bool dg_only = true;
+    if constexpr (VOL == VI_HG) {
+        double al = a.alpha_raw[ec];
+        if (a.smooth) {
+#pragma unroll
+            for (int f = 0; f < 2 * DIM; ++f) {
+                const double an = 0.5 * a.alpha_raw[a.nbr[(int64_t)f * E + ec]];
+                al = an > al ? an : al;
+            }
+        }
+        dg_only = fabs(al) <= a.atol;
+        alpha = (R)al;
+    }
+
+    auto load_node = [&](int n) {
+        R f[NRF];
+#pragma unroll
+        for (int k = 0; k < NRF; ++k) f[k] = rec[(k * NN + n) * M + el];
+        const L lr = NODE_LOGS ? recd[n * M + el] : L(0);
+        const L lq = NODE_LOGS ? recd[(NN + n) * M + el] : L(0);
+        return EQ::from_rec(f, lr, lq, P);
+    };
+    const Node me = load_node(node);
+
+    R acl[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) acl[v] = R(0);
+
+    // this node's row of D_split and its inverse weight, read with constant
+    // indices (a dynamic index into the kernel arguments goes through scratch)
+    R dsr[N];
+    R invw_p = R(0);
+#pragma unroll
+    for (int j = 0; j < N; ++j) dsr[j] = R(0);
+#pragma unroll
+    for (int pp = 0; pp < N; ++pp) {
+        if (pp != p) continue;
+#pragma unroll
+        for (int j = 0; j < N; ++j) dsr[j] = (R)a.Ds[pp * N + j];
+        invw_p = (R)a.invw[pp];
+    }
+
+    // the direction's terms with the direction a compile-time constant (a
+    // runtime direction would index the node records' vector fields
+    // dynamically, which goes through scratch); d is uniform over each wave
+    // of the usual shapes, so the dispatch below does not diverge
+    auto dir_terms = [&](auto dc) {
+        constexpr int D = decltype(dc)::value;
+        constexpr int SD = cpow(N, D);
+        // volume: this node's row of the split form, partners in increasing
+        // order; every lane runs N-1 pairs with the lower node first (operands
+        // selected, not branched, so that a wave evaluates one flux per pair)
+        const R sc = (VOL == VI_HG && !dg_only) ? R(1) - alpha : R(1);
+    #pragma unroll
+        for (int q = 0; q < N - 1; ++q) {
+            const bool below = q < p;  // partner q below this node, else q + 1 above
+            const Node other = load_node(base + (below ? q : q + 1) * SD);
+            const Node lo = select_node<EQ, R, L>(below, other, me, P);
+            const Node hi = select_node<EQ, R, L>(below, me, other, P);
+            R f[NV];
+            EQ::template two_point<VF>(lo, hi, D, P, f);
+            const R w = sc * (below ? dsr[q] : dsr[q + 1]);
+    #pragma unroll
+            for (int v = 0; v < NV; ++v) acl[v] += w * f[v];
+        }
+        if constexpr (EQ::NONCONS) {
+            R integral[NV];
+    #pragma unroll
+            for (int v = 0; v < NV; ++v) integral[v] = R(0);
+    #pragma unroll
+            for (int j = 0; j < N; ++j) {
+                const Node other = load_node(base + j * SD);
+                R f[NV];
+                EQ::template noncons<-1>(me, other, D, P, f);
+                const R w = dsr[j];
+    #pragma unroll
+                for (int v = 0; v < NV; ++v) integral[v] += w * f[v];
+            }
+            const R h = sc * R(0.5);
+    #pragma unroll
+            for (int v = 0; v < NV; ++v) acl[v] += h * integral[v];
+        }
+        if (VOL == VI_HG && !dg_only) {
+            const R w = alpha * invw_p;
+            if (p > 0) {  // subcell interface p - 1/2: this node is its right node
+                const Node left = load_node(node - SD);
+                R F[NV];
+                EQ::template two_point<FVF>(left, me, D, P, F);
+                if constexpr (EQ::NONCONS) {
+                    R nc[NV];
+                    EQ::template noncons<FVF>(me, left, D, P, nc);
+    #pragma unroll
+                    for (int v = 0; v < NV; ++v) acl[v] -= w * (F[v] + R(0.5) * nc[v]);
+                } else {
+    #pragma unroll
+                    for (int v = 0; v < NV; ++v) acl[v] -= w * F[v];
+                }
+            }
+            if (p < N - 1) {  // subcell interface p + 1/2: this node is its left node
+                const Node right = load_node(node + SD);
+                R F[NV];
+                EQ::template two_point<FVF>(me, right, D, P, F);
+                if constexpr (EQ::NONCONS) {
+                    R nc[NV];
+                    EQ::template noncons<FVF>(me, right, D, P, nc);
+    #pragma unroll
+                    for (int v = 0; v < NV; ++v) acl[v] += w * (F[v] + R(0.5) * nc[v]);
+                } else {
+    #pragma unroll
+                    for (int v = 0; v < NV; ++v) acl[v] += w * F[v];
+                }
+            }
+        }
+
+        // interface flux at the line's ends and its surface-integral weight
+        // (this element is "right" at its -d face, "left" at its +d face)
+        if (face) {
+            const Node nb = EQ::template node<R, SF_LOGS, L>(ub, P);
+            const bool minus = p == 0;
+            const Node lo = select_node<EQ, R, L>(minus, nb, me, P);
+            const Node hi = select_node<EQ, R, L>(minus, me, nb, P);
+            R fs[NV];
+            EQ::template two_point<SF>(lo, hi, D, P, fs);
+            if constexpr (EQ::NONCONS) {
+                R nc[NV];
+                EQ::template noncons<SF>(me, nb, D, P, nc);
+    #pragma unroll
+                for (int v = 0; v < NV; ++v) fs[v] = fs[v] + R(0.5) * nc[v];
+            }
+            const R lw = minus ? -(R)a.lhat_m : (R)a.lhat_p;
+    #pragma unroll
+            for (int v = 0; v < NV; ++v) acl[v] = acl[v] + fs[v] * lw;
+        }
+
+    };
+    if (d == 0) dir_terms(std::integral_constant<int, 0>{});
+    if constexpr (DIM > 1) {
+        if (d == 1) dir_terms(std::integral_constant<int, 1>{});
+    }
+    if constexpr (DIM > 2) {
+        if (d == 2) dir_terms(std::integral_constant<int, 2>{});
+    }
+
+    // directions meet in LDS; the last direction's lane adds them in order
+    if (!last) {
+#pragma unroll
+        for (int v = 0; v < NV; ++v) acc[((d * NV + v) * NN + node) * M + el] = acl[v];
+    }
+    if (DIM > 1) __syncthreads();
+    if (!last || !valid) return;
+    const double fac = -a.invJ[e];
+    double t_dt = 0.0;
+    if (RK) t_dt = a.time[1];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+        R total = acl[v];
+        if constexpr (DIM > 1) {
+            R sum = acc[((0 * NV + v) * NN + node) * M + el];
+#pragma unroll
+            for (int dd = 1; dd < DIM - 1; ++dd) sum = sum + acc[((dd * NV + v) * NN + node) * M + el];
+            total = sum + acl[v];
+        }
+        const int64_t g = (int64_t)(v * NN + node) * E + e;
+        const double val = (double)total * fac;
+        if constexpr (!RK) {
+            a.du[g] = val;
+        } else {
+            // u and k again (cache resident on the small meshes this form runs)
+            const double kk = a.rk_first ? t_dt * val : a.rk_a * a.k[g] + t_dt * val;
+            a.k[g] = kk;
+            a.u_out[g] = a.u[g] + a.rk_b * kk;
+        }
+    }
+}
+
+// Shapes the node-parallel form is built for: periodic conforming meshes in
+// 2D and 3D, flux differencing or shock capturing, at most five variables
+// (the nine-variable MHD system is faster in the line form), whole
+// workgroups within 1024 lanes.  The host sets RhsArgs::np only for these.
+template <int DIM, int N, int NV, int VOL, bool EXT>
+__host__ __device__ constexpr bool np_eligible() {
+    return !EXT && VOL != VI_WEAK && DIM > 1 && NV <= 5 && cpow(N, DIM) * DIM <= 1024;
+}
+
+// Launch helper used by the instantiation units: the node-parallel form for
+// small periodic conforming meshes (RhsArgs::np), the line form otherwise.
 template <int DIM, int N, class EQ, int VOL, int VF, int SF, int FVF, typename R, int M, bool RK, bool EXT = false,
           typename L = double>
 hipError_t launch_rhs(const RhsArgs& a, hipStream_t stream) {
-    constexpr int NL = cpow(N, DIM - 1);
+    if constexpr (np_eligible<DIM, N, EQ::NV, VOL, EXT>()) {
+        if (a.np) {
+            constexpr int MN = np_elems<DIM, N>();
+            int64_t blocks = (a.E + MN - 1) / MN;
+            if (a.nxcd > 1) blocks = (blocks + a.nxcd - 1) / a.nxcd * a.nxcd;
+            hipLaunchKernelGGL((rhs_np_kernel<DIM, N, EQ, VOL, VF, SF, FVF, R, RK, L>), dim3((unsigned)blocks),
+                               dim3(MN * cpow(N, DIM) * DIM), 0, stream, a);
+            return hipGetLastError();
+        }
+    }
     int64_t blocks = (a.E + M - 1) / M;
     if (a.nxcd > 1) blocks = (blocks + a.nxcd - 1) / a.nxcd * a.nxcd;
     hipLaunchKernelGGL((rhs_kernel<DIM, N, EQ, VOL, VF, SF, FVF, R, M, RK, EXT, L>), dim3((unsigned)blocks),
                        dim3(rhs_threads<DIM, N, M, R, RK, EQ::NV>()), 0, stream, a);
-    (void)NL;
     return hipGetLastError();
 }
 

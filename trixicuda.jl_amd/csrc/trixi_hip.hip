@@ -15,6 +15,7 @@
 
 #include "registry.hpp"
 #include "initial_conditions.hpp"
+#include "indicator.hpp"
 
 namespace trixi_hip {
 
@@ -365,83 +366,6 @@ constexpr int ind_epb() {
     return ind_block<DIM, N>() / cpow(N, DIM);
 }
 
-// internal indicator-variable code of shallow water's waterheight_pressure
-// (its configuration value is density_pressure, the same position in Trixi's
-// API)
-constexpr int IND_WATERHEIGHT_PRESSURE = 3;
-
-template <int DIM, int N>
-__device__ __forceinline__ int max_mode(int n) {
-    int maxc = 0;
-#pragma unroll
-    for (int d = 0; d < DIM; ++d) {
-        const int c = (n / cpow(N, d)) % N;
-        maxc = c > maxc ? c : maxc;
-    }
-    return maxc;
-}
-
-// modal coefficients of the indicator variable (density * pressure,
-// pressure or density) at this lane's node (all lanes of the block take part:
-// it synchronises)
-template <int DIM, int N>
-__device__ __forceinline__ double modal_indicator(const double* __restrict__ u, int64_t ec, int64_t E,
-                                                  const PhysR<double>& P, const double* __restrict__ Vinv,
-                                                  double* buf, int el, int n, int var, int nv) {
-    constexpr int NN = cpow(N, DIM);
-    // compressible Euler (DIM + 2 variables) or a scalar equation (nv = 1:
-    // the variable is the solution itself)
-    double m;
-    if (var == IND_WATERHEIGHT_PRESSURE) {
-        // shallow water: waterheight_pressure = h g h^2 / 2
-        const double h = u[(int64_t)n * E + ec];
-        m = h * (0.5 * P.grav * h * h);
-    } else if (nv == 1) {
-        m = u[(int64_t)n * E + ec];
-    } else if (nv == DIM + 2) {
-        double uc[DIM + 2];
-#pragma unroll
-        for (int v = 0; v < DIM + 2; ++v) uc[v] = u[((int64_t)v * NN + n) * E + ec];
-        m = Euler<DIM>::indicator_variable(uc, P, var);
-    } else {
-        // ideal GLM-MHD (nine variables): density_pressure with the magnetic
-        // and cleaning energies taken out of the pressure
-        double uc[9];
-#pragma unroll
-        for (int v = 0; v < 9; ++v) uc[v] = u[((int64_t)v * NN + n) * E + ec];
-        const double ke = (uc[1] * uc[1] + uc[2] * uc[2] + uc[3] * uc[3]) / uc[0];
-        const double me = uc[5] * uc[5] + uc[6] * uc[6] + uc[7] * uc[7] + uc[8] * uc[8];
-        m = uc[0] * P.gm1 * (uc[4] - 0.5 * ke - 0.5 * me);
-    }
-#pragma unroll
-    for (int d = 0; d < DIM; ++d) {
-        const int s = cpow(N, d);
-        const int idx = (n / s) % N, base = n - idx * s;
-        buf[threadIdx.x] = m;
-        __syncthreads();
-        double r = 0.0;
-#pragma unroll
-        for (int ii = 0; ii < N; ++ii) r += Vinv[idx * N + ii] * buf[el * NN + base + ii * s];
-        __syncthreads();
-        m = r;
-    }
-    return m;
-}
-
-template <int DIM, int N>
-__device__ __forceinline__ void modal_energies(const double* e2, int el, double& total, double& clip1,
-                                               double& clip2) {
-    constexpr int NN = cpow(N, DIM);
-    total = clip1 = clip2 = 0.0;
-    for (int q = 0; q < NN; ++q) {
-        const int maxc = max_mode<DIM, N>(q);
-        const double v = e2[el * NN + q];
-        total += v;
-        if (maxc < N - 1) clip1 += v;
-        if (maxc < N - 2) clip2 += v;
-    }
-}
-
 template <int DIM, int N>
 __global__ void __launch_bounds__(256) indicator_kernel(const double* __restrict__ u, double* __restrict__ alpha_raw,
                                                         int64_t E, PhysParams phys, const double* __restrict__ Vinv,
@@ -466,13 +390,7 @@ __global__ void __launch_bounds__(256) indicator_kernel(const double* __restrict
     if (n != 0 || el >= EPB || e >= E) return;
     double total, clip1, clip2;
     modal_energies<DIM, N>(e2, el, total, clip1, clip2);
-    const double frac1 = (total != 0.0) ? (total - clip1) / total : 0.0;
-    const double frac2 = (clip1 != 0.0) ? (clip1 - clip2) / clip1 : 0.0;
-    const double energy = frac2 > frac1 ? frac2 : frac1;
-    double a = 1.0 / (1.0 + exp(-parameter_s / threshold * (energy - threshold)));
-    if (a < alpha_min) a = 0.0;
-    if (a > 1.0 - alpha_min) a = 1.0;
-    alpha_raw[e] = a < alpha_max ? a : alpha_max;
+    alpha_raw[e] = hg_alpha(total, clip1, clip2, threshold, parameter_s, alpha_min, alpha_max);
 }
 
 // alpha_e = max(raw_e, max over the face neighbours n of raw_n / 2), the
@@ -1415,6 +1333,22 @@ int trixi_hip_create(const trixi_hip_config* config, const trixi_hip_mesh* mesh,
         }
         const char* env = getenv("TRIXI_HIP_NXCD");
         if (env) a.nxcd = atoi(env) > 0 ? atoi(env) : 1;
+    }
+    {
+        // The node-parallel form for meshes whose line form (one lane per
+        // element line and direction) has fewer lanes than two waves per SIMD
+        // (256 CUs x 4 SIMDs x 2 waves x 64 lanes); TRIXI_HIP_NP_LANES
+        // overrides the threshold, 0 turns the form off.  Eligible shapes as
+        // np_eligible() in rhs_kernel.hpp: periodic, conforming, no sources,
+        // 2D/3D, flux differencing or HG, at most five variables.
+        int64_t np_lanes = 131072;
+        const char* env = getenv("TRIXI_HIP_NP_LANES");
+        if (env) np_lanes = atoll(env) > 0 ? atoll(env) : 0;
+        const KernelKey key = make_key(cfg, 0, NM > 0);
+        const bool eligible = key.ext == 0 && cfg.volume_integral != TRIXI_HIP_VOLUME_WEAK_FORM && dim > 1 &&
+                              nv <= 5 && nn * dim <= 1024;
+        const int64_t line_lanes = E * (nn / N) * dim;
+        a.np = eligible && line_lanes < np_lanes ? 1 : 0;
     }
     {
         // look up this device's static scratch once, at set-up, with the
