@@ -8,8 +8,9 @@ state.  A step is one rhs! evaluation (u -> du) with u resident in HBM.
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision mixed|fp64]
 
 `value` is DOF-updates/s of the mixed-precision rhs! (FP32 arithmetic, FP64
-state / log-means); the FP64 rhs! is reported beside it with its own
-roofline.  The CPU baseline is the FP64 oracle restatement (OpenMP) on a
+state / log-means); the FP64 rhs! and the FP32-state rhs! (the reference's
+RealT = Float32 mode: FP32 state, du and logarithms) are reported beside it,
+each with its own roofline.  The CPU baseline is the FP64 oracle restatement (OpenMP) on a
 bounded sample of the same workload and state.
 """
 import argparse
@@ -249,7 +250,8 @@ def main():
     ap.add_argument("--level", type=int, default=5)
     ap.add_argument("--precision", choices=("mixed", "fp64"), default="mixed",
                     help="mode reported as `value` (both are measured)")
-    ap.add_argument("--only", choices=("mixed", "fp64"), default=None, help="measure one mode only (profiling)")
+    ap.add_argument("--only", choices=("mixed", "fp64", "fp32"), default=None,
+                    help="measure one mode only (profiling)")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--rk-steps", type=int, default=20)
@@ -289,16 +291,23 @@ def main():
     flops_dof, bytes_dof = fm.FLOPS_PER_DOF[key], fm.BYTES_PER_DOF[key]
 
     stream = torch.cuda.Stream()
-    modes = [args.only] if args.only else ["fp64", "mixed"]
+    modes = [args.only] if args.only else ["fp64", "mixed", "fp32"]
     res, du_out = {}, {}
     u = None
+    semi_keep = None
     for mode in modes:
         semi = make_semi(th, args.level, mode)
         if u is None:
             u = th.compute_coefficients(0.0, semi)
+            if semi.fp32_state:
+                u = u.double()
             torch.cuda.synchronize()
-        du = torch.empty_like(u)
-        ms, wall_ms = time_rhs(torch, th, semi, u, du, args.steps, args.warmup, stream)
+        # the FP32-state mode reads the same state rounded to FP32 and writes
+        # FP32 du: half the bytes per DOF
+        um = u.float() if semi.fp32_state else u
+        bytes_dof = fm.BYTES_PER_DOF[key] // (2 if semi.fp32_state else 1)
+        du = torch.empty_like(um)
+        ms, wall_ms = time_rhs(torch, th, semi, um, du, args.steps, args.warmup, stream)
         dofs = semi.ndofs
         secs = ms * 1e-3
         fpeak = fm.PEAK["fp64_TFLOPs"] if mode == "fp64" else fm.PEAK["fp32_TFLOPs"]
@@ -311,20 +320,22 @@ def main():
             "roofline": {"bound": "hbm", "achieved": gbps, "peak": fm.PEAK["hbm_GBps"], "unit": "GB/s",
                          "frac": gbps / fm.PEAK["hbm_GBps"], "traffic": traffic.get(mode),
                          "kernel": "rhs_kernel<3, 4, Euler<3>, FD, flux_ranocha, LLF, "
-                                   + ("double, M=4>" if mode == "fp64" else "float, M=8>"),
+                                   + {"fp64": "double, M=4>", "mixed": "float, M=8, logs double>",
+                                      "fp32": "float, M=8, logs float, state float>"}[mode],
                          "algorithmic_bytes": bytes_dof * dofs, "bytes_per_dof": bytes_dof,
                          "flop_per_dof": flops_dof, "flop_TFLOPs": achieved_tf, "flop_peak_TFLOPs": fpeak,
                          "flop_frac": achieved_tf / fpeak},
         }
         if args.rk_steps > 0:
-            res[mode]["rk_step_ms"] = time_rk(torch, th, semi, u, args.rk_steps, stream)
+            res[mode]["rk_step_ms"] = time_rk(torch, th, semi, um, args.rk_steps, stream)
         if mode == "fp64" and args.vjp_steps > 0:
             vms = time_vjp(torch, th, semi, u, args.vjp_steps, 3, stream)
             res[mode]["vjp_ms"] = vms
             res[mode]["adjoint_over_forward"] = vms / ms
-        du_out[mode] = du.clone()
-        semi_keep = semi
-        if mode != modes[-1]:
+        du_out[mode] = du.double() if semi.fp32_state else du.clone()
+        if semi_keep is None and not semi.fp32_state:
+            semi_keep = semi  # FP64 vectors: the CPU baseline's unpacking
+        else:
             semi.close()
     torch.cuda.synchronize()
 
@@ -333,11 +344,14 @@ def main():
         extra = measure_configs(torch, th, fm, stream, 100)
 
     errors = {}
-    if "fp64" in du_out and "mixed" in du_out:
-        a, b = du_out["mixed"], du_out["fp64"]
-        errors["mixed_vs_fp64_rel_max"] = float((a - b).abs().max() / b.abs().max())
-        errors["mixed_vs_fp64_rel_l2"] = float(torch.linalg.norm(a - b) / torch.linalg.norm(b))
+    for m in ("mixed", "fp32"):
+        if "fp64" in du_out and m in du_out:
+            a, b = du_out[m], du_out["fp64"]
+            errors[f"{m}_vs_fp64_rel_max"] = float((a - b).abs().max() / b.abs().max())
+            errors[f"{m}_vs_fp64_rel_l2"] = float(torch.linalg.norm(a - b) / torch.linalg.norm(b))
 
+    if semi_keep is None:
+        semi_keep = make_semi(th, args.level, "fp64")
     cpu = None
     if not args.no_cpu_baseline:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -381,8 +395,9 @@ def main():
         "errors": errors,
         "other_configs": extra,
     }
-    if "fp64" in res and "mixed" in res:
-        line["mixed_over_fp64"] = res["mixed"]["value"] / res["fp64"]["value"]
+    for m in ("mixed", "fp32"):
+        if "fp64" in res and m in res:
+            line[f"{m}_over_fp64"] = res[m]["value"] / res["fp64"]["value"]
     print(json.dumps(line))
 
 

@@ -101,11 +101,23 @@ enum trixi_hip_precision {
     TRIXI_HIP_MIXED = 1, /* FP32 volume/surface arithmetic; FP64 state, the logarithms
                             of the ln-means (per node, and their differences), RK
                             update, shock indicator and reductions                   */
-    TRIXI_HIP_MIXED_F32LOG = 2 /* as TRIXI_HIP_MIXED with the ln-mean logarithms in
+    TRIXI_HIP_MIXED_F32LOG = 2, /* as TRIXI_HIP_MIXED with the ln-mean logarithms in
                             FP32 as well (built for 3D Euler flux_ranocha + LLF, 2D
                             Euler flux_ranocha + LLF with flux differencing or HG
                             shock capturing, 3D GLM-MHD flux_hindenlang_gassner +
                             LLF; periodic meshes): faster, less accurate             */
+    TRIXI_HIP_FP32 = 3   /* Trixi's RealT = uEltype = Float32: state, du, the RK
+                            register and all rhs! arithmetic in FP32 (logarithms
+                            included), the RK update and the shock indicator
+                            evaluated in FP64 from the FP32 values.  State vectors
+                            of such a context are float: trixi_hip_rhs_f32 and
+                            trixi_hip_rk_step_f32 (the FP64-vector entry points
+                            reject it; pack/unpack, compute_coefficients and the
+                            callbacks still take FP64 vectors).  Built for 2D/3D
+                            Euler flux differencing (flux_ranocha + LLF or
+                            flux_ranocha) and 2D Euler HG shock capturing
+                            (flux_ranocha or flux_shima_etal + LLF) on periodic
+                            meshes: the reference's benchmark configurations  */
 };
 enum trixi_hip_initial_condition {
     TRIXI_HIP_IC_CONSTANT = 0,
@@ -295,6 +307,20 @@ int trixi_hip_rk_stage(trixi_hip_ctx* ctx, const double* u, double* k, double* u
  * counterpart of CUDA.synchronize() in the reference benchmarks
  * (benchmark/euler_ec_3d.jl:58-72). */
 int trixi_hip_sync(trixi_hip_stream stream);
+
+/* rhs! of an FP32-state context (precision TRIXI_HIP_FP32): as trixi_hip_rhs
+ * with float state vectors.  Replaces rhs_gpu! with RealT = uEltype =
+ * Float32 (solvers.jl:18-31; semidiscretization_hyperbolic.jl:60-66;
+ * benchmark/euler_ec_3d.jl:6). */
+int trixi_hip_rhs_f32(trixi_hip_ctx* ctx, float* du, const float* u, double t,
+                      trixi_hip_stream stream);
+
+/* One CarpenterKennedy2N54 step of an FP32-state context, as
+ * trixi_hip_rk_step with float u and k (the update itself is evaluated in
+ * FP64 and rounded to FP32).  Replaces the OrdinaryDiffEq step on Float32
+ * CuArrays (examples/*.jl, e.g. euler_ec_3d.jl:53-55). */
+int trixi_hip_rk_step_f32(trixi_hip_ctx* ctx, float* u, float* k, double t, double dt,
+                          int32_t use_graph, trixi_hip_stream stream);
 
 /* max_dt = 2 / (N * max_e invJ_e * sum_d lambda_d) on the device; blocks until
  * the scalar is on the host.  Replaces max_dt(u::CuArray, ...)

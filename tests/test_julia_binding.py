@@ -118,6 +118,7 @@ def _c_param_type(p):
 JL_ARG_OK = {
     "Ptr{Cvoid}": {"trixi_hip_ctx*", "trixi_hip_stream"},
     "Ptr{Float64}": {"double*"},
+    "Ptr{Float32}": {"float*"},
     "Ref{Float64}": {"double*"},
     "Ref{Int32}": {"int32_t*"},
     "Ref{Int64}": {"int64_t*"},

@@ -85,7 +85,9 @@ def test_config_mapping_and_rejections(th):
     s32 = th.DGSEMHIP(polydeg=3, surface_flux=th.FluxLaxFriedrichs(th.max_abs_speed_naive),
                       volume_integral=th.VolumeIntegralFluxDifferencing(th.flux_shima_etal), RealT=np.float32)
     c = s32.config(eq)
-    assert (c.surface_flux, c.volume_flux, c.precision) == (2, 4, 1)
+    # RealT = Float32 is the FP32-state precision (TRIXI_HIP_FP32 = 3)
+    assert (c.surface_flux, c.volume_flux, c.precision) == (2, 4, 3)
+    assert th.DGSEMHIP(polydeg=3, precision="mixed").config(eq).precision == 1
     mhd = th.IdealGlmMhdEquations3D(5 / 3)
     with pytest.raises(NotImplementedError):
         th.DGSEMHIP(polydeg=3, surface_flux=th.flux_lax_friedrichs).config(mhd)

@@ -15,7 +15,7 @@ EQ_LINADV, EQ_EULER, EQ_GLM_MHD, EQ_BURGERS, EQ_HYPDIFF, EQ_EULER_MULTI, EQ_EULE
 VI_WEAK, VI_FD, VI_HG = 0, 1, 2
 (FLUX_CENTRAL, FLUX_LLF, FLUX_LLF_NAIVE, FLUX_RANOCHA, FLUX_SHIMA_ETAL, FLUX_HINDENLANG_GASSNER, FLUX_EC, FLUX_HLL,
  FLUX_CHAN_ETAL, FLUX_HLLE, FLUX_WINTERMEYER_ETAL, FLUX_FJORDHOLM_ETAL, FLUX_HYDROSTATIC_LLF) = range(13)
-PREC_FP64, PREC_MIXED, PREC_MIXED_F32LOG = 0, 1, 2
+PREC_FP64, PREC_MIXED, PREC_MIXED_F32LOG, PREC_FP32 = 0, 1, 2, 3
 BC_PERIODIC, BC_DIRICHLET, BC_SLIP_WALL = 0, 1, 2
 SOURCE_NONE, SOURCE_CONVERGENCE_TEST, SOURCE_EOC_TEST_EULER, SOURCE_POISSON_NONPERIODIC, SOURCE_HARMONIC = range(5)
 (IC_CONSTANT, IC_CONVERGENCE_TEST, IC_WEAK_BLAST_WAVE, IC_SEEDED_RANDOM, IC_SMOOTH_PERIODIC,
@@ -89,10 +89,12 @@ SIGNATURES = {
     "trixi_hip_destroy": (ctypes.c_int, [_P]),
     "trixi_hip_sizes": (ctypes.c_int, [_P, _P, _P, _P]),
     "trixi_hip_rhs": (ctypes.c_int, [_P, _P, _P, _D, _P]),
+    "trixi_hip_rhs_f32": (ctypes.c_int, [_P, _P, _P, _D, _P]),
     "trixi_hip_pack": (ctypes.c_int, [_P, _P, _P, _P]),
     "trixi_hip_unpack": (ctypes.c_int, [_P, _P, _P, _P]),
     "trixi_hip_compute_coefficients": (ctypes.c_int, [_P, _P, _I32, _D, _U64, _P]),
     "trixi_hip_rk_step": (ctypes.c_int, [_P, _P, _P, _D, _D, _I32, _P]),
+    "trixi_hip_rk_step_f32": (ctypes.c_int, [_P, _P, _P, _D, _D, _I32, _P]),
     "trixi_hip_rk_stage": (ctypes.c_int, [_P, _P, _P, _P, _I32, _D, _D, _P]),
     "trixi_hip_sync": (ctypes.c_int, [_P]),
     "trixi_hip_max_dt": (ctypes.c_int, [_P, _P, _P, _P]),

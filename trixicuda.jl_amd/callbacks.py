@@ -14,7 +14,7 @@ from .semidiscretization import _ptr, _stream
 
 
 def max_dt(u_ode, semi):
-    semi.check_vector(u_ode, "u_ode")
+    u_ode = semi.as_f64(u_ode, "u_ode")
     out = ctypes.c_double()
     L.check(L.lib().trixi_hip_max_dt(semi.ctx, _ptr(u_ode), ctypes.byref(out), _stream()))
     return out.value
@@ -22,7 +22,7 @@ def max_dt(u_ode, semi):
 
 def calc_error_norms(u_ode, t, semi, initial_condition=None):
     """(l2, linf) per variable against the initial condition at time t."""
-    semi.check_vector(u_ode, "u_ode")
+    u_ode = semi.as_f64(u_ode, "u_ode")
     ic = initial_condition or semi.initial_condition
     if not hasattr(ic, "ic_id"):
         raise NotImplementedError("error norms are computed on the device against a built initial condition")
@@ -35,7 +35,7 @@ def calc_error_norms(u_ode, t, semi, initial_condition=None):
 
 
 def integrate(u_ode, semi):
-    semi.check_vector(u_ode, "u_ode")
+    u_ode = semi.as_f64(u_ode, "u_ode")
     out = np.zeros(semi.nvariables)
     L.check(L.lib().trixi_hip_integrate(semi.ctx, _ptr(u_ode), out.ctypes.data_as(ctypes.c_void_p), _stream()))
     return out
@@ -44,7 +44,7 @@ def integrate(u_ode, semi):
 def blending_factors(u_ode, semi):
     """Hennemann-Gassner blending factors (after smoothing) as a device tensor."""
     import torch
-    semi.check_vector(u_ode, "u_ode")
+    u_ode = semi.as_f64(u_ode, "u_ode")
     alpha = torch.empty(semi.nelements, dtype=torch.float64, device=u_ode.device)
     L.check(L.lib().trixi_hip_indicator(semi.ctx, _ptr(u_ode), _ptr(alpha), _stream()))
     return alpha
@@ -53,7 +53,7 @@ def blending_factors(u_ode, semi):
 def check_state(u_ode, semi):
     """(nbad, flags): per-element flags bit0 non-finite, bit1 rho<=0, bit2 p<=0."""
     import torch
-    semi.check_vector(u_ode, "u_ode")
+    u_ode = semi.as_f64(u_ode, "u_ode")
     flags = torch.zeros(semi.nelements, dtype=torch.int32, device=u_ode.device)
     nbad = ctypes.c_int64()
     rc = L.lib().trixi_hip_check_state(semi.ctx, _ptr(u_ode), _ptr(flags), ctypes.byref(nbad), _stream())

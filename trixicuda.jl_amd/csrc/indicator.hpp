@@ -32,8 +32,8 @@ __device__ __forceinline__ int max_mode(int n) {
 // it synchronises)
 // the indicator variable (density * pressure, pressure or density) at node n
 // of element ec
-template <int DIM, int N>
-__device__ __forceinline__ double indicator_value(const double* __restrict__ u, int64_t ec, int64_t E,
+template <int DIM, int N, typename S>
+__device__ __forceinline__ double indicator_value(const S* __restrict__ u, int64_t ec, int64_t E,
                                                   const PhysR<double>& P, int n, int var, int nv) {
     constexpr int NN = cpow(N, DIM);
     // compressible Euler (DIM + 2 variables) or a scalar equation (nv = 1:
@@ -41,21 +41,21 @@ __device__ __forceinline__ double indicator_value(const double* __restrict__ u, 
     double m;
     if (var == IND_WATERHEIGHT_PRESSURE) {
         // shallow water: waterheight_pressure = h g h^2 / 2
-        const double h = u[(int64_t)n * E + ec];
+        const double h = (double)u[(int64_t)n * E + ec];
         m = h * (0.5 * P.grav * h * h);
     } else if (nv == 1) {
-        m = u[(int64_t)n * E + ec];
+        m = (double)u[(int64_t)n * E + ec];
     } else if (nv == DIM + 2) {
         double uc[DIM + 2];
 #pragma unroll
-        for (int v = 0; v < DIM + 2; ++v) uc[v] = u[((int64_t)v * NN + n) * E + ec];
+        for (int v = 0; v < DIM + 2; ++v) uc[v] = (double)u[((int64_t)v * NN + n) * E + ec];
         m = Euler<DIM>::indicator_variable(uc, P, var);
     } else {
         // ideal GLM-MHD (nine variables): density_pressure with the magnetic
         // and cleaning energies taken out of the pressure
         double uc[9];
 #pragma unroll
-        for (int v = 0; v < 9; ++v) uc[v] = u[((int64_t)v * NN + n) * E + ec];
+        for (int v = 0; v < 9; ++v) uc[v] = (double)u[((int64_t)v * NN + n) * E + ec];
         const double ke = (uc[1] * uc[1] + uc[2] * uc[2] + uc[3] * uc[3]) / uc[0];
         const double me = uc[5] * uc[5] + uc[6] * uc[6] + uc[7] * uc[7] + uc[8] * uc[8];
         m = uc[0] * P.gm1 * (uc[4] - 0.5 * ke - 0.5 * me);
@@ -86,11 +86,11 @@ __device__ __forceinline__ double modal_transform(double m, const double* __rest
 }
 
 // modal coefficients of the indicator variable at this lane's node
-template <int DIM, int N>
-__device__ __forceinline__ double modal_indicator(const double* __restrict__ u, int64_t ec, int64_t E,
+template <int DIM, int N, typename S>
+__device__ __forceinline__ double modal_indicator(const S* __restrict__ u, int64_t ec, int64_t E,
                                                   const PhysR<double>& P, const double* __restrict__ Vinv,
                                                   double* buf, int el, int n, int var, int nv) {
-    return modal_transform<DIM, N>(indicator_value<DIM, N>(u, ec, E, P, n, var, nv), Vinv, buf, el, n);
+    return modal_transform<DIM, N>(indicator_value<DIM, N, S>(u, ec, E, P, n, var, nv), Vinv, buf, el, n);
 }
 
 template <int DIM, int N>

@@ -25,7 +25,13 @@
 
 namespace trixi_hip {
 
-__host__ __device__ constexpr int cpow(int b, int e) { return e == 0 ? 1 : b * cpow(b, e - 1); }
+// b^e; a loop rather than recursion, so that a call with a runtime exponent
+// inlines instead of becoming a recursive device call (with a dynamic stack)
+__host__ __device__ constexpr int cpow(int b, int e) {
+    int r = 1;
+    for (int i = 0; i < e; ++i) r *= b;
+    return r;
+}
 
 enum Equation : int {
     EQ_LINADV = 0,

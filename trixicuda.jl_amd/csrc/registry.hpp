@@ -135,4 +135,17 @@ struct RegistrarF32Log {
     }
 };
 
+// Precision 3 (TRIXI_HIP_FP32): FP32 state vectors and FP32 arithmetic,
+// logarithms included (Trixi's RealT = uEltype = Float32), periodic meshes.
+template <int DIM, int N, class EQ, int VOL, int VF, int SF, int FVF>
+struct RegistrarF32State {
+    static void add(int eq) {
+        constexpr int MF = pick_m<DIM, N, EQ::NV, EQ::NRF, 4, 4>();
+        register_kernel({DIM, N, eq, VOL, VF, SF, FVF, 3, 0},
+                        &launch_rhs<DIM, N, EQ, VOL, VF, SF, FVF, float, MF, false, false, float, float>);
+        register_kernel({DIM, N, eq, VOL, VF, SF, FVF, 3, 1},
+                        &launch_rhs<DIM, N, EQ, VOL, VF, SF, FVF, float, MF, true, false, float, float>);
+    }
+};
+
 }  // namespace trixi_hip

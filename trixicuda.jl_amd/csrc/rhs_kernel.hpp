@@ -45,6 +45,11 @@ constexpr int MAXN = 8;
 #ifndef TRIXI_HIP_MIN_WAVES
 #define TRIXI_HIP_MIN_WAVES 1
 #endif
+// Fused RK stage: 1 keeps u and k of the node pass in registers for the
+// register update, 0 reads them again there.
+#ifndef TRIXI_HIP_RK_KEEP
+#define TRIXI_HIP_RK_KEEP 1
+#endif
 // 1: node records are computed once per node into LDS; 0: every lane
 // computes the node data of its line in each direction phase (no record LDS).
 #ifndef TRIXI_HIP_STAGE_NODES
@@ -61,10 +66,12 @@ constexpr int MAXN = 8;
 #endif
 
 struct RhsArgs {
-    const double* u;          // SoA state  u[(v*NN + node)*E + e]
-    double* du;               // SoA output (rhs mode)
-    double* k;                // 2N register (RK mode)
-    double* u_out;            // RK mode: u_out = u + b k
+    // state vectors of the kernel's state type S (double, or float for the
+    // FP32-state precision)
+    const void* u;            // SoA state  u[(v*NN + node)*E + e]
+    void* du;                 // SoA output (rhs mode)
+    void* k;                  // 2N register (RK mode)
+    void* u_out;              // RK mode: u_out = u + b k
     const double* time;       // RK mode: device scalars {t, dt}
     const int32_t* nbr;       // [2*DIM][E] neighbour element per face (-x,+x,-y,+y,-z,+z)
     const double* invJ;       // [E]
@@ -126,7 +133,7 @@ __device__ __forceinline__ void source_at(const RhsArgs& a, int64_t e, int node,
         // hyperbolic diffusion: the relaxation -q / T_r reads this node's q
         double qv[3] = {0.0, 0.0, 0.0};
 #pragma unroll
-        for (int d = 0; d < DIM; ++d) qv[d] = a.u[((int64_t)(1 + d) * NN + node) * a.E + e];
+        for (int d = 0; d < DIM; ++d) qv[d] = static_cast<const double*>(a.u)[((int64_t)(1 + d) * NN + node) * a.E + e];
         double s4[4];
         source_hypdiff(DIM, a.src, a.phys.nu, a.phys.inv_Tr, xx, qv, s4);
 #pragma unroll
@@ -197,9 +204,14 @@ __host__ __device__ constexpr int rhs_threads() {
 // mortar faces (nbr = -2 - slot, fluxes from the mortar pass) and source
 // terms; the periodic, conforming, source-free kernels carry none of that code.
 template <int DIM, int N, class EQ, int VOL, int VF, int SF, int FVF, typename R, int M, bool RK, bool EXT,
-          typename L>
+          typename L, typename S>
 __global__ void __launch_bounds__((rhs_threads<DIM, N, M, R, RK, EQ::NV>()), TRIXI_HIP_MIN_WAVES)
 rhs_kernel(const RhsArgs a) {
+    static_assert(!EXT || std::is_same<S, double>::value, "boundaries, mortars and sources take FP64 states");
+    const S* __restrict__ U = static_cast<const S*>(a.u);
+    S* __restrict__ DU = static_cast<S*>(a.du);
+    S* __restrict__ K = static_cast<S*>(a.k);
+    S* __restrict__ UO = static_cast<S*>(a.u_out);
     constexpr int NL = cpow(N, DIM - 1);
     constexpr int NN = cpow(N, DIM);
     // PD: lane group g (whole waves) evaluates direction g; the groups meet in
@@ -275,14 +287,17 @@ rhs_kernel(const RhsArgs a) {
             }
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
-            um_next[v] = a.u[(int64_t)(v * NN + base0 + (N - 1) * s0) * E + nm0];
-            up_next[v] = a.u[(int64_t)(v * NN + base0) * E + np0];
+            um_next[v] = (double)U[(int64_t)(v * NN + base0 + (N - 1) * s0) * E + nm0];
+            up_next[v] = (double)U[(int64_t)(v * NN + base0) * E + np0];
         }
     }
 
     // ---- node data, once per node: coalesced loads (element innermost) -----
     // Every lane visits NIT (node, element) pairs: NN * M / NT, rounded up.
-    double ukeep[COOP ? NIT : 1][NV], kkeep[COOP ? NIT : 1][NV];
+    // RK_KEEP: the node pass keeps u and k in registers for the update;
+    // otherwise the update reads them again with the same coalesced pattern
+    constexpr bool RK_KEEP = COOP && TRIXI_HIP_RK_KEEP != 0;
+    double ukeep[RK_KEEP ? NIT : 1][NV], kkeep[RK_KEEP ? NIT : 1][NV];
 #pragma unroll
     for (int it = 0; it < (STAGE ? NIT : 0); ++it) {
         const int q = tid + it * NT;
@@ -295,11 +310,11 @@ rhs_kernel(const RhsArgs a) {
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
             const int64_t g = (int64_t)(v * NN + n) * E + ee;
-            const double ud = a.u[g];
+            const double ud = (double)U[g];
             uc[v] = (R)ud;
-            if constexpr (COOP) {
+            if constexpr (RK_KEEP) {
                 ukeep[it][v] = ud;
-                kkeep[it][v] = a.rk_first ? 0.0 : a.k[g];
+                kkeep[it][v] = a.rk_first ? 0.0 : (double)K[g];
             }
         }
         const Node nd = EQ::template node<R, NODE_LOGS, L>(uc, P);
@@ -360,11 +375,11 @@ rhs_kernel(const RhsArgs a) {
                 const int64_t g = (int64_t)(v * NN + node) * E + e;
                 const double val = (double)total * fac + sv[v];
                 if (!RK) {
-                    a.du[g] = val;
+                    DU[g] = (S)val;
                 } else {
-                    const double kk = a.rk_first ? t_dt * val : a.rk_a * a.k[g] + t_dt * val;
-                    a.k[g] = kk;
-                    a.u_out[g] = a.u[g] + a.rk_b * kk;
+                    const S kk = (S)(a.rk_first ? t_dt * val : a.rk_a * (double)K[g] + t_dt * val);
+                    K[g] = kk;
+                    UO[g] = (S)((double)U[g] + a.rk_b * (double)kk);
                 }
             }
         }
@@ -389,11 +404,11 @@ rhs_kernel(const RhsArgs a) {
             if (nb[2 * d] == -1) {
                 if ((a.outflow >> (2 * d)) & 1) {
 #pragma unroll
-                    for (int v = 0; v < NV; ++v) um[v] = (R)a.u[(int64_t)(v * NN + base) * E + ec];
+                    for (int v = 0; v < NV; ++v) um[v] = (R)U[(int64_t)(v * NN + base) * E + ec];
                 } else if (a.bc == TRIXI_HIP_BC_SLIP_WALL) {
                     // slip wall: the inner state with its normal momentum reversed
 #pragma unroll
-                    for (int v = 0; v < NV; ++v) um[v] = (R)a.u[(int64_t)(v * NN + base) * E + ec];
+                    for (int v = 0; v < NV; ++v) um[v] = (R)U[(int64_t)(v * NN + base) * E + ec];
                     um[1 + d] = -um[1 + d];
                 } else {
                     boundary_state<DIM, NN, NV, R>(a, ec, base, tnow, um);
@@ -402,10 +417,10 @@ rhs_kernel(const RhsArgs a) {
             if (nb[2 * d + 1] == -1) {
                 if ((a.outflow >> (2 * d + 1)) & 1) {
 #pragma unroll
-                    for (int v = 0; v < NV; ++v) up[v] = (R)a.u[(int64_t)(v * NN + base + (N - 1) * s) * E + ec];
+                    for (int v = 0; v < NV; ++v) up[v] = (R)U[(int64_t)(v * NN + base + (N - 1) * s) * E + ec];
                 } else if (a.bc == TRIXI_HIP_BC_SLIP_WALL) {
 #pragma unroll
-                    for (int v = 0; v < NV; ++v) up[v] = (R)a.u[(int64_t)(v * NN + base + (N - 1) * s) * E + ec];
+                    for (int v = 0; v < NV; ++v) up[v] = (R)U[(int64_t)(v * NN + base + (N - 1) * s) * E + ec];
                     up[1 + d] = -up[1 + d];
                 } else {
                     boundary_state<DIM, NN, NV, R>(a, ec, base + (N - 1) * s, tnow, up);
@@ -417,8 +432,8 @@ rhs_kernel(const RhsArgs a) {
             const int bn = line_base<DIM, N>(line, d + 1);
 #pragma unroll
             for (int v = 0; v < NV; ++v) {
-                um_next[v] = a.u[(int64_t)(v * NN + bn + (N - 1) * sn) * E + nbl[2 * (d + 1)]];
-                up_next[v] = a.u[(int64_t)(v * NN + bn) * E + nbl[2 * (d + 1) + 1]];
+                um_next[v] = (double)U[(int64_t)(v * NN + bn + (N - 1) * sn) * E + nbl[2 * (d + 1)]];
+                up_next[v] = (double)U[(int64_t)(v * NN + bn) * E + nbl[2 * (d + 1) + 1]];
             }
         }
 
@@ -436,7 +451,7 @@ rhs_kernel(const RhsArgs a) {
             } else {
                 R uc[NV];
 #pragma unroll
-                for (int v = 0; v < NV; ++v) uc[v] = (R)a.u[(int64_t)(v * NN + node) * E + ec];
+                for (int v = 0; v < NV; ++v) uc[v] = (R)U[(int64_t)(v * NN + node) * E + ec];
                 nd[p] = EQ::template node<R, NODE_LOGS, L>(uc, P);
             }
         }
@@ -656,9 +671,11 @@ rhs_kernel(const RhsArgs a) {
             for (int v = 0; v < NV; ++v) {
                 const int64_t g = (int64_t)(v * NN + n) * E + ee;
                 const double val = (double)acc[(v * NN + n) * LDM + c] * fac + sv[v];
-                const double kk = a.rk_first ? t_dt * val : a.rk_a * kkeep[it][v] + t_dt * val;
-                a.k[g] = kk;
-                a.u_out[g] = ukeep[it][v] + a.rk_b * kk;
+                const double kold = RK_KEEP ? kkeep[it][v] : (a.rk_first ? 0.0 : (double)K[g]);
+                const double uold = RK_KEEP ? ukeep[it][v] : (double)U[g];
+                const S kk = (S)(a.rk_first ? t_dt * val : a.rk_a * kold + t_dt * val);
+                K[g] = kk;
+                UO[g] = (S)(uold + a.rk_b * (double)kk);
             }
         }
     }
@@ -703,9 +720,13 @@ __host__ __device__ constexpr int np_elems() {
     return (256 / (cpow(N, DIM) * DIM)) > 0 ? 256 / (cpow(N, DIM) * DIM) : 1;
 }
 
-template <int DIM, int N, class EQ, int VOL, int VF, int SF, int FVF, typename R, bool RK, typename L>
+template <int DIM, int N, class EQ, int VOL, int VF, int SF, int FVF, typename R, bool RK, typename L, typename S>
 __global__ void __launch_bounds__((np_elems<DIM, N>() * cpow(N, DIM) * DIM))
 rhs_np_kernel(const RhsArgs a) {
+    const S* __restrict__ U = static_cast<const S*>(a.u);
+    S* __restrict__ DU = static_cast<S*>(a.du);
+    S* __restrict__ K = static_cast<S*>(a.k);
+    S* __restrict__ UO = static_cast<S*>(a.u_out);
     constexpr int NN = cpow(N, DIM);
     constexpr int M = np_elems<DIM, N>();
     constexpr int NV = EQ::NV;
@@ -746,13 +767,13 @@ rhs_np_kernel(const RhsArgs a) {
         const int32_t nbe = a.nbr[(int64_t)(2 * d + (p == 0 ? 0 : 1)) * E + ec];
         const int nn_face = p == 0 ? node + (N - 1) * s : node - (N - 1) * s;
 #pragma unroll
-        for (int v = 0; v < NV; ++v) ub[v] = (R)a.u[(int64_t)(v * NN + nn_face) * E + nbe];
+        for (int v = 0; v < NV; ++v) ub[v] = (R)U[(int64_t)(v * NN + nn_face) * E + nbe];
     }
 
     if (last) {
         R uc[NV];
 #pragma unroll
-        for (int v = 0; v < NV; ++v) uc[v] = (R)a.u[(int64_t)(v * NN + node) * E + ec];
+        for (int v = 0; v < NV; ++v) uc[v] = (R)U[(int64_t)(v * NN + node) * E + ec];
         const Node nd = EQ::template node<R, NODE_LOGS, L>(uc, P);
         R f[NRF];
         EQ::to_rec(nd, f);
@@ -931,12 +952,12 @@ rhs_np_kernel(const RhsArgs a) {
         const int64_t g = (int64_t)(v * NN + node) * E + e;
         const double val = (double)total * fac;
         if constexpr (!RK) {
-            a.du[g] = val;
+            DU[g] = (S)val;
         } else {
             // u and k again (cache resident on the small meshes this form runs)
-            const double kk = a.rk_first ? t_dt * val : a.rk_a * a.k[g] + t_dt * val;
-            a.k[g] = kk;
-            a.u_out[g] = a.u[g] + a.rk_b * kk;
+            const S kk = (S)(a.rk_first ? t_dt * val : a.rk_a * (double)K[g] + t_dt * val);
+            K[g] = kk;
+            UO[g] = (S)((double)U[g] + a.rk_b * (double)kk);
         }
     }
 }
@@ -953,21 +974,21 @@ __host__ __device__ constexpr bool np_eligible() {
 // Launch helper used by the instantiation units: the node-parallel form for
 // small periodic conforming meshes (RhsArgs::np), the line form otherwise.
 template <int DIM, int N, class EQ, int VOL, int VF, int SF, int FVF, typename R, int M, bool RK, bool EXT = false,
-          typename L = double>
+          typename L = double, typename S = double>
 hipError_t launch_rhs(const RhsArgs& a, hipStream_t stream) {
     if constexpr (np_eligible<DIM, N, EQ::NV, VOL, EXT>()) {
         if (a.np) {
             constexpr int MN = np_elems<DIM, N>();
             int64_t blocks = (a.E + MN - 1) / MN;
             if (a.nxcd > 1) blocks = (blocks + a.nxcd - 1) / a.nxcd * a.nxcd;
-            hipLaunchKernelGGL((rhs_np_kernel<DIM, N, EQ, VOL, VF, SF, FVF, R, RK, L>), dim3((unsigned)blocks),
+            hipLaunchKernelGGL((rhs_np_kernel<DIM, N, EQ, VOL, VF, SF, FVF, R, RK, L, S>), dim3((unsigned)blocks),
                                dim3(MN * cpow(N, DIM) * DIM), 0, stream, a);
             return hipGetLastError();
         }
     }
     int64_t blocks = (a.E + M - 1) / M;
     if (a.nxcd > 1) blocks = (blocks + a.nxcd - 1) / a.nxcd * a.nxcd;
-    hipLaunchKernelGGL((rhs_kernel<DIM, N, EQ, VOL, VF, SF, FVF, R, M, RK, EXT, L>), dim3((unsigned)blocks),
+    hipLaunchKernelGGL((rhs_kernel<DIM, N, EQ, VOL, VF, SF, FVF, R, M, RK, EXT, L, S>), dim3((unsigned)blocks),
                        dim3(rhs_threads<DIM, N, M, R, RK, EQ::NV>()), 0, stream, a);
     return hipGetLastError();
 }

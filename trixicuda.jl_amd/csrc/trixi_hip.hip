@@ -74,6 +74,7 @@ void register_vjp_mhd3d();
 void register_euler_ext();
 void register_vjp_euler_ext();
 void register_mortar_kernels();
+void register_fp32();
 
 static void ensure_registered() {
     static std::once_flag once;
@@ -102,6 +103,7 @@ static void ensure_registered() {
         register_euler_ext();
         register_vjp_euler_ext();
         register_mortar_kernels();
+        register_fp32();
     });
 }
 
@@ -366,8 +368,8 @@ constexpr int ind_epb() {
     return ind_block<DIM, N>() / cpow(N, DIM);
 }
 
-template <int DIM, int N>
-__global__ void __launch_bounds__(256) indicator_kernel(const double* __restrict__ u, double* __restrict__ alpha_raw,
+template <int DIM, int N, typename S = double>
+__global__ void __launch_bounds__(256) indicator_kernel(const S* __restrict__ u, double* __restrict__ alpha_raw,
                                                         int64_t E, PhysParams phys, const double* __restrict__ Vinv,
                                                         double threshold, double parameter_s, double alpha_min,
                                                         double alpha_max, int var, int nv) {
@@ -384,7 +386,7 @@ __global__ void __launch_bounds__(256) indicator_kernel(const double* __restrict
     const int64_t e = (int64_t)blockIdx.x * EPB + elc;
     const int64_t ec = e < E ? e : E - 1;
     const PhysR<double> P = PhysR<double>::from(phys);
-    const double m = modal_indicator<DIM, N>(u, ec, E, P, Vinv, buf, elc, n, var, nv);
+    const double m = modal_indicator<DIM, N, S>(u, ec, E, P, Vinv, buf, elc, n, var, nv);
     e2[threadIdx.x] = m * m;
     __syncthreads();
     if (n != 0 || el >= EPB || e >= E) return;
@@ -446,7 +448,7 @@ __global__ void __launch_bounds__(64) indicator_vjp_kernel(
     const bool valid = e < E;
     const int64_t ec = valid ? e : E - 1;
     const PhysR<double> P = PhysR<double>::from(phys);
-    const double m = modal_indicator<DIM, N>(u, ec, E, P, Vinv, buf, el, n, var, nv);
+    const double m = modal_indicator<DIM, N, double>(u, ec, E, P, Vinv, buf, el, n, var, nv);
     e2[threadIdx.x] = m * m;
     __syncthreads();
     if (n == 0) {
@@ -835,8 +837,8 @@ struct trixi_hip_ctx {
     // graph cache for rk steps
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
-    double* graph_u = nullptr;
-    double* graph_k = nullptr;
+    void* graph_u = nullptr;
+    void* graph_k = nullptr;
     hipStream_t graph_stream = nullptr;
     bool has_graph = false;
 };
@@ -889,23 +891,32 @@ static void free_ctx(trixi_hip_ctx* c) {
     delete c;
 }
 
-static int launch_indicator(trixi_hip_ctx* c, const double* u, hipStream_t s) {
+static bool fp32_state(const trixi_hip_ctx* c) { return c->cfg.precision == TRIXI_HIP_FP32; }
+
+// Raw blending factors of state u (FP64, or FP32 when f32).
+static int launch_indicator(trixi_hip_ctx* c, const void* u, hipStream_t s, bool f32 = false) {
     const double threshold = 0.5 * pow(10.0, -1.8 * pow((double)c->N, 0.25));
     const double parameter_s = log((1.0 - 0.0001) / 0.0001);
     const PhysParams& P = c->args.phys;
     if (c->N != 4 && c->N != 5) return fail(TRIXI_HIP_EUNSUPPORTED, "shock capturing is built for polydeg 3 and 4");
     const int var = c->cfg.equation == TRIXI_HIP_SHALLOW_WATER ? IND_WATERHEIGHT_PRESSURE : c->cfg.indicator_variable;
-#define TRIXI_HIP_IND(D, NQ)                                                                                  \
-    hipLaunchKernelGGL((indicator_kernel<D, NQ>),                                                             \
+#define TRIXI_HIP_IND_S(D, NQ, ST)                                                                             \
+    hipLaunchKernelGGL((indicator_kernel<D, NQ, ST>),                                                         \
                        dim3((unsigned)((c->E + ind_epb<D, NQ>() - 1) / ind_epb<D, NQ>())),                    \
-                       dim3(ind_block<D, NQ>()), 0, s, u, c->d_alpha_raw, c->E, P, c->d_Vinv, threshold,     \
-                       parameter_s, c->cfg.alpha_min, c->cfg.alpha_max, var, c->nv)
+                       dim3(ind_block<D, NQ>()), 0, s, static_cast<const ST*>(u), c->d_alpha_raw, c->E, P,   \
+                       c->d_Vinv, threshold, parameter_s, c->cfg.alpha_min, c->cfg.alpha_max, var, c->nv)
+#define TRIXI_HIP_IND(D, NQ)                                  \
+    do {                                                      \
+        if (f32) TRIXI_HIP_IND_S(D, NQ, float);                \
+        else TRIXI_HIP_IND_S(D, NQ, double);                   \
+    } while (0)
     if (c->N == 4) {
         if (c->dim == 1) TRIXI_HIP_IND(1, 4); else if (c->dim == 2) TRIXI_HIP_IND(2, 4); else TRIXI_HIP_IND(3, 4);
     } else {
         if (c->dim == 1) TRIXI_HIP_IND(1, 5); else if (c->dim == 2) TRIXI_HIP_IND(2, 5); else TRIXI_HIP_IND(3, 5);
     }
 #undef TRIXI_HIP_IND
+#undef TRIXI_HIP_IND_S
     HIP_TRY(hipGetLastError());
     return TRIXI_HIP_OK;
 }
@@ -1378,17 +1389,32 @@ int trixi_hip_sizes(const trixi_hip_ctx* ctx, int32_t* nvars, int32_t* nnodes, i
     return TRIXI_HIP_OK;
 }
 
+static const char* const kNeedF32 = "this context's state is FP32: use trixi_hip_rhs_f32 / trixi_hip_rk_step_f32";
+static const char* const kNeedF64 = "this context's state is FP64: use trixi_hip_rhs / trixi_hip_rk_step";
+
+// rhs! of state u into du (state type of the context)
+static int enqueue_rhs(trixi_hip_ctx* ctx, void* du, const void* u, double t, hipStream_t s);
+
 int trixi_hip_rhs(trixi_hip_ctx* ctx, double* du, const double* u, double t, trixi_hip_stream stream) {
     if (!ctx || !du || !u) return fail(TRIXI_HIP_EINVAL, "null argument");
+    if (fp32_state(ctx)) return fail(TRIXI_HIP_EINVAL, kNeedF32);
+    return enqueue_rhs(ctx, du, u, t, (hipStream_t)stream);
+}
+
+int trixi_hip_rhs_f32(trixi_hip_ctx* ctx, float* du, const float* u, double t, trixi_hip_stream stream) {
+    if (!ctx || !du || !u) return fail(TRIXI_HIP_EINVAL, "null argument");
+    if (!fp32_state(ctx)) return fail(TRIXI_HIP_EINVAL, kNeedF64);
+    return enqueue_rhs(ctx, du, u, t, (hipStream_t)stream);
+}
+
+static int enqueue_rhs(trixi_hip_ctx* ctx, void* du, const void* u, double t, hipStream_t s) {
     if (du == u) return fail(TRIXI_HIP_EINVAL, "du and u must not alias");
-    (void)t;
-    hipStream_t s = (hipStream_t)stream;
     if (ctx->cfg.volume_integral == TRIXI_HIP_VOLUME_SHOCK_CAPTURING_HG) {
-        int rc = launch_indicator(ctx, u, s);
+        int rc = launch_indicator(ctx, u, s, fp32_state(ctx));
         if (rc) return rc;
     }
-    {
-        int rc = launch_mortars(ctx, u, s);
+    if (!fp32_state(ctx)) {
+        int rc = launch_mortars(ctx, static_cast<const double*>(u), s);
         if (rc) return rc;
     }
     RhsArgs a = ctx->args;
@@ -1452,20 +1478,22 @@ int trixi_hip_compute_coefficients(trixi_hip_ctx* ctx, double* u, int32_t ic, do
 // it cannot update in place: stage 0 reads u and writes buffer A, then the
 // stages alternate A -> B -> A -> B, and stage 4 reads B and writes the
 // caller's u, which no stage after 0 reads.
-static int enqueue_rk_stages(trixi_hip_ctx* c, double* u, double* k, hipStream_t s) {
-    double* bufA = c->d_tmp;
-    double* bufB = c->d_tmp + c->nvalues;
-    double* const in[5] = {u, bufA, bufB, bufA, bufB};
-    double* const out[5] = {bufA, bufB, bufA, bufB, u};
+static int enqueue_rk_stages(trixi_hip_ctx* c, void* u, void* k, hipStream_t s) {
+    // two stage buffers of the context's state type in d_tmp
+    const size_t bytes = (size_t)c->nvalues * (fp32_state(c) ? sizeof(float) : sizeof(double));
+    void* bufA = c->d_tmp;
+    void* bufB = static_cast<char*>(static_cast<void*>(c->d_tmp)) + bytes;
+    void* const in[5] = {u, bufA, bufB, bufA, bufB};
+    void* const out[5] = {bufA, bufB, bufA, bufB, u};
     for (int st = 0; st < 5; ++st) {
-        double* cur = in[st];
-        double* nxt = out[st];
+        void* cur = in[st];
+        void* nxt = out[st];
         if (c->cfg.volume_integral == TRIXI_HIP_VOLUME_SHOCK_CAPTURING_HG) {
-            int rc = launch_indicator(c, cur, s);
+            int rc = launch_indicator(c, cur, s, fp32_state(c));
             if (rc) return rc;
         }
-        {
-            int rc = launch_mortars(c, cur, s);
+        if (!fp32_state(c)) {
+            int rc = launch_mortars(c, static_cast<const double*>(cur), s);
             if (rc) return rc;
         }
         RhsArgs a = c->args;
@@ -1486,6 +1514,7 @@ int trixi_hip_rk_stage(trixi_hip_ctx* ctx, const double* u, double* k, double* u
     if (!ctx || !u || !k || !u_out) return fail(TRIXI_HIP_EINVAL, "null argument");
     if (stage < 0 || stage > 4) return fail(TRIXI_HIP_EINVAL, "stage must be 0..4");
     if (u_out == u || k == u || k == u_out) return fail(TRIXI_HIP_EINVAL, "u, k and u_out must be distinct");
+    if (fp32_state(ctx)) return fail(TRIXI_HIP_EINVAL, kNeedF32);
     hipStream_t s = (hipStream_t)stream;
     hipLaunchKernelGGL(set_time_kernel, dim3(1), dim3(1), 0, s, ctx->d_time, t, dt, CK_C[0], CK_C[1], CK_C[2],
                        CK_C[3], CK_C[4]);
@@ -1516,10 +1545,26 @@ int trixi_hip_sync(trixi_hip_stream stream) {
     return TRIXI_HIP_OK;
 }
 
+static int rk_step_any(trixi_hip_ctx* ctx, void* u, void* k, double t, double dt, int32_t use_graph,
+                       hipStream_t s);
+
 int trixi_hip_rk_step(trixi_hip_ctx* ctx, double* u, double* k, double t, double dt, int32_t use_graph,
                       trixi_hip_stream stream) {
     if (!ctx || !u || !k) return fail(TRIXI_HIP_EINVAL, "null argument");
-    hipStream_t s = (hipStream_t)stream;
+    if (fp32_state(ctx)) return fail(TRIXI_HIP_EINVAL, kNeedF32);
+    return rk_step_any(ctx, u, k, t, dt, use_graph, (hipStream_t)stream);
+}
+
+int trixi_hip_rk_step_f32(trixi_hip_ctx* ctx, float* u, float* k, double t, double dt, int32_t use_graph,
+                          trixi_hip_stream stream) {
+    if (!ctx || !u || !k) return fail(TRIXI_HIP_EINVAL, "null argument");
+    if (!fp32_state(ctx)) return fail(TRIXI_HIP_EINVAL, kNeedF64);
+    return rk_step_any(ctx, u, k, t, dt, use_graph, (hipStream_t)stream);
+}
+
+static int rk_step_any(trixi_hip_ctx* ctx, void* u, void* k, double t, double dt, int32_t use_graph,
+                       hipStream_t s) {
+    if (u == k) return fail(TRIXI_HIP_EINVAL, "u and k must be distinct");
     hipLaunchKernelGGL(set_time_kernel, dim3(1), dim3(1), 0, s, ctx->d_time, t, dt, CK_C[0], CK_C[1], CK_C[2],
                        CK_C[3], CK_C[4]);
     HIP_TRY(hipGetLastError());
@@ -1728,6 +1773,7 @@ static bool is_hg(const trixi_hip_ctx* c) {
 
 static int require_vjp(trixi_hip_ctx* c) {
     if (c->vjp_fn) return TRIXI_HIP_OK;
+    if (fp32_state(c)) return fail(TRIXI_HIP_EUNSUPPORTED, "the adjoint takes FP64 states (no FP32-state adjoint)");
     if (c->nmortars > 0) return fail(TRIXI_HIP_EUNSUPPORTED, "no adjoint kernel for meshes with mortars");
     if (c->cfg.boundary_outflow_faces != 0)
         return fail(TRIXI_HIP_EUNSUPPORTED, "no adjoint kernel for outflow boundaries");

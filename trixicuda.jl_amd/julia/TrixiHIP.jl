@@ -147,16 +147,16 @@ struct DGSEMHIP{VI, SF}
     polydeg::Int
     surface_flux::SF
     volume_integral::VI
-    precision::Symbol      # :fp64, :mixed or :mixed_f32log
+    precision::Symbol      # :fp64, :mixed, :mixed_f32log or :fp32 (Float32 state vectors)
 end
 
 # enum trixi_hip_precision
 precision_id(p::Symbol) = p === :fp64 ? Int32(0) : p === :mixed ? Int32(1) : p === :mixed_f32log ? Int32(2) :
-    throw(ArgumentError("precision must be :fp64, :mixed or :mixed_f32log"))
+    p === :fp32 ? Int32(3) : throw(ArgumentError("precision must be :fp64, :mixed, :mixed_f32log or :fp32"))
 
 function DGSEMHIP(; RealT = Float64, polydeg::Integer, surface_flux = flux_central,
                   volume_integral = VolumeIntegralWeakForm(), precision = nothing)
-    prec = precision === nothing ? (RealT === Float32 ? :mixed : :fp64) : precision
+    prec = precision === nothing ? (RealT === Float32 ? :fp32 : :fp64) : precision
     DGSEMHIP(Int(polydeg), surface_flux, volume_integral, prec)
 end
 
@@ -302,6 +302,15 @@ function rhs_hip!(du_ode::ROCVector{Float64}, u_ode::ROCVector{Float64},
     nothing
 end
 
+# RealT = Float32 (precision :fp32): Float32 state vectors
+function rhs_hip!(du_ode::ROCVector{Float32}, u_ode::ROCVector{Float32},
+                  semi::SemidiscretizationHyperbolicHIP, t)
+    check(ccall((:trixi_hip_rhs_f32, libtrixihip), Cint,
+                (Ptr{Cvoid}, Ptr{Float32}, Ptr{Float32}, Float64, Ptr{Cvoid}),
+                semi.ctx, pointer(du_ode), pointer(u_ode), t, stream_ptr()))
+    nothing
+end
+
 function compute_coefficients_hip(t, semi::SemidiscretizationHyperbolicHIP)
     u = ROCVector{Float64}(undef, semi.nvalues)
     if !is_builtin_ic(semi.initial_condition)
@@ -367,6 +376,14 @@ function rk_step_hip!(u::ROCVector{Float64}, k::ROCVector{Float64}, t, dt,
                       semi::SemidiscretizationHyperbolicHIP; use_graph = true)
     check(ccall((:trixi_hip_rk_step, libtrixihip), Cint,
                 (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Float64, Float64, Int32, Ptr{Cvoid}),
+                semi.ctx, pointer(u), pointer(k), t, dt, use_graph ? 1 : 0, stream_ptr()))
+    nothing
+end
+
+function rk_step_hip!(u::ROCVector{Float32}, k::ROCVector{Float32}, t, dt,
+                      semi::SemidiscretizationHyperbolicHIP; use_graph = true)
+    check(ccall((:trixi_hip_rk_step_f32, libtrixihip), Cint,
+                (Ptr{Cvoid}, Ptr{Float32}, Ptr{Float32}, Float64, Float64, Int32, Ptr{Cvoid}),
                 semi.ctx, pointer(u), pointer(k), t, dt, use_graph ? 1 : 0, stream_ptr()))
     nothing
 end

@@ -160,16 +160,33 @@ class SemidiscretizationHyperbolicHIP:
         """Trixi's u[v, i, j, k, e] as a C-ordered array (axes reversed)."""
         return (self.nelements,) + (self.nnodes,) * self.mesh.ndims + (self.nvariables,)
 
-    def allocate_coefficients(self):
-        torch = _torch()
-        return torch.empty(self.nvalues, dtype=torch.float64, device="cuda")
+    @property
+    def fp32_state(self):
+        """True for the FP32-state precision (float32 state vectors)."""
+        return self.solver.precision == "fp32"
 
-    def check_vector(self, x, name):
+    @property
+    def state_dtype(self):
         torch = _torch()
-        if not (isinstance(x, torch.Tensor) and x.is_cuda and x.dtype == torch.float64 and x.is_contiguous()
+        return torch.float32 if self.fp32_state else torch.float64
+
+    def allocate_coefficients(self, dtype=None):
+        torch = _torch()
+        return torch.empty(self.nvalues, dtype=dtype or self.state_dtype, device="cuda")
+
+    def check_vector(self, x, name, dtype=None):
+        torch = _torch()
+        dtype = dtype or self.state_dtype
+        if not (isinstance(x, torch.Tensor) and x.is_cuda and x.dtype == dtype and x.is_contiguous()
                 and x.numel() == self.nvalues):
-            raise L.TrixiHipError(L.EINVAL, f"{name} must be a contiguous float64 CUDA tensor with "
-                                            f"{self.nvalues} values")
+            raise L.TrixiHipError(L.EINVAL, f"{name} must be a contiguous {str(dtype).split('.')[-1]} CUDA tensor "
+                                            f"with {self.nvalues} values")
+
+    def as_f64(self, x, name):
+        """A state vector as float64 (the callbacks, pack and unpack work on
+        FP64 vectors; an FP32 state is widened, a copy)."""
+        self.check_vector(x, name)
+        return x.double() if self.fp32_state else x
 
     def __repr__(self):
         return (f"SemidiscretizationHyperbolicHIP({self.mesh}, {type(self.equations).__name__}, "
@@ -193,9 +210,12 @@ def compute_coefficients(t, semi, initial_condition=None):
         torch = _torch()
         host = host_initial_state(ic, t, semi)
         return pack(torch.from_numpy(host).to("cuda"), semi)
-    u = semi.allocate_coefficients()
+    torch = _torch()
+    u = semi.allocate_coefficients(torch.float64)
     L.check(L.lib().trixi_hip_compute_coefficients(semi.ctx, _ptr(u), ic.ic_id, float(t), ic.seed, _stream()))
-    return u
+    # an FP32 state is the FP64 one rounded (Trixi evaluates the initial
+    # condition in RealT; rounding the FP64 value is within its last bit)
+    return u.float() if semi.fp32_state else u
 
 
 def host_initial_state(initial_condition, t, semi):
@@ -221,7 +241,8 @@ def rhs_hip_(du_ode, u_ode, semi, t):
     """du_ode = rhs(u_ode, t), asynchronous on the current stream (rhs_gpu!)."""
     semi.check_vector(du_ode, "du_ode")
     semi.check_vector(u_ode, "u_ode")
-    L.check(L.lib().trixi_hip_rhs(semi.ctx, _ptr(du_ode), _ptr(u_ode), float(t), _stream()))
+    fn = L.lib().trixi_hip_rhs_f32 if semi.fp32_state else L.lib().trixi_hip_rhs
+    L.check(fn(semi.ctx, _ptr(du_ode), _ptr(u_ode), float(t), _stream()))
 
 
 class ODEProblem:
@@ -248,17 +269,19 @@ def pack(u_trixi, semi):
     """Trixi-layout device tensor -> new native-layout state vector."""
     torch = _torch()
     src = u_trixi.contiguous().view(-1)
+    if semi.fp32_state and src.dtype in (torch.float32, torch.float64):
+        src = src.double()
     if src.numel() != semi.nvalues or src.dtype != torch.float64 or not src.is_cuda:
         raise L.TrixiHipError(L.EINVAL, "u_trixi has the wrong size, dtype or device")
-    out = semi.allocate_coefficients()
+    out = semi.allocate_coefficients(torch.float64)
     L.check(L.lib().trixi_hip_pack(semi.ctx, _ptr(out), _ptr(src), _stream()))
-    return out
+    return out.float() if semi.fp32_state else out
 
 
 def wrap_array_native(u_ode, semi):
     """Trixi-layout copy u[e, (k, j,) i, v] of a native state vector."""
     torch = _torch()
-    semi.check_vector(u_ode, "u_ode")
+    u64 = semi.as_f64(u_ode, "u_ode")
     out = torch.empty(semi.trixi_shape, dtype=torch.float64, device=u_ode.device)
-    L.check(L.lib().trixi_hip_unpack(semi.ctx, _ptr(out), _ptr(u_ode), _stream()))
-    return out
+    L.check(L.lib().trixi_hip_unpack(semi.ctx, _ptr(out), _ptr(u64), _stream()))
+    return out.float() if semi.fp32_state else out

@@ -69,7 +69,8 @@ class VolumeIntegralShockCapturingHG:
 
 
 # precision name -> enum trixi_hip_precision
-PRECISIONS = {"fp64": L.PREC_FP64, "mixed": L.PREC_MIXED, "mixed_f32log": L.PREC_MIXED_F32LOG}
+PRECISIONS = {"fp64": L.PREC_FP64, "mixed": L.PREC_MIXED, "mixed_f32log": L.PREC_MIXED_F32LOG,
+              "fp32": L.PREC_FP32}
 
 
 class DGSEMHIP:
@@ -81,8 +82,10 @@ class DGSEMHIP:
     indicator in FP64 and does the volume/surface arithmetic in FP32;
     "mixed_f32log" is "mixed" with the ln-mean logarithms in FP32 too (built
     for the benchmark's periodic flux_ranocha / flux_hindenlang_gassner
-    configurations).  RealT=np.float32 selects "mixed" (the reference's
-    Float32 mode)."""
+    configurations); "fp32" is the reference's RealT = Float32 mode: float32
+    state vectors and FP32 arithmetic throughout (built for the reference's
+    benchmark configurations, periodic 2D/3D Euler flux differencing and 2D
+    shock capturing).  RealT=np.float32 selects "fp32"."""
 
     def __init__(self, polydeg, surface_flux=flux_central, volume_integral=None, RealT=np.float64,
                  precision=None):
@@ -90,7 +93,7 @@ class DGSEMHIP:
         self.surface_flux = surface_flux
         self.volume_integral = volume_integral if volume_integral is not None else VolumeIntegralWeakForm()
         if precision is None:
-            precision = "mixed" if RealT == np.float32 else "fp64"
+            precision = "fp32" if RealT == np.float32 else "fp64"
         if precision not in PRECISIONS:
             raise ValueError("precision must be one of " + ", ".join(repr(p) for p in PRECISIONS))
         self.precision = precision

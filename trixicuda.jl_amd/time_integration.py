@@ -33,8 +33,8 @@ def rk_step(u, k, t, dt, semi, use_graph=True, stream=None):
     s = stream if stream is not None else torch.cuda.current_stream()
     if use_graph and s.cuda_stream == 0:
         raise L.TrixiHipError(L.EINVAL, "graph replay needs a non-default stream")
-    L.check(L.lib().trixi_hip_rk_step(semi.ctx, _ptr(u), _ptr(k), float(t), float(dt), 1 if use_graph else 0,
-                                      ctypes.c_void_p(s.cuda_stream)))
+    fn = L.lib().trixi_hip_rk_step_f32 if semi.fp32_state else L.lib().trixi_hip_rk_step
+    L.check(fn(semi.ctx, _ptr(u), _ptr(k), float(t), float(dt), 1 if use_graph else 0, ctypes.c_void_p(s.cuda_stream)))
 
 
 def rk_stage(u, k, u_out, stage, t, dt, semi, stream=None):
