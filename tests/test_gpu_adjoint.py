@@ -389,3 +389,39 @@ def test_adjoint_unsupported_is_an_error(th, torch, c):
         th.rhs_vjp(torch.empty_like(u), u, torch.ones_like(u), semi)
     assert ei.value.code == -2
     semi.close()
+
+
+@pytest.mark.parametrize("case", [c for c in TINY if c.ndims > 1 and not c.patches], ids=lambda c: c.name)
+def test_vjp_direction_groups_equal_one_group(th, torch, case):
+    """The direction-group form of the VJP kernel (small meshes: one lane
+    group and LDS accumulator per direction, summed in direction order)
+    against the one-group form (directions one after the other in every
+    lane): the same terms, summed in another order, so they agree to
+    round-off (relative max-norm 1e-13)."""
+    import os
+    out = {}
+    for grouped in (True, False):
+        old = os.environ.get("TRIXI_HIP_VJP_PD_LANES")
+        os.environ["TRIXI_HIP_VJP_PD_LANES"] = str(1 << 40) if grouped else "0"
+        try:
+            semi = product_semi(th, case)
+        finally:
+            if old is None:
+                del os.environ["TRIXI_HIP_VJP_PD_LANES"]
+            else:
+                os.environ["TRIXI_HIP_VJP_PD_LANES"] = old
+        try:
+            u = th.compute_coefficients(0.0, semi)
+            gen = torch.Generator(device="cuda").manual_seed(20261202)
+            vb = torch.randn(u.numel(), dtype=torch.float64, device="cuda", generator=gen)
+            ub = torch.empty_like(u)
+            th.rhs_vjp(ub, u, vb, semi)
+            torch.cuda.synchronize()
+            out[grouped] = ub.cpu().numpy()
+        finally:
+            semi.close()
+    a, b = out[True], out[False]
+    err = float(np.abs(a - b).max() / np.abs(b).max())
+    print(f"{case.name}: grouped vs one-group VJP rel max-norm {err:.3e}", flush=True)
+    assert np.all(np.isfinite(a))
+    assert err <= 1e-13, err

@@ -43,16 +43,20 @@ struct MortarRegistrar {
 #define TRIXI_HIP_VJP_LDS_KIB 64
 #endif
 
-// Workgroup width of the adjoint kernel (FP64): node records with 1/rho and
-// 1/p, FP64 logarithms, the scaled cotangents and the accumulator within
-// TRIXI_HIP_VJP_LDS_KIB.
-template <int DIM, int N, int NV, int NRF>
+// Workgroup width of the adjoint kernel (FP64), a power of two: node records
+// with 1/rho and 1/p, FP64 logarithms, the scaled cotangents and G
+// accumulators (one per direction group) within TRIXI_HIP_VJP_LDS_KIB, or
+// twice that for the direction groups of systems of more than five variables
+// (the nine-variable MHD adjoint: whole waves per direction group, one
+// 100 KiB workgroup per CU; gfx950 gives a workgroup up to 160 KiB of LDS).
+template <int DIM, int N, int NV, int NRF, int G = 1>
 constexpr int pick_m_vjp() {
     constexpr int NL = cpow(N, DIM - 1), NN = cpow(N, DIM);
-    constexpr int per_elem = NN * (NRF * 8 + 2 * 8 + 2 * NV * 8);
-    int m = 256 / NL;
-    if (m < 1) m = 1;
-    while (m > 1 && m * per_elem > TRIXI_HIP_VJP_LDS_KIB * 1024) m /= 2;
+    constexpr int per_elem = NN * (NRF * 8 + 2 * 8 + (1 + G) * NV * 8);
+    constexpr int budget = (G > 1 && NV > 5 ? 2 : 1) * TRIXI_HIP_VJP_LDS_KIB * 1024;
+    int m = 1;
+    while (2 * m * NL * G <= 256) m *= 2;
+    while (m > 1 && m * per_elem > budget) m /= 2;
     return m;
 }
 
@@ -60,8 +64,9 @@ template <int DIM, int N, class EQ, int VOL, int VF, int SF, int FVF = FLUX_CENT
 struct VjpRegistrar {
     static void add(int eq) {
         constexpr int M = pick_m_vjp<DIM, N, EQ::NV, EQ::NRF>();
+        constexpr int MG = pick_m_vjp<DIM, N, EQ::NV, EQ::NRF, DIM>();
         register_vjp({DIM, N, eq, VOL, VF, SF, FVF, 0, 0, EXT ? 1 : 0},
-                     &launch_vjp<DIM, N, EQ, VOL, VF, SF, FVF, double, M, EXT>);
+                     &launch_vjp<DIM, N, EQ, VOL, VF, SF, FVF, double, M, EXT, MG>);
     }
 };
 

@@ -1756,6 +1756,10 @@ static VjpArgs vjp_args(const trixi_hip_ctx* c) {
     v.lhat_m = c->args.lhat_m;
     v.lhat_p = c->args.lhat_p;
     v.nxcd = c->args.nxcd;
+    // the direction-group form below two waves per SIMD of the one-group
+    // form (as the rhs kernel's node-parallel form); 0 turns it off
+    v.pd_lanes = 131072;
+    if (const char* env = getenv("TRIXI_HIP_VJP_PD_LANES")) v.pd_lanes = atoll(env) > 0 ? atoll(env) : 0;
     v.alpha_raw = c->d_alpha_raw;
     v.alphabar = c->d_alphabar;
     v.winner = c->d_winner;

@@ -50,15 +50,23 @@ struct VjpArgs {
     IcParams bc_ic;
     int bc;
     double t;                 // time of the linearisation point
+    int64_t pd_lanes;         // meshes with fewer element lines run the direction-group form
 };
 
-template <int DIM, int N, class EQ, int VOL, int VF, int SF, int FVF, typename R, int M, bool EXT>
-__global__ void __launch_bounds__(M * cpow(N, DIM - 1))
+// G direction groups: G = 1 runs the directions one after the other in every
+// lane; G = DIM gives each direction its own lanes (M * N^(d-1) per group)
+// and its own LDS accumulator, summed in direction order at the end, for
+// meshes too small to fill the GPU with one lane per element line.
+template <int DIM, int N, class EQ, int VOL, int VF, int SF, int FVF, typename R, int M, bool EXT, int G>
+__global__ void __launch_bounds__(M * cpow(N, DIM - 1) * G)
 vjp_kernel(const VjpArgs a) {
     constexpr int NL = cpow(N, DIM - 1);
     constexpr int NN = cpow(N, DIM);
     constexpr int NV = EQ::NV;
     constexpr int NRF = EQ::NRF;
+    constexpr bool PD = G > 1;
+    static_assert(G == 1 || G == DIM, "one group, or one group per direction");
+    constexpr int NT = M * NL * G;  // workgroup size
     using Node = typename EQ::template Node<R>;
     using Bar = typename EQ::template Bar<R>;
     // flux differencing with node records of more than 10 values, and shock
@@ -69,12 +77,15 @@ vjp_kernel(const VjpArgs a) {
     __shared__ R rec[NRF * NN * M];
     __shared__ double recd[2 * NN * M];
     __shared__ R ws[NV * NN * M];   // w = -invJ alpha vbar of the tile
-    __shared__ R acc[NV * NN * M];  // primitive cotangents, summed over directions
-    __shared__ double abar_s[VOL == VI_HG ? NL * M : 1];
+    __shared__ R acc[G * NV * NN * M];  // primitive cotangents, summed over directions (per group)
+    __shared__ double abar_s[VOL == VI_HG ? NT : 1];
 
     const int tid = threadIdx.x;
-    const int el = tid % M;
-    const int line = tid / M;
+    const int grp = tid / (M * NL);  // direction of this lane's group (PD)
+    const int gtid = tid - grp * (M * NL);
+    const int el = gtid % M;
+    const int line = gtid / M;
+    R* const accg = acc + (PD ? grp * NV * NN * M : 0);
     const int64_t E = a.E;
     const int64_t e0 = xcd_tile(a.nxcd) * M;
     if (e0 >= E) return;
@@ -110,7 +121,7 @@ vjp_kernel(const VjpArgs a) {
         alpha = (R)al;
     }
 
-    for (int q = tid; q < NN * M; q += M * NL) {
+    for (int q = tid; q < NN * M; q += NT) {
         const int c = q % M;
         const int n = q / M;
         int64_t ee = e0 + c;
@@ -130,32 +141,36 @@ vjp_kernel(const VjpArgs a) {
         for (int v = 0; v < NV; ++v) ws[(v * NN + n) * M + c] = wsc * (R)a.vbar[(int64_t)(v * NN + n) * E + ee];
         if constexpr (RELOAD)
 #pragma unroll
-            for (int v = 0; v < NV; ++v) acc[(v * NN + n) * M + c] = R(0);
+            for (int g = 0; g < G; ++g)
+#pragma unroll
+                for (int v = 0; v < NV; ++v) acc[((g * NV + v) * NN + n) * M + c] = R(0);
     }
     __syncthreads();
 
     const R lm = (R)a.lhat_m, lp = (R)a.lhat_p;
+    auto Wn = [&](int node, int v) -> R { return ws[(v * NN + node) * M + el]; };
     auto load_node = [&](int node) -> Node {
         R f[NRF];
 #pragma unroll
         for (int k = 0; k < NRF; ++k) f[k] = rec[(k * NN + node) * M + el];
         return EQ::template from_rec<R>(f, recd[node * M + el], recd[(NN + node) * M + el]);
     };
-    // acc (+)= b at a node of this lane's element; the first direction stores
+    // acc (+)= b at a node of this lane's element; the first direction of
+    // the group stores
     auto flush = [&](int d, int node, const Bar& b) {
         R t[NV];
         EQ::bar_to_array(b, t);
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
             const int idx = (v * NN + node) * M + el;
-            acc[idx] = d == 0 ? t[v] : acc[idx] + t[v];
+            accg[idx] = (PD || d == 0) ? t[v] : accg[idx] + t[v];
         }
     };
     auto add_acc = [&](int node, const Bar& b) {
         R t[NV];
         EQ::bar_to_array(b, t);
 #pragma unroll
-        for (int v = 0; v < NV; ++v) acc[(v * NN + node) * M + el] += t[v];
+        for (int v = 0; v < NV; ++v) accg[(v * NN + node) * M + el] += t[v];
     };
     // Interface at the -/+ face of the line: neighbour face node state and
     // cotangent, then the flux VJP keeping only this element's side.
@@ -174,7 +189,7 @@ vjp_kernel(const VjpArgs a) {
             const int64_t r = (int64_t)(v * NN + nnode) * E + fn;
             un[v] = (R)a.u[r];
             // cotangents of this element's and the neighbour's face flux
-            go[v] = (plus ? lp : -lm) * ws[(v * NN + onode) * M + el];
+            go[v] = (plus ? lp : -lm) * Wn(onode, v);
             gn[v] = (plus ? -lm : lp) * wn * (R)a.vbar[r];
             g[v] = go[v] + gn[v];
         }
@@ -218,7 +233,7 @@ vjp_kernel(const VjpArgs a) {
         constexpr int d = decltype(dc)::value;
         const int s = cpow(N, d);
         const int base = line_base<DIM, N>(line, d);
-        auto W = [&](int p, int v) -> R { return ws[(v * NN + base + p * s) * M + el]; };
+        auto W = [&](int p, int v) -> R { return Wn(base + p * s, v); };
 
         if constexpr (RELOAD) {
             // Large node records: the pair's nodes are reloaded from LDS (the
@@ -411,26 +426,36 @@ vjp_kernel(const VjpArgs a) {
             }
             flush(d, base + (N - 1) * s, bb[N - 1]);
         }
-        __syncthreads();
+        // one group: the next direction adds to nodes other lanes stored
+        if constexpr (!PD) __syncthreads();
     };
-    phase(std::integral_constant<int, 0>{});
-    if constexpr (DIM > 1) phase(std::integral_constant<int, 1>{});
-    if constexpr (DIM > 2) phase(std::integral_constant<int, 2>{});
+    // (with groups, each runs its own direction into its own accumulator;
+    // no barrier inside a phase, one after all)
+    if (!PD || grp == 0) phase(std::integral_constant<int, 0>{});
+    if constexpr (DIM > 1) {
+        if (!PD || grp == 1) phase(std::integral_constant<int, 1>{});
+    }
+    if constexpr (DIM > 2) {
+        if (!PD || grp == 2) phase(std::integral_constant<int, 2>{});
+    }
+    if constexpr (PD) __syncthreads();
 
     if constexpr (HG) {
-        // alphabar of each element: sum of its lines' parts in a fixed order
+        // alphabar of each element: sum of its lines' (and groups') parts in
+        // a fixed order
         abar_s[tid] = abar_part;
         __syncthreads();
-        if (line == 0 && e < E) {
+        if (grp == 0 && line == 0 && e < E) {
             double sum = 0.0;
-            for (int l = 0; l < NL; ++l) sum += abar_s[l * M + el];
+            for (int g = 0; g < G; ++g)
+                for (int l = 0; l < NL; ++l) sum += abar_s[g * M * NL + l * M + el];
             a.alphabar[e] = dg_only ? 0.0 : sum;
             a.winner[e] = win;
         }
     }
 
     // (d prim / d cons)^T per node, ubar = beta ubar + result
-    for (int q = tid; q < NN * M; q += M * NL) {
+    for (int q = tid; q < NN * M; q += NT) {
         const int c = q % M;
         const int n = q / M;
         const int64_t ee = e0 + c;
@@ -441,7 +466,11 @@ vjp_kernel(const VjpArgs a) {
         const Node nd = EQ::template from_rec<R>(f, recd[n * M + c], recd[(NN + n) * M + c]);
         R t[NV];
 #pragma unroll
-        for (int v = 0; v < NV; ++v) t[v] = acc[(v * NN + n) * M + c];
+        for (int v = 0; v < NV; ++v) {
+            t[v] = acc[(v * NN + n) * M + c];
+#pragma unroll
+            for (int g = 1; g < G; ++g) t[v] += acc[((g * NV + v) * NN + n) * M + c];
+        }
         Bar tot;
         EQ::array_to_bar(t, tot);
         R ub[NV];
@@ -455,13 +484,26 @@ vjp_kernel(const VjpArgs a) {
     }
 }
 
-template <int DIM, int N, class EQ, int VOL, int VF, int SF, int FVF, typename R, int M, bool EXT = false>
+// M: elements per workgroup of the one-group form; MG: of the
+// direction-group form, used for meshes whose one-group form would launch
+// fewer lanes than a.pd_lanes (on 2D and 3D meshes).
+template <int DIM, int N, class EQ, int VOL, int VF, int SF, int FVF, typename R, int M, bool EXT = false,
+          int MG = M>
 hipError_t launch_vjp(const VjpArgs& a, hipStream_t stream) {
     constexpr int NL = cpow(N, DIM - 1);
+    if constexpr (DIM > 1) {
+        if (a.E * NL < a.pd_lanes) {
+            int64_t blocks = (a.E + MG - 1) / MG;
+            if (a.nxcd > 1) blocks = (blocks + a.nxcd - 1) / a.nxcd * a.nxcd;
+            hipLaunchKernelGGL((vjp_kernel<DIM, N, EQ, VOL, VF, SF, FVF, R, MG, EXT, DIM>), dim3((unsigned)blocks),
+                               dim3(MG * NL * DIM), 0, stream, a);
+            return hipGetLastError();
+        }
+    }
     int64_t blocks = (a.E + M - 1) / M;
     if (a.nxcd > 1) blocks = (blocks + a.nxcd - 1) / a.nxcd * a.nxcd;
-    hipLaunchKernelGGL((vjp_kernel<DIM, N, EQ, VOL, VF, SF, FVF, R, M, EXT>), dim3((unsigned)blocks), dim3(M * NL), 0,
-                       stream, a);
+    hipLaunchKernelGGL((vjp_kernel<DIM, N, EQ, VOL, VF, SF, FVF, R, M, EXT, 1>), dim3((unsigned)blocks), dim3(M * NL),
+                       0, stream, a);
     return hipGetLastError();
 }
 
