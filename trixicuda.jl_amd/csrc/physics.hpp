@@ -23,6 +23,8 @@
 #include <cstdint>
 #include <type_traits>
 
+#include "fastlog.hpp"
+
 namespace trixi_hip {
 
 // b^e; a loop rather than recursion, so that a call with a runtime exponent
@@ -122,6 +124,12 @@ __device__ __forceinline__ float rsqrt_(float x) { return __builtin_amdgcn_sqrtf
 __device__ __forceinline__ double dabs(double x) { return fabs(x); }
 __device__ __forceinline__ float dabs(float x) { return fabsf(x); }
 template <typename R> __device__ __forceinline__ R dmax(R a, R b) { return b > a ? b : a; }
+
+// Logarithms of the node records: fast_log in FP64 (about half the
+// instructions of the library's log, within 2 ulp), the library's FP32 log
+// in FP32.
+__device__ __forceinline__ double node_log(double x) { return fast_log(x); }
+__device__ __forceinline__ float node_log(float x) { return logf(x); }
 
 // ---------------------------------------------------------------------------
 // Logarithmic means (Trixi.jl ln_mean / inv_ln_mean) from per-node
@@ -375,8 +383,8 @@ struct Euler {
         n.c = rsqrt_(P.gamma * n.p * inv_rho);
         n.inv_gm1 = P.inv_gm1;
         if (LOGS) {
-            n.lrho = log((L)n.rho);
-            n.lp = log((L)n.p);
+            n.lrho = node_log((L)n.rho);
+            n.lp = node_log((L)n.p);
         } else {
             n.lrho = n.lp = 0.0;
         }
@@ -595,8 +603,8 @@ struct EulerMulti {
         n.c = rsqrt_(gamma * n.p * inv_rho);
         n.inv_gm1 = rdiv(R(1), gamma - R(1));
         if (LOGS) {
-            n.lrho = log((L)n.rho);
-            n.lp = log((L)n.p);
+            n.lrho = node_log((L)n.rho);
+            n.lp = node_log((L)n.p);
         } else {
             n.lrho = n.lp = 0.0;
         }
@@ -646,7 +654,7 @@ struct EulerMulti {
             const R vo_avg = R(0.5) * (l.v[o] + r.v[o]);
 #pragma unroll
             for (int k = 0; k < NC; ++k) {
-                const R fk = ln_mean_logs<R>(l.rk[k], r.rk[k], log((L)l.rk[k]), log((L)r.rk[k])) * vo_avg;
+                const R fk = ln_mean_logs<R>(l.rk[k], r.rk[k], node_log((L)l.rk[k]), node_log((L)r.rk[k])) * vo_avg;
                 f[DIM + 1 + k] = fk;
                 f_rho_sum += fk;
                 ravg[k] = R(0.5) * (l.rk[k] + r.rk[k]);
@@ -732,8 +740,8 @@ struct EulerQuasi1D {
         n.c = rsqrt_(P.gamma * n.p * rdiv(R(1), n.rho));
         n.inv_gm1 = P.inv_gm1;
         if (LOGS) {
-            n.lrho = log((L)n.rho);
-            n.lp = log((L)n.p);
+            n.lrho = node_log((L)n.rho);
+            n.lp = node_log((L)n.p);
         } else {
             n.lrho = n.lp = 0.0;
         }
@@ -995,8 +1003,8 @@ struct GlmMhd {
                        R(0.5) * (u[5] * u[5] + u[6] * u[6] + u[7] * u[7]) - R(0.5) * u[8] * u[8]);
         n.inv_gm1 = P.inv_gm1;
         if (LOGS) {
-            n.lrho = log((L)n.rho);
-            n.lp = log((L)n.p);
+            n.lrho = node_log((L)n.rho);
+            n.lp = node_log((L)n.p);
         } else {
             n.lrho = n.lp = 0.0;
         }

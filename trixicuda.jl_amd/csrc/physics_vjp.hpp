@@ -159,8 +159,8 @@ struct EulerVjp {
         n.p = P.gm1 * (u[DIM + 1] - R(0.5) * ke);
         n.ip = rrcp(n.p);
         n.c = rsqrt_(P.gamma * n.p * n.irho);
-        n.lrho = log((double)n.rho);
-        n.lp = log((double)n.p);
+        n.lrho = node_log((double)n.rho);
+        n.lp = node_log((double)n.p);
         return n;
     }
     template <typename R>
@@ -456,8 +456,8 @@ struct GlmMhdVjp {
         n.p = P.gm1 * (u[4] - R(0.5) * (u[1] * n.v[0] + u[2] * n.v[1] + u[3] * n.v[2]) -
                        R(0.5) * (u[5] * u[5] + u[6] * u[6] + u[7] * u[7]) - R(0.5) * u[8] * u[8]);
         n.ip = rrcp(n.p);
-        n.lrho = log((double)n.rho);
-        n.lp = log((double)n.p);
+        n.lrho = node_log((double)n.rho);
+        n.lp = node_log((double)n.p);
         return n;
     }
     template <typename R>

@@ -4,6 +4,7 @@
     python trixicuda.jl_amd/profiling.py traffic --match S --fetch DB --write DB \
         --copy-fetch DB --copy-write DB --copy-mib MIB --label L --algorithmic-bytes B [--command CMD]
     python trixicuda.jl_amd/profiling.py flops --match S --f64 DB [--f32 DB] --dofs N --ms T [--label L]
+    python trixicuda.jl_amd/profiling.py valu --match S --db DB --dofs N --ms T [--label L]
 
 ``kernels`` lists per-kernel dispatch counts and durations (the numbers of
 ``rocprofv3 --kernel-trace --stats``) with the launch resources.  ``traffic``
@@ -13,7 +14,10 @@ device copy of known size (the largest ``__amd_rocclr_copyBuffer`` dispatch of
 a ``bench.py --copy-calibration-mib MIB`` run), and prints a
 ``traffic_bytes <label> <bytes>`` line that bench.py reads.  ``flops`` turns
 SQ_INSTS_VALU_{ADD,MUL,FMA,TRANS}_F64 (and _F32) wave-instruction counts into
-executed flop per dispatch: 64 lanes x (ADD + MUL + TRANS + 2 FMA).
+executed flop per dispatch: 64 lanes x (ADD + MUL + TRANS + 2 FMA).  ``valu``
+prints instruction counts (SQ_INSTS_VALU, _CVT, _INT32, SALU, LDS, LDS bank
+conflicts, VMEM) per dispatch and per DOF, and the fraction of the SIMDs'
+VALU issue cycles the kernel's VALU instructions take.
 """
 import argparse
 import collections
@@ -135,6 +139,32 @@ def read_traffic(path):
     return out
 
 
+# MI355X: 256 CUs x 4 SIMDs, 2.4 GHz; a VALU instruction of a 64-lane wave
+# takes 4 cycles on a 16-lane SIMD (FP64, and FP32 without packing)
+SIMDS, CLOCK_HZ, VALU_CYCLES = 1024, 2.4e9, 4
+ISSUE_COUNTERS = ("SQ_INSTS_VALU", "SQ_INSTS_VALU_CVT", "SQ_INSTS_VALU_INT32", "SQ_INSTS_SALU", "SQ_INSTS_LDS",
+                  "SQ_LDS_BANK_CONFLICT", "SQ_INSTS_VMEM_RD", "SQ_INSTS_VMEM_WR")
+
+
+def cmd_valu(a):
+    """Instruction counts per dispatch (wave instructions, summed over the
+    counter instances) and per DOF (lane instructions), and the VALU issue
+    fraction: SQ_INSTS_VALU x 4 cycles / (1024 SIMDs x 2.4 GHz x kernel time)."""
+    name = None
+    print(f"## {a.label}")
+    for ctr in ISSUE_COUNTERS:
+        by = counter_per_dispatch(a.db, ctr)
+        if not any(a.match in n for n in by):
+            continue
+        name, vals = _pick(by, a.match)
+        v = statistics.fmean(vals)
+        print(f"{ctr:24s} {v:14.6g} per dispatch  {v * 64 / a.dofs:8.1f} per DOF")
+        if ctr == "SQ_INSTS_VALU":
+            frac = v * VALU_CYCLES / (SIMDS * CLOCK_HZ * a.ms * 1e-3)
+            print(f"VALU issue at {a.ms * 1e3:.1f} us per launch: {frac:.3f} of the 1024 SIMDs' VALU cycles")
+    print(f"kernel: {name}")
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     sub = ap.add_subparsers(dest="cmd", required=True)
@@ -155,8 +185,14 @@ def main(argv=None):
     f.add_argument("--dofs", type=float, required=True)
     f.add_argument("--ms", type=float, required=True)
     f.add_argument("--label", default="")
+    v = sub.add_parser("valu")
+    v.add_argument("--match", required=True)
+    v.add_argument("--db", required=True)
+    v.add_argument("--dofs", type=float, required=True)
+    v.add_argument("--ms", type=float, required=True)
+    v.add_argument("--label", default="")
     a = ap.parse_args(argv)
-    {"kernels": cmd_kernels, "traffic": cmd_traffic, "flops": cmd_flops}[a.cmd](a)
+    {"kernels": cmd_kernels, "traffic": cmd_traffic, "flops": cmd_flops, "valu": cmd_valu}[a.cmd](a)
 
 
 if __name__ == "__main__":
