@@ -48,14 +48,14 @@ __device__ __forceinline__ double indicator_value(const S* __restrict__ u, int64
     } else if (nv == DIM + 2) {
         double uc[DIM + 2];
 #pragma unroll
-        for (int v = 0; v < DIM + 2; ++v) uc[v] = (double)u[((int64_t)v * NN + n) * E + ec];
+        for (int v = 0; v < DIM + 2; ++v) uc[v] = (double)u[soa(v, n, ec, (int64_t)NN * E, (uint32_t)E)];
         m = Euler<DIM>::indicator_variable(uc, P, var);
     } else {
         // ideal GLM-MHD (nine variables): density_pressure with the magnetic
         // and cleaning energies taken out of the pressure
         double uc[9];
 #pragma unroll
-        for (int v = 0; v < 9; ++v) uc[v] = (double)u[((int64_t)v * NN + n) * E + ec];
+        for (int v = 0; v < 9; ++v) uc[v] = (double)u[soa(v, n, ec, (int64_t)NN * E, (uint32_t)E)];
         const double ke = (uc[1] * uc[1] + uc[2] * uc[2] + uc[3] * uc[3]) / uc[0];
         const double me = uc[5] * uc[5] + uc[6] * uc[6] + uc[7] * uc[7] + uc[8] * uc[8];
         m = uc[0] * P.gm1 * (uc[4] - 0.5 * ke - 0.5 * me);

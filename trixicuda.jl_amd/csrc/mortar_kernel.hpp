@@ -65,6 +65,8 @@ __global__ void __launch_bounds__(256) mortar_kernel(const MortarArgs a) {
     const int64_t m = (int64_t)blockIdx.x * MPB + lm;
     const bool valid = m < a.nmortars;
     const int64_t E = a.E;
+    const uint32_t E32 = (uint32_t)E;
+    const int64_t slab = (int64_t)NN * E;
     const PhysR<double> P = PhysR<double>::from(a.phys);
 
     int od = 0, side = 1;
@@ -81,7 +83,7 @@ __global__ void __launch_bounds__(256) mortar_kernel(const MortarArgs a) {
     if (valid && p == 0) {
         const int node = face_base<DIM, N>(q, od) + pos_large * sod;
 #pragma unroll
-        for (int v = 0; v < NV; ++v) large_face[lm][v][q] = a.u[(int64_t)(v * NN + node) * E + large];
+        for (int v = 0; v < NV; ++v) large_face[lm][v][q] = a.u[soa(v, node, large, slab, E32)];
     }
     __syncthreads();
     if (valid) {
@@ -100,7 +102,7 @@ __global__ void __launch_bounds__(256) mortar_kernel(const MortarArgs a) {
         }
         const int node = face_base<DIM, N>(q, od) + pos_small * sod;
 #pragma unroll
-        for (int v = 0; v < NV; ++v) uS[v] = a.u[(int64_t)(v * NN + node) * E + small];
+        for (int v = 0; v < NV; ++v) uS[v] = a.u[soa(v, node, small, slab, E32)];
         const Node nl = EQ::template node<double, LOGS>(uL, P);
         const Node ns = EQ::template node<double, LOGS>(uS, P);
         double f[NV], nc_small[NV], nc_large[NV];

@@ -98,6 +98,14 @@ struct PhysR {
     }
 };
 
+// Index (v*NN + node)*E + e of an element-innermost state array: the
+// uniform v*NN*E (`slab` = NN*E) stays in scalar registers and node*E + e is
+// one 32x32->64-bit multiply-add per lane.  Exact for node, e >= 0 and
+// E < 2^31 (element ids are int32).
+__device__ __forceinline__ int64_t soa(int v, int node, int64_t e, int64_t slab, uint32_t E32) {
+    return (int64_t)v * slab + (int64_t)((uint64_t)(uint32_t)node * (uint64_t)E32 + (uint64_t)e);
+}
+
 // ---------------------------------------------------------------------------
 // arithmetic helpers.
 //   FP64: reciprocal = v_rcp_f64 refined by two Newton steps, quotient

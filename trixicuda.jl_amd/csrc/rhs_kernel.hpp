@@ -50,6 +50,12 @@ constexpr int MAXN = 8;
 #ifndef TRIXI_HIP_RK_KEEP
 #define TRIXI_HIP_RK_KEEP 1
 #endif
+// 1: the node pass's state loads are issued before the face loads (systems
+// of up to five variables); 0: each (node, element) pair is loaded as the
+// node pass reaches it.
+#ifndef TRIXI_HIP_EARLY_LOADS
+#define TRIXI_HIP_EARLY_LOADS 1
+#endif
 // 1: node records are computed once per node into LDS; 0: every lane
 // computes the node data of its line in each direction phase (no record LDS).
 #ifndef TRIXI_HIP_STAGE_NODES
@@ -251,6 +257,8 @@ rhs_kernel(const RhsArgs a) {
     const int el = gtid % M;
     const int line = gtid / M;
     const int64_t E = a.E;
+    const uint32_t E32 = (uint32_t)E;
+    const int64_t slab = (int64_t)NN * E;
     const int64_t e0 = xcd_tile(a.nxcd) * M;
     if (e0 >= E) return;  // whole workgroup: padding tile of the XCD mapping
     const int64_t e = e0 + el;
@@ -271,6 +279,36 @@ rhs_kernel(const RhsArgs a) {
         nb[f] = a.nbr[(int64_t)f * E + ec];
         nbl[f] = (!EXT || nb[f] >= 0) ? nb[f] : (int32_t)ec;
     }
+    // ---- node pass loads: where they take few registers (at most 24 VGPRs,
+    // systems of up to five variables), every (node, element) pair of this
+    // lane is loaded here, before anything waits on memory, so that a guarded
+    // last pair is not loaded only after the others are computed (indices
+    // past the tile are clamped to a valid pair and not used).  Otherwise
+    // the node pass below loads each pair as it goes.
+    constexpr bool RK_KEEP = COOP && TRIXI_HIP_RK_KEEP != 0;
+    constexpr bool EARLY =
+        TRIXI_HIP_EARLY_LOADS != 0 && STAGE && NV <= 5 && NIT * NV * (int)sizeof(S) <= 96;
+    S ul[EARLY ? NIT : 1][NV];
+    double ukeep[RK_KEEP && !EARLY ? NIT : 1][NV], kkeep[RK_KEEP ? NIT : 1][NV];
+#pragma unroll
+    for (int it = 0; it < (EARLY ? NIT : 0); ++it) {
+        int q = tid + it * NT;
+        if ((it + 1) * NT > NN * M && q >= NN * M) q = NN * M - 1;
+        const int c = q % M;
+        const int n = q / M;
+        int64_t ee = e0 + c;
+        if (ee >= E) ee = E - 1;
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            const int64_t g = soa(v, n, ee, slab, E32);
+            ul[it][v] = U[g];
+            if constexpr (RK_KEEP) kkeep[it][v] = a.rk_first ? 0.0 : (double)K[g];
+        }
+    }
+    // keep these loads ahead of the face loads below: loads complete in issue
+    // order, so the node pass then waits for its own data only
+    if constexpr (EARLY) __builtin_amdgcn_sched_barrier(0);
+
     double tnow = 0.0;
     if constexpr (EXT) tnow = RK ? a.time[0] : a.t_host;
     double um_next[NV], up_next[NV];
@@ -287,8 +325,8 @@ rhs_kernel(const RhsArgs a) {
             }
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
-            um_next[v] = (double)U[(int64_t)(v * NN + base0 + (N - 1) * s0) * E + nm0];
-            up_next[v] = (double)U[(int64_t)(v * NN + base0) * E + np0];
+            um_next[v] = (double)U[soa(v, base0 + (N - 1) * s0, nm0, slab, E32)];
+            up_next[v] = (double)U[soa(v, base0, np0, slab, E32)];
         }
     }
 
@@ -296,25 +334,28 @@ rhs_kernel(const RhsArgs a) {
     // Every lane visits NIT (node, element) pairs: NN * M / NT, rounded up.
     // RK_KEEP: the node pass keeps u and k in registers for the update;
     // otherwise the update reads them again with the same coalesced pattern
-    constexpr bool RK_KEEP = COOP && TRIXI_HIP_RK_KEEP != 0;
-    double ukeep[RK_KEEP ? NIT : 1][NV], kkeep[RK_KEEP ? NIT : 1][NV];
 #pragma unroll
     for (int it = 0; it < (STAGE ? NIT : 0); ++it) {
         const int q = tid + it * NT;
-        if ((NN * M) % NT != 0 && q >= NN * M) continue;
+        if ((it + 1) * NT > NN * M && q >= NN * M) continue;
         const int c = q % M;
         const int n = q / M;
-        int64_t ee = e0 + c;
-        if (ee >= E) ee = E - 1;
         R uc[NV];
+        if constexpr (EARLY) {
 #pragma unroll
-        for (int v = 0; v < NV; ++v) {
-            const int64_t g = (int64_t)(v * NN + n) * E + ee;
-            const double ud = (double)U[g];
-            uc[v] = (R)ud;
-            if constexpr (RK_KEEP) {
-                ukeep[it][v] = ud;
-                kkeep[it][v] = a.rk_first ? 0.0 : (double)K[g];
+            for (int v = 0; v < NV; ++v) uc[v] = (R)(double)ul[it][v];
+        } else {
+            int64_t ee = e0 + c;
+            if (ee >= E) ee = E - 1;
+#pragma unroll
+            for (int v = 0; v < NV; ++v) {
+                const int64_t g = soa(v, n, ee, slab, E32);
+                const double ud = (double)U[g];
+                uc[v] = (R)ud;
+                if constexpr (RK_KEEP) {
+                    ukeep[it][v] = ud;
+                    kkeep[it][v] = a.rk_first ? 0.0 : (double)K[g];
+                }
             }
         }
         const Node nd = EQ::template node<R, NODE_LOGS, L>(uc, P);
@@ -372,7 +413,7 @@ rhs_kernel(const RhsArgs a) {
             for (int v = 0; v < NV; ++v) {
                 R total = al[p][v];
                 if (DIM > 1) total = acc[(v * NN + node) * LDM + el] + total;
-                const int64_t g = (int64_t)(v * NN + node) * E + e;
+                const int64_t g = soa(v, node, e, slab, E32);
                 const double val = (double)total * fac + sv[v];
                 if (!RK) {
                     DU[g] = (S)val;
@@ -404,11 +445,11 @@ rhs_kernel(const RhsArgs a) {
             if (nb[2 * d] == -1) {
                 if ((a.outflow >> (2 * d)) & 1) {
 #pragma unroll
-                    for (int v = 0; v < NV; ++v) um[v] = (R)U[(int64_t)(v * NN + base) * E + ec];
+                    for (int v = 0; v < NV; ++v) um[v] = (R)U[soa(v, base, ec, slab, E32)];
                 } else if (a.bc == TRIXI_HIP_BC_SLIP_WALL) {
                     // slip wall: the inner state with its normal momentum reversed
 #pragma unroll
-                    for (int v = 0; v < NV; ++v) um[v] = (R)U[(int64_t)(v * NN + base) * E + ec];
+                    for (int v = 0; v < NV; ++v) um[v] = (R)U[soa(v, base, ec, slab, E32)];
                     um[1 + d] = -um[1 + d];
                 } else {
                     boundary_state<DIM, NN, NV, R>(a, ec, base, tnow, um);
@@ -417,10 +458,10 @@ rhs_kernel(const RhsArgs a) {
             if (nb[2 * d + 1] == -1) {
                 if ((a.outflow >> (2 * d + 1)) & 1) {
 #pragma unroll
-                    for (int v = 0; v < NV; ++v) up[v] = (R)U[(int64_t)(v * NN + base + (N - 1) * s) * E + ec];
+                    for (int v = 0; v < NV; ++v) up[v] = (R)U[soa(v, base + (N - 1) * s, ec, slab, E32)];
                 } else if (a.bc == TRIXI_HIP_BC_SLIP_WALL) {
 #pragma unroll
-                    for (int v = 0; v < NV; ++v) up[v] = (R)U[(int64_t)(v * NN + base + (N - 1) * s) * E + ec];
+                    for (int v = 0; v < NV; ++v) up[v] = (R)U[soa(v, base + (N - 1) * s, ec, slab, E32)];
                     up[1 + d] = -up[1 + d];
                 } else {
                     boundary_state<DIM, NN, NV, R>(a, ec, base + (N - 1) * s, tnow, up);
@@ -432,8 +473,8 @@ rhs_kernel(const RhsArgs a) {
             const int bn = line_base<DIM, N>(line, d + 1);
 #pragma unroll
             for (int v = 0; v < NV; ++v) {
-                um_next[v] = (double)U[(int64_t)(v * NN + bn + (N - 1) * sn) * E + nbl[2 * (d + 1)]];
-                up_next[v] = (double)U[(int64_t)(v * NN + bn) * E + nbl[2 * (d + 1) + 1]];
+                um_next[v] = (double)U[soa(v, bn + (N - 1) * sn, nbl[2 * (d + 1)], slab, E32)];
+                up_next[v] = (double)U[soa(v, bn, nbl[2 * (d + 1) + 1], slab, E32)];
             }
         }
 
@@ -451,7 +492,7 @@ rhs_kernel(const RhsArgs a) {
             } else {
                 R uc[NV];
 #pragma unroll
-                for (int v = 0; v < NV; ++v) uc[v] = (R)U[(int64_t)(v * NN + node) * E + ec];
+                for (int v = 0; v < NV; ++v) uc[v] = (R)U[soa(v, node, ec, slab, E32)];
                 nd[p] = EQ::template node<R, NODE_LOGS, L>(uc, P);
             }
         }
@@ -669,10 +710,10 @@ rhs_kernel(const RhsArgs a) {
             if (EXT && a.src) source_at<DIM, NN, NV>(a, ee, n, tnow, sv);
 #pragma unroll
             for (int v = 0; v < NV; ++v) {
-                const int64_t g = (int64_t)(v * NN + n) * E + ee;
+                const int64_t g = soa(v, n, ee, slab, E32);
                 const double val = (double)acc[(v * NN + n) * LDM + c] * fac + sv[v];
                 const double kold = RK_KEEP ? kkeep[it][v] : (a.rk_first ? 0.0 : (double)K[g]);
-                const double uold = RK_KEEP ? ukeep[it][v] : (double)U[g];
+                const double uold = RK_KEEP ? (EARLY ? (double)ul[it][v] : ukeep[it][v]) : (double)U[g];
                 const S kk = (S)(a.rk_first ? t_dt * val : a.rk_a * kold + t_dt * val);
                 K[g] = kk;
                 UO[g] = (S)(uold + a.rk_b * (double)kk);
@@ -748,6 +789,8 @@ rhs_np_kernel(const RhsArgs a) {
     const int node = (tid / M) % NN;
     const int d = tid / (M * NN);
     const int64_t E = a.E;
+    const uint32_t E32 = (uint32_t)E;
+    const int64_t slab = (int64_t)NN * E;
     const int64_t e0 = xcd_tile(a.nxcd) * M;
     if (e0 >= E) return;  // whole workgroup: padding tile of the XCD mapping
     const int64_t e = e0 + el;
@@ -767,13 +810,13 @@ rhs_np_kernel(const RhsArgs a) {
         const int32_t nbe = a.nbr[(int64_t)(2 * d + (p == 0 ? 0 : 1)) * E + ec];
         const int nn_face = p == 0 ? node + (N - 1) * s : node - (N - 1) * s;
 #pragma unroll
-        for (int v = 0; v < NV; ++v) ub[v] = (R)U[(int64_t)(v * NN + nn_face) * E + nbe];
+        for (int v = 0; v < NV; ++v) ub[v] = (R)U[soa(v, nn_face, nbe, slab, E32)];
     }
 
     if (last) {
         R uc[NV];
 #pragma unroll
-        for (int v = 0; v < NV; ++v) uc[v] = (R)U[(int64_t)(v * NN + node) * E + ec];
+        for (int v = 0; v < NV; ++v) uc[v] = (R)U[soa(v, node, ec, slab, E32)];
         const Node nd = EQ::template node<R, NODE_LOGS, L>(uc, P);
         R f[NRF];
         EQ::to_rec(nd, f);
@@ -949,7 +992,7 @@ rhs_np_kernel(const RhsArgs a) {
             for (int dd = 1; dd < DIM - 1; ++dd) sum = sum + acc[((dd * NV + v) * NN + node) * M + el];
             total = sum + acl[v];
         }
-        const int64_t g = (int64_t)(v * NN + node) * E + e;
+        const int64_t g = soa(v, node, e, slab, E32);
         const double val = (double)total * fac;
         if constexpr (!RK) {
             DU[g] = (S)val;

@@ -87,6 +87,8 @@ vjp_kernel(const VjpArgs a) {
     const int line = gtid / M;
     R* const accg = acc + (PD ? grp * NV * NN * M : 0);
     const int64_t E = a.E;
+    const uint32_t E32 = (uint32_t)E;
+    const int64_t slab = (int64_t)NN * E;
     const int64_t e0 = xcd_tile(a.nxcd) * M;
     if (e0 >= E) return;
     const int64_t e = e0 + el;
@@ -128,7 +130,7 @@ vjp_kernel(const VjpArgs a) {
         if (ee >= E) ee = E - 1;
         R uc[NV];
 #pragma unroll
-        for (int v = 0; v < NV; ++v) uc[v] = (R)a.u[(int64_t)(v * NN + n) * E + ee];
+        for (int v = 0; v < NV; ++v) uc[v] = (R)a.u[soa(v, n, ee, slab, E32)];
         const Node nd = EQ::node(uc, P);
         R f[NRF];
         EQ::to_rec(nd, f);
@@ -138,7 +140,7 @@ vjp_kernel(const VjpArgs a) {
         recd[(NN + n) * M + c] = nd.lp;
         const R wsc = (R)(-a.invJ[ee] * a.alpha);
 #pragma unroll
-        for (int v = 0; v < NV; ++v) ws[(v * NN + n) * M + c] = wsc * (R)a.vbar[(int64_t)(v * NN + n) * E + ee];
+        for (int v = 0; v < NV; ++v) ws[(v * NN + n) * M + c] = wsc * (R)a.vbar[soa(v, n, ee, slab, E32)];
         if constexpr (RELOAD)
 #pragma unroll
             for (int g = 0; g < G; ++g)
@@ -186,7 +188,7 @@ vjp_kernel(const VjpArgs a) {
         R un[NV], g[NV], go[NV], gn[NV];
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
-            const int64_t r = (int64_t)(v * NN + nnode) * E + fn;
+            const int64_t r = soa(v, nnode, fn, slab, E32);
             un[v] = (R)a.u[r];
             // cotangents of this element's and the neighbour's face flux
             go[v] = (plus ? lp : -lm) * Wn(onode, v);
@@ -477,7 +479,7 @@ vjp_kernel(const VjpArgs a) {
         EQ::to_cons_bar(nd, tot, ub, P);
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
-            const int64_t g = (int64_t)(v * NN + n) * E + ee;
+            const int64_t g = soa(v, n, ee, slab, E32);
             const double prev = a.beta != 0.0 ? a.beta * a.ubar[g] : 0.0;
             a.ubar[g] = prev + (double)ub[v];
         }
