@@ -143,7 +143,9 @@ __device__ __forceinline__ float node_log(float x) { return logf(x); }
 // Logarithmic means (Trixi.jl ln_mean / inv_ln_mean) from per-node
 // logarithms, branch free with a single reciprocal each.
 //   f2 = ((y - x) / (x + y))^2; the series branch is taken for f2 < 1e-4,
-//   tested as (y-x)^2 < 1e-4 (x+y)^2 (x, y > 0).
+//   tested as |y - x| < 0.01 (x + y) (x, y > 0; one product, the absolute
+//   value is a source modifier; both branches are accurate to round-off on
+//   either side of the threshold, so its exact placement does not matter).
 //   series:  ln_mean     = (x + y) Q(f2),  Q = 1/P expanded to the order of
 //                          P(f2) = 2 + 2f2/3 + 2f2^2/5 + 2f2^3/7:
 //                          Q = 1/2 - f2/6 - 2 f2^2/45 - 22 f2^3/945
@@ -159,7 +161,7 @@ __device__ __forceinline__ float node_log(float x) { return logf(x); }
 template <typename R, typename L>
 __device__ __forceinline__ R ln_mean_logs(R x, R y, L lx, L ly) {
     const R dxy = y - x, s = x + y;
-    const bool series = dxy * dxy < R(1.0e-4) * (s * s);
+    const bool series = dabs(dxy) < R(0.01) * s;
     const R r = rrcp(series ? s : (R)(ly - lx));
     const R z = dxy * r, f2 = z * z;
     const R q = R(0.5) + f2 * (R(-1.0 / 6.0) + f2 * (R(-2.0 / 45.0) + f2 * R(-22.0 / 945.0)));
@@ -169,7 +171,7 @@ __device__ __forceinline__ R ln_mean_logs(R x, R y, L lx, L ly) {
 template <typename R, typename L>
 __device__ __forceinline__ R inv_ln_mean_logs(R x, R y, L lx, L ly) {
     const R dxy = y - x, s = x + y;
-    const bool series = dxy * dxy < R(1.0e-4) * (s * s);
+    const bool series = dabs(dxy) < R(0.01) * s;
     const R dl = (R)(ly - lx);
     const R r = rrcp(series ? s : dxy);
     const R z = dxy * r, f2 = z * z;

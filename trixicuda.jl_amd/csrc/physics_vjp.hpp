@@ -24,7 +24,7 @@ namespace trixi_hip {
 template <typename R>
 __device__ __forceinline__ void ln_mean_grad(R x, R y, double lx, double ly, R ix, R iy, R& L, R& Lx, R& Ly) {
     const R dxy = y - x, s = x + y;
-    const bool series = dxy * dxy < R(1.0e-4) * (s * s);
+    const bool series = dabs(dxy) < R(0.01) * s;  // the forward mean's branch test
     if (series) {
         const R r = rrcp(s);
         const R z = dxy * r, f2 = z * z;
@@ -48,7 +48,7 @@ template <typename R>
 __device__ __forceinline__ void inv_ln_mean_grad(R x, R y, double lx, double ly, R ix, R iy, R& I, R& Ix,
                                                  R& Iy) {
     const R dxy = y - x, s = x + y;
-    const bool series = dxy * dxy < R(1.0e-4) * (s * s);
+    const bool series = dabs(dxy) < R(0.01) * s;  // the forward mean's branch test
     if (series) {
         const R r = rrcp(s);
         const R z = dxy * r, f2 = z * z;
