@@ -17,7 +17,9 @@ SQ_INSTS_VALU_{ADD,MUL,FMA,TRANS}_F64 (and _F32) wave-instruction counts into
 executed flop per dispatch: 64 lanes x (ADD + MUL + TRANS + 2 FMA).  ``valu``
 prints instruction counts (SQ_INSTS_VALU, _CVT, _INT32, SALU, LDS, LDS bank
 conflicts, VMEM) per dispatch and per DOF, and the fraction of the SIMDs'
-VALU issue cycles the kernel's VALU instructions take.
+VALU issue cycles the kernel's VALU instructions take; with ``--waits``, the
+wave-cycle breakdown (waiting at s_waitcnt or a barrier, waiting to issue,
+issuing VALU / LDS) of a second run's SQ_WAVE_CYCLES group.
 """
 import argparse
 import collections
@@ -146,6 +148,10 @@ ISSUE_COUNTERS = ("SQ_INSTS_VALU", "SQ_INSTS_VALU_CVT", "SQ_INSTS_VALU_INT32", "
                   "SQ_LDS_BANK_CONFLICT", "SQ_INSTS_VMEM_RD", "SQ_INSTS_VMEM_WR")
 
 
+WAIT_COUNTERS = ("SQ_WAVE_CYCLES", "SQ_WAIT_ANY", "SQ_WAIT_INST_ANY", "SQ_WAIT_INST_LDS", "SQ_ACTIVE_INST_VALU",
+                 "SQ_ACTIVE_INST_LDS", "SQ_ACTIVE_INST_VMEM")
+
+
 def cmd_valu(a):
     """Instruction counts per dispatch (wave instructions, summed over the
     counter instances) and per DOF (lane instructions), and the VALU issue
@@ -163,6 +169,16 @@ def cmd_valu(a):
             frac = v * VALU_CYCLES / (SIMDS * CLOCK_HZ * a.ms * 1e-3)
             print(f"VALU issue at {a.ms * 1e3:.1f} us per launch: {frac:.3f} of the 1024 SIMDs' VALU cycles")
     print(f"kernel: {name}")
+    if a.waits:
+        vals = {}
+        for ctr in WAIT_COUNTERS:
+            by = counter_per_dispatch(a.waits, ctr)
+            wname, vs = _pick(by, a.match)
+            vals[ctr] = statistics.fmean(vs)
+        wc = vals["SQ_WAVE_CYCLES"]
+        print("wave-cycle breakdown (fractions of SQ_WAVE_CYCLES):")
+        for ctr in WAIT_COUNTERS[1:]:
+            print(f"  {ctr:22s} {vals[ctr] / wc:6.3f}")
 
 
 def main(argv=None):
@@ -190,6 +206,7 @@ def main(argv=None):
     v.add_argument("--db", required=True)
     v.add_argument("--dofs", type=float, required=True)
     v.add_argument("--ms", type=float, required=True)
+    v.add_argument("--waits", default="")
     v.add_argument("--label", default="")
     a = ap.parse_args(argv)
     {"kernels": cmd_kernels, "traffic": cmd_traffic, "flops": cmd_flops, "valu": cmd_valu}[a.cmd](a)
