@@ -10,7 +10,9 @@ state.  A step is one rhs! evaluation (u -> du) with u resident in HBM.
 `value` is DOF-updates/s of the mixed-precision rhs! (FP32 arithmetic, FP64
 state / log-means); the FP64 rhs! and the FP32-state rhs! (the reference's
 RealT = Float32 mode: FP32 state, du and logarithms) are reported beside it,
-each with its own roofline.  The CPU baseline is the FP64 oracle restatement (OpenMP) on a
+each with its own roofline; each mode also times one CarpenterKennedy2N54
+step (five fused rhs! + register-update stages, hipGraph) with its HBM
+roofline.  The CPU baseline is the FP64 oracle restatement (OpenMP) on a
 bounded sample of the same workload and state.
 """
 import argparse
@@ -242,6 +244,18 @@ def time_rk(torch, th, semi, u, steps, stream):
     return start.elapsed_time(stop) / steps
 
 
+def trajectory(torch, th, semi, u0, dt, steps, stream):
+    """`steps` CK2N54 steps of size dt from u0 (hipGraph per step); the
+    final state in FP64."""
+    k = torch.zeros_like(u0)
+    uu = u0.clone()
+    with torch.cuda.stream(stream):
+        for i in range(steps):
+            th.rk_step(uu, k, i * dt, dt, semi, use_graph=True, stream=stream)
+        torch.cuda.synchronize()
+    return uu.double()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -255,6 +269,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--rk-steps", type=int, default=20)
+    ap.add_argument("--traj-steps", type=int, default=100,
+                    help="RK steps of the precision-comparison trajectory (0: none)")
     ap.add_argument("--vjp-steps", type=int, default=50)
     ap.add_argument("--no-extra", action="store_true", help="skip the C2/C4/C5 measurements")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
@@ -293,6 +309,7 @@ def main():
     stream = torch.cuda.Stream()
     modes = [args.only] if args.only else ["fp64", "mixed", "fp32"]
     res, du_out = {}, {}
+    traj, traj_dt = {}, None
     u = None
     semi_keep = None
     for mode in modes:
@@ -327,7 +344,23 @@ def main():
                          "flop_frac": achieved_tf / fpeak},
         }
         if args.rk_steps > 0:
-            res[mode]["rk_step_ms"] = time_rk(torch, th, semi, um, args.rk_steps, stream)
+            rk_ms = time_rk(torch, th, semi, um, args.rk_steps, stream)
+            res[mode]["rk_step_ms"] = rk_ms
+            # One CarpenterKennedy2N54 step: five fused stages; the first reads u
+            # and writes k and the stage output, the other four also read k.
+            vb = 5 * (4 if semi.fp32_state else 8)  # bytes of one node's state
+            rk_bytes = (3 + 4 * 4) * vb * dofs
+            rk_gbps = rk_bytes / (rk_ms * 1e-3) / 1e9
+            res[mode]["rk_step_roofline"] = {"bound": "hbm", "achieved": rk_gbps, "peak": fm.PEAK["hbm_GBps"],
+                                             "unit": "GB/s", "frac": rk_gbps / fm.PEAK["hbm_GBps"],
+                                             "algorithmic_bytes": rk_bytes, "bytes_per_dof": rk_bytes // dofs}
+        if not args.only and args.traj_steps > 0:
+            # every mode advances the same initial state with the same steps
+            # (CFL 1 from the FP64 state's max_dt), for the solution-level
+            # difference between the precisions
+            if traj_dt is None:
+                traj_dt = th.max_dt(u, semi)
+            traj[mode] = trajectory(torch, th, semi, um, traj_dt, args.traj_steps, stream)
         if mode == "fp64" and args.vjp_steps > 0:
             vms = time_vjp(torch, th, semi, u, args.vjp_steps, 3, stream)
             res[mode]["vjp_ms"] = vms
@@ -349,6 +382,21 @@ def main():
             a, b = du_out[m], du_out["fp64"]
             errors[f"{m}_vs_fp64_rel_max"] = float((a - b).abs().max() / b.abs().max())
             errors[f"{m}_vs_fp64_rel_l2"] = float(torch.linalg.norm(a - b) / torch.linalg.norm(b))
+    trajectory_errors = None
+    if "fp64" in traj:
+        b = traj["fp64"]
+        trajectory_errors = {"steps": args.traj_steps, "dt": traj_dt, "t_end": traj_dt * args.traj_steps,
+                             "cfl": 1.0}
+        for m in ("mixed", "fp32"):
+            if m in traj:
+                a = traj[m]
+                trajectory_errors[f"{m}_vs_fp64_rel_max"] = float((a - b).abs().max() / b.abs().max())
+                trajectory_errors[f"{m}_vs_fp64_rel_l2"] = float(torch.linalg.norm(a - b) / torch.linalg.norm(b))
+                # relative to the state's change over the trajectory
+                db = b - u
+                trajectory_errors[f"{m}_vs_fp64_increment_rel_l2"] = float(torch.linalg.norm(a - b)
+                                                                           / torch.linalg.norm(db))
+                trajectory_errors[f"{m}_vs_fp64_increment_rel_max"] = float((a - b).abs().max() / db.abs().max())
 
     if semi_keep is None:
         semi_keep = make_semi(th, args.level, "fp64")
@@ -393,6 +441,7 @@ def main():
         "cpu_baseline": cpu,
         "modes": res,
         "errors": errors,
+        "trajectory_errors": trajectory_errors,
         "other_configs": extra,
     }
     for m in ("mixed", "fp32"):
