@@ -374,7 +374,9 @@ int trixi_hip_rhs_vjp(trixi_hip_ctx* ctx, double* ubar, const double* u, const d
                       double t, double alpha, double beta, trixi_hip_stream stream);
 
 /* Allocate the adjoint's work buffers (7 state vectors).  Call once before
- * trixi_hip_rk_step_vjp; this is the only allocating call after create. */
+ * trixi_hip_rk_step_vjp.  Besides this call, only the first
+ * trixi_hip_rhs_vjp of a shock-capturing context allocates after create
+ * (two per-element arrays). */
 int trixi_hip_enable_adjoint(trixi_hip_ctx* ctx);
 
 /* Reverse of one CarpenterKennedy2N54 step from the step's initial state
