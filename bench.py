@@ -421,7 +421,7 @@ def main():
         dofs = mesh.nelements * 64
         cpu = {"value": dofs * n / el, "unit": "DOF-updates/s", "cores": threads, "nproc": threads,
                "os_cpu_count": os.cpu_count(), "kind": "port",
-               "sample": f"{n} x rhs! of the full workload state ({dofs} DOF), FP64 C++ oracle, OpenMP"}
+               "sample": f"{n} x rhs! of the full workload state ({dofs} DOF), FP64 C++ oracle (C++ restatement, not Trixi), OpenMP"}
         for mode, d in du_out.items():
             g = th.wrap_array_native(d, semi_keep).cpu().numpy()
             errors[f"{mode}_vs_oracle_rel_max"] = float(np.abs(g - du_ref).max() / np.abs(du_ref).max())
