@@ -24,6 +24,7 @@
 
 #include <type_traits>
 
+#include "indicator.hpp"
 #include "initial_conditions.hpp"
 #include "physics.hpp"
 
@@ -108,6 +109,14 @@ struct RhsArgs {
     int mK;
     // node-parallel form (small periodic conforming meshes, chosen on the host)
     int np;
+    // fused RK stage of the node-parallel form with shock capturing
+    // (compressible Euler): the raw blending factors of u_out, the next
+    // stage's input, go to alpha_next (nullptr: not computed), with the
+    // indicator's inverse Vandermonde and parameters
+    double* alpha_next;
+    const double* Vinv;
+    double ind_threshold, ind_s, ind_alpha_min, ind_alpha_max;
+    int ind_var;
 };
 
 // Dirichlet state boundary_ic(x, t) at node `node` of element e (its face
@@ -783,6 +792,10 @@ rhs_np_kernel(const RhsArgs a) {
     __shared__ R rec[NRF * NN * M];
     __shared__ L recd[NDF > 0 ? NDF * NN * M : 1];
     __shared__ R acc[DIM > 1 ? (DIM - 1) * NV * NN * M : 1];
+    // fused RK stage with shock capturing: the next stage's raw blending
+    // factors from u_out (indicator variable, modal coefficients, energies)
+    constexpr bool FUSE_IND = RK && VOL == VI_HG && std::is_same<EQ, Euler<DIM>>::value;
+    __shared__ double ind_buf[FUSE_IND ? 2 * M * NN : 1];
 
     const int tid = threadIdx.x;
     const int el = tid % M;
@@ -979,28 +992,66 @@ rhs_np_kernel(const RhsArgs a) {
         for (int v = 0; v < NV; ++v) acc[((d * NV + v) * NN + node) * M + el] = acl[v];
     }
     if (DIM > 1) __syncthreads();
-    if (!last || !valid) return;
-    const double fac = -a.invJ[e];
-    double t_dt = 0.0;
-    if (RK) t_dt = a.time[1];
+    // (no early exit: the fused indicator below synchronises the workgroup)
+    double uo[FUSE_IND ? NV : 1];
+    if (last && valid) {
+        const double fac = -a.invJ[e];
+        double t_dt = 0.0;
+        if (RK) t_dt = a.time[1];
 #pragma unroll
-    for (int v = 0; v < NV; ++v) {
-        R total = acl[v];
-        if constexpr (DIM > 1) {
-            R sum = acc[((0 * NV + v) * NN + node) * M + el];
+        for (int v = 0; v < NV; ++v) {
+            R total = acl[v];
+            if constexpr (DIM > 1) {
+                R sum = acc[((0 * NV + v) * NN + node) * M + el];
 #pragma unroll
-            for (int dd = 1; dd < DIM - 1; ++dd) sum = sum + acc[((dd * NV + v) * NN + node) * M + el];
-            total = sum + acl[v];
+                for (int dd = 1; dd < DIM - 1; ++dd) sum = sum + acc[((dd * NV + v) * NN + node) * M + el];
+                total = sum + acl[v];
+            }
+            const int64_t g = soa(v, node, e, slab, E32);
+            const double val = (double)total * fac;
+            if constexpr (!RK) {
+                DU[g] = (S)val;
+            } else {
+                // u and k again (cache resident on the small meshes this form runs)
+                const S kk = (S)(a.rk_first ? t_dt * val : a.rk_a * (double)K[g] + t_dt * val);
+                K[g] = kk;
+                const S un = (S)((double)U[g] + a.rk_b * (double)kk);
+                UO[g] = un;
+                if constexpr (FUSE_IND) uo[v] = (double)un;
+            }
         }
-        const int64_t g = soa(v, node, e, slab, E32);
-        const double val = (double)total * fac;
-        if constexpr (!RK) {
-            DU[g] = (S)val;
-        } else {
-            // u and k again (cache resident on the small meshes this form runs)
-            const S kk = (S)(a.rk_first ? t_dt * val : a.rk_a * (double)K[g] + t_dt * val);
-            K[g] = kk;
-            UO[g] = (S)((double)U[g] + a.rk_b * (double)kk);
+    }
+    if constexpr (FUSE_IND) {
+        if (a.alpha_next == nullptr) return;  // uniform: the whole workgroup
+        // the indicator kernel's steps on the stored (rounded) u_out, element
+        // slot el of the node-pass lanes: the same values in the same order,
+        // so the factors equal a separate indicator launch's bit for bit
+        double* const buf = ind_buf;
+        double* const e2 = ind_buf + M * NN;
+        const PhysR<double> P64 = PhysR<double>::from(a.phys);
+        double m = 0.0;
+        if (last && valid) m = Euler<DIM>::indicator_variable(uo, P64, a.ind_var);
+#pragma unroll
+        for (int dd = 0; dd < DIM; ++dd) {
+            const int s2 = cpow(N, dd);
+            const int idx = (node / s2) % N, b2 = node - idx * s2;
+            if (last) buf[el * NN + node] = m;
+            __syncthreads();
+            if (last) {
+                double r = 0.0;
+#pragma unroll
+                for (int ii = 0; ii < N; ++ii) r += a.Vinv[idx * N + ii] * buf[el * NN + b2 + ii * s2];
+                m = r;
+            }
+            __syncthreads();
+        }
+        if (last) e2[el * NN + node] = m * m;
+        __syncthreads();
+        if (last && node == 0 && valid) {
+            double total, clip1, clip2;
+            modal_energies<DIM, N>(e2, el, total, clip1, clip2);
+            a.alpha_next[e] = hg_alpha(total, clip1, clip2, a.ind_threshold, a.ind_s, a.ind_alpha_min,
+                                       a.ind_alpha_max);
         }
     }
 }

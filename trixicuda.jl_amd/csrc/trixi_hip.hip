@@ -822,6 +822,7 @@ struct trixi_hip_ctx {
     double* d_k = nullptr;        // forward register during recomputation
     double* d_kbar = nullptr;     // adjoint register
     double* d_alphabar = nullptr; // HG: cotangent of the smoothed blending factor
+    double* d_stage_alpha = nullptr;  // fused HG adjoint: raw blending factors of the 5 stage inputs
     int32_t* d_winner = nullptr;  // HG: argument of the smoothing max
     bool adjoint = false;
     // L2 mortars
@@ -841,6 +842,9 @@ struct trixi_hip_ctx {
     void* graph_k = nullptr;
     hipStream_t graph_stream = nullptr;
     bool has_graph = false;
+    // the fused RK stages compute the next stage's raw blending factors
+    // (node-parallel form, compressible Euler shock capturing)
+    bool fuse_ind = false;
 };
 
 static const double CK_A[5] = {0.0, -567301805773.0 / 1357537059087.0, -2404267990393.0 / 2016746695238.0,
@@ -884,7 +888,7 @@ static void free_ctx(trixi_hip_ctx* c) {
     if (c->graph) (void)hipGraphDestroy(c->graph);
     void* bufs[] = {c->d_nbr, c->d_invJ, c->d_x,     c->d_alpha_raw, c->d_alpha, c->d_time,
                     c->d_partial, c->d_result, c->d_Vinv, c->d_Vdm,   c->d_wa,    c->d_w,
-                    c->d_tmp, c->d_stage, c->d_k, c->d_kbar, c->d_alphabar, c->d_winner,
+                    c->d_tmp, c->d_stage, c->d_k, c->d_kbar, c->d_alphabar, c->d_winner, c->d_stage_alpha,
                     c->d_mids, c->d_mori, c->d_mls, c->d_mnb, c->d_mflux};
     for (void* p : bufs)
         if (p) (void)hipFree(p);
@@ -894,7 +898,9 @@ static void free_ctx(trixi_hip_ctx* c) {
 static bool fp32_state(const trixi_hip_ctx* c) { return c->cfg.precision == TRIXI_HIP_FP32; }
 
 // Raw blending factors of state u (FP64, or FP32 when f32).
-static int launch_indicator(trixi_hip_ctx* c, const void* u, hipStream_t s, bool f32 = false) {
+static int launch_indicator(trixi_hip_ctx* c, const void* u, hipStream_t s, bool f32 = false,
+                            double* out = nullptr) {
+    double* const araw = out ? out : c->d_alpha_raw;
     const double threshold = 0.5 * pow(10.0, -1.8 * pow((double)c->N, 0.25));
     const double parameter_s = log((1.0 - 0.0001) / 0.0001);
     const PhysParams& P = c->args.phys;
@@ -903,7 +909,7 @@ static int launch_indicator(trixi_hip_ctx* c, const void* u, hipStream_t s, bool
 #define TRIXI_HIP_IND_S(D, NQ, ST)                                                                             \
     hipLaunchKernelGGL((indicator_kernel<D, NQ, ST>),                                                         \
                        dim3((unsigned)((c->E + ind_epb<D, NQ>() - 1) / ind_epb<D, NQ>())),                    \
-                       dim3(ind_block<D, NQ>()), 0, s, static_cast<const ST*>(u), c->d_alpha_raw, c->E, P,   \
+                       dim3(ind_block<D, NQ>()), 0, s, static_cast<const ST*>(u), araw, c->E, P,             \
                        c->d_Vinv, threshold, parameter_s, c->cfg.alpha_min, c->cfg.alpha_max, var, c->nv)
 #define TRIXI_HIP_IND(D, NQ)                                  \
     do {                                                      \
@@ -1360,6 +1366,21 @@ int trixi_hip_create(const trixi_hip_config* config, const trixi_hip_mesh* mesh,
                               nv <= 5 && nn * dim <= 1024;
         const int64_t line_lanes = E * (nn / N) * dim;
         a.np = eligible && line_lanes < np_lanes ? 1 : 0;
+        // shock capturing: the indicator's parameters (Trixi's threshold
+        // 0.5 * 10^(-1.8 N^0.25) and logistic steepness log((1 - 1e-4) / 1e-4)),
+        // for the fused RK stages of the node-parallel form, which write the
+        // next stage's raw blending factors; TRIXI_HIP_FUSE_INDICATOR=0 keeps
+        // a separate indicator launch per stage
+        a.Vinv = c->d_Vinv;
+        a.ind_threshold = 0.5 * pow(10.0, -1.8 * pow((double)N, 0.25));
+        a.ind_s = log((1.0 - 0.0001) / 0.0001);
+        a.ind_alpha_min = cfg.alpha_min;
+        a.ind_alpha_max = cfg.alpha_max;
+        a.ind_var = cfg.indicator_variable;
+        const char* fenv = getenv("TRIXI_HIP_FUSE_INDICATOR");
+        c->fuse_ind = a.np && cfg.volume_integral == TRIXI_HIP_VOLUME_SHOCK_CAPTURING_HG &&
+                      cfg.equation == TRIXI_HIP_COMPRESSIBLE_EULER && (N == 4 || N == 5) &&
+                      !(fenv && atoi(fenv) == 0);
     }
     {
         // look up this device's static scratch once, at set-up, with the
@@ -1485,10 +1506,15 @@ static int enqueue_rk_stages(trixi_hip_ctx* c, void* u, void* k, hipStream_t s) 
     void* bufB = static_cast<char*>(static_cast<void*>(c->d_tmp)) + bytes;
     void* const in[5] = {u, bufA, bufB, bufA, bufB};
     void* const out[5] = {bufA, bufB, bufA, bufB, u};
+    // fused shock-capturing stages: stage st reads the raw blending factors
+    // of its input from alpha[st % 2] and writes those of its output (the
+    // next stage's input) to alpha[(st + 1) % 2]; only the first stage needs
+    // an indicator launch
+    double* const alpha[2] = {c->d_alpha_raw, c->d_alpha};
     for (int st = 0; st < 5; ++st) {
         void* cur = in[st];
         void* nxt = out[st];
-        if (c->cfg.volume_integral == TRIXI_HIP_VOLUME_SHOCK_CAPTURING_HG) {
+        if (c->cfg.volume_integral == TRIXI_HIP_VOLUME_SHOCK_CAPTURING_HG && (!c->fuse_ind || st == 0)) {
             int rc = launch_indicator(c, cur, s, fp32_state(c));
             if (rc) return rc;
         }
@@ -1504,6 +1530,10 @@ static int enqueue_rk_stages(trixi_hip_ctx* c, void* u, void* k, hipStream_t s) 
         a.rk_a = CK_A[st];
         a.rk_b = CK_B[st];
         a.rk_first = (st == 0);
+        if (c->fuse_ind) {
+            a.alpha_raw = alpha[st % 2];
+            a.alpha_next = st < 4 ? alpha[(st + 1) % 2] : nullptr;
+        }
         HIP_TRY(c->rk_fn(a, s));
     }
     return TRIXI_HIP_OK;
@@ -1811,8 +1841,9 @@ static int require_vjp(trixi_hip_ctx* c) {
 
 // ubar = beta ubar + J(v.u)^T (alpha vbar): the VJP kernel and, for HG shock
 // capturing, the blending factor of v.u before it and its reverse after it.
-static int enqueue_vjp(trixi_hip_ctx* c, const VjpArgs& v, hipStream_t s) {
-    if (is_hg(c)) {
+// alpha_known: v.alpha_raw already holds the raw blending factors of v.u
+static int enqueue_vjp(trixi_hip_ctx* c, const VjpArgs& v, hipStream_t s, bool alpha_known = false) {
+    if (is_hg(c) && !alpha_known) {
         int rc = launch_indicator(c, v.u, s);
         if (rc) return rc;
     }
@@ -1867,10 +1898,12 @@ int trixi_hip_enable_adjoint(trixi_hip_ctx* ctx) {
     ctx->rk64_fn = find_kernel(make_key(c64, 1));
     if (!ctx->rk64_fn) return fail(TRIXI_HIP_EUNSUPPORTED, "no FP64 stage kernel for the adjoint");
     const size_t bytes = (size_t)ctx->nvalues * sizeof(double);
-    double** bufs[3] = {&ctx->d_stage, &ctx->d_k, &ctx->d_kbar};
-    const size_t sizes[3] = {5 * bytes, bytes, bytes};
-    for (int b = 0; b < 3; ++b) {
-        if (*bufs[b]) continue;
+    double** bufs[4] = {&ctx->d_stage, &ctx->d_k, &ctx->d_kbar, &ctx->d_stage_alpha};
+    // the stage inputs' blending factors only where the stages compute them
+    const size_t sizes[4] = {5 * bytes, bytes, bytes,
+                             is_hg(ctx) && ctx->fuse_ind ? 5 * (size_t)ctx->E * sizeof(double) : 0};
+    for (int b = 0; b < 4; ++b) {
+        if (*bufs[b] || sizes[b] == 0) continue;
         if (hipMalloc((void**)bufs[b], sizes[b]) != hipSuccess) {
             // leave no partial set behind: a retry starts from scratch
             *bufs[b] = nullptr;
@@ -1898,9 +1931,15 @@ int trixi_hip_rk_step_vjp(trixi_hip_ctx* ctx, const double* u_n, double* ubar, d
                        CK_C[3], CK_C[4]);
     hipLaunchKernelGGL(copy_kernel, dim3(1024), dim3(256), 0, s, U[0], u_n, n);
     HIP_TRY(hipGetLastError());
+    // fused shock-capturing stages: one indicator launch for U_0, and every
+    // stage writes the raw blending factors of the next input; they are kept
+    // per stage for the reverse sweep
+    const bool fused = is_hg(ctx) && ctx->fuse_ind && ctx->d_stage_alpha;
+    double* A[5];
+    for (int st = 0; st < 5; ++st) A[st] = fused ? ctx->d_stage_alpha + (size_t)st * ctx->E : nullptr;
     for (int st = 0; st < 4; ++st) {
-        if (is_hg(ctx)) {
-            int rc = launch_indicator(ctx, U[st], s);
+        if (is_hg(ctx) && (!fused || st == 0)) {
+            int rc = launch_indicator(ctx, U[st], s, false, A[st]);
             if (rc) return rc;
         }
         RhsArgs a = ctx->args;
@@ -1911,6 +1950,10 @@ int trixi_hip_rk_step_vjp(trixi_hip_ctx* ctx, const double* u_n, double* ubar, d
         a.rk_a = CK_A[st];
         a.rk_b = CK_B[st];
         a.rk_first = (st == 0);
+        if (fused) {
+            a.alpha_raw = A[st];
+            a.alpha_next = A[st + 1];
+        }
         HIP_TRY(ctx->rk64_fn(a, s));
     }
     // reverse the stages
@@ -1925,7 +1968,8 @@ int trixi_hip_rk_step_vjp(trixi_hip_ctx* ctx, const double* u_n, double* ubar, d
         v.alpha = dt;
         v.beta = 1.0;
         v.t = t + CK_C[st] * dt;
-        int rc = enqueue_vjp(ctx, v, s);
+        if (fused) v.alpha_raw = A[st];
+        int rc = enqueue_vjp(ctx, v, s, fused);
         if (rc) return rc;
     }
     return TRIXI_HIP_OK;
