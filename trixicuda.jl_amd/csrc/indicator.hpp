@@ -85,6 +85,44 @@ __device__ __forceinline__ double modal_transform(double m, const double* __rest
     return m;
 }
 
+// The rows of the inverse Vandermonde matrix a lane's modal transform uses
+// (row (n / N^d) % N for direction d), loaded before the state so that their
+// latency overlaps it instead of following the transform's barriers.
+template <int DIM, int N>
+struct VinvRows {
+    double r[DIM][N];
+};
+template <int DIM, int N>
+__device__ __forceinline__ VinvRows<DIM, N> vinv_rows(const double* __restrict__ Vinv, int n) {
+    VinvRows<DIM, N> v;
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+        const int idx = (n / cpow(N, d)) % N;
+#pragma unroll
+        for (int ii = 0; ii < N; ++ii) v.r[d][ii] = Vinv[idx * N + ii];
+    }
+    return v;
+}
+// modal_transform with preloaded rows (the same products in the same order)
+template <int DIM, int N>
+__device__ __forceinline__ double modal_transform_rows(double m, const VinvRows<DIM, N>& V, double* buf, int el,
+                                                       int n) {
+    constexpr int NN = cpow(N, DIM);
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+        const int s = cpow(N, d);
+        const int idx = (n / s) % N, base = n - idx * s;
+        buf[threadIdx.x] = m;
+        __syncthreads();
+        double r = 0.0;
+#pragma unroll
+        for (int ii = 0; ii < N; ++ii) r += V.r[d][ii] * buf[el * NN + base + ii * s];
+        __syncthreads();
+        m = r;
+    }
+    return m;
+}
+
 // modal coefficients of the indicator variable at this lane's node
 template <int DIM, int N, typename S>
 __device__ __forceinline__ double modal_indicator(const S* __restrict__ u, int64_t ec, int64_t E,

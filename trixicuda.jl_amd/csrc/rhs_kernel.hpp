@@ -1029,6 +1029,7 @@ rhs_np_kernel(const RhsArgs a) {
         double* const buf = ind_buf;
         double* const e2 = ind_buf + M * NN;
         const PhysR<double> P64 = PhysR<double>::from(a.phys);
+        const VinvRows<DIM, N> V = vinv_rows<DIM, N>(a.Vinv, node);
         double m = 0.0;
         if (last && valid) m = Euler<DIM>::indicator_variable(uo, P64, a.ind_var);
 #pragma unroll
@@ -1040,7 +1041,7 @@ rhs_np_kernel(const RhsArgs a) {
             if (last) {
                 double r = 0.0;
 #pragma unroll
-                for (int ii = 0; ii < N; ++ii) r += a.Vinv[idx * N + ii] * buf[el * NN + b2 + ii * s2];
+                for (int ii = 0; ii < N; ++ii) r += V.r[dd][ii] * buf[el * NN + b2 + ii * s2];
                 m = r;
             }
             __syncthreads();

@@ -386,7 +386,9 @@ __global__ void __launch_bounds__(256) indicator_kernel(const S* __restrict__ u,
     const int64_t e = (int64_t)blockIdx.x * EPB + elc;
     const int64_t ec = e < E ? e : E - 1;
     const PhysR<double> P = PhysR<double>::from(phys);
-    const double m = modal_indicator<DIM, N, S>(u, ec, E, P, Vinv, buf, elc, n, var, nv);
+    const VinvRows<DIM, N> V = vinv_rows<DIM, N>(Vinv, n);
+    const double m =
+        modal_transform_rows<DIM, N>(indicator_value<DIM, N, S>(u, ec, E, P, n, var, nv), V, buf, elc, n);
     e2[threadIdx.x] = m * m;
     __syncthreads();
     if (n != 0 || el >= EPB || e >= E) return;
