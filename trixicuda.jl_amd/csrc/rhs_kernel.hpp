@@ -47,7 +47,11 @@ constexpr int MAXN = 8;
 #define TRIXI_HIP_MIN_WAVES 1
 #endif
 // Fused RK stage: 1 keeps u and k of the node pass in registers for the
-// register update, 0 reads them again there.
+// register update in the mixed mode (FP32 arithmetic, FP64 state), where
+// that measured faster, and reads them again there in FP64 and with an FP32
+// state (fewer registers: 3D Euler level 5, RK step 0.588 -> 0.573 ms FP64,
+// 0.340 -> 0.330 ms FP32 state, against 0.476 -> 0.498 ms mixed); 0 always
+// reads them again.
 #ifndef TRIXI_HIP_RK_KEEP
 #define TRIXI_HIP_RK_KEEP 1
 #endif
@@ -70,6 +74,11 @@ constexpr int MAXN = 8;
 // ... also in the mixed (FP32) mode
 #ifndef TRIXI_HIP_PAR_DIRS_F32
 #define TRIXI_HIP_PAR_DIRS_F32 0
+#endif
+// ... also in the fused RK stage (FP64: 256 -> 198 VGPRs, 1 -> 2 waves per
+// SIMD; 3D Euler level 5 RK step 0.663 -> 0.588 ms)
+#ifndef TRIXI_HIP_PAR_DIRS_RK
+#define TRIXI_HIP_PAR_DIRS_RK 1
 #endif
 
 struct RhsArgs {
@@ -199,16 +208,18 @@ __device__ __forceinline__ int line_base(int line, int d) {
 // Lane groups of the rhs kernel: with parallel directions each direction has
 // its own M * N^(d-1) lanes (whole waves), otherwise one group runs the
 // directions one after the other.  Parallel directions triple the waves per
-// element, which pays where LDS limits the occupancy of the sequential form:
-// the FP64 rhs! (3D Euler flux differencing at level 5: 93.8 -> 89.5 us).  In
-// FP32 (63.4 -> 76.8 us) and in the fused RK stage (no gain) the sequential
-// form stays, with one copy of each direction's code live at a time, and so
-// it does for the nine-variable MHD system, whose parallel form spills
-// (3D GLM-MHD at level 4: 28.6 -> 45.5 us).
+// element, which pays where LDS or registers limit the occupancy of the
+// sequential form: the FP64 rhs! (3D Euler flux differencing at level 5:
+// 93.8 -> 89.5 us) and the FP64 fused RK stage (fewer registers than the
+// sequential form's 256).  In FP32 arithmetic (63.4 -> 76.8 us) the
+// sequential form stays, with one copy of each direction's code live at a
+// time, and so it does for the nine-variable MHD system, whose parallel form
+// spills (3D GLM-MHD at level 4: 28.6 -> 45.5 us).
 template <int DIM, int N, int M, typename R, bool RK, int NV>
 __host__ __device__ constexpr bool rhs_par_dirs() {
     return TRIXI_HIP_PAR_DIRS != 0 && DIM > 1 && (M * cpow(N, DIM - 1)) % 64 == 0 &&
-           (std::is_same<R, double>::value || TRIXI_HIP_PAR_DIRS_F32 != 0) && !RK && NV <= 5;
+           (std::is_same<R, double>::value || TRIXI_HIP_PAR_DIRS_F32 != 0) && (!RK || TRIXI_HIP_PAR_DIRS_RK != 0) &&
+           NV <= 5;
 }
 template <int DIM, int N, int M, typename R, bool RK, int NV>
 __host__ __device__ constexpr int rhs_threads() {
@@ -294,7 +305,8 @@ rhs_kernel(const RhsArgs a) {
     // last pair is not loaded only after the others are computed (indices
     // past the tile are clamped to a valid pair and not used).  Otherwise
     // the node pass below loads each pair as it goes.
-    constexpr bool RK_KEEP = COOP && TRIXI_HIP_RK_KEEP != 0;
+    constexpr bool RK_KEEP = COOP && TRIXI_HIP_RK_KEEP != 0 && std::is_same<R, float>::value &&
+                             std::is_same<S, double>::value;
     constexpr bool EARLY =
         TRIXI_HIP_EARLY_LOADS != 0 && STAGE && NV <= 5 && NIT * NV * (int)sizeof(S) <= 96;
     S ul[EARLY ? NIT : 1][NV];
