@@ -72,9 +72,17 @@ struct VjpRegistrar {
 
 // Workgroup width in elements: fill about 256 lanes and keep the LDS tiles
 // (node records with their two logarithms, direction accumulator) within
-// 40 KiB so that several workgroups share a CU.
+// 40 KiB (FP64 arithmetic) so that several workgroups share a CU, or 64 KiB
+// with FP32 arithmetic and up to five variables, whose wider tiles (3D Euler: 16 elements, whole
+// 128-byte rows of an FP64 state) measured faster despite two waves per SIMD
+// instead of 2.5: 3D Euler level 5, mixed rhs! 57.2 -> 54.9 us, FP32-state
+// rhs! 45.0 -> 40.4 us, RK steps -11 % and -8 % (the FP64 kernel at 8
+// elements: 80 -> 101 us).
 #ifndef TRIXI_HIP_LDS_BUDGET_KIB
 #define TRIXI_HIP_LDS_BUDGET_KIB 40
+#endif
+#ifndef TRIXI_HIP_LDS_BUDGET_F32_KIB
+#define TRIXI_HIP_LDS_BUDGET_F32_KIB 64
 #endif
 #ifndef TRIXI_HIP_STAGE_NODES
 #define TRIXI_HIP_STAGE_NODES 1
@@ -89,7 +97,10 @@ constexpr int pick_m() {
     constexpr int per_elem = NN * ((TRIXI_HIP_STAGE_NODES ? NRF * RS + 2 * LS : 0) + (DIM > 1 ? NV * RS : 0));
     int m = TRIXI_HIP_MAX_THREADS / NL;
     if (m < 1) m = 1;
-    while (m > 1 && m * per_elem > TRIXI_HIP_LDS_BUDGET_KIB * 1024) m /= 2;
+    // (the nine-variable MHD system keeps the FP64 budget: its wider tiles
+    // take 256 VGPRs)
+    constexpr int budget = (RS == 4 && NV <= 5 ? TRIXI_HIP_LDS_BUDGET_F32_KIB : TRIXI_HIP_LDS_BUDGET_KIB) * 1024;
+    while (m > 1 && m * per_elem > budget) m /= 2;
     return m;
 }
 
