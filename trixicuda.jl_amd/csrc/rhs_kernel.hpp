@@ -66,6 +66,11 @@ constexpr int MAXN = 8;
 #ifndef TRIXI_HIP_STAGE_NODES
 #define TRIXI_HIP_STAGE_NODES 1
 #endif
+// Workgroup size of the node-parallel form (lanes; whole elements of
+// N^d * d lanes each)
+#ifndef TRIXI_HIP_NP_THREADS
+#define TRIXI_HIP_NP_THREADS 256
+#endif
 // Direction phases in parallel: one lane group (whole waves) per direction,
 // combined through the LDS accumulator in direction order.
 #ifndef TRIXI_HIP_PAR_DIRS
@@ -779,7 +784,7 @@ __device__ __forceinline__ typename EQ::template Node<R, L> select_node(
 
 template <int DIM, int N>
 __host__ __device__ constexpr int np_elems() {
-    return (256 / (cpow(N, DIM) * DIM)) > 0 ? 256 / (cpow(N, DIM) * DIM) : 1;
+    return (TRIXI_HIP_NP_THREADS / (cpow(N, DIM) * DIM)) > 0 ? TRIXI_HIP_NP_THREADS / (cpow(N, DIM) * DIM) : 1;
 }
 
 template <int DIM, int N, class EQ, int VOL, int VF, int SF, int FVF, typename R, bool RK, typename L, typename S>
@@ -1093,6 +1098,8 @@ hipError_t launch_rhs(const RhsArgs& a, hipStream_t stream) {
             return hipGetLastError();
         }
     }
+    // only the node-parallel form computes the next stage's blending factors
+    if (a.alpha_next != nullptr) return hipErrorInvalidValue;
     int64_t blocks = (a.E + M - 1) / M;
     if (a.nxcd > 1) blocks = (blocks + a.nxcd - 1) / a.nxcd * a.nxcd;
     hipLaunchKernelGGL((rhs_kernel<DIM, N, EQ, VOL, VF, SF, FVF, R, M, RK, EXT, L, S>), dim3((unsigned)blocks),
