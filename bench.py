@@ -337,8 +337,8 @@ def main():
             "roofline": {"bound": "hbm", "achieved": gbps, "peak": fm.PEAK["hbm_GBps"], "unit": "GB/s",
                          "frac": gbps / fm.PEAK["hbm_GBps"], "traffic": traffic.get(mode),
                          "kernel": "rhs_kernel<3, 4, Euler<3>, FD, flux_ranocha, LLF, "
-                                   + {"fp64": "double, M=4>", "mixed": "float, M=8, logs double>",
-                                      "fp32": "float, M=8, logs float, state float>"}[mode],
+                                   + {"fp64": "double, M=4>", "mixed": "float, M=16, logs double>",
+                                      "fp32": "float, M=16, logs float, state float>"}[mode],
                          "algorithmic_bytes": bytes_dof * dofs, "bytes_per_dof": bytes_dof,
                          "flop_per_dof": flops_dof, "flop_TFLOPs": achieved_tf, "flop_peak_TFLOPs": fpeak,
                          "flop_frac": achieved_tf / fpeak},

@@ -394,9 +394,10 @@ def test_adjoint_unsupported_is_an_error(th, torch, c):
 @pytest.mark.parametrize("case", [c for c in TINY if c.ndims > 1 and not c.patches], ids=lambda c: c.name)
 def test_vjp_direction_groups_equal_one_group(th, torch, case):
     """The direction-group form of the VJP kernel (small meshes: one lane
-    group and LDS accumulator per direction, summed in direction order)
-    against the one-group form (directions one after the other in every
-    lane): the same terms, summed in another order, so they agree to
+    group and LDS accumulator per direction, summed in direction order; for
+    GLM-MHD and 2D shock capturing one more group with all the interface
+    fluxes) against the one-group form (directions one after the other in
+    every lane): the same terms, summed in another order, so they agree to
     round-off (relative max-norm 1e-13)."""
     import os
     out = {}

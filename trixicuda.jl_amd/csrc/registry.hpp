@@ -39,39 +39,11 @@ struct MortarRegistrar {
     static void add(int eq) { register_mortar({DIM, N, eq, 0, 0, SF, 0, 0, 0, 0}, &launch_mortar<DIM, N, EQ, SF>); }
 };
 
-#ifndef TRIXI_HIP_VJP_LDS_KIB
-#define TRIXI_HIP_VJP_LDS_KIB 64
-#endif
-#ifndef TRIXI_HIP_VJP_LDS_ONE_GROUP_KIB
-#define TRIXI_HIP_VJP_LDS_ONE_GROUP_KIB 128
-#endif
-
-// Workgroup width of the adjoint kernel (FP64), a power of two: node records
-// with 1/rho and 1/p, FP64 logarithms, the scaled cotangents and G
-// accumulators (one per direction group) within TRIXI_HIP_VJP_LDS_KIB, or
-// twice that for the direction groups of systems of more than five variables
-// (the nine-variable MHD adjoint: whole waves per direction group, one
-// 100 KiB workgroup per CU; gfx950 gives a workgroup up to 160 KiB of LDS).
-// The one-group form (large meshes) takes up to 128 KiB: 3D Euler at level 5
-// with 8-element tiles, VJP 165 -> 159 us (the direction-group forms of the
-// small meshes measured slower with wider tiles).
-template <int DIM, int N, int NV, int NRF, int G = 1>
-constexpr int pick_m_vjp() {
-    constexpr int NL = cpow(N, DIM - 1), NN = cpow(N, DIM);
-    constexpr int per_elem = NN * (NRF * 8 + 2 * 8 + (1 + G) * NV * 8);
-    constexpr int budget =
-        (G == 1 ? TRIXI_HIP_VJP_LDS_ONE_GROUP_KIB : (NV > 5 ? 2 : 1) * TRIXI_HIP_VJP_LDS_KIB) * 1024;
-    int m = 1;
-    while (2 * m * NL * G <= 256) m *= 2;
-    while (m > 1 && m * per_elem > budget) m /= 2;
-    return m;
-}
-
 template <int DIM, int N, class EQ, int VOL, int VF, int SF, int FVF = FLUX_CENTRAL, bool EXT = false>
 struct VjpRegistrar {
     static void add(int eq) {
-        constexpr int M = pick_m_vjp<DIM, N, EQ::NV, EQ::NRF>();
-        constexpr int MG = pick_m_vjp<DIM, N, EQ::NV, EQ::NRF, DIM>();
+        constexpr int M = pick_m_vjp<DIM, N, VOL, EQ::NV, EQ::NRF>();
+        constexpr int MG = pick_m_vjp<DIM, N, VOL, EQ::NV, EQ::NRF, DIM>();
         register_vjp({DIM, N, eq, VOL, VF, SF, FVF, 0, 0, EXT ? 1 : 0},
                      &launch_vjp<DIM, N, EQ, VOL, VF, SF, FVF, double, M, EXT, MG>);
     }

@@ -53,12 +53,71 @@ struct VjpArgs {
     int64_t pd_lanes;         // meshes with fewer element lines run the direction-group form
 };
 
+#ifndef TRIXI_HIP_VJP_LDS_KIB
+#define TRIXI_HIP_VJP_LDS_KIB 64
+#endif
+#ifndef TRIXI_HIP_VJP_LDS_ONE_GROUP_KIB
+#define TRIXI_HIP_VJP_LDS_ONE_GROUP_KIB 128
+#endif
+
+// Workgroup width of the adjoint kernel (FP64), a power of two, with GA LDS
+// accumulators: node records with 1/rho and 1/p, FP64 logarithms, the scaled
+// cotangents and the accumulators within TRIXI_HIP_VJP_LDS_KIB, or twice that
+// for the direction groups of systems of more than five variables (the
+// nine-variable MHD adjoint: whole waves per direction group; gfx950 gives a
+// workgroup up to 160 KiB of LDS).  The one-group form (large meshes) takes
+// up to 128 KiB: 3D Euler at level 5 with 8-element tiles, VJP 165 -> 159 us
+// (the direction-group forms of the small meshes measured slower with wider
+// tiles).
+template <int DIM, int N, int NV, int NRF, int G, int GA>
+__host__ __device__ constexpr int vjp_width() {
+    constexpr int NL = cpow(N, DIM - 1), NN = cpow(N, DIM);
+    constexpr int per_elem = NN * (NRF * 8 + 2 * 8 + (1 + GA) * NV * 8);
+    constexpr int budget =
+        (G == 1 ? TRIXI_HIP_VJP_LDS_ONE_GROUP_KIB : (NV > 5 ? 2 : 1) * TRIXI_HIP_VJP_LDS_KIB) * 1024;
+    int m = 1;
+    while (2 * m * NL * G <= 256) m *= 2;
+    while (m > 1 && m * per_elem > budget) m /= 2;
+    return m;
+}
+
+// Face group of the direction-group form: one more lane group evaluates the
+// interface fluxes of every direction into an accumulator of its own, beside
+// the direction groups' volume terms.  A workgroup of the nine-variable MHD
+// adjoint (three 64-lane direction groups, one workgroup per CU) then keeps
+// all four SIMDs of its CU busy: 3D GLM-MHD at level 4, VJP 69.1 -> 54.4 us;
+// 2D Euler shock capturing at level 6, 20.9 -> 19.9 us.
+// TRIXI_HIP_VJP_FACE_GROUP: 2, both faces of every line (54.4 us); 1, the +
+// faces, the direction groups keeping the - faces (60.3 us); 0, none.  It
+// applies to the reloading form (LDS accumulators, zeroed in the node pass)
+// where the extra accumulator leaves the tile width unchanged (not 3D Euler
+// shock capturing, whose tiles would halve).
+#ifndef TRIXI_HIP_VJP_FACE_GROUP
+#define TRIXI_HIP_VJP_FACE_GROUP 2
+#endif
+template <int DIM, int N, int VOL, int NV, int NRF, int G>
+__host__ __device__ constexpr int vjp_face_group() {
+    return G > 1 && VOL != VI_WEAK && (NRF > 10 || VOL == VI_HG) &&
+                   vjp_width<DIM, N, NV, NRF, G, G + 1>() == vjp_width<DIM, N, NV, NRF, G, G>()
+               ? TRIXI_HIP_VJP_FACE_GROUP
+               : 0;
+}
+// lane groups of a workgroup: G, plus the face group
+template <int DIM, int N, int VOL, int NV, int NRF, int G>
+__host__ __device__ constexpr int vjp_groups() {
+    return G + (vjp_face_group<DIM, N, VOL, NV, NRF, G>() != 0 ? 1 : 0);
+}
+template <int DIM, int N, int VOL, int NV, int NRF, int G = 1>
+__host__ __device__ constexpr int pick_m_vjp() {
+    return vjp_width<DIM, N, NV, NRF, G, vjp_groups<DIM, N, VOL, NV, NRF, G>()>();
+}
+
 // G direction groups: G = 1 runs the directions one after the other in every
 // lane; G = DIM gives each direction its own lanes (M * N^(d-1) per group)
 // and its own LDS accumulator, summed in direction order at the end, for
 // meshes too small to fill the GPU with one lane per element line.
 template <int DIM, int N, class EQ, int VOL, int VF, int SF, int FVF, typename R, int M, bool EXT, int G>
-__global__ void __launch_bounds__(M * cpow(N, DIM - 1) * G)
+__global__ void __launch_bounds__((M * cpow(N, DIM - 1) * vjp_groups<DIM, N, VOL, EQ::NV, EQ::NRF, G>()))
 vjp_kernel(const VjpArgs a) {
     constexpr int NL = cpow(N, DIM - 1);
     constexpr int NN = cpow(N, DIM);
@@ -66,7 +125,12 @@ vjp_kernel(const VjpArgs a) {
     constexpr int NRF = EQ::NRF;
     constexpr bool PD = G > 1;
     static_assert(G == 1 || G == DIM, "one group, or one group per direction");
-    constexpr int NT = M * NL * G;  // workgroup size
+    constexpr int FGF = vjp_face_group<DIM, N, VOL, NV, NRF, G>();
+    constexpr int GT = vjp_groups<DIM, N, VOL, NV, NRF, G>();  // lane groups (and LDS accumulators)
+    // a group lies within one wave (its directions' accumulations into
+    // shared face nodes stay in program order)
+    static_assert(FGF == 0 || (M * NL <= 64 && 64 % (M * NL) == 0), "face group within one wave");
+    constexpr int NT = M * NL * GT;  // workgroup size
     using Node = typename EQ::template Node<R>;
     using Bar = typename EQ::template Bar<R>;
     // flux differencing with node records of more than 10 values, and shock
@@ -77,11 +141,11 @@ vjp_kernel(const VjpArgs a) {
     __shared__ R rec[NRF * NN * M];
     __shared__ double recd[2 * NN * M];
     __shared__ R ws[NV * NN * M];   // w = -invJ alpha vbar of the tile
-    __shared__ R acc[G * NV * NN * M];  // primitive cotangents, summed over directions (per group)
+    __shared__ R acc[GT * NV * NN * M];  // primitive cotangents, summed over directions (per group)
     __shared__ double abar_s[VOL == VI_HG ? NT : 1];
 
     const int tid = threadIdx.x;
-    const int grp = tid / (M * NL);  // direction of this lane's group (PD)
+    const int grp = tid / (M * NL);  // direction of this lane's group (PD; G: the face group)
     const int gtid = tid - grp * (M * NL);
     const int el = gtid % M;
     const int line = gtid / M;
@@ -143,7 +207,7 @@ vjp_kernel(const VjpArgs a) {
         for (int v = 0; v < NV; ++v) ws[(v * NN + n) * M + c] = wsc * (R)a.vbar[soa(v, n, ee, slab, E32)];
         if constexpr (RELOAD)
 #pragma unroll
-            for (int g = 0; g < G; ++g)
+            for (int g = 0; g < GT; ++g)
 #pragma unroll
                 for (int v = 0; v < NV; ++v) acc[((g * NV + v) * NN + n) * M + c] = R(0);
     }
@@ -338,14 +402,14 @@ vjp_kernel(const VjpArgs a) {
                     add_acc(base + j * s, t);
                 }
             }
-            {
+            if constexpr (FGF < 2) {
                 asm volatile("" ::: "memory");
                 Bar b;
                 EQ::zero(b);
                 face(d, base, s, false, load_node(base), b);
                 add_acc(base, b);
             }
-            {
+            if constexpr (FGF == 0) {
                 asm volatile("" ::: "memory");
                 Bar b;
                 EQ::zero(b);
@@ -440,6 +504,34 @@ vjp_kernel(const VjpArgs a) {
     if constexpr (DIM > 2) {
         if (!PD || grp == 2) phase(std::integral_constant<int, 2>{});
     }
+    if constexpr (FGF != 0) {
+        // the face group: the interfaces of this lane's line in every
+        // direction (both faces, or the + faces), into its own accumulator
+        auto face_phase = [&](auto dc) {
+            constexpr int d = decltype(dc)::value;
+            const int s = cpow(N, d);
+            const int base = line_base<DIM, N>(line, d);
+            if constexpr (FGF == 2) {
+                asm volatile("" ::: "memory");
+                Bar b;
+                EQ::zero(b);
+                face(d, base, s, false, load_node(base), b);
+                add_acc(base, b);
+            }
+            {
+                asm volatile("" ::: "memory");
+                Bar b;
+                EQ::zero(b);
+                face(d, base, s, true, load_node(base + (N - 1) * s), b);
+                add_acc(base + (N - 1) * s, b);
+            }
+        };
+        if (grp == G) {
+            face_phase(std::integral_constant<int, 0>{});
+            if constexpr (DIM > 1) face_phase(std::integral_constant<int, 1>{});
+            if constexpr (DIM > 2) face_phase(std::integral_constant<int, 2>{});
+        }
+    }
     if constexpr (PD) __syncthreads();
 
     if constexpr (HG) {
@@ -471,7 +563,7 @@ vjp_kernel(const VjpArgs a) {
         for (int v = 0; v < NV; ++v) {
             t[v] = acc[(v * NN + n) * M + c];
 #pragma unroll
-            for (int g = 1; g < G; ++g) t[v] += acc[((g * NV + v) * NN + n) * M + c];
+            for (int g = 1; g < GT; ++g) t[v] += acc[((g * NV + v) * NN + n) * M + c];
         }
         Bar tot;
         EQ::array_to_bar(t, tot);
@@ -498,7 +590,7 @@ hipError_t launch_vjp(const VjpArgs& a, hipStream_t stream) {
             int64_t blocks = (a.E + MG - 1) / MG;
             if (a.nxcd > 1) blocks = (blocks + a.nxcd - 1) / a.nxcd * a.nxcd;
             hipLaunchKernelGGL((vjp_kernel<DIM, N, EQ, VOL, VF, SF, FVF, R, MG, EXT, DIM>), dim3((unsigned)blocks),
-                               dim3(MG * NL * DIM), 0, stream, a);
+                               dim3(MG * NL * vjp_groups<DIM, N, VOL, EQ::NV, EQ::NRF, DIM>()), 0, stream, a);
             return hipGetLastError();
         }
     }
