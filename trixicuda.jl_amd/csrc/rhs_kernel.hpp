@@ -55,6 +55,15 @@ constexpr int MAXN = 8;
 #ifndef TRIXI_HIP_RK_KEEP
 #define TRIXI_HIP_RK_KEEP 1
 #endif
+// Fused RK stage of the sequential form with an FP32 state (u and k read
+// again): 1 requests them once the lane's last direction is computed, before
+// the barrier ahead of the update, so that their latency overlaps the wait;
+// 0 reads them in the update itself.  3D Euler level 5, FP32-state RK step
+// 0.306 -> 0.260 ms (three rounds on one box).  The FP64 parallel-direction
+// form measured 0.578 -> 0.588 ms with it and reads them in the update.
+#ifndef TRIXI_HIP_RK_PREFETCH
+#define TRIXI_HIP_RK_PREFETCH 1
+#endif
 // 1: the node pass's state loads are issued before the face loads (systems
 // of up to five variables); 0: each (node, element) pair is loaded as the
 // node pass reaches it.
@@ -316,6 +325,28 @@ rhs_kernel(const RhsArgs a) {
         TRIXI_HIP_EARLY_LOADS != 0 && STAGE && NV <= 5 && NIT * NV * (int)sizeof(S) <= 96;
     S ul[EARLY ? NIT : 1][NV];
     double ukeep[RK_KEEP && !EARLY ? NIT : 1][NV], kkeep[RK_KEEP ? NIT : 1][NV];
+    // RK_PRE: u and k of the update's (node, element) pairs, requested after
+    // the direction work (few registers are live there; at most 40 VGPRs)
+    constexpr bool RK_PRE = COOP && !RK_KEEP && !PD && TRIXI_HIP_RK_PREFETCH != 0 && std::is_same<S, float>::value &&
+                            NV <= 5 && NIT * NV * (int)sizeof(S) <= 80;
+    S upre[RK_PRE ? NIT : 1][NV], kpre[RK_PRE ? NIT : 1][NV];
+    auto rk_prefetch = [&]() {
+#pragma unroll
+        for (int it = 0; it < (RK_PRE ? NIT : 0); ++it) {
+            int q = tid + it * NT;
+            if ((it + 1) * NT > NN * M && q >= NN * M) q = NN * M - 1;
+            const int c = q % M;
+            const int n = q / M;
+            int64_t ee = e0 + c;
+            if (ee >= E) ee = E - 1;
+#pragma unroll
+            for (int v = 0; v < NV; ++v) {
+                const int64_t g = soa(v, n, ee, slab, E32);
+                upre[it][v] = U[g];
+                kpre[it][v] = a.rk_first ? S(0) : K[g];
+            }
+        }
+    };
 #pragma unroll
     for (int it = 0; it < (EARLY ? NIT : 0); ++it) {
         int q = tid + it * NT;
@@ -673,6 +704,7 @@ rhs_kernel(const RhsArgs a) {
             base_g = base;
             s_g = s;
         } else if (d + 1 < DIM || COOP) {
+            if (d + 1 == DIM) rk_prefetch();
 #pragma unroll
             for (int p = 0; p < N; ++p)
 #pragma unroll
@@ -738,8 +770,12 @@ rhs_kernel(const RhsArgs a) {
             for (int v = 0; v < NV; ++v) {
                 const int64_t g = soa(v, n, ee, slab, E32);
                 const double val = (double)acc[(v * NN + n) * LDM + c] * fac + sv[v];
-                const double kold = RK_KEEP ? kkeep[it][v] : (a.rk_first ? 0.0 : (double)K[g]);
-                const double uold = RK_KEEP ? (EARLY ? (double)ul[it][v] : ukeep[it][v]) : (double)U[g];
+                const double kold = RK_KEEP  ? kkeep[it][v]
+                                    : RK_PRE ? (double)kpre[it][v]
+                                             : (a.rk_first ? 0.0 : (double)K[g]);
+                const double uold = RK_KEEP  ? (EARLY ? (double)ul[it][v] : ukeep[it][v])
+                                    : RK_PRE ? (double)upre[it][v]
+                                             : (double)U[g];
                 const S kk = (S)(a.rk_first ? t_dt * val : a.rk_a * kold + t_dt * val);
                 K[g] = kk;
                 UO[g] = (S)(uold + a.rk_b * (double)kk);
