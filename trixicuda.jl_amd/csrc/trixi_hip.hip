@@ -445,28 +445,50 @@ __global__ void __launch_bounds__(64) indicator_vjp_kernel(
     __shared__ double buf[64];
     __shared__ double e2[64];
     __shared__ double coef[EPB][3];  // cotangents of total, clip1, clip2 (0: nothing to do)
+    __shared__ double ab_s[EPB];     // cotangent of each element's raw factor
     const int el = threadIdx.x / NN, n = threadIdx.x % NN;
     const int64_t e = (int64_t)blockIdx.x * EPB + el;
     const bool valid = e < E;
     const int64_t ec = valid ? e : E - 1;
+    // the gathered cotangent first (all loads issued together, the
+    // neighbours' unconditionally), so that a block whose elements all have
+    // none (smoothed factors of 0: most elements of a mesh) ends here
+    if (n == 0) {
+        double ab = 0.0;
+        if (valid) {
+            if (smooth) {
+                int32_t nbv[2 * DIM], wn[2 * DIM];
+                double an[2 * DIM];
+#pragma unroll
+                for (int f = 0; f < 2 * DIM; ++f) nbv[f] = nbr[(int64_t)f * E + e];
+#pragma unroll
+                for (int f = 0; f < 2 * DIM; ++f) {
+                    const int64_t q = nbv[f] >= 0 ? (int64_t)nbv[f] : e;
+                    wn[f] = winner[q];
+                    an[f] = alphabar[q];
+                }
+                ab = winner[e] < 0 ? alphabar[e] : 0.0;
+#pragma unroll
+                for (int f = 0; f < 2 * DIM; ++f)
+                    if (nbv[f] >= 0 && wn[f] == (f ^ 1)) ab += 0.5 * an[f];
+            } else {
+                ab = alphabar[e];
+            }
+        }
+        ab_s[el] = ab;
+    }
+    __syncthreads();
+    bool any = false;
+#pragma unroll
+    for (int k = 0; k < EPB; ++k) any = any || ab_s[k] != 0.0;
+    if (!any) return;  // block-uniform
     const PhysR<double> P = PhysR<double>::from(phys);
     const double m = modal_indicator<DIM, N, double>(u, ec, E, P, Vinv, buf, el, n, var, nv);
     e2[threadIdx.x] = m * m;
     __syncthreads();
     if (n == 0) {
         double tb = 0.0, c1b = 0.0, c2b = 0.0;
-        double ab = 0.0;
-        if (valid) {
-            if (smooth) {
-                ab = winner[e] < 0 ? alphabar[e] : 0.0;
-                for (int f = 0; f < 2 * DIM; ++f) {
-                    const int32_t nb = nbr[(int64_t)f * E + e];
-                    if (nb >= 0 && winner[nb] == (f ^ 1)) ab += 0.5 * alphabar[nb];
-                }
-            } else {
-                ab = alphabar[e];
-            }
-        }
+        const double ab = ab_s[el];
         if (ab != 0.0) {
             double total, clip1, clip2;
             modal_energies<DIM, N>(e2, el, total, clip1, clip2);
