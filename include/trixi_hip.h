@@ -383,8 +383,8 @@ int trixi_hip_enable_adjoint(trixi_hip_ctx* ctx);
  * u_n (a checkpoint, not modified): on entry ubar holds the cotangent of
  * u_{n+1}, on exit that of u_n.  The five stage inputs are recomputed in
  * FP64 from u_n, then the stages are reversed (register cotangent update +
- * rhs VJP per stage).  No reference counterpart (multi-step gradients with
- * RK-stage checkpointing). */
+ * rhs VJP per stage).  ubar must not alias u_n.  No reference counterpart
+ * (multi-step gradients with RK-stage checkpointing). */
 int trixi_hip_rk_step_vjp(trixi_hip_ctx* ctx, const double* u_n, double* ubar, double t,
                           double dt, trixi_hip_stream stream);
 

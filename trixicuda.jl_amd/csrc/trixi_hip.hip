@@ -437,7 +437,7 @@ __global__ void __launch_bounds__(64) indicator_vjp_kernel(
     const double* __restrict__ u, double* __restrict__ ubar, const double* __restrict__ alphabar,
     const int32_t* __restrict__ winner, const int32_t* __restrict__ nbr, int smooth, int64_t E, PhysParams phys,
     const double* __restrict__ Vinv, double threshold, double parameter_s, double alpha_min, double alpha_max,
-    int var, int nv) {
+    int var, int nv, double* __restrict__ kbar_out, double kb) {
     constexpr int NN = cpow(N, DIM);
     constexpr int NV = DIM + 2;
     constexpr int EPB = 64 / NN;
@@ -450,6 +450,13 @@ __global__ void __launch_bounds__(64) indicator_vjp_kernel(
     const int64_t e = (int64_t)blockIdx.x * EPB + el;
     const bool valid = e < E;
     const int64_t ec = valid ? e : E - 1;
+    // ubar[i] += d, and in a reverse RK stage the earlier stage's register
+    // cotangent (kbar_out = a kbar + kb ubar, written by the VJP kernel
+    // before this one) follows: kbar_out[i] += kb d
+    auto bump = [&](int64_t i, double d) {
+        ubar[i] += d;
+        if (kbar_out != nullptr) kbar_out[i] += kb * d;
+    };
     // the gathered cotangent first (all loads issued together, the
     // neighbours' unconditionally), so that a block whose elements all have
     // none (smoothed factors of 0: most elements of a mesh) ends here
@@ -537,7 +544,7 @@ __global__ void __launch_bounds__(64) indicator_vjp_kernel(
     if (!active) return;
     const int64_t r0 = (int64_t)n * E + e;
     if (var == 2) {  // density
-        ubar[r0] += mb;
+        bump(r0, mb);
         return;
     }
     const double g = mb * P.gm1;
@@ -551,20 +558,20 @@ __global__ void __launch_bounds__(64) indicator_vjp_kernel(
             const int64_t rd = (int64_t)(1 + d) * NN * E + r0;
             const double vd = u[rd] * irho;
             ke += vd * vd;
-            ubar[rd] -= g * vd;
+            bump(rd, -(g * vd));
         }
-        ubar[r0] += g * 0.5 * ke;
-        ubar[(int64_t)(DIM + 1) * NN * E + r0] += g;
+        bump(r0, g * 0.5 * ke);
+        bump((int64_t)(DIM + 1) * NN * E + r0, g);
         return;
     }
     // density * pressure = (gamma - 1) (rho E - |rho v|^2 / 2)
-    ubar[r0] += g * en;
+    bump(r0, g * en);
 #pragma unroll
     for (int d = 0; d < DIM; ++d) {
         const int64_t rd = (int64_t)(1 + d) * NN * E + r0;
-        ubar[rd] -= g * u[rd];
+        bump(rd, -(g * u[rd]));
     }
-    ubar[(int64_t)(DIM + 1) * NN * E + r0] += g * rho;
+    bump((int64_t)(DIM + 1) * NN * E + r0, g * rho);
     (void)NV;
 }
 
@@ -1883,7 +1890,8 @@ static int enqueue_vjp(trixi_hip_ctx* c, const VjpArgs& v, hipStream_t s, bool a
     hipLaunchKernelGGL((indicator_vjp_kernel<D, 4>), dim3(blocks), dim3(threads), 0, s, v.u, v.ubar,            \
                        (const double*)c->d_alphabar, (const int32_t*)c->d_winner, (const int32_t*)c->d_nbr,    \
                        c->cfg.alpha_smooth, c->E, P, (const double*)c->d_Vinv, threshold, parameter_s,       \
-                       c->cfg.alpha_min, c->cfg.alpha_max, c->cfg.indicator_variable, c->nv)
+                       c->cfg.alpha_min, c->cfg.alpha_max, c->cfg.indicator_variable, c->nv, v.kbar_out,     \
+                       v.kbar_b)
         if (c->dim == 1)
             TRIXI_HIP_IND_VJP(1);
         else if (c->dim == 2)
@@ -1948,12 +1956,14 @@ int trixi_hip_rk_step_vjp(trixi_hip_ctx* ctx, const double* u_n, double* ubar, d
     if (!ctx->adjoint) return fail(TRIXI_HIP_EINVAL, "call trixi_hip_enable_adjoint first");
     hipStream_t s = (hipStream_t)stream;
     const int64_t n = ctx->nvalues;
-    double* U[5];
-    for (int st = 0; st < 5; ++st) U[st] = ctx->d_stage + (size_t)st * n;
-    // recompute the stage inputs U_0 .. U_4 in FP64
+    // stage inputs: U_0 is the checkpoint itself (only read), U_1 .. U_4 are
+    // recomputed in FP64 into the stage buffers
+    const double* U[5];
+    double* Uw[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+    U[0] = u_n;
+    for (int st = 1; st < 5; ++st) U[st] = Uw[st] = ctx->d_stage + (size_t)st * n;
     hipLaunchKernelGGL(set_time_kernel, dim3(1), dim3(1), 0, s, ctx->d_time, t, dt, CK_C[0], CK_C[1], CK_C[2],
                        CK_C[3], CK_C[4]);
-    hipLaunchKernelGGL(copy_kernel, dim3(1024), dim3(256), 0, s, U[0], u_n, n);
     HIP_TRY(hipGetLastError());
     // fused shock-capturing stages: one indicator launch for U_0, and every
     // stage writes the raw blending factors of the next input; they are kept
@@ -1969,7 +1979,7 @@ int trixi_hip_rk_step_vjp(trixi_hip_ctx* ctx, const double* u_n, double* ubar, d
         RhsArgs a = ctx->args;
         a.u = U[st];
         a.k = ctx->d_k;
-        a.u_out = U[st + 1];
+        a.u_out = Uw[st + 1];
         a.time = ctx->d_time + 2 * st;
         a.rk_a = CK_A[st];
         a.rk_b = CK_B[st];
@@ -1980,14 +1990,24 @@ int trixi_hip_rk_step_vjp(trixi_hip_ctx* ctx, const double* u_n, double* ubar, d
         }
         HIP_TRY(ctx->rk64_fn(a, s));
     }
-    // reverse the stages
+    // reverse the stages: kbar_4 = B_4 ubar here, then each stage's VJP
+    // leaves kbar_{s-1} = A_s kbar_s + B_{s-1} ubar in the other register
+    // buffer as it finishes each node (the stage buffers' first slot is free:
+    // U_0 is u_n)
+    double* Kb[2] = {ctx->d_kbar, ctx->d_stage};
+    int cur = 0;
+    hipLaunchKernelGGL(kbar_update_kernel, dim3(1024), dim3(256), 0, s, Kb[cur], (const double*)ubar, 0.0, CK_B[4],
+                       1, n);
+    HIP_TRY(hipGetLastError());
     for (int st = 4; st >= 0; --st) {
-        hipLaunchKernelGGL(kbar_update_kernel, dim3(1024), dim3(256), 0, s, ctx->d_kbar, (const double*)ubar,
-                           st < 4 ? CK_A[st + 1] : 0.0, CK_B[st], st == 4 ? 1 : 0, n);
-        HIP_TRY(hipGetLastError());
         VjpArgs v = vjp_args(ctx);
         v.u = U[st];
-        v.vbar = ctx->d_kbar;
+        v.vbar = Kb[cur];
+        if (st > 0) {
+            v.kbar_out = Kb[cur ^ 1];
+            v.kbar_a = CK_A[st];
+            v.kbar_b = CK_B[st - 1];
+        }
         v.ubar = ubar;
         v.alpha = dt;
         v.beta = 1.0;
@@ -1995,6 +2015,7 @@ int trixi_hip_rk_step_vjp(trixi_hip_ctx* ctx, const double* u_n, double* ubar, d
         if (fused) v.alpha_raw = A[st];
         int rc = enqueue_vjp(ctx, v, s, fused);
         if (rc) return rc;
+        cur ^= 1;
     }
     return TRIXI_HIP_OK;
 }

@@ -51,6 +51,12 @@ struct VjpArgs {
     int bc;
     double t;                 // time of the linearisation point
     int64_t pd_lanes;         // meshes with fewer element lines run the direction-group form
+    // reverse RK stage: the next (earlier) stage's register cotangent
+    // kbar_out = kbar_a * vbar + kbar_b * ubar (ubar as this call leaves it),
+    // written with each node's ubar (with shock capturing the reverse
+    // indicator then adds kbar_b times its own ubar increments); nullptr: none
+    double* kbar_out;
+    double kbar_a, kbar_b;
 };
 
 #ifndef TRIXI_HIP_VJP_LDS_KIB
@@ -573,7 +579,11 @@ vjp_kernel(const VjpArgs a) {
         for (int v = 0; v < NV; ++v) {
             const int64_t g = soa(v, n, ee, slab, E32);
             const double prev = a.beta != 0.0 ? a.beta * a.ubar[g] : 0.0;
-            a.ubar[g] = prev + (double)ub[v];
+            const double ubn = prev + (double)ub[v];
+            a.ubar[g] = ubn;
+            // (vbar is not written by this kernel: the neighbours' reads of
+            // it stay valid, kbar_out being another buffer)
+            if (a.kbar_out != nullptr) a.kbar_out[g] = a.kbar_a * a.vbar[g] + a.kbar_b * ubn;
         }
     }
 }
