@@ -66,15 +66,28 @@ def test_fp32_state_rhs_matches_oracle(th, oracle, torch, case):
     assert err <= FP32_TOL, err
 
 
+@pytest.mark.parametrize("form", ["default", "line"])
 @pytest.mark.parametrize("use_graph", [False, True])
 @pytest.mark.parametrize("case", [FP32_CASES[0], FP32_CASES[5]], ids=lambda c: c.name)
-def test_fp32_state_rk_step_matches_oracle(th, oracle, torch, case, use_graph):
+def test_fp32_state_rk_step_matches_oracle(th, oracle, torch, case, use_graph, form):
     """Two CK2N54 steps in FP32 state against the oracle's FP64 steps from the
-    same FP32 initial state (graph replays checked after each step)."""
+    same FP32 initial state (graph replays checked after each step); `line`
+    forces the line form of the fused stage (the form of large meshes) on
+    these small ones, which otherwise take the node-parallel form."""
+    import os
     O = oracle
     cfg = oracle_cfg(O, case)
     mesh, u = _fp32_state(O, case)
-    semi = product_semi(th, case, "fp32")
+    old = os.environ.get("TRIXI_HIP_NP_LANES")
+    if form == "line":
+        os.environ["TRIXI_HIP_NP_LANES"] = "0"
+    try:
+        semi = product_semi(th, case, "fp32")
+    finally:
+        if old is None:
+            os.environ.pop("TRIXI_HIP_NP_LANES", None)
+        else:
+            os.environ["TRIXI_HIP_NP_LANES"] = old
     try:
         ud = th.pack(torch.from_numpy(u).cuda(), semi)
         kd = torch.zeros_like(ud)

@@ -86,3 +86,40 @@ def test_node_parallel_equals_line_form(th, torch, case, precision):
         err = float(np.abs(a - b).max() / np.abs(b).max())
         print(f"{case.name} {precision} {name}: node-parallel vs line rel max-norm {err:.3e}", flush=True)
         assert err <= tol, (name, err)
+
+
+# the cases built with an FP32 state (Trixi's RealT = uEltype = Float32)
+FP32_NP_CASES = [NP_CASES[0], NP_CASES[2], NP_CASES[3]]
+
+
+@pytest.mark.parametrize("case", FP32_NP_CASES, ids=lambda c: c.name)
+def test_node_parallel_equals_line_form_fp32_state(th, torch, case):
+    """The same with an FP32 state: du, and u and k after one CK2N54 step
+    of fused stages (the line form's update requests u and k ahead of its
+    last barrier) against the node-parallel form, within the FP32-state
+    parity tolerance (1e-4, each form's bound against the FP64 oracle).  k is
+    measured against the scale of its terms, dt max|du|:
+    the step's k is a cancellation of the stages' terms, so its own maximum
+    is smaller and would magnify the forms' FP32 rounding differences."""
+    out = {}
+    dt = 1e-3
+    for npf in (True, False):
+        semi = _semi(th, case, "fp32", npf)
+        try:
+            u = th.compute_coefficients(0.0, semi)
+            assert u.dtype == torch.float32
+            du = torch.empty_like(u)
+            th.rhs_hip_(du, u, semi, 0.0)
+            k = torch.zeros_like(u)
+            th.rk_step(u, k, 0.0, dt, semi, use_graph=False)
+            torch.cuda.synchronize()
+            out[npf] = (du.double().cpu().numpy(), u.double().cpu().numpy(), k.double().cpu().numpy())
+        finally:
+            semi.close()
+    scale = {"du": np.abs(out[False][0]).max(), "u": np.abs(out[False][1]).max(),
+             "k": dt * np.abs(out[False][0]).max()}
+    for name, a, b in zip(("du", "u", "k"), out[True], out[False]):
+        assert np.all(np.isfinite(a)), name
+        err = float(np.abs(a - b).max() / scale[name])
+        print(f"{case.name} fp32 state {name}: node-parallel vs line rel max-norm {err:.3e}", flush=True)
+        assert err <= 1e-4, (name, err)
