@@ -87,31 +87,35 @@ __host__ __device__ constexpr int vjp_width() {
     return m;
 }
 
-// Face group of the direction-group form: one more lane group evaluates the
-// interface fluxes of every direction into an accumulator of its own, beside
-// the direction groups' volume terms.  A workgroup of the nine-variable MHD
-// adjoint (three 64-lane direction groups, one workgroup per CU) then keeps
-// all four SIMDs of its CU busy: 3D GLM-MHD at level 4, VJP 69.1 -> 54.4 us;
-// 2D Euler shock capturing at level 6, 20.9 -> 19.9 us.
-// TRIXI_HIP_VJP_FACE_GROUP: 2, both faces of every line (54.4 us); 1, the +
-// faces, the direction groups keeping the - faces (60.3 us); 0, none.  It
-// applies to the reloading form (LDS accumulators, zeroed in the node pass)
-// where the extra accumulator leaves the tile width unchanged (not 3D Euler
-// shock capturing, whose tiles would halve).
+// Surface groups of the direction-group form: beside the direction groups'
+// volume terms, more lane groups evaluate the interface fluxes, each into an
+// LDS accumulator of its own.  A workgroup of the nine-variable MHD adjoint
+// (three 64-lane direction groups, one workgroup per CU) then keeps all four
+// SIMDs of its CU busy: one face group for every direction, 3D GLM-MHD at
+// level 4, VJP 69.1 -> 54.4 us (the + faces alone in it, the direction groups
+// keeping the - faces, 60.3 us).  With shock capturing, one group per
+// direction takes that direction's faces and its FV subcells, leaving the
+// direction groups the DG pairs.
+// TRIXI_HIP_VJP_FACE_GROUP: 2, per-direction groups with shock capturing
+// where they fit, else one face group; 1, one face group; 0, none.  They
+// apply to the reloading form (LDS accumulators, zeroed in the node pass)
+// where the extra accumulators leave the tile width unchanged.
 #ifndef TRIXI_HIP_VJP_FACE_GROUP
 #define TRIXI_HIP_VJP_FACE_GROUP 2
 #endif
+// the number of surface groups: 0, 1 (one face group) or G (per direction)
 template <int DIM, int N, int VOL, int NV, int NRF, int G>
-__host__ __device__ constexpr int vjp_face_group() {
-    return G > 1 && VOL != VI_WEAK && (NRF > 10 || VOL == VI_HG) &&
-                   vjp_width<DIM, N, NV, NRF, G, G + 1>() == vjp_width<DIM, N, NV, NRF, G, G>()
-               ? TRIXI_HIP_VJP_FACE_GROUP
-               : 0;
+__host__ __device__ constexpr int vjp_surface_groups() {
+    if (!(G > 1 && VOL != VI_WEAK && (NRF > 10 || VOL == VI_HG)) || TRIXI_HIP_VJP_FACE_GROUP == 0) return 0;
+    constexpr int w = vjp_width<DIM, N, NV, NRF, G, G>();
+    if (TRIXI_HIP_VJP_FACE_GROUP >= 2 && VOL == VI_HG && vjp_width<DIM, N, NV, NRF, G, 2 * G>() == w) return G;
+    if (vjp_width<DIM, N, NV, NRF, G, G + 1>() == w) return 1;
+    return 0;
 }
-// lane groups of a workgroup: G, plus the face group
+// lane groups of a workgroup: G, plus the surface groups
 template <int DIM, int N, int VOL, int NV, int NRF, int G>
 __host__ __device__ constexpr int vjp_groups() {
-    return G + (vjp_face_group<DIM, N, VOL, NV, NRF, G>() != 0 ? 1 : 0);
+    return G + vjp_surface_groups<DIM, N, VOL, NV, NRF, G>();
 }
 template <int DIM, int N, int VOL, int NV, int NRF, int G = 1>
 __host__ __device__ constexpr int pick_m_vjp() {
@@ -131,11 +135,11 @@ vjp_kernel(const VjpArgs a) {
     constexpr int NRF = EQ::NRF;
     constexpr bool PD = G > 1;
     static_assert(G == 1 || G == DIM, "one group, or one group per direction");
-    constexpr int FGF = vjp_face_group<DIM, N, VOL, NV, NRF, G>();
-    constexpr int GT = vjp_groups<DIM, N, VOL, NV, NRF, G>();  // lane groups (and LDS accumulators)
-    // a group lies within one wave (its directions' accumulations into
-    // shared face nodes stay in program order)
-    static_assert(FGF == 0 || (M * NL <= 64 && 64 % (M * NL) == 0), "face group within one wave");
+    constexpr int NS = vjp_surface_groups<DIM, N, VOL, NV, NRF, G>();
+    constexpr int GT = G + NS;  // lane groups (and LDS accumulators)
+    // one face group lies within one wave (its directions' accumulations
+    // into shared face nodes stay in program order)
+    static_assert(NS != 1 || (M * NL <= 64 && 64 % (M * NL) == 0), "face group within one wave");
     constexpr int NT = M * NL * GT;  // workgroup size
     using Node = typename EQ::template Node<R>;
     using Bar = typename EQ::template Bar<R>;
@@ -151,7 +155,7 @@ vjp_kernel(const VjpArgs a) {
     __shared__ double abar_s[VOL == VI_HG ? NT : 1];
 
     const int tid = threadIdx.x;
-    const int grp = tid / (M * NL);  // direction of this lane's group (PD; G: the face group)
+    const int grp = tid / (M * NL);  // direction of this lane's group (PD; >= G: surface groups)
     const int gtid = tid - grp * (M * NL);
     const int el = gtid % M;
     const int line = gtid / M;
@@ -299,6 +303,40 @@ vjp_kernel(const VjpArgs a) {
         }
     };
 
+    // FV subcells of the shock-capturing blend along this lane's line in
+    // direction d: acc_p += alpha invw_p F, acc_{p+1} -= alpha invw_{p+1} F
+    auto fv_block = [&](auto dc) {
+        constexpr int d = decltype(dc)::value;
+        const int s = cpow(N, d);
+        const int base = line_base<DIM, N>(line, d);
+        auto W = [&](int p, int v) -> R { return Wn(base + p * s, v); };
+        const bool blend = HG && !dg_only;
+        if constexpr (HG) {
+            if (blend) {
+#pragma unroll
+                for (int p = 0; p + 1 < N; ++p) {
+                    asm volatile("" ::: "memory");
+                    const Node np0 = load_node(base + p * s);
+                    const Node np1 = load_node(base + (p + 1) * s);
+                    R h[NV], F[NV];
+                    EQ::template two_point_value<FVF>(np0, np1, d, P, F);
+#pragma unroll
+                    for (int v = 0; v < NV; ++v) {
+                        h[v] = (R)a.invw[p] * W(p, v) - (R)a.invw[p + 1] * W(p + 1, v);
+                        abar_part += (double)(h[v] * F[v]);
+                        h[v] *= alpha;
+                    }
+                    Bar b0, b1;
+                    EQ::zero(b0);
+                    EQ::zero(b1);
+                    EQ::template two_point_vjp<FVF>(np0, np1, d, h, b0, b1, P);
+                    add_acc(base + p * s, b0);
+                    add_acc(base + (p + 1) * s, b1);
+                }
+            }
+        }
+    };
+
     // one phase per direction; the direction is a compile-time constant so
     // that every component index (v[d], B[d], g[1 + d]) stays static
     auto phase = [&](auto dc) {
@@ -346,31 +384,7 @@ vjp_kernel(const VjpArgs a) {
                 }
                 add_acc(base + i * s, bi);
             }
-            if constexpr (HG) {
-                if (blend) {
-                    // FV subcells: acc_p += alpha invw_p F, acc_{p+1} -= alpha invw_{p+1} F
-#pragma unroll
-                    for (int p = 0; p + 1 < N; ++p) {
-                        asm volatile("" ::: "memory");
-                        const Node np0 = load_node(base + p * s);
-                        const Node np1 = load_node(base + (p + 1) * s);
-                        R h[NV], F[NV];
-                        EQ::template two_point_value<FVF>(np0, np1, d, P, F);
-#pragma unroll
-                        for (int v = 0; v < NV; ++v) {
-                            h[v] = (R)a.invw[p] * W(p, v) - (R)a.invw[p + 1] * W(p + 1, v);
-                            abar_part += (double)(h[v] * F[v]);
-                            h[v] *= alpha;
-                        }
-                        Bar b0, b1;
-                        EQ::zero(b0);
-                        EQ::zero(b1);
-                        EQ::template two_point_vjp<FVF>(np0, np1, d, h, b0, b1, P);
-                        add_acc(base + p * s, b0);
-                        add_acc(base + (p + 1) * s, b1);
-                    }
-                }
-            }
+            if constexpr (NS != G) fv_block(dc);
             if constexpr (EQ::NONCONS) {
                 R ca[N], cc[N];
 #pragma unroll
@@ -408,14 +422,14 @@ vjp_kernel(const VjpArgs a) {
                     add_acc(base + j * s, t);
                 }
             }
-            if constexpr (FGF < 2) {
+            if constexpr (NS == 0) {
                 asm volatile("" ::: "memory");
                 Bar b;
                 EQ::zero(b);
                 face(d, base, s, false, load_node(base), b);
                 add_acc(base, b);
             }
-            if constexpr (FGF == 0) {
+            if constexpr (NS == 0) {
                 asm volatile("" ::: "memory");
                 Bar b;
                 EQ::zero(b);
@@ -510,14 +524,14 @@ vjp_kernel(const VjpArgs a) {
     if constexpr (DIM > 2) {
         if (!PD || grp == 2) phase(std::integral_constant<int, 2>{});
     }
-    if constexpr (FGF != 0) {
-        // the face group: the interfaces of this lane's line in every
-        // direction (both faces, or the + faces), into its own accumulator
-        auto face_phase = [&](auto dc) {
+    if constexpr (NS != 0) {
+        // surface groups: the interfaces of this lane's line (and with one
+        // group per direction the FV subcells), into their own accumulators
+        auto faces_of_line = [&](auto dc) {
             constexpr int d = decltype(dc)::value;
             const int s = cpow(N, d);
             const int base = line_base<DIM, N>(line, d);
-            if constexpr (FGF == 2) {
+            {
                 asm volatile("" ::: "memory");
                 Bar b;
                 EQ::zero(b);
@@ -532,10 +546,29 @@ vjp_kernel(const VjpArgs a) {
                 add_acc(base + (N - 1) * s, b);
             }
         };
-        if (grp == G) {
-            face_phase(std::integral_constant<int, 0>{});
-            if constexpr (DIM > 1) face_phase(std::integral_constant<int, 1>{});
-            if constexpr (DIM > 2) face_phase(std::integral_constant<int, 2>{});
+        if constexpr (NS == 1) {
+            if (grp == G) {
+                faces_of_line(std::integral_constant<int, 0>{});
+                if constexpr (DIM > 1) faces_of_line(std::integral_constant<int, 1>{});
+                if constexpr (DIM > 2) faces_of_line(std::integral_constant<int, 2>{});
+            }
+        } else {
+            if (grp == G) {
+                fv_block(std::integral_constant<int, 0>{});
+                faces_of_line(std::integral_constant<int, 0>{});
+            }
+            if constexpr (DIM > 1) {
+                if (grp == G + 1) {
+                    fv_block(std::integral_constant<int, 1>{});
+                    faces_of_line(std::integral_constant<int, 1>{});
+                }
+            }
+            if constexpr (DIM > 2) {
+                if (grp == G + 2) {
+                    fv_block(std::integral_constant<int, 2>{});
+                    faces_of_line(std::integral_constant<int, 2>{});
+                }
+            }
         }
     }
     if constexpr (PD) __syncthreads();
@@ -547,7 +580,7 @@ vjp_kernel(const VjpArgs a) {
         __syncthreads();
         if (grp == 0 && line == 0 && e < E) {
             double sum = 0.0;
-            for (int g = 0; g < G; ++g)
+            for (int g = 0; g < GT; ++g)
                 for (int l = 0; l < NL; ++l) sum += abar_s[g * M * NL + l * M + el];
             a.alphabar[e] = dg_only ? 0.0 : sum;
             a.winner[e] = win;
