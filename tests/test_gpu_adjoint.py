@@ -62,6 +62,8 @@ TINY = [
          source="convergence_test", t=0.3),
     Case("euler1d_fd_dirichlet", 1, "euler", 2, "fd", "ranocha", cmin=0.0, cmax=2.0, periodic=False,
          bc_state="convergence_test", t=0.6),
+    Case("euler2d_fd_dirichlet", 2, "euler", 2, "fd", "ranocha", cmin=0.0, cmax=2.0, periodic=False,
+         bc_state="convergence_test", t=0.6),
     Case("mhd3d_fd_hg_llfnaive", 3, "mhd", 1, "fd", "hindenlang_gassner", "llf_naive", cmin=-1.0, cmax=1.0,
          gamma=5 / 3),
     Case("mhd3d_fd_hg_hg", 3, "mhd", 1, "fd", "hindenlang_gassner", "hindenlang_gassner", cmin=-1.0, cmax=1.0,

@@ -93,22 +93,23 @@ __host__ __device__ constexpr int vjp_width() {
 // (three 64-lane direction groups, one workgroup per CU) then keeps all four
 // SIMDs of its CU busy: one face group for every direction, 3D GLM-MHD at
 // level 4, VJP 69.1 -> 54.4 us (the + faces alone in it, the direction groups
-// keeping the - faces, 60.3 us).  With shock capturing, one group per
-// direction takes that direction's faces and its FV subcells, leaving the
-// direction groups the DG pairs.
-// TRIXI_HIP_VJP_FACE_GROUP: 2, per-direction groups with shock capturing
-// where they fit, else one face group; 1, one face group; 0, none.  They
-// apply to the reloading form (LDS accumulators, zeroed in the node pass)
-// where the extra accumulators leave the tile width unchanged.
+// keeping the - faces, 60.3 us).  For systems of up to five variables one
+// group per direction takes that direction's faces (and with shock capturing
+// its FV subcells), leaving the direction groups the volume pairs.
+// TRIXI_HIP_VJP_FACE_GROUP: 2, per-direction groups for up to five
+// variables where they fit, else one face group; 1, one face group; 0, none.
+// They apply where the extra accumulators leave the tile width unchanged,
+// and the kernel then takes the reloading form (LDS accumulators, zeroed in
+// the node pass).
 #ifndef TRIXI_HIP_VJP_FACE_GROUP
 #define TRIXI_HIP_VJP_FACE_GROUP 2
 #endif
 // the number of surface groups: 0, 1 (one face group) or G (per direction)
 template <int DIM, int N, int VOL, int NV, int NRF, int G>
 __host__ __device__ constexpr int vjp_surface_groups() {
-    if (!(G > 1 && VOL != VI_WEAK && (NRF > 10 || VOL == VI_HG)) || TRIXI_HIP_VJP_FACE_GROUP == 0) return 0;
+    if (!(G > 1 && VOL != VI_WEAK) || TRIXI_HIP_VJP_FACE_GROUP == 0) return 0;
     constexpr int w = vjp_width<DIM, N, NV, NRF, G, G>();
-    if (TRIXI_HIP_VJP_FACE_GROUP >= 2 && VOL == VI_HG && vjp_width<DIM, N, NV, NRF, G, 2 * G>() == w) return G;
+    if (TRIXI_HIP_VJP_FACE_GROUP >= 2 && NV <= 5 && vjp_width<DIM, N, NV, NRF, G, 2 * G>() == w) return G;
     if (vjp_width<DIM, N, NV, NRF, G, G + 1>() == w) return 1;
     return 0;
 }
@@ -143,10 +144,11 @@ vjp_kernel(const VjpArgs a) {
     constexpr int NT = M * NL * GT;  // workgroup size
     using Node = typename EQ::template Node<R>;
     using Bar = typename EQ::template Bar<R>;
-    // flux differencing with node records of more than 10 values, and shock
-    // capturing (which also recomputes the forward fluxes), reload the nodes
-    // per pair and accumulate cotangents in LDS
-    constexpr bool RELOAD = VOL != VI_WEAK && (NRF > 10 || VOL == VI_HG);
+    // flux differencing with node records of more than 10 values, shock
+    // capturing (which also recomputes the forward fluxes) and the forms
+    // with surface groups reload the nodes per pair and accumulate
+    // cotangents in LDS
+    constexpr bool RELOAD = VOL != VI_WEAK && (NRF > 10 || VOL == VI_HG || NS > 0);
 
     __shared__ R rec[NRF * NN * M];
     __shared__ double recd[2 * NN * M];
