@@ -490,7 +490,12 @@ __global__ void __launch_bounds__(64) indicator_vjp_kernel(
     for (int k = 0; k < EPB; ++k) any = any || ab_s[k] != 0.0;
     if (!any) return;  // block-uniform
     const PhysR<double> P = PhysR<double>::from(phys);
-    const double m = modal_indicator<DIM, N, double>(u, ec, E, P, Vinv, buf, el, n, var, nv);
+    // this node's state, kept for the reverse of the indicator variable below
+    // (compressible Euler: the shock-capturing adjoint is built for it)
+    double uc[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) uc[v] = u[((int64_t)v * NN + n) * E + ec];
+    const double m = modal_transform<DIM, N>(Euler<DIM>::indicator_variable(uc, P, var), Vinv, buf, el, n);
     e2[threadIdx.x] = m * m;
     __syncthreads();
     if (n == 0) {
@@ -548,7 +553,7 @@ __global__ void __launch_bounds__(64) indicator_vjp_kernel(
         return;
     }
     const double g = mb * P.gm1;
-    const double rho = u[r0], en = u[(int64_t)(DIM + 1) * NN * E + r0];
+    const double rho = uc[0], en = uc[DIM + 1];
     if (var == 1) {
         // pressure = (gamma - 1) (rho E - |rho v|^2 / (2 rho))
         const double irho = 1.0 / rho;
@@ -556,7 +561,7 @@ __global__ void __launch_bounds__(64) indicator_vjp_kernel(
 #pragma unroll
         for (int d = 0; d < DIM; ++d) {
             const int64_t rd = (int64_t)(1 + d) * NN * E + r0;
-            const double vd = u[rd] * irho;
+            const double vd = uc[1 + d] * irho;
             ke += vd * vd;
             bump(rd, -(g * vd));
         }
@@ -569,10 +574,9 @@ __global__ void __launch_bounds__(64) indicator_vjp_kernel(
 #pragma unroll
     for (int d = 0; d < DIM; ++d) {
         const int64_t rd = (int64_t)(1 + d) * NN * E + r0;
-        bump(rd, -(g * u[rd]));
+        bump(rd, -(g * uc[1 + d]));
     }
     bump((int64_t)(DIM + 1) * NN * E + r0, g * rho);
-    (void)NV;
 }
 
 __global__ void copy_kernel(double* __restrict__ dst, const double* __restrict__ src, int64_t n) {
